@@ -1,0 +1,130 @@
+/*
+ * sol105.h -- C ABI of the MI355X SOL 105 hot path (libsol105_hip.so).
+ *
+ * The functions replace, for the supported decks, the reference's Julia
+ * entry points listed beside each declaration (paths relative to the
+ * reference root).  Conventions:
+ *   - every function returns SOL105_OK (0) or a negative SOL105_E_* code;
+ *     sol105_last_error(ctx) gives the message of the last failure;
+ *   - host buffers are caller-owned; device memory is owned by the context;
+ *   - one context per host thread (a context serialises its own work on its
+ *     own HIP stream);
+ *   - node indices are 0-based ranks of the GRID ids sorted ascending
+ *     (JFEM/src/solver/assembly.jl:1958-1961); DOF index = 6*rank + component;
+ *   - matrices are returned in 6x6 block-sparse-row form (BSR) over all
+ *     6*n_nodes DOFs: block row pointers over nodes, block column indices,
+ *     36 values per block stored row-major.
+ */
+#ifndef SOL105_H
+#define SOL105_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SOL105_OK 0
+#define SOL105_E_INVALID (-1)      /* bad argument / inconsistent model arrays  */
+#define SOL105_E_UNSUPPORTED (-2)  /* feature outside the implemented branch    */
+#define SOL105_E_DEVICE (-3)       /* HIP runtime error or no device            */
+#define SOL105_E_SINGULAR (-4)     /* stiffness not positive definite           */
+#define SOL105_E_NOCONV (-5)       /* eigensolver did not converge              */
+#define SOL105_E_STATE (-6)        /* call order violated (e.g. KG before u)    */
+
+typedef struct sol105_ctx sol105_ctx;
+typedef struct sol105_model sol105_model;
+
+/* Model arrays (structure of arrays).  All pointers are host pointers and are
+ * copied during sol105_model_upload.  Replaces the model Dict consumed by
+ * Solver.assemble_stiffness (JFEM/src/solver/assembly.jl:1953-2530). */
+typedef struct sol105_model_desc {
+    int32_t n_nodes;
+    const double *xyz;          /* n_nodes x 3 basic-frame coordinates            */
+    int32_t n_q4;
+    const int32_t *q4_nodes;    /* n_q4 x 4 node ranks (CQUAD4 G1..G4)            */
+    const double *q4_prop;      /* n_q4 x 6: E, nu, T, 12I/T^3, TS/T, G_shear     */
+    int32_t n_bar;
+    const int32_t *bar_nodes;   /* n_bar x 2 node ranks (GA, GB)                  */
+    const double *bar_vref;     /* n_bar x 3 orientation vector (V, or G0-GA)     */
+    const double *bar_prop;     /* n_bar x 8: A, I1, I2, J, K1, K2, E, G          */
+    const uint8_t *spc_mask;    /* 6*n_nodes, 1 = constrained by SPC1/SPC/PS      */
+    const double *force;        /* 6*n_nodes applied load (analysis frame)        */
+    const uint8_t *node_has_shell; /* n_nodes (AUTOSPC rotational topology)       */
+    const uint8_t *node_has_bar;   /* n_nodes                                     */
+    double k6rot;               /* PARAM,K6ROT (default 100)                      */
+    int32_t autospc;            /* PARAM,AUTOSPC (default 1)                      */
+    int32_t nd;                 /* EIGRL ND                                       */
+} sol105_model_desc;
+
+/* Device selection and context.  No reference counterpart (the reference is
+ * CPU-only); one context owns one HIP stream and its scratch. */
+int sol105_ctx_create(int device, sol105_ctx **out);
+int sol105_ctx_destroy(sol105_ctx *ctx);
+const char *sol105_last_error(const sol105_ctx *ctx);
+
+/* Upload a model, build the BSR pattern, the deterministic element->block
+ * scatter map and the fill-reducing ordering.  Replaces the indexing part of
+ * Solver.assemble_stiffness (JFEM/src/solver/assembly.jl:1957-1989). */
+int sol105_model_upload(sol105_ctx *ctx, const sol105_model_desc *desc, sol105_model **out);
+int sol105_model_free(sol105_ctx *ctx, sol105_model *model);
+
+/* Replace the CQUAD4 thickness column (design update) without re-analysis. */
+int sol105_model_set_q4_thickness(sol105_ctx *ctx, sol105_model *model, const double *t /* n_q4 */);
+
+/* Pattern query: n_blocks, then block row pointers (n_nodes+1) and block
+ * columns (n_blocks).  Either array may be NULL. */
+int sol105_pattern(sol105_ctx *ctx, const sol105_model *model, int64_t *n_blocks,
+                   int32_t *block_row_ptr, int32_t *block_col);
+
+/* K assembly: element Ke kernels + deterministic block gather.  Replaces
+ * Solver.assemble_stiffness (JFEM/src/solver/assembly.jl:1953-4375).
+ * vals (n_blocks*36) may be NULL (keeps K on the device). */
+int sol105_assemble_K(sol105_ctx *ctx, sol105_model *model, double *vals);
+
+/* Static prestress solve: AUTOSPC, sparse Cholesky (device), solve.
+ * Replaces Solver.solve_case / apply_bc_and_solve
+ * (JFEM/src/solver/solve_case.jl:845-871, boundary_conditions.jl:530-1012).
+ * u (6*n_nodes) may be NULL; free_mask (6*n_nodes) may be NULL. */
+int sol105_static_solve(sol105_ctx *ctx, sol105_model *model, double *u, uint8_t *free_mask);
+
+/* KG assembly from the device-resident static displacement (or from u if
+ * non-NULL).  Replaces Solver.assemble_geometric_stiffness
+ * (JFEM/src/solver/assembly.jl:4382-6635).  vals may be NULL. */
+int sol105_assemble_KG(sol105_ctx *ctx, sol105_model *model, const double *u, double *vals);
+
+/* Buckling eigensolve with the reference's output semantics (mode count,
+ * positive and localization filters, max-abs normalisation).  Replaces
+ * Solver.solve_buckling (JFEM/src/solver/solve_case.jl:1863-3205).
+ * lambda_out has room for max_modes values, modes_out (may be NULL) for
+ * max_modes x 6*n_nodes doubles (mode-major).  *n_out receives the count. */
+int sol105_buckling(sol105_ctx *ctx, sol105_model *model, int32_t max_modes,
+                    double *lambda_out, double *modes_out, int32_t *n_out);
+
+/* End-to-end SOL 105 solve (K -> u -> KG -> buckling) on the device.
+ * Replaces the SOL 105 part of solve_model/_solve_sol105
+ * (JFEM/src/JFEMSolver.jl:789-872, 1296-1568). */
+int sol105_solve(sol105_ctx *ctx, sol105_model *model, int32_t max_modes,
+                 double *lambda_out, double *modes_out, int32_t *n_out);
+
+/* Per-element thickness sensitivity d(lambda_m)/dt_e of buckling mode m
+ * (0-based, after sol105_buckling) for every CQUAD4, including the prestress
+ * (adjoint) term.  Replaces solve_adjoint_buckling with one design variable
+ * per element (JFEM/src/solver/adjoint_buckling.jl:2406-2725). */
+int sol105_sensitivity_thickness(sol105_ctx *ctx, sol105_model *model, int32_t mode,
+                                 double *dlam_dt /* n_q4 */);
+
+/* Phase timings of the last sol105_solve in milliseconds (device events):
+ * [0] K assembly, [1] AUTOSPC+symbolic, [2] factorization, [3] static solve,
+ * [4] KG assembly, [5] eigensolve, [6] post-processing, [7] total. */
+int sol105_last_timings(const sol105_ctx *ctx, double *ms /* 8 */);
+
+/* Statistics of the factorization of the last solve: [0] n_free, [1] nnz(L),
+ * [2] factor flops, [3] number of fronts, [4] tree height, [5] Lanczos basis
+ * size, [6] Lanczos block steps, [7] bytes of front storage. */
+int sol105_last_stats(const sol105_ctx *ctx, double *stats /* 8 */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SOL105_H */

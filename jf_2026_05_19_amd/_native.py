@@ -1,0 +1,232 @@
+"""ctypes binding of libsol105_hip.so (the C ABI in include/sol105.h).
+
+The library is required: importing succeeds without it (so CPU-only tooling
+can inspect the package), but every call raises ``NativeLibraryMissing``
+when the built library is absent.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsol105_hip.so")
+
+SOL105_OK = 0
+ERRORS = {-1: "SOL105_E_INVALID", -2: "SOL105_E_UNSUPPORTED", -3: "SOL105_E_DEVICE", -4: "SOL105_E_SINGULAR",
+          -5: "SOL105_E_NOCONV", -6: "SOL105_E_STATE"}
+
+EXPORTS = [
+    "sol105_ctx_create", "sol105_ctx_destroy", "sol105_last_error", "sol105_model_upload", "sol105_model_free",
+    "sol105_model_set_q4_thickness", "sol105_pattern", "sol105_assemble_K", "sol105_static_solve",
+    "sol105_assemble_KG", "sol105_buckling", "sol105_solve", "sol105_sensitivity_thickness",
+    "sol105_last_timings", "sol105_last_stats",
+]
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+class Sol105Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (ERRORS.get(code, str(code)), msg))
+        self.code = code
+
+
+class ModelDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_nodes", ctypes.c_int32), ("xyz", ctypes.POINTER(ctypes.c_double)),
+        ("n_q4", ctypes.c_int32), ("q4_nodes", ctypes.POINTER(ctypes.c_int32)),
+        ("q4_prop", ctypes.POINTER(ctypes.c_double)),
+        ("n_bar", ctypes.c_int32), ("bar_nodes", ctypes.POINTER(ctypes.c_int32)),
+        ("bar_vref", ctypes.POINTER(ctypes.c_double)), ("bar_prop", ctypes.POINTER(ctypes.c_double)),
+        ("spc_mask", ctypes.POINTER(ctypes.c_uint8)), ("force", ctypes.POINTER(ctypes.c_double)),
+        ("node_has_shell", ctypes.POINTER(ctypes.c_uint8)), ("node_has_bar", ctypes.POINTER(ctypes.c_uint8)),
+        ("k6rot", ctypes.c_double), ("autospc", ctypes.c_int32), ("nd", ctypes.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load the native library (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeLibraryMissing(
+                "libsol105_hip.so is not built: run `python -m jf_2026_05_19_amd.build` (no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.sol105_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.sol105_ctx_destroy.argtypes = [vp]
+        L.sol105_last_error.argtypes = [vp]
+        L.sol105_last_error.restype = ctypes.c_char_p
+        L.sol105_model_upload.argtypes = [vp, ctypes.POINTER(ModelDesc), ctypes.POINTER(vp)]
+        L.sol105_model_free.argtypes = [vp, vp]
+        L.sol105_model_set_q4_thickness.argtypes = [vp, vp, dp]
+        L.sol105_pattern.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
+                                     ctypes.POINTER(ctypes.c_int32)]
+        L.sol105_assemble_K.argtypes = [vp, vp, dp]
+        L.sol105_static_solve.argtypes = [vp, vp, dp, ctypes.POINTER(ctypes.c_uint8)]
+        L.sol105_assemble_KG.argtypes = [vp, vp, dp, dp]
+        L.sol105_buckling.argtypes = [vp, vp, ctypes.c_int32, dp, dp, ctypes.POINTER(ctypes.c_int32)]
+        L.sol105_solve.argtypes = [vp, vp, ctypes.c_int32, dp, dp, ctypes.POINTER(ctypes.c_int32)]
+        L.sol105_sensitivity_thickness.argtypes = [vp, vp, ctypes.c_int32, dp]
+        L.sol105_last_timings.argtypes = [vp, dp]
+        L.sol105_last_stats.argtypes = [vp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Context:
+    """One HIP device context (stream + error state)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.sol105_ctx_create(int(device), ctypes.byref(h))
+        if rc != SOL105_OK:
+            raise Sol105Error(rc, "could not create a HIP context on device %d" % device)
+        self.h = h
+        self.L = L
+
+    def check(self, rc):
+        if rc != SOL105_OK:
+            raise Sol105Error(rc, self.L.sol105_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.L.sol105_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def timings(self):
+        out = np.zeros(8)
+        self.check(self.L.sol105_last_timings(self.h, _dp(out)))
+        return out
+
+    def stats(self):
+        out = np.zeros(8)
+        self.check(self.L.sol105_last_stats(self.h, _dp(out)))
+        return out
+
+
+class DeviceModel:
+    """A model uploaded to the device (pattern, maps and ordering built)."""
+
+    def __init__(self, ctx: Context, arr):
+        self.ctx = ctx
+        self.arr = arr
+        L = ctx.L
+        self._keep = dict(
+            xyz=np.ascontiguousarray(arr.xyz, dtype=np.float64),
+            q4=np.ascontiguousarray(arr.q4, dtype=np.int32),
+            q4prop=np.ascontiguousarray(arr.q4prop, dtype=np.float64),
+            bar=np.ascontiguousarray(arr.bar, dtype=np.int32),
+            vref=np.ascontiguousarray(arr.bar_vref, dtype=np.float64),
+            barprop=np.ascontiguousarray(arr.barprop, dtype=np.float64),
+            spc=np.ascontiguousarray(arr.spc_mask, dtype=np.uint8),
+            F=np.ascontiguousarray(arr.F, dtype=np.float64),
+            hs=np.ascontiguousarray(arr.node_has_shell, dtype=np.uint8),
+            hb=np.ascontiguousarray(arr.node_has_bar, dtype=np.uint8),
+        )
+        k = self._keep
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        up = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        d = ModelDesc(arr.n_nodes, _dp(k["xyz"]), arr.n_q4, ip(k["q4"]), _dp(k["q4prop"]),
+                      arr.n_bar, ip(k["bar"]), _dp(k["vref"]), _dp(k["barprop"]),
+                      up(k["spc"]), _dp(k["F"]), up(k["hs"]), up(k["hb"]),
+                      float(arr.k6rot), int(arr.autospc), int(arr.nd))
+        h = ctypes.c_void_p()
+        ctx.check(L.sol105_model_upload(ctx.h, ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        nb = ctypes.c_int64()
+        ctx.check(L.sol105_pattern(ctx.h, h, ctypes.byref(nb), None, None))
+        self.n_blocks = int(nb.value)
+        self.row_ptr = np.zeros(arr.n_nodes + 1, np.int32)
+        self.col = np.zeros(self.n_blocks, np.int32)
+        ctx.check(L.sol105_pattern(ctx.h, h, None, self.row_ptr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                   self.col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+
+    def free(self):
+        if self.h:
+            self.ctx.L.sol105_model_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # --- phases -----------------------------------------------------------
+    def set_q4_thickness(self, t):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        self.ctx.check(self.ctx.L.sol105_model_set_q4_thickness(self.ctx.h, self.h, _dp(t)))
+
+    def assemble_K(self, want=True):
+        vals = np.zeros(self.n_blocks * 36) if want else None
+        self.ctx.check(self.ctx.L.sol105_assemble_K(self.ctx.h, self.h, _dp(vals) if want else None))
+        return vals
+
+    def static_solve(self):
+        u = np.zeros(self.arr.ndof)
+        fm = np.zeros(self.arr.ndof, np.uint8)
+        self.ctx.check(self.ctx.L.sol105_static_solve(self.ctx.h, self.h, _dp(u),
+                                                      fm.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return u, fm.astype(bool)
+
+    def assemble_KG(self, u=None, want=True):
+        vals = np.zeros(self.n_blocks * 36) if want else None
+        uu = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
+        self.ctx.check(self.ctx.L.sol105_assemble_KG(self.ctx.h, self.h, _dp(uu) if uu is not None else None,
+                                                     _dp(vals) if want else None))
+        return vals
+
+    def buckling(self, max_modes=64, want_modes=True):
+        lam = np.zeros(max_modes)
+        modes = np.zeros((max_modes, self.arr.ndof)) if want_modes else None
+        n = ctypes.c_int32()
+        self.ctx.check(self.ctx.L.sol105_buckling(self.ctx.h, self.h, max_modes, _dp(lam),
+                                                  _dp(modes) if want_modes else None, ctypes.byref(n)))
+        k = n.value
+        return lam[:k].copy(), (modes[:k].T.copy() if want_modes else None)
+
+    def solve(self, max_modes=64, want_modes=True):
+        lam = np.zeros(max_modes)
+        modes = np.zeros((max_modes, self.arr.ndof)) if want_modes else None
+        n = ctypes.c_int32()
+        self.ctx.check(self.ctx.L.sol105_solve(self.ctx.h, self.h, max_modes, _dp(lam),
+                                               _dp(modes) if want_modes else None, ctypes.byref(n)))
+        k = n.value
+        return lam[:k].copy(), (modes[:k].T.copy() if want_modes else None)
+
+    def sensitivity_thickness(self, mode=0):
+        out = np.zeros(self.arr.n_q4)
+        self.ctx.check(self.ctx.L.sol105_sensitivity_thickness(self.ctx.h, self.h, int(mode), _dp(out)))
+        return out
+
+    def bsr_to_csr(self, vals):
+        """BSR values (n_blocks*36) -> scipy CSR over all DOFs."""
+        import scipy.sparse as sp
+        nb = self.n_blocks
+        brow = np.repeat(np.arange(self.arr.n_nodes), np.diff(self.row_ptr))
+        ci = np.arange(36) // 6
+        cj = np.arange(36) % 6
+        I = (6 * brow[:, None] + ci[None, :]).ravel()
+        J = (6 * self.col[:, None].astype(np.int64) + cj[None, :]).ravel()
+        return sp.csr_matrix((np.asarray(vals).reshape(nb * 36), (I, J)), shape=(self.arr.ndof, self.arr.ndof))

@@ -1,0 +1,806 @@
+// api.cpp -- the C ABI (include/sol105.h): context, model upload, and the
+// orchestration of one SOL 105 solve on a single HIP stream.
+//
+// Phase map to the reference:
+//   sol105_assemble_K    Solver.assemble_stiffness     JFEM/src/solver/assembly.jl:1953-4375
+//   sol105_static_solve  Solver.apply_bc_and_solve     JFEM/src/solver/boundary_conditions.jl:530-1012
+//   sol105_assemble_KG   Solver.assemble_geometric_stiffness  JFEM/src/solver/assembly.jl:4382-6635
+//   sol105_buckling      Solver.solve_buckling         JFEM/src/solver/solve_case.jl:1863-3205
+//   sol105_sensitivity_thickness  solve_adjoint_buckling  JFEM/src/solver/adjoint_buckling.jl:2406-2725
+#include "../../include/sol105.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "dense.h"
+#include "factor.h"
+#include "symbolic.h"
+
+struct sol105_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    double timings[8] = {0};
+    double stats[8] = {0};
+    hipEvent_t ev[9] = {nullptr};
+};
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void free_() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { free_(); }
+    hipError_t alloc(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        free_();
+        n = count;
+        return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+    }
+    hipError_t upload(const T *h, size_t count, hipStream_t s) {
+        hipError_t e = alloc(count);
+        if (e != hipSuccess) return e;
+        if (count == 0) return hipSuccess;
+        return hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s);
+    }
+};
+
+}  // namespace
+
+struct sol105_model {
+    // host copies
+    int n_nodes = 0, n_q4 = 0, n_bar = 0;
+    std::vector<double> xyz, q4prop, bar_vref, barprop, force;
+    std::vector<int32_t> q4, bar;
+    std::vector<uint8_t> spc_mask, has_shell, has_bar;
+    double k6rot = 100.0;
+    int autospc = 1, nd = 3;
+    BlockPattern pat;
+    std::vector<int32_t> diag_blk;
+    // device: model + assembly
+    DevBuf<double> d_xyz, d_q4prop, d_bar_vref, d_barprop, d_force;
+    DevBuf<int32_t> d_q4, d_bar, d_rp, d_col, d_cptr, d_cidx, d_diag;
+    DevBuf<double> d_eb, d_K, d_KG, d_u, d_kdiag;
+    // factor
+    bool have_sym = false;
+    std::vector<uint8_t> free_mask;
+    Symbolic sym;
+    FactorPlan plan;
+    DevBuf<int32_t> f_ns, f_nu, f_m, f_piv0, f_rows, f_rel, f_cptr, f_cidx, f_flist, f_ksrc, d_elim_of_dof,
+        d_dof_of_elim;
+    DevBuf<int64_t> f_off, f_rowoff, f_vecoff, f_reloff, f_kdst;
+    DevBuf<uint8_t> d_free, d_flag;
+    DevBuf<double> d_front, d_piv, d_W, d_kdiag_elim;
+    DevBuf<int> d_err;
+    DevFronts dev;
+    // state
+    bool K_valid = false, factor_valid = false, u_valid = false, KG_valid = false;
+    // eigen results
+    std::vector<double> lambda;
+    DevBuf<double> d_modes;  // ndof x n_modes (normalised, global frame)
+    int n_modes = 0;
+    double lanczos_m = 0, lanczos_steps = 0;
+    int ndof() const { return 6 * n_nodes; }
+};
+
+namespace {
+
+int set_err(sol105_ctx *ctx, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                                      \
+    do {                                                                                                  \
+        hipError_t e_ = (call);                                                                           \
+        if (e_ != hipSuccess) return set_err(ctx, SOL105_E_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+ElemArgs elem_args(sol105_model *M) {
+    ElemArgs a;
+    a.n_q4 = M->n_q4;
+    a.n_bar = M->n_bar;
+    a.xyz = M->d_xyz.p;
+    a.q4 = M->d_q4.p;
+    a.q4prop = M->d_q4prop.p;
+    a.bar = M->d_bar.p;
+    a.bar_vref = M->d_bar_vref.p;
+    a.barprop = M->d_barprop.p;
+    a.k6rot = M->k6rot;
+    return a;
+}
+
+// AUTOSPC on the diagonal (JFEM/src/solver/boundary_conditions.jl:296-383).
+void autospc_mask(const sol105_model *M, const std::vector<double> &kd, std::vector<uint8_t> &free_out) {
+    int ndof = M->ndof();
+    std::vector<uint8_t> fixed(M->spc_mask.begin(), M->spc_mask.end());
+    if (M->autospc) {
+        double max_t = 1.0, max_r = 1.0;
+        for (int i = 0; i < ndof; i++) {
+            if (i % 6 < 3) max_t = std::max(max_t, std::fabs(kd[i]));
+            else max_r = std::max(max_r, std::fabs(kd[i]));
+        }
+        for (int i = 0; i < ndof; i++) {
+            if (fixed[i]) continue;
+            int c = i % 6, v = i / 6;
+            double th;
+            if (c < 3) th = 1e-8 * std::max(max_t, 1.0);
+            else {
+                double mult = 1.0;
+                bool hs = M->has_shell[v], hb = M->has_bar[v];
+                if (hs && hb) mult = 0.1;
+                else if (hs) mult = 4.0;
+                else if (hb) mult = 0.25;
+                th = 1e-8 * std::max(max_r, 1.0) * mult;
+            }
+            if (std::fabs(kd[i]) < th || kd[i] < 0) fixed[i] = 1;
+        }
+    }
+    free_out.resize(ndof);
+    for (int i = 0; i < ndof; i++) free_out[i] = fixed[i] ? 0 : 1;
+}
+
+int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
+    hipStream_t s = ctx->stream;
+    Symbolic &S = M->sym;
+    int nf = (int)S.fronts.size();
+    std::vector<int32_t> ns(nf), nu(nf), m(nf), piv0(nf), cptr(nf + 1, 0), cidx, flist;
+    std::vector<int64_t> off(nf), rowoff(nf), vecoff(nf), reloff(nf);
+    for (int f = 0; f < nf; f++) {
+        const FrontInfo &F = S.fronts[f];
+        ns[f] = F.ns; nu[f] = F.nu; m[f] = F.m; piv0[f] = F.piv0;
+        off[f] = F.off; rowoff[f] = F.row_off; vecoff[f] = F.vec_off; reloff[f] = F.rel_off;
+        cptr[f + 1] = cptr[f] + (int32_t)S.children[f].size();
+        for (int c : S.children[f]) cidx.push_back(c);
+    }
+    FactorPlan &P = M->plan;
+    P.n_free = S.n_free;
+    P.height = S.height;
+    P.front_storage = S.front_storage;
+    P.vec_storage = S.vec_storage;
+    P.n_kmap = (int64_t)S.k_src.size();
+    P.heights.assign(S.height + 1, HeightInfo());
+    for (int h = 0; h <= S.height; h++) {
+        HeightInfo &H = P.heights[h];
+        H.n = (int)S.by_height[h].size();
+        H.list_off = (int64_t)flist.size();
+        for (int f : S.by_height[h]) {
+            flist.push_back(f);
+            H.max_ns = std::max(H.max_ns, S.fronts[f].ns);
+            H.max_m = std::max(H.max_m, S.fronts[f].m);
+            H.max_children = std::max(H.max_children, (int)S.children[f].size());
+            for (int c : S.children[f])
+                H.max_child_nu2 = std::max<int64_t>(H.max_child_nu2, (int64_t)S.fronts[c].nu * S.fronts[c].nu);
+        }
+    }
+    HIPCHK(M->f_ns.upload(ns.data(), nf, s));
+    HIPCHK(M->f_nu.upload(nu.data(), nf, s));
+    HIPCHK(M->f_m.upload(m.data(), nf, s));
+    HIPCHK(M->f_piv0.upload(piv0.data(), nf, s));
+    HIPCHK(M->f_off.upload(off.data(), nf, s));
+    HIPCHK(M->f_rowoff.upload(rowoff.data(), nf, s));
+    HIPCHK(M->f_vecoff.upload(vecoff.data(), nf, s));
+    HIPCHK(M->f_reloff.upload(reloff.data(), nf, s));
+    HIPCHK(M->f_rows.upload(S.rows.data(), S.rows.size(), s));
+    HIPCHK(M->f_rel.upload(S.rel.data(), S.rel.size(), s));
+    HIPCHK(M->f_cptr.upload(cptr.data(), cptr.size(), s));
+    HIPCHK(M->f_cidx.upload(cidx.data(), cidx.size(), s));
+    HIPCHK(M->f_flist.upload(flist.data(), flist.size(), s));
+    HIPCHK(M->f_ksrc.upload(S.k_src.data(), S.k_src.size(), s));
+    HIPCHK(M->f_kdst.upload(S.k_dst.data(), S.k_dst.size(), s));
+    HIPCHK(M->d_elim_of_dof.upload(S.elim_of_dof.data(), S.elim_of_dof.size(), s));
+    HIPCHK(M->d_dof_of_elim.upload(S.dof_of_elim.data(), S.dof_of_elim.size(), s));
+    HIPCHK(M->d_free.upload(M->free_mask.data(), M->free_mask.size(), s));
+    HIPCHK(M->d_front.alloc(std::max<int64_t>(S.front_storage, 1)));
+    HIPCHK(M->d_piv.alloc(std::max(S.n_free, 1)));
+    HIPCHK(M->d_kdiag_elim.alloc(std::max(S.n_free, 1)));
+    HIPCHK(M->d_flag.alloc(std::max(S.n_free, 1)));
+    HIPCHK(M->d_W.alloc(std::max<int64_t>(S.vec_storage * MAXR, 1)));
+    HIPCHK(M->d_err.alloc(4));
+    DevFronts &D = M->dev;
+    D.ns = M->f_ns.p; D.nu = M->f_nu.p; D.m = M->f_m.p; D.piv0 = M->f_piv0.p;
+    D.off = M->f_off.p; D.row_off = M->f_rowoff.p; D.vec_off = M->f_vecoff.p; D.rel_off = M->f_reloff.p;
+    D.rows = M->f_rows.p; D.rel = M->f_rel.p; D.child_ptr = M->f_cptr.p; D.child_idx = M->f_cidx.p;
+    D.flist = M->f_flist.p; D.k_src = M->f_ksrc.p; D.k_dst = M->f_kdst.p;
+    M->have_sym = true;
+    return SOL105_OK;
+}
+
+double ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return (double)ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sol105_ctx_create(int device, sol105_ctx **out) {
+    if (!out) return SOL105_E_INVALID;
+    *out = nullptr;
+    auto ctx = std::make_unique<sol105_ctx>();
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SOL105_E_DEVICE;
+    if (device < 0 || device >= ndev) return SOL105_E_INVALID;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) return SOL105_E_DEVICE;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return SOL105_E_DEVICE;
+    for (auto &e : ctx->ev)
+        if (hipEventCreate(&e) != hipSuccess) return SOL105_E_DEVICE;
+    *out = ctx.release();
+    return SOL105_OK;
+}
+
+int sol105_ctx_destroy(sol105_ctx *ctx) {
+    if (!ctx) return SOL105_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SOL105_OK;
+}
+
+const char *sol105_last_error(const sol105_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int sol105_model_upload(sol105_ctx *ctx, const sol105_model_desc *d, sol105_model **out) {
+    if (!ctx || !d || !out) return set_err(ctx, SOL105_E_INVALID, "null argument");
+    *out = nullptr;
+    if (d->n_nodes <= 0 || !d->xyz) return set_err(ctx, SOL105_E_INVALID, "model has no nodes");
+    if (d->n_q4 < 0 || d->n_bar < 0) return set_err(ctx, SOL105_E_INVALID, "negative element count");
+    if ((d->n_q4 > 0 && (!d->q4_nodes || !d->q4_prop)) || (d->n_bar > 0 && (!d->bar_nodes || !d->bar_prop || !d->bar_vref)))
+        return set_err(ctx, SOL105_E_INVALID, "missing element arrays");
+    if (!d->spc_mask || !d->force || !d->node_has_shell || !d->node_has_bar)
+        return set_err(ctx, SOL105_E_INVALID, "missing SPC / load / topology arrays");
+    if (d->nd <= 0) return set_err(ctx, SOL105_E_INVALID, "EIGRL ND must be positive");
+    (void)hipSetDevice(ctx->device);
+    auto M = std::make_unique<sol105_model>();
+    int n = d->n_nodes;
+    M->n_nodes = n;
+    M->n_q4 = d->n_q4;
+    M->n_bar = d->n_bar;
+    M->xyz.assign(d->xyz, d->xyz + 3 * (size_t)n);
+    M->q4.assign(d->q4_nodes, d->q4_nodes + 4 * (size_t)d->n_q4);
+    M->q4prop.assign(d->q4_prop, d->q4_prop + 6 * (size_t)d->n_q4);
+    M->bar.assign(d->bar_nodes, d->bar_nodes + 2 * (size_t)d->n_bar);
+    M->bar_vref.assign(d->bar_vref, d->bar_vref + 3 * (size_t)d->n_bar);
+    M->barprop.assign(d->bar_prop, d->bar_prop + 8 * (size_t)d->n_bar);
+    M->spc_mask.assign(d->spc_mask, d->spc_mask + 6 * (size_t)n);
+    M->force.assign(d->force, d->force + 6 * (size_t)n);
+    M->has_shell.assign(d->node_has_shell, d->node_has_shell + n);
+    M->has_bar.assign(d->node_has_bar, d->node_has_bar + n);
+    M->k6rot = d->k6rot;
+    M->autospc = d->autospc;
+    M->nd = d->nd;
+    for (int32_t v : M->q4)
+        if (v < 0 || v >= n) return set_err(ctx, SOL105_E_INVALID, "CQUAD4 node rank %d out of range", v);
+    for (int32_t v : M->bar)
+        if (v < 0 || v >= n) return set_err(ctx, SOL105_E_INVALID, "CBAR node rank %d out of range", v);
+    for (int e = 0; e < M->n_q4; e++) {
+        const double *p = &M->q4prop[6 * (size_t)e];
+        if (!(p[2] > 0.0) || !(p[0] > 0.0)) return set_err(ctx, SOL105_E_INVALID, "CQUAD4 %d: T and E must be positive", e);
+    }
+    build_block_pattern(n, M->n_q4, M->q4.data(), M->n_bar, M->bar.data(), M->pat);
+    M->diag_blk.resize(n);
+    for (int v = 0; v < n; v++) M->diag_blk[v] = (int32_t)M->pat.find(v, v);
+    hipStream_t s = ctx->stream;
+    HIPCHK(M->d_xyz.upload(M->xyz.data(), M->xyz.size(), s));
+    HIPCHK(M->d_q4.upload(M->q4.data(), M->q4.size(), s));
+    HIPCHK(M->d_q4prop.upload(M->q4prop.data(), M->q4prop.size(), s));
+    HIPCHK(M->d_bar.upload(M->bar.data(), M->bar.size(), s));
+    HIPCHK(M->d_bar_vref.upload(M->bar_vref.data(), M->bar_vref.size(), s));
+    HIPCHK(M->d_barprop.upload(M->barprop.data(), M->barprop.size(), s));
+    HIPCHK(M->d_force.upload(M->force.data(), M->force.size(), s));
+    HIPCHK(M->d_rp.upload(M->pat.row_ptr.data(), M->pat.row_ptr.size(), s));
+    HIPCHK(M->d_col.upload(M->pat.col.data(), M->pat.col.size(), s));
+    HIPCHK(M->d_cptr.upload(M->pat.cptr.data(), M->pat.cptr.size(), s));
+    HIPCHK(M->d_cidx.upload(M->pat.cidx.data(), M->pat.cidx.size(), s));
+    HIPCHK(M->d_diag.upload(M->diag_blk.data(), M->diag_blk.size(), s));
+    size_t neb = ((size_t)M->n_q4 * 16 + (size_t)M->n_bar * 4) * 36;
+    HIPCHK(M->d_eb.alloc(neb));
+    HIPCHK(M->d_K.alloc(M->pat.n_blocks() * 36));
+    HIPCHK(M->d_KG.alloc(M->pat.n_blocks() * 36));
+    HIPCHK(M->d_u.alloc(6 * (size_t)n));
+    HIPCHK(M->d_kdiag.alloc(6 * (size_t)n));
+    HIPCHK(hipStreamSynchronize(s));
+    *out = M.release();
+    return SOL105_OK;
+}
+
+int sol105_model_free(sol105_ctx *ctx, sol105_model *model) {
+    if (ctx) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    delete model;
+    return SOL105_OK;
+}
+
+int sol105_model_set_q4_thickness(sol105_ctx *ctx, sol105_model *M, const double *t) {
+    if (!ctx || !M || (!t && M->n_q4 > 0)) return set_err(ctx, SOL105_E_INVALID, "null argument");
+    for (int e = 0; e < M->n_q4; e++) {
+        if (!(t[e] > 0.0)) return set_err(ctx, SOL105_E_INVALID, "thickness of CQUAD4 %d must be positive", e);
+        M->q4prop[6 * (size_t)e + 2] = t[e];
+    }
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(M->d_q4prop.upload(M->q4prop.data(), M->q4prop.size(), ctx->stream));
+    M->K_valid = M->factor_valid = M->u_valid = M->KG_valid = false;
+    M->n_modes = 0;
+    return SOL105_OK;
+}
+
+int sol105_pattern(sol105_ctx *ctx, const sol105_model *M, int64_t *n_blocks, int32_t *rp, int32_t *col) {
+    if (!M) return set_err(ctx, SOL105_E_INVALID, "null model");
+    if (n_blocks) *n_blocks = M->pat.n_blocks();
+    if (rp) std::memcpy(rp, M->pat.row_ptr.data(), M->pat.row_ptr.size() * sizeof(int32_t));
+    if (col) std::memcpy(col, M->pat.col.data(), M->pat.col.size() * sizeof(int32_t));
+    return SOL105_OK;
+}
+
+int sol105_assemble_K(sol105_ctx *ctx, sol105_model *M, double *vals) {
+    if (!ctx || !M) return set_err(ctx, SOL105_E_INVALID, "null argument");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    ElemArgs a = elem_args(M);
+    launch_q4_ke(a, M->d_eb.p, s);
+    launch_bar_ke(a, M->d_eb.p, s);
+    launch_block_gather(M->pat.n_blocks(), M->d_cptr.p, M->d_cidx.p, M->d_eb.p, M->d_K.p, s);
+    HIPCHK(hipGetLastError());
+    M->K_valid = true;
+    M->factor_valid = M->u_valid = M->KG_valid = false;
+    if (vals) {
+        HIPCHK(hipMemcpyAsync(vals, M->d_K.p, M->pat.n_blocks() * 36 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    return SOL105_OK;
+}
+
+static int factorize(sol105_ctx *ctx, sol105_model *M) {
+    hipStream_t s = ctx->stream;
+    int ndof = M->ndof();
+    // AUTOSPC from the assembled diagonal
+    extract_kdiag(M->n_nodes, M->d_diag.p, M->d_K.p, M->d_kdiag.p, s);
+    std::vector<double> kd(ndof);
+    HIPCHK(hipMemcpyAsync(kd.data(), M->d_kdiag.p, ndof * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<uint8_t> fm;
+    autospc_mask(M, kd, fm);
+    for (int attempt = 0; attempt < 2; attempt++) {
+        if (!M->have_sym || fm != M->free_mask) {
+            M->free_mask = fm;
+            try {
+                symbolic_analyze(M->pat, M->xyz.data(), M->free_mask.data(), 32, M->sym);
+            } catch (const std::exception &ex) {
+                return set_err(ctx, SOL105_E_INVALID, "symbolic analysis failed: %s", ex.what());
+            }
+            if (M->sym.n_free == 0) return set_err(ctx, SOL105_E_SINGULAR, "no free DOFs");
+            int rc = upload_symbolic(ctx, M);
+            if (rc) return rc;
+        }
+        HIPCHK(hipMemsetAsync(M->d_err.p, 0, sizeof(int) * 4, s));
+        factor_numeric(M->plan, M->dev, M->d_K.p, M->d_front.p, M->d_piv.p, M->d_err.p, s);
+        // post-factorisation singular-pivot check: d_i / K_ii < 1e-7
+        launch_kdiag_elim(M->sym.n_free, M->d_dof_of_elim.p, M->d_diag.p, M->d_K.p, M->d_kdiag_elim.p, s);
+        launch_pivot_ratio(M->sym.n_free, M->d_piv.p, M->d_kdiag_elim.p, M->d_err.p + 1, M->d_flag.p, s);
+        HIPCHK(hipGetLastError());
+        int herr[4];
+        HIPCHK(hipMemcpyAsync(herr, M->d_err.p, sizeof(herr), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (herr[1] == 0 && herr[0] == 0) {
+            M->factor_valid = true;
+            return SOL105_OK;
+        }
+        if (attempt == 1) break;
+        // fix the flagged DOFs (reference: boundary_conditions.jl:742-782) and refactor
+        std::vector<uint8_t> flag(M->sym.n_free);
+        HIPCHK(hipMemcpy(flag.data(), M->d_flag.p, flag.size(), hipMemcpyDeviceToHost));
+        int nfix = 0;
+        for (int e = 0; e < M->sym.n_free; e++)
+            if (flag[e]) { fm[M->sym.dof_of_elim[e]] = 0; nfix++; }
+        if (nfix == 0) break;
+    }
+    return set_err(ctx, SOL105_E_SINGULAR, "stiffness matrix is not positive definite on the free set");
+}
+
+int sol105_static_solve(sol105_ctx *ctx, sol105_model *M, double *u, uint8_t *free_mask) {
+    if (!ctx || !M) return set_err(ctx, SOL105_E_INVALID, "null argument");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    if (!M->K_valid) {
+        int rc = sol105_assemble_K(ctx, M, nullptr);
+        if (rc) return rc;
+    }
+    if (!M->factor_valid) {
+        int rc = factorize(ctx, M);
+        if (rc) return rc;
+    }
+    int nfree = M->sym.n_free, ndof = M->ndof();
+    double *X = nullptr;
+    HIPCHK(hipMallocAsync((void **)&X, sizeof(double) * nfree, s));
+    launch_gather_elim(nfree, ndof, 1, M->d_dof_of_elim.p, M->d_force.p, X, s);
+    factor_solve(M->plan, M->dev, M->d_front.p, M->d_W.p, X, 1, s);
+    launch_scatter_elim(nfree, ndof, 1, M->d_elim_of_dof.p, X, M->d_u.p, s);
+    HIPCHK(hipFreeAsync(X, s));
+    HIPCHK(hipGetLastError());
+    M->u_valid = true;
+    M->KG_valid = false;
+    if (u) HIPCHK(hipMemcpyAsync(u, M->d_u.p, ndof * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (free_mask) std::memcpy(free_mask, M->free_mask.data(), ndof);
+    HIPCHK(hipStreamSynchronize(s));
+    return SOL105_OK;
+}
+
+int sol105_assemble_KG(sol105_ctx *ctx, sol105_model *M, const double *u, double *vals) {
+    if (!ctx || !M) return set_err(ctx, SOL105_E_INVALID, "null argument");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    if (u) {
+        HIPCHK(hipMemcpyAsync(M->d_u.p, u, M->ndof() * sizeof(double), hipMemcpyHostToDevice, s));
+        M->u_valid = true;
+    }
+    if (!M->u_valid) return set_err(ctx, SOL105_E_STATE, "no static displacement: run sol105_static_solve first");
+    ElemArgs a = elem_args(M);
+    launch_q4_kg(a, M->d_u.p, M->d_eb.p, s);
+    launch_bar_kg(a, M->d_u.p, M->d_eb.p, s);
+    launch_block_gather(M->pat.n_blocks(), M->d_cptr.p, M->d_cidx.p, M->d_eb.p, M->d_KG.p, s);
+    HIPCHK(hipGetLastError());
+    M->KG_valid = true;
+    if (vals) {
+        HIPCHK(hipMemcpyAsync(vals, M->d_KG.p, M->pat.n_blocks() * 36 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    return SOL105_OK;
+}
+
+namespace {
+
+// Block Lanczos on C = L^-1 (-KG) L^-T with full (CGS2) reorthogonalisation.
+// Returns the Ritz values / vectors of the `want` largest |theta|.
+struct LanczosOut {
+    std::vector<double> theta;  // selected Ritz values (order of selection)
+    std::vector<double> resid;
+    int m = 0, steps = 0;
+};
+
+int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want, elimination order */,
+            LanczosOut &out) {
+    hipStream_t s = ctx->stream;
+    int n = M->sym.n_free, ndof = M->ndof();
+    int b = want <= 8 ? 8 : 16;
+    int mmax = std::min(384, (n / b) * b);
+    if (mmax < b) return set_err(ctx, SOL105_E_NOCONV, "free set too small for the eigensolver (%d)", n);
+    int guard = 2;
+    int kneed = std::min(want + guard, mmax);
+    double tol = 1e-10;
+    DevBuf<double> V, Wb, xf, yf, T, A, Z, part, small, H2;
+    DevBuf<int> ierr, sw;
+    HIPCHK(V.alloc((size_t)n * (mmax + b)));
+    HIPCHK(Wb.alloc((size_t)n * b));
+    HIPCHK(xf.alloc((size_t)ndof * b));
+    HIPCHK(yf.alloc((size_t)ndof * b));
+    HIPCHK(T.alloc((size_t)(mmax + b) * (mmax + b)));
+    HIPCHK(A.alloc((size_t)mmax * mmax));
+    HIPCHK(Z.alloc((size_t)mmax * mmax));
+    HIPCHK(part.alloc(std::max<int64_t>(ts_gemm_tn_part_size(n, mmax + b, b), 1)));
+    HIPCHK(small.alloc(4 * 16 * 16));
+    HIPCHK(ierr.alloc(4));
+    HIPCHK(sw.alloc(1));
+    HIPCHK(H2.alloc((size_t)(mmax + b) * b));
+    int ldT = mmax + b;
+    dev_fill(T.p, (int64_t)ldT * ldT, 0.0, s);
+    HIPCHK(hipMemsetAsync(ierr.p, 0, sizeof(int) * 4, s));
+    double *G1 = small.p, *G2 = small.p + 256, *R = small.p + 512;
+    // CholQR2 of block Q (n x b) -> R_out (b x b, ld ldr)
+    auto cholqr2 = [&](double *Q, double *Rout, int ldr) {
+        ts_gemm_tn(n, b, b, Q, n, Q, n, G1, 16, 1.0, 0.0, part.p, s);
+        small_chol_upper(b, G1, 16, ierr.p, s);
+        trsm_right_upper(n, b, Q, n, G1, 16, s);
+        ts_gemm_tn(n, b, b, Q, n, Q, n, G2, 16, 1.0, 0.0, part.p, s);
+        small_chol_upper(b, G2, 16, ierr.p, s);
+        trsm_right_upper(n, b, Q, n, G2, 16, s);
+        if (Rout) upper_mul(b, G2, 16, G1, 16, Rout, ldr, s);
+    };
+    start_block(n, b, V.p, n, 0, s);
+    cholqr2(V.p, nullptr, 0);
+    int j = 0;
+    int next_check = std::min(mmax, std::max(2 * b, ((kneed * 3 + b - 1) / b) * b));
+    std::vector<double> hA, hZlast, hR;
+    std::vector<int> sel;
+    bool converged = false;
+    int m = 0;
+    while (true) {
+        double *Qj = V.p + (size_t)j * b * n;
+        // W = L^-1 (-KG) L^-T Qj
+        HIPCHK(hipMemcpyAsync(Wb.p, Qj, sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
+        factor_solve_bwd(M->plan, M->dev, M->d_front.p, M->d_W.p, Wb.p, b, s);
+        launch_scatter_elim(n, ndof, b, M->d_elim_of_dof.p, Wb.p, xf.p, s);
+        launch_bsr_spmv(M->n_nodes, M->d_rp.p, M->d_col.p, M->d_KG.p, xf.p, yf.p, b, -1.0, s);
+        launch_gather_elim(n, ndof, b, M->d_dof_of_elim.p, yf.p, Wb.p, s);
+        factor_solve_fwd(M->plan, M->dev, M->d_front.p, M->d_W.p, Wb.p, b, s);
+        int kcols = (j + 1) * b;
+        double *Tcol = T.p + (size_t)j * b * ldT;
+        ts_gemm_tn(n, kcols, b, V.p, n, Wb.p, n, Tcol, ldT, 1.0, 0.0, part.p, s);
+        ts_gemm_nn(n, kcols, b, V.p, n, Tcol, ldT, Wb.p, n, -1.0, 1.0, s);
+        // second Gram-Schmidt pass, accumulated into the same column block of T
+        ts_gemm_tn(n, kcols, b, V.p, n, Wb.p, n, H2.p, kcols, 1.0, 0.0, part.p, s);
+        ts_gemm_nn(n, kcols, b, V.p, n, H2.p, kcols, Wb.p, n, -1.0, 1.0, s);
+        mat_add(kcols, b, H2.p, kcols, Tcol, ldT, s);
+        j++;
+        m = j * b;
+        {
+            double *Qn = V.p + (size_t)j * b * n;
+            HIPCHK(hipMemcpyAsync(Qn, Wb.p, sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
+            cholqr2(Qn, T.p + (size_t)j * b + (size_t)(j - 1) * b * ldT, ldT);
+        }
+        bool last = (m >= mmax);
+        if (m >= next_check || last) {
+            symmetrize_copy(m, ldT, T.p, A.p, s);
+            // m is a multiple of the (even) block size
+            jacobi_eig(m, A.p, Z.p, 40, 1e-15, sw.p, s);
+            HIPCHK(hipGetLastError());
+            hA.assign((size_t)m * m, 0.0);
+            HIPCHK(hipMemcpyAsync(hA.data(), A.p, sizeof(double) * m * m, hipMemcpyDeviceToHost, s));
+            std::vector<double> hZ((size_t)m * m);
+            HIPCHK(hipMemcpyAsync(hZ.data(), Z.p, sizeof(double) * m * m, hipMemcpyDeviceToHost, s));
+            hR.assign((size_t)b * b, 0.0);
+            HIPCHK(hipMemcpy2DAsync(hR.data(), b * sizeof(double), T.p + (size_t)m + (size_t)(m - b) * ldT,
+                                    ldT * sizeof(double), b * sizeof(double), b, hipMemcpyDeviceToHost, s));
+            int herr[4];
+            HIPCHK(hipMemcpyAsync(herr, ierr.p, sizeof(herr), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            std::vector<double> th(m);
+            for (int i = 0; i < m; i++) th[i] = hA[(size_t)i * m + i];
+            std::vector<int> idx(m);
+            std::iota(idx.begin(), idx.end(), 0);
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) { return std::fabs(th[a]) > std::fabs(th[c]); });
+            // residual of Ritz pair i: || R_j * Z[m-b:m, i] ||
+            auto resid = [&](int i) {
+                double r2 = 0.0;
+                for (int p = 0; p < b; p++) {
+                    double v = 0.0;
+                    for (int q = 0; q < b; q++) v += hR[(size_t)q * b + p] * hZ[(size_t)i * m + (m - b) + q];
+                    r2 += v * v;
+                }
+                return std::sqrt(r2);
+            };
+            bool ok = true;
+            int kchk = std::min(kneed, m);
+            for (int t = 0; t < kchk; t++) {
+                int i = idx[t];
+                if (std::fabs(th[i]) < 1e-300) continue;
+                if (resid(i) > tol * std::fabs(th[i])) { ok = false; break; }
+            }
+            bool breakdown = (herr[0] & 2) != 0;
+            if (ok || last || breakdown) {
+                converged = ok;
+                int k = std::min(want, m);
+                sel.assign(idx.begin(), idx.begin() + k);
+                out.theta.clear();
+                out.resid.clear();
+                for (int i : sel) { out.theta.push_back(th[i]); out.resid.push_back(resid(i)); }
+                // Ritz vectors Y = V_m Z[:, sel]
+                std::vector<double> S((size_t)m * k);
+                for (int c = 0; c < k; c++)
+                    for (int r = 0; r < m; r++) S[(size_t)c * m + r] = hZ[(size_t)sel[c] * m + r];
+                DevBuf<double> dS;
+                HIPCHK(dS.upload(S.data(), S.size(), s));
+                ts_gemm_nn(n, m, k, V.p, n, dS.p, m, d_Y, n, 1.0, 0.0, s);
+                HIPCHK(hipStreamSynchronize(s));
+                break;
+            }
+            next_check = std::min(mmax, m + 2 * b);
+        }
+    }
+    out.m = m;
+    out.steps = j;
+    if (!converged) return set_err(ctx, SOL105_E_NOCONV, "block Lanczos did not converge in %d vectors", m);
+    return SOL105_OK;
+}
+
+}  // namespace
+
+int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double *lambda_out, double *modes_out,
+                    int32_t *n_out) {
+    if (!ctx || !M || !n_out) return set_err(ctx, SOL105_E_INVALID, "null argument");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    if (!M->factor_valid) return set_err(ctx, SOL105_E_STATE, "no factorization: run sol105_static_solve first");
+    if (!M->KG_valid) return set_err(ctx, SOL105_E_STATE, "no KG: run sol105_assemble_KG first");
+    int n = M->sym.n_free, ndof = M->ndof(), nd = M->nd;
+    bool dense = n <= 4000;
+    int req = dense ? 3 * nd : nd;
+    req = std::min(req, std::max(n - 2, 1));
+    if (req > MAXR) return set_err(ctx, SOL105_E_UNSUPPORTED, "more than %d requested modes", MAXR);
+    DevBuf<double> Y, Xf;
+    HIPCHK(Y.alloc((size_t)n * req));
+    LanczosOut lo;
+    int rc = lanczos(ctx, M, req, Y.p, lo);
+    if (rc) return rc;
+    int k = (int)lo.theta.size();
+    M->lanczos_m = lo.m;
+    M->lanczos_steps = lo.steps;
+    // x = L^-T y, expand to all DOFs
+    factor_solve_bwd(M->plan, M->dev, M->d_front.p, M->d_W.p, Y.p, k, s);
+    HIPCHK(Xf.alloc((size_t)ndof * k));
+    launch_scatter_elim(n, ndof, k, M->d_elim_of_dof.p, Y.p, Xf.p, s);
+    // eigenvalue semantics: lambda = 1/theta, |theta| > 1e-12 (dense) or
+    // >= 1e-14 and |lambda| > 1e-6 (Krylov); order by |lambda|, keep req
+    std::vector<int> cand;
+    std::vector<double> lam(k);
+    for (int i = 0; i < k; i++) {
+        double th = lo.theta[i];
+        lam[i] = 1.0 / th;
+        bool ok = dense ? (std::isfinite(th) && std::fabs(th) > 1e-12)
+                        : (std::fabs(th) >= 1e-14 && std::fabs(lam[i]) > 1e-6);
+        if (ok) cand.push_back(i);
+    }
+    std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) { return std::fabs(lam[a]) < std::fabs(lam[b]); });
+    if ((int)cand.size() > req) cand.resize(req);
+    // positive filter (lambda > 1e-10), ascending
+    std::vector<int> pos;
+    for (int i : cand)
+        if (lam[i] > 1e-10) pos.push_back(i);
+    std::stable_sort(pos.begin(), pos.end(), [&](int a, int b) { return lam[a] < lam[b]; });
+    // localization filter (defaults: >= 100 shells, share 0.10, scan ND + 16)
+    std::vector<int> surv;
+    if (M->n_q4 >= 100 && !pos.empty()) {
+        int limit = std::min((int)pos.size(), std::max(nd + 16, nd));
+        DevBuf<double> scr, sh;
+        HIPCHK(scr.alloc(2 * ((M->n_q4 + 255) / 256) + 2));
+        HIPCHK(sh.alloc(2 * (size_t)limit));
+        for (int t = 0; t < limit; t++) elem_energy_share(M->n_q4, M->d_q4.p, ndof, Xf.p, pos[t], scr.p, sh.p + 2 * t, s);
+        std::vector<double> hsh(2 * (size_t)limit);
+        HIPCHK(hipMemcpyAsync(hsh.data(), sh.p, hsh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int t = 0; t < limit; t++) {
+            double mx = hsh[2 * t], tot = hsh[2 * t + 1];
+            if (tot > 0 && mx / tot > 0.10) continue;
+            surv.push_back(pos[t]);
+        }
+        for (int t = limit; t < (int)pos.size(); t++) surv.push_back(pos[t]);
+    } else {
+        surv = pos;
+    }
+    int nout = std::min(req, (int)surv.size());
+    nout = std::min(nout, max_modes);
+    HIPCHK(M->d_modes.alloc((size_t)ndof * std::max(nout, 1)));
+    M->lambda.assign(nout, 0.0);
+    DevBuf<double> scr;
+    HIPCHK(scr.alloc(1024));
+    for (int t = 0; t < nout; t++) {
+        int i = surv[t];
+        M->lambda[t] = lam[i];
+        HIPCHK(hipMemcpyAsync(M->d_modes.p + (size_t)t * ndof, Xf.p + (size_t)i * ndof, ndof * sizeof(double),
+                              hipMemcpyDeviceToDevice, s));
+        normalize_maxabs(ndof, M->d_modes.p + (size_t)t * ndof, scr.p, s);
+    }
+    M->n_modes = nout;
+    *n_out = nout;
+    if (lambda_out)
+        for (int t = 0; t < nout; t++) lambda_out[t] = M->lambda[t];
+    if (modes_out && nout > 0)
+        HIPCHK(hipMemcpyAsync(modes_out, M->d_modes.p, (size_t)ndof * nout * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+    return SOL105_OK;
+}
+
+int sol105_solve(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double *lambda_out, double *modes_out,
+                 int32_t *n_out) {
+    if (!ctx || !M || !n_out) return set_err(ctx, SOL105_E_INVALID, "null argument");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipEventRecord(ctx->ev[0], s));
+    int rc = sol105_assemble_K(ctx, M, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->ev[1], s));
+    rc = factorize(ctx, M);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->ev[3], s));
+    rc = sol105_static_solve(ctx, M, nullptr, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->ev[4], s));
+    rc = sol105_assemble_KG(ctx, M, nullptr, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->ev[5], s));
+    rc = sol105_buckling(ctx, M, max_modes, lambda_out, modes_out, n_out);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->ev[6], s));
+    HIPCHK(hipEventSynchronize(ctx->ev[6]));
+    ctx->timings[0] = ev_ms(ctx->ev[0], ctx->ev[1]);
+    ctx->timings[1] = 0.0;
+    ctx->timings[2] = ev_ms(ctx->ev[1], ctx->ev[3]);
+    ctx->timings[3] = ev_ms(ctx->ev[3], ctx->ev[4]);
+    ctx->timings[4] = ev_ms(ctx->ev[4], ctx->ev[5]);
+    ctx->timings[5] = ev_ms(ctx->ev[5], ctx->ev[6]);
+    ctx->timings[6] = 0.0;
+    ctx->timings[7] = ev_ms(ctx->ev[0], ctx->ev[6]);
+    ctx->stats[0] = M->sym.n_free;
+    ctx->stats[1] = (double)M->sym.nnz_L;
+    ctx->stats[2] = M->sym.flops;
+    ctx->stats[3] = (double)M->sym.fronts.size();
+    ctx->stats[4] = M->sym.height;
+    ctx->stats[5] = M->lanczos_m;
+    ctx->stats[6] = M->lanczos_steps;
+    ctx->stats[7] = (double)M->sym.front_storage * 8.0;
+    return SOL105_OK;
+}
+
+int sol105_sensitivity_thickness(sol105_ctx *ctx, sol105_model *M, int32_t mode, double *dlam_dt) {
+    if (!ctx || !M || !dlam_dt) return set_err(ctx, SOL105_E_INVALID, "null argument");
+    if (mode < 0 || mode >= M->n_modes) return set_err(ctx, SOL105_E_STATE, "mode %d not available", mode);
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    int ndof = M->ndof(), n = M->sym.n_free;
+    const double *phi = M->d_modes.p + (size_t)mode * ndof;
+    double lam = M->lambda[mode];
+    ElemArgs a = elem_args(M);
+    DevBuf<double> egq, egb, g, ge, psi, kgphi, scr, dots;
+    HIPCHK(egq.alloc((size_t)std::max(M->n_q4, 1) * 24));
+    HIPCHK(egb.alloc((size_t)std::max(M->n_bar, 1) * 12));
+    HIPCHK(g.alloc(ndof));
+    HIPCHK(ge.alloc(std::max(n, 1)));
+    HIPCHK(psi.alloc(ndof));
+    HIPCHK(kgphi.alloc(ndof));
+    HIPCHK(scr.alloc(512));
+    HIPCHK(dots.alloc(2));
+    launch_q4_adjoint_rhs(a, M->d_u.p, phi, lam, egq.p, s);
+    launch_bar_adjoint_rhs(a, phi, lam, egb.p, s);
+    node_gather_vec(M->n_nodes, M->n_q4, M->d_diag.p, M->d_cptr.p, M->d_cidx.p, egq.p, egb.p, M->d_free.p, g.p, s);
+    launch_gather_elim(n, ndof, 1, M->d_dof_of_elim.p, g.p, ge.p, s);
+    factor_solve(M->plan, M->dev, M->d_front.p, M->d_W.p, ge.p, 1, s);
+    launch_scatter_elim(n, ndof, 1, M->d_elim_of_dof.p, ge.p, psi.p, s);
+    // phi' KG phi on the free set
+    DevBuf<double> phif;
+    HIPCHK(phif.alloc(ndof));
+    launch_gather_elim(n, ndof, 1, M->d_dof_of_elim.p, phi, ge.p, s);
+    launch_scatter_elim(n, ndof, 1, M->d_elim_of_dof.p, ge.p, phif.p, s);
+    launch_bsr_spmv(M->n_nodes, M->d_rp.p, M->d_col.p, M->d_KG.p, phif.p, kgphi.p, 1, 1.0, s);
+    dot_det(ndof, phif.p, kgphi.p, scr.p, dots.p, s);
+    double pkp = 0.0;
+    HIPCHK(hipMemcpyAsync(&pkp, dots.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    DevBuf<double> dl;
+    HIPCHK(dl.alloc(std::max(M->n_q4, 1)));
+    launch_q4_sens(a, M->d_u.p, phif.p, psi.p, lam, pkp, dl.p, s);
+    HIPCHK(hipMemcpyAsync(dlam_dt, dl.p, sizeof(double) * M->n_q4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+    return SOL105_OK;
+}
+
+int sol105_last_timings(const sol105_ctx *ctx, double *ms) {
+    if (!ctx || !ms) return SOL105_E_INVALID;
+    std::memcpy(ms, ctx->timings, sizeof(ctx->timings));
+    return SOL105_OK;
+}
+
+int sol105_last_stats(const sol105_ctx *ctx, double *st) {
+    if (!ctx || !st) return SOL105_E_INVALID;
+    std::memcpy(st, ctx->stats, sizeof(ctx->stats));
+    return SOL105_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,41 @@
+// common.h -- shared definitions of the MI355X SOL 105 library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#define S105_WAVE 64
+
+// Element-block layout: every element writes its global stiffness as dense
+// 6x6 node blocks (row-major), CQUAD4 first (16 blocks per element, index
+// e*16 + a*4 + b for corner nodes a, b), then CBAR (4 blocks per bar,
+// index q4_blocks + b*4 + a*2 + c).  The block gather sums these into the
+// 6x6 block-sparse-row matrix in a fixed element order.
+struct ElemArgs {
+    int n_q4;
+    int n_bar;
+    const double *xyz;        // n_nodes x 3
+    const int32_t *q4;        // n_q4 x 4
+    const double *q4prop;     // n_q4 x 6
+    const int32_t *bar;       // n_bar x 2
+    const double *bar_vref;   // n_bar x 3
+    const double *barprop;    // n_bar x 8
+    double k6rot;
+};
+
+// launchers (elements.hip)
+void launch_q4_ke(const ElemArgs &a, double *eb, hipStream_t s);
+void launch_q4_kg(const ElemArgs &a, const double *u, double *eb, hipStream_t s);
+void launch_bar_ke(const ElemArgs &a, double *eb, hipStream_t s);
+void launch_bar_kg(const ElemArgs &a, const double *u, double *eb, hipStream_t s);
+void launch_block_gather(int64_t n_blocks, const int32_t *cptr, const int32_t *cidx, const double *eb,
+                         double *vals, hipStream_t s);
+
+// dlambda/dt kernels (elements.hip)
+void launch_q4_sens(const ElemArgs &a, const double *u, const double *phi, const double *psi,
+                    double lam, double phiKGphi, double *dldt, hipStream_t s);
+void launch_q4_adjoint_rhs(const ElemArgs &a, const double *u, const double *phi, double lam,
+                           double *eg /* n_q4 x 24 */, hipStream_t s);
+void launch_bar_adjoint_rhs(const ElemArgs &a, const double *phi, double lam, double *eg /* n_bar x 12 */,
+                            hipStream_t s);

@@ -1,0 +1,28 @@
+// dense.h -- tall-skinny dense kernels of the block Lanczos eigensolver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <algorithm>
+
+// out = alpha * V^T W + beta * out   (V: n x P, W: n x Q <= 16; out ld ldo)
+void ts_gemm_tn(int n, int P, int Q, const double *V, int ldv, const double *W, int ldw, double *out, int ldo,
+                double alpha, double beta, double *part, hipStream_t s);
+int64_t ts_gemm_tn_part_size(int n, int P, int Q);
+// Y = alpha * V H + beta * Y   (Q <= 16)
+void ts_gemm_nn(int n, int P, int Q, const double *V, int ldv, const double *H, int ldh, double *Y, int ldy,
+                double alpha, double beta, hipStream_t s);
+void small_chol_upper(int b, double *G, int ldg, int *err, hipStream_t s);
+void trsm_right_upper(int n, int b, double *W, int ldw, const double *R, int ldr, hipStream_t s);
+void upper_mul(int b, const double *R2, int ld2, const double *R1, int ld1, double *out, int ldo, hipStream_t s);
+void start_block(int n, int b, double *V, int ldv, int seed, hipStream_t s);
+void jacobi_eig(int m, double *A, double *Z, int max_sweeps, double tol, int *sweeps, hipStream_t s);
+void elem_energy_share(int n_q4, const int32_t *q4, int ndof, const double *modes, int k, double *scratch,
+                       double *out2, hipStream_t s);
+void normalize_maxabs(int64_t n, double *x, double *scratch, hipStream_t s);
+void symmetrize_copy(int m, int ld, const double *T, double *A, hipStream_t s);
+void dot_det(int64_t n, const double *x, const double *y, double *scratch, double *out, hipStream_t s);
+void node_gather_vec(int n_nodes, int n_q4, const int32_t *diag_blk, const int32_t *cptr, const int32_t *cidx,
+                     const double *eg_q4, const double *eg_bar, const uint8_t *free_mask, double *g, hipStream_t s);
+void extract_kdiag(int n_nodes, const int32_t *diag_blk, const double *kv, double *out, hipStream_t s);
+// dst[i + j*ldd] += src[i + j*lds]
+void mat_add(int rows, int cols, const double *src, int lds, double *dst, int ldd, hipStream_t s);

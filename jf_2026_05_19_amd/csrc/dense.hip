@@ -1,0 +1,464 @@
+// dense.hip -- tall-skinny dense kernels for the block Lanczos eigensolver:
+// V^T W products (two-stage, fixed reduction order), W -= V H updates,
+// Cholesky-QR of a block, a parallel cyclic Jacobi eigensolver for the
+// projected matrix, the deterministic start block, and the post-processing
+// reductions (element energy shares, max-abs normalisation).
+#include "dense.h"
+
+#include <cmath>
+
+namespace {
+
+constexpr int CH = 1024;  // rows per partial-sum chunk
+
+// partial[c][p][q] = sum_{rows of chunk c} V[row, p] * W[row, q]
+__global__ void __launch_bounds__(256) tn_partial_kernel(int n, int P, int Q, const double *V, int ldv, const double *W,
+                                                         int ldw, double *part) {
+    int c = blockIdx.x;
+    int p0 = blockIdx.y * 16;
+    int pl = threadIdx.x % 16, ql = threadIdx.x / 16;
+    __shared__ double Vs[64][17];
+    __shared__ double Ws[64][17];
+    double acc = 0.0;
+    int r0 = c * CH, r1 = min(n, r0 + CH);
+    for (int rb = r0; rb < r1; rb += 64) {
+        for (int t = threadIdx.x; t < 64 * 16; t += 256) {
+            int r = t % 64, k = t / 64;
+            int row = rb + r;
+            Vs[r][k] = (row < r1 && p0 + k < P) ? V[row + (int64_t)(p0 + k) * ldv] : 0.0;
+            Ws[r][k] = (row < r1 && k < Q) ? W[row + (int64_t)k * ldw] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int r = 0; r < 64; r++) acc += Vs[r][pl] * Ws[r][ql];
+        __syncthreads();
+    }
+    if (p0 + pl < P && ql < Q) part[((int64_t)c * P + p0 + pl) * Q + ql] = acc;
+}
+
+// out[p + q*ldo] = alpha * sum_c part[c][p][q] + beta * out[...]
+__global__ void tn_reduce_kernel(int nch, int P, int Q, const double *part, double *out, int ldo, double alpha,
+                                 double beta) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P * Q) return;
+    int p = t / Q, q = t % Q;
+    double s = 0.0;
+    for (int c = 0; c < nch; c++) s += part[((int64_t)c * P + p) * Q + q];
+    double *o = out + p + (int64_t)q * ldo;
+    *o = alpha * s + (beta == 0.0 ? 0.0 : beta * *o);
+}
+
+// Y[row, q] = beta*Y + alpha * sum_p V[row, p] H[p, q]   (Q <= 16)
+__global__ void __launch_bounds__(256) nn_kernel(int n, int P, int Q, const double *V, int ldv, const double *H,
+                                                 int ldh, double *Y, int ldy, double alpha, double beta) {
+    __shared__ double Hs[256][16];
+    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc[q] = 0.0;
+    for (int pb = 0; pb < P; pb += 256) {
+        int pn = min(256, P - pb);
+        __syncthreads();
+        for (int t = threadIdx.x; t < 256 * 16; t += blockDim.x) {
+            int p = t / 16, q = t % 16;
+            Hs[p][q] = (p < pn && q < Q) ? H[(pb + p) + (int64_t)q * ldh] : 0.0;
+        }
+        __syncthreads();
+        if (row < n)
+            for (int p = 0; p < pn; p++) {
+                double v = V[row + (int64_t)(pb + p) * ldv];
+#pragma unroll
+                for (int q = 0; q < 16; q++) acc[q] += v * Hs[p][q];
+            }
+    }
+    if (row < n)
+        for (int q = 0; q < Q; q++) {
+            double *y = Y + row + (int64_t)q * ldy;
+            *y = alpha * acc[q] + (beta == 0.0 ? 0.0 : beta * *y);
+        }
+}
+
+// Cholesky G = R^T R (upper R, b <= 16) in place; single thread.
+__global__ void chol_upper_kernel(int b, double *G, int ldg, int *err) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int j = 0; j < b; j++) {
+        double d = G[j + j * ldg];
+        for (int k = 0; k < j; k++) d -= G[k + j * ldg] * G[k + j * ldg];
+        if (!(d > 0.0)) { *err |= 2; d = 1e-300; }
+        double r = sqrt(d);
+        G[j + j * ldg] = r;
+        for (int i = j + 1; i < b; i++) {
+            double s = G[j + i * ldg];
+            for (int k = 0; k < j; k++) s -= G[k + j * ldg] * G[k + i * ldg];
+            G[j + i * ldg] = s / r;
+        }
+        for (int i = j + 1; i < b; i++) G[i + j * ldg] = 0.0;
+    }
+}
+
+// W = W R^-1 (R upper b x b), one thread per row
+__global__ void trsm_right_upper_kernel(int n, int b, double *W, int ldw, const double *R, int ldr) {
+    __shared__ double Rs[16][16];
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) {
+        int i = t % 16, j = t / 16;
+        Rs[i][j] = (i < b && j < b) ? R[i + j * ldr] : 0.0;
+    }
+    __syncthreads();
+    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    double x[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) x[j] = (j < b) ? W[row + (int64_t)j * ldw] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        if (j < b) {
+            double s = x[j];
+#pragma unroll
+            for (int k = 0; k < j; k++) s -= x[k] * Rs[k][j];
+            x[j] = s / Rs[j][j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+        if (j < b) W[row + (int64_t)j * ldw] = x[j];
+}
+
+// R_prod = R2 * R1 (upper b x b): used for CholQR2 (R = R2 R1)
+__global__ void upper_mul_kernel(int b, const double *R2, int ld2, const double *R1, int ld1, double *out, int ldo) {
+    int t = threadIdx.x;
+    if (t >= b * b) return;
+    int i = t % b, j = t / b;
+    double s = 0.0;
+    for (int k = 0; k < b; k++) s += R2[i + k * ld2] * R1[k + j * ld1];
+    out[i + j * ldo] = s;
+}
+
+// Start block (elimination order), seed 0: sin(0.75487... x) + 0.5 cos(1.24697... x),
+// x = i + 0.137 (seed + 1) + 0.731 * ordinal  (JFEM/src/solver/solve_case.jl:59-68)
+__global__ void start_block_kernel(int n, int b, double *V, int ldv, int seed) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * b) return;
+    int i = (int)(t % n), r = (int)(t / n);
+    double x = (i + 1) + 0.137 * (seed + 1) + 0.731 * (r + 1);
+    V[i + (int64_t)r * ldv] = sin(0.7548776662466927 * x) + 0.5 * cos(1.246979603717467 * x);
+}
+
+// ---------------------------------------------------------------------------
+// Parallel cyclic Jacobi eigensolver (one workgroup).  Matrix A (m x m, ld m,
+// symmetric) and eigenvectors Z (m x m) live in global memory; each round
+// rotates m/2 disjoint index pairs of the round-robin tournament.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void rr_pair(int m, int round, int k, int &p, int &q) {
+    // circle method: position 0 fixed, others rotate
+    int n1 = m - 1;
+    int a = (k == 0) ? 0 : 1 + ((k - 1 + round) % n1);
+    int bpos = m - 1 - k;
+    int b = 1 + ((bpos - 1 + round) % n1);
+    p = min(a, b);
+    q = max(a, b);
+}
+
+__global__ void __launch_bounds__(1024) jacobi_kernel(int m, double *A, double *Z, int max_sweeps, double tol,
+                                                      int *sweeps_out) {
+    __shared__ double cs[512], sn[512];
+    __shared__ int pp[512], qq[512];
+    __shared__ double red[1024];
+    __shared__ int done;
+    int tid = threadIdx.x, nt = blockDim.x;
+    for (int t = tid; t < m * m; t += nt) Z[t] = ((t % m) == (t / m)) ? 1.0 : 0.0;
+    if (tid == 0) done = 0;
+    __syncthreads();
+    int half = m / 2;
+    int sweep = 0;
+    for (; sweep < max_sweeps; sweep++) {
+        // convergence: off-diagonal Frobenius vs diagonal
+        double off = 0.0, dia = 0.0;
+        for (int t = tid; t < m * m; t += nt) {
+            int i = t % m, j = t / m;
+            double v = A[t];
+            if (i == j) dia += v * v; else off += v * v;
+        }
+        red[tid] = off;
+        __syncthreads();
+        for (int s = nt / 2; s > 0; s >>= 1) {
+            if (tid < s) red[tid] += red[tid + s];
+            __syncthreads();
+        }
+        double offs = red[0];
+        __syncthreads();
+        red[tid] = dia;
+        __syncthreads();
+        for (int s = nt / 2; s > 0; s >>= 1) {
+            if (tid < s) red[tid] += red[tid + s];
+            __syncthreads();
+        }
+        double dias = red[0];
+        __syncthreads();
+        if (offs <= tol * tol * dias) break;
+        for (int round = 0; round < m - 1; round++) {
+            for (int k = tid; k < half; k += nt) {
+                int p, q;
+                rr_pair(m, round, k, p, q);
+                double apq = A[p + q * m], app = A[p + p * m], aqq = A[q + q * m];
+                double c = 1.0, s = 0.0;
+                if (apq != 0.0) {
+                    double tau = (aqq - app) / (2.0 * apq);
+                    double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    c = 1.0 / sqrt(1.0 + t * t);
+                    s = t * c;
+                }
+                cs[k] = c; sn[k] = s; pp[k] = p; qq[k] = q;
+            }
+            __syncthreads();
+            // rows: A[p,:], A[q,:]
+            for (int t = tid; t < half * m; t += nt) {
+                int k = t % half, j = t / half;
+                int p = pp[k], q = qq[k];
+                double c = cs[k], s = sn[k];
+                double ap = A[p + j * m], aq = A[q + j * m];
+                A[p + j * m] = c * ap - s * aq;
+                A[q + j * m] = s * ap + c * aq;
+            }
+            __syncthreads();
+            // columns: A[:,p], A[:,q] and Z[:,p], Z[:,q]
+            for (int t = tid; t < half * m; t += nt) {
+                int k = t / m, i = t % m;
+                int p = pp[k], q = qq[k];
+                double c = cs[k], s = sn[k];
+                double ap = A[i + p * m], aq = A[i + q * m];
+                A[i + p * m] = c * ap - s * aq;
+                A[i + q * m] = s * ap + c * aq;
+                double zp = Z[i + p * m], zq = Z[i + q * m];
+                Z[i + p * m] = c * zp - s * zq;
+                Z[i + q * m] = s * zp + c * zq;
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) *sweeps_out = sweep;
+}
+
+// per-element translational energy share of each mode (localization filter,
+// JFEM/src/solver/solve_case.jl:2795-2893): e = mean over corner nodes of
+// tx^2+ty^2+tz^2; out[k] = max_e / sum_e for mode k (two-stage).
+__global__ void elem_energy_partial_kernel(int n_q4, const int32_t *q4, int ndof, const double *modes, int k,
+                                           double *pmax, double *psum) {
+    __shared__ double smax[256], ssum[256];
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const double *x = modes + (int64_t)k * ndof;
+    double v = 0.0;
+    if (e < n_q4) {
+        for (int a = 0; a < 4; a++) {
+            const double *t = x + 6 * (int64_t)q4[4 * e + a];
+            v += t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+        }
+        v /= 4.0;
+    }
+    smax[threadIdx.x] = v;
+    ssum[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+            ssum[threadIdx.x] += ssum[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { pmax[blockIdx.x] = smax[0]; psum[blockIdx.x] = ssum[0]; }
+}
+
+__global__ void final_share_kernel(int nb, const double *pmax, const double *psum, double *out) {
+    __shared__ double smax[256], ssum[256];
+    double mx = 0.0, sm = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) { mx = fmax(mx, pmax[i]); }
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+        __syncthreads();
+    }
+    // sequential (fixed-order) sum for determinism
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < nb; i++) sm += psum[i];
+        out[0] = smax[0];
+        out[1] = sm;
+    }
+}
+
+// column max-abs (two-stage) and scale
+__global__ void colmaxabs_partial_kernel(int64_t n, const double *x, double *part) {
+    __shared__ double sm[256];
+    double v = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) v = fmax(v, fabs(x[i]));
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+__global__ void scale_by_max_kernel(int64_t n, double *x, const double *part, int np) {
+    __shared__ double mx;
+    if (threadIdx.x == 0) {
+        double v = 0.0;
+        for (int i = 0; i < np; i++) v = fmax(v, part[i]);
+        mx = v;
+    }
+    __syncthreads();
+    double m = mx;
+    if (!(m > 1e-30)) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] /= m;
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+void ts_gemm_tn(int n, int P, int Q, const double *V, int ldv, const double *W, int ldw, double *out, int ldo,
+                double alpha, double beta, double *part, hipStream_t s) {
+    int nch = (n + CH - 1) / CH;
+    tn_partial_kernel<<<dim3(nch, (P + 15) / 16), 256, 0, s>>>(n, P, Q, V, ldv, W, ldw, part);
+    tn_reduce_kernel<<<nblk((int64_t)P * Q, 256), 256, 0, s>>>(nch, P, Q, part, out, ldo, alpha, beta);
+}
+
+int64_t ts_gemm_tn_part_size(int n, int P, int Q) { return (int64_t)((n + CH - 1) / CH) * P * Q; }
+
+void ts_gemm_nn(int n, int P, int Q, const double *V, int ldv, const double *H, int ldh, double *Y, int ldy,
+                double alpha, double beta, hipStream_t s) {
+    nn_kernel<<<nblk(n, 256), 256, 0, s>>>(n, P, Q, V, ldv, H, ldh, Y, ldy, alpha, beta);
+}
+
+void small_chol_upper(int b, double *G, int ldg, int *err, hipStream_t s) {
+    chol_upper_kernel<<<1, 64, 0, s>>>(b, G, ldg, err);
+}
+
+void trsm_right_upper(int n, int b, double *W, int ldw, const double *R, int ldr, hipStream_t s) {
+    trsm_right_upper_kernel<<<nblk(n, 256), 256, 0, s>>>(n, b, W, ldw, R, ldr);
+}
+
+void upper_mul(int b, const double *R2, int ld2, const double *R1, int ld1, double *out, int ldo, hipStream_t s) {
+    upper_mul_kernel<<<1, 256, 0, s>>>(b, R2, ld2, R1, ld1, out, ldo);
+}
+
+void start_block(int n, int b, double *V, int ldv, int seed, hipStream_t s) {
+    start_block_kernel<<<nblk((int64_t)n * b, 256), 256, 0, s>>>(n, b, V, ldv, seed);
+}
+
+void jacobi_eig(int m, double *A, double *Z, int max_sweeps, double tol, int *sweeps, hipStream_t s) {
+    jacobi_kernel<<<1, 1024, 0, s>>>(m, A, Z, max_sweeps, tol, sweeps);
+}
+
+void elem_energy_share(int n_q4, const int32_t *q4, int ndof, const double *modes, int k, double *scratch,
+                       double *out2, hipStream_t s) {
+    unsigned nb = nblk(n_q4, 256);
+    elem_energy_partial_kernel<<<nb, 256, 0, s>>>(n_q4, q4, ndof, modes, k, scratch, scratch + nb);
+    final_share_kernel<<<1, 256, 0, s>>>((int)nb, scratch, scratch + nb, out2);
+}
+
+void normalize_maxabs(int64_t n, double *x, double *scratch, hipStream_t s) {
+    unsigned nb = (unsigned)std::min<int64_t>(1024, (n + 255) / 256);
+    colmaxabs_partial_kernel<<<nb, 256, 0, s>>>(n, x, scratch);
+    scale_by_max_kernel<<<nb, 256, 0, s>>>(n, x, scratch, (int)nb);
+}
+
+namespace {
+
+__global__ void symmetrize_kernel(int m, int ld, const double *T, double *A) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m * m) return;
+    int i = t % m, j = t / m;
+    A[t] = 0.5 * (T[i + (int64_t)j * ld] + T[j + (int64_t)i * ld]);
+}
+
+__global__ void dot_partial_kernel(int64_t n, const double *x, const double *y, double *part) {
+    __shared__ double sm[256];
+    double v = 0.0;
+    int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    int64_t b = (int64_t)blockIdx.x * per, e = std::min(n, b + per);
+    for (int64_t i = b + threadIdx.x; i < e; i += 256) v += x[i] * y[i];
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+__global__ void dot_final_kernel(int np, const double *part, double *out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < np; i++) s += part[i];
+        *out = s;
+    }
+}
+
+// g[6v + c] = sum over the node's incident element vectors (fixed order)
+__global__ void node_gather_kernel(int n_nodes, int n_q4, const int32_t *diag_blk, const int32_t *cptr,
+                                   const int32_t *cidx, const double *eg_q4, const double *eg_bar,
+                                   const uint8_t *free_mask, double *g) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_nodes * 6) return;
+    int v = t / 6, c = t % 6;
+    int d = diag_blk[v];
+    double s = 0.0;
+    int64_t q4b = (int64_t)n_q4 * 16;
+    for (int k = cptr[d]; k < cptr[d + 1]; k++) {
+        int64_t idx = cidx[k];
+        if (idx < q4b) {
+            int64_t e = idx / 16;
+            int a = (int)((idx % 16) / 4);
+            s += eg_q4[e * 24 + a * 6 + c];
+        } else {
+            int64_t r = idx - q4b;
+            int64_t b = r / 4;
+            int a = (int)((r % 4) / 2);
+            s += eg_bar[b * 12 + a * 6 + c];
+        }
+    }
+    g[t] = free_mask[t] ? s : 0.0;
+}
+
+__global__ void kdiag_kernel(int n_nodes, const int32_t *diag_blk, const double *kv, double *out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_nodes * 6) return;
+    int v = t / 6, c = t % 6;
+    out[t] = kv[(int64_t)diag_blk[v] * 36 + c * 7];
+}
+
+}  // namespace
+
+void symmetrize_copy(int m, int ld, const double *T, double *A, hipStream_t s) {
+    symmetrize_kernel<<<(m * m + 255) / 256, 256, 0, s>>>(m, ld, T, A);
+}
+
+void dot_det(int64_t n, const double *x, const double *y, double *scratch, double *out, hipStream_t s) {
+    int np = 256;
+    dot_partial_kernel<<<np, 256, 0, s>>>(n, x, y, scratch);
+    dot_final_kernel<<<1, 64, 0, s>>>(np, scratch, out);
+}
+
+void node_gather_vec(int n_nodes, int n_q4, const int32_t *diag_blk, const int32_t *cptr, const int32_t *cidx,
+                     const double *eg_q4, const double *eg_bar, const uint8_t *free_mask, double *g, hipStream_t s) {
+    node_gather_kernel<<<(n_nodes * 6 + 255) / 256, 256, 0, s>>>(n_nodes, n_q4, diag_blk, cptr, cidx, eg_q4, eg_bar,
+                                                                free_mask, g);
+}
+
+void extract_kdiag(int n_nodes, const int32_t *diag_blk, const double *kv, double *out, hipStream_t s) {
+    kdiag_kernel<<<(n_nodes * 6 + 255) / 256, 256, 0, s>>>(n_nodes, diag_blk, kv, out);
+}
+
+namespace {
+__global__ void mat_add_kernel(int rows, int cols, const double *src, int lds, double *dst, int ldd) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows * cols) return;
+    int i = t % rows, j = t / rows;
+    dst[i + (int64_t)j * ldd] += src[i + (int64_t)j * lds];
+}
+}  // namespace
+
+void mat_add(int rows, int cols, const double *src, int lds, double *dst, int ldd, hipStream_t s) {
+    if (rows * cols > 0) mat_add_kernel<<<(rows * cols + 255) / 256, 256, 0, s>>>(rows, cols, src, lds, dst, ldd);
+}

@@ -1,0 +1,1200 @@
+// elements.hip -- element stiffness / geometric stiffness kernels for gfx950.
+//
+// CQUAD4 (MacNeal, default flat-isotropic branch) is computed by one 64-lane
+// wavefront per element: lane l owns the 3x3 sub-block (l/8, l%8) of the
+// 24x24 element matrix, so the per-element set-up (Gauss-point Jacobians,
+// incompatible-mode couplings, MacNeal shear compliances) is staged once in
+// LDS and the 576 outputs are produced without redundancy.  Results go to an
+// element-block buffer through an LDS transpose so every wavefront stores
+// 4.6 KB contiguously.  Four elements share a 256-thread workgroup.
+//
+// Mathematics restated from the reference:
+//   frame .......... JFEM/src/solver/helpers.jl:1134-1193 (:diag mode)
+//   Ke ............. JFEM/src/FEMKernels.jl:3185-4034 (macneal, twist row,
+//                    Hughes-Brezzi drilling k6rot/1e5, Wilson membrane and
+//                    bending incompatible modes condensed)
+//   MacNeal shear .. JFEM/src/FEMKernels.jl:4715-4978 (eps 0.04, zb 0.65)
+//   T'KT ........... JFEM/src/solver/assembly.jl:3808-3837
+//   KG ............. JFEM/src/FEMKernels.jl:6341-6539, 6708-6785, 6891-6972,
+//                    7554-7985; JFEM/src/solver/assembly.jl:5175-6007
+//   CBAR ........... JFEM/src/FEMKernels.jl:140-229, 7441-7471;
+//                    JFEM/src/solver/assembly.jl:3960-4020, 6272-6345
+#include "common.h"
+#include <cmath>
+
+namespace {
+
+constexpr double PT = 0.57735026918962576451;  // 1/sqrt(3)
+constexpr int EPB = 4;                         // elements (wavefronts) per workgroup
+
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ double dot3(const double *a, const double *b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+__device__ __forceinline__ void normalize3(double *a) {
+    double n = sqrt(dot3(a, a));
+    a[0] /= n; a[1] /= n; a[2] /= n;
+}
+
+// CQUAD4 element frame (rows of R = v1, v2, v3) and local corner coordinates.
+__device__ void q4_frame(const double *p /*12*/, double *R /*9*/, double *lc /*8*/) {
+    double gxi[3], geta[3], n[3], d13[3], d24[3];
+    for (int i = 0; i < 3; i++) {
+        gxi[i] = 0.25 * (-p[i] + p[3 + i] + p[6 + i] - p[9 + i]);
+        geta[i] = 0.25 * (-p[i] - p[3 + i] + p[6 + i] + p[9 + i]);
+        d13[i] = p[6 + i] - p[i];
+        d24[i] = p[9 + i] - p[3 + i];
+    }
+    cross3(gxi, geta, n);
+    if (dot3(n, n) <= 1e-24) cross3(d13, d24, n);
+    normalize3(n);
+    normalize3(d13);
+    normalize3(d24);
+    double x[3] = {d13[0] - d24[0], d13[1] - d24[1], d13[2] - d24[2]};
+    if (dot3(x, x) < 1e-20) { x[0] = d13[0] + d24[0]; x[1] = d13[1] + d24[1]; x[2] = d13[2] + d24[2]; }
+    double xd = dot3(x, n);
+    double xp[3] = {x[0] - xd * n[0], x[1] - xd * n[1], x[2] - xd * n[2]};
+    if (dot3(xp, xp) <= 1e-24) {
+        double gd = dot3(geta, n);
+        for (int i = 0; i < 3; i++) xp[i] = geta[i] - gd * n[i];
+        if (dot3(xp, xp) <= 1e-24) {
+            double e[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]};
+            double ed = dot3(e, n);
+            for (int i = 0; i < 3; i++) xp[i] = e[i] - ed * n[i];
+        }
+    }
+    normalize3(xp);
+    double v2[3];
+    cross3(n, xp, v2);
+    for (int i = 0; i < 3; i++) { R[i] = xp[i]; R[3 + i] = v2[i]; R[6 + i] = n[i]; }
+    double c[3];
+    for (int i = 0; i < 3; i++) c[i] = (p[i] + p[3 + i] + p[6 + i] + p[9 + i]) / 4.0;
+    for (int k = 0; k < 4; k++) {
+        double d[3] = {p[3 * k] - c[0], p[3 * k + 1] - c[1], p[3 * k + 2] - c[2]};
+        lc[2 * k] = dot3(d, xp);
+        lc[2 * k + 1] = dot3(d, v2);
+    }
+}
+
+__device__ __forceinline__ void shape_derivs(double r, double s, double *dNr, double *dNs) {
+    dNr[0] = -0.25 * (1 - s); dNr[1] = 0.25 * (1 - s); dNr[2] = 0.25 * (1 + s); dNr[3] = -0.25 * (1 + s);
+    dNs[0] = -0.25 * (1 - r); dNs[1] = -0.25 * (1 + r); dNs[2] = 0.25 * (1 + r); dNs[3] = 0.25 * (1 - r);
+}
+__device__ __forceinline__ void jacobian(const double *lc, const double *dNr, const double *dNs, double &J11,
+                                         double &J12, double &J21, double &J22) {
+    J11 = dNr[0] * lc[0] + dNr[1] * lc[2] + dNr[2] * lc[4] + dNr[3] * lc[6];
+    J12 = dNr[0] * lc[1] + dNr[1] * lc[3] + dNr[2] * lc[5] + dNr[3] * lc[7];
+    J21 = dNs[0] * lc[0] + dNs[1] * lc[2] + dNs[2] * lc[4] + dNs[3] * lc[6];
+    J22 = dNs[0] * lc[1] + dNs[1] * lc[3] + dNs[2] * lc[5] + dNs[3] * lc[7];
+}
+
+// 4x4 inverse by Gauss-Jordan with partial pivoting (single lane).
+__device__ void inv4(const double *a, double *ai) {
+    double m[4][8];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) { m[i][j] = a[i * 4 + j]; m[i][4 + j] = (i == j) ? 1.0 : 0.0; }
+    for (int c = 0; c < 4; c++) {
+        int p = c;
+        double best = fabs(m[c][c]);
+        for (int r = c + 1; r < 4; r++)
+            if (fabs(m[r][c]) > best) { best = fabs(m[r][c]); p = r; }
+        if (p != c)
+            for (int j = 0; j < 8; j++) { double t = m[c][j]; m[c][j] = m[p][j]; m[p][j] = t; }
+        double d = m[c][c];
+        for (int j = 0; j < 8; j++) m[c][j] /= d;
+        for (int r = 0; r < 4; r++) {
+            if (r == c) continue;
+            double f = m[r][c];
+            for (int j = 0; j < 8; j++) m[r][j] -= f * m[c][j];
+        }
+    }
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) ai[i * 4 + j] = m[i][4 + j];
+}
+
+// Per-element LDS staging of the CQUAD4 kernels.
+struct Q4Stage {
+    double lc[8];
+    double R[9];
+    double gdx[4][4], gdy[4][4], gN[4][4];  // per Gauss point, per corner
+    double gadj[4];                          // |det J| (clamped at 1e-12)
+    double gdp[4][4];                        // dphi1/dx, dphi1/dy, dphi2/dx, dphi2/dy
+    double dNdx_c[4], dNdy_c[4];
+    double Kab[24][4];                       // membrane incompatible coupling
+    double Kabb[24][4];                      // bending incompatible coupling
+    double Kbb[16], Kbbb[16];
+    double Kbbi[16], Kbbbi[16];
+    double D[4][12];                         // MacNeal shear strain-displacement
+    double Jp[4];
+    double Zi[16];                           // (Zs + Zb)^-1
+    double Zs[16], Zb[16];
+    double ZiD[4][12];
+    double out[576];                         // 24x24 staging (element-block order)
+};
+
+// membrane B column (3-vector) of element DOF a at Gauss point g
+__device__ __forceinline__ void bm_col(const Q4Stage &S, int g, int a, double *b) {
+    int k = a / 6, c = a % 6;
+    double dx = S.gdx[g][k], dy = S.gdy[g][k];
+    b[0] = (c == 0) ? dx : 0.0;
+    b[1] = (c == 1) ? dy : 0.0;
+    b[2] = (c == 0) ? dy : ((c == 1) ? dx : 0.0);
+}
+// bending B column with the MacNeal twist row
+__device__ __forceinline__ void bb_col(const Q4Stage &S, int g, int a, double *b) {
+    int k = a / 6, c = a % 6;
+    double dx = S.gdx[g][k], dy = S.gdy[g][k];
+    if (c == 4) { b[0] = dx; b[1] = 0.0; b[2] = 2.0 * dy - S.dNdy_c[k]; }
+    else if (c == 3) { b[0] = 0.0; b[1] = -dy; b[2] = 2.0 * (-dx) - (-S.dNdx_c[k]); }
+    else { b[0] = 0.0; b[1] = 0.0; b[2] = 0.0; }
+}
+__device__ __forceinline__ double bd_val(const Q4Stage &S, int g, int a) {
+    int k = a / 6, c = a % 6;
+    if (c == 0) return 0.5 * S.gdy[g][k];
+    if (c == 1) return -0.5 * S.gdx[g][k];
+    if (c == 5) return S.gN[g][k];
+    return 0.0;
+}
+// Wilson membrane mode column m
+__device__ __forceinline__ void bi_col(const Q4Stage &S, int g, int m, double *b) {
+    double p1x = S.gdp[g][0], p1y = S.gdp[g][1], p2x = S.gdp[g][2], p2y = S.gdp[g][3];
+    switch (m) {
+        case 0: b[0] = p1x; b[1] = 0.0; b[2] = p1y; break;
+        case 1: b[0] = 0.0; b[1] = p1y; b[2] = p1x; break;
+        case 2: b[0] = p2x; b[1] = 0.0; b[2] = p2y; break;
+        default: b[0] = 0.0; b[1] = p2y; b[2] = p2x; break;
+    }
+}
+// bending incompatible mode column m
+__device__ __forceinline__ void bib_col(const Q4Stage &S, int g, int m, double *b) {
+    double p1x = S.gdp[g][0], p1y = S.gdp[g][1], p2x = S.gdp[g][2], p2y = S.gdp[g][3];
+    switch (m) {
+        case 0: b[0] = 0.0; b[1] = -p1y; b[2] = -p1x; break;
+        case 1: b[0] = 0.0; b[1] = -p2y; b[2] = -p2x; break;
+        case 2: b[0] = p1x; b[1] = 0.0; b[2] = p1y; break;
+        default: b[0] = p2x; b[1] = 0.0; b[2] = p2y; break;
+    }
+}
+__device__ __forceinline__ void mat3v(const double *C, const double *b, double *o) {
+    o[0] = C[0] * b[0] + C[1] * b[1] + C[2] * b[2];
+    o[1] = C[3] * b[0] + C[4] * b[1] + C[5] * b[2];
+    o[2] = C[6] * b[0] + C[7] * b[1] + C[8] * b[2];
+}
+
+struct ShellProps {
+    double Cm[9], Cb[9], gs;  // Cs = gs * I (isotropic)
+    double h, E, alpha_drill;
+    bool bend;                // 12I/T^3 > 1e-12
+};
+
+__device__ void shell_props(const double *prop, double k6rot, ShellProps &P) {
+    double E = prop[0], nu = prop[1], h = prop[2], br = prop[3], tst = prop[4], Gs = prop[5];
+    double cm = E * h / (1 - nu * nu);
+    double cb = br * (E * h * h * h) / (12 * (1 - nu * nu));
+    const double D[9] = {1.0, nu, 0.0, nu, 1.0, 0.0, 0.0, 0.0, (1 - nu) / 2};
+    for (int i = 0; i < 9; i++) { P.Cm[i] = cm * D[i]; P.Cb[i] = cb * D[i]; }
+    P.bend = br > 1e-12;
+    P.gs = P.bend ? tst * h * Gs : 0.0;
+    P.h = h;
+    P.E = E;
+    double ek6 = P.bend ? k6rot : 0.0;
+    double eds = P.bend ? 1.0 : 0.0;
+    double Gd = P.Cm[8] / h;
+    if (Gd < 1e-6) Gd = E / (2 * 3.0);
+    P.alpha_drill = eds * (ek6 / 1e5) * Gd * h;
+}
+
+// Stage Gauss-point data (lanes 0-3), centre derivatives (lane 4) and MacNeal
+// sample points (lanes 8-11).
+__device__ void q4_stage_geometry(Q4Stage &S, int lane) {
+    if (lane < 4) {
+        const double gps[4][2] = {{-PT, -PT}, {PT, -PT}, {PT, PT}, {-PT, PT}};
+        double r = gps[lane][0], s = gps[lane][1];
+        double dNr[4], dNs[4], J11, J12, J21, J22;
+        shape_derivs(r, s, dNr, dNs);
+        jacobian(S.lc, dNr, dNs, J11, J12, J21, J22);
+        double detJ = J11 * J22 - J12 * J21;
+        double adj = fabs(detJ);
+        if (adj < 1e-12) adj = 1e-12;
+        double id = 1.0 / detJ;
+        double iJ11 = J22 * id, iJ12 = -J12 * id, iJ21 = -J21 * id, iJ22 = J11 * id;
+        for (int k = 0; k < 4; k++) {
+            S.gdx[lane][k] = iJ11 * dNr[k] + iJ12 * dNs[k];
+            S.gdy[lane][k] = iJ21 * dNr[k] + iJ22 * dNs[k];
+            S.gN[lane][k] = 0.25 * (1 + ((k == 1 || k == 2) ? r : -r)) * (1 + (k >= 2 ? s : -s));
+        }
+        S.gadj[lane] = adj;
+        S.gdp[lane][0] = iJ11 * (-2.0 * r);
+        S.gdp[lane][1] = iJ21 * (-2.0 * r);
+        S.gdp[lane][2] = iJ12 * (-2.0 * s);
+        S.gdp[lane][3] = iJ22 * (-2.0 * s);
+    } else if (lane == 4) {
+        const double dNr_c[4] = {-0.25, 0.25, 0.25, -0.25}, dNs_c[4] = {-0.25, -0.25, 0.25, 0.25};
+        double J11, J12, J21, J22;
+        jacobian(S.lc, dNr_c, dNs_c, J11, J12, J21, J22);
+        double id = 1.0 / (J11 * J22 - J12 * J21);
+        double iJ11 = J22 * id, iJ12 = -J12 * id, iJ21 = -J21 * id, iJ22 = J11 * id;
+        for (int k = 0; k < 4; k++) {
+            S.dNdx_c[k] = iJ11 * dNr_c[k] + iJ12 * dNs_c[k];
+            S.dNdy_c[k] = iJ21 * dNr_c[k] + iJ22 * dNs_c[k];
+        }
+    } else if (lane >= 8 && lane < 12) {
+        int p = lane - 8;
+        const double sp[4][2] = {{0.0, -PT}, {0.0, PT}, {-PT, 0.0}, {PT, 0.0}};
+        double xi = sp[p][0], eta = sp[p][1];
+        double dNr[4], dNs[4], J11, J12, J21, J22;
+        shape_derivs(xi, eta, dNr, dNs);
+        jacobian(S.lc, dNr, dNs, J11, J12, J21, J22);
+        double detJ = J11 * J22 - J12 * J21;
+        if (fabs(detJ) < 1e-14) detJ = detJ < 0.0 ? -1e-14 : 1e-14;
+        S.Jp[p] = fabs(detJ);
+        double id = 1.0 / detJ;
+        double iJ11 = J22 * id, iJ12 = -J12 * id, iJ21 = -J21 * id, iJ22 = J11 * id;
+        double N[4] = {0.25 * (1 - xi) * (1 - eta), 0.25 * (1 + xi) * (1 - eta), 0.25 * (1 + xi) * (1 + eta),
+                       0.25 * (1 - xi) * (1 + eta)};
+        for (int k = 0; k < 4; k++) {
+            double dx = iJ11 * dNr[k] + iJ12 * dNs[k];
+            double dy = iJ21 * dNr[k] + iJ22 * dNs[k];
+            if (p < 2) { S.D[p][3 * k] = dx; S.D[p][3 * k + 1] = 0.0; S.D[p][3 * k + 2] = N[k]; }
+            else { S.D[p][3 * k] = dy; S.D[p][3 * k + 1] = -N[k]; S.D[p][3 * k + 2] = 0.0; }
+        }
+    }
+}
+
+// MacNeal compliance Zs, Zb and Zi = (Zs + Zb)^-1 (single lane).
+__device__ void q4_macneal_z(Q4Stage &S, const ShellProps &P) {
+    const double *lc = S.lc;
+    double Dx = 0.5 * (lc[2] + lc[4] - lc[0] - lc[6]);
+    double Dy = 0.5 * (lc[5] + lc[7] - lc[1] - lc[3]);
+    double Dx2 = Dx * Dx, Dy2 = Dy * Dy;
+    const double dNr_c[4] = {-0.25, 0.25, 0.25, -0.25}, dNs_c[4] = {-0.25, -0.25, 0.25, 0.25};
+    double J11, J12, J21, J22;
+    jacobian(lc, dNr_c, dNs_c, J11, J12, J21, J22);
+    double A_elem = 4.0 * fabs(J11 * J22 - J12 * J21);
+    const double eps = 0.04, zb = 0.65;
+    double a_par = eps / (eps + (1.0 - eps) * Dx2 / fmax(Dy2, 1e-30));
+    double b_par = eps / (eps + (1.0 - eps) * Dy2 / fmax(Dx2, 1e-30));
+    double flex_x = 1.0 / fmax(fabs(P.Cb[0]), 1e-30);
+    double flex_y = 1.0 / fmax(fabs(P.Cb[4]), 1e-30);
+    double inv12A = 1.0 / (12.0 * A_elem);
+    double zbx_u = zb * inv12A * Dx2 * flex_x;
+    double zbx_d = zb * inv12A * a_par * Dx2 * flex_x;
+    double zby_u = zb * inv12A * Dy2 * flex_y;
+    double zby_d = zb * inv12A * b_par * Dy2 * flex_y;
+    for (int i = 0; i < 16; i++) S.Zb[i] = 0.0;
+    S.Zb[0] = zbx_u + zbx_d; S.Zb[1] = zbx_u - zbx_d; S.Zb[4] = S.Zb[1]; S.Zb[5] = S.Zb[0];
+    S.Zb[10] = zby_u + zby_d; S.Zb[11] = zby_u - zby_d; S.Zb[14] = S.Zb[11]; S.Zb[15] = S.Zb[10];
+    // [V G V] is diagonal for an isotropic Cs (G_xy = 0)
+    double VGV[16];
+    for (int i = 0; i < 16; i++) VGV[i] = 0.0;
+    for (int i = 0; i < 4; i++) {
+        double Jf = sqrt(2.0 * S.Jp[i]) * sqrt(2.0 * S.Jp[i]);
+        double v = Jf * P.gs;
+        VGV[i * 5] = v < 1e-30 ? 1e-30 : v;
+    }
+    double Zs[16];
+    inv4(VGV, Zs);
+    double Z[16];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            double zs = 0.5 * (Zs[i * 4 + j] + Zs[j * 4 + i]);
+            S.Zs[i * 4 + j] = zs;
+        }
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            double a = S.Zs[i * 4 + j] + S.Zb[i * 4 + j];
+            double b = S.Zs[j * 4 + i] + S.Zb[j * 4 + i];
+            Z[i * 4 + j] = 0.5 * (a + b);
+        }
+    inv4(Z, S.Zi);
+}
+
+// Rotate a local 3x3 block to the global frame: out = R^T blk R.
+__device__ __forceinline__ void rot_block(const double *R, const double *blk, double *out) {
+    double tmp[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) tmp[i * 3 + j] = blk[i * 3] * R[j] + blk[i * 3 + 1] * R[3 + j] + blk[i * 3 + 2] * R[6 + j];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) out[i * 3 + j] = R[i] * tmp[j] + R[3 + i] * tmp[3 + j] + R[6 + i] * tmp[6 + j];
+}
+
+// position of entry (a, b) of a 24x24 element matrix in the 16 x 36 block layout
+__device__ __forceinline__ int eb_pos(int a, int b) {
+    return ((a / 6) * 4 + (b / 6)) * 36 + (a % 6) * 6 + (b % 6);
+}
+
+// Stores S.out (576 values in element-block order) coalesced.
+__device__ __forceinline__ void store_576(const Q4Stage &S, double *dst, int lane) {
+    for (int t = lane; t < 576; t += S105_WAVE) dst[t] = S.out[t];
+}
+
+// Sum over the element's 2x2 Gauss points of the Ke entry (a, b) parts that
+// do not involve condensation or shear: membrane + bending + drilling.
+// Components are returned separately (needed by the thickness derivative).
+__device__ void q4_gauss_entry(const Q4Stage &S, const ShellProps &P, int a, int b, double &km, double &kb,
+                               double &kd) {
+    km = kb = kd = 0.0;
+    for (int g = 0; g < 4; g++) {
+        double adj = S.gadj[g];
+        double ba[3], bb[3], t[3];
+        bm_col(S, g, a, ba);
+        bm_col(S, g, b, bb);
+        mat3v(P.Cm, bb, t);
+        double vm = ba[0] * t[0] + ba[1] * t[1] + ba[2] * t[2];
+        bb_col(S, g, a, ba);
+        bb_col(S, g, b, bb);
+        mat3v(P.Cb, bb, t);
+        double vb = ba[0] * t[0] + ba[1] * t[1] + ba[2] * t[2];
+        km += adj * vm;
+        kb += adj * vb;
+        kd += adj * P.alpha_drill * bd_val(S, g, a) * bd_val(S, g, b);
+    }
+}
+
+// Condensation corrections (membrane, bending) of entry (a, b).
+__device__ __forceinline__ void q4_condense_entry(const Q4Stage &S, int a, int b, bool bend_c, double &cm,
+                                                  double &cb) {
+    cm = 0.0;
+    cb = 0.0;
+    for (int l = 0; l < 4; l++) {
+        double tm = 0.0, tb = 0.0;
+        for (int q = 0; q < 4; q++) {
+            tm += S.Kbbi[l * 4 + q] * S.Kab[b][q];
+            tb += S.Kbbbi[l * 4 + q] * S.Kabb[b][q];
+        }
+        cm += S.Kab[a][l] * tm;
+        cb += S.Kabb[a][l] * tb;
+    }
+    if (!bend_c) cb = 0.0;
+}
+
+// plate DOF index (w, thx, thy per node) of element DOF a, or -1
+__device__ __forceinline__ int plate_idx(int a) {
+    int c = a % 6;
+    return (c >= 2 && c <= 4) ? 3 * (a / 6) + (c - 2) : -1;
+}
+
+// Common per-element set-up of the Ke / sensitivity kernels (all lanes of the
+// element's wavefront participate; contains the workgroup barriers).
+__device__ bool q4_setup(Q4Stage &S, ShellProps &P, const ElemArgs &A, int e, int lane, bool &bend_c,
+                         bool &has_shear) {
+    bool valid = e < A.n_q4;
+    double p[12];
+    if (valid) {
+        for (int k = 0; k < 4; k++) {
+            int nd = A.q4[4 * e + k];
+            for (int i = 0; i < 3; i++) p[3 * k + i] = A.xyz[3 * nd + i];
+        }
+        shell_props(A.q4prop + 6 * e, A.k6rot, P);
+        if (lane == 0) q4_frame(p, S.R, S.lc);
+    }
+    __syncthreads();
+    if (valid) q4_stage_geometry(S, lane);
+    __syncthreads();
+    if (valid) {
+        // couplings: 96 membrane + 96 bending entries, then 16 + 16 self terms
+        for (int t = lane; t < 224; t += S105_WAVE) {
+            if (t < 192) {
+                int bend = t >= 96;
+                int idx = bend ? t - 96 : t;
+                int a = idx / 4, m = idx % 4;
+                double s = 0.0;
+                for (int g = 0; g < 4; g++) {
+                    double ba[3], bi[3], cbi[3];
+                    if (bend) { bb_col(S, g, a, ba); bib_col(S, g, m, bi); mat3v(P.Cb, bi, cbi); }
+                    else { bm_col(S, g, a, ba); bi_col(S, g, m, bi); mat3v(P.Cm, bi, cbi); }
+                    s += S.gadj[g] * (ba[0] * cbi[0] + ba[1] * cbi[1] + ba[2] * cbi[2]);
+                }
+                if (bend) S.Kabb[a][m] = s; else S.Kab[a][m] = s;
+            } else {
+                int idx = t - 192;
+                int bend = idx >= 16;
+                int l = (idx % 16) / 4, m = idx % 4;
+                double s = 0.0;
+                for (int g = 0; g < 4; g++) {
+                    double bl[3], bm[3], cbm[3];
+                    if (bend) { bib_col(S, g, l, bl); bib_col(S, g, m, bm); mat3v(P.Cb, bm, cbm); }
+                    else { bi_col(S, g, l, bl); bi_col(S, g, m, bm); mat3v(P.Cm, bm, cbm); }
+                    s += S.gadj[g] * (bl[0] * cbm[0] + bl[1] * cbm[1] + bl[2] * cbm[2]);
+                }
+                if (bend) S.Kbbb[l * 4 + m] = s; else S.Kbb[l * 4 + m] = s;
+            }
+        }
+    }
+    __syncthreads();
+    has_shear = valid && P.h >= 1e-30 && P.gs >= 1e-30;
+    if (valid) {
+        if (lane == 0) inv4(S.Kbb, S.Kbbi);
+        if (lane == 1) inv4(S.Kbbb, S.Kbbbi);
+        if (lane == 2 && has_shear) q4_macneal_z(S, P);
+    }
+    double kmax = 0.0;
+    if (valid)
+        for (int i = 0; i < 16; i++) kmax = fmax(kmax, fabs(S.Kbbb[i]));
+    bend_c = kmax > 0.0;
+    __syncthreads();
+    if (valid && has_shear)
+        for (int t = lane; t < 48; t += S105_WAVE) {
+            int i = t / 12, c = t % 12;
+            double s = 0.0;
+            for (int l = 0; l < 4; l++) s += S.Zi[i * 4 + l] * S.D[l][c];
+            S.ZiD[i][c] = s;
+        }
+    __syncthreads();
+    return valid;
+}
+
+// Element stiffness of lane's 3x3 sub-block in the local frame.
+__device__ void q4_local_block(const Q4Stage &S, const ShellProps &P, int lane, bool bend_c, bool has_shear,
+                               double *blk) {
+    int bi = lane / 8, bj = lane % 8;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            int a = 3 * bi + i, b = 3 * bj + j;
+            double km, kb, kd, cm, cb;
+            q4_gauss_entry(S, P, a, b, km, kb, kd);
+            double v = km + kb + kd;
+            if (has_shear) {
+                int pa = plate_idx(a), pb = plate_idx(b);
+                if (pa >= 0 && pb >= 0) {
+                    double s1 = 0.0, s2 = 0.0;
+                    for (int l = 0; l < 4; l++) {
+                        s1 += S.D[l][pa] * S.ZiD[l][pb];
+                        s2 += S.D[l][pb] * S.ZiD[l][pa];
+                    }
+                    v += 0.5 * (s1 + s2);
+                }
+            }
+            q4_condense_entry(S, a, b, bend_c, cm, cb);
+            v -= cb;
+            v -= cm;
+            blk[i * 3 + j] = v;
+        }
+}
+
+__global__ void __launch_bounds__(256) q4_ke_kernel(ElemArgs A, double *eb) {
+    __shared__ Q4Stage stage[EPB];
+    int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
+    int e = blockIdx.x * EPB + w;
+    Q4Stage &S = stage[w];
+    ShellProps P;
+    bool bend_c, has_shear;
+    bool valid = q4_setup(S, P, A, e, lane, bend_c, has_shear);
+    if (valid) {
+        double blk[9], g[9];
+        q4_local_block(S, P, lane, bend_c, has_shear, blk);
+        rot_block(S.R, blk, g);
+        int bi = lane / 8, bj = lane % 8;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) S.out[eb_pos(3 * bi + i, 3 * bj + j)] = g[i * 3 + j];
+    }
+    __syncthreads();
+    if (valid) store_576(S, eb + (size_t)e * 576, lane);
+}
+
+// ---------------------------------------------------------------------------
+// CQUAD4 geometric stiffness
+// ---------------------------------------------------------------------------
+
+struct Q4KGStage {
+    double lc[8];
+    double R[9];
+    double ul[24];
+    double gdx[4][4], gdy[4][4], gadj[4], gdp[4][4];
+    double Kab[24][4];
+    double Kbb[16], Kbbi[16];
+    double Am[4][24];   // condensation map A = -Kbb^-1 Kab^T
+    double alpha[4];
+    double sig[4][3];
+    double pr[4][6];    // principal stresses per GP: l1 c1 s1 l2 c2 s2
+    double va[8][24];   // (s gx - c gy) per (gp, direction)
+    double vb[8][24];   // (c gzx + s gzy)
+    double wt[8];
+    double out[576];
+};
+
+__device__ void q4kg_gp_stage(Q4KGStage &S, int lane) {
+    const double gps[4][2] = {{-PT, -PT}, {PT, -PT}, {PT, PT}, {-PT, PT}};
+    double r = gps[lane][0], s = gps[lane][1];
+    double dNr[4], dNs[4], J11, J12, J21, J22;
+    shape_derivs(r, s, dNr, dNs);
+    jacobian(S.lc, dNr, dNs, J11, J12, J21, J22);
+    double detJ = J11 * J22 - J12 * J21;
+    double adj = fabs(detJ);
+    if (adj < 1e-12) adj = 1e-12;
+    double id = 1.0 / detJ;
+    double iJ11 = J22 * id, iJ12 = -J12 * id, iJ21 = -J21 * id, iJ22 = J11 * id;
+    for (int k = 0; k < 4; k++) {
+        S.gdx[lane][k] = iJ11 * dNr[k] + iJ12 * dNs[k];
+        S.gdy[lane][k] = iJ21 * dNr[k] + iJ22 * dNs[k];
+    }
+    S.gadj[lane] = adj;
+    S.gdp[lane][0] = iJ11 * (-2.0 * r);
+    S.gdp[lane][1] = iJ21 * (-2.0 * r);
+    S.gdp[lane][2] = iJ12 * (-2.0 * s);
+    S.gdp[lane][3] = iJ22 * (-2.0 * s);
+}
+
+// principal stress decomposition, radius <= 1e-30 returns the axes
+__device__ void principal2d(double sxx, double syy, double sxy, double *o) {
+    double mean = 0.5 * (sxx + syy), hd = 0.5 * (sxx - syy);
+    double rad = sqrt(hd * hd + sxy * sxy);
+    if (rad <= 1e-30) { o[0] = mean; o[1] = 1.0; o[2] = 0.0; o[3] = mean; o[4] = 0.0; o[5] = 1.0; return; }
+    double th = 0.5 * atan2(2.0 * sxy, sxx - syy);
+    double c = cos(th), s = sin(th);
+    o[0] = mean + rad; o[1] = c; o[2] = s; o[3] = mean - rad; o[4] = -s; o[5] = c;
+}
+
+// gradient vectors of (ux, uy, uz) along x/y at GP g for element DOF a,
+// including the condensed Wilson modes.
+__device__ __forceinline__ void q4kg_grads(const Q4KGStage &S, int g, int a, double *gr /*6*/) {
+    int k = a / 6, c = a % 6;
+    double dx = S.gdx[g][k], dy = S.gdy[g][k];
+    double uxx = (c == 0) ? dx : 0.0, uxy = (c == 0) ? dy : 0.0;
+    double uyx = (c == 1) ? dx : 0.0, uyy = (c == 1) ? dy : 0.0;
+    double uzx = (c == 2) ? dx : 0.0, uzy = (c == 2) ? dy : 0.0;
+    double p1x = S.gdp[g][0], p1y = S.gdp[g][1], p2x = S.gdp[g][2], p2y = S.gdp[g][3];
+    double a1 = S.Am[0][a], a2 = S.Am[1][a], a3 = S.Am[2][a], a4 = S.Am[3][a];
+    uxx += p1x * a1 + p2x * a3;
+    uxy += p1y * a1 + p2y * a3;
+    uyx += p1x * a2 + p2x * a4;
+    uyy += p1y * a2 + p2y * a4;
+    gr[0] = uxx; gr[1] = uxy; gr[2] = uyx; gr[3] = uyy; gr[4] = uzx; gr[5] = uzy;
+}
+
+// Shared set-up of the KG and sensitivity kernels: frame, local u, GP data,
+// condensation map, membrane stresses and principal directions.
+__device__ bool q4kg_setup(Q4KGStage &S, const ElemArgs &A, const double *u, int e, int lane, double &h) {
+    bool valid = e < A.n_q4;
+    double Cm[9];
+    h = 1.0;
+    if (valid) {
+        const double *prop = A.q4prop + 6 * e;
+        double E = prop[0], nu = prop[1];
+        h = prop[2];
+        double cm = E / (1 - nu * nu);
+        const double D[9] = {1.0, nu, 0.0, nu, 1.0, 0.0, 0.0, 0.0, (1 - nu) / 2};
+        for (int i = 0; i < 9; i++) Cm[i] = (cm * D[i]) * h;
+        if (lane == 0) {
+            double p[12];
+            for (int k = 0; k < 4; k++) {
+                int nd = A.q4[4 * e + k];
+                for (int i = 0; i < 3; i++) p[3 * k + i] = A.xyz[3 * nd + i];
+            }
+            q4_frame(p, S.R, S.lc);
+        }
+    }
+    __syncthreads();
+    if (valid) {
+        if (lane < 4) q4kg_gp_stage(S, lane);
+        if (lane >= 8 && lane < 16) {
+            int b = lane - 8;  // 3-vector block b of the element DOFs
+            int nd = A.q4[4 * e + b / 2];
+            const double *ug = u + 6 * (size_t)nd + 3 * (b % 2);
+            for (int i = 0; i < 3; i++) S.ul[3 * b + i] = S.R[i * 3] * ug[0] + S.R[i * 3 + 1] * ug[1] + S.R[i * 3 + 2] * ug[2];
+        }
+    }
+    __syncthreads();
+    if (valid) {
+        for (int t = lane; t < 112; t += S105_WAVE) {
+            if (t < 96) {
+                int a = t / 4, m = t % 4;
+                int k = a / 6, c = a % 6;
+                double s = 0.0;
+                for (int g = 0; g < 4; g++) {
+                    double dx = S.gdx[g][k], dy = S.gdy[g][k];
+                    double ba[3] = {(c == 0) ? dx : 0.0, (c == 1) ? dy : 0.0, (c == 0) ? dy : ((c == 1) ? dx : 0.0)};
+                    double p1x = S.gdp[g][0], p1y = S.gdp[g][1], p2x = S.gdp[g][2], p2y = S.gdp[g][3];
+                    double bi[3];
+                    switch (m) {
+                        case 0: bi[0] = p1x; bi[1] = 0.0; bi[2] = p1y; break;
+                        case 1: bi[0] = 0.0; bi[1] = p1y; bi[2] = p1x; break;
+                        case 2: bi[0] = p2x; bi[1] = 0.0; bi[2] = p2y; break;
+                        default: bi[0] = 0.0; bi[1] = p2y; bi[2] = p2x; break;
+                    }
+                    double cbi[3];
+                    mat3v(Cm, bi, cbi);
+                    s += (ba[0] * cbi[0] + ba[1] * cbi[1] + ba[2] * cbi[2]) * S.gadj[g];
+                }
+                S.Kab[a][m] = s;
+            } else {
+                int idx = t - 96;
+                int l = idx / 4, m = idx % 4;
+                double s = 0.0;
+                for (int g = 0; g < 4; g++) {
+                    double p1x = S.gdp[g][0], p1y = S.gdp[g][1], p2x = S.gdp[g][2], p2y = S.gdp[g][3];
+                    double cols[4][3] = {{p1x, 0.0, p1y}, {0.0, p1y, p1x}, {p2x, 0.0, p2y}, {0.0, p2y, p2x}};
+                    double cbm[3];
+                    mat3v(Cm, cols[m], cbm);
+                    s += (cols[l][0] * cbm[0] + cols[l][1] * cbm[1] + cols[l][2] * cbm[2]) * S.gadj[g];
+                }
+                S.Kbb[l * 4 + m] = s;
+            }
+        }
+    }
+    __syncthreads();
+    if (valid && lane == 0) inv4(S.Kbb, S.Kbbi);
+    __syncthreads();
+    if (valid)
+        for (int t = lane; t < 96; t += S105_WAVE) {
+            int l = t / 24, a = t % 24;
+            double s = 0.0;
+            for (int q = 0; q < 4; q++) s += S.Kbbi[l * 4 + q] * S.Kab[a][q];
+            S.Am[l][a] = -s;
+        }
+    __syncthreads();
+    if (valid && lane < 4) {
+        double s = 0.0;
+        for (int a = 0; a < 24; a++) s += S.Am[lane][a] * S.ul[a];
+        S.alpha[lane] = s;
+    }
+    __syncthreads();
+    if (valid && lane < 4) {
+        int g = lane;
+        double eps[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < 4; k++) {
+            double dx = S.gdx[g][k], dy = S.gdy[g][k];
+            eps[0] += dx * S.ul[6 * k];
+            eps[1] += dy * S.ul[6 * k + 1];
+            eps[2] += dy * S.ul[6 * k] + dx * S.ul[6 * k + 1];
+        }
+        double p1x = S.gdp[g][0], p1y = S.gdp[g][1], p2x = S.gdp[g][2], p2y = S.gdp[g][3];
+        eps[0] += p1x * S.alpha[0] + p2x * S.alpha[2];
+        eps[1] += p1y * S.alpha[1] + p2y * S.alpha[3];
+        eps[2] += p1y * S.alpha[0] + p1x * S.alpha[1] + p2y * S.alpha[2] + p2x * S.alpha[3];
+        double N[3];
+        mat3v(Cm, eps, N);
+        for (int i = 0; i < 3; i++) S.sig[g][i] = N[i] / h;
+        principal2d(S.sig[g][0], S.sig[g][1], S.sig[g][2], S.pr[g]);
+    }
+    __syncthreads();
+    return valid;
+}
+
+// va/vb vectors and weights for the 8 (gp, direction) rank-2 updates.
+__device__ void q4kg_vectors(Q4KGStage &S, int lane, double h) {
+    for (int t = lane; t < 8 * 24; t += S105_WAVE) {
+        int q = t / 24, a = t % 24;
+        int g = q / 2, k = q % 2;
+        double c = S.pr[g][3 * k + 1], s = S.pr[g][3 * k + 2];
+        double gr[6];
+        q4kg_grads(S, g, a, gr);
+        double gx = c * gr[0] + s * gr[1], gy = c * gr[2] + s * gr[3], gz = c * gr[4] + s * gr[5];
+        S.va[q][a] = s * gx - c * gy;
+        S.vb[q][a] = gz;
+    }
+    if (lane < 8) {
+        int g = lane / 2, k = lane % 2;
+        double lam = S.pr[g][3 * k];
+        S.wt[lane] = (fabs(lam) <= 1e-30) ? 0.0 : h * S.gadj[g] * lam;
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) q4_kg_kernel(ElemArgs A, const double *u, double *eb) {
+    __shared__ Q4KGStage stage[EPB];
+    int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
+    int e = blockIdx.x * EPB + w;
+    Q4KGStage &S = stage[w];
+    double h;
+    bool valid = q4kg_setup(S, A, u, e, lane, h);
+    if (valid) q4kg_vectors(S, lane, h);
+    else __syncthreads();
+    if (valid) {
+        int bi = lane / 8, bj = lane % 8;
+        double blk[9], g[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                int a = 3 * bi + i, b = 3 * bj + j;
+                double v = 0.0;
+                for (int q = 0; q < 8; q++) v += S.wt[q] * (S.va[q][a] * S.va[q][b] + S.vb[q][a] * S.vb[q][b]);
+                blk[i * 3 + j] = v;
+            }
+        rot_block(S.R, blk, g);
+        // drilling rows/columns (global component 6 of each node) are zeroed
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                bool drill = ((bi % 2) == 1 && i == 2) || ((bj % 2) == 1 && j == 2);
+                S.out[eb_pos(3 * bi + i, 3 * bj + j)] = drill ? 0.0 : g[i * 3 + j];
+            }
+    }
+    __syncthreads();
+    if (valid)
+        for (int t = lane; t < 576; t += S105_WAVE) eb[(size_t)e * 576 + t] = S.out[t];
+}
+
+// ---------------------------------------------------------------------------
+// CBAR (one thread per bar; 12x12 local matrices are analytic and sparse)
+// ---------------------------------------------------------------------------
+
+struct BarLocal {
+    double R[9];
+    double L, X, T, ay, by, cy, dy, az, bz, cz, dz;
+};
+
+__device__ void bar_setup(const ElemArgs &A, int b, BarLocal &B) {
+    const double *p1 = A.xyz + 3 * A.bar[2 * b], *p2 = A.xyz + 3 * A.bar[2 * b + 1];
+    double vx[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+    B.L = sqrt(dot3(vx, vx));
+    normalize3(vx);
+    double vr[3] = {A.bar_vref[3 * b], A.bar_vref[3 * b + 1], A.bar_vref[3 * b + 2]};
+    if (sqrt(dot3(vr, vr)) < 1e-6) {
+        vr[0] = 0.0; vr[1] = 0.0; vr[2] = 1.0;
+        if (fabs(dot3(vx, vr)) > 0.9) { vr[1] = 1.0; vr[2] = 0.0; }
+    }
+    double vz[3], vy[3];
+    cross3(vx, vr, vz);
+    normalize3(vz);
+    cross3(vz, vx, vy);
+    for (int i = 0; i < 3; i++) { B.R[i] = vx[i]; B.R[3 + i] = vy[i]; B.R[6 + i] = vz[i]; }
+    const double *bp = A.barprop + 8 * b;
+    double Ar = bp[0], I1 = bp[1], I2 = bp[2], J = bp[3], K1 = bp[4], K2 = bp[5], E = bp[6], G = bp[7];
+    double Iy = I2, Iz = I1;
+    if (Iy == 0.0) Iy = Iz;
+    if (Iz == 0.0) Iz = Iy;
+    double L = B.L;
+    bool ok = L >= 1e-9;
+    B.X = ok ? E * Ar / L : 0.0;
+    B.T = ok ? G * J / L : 0.0;
+    double Py = (K2 > 0.0) ? 12 * E * Iy / (G * (K2 * Ar) * L * L) : 0.0;
+    double Pz = (K1 > 0.0) ? 12 * E * Iz / (G * (K1 * Ar) * L * L) : 0.0;
+    B.ay = ok ? 12 * E * Iy / (L * L * L * (1 + Py)) : 0.0;
+    B.by = ok ? 6 * E * Iy / (L * L * (1 + Py)) : 0.0;
+    B.cy = ok ? (4 + Py) * E * Iy / (L * (1 + Py)) : 0.0;
+    B.dy = ok ? (2 - Py) * E * Iy / (L * (1 + Py)) : 0.0;
+    B.az = ok ? 12 * E * Iz / (L * L * L * (1 + Pz)) : 0.0;
+    B.bz = ok ? 6 * E * Iz / (L * L * (1 + Pz)) : 0.0;
+    B.cz = ok ? (4 + Pz) * E * Iz / (L * (1 + Pz)) : 0.0;
+    B.dz = ok ? (2 - Pz) * E * Iz / (L * (1 + Pz)) : 0.0;
+}
+
+// local Timoshenko stiffness entry (0-based i, j)
+__device__ double bar_k(const BarLocal &B, int i, int j) {
+    int I = i + 1, J = j + 1;
+    auto pair = [&](int a, int b) { return (I == a && J == b) || (I == b && J == a); };
+    if (pair(1, 1) || pair(7, 7)) return B.X;
+    if (pair(1, 7)) return -B.X;
+    if (pair(4, 4) || pair(10, 10)) return B.T;
+    if (pair(4, 10)) return -B.T;
+    if (pair(3, 3) || pair(9, 9)) return B.ay;
+    if (pair(3, 9)) return -B.ay;
+    if (pair(3, 5) || pair(3, 11)) return -B.by;
+    if (pair(9, 5) || pair(9, 11)) return B.by;
+    if (pair(5, 5) || pair(11, 11)) return B.cy;
+    if (pair(5, 11)) return B.dy;
+    if (pair(2, 2) || pair(8, 8)) return B.az;
+    if (pair(2, 8)) return -B.az;
+    if (pair(2, 6) || pair(2, 12)) return B.bz;
+    if (pair(8, 6) || pair(8, 12)) return -B.bz;
+    if (pair(6, 6) || pair(12, 12)) return B.cz;
+    if (pair(6, 12)) return B.dz;
+    return 0.0;
+}
+
+// local beam-column KG entry for axial force P
+__device__ double bar_kg(double L, double P, int i, int j) {
+    if (L < 1e-9 || fabs(P) < 1e-30) return 0.0;
+    double c1 = 6.0 * P / (5.0 * L), c2 = P / 10.0, c3 = 2.0 * P * L / 15.0, c4 = -P * L / 30.0;
+    int I = i + 1, J = j + 1;
+    auto pair = [&](int a, int b) { return (I == a && J == b) || (I == b && J == a); };
+    if (pair(2, 2) || pair(8, 8) || pair(3, 3) || pair(9, 9)) return c1;
+    if (pair(2, 8) || pair(3, 9)) return -c1;
+    if (pair(2, 6) || pair(2, 12)) return c2;
+    if (pair(6, 6) || pair(12, 12) || pair(5, 5) || pair(11, 11)) return c3;
+    if (pair(6, 12) || pair(5, 11)) return c4;
+    if (pair(8, 6) || pair(8, 12)) return -c2;
+    if (pair(3, 5) || pair(3, 11)) return -c2;
+    if (pair(9, 5) || pair(9, 11)) return c2;
+    return 0.0;
+}
+
+__global__ void bar_ke_kernel(ElemArgs A, double *eb) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= A.n_bar) return;
+    BarLocal B;
+    bar_setup(A, b, B);
+    double *dst = eb + (size_t)b * 144;
+    for (int bi = 0; bi < 4; bi++)
+        for (int bj = 0; bj < 4; bj++) {
+            double blk[9], g[9];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) blk[i * 3 + j] = bar_k(B, 3 * bi + i, 3 * bj + j);
+            rot_block(B.R, blk, g);
+            int na = bi / 2, nb = bj / 2;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    dst[(na * 2 + nb) * 36 + ((bi % 2) * 3 + i) * 6 + (bj % 2) * 3 + j] = g[i * 3 + j];
+        }
+}
+
+__device__ double bar_axial(const ElemArgs &A, const BarLocal &B, int b, const double *u) {
+    double ul[12];
+    for (int blk = 0; blk < 4; blk++) {
+        int nd = A.bar[2 * b + blk / 2];
+        const double *ug = u + 6 * (size_t)nd + 3 * (blk % 2);
+        for (int i = 0; i < 3; i++)
+            ul[3 * blk + i] = B.R[i * 3] * ug[0] + B.R[i * 3 + 1] * ug[1] + B.R[i * 3 + 2] * ug[2];
+    }
+    double P = 0.0;
+    for (int j = 0; j < 12; j++) P += bar_k(B, 6, j) * ul[j];
+    return P;
+}
+
+__global__ void bar_kg_kernel(ElemArgs A, const double *u, double *eb) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= A.n_bar) return;
+    BarLocal B;
+    bar_setup(A, b, B);
+    double P = bar_axial(A, B, b, u);
+    double *dst = eb + (size_t)b * 144;
+    for (int bi = 0; bi < 4; bi++)
+        for (int bj = 0; bj < 4; bj++) {
+            double blk[9], g[9];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) blk[i * 3 + j] = bar_kg(B.L, P, 3 * bi + i, 3 * bj + j);
+            rot_block(B.R, blk, g);
+            int na = bi / 2, nb = bj / 2;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    dst[(na * 2 + nb) * 36 + ((bi % 2) * 3 + i) * 6 + (bj % 2) * 3 + j] = g[i * 3 + j];
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic block gather: K[blk][t] = sum of contributing element blocks
+// in element order (no atomics: the summation order is fixed run to run).
+// ---------------------------------------------------------------------------
+__global__ void block_gather_kernel(int64_t n_blocks, const int32_t *cptr, const int32_t *cidx,
+                                    const double *eb, double *vals) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n_blocks * 36) return;
+    int64_t blk = gid / 36;
+    int t = (int)(gid % 36);
+    double s = 0.0;
+    for (int k = cptr[blk]; k < cptr[blk + 1]; k++) s += eb[(size_t)cidx[k] * 36 + t];
+    vals[gid] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Thickness sensitivity of one buckling mode (per CQUAD4):
+//   dlam/dt_e = -(phi' dKe phi + lam phi' dKGe|u phi - psi' dKe u) / (phi' KG phi)
+// with dKe/dt = (Km + Kd)/t + 3 Kb/t + D' Z^-1 (Zs/t + 3 Zb/t) Z^-1 D
+// (membrane and drilling terms scale with t, bending with t^3, the MacNeal
+// shear compliances with 1/t and 1/t^3) and dKGe/dt|u = KGe/t.
+// Restates the quantities of JFEM/src/solver/adjoint_buckling.jl:2406-2637
+// for one design variable per element, differentiating the forward kernel.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// element DOF a of element e in the global DOF vector
+__device__ __forceinline__ double q4_gvec(const ElemArgs &A, const double *x, int e, int a) {
+    return x[6 * (size_t)A.q4[4 * e + a / 6] + a % 6];
+}
+
+__global__ void __launch_bounds__(256) q4_dke_kernel(ElemArgs A, const double *u, const double *phi,
+                                                     const double *psi, double *out2 /* n_q4 x 2 */) {
+    __shared__ Q4Stage stage[EPB];
+    int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
+    int e = blockIdx.x * EPB + w;
+    Q4Stage &S = stage[w];
+    ShellProps P;
+    bool bend_c, has_shear;
+    bool valid = q4_setup(S, P, A, e, lane, bend_c, has_shear);
+    double f1 = 0.0, f2 = 0.0;
+    if (valid) {
+        double t = P.h;
+        int bi = lane / 8, bj = lane % 8;
+        double blk[9], g[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                int a = 3 * bi + i, b = 3 * bj + j;
+                double km, kb, kd, cm, cb;
+                q4_gauss_entry(S, P, a, b, km, kb, kd);
+                q4_condense_entry(S, a, b, bend_c, cm, cb);
+                double v = (km - cm + kd) / t + 3.0 * (kb - cb) / t;
+                if (has_shear) {
+                    int pa = plate_idx(a), pb = plate_idx(b);
+                    if (pa >= 0 && pb >= 0) {
+                        // D' Zi (Zs/t + 3 Zb/t) Zi D, symmetrised like the forward block
+                        double za[4], zb[4];
+                        for (int l = 0; l < 4; l++) { za[l] = S.ZiD[l][pa]; zb[l] = S.ZiD[l][pb]; }
+                        double s = 0.0;
+                        for (int l = 0; l < 4; l++)
+                            for (int q = 0; q < 4; q++)
+                                s += za[l] * (S.Zs[l * 4 + q] / t + 3.0 * S.Zb[l * 4 + q] / t) * zb[q];
+                        v += s;
+                    }
+                }
+                blk[i * 3 + j] = v;
+            }
+        rot_block(S.R, blk, g);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double pa = q4_gvec(A, phi, e, 3 * bi + i), pb = q4_gvec(A, phi, e, 3 * bj + j);
+                double sa = q4_gvec(A, psi, e, 3 * bi + i), ub = q4_gvec(A, u, e, 3 * bj + j);
+                f1 += pa * g[i * 3 + j] * pb;
+                f2 += sa * g[i * 3 + j] * ub;
+            }
+    }
+    f1 = wave_sum(f1);
+    f2 = wave_sum(f2);
+    if (valid && lane == 0) { out2[2 * e] = f1; out2[2 * e + 1] = f2; }
+}
+
+// phi' KGe phi / t per element (the dKG/dt|u term)
+__global__ void __launch_bounds__(256) q4_kgt_kernel(ElemArgs A, const double *u, const double *phi,
+                                                     double *out /* n_q4 */) {
+    __shared__ Q4KGStage stage[EPB];
+    int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
+    int e = blockIdx.x * EPB + w;
+    Q4KGStage &S = stage[w];
+    double h;
+    bool valid = q4kg_setup(S, A, u, e, lane, h);
+    if (valid) q4kg_vectors(S, lane, h);
+    else __syncthreads();
+    double f = 0.0;
+    if (valid) {
+        int bi = lane / 8, bj = lane % 8;
+        double blk[9], g[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                int a = 3 * bi + i, b = 3 * bj + j;
+                double v = 0.0;
+                for (int q = 0; q < 8; q++) v += S.wt[q] * (S.va[q][a] * S.va[q][b] + S.vb[q][a] * S.vb[q][b]);
+                blk[i * 3 + j] = v;
+            }
+        rot_block(S.R, blk, g);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                bool drill = ((bi % 2) == 1 && i == 2) || ((bj % 2) == 1 && j == 2);
+                if (drill) continue;
+                f += q4_gvec(A, phi, e, 3 * bi + i) * g[i * 3 + j] * q4_gvec(A, phi, e, 3 * bj + j);
+            }
+    }
+    f = wave_sum(f);
+    if (valid && lane == 0) out[e] = f / h;
+}
+
+// Forward-mode dual number with the three membrane-stress directions.
+struct Dual3 {
+    double v, d[3];
+};
+__device__ __forceinline__ Dual3 dconst(double v) { return Dual3{v, {0.0, 0.0, 0.0}}; }
+__device__ __forceinline__ Dual3 operator+(const Dual3 &a, const Dual3 &b) {
+    return Dual3{a.v + b.v, {a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2]}};
+}
+__device__ __forceinline__ Dual3 operator-(const Dual3 &a, const Dual3 &b) {
+    return Dual3{a.v - b.v, {a.d[0] - b.d[0], a.d[1] - b.d[1], a.d[2] - b.d[2]}};
+}
+__device__ __forceinline__ Dual3 operator*(const Dual3 &a, const Dual3 &b) {
+    return Dual3{a.v * b.v, {a.d[0] * b.v + a.v * b.d[0], a.d[1] * b.v + a.v * b.d[1], a.d[2] * b.v + a.v * b.d[2]}};
+}
+__device__ __forceinline__ Dual3 operator*(double s, const Dual3 &a) {
+    return Dual3{s * a.v, {s * a.d[0], s * a.d[1], s * a.d[2]}};
+}
+__device__ __forceinline__ Dual3 dsqrt(const Dual3 &a) {
+    double r = sqrt(a.v);
+    double f = r > 0.0 ? 0.5 / r : 0.0;
+    return Dual3{r, {f * a.d[0], f * a.d[1], f * a.d[2]}};
+}
+__device__ __forceinline__ Dual3 datan2(const Dual3 &y, const Dual3 &x) {
+    double den = x.v * x.v + y.v * y.v;
+    double fx = den > 0.0 ? -y.v / den : 0.0, fy = den > 0.0 ? x.v / den : 0.0;
+    return Dual3{atan2(y.v, x.v), {fx * x.d[0] + fy * y.d[0], fx * x.d[1] + fy * y.d[1], fx * x.d[2] + fy * y.d[2]}};
+}
+__device__ __forceinline__ Dual3 dcos(const Dual3 &a) {
+    double s = -sin(a.v);
+    return Dual3{cos(a.v), {s * a.d[0], s * a.d[1], s * a.d[2]}};
+}
+__device__ __forceinline__ Dual3 dsin(const Dual3 &a) {
+    double c = cos(a.v);
+    return Dual3{sin(a.v), {c * a.d[0], c * a.d[1], c * a.d[2]}};
+}
+
+// Adjoint right-hand side per element: d(phi' KGe(u) phi)/du_e (24, global
+// frame), differentiated through the principal-stress decomposition.
+__global__ void __launch_bounds__(256) q4_adjrhs_kernel(ElemArgs A, const double *u, const double *phi, double lam,
+                                                        double *eg /* n_q4 x 24 */) {
+    __shared__ Q4KGStage stage[EPB];
+    __shared__ double dphi_s[EPB][4][3];  // dPhi_gp/dsigma
+    __shared__ double dsdu[EPB][4][3][24];
+    __shared__ double phil[EPB][24];
+    int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
+    int e = blockIdx.x * EPB + w;
+    Q4KGStage &S = stage[w];
+    double h;
+    bool valid = q4kg_setup(S, A, u, e, lane, h);
+    // phi in the local frame with the global drilling component removed
+    if (valid && lane < 8) {
+        int b = lane;
+        int nd = A.q4[4 * e + b / 2];
+        double pg[3];
+        for (int i = 0; i < 3; i++) pg[i] = phi[6 * (size_t)nd + 3 * (b % 2) + i];
+        if (b % 2 == 1) pg[2] = 0.0;
+        for (int i = 0; i < 3; i++) phil[w][3 * b + i] = S.R[i * 3] * pg[0] + S.R[i * 3 + 1] * pg[1] + S.R[i * 3 + 2] * pg[2];
+    }
+    __syncthreads();
+    if (valid && lane < 4) {
+        int g = lane;
+        // gradients of phi at this GP (incl. condensed Wilson modes)
+        double X1 = 0, X2 = 0, Y1 = 0, Y2 = 0, Z1 = 0, Z2 = 0;
+        for (int a = 0; a < 24; a++) {
+            double gr[6];
+            q4kg_grads(S, g, a, gr);
+            double pa = phil[w][a];
+            X1 += gr[0] * pa; X2 += gr[1] * pa; Y1 += gr[2] * pa; Y2 += gr[3] * pa; Z1 += gr[4] * pa; Z2 += gr[5] * pa;
+        }
+        Dual3 sxx = Dual3{S.sig[g][0], {1.0, 0.0, 0.0}};
+        Dual3 syy = Dual3{S.sig[g][1], {0.0, 1.0, 0.0}};
+        Dual3 sxy = Dual3{S.sig[g][2], {0.0, 0.0, 1.0}};
+        Dual3 mean = 0.5 * (sxx + syy), hd = 0.5 * (sxx - syy);
+        Dual3 rad = dsqrt(hd * hd + sxy * sxy);
+        Dual3 l[2], c[2], s[2];
+        if (rad.v <= 1e-30) {
+            l[0] = mean; l[1] = mean;
+            c[0] = dconst(1.0); s[0] = dconst(0.0);
+            c[1] = dconst(0.0); s[1] = dconst(1.0);
+        } else {
+            Dual3 th = 0.5 * datan2(2.0 * sxy, sxx - syy);
+            Dual3 cc = dcos(th), ss = dsin(th);
+            l[0] = mean + rad; c[0] = cc; s[0] = ss;
+            l[1] = mean - rad; c[1] = dconst(0.0) - ss; s[1] = cc;
+        }
+        Dual3 Phi = dconst(0.0);
+        for (int k = 0; k < 2; k++) {
+            if (fabs(l[k].v) <= 1e-30) continue;
+            Dual3 gx = c[k] * dconst(X1) + s[k] * dconst(X2);
+            Dual3 gy = c[k] * dconst(Y1) + s[k] * dconst(Y2);
+            Dual3 gz = c[k] * dconst(Z1) + s[k] * dconst(Z2);
+            Dual3 a = s[k] * gx - c[k] * gy;
+            Phi = Phi + l[k] * (a * a + gz * gz);
+        }
+        double sc = h * S.gadj[g];
+        for (int i = 0; i < 3; i++) dphi_s[w][g][i] = sc * Phi.d[i];
+    }
+    // dsigma_gp/du_local (3 x 24 per GP): Cm (Bm + Bi A) / h
+    if (valid) {
+        const double *prop = A.q4prop + 6 * e;
+        double E = prop[0], nu = prop[1];
+        double cm = E / (1 - nu * nu);
+        const double D[9] = {1.0, nu, 0.0, nu, 1.0, 0.0, 0.0, 0.0, (1 - nu) / 2};
+        double Cm[9];
+        for (int i = 0; i < 9; i++) Cm[i] = (cm * D[i]) * h;
+        for (int t = lane; t < 96; t += S105_WAVE) {
+            int g = t / 24, a = t % 24;
+            int k = a / 6, cpt = a % 6;
+            double dx = S.gdx[g][k], dy = S.gdy[g][k];
+            double eps[3] = {(cpt == 0) ? dx : 0.0, (cpt == 1) ? dy : 0.0, (cpt == 0) ? dy : ((cpt == 1) ? dx : 0.0)};
+            double p1x = S.gdp[g][0], p1y = S.gdp[g][1], p2x = S.gdp[g][2], p2y = S.gdp[g][3];
+            double a1 = S.Am[0][a], a2 = S.Am[1][a], a3 = S.Am[2][a], a4 = S.Am[3][a];
+            eps[0] += p1x * a1 + p2x * a3;
+            eps[1] += p1y * a2 + p2y * a4;
+            eps[2] += p1y * a1 + p1x * a2 + p2y * a3 + p2x * a4;
+            double N[3];
+            mat3v(Cm, eps, N);
+            for (int i = 0; i < 3; i++) dsdu[w][g][i][a] = N[i] / h;
+        }
+    }
+    __syncthreads();
+    // d/du_local = sum_gp dPhi/dsigma . dsigma/du ; rotate to global: g_glob = R^T g_loc
+    if (valid && lane < 8) {
+        int b = lane;
+        double gl[3];
+        for (int i = 0; i < 3; i++) {
+            int a = 3 * b + i;
+            double s = 0.0;
+            for (int g = 0; g < 4; g++)
+                for (int q = 0; q < 3; q++) s += dphi_s[w][g][q] * dsdu[w][g][q][a];
+            gl[i] = s;
+        }
+        for (int j = 0; j < 3; j++) {
+            double v = S.R[j] * gl[0] + S.R[3 + j] * gl[1] + S.R[6 + j] * gl[2];
+            eg[(size_t)e * 24 + 3 * b + j] = lam * v;
+        }
+    }
+}
+
+// bar adjoint RHS: d(phi' KGb(P(u)) phi)/du = (phi_l' KG_l(P=1) phi_l) dP/du
+__global__ void bar_adjrhs_kernel(ElemArgs A, const double *phi, double lam, double *eg /* n_bar x 12 */) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= A.n_bar) return;
+    BarLocal B;
+    bar_setup(A, b, B);
+    double pl[12];
+    for (int blk = 0; blk < 4; blk++) {
+        int nd = A.bar[2 * b + blk / 2];
+        const double *pg = phi + 6 * (size_t)nd + 3 * (blk % 2);
+        for (int i = 0; i < 3; i++) pl[3 * blk + i] = B.R[i * 3] * pg[0] + B.R[i * 3 + 1] * pg[1] + B.R[i * 3 + 2] * pg[2];
+    }
+    double q = 0.0;
+    for (int i = 0; i < 12; i++)
+        for (int j = 0; j < 12; j++) q += pl[i] * bar_kg(B.L, 1.0, i, j) * pl[j];
+    // dP/du_global = row 7 of k_local times T
+    for (int blk = 0; blk < 4; blk++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0.0;
+            for (int i = 0; i < 3; i++) s += bar_k(B, 6, 3 * blk + i) * B.R[i * 3 + j];
+            eg[(size_t)b * 12 + 3 * blk + j] = lam * q * s;
+        }
+}
+
+__global__ void sens_combine_kernel(int n, const double *dke2, const double *kgt, double lam, double phiKGphi,
+                                    double *dldt) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double num = dke2[2 * e] + lam * kgt[e] - dke2[2 * e + 1];
+    dldt[e] = -num / phiKGphi;
+}
+
+}  // namespace
+
+void launch_q4_ke(const ElemArgs &a, double *eb, hipStream_t s) {
+    if (a.n_q4 == 0) return;
+    q4_ke_kernel<<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, eb);
+}
+void launch_q4_kg(const ElemArgs &a, const double *u, double *eb, hipStream_t s) {
+    if (a.n_q4 == 0) return;
+    q4_kg_kernel<<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, eb);
+}
+void launch_bar_ke(const ElemArgs &a, double *eb, hipStream_t s) {
+    if (a.n_bar == 0) return;
+    bar_ke_kernel<<<(a.n_bar + 127) / 128, 128, 0, s>>>(a, eb + (size_t)a.n_q4 * 576);
+}
+void launch_bar_kg(const ElemArgs &a, const double *u, double *eb, hipStream_t s) {
+    if (a.n_bar == 0) return;
+    bar_kg_kernel<<<(a.n_bar + 127) / 128, 128, 0, s>>>(a, u, eb + (size_t)a.n_q4 * 576);
+}
+void launch_block_gather(int64_t n_blocks, const int32_t *cptr, const int32_t *cidx, const double *eb,
+                         double *vals, hipStream_t s) {
+    int64_t n = n_blocks * 36;
+    if (n == 0) return;
+    block_gather_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n_blocks, cptr, cidx, eb, vals);
+}
+void launch_q4_sens(const ElemArgs &a, const double *u, const double *phi, const double *psi, double lam,
+                    double phiKGphi, double *dldt, hipStream_t s) {
+    if (a.n_q4 == 0) return;
+    double *tmp = nullptr;
+    (void)hipMallocAsync((void **)&tmp, sizeof(double) * 3 * (size_t)a.n_q4, s);
+    q4_dke_kernel<<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, psi, tmp);
+    q4_kgt_kernel<<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, tmp + 2 * (size_t)a.n_q4);
+    sens_combine_kernel<<<(a.n_q4 + 255) / 256, 256, 0, s>>>(a.n_q4, tmp, tmp + 2 * (size_t)a.n_q4, lam, phiKGphi,
+                                                             dldt);
+    (void)hipFreeAsync(tmp, s);
+}
+void launch_q4_adjoint_rhs(const ElemArgs &a, const double *u, const double *phi, double lam, double *eg,
+                           hipStream_t s) {
+    if (a.n_q4 == 0) return;
+    q4_adjrhs_kernel<<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, lam, eg);
+}
+void launch_bar_adjoint_rhs(const ElemArgs &a, const double *phi, double lam, double *eg, hipStream_t s) {
+    if (a.n_bar == 0) return;
+    bar_adjrhs_kernel<<<(a.n_bar + 127) / 128, 128, 0, s>>>(a, phi, lam, eg);
+}

@@ -1,0 +1,123 @@
+"""Seeded synthetic SOL 105 decks (free-field BDF text) for the workloads the
+path is measured on.  The decks use only the card subset of
+``jf_2026_05_19_amd.bdf`` so the same text can be fed to the reference's
+parser (``JFEM/src/parsing/card_processor.jl``).
+
+* ``plate_deck``  -- flat square plate, CQUAD4 mesh, simply supported on all
+  edges, uniform compressive edge load on x = a reacted at x = 0, consistent
+  nodal lumping (half load at the corners).
+* ``panel_deck``  -- flat panel with CBAR stringers (PBARL BAR) on every k-th
+  longitudinal mesh line, clamped loaded edges, axial compression.
+* ``variant_sweep`` -- seeded design variants of the plate (mesh size,
+  thickness, modulus).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+__all__ = ["plate_deck", "panel_deck", "variant_sweep"]
+
+
+def _fmt(v):
+    if isinstance(v, (int, np.integer)):
+        return str(int(v))
+    if isinstance(v, str):
+        return v
+    return repr(float(v))
+
+
+def _card(*fields):
+    return ",".join(_fmt(f) for f in fields)
+
+
+def _header(nd, sol=105):
+    return [
+        f"SOL {sol}",
+        "CEND",
+        "SUBCASE 1",
+        "  SPC = 1",
+        "  LOAD = 10",
+        "SUBCASE 2",
+        "  SPC = 1",
+        "  METHOD = 100",
+        "  STATSUB = 1",
+        "BEGIN BULK",
+        _card("EIGRL", 100, "", "", nd),
+    ]
+
+
+def plate_deck(nx=20, ny=20, a=500.0, b=500.0, t=2.0, E=70e3, nu=0.33, line_load=1.0, nd=5,
+               k6rot=None):
+    """Simply supported plate under uniaxial edge compression (N/mm)."""
+    L = _header(nd)
+    if k6rot is not None:
+        L.append(_card("PARAM", "K6ROT", float(k6rot)))
+    L.append(_card("MAT1", 1, float(E), "", float(nu)))
+    L.append(_card("PSHELL", 1, 1, float(t), 1, "", ""))
+    gid = lambda i, j: j * (nx + 1) + i + 1
+    for j in range(ny + 1):
+        for i in range(nx + 1):
+            L.append(_card("GRID", gid(i, j), "", a * i / nx, b * j / ny, 0.0))
+    eid = 1
+    for j in range(ny):
+        for i in range(nx):
+            L.append(_card("CQUAD4", eid, 1, gid(i, j), gid(i + 1, j), gid(i + 1, j + 1), gid(i, j + 1)))
+            eid += 1
+    edge = sorted({gid(i, j) for j in range(ny + 1) for i in range(nx + 1)
+                   if i in (0, nx) or j in (0, ny)})
+    for g in edge:
+        L.append(_card("SPC1", 1, "3", g))
+    for j in range(ny + 1):
+        L.append(_card("SPC1", 1, "1", gid(0, j)))
+    L.append(_card("SPC1", 1, "2", gid(0, 0)))
+    dy = b / ny
+    for j in range(ny + 1):
+        w = 0.5 if j in (0, ny) else 1.0
+        L.append(_card("FORCE", 10, gid(nx, j), 0, line_load * dy * w, -1.0, 0.0, 0.0))
+    L.append("ENDDATA")
+    return "\n".join(L) + "\n"
+
+
+def panel_deck(nx=200, ny=100, a=1000.0, b=500.0, t=1.5, E=70e3, nu=0.33, stringer_every=10,
+               bar_dims=(3.0, 20.0), line_load=1.0, nd=10):
+    """Stringer-stiffened panel: clamped loaded edges, axial compression."""
+    L = _header(nd)
+    L.append(_card("MAT1", 1, float(E), "", float(nu)))
+    L.append(_card("PSHELL", 1, 1, float(t), 1, "", ""))
+    L.append(_card("PBARL", 2, 1, "", "BAR", float(bar_dims[0]), float(bar_dims[1])))
+    gid = lambda i, j: j * (nx + 1) + i + 1
+    for j in range(ny + 1):
+        for i in range(nx + 1):
+            L.append(_card("GRID", gid(i, j), "", a * i / nx, b * j / ny, 0.0))
+    eid = 1
+    for j in range(ny):
+        for i in range(nx):
+            L.append(_card("CQUAD4", eid, 1, gid(i, j), gid(i + 1, j), gid(i + 1, j + 1), gid(i, j + 1)))
+            eid += 1
+    bid = 1000001
+    for j in range(0, ny + 1, stringer_every):
+        for i in range(nx):
+            L.append(_card("CBAR", bid, 2, gid(i, j), gid(i + 1, j), 0.0, 0.0, 1.0))
+            bid += 1
+    for j in range(ny + 1):
+        L.append(_card("SPC1", 1, "123456", gid(0, j)))
+        L.append(_card("SPC1", 1, "23456", gid(nx, j)))
+    dy = b / ny
+    for j in range(ny + 1):
+        w = 0.5 if j in (0, ny) else 1.0
+        L.append(_card("FORCE", 10, gid(nx, j), 0, line_load * dy * w, -1.0, 0.0, 0.0))
+    L.append("ENDDATA")
+    return "\n".join(L) + "\n"
+
+
+def variant_sweep(n=256, seed=20260519, n_min=10, n_max=80):
+    """Seeded plate design variants: (deck text, description) pairs."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        m = int(rng.integers(n_min, n_max + 1))
+        t = float(rng.uniform(1.0, 4.0))
+        steel = bool(rng.integers(0, 2))
+        E, nu = (210e3, 0.30) if steel else (70e3, 0.33)
+        out.append((plate_deck(nx=m, ny=m, t=t, E=E, nu=nu), dict(mesh=m, t=t, E=E, nu=nu)))
+    return out

@@ -1,0 +1,268 @@
+"""Model dictionary -> typed structure-of-arrays for the SOL 105 path.
+
+The reference keeps its model as a ``Dict{String,Any}`` (built by
+``JFEM/src/ModelBuilder.jl:421-823``) and re-derives every per-element array
+inside ``assemble_stiffness`` (``JFEM/src/solver/assembly.jl:1953-2530``).
+Here that derivation happens once, on the host, into contiguous arrays that
+the C-ABI takes by pointer:
+
+* node order = ascending GRID id (``assembly.jl:1958-1961``); DOF ``6*rank+c``;
+* per-CQUAD4 property row ``[E, nu, T, BEND_RATIO, TS_T, G_shear]``
+  (``assembly.jl:2393-2476``: MID3 falls back to MID1 for the shear modulus);
+* per-CBAR property row ``[A, I1, I2, J, K1, K2, E, G]`` and the resolved
+  orientation vector (G0 - GA, or V; ``helpers.jl:8-18``);
+* SPC1/SPC set of the static subcase (``boundary_conditions.jl:634-677``);
+* FORCE vector of the static subcase LOAD (``solve_case.jl:3-57``,
+  ``loads.jl:50-53``), rotated into the analysis frame (identity for CD = 0);
+* subcase pairing and EIGRL ND/V1/V2 (``JFEMSolver.jl:1302-1326, 1464-1472``).
+
+Features outside the supported default branch raise ``UnsupportedModel``
+(the C-ABI returns SOL105_E_UNSUPPORTED for the same cases).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+__all__ = ["UnsupportedModel", "Sol105Arrays", "build_arrays"]
+
+
+class UnsupportedModel(ValueError):
+    pass
+
+
+@dataclass
+class Sol105Arrays:
+    node_ids: np.ndarray          # (n,) int64, ascending GRID ids
+    xyz: np.ndarray               # (n, 3) float64
+    q4: np.ndarray                # (n_q4, 4) int32 node ranks
+    q4_eid: np.ndarray            # (n_q4,) int64
+    q4_pid: np.ndarray            # (n_q4,) int64
+    q4prop: np.ndarray            # (n_q4, 6) float64  E, nu, T, BR, TS/T, G_shear
+    bar: np.ndarray               # (n_bar, 2) int32
+    bar_eid: np.ndarray
+    bar_vref: np.ndarray          # (n_bar, 3)
+    barprop: np.ndarray           # (n_bar, 8)  A, I1, I2, J, K1, K2, E, G
+    spc_mask: np.ndarray          # (ndof,) uint8 1 = fixed by SPC1/SPC
+    F: np.ndarray                 # (ndof,) float64
+    nd: int = 3
+    v1: float = 0.0
+    v2: float = 0.0
+    k6rot: float = 100.0
+    autospc: bool = True
+    node_has_shell: np.ndarray = field(default=None)
+    node_has_bar: np.ndarray = field(default=None)
+    buckling_subcase: int = 1
+    static_subcase: int = 1
+
+    @property
+    def n_nodes(self):
+        return int(self.xyz.shape[0])
+
+    @property
+    def ndof(self):
+        return 6 * self.n_nodes
+
+    @property
+    def n_q4(self):
+        return int(self.q4.shape[0])
+
+    @property
+    def n_bar(self):
+        return int(self.bar.shape[0])
+
+    def with_q4_thickness(self, t: np.ndarray) -> "Sol105Arrays":
+        """Copy with per-element CQUAD4 thickness replaced (design variants)."""
+        import copy
+        out = copy.copy(self)
+        out.q4prop = self.q4prop.copy()
+        out.q4prop[:, 2] = t
+        return out
+
+
+def _flat_dev(p):
+    """max out-of-plane corner deviation / diagonal length, assembly.jl:2815-2826."""
+    d13 = p[2] - p[0]
+    d24 = p[3] - p[1]
+    n = np.cross(d13, d24)
+    ln = np.linalg.norm(n)
+    c = p.mean(axis=0)
+    L = max(np.linalg.norm(d13), np.linalg.norm(d24))
+    if ln <= 1e-12:
+        return 0.0, L
+    n = n / ln
+    return float(np.max(np.abs((p - c) @ n))), L
+
+
+def _subcase_pairing(cc):
+    subs = cc["SUBCASES"]
+    sids = sorted(subs)
+    buck = []
+    for sid in sids:
+        sub = subs[sid]
+        if sub.get("METHOD") is not None:
+            st = sub.get("STATSUB")
+            buck.append((sid, int(st) if st is not None else 0))
+    if buck and all(p[1] == 0 for p in buck):
+        default = None
+        for sid in sids:
+            if subs[sid].get("LOAD") is not None and not any(p[0] == sid for p in buck):
+                default = sid
+                break
+        if default is None:
+            default = sids[0]
+        buck = [(p[0], default) for p in buck]
+    if not buck:
+        buck = [(sids[1] if len(sids) >= 2 else sids[0], sids[0])]
+    return buck
+
+
+def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
+    if model.get("_UNSUPPORTED_CARDS"):
+        raise UnsupportedModel(f"unsupported cards: {model['_UNSUPPORTED_CARDS']}")
+    grids = model["GRIDs"]
+    ids = sorted(int(k) for k in grids)
+    rank = {g: i for i, g in enumerate(ids)}
+    n = len(ids)
+    xyz = np.zeros((n, 3))
+    for g in ids:
+        gr = grids[str(g)]
+        if (gr.get("CD", 0) or 0) != 0 or (gr.get("CP", 0) or 0) != 0:
+            raise UnsupportedModel(f"GRID {g}: CP/CD coordinate systems are not supported")
+        xyz[rank[g]] = gr["X"]
+    mats = model["MATs"]
+    pshells = model["PSHELLs"]
+    q4, q4eid, q4pid, q4prop = [], [], [], []
+    for key in sorted(model["CSHELLs"], key=lambda k: int(k)):
+        el = model["CSHELLs"][key]
+        nodes = el["NODES"]
+        if len(nodes) != 4:
+            raise UnsupportedModel(f"element {el['ID']}: CTRIA3 is not supported by this build")
+        prop = pshells.get(str(el["PID"]))
+        if prop is None:
+            continue
+        mat = mats.get(str(prop["MID"]))
+        if mat is None:
+            continue
+        if any(nd not in rank for nd in nodes):
+            continue
+        mid3 = prop.get("MID3", 0) or 0
+        smat = mats.get(str(mid3), mat) if mid3 else mat
+        Gs = float(smat.get("G", 0.0))
+        if Gs <= 0.0:
+            E_ = float(smat.get("E", 0.0)); nu_ = float(smat.get("NU", 0.0))
+            Gs = E_ / (2 * (1 + nu_)) if E_ > 0 else 0.0
+        if abs(float(el.get("THETA", 0.0) or 0.0)) > 1e-10 and False:
+            pass
+        p = xyz[[rank[v] for v in nodes]]
+        dev, L = _flat_dev(p)
+        if dev >= 1e-6 * max(L, 1e-12):
+            raise UnsupportedModel(f"element {el['ID']}: warped CQUAD4 (curved-shell gates) not supported")
+        q4.append([rank[v] for v in nodes])
+        q4eid.append(int(el["ID"]))
+        q4pid.append(int(el["PID"]))
+        q4prop.append([float(mat["E"]), float(mat["NU"]), float(prop["T"]),
+                       float(prop.get("BEND_RATIO", 1.0)), float(prop.get("TS_T", 5.0 / 6.0)), Gs])
+    pbarls = model.get("PBARLs", {})
+    bars, bareid, vref, barprop = [], [], [], []
+    for group in ("CBARs", "CBEAMs"):
+        for key in sorted(model.get(group, {}), key=lambda k: int(k)):
+            b = model[group][key]
+            prop = pbarls.get(str(b["PID"]))
+            if prop is None:
+                continue
+            if prop.get("UNSUPPORTED"):
+                raise UnsupportedModel(f"PBARL {prop['PID']}: only TYPE=BAR is supported")
+            mat = mats.get(str(prop["MID"]))
+            if mat is None or b["GA"] not in rank or b["GB"] not in rank:
+                continue
+            if (b.get("PA", 0) or 0) != 0 or (b.get("PB", 0) or 0) != 0:
+                raise UnsupportedModel(f"CBAR {b['ID']}: pin flags are not supported")
+            if any(abs(v) > 1e-10 for v in list(b.get("WA", [0, 0, 0])) + list(b.get("WB", [0, 0, 0]))):
+                raise UnsupportedModel(f"CBAR {b['ID']}: offsets are not supported")
+            g0 = b.get("G0", 0) or 0
+            if g0 > 0 and g0 in rank:
+                v = xyz[rank[g0]] - xyz[rank[b["GA"]]]
+            else:
+                v = np.asarray(b.get("V", [0.0, 0.0, 0.0]), dtype=float)
+            bars.append([rank[b["GA"]], rank[b["GB"]]])
+            bareid.append(int(b["ID"]))
+            vref.append(v)
+            barprop.append([prop["A"], prop.get("I1", prop.get("I", 0.0)), prop.get("I2", prop.get("I", 0.0)),
+                            prop["J"], prop.get("K1", 0.0), prop.get("K2", 0.0), float(mat["E"]), float(mat["G"])])
+    ndof = 6 * n
+    cc = model["CASE_CONTROL"]
+    pairs = _subcase_pairing(cc)
+    buck_sid, stat_sid = pairs[buckling_index]
+    subs = cc["SUBCASES"]
+    sub_static = subs.get(stat_sid, subs[sorted(subs)[0]])
+    sub_buck = subs[buck_sid]
+    load_id = sub_static.get("LOAD")
+    spc_id = sub_buck.get("SPC", sub_static.get("SPC"))
+    spc_mask = np.zeros(ndof, dtype=np.uint8)
+    if spc_id is not None:
+        sets = set(model.get("SPCADDs", {}).get(int(spc_id), [int(spc_id)]))
+        for spc in model["SPC1s"]:
+            if int(spc["SID"]) in sets:
+                if abs(float(spc.get("D", 0.0))) > 0:
+                    raise UnsupportedModel("enforced displacements are not supported")
+                for g in spc["NODES"]:
+                    if g in rank:
+                        for c in spc["C"]:
+                            spc_mask[6 * rank[g] + int(c) - 1] = 1
+    for g in ids:
+        ps = str(grids[str(g)].get("PS", "") or "").strip()
+        for c in ps:
+            if c.isdigit() and c != "0":
+                spc_mask[6 * rank[g] + int(c) - 1] = 1
+    F = np.zeros(ndof)
+    if load_id is not None:
+        for f in model["FORCEs"]:
+            if int(f["SID"]) == int(load_id):
+                if (f.get("CID", 0) or 0) != 0:
+                    raise UnsupportedModel("FORCE with CID != 0 is not supported")
+                g = f["GID"]
+                F[6 * rank[g]:6 * rank[g] + 3] += np.asarray(f["Dir"], dtype=float) * f["Mag"]
+        for m in model.get("MOMENTs", []):
+            if int(m["SID"]) == int(load_id):
+                raise UnsupportedModel("MOMENT loads are not supported")
+    nd, v1, v2 = 3, 0.0, 0.0
+    method = sub_buck.get("METHOD")
+    if method is not None:
+        e = model["EIGRLs"].get(str(int(method)))
+        if e is not None:
+            nd, v1, v2 = int(e.get("ND", 3)), float(e.get("V1", 0.0)), float(e.get("V2", 0.0))
+    if (v1 != 0.0 or v2 != 0.0) and v2 > v1:
+        raise UnsupportedModel("EIGRL V1/V2 range mode is not supported")
+    k6 = model.get("PARAM_K6ROT", 100.0)
+    k6rot = float(k6) if isinstance(k6, (int, float)) else 100.0
+    asp = model.get("PARAM_AUTOSPC", True)
+    if isinstance(asp, str):
+        autospc = asp.strip().upper() not in ("", "NO", "N", "FALSE", "F", "OFF", "0")
+    elif isinstance(asp, (int, float)) and not isinstance(asp, bool):
+        autospc = abs(float(asp)) > 1e-12
+    else:
+        autospc = bool(asp)
+    snorm = model.get("PARAM_SNORM", 0.0)
+    if isinstance(snorm, (int, float)) and snorm > 0:
+        raise UnsupportedModel("PARAM,SNORM > 0 is not supported")
+    q4a = np.asarray(q4, dtype=np.int32).reshape(-1, 4)
+    bara = np.asarray(bars, dtype=np.int32).reshape(-1, 2)
+    has_shell = np.zeros(n, dtype=np.uint8)
+    has_bar = np.zeros(n, dtype=np.uint8)
+    if q4a.size:
+        has_shell[q4a.ravel()] = 1
+    if bara.size:
+        has_bar[bara.ravel()] = 1
+    return Sol105Arrays(
+        node_ids=np.asarray(ids, dtype=np.int64), xyz=xyz,
+        q4=q4a, q4_eid=np.asarray(q4eid, dtype=np.int64), q4_pid=np.asarray(q4pid, dtype=np.int64),
+        q4prop=np.asarray(q4prop, dtype=float).reshape(-1, 6),
+        bar=bara, bar_eid=np.asarray(bareid, dtype=np.int64),
+        bar_vref=np.asarray(vref, dtype=float).reshape(-1, 3),
+        barprop=np.asarray(barprop, dtype=float).reshape(-1, 8),
+        spc_mask=spc_mask, F=F, nd=nd, v1=v1, v2=v2, k6rot=k6rot, autospc=autospc,
+        node_has_shell=has_shell, node_has_bar=has_bar,
+        buckling_subcase=int(buck_sid), static_subcase=int(stat_sid),
+    )

@@ -1,0 +1,316 @@
+"""CPU oracle for the SOL 105 path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
+leg import this module, and only as the checker.  The product path
+(``jf_2026_05_19_amd``) never imports it.
+
+Element and assembly arithmetic comes from ``sol105_oracle.c`` (a plain-C
+restatement, see its header for the reference lines).  The solves use SciPy
+as a textbook reference for what the reference's third-party libraries
+compute on the same matrices:
+
+* static solve: sparse LU (SuperLU) of ``K_ff`` -- the reference factors the
+  same matrix with CHOLMOD (``JFEM/src/solver/boundary_conditions.jl:740``);
+* buckling, ``n_free <= 4000``: dense symmetric-definite ``eigh(-KG, K)``,
+  i.e. the reference's ``C = L^-1 (-KG) L^-T`` route
+  (``JFEM/src/solver/solve_case.jl:2131-2196``);
+* buckling, ``n_free > 4000``: ARPACK ``eigsh(-KG, M=K, which='LM')`` for the
+  largest |theta| of ``K^-1 (-KG)``, the operator KrylovKit iterates
+  (``solve_case.jl:2202-2315``; KrylovKit 0.10.2, Arnoldi, tol 1e-12);
+* post-processing restated from ``solve_case.jl:2388-2420`` (positive filter),
+  ``:2582-2933`` (localization filter defaults), ``:2966-2987`` (mode count),
+  ``:3146-3192`` (expansion + max-abs normalisation);
+* AUTOSPC restated from ``boundary_conditions.jl:205-383``.
+
+Sensitivities are central finite differences of this forward path with MAC
+mode tracking, as the reference's own fallback does
+(``JFEM/src/solver/adjoint_buckling.jl:374-436``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_lp = ctypes.POINTER(ctypes.c_int64)
+
+
+def build(force=False):
+    """Compile the C restatement (gcc, no fast-math)."""
+    if os.path.exists(LIB_PATH) and not force:
+        return LIB_PATH
+    os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
+    src = os.path.join(HERE, "sol105_oracle.c")
+    tmp = LIB_PATH + ".tmp%d" % os.getpid()
+    subprocess.check_call(["gcc", "-O2", "-fPIC", "-shared", "-std=c99", "-fno-fast-math",
+                           "-ffp-contract=off", src, "-o", tmp, "-lm"])
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.or_q4_ke_local.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp]
+        L.or_q4_ke_global.argtypes = [_dp, _dp, ctypes.c_double, _dp, _dp, _dp]
+        L.or_q4_frame.argtypes = [_dp, _dp, _dp, _dp]
+        L.or_bar_ke_global.argtypes = [_dp, _dp, _dp, _dp, _dp]
+        L.or_frame3d_ke.argtypes = [ctypes.c_double] * 9 + [_dp]
+        L.or_frame3d_kg.argtypes = [ctypes.c_double, ctypes.c_double, _dp]
+        L.or_q4_kg_global.argtypes = [_dp, _dp, _dp, _dp]
+        L.or_q4_kg_local.argtypes = [_dp, _dp, ctypes.c_double, _dp, _dp]
+        L.or_q4_membrane_force_field.argtypes = [_dp, _dp, _dp, _dp]
+        L.or_q4_condensation_map.argtypes = [_dp, _dp, _dp]
+        L.or_bar_kg_global.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.or_assemble_K.restype = ctypes.c_int64
+        L.or_assemble_K.argtypes = [ctypes.c_int, _ip, _dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp,
+                                    _lp, _lp, _dp]
+        L.or_assemble_KG.restype = ctypes.c_int64
+        L.or_assemble_KG.argtypes = [ctypes.c_int, _ip, _dp, _dp, ctypes.c_int, _ip, _dp, _dp, _dp,
+                                     _lp, _lp, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# --------------------------------------------------------------------------
+# element level
+# --------------------------------------------------------------------------
+
+def q4_ke_local(lc, Cm, Cb, Cs, h, E, k6rot=100.0, drill_scale=1.0, membrane_incomp=True, bending_incomp=True):
+    out = np.zeros((24, 24))
+    lib().or_q4_ke_local(_p(_c(lc, float)), _p(_c(Cm, float)), _p(_c(Cb, float)), _p(_c(Cs, float)),
+                         float(h), float(E), float(k6rot), float(drill_scale), int(membrane_incomp),
+                         int(bending_incomp), _p(out))
+    return out
+
+
+def q4_ke_global(p, prop, k6rot=100.0):
+    out = np.zeros((24, 24)); R = np.zeros(9); lc = np.zeros(8)
+    lib().or_q4_ke_global(_p(_c(p, float)), _p(_c(prop, float)), float(k6rot), _p(out), _p(R), _p(lc))
+    return out, R.reshape(3, 3), lc.reshape(4, 2)
+
+
+def q4_kg_global(p, prop, ug):
+    out = np.zeros((24, 24))
+    lib().or_q4_kg_global(_p(_c(p, float)), _p(_c(prop, float)), _p(_c(ug, float)), _p(out))
+    return out
+
+
+def q4_kg_local(lc, sig, h, Amap=None):
+    out = np.zeros((24, 24))
+    A = _p(_c(Amap, float)) if Amap is not None else None
+    lib().or_q4_kg_local(_p(_c(lc, float)), _p(_c(sig, float)), float(h), A, _p(out))
+    return out
+
+
+def q4_membrane_force_field(lc, u, Cm):
+    out = np.zeros(12)
+    lib().or_q4_membrane_force_field(_p(_c(lc, float)), _p(_c(u, float)), _p(_c(Cm, float)), _p(out))
+    return out.reshape(4, 3)
+
+
+def frame3d_ke(L, A, Iy, Iz, J, E, G, Asy=np.inf, Asz=np.inf):
+    out = np.zeros((12, 12))
+    lib().or_frame3d_ke(L, A, Iy, Iz, J, E, G, Asy, Asz, _p(out))
+    return out
+
+
+def frame3d_kg(L, P):
+    out = np.zeros((12, 12))
+    lib().or_frame3d_kg(L, P, _p(out))
+    return out
+
+
+def bar_ke_global(p1, p2, vref, bp):
+    out = np.zeros((12, 12))
+    lib().or_bar_ke_global(_p(_c(p1, float)), _p(_c(p2, float)), _p(_c(vref, float)), _p(_c(bp, float)), _p(out))
+    return out
+
+
+# --------------------------------------------------------------------------
+# assembly
+# --------------------------------------------------------------------------
+
+def assemble_K(arr):
+    n = 576 * arr.n_q4 + 144 * arr.n_bar
+    I = np.zeros(n, np.int64); J = np.zeros(n, np.int64); V = np.zeros(n)
+    q4 = _c(arr.q4, np.int32); bar = _c(arr.bar, np.int32)
+    cnt = lib().or_assemble_K(arr.n_q4, _p(q4, _ip), _p(_c(arr.xyz, float)), _p(_c(arr.q4prop, float)),
+                              float(arr.k6rot), arr.n_bar, _p(bar, _ip), _p(_c(arr.bar_vref, float)),
+                              _p(_c(arr.barprop, float)), _p(I, _lp), _p(J, _lp), _p(V))
+    assert cnt == n
+    return sp.csr_matrix((V, (I, J)), shape=(arr.ndof, arr.ndof))
+
+
+def assemble_KG(arr, u):
+    n = 576 * arr.n_q4 + 144 * arr.n_bar
+    I = np.zeros(n, np.int64); J = np.zeros(n, np.int64); V = np.zeros(n)
+    q4 = _c(arr.q4, np.int32); bar = _c(arr.bar, np.int32)
+    cnt = lib().or_assemble_KG(arr.n_q4, _p(q4, _ip), _p(_c(arr.xyz, float)), _p(_c(arr.q4prop, float)),
+                               arr.n_bar, _p(bar, _ip), _p(_c(arr.bar_vref, float)), _p(_c(arr.barprop, float)),
+                               _p(_c(u, float)), _p(I, _lp), _p(J, _lp), _p(V))
+    assert cnt == n
+    return sp.csr_matrix((V, (I, J)), shape=(arr.ndof, arr.ndof))
+
+
+# --------------------------------------------------------------------------
+# boundary conditions, static solve
+# --------------------------------------------------------------------------
+
+def autospc(arr, Kdiag):
+    """Diagonal AUTOSPC.  boundary_conditions.jl:296-383 (defaults)."""
+    fixed = arr.spc_mask.astype(bool).copy()
+    if not arr.autospc:
+        return fixed
+    ndof = arr.ndof
+    comp = np.arange(ndof) % 6
+    ad = np.abs(Kdiag)
+    max_t = max(1.0, ad[comp < 3].max(initial=1.0))
+    max_r = max(1.0, ad[comp >= 3].max(initial=1.0))
+    node = np.arange(ndof) // 6
+    hs = arr.node_has_shell[node].astype(bool)
+    hb = arr.node_has_bar[node].astype(bool)
+    mult = np.ones(ndof)
+    mult[hs & hb] = 0.1
+    mult[hs & ~hb] = 4.0
+    mult[~hs & hb] = 0.25
+    thresh = np.where(comp < 3, 1e-8 * max(max_t, 1.0), 1e-8 * max(max_r, 1.0) * mult)
+    new = (~fixed) & ((ad < thresh) | (Kdiag < 0))
+    return fixed | new
+
+
+def static_solve(arr, K):
+    Kd = K.diagonal()
+    fixed = autospc(arr, Kd)
+    free = np.flatnonzero(~fixed)
+    Kff = K[free][:, free].tocsc()
+    lu = spla.splu(Kff)
+    u = np.zeros(arr.ndof)
+    u[free] = lu.solve(arr.F[free])
+    return u, free
+
+
+# --------------------------------------------------------------------------
+# buckling
+# --------------------------------------------------------------------------
+
+def buckling(arr, K, KG, free, krylov_tol=1e-13):
+    nd = arr.nd
+    nf = len(free)
+    Kff = K[free][:, free]
+    Kff = (0.5 * (Kff + Kff.T)).tocsc()
+    KGff = KG[free][:, free]
+    KGff = (0.5 * (KGff + KGff.T)).tocsc()
+    dense = nf <= 4000
+    req = 3 * nd if dense else nd
+    req = min(req, max(nf - 2, 1))
+    if dense:
+        th, V = sla.eigh(-KGff.toarray(), Kff.toarray())
+        ok = np.isfinite(th) & (np.abs(th) > 1e-12)
+        th, V = th[ok], V[:, ok]
+        lam = 1.0 / th
+        perm = np.argsort(np.abs(lam), kind="stable")
+        n_out = min(req, len(perm))
+        lam, V = lam[perm[:n_out]], V[:, perm[:n_out]]
+    else:
+        nev = min(req + 5, nf - 1)
+        lu = spla.splu(Kff)
+        Minv = spla.LinearOperator(Kff.shape, matvec=lu.solve, dtype=float)
+        th, V = spla.eigsh(-KGff, k=nev, M=Kff, Minv=Minv, which="LM", tol=krylov_tol,
+                           ncv=max(2 * nev + 20, 40))
+        ok = np.abs(th) >= 1e-14
+        th, V = th[ok], V[:, ok]
+        lam = 1.0 / th
+        keep = np.abs(lam) > 1e-6
+        lam, V = lam[keep], V[:, keep]
+        perm = np.argsort(np.abs(lam), kind="stable")
+        n_out = min(req, len(perm))
+        lam, V = lam[perm[:n_out]], V[:, perm[:n_out]]
+    raw_lam = lam.copy()
+    pos = np.flatnonzero(lam > 1e-10)
+    order = pos[np.argsort(lam[pos], kind="stable")]
+    dropped = []
+    if arr.n_q4 >= 100 and len(order):
+        limit = min(len(order), max(nd + 16, nd))
+        surv = []
+        full = np.zeros(arr.ndof)
+        for k in order[:limit]:
+            full[:] = 0.0
+            full[free] = V[:, k]
+            t2 = (full.reshape(-1, 6)[:, :3] ** 2).sum(axis=1)
+            e = t2[arr.q4].mean(axis=1)
+            tot = e.sum()
+            if tot > 0 and e.max() / tot > 0.10:
+                dropped.append(int(k))
+                continue
+            surv.append(int(k))
+        order = np.array(surv + list(order[limit:]), dtype=int)
+    n_out = min(req, len(order))
+    sel = order[:n_out]
+    modes = np.zeros((arr.ndof, n_out))
+    modes[free] = V[:, sel]
+    for m in range(n_out):
+        mx = np.abs(modes[:, m]).max()
+        if mx > 1e-30:
+            modes[:, m] /= mx
+    return dict(eigenvalues=lam[sel], modes=modes, raw_eigenvalues=raw_lam, dense=dense,
+                localization_dropped=len(dropped), requested=req)
+
+
+def solve(arr):
+    """Full forward SOL 105 path: K -> u -> KG -> (lambda, phi)."""
+    K = assemble_K(arr)
+    u, free = static_solve(arr, K)
+    KG = assemble_KG(arr, u)
+    res = buckling(arr, K, KG, free)
+    res.update(K=K, KG=KG, u=u, free=free)
+    return res
+
+
+def mac(a, b):
+    return float((a @ b) ** 2 / ((a @ a) * (b @ b)))
+
+
+def sensitivity_fd(arr, elems, mode=0, rel_step=1e-5):
+    """Central FD of lambda_mode w.r.t. the thickness of each listed CQUAD4,
+    tracking the mode by MAC against the unperturbed shape."""
+    base = solve(arr)
+    phi0 = base["modes"][:, mode]
+    out = []
+    for e in elems:
+        h = arr.q4prop[e, 2]
+        d = rel_step * h
+        vals = []
+        for sgn in (+1, -1):
+            t = arr.q4prop[:, 2].copy()
+            t[e] = h + sgn * d
+            r = solve(arr.with_q4_thickness(t))
+            macs = [mac(phi0, r["modes"][:, j]) for j in range(r["modes"].shape[1])]
+            j = int(np.argmax(macs))
+            vals.append(r["eigenvalues"][j])
+        out.append((vals[0] - vals[1]) / (2 * d))
+    return np.array(out), base
