@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -488,6 +489,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
             LanczosOut &out) {
     hipStream_t s = ctx->stream;
     int n = M->sym.n_free, ndof = M->ndof();
+    const bool debug = std::getenv("SOL105_DEBUG") != nullptr;
     int b = want <= 8 ? 8 : 16;
     int mmax = std::min(384, (n / b) * b);
     if (mmax < b) return set_err(ctx, SOL105_E_NOCONV, "free set too small for the eigensolver (%d)", n);
@@ -539,6 +541,18 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
         launch_bsr_spmv(M->n_nodes, M->d_rp.p, M->d_col.p, M->d_KG.p, xf.p, yf.p, b, -1.0, s);
         launch_gather_elim(n, ndof, b, M->d_dof_of_elim.p, yf.p, Wb.p, s);
         factor_solve_fwd(M->plan, M->dev, M->d_front.p, M->d_W.p, Wb.p, b, s);
+        if (debug && j < 2) {
+            std::vector<double> hq((size_t)n * b), hw((size_t)n * b);
+            HIPCHK(hipMemcpyAsync(hq.data(), Qj, hq.size() * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(hw.data(), Wb.p, hw.size() * 8, hipMemcpyDeviceToHost, s));
+            int herr[4];
+            HIPCHK(hipMemcpyAsync(herr, ierr.p, sizeof(herr), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            double nq = 0, nw = 0, q01 = 0;
+            for (int i = 0; i < n; i++) { nq += hq[i] * hq[i]; nw += hw[i] * hw[i]; q01 += hq[i] * hq[n + i]; }
+            fprintf(stderr, "[sol105] lanczos step %d: |q0|^2=%.6e q0.q1=%.3e |w0|^2=%.6e err=%d\n", j, nq, q01, nw,
+                    herr[0]);
+        }
         int kcols = (j + 1) * b;
         double *Tcol = T.p + (size_t)j * b * ldT;
         ts_gemm_tn(n, kcols, b, V.p, n, Wb.p, n, Tcol, ldT, 1.0, 0.0, part.p, s);
@@ -587,12 +601,21 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
             };
             bool ok = true;
             int kchk = std::min(kneed, m);
+            if (debug) {
+                fprintf(stderr, "[sol105] RR m=%d err=%d theta:", m, herr[0]);
+                for (int t = 0; t < std::min(kchk, 6); t++) fprintf(stderr, " %.6e(r=%.2e)", th[idx[t]], resid(idx[t]));
+                fprintf(stderr, "\n");
+            }
             for (int t = 0; t < kchk; t++) {
                 int i = idx[t];
+                double r = resid(i);
+                if (!std::isfinite(th[i]) || !std::isfinite(r)) { ok = false; break; }
                 if (std::fabs(th[i]) < 1e-300) continue;
-                if (resid(i) > tol * std::fabs(th[i])) { ok = false; break; }
+                if (r > tol * std::fabs(th[i])) { ok = false; break; }
             }
             bool breakdown = (herr[0] & 2) != 0;
+            if (breakdown && m < n)
+                return set_err(ctx, SOL105_E_NOCONV, "block Lanczos breakdown (rank-deficient block) at %d vectors", m);
             if (ok || last || breakdown) {
                 converged = ok;
                 int k = std::min(want, m);

@@ -133,14 +133,31 @@ __global__ void upper_mul_kernel(int b, const double *R2, int ld2, const double 
     out[i + j * ldo] = s;
 }
 
-// Start block (elimination order), seed 0: sin(0.75487... x) + 0.5 cos(1.24697... x),
-// x = i + 0.137 (seed + 1) + 0.731 * ordinal  (JFEM/src/solver/solve_case.jl:59-68)
+// Start block (elimination order).  Column 0 is the reference's deterministic
+// start vector, seed 0: sin(0.75487... x) + 0.5 cos(1.24697... x),
+// x = i + 0.137 (seed + 1) + 0.731 (JFEM/src/solver/solve_case.jl:59-68).
+// Shifted copies of that vector only span a 4-dimensional space, so the other
+// block columns use a counter-based hash (splitmix64) mapped to [-1, 1).
+__device__ __forceinline__ double hash_unit(uint64_t k) {
+    uint64_t z = k + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
 __global__ void start_block_kernel(int n, int b, double *V, int ldv, int seed) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)n * b) return;
     int i = (int)(t % n), r = (int)(t / n);
-    double x = (i + 1) + 0.137 * (seed + 1) + 0.731 * (r + 1);
-    V[i + (int64_t)r * ldv] = sin(0.7548776662466927 * x) + 0.5 * cos(1.246979603717467 * x);
+    double v;
+    if (r == 0) {
+        double x = (i + 1) + 0.137 * (seed + 1) + 0.731 * 1;
+        v = sin(0.7548776662466927 * x) + 0.5 * cos(1.246979603717467 * x);
+    } else {
+        v = hash_unit(((uint64_t)(seed + 1) << 48) ^ ((uint64_t)r << 32) ^ (uint64_t)i);
+    }
+    V[i + (int64_t)r * ldv] = v;
 }
 
 // ---------------------------------------------------------------------------
