@@ -24,6 +24,7 @@
 #include "common.h"
 #include "dense.h"
 #include "factor.h"
+#include "solve.h"
 #include "symbolic.h"
 
 struct sol105_ctx {
@@ -89,6 +90,11 @@ struct sol105_model {
     DevBuf<double> d_front, d_piv, d_W, d_kdiag_elim;
     DevBuf<int> d_err;
     DevFronts dev;
+    SolvePlan splan;
+    DevBuf<int64_t> s_poff;
+    DevBuf<int32_t> s_ftiles, s_btiles, s_ftbase, s_btbase, s_fnks, s_bnks;
+    DevBuf<double> d_P, d_PT, d_part;
+    DevSolve dsolve;
     // state
     bool K_valid = false, factor_valid = false, u_valid = false, KG_valid = false;
     // eigen results
@@ -175,25 +181,7 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
         for (int c : S.children[f]) cidx.push_back(c);
     }
     FactorPlan &P = M->plan;
-    P.n_free = S.n_free;
-    P.height = S.height;
-    P.front_storage = S.front_storage;
-    P.vec_storage = S.vec_storage;
-    P.n_kmap = (int64_t)S.k_src.size();
-    P.heights.assign(S.height + 1, HeightInfo());
-    for (int h = 0; h <= S.height; h++) {
-        HeightInfo &H = P.heights[h];
-        H.n = (int)S.by_height[h].size();
-        H.list_off = (int64_t)flist.size();
-        for (int f : S.by_height[h]) {
-            flist.push_back(f);
-            H.max_ns = std::max(H.max_ns, S.fronts[f].ns);
-            H.max_m = std::max(H.max_m, S.fronts[f].m);
-            H.max_children = std::max(H.max_children, (int)S.children[f].size());
-            for (int c : S.children[f])
-                H.max_child_nu2 = std::max<int64_t>(H.max_child_nu2, (int64_t)S.fronts[c].nu * S.fronts[c].nu);
-        }
-    }
+    build_plans(S, P, M->splan, flist);
     HIPCHK(M->f_ns.upload(ns.data(), nf, s));
     HIPCHK(M->f_nu.upload(nu.data(), nf, s));
     HIPCHK(M->f_m.upload(m.data(), nf, s));
@@ -223,8 +211,36 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
     D.off = M->f_off.p; D.row_off = M->f_rowoff.p; D.vec_off = M->f_vecoff.p; D.rel_off = M->f_reloff.p;
     D.rows = M->f_rows.p; D.rel = M->f_rel.p; D.child_ptr = M->f_cptr.p; D.child_idx = M->f_cidx.p;
     D.flist = M->f_flist.p; D.k_src = M->f_ksrc.p; D.k_dst = M->f_kdst.p;
+    // GEMM-form solve plan
+    {
+        SolvePlan &SP = M->splan;
+        HIPCHK(M->s_poff.upload(SP.poff.data(), SP.poff.size(), s));
+        HIPCHK(M->s_ftiles.upload(SP.fwd_tiles.data(), SP.fwd_tiles.size(), s));
+        HIPCHK(M->s_btiles.upload(SP.bwd_tiles.data(), SP.bwd_tiles.size(), s));
+        HIPCHK(M->s_ftbase.upload(SP.fwd_tbase.data(), SP.fwd_tbase.size(), s));
+        HIPCHK(M->s_btbase.upload(SP.bwd_tbase.data(), SP.bwd_tbase.size(), s));
+        HIPCHK(M->s_fnks.upload(SP.fwd_nks.data(), SP.fwd_nks.size(), s));
+        HIPCHK(M->s_bnks.upload(SP.bwd_nks.data(), SP.bwd_nks.size(), s));
+        HIPCHK(M->d_P.alloc(std::max<int64_t>(SP.p_storage, 1)));
+        HIPCHK(M->d_PT.alloc(std::max<int64_t>(SP.p_storage, 1)));
+        HIPCHK(M->d_part.alloc(std::max<int64_t>(solve_part_size(SP), 1)));
+        DevSolve &DS = M->dsolve;
+        DS.poff = M->s_poff.p;
+        DS.fwd_tiles = M->s_ftiles.p;
+        DS.bwd_tiles = M->s_btiles.p;
+        DS.fwd_tbase_rel = M->s_ftbase.p;
+        DS.bwd_tbase_rel = M->s_btbase.p;
+        DS.fwd_nks = M->s_fnks.p;
+        DS.bwd_nks = M->s_bnks.p;
+    }
     M->have_sym = true;
     return SOL105_OK;
+}
+
+// x <- K_ff^-1 x (elimination order, nrhs columns)
+void tri_solve(sol105_model *M, double *X, int nrhs, hipStream_t s) {
+    solve_fwd(M->plan, M->dev, M->dsolve, M->splan, M->d_P.p, M->d_W.p, M->d_part.p, X, nrhs, s);
+    solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, X, nrhs, s);
 }
 
 double ev_ms(hipEvent_t a, hipEvent_t b) {
@@ -405,6 +421,8 @@ static int factorize(sol105_ctx *ctx, sol105_model *M) {
         // post-factorisation singular-pivot check: d_i / K_ii < 1e-7
         launch_kdiag_elim(M->sym.n_free, M->d_dof_of_elim.p, M->d_diag.p, M->d_K.p, M->d_kdiag_elim.p, s);
         launch_pivot_ratio(M->sym.n_free, M->d_piv.p, M->d_kdiag_elim.p, M->d_err.p + 1, M->d_flag.p, s);
+        // explicit-inverse solve operators (only used when the factor is accepted)
+        solve_prepare(M->plan, M->dev, M->dsolve, M->splan, M->d_front.p, M->d_P.p, M->d_PT.p, s);
         HIPCHK(hipGetLastError());
         int herr[4];
         HIPCHK(hipMemcpyAsync(herr, M->d_err.p, sizeof(herr), hipMemcpyDeviceToHost, s));
@@ -441,7 +459,7 @@ int sol105_static_solve(sol105_ctx *ctx, sol105_model *M, double *u, uint8_t *fr
     double *X = nullptr;
     HIPCHK(hipMallocAsync((void **)&X, sizeof(double) * nfree, s));
     launch_gather_elim(nfree, ndof, 1, M->d_dof_of_elim.p, M->d_force.p, X, s);
-    factor_solve(M->plan, M->dev, M->d_front.p, M->d_W.p, X, 1, s);
+    tri_solve(M, X, 1, s);
     launch_scatter_elim(nfree, ndof, 1, M->d_elim_of_dof.p, X, M->d_u.p, s);
     HIPCHK(hipFreeAsync(X, s));
     HIPCHK(hipGetLastError());
@@ -536,11 +554,11 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
         double *Qj = V.p + (size_t)j * b * n;
         // W = L^-1 (-KG) L^-T Qj
         HIPCHK(hipMemcpyAsync(Wb.p, Qj, sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
-        factor_solve_bwd(M->plan, M->dev, M->d_front.p, M->d_W.p, Wb.p, b, s);
+        solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Wb.p, b, s);
         launch_scatter_elim(n, ndof, b, M->d_elim_of_dof.p, Wb.p, xf.p, s);
         launch_bsr_spmv(M->n_nodes, M->d_rp.p, M->d_col.p, M->d_KG.p, xf.p, yf.p, b, -1.0, s);
         launch_gather_elim(n, ndof, b, M->d_dof_of_elim.p, yf.p, Wb.p, s);
-        factor_solve_fwd(M->plan, M->dev, M->d_front.p, M->d_W.p, Wb.p, b, s);
+        solve_fwd(M->plan, M->dev, M->dsolve, M->splan, M->d_P.p, M->d_W.p, M->d_part.p, Wb.p, b, s);
         if (debug && j < 2) {
             std::vector<double> hq((size_t)n * b), hw((size_t)n * b);
             HIPCHK(hipMemcpyAsync(hq.data(), Qj, hq.size() * 8, hipMemcpyDeviceToHost, s));
@@ -572,7 +590,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
         if (m >= next_check || last) {
             symmetrize_copy(m, ldT, T.p, A.p, s);
             // m is a multiple of the (even) block size
-            jacobi_eig(m, A.p, Z.p, 40, 1e-15, sw.p, s);
+            jacobi_eig(m, A.p, Z.p, 30, 1e-13, sw.p, s);
             HIPCHK(hipGetLastError());
             hA.assign((size_t)m * m, 0.0);
             HIPCHK(hipMemcpyAsync(hA.data(), A.p, sizeof(double) * m * m, hipMemcpyDeviceToHost, s));
@@ -633,7 +651,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
                 HIPCHK(hipStreamSynchronize(s));
                 break;
             }
-            next_check = std::min(mmax, m + 2 * b);
+            next_check = std::min(mmax, m + 4 * b);
         }
     }
     out.m = m;
@@ -665,7 +683,7 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
     M->lanczos_m = lo.m;
     M->lanczos_steps = lo.steps;
     // x = L^-T y, expand to all DOFs
-    factor_solve_bwd(M->plan, M->dev, M->d_front.p, M->d_W.p, Y.p, k, s);
+    solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Y.p, k, s);
     HIPCHK(Xf.alloc((size_t)ndof * k));
     launch_scatter_elim(n, ndof, k, M->d_elim_of_dof.p, Y.p, Xf.p, s);
     // eigenvalue semantics: lambda = 1/theta, |theta| > 1e-12 (dense) or
@@ -793,7 +811,7 @@ int sol105_sensitivity_thickness(sol105_ctx *ctx, sol105_model *M, int32_t mode,
     launch_bar_adjoint_rhs(a, phi, lam, egb.p, s);
     node_gather_vec(M->n_nodes, M->n_q4, M->d_diag.p, M->d_cptr.p, M->d_cidx.p, egq.p, egb.p, M->d_free.p, g.p, s);
     launch_gather_elim(n, ndof, 1, M->d_dof_of_elim.p, g.p, ge.p, s);
-    factor_solve(M->plan, M->dev, M->d_front.p, M->d_W.p, ge.p, 1, s);
+    tri_solve(M, ge.p, 1, s);
     launch_scatter_elim(n, ndof, 1, M->d_elim_of_dof.p, ge.p, psi.p, s);
     // phi' KG phi on the free set
     DevBuf<double> phif;

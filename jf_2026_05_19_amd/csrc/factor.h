@@ -4,7 +4,7 @@
 #include <cstdint>
 #include <vector>
 
-#define MAXR 16  // right-hand sides per triangular-solve call
+#include "plan.h"
 
 // Per-front arrays on the device (indexed by front id).
 struct DevFronts {
@@ -17,32 +17,9 @@ struct DevFronts {
     const int64_t *k_dst;
 };
 
-struct HeightInfo {
-    int n = 0;              // fronts at this height
-    int64_t list_off = 0;   // offset in flist
-    int max_ns = 0, max_m = 0;
-    int max_children = 0;
-    int64_t max_child_nu2 = 0;
-};
-
-struct FactorPlan {
-    int n_free = 0;
-    int height = 0;
-    int64_t front_storage = 0;
-    int64_t vec_storage = 0;
-    int64_t n_kmap = 0;
-    std::vector<HeightInfo> heights;
-};
-
 void dev_fill(double *p, int64_t n, double v, hipStream_t s);
 void factor_numeric(const FactorPlan &P, const DevFronts &D, const double *kvals, double *F, double *piv, int *err,
                     hipStream_t s);
-void factor_solve(const FactorPlan &P, const DevFronts &D, const double *F, double *W, double *X, int nrhs,
-                  hipStream_t s);
-void factor_solve_fwd(const FactorPlan &P, const DevFronts &D, const double *F, double *W, double *X, int nrhs,
-                      hipStream_t s);
-void factor_solve_bwd(const FactorPlan &P, const DevFronts &D, const double *F, double *W, double *X, int nrhs,
-                      hipStream_t s);
 void launch_gather_elim(int n_free, int ndof, int nrhs, const int32_t *dof_of_elim, const double *xf, double *xe,
                         hipStream_t s);
 void launch_scatter_elim(int n_free, int ndof, int nrhs, const int32_t *elim_of_dof, const double *xe, double *xf,

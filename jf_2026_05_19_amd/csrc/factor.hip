@@ -188,149 +188,6 @@ __global__ void pivot_ratio_kernel(int n, const double *piv, const double *kdiag
     if (bad) atomicAdd(count, 1);
 }
 
-// ---------------------------------------------------------------------------
-// triangular solves with up to MAXR right-hand sides (X: n_free x nrhs, ld n)
-// ---------------------------------------------------------------------------
-
-__global__ void __launch_bounds__(256) fwd_front_kernel(DevFronts D, const int32_t *flist, const double *F,
-                                                        double *W, double *X, int n_free, int nrhs) {
-    int f = flist[blockIdx.x];
-    int ns = D.ns[f], m = D.m[f], piv0 = D.piv0[f];
-    const double *A = F + D.off[f];
-    double *V = W + D.vec_off[f] * nrhs;
-    int tid = threadIdx.x;
-    for (int t = tid; t < m * nrhs; t += blockDim.x) {
-        int i = t % m, r = t / m;
-        V[i + (int64_t)r * m] = (i < ns) ? X[(piv0 + i) + (int64_t)r * n_free] : 0.0;
-    }
-    __syncthreads();
-    for (int k = D.child_ptr[f]; k < D.child_ptr[f + 1]; k++) {
-        int c = D.child_idx[k];
-        int nu = D.nu[c], nsc = D.ns[c], mc = D.m[c];
-        const int32_t *rel = D.rel + D.rel_off[c];
-        const double *Vc = W + D.vec_off[c] * nrhs;
-        for (int t = tid; t < nu * nrhs; t += blockDim.x) {
-            int i = t % nu, r = t / nu;
-            V[rel[i] + (int64_t)r * m] += Vc[(nsc + i) + (int64_t)r * mc];
-        }
-        __syncthreads();
-    }
-    __shared__ double L[NB][NB + 1];
-    __shared__ double Y[NB][MAXR];
-    for (int k0 = 0; k0 < ns; k0 += NB) {
-        int nb = min(NB, ns - k0);
-        for (int t = tid; t < nb * nb; t += blockDim.x) {
-            int i = t % nb, j = t / nb;
-            L[i][j] = (i >= j) ? A[(k0 + i) + (int64_t)(k0 + j) * m] : 0.0;
-        }
-        for (int t = tid; t < nb * nrhs; t += blockDim.x) {
-            int i = t % nb, r = t / nb;
-            Y[i][r] = V[(k0 + i) + (int64_t)r * m];
-        }
-        __syncthreads();
-        if (tid < nrhs) {
-            int r = tid;
-            for (int j = 0; j < nb; j++) {
-                double s = Y[j][r];
-                for (int l = 0; l < j; l++) s -= L[j][l] * Y[l][r];
-                Y[j][r] = s / L[j][j];
-            }
-        }
-        __syncthreads();
-        for (int t = tid; t < nb * nrhs; t += blockDim.x) {
-            int i = t % nb, r = t / nb;
-            V[(k0 + i) + (int64_t)r * m] = Y[i][r];
-        }
-        int rows = m - k0 - nb;
-        for (int t = tid; t < rows * nrhs; t += blockDim.x) {
-            int i = k0 + nb + t % rows, r = t / rows;
-            double s = 0.0;
-            for (int l = 0; l < nb; l++) s += A[i + (int64_t)(k0 + l) * m] * Y[l][r];
-            V[i + (int64_t)r * m] -= s;
-        }
-        __syncthreads();
-    }
-    for (int t = tid; t < ns * nrhs; t += blockDim.x) {
-        int i = t % ns, r = t / ns;
-        X[(piv0 + i) + (int64_t)r * n_free] = V[i + (int64_t)r * m];
-    }
-}
-
-__global__ void __launch_bounds__(256) bwd_front_kernel(DevFronts D, const int32_t *flist, const double *F,
-                                                        double *W, double *X, int n_free, int nrhs) {
-    int f = flist[blockIdx.x];
-    int ns = D.ns[f], m = D.m[f], nu = D.nu[f], piv0 = D.piv0[f];
-    const double *A = F + D.off[f];
-    const int32_t *rows = D.rows + D.row_off[f];
-    double *V = W + D.vec_off[f] * nrhs;
-    int tid = threadIdx.x;
-    for (int t = tid; t < m * nrhs; t += blockDim.x) {
-        int i = t % m, r = t / m;
-        V[i + (int64_t)r * m] = X[rows[i] + (int64_t)r * n_free];
-    }
-    __syncthreads();
-    // z_j = y_j - sum_i L21[i][j] x_upd[i]  (one wavefront per pivot column)
-    int wv = tid / 64, lane = tid % 64, nw = blockDim.x / 64;
-    for (int j = wv; j < ns; j += nw) {
-        double acc[MAXR];
-#pragma unroll
-        for (int r = 0; r < MAXR; r++) acc[r] = 0.0;
-        for (int i = lane; i < nu; i += 64) {
-            double l = A[(ns + i) + (int64_t)j * m];
-#pragma unroll
-            for (int r = 0; r < MAXR; r++)
-                if (r < nrhs) acc[r] += l * V[(ns + i) + (int64_t)r * m];
-        }
-#pragma unroll
-        for (int r = 0; r < MAXR; r++) {
-            if (r >= nrhs) break;
-            double v = acc[r];
-            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-            if (lane == 0) V[j + (int64_t)r * m] -= v;
-        }
-    }
-    __syncthreads();
-    __shared__ double L[NB][NB + 1];
-    __shared__ double Y[NB][MAXR];
-    int last = ((ns - 1) / NB) * NB;
-    for (int k0 = last; k0 >= 0 && ns > 0; k0 -= NB) {
-        int nb = min(NB, ns - k0);
-        for (int t = tid; t < nb * nb; t += blockDim.x) {
-            int i = t % nb, j = t / nb;
-            L[i][j] = (i >= j) ? A[(k0 + i) + (int64_t)(k0 + j) * m] : 0.0;
-        }
-        for (int t = tid; t < nb * nrhs; t += blockDim.x) {
-            int i = t % nb, r = t / nb;
-            Y[i][r] = V[(k0 + i) + (int64_t)r * m];
-        }
-        __syncthreads();
-        if (tid < nrhs) {
-            int r = tid;
-            for (int j = nb - 1; j >= 0; j--) {
-                double s = Y[j][r];
-                for (int l = j + 1; l < nb; l++) s -= L[l][j] * Y[l][r];
-                Y[j][r] = s / L[j][j];
-            }
-        }
-        __syncthreads();
-        for (int t = tid; t < nb * nrhs; t += blockDim.x) {
-            int i = t % nb, r = t / nb;
-            V[(k0 + i) + (int64_t)r * m] = Y[i][r];
-        }
-        for (int t = tid; t < k0 * nrhs; t += blockDim.x) {
-            int j = t % k0, r = t / k0;
-            double s = 0.0;
-            for (int l = 0; l < nb; l++) s += A[(k0 + l) + (int64_t)j * m] * Y[l][r];
-            V[j + (int64_t)r * m] -= s;
-        }
-        __syncthreads();
-    }
-    for (int t = tid; t < ns * nrhs; t += blockDim.x) {
-        int i = t % ns, r = t / ns;
-        X[(piv0 + i) + (int64_t)r * n_free] = V[i + (int64_t)r * m];
-    }
-}
-
 __global__ void gather_elim_kernel(int n_free, int ndof, int nrhs, const int32_t *dof_of_elim, const double *xf,
                                    double *xe) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -420,34 +277,6 @@ void factor_numeric(const FactorPlan &P, const DevFronts &D, const double *kvals
                 syrk_update_kernel<<<dim3((unsigned)(T * (T + 1) / 2), H.n), 256, 0, s>>>(D, fl, k0, F);
             }
         }
-    }
-}
-
-void factor_solve(const FactorPlan &P, const DevFronts &D, const double *F, double *W, double *X, int nrhs,
-                  hipStream_t s) {
-    for (int h = 0; h <= P.height; h++) {
-        const HeightInfo &H = P.heights[h];
-        fwd_front_kernel<<<H.n, 256, 0, s>>>(D, D.flist + H.list_off, F, W, X, P.n_free, nrhs);
-    }
-    for (int h = P.height; h >= 0; h--) {
-        const HeightInfo &H = P.heights[h];
-        bwd_front_kernel<<<H.n, 256, 0, s>>>(D, D.flist + H.list_off, F, W, X, P.n_free, nrhs);
-    }
-}
-
-void factor_solve_fwd(const FactorPlan &P, const DevFronts &D, const double *F, double *W, double *X, int nrhs,
-                      hipStream_t s) {
-    for (int h = 0; h <= P.height; h++) {
-        const HeightInfo &H = P.heights[h];
-        fwd_front_kernel<<<H.n, 256, 0, s>>>(D, D.flist + H.list_off, F, W, X, P.n_free, nrhs);
-    }
-}
-
-void factor_solve_bwd(const FactorPlan &P, const DevFronts &D, const double *F, double *W, double *X, int nrhs,
-                      hipStream_t s) {
-    for (int h = P.height; h >= 0; h--) {
-        const HeightInfo &H = P.heights[h];
-        bwd_front_kernel<<<H.n, 256, 0, s>>>(D, D.flist + H.list_off, F, W, X, P.n_free, nrhs);
     }
 }
 

@@ -1,0 +1,51 @@
+// plan.h -- host-only plans of the multifrontal factorization and solves
+// (no HIP types, so host tools and tests can build them with a plain C++
+// compiler).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#define MAXR 16          // right-hand sides per triangular-solve call
+constexpr int SOLVE_RT = 64;   // rows per product tile of the solves
+constexpr int SOLVE_KS = 128;  // k-slice of the solve products
+
+struct HeightInfo {
+    int n = 0;              // fronts at this height
+    int64_t list_off = 0;   // offset in flist
+    int max_ns = 0, max_m = 0;
+    int max_children = 0;
+    int64_t max_child_nu2 = 0;
+};
+
+struct FactorPlan {
+    int n_free = 0;
+    int height = 0;
+    int64_t front_storage = 0;
+    int64_t vec_storage = 0;
+    int64_t n_kmap = 0;
+    std::vector<HeightInfo> heights;
+};
+
+struct SolveHeight {
+    int64_t fwd_off = 0, bwd_off = 0;  // first tile of this height
+    int n_fwd = 0, n_bwd = 0;          // tiles of this height
+    int max_m = 0, max_ns = 0;
+};
+
+struct SolvePlan {
+    std::vector<int64_t> poff;                  // per-front offset of P / PT (m * ns doubles)
+    int64_t p_storage = 0;
+    std::vector<int32_t> fwd_tiles, bwd_tiles;  // (front, row tile, k slice) triples
+    std::vector<int32_t> fwd_tbase, bwd_tbase;  // per front: first tile, relative to its height
+    std::vector<int32_t> fwd_nks, bwd_nks;      // per front: k slices
+    std::vector<SolveHeight> heights;
+    int64_t max_height_tiles = 0;
+};
+
+inline int64_t solve_part_size(const SolvePlan &S) { return S.max_height_tiles * SOLVE_RT * MAXR; }
+
+void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std::vector<int> &m,
+                      const std::vector<int32_t> &flist, SolvePlan &S);
+
+struct Symbolic;
+void build_plans(const Symbolic &S, FactorPlan &P, SolvePlan &SP, std::vector<int32_t> &flist);

@@ -1,0 +1,344 @@
+// solve.hip -- triangular solves with the multifrontal factor as GEMMs.
+//
+// After the numeric factorization every front f holds [L11; L21] (m x ns).
+// Once per factorization we form
+//     P_f = [ L11^-1 ; L21 L11^-1 ]      (m x ns, column-major)
+// and its transpose PT_f (ns x m, column-major).  Then, per tree height,
+//   forward:  v = [x_piv ; 0] + sum_children w_c ;  z = P v_top ;
+//             x_piv = z_top ;  w_f = v_bot - z_bot     (w_f goes to the parent)
+//   backward: u = [x_piv ; -x(update rows)] ;  x_piv = PT u
+// so each direction is one tall-skinny product per height, split into
+// 64-row tiles and 256-wide k-slices over many workgroups (memory bound: the
+// factor is streamed once per solve for up to MAXR right-hand sides), with a
+// fixed-order reduction of the k-slices (bit-reproducible).
+#include "factor.h"
+#include "solve.h"
+
+#include <cmath>
+
+namespace {
+
+constexpr int TB = 32;        // inversion block
+constexpr int RT = SOLVE_RT;  // rows per product tile
+constexpr int KS = SOLVE_KS;  // k-slice of the product
+
+// --- inverse of L11 (block column per workgroup) ---------------------------
+
+// lower-triangular 32x32 inverse in LDS: column j solved by thread j
+__device__ void tri_inv32(double (*L)[TB + 1], double (*X)[TB + 1], int nb) {
+    int j = threadIdx.x;
+    if (j < nb) {
+        for (int i = 0; i < nb; i++) {
+            double s = (i == j) ? 1.0 : 0.0;
+            if (i >= j) {
+                for (int l = j; l < i; l++) s -= L[i][l] * X[l][j];
+                X[i][j] = s / L[i][i];
+            } else {
+                X[i][j] = 0.0;
+            }
+        }
+    }
+}
+
+// P[top] = L11^-1, one workgroup per (front, 32-wide block column).
+__global__ void __launch_bounds__(256) linv_kernel(DevFronts D, const int32_t *flist, const double *F, double *P,
+                                                   const int64_t *poff) {
+    int f = flist[blockIdx.y];
+    int ns = D.ns[f], m = D.m[f];
+    int j0 = blockIdx.x * TB;
+    if (j0 >= ns) return;
+    const double *A = F + D.off[f];
+    double *Pf = P + poff[f];
+    __shared__ double Lt[TB][TB + 1], Xt[TB][TB + 1], Dg[TB][TB + 1], Di[TB][TB + 1], Acc[TB][TB + 1];
+    int tid = threadIdx.x;
+    int nbj = min(TB, ns - j0);
+    // diagonal block inverse
+    for (int t = tid; t < TB * TB; t += blockDim.x) {
+        int i = t % TB, j = t / TB;
+        Dg[i][j] = (i < nbj && j < nbj && i >= j) ? A[(j0 + i) + (int64_t)(j0 + j) * m] : 0.0;
+    }
+    __syncthreads();
+    tri_inv32(Dg, Di, nbj);
+    __syncthreads();
+    for (int t = tid; t < TB * TB; t += blockDim.x) {
+        int i = t % TB, j = t / TB;
+        if (i < nbj && j < nbj) Pf[(j0 + i) + (int64_t)(j0 + j) * m] = Di[i][j];
+    }
+    // rows above the block column are zero
+    for (int t = tid; t < j0 * nbj; t += blockDim.x) {
+        int i = t % j0, j = t / j0;
+        Pf[i + (int64_t)(j0 + j) * m] = 0.0;
+    }
+    __threadfence();
+    __syncthreads();
+    for (int i0 = j0 + TB; i0 < ns; i0 += TB) {
+        int nbi = min(TB, ns - i0);
+        for (int t = tid; t < TB * TB; t += blockDim.x) Acc[t % TB][t / TB] = 0.0;
+        __syncthreads();
+        // Acc = sum_{k0 in [j0, i0)} L[i0, k0] X[k0, j0]
+        for (int k0 = j0; k0 < i0; k0 += TB) {
+            int nbk = min(TB, ns - k0);
+            for (int t = tid; t < TB * TB; t += blockDim.x) {
+                int i = t % TB, j = t / TB;
+                Lt[i][j] = (i < nbi && j < nbk) ? A[(i0 + i) + (int64_t)(k0 + j) * m] : 0.0;
+                Xt[i][j] = (i < nbk && j < nbj) ? Pf[(k0 + i) + (int64_t)(j0 + j) * m] : 0.0;
+            }
+            __syncthreads();
+            for (int t = tid; t < TB * TB; t += blockDim.x) {
+                int i = t % TB, j = t / TB;
+                double s = 0.0;
+#pragma unroll 8
+                for (int l = 0; l < TB; l++) s += Lt[i][l] * Xt[l][j];
+                Acc[i][j] += s;
+            }
+            __syncthreads();
+        }
+        // X[i0] = -inv(L[i0,i0]) Acc
+        for (int t = tid; t < TB * TB; t += blockDim.x) {
+            int i = t % TB, j = t / TB;
+            Dg[i][j] = (i < nbi && j < nbi && i >= j) ? A[(i0 + i) + (int64_t)(i0 + j) * m] : 0.0;
+        }
+        __syncthreads();
+        tri_inv32(Dg, Di, nbi);
+        __syncthreads();
+        for (int t = tid; t < TB * TB; t += blockDim.x) {
+            int i = t % TB, j = t / TB;
+            if (i < nbi && j < nbj) {
+                double s = 0.0;
+                for (int l = 0; l <= i; l++) s += Di[i][l] * Acc[l][j];
+                Pf[(i0 + i) + (int64_t)(j0 + j) * m] = -s;
+            }
+        }
+        __threadfence();
+        __syncthreads();
+    }
+}
+
+// P[bottom] = L21 * L11^-1 : 64 x 64 output tiles, k from the tile column
+// start (L11^-1 is lower triangular).
+__global__ void __launch_bounds__(256) m21_kernel(DevFronts D, const int32_t *flist, const double *F, double *P,
+                                                  const int64_t *poff) {
+    int f = flist[blockIdx.z];
+    int ns = D.ns[f], m = D.m[f], nu = m - ns;
+    int r0 = ns + blockIdx.x * 64, c0 = blockIdx.y * 64;
+    if (nu <= 0 || r0 >= m || c0 >= ns) return;
+    const double *A = F + D.off[f];
+    double *Pf = P + poff[f];
+    __shared__ double As[64][33], Bs[32][65];
+    int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+    for (int k0 = c0; k0 < ns; k0 += 32) {
+        for (int t = threadIdx.x; t < 64 * 32; t += 256) {
+            int r = t % 64, k = t / 64;
+            As[r][k] = (r0 + r < m && k0 + k < ns) ? A[(r0 + r) + (int64_t)(k0 + k) * m] : 0.0;
+            int kk = t % 32, c = t / 32;
+            Bs[kk][c] = (k0 + kk < ns && c0 + c < ns) ? Pf[(k0 + kk) + (int64_t)(c0 + c) * m] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int k = 0; k < 32; k++) {
+            double ra[4], cb[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) ra[a] = As[tx + 16 * a][k];
+#pragma unroll
+            for (int b = 0; b < 4; b++) cb[b] = Bs[k][ty + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[a][b] += ra[a] * cb[b];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            int i = r0 + tx + 16 * a, j = c0 + ty + 16 * b;
+            if (i < m && j < ns) Pf[i + (int64_t)j * m] = acc[a][b];
+        }
+}
+
+// PT[j + i*ns] = P[i + j*m]  (64 x 64 tiles through LDS)
+__global__ void transpose_kernel(DevFronts D, const int32_t *flist, const double *P, double *PT, const int64_t *poff) {
+    int f = flist[blockIdx.z];
+    int ns = D.ns[f], m = D.m[f];
+    int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+    if (i0 >= m || j0 >= ns) return;
+    const double *Pf = P + poff[f];
+    double *Tf = PT + poff[f];
+    __shared__ double T[64][65];
+    for (int t = threadIdx.x; t < 64 * 64; t += blockDim.x) {
+        int i = t % 64, j = t / 64;
+        T[i][j] = (i0 + i < m && j0 + j < ns) ? Pf[(i0 + i) + (int64_t)(j0 + j) * m] : 0.0;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 64 * 64; t += blockDim.x) {
+        int j = t % 64, i = t / 64;
+        if (i0 + i < m && j0 + j < ns) Tf[(j0 + j) + (int64_t)(i0 + i) * ns] = T[i][j];
+    }
+}
+
+// --- solves ----------------------------------------------------------------
+
+// forward assembly: V_f = [x_piv ; 0] + sum over children of their w (fixed child order)
+__global__ void __launch_bounds__(256) fwd_assemble_kernel(DevFronts D, const int32_t *flist, double *W,
+                                                           const double *X, int n_free, int nrhs) {
+    int f = flist[blockIdx.x];
+    int ns = D.ns[f], m = D.m[f], piv0 = D.piv0[f];
+    double *V = W + D.vec_off[f] * nrhs;
+    for (int t = threadIdx.x; t < m * nrhs; t += blockDim.x) {
+        int i = t % m, r = t / m;
+        V[i + (int64_t)r * m] = (i < ns) ? X[(piv0 + i) + (int64_t)r * n_free] : 0.0;
+    }
+    __syncthreads();
+    for (int k = D.child_ptr[f]; k < D.child_ptr[f + 1]; k++) {
+        int c = D.child_idx[k];
+        int nu = D.nu[c], nsc = D.ns[c], mc = D.m[c];
+        const int32_t *rel = D.rel + D.rel_off[c];
+        const double *Vc = W + D.vec_off[c] * nrhs;
+        for (int t = threadIdx.x; t < nu * nrhs; t += blockDim.x) {
+            int i = t % nu, r = t / nu;
+            V[rel[i] + (int64_t)r * m] += Vc[(nsc + i) + (int64_t)r * mc];
+        }
+        __syncthreads();
+    }
+}
+
+// backward gather: V_f = [x_piv ; -x(update rows)]
+__global__ void bwd_gather_kernel(DevFronts D, const int32_t *flist, double *W, const double *X, int n_free,
+                                  int nrhs) {
+    int f = flist[blockIdx.y];
+    int ns = D.ns[f], m = D.m[f];
+    const int32_t *rows = D.rows + D.row_off[f];
+    double *V = W + D.vec_off[f] * nrhs;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m * nrhs; t += gridDim.x * blockDim.x) {
+        int i = t % m, r = t / m;
+        double v = X[rows[i] + (int64_t)r * n_free];
+        V[i + (int64_t)r * m] = (i < ns) ? v : -v;
+    }
+}
+
+// partial[tile][64][nrhs] = A[rows of tile, k-slice] * V[k-slice]
+// fwd: A = P_f (m x ns, ld m), input = V_f[0:ns];  bwd: A = PT_f (ns x m, ld ns), input = V_f[0:m]
+__global__ void __launch_bounds__(256) prod_kernel(DevFronts D, const int32_t *tiles, const double *A, const int64_t *poff,
+                                                   const double *W, double *part, int nrhs, int bwd) {
+    int f = tiles[3 * blockIdx.x], rt = tiles[3 * blockIdx.x + 1], ks = tiles[3 * blockIdx.x + 2];
+    int ns = D.ns[f], m = D.m[f];
+    int nrows = bwd ? ns : m;
+    int inner = bwd ? m : ns;
+    int lda = bwd ? ns : m;
+    const double *Af = A + poff[f];
+    const double *V = W + D.vec_off[f] * nrhs;
+    int k0 = ks * KS, k1 = min(inner, k0 + KS);
+    __shared__ double Vs[KS][MAXR];
+    __shared__ double red[4][RT][MAXR + 1];
+    for (int t = threadIdx.x; t < (k1 - k0) * nrhs; t += blockDim.x) {
+        int k = t % (k1 - k0), r = t / (k1 - k0);
+        Vs[k][r] = V[(k0 + k) + (int64_t)r * m];
+    }
+    __syncthreads();
+    int row = threadIdx.x % RT, g = threadIdx.x / RT;
+    int i = rt * RT + row;
+    double acc[MAXR];
+#pragma unroll
+    for (int r = 0; r < MAXR; r++) acc[r] = 0.0;
+    if (i < nrows) {
+        for (int k = k0 + g; k < k1; k += 4) {
+            double a = Af[i + (int64_t)k * lda];
+#pragma unroll
+            for (int r = 0; r < MAXR; r++) acc[r] += a * Vs[k - k0][r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < MAXR; r++) red[g][row][r] = acc[r];
+    __syncthreads();
+    for (int t = threadIdx.x; t < RT * nrhs; t += blockDim.x) {
+        int rr = t % RT, r = t / RT;
+        double s = red[0][rr][r] + red[1][rr][r] + red[2][rr][r] + red[3][rr][r];
+        part[((int64_t)blockIdx.x * RT + rr) * MAXR + r] = s;
+    }
+}
+
+// fixed-order reduction of the k-slices and write-back
+__global__ void fwd_finish_kernel(DevFronts D, const int32_t *flist, const int32_t *tbase, const int32_t *nks,
+                                  const double *part, double *W, double *X, int n_free, int nrhs) {
+    int f = flist[blockIdx.y];
+    int ns = D.ns[f], m = D.m[f], piv0 = D.piv0[f];
+    double *V = W + D.vec_off[f] * nrhs;
+    int nk = nks[f];
+    int64_t base = tbase[f];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m * nrhs; t += gridDim.x * blockDim.x) {
+        int i = t % m, r = t / m;
+        int rt = i / RT, rr = i % RT;
+        double s = 0.0;
+        for (int k = 0; k < nk; k++) s += part[((base + (int64_t)rt * nk + k) * RT + rr) * MAXR + r];
+        if (i < ns) X[(piv0 + i) + (int64_t)r * n_free] = s;
+        else V[i + (int64_t)r * m] -= s;
+    }
+}
+
+__global__ void bwd_finish_kernel(DevFronts D, const int32_t *flist, const int32_t *tbase, const int32_t *nks,
+                                  const double *part, double *X, int n_free, int nrhs) {
+    int f = flist[blockIdx.y];
+    int ns = D.ns[f], piv0 = D.piv0[f];
+    int nk = nks[f];
+    int64_t base = tbase[f];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ns * nrhs; t += gridDim.x * blockDim.x) {
+        int i = t % ns, r = t / ns;
+        int rt = i / RT, rr = i % RT;
+        double s = 0.0;
+        for (int k = 0; k < nk; k++) s += part[((base + (int64_t)rt * nk + k) * RT + rr) * MAXR + r];
+        X[(piv0 + i) + (int64_t)r * n_free] = s;
+    }
+}
+
+inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+void solve_prepare(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *F,
+                   double *Pbuf, double *PTbuf, hipStream_t s) {
+    for (int h = 0; h <= P.height; h++) {
+        const HeightInfo &H = P.heights[h];
+        const int32_t *fl = D.flist + H.list_off;
+        if (H.n == 0) continue;
+        linv_kernel<<<dim3(cdiv(H.max_ns, TB), H.n), 256, 0, s>>>(D, fl, F, Pbuf, DS.poff);
+        // update rows of each front: m - ns <= max_m (kernel clips)
+        m21_kernel<<<dim3(cdiv(H.max_m, 64), cdiv(H.max_ns, 64), H.n), 256, 0, s>>>(D, fl, F, Pbuf, DS.poff);
+        transpose_kernel<<<dim3(cdiv(H.max_m, 64), cdiv(H.max_ns, 64), H.n), 256, 0, s>>>(D, fl, Pbuf, PTbuf, DS.poff);
+    }
+}
+
+void solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *Pbuf,
+               double *W, double *part, double *X, int nrhs, hipStream_t s) {
+    for (int h = 0; h <= P.height; h++) {
+        const HeightInfo &H = P.heights[h];
+        const SolveHeight &SH = S.heights[h];
+        if (H.n == 0) continue;
+        const int32_t *fl = D.flist + H.list_off;
+        fwd_assemble_kernel<<<H.n, 256, 0, s>>>(D, fl, W, X, P.n_free, nrhs);
+        prod_kernel<<<SH.n_fwd, 256, 0, s>>>(D, DS.fwd_tiles + 3 * SH.fwd_off, Pbuf, DS.poff, W, part, nrhs, 0);
+        fwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.fwd_tbase_rel,
+                                                                                       DS.fwd_nks, part, W, X,
+                                                                                       P.n_free, nrhs);
+    }
+}
+
+void solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *PTbuf,
+               double *W, double *part, double *X, int nrhs, hipStream_t s) {
+    for (int h = P.height; h >= 0; h--) {
+        const HeightInfo &H = P.heights[h];
+        const SolveHeight &SH = S.heights[h];
+        if (H.n == 0) continue;
+        const int32_t *fl = D.flist + H.list_off;
+        bwd_gather_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, fl, W, X, P.n_free, nrhs);
+        prod_kernel<<<SH.n_bwd, 256, 0, s>>>(D, DS.bwd_tiles + 3 * SH.bwd_off, PTbuf, DS.poff, W, part, nrhs, 1);
+        bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.bwd_tbase_rel,
+                                                                                        DS.bwd_nks, part, X,
+                                                                                        P.n_free, nrhs);
+    }
+}

@@ -1,0 +1,93 @@
+// solve_plan.cpp -- tile plan of the GEMM-form triangular solves (host).
+//
+// Per front f (pivots ns, rows m): the forward product P_f (m x ns) is cut
+// in 64-row tiles x 128-column k-slices, the backward product PT_f (ns x m)
+// in 64-row tiles x 128-column k-slices.  Tiles of one tree height form one
+// launch; the per-front tile base is relative to the height's first tile so
+// the partial-sum buffer only has to hold the largest height.
+#include "plan.h"
+#include "symbolic.h"
+
+#include <algorithm>
+
+void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std::vector<int> &m,
+                      const std::vector<int32_t> &flist, SolvePlan &S) {
+    const int RT = SOLVE_RT, KS = SOLVE_KS;
+    int nf = (int)ns.size();
+    S.poff.assign(nf, 0);
+    int64_t off = 0;
+    for (int f = 0; f < nf; f++) {
+        S.poff[f] = off;
+        off += (int64_t)m[f] * ns[f];
+    }
+    S.p_storage = off;
+    S.fwd_tiles.clear();
+    S.bwd_tiles.clear();
+    S.fwd_tbase.assign(nf, 0);
+    S.bwd_tbase.assign(nf, 0);
+    S.fwd_nks.assign(nf, 0);
+    S.bwd_nks.assign(nf, 0);
+    S.heights.assign(P.height + 1, SolveHeight());
+    S.max_height_tiles = 0;
+    for (int h = 0; h <= P.height; h++) {
+        const HeightInfo &H = P.heights[h];
+        SolveHeight &SH = S.heights[h];
+        SH.fwd_off = (int64_t)S.fwd_tiles.size() / 3;
+        SH.bwd_off = (int64_t)S.bwd_tiles.size() / 3;
+        for (int q = 0; q < H.n; q++) {
+            int f = flist[H.list_off + q];
+            int nrt = (m[f] + RT - 1) / RT, nk = (ns[f] + KS - 1) / KS;
+            S.fwd_tbase[f] = (int32_t)(S.fwd_tiles.size() / 3 - SH.fwd_off);
+            S.fwd_nks[f] = nk;
+            for (int rt = 0; rt < nrt; rt++)
+                for (int k = 0; k < nk; k++) {
+                    S.fwd_tiles.push_back(f);
+                    S.fwd_tiles.push_back(rt);
+                    S.fwd_tiles.push_back(k);
+                }
+            int nrtb = (ns[f] + RT - 1) / RT, nkb = (m[f] + KS - 1) / KS;
+            S.bwd_tbase[f] = (int32_t)(S.bwd_tiles.size() / 3 - SH.bwd_off);
+            S.bwd_nks[f] = nkb;
+            for (int rt = 0; rt < nrtb; rt++)
+                for (int k = 0; k < nkb; k++) {
+                    S.bwd_tiles.push_back(f);
+                    S.bwd_tiles.push_back(rt);
+                    S.bwd_tiles.push_back(k);
+                }
+        }
+        SH.n_fwd = (int)(S.fwd_tiles.size() / 3 - SH.fwd_off);
+        SH.n_bwd = (int)(S.bwd_tiles.size() / 3 - SH.bwd_off);
+        SH.max_m = H.max_m;
+        SH.max_ns = H.max_ns;
+        S.max_height_tiles = std::max<int64_t>(S.max_height_tiles, std::max(SH.n_fwd, SH.n_bwd));
+    }
+}
+
+// Factorization plan (per-height launch bounds) and solve plan from the
+// symbolic analysis.  flist receives the front ids grouped by height.
+void build_plans(const Symbolic &S, FactorPlan &P, SolvePlan &SP, std::vector<int32_t> &flist) {
+    int nf = (int)S.fronts.size();
+    P.n_free = S.n_free;
+    P.height = S.height;
+    P.front_storage = S.front_storage;
+    P.vec_storage = S.vec_storage;
+    P.n_kmap = (int64_t)S.k_src.size();
+    P.heights.assign(S.height + 1, HeightInfo());
+    flist.clear();
+    for (int h = 0; h <= S.height; h++) {
+        HeightInfo &H = P.heights[h];
+        H.n = (int)S.by_height[h].size();
+        H.list_off = (int64_t)flist.size();
+        for (int f : S.by_height[h]) {
+            flist.push_back(f);
+            H.max_ns = std::max(H.max_ns, S.fronts[f].ns);
+            H.max_m = std::max(H.max_m, S.fronts[f].m);
+            H.max_children = std::max(H.max_children, (int)S.children[f].size());
+            for (int c : S.children[f])
+                H.max_child_nu2 = std::max<int64_t>(H.max_child_nu2, (int64_t)S.fronts[c].nu * S.fronts[c].nu);
+        }
+    }
+    std::vector<int> ns(nf), m(nf);
+    for (int f = 0; f < nf; f++) { ns[f] = S.fronts[f].ns; m[f] = S.fronts[f].m; }
+    solve_plan_build(P, ns, m, flist, SP);
+}

@@ -62,3 +62,55 @@ void probe_fronts(void *h, int64_t *info, int32_t *rows, int32_t *rel, int32_t *
     std::memcpy(k_dst, S.k_dst.data(), S.k_dst.size() * 8);
 }
 }
+
+#include "../../jf_2026_05_19_amd/csrc/plan.h"
+
+struct PlanProbe {
+    FactorPlan P;
+    SolvePlan SP;
+    std::vector<int32_t> flist;
+};
+
+extern "C" {
+
+void *probe_plans(void *h) {
+    Probe *p = (Probe *)h;
+    PlanProbe *q = new PlanProbe();
+    build_plans(p->S, q->P, q->SP, q->flist);
+    return q;
+}
+
+void probe_plans_free(void *q) { delete (PlanProbe *)q; }
+
+// [n_fwd_tiles, n_bwd_tiles, height, max_height_tiles, p_storage]
+void probe_plan_sizes(void *qh, int64_t *out) {
+    PlanProbe *q = (PlanProbe *)qh;
+    out[0] = (int64_t)q->SP.fwd_tiles.size() / 3;
+    out[1] = (int64_t)q->SP.bwd_tiles.size() / 3;
+    out[2] = q->P.height;
+    out[3] = q->SP.max_height_tiles;
+    out[4] = q->SP.p_storage;
+}
+
+// per height (8 values): list_off, n, fwd_off, bwd_off, n_fwd, n_bwd, max_m, max_ns
+void probe_plan(void *qh, int32_t *ft, int32_t *bt, int32_t *ftb, int32_t *btb, int32_t *fnk, int32_t *bnk,
+                int32_t *flist, int64_t *hinfo, int64_t *poff) {
+    PlanProbe *q = (PlanProbe *)qh;
+    const SolvePlan &S = q->SP;
+    std::memcpy(ft, S.fwd_tiles.data(), S.fwd_tiles.size() * 4);
+    std::memcpy(bt, S.bwd_tiles.data(), S.bwd_tiles.size() * 4);
+    std::memcpy(ftb, S.fwd_tbase.data(), S.fwd_tbase.size() * 4);
+    std::memcpy(btb, S.bwd_tbase.data(), S.bwd_tbase.size() * 4);
+    std::memcpy(fnk, S.fwd_nks.data(), S.fwd_nks.size() * 4);
+    std::memcpy(bnk, S.bwd_nks.data(), S.bwd_nks.size() * 4);
+    std::memcpy(flist, q->flist.data(), q->flist.size() * 4);
+    for (int h = 0; h <= q->P.height; h++) {
+        const HeightInfo &H = q->P.heights[h];
+        const SolveHeight &SH = S.heights[h];
+        int64_t *o = hinfo + 8 * h;
+        o[0] = H.list_off; o[1] = H.n; o[2] = SH.fwd_off; o[3] = SH.bwd_off;
+        o[4] = SH.n_fwd; o[5] = SH.n_bwd; o[6] = H.max_m; o[7] = H.max_ns;
+    }
+    std::memcpy(poff, S.poff.data(), S.poff.size() * 8);
+}
+}

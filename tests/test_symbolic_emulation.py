@@ -27,9 +27,10 @@ LIB = os.path.join(LIBDIR, "libsymprobe.so")
 
 def _probe():
     os.makedirs(LIBDIR, exist_ok=True)
-    sym_cpp = os.path.join(HERE, "..", "jf_2026_05_19_amd", "csrc", "symbolic.cpp")
+    csrc = os.path.join(HERE, "..", "jf_2026_05_19_amd", "csrc")
     tmp = LIB + ".tmp%d" % os.getpid()
-    subprocess.check_call(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", SRC, sym_cpp, "-o", tmp])
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", SRC, os.path.join(csrc, "symbolic.cpp"),
+                           os.path.join(csrc, "solve_plan.cpp"), "-o", tmp])
     os.replace(tmp, LIB)
     L = ctypes.CDLL(LIB)
     vp = ctypes.c_void_p
@@ -39,6 +40,11 @@ def _probe():
     L.probe_sizes.argtypes = [vp, vp]
     L.probe_pattern.argtypes = [vp, vp, vp, vp, vp]
     L.probe_fronts.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.probe_plans.restype = vp
+    L.probe_plans.argtypes = [vp]
+    L.probe_plans_free.argtypes = [vp]
+    L.probe_plan_sizes.argtypes = [vp, vp]
+    L.probe_plan.argtypes = [vp] * 10
     return L
 
 
@@ -66,9 +72,19 @@ def analyze(L, arr, free_mask, leaf=8):
     info = np.zeros((nf, 9), np.int64); rows = np.zeros(nrows, np.int32); rel = np.zeros(nrel, np.int32)
     eod = np.zeros(arr.ndof, np.int32); ksrc = np.zeros(nk, np.int32); kdst = np.zeros(nk, np.int64)
     L.probe_fronts(h, _ptr(info), _ptr(rows), _ptr(rel), _ptr(eod), _ptr(ksrc), _ptr(kdst))
+    q = L.probe_plans(h)
+    ps = np.zeros(5, np.int64)
+    L.probe_plan_sizes(q, _ptr(ps))
+    nft, nbt, H, mht, pst = [int(x) for x in ps]
+    plan = dict(ft=np.zeros(3 * nft, np.int32), bt=np.zeros(3 * nbt, np.int32), ftb=np.zeros(nf, np.int32),
+                btb=np.zeros(nf, np.int32), fnk=np.zeros(nf, np.int32), bnk=np.zeros(nf, np.int32),
+                flist=np.zeros(nf, np.int32), hinfo=np.zeros((H + 1, 8), np.int64), poff=np.zeros(nf, np.int64),
+                max_height_tiles=mht, p_storage=pst)
+    L.probe_plan(q, *[_ptr(plan[k]) for k in ("ft", "bt", "ftb", "btb", "fnk", "bnk", "flist", "hinfo", "poff")])
+    L.probe_plans_free(q)
     L.probe_free(h)
     return dict(rp=rp, col=col, cptr=cptr, cidx=cidx, nfree=nfree, info=info, rows=rows, rel=rel, eod=eod,
-                ksrc=ksrc, kdst=kdst, fs=fs, height=height)
+                ksrc=ksrc, kdst=kdst, fs=fs, height=height, plan=plan)
 
 
 def element_blocks(arr):
@@ -219,3 +235,105 @@ def test_gather_and_multifrontal_emulation(probe, deck):
         loc = S["kdst"][sel] - off
         r, c = loc % m, loc // m
         assert np.all(r >= c) and np.all(c < ns)
+
+
+RT, KS = 64, 128  # product tile rows / k-slice (plan.h SOLVE_RT / SOLVE_KS)
+
+
+def emulate_tiled_solve(S, F, b):
+    """The GEMM-form solve exactly as the device tiles it: P = [L11^-1; L21 L11^-1],
+    per-height tile lists, k-slice partial sums and the fixed-order finish."""
+    info, plan = S["info"], S["plan"]
+    nf = len(info)
+    Pf, PTf = {}, {}
+    for f in range(nf):
+        ns, nu, m, off = info[f, 0], info[f, 1], info[f, 2], info[f, 6]
+        Fp = F[off:off + m * m].reshape(m, m, order="F")
+        Linv = np.tril(np.linalg.inv(np.tril(Fp[:ns, :ns])))
+        P = np.vstack([Linv, Fp[ns:, :ns] @ Linv])
+        Pf[f], PTf[f] = P, P.T.copy()
+    children = [[] for _ in range(nf)]
+    for f in range(nf):
+        if info[f, 4] >= 0:
+            children[info[f, 4]].append(f)
+    rel_of = lambda c: S["rel"][info[c, 8]:info[c, 8] + info[c, 1]]
+    x = b.copy()
+    V = {}
+    H = plan["hinfo"]
+    for h in range(len(H)):
+        lo, n, fo, bo, nfw, nbw = H[h, :6]
+        fl = plan["flist"][lo:lo + n]
+        for f in fl:
+            ns, m, piv0 = info[f, 0], info[f, 2], info[f, 3]
+            v = np.zeros(m)
+            v[:ns] = x[piv0:piv0 + ns]
+            for c in children[f]:
+                v[rel_of(c)] += V[c][info[c, 0]:]
+            V[f] = v
+        part = np.zeros((nfw, RT))
+        for tt in range(nfw):
+            f, rt, ks = plan["ft"][3 * (fo + tt):3 * (fo + tt) + 3]
+            ns, m = info[f, 0], info[f, 2]
+            r0, r1 = rt * RT, min(m, rt * RT + RT)
+            k0, k1 = ks * KS, min(ns, ks * KS + KS)
+            part[tt, :r1 - r0] = Pf[f][r0:r1, k0:k1] @ V[f][k0:k1]
+        for f in fl:
+            ns, m, piv0 = info[f, 0], info[f, 2], info[f, 3]
+            nk, base = plan["fnk"][f], plan["ftb"][f]
+            for i in range(m):
+                s = sum(part[base + (i // RT) * nk + k, i % RT] for k in range(nk))
+                if i < ns:
+                    x[piv0 + i] = s
+                else:
+                    V[f][i] -= s
+    for h in reversed(range(len(H))):
+        lo, n, fo, bo, nfw, nbw = H[h, :6]
+        fl = plan["flist"][lo:lo + n]
+        U = {}
+        for f in fl:
+            ns, m, roff = info[f, 0], info[f, 2], info[f, 7]
+            rows = S["rows"][roff:roff + m]
+            u = x[rows].copy()
+            u[ns:] *= -1.0
+            U[f] = u
+        part = np.zeros((nbw, RT))
+        for tt in range(nbw):
+            f, rt, ks = plan["bt"][3 * (bo + tt):3 * (bo + tt) + 3]
+            ns, m = info[f, 0], info[f, 2]
+            r0, r1 = rt * RT, min(ns, rt * RT + RT)
+            k0, k1 = ks * KS, min(m, ks * KS + KS)
+            part[tt, :r1 - r0] = PTf[f][r0:r1, k0:k1] @ U[f][k0:k1]
+        for f in fl:
+            ns, piv0 = info[f, 0], info[f, 3]
+            nk, base = plan["bnk"][f], plan["btb"][f]
+            for i in range(ns):
+                x[piv0 + i] = sum(part[base + (i // RT) * nk + k, i % RT] for k in range(nk))
+    return x
+
+
+def test_tiled_gemm_solve_emulation(probe):
+    txt = decks.panel_deck(nx=14, ny=10, a=280.0, b=200.0, stringer_every=5, nd=4)
+    arr = M.build_arrays(bdf.bdf_to_model_text(txt))
+    K_or = O.assemble_K(arr)
+    free_mask = ~O.autospc(arr, K_or.diagonal())
+    S = analyze(probe, arr, free_mask, leaf=4)
+    plan = S["plan"]
+    # tile bookkeeping: every (front, row tile, k slice) appears exactly once
+    info = S["info"]
+    want_f = sum(-(-info[f, 2] // RT) * -(-info[f, 0] // KS) for f in range(len(info)))
+    want_b = sum(-(-info[f, 0] // RT) * -(-info[f, 2] // KS) for f in range(len(info)))
+    assert len(plan["ft"]) // 3 == want_f and len(plan["bt"]) // 3 == want_b
+    assert plan["max_height_tiles"] == max(plan["hinfo"][:, 4].max(), plan["hinfo"][:, 5].max())
+    eb = element_blocks(arr)
+    kv = np.zeros((len(S["col"]), 6, 6))
+    for blk in range(len(S["col"])):
+        for k in range(S["cptr"][blk], S["cptr"][blk + 1]):
+            kv[blk] += eb[S["cidx"][k]]
+    F = emulate_factor(S, kv.reshape(-1))
+    eod = S["eod"]
+    free = np.flatnonzero(free_mask)
+    perm = np.empty(S["nfree"], np.int64)
+    perm[eod[free]] = free
+    x = emulate_tiled_solve(S, F, arr.F[perm])
+    u_ref, _ = O.static_solve(arr, K_or)
+    assert np.abs(x - u_ref[perm]).max() <= 1e-9 * np.abs(u_ref).max()
