@@ -514,7 +514,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     int guard = 2;
     int kneed = std::min(want + guard, mmax);
     double tol = 1e-10;
-    DevBuf<double> V, Wb, xf, yf, T, A, Z, part, small, H2;
+    DevBuf<double> V, Wb, xf, yf, T, A, Zs, thd, ework, part, small, H2;
     DevBuf<int> ierr, sw;
     HIPCHK(V.alloc((size_t)n * (mmax + b)));
     HIPCHK(Wb.alloc((size_t)n * b));
@@ -522,11 +522,15 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     HIPCHK(yf.alloc((size_t)ndof * b));
     HIPCHK(T.alloc((size_t)(mmax + b) * (mmax + b)));
     HIPCHK(A.alloc((size_t)mmax * mmax));
-    HIPCHK(Z.alloc((size_t)mmax * mmax));
     HIPCHK(part.alloc(std::max<int64_t>(ts_gemm_tn_part_size(n, mmax + b, b), 1)));
     HIPCHK(small.alloc(4 * 16 * 16));
     HIPCHK(ierr.alloc(4));
     HIPCHK(sw.alloc(1));
+    if (kneed > SYMEIG_MAX_K || mmax > SYMEIG_MAX_M)
+        return set_err(ctx, SOL105_E_UNSUPPORTED, "Rayleigh-Ritz size out of range (k=%d, m=%d)", kneed, mmax);
+    HIPCHK(Zs.alloc((size_t)mmax * kneed));
+    HIPCHK(thd.alloc(kneed));
+    HIPCHK(ework.alloc(symeig_work_size(mmax, kneed)));
     HIPCHK(H2.alloc((size_t)(mmax + b) * b));
     int ldT = mmax + b;
     dev_fill(T.p, (int64_t)ldT * ldT, 0.0, s);
@@ -546,8 +550,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     cholqr2(V.p, nullptr, 0);
     int j = 0;
     int next_check = std::min(mmax, std::max(2 * b, ((kneed * 3 + b - 1) / b) * b));
-    std::vector<double> hA, hZlast, hR;
-    std::vector<int> sel;
+    std::vector<double> hR;
     bool converged = false;
     int m = 0;
     while (true) {
@@ -589,65 +592,58 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
         bool last = (m >= mmax);
         if (m >= next_check || last) {
             symmetrize_copy(m, ldT, T.p, A.p, s);
-            // m is a multiple of the (even) block size
-            jacobi_eig(m, A.p, Z.p, 30, 1e-13, sw.p, s);
+            // the kchk Ritz pairs of largest |theta| (A is overwritten)
+            const int kchk = std::min(kneed, m);
+            sym_eig_top(m, A.p, kchk, thd.p, Zs.p, ework.p, sw.p, s);
             HIPCHK(hipGetLastError());
-            hA.assign((size_t)m * m, 0.0);
-            HIPCHK(hipMemcpyAsync(hA.data(), A.p, sizeof(double) * m * m, hipMemcpyDeviceToHost, s));
-            std::vector<double> hZ((size_t)m * m);
-            HIPCHK(hipMemcpyAsync(hZ.data(), Z.p, sizeof(double) * m * m, hipMemcpyDeviceToHost, s));
+            std::vector<double> th(kchk), hZl((size_t)b * kchk);
+            HIPCHK(hipMemcpyAsync(th.data(), thd.p, sizeof(double) * kchk, hipMemcpyDeviceToHost, s));
+            int eig_info = 0;
+            HIPCHK(hipMemcpyAsync(&eig_info, sw.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            // last block row of the Ritz vectors in the Lanczos basis
+            HIPCHK(hipMemcpy2DAsync(hZl.data(), b * sizeof(double), Zs.p + (m - b), (size_t)m * sizeof(double),
+                                    b * sizeof(double), kchk, hipMemcpyDeviceToHost, s));
             hR.assign((size_t)b * b, 0.0);
             HIPCHK(hipMemcpy2DAsync(hR.data(), b * sizeof(double), T.p + (size_t)m + (size_t)(m - b) * ldT,
                                     ldT * sizeof(double), b * sizeof(double), b, hipMemcpyDeviceToHost, s));
             int herr[4];
             HIPCHK(hipMemcpyAsync(herr, ierr.p, sizeof(herr), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
-            std::vector<double> th(m);
-            for (int i = 0; i < m; i++) th[i] = hA[(size_t)i * m + i];
-            std::vector<int> idx(m);
-            std::iota(idx.begin(), idx.end(), 0);
-            std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) { return std::fabs(th[a]) > std::fabs(th[c]); });
-            // residual of Ritz pair i: || R_j * Z[m-b:m, i] ||
-            auto resid = [&](int i) {
+            if (eig_info != 0)
+                return set_err(ctx, SOL105_E_NOCONV, "projected eigenproblem: inverse iteration failed (m=%d)", m);
+            // residual of Ritz pair t: || R_j * Z[m-b:m, t] ||
+            auto resid = [&](int t) {
                 double r2 = 0.0;
                 for (int p = 0; p < b; p++) {
                     double v = 0.0;
-                    for (int q = 0; q < b; q++) v += hR[(size_t)q * b + p] * hZ[(size_t)i * m + (m - b) + q];
+                    for (int q = 0; q < b; q++) v += hR[(size_t)q * b + p] * hZl[(size_t)t * b + q];
                     r2 += v * v;
                 }
                 return std::sqrt(r2);
             };
             bool ok = true;
-            int kchk = std::min(kneed, m);
             if (debug) {
                 fprintf(stderr, "[sol105] RR m=%d err=%d theta:", m, herr[0]);
-                for (int t = 0; t < std::min(kchk, 6); t++) fprintf(stderr, " %.6e(r=%.2e)", th[idx[t]], resid(idx[t]));
+                for (int t = 0; t < std::min(kchk, 6); t++) fprintf(stderr, " %.6e(r=%.2e)", th[t], resid(t));
                 fprintf(stderr, "\n");
             }
             for (int t = 0; t < kchk; t++) {
-                int i = idx[t];
-                double r = resid(i);
-                if (!std::isfinite(th[i]) || !std::isfinite(r)) { ok = false; break; }
-                if (std::fabs(th[i]) < 1e-300) continue;
-                if (r > tol * std::fabs(th[i])) { ok = false; break; }
+                double r = resid(t);
+                if (!std::isfinite(th[t]) || !std::isfinite(r)) { ok = false; break; }
+                if (std::fabs(th[t]) < 1e-300) continue;
+                if (r > tol * std::fabs(th[t])) { ok = false; break; }
             }
             bool breakdown = (herr[0] & 2) != 0;
             if (breakdown && m < n)
                 return set_err(ctx, SOL105_E_NOCONV, "block Lanczos breakdown (rank-deficient block) at %d vectors", m);
             if (ok || last || breakdown) {
                 converged = ok;
-                int k = std::min(want, m);
-                sel.assign(idx.begin(), idx.begin() + k);
-                out.theta.clear();
+                int k = std::min(want, kchk);
+                out.theta.assign(th.begin(), th.begin() + k);
                 out.resid.clear();
-                for (int i : sel) { out.theta.push_back(th[i]); out.resid.push_back(resid(i)); }
-                // Ritz vectors Y = V_m Z[:, sel]
-                std::vector<double> S((size_t)m * k);
-                for (int c = 0; c < k; c++)
-                    for (int r = 0; r < m; r++) S[(size_t)c * m + r] = hZ[(size_t)sel[c] * m + r];
-                DevBuf<double> dS;
-                HIPCHK(dS.upload(S.data(), S.size(), s));
-                ts_gemm_nn(n, m, k, V.p, n, dS.p, m, d_Y, n, 1.0, 0.0, s);
+                for (int t = 0; t < k; t++) out.resid.push_back(resid(t));
+                // Ritz vectors Y = V_m Z[:, 0:k]
+                ts_gemm_nn(n, m, k, V.p, n, Zs.p, m, d_Y, n, 1.0, 0.0, s);
                 HIPCHK(hipStreamSynchronize(s));
                 break;
             }

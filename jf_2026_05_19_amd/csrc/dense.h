@@ -15,7 +15,6 @@ void small_chol_upper(int b, double *G, int ldg, int *err, hipStream_t s);
 void trsm_right_upper(int n, int b, double *W, int ldw, const double *R, int ldr, hipStream_t s);
 void upper_mul(int b, const double *R2, int ld2, const double *R1, int ld1, double *out, int ldo, hipStream_t s);
 void start_block(int n, int b, double *V, int ldv, int seed, hipStream_t s);
-void jacobi_eig(int m, double *A, double *Z, int max_sweeps, double tol, int *sweeps, hipStream_t s);
 void elem_energy_share(int n_q4, const int32_t *q4, int ndof, const double *modes, int k, double *scratch,
                        double *out2, hipStream_t s);
 void normalize_maxabs(int64_t n, double *x, double *scratch, hipStream_t s);
@@ -26,3 +25,12 @@ void node_gather_vec(int n_nodes, int n_q4, const int32_t *diag_blk, const int32
 void extract_kdiag(int n_nodes, const int32_t *diag_blk, const double *kv, double *out, hipStream_t s);
 // dst[i + j*ldd] += src[i + j*lds]
 void mat_add(int rows, int cols, const double *src, int lds, double *dst, int ldd, hipStream_t s);
+// The k eigenpairs of largest |theta| of a symmetric m x m matrix A
+// (column-major, overwritten), one workgroup; m <= SYMEIG_MAX_M,
+// k <= SYMEIG_MAX_K.  theta[t] in descending |theta| order, Zs (m x k, ld m)
+// the orthonormal eigenvectors; work >= symeig_work_size(m, k) doubles;
+// *info = 0 on success.
+constexpr int SYMEIG_MAX_M = 1024;
+constexpr int SYMEIG_MAX_K = 64;
+int64_t symeig_work_size(int m, int k);
+void sym_eig_top(int m, double *A, int k, double *theta, double *Zs, double *work, int *info, hipStream_t s);

@@ -1,7 +1,6 @@
 // dense.hip -- tall-skinny dense kernels for the block Lanczos eigensolver:
 // V^T W products (two-stage, fixed reduction order), W -= V H updates,
-// Cholesky-QR of a block, a parallel cyclic Jacobi eigensolver for the
-// projected matrix, the deterministic start block, and the post-processing
+// Cholesky-QR of a block, the deterministic start block, and the post-processing
 // reductions (element energy shares, max-abs normalisation).
 #include "dense.h"
 
@@ -160,101 +159,6 @@ __global__ void start_block_kernel(int n, int b, double *V, int ldv, int seed) {
     V[i + (int64_t)r * ldv] = v;
 }
 
-// ---------------------------------------------------------------------------
-// Parallel cyclic Jacobi eigensolver (one workgroup).  Matrix A (m x m, ld m,
-// symmetric) and eigenvectors Z (m x m) live in global memory; each round
-// rotates m/2 disjoint index pairs of the round-robin tournament.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void rr_pair(int m, int round, int k, int &p, int &q) {
-    // circle method: position 0 fixed, others rotate
-    int n1 = m - 1;
-    int a = (k == 0) ? 0 : 1 + ((k - 1 + round) % n1);
-    int bpos = m - 1 - k;
-    int b = 1 + ((bpos - 1 + round) % n1);
-    p = min(a, b);
-    q = max(a, b);
-}
-
-__global__ void __launch_bounds__(1024) jacobi_kernel(int m, double *A, double *Z, int max_sweeps, double tol,
-                                                      int *sweeps_out) {
-    __shared__ double cs[512], sn[512];
-    __shared__ int pp[512], qq[512];
-    __shared__ double red[1024];
-    __shared__ int done;
-    int tid = threadIdx.x, nt = blockDim.x;
-    for (int t = tid; t < m * m; t += nt) Z[t] = ((t % m) == (t / m)) ? 1.0 : 0.0;
-    if (tid == 0) done = 0;
-    __syncthreads();
-    int half = m / 2;
-    int sweep = 0;
-    for (; sweep < max_sweeps; sweep++) {
-        // convergence: off-diagonal Frobenius vs diagonal
-        double off = 0.0, dia = 0.0;
-        for (int t = tid; t < m * m; t += nt) {
-            int i = t % m, j = t / m;
-            double v = A[t];
-            if (i == j) dia += v * v; else off += v * v;
-        }
-        red[tid] = off;
-        __syncthreads();
-        for (int s = nt / 2; s > 0; s >>= 1) {
-            if (tid < s) red[tid] += red[tid + s];
-            __syncthreads();
-        }
-        double offs = red[0];
-        __syncthreads();
-        red[tid] = dia;
-        __syncthreads();
-        for (int s = nt / 2; s > 0; s >>= 1) {
-            if (tid < s) red[tid] += red[tid + s];
-            __syncthreads();
-        }
-        double dias = red[0];
-        __syncthreads();
-        if (offs <= tol * tol * dias) break;
-        for (int round = 0; round < m - 1; round++) {
-            for (int k = tid; k < half; k += nt) {
-                int p, q;
-                rr_pair(m, round, k, p, q);
-                double apq = A[p + q * m], app = A[p + p * m], aqq = A[q + q * m];
-                double c = 1.0, s = 0.0;
-                if (apq != 0.0) {
-                    double tau = (aqq - app) / (2.0 * apq);
-                    double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    c = 1.0 / sqrt(1.0 + t * t);
-                    s = t * c;
-                }
-                cs[k] = c; sn[k] = s; pp[k] = p; qq[k] = q;
-            }
-            __syncthreads();
-            // rows: A[p,:], A[q,:]
-            for (int t = tid; t < half * m; t += nt) {
-                int k = t % half, j = t / half;
-                int p = pp[k], q = qq[k];
-                double c = cs[k], s = sn[k];
-                double ap = A[p + j * m], aq = A[q + j * m];
-                A[p + j * m] = c * ap - s * aq;
-                A[q + j * m] = s * ap + c * aq;
-            }
-            __syncthreads();
-            // columns: A[:,p], A[:,q] and Z[:,p], Z[:,q]
-            for (int t = tid; t < half * m; t += nt) {
-                int k = t / m, i = t % m;
-                int p = pp[k], q = qq[k];
-                double c = cs[k], s = sn[k];
-                double ap = A[i + p * m], aq = A[i + q * m];
-                A[i + p * m] = c * ap - s * aq;
-                A[i + q * m] = s * ap + c * aq;
-                double zp = Z[i + p * m], zq = Z[i + q * m];
-                Z[i + p * m] = c * zp - s * zq;
-                Z[i + q * m] = s * zp + c * zq;
-            }
-            __syncthreads();
-        }
-    }
-    if (tid == 0) *sweeps_out = sweep;
-}
-
 // per-element translational energy share of each mode (localization filter,
 // JFEM/src/solver/solve_case.jl:2795-2893): e = mean over corner nodes of
 // tx^2+ty^2+tz^2; out[k] = max_e / sum_e for mode k (two-stage).
@@ -361,10 +265,6 @@ void upper_mul(int b, const double *R2, int ld2, const double *R1, int ld1, doub
 
 void start_block(int n, int b, double *V, int ldv, int seed, hipStream_t s) {
     start_block_kernel<<<nblk((int64_t)n * b, 256), 256, 0, s>>>(n, b, V, ldv, seed);
-}
-
-void jacobi_eig(int m, double *A, double *Z, int max_sweeps, double tol, int *sweeps, hipStream_t s) {
-    jacobi_kernel<<<1, 1024, 0, s>>>(m, A, Z, max_sweeps, tol, sweeps);
 }
 
 void elem_energy_share(int n_q4, const int32_t *q4, int ndof, const double *modes, int k, double *scratch,
@@ -479,3 +379,4 @@ __global__ void mat_add_kernel(int rows, int cols, const double *src, int lds, d
 void mat_add(int rows, int cols, const double *src, int lds, double *dst, int ldd, hipStream_t s) {
     if (rows * cols > 0) mat_add_kernel<<<(rows * cols + 255) / 256, 256, 0, s>>>(rows, cols, src, lds, dst, ldd);
 }
+
