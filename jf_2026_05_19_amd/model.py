@@ -153,8 +153,6 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
         if Gs <= 0.0:
             E_ = float(smat.get("E", 0.0)); nu_ = float(smat.get("NU", 0.0))
             Gs = E_ / (2 * (1 + nu_)) if E_ > 0 else 0.0
-        if abs(float(el.get("THETA", 0.0) or 0.0)) > 1e-10 and False:
-            pass
         p = xyz[[rank[v] for v in nodes]]
         dev, L = _flat_dev(p)
         if dev >= 1e-6 * max(L, 1e-12):

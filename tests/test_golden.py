@@ -1,0 +1,50 @@
+"""The oracle reproduces the committed golden fixtures (tests/golden/*.json).
+
+The fixtures were written by tests/golden/make_fixtures.py from the pinned
+oracle; this keeps later oracle edits honest.  The GPU kernels are compared
+against the same vectors in tests/test_gpu_golden.py.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from jf_2026_05_19_amd import bdf, model as M
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+ELEMS = json.load(open(os.path.join(GOLDEN, "elements.json")))
+SOLVES = json.load(open(os.path.join(GOLDEN, "solves.json")))
+
+
+@pytest.mark.parametrize("i", range(len(ELEMS["quads"])))
+def test_oracle_quad_fixtures(i):
+    q = ELEMS["quads"][i]
+    p = np.asarray(q["xyz"]).ravel()
+    Ke, _, _ = O.q4_ke_global(p, np.asarray(q["prop"]))
+    KG = O.q4_kg_global(p, np.asarray(q["prop"]), np.asarray(q["u"]))
+    for got, ref in ((Ke.ravel(), q["Ke"]), (KG.ravel(), q["KG"])):
+        ref = np.asarray(ref)
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("i", range(len(ELEMS["bars"])))
+def test_oracle_bar_fixtures(i):
+    b = ELEMS["bars"][i]
+    p1, p2 = (np.asarray(x) for x in b["xyz"])
+    Ke = O.bar_ke_global(p1, p2, np.asarray(b["vref"]), np.asarray(b["prop"]))
+    ref = np.asarray(b["Ke"])
+    assert np.abs(Ke.ravel() - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("name", sorted(SOLVES))
+def test_oracle_solve_fixtures(name):
+    s = SOLVES[name]
+    arr = M.build_arrays(bdf.bdf_to_model_text(s["deck"]))
+    r = O.solve(arr)
+    assert len(r["free"]) == s["n_free"]
+    lam = np.asarray(r["eigenvalues"])
+    ref = np.asarray(s["eigenvalues"])
+    assert lam.shape == ref.shape
+    assert np.all(np.abs(lam - ref) <= 1e-10 * np.abs(ref))
