@@ -114,6 +114,18 @@ int sol105_solve(sol105_ctx *ctx, sol105_model *model, int32_t max_modes,
 int sol105_sensitivity_thickness(sol105_ctx *ctx, sol105_model *model, int32_t mode,
                                  double *dlam_dt /* n_q4 */);
 
+/* The terms of d(lambda_m)/dt_e separately, for checking the adjoint formula
+ * term by term (JFEM/src/solver/adjoint_buckling.jl:2611):
+ *   terms (n_q4 x 3, may be NULL) = [phi' dKe/dt phi, phi' KGe phi / t,
+ *                                    psi' dKe/dt u] per CQUAD4,
+ *   adjoint_load (6*n_nodes, may be NULL) = lambda d(phi' KG(u) phi)/du on
+ *                                    the free DOFs (zero elsewhere),
+ *   *phi_kg_phi (may be NULL) = phi' KG phi,
+ * so that d(lambda)/dt_e = -(terms[0] + lambda terms[1] - terms[2]) / phi_kg_phi
+ * with psi = K_ff^-1 adjoint_load. */
+int sol105_sensitivity_terms(sol105_ctx *ctx, sol105_model *model, int32_t mode, double *terms,
+                             double *adjoint_load, double *phi_kg_phi);
+
 /* Phase timings of the last sol105_solve in milliseconds (device events):
  * [0] K assembly, [1] AUTOSPC+symbolic, [2] factorization, [3] static solve,
  * [4] KG assembly, [5] eigensolve, [6] post-processing, [7] total. */

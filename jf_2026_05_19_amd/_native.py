@@ -22,7 +22,7 @@ EXPORTS = [
     "sol105_ctx_create", "sol105_ctx_destroy", "sol105_last_error", "sol105_model_upload", "sol105_model_free",
     "sol105_model_set_q4_thickness", "sol105_pattern", "sol105_assemble_K", "sol105_static_solve",
     "sol105_assemble_KG", "sol105_buckling", "sol105_solve", "sol105_sensitivity_thickness",
-    "sol105_last_timings", "sol105_last_stats",
+    "sol105_sensitivity_terms", "sol105_last_timings", "sol105_last_stats",
 ]
 
 
@@ -77,6 +77,7 @@ def lib():
         L.sol105_buckling.argtypes = [vp, vp, ctypes.c_int32, dp, dp, ctypes.POINTER(ctypes.c_int32)]
         L.sol105_solve.argtypes = [vp, vp, ctypes.c_int32, dp, dp, ctypes.POINTER(ctypes.c_int32)]
         L.sol105_sensitivity_thickness.argtypes = [vp, vp, ctypes.c_int32, dp]
+        L.sol105_sensitivity_terms.argtypes = [vp, vp, ctypes.c_int32, dp, dp, dp]
         L.sol105_last_timings.argtypes = [vp, dp]
         L.sol105_last_stats.argtypes = [vp, dp]
         _lib = L
@@ -219,6 +220,15 @@ class DeviceModel:
         out = np.zeros(self.arr.n_q4)
         self.ctx.check(self.ctx.L.sol105_sensitivity_thickness(self.ctx.h, self.h, int(mode), _dp(out)))
         return out
+
+    def sensitivity_terms(self, mode=0):
+        """(terms n_q4 x 3, adjoint load ndof, phi' KG phi) of d(lambda)/dt."""
+        terms = np.zeros((self.arr.n_q4, 3))
+        g = np.zeros(self.arr.ndof)
+        pkp = np.zeros(1)
+        self.ctx.check(self.ctx.L.sol105_sensitivity_terms(self.ctx.h, self.h, int(mode), _dp(terms), _dp(g),
+                                                           _dp(pkp)))
+        return terms, g, float(pkp[0])
 
     def bsr_to_csr(self, vals):
         """BSR values (n_blocks*36) -> scipy CSR over all DOFs."""

@@ -497,12 +497,71 @@ namespace {
 
 // Block Lanczos on C = L^-1 (-KG) L^-T with full (CGS2) reorthogonalisation.
 // Returns the Ritz values / vectors of the `want` largest |theta|.
+// Eigen-decomposition of a small symmetric matrix (b <= 16, column-major) on
+// the host by cyclic Jacobi; w descending, U column-major with matching
+// columns.  Used only on the rare rank-deficient Lanczos step.
+void small_sym_eig_host(int b, std::vector<double> A, std::vector<double> &w, std::vector<double> &U) {
+    U.assign((size_t)b * b, 0.0);
+    for (int i = 0; i < b; i++) U[(size_t)i * b + i] = 1.0;
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0.0, dia = 0.0;
+        for (int q = 0; q < b; q++)
+            for (int p = 0; p < b; p++) (p == q ? dia : off) += A[(size_t)q * b + p] * A[(size_t)q * b + p];
+        if (off <= 1e-32 * dia || off == 0.0) break;
+        for (int p = 0; p < b - 1; p++)
+            for (int q = p + 1; q < b; q++) {
+                double apq = A[(size_t)q * b + p];
+                if (apq == 0.0) continue;
+                double app = A[(size_t)p * b + p], aqq = A[(size_t)q * b + q];
+                double tau = (aqq - app) / (2.0 * apq);
+                double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                double c = 1.0 / std::sqrt(1.0 + t * t), sn = t * c;
+                for (int k = 0; k < b; k++) {  // A <- J^T A J
+                    double akp = A[(size_t)p * b + k], akq = A[(size_t)q * b + k];
+                    A[(size_t)p * b + k] = c * akp - sn * akq;
+                    A[(size_t)q * b + k] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < b; k++) {
+                    double apk = A[(size_t)k * b + p], aqk = A[(size_t)k * b + q];
+                    A[(size_t)k * b + p] = c * apk - sn * aqk;
+                    A[(size_t)k * b + q] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < b; k++) {
+                    double ukp = U[(size_t)p * b + k], ukq = U[(size_t)q * b + k];
+                    U[(size_t)p * b + k] = c * ukp - sn * ukq;
+                    U[(size_t)q * b + k] = sn * ukp + c * ukq;
+                }
+            }
+    }
+    std::vector<int> idx(b);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return A[(size_t)x * b + x] > A[(size_t)y * b + y]; });
+    std::vector<double> U2((size_t)b * b);
+    w.resize(b);
+    for (int c = 0; c < b; c++) {
+        w[c] = A[(size_t)idx[c] * b + idx[c]];
+        for (int k = 0; k < b; k++) U2[(size_t)c * b + k] = U[(size_t)idx[c] * b + k];
+    }
+    U.swap(U2);
+}
+
 struct LanczosOut {
-    std::vector<double> theta;  // selected Ritz values (order of selection)
+    std::vector<double> theta;  // selected Ritz values (descending |theta|)
     std::vector<double> resid;
-    int m = 0, steps = 0;
+    int m = 0, steps = 0, injected = 0;
 };
 
+// Block Lanczos on C = L^-1 (-KG) L^-T with full CGS2 reorthogonalisation.
+// Each new block is orthonormalised by CholQR2 whose first Cholesky also
+// checks every pivot against the block's norm after the first Gram-Schmidt
+// pass: a direction left below 1e-7 of it is numerically inside the basis
+// (the Krylov space is exhausted or a block multiplicity exceeded b).  Such a
+// block is detected at the next Rayleigh-Ritz check; the iteration then
+// returns to it and rebuilds it with an eigen-decomposition of its Gram
+// matrix, keeping the independent directions and replacing the others by
+// seeded random vectors orthogonalised against the whole basis.  The
+// Lanczos relation C V = V T + Q R E^T stays exact (the new directions enter
+// with zero coefficients), so Rayleigh-Ritz and the residual bound hold.
 int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want, elimination order */,
             LanczosOut &out) {
     hipStream_t s = ctx->stream;
@@ -513,7 +572,8 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     if (mmax < b) return set_err(ctx, SOL105_E_NOCONV, "free set too small for the eigensolver (%d)", n);
     int guard = 2;
     int kneed = std::min(want + guard, mmax);
-    double tol = 1e-10;
+    const double tol = 1e-10;
+    const double dep_tol2 = 1e-14;
     DevBuf<double> V, Wb, xf, yf, T, A, Zs, thd, ework, part, small, H2;
     DevBuf<int> ierr, sw;
     HIPCHK(V.alloc((size_t)n * (mmax + b)));
@@ -523,7 +583,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     HIPCHK(T.alloc((size_t)(mmax + b) * (mmax + b)));
     HIPCHK(A.alloc((size_t)mmax * mmax));
     HIPCHK(part.alloc(std::max<int64_t>(ts_gemm_tn_part_size(n, mmax + b, b), 1)));
-    HIPCHK(small.alloc(4 * 16 * 16));
+    HIPCHK(small.alloc(5 * 16 * 16));
     HIPCHK(ierr.alloc(4));
     HIPCHK(sw.alloc(1));
     if (kneed > SYMEIG_MAX_K || mmax > SYMEIG_MAX_M)
@@ -535,123 +595,186 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     int ldT = mmax + b;
     dev_fill(T.p, (int64_t)ldT * ldT, 0.0, s);
     HIPCHK(hipMemsetAsync(ierr.p, 0, sizeof(int) * 4, s));
-    double *G1 = small.p, *G2 = small.p + 256, *R = small.p + 512;
-    // CholQR2 of block Q (n x b) -> R_out (b x b, ld ldr)
-    auto cholqr2 = [&](double *Q, double *Rout, int ldr) {
+    double *G1 = small.p, *G2 = small.p + 256, *Rt = small.p + 512, *G0 = small.p + 768, *Hs = small.p + 1024;
+    auto blk = [&](int jj) { return V.p + (size_t)jj * b * n; };
+    auto Tsub = [&](int jj) { return T.p + (size_t)(jj + 1) * b + (size_t)jj * b * ldT; };  // R of block jj+1
+    // CholQR2 of Q (n x b); with g0 the first Cholesky checks for dependent directions
+    auto cholqr2 = [&](double *Q, double *Rout, int ldr, const double *g0, int bidx) {
         ts_gemm_tn(n, b, b, Q, n, Q, n, G1, 16, 1.0, 0.0, part.p, s);
-        small_chol_upper(b, G1, 16, ierr.p, s);
+        if (g0) small_chol_upper_checked(b, G1, 16, g0, 16, dep_tol2, bidx, ierr.p, s);
+        else small_chol_upper(b, G1, 16, ierr.p, s);
         trsm_right_upper(n, b, Q, n, G1, 16, s);
         ts_gemm_tn(n, b, b, Q, n, Q, n, G2, 16, 1.0, 0.0, part.p, s);
         small_chol_upper(b, G2, 16, ierr.p, s);
         trsm_right_upper(n, b, Q, n, G2, 16, s);
         if (Rout) upper_mul(b, G2, 16, G1, 16, Rout, ldr, s);
     };
-    start_block(n, b, V.p, n, 0, s);
-    cholqr2(V.p, nullptr, 0);
-    int j = 0;
-    int next_check = std::min(mmax, std::max(2 * b, ((kneed * 3 + b - 1) / b) * b));
-    std::vector<double> hR;
-    bool converged = false;
-    int m = 0;
-    while (true) {
-        double *Qj = V.p + (size_t)j * b * n;
-        // W = L^-1 (-KG) L^-T Qj
-        HIPCHK(hipMemcpyAsync(Wb.p, Qj, sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
+    // Wb = C Q_jj, orthogonalised (CGS2) against V[:, 0:(jj+1)b]; coefficients
+    // into T's column block jj; G0 = Gram matrix after the first pass
+    auto expand = [&](int jj) {
+        HIPCHK(hipMemcpyAsync(Wb.p, blk(jj), sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
         solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Wb.p, b, s);
         launch_scatter_elim(n, ndof, b, M->d_elim_of_dof.p, Wb.p, xf.p, s);
         launch_bsr_spmv(M->n_nodes, M->d_rp.p, M->d_col.p, M->d_KG.p, xf.p, yf.p, b, -1.0, s);
         launch_gather_elim(n, ndof, b, M->d_dof_of_elim.p, yf.p, Wb.p, s);
         solve_fwd(M->plan, M->dev, M->dsolve, M->splan, M->d_P.p, M->d_W.p, M->d_part.p, Wb.p, b, s);
-        if (debug && j < 2) {
-            std::vector<double> hq((size_t)n * b), hw((size_t)n * b);
-            HIPCHK(hipMemcpyAsync(hq.data(), Qj, hq.size() * 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipMemcpyAsync(hw.data(), Wb.p, hw.size() * 8, hipMemcpyDeviceToHost, s));
-            int herr[4];
-            HIPCHK(hipMemcpyAsync(herr, ierr.p, sizeof(herr), hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            double nq = 0, nw = 0, q01 = 0;
-            for (int i = 0; i < n; i++) { nq += hq[i] * hq[i]; nw += hw[i] * hw[i]; q01 += hq[i] * hq[n + i]; }
-            fprintf(stderr, "[sol105] lanczos step %d: |q0|^2=%.6e q0.q1=%.3e |w0|^2=%.6e err=%d\n", j, nq, q01, nw,
-                    herr[0]);
-        }
-        int kcols = (j + 1) * b;
-        double *Tcol = T.p + (size_t)j * b * ldT;
+        int kcols = (jj + 1) * b;
+        double *Tcol = T.p + (size_t)jj * b * ldT;
         ts_gemm_tn(n, kcols, b, V.p, n, Wb.p, n, Tcol, ldT, 1.0, 0.0, part.p, s);
         ts_gemm_nn(n, kcols, b, V.p, n, Tcol, ldT, Wb.p, n, -1.0, 1.0, s);
-        // second Gram-Schmidt pass, accumulated into the same column block of T
+        ts_gemm_tn(n, b, b, Wb.p, n, Wb.p, n, G0, 16, 1.0, 0.0, part.p, s);
         ts_gemm_tn(n, kcols, b, V.p, n, Wb.p, n, H2.p, kcols, 1.0, 0.0, part.p, s);
         ts_gemm_nn(n, kcols, b, V.p, n, H2.p, kcols, Wb.p, n, -1.0, 1.0, s);
         mat_add(kcols, b, H2.p, kcols, Tcol, ldT, s);
+        return SOL105_OK;
+    };
+    auto fast_step = [&](int jj) {
+        int rc = expand(jj);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(blk(jj + 1), Wb.p, sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
+        cholqr2(blk(jj + 1), Tsub(jj), ldT, G0, jj + 1);
+        return SOL105_OK;
+    };
+    // rank-revealing rebuild of block jj+1 (host-assisted; rare)
+    auto robust_step = [&](int jj, int &rank) {
+        int rc = expand(jj);
+        if (rc) return rc;
+        ts_gemm_tn(n, b, b, Wb.p, n, Wb.p, n, G1, 16, 1.0, 0.0, part.p, s);
+        std::vector<double> hG0(256), hG1(256);
+        HIPCHK(hipMemcpyAsync(hG0.data(), G0, 256 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(hG1.data(), G1, 256 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::vector<double> g((size_t)b * b), w, U;
+        double gmax = 0.0;
+        for (int c = 0; c < b; c++) {
+            gmax = std::max(gmax, hG0[(size_t)c * 16 + c]);
+            for (int r = 0; r < b; r++) g[(size_t)c * b + r] = 0.5 * (hG1[(size_t)c * 16 + r] + hG1[(size_t)r * 16 + c]);
+        }
+        small_sym_eig_host(b, g, w, U);
+        int r = 0;
+        while (r < b && w[r] > dep_tol2 * gmax && w[r] > 0.0) r++;
+        rank = r;
+        std::vector<double> hH(256, 0.0), hR(256, 0.0);
+        for (int c = 0; c < r; c++) {
+            double sq = std::sqrt(w[c]);
+            for (int k = 0; k < b; k++) {
+                hH[(size_t)c * 16 + k] = U[(size_t)c * b + k] / sq;   // H = U_r S^-1/2
+                hR[(size_t)k * 16 + c] = sq * U[(size_t)c * b + k];   // R rows = S^1/2 U_r^T
+            }
+        }
+        HIPCHK(hipMemcpyAsync(Hs, hH.data(), 256 * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(Rt, hR.data(), 256 * sizeof(double), hipMemcpyHostToDevice, s));
+        double *Qn = blk(jj + 1);
+        if (r > 0) ts_gemm_nn(n, b, r, Wb.p, n, Hs, 16, Qn, n, 1.0, 0.0, s);
+        if (r < b) {
+            double *Z = Qn + (size_t)r * n;
+            int nz = b - r, kc = (jj + 1) * b + r;
+            hash_fill(n, nz, Z, n, 0x5EED00ull + (uint64_t)jj, s);
+            for (int pass = 0; pass < 2; pass++) {
+                ts_gemm_tn(n, kc, nz, V.p, n, Z, n, H2.p, kc, 1.0, 0.0, part.p, s);
+                ts_gemm_nn(n, kc, nz, V.p, n, H2.p, kc, Z, n, -1.0, 1.0, s);
+            }
+            for (int pass = 0; pass < 2; pass++) {
+                ts_gemm_tn(n, nz, nz, Z, n, Z, n, G1, 16, 1.0, 0.0, part.p, s);
+                small_chol_upper(nz, G1, 16, ierr.p, s);
+                trsm_right_upper(n, nz, Z, n, G1, 16, s);
+            }
+        }
+        // refinement of the whole block: Qn = Qn R2^-1, R = R2 R
+        ts_gemm_tn(n, b, b, Qn, n, Qn, n, G2, 16, 1.0, 0.0, part.p, s);
+        small_chol_upper(b, G2, 16, ierr.p, s);
+        trsm_right_upper(n, b, Qn, n, G2, 16, s);
+        upper_mul(b, G2, 16, Rt, 16, Tsub(jj), ldT, s);
+        return SOL105_OK;
+    };
+    start_block(n, b, V.p, n, 0, s);
+    cholqr2(V.p, nullptr, 0, nullptr, 0);
+    int j = 0;
+    int next_check = std::min(mmax, std::max(2 * b, ((kneed * 3 + b - 1) / b) * b));
+    std::vector<double> hR;
+    bool converged = false;
+    int m = 0, injected = 0;
+    while (true) {
+        int rc = fast_step(j);
+        if (rc) return rc;
         j++;
         m = j * b;
-        {
-            double *Qn = V.p + (size_t)j * b * n;
-            HIPCHK(hipMemcpyAsync(Qn, Wb.p, sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
-            cholqr2(Qn, T.p + (size_t)j * b + (size_t)(j - 1) * b * ldT, ldT);
-        }
         bool last = (m >= mmax);
-        if (m >= next_check || last) {
-            symmetrize_copy(m, ldT, T.p, A.p, s);
-            // the kchk Ritz pairs of largest |theta| (A is overwritten)
-            const int kchk = std::min(kneed, m);
-            sym_eig_top(m, A.p, kchk, thd.p, Zs.p, ework.p, sw.p, s);
-            HIPCHK(hipGetLastError());
-            std::vector<double> th(kchk), hZl((size_t)b * kchk);
-            HIPCHK(hipMemcpyAsync(th.data(), thd.p, sizeof(double) * kchk, hipMemcpyDeviceToHost, s));
-            int eig_info = 0;
-            HIPCHK(hipMemcpyAsync(&eig_info, sw.p, sizeof(int), hipMemcpyDeviceToHost, s));
-            // last block row of the Ritz vectors in the Lanczos basis
-            HIPCHK(hipMemcpy2DAsync(hZl.data(), b * sizeof(double), Zs.p + (m - b), (size_t)m * sizeof(double),
-                                    b * sizeof(double), kchk, hipMemcpyDeviceToHost, s));
-            hR.assign((size_t)b * b, 0.0);
-            HIPCHK(hipMemcpy2DAsync(hR.data(), b * sizeof(double), T.p + (size_t)m + (size_t)(m - b) * ldT,
-                                    ldT * sizeof(double), b * sizeof(double), b, hipMemcpyDeviceToHost, s));
-            int herr[4];
-            HIPCHK(hipMemcpyAsync(herr, ierr.p, sizeof(herr), hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            if (eig_info != 0)
-                return set_err(ctx, SOL105_E_NOCONV, "projected eigenproblem: inverse iteration failed (m=%d)", m);
-            // residual of Ritz pair t: || R_j * Z[m-b:m, t] ||
-            auto resid = [&](int t) {
-                double r2 = 0.0;
-                for (int p = 0; p < b; p++) {
-                    double v = 0.0;
-                    for (int q = 0; q < b; q++) v += hR[(size_t)q * b + p] * hZl[(size_t)t * b + q];
-                    r2 += v * v;
-                }
-                return std::sqrt(r2);
-            };
-            bool ok = true;
-            if (debug) {
-                fprintf(stderr, "[sol105] RR m=%d err=%d theta:", m, herr[0]);
-                for (int t = 0; t < std::min(kchk, 6); t++) fprintf(stderr, " %.6e(r=%.2e)", th[t], resid(t));
-                fprintf(stderr, "\n");
-            }
-            for (int t = 0; t < kchk; t++) {
-                double r = resid(t);
-                if (!std::isfinite(th[t]) || !std::isfinite(r)) { ok = false; break; }
-                if (std::fabs(th[t]) < 1e-300) continue;
-                if (r > tol * std::fabs(th[t])) { ok = false; break; }
-            }
-            bool breakdown = (herr[0] & 2) != 0;
-            if (breakdown && m < n)
-                return set_err(ctx, SOL105_E_NOCONV, "block Lanczos breakdown (rank-deficient block) at %d vectors", m);
-            if (ok || last || breakdown) {
-                converged = ok;
-                int k = std::min(want, kchk);
-                out.theta.assign(th.begin(), th.begin() + k);
-                out.resid.clear();
-                for (int t = 0; t < k; t++) out.resid.push_back(resid(t));
-                // Ritz vectors Y = V_m Z[:, 0:k]
-                ts_gemm_nn(n, m, k, V.p, n, Zs.p, m, d_Y, n, 1.0, 0.0, s);
-                HIPCHK(hipStreamSynchronize(s));
-                break;
-            }
-            next_check = std::min(mmax, m + 4 * b);
+        if (!(m >= next_check || last)) continue;
+        symmetrize_copy(m, ldT, T.p, A.p, s);
+        // the kchk Ritz pairs of largest |theta| (A is overwritten)
+        const int kchk = std::min(kneed, m);
+        sym_eig_top(m, A.p, kchk, thd.p, Zs.p, ework.p, sw.p, s);
+        HIPCHK(hipGetLastError());
+        std::vector<double> th(kchk), hZl((size_t)b * kchk);
+        HIPCHK(hipMemcpyAsync(th.data(), thd.p, sizeof(double) * kchk, hipMemcpyDeviceToHost, s));
+        int eig_info = 0;
+        HIPCHK(hipMemcpyAsync(&eig_info, sw.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        // last block row of the Ritz vectors in the Lanczos basis
+        HIPCHK(hipMemcpy2DAsync(hZl.data(), b * sizeof(double), Zs.p + (m - b), (size_t)m * sizeof(double),
+                                b * sizeof(double), kchk, hipMemcpyDeviceToHost, s));
+        hR.assign((size_t)b * b, 0.0);
+        HIPCHK(hipMemcpy2DAsync(hR.data(), b * sizeof(double), T.p + (size_t)m + (size_t)(m - b) * ldT,
+                                ldT * sizeof(double), b * sizeof(double), b, hipMemcpyDeviceToHost, s));
+        int herr[4];
+        HIPCHK(hipMemcpyAsync(herr, ierr.p, sizeof(herr), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (herr[0] & 4) {
+            // a block became rank-deficient: rebuild it and continue from there
+            int jb = herr[1] - 1;
+            HIPCHK(hipMemsetAsync(ierr.p, 0, sizeof(int) * 4, s));
+            int rank = b;
+            rc = robust_step(jb - 1, rank);
+            if (rc) return rc;
+            injected += b - rank;
+            if (debug) fprintf(stderr, "[sol105] block %d rank-deficient: rank %d, %d directions injected\n", jb, rank,
+                               b - rank);
+            j = jb;
+            m = j * b;
+            continue;
         }
+        if (herr[0] & 2)
+            return set_err(ctx, SOL105_E_NOCONV, "block Lanczos: Cholesky-QR breakdown at %d vectors", m);
+        if (eig_info != 0)
+            return set_err(ctx, SOL105_E_NOCONV, "projected eigenproblem: inverse iteration failed (m=%d)", m);
+        // residual of Ritz pair t: || R_j * Z[m-b:m, t] ||
+        auto resid = [&](int t) {
+            double r2 = 0.0;
+            for (int p = 0; p < b; p++) {
+                double v = 0.0;
+                for (int q = 0; q < b; q++) v += hR[(size_t)q * b + p] * hZl[(size_t)t * b + q];
+                r2 += v * v;
+            }
+            return std::sqrt(r2);
+        };
+        bool ok = true;
+        if (debug) {
+            fprintf(stderr, "[sol105] RR m=%d theta:", m);
+            for (int t = 0; t < std::min(kchk, 6); t++) fprintf(stderr, " %.6e(r=%.2e)", th[t], resid(t));
+            fprintf(stderr, "\n");
+        }
+        for (int t = 0; t < kchk; t++) {
+            double r = resid(t);
+            if (!std::isfinite(th[t]) || !std::isfinite(r)) { ok = false; break; }
+            if (std::fabs(th[t]) < 1e-300) continue;
+            if (r > tol * std::fabs(th[t])) { ok = false; break; }
+        }
+        if (ok || last) {
+            converged = ok;
+            int k = std::min(want, kchk);
+            out.theta.assign(th.begin(), th.begin() + k);
+            out.resid.clear();
+            for (int t = 0; t < k; t++) out.resid.push_back(resid(t));
+            // Ritz vectors Y = V_m Z[:, 0:k]
+            ts_gemm_nn(n, m, k, V.p, n, Zs.p, m, d_Y, n, 1.0, 0.0, s);
+            HIPCHK(hipStreamSynchronize(s));
+            break;
+        }
+        next_check = std::min(mmax, m + 4 * b);
     }
     out.m = m;
     out.steps = j;
+    out.injected = injected;
     if (!converged) return set_err(ctx, SOL105_E_NOCONV, "block Lanczos did not converge in %d vectors", m);
     return SOL105_OK;
 }
@@ -785,8 +908,16 @@ int sol105_solve(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double *la
     return SOL105_OK;
 }
 
-int sol105_sensitivity_thickness(sol105_ctx *ctx, sol105_model *M, int32_t mode, double *dlam_dt) {
-    if (!ctx || !M || !dlam_dt) return set_err(ctx, SOL105_E_INVALID, "null argument");
+}  // extern "C"
+
+namespace {
+
+// Shared part of the sensitivity entry points: adjoint load g (ndof, global
+// frame, zero on fixed DOFs), psi = K_ff^-1 g, the three per-element terms
+// and phi' KG phi.
+int sens_terms(sol105_ctx *ctx, sol105_model *M, int32_t mode, DevBuf<double> &terms, DevBuf<double> &g,
+               double &pkp) {
+    if (!ctx || !M) return set_err(ctx, SOL105_E_INVALID, "null argument");
     if (mode < 0 || mode >= M->n_modes) return set_err(ctx, SOL105_E_STATE, "mode %d not available", mode);
     (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
@@ -794,7 +925,7 @@ int sol105_sensitivity_thickness(sol105_ctx *ctx, sol105_model *M, int32_t mode,
     const double *phi = M->d_modes.p + (size_t)mode * ndof;
     double lam = M->lambda[mode];
     ElemArgs a = elem_args(M);
-    DevBuf<double> egq, egb, g, ge, psi, kgphi, scr, dots;
+    DevBuf<double> egq, egb, ge, psi, kgphi, scr, dots, phif;
     HIPCHK(egq.alloc((size_t)std::max(M->n_q4, 1) * 24));
     HIPCHK(egb.alloc((size_t)std::max(M->n_bar, 1) * 12));
     HIPCHK(g.alloc(ndof));
@@ -803,6 +934,8 @@ int sol105_sensitivity_thickness(sol105_ctx *ctx, sol105_model *M, int32_t mode,
     HIPCHK(kgphi.alloc(ndof));
     HIPCHK(scr.alloc(512));
     HIPCHK(dots.alloc(2));
+    HIPCHK(phif.alloc(ndof));
+    HIPCHK(terms.alloc((size_t)std::max(M->n_q4, 1) * 3));
     launch_q4_adjoint_rhs(a, M->d_u.p, phi, lam, egq.p, s);
     launch_bar_adjoint_rhs(a, phi, lam, egb.p, s);
     node_gather_vec(M->n_nodes, M->n_q4, M->d_diag.p, M->d_cptr.p, M->d_cidx.p, egq.p, egb.p, M->d_free.p, g.p, s);
@@ -810,21 +943,46 @@ int sol105_sensitivity_thickness(sol105_ctx *ctx, sol105_model *M, int32_t mode,
     tri_solve(M, ge.p, 1, s);
     launch_scatter_elim(n, ndof, 1, M->d_elim_of_dof.p, ge.p, psi.p, s);
     // phi' KG phi on the free set
-    DevBuf<double> phif;
-    HIPCHK(phif.alloc(ndof));
     launch_gather_elim(n, ndof, 1, M->d_dof_of_elim.p, phi, ge.p, s);
     launch_scatter_elim(n, ndof, 1, M->d_elim_of_dof.p, ge.p, phif.p, s);
     launch_bsr_spmv(M->n_nodes, M->d_rp.p, M->d_col.p, M->d_KG.p, phif.p, kgphi.p, 1, 1.0, s);
     dot_det(ndof, phif.p, kgphi.p, scr.p, dots.p, s);
-    double pkp = 0.0;
+    launch_q4_sens_terms(a, M->d_u.p, phif.p, psi.p, terms.p, s);
     HIPCHK(hipMemcpyAsync(&pkp, dots.p, sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    DevBuf<double> dl;
+    HIPCHK(hipGetLastError());
+    return SOL105_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sol105_sensitivity_thickness(sol105_ctx *ctx, sol105_model *M, int32_t mode, double *dlam_dt) {
+    if (!dlam_dt) return set_err(ctx, SOL105_E_INVALID, "null argument");
+    DevBuf<double> terms, g, dl;
+    double pkp = 0.0;
+    int rc = sens_terms(ctx, M, mode, terms, g, pkp);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
     HIPCHK(dl.alloc(std::max(M->n_q4, 1)));
-    launch_q4_sens(a, M->d_u.p, phif.p, psi.p, lam, pkp, dl.p, s);
+    launch_sens_combine(M->n_q4, terms.p, M->lambda[mode], pkp, dl.p, s);
     HIPCHK(hipMemcpyAsync(dlam_dt, dl.p, sizeof(double) * M->n_q4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    HIPCHK(hipGetLastError());
+    return SOL105_OK;
+}
+
+int sol105_sensitivity_terms(sol105_ctx *ctx, sol105_model *M, int32_t mode, double *terms_out,
+                             double *adjoint_load, double *phi_kg_phi) {
+    DevBuf<double> terms, g;
+    double pkp = 0.0;
+    int rc = sens_terms(ctx, M, mode, terms, g, pkp);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    if (terms_out) HIPCHK(hipMemcpyAsync(terms_out, terms.p, sizeof(double) * 3 * M->n_q4, hipMemcpyDeviceToHost, s));
+    if (adjoint_load) HIPCHK(hipMemcpyAsync(adjoint_load, g.p, sizeof(double) * M->ndof(), hipMemcpyDeviceToHost, s));
+    if (phi_kg_phi) *phi_kg_phi = pkp;
+    HIPCHK(hipStreamSynchronize(s));
     return SOL105_OK;
 }
 

@@ -33,8 +33,11 @@ void launch_block_gather(int64_t n_blocks, const int32_t *cptr, const int32_t *c
                          double *vals, hipStream_t s);
 
 // dlambda/dt kernels (elements.hip)
-void launch_q4_sens(const ElemArgs &a, const double *u, const double *phi, const double *psi,
-                    double lam, double phiKGphi, double *dldt, hipStream_t s);
+// terms[3e..3e+2] = phi' dKe/dt phi, phi' KGe phi / t, psi' dKe/dt u (global frame vectors)
+void launch_q4_sens_terms(const ElemArgs &a, const double *u, const double *phi, const double *psi,
+                          double *terms /* n_q4 x 3 */, hipStream_t s);
+// dldt[e] = -(terms[3e] + lam terms[3e+1] - terms[3e+2]) / phiKGphi
+void launch_sens_combine(int n_q4, const double *terms, double lam, double phiKGphi, double *dldt, hipStream_t s);
 void launch_q4_adjoint_rhs(const ElemArgs &a, const double *u, const double *phi, double lam,
                            double *eg /* n_q4 x 24 */, hipStream_t s);
 void launch_bar_adjoint_rhs(const ElemArgs &a, const double *phi, double lam, double *eg /* n_bar x 12 */,

@@ -12,6 +12,12 @@ int64_t ts_gemm_tn_part_size(int n, int P, int Q);
 void ts_gemm_nn(int n, int P, int Q, const double *V, int ldv, const double *H, int ldh, double *Y, int ldy,
                 double alpha, double beta, hipStream_t s);
 void small_chol_upper(int b, double *G, int ldg, int *err, hipStream_t s);
+// as small_chol_upper, but a pivot <= tol2 * max diag(g0) flags block blk as
+// rank-deficient (err[0] |= 4, err[1] = min(blk + 1))
+void small_chol_upper_checked(int b, double *G, int ldg, const double *g0, int ldg0, double tol2, int blk, int *err,
+                              hipStream_t s);
+// V[:, 0:cols] = seeded uniform values in [-1, 1)
+void hash_fill(int n, int cols, double *V, int ldv, uint64_t seed, hipStream_t s);
 void trsm_right_upper(int n, int b, double *W, int ldw, const double *R, int ldr, hipStream_t s);
 void upper_mul(int b, const double *R2, int ld2, const double *R1, int ld1, double *out, int ldo, hipStream_t s);
 void start_block(int n, int b, double *V, int ldv, int seed, hipStream_t s);

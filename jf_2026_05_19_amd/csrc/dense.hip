@@ -77,21 +77,57 @@ __global__ void __launch_bounds__(256) nn_kernel(int n, int P, int Q, const doub
         }
 }
 
-// Cholesky G = R^T R (upper R, b <= 16) in place; single thread.
-__global__ void chol_upper_kernel(int b, double *G, int ldg, int *err) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Cholesky G = R^T R (upper R, b <= 16) in place, one wavefront: step j
+// forms the Schur-complement row j on lanes j..b-1.  Without g0, a pivot
+// d <= 0 sets err[0] |= 2.  With g0 (the Gram matrix of the block before
+// Gram-Schmidt, ld ldg0), a pivot d <= tol2 * max_i g0_ii marks a direction
+// that Gram-Schmidt reduced to rounding noise: err[0] |= 4 and err[1] keeps
+// the smallest (blk + 1) seen.  Pivots are floored so no NaN is produced.
+__global__ void __launch_bounds__(64) chol_upper_kernel(int b, double *G, int ldg, const double *g0, int ldg0,
+                                                        double tol2, int blk, int *err) {
+    __shared__ double R[16][17];
+    __shared__ double dsh;
+    const int t = threadIdx.x;
+    for (int q = t; q < 256; q += 64) {
+        int i = q % 16, j = q / 16;
+        R[i][j] = (i < b && j < b) ? G[i + j * ldg] : 0.0;
+    }
+    double thr = 0.0;
+    if (g0)
+        for (int i = 0; i < b; i++) thr = fmax(thr, g0[i + i * ldg0]);
+    thr *= tol2;
+    int flag = 0;
+    __syncthreads();
     for (int j = 0; j < b; j++) {
-        double d = G[j + j * ldg];
-        for (int k = 0; k < j; k++) d -= G[k + j * ldg] * G[k + j * ldg];
-        if (!(d > 0.0)) { *err |= 2; d = 1e-300; }
-        double r = sqrt(d);
-        G[j + j * ldg] = r;
-        for (int i = j + 1; i < b; i++) {
-            double s = G[j + i * ldg];
-            for (int k = 0; k < j; k++) s -= G[k + j * ldg] * G[k + i * ldg];
-            G[j + i * ldg] = s / r;
+        double sv = 0.0;
+        if (t >= j && t < b) {
+            sv = R[j][t];
+            for (int k = 0; k < j; k++) sv -= R[k][j] * R[k][t];
         }
-        for (int i = j + 1; i < b; i++) G[i + j * ldg] = 0.0;
+        if (t == j) dsh = sv;
+        __syncthreads();
+        double d = dsh;
+        if (g0 ? !(d > thr) : !(d > 0.0)) {
+            flag = 1;
+            d = fmax(fmax(d, thr), 1e-300);
+        }
+        const double r = sqrt(d);
+        if (t > j && t < b) R[j][t] = sv / r;
+        if (t == j) R[j][j] = r;
+        if (t < j) R[j][t] = 0.0;
+        __syncthreads();
+    }
+    for (int q = t; q < b * b; q += 64) {
+        int i = q % b, j = q / b;
+        G[i + j * ldg] = (i <= j) ? R[i][j] : 0.0;
+    }
+    if (t == 0 && flag) {
+        if (g0) {
+            err[0] |= 4;
+            if (err[1] == 0 || blk + 1 < err[1]) err[1] = blk + 1;
+        } else {
+            err[0] |= 2;
+        }
     }
 }
 
@@ -143,6 +179,14 @@ __device__ __forceinline__ double hash_unit(uint64_t k) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z = z ^ (z >> 31);
     return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+// V[:, c] = uniform [-1, 1) hash values (counter-based, seeded)
+__global__ void hash_fill_kernel(int n, int cols, double *V, int ldv, uint64_t seed) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * cols) return;
+    int i = (int)(t % n), c = (int)(t / n);
+    V[i + (int64_t)c * ldv] = hash_unit((seed << 40) ^ ((uint64_t)c << 32) ^ (uint64_t)i);
 }
 
 __global__ void start_block_kernel(int n, int b, double *V, int ldv, int seed) {
@@ -252,7 +296,16 @@ void ts_gemm_nn(int n, int P, int Q, const double *V, int ldv, const double *H, 
 }
 
 void small_chol_upper(int b, double *G, int ldg, int *err, hipStream_t s) {
-    chol_upper_kernel<<<1, 64, 0, s>>>(b, G, ldg, err);
+    chol_upper_kernel<<<1, 64, 0, s>>>(b, G, ldg, nullptr, 0, 0.0, 0, err);
+}
+
+void small_chol_upper_checked(int b, double *G, int ldg, const double *g0, int ldg0, double tol2, int blk, int *err,
+                              hipStream_t s) {
+    chol_upper_kernel<<<1, 64, 0, s>>>(b, G, ldg, g0, ldg0, tol2, blk, err);
+}
+
+void hash_fill(int n, int cols, double *V, int ldv, uint64_t seed, hipStream_t s) {
+    if ((int64_t)n * cols > 0) hash_fill_kernel<<<nblk((int64_t)n * cols, 256), 256, 0, s>>>(n, cols, V, ldv, seed);
 }
 
 void trsm_right_upper(int n, int b, double *W, int ldw, const double *R, int ldr, hipStream_t s) {

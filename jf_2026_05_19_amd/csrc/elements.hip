@@ -900,7 +900,7 @@ __device__ __forceinline__ double q4_gvec(const ElemArgs &A, const double *x, in
 }
 
 __global__ void __launch_bounds__(256) q4_dke_kernel(ElemArgs A, const double *u, const double *phi,
-                                                     const double *psi, double *out2 /* n_q4 x 2 */) {
+                                                     const double *psi, double *terms /* n_q4 x 3 */) {
     __shared__ Q4Stage stage[EPB];
     int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
     int e = blockIdx.x * EPB + w;
@@ -946,12 +946,12 @@ __global__ void __launch_bounds__(256) q4_dke_kernel(ElemArgs A, const double *u
     }
     f1 = wave_sum(f1);
     f2 = wave_sum(f2);
-    if (valid && lane == 0) { out2[2 * e] = f1; out2[2 * e + 1] = f2; }
+    if (valid && lane == 0) { terms[3 * e] = f1; terms[3 * e + 2] = f2; }
 }
 
 // phi' KGe phi / t per element (the dKG/dt|u term)
 __global__ void __launch_bounds__(256) q4_kgt_kernel(ElemArgs A, const double *u, const double *phi,
-                                                     double *out /* n_q4 */) {
+                                                     double *terms /* n_q4 x 3 */) {
     __shared__ Q4KGStage stage[EPB];
     int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
     int e = blockIdx.x * EPB + w;
@@ -980,7 +980,7 @@ __global__ void __launch_bounds__(256) q4_kgt_kernel(ElemArgs A, const double *u
             }
     }
     f = wave_sum(f);
-    if (valid && lane == 0) out[e] = f / h;
+    if (valid && lane == 0) terms[3 * e + 1] = f / h;
 }
 
 // Forward-mode dual number with the three membrane-stress directions.
@@ -1068,9 +1068,12 @@ __global__ void __launch_bounds__(256) q4_adjrhs_kernel(ElemArgs A, const double
             l[0] = mean + rad; c[0] = cc; s[0] = ss;
             l[1] = mean - rad; c[1] = dconst(0.0) - ss; s[1] = cc;
         }
+        // The forward operator drops a principal stress with |l| <= 1e-30;
+        // its value is then zero either way, but its derivative dl/dsigma * q
+        // is not, so every direction is kept here (uniaxial states have a
+        // minor principal stress at rounding level, often exactly 0).
         Dual3 Phi = dconst(0.0);
         for (int k = 0; k < 2; k++) {
-            if (fabs(l[k].v) <= 1e-30) continue;
             Dual3 gx = c[k] * dconst(X1) + s[k] * dconst(X2);
             Dual3 gy = c[k] * dconst(Y1) + s[k] * dconst(Y2);
             Dual3 gz = c[k] * dconst(Z1) + s[k] * dconst(Z2);
@@ -1146,11 +1149,11 @@ __global__ void bar_adjrhs_kernel(ElemArgs A, const double *phi, double lam, dou
         }
 }
 
-__global__ void sens_combine_kernel(int n, const double *dke2, const double *kgt, double lam, double phiKGphi,
-                                    double *dldt) {
+// terms[3e..3e+2] = phi' dKe/dt phi, phi' KGe phi / t, psi' dKe/dt u
+__global__ void sens_combine_kernel(int n, const double *terms, double lam, double phiKGphi, double *dldt) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
-    double num = dke2[2 * e] + lam * kgt[e] - dke2[2 * e + 1];
+    double num = terms[3 * e] + lam * terms[3 * e + 1] - terms[3 * e + 2];
     dldt[e] = -num / phiKGphi;
 }
 
@@ -1178,16 +1181,15 @@ void launch_block_gather(int64_t n_blocks, const int32_t *cptr, const int32_t *c
     if (n == 0) return;
     block_gather_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n_blocks, cptr, cidx, eb, vals);
 }
-void launch_q4_sens(const ElemArgs &a, const double *u, const double *phi, const double *psi, double lam,
-                    double phiKGphi, double *dldt, hipStream_t s) {
+void launch_q4_sens_terms(const ElemArgs &a, const double *u, const double *phi, const double *psi,
+                          double *terms /* n_q4 x 3 */, hipStream_t s) {
     if (a.n_q4 == 0) return;
-    double *tmp = nullptr;
-    (void)hipMallocAsync((void **)&tmp, sizeof(double) * 3 * (size_t)a.n_q4, s);
-    q4_dke_kernel<<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, psi, tmp);
-    q4_kgt_kernel<<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, tmp + 2 * (size_t)a.n_q4);
-    sens_combine_kernel<<<(a.n_q4 + 255) / 256, 256, 0, s>>>(a.n_q4, tmp, tmp + 2 * (size_t)a.n_q4, lam, phiKGphi,
-                                                             dldt);
-    (void)hipFreeAsync(tmp, s);
+    q4_dke_kernel<<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, psi, terms);
+    q4_kgt_kernel<<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, terms);
+}
+void launch_sens_combine(int n_q4, const double *terms, double lam, double phiKGphi, double *dldt, hipStream_t s) {
+    if (n_q4 == 0) return;
+    sens_combine_kernel<<<(n_q4 + 255) / 256, 256, 0, s>>>(n_q4, terms, lam, phiKGphi, dldt);
 }
 void launch_q4_adjoint_rhs(const ElemArgs &a, const double *u, const double *phi, double lam, double *eg,
                            hipStream_t s) {
