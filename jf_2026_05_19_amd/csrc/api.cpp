@@ -92,7 +92,7 @@ struct sol105_model {
     DevFronts dev;
     SolvePlan splan;
     DevBuf<int64_t> s_poff;
-    DevBuf<int32_t> s_ftiles, s_btiles, s_ftbase, s_btbase, s_fnks, s_bnks;
+    DevBuf<int32_t> s_ftiles, s_btiles, s_ftbase, s_btbase, s_fnks, s_bnks, s_pullptr, s_pullsrc;
     DevBuf<double> d_P, d_PT, d_part;
     DevSolve dsolve;
     // state
@@ -102,6 +102,7 @@ struct sol105_model {
     DevBuf<double> d_modes;  // ndof x n_modes (normalised, global frame)
     int n_modes = 0;
     double lanczos_m = 0, lanczos_steps = 0;
+    int warm_m = 0, warm_b = 0;  // Lanczos size at the last convergence (first check there next time)
     int ndof() const { return 6 * n_nodes; }
 };
 
@@ -221,6 +222,8 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
         HIPCHK(M->s_btbase.upload(SP.bwd_tbase.data(), SP.bwd_tbase.size(), s));
         HIPCHK(M->s_fnks.upload(SP.fwd_nks.data(), SP.fwd_nks.size(), s));
         HIPCHK(M->s_bnks.upload(SP.bwd_nks.data(), SP.bwd_nks.size(), s));
+        HIPCHK(M->s_pullptr.upload(SP.pull_ptr.data(), SP.pull_ptr.size(), s));
+        HIPCHK(M->s_pullsrc.upload(SP.pull_src.data(), SP.pull_src.size(), s));
         HIPCHK(M->d_P.alloc(std::max<int64_t>(SP.p_storage, 1)));
         HIPCHK(M->d_PT.alloc(std::max<int64_t>(SP.p_storage, 1)));
         HIPCHK(M->d_part.alloc(std::max<int64_t>(solve_part_size(SP), 1)));
@@ -232,6 +235,8 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
         DS.bwd_tbase_rel = M->s_btbase.p;
         DS.fwd_nks = M->s_fnks.p;
         DS.bwd_nks = M->s_bnks.p;
+        DS.pull_ptr = M->s_pullptr.p;
+        DS.pull_src = M->s_pullsrc.p;
     }
     M->have_sym = true;
     return SOL105_OK;
@@ -690,7 +695,11 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     start_block(n, b, V.p, n, 0, s);
     cholqr2(V.p, nullptr, 0, nullptr, 0);
     int j = 0;
-    int next_check = std::min(mmax, std::max(2 * b, ((kneed * 3 + b - 1) / b) * b));
+    // first Rayleigh-Ritz check: where the last solve of this model converged
+    // (design loops and repeated solves), else at about 3 vectors per wanted pair
+    const bool warm = M->warm_m > 0 && M->warm_b == b;
+    int next_check = warm ? std::min(mmax, std::max(b, M->warm_m))
+                          : std::min(mmax, std::max(2 * b, ((kneed * 3 + b - 1) / b) * b));
     std::vector<double> hR;
     bool converged = false;
     int m = 0, injected = 0;
@@ -770,12 +779,14 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
             HIPCHK(hipStreamSynchronize(s));
             break;
         }
-        next_check = std::min(mmax, m + 4 * b);
+        next_check = std::min(mmax, m + (warm ? b : 4 * b));
     }
     out.m = m;
     out.steps = j;
     out.injected = injected;
     if (!converged) return set_err(ctx, SOL105_E_NOCONV, "block Lanczos did not converge in %d vectors", m);
+    M->warm_m = m;
+    M->warm_b = b;
     return SOL105_OK;
 }
 

@@ -40,6 +40,10 @@ struct SolvePlan {
     std::vector<int32_t> fwd_nks, bwd_nks;      // per front: k slices
     std::vector<SolveHeight> heights;
     int64_t max_height_tiles = 0;
+    // forward-assembly pull map: row i of front f (vector slot vec_off[f] + i)
+    // adds the child update rows pull_src[pull_ptr[slot] .. pull_ptr[slot+1])
+    // as (child front, row in the child) pairs, in fixed child order
+    std::vector<int32_t> pull_ptr, pull_src;
 };
 
 inline int64_t solve_part_size(const SolvePlan &S) { return S.max_height_tiles * SOLVE_RT * MAXR; }

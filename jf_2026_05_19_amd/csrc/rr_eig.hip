@@ -2,16 +2,23 @@
 // the k eigenpairs of largest |theta| of the symmetric projected matrix
 // (m x m, m <= 1024), in one 1024-thread workgroup.
 //   1. Householder tridiagonalisation (Golub & Van Loan, Matrix Computations,
-//      4th ed., Alg. 5.1.1 `house` and Alg. 8.3.1): one wavefront per column
-//      for the symmetric matrix-vector product, coalesced rank-2 update;
+//      4th ed., Alg. 5.1.1 `house` and Alg. 8.3.1).  The matrix stays in
+//      global memory (L2); every wavefront handles CPI columns per round
+//      with all of its loads issued before any use, so one L2 round trip
+//      covers CPI columns of the symmetric matrix-vector product and of the
+//      rank-2 update;
 //   2. eigenvalues of the tridiagonal matrix at the 2k extreme indices by
-//      Sturm-sequence bisection (Wilkinson, The Algebraic Eigenvalue Problem,
-//      ch. 5), one lane per index; the k of largest |theta| are kept;
+//      Sturm-sequence multisection (Wilkinson, The Algebraic Eigenvalue
+//      Problem, ch. 5): G lanes per index evaluate G interior points per
+//      round, a ballot picks the subinterval (log2(G+1) bits per round);
 //   3. eigenvectors by inverse iteration with a partially pivoted
-//      tridiagonal LU (Golub & Van Loan Alg. 4.3.x / 8.2), one lane per
-//      cluster of close eigenvalues, modified Gram-Schmidt inside a cluster
-//      (the scheme of Wilkinson and of LAPACK's stein);
-//   4. back-transformation of the k vectors with the stored reflectors.
+//      tridiagonal LU (Golub & Van Loan sec. 4.3), one lane per cluster of
+//      close eigenvalues with its LU factors and vector in LDS, modified
+//      Gram-Schmidt inside a cluster (the scheme of Wilkinson and of
+//      LAPACK's stein);
+//   4. back-transformation of the k vectors (held in LDS) with the stored
+//      reflectors, the next reflector prefetched while the current one is
+//      applied.
 // Only k vectors are formed, so the cost is the O(m^3) reduction of step 1
 // plus O(k m^2); the large cluster of near-zero Ritz values that a Lanczos
 // basis carries does not slow any step down.
@@ -25,6 +32,7 @@ namespace {
 constexpr int NT = 1024;
 constexpr int NW = NT / 64;
 constexpr double EPS = DBL_EPSILON;
+constexpr int SCR = 12288;  // LDS scratch (doubles) for steps 3 and 4
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -65,11 +73,14 @@ __device__ __forceinline__ double hash_unit(uint64_t k) {
     return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
+// RPL: rows per lane of a column (m <= 64 RPL + 1); CPI: columns per wavefront round
+template <int RPL, int CPI>
 __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k, double *theta, double *Zs,
                                                         double *work, int *info) {
     __shared__ double vsh[NT];
     __shared__ double psh[NT];
     __shared__ double dd[NT], ee[NT], e2[NT];
+    __shared__ double scr[SCR];
     __shared__ double ev[2 * SYMEIG_MAX_K];  // eigenvalues at the extreme indices
     __shared__ int sel[SYMEIG_MAX_K];        // chosen index, descending |theta|
     __shared__ int asc[SYMEIG_MAX_K];        // chosen indices, ascending
@@ -77,6 +88,7 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
     __shared__ int cl_start[SYMEIG_MAX_K + 1];
     __shared__ double wred[NW], wlo[NW], whi[NW], wem[NW];
     __shared__ double bnd[4];
+    __shared__ double sx1;
     __shared__ int sint[4];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     double *tau = work;
@@ -86,34 +98,57 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
     for (int j = 0; j + 2 < m; j++) {
         const int n2 = m - j - 1;
         double *x = A + (j + 1) + (int64_t)j * m;
-        double part = 0.0;
-        for (int i = 1 + tid; i < n2; i += NT) part += x[i] * x[i];
-        const double sigma = block_sum(part, wred);
-        const double x1 = x[0];
-        double beta = 0.0, mu = x1;
+        const double xi = (tid < n2) ? x[tid] : 0.0;
+        if (tid == 0) sx1 = xi;
+        const double sigma = block_sum((tid >= 1 && tid < n2) ? xi * xi : 0.0, wred);
+        const double x1 = sx1;
+        double beta = 0.0, mu = x1, v1 = 1.0;
         if (sigma > 0.0) {
             mu = sqrt(x1 * x1 + sigma);
-            double v1 = (x1 <= 0.0) ? x1 - mu : -sigma / (x1 + mu);
+            v1 = (x1 <= 0.0) ? x1 - mu : -sigma / (x1 + mu);
             beta = 2.0 * v1 * v1 / (sigma + v1 * v1);
-            for (int i = tid; i < n2; i += NT) vsh[i] = (i == 0) ? 1.0 : x[i] / v1;
-        } else {
-            for (int i = tid; i < n2; i += NT) vsh[i] = (i == 0) ? 1.0 : 0.0;
         }
-        __syncthreads();
+        if (tid < n2) {
+            const double v = (tid == 0) ? 1.0 : ((sigma > 0.0) ? xi / v1 : 0.0);
+            vsh[tid] = v;
+            x[tid] = v;
+        }
         if (tid == 0) {
             dd[j] = A[j + (int64_t)j * m];
             ee[j] = mu;
             tau[j] = beta;
         }
-        for (int i = tid; i < n2; i += NT) x[i] = vsh[i];
+        __syncthreads();
         if (beta != 0.0) {
-            // p = beta * A22 v, one wavefront per column of the symmetric A22
-            for (int i = wid; i < n2; i += NW) {
-                const double *col = A + (j + 1) + (int64_t)(j + 1 + i) * m;
-                double sacc = 0.0;
-                for (int r = lane; r < n2; r += 64) sacc += col[r] * vsh[r];
-                sacc = wave_sum(sacc);
-                if (lane == 0) psh[i] = beta * sacc;
+            // p = beta * A22 v (A22 symmetric: column dot products)
+            for (int cb = wid; cb < n2; cb += NW * CPI) {
+                double a[CPI][RPL];
+#pragma unroll
+                for (int q = 0; q < CPI; q++) {
+                    const int c = min(cb + NW * q, n2 - 1);
+                    const double *col = A + (j + 1) + (int64_t)(j + 1 + c) * m;
+#pragma unroll
+                    for (int r = 0; r < RPL; r++) {
+                        const int i = lane + 64 * r;
+                        a[q][r] = (i < n2) ? col[i] : 0.0;
+                    }
+                }
+                double s[CPI];
+#pragma unroll
+                for (int q = 0; q < CPI; q++) s[q] = 0.0;
+#pragma unroll
+                for (int r = 0; r < RPL; r++) {
+                    const int i = lane + 64 * r;
+                    const double vi = (i < n2) ? vsh[i] : 0.0;
+#pragma unroll
+                    for (int q = 0; q < CPI; q++) s[q] += a[q][r] * vi;
+                }
+#pragma unroll
+                for (int q = 0; q < CPI; q++) {
+                    const double t = wave_sum(s[q]);
+                    const int c = cb + NW * q;
+                    if (lane == 0 && c < n2) psh[c] = beta * t;
+                }
             }
             __syncthreads();
             double pv = 0.0;
@@ -122,9 +157,31 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
             for (int i = tid; i < n2; i += NT) psh[i] -= K * vsh[i];
             __syncthreads();
             // A22 -= v w^T + w v^T
-            for (int64_t t = tid; t < (int64_t)n2 * n2; t += NT) {
-                int i = (int)(t % n2), c = (int)(t / n2);
-                A[(j + 1 + i) + (int64_t)(j + 1 + c) * m] -= vsh[i] * psh[c] + psh[i] * vsh[c];
+            for (int cb = wid; cb < n2; cb += NW * CPI) {
+                double a[CPI][RPL];
+#pragma unroll
+                for (int q = 0; q < CPI; q++) {
+                    const int c = cb + NW * q;
+                    const double *col = A + (j + 1) + (int64_t)(j + 1 + min(c, n2 - 1)) * m;
+#pragma unroll
+                    for (int r = 0; r < RPL; r++) {
+                        const int i = lane + 64 * r;
+                        a[q][r] = (i < n2 && c < n2) ? col[i] : 0.0;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < CPI; q++) {
+                    const int c = cb + NW * q;
+                    if (c < n2) {
+                        double *col = A + (j + 1) + (int64_t)(j + 1 + c) * m;
+                        const double vc = vsh[c], pc = psh[c];
+#pragma unroll
+                        for (int r = 0; r < RPL; r++) {
+                            const int i = lane + 64 * r;
+                            if (i < n2) col[i] = a[q][r] - (vsh[i] * pc + psh[i] * vc);
+                        }
+                    }
+                }
             }
         }
         __syncthreads();
@@ -141,7 +198,7 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
     }
     __syncthreads();
 
-    // ---- 2. Gershgorin interval, then bisection at the extreme indices ----
+    // ---- 2. Gershgorin interval, then multisection at the extreme indices ----
     for (int i = tid; i < m; i += NT) e2[i] = ee[i] * ee[i];
     {
         double lo = INFINITY, hi = -INFINITY, em = 0.0;
@@ -179,18 +236,30 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
     const double tnorm = bnd[2], pivmin = bnd[3];
     const int both = (2 * k >= m);          // every index is an extreme one
     const int nidx = both ? m : 2 * k;
-    if (tid < nidx) {
-        int idx = both ? tid : (tid < k ? tid : m - 2 * k + tid);
+    {
+        int G = 32;
+        while (G > 1 && nidx * G > NT) G >>= 1;
+        const int grp = tid / G, gl = tid % G;
+        const bool active = grp < nidx;
+        const int idx = both ? grp : (grp < k ? grp : m - 2 * k + grp);
+        const int shift = (lane / G) * G;
+        const uint64_t gmask = (1ull << G) - 1;
         double lo = bnd[0], hi = bnd[1];
         const double abstol = EPS * tnorm;
         for (int it = 0; it < 200; it++) {
-            double tol = fmax(abstol, 2.0 * EPS * fmax(fabs(lo), fabs(hi)));
-            if (hi - lo <= tol) break;
-            double mid = 0.5 * (lo + hi);
-            if (sturm_count(m, dd, e2, mid, pivmin) <= idx) lo = mid;
-            else hi = mid;
+            const double tol = fmax(abstol, 2.0 * EPS * fmax(fabs(lo), fabs(hi)));
+            if (!active || hi - lo <= tol) break;
+            const double w = hi - lo;
+            const double xg = lo + w * (double)(gl + 1) / (double)(G + 1);
+            const bool le = sturm_count(m, dd, e2, xg, pivmin) <= idx;
+            const uint64_t bal = (uint64_t)__ballot(le);
+            const int s = __popcll((bal >> shift) & gmask);
+            const double nlo = (s > 0) ? lo + w * (double)s / (double)(G + 1) : lo;
+            const double nhi = (s < G) ? lo + w * (double)(s + 1) / (double)(G + 1) : hi;
+            lo = nlo;
+            hi = fmax(nhi, nlo);
         }
-        ev[tid] = 0.5 * (lo + hi);
+        if (active && gl == 0) ev[grp] = 0.5 * (lo + hi);
     }
     __syncthreads();
 
@@ -223,112 +292,140 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
     }
     __syncthreads();
 
-    // ---- 3. inverse iteration, one lane per cluster ----
+    // ---- 3. inverse iteration, one lane per cluster, LU and vector in LDS ----
     const int nc = sint[1];
-    if (tid < nc) {
-        double *D = work + m + (int64_t)tid * 5 * m;
-        double *DL = D + m, *DU = DL + m, *DU2 = DU + m, *PV = DU2 + m;
-        const double tiny = EPS * tnorm + pivmin;
-        double xprev = 0.0;
-        for (int t = cl_start[tid]; t < cl_start[tid + 1]; t++) {
-            double x = ev_at(asc[t]);
-            if (t > cl_start[tid]) {
-                double pertol = 10.0 * EPS * fmax(fabs(x), tnorm * 1e-3);
-                if (x <= xprev + pertol) x = xprev + pertol;
-            }
-            xprev = x;
-            // LU with partial pivoting of T - x I (rows i, i+1 swapped when PV[i] = 1)
-            for (int i = 0; i < m; i++) {
-                D[i] = dd[i] - x;
-                DL[i] = ee[i];
-                DU[i] = ee[i];
-                DU2[i] = 0.0;
-                PV[i] = 0.0;
-            }
-            for (int i = 0; i + 1 < m; i++) {
-                if (fabs(D[i]) >= fabs(DL[i])) {
-                    double f = (D[i] != 0.0) ? DL[i] / D[i] : 0.0;
-                    DL[i] = f;
-                    D[i + 1] -= f * DU[i];
-                } else {
-                    double f = D[i] / DL[i];
-                    D[i] = DL[i];
-                    DL[i] = f;
-                    double tmp = DU[i];
-                    DU[i] = D[i + 1];
-                    D[i + 1] = tmp - f * D[i + 1];
-                    if (i + 2 < m) {
-                        DU2[i] = DU[i + 1];
-                        DU[i + 1] = -f * DU[i + 1];
-                    }
-                    PV[i] = 1.0;
+    const int CB = SCR / (6 * m);  // clusters per batch (>= 2 for m <= 1024)
+    for (int c0 = 0; c0 < nc; c0 += CB) {
+        const int cid = c0 + tid;
+        if (tid < CB && cid < nc) {
+            double *D = scr + (size_t)tid * 6 * m;
+            double *DL = D + m, *DU = DL + m, *DU2 = DU + m, *PV = DU2 + m, *z = PV + m;
+            const double tiny = EPS * tnorm + pivmin;
+            double xprev = 0.0;
+            for (int t = cl_start[cid]; t < cl_start[cid + 1]; t++) {
+                double x = ev_at(asc[t]);
+                if (t > cl_start[cid]) {
+                    double pertol = 10.0 * EPS * fmax(fabs(x), tnorm * 1e-3);
+                    if (x <= xprev + pertol) x = xprev + pertol;
                 }
-            }
-            for (int i = 0; i < m; i++)
-                if (fabs(D[i]) < tiny) D[i] = (D[i] < 0.0) ? -tiny : tiny;
-            double *z = Zs + (int64_t)ocol[t] * m;
-            for (int i = 0; i < m; i++) z[i] = hash_unit(((uint64_t)(asc[t] + 1) << 32) ^ (uint64_t)i);
-            for (int it = 0; it < 4; it++) {
-                // L solve
+                xprev = x;
+                // LU with partial pivoting of T - x I (rows i, i+1 swapped when PV[i] = 1);
+                // dcur/ucur carry D[i]/DU[i] of the row being eliminated
+                double dcur = dd[0] - x, ucur = ee[0];
                 for (int i = 0; i + 1 < m; i++) {
-                    if (PV[i] == 0.0) {
-                        z[i + 1] -= DL[i] * z[i];
+                    const double l = ee[i];
+                    const double dn = dd[i + 1] - x, un = (i + 2 < m) ? ee[i + 1] : 0.0;
+                    if (fabs(dcur) >= fabs(l)) {
+                        const double f = (dcur != 0.0) ? l / dcur : 0.0;
+                        D[i] = dcur; DL[i] = f; DU[i] = ucur; DU2[i] = 0.0; PV[i] = 0.0;
+                        dcur = dn - f * ucur;
+                        ucur = un;
                     } else {
-                        double tmp = z[i];
-                        z[i] = z[i + 1];
-                        z[i + 1] = tmp - DL[i] * z[i];
+                        const double f = dcur / l;
+                        D[i] = l; DL[i] = f; DU[i] = dn; DU2[i] = un; PV[i] = 1.0;
+                        dcur = ucur - f * dn;
+                        ucur = -f * un;
                     }
                 }
-                // U solve
-                z[m - 1] /= D[m - 1];
-                if (m >= 2) z[m - 2] = (z[m - 2] - DU[m - 2] * z[m - 1]) / D[m - 2];
-                for (int i = m - 3; i >= 0; i--) z[i] = (z[i] - DU[i] * z[i + 1] - DU2[i] * z[i + 2]) / D[i];
-                // orthogonalise against the cluster's earlier vectors
-                for (int u = cl_start[tid]; u < t; u++) {
-                    const double *zu = Zs + (int64_t)ocol[u] * m;
-                    double s = 0.0;
-                    for (int i = 0; i < m; i++) s += zu[i] * z[i];
-                    for (int i = 0; i < m; i++) z[i] -= s * zu[i];
+                D[m - 1] = dcur;
+                DU[m - 1] = 0.0;
+                for (int i = 0; i < m; i++)
+                    if (fabs(D[i]) < tiny) D[i] = (D[i] < 0.0) ? -tiny : tiny;
+                for (int i = 0; i < m; i++) z[i] = hash_unit(((uint64_t)(asc[t] + 1) << 32) ^ (uint64_t)i);
+                for (int it = 0; it < 4; it++) {
+                    // L solve, carrying the current row's value
+                    double cur = z[0];
+                    for (int i = 0; i + 1 < m; i++) {
+                        const double nxt = z[i + 1];
+                        if (PV[i] == 0.0) {
+                            z[i] = cur;
+                            cur = nxt - DL[i] * cur;
+                        } else {
+                            z[i] = nxt;
+                            cur = cur - DL[i] * nxt;
+                        }
+                    }
+                    z[m - 1] = cur;
+                    // U solve, carrying z[i+1], z[i+2]
+                    double z1 = z[m - 1] / D[m - 1], z2 = 0.0;
+                    z[m - 1] = z1;
+                    for (int i = m - 2; i >= 0; i--) {
+                        const double zi = (z[i] - DU[i] * z1 - DU2[i] * z2) / D[i];
+                        z[i] = zi;
+                        z2 = z1;
+                        z1 = zi;
+                    }
+                    // orthogonalise against the cluster's earlier vectors
+                    for (int u = cl_start[cid]; u < t; u++) {
+                        const double *zu = Zs + (int64_t)ocol[u] * m;
+                        double sdot = 0.0;
+                        for (int i = 0; i < m; i++) sdot += zu[i] * z[i];
+                        for (int i = 0; i < m; i++) z[i] -= sdot * zu[i];
+                    }
+                    double nrm = 0.0;
+                    for (int i = 0; i < m; i++) nrm += z[i] * z[i];
+                    nrm = sqrt(nrm);
+                    if (!(nrm > 0.0) || !isfinite(nrm)) {
+                        atomicOr(&sint[0], 2);
+                        break;
+                    }
+                    const double inv = 1.0 / nrm;
+                    for (int i = 0; i < m; i++) z[i] *= inv;
                 }
-                double nrm = 0.0;
-                for (int i = 0; i < m; i++) nrm += z[i] * z[i];
-                nrm = sqrt(nrm);
-                if (!(nrm > 0.0) || !isfinite(nrm)) {
-                    atomicOr(&sint[0], 2);
-                    break;
-                }
-                double inv = 1.0 / nrm;
-                for (int i = 0; i < m; i++) z[i] *= inv;
+                double *zo = Zs + (int64_t)ocol[t] * m;
+                for (int i = 0; i < m; i++) zo[i] = z[i];
             }
         }
+        __syncthreads();
     }
-    __syncthreads();
 
-    // ---- 4. back-transformation Zs <- H_0 ... H_{m-3} Zs, one wavefront per vector ----
-    for (int c = wid; c < k; c += NW) {
-        double *z = Zs + (int64_t)c * m;
+    // ---- 4. back-transformation Zs <- H_0 ... H_{m-3} Zs, vectors in LDS ----
+    const int cbat = SCR / m;  // vectors per batch
+    for (int c0 = 0; c0 < k; c0 += cbat) {
+        const int nv = min(cbat, k - c0);
+        for (int t = tid; t < nv * m; t += NT) scr[t] = Zs[(int64_t)c0 * m + t];
+        // reflector j lives in A[j+1:, j] (length m-j-1); double buffer vsh/psh
+        double *bcur = vsh, *bnext = psh;
+        if (m >= 3 && tid < 2) bcur[tid] = A[(m - 2) + tid + (int64_t)(m - 3) * m];
+        __syncthreads();
         for (int j = m - 3; j >= 0; j--) {
+            // prefetch reflector j-1 (length m-j) into a register
+            double pre = 0.0;
+            const int n2n = m - j;
+            if (j > 0 && tid < n2n) pre = A[j + tid + (int64_t)(j - 1) * m];
             const double beta = tau[j];
-            if (beta == 0.0) continue;
-            const double *v = A + (j + 1) + (int64_t)j * m;
             const int n2 = m - j - 1;
-            double sacc = 0.0;
-            for (int i = lane; i < n2; i += 64) sacc += v[i] * z[j + 1 + i];
-            sacc = beta * wave_sum(sacc);
-            for (int i = lane; i < n2; i += 64) z[j + 1 + i] -= sacc * v[i];
-            // the next reflector's dot product reads entries other lanes of
-            // this wavefront just wrote: keep the compiler from reordering
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
+            const double *v = bcur;
+            if (beta != 0.0)
+                for (int c = wid; c < nv; c += NW) {
+                    double *z = scr + (size_t)c * m + (j + 1);
+                    double sacc = 0.0;
+                    for (int i = lane; i < n2; i += 64) sacc += v[i] * z[i];
+                    sacc = beta * wave_sum(sacc);
+                    for (int i = lane; i < n2; i += 64) z[i] -= sacc * v[i];
+                }
+            if (j > 0 && tid < n2n) bnext[tid] = pre;
+            double *tsw = bcur;
+            bcur = bnext;
+            bnext = tsw;
+            __syncthreads();
         }
+        for (int t = tid; t < nv * m; t += NT) Zs[(int64_t)c0 * m + t] = scr[t];
+        __syncthreads();
     }
     if (tid == 0) *info = sint[0];
 }
 
 }  // namespace
 
-int64_t symeig_work_size(int m, int k) { return (int64_t)m + (int64_t)k * 5 * m; }
+int64_t symeig_work_size(int m, int k) {
+    (void)k;
+    return (int64_t)m;
+}
 
 void sym_eig_top(int m, double *A, int k, double *theta, double *Zs, double *work, int *info, hipStream_t s) {
-    symeig_top_kernel<<<1, NT, 0, s>>>(m, A, k, theta, Zs, work, info);
+    if (m <= 385)
+        symeig_top_kernel<6, 4><<<1, NT, 0, s>>>(m, A, k, theta, Zs, work, info);
+    else
+        symeig_top_kernel<16, 1><<<1, NT, 0, s>>>(m, A, k, theta, Zs, work, info);
 }

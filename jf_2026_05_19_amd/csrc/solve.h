@@ -12,6 +12,7 @@ struct DevSolve {
     const int32_t *fwd_tiles, *bwd_tiles;
     const int32_t *fwd_tbase_rel, *bwd_tbase_rel;
     const int32_t *fwd_nks, *bwd_nks;
+    const int32_t *pull_ptr, *pull_src;
 };
 
 void solve_prepare(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *F,

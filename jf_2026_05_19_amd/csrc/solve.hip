@@ -184,27 +184,22 @@ __global__ void transpose_kernel(DevFronts D, const int32_t *flist, const double
 
 // --- solves ----------------------------------------------------------------
 
-// forward assembly: V_f = [x_piv ; 0] + sum over children of their w (fixed child order)
-__global__ void __launch_bounds__(256) fwd_assemble_kernel(DevFronts D, const int32_t *flist, double *W,
-                                                           const double *X, int n_free, int nrhs) {
-    int f = flist[blockIdx.x];
+// forward assembly: V_f = [x_piv ; 0] + sum over children of their w, one
+// thread per (row, rhs); each row pulls its child rows in fixed child order
+__global__ void fwd_pull_kernel(DevFronts D, DevSolve DS, const int32_t *flist, double *W, const double *X,
+                                int n_free, int nrhs) {
+    int f = flist[blockIdx.y];
     int ns = D.ns[f], m = D.m[f], piv0 = D.piv0[f];
-    double *V = W + D.vec_off[f] * nrhs;
-    for (int t = threadIdx.x; t < m * nrhs; t += blockDim.x) {
+    int64_t slot0 = D.vec_off[f];
+    double *V = W + slot0 * nrhs;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m * nrhs; t += gridDim.x * blockDim.x) {
         int i = t % m, r = t / m;
-        V[i + (int64_t)r * m] = (i < ns) ? X[(piv0 + i) + (int64_t)r * n_free] : 0.0;
-    }
-    __syncthreads();
-    for (int k = D.child_ptr[f]; k < D.child_ptr[f + 1]; k++) {
-        int c = D.child_idx[k];
-        int nu = D.nu[c], nsc = D.ns[c], mc = D.m[c];
-        const int32_t *rel = D.rel + D.rel_off[c];
-        const double *Vc = W + D.vec_off[c] * nrhs;
-        for (int t = threadIdx.x; t < nu * nrhs; t += blockDim.x) {
-            int i = t % nu, r = t / nu;
-            V[rel[i] + (int64_t)r * m] += Vc[(nsc + i) + (int64_t)r * mc];
+        double v = (i < ns) ? X[(piv0 + i) + (int64_t)r * n_free] : 0.0;
+        for (int q = DS.pull_ptr[slot0 + i]; q < DS.pull_ptr[slot0 + i + 1]; q++) {
+            int c = DS.pull_src[2 * q], row = DS.pull_src[2 * q + 1];
+            v += W[D.vec_off[c] * nrhs + (int64_t)r * D.m[c] + row];
         }
-        __syncthreads();
+        V[i + (int64_t)r * m] = v;
     }
 }
 
@@ -224,8 +219,14 @@ __global__ void bwd_gather_kernel(DevFronts D, const int32_t *flist, double *W, 
 
 // partial[tile][64][nrhs] = A[rows of tile, k-slice] * V[k-slice]
 // fwd: A = P_f (m x ns, ld m), input = V_f[0:ns];  bwd: A = PT_f (ns x m, ld ns), input = V_f[0:m]
-__global__ void __launch_bounds__(256) prod_kernel(DevFronts D, const int32_t *tiles, const double *A, const int64_t *poff,
-                                                   const double *W, double *part, int nrhs, int bwd) {
+// Wavefront g of the workgroup streams k in [k0 + 32g, k0 + 32g + 32) for the
+// tile's 64 rows (one row per lane, 8 loads in flight per lane); the four
+// partial sums are added in wavefront order (deterministic).
+template <int NR>
+__global__ void __launch_bounds__(256) prod_kernel(DevFronts D, const int32_t *tiles, const double *A,
+                                                   const int64_t *poff, const double *W, double *part, int nrhs,
+                                                   int bwd) {
+    constexpr int KW = KS / 4;  // k per wavefront
     int f = tiles[3 * blockIdx.x], rt = tiles[3 * blockIdx.x + 1], ks = tiles[3 * blockIdx.x + 2];
     int ns = D.ns[f], m = D.m[f];
     int nrows = bwd ? ns : m;
@@ -234,32 +235,47 @@ __global__ void __launch_bounds__(256) prod_kernel(DevFronts D, const int32_t *t
     const double *Af = A + poff[f];
     const double *V = W + D.vec_off[f] * nrhs;
     int k0 = ks * KS, k1 = min(inner, k0 + KS);
-    __shared__ double Vs[KS][MAXR];
-    __shared__ double red[4][RT][MAXR + 1];
-    for (int t = threadIdx.x; t < (k1 - k0) * nrhs; t += blockDim.x) {
-        int k = t % (k1 - k0), r = t / (k1 - k0);
-        Vs[k][r] = V[(k0 + k) + (int64_t)r * m];
+    __shared__ double Vs[KS][NR];
+    __shared__ double red[3][RT][NR + 1];
+    for (int t = threadIdx.x; t < KS * NR; t += blockDim.x) {
+        int k = t / NR, r = t % NR;
+        Vs[k][r] = (k0 + k < k1 && r < nrhs) ? V[(k0 + k) + (int64_t)r * m] : 0.0;
     }
     __syncthreads();
     int row = threadIdx.x % RT, g = threadIdx.x / RT;
     int i = rt * RT + row;
-    double acc[MAXR];
+    int kb = k0 + g * KW, ke = min(k1, kb + KW);
+    double acc[NR];
 #pragma unroll
-    for (int r = 0; r < MAXR; r++) acc[r] = 0.0;
-    if (i < nrows) {
-        for (int k = k0 + g; k < k1; k += 4) {
-            double a = Af[i + (int64_t)k * lda];
+    for (int r = 0; r < NR; r++) acc[r] = 0.0;
+    if (i < nrows && kb < ke) {
+        const double *a = Af + i + (int64_t)kb * lda;
+        int k = kb;
+        for (; k + 8 <= ke; k += 8) {
+            double av[8];
 #pragma unroll
-            for (int r = 0; r < MAXR; r++) acc[r] += a * Vs[k - k0][r];
+            for (int u = 0; u < 8; u++) av[u] = a[(int64_t)(k - kb + u) * lda];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+#pragma unroll
+                for (int r = 0; r < NR; r++) acc[r] += av[u] * Vs[k - k0 + u][r];
+        }
+        for (; k < ke; k++) {
+            double av = a[(int64_t)(k - kb) * lda];
+#pragma unroll
+            for (int r = 0; r < NR; r++) acc[r] += av * Vs[k - k0][r];
         }
     }
+    if (g > 0) {
 #pragma unroll
-    for (int r = 0; r < MAXR; r++) red[g][row][r] = acc[r];
+        for (int r = 0; r < NR; r++) red[g - 1][row][r] = acc[r];
+    }
     __syncthreads();
-    for (int t = threadIdx.x; t < RT * nrhs; t += blockDim.x) {
-        int rr = t % RT, r = t / RT;
-        double s = red[0][rr][r] + red[1][rr][r] + red[2][rr][r] + red[3][rr][r];
-        part[((int64_t)blockIdx.x * RT + rr) * MAXR + r] = s;
+    if (g == 0) {
+        for (int r = 0; r < nrhs; r++) {
+            double s = acc[r] + red[0][row][r] + red[1][row][r] + red[2][row][r];
+            part[((int64_t)blockIdx.x * RT + row) * MAXR + r] = s;
+        }
     }
 }
 
@@ -320,8 +336,12 @@ void solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, cons
         const SolveHeight &SH = S.heights[h];
         if (H.n == 0) continue;
         const int32_t *fl = D.flist + H.list_off;
-        fwd_assemble_kernel<<<H.n, 256, 0, s>>>(D, fl, W, X, P.n_free, nrhs);
-        prod_kernel<<<SH.n_fwd, 256, 0, s>>>(D, DS.fwd_tiles + 3 * SH.fwd_off, Pbuf, DS.poff, W, part, nrhs, 0);
+        fwd_pull_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, DS, fl, W, X, P.n_free, nrhs);
+        if (nrhs == 1)
+            prod_kernel<1><<<SH.n_fwd, 256, 0, s>>>(D, DS.fwd_tiles + 3 * SH.fwd_off, Pbuf, DS.poff, W, part, nrhs, 0);
+        else
+            prod_kernel<MAXR><<<SH.n_fwd, 256, 0, s>>>(D, DS.fwd_tiles + 3 * SH.fwd_off, Pbuf, DS.poff, W, part, nrhs,
+                                                       0);
         fwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.fwd_tbase_rel,
                                                                                        DS.fwd_nks, part, W, X,
                                                                                        P.n_free, nrhs);
@@ -336,7 +356,11 @@ void solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, cons
         if (H.n == 0) continue;
         const int32_t *fl = D.flist + H.list_off;
         bwd_gather_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, fl, W, X, P.n_free, nrhs);
-        prod_kernel<<<SH.n_bwd, 256, 0, s>>>(D, DS.bwd_tiles + 3 * SH.bwd_off, PTbuf, DS.poff, W, part, nrhs, 1);
+        if (nrhs == 1)
+            prod_kernel<1><<<SH.n_bwd, 256, 0, s>>>(D, DS.bwd_tiles + 3 * SH.bwd_off, PTbuf, DS.poff, W, part, nrhs, 1);
+        else
+            prod_kernel<MAXR><<<SH.n_bwd, 256, 0, s>>>(D, DS.bwd_tiles + 3 * SH.bwd_off, PTbuf, DS.poff, W, part,
+                                                       nrhs, 1);
         bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.bwd_tbase_rel,
                                                                                         DS.bwd_nks, part, X,
                                                                                         P.n_free, nrhs);

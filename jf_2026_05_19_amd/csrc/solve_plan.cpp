@@ -90,4 +90,23 @@ void build_plans(const Symbolic &S, FactorPlan &P, SolvePlan &SP, std::vector<in
     std::vector<int> ns(nf), m(nf);
     for (int f = 0; f < nf; f++) { ns[f] = S.fronts[f].ns; m[f] = S.fronts[f].m; }
     solve_plan_build(P, ns, m, flist, SP);
+    // pull map: count, prefix sum, fill (children in their stored order)
+    SP.pull_ptr.assign(S.vec_storage + 1, 0);
+    for (int f = 0; f < nf; f++)
+        for (int c : S.children[f]) {
+            const FrontInfo &C = S.fronts[c];
+            for (int j = 0; j < C.nu; j++) SP.pull_ptr[S.fronts[f].vec_off + S.rel[C.rel_off + j] + 1]++;
+        }
+    for (int64_t i = 0; i < S.vec_storage; i++) SP.pull_ptr[i + 1] += SP.pull_ptr[i];
+    SP.pull_src.assign(2 * (size_t)SP.pull_ptr[S.vec_storage], 0);
+    std::vector<int32_t> cur(SP.pull_ptr.begin(), SP.pull_ptr.end() - 1);
+    for (int f = 0; f < nf; f++)
+        for (int c : S.children[f]) {
+            const FrontInfo &C = S.fronts[c];
+            for (int j = 0; j < C.nu; j++) {
+                int32_t q = cur[S.fronts[f].vec_off + S.rel[C.rel_off + j]]++;
+                SP.pull_src[2 * (size_t)q] = c;
+                SP.pull_src[2 * (size_t)q + 1] = C.ns + j;
+            }
+        }
 }

@@ -114,3 +114,19 @@ void probe_plan(void *qh, int32_t *ft, int32_t *bt, int32_t *ftb, int32_t *btb, 
     std::memcpy(poff, S.poff.data(), S.poff.size() * 8);
 }
 }
+
+extern "C" {
+
+// pull map of the forward assembly: sizes [n_slots + 1, n_entries]
+void probe_pull_sizes(void *qh, int64_t *out) {
+    PlanProbe *q = (PlanProbe *)qh;
+    out[0] = (int64_t)q->SP.pull_ptr.size();
+    out[1] = (int64_t)q->SP.pull_src.size() / 2;
+}
+
+void probe_pull(void *qh, int32_t *ptr, int32_t *src) {
+    PlanProbe *q = (PlanProbe *)qh;
+    std::memcpy(ptr, q->SP.pull_ptr.data(), q->SP.pull_ptr.size() * 4);
+    std::memcpy(src, q->SP.pull_src.data(), q->SP.pull_src.size() * 4);
+}
+}

@@ -45,6 +45,8 @@ def _probe():
     L.probe_plans_free.argtypes = [vp]
     L.probe_plan_sizes.argtypes = [vp, vp]
     L.probe_plan.argtypes = [vp] * 10
+    L.probe_pull_sizes.argtypes = [vp, vp]
+    L.probe_pull.argtypes = [vp, vp, vp]
     return L
 
 
@@ -81,6 +83,11 @@ def analyze(L, arr, free_mask, leaf=8):
                 flist=np.zeros(nf, np.int32), hinfo=np.zeros((H + 1, 8), np.int64), poff=np.zeros(nf, np.int64),
                 max_height_tiles=mht, p_storage=pst)
     L.probe_plan(q, *[_ptr(plan[k]) for k in ("ft", "bt", "ftb", "btb", "fnk", "bnk", "flist", "hinfo", "poff")])
+    psz = np.zeros(2, np.int64)
+    L.probe_pull_sizes(q, _ptr(psz))
+    plan["pull_ptr"] = np.zeros(int(psz[0]), np.int32)
+    plan["pull_src"] = np.zeros(2 * int(psz[1]), np.int32)
+    L.probe_pull(q, _ptr(plan["pull_ptr"]), _ptr(plan["pull_src"]))
     L.probe_plans_free(q)
     L.probe_free(h)
     return dict(rp=rp, col=col, cptr=cptr, cidx=cidx, nfree=nfree, info=info, rows=rows, rel=rel, eod=eod,
@@ -337,3 +344,42 @@ def test_tiled_gemm_solve_emulation(probe):
     x = emulate_tiled_solve(S, F, arr.F[perm])
     u_ref, _ = O.static_solve(arr, K_or)
     assert np.abs(x - u_ref[perm]).max() <= 1e-9 * np.abs(u_ref).max()
+
+
+@pytest.mark.parametrize("deck", ["plate", "panel"])
+def test_forward_pull_map_equals_extend_add(probe, deck):
+    """The forward solve assembles each front vector by pulling child update
+    rows through a precomputed map; it must equal the push form
+    V_f[rel_c[j]] += w_c[j] with children in their stored order, bit for bit."""
+    if deck == "plate":
+        arr = M.build_arrays(bdf.bdf_to_model_text(decks.plate_deck(nx=12, ny=10, nd=2)))
+    else:
+        arr = M.build_arrays(bdf.bdf_to_model_text(decks.panel_deck(nx=24, ny=12, stringer_every=4, nd=2)))
+    free = (~arr.spc_mask.astype(bool)).astype(np.uint8)
+    S = analyze(probe, arr, free, leaf=8)
+    info = S["info"]
+    ns, nu, m, parent = info[:, 0], info[:, 1], info[:, 2], info[:, 4]
+    vec_off = np.concatenate([[0], np.cumsum(m)])[:-1]
+    nslot = int(m.sum())
+    pp, ps = S["plan"]["pull_ptr"], S["plan"]["pull_src"]
+    assert len(pp) == nslot + 1 and pp[-1] == nu[parent >= 0].sum()
+    rng = np.random.default_rng(5)
+    W = rng.standard_normal(nslot)
+    push = np.zeros(nslot)
+    children = [[] for _ in range(len(m))]
+    for c in range(len(m)):
+        if parent[c] >= 0:
+            children[parent[c]].append(c)
+    for f in range(len(m)):
+        for c in children[f]:
+            rel = S["rel"][info[c, 8]:info[c, 8] + nu[c]]
+            for j in range(nu[c]):
+                push[vec_off[f] + rel[j]] += W[vec_off[c] + ns[c] + j]
+    pull = np.zeros(nslot)
+    for t in range(nslot):
+        v = 0.0
+        for q in range(pp[t], pp[t + 1]):
+            c, row = ps[2 * q], ps[2 * q + 1]
+            v += W[vec_off[c] + row]
+        pull[t] = v
+    assert np.array_equal(push, pull)
