@@ -619,9 +619,9 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     auto expand = [&](int jj) {
         HIPCHK(hipMemcpyAsync(Wb.p, blk(jj), sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
         solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Wb.p, b, s);
-        launch_scatter_elim(n, ndof, b, M->d_elim_of_dof.p, Wb.p, xf.p, s);
+        launch_scatter_elim_il(n, ndof, b, M->d_elim_of_dof.p, Wb.p, xf.p, s);
         launch_bsr_spmv(M->n_nodes, M->d_rp.p, M->d_col.p, M->d_KG.p, xf.p, yf.p, b, -1.0, s);
-        launch_gather_elim(n, ndof, b, M->d_dof_of_elim.p, yf.p, Wb.p, s);
+        launch_gather_elim_il(n, b, M->d_dof_of_elim.p, yf.p, Wb.p, s);
         solve_fwd(M->plan, M->dev, M->dsolve, M->splan, M->d_P.p, M->d_W.p, M->d_part.p, Wb.p, b, s);
         int kcols = (jj + 1) * b;
         double *Tcol = T.p + (size_t)jj * b * ldT;

@@ -8,69 +8,145 @@
 
 namespace {
 
-constexpr int CH = 1024;  // rows per partial-sum chunk
+constexpr int CH = 512;    // rows per partial-sum chunk
+constexpr int TR = 64;     // rows per LDS tile of the V^T W product
+constexpr int PC = 64;     // columns of V per workgroup
 
-// partial[c][p][q] = sum_{rows of chunk c} V[row, p] * W[row, q]
+// partial[c][p][q] = sum_{rows of chunk c} V[row, p] * W[row, q]   (Q <= 16)
+// Workgroup (c, g) takes rows [c CH, (c+1) CH) and columns [g PC, (g+1) PC) of
+// V.  Each 64-row tile of V (64 x 64) and W (64 x 16) is staged in LDS with
+// coalesced loads; lane (pl, ql) accumulates the four outputs
+// (pl + 16 i, ql), i = 0..3, so W is re-read once per 64 columns of V.
 __global__ void __launch_bounds__(256) tn_partial_kernel(int n, int P, int Q, const double *V, int ldv, const double *W,
                                                          int ldw, double *part) {
-    int c = blockIdx.x;
-    int p0 = blockIdx.y * 16;
-    int pl = threadIdx.x % 16, ql = threadIdx.x / 16;
-    __shared__ double Vs[64][17];
-    __shared__ double Ws[64][17];
-    double acc = 0.0;
-    int r0 = c * CH, r1 = min(n, r0 + CH);
-    for (int rb = r0; rb < r1; rb += 64) {
-        for (int t = threadIdx.x; t < 64 * 16; t += 256) {
-            int r = t % 64, k = t / 64;
-            int row = rb + r;
-            Vs[r][k] = (row < r1 && p0 + k < P) ? V[row + (int64_t)(p0 + k) * ldv] : 0.0;
-            Ws[r][k] = (row < r1 && k < Q) ? W[row + (int64_t)k * ldw] : 0.0;
+    __shared__ double Vs[TR][PC + 1];
+    __shared__ double Ws[TR][17];
+    const int c = blockIdx.x, p0 = blockIdx.y * PC, t = threadIdx.x;
+    const int pl = t % 16, ql = t / 16;
+    const int r0 = c * CH, r1 = min(n, r0 + CH);
+    const int lr = t % TR, lc = t / TR;  // this lane's load row and first column
+    double vreg[PC / 4], wreg[4];
+    // tile loads go to registers first, so the next tile is in flight while
+    // the current one is multiplied out of LDS
+    auto load_tile = [&](int rb) {
+        const int row = rb + lr;
+#pragma unroll
+        for (int u = 0; u < PC / 4; u++) {
+            const int col = lc + 4 * u;
+            vreg[u] = (row < r1 && p0 + col < P) ? V[row + (int64_t)(p0 + col) * ldv] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int q = lc + 4 * u;
+            wreg[u] = (row < r1 && q < Q) ? W[row + (int64_t)q * ldw] : 0.0;
+        }
+    };
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (r0 < r1) load_tile(r0);
+    for (int rb = r0; rb < r1; rb += TR) {
+#pragma unroll
+        for (int u = 0; u < PC / 4; u++) Vs[lr][lc + 4 * u] = vreg[u];
+#pragma unroll
+        for (int u = 0; u < 4; u++) Ws[lr][lc + 4 * u] = wreg[u];
+        __syncthreads();
+        if (rb + TR < r1) load_tile(rb + TR);
+#pragma unroll 8
+        for (int r = 0; r < TR; r++) {
+            const double w = Ws[r][ql];
+#pragma unroll
+            for (int i = 0; i < 4; i++) acc[i] += Vs[r][pl + 16 * i] * w;
         }
         __syncthreads();
-#pragma unroll 8
-        for (int r = 0; r < 64; r++) acc += Vs[r][pl] * Ws[r][ql];
-        __syncthreads();
     }
-    if (p0 + pl < P && ql < Q) part[((int64_t)c * P + p0 + pl) * Q + ql] = acc;
+    if (ql < Q)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int p = p0 + pl + 16 * i;
+            if (p < P) part[((int64_t)c * P + p) * Q + ql] = acc[i];
+        }
 }
 
-// out[p + q*ldo] = alpha * sum_c part[c][p][q] + beta * out[...]
-__global__ void tn_reduce_kernel(int nch, int P, int Q, const double *part, double *out, int ldo, double alpha,
-                                 double beta) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= P * Q) return;
-    int p = t / Q, q = t % Q;
+// P, Q <= 16 (Gram matrices of one block): one 256-row chunk per workgroup,
+// both tiles in LDS after a single coalesced load each
+constexpr int CHS = 256;
+__global__ void __launch_bounds__(256) tn_small_kernel(int n, int P, int Q, const double *V, int ldv, const double *W,
+                                                       int ldw, double *part) {
+    __shared__ double Vs[CHS][17];
+    __shared__ double Ws[CHS][17];
+    const int c = blockIdx.x, t = threadIdx.x, row = c * CHS + t;
+#pragma unroll
+    for (int p = 0; p < 16; p++) Vs[t][p] = (row < n && p < P) ? V[row + (int64_t)p * ldv] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) Ws[t][q] = (row < n && q < Q) ? W[row + (int64_t)q * ldw] : 0.0;
+    __syncthreads();
+    const int pl = t % 16, ql = t / 16;
+    double acc = 0.0;
+#pragma unroll 8
+    for (int r = 0; r < CHS; r++) acc += Vs[r][pl] * Ws[r][ql];
+    if (pl < P && ql < Q) part[((int64_t)c * P + pl) * Q + ql] = acc;
+}
+
+// out[p + q*ldo] = alpha * sum_c part[c][p][q] + beta * out[...]; sixteen lanes
+// per output sum interleaved chunk subsets, added in a fixed order
+__global__ void __launch_bounds__(256) tn_reduce_kernel(int nch, int P, int Q, const double *part, double *out,
+                                                        int ldo, double alpha, double beta) {
+    __shared__ double red[16][17];
+    const int ol = threadIdx.x % 16, sg = threadIdx.x / 16;
+    const int o = blockIdx.x * 16 + ol;
+    const int PQ = P * Q;
     double s = 0.0;
-    for (int c = 0; c < nch; c++) s += part[((int64_t)c * P + p) * Q + q];
-    double *o = out + p + (int64_t)q * ldo;
-    *o = alpha * s + (beta == 0.0 ? 0.0 : beta * *o);
+    if (o < PQ)
+        for (int c = sg; c < nch; c += 16) s += part[(int64_t)c * PQ + o];
+    red[sg][ol] = s;
+    __syncthreads();
+    if (sg == 0 && o < PQ) {
+        double tot = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) tot += red[i][ol];
+        const int p = o / Q, q = o % Q;
+        double *y = out + p + (int64_t)q * ldo;
+        *y = alpha * tot + (beta == 0.0 ? 0.0 : beta * *y);
+    }
 }
 
-// Y[row, q] = beta*Y + alpha * sum_p V[row, p] H[p, q]   (Q <= 16)
+// Y[row, q] = beta*Y + alpha * sum_p V[row, p] H[p, q]   (Q <= 16), one row
+// per lane, H staged in LDS, eight columns of V in flight per lane
 __global__ void __launch_bounds__(256) nn_kernel(int n, int P, int Q, const double *V, int ldv, const double *H,
                                                  int ldh, double *Y, int ldy, double alpha, double beta) {
     __shared__ double Hs[256][16];
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = row < n;
+    const double *vr = V + (live ? row : 0);
     double acc[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[q] = 0.0;
     for (int pb = 0; pb < P; pb += 256) {
-        int pn = min(256, P - pb);
+        const int pn = min(256, P - pb);
         __syncthreads();
         for (int t = threadIdx.x; t < 256 * 16; t += blockDim.x) {
             int p = t / 16, q = t % 16;
             Hs[p][q] = (p < pn && q < Q) ? H[(pb + p) + (int64_t)q * ldh] : 0.0;
         }
         __syncthreads();
-        if (row < n)
-            for (int p = 0; p < pn; p++) {
-                double v = V[row + (int64_t)(pb + p) * ldv];
+        if (live) {
+            int p = 0;
+            for (; p + 8 <= pn; p += 8) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = vr[(int64_t)(pb + p + u) * ldv];
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+#pragma unroll
+                    for (int q = 0; q < 16; q++) acc[q] += v[u] * Hs[p + u][q];
+            }
+            for (; p < pn; p++) {
+                const double v = vr[(int64_t)(pb + p) * ldv];
 #pragma unroll
                 for (int q = 0; q < 16; q++) acc[q] += v * Hs[p][q];
             }
+        }
     }
-    if (row < n)
+    if (live)
         for (int q = 0; q < Q; q++) {
             double *y = Y + row + (int64_t)q * ldy;
             *y = alpha * acc[q] + (beta == 0.0 ? 0.0 : beta * *y);
@@ -283,12 +359,21 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 void ts_gemm_tn(int n, int P, int Q, const double *V, int ldv, const double *W, int ldw, double *out, int ldo,
                 double alpha, double beta, double *part, hipStream_t s) {
-    int nch = (n + CH - 1) / CH;
-    tn_partial_kernel<<<dim3(nch, (P + 15) / 16), 256, 0, s>>>(n, P, Q, V, ldv, W, ldw, part);
-    tn_reduce_kernel<<<nblk((int64_t)P * Q, 256), 256, 0, s>>>(nch, P, Q, part, out, ldo, alpha, beta);
+    if (P <= 0 || Q <= 0) return;
+    int nch;
+    if (P <= 16 && Q <= 16) {
+        nch = (n + CHS - 1) / CHS;
+        tn_small_kernel<<<nch, 256, 0, s>>>(n, P, Q, V, ldv, W, ldw, part);
+    } else {
+        nch = (n + CH - 1) / CH;
+        tn_partial_kernel<<<dim3(nch, (P + PC - 1) / PC), 256, 0, s>>>(n, P, Q, V, ldv, W, ldw, part);
+    }
+    tn_reduce_kernel<<<nblk((int64_t)P * Q, 16), 256, 0, s>>>(nch, P, Q, part, out, ldo, alpha, beta);
 }
 
-int64_t ts_gemm_tn_part_size(int n, int P, int Q) { return (int64_t)((n + CH - 1) / CH) * P * Q; }
+int64_t ts_gemm_tn_part_size(int n, int P, int Q) {
+    return std::max<int64_t>((int64_t)((n + CH - 1) / CH) * P * Q, (int64_t)((n + CHS - 1) / CHS) * 16 * 16);
+}
 
 void ts_gemm_nn(int n, int P, int Q, const double *V, int ldv, const double *H, int ldh, double *Y, int ldy,
                 double alpha, double beta, hipStream_t s) {

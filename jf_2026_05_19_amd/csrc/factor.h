@@ -24,8 +24,14 @@ void launch_gather_elim(int n_free, int ndof, int nrhs, const int32_t *dof_of_el
                         hipStream_t s);
 void launch_scatter_elim(int n_free, int ndof, int nrhs, const int32_t *elim_of_dof, const double *xe, double *xf,
                          hipStream_t s);
+// y = scale * A x, A 6x6 BSR; x and y interleaved: x[dof * nrhs + r] (nrhs <= 32)
 void launch_bsr_spmv(int n_nodes, const int32_t *rp, const int32_t *col, const double *val, const double *x, double *y,
                      int nrhs, double scale, hipStream_t s);
+// elimination order (column-major, ld n_free) <-> interleaved DOF order
+void launch_scatter_elim_il(int n_free, int ndof, int nrhs, const int32_t *elim_of_dof, const double *xe, double *xf,
+                            hipStream_t s);
+void launch_gather_elim_il(int n_free, int nrhs, const int32_t *dof_of_elim, const double *xf, double *xe,
+                           hipStream_t s);
 void launch_kdiag_elim(int n_free, const int32_t *dof_of_elim, const int32_t *diag_blk, const double *kv, double *out,
                        hipStream_t s);
 void launch_pivot_ratio(int n, const double *piv, const double *kdiag_elim, int *count, uint8_t *flag, hipStream_t s);

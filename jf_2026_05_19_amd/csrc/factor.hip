@@ -16,6 +16,8 @@
 // shift-invert operator (JFEM/src/solver/solve_case.jl:2213-2233).
 #include "factor.h"
 
+#include <algorithm>
+
 #include <cmath>
 
 namespace {
@@ -205,32 +207,51 @@ __global__ void scatter_elim_kernel(int n_free, int ndof, int nrhs, const int32_
     xf[t] = e >= 0 ? xe[e + (int64_t)r * n_free] : 0.0;
 }
 
-// y = s * A x for a 6x6 BSR matrix, nrhs vectors (ld ndof).  One thread per
-// (block row, component) keeps all right-hand sides in registers so each
-// matrix value is read once.
-__global__ void bsr_spmv_kernel(int n_nodes, const int32_t *rp, const int32_t *col, const double *val,
-                                const double *x, double *y, int nrhs, double s) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_nodes * 6) return;
-    int I = t / 6, ci = t % 6;
-    int64_t ndof = (int64_t)n_nodes * 6;
-    double acc[MAXR];
-#pragma unroll
-    for (int r = 0; r < MAXR; r++) acc[r] = 0.0;
+// Interleaved DOF-order vectors (x[d * nrhs + r]) for the BSR product: the
+// nrhs values of one DOF are adjacent, so a block's 6 x nrhs inputs are
+// 6 * nrhs contiguous doubles.
+__global__ void scatter_elim_il_kernel(int n_free, int ndof, int nrhs, const int32_t *elim_of_dof, const double *xe,
+                                       double *xf) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)ndof * nrhs) return;
+    int d = (int)(t / nrhs), r = (int)(t % nrhs);
+    int e = elim_of_dof[d];
+    xf[t] = e >= 0 ? xe[e + (int64_t)r * n_free] : 0.0;
+}
+
+__global__ void gather_elim_il_kernel(int n_free, int nrhs, const int32_t *dof_of_elim, const double *xf, double *xe) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_free * nrhs) return;
+    int e = (int)(t % n_free), r = (int)(t / n_free);
+    xe[t] = xf[(int64_t)dof_of_elim[e] * nrhs + r];
+}
+
+// y = s * A x for a 6x6 BSR matrix, x and y interleaved (x[d * nrhs + r]).
+// 6 * nrhs lanes per block row, lane (ci, r): the block-row values of row ci
+// are read by the nrhs lanes of that row together, the inputs x[6J + cj][r]
+// by consecutive lanes.
+__global__ void __launch_bounds__(192) bsr_spmv_kernel(int n_nodes, const int32_t *rp, const int32_t *col,
+                                                       const double *val, const double *x, double *y, int nrhs,
+                                                       double s) {
+    const int tpr = 6 * nrhs, rpw = max(1, 192 / tpr);
+    const int lr = threadIdx.x / tpr, l = threadIdx.x % tpr;
+    const int I = blockIdx.x * rpw + lr;
+    if (lr >= rpw || I >= n_nodes) return;
+    const int ci = l / nrhs, r = l % nrhs;
+    double acc = 0.0;
     for (int k = rp[I]; k < rp[I + 1]; k++) {
         const double *v = val + (int64_t)k * 36 + ci * 6;
-        int64_t J6 = (int64_t)col[k] * 6;
+        const double *xb = x + (int64_t)col[k] * 6 * nrhs + r;
+        double a[6], xv[6];
 #pragma unroll
         for (int cj = 0; cj < 6; cj++) {
-            double a = v[cj];
-#pragma unroll
-            for (int r = 0; r < MAXR; r++)
-                if (r < nrhs) acc[r] += a * x[J6 + cj + r * ndof];
+            a[cj] = v[cj];
+            xv[cj] = xb[cj * nrhs];
         }
-    }
 #pragma unroll
-    for (int r = 0; r < MAXR; r++)
-        if (r < nrhs) y[t + r * ndof] = s * acc[r];
+        for (int cj = 0; cj < 6; cj++) acc += a[cj] * xv[cj];
+    }
+    y[((int64_t)I * 6 + ci) * nrhs + r] = s * acc;
 }
 
 __global__ void kdiag_elim_kernel(int n_free, const int32_t *dof_of_elim, const int32_t *diag_blk, const double *kv,
@@ -294,7 +315,20 @@ void launch_scatter_elim(int n_free, int ndof, int nrhs, const int32_t *elim_of_
 
 void launch_bsr_spmv(int n_nodes, const int32_t *rp, const int32_t *col, const double *val, const double *x, double *y,
                      int nrhs, double scale, hipStream_t s) {
-    bsr_spmv_kernel<<<nblk((int64_t)n_nodes * 6, 256), 256, 0, s>>>(n_nodes, rp, col, val, x, y, nrhs, scale);
+    const int rpw = std::max(1, 192 / (6 * nrhs));
+    bsr_spmv_kernel<<<nblk(n_nodes, rpw), 192, 0, s>>>(n_nodes, rp, col, val, x, y, nrhs, scale);
+}
+
+void launch_scatter_elim_il(int n_free, int ndof, int nrhs, const int32_t *elim_of_dof, const double *xe, double *xf,
+                            hipStream_t s) {
+    int64_t n = (int64_t)ndof * nrhs;
+    if (n > 0) scatter_elim_il_kernel<<<nblk(n, 256), 256, 0, s>>>(n_free, ndof, nrhs, elim_of_dof, xe, xf);
+}
+
+void launch_gather_elim_il(int n_free, int nrhs, const int32_t *dof_of_elim, const double *xf, double *xe,
+                           hipStream_t s) {
+    int64_t n = (int64_t)n_free * nrhs;
+    if (n > 0) gather_elim_il_kernel<<<nblk(n, 256), 256, 0, s>>>(n_free, nrhs, dof_of_elim, xf, xe);
 }
 
 void launch_kdiag_elim(int n_free, const int32_t *dof_of_elim, const int32_t *diag_blk, const double *kv, double *out,

@@ -332,7 +332,11 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
                 for (int i = 0; i < m; i++)
                     if (fabs(D[i]) < tiny) D[i] = (D[i] < 0.0) ? -tiny : tiny;
                 for (int i = 0; i < m; i++) z[i] = hash_unit(((uint64_t)(asc[t] + 1) << 32) ^ (uint64_t)i);
-                for (int it = 0; it < 4; it++) {
+                // the eigenvalue is accurate to rounding, so two solves give the
+                // vector to working precision; a clustered one gets a third with
+                // re-orthogonalisation.  Solves are linear: normalise once at the end.
+                const int nit = (cl_start[cid + 1] - cl_start[cid] > 1) ? 3 : 2;
+                for (int it = 0; it < nit; it++) {
                     // L solve, carrying the current row's value
                     double cur = z[0];
                     for (int i = 0; i + 1 < m; i++) {
@@ -355,23 +359,23 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
                         z2 = z1;
                         z1 = zi;
                     }
-                    // orthogonalise against the cluster's earlier vectors
+                    // orthogonalise against the cluster's earlier (unit) vectors
                     for (int u = cl_start[cid]; u < t; u++) {
                         const double *zu = Zs + (int64_t)ocol[u] * m;
                         double sdot = 0.0;
                         for (int i = 0; i < m; i++) sdot += zu[i] * z[i];
                         for (int i = 0; i < m; i++) z[i] -= sdot * zu[i];
                     }
-                    double nrm = 0.0;
-                    for (int i = 0; i < m; i++) nrm += z[i] * z[i];
-                    nrm = sqrt(nrm);
-                    if (!(nrm > 0.0) || !isfinite(nrm)) {
-                        atomicOr(&sint[0], 2);
-                        break;
-                    }
-                    const double inv = 1.0 / nrm;
-                    for (int i = 0; i < m; i++) z[i] *= inv;
                 }
+                double nrm = 0.0;
+                for (int i = 0; i < m; i++) nrm += z[i] * z[i];
+                nrm = sqrt(nrm);
+                if (!(nrm > 0.0) || !isfinite(nrm)) {
+                    atomicOr(&sint[0], 2);
+                    nrm = 1.0;
+                }
+                const double inv = 1.0 / nrm;
+                for (int i = 0; i < m; i++) z[i] *= inv;
                 double *zo = Zs + (int64_t)ocol[t] * m;
                 for (int i = 0; i < m; i++) zo[i] = z[i];
             }
