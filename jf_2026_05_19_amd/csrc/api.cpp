@@ -92,7 +92,7 @@ struct sol105_model {
     DevFronts dev;
     SolvePlan splan;
     DevBuf<int64_t> s_poff;
-    DevBuf<int32_t> s_ftiles, s_btiles, s_ftbase, s_btbase, s_fnks, s_bnks, s_pullptr, s_pullsrc;
+    DevBuf<int32_t> s_ftiles, s_btiles, s_ftbase, s_btbase, s_fnks, s_bnks, s_pullptr, s_pullsrc, s_lblk;
     DevBuf<double> d_P, d_PT, d_part;
     DevSolve dsolve;
     // state
@@ -224,6 +224,7 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
         HIPCHK(M->s_bnks.upload(SP.bwd_nks.data(), SP.bwd_nks.size(), s));
         HIPCHK(M->s_pullptr.upload(SP.pull_ptr.data(), SP.pull_ptr.size(), s));
         HIPCHK(M->s_pullsrc.upload(SP.pull_src.data(), SP.pull_src.size(), s));
+        HIPCHK(M->s_lblk.upload(SP.lblk.data(), SP.lblk.size(), s));
         HIPCHK(M->d_P.alloc(std::max<int64_t>(SP.p_storage, 1)));
         HIPCHK(M->d_PT.alloc(std::max<int64_t>(SP.p_storage, 1)));
         HIPCHK(M->d_part.alloc(std::max<int64_t>(solve_part_size(SP), 1)));
@@ -237,6 +238,8 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
         DS.bwd_nks = M->s_bnks.p;
         DS.pull_ptr = M->s_pullptr.p;
         DS.pull_src = M->s_pullsrc.p;
+        DS.lblk = M->s_lblk.p;
+        DS.n_lblk = (int)(SP.lblk.size() / 2);
     }
     M->have_sym = true;
     return SOL105_OK;

@@ -44,6 +44,9 @@ struct SolvePlan {
     // adds the child update rows pull_src[pull_ptr[slot] .. pull_ptr[slot+1])
     // as (child front, row in the child) pairs, in fixed child order
     std::vector<int32_t> pull_ptr, pull_src;
+    // 32-wide column blocks of every L11, (front, first column) pairs, longest
+    // block-column chain first (the explicit inverse is built from these)
+    std::vector<int32_t> lblk;
 };
 
 inline int64_t solve_part_size(const SolvePlan &S) { return S.max_height_tiles * SOLVE_RT * MAXR; }

@@ -13,6 +13,8 @@ struct DevSolve {
     const int32_t *fwd_tbase_rel, *bwd_tbase_rel;
     const int32_t *fwd_nks, *bwd_nks;
     const int32_t *pull_ptr, *pull_src;
+    const int32_t *lblk;  // (front, first column) of every 32-wide L11 column block
+    int n_lblk;
 };
 
 void solve_prepare(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *F,

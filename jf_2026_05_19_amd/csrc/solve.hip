@@ -8,7 +8,7 @@
 //             x_piv = z_top ;  w_f = v_bot - z_bot     (w_f goes to the parent)
 //   backward: u = [x_piv ; -x(update rows)] ;  x_piv = PT u
 // so each direction is one tall-skinny product per height, split into
-// 64-row tiles and 256-wide k-slices over many workgroups (memory bound: the
+// 64-row tiles and 128-wide k-slices over many workgroups (memory bound: the
 // factor is streamed once per solve for up to MAXR right-hand sides), with a
 // fixed-order reduction of the k-slices (bit-reproducible).
 #include "factor.h"
@@ -22,93 +22,92 @@ constexpr int TB = 32;        // inversion block
 constexpr int RT = SOLVE_RT;  // rows per product tile
 constexpr int KS = SOLVE_KS;  // k-slice of the product
 
-// --- inverse of L11 (block column per workgroup) ---------------------------
+// --- inverse of L11 ----------------------------------------------------------
+// Every front is final once the factorization has finished, so all fronts'
+// inverses are built in two launches: the 32 x 32 diagonal blocks first, then
+// each 32-wide column block of L11^-1 by block forward substitution
+//     X[i,j] = -inv(L[i,i]) * sum_{k=j}^{i-1} L[i,k] X[k,j]
+// (one workgroup per column block, longest chains dispatched first).
 
-// lower-triangular 32x32 inverse in LDS: column j solved by thread j
-__device__ void tri_inv32(double (*L)[TB + 1], double (*X)[TB + 1], int nb) {
-    int j = threadIdx.x;
-    if (j < nb) {
-        for (int i = 0; i < nb; i++) {
-            double s = (i == j) ? 1.0 : 0.0;
-            if (i >= j) {
-                for (int l = j; l < i; l++) s -= L[i][l] * X[l][j];
-                X[i][j] = s / L[i][i];
-            } else {
-                X[i][j] = 0.0;
-            }
-        }
+// inverse of one 32 x 32 lower-triangular diagonal block, one wavefront:
+// lane j keeps column j of the inverse in registers, rows of L broadcast from LDS
+__global__ void __launch_bounds__(64) diag_inv_kernel(DevFronts D, DevSolve DS, const double *F, double *P) {
+    const int f = DS.lblk[2 * blockIdx.x], b0 = DS.lblk[2 * blockIdx.x + 1];
+    const int ns = D.ns[f], m = D.m[f];
+    const int nb = min(TB, ns - b0);
+    const double *A = F + D.off[f];
+    double *Pf = P + DS.poff[f];
+    __shared__ double L[TB][TB + 1];
+    for (int t = threadIdx.x; t < TB * TB; t += 64) {
+        int i = t % TB, j = t / TB;
+        L[i][j] = (i < nb && j < nb && i >= j) ? A[(b0 + i) + (int64_t)(b0 + j) * m] : ((i == j) ? 1.0 : 0.0);
     }
+    __syncthreads();
+    const int j = threadIdx.x;
+    double x[TB];
+#pragma unroll
+    for (int i = 0; i < TB; i++) {
+        double sacc = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+        for (int l = 0; l < i; l++) sacc -= L[i][l] * x[l];
+        x[i] = (i >= j) ? sacc / L[i][i] : 0.0;
+    }
+    if (j < nb)
+#pragma unroll
+        for (int i = 0; i < TB; i++)
+            if (i < nb) Pf[(b0 + i) + (int64_t)(b0 + j) * m] = x[i];
 }
 
-// P[top] = L11^-1, one workgroup per (front, 32-wide block column).
-__global__ void __launch_bounds__(256) linv_kernel(DevFronts D, const int32_t *flist, const double *F, double *P,
-                                                   const int64_t *poff) {
-    int f = flist[blockIdx.y];
-    int ns = D.ns[f], m = D.m[f];
-    int j0 = blockIdx.x * TB;
-    if (j0 >= ns) return;
+// column block [j0, j0 + 32) of L11^-1 below its diagonal block; rows above it are zero
+__global__ void __launch_bounds__(256) linv_col_kernel(DevFronts D, DevSolve DS, const double *F, double *P) {
+    const int f = DS.lblk[2 * blockIdx.x], j0 = DS.lblk[2 * blockIdx.x + 1];
+    const int ns = D.ns[f], m = D.m[f];
+    const int nbj = min(TB, ns - j0);
     const double *A = F + D.off[f];
-    double *Pf = P + poff[f];
-    __shared__ double Lt[TB][TB + 1], Xt[TB][TB + 1], Dg[TB][TB + 1], Di[TB][TB + 1], Acc[TB][TB + 1];
-    int tid = threadIdx.x;
-    int nbj = min(TB, ns - j0);
-    // diagonal block inverse
-    for (int t = tid; t < TB * TB; t += blockDim.x) {
-        int i = t % TB, j = t / TB;
-        Dg[i][j] = (i < nbj && j < nbj && i >= j) ? A[(j0 + i) + (int64_t)(j0 + j) * m] : 0.0;
-    }
-    __syncthreads();
-    tri_inv32(Dg, Di, nbj);
-    __syncthreads();
-    for (int t = tid; t < TB * TB; t += blockDim.x) {
-        int i = t % TB, j = t / TB;
-        if (i < nbj && j < nbj) Pf[(j0 + i) + (int64_t)(j0 + j) * m] = Di[i][j];
-    }
-    // rows above the block column are zero
+    double *Pf = P + DS.poff[f];
+    __shared__ double Lt[TB][TB + 1], Xt[TB][TB + 1];
+    const int tid = threadIdx.x, ti = tid % TB, tq = tid / TB;  // output row, column quad
     for (int t = tid; t < j0 * nbj; t += blockDim.x) {
-        int i = t % j0, j = t / j0;
-        Pf[i + (int64_t)(j0 + j) * m] = 0.0;
+        int i = t % j0, jj = t / j0;
+        Pf[i + (int64_t)(j0 + jj) * m] = 0.0;
     }
-    __threadfence();
-    __syncthreads();
     for (int i0 = j0 + TB; i0 < ns; i0 += TB) {
-        int nbi = min(TB, ns - i0);
-        for (int t = tid; t < TB * TB; t += blockDim.x) Acc[t % TB][t / TB] = 0.0;
-        __syncthreads();
-        // Acc = sum_{k0 in [j0, i0)} L[i0, k0] X[k0, j0]
+        const int nbi = min(TB, ns - i0);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
         for (int k0 = j0; k0 < i0; k0 += TB) {
-            int nbk = min(TB, ns - k0);
+            const int nbk = min(TB, ns - k0);
             for (int t = tid; t < TB * TB; t += blockDim.x) {
-                int i = t % TB, j = t / TB;
-                Lt[i][j] = (i < nbi && j < nbk) ? A[(i0 + i) + (int64_t)(k0 + j) * m] : 0.0;
-                Xt[i][j] = (i < nbk && j < nbj) ? Pf[(k0 + i) + (int64_t)(j0 + j) * m] : 0.0;
+                int i = t % TB, jj = t / TB;
+                Lt[i][jj] = (i < nbi && jj < nbk) ? A[(i0 + i) + (int64_t)(k0 + jj) * m] : 0.0;
+                Xt[i][jj] = (i < nbk && jj < nbj) ? Pf[(k0 + i) + (int64_t)(j0 + jj) * m] : 0.0;
             }
             __syncthreads();
-            for (int t = tid; t < TB * TB; t += blockDim.x) {
-                int i = t % TB, j = t / TB;
-                double s = 0.0;
 #pragma unroll 8
-                for (int l = 0; l < TB; l++) s += Lt[i][l] * Xt[l][j];
-                Acc[i][j] += s;
+            for (int l = 0; l < TB; l++) {
+                const double a = Lt[ti][l];
+#pragma unroll
+                for (int u = 0; u < 4; u++) acc[u] += a * Xt[l][4 * tq + u];
             }
             __syncthreads();
         }
-        // X[i0] = -inv(L[i0,i0]) Acc
+        // X[i0, j0] = -inv(L[i0,i0]) acc, inv(L[i0,i0]) from the diagonal-block pass
+#pragma unroll
+        for (int u = 0; u < 4; u++) Xt[ti][4 * tq + u] = acc[u];
         for (int t = tid; t < TB * TB; t += blockDim.x) {
-            int i = t % TB, j = t / TB;
-            Dg[i][j] = (i < nbi && j < nbi && i >= j) ? A[(i0 + i) + (int64_t)(i0 + j) * m] : 0.0;
+            int i = t % TB, jj = t / TB;
+            Lt[i][jj] = (i < nbi && jj < nbi && jj <= i) ? Pf[(i0 + i) + (int64_t)(i0 + jj) * m] : 0.0;
         }
         __syncthreads();
-        tri_inv32(Dg, Di, nbi);
-        __syncthreads();
-        for (int t = tid; t < TB * TB; t += blockDim.x) {
-            int i = t % TB, j = t / TB;
-            if (i < nbi && j < nbj) {
-                double s = 0.0;
-                for (int l = 0; l <= i; l++) s += Di[i][l] * Acc[l][j];
-                Pf[(i0 + i) + (int64_t)(j0 + j) * m] = -s;
+        if (ti < nbi)
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int c = 4 * tq + u;
+                if (c < nbj) {
+                    double sacc = 0.0;
+                    for (int l = 0; l <= ti; l++) sacc += Lt[ti][l] * Xt[l][c];
+                    Pf[(i0 + ti) + (int64_t)(j0 + c) * m] = -sacc;
+                }
             }
-        }
         __threadfence();
         __syncthreads();
     }
@@ -318,11 +317,14 @@ inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b);
 
 void solve_prepare(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *F,
                    double *Pbuf, double *PTbuf, hipStream_t s) {
+    if (DS.n_lblk > 0) {
+        diag_inv_kernel<<<DS.n_lblk, 64, 0, s>>>(D, DS, F, Pbuf);
+        linv_col_kernel<<<DS.n_lblk, 256, 0, s>>>(D, DS, F, Pbuf);
+    }
     for (int h = 0; h <= P.height; h++) {
         const HeightInfo &H = P.heights[h];
         const int32_t *fl = D.flist + H.list_off;
         if (H.n == 0) continue;
-        linv_kernel<<<dim3(cdiv(H.max_ns, TB), H.n), 256, 0, s>>>(D, fl, F, Pbuf, DS.poff);
         // update rows of each front: m - ns <= max_m (kernel clips)
         m21_kernel<<<dim3(cdiv(H.max_m, 64), cdiv(H.max_ns, 64), H.n), 256, 0, s>>>(D, fl, F, Pbuf, DS.poff);
         transpose_kernel<<<dim3(cdiv(H.max_m, 64), cdiv(H.max_ns, 64), H.n), 256, 0, s>>>(D, fl, Pbuf, PTbuf, DS.poff);

@@ -61,6 +61,18 @@ void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std
         SH.max_ns = H.max_ns;
         S.max_height_tiles = std::max<int64_t>(S.max_height_tiles, std::max(SH.n_fwd, SH.n_bwd));
     }
+    // column blocks of the L11 inverses, longest dependency chain first
+    std::vector<std::pair<int, std::pair<int32_t, int32_t>>> blocks;
+    for (int f = 0; f < nf; f++) {
+        int nb = (ns[f] + 31) / 32;
+        for (int j = 0; j < nb; j++) blocks.push_back({nb - j, {f, 32 * j}});
+    }
+    std::stable_sort(blocks.begin(), blocks.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+    S.lblk.clear();
+    for (auto &b : blocks) {
+        S.lblk.push_back(b.second.first);
+        S.lblk.push_back(b.second.second);
+    }
 }
 
 // Factorization plan (per-height launch bounds) and solve plan from the
