@@ -126,6 +126,17 @@ int sol105_sensitivity_thickness(sol105_ctx *ctx, sol105_model *model, int32_t m
 int sol105_sensitivity_terms(sol105_ctx *ctx, sol105_model *model, int32_t mode, double *terms,
                              double *adjoint_load, double *phi_kg_phi);
 
+/* Diagnostics, no counterpart in the reference (which has no device code):
+ * HIP-event timing of the block triangular-solve product kernel, the
+ * dominant kernel of the eigensolve (one launch per tree height and solve
+ * direction).  sol105_profile_enable resets the counters and turns timing
+ * on (on != 0) or off; sol105_profile_read returns
+ *   out[0] launches timed, out[1] their summed duration in ms,
+ *   out[2] their algorithmic bytes (8 x (factor entries + nrhs x vector
+ *          rows read and written)), out[3] = 0. */
+int sol105_profile_enable(sol105_ctx *ctx, int32_t on);
+int sol105_profile_read(sol105_ctx *ctx, double *out /* 4 */);
+
 /* Phase timings of the last sol105_solve in milliseconds (device events):
  * [0] K assembly, [1] AUTOSPC+symbolic, [2] factorization, [3] static solve,
  * [4] KG assembly, [5] eigensolve, [6] post-processing, [7] total. */

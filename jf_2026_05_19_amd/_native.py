@@ -22,7 +22,8 @@ EXPORTS = [
     "sol105_ctx_create", "sol105_ctx_destroy", "sol105_last_error", "sol105_model_upload", "sol105_model_free",
     "sol105_model_set_q4_thickness", "sol105_pattern", "sol105_assemble_K", "sol105_static_solve",
     "sol105_assemble_KG", "sol105_buckling", "sol105_solve", "sol105_sensitivity_thickness",
-    "sol105_sensitivity_terms", "sol105_last_timings", "sol105_last_stats",
+    "sol105_sensitivity_terms", "sol105_profile_enable", "sol105_profile_read", "sol105_last_timings",
+    "sol105_last_stats",
 ]
 
 
@@ -78,6 +79,8 @@ def lib():
         L.sol105_solve.argtypes = [vp, vp, ctypes.c_int32, dp, dp, ctypes.POINTER(ctypes.c_int32)]
         L.sol105_sensitivity_thickness.argtypes = [vp, vp, ctypes.c_int32, dp]
         L.sol105_sensitivity_terms.argtypes = [vp, vp, ctypes.c_int32, dp, dp, dp]
+        L.sol105_profile_enable.argtypes = [vp, ctypes.c_int32]
+        L.sol105_profile_read.argtypes = [vp, dp]
         L.sol105_last_timings.argtypes = [vp, dp]
         L.sol105_last_stats.argtypes = [vp, dp]
         _lib = L
@@ -114,6 +117,15 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def profile_enable(self, on=True):
+        self.check(self.L.sol105_profile_enable(self.h, 1 if on else 0))
+
+    def profile_read(self):
+        """(launches, total ms, algorithmic bytes) of the timed product launches."""
+        out = np.zeros(4)
+        self.check(self.L.sol105_profile_read(self.h, _dp(out)))
+        return int(out[0]), float(out[1]), float(out[2])
 
     def timings(self):
         out = np.zeros(8)

@@ -34,6 +34,9 @@ struct sol105_ctx {
     double timings[8] = {0};
     double stats[8] = {0};
     hipEvent_t ev[9] = {nullptr};
+    bool profiling = false;
+    ProdTimer ptimer;  // HIP-event timing of the block triangular-solve products
+    ProdTimer *timer() { return profiling ? &ptimer : nullptr; }
 };
 
 namespace {
@@ -92,7 +95,7 @@ struct sol105_model {
     DevFronts dev;
     SolvePlan splan;
     DevBuf<int64_t> s_poff;
-    DevBuf<int32_t> s_ftiles, s_btiles, s_ftbase, s_btbase, s_fnks, s_bnks, s_pullptr, s_pullsrc, s_lblk;
+    DevBuf<int32_t> s_ftiles, s_btiles, s_ftbase, s_btbase, s_fnks, s_bnks, s_fkw, s_bkw, s_pullptr, s_pullsrc, s_lblk;
     DevBuf<double> d_P, d_PT, d_part;
     DevSolve dsolve;
     // state
@@ -222,6 +225,8 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
         HIPCHK(M->s_btbase.upload(SP.bwd_tbase.data(), SP.bwd_tbase.size(), s));
         HIPCHK(M->s_fnks.upload(SP.fwd_nks.data(), SP.fwd_nks.size(), s));
         HIPCHK(M->s_bnks.upload(SP.bwd_nks.data(), SP.bwd_nks.size(), s));
+        HIPCHK(M->s_fkw.upload(SP.fwd_kw.data(), SP.fwd_kw.size(), s));
+        HIPCHK(M->s_bkw.upload(SP.bwd_kw.data(), SP.bwd_kw.size(), s));
         HIPCHK(M->s_pullptr.upload(SP.pull_ptr.data(), SP.pull_ptr.size(), s));
         HIPCHK(M->s_pullsrc.upload(SP.pull_src.data(), SP.pull_src.size(), s));
         HIPCHK(M->s_lblk.upload(SP.lblk.data(), SP.lblk.size(), s));
@@ -236,6 +241,8 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
         DS.bwd_tbase_rel = M->s_btbase.p;
         DS.fwd_nks = M->s_fnks.p;
         DS.bwd_nks = M->s_bnks.p;
+        DS.fwd_kw = M->s_fkw.p;
+        DS.bwd_kw = M->s_bkw.p;
         DS.pull_ptr = M->s_pullptr.p;
         DS.pull_src = M->s_pullsrc.p;
         DS.lblk = M->s_lblk.p;
@@ -621,11 +628,11 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     // into T's column block jj; G0 = Gram matrix after the first pass
     auto expand = [&](int jj) {
         HIPCHK(hipMemcpyAsync(Wb.p, blk(jj), sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
-        solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Wb.p, b, s);
+        solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Wb.p, b, s, ctx->timer());
         launch_scatter_elim_il(n, ndof, b, M->d_elim_of_dof.p, Wb.p, xf.p, s);
         launch_bsr_spmv(M->n_nodes, M->d_rp.p, M->d_col.p, M->d_KG.p, xf.p, yf.p, b, -1.0, s);
         launch_gather_elim_il(n, b, M->d_dof_of_elim.p, yf.p, Wb.p, s);
-        solve_fwd(M->plan, M->dev, M->dsolve, M->splan, M->d_P.p, M->d_W.p, M->d_part.p, Wb.p, b, s);
+        solve_fwd(M->plan, M->dev, M->dsolve, M->splan, M->d_P.p, M->d_W.p, M->d_part.p, Wb.p, b, s, ctx->timer());
         int kcols = (jj + 1) * b;
         double *Tcol = T.p + (size_t)jj * b * ldT;
         ts_gemm_tn(n, kcols, b, V.p, n, Wb.p, n, Tcol, ldT, 1.0, 0.0, part.p, s);
@@ -816,7 +823,7 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
     M->lanczos_m = lo.m;
     M->lanczos_steps = lo.steps;
     // x = L^-T y, expand to all DOFs
-    solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Y.p, k, s);
+    solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Y.p, k, s, ctx->timer());
     HIPCHK(Xf.alloc((size_t)ndof * k));
     launch_scatter_elim(n, ndof, k, M->d_elim_of_dof.p, Y.p, Xf.p, s);
     // eigenvalue semantics: lambda = 1/theta, |theta| > 1e-12 (dense) or
@@ -997,6 +1004,25 @@ int sol105_sensitivity_terms(sol105_ctx *ctx, sol105_model *M, int32_t mode, dou
     if (adjoint_load) HIPCHK(hipMemcpyAsync(adjoint_load, g.p, sizeof(double) * M->ndof(), hipMemcpyDeviceToHost, s));
     if (phi_kg_phi) *phi_kg_phi = pkp;
     HIPCHK(hipStreamSynchronize(s));
+    return SOL105_OK;
+}
+
+int sol105_profile_enable(sol105_ctx *ctx, int32_t on) {
+    if (!ctx) return SOL105_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    ctx->ptimer.reset();
+    ctx->profiling = on != 0;
+    return SOL105_OK;
+}
+
+int sol105_profile_read(sol105_ctx *ctx, double *out) {
+    if (!ctx || !out) return SOL105_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    ctx->ptimer.collect();
+    out[0] = (double)ctx->ptimer.launches;
+    out[1] = ctx->ptimer.total_ms;
+    out[2] = ctx->ptimer.total_bytes;
+    out[3] = 0.0;
     return SOL105_OK;
 }
 

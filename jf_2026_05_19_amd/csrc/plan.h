@@ -7,7 +7,8 @@
 
 #define MAXR 16          // right-hand sides per triangular-solve call
 constexpr int SOLVE_RT = 64;   // rows per product tile of the solves
-constexpr int SOLVE_KS = 128;  // k-slice of the solve products
+constexpr int SOLVE_KS = 128;  // k-slice of the solve products (fronts split over k)
+constexpr int SOLVE_KMAX = 384;  // fronts with inner dimension up to this are not split
 
 struct HeightInfo {
     int n = 0;              // fronts at this height
@@ -30,6 +31,7 @@ struct SolveHeight {
     int64_t fwd_off = 0, bwd_off = 0;  // first tile of this height
     int n_fwd = 0, n_bwd = 0;          // tiles of this height
     int max_m = 0, max_ns = 0;
+    int64_t sum_mns = 0, sum_m = 0, sum_ns = 0;  // factor entries and vector rows of this height
 };
 
 struct SolvePlan {
@@ -38,6 +40,7 @@ struct SolvePlan {
     std::vector<int32_t> fwd_tiles, bwd_tiles;  // (front, row tile, k slice) triples
     std::vector<int32_t> fwd_tbase, bwd_tbase;  // per front: first tile, relative to its height
     std::vector<int32_t> fwd_nks, bwd_nks;      // per front: k slices
+    std::vector<int32_t> fwd_kw, bwd_kw;        // per front: k-slice width
     std::vector<SolveHeight> heights;
     int64_t max_height_tiles = 0;
     // forward-assembly pull map: row i of front f (vector slot vec_off[f] + i)

@@ -12,14 +12,32 @@ struct DevSolve {
     const int32_t *fwd_tiles, *bwd_tiles;
     const int32_t *fwd_tbase_rel, *bwd_tbase_rel;
     const int32_t *fwd_nks, *bwd_nks;
+    const int32_t *fwd_kw, *bwd_kw;
     const int32_t *pull_ptr, *pull_src;
     const int32_t *lblk;  // (front, first column) of every 32-wide L11 column block
     int n_lblk;
 };
 
+// Optional HIP-event timing of the solve product launches (diagnostics):
+// every product launch is bracketed by two events on its stream; collect()
+// waits for them and accumulates the elapsed time and the launches'
+// algorithmic bytes (factor entries + vector rows in and out).
+struct ProdTimer {
+    std::vector<hipEvent_t> pool;
+    std::vector<double> pending_bytes;
+    size_t used = 0;
+    double total_ms = 0.0, total_bytes = 0.0;
+    long launches = 0;
+    void begin(hipStream_t s, double bytes);
+    void end(hipStream_t s);
+    void collect();
+    void reset();
+    ~ProdTimer();
+};
+
 void solve_prepare(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *F,
                    double *Pbuf, double *PTbuf, hipStream_t s);
 void solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *Pbuf,
-               double *W, double *part, double *X, int nrhs, hipStream_t s);
+               double *W, double *part, double *X, int nrhs, hipStream_t s, ProdTimer *pt = nullptr);
 void solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *PTbuf,
-               double *W, double *part, double *X, int nrhs, hipStream_t s);
+               double *W, double *part, double *X, int nrhs, hipStream_t s, ProdTimer *pt = nullptr);

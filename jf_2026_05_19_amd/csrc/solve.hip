@@ -221,24 +221,39 @@ __global__ void bwd_gather_kernel(DevFronts D, const int32_t *flist, double *W, 
 // Wavefront g of the workgroup streams k in [k0 + 32g, k0 + 32g + 32) for the
 // tile's 64 rows (one row per lane, 8 loads in flight per lane); the four
 // partial sums are added in wavefront order (deterministic).
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
+// on XCD b % 8), each with its own L2.  This bijection hands every XCD a
+// contiguous run of logical tiles, so the row tiles of one front (which all
+// read that front's V slice) share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int q = n / 8, r = n % 8, x = b % 8, i = b / 8;
+    return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 template <int NR>
 __global__ void __launch_bounds__(256) prod_kernel(DevFronts D, const int32_t *tiles, const double *A,
-                                                   const int64_t *poff, const double *W, double *part, int nrhs,
-                                                   int bwd) {
-    constexpr int KW = KS / 4;  // k per wavefront
-    int f = tiles[3 * blockIdx.x], rt = tiles[3 * blockIdx.x + 1], ks = tiles[3 * blockIdx.x + 2];
+                                                   const int64_t *poff, double *W, double *part, int nrhs, int bwd,
+                                                   const int32_t *kwid, const int32_t *nks, double *X, int n_free) {
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    int f = tiles[3 * tile], rt = tiles[3 * tile + 1], ks = tiles[3 * tile + 2];
     int ns = D.ns[f], m = D.m[f];
     int nrows = bwd ? ns : m;
     int inner = bwd ? m : ns;
     int lda = bwd ? ns : m;
     const double *Af = A + poff[f];
-    const double *V = W + D.vec_off[f] * nrhs;
-    int k0 = ks * KS, k1 = min(inner, k0 + KS);
-    __shared__ double Vs[KS][NR];
-    __shared__ double red[3][RT][NR + 1];
-    for (int t = threadIdx.x; t < KS * NR; t += blockDim.x) {
+    double *V = W + D.vec_off[f] * nrhs;
+    const int kw = kwid[f];
+    const int KW = (kw + 3) / 4;  // k per wavefront
+    int k0 = ks * kw, k1 = min(inner, k0 + kw);
+    // the V slice and, after the products, the cross-wavefront partial sums
+    // share one LDS buffer
+    constexpr int VS_N = SOLVE_KMAX * NR, RED_N = 3 * RT * (NR + 1);
+    __shared__ double lds[VS_N > RED_N ? VS_N : RED_N];
+    double (*Vs)[NR] = reinterpret_cast<double (*)[NR]>(lds);
+    double (*red)[RT][NR + 1] = reinterpret_cast<double (*)[RT][NR + 1]>(lds);
+    for (int t = threadIdx.x; t < (k1 - k0) * NR; t += blockDim.x) {
         int k = t / NR, r = t % NR;
-        Vs[k][r] = (k0 + k < k1 && r < nrhs) ? V[(k0 + k) + (int64_t)r * m] : 0.0;
+        Vs[k][r] = (r < nrhs) ? V[(k0 + k) + (int64_t)r * m] : 0.0;
     }
     __syncthreads();
     int row = threadIdx.x % RT, g = threadIdx.x / RT;
@@ -250,12 +265,12 @@ __global__ void __launch_bounds__(256) prod_kernel(DevFronts D, const int32_t *t
     if (i < nrows && kb < ke) {
         const double *a = Af + i + (int64_t)kb * lda;
         int k = kb;
-        for (; k + 8 <= ke; k += 8) {
-            double av[8];
+        for (; k + 16 <= ke; k += 16) {
+            double av[16];
 #pragma unroll
-            for (int u = 0; u < 8; u++) av[u] = a[(int64_t)(k - kb + u) * lda];
+            for (int u = 0; u < 16; u++) av[u] = a[(int64_t)(k - kb + u) * lda];
 #pragma unroll
-            for (int u = 0; u < 8; u++)
+            for (int u = 0; u < 16; u++)
 #pragma unroll
                 for (int r = 0; r < NR; r++) acc[r] += av[u] * Vs[k - k0 + u][r];
         }
@@ -265,15 +280,27 @@ __global__ void __launch_bounds__(256) prod_kernel(DevFronts D, const int32_t *t
             for (int r = 0; r < NR; r++) acc[r] += av * Vs[k - k0][r];
         }
     }
+    __syncthreads();  // every wavefront is done with Vs
     if (g > 0) {
 #pragma unroll
         for (int r = 0; r < NR; r++) red[g - 1][row][r] = acc[r];
     }
     __syncthreads();
-    if (g == 0) {
-        for (int r = 0; r < nrhs; r++) {
-            double s = acc[r] + red[0][row][r] + red[1][row][r] + red[2][row][r];
-            part[((int64_t)blockIdx.x * RT + row) * MAXR + r] = s;
+    if (g == 0 && i < nrows) {
+        if (nks[f] == 1) {
+            // single k-slice: the sum is final.  Nothing else in this launch
+            // reads x_piv or the update rows of V (fwd reads V[0:ns] only).
+            const int piv0 = D.piv0[f];
+            for (int r = 0; r < nrhs; r++) {
+                double s = acc[r] + red[0][row][r] + red[1][row][r] + red[2][row][r];
+                if (bwd || i < ns) X[(piv0 + i) + (int64_t)r * n_free] = s;
+                else V[i + (int64_t)r * m] -= s;
+            }
+        } else {
+            for (int r = 0; r < nrhs; r++) {
+                double s = acc[r] + red[0][row][r] + red[1][row][r] + red[2][row][r];
+                part[((int64_t)tile * RT + row) * MAXR + r] = s;
+            }
         }
     }
 }
@@ -285,6 +312,7 @@ __global__ void fwd_finish_kernel(DevFronts D, const int32_t *flist, const int32
     int ns = D.ns[f], m = D.m[f], piv0 = D.piv0[f];
     double *V = W + D.vec_off[f] * nrhs;
     int nk = nks[f];
+    if (nk == 1) return;  // written by the product kernel
     int64_t base = tbase[f];
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m * nrhs; t += gridDim.x * blockDim.x) {
         int i = t % m, r = t / m;
@@ -301,6 +329,7 @@ __global__ void bwd_finish_kernel(DevFronts D, const int32_t *flist, const int32
     int f = flist[blockIdx.y];
     int ns = D.ns[f], piv0 = D.piv0[f];
     int nk = nks[f];
+    if (nk == 1) return;  // written by the product kernel
     int64_t base = tbase[f];
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ns * nrhs; t += gridDim.x * blockDim.x) {
         int i = t % ns, r = t / ns;
@@ -331,19 +360,67 @@ void solve_prepare(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, 
     }
 }
 
+void ProdTimer::begin(hipStream_t s, double bytes) {
+    while (pool.size() < used + 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        pool.push_back(e);
+    }
+    (void)hipEventRecord(pool[used], s);
+    pending_bytes.push_back(bytes);
+}
+
+void ProdTimer::end(hipStream_t s) {
+    if (pool.size() < used + 2) return;
+    (void)hipEventRecord(pool[used + 1], s);
+    used += 2;
+}
+
+void ProdTimer::collect() {
+    for (size_t i = 0; i + 1 < used; i += 2) {
+        float ms = 0.f;
+        (void)hipEventSynchronize(pool[i + 1]);
+        if (hipEventElapsedTime(&ms, pool[i], pool[i + 1]) == hipSuccess) total_ms += ms;
+        total_bytes += pending_bytes[i / 2];
+        launches++;
+    }
+    used = 0;
+    pending_bytes.clear();
+}
+
+void ProdTimer::reset() {
+    collect();
+    total_ms = total_bytes = 0.0;
+    launches = 0;
+}
+
+ProdTimer::~ProdTimer() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+// algorithmic bytes of one product launch: factor entries, vector rows in and out
+static double prod_bytes(const SolveHeight &SH, int nrhs) {
+    return 8.0 * ((double)SH.sum_mns + (double)nrhs * (double)(SH.sum_m + SH.sum_ns));
+}
+
 void solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *Pbuf,
-               double *W, double *part, double *X, int nrhs, hipStream_t s) {
+               double *W, double *part, double *X, int nrhs, hipStream_t s, ProdTimer *pt) {
     for (int h = 0; h <= P.height; h++) {
         const HeightInfo &H = P.heights[h];
         const SolveHeight &SH = S.heights[h];
         if (H.n == 0) continue;
         const int32_t *fl = D.flist + H.list_off;
         fwd_pull_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, DS, fl, W, X, P.n_free, nrhs);
+        const bool timed = pt && nrhs > 1 && h == 0;
+        if (timed) pt->begin(s, prod_bytes(SH, nrhs));
+        const int32_t *tl = DS.fwd_tiles + 3 * SH.fwd_off;
         if (nrhs == 1)
-            prod_kernel<1><<<SH.n_fwd, 256, 0, s>>>(D, DS.fwd_tiles + 3 * SH.fwd_off, Pbuf, DS.poff, W, part, nrhs, 0);
+            prod_kernel<1><<<SH.n_fwd, 256, 0, s>>>(D, tl, Pbuf, DS.poff, W, part, nrhs, 0, DS.fwd_kw, DS.fwd_nks, X,
+                                                    P.n_free);
         else
-            prod_kernel<MAXR><<<SH.n_fwd, 256, 0, s>>>(D, DS.fwd_tiles + 3 * SH.fwd_off, Pbuf, DS.poff, W, part, nrhs,
-                                                       0);
+            prod_kernel<MAXR><<<SH.n_fwd, 256, 0, s>>>(D, tl, Pbuf, DS.poff, W, part, nrhs, 0, DS.fwd_kw, DS.fwd_nks,
+                                                       X, P.n_free);
+        if (timed) pt->end(s);
         fwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.fwd_tbase_rel,
                                                                                        DS.fwd_nks, part, W, X,
                                                                                        P.n_free, nrhs);
@@ -351,18 +428,23 @@ void solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, cons
 }
 
 void solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *PTbuf,
-               double *W, double *part, double *X, int nrhs, hipStream_t s) {
+               double *W, double *part, double *X, int nrhs, hipStream_t s, ProdTimer *pt) {
     for (int h = P.height; h >= 0; h--) {
         const HeightInfo &H = P.heights[h];
         const SolveHeight &SH = S.heights[h];
         if (H.n == 0) continue;
         const int32_t *fl = D.flist + H.list_off;
         bwd_gather_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, fl, W, X, P.n_free, nrhs);
+        const bool timed = pt && nrhs > 1 && h == 0;
+        if (timed) pt->begin(s, prod_bytes(SH, nrhs));
+        const int32_t *tl = DS.bwd_tiles + 3 * SH.bwd_off;
         if (nrhs == 1)
-            prod_kernel<1><<<SH.n_bwd, 256, 0, s>>>(D, DS.bwd_tiles + 3 * SH.bwd_off, PTbuf, DS.poff, W, part, nrhs, 1);
+            prod_kernel<1><<<SH.n_bwd, 256, 0, s>>>(D, tl, PTbuf, DS.poff, W, part, nrhs, 1, DS.bwd_kw, DS.bwd_nks, X,
+                                                    P.n_free);
         else
-            prod_kernel<MAXR><<<SH.n_bwd, 256, 0, s>>>(D, DS.bwd_tiles + 3 * SH.bwd_off, PTbuf, DS.poff, W, part,
-                                                       nrhs, 1);
+            prod_kernel<MAXR><<<SH.n_bwd, 256, 0, s>>>(D, tl, PTbuf, DS.poff, W, part, nrhs, 1, DS.bwd_kw, DS.bwd_nks,
+                                                       X, P.n_free);
+        if (timed) pt->end(s);
         bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.bwd_tbase_rel,
                                                                                         DS.bwd_nks, part, X,
                                                                                         P.n_free, nrhs);

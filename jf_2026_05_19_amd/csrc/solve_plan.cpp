@@ -27,6 +27,8 @@ void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std
     S.bwd_tbase.assign(nf, 0);
     S.fwd_nks.assign(nf, 0);
     S.bwd_nks.assign(nf, 0);
+    S.fwd_kw.assign(nf, KS);
+    S.bwd_kw.assign(nf, KS);
     S.heights.assign(P.height + 1, SolveHeight());
     S.max_height_tiles = 0;
     for (int h = 0; h <= P.height; h++) {
@@ -36,7 +38,15 @@ void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std
         SH.bwd_off = (int64_t)S.bwd_tiles.size() / 3;
         for (int q = 0; q < H.n; q++) {
             int f = flist[H.list_off + q];
-            int nrt = (m[f] + RT - 1) / RT, nk = (ns[f] + KS - 1) / KS;
+            SH.sum_mns += (int64_t)m[f] * ns[f];
+            SH.sum_m += m[f];
+            SH.sum_ns += ns[f];
+            // one k-slice (and no partial sums) when the inner dimension fits
+            const int kwf = ns[f] <= SOLVE_KMAX ? std::max(ns[f], 1) : KS;
+            const int kwb = m[f] <= SOLVE_KMAX ? std::max(m[f], 1) : KS;
+            S.fwd_kw[f] = kwf;
+            S.bwd_kw[f] = kwb;
+            int nrt = (m[f] + RT - 1) / RT, nk = (ns[f] + kwf - 1) / kwf;
             S.fwd_tbase[f] = (int32_t)(S.fwd_tiles.size() / 3 - SH.fwd_off);
             S.fwd_nks[f] = nk;
             for (int rt = 0; rt < nrt; rt++)
@@ -45,7 +55,7 @@ void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std
                     S.fwd_tiles.push_back(rt);
                     S.fwd_tiles.push_back(k);
                 }
-            int nrtb = (ns[f] + RT - 1) / RT, nkb = (m[f] + KS - 1) / KS;
+            int nrtb = (ns[f] + RT - 1) / RT, nkb = (m[f] + kwb - 1) / kwb;
             S.bwd_tbase[f] = (int32_t)(S.bwd_tiles.size() / 3 - SH.bwd_off);
             S.bwd_nks[f] = nkb;
             for (int rt = 0; rt < nrtb; rt++)
