@@ -113,10 +113,11 @@ def main():
     print(json.dumps(result))
 
 
-# HBM bytes per launch of the dominant kernel from rocprofv3 --pmc FETCH_SIZE +
-# WRITE_SIZE (profiles/prod_kernel_c2_panel_pmc.txt), each scaled by the same
-# counter's reading for a plain device copy of known size; None until measured.
-PROD_TRAFFIC_BYTES = None
+# HBM bytes per leaf-height launch of the dominant kernel from rocprofv3 --pmc
+# FETCH_SIZE (224.34 MB) + WRITE_SIZE (15.85 MB), each scaled by the same
+# counter's reading for a 256 MiB device copy (factors 2.0 and 1.0); the runs
+# and the per-grid figures are in profiles/prod_kernel_c2_panel_pmc.txt.
+PROD_TRAFFIC_BYTES = 240.18e6
 
 
 def roofline(prof):
@@ -124,7 +125,8 @@ def roofline(prof):
     (prod_kernel<16>): algorithmic bytes per launch (factor entries streamed
     once + the right-hand-side rows read and written) over the average launch
     duration, both from HIP events recorded on the solve stream around every
-    launch in the timed steps."""
+    leaf-height launch (the two largest grids, forward and backward) of the
+    block Lanczos steps in the timed region."""
     launches, ms, nbytes = prof
     if launches == 0 or ms <= 0:
         return None
