@@ -44,6 +44,9 @@ typedef struct sol105_model_desc {
     int32_t n_q4;
     const int32_t *q4_nodes;    /* n_q4 x 4 node ranks (CQUAD4 G1..G4)            */
     const double *q4_prop;      /* n_q4 x 6: E, nu, T, 12I/T^3, TS/T, G_shear     */
+    int32_t n_t3;
+    const int32_t *t3_nodes;    /* n_t3 x 3 node ranks (CTRIA3 G1..G3)            */
+    const double *t3_prop;      /* n_t3 x 6, columns as q4_prop                   */
     int32_t n_bar;
     const int32_t *bar_nodes;   /* n_bar x 2 node ranks (GA, GB)                  */
     const double *bar_vref;     /* n_bar x 3 orientation vector (V, or G0-GA)     */

@@ -42,6 +42,8 @@ class ModelDesc(ctypes.Structure):
         ("n_nodes", ctypes.c_int32), ("xyz", ctypes.POINTER(ctypes.c_double)),
         ("n_q4", ctypes.c_int32), ("q4_nodes", ctypes.POINTER(ctypes.c_int32)),
         ("q4_prop", ctypes.POINTER(ctypes.c_double)),
+        ("n_t3", ctypes.c_int32), ("t3_nodes", ctypes.POINTER(ctypes.c_int32)),
+        ("t3_prop", ctypes.POINTER(ctypes.c_double)),
         ("n_bar", ctypes.c_int32), ("bar_nodes", ctypes.POINTER(ctypes.c_int32)),
         ("bar_vref", ctypes.POINTER(ctypes.c_double)), ("bar_prop", ctypes.POINTER(ctypes.c_double)),
         ("spc_mask", ctypes.POINTER(ctypes.c_uint8)), ("force", ctypes.POINTER(ctypes.c_double)),
@@ -149,6 +151,8 @@ class DeviceModel:
             xyz=np.ascontiguousarray(arr.xyz, dtype=np.float64),
             q4=np.ascontiguousarray(arr.q4, dtype=np.int32),
             q4prop=np.ascontiguousarray(arr.q4prop, dtype=np.float64),
+            t3=np.ascontiguousarray(arr.t3, dtype=np.int32),
+            t3prop=np.ascontiguousarray(arr.t3prop, dtype=np.float64),
             bar=np.ascontiguousarray(arr.bar, dtype=np.int32),
             vref=np.ascontiguousarray(arr.bar_vref, dtype=np.float64),
             barprop=np.ascontiguousarray(arr.barprop, dtype=np.float64),
@@ -161,7 +165,7 @@ class DeviceModel:
         ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
         up = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
         d = ModelDesc(arr.n_nodes, _dp(k["xyz"]), arr.n_q4, ip(k["q4"]), _dp(k["q4prop"]),
-                      arr.n_bar, ip(k["bar"]), _dp(k["vref"]), _dp(k["barprop"]),
+                      arr.n_t3, ip(k["t3"]), _dp(k["t3prop"]), arr.n_bar, ip(k["bar"]), _dp(k["vref"]), _dp(k["barprop"]),
                       up(k["spc"]), _dp(k["F"]), up(k["hs"]), up(k["hb"]),
                       float(arr.k6rot), int(arr.autospc), int(arr.nd))
         h = ctypes.c_void_p()

@@ -69,17 +69,17 @@ struct DevBuf {
 
 struct sol105_model {
     // host copies
-    int n_nodes = 0, n_q4 = 0, n_bar = 0;
-    std::vector<double> xyz, q4prop, bar_vref, barprop, force;
-    std::vector<int32_t> q4, bar;
+    int n_nodes = 0, n_q4 = 0, n_t3 = 0, n_bar = 0;
+    std::vector<double> xyz, q4prop, t3prop, bar_vref, barprop, force;
+    std::vector<int32_t> q4, t3, bar;
     std::vector<uint8_t> spc_mask, has_shell, has_bar;
     double k6rot = 100.0;
     int autospc = 1, nd = 3;
     BlockPattern pat;
     std::vector<int32_t> diag_blk;
     // device: model + assembly
-    DevBuf<double> d_xyz, d_q4prop, d_bar_vref, d_barprop, d_force;
-    DevBuf<int32_t> d_q4, d_bar, d_rp, d_col, d_cptr, d_cidx, d_diag;
+    DevBuf<double> d_xyz, d_q4prop, d_t3prop, d_bar_vref, d_barprop, d_force;
+    DevBuf<int32_t> d_q4, d_t3, d_bar, d_rp, d_col, d_cptr, d_cidx, d_diag;
     DevBuf<double> d_eb, d_K, d_KG, d_u, d_kdiag;
     // factor
     bool have_sym = false;
@@ -130,10 +130,13 @@ int set_err(sol105_ctx *ctx, int code, const char *fmt, ...) {
 ElemArgs elem_args(sol105_model *M) {
     ElemArgs a;
     a.n_q4 = M->n_q4;
+    a.n_t3 = M->n_t3;
     a.n_bar = M->n_bar;
     a.xyz = M->d_xyz.p;
     a.q4 = M->d_q4.p;
     a.q4prop = M->d_q4prop.p;
+    a.t3 = M->d_t3.p;
+    a.t3prop = M->d_t3prop.p;
     a.bar = M->d_bar.p;
     a.bar_vref = M->d_bar_vref.p;
     a.barprop = M->d_barprop.p;
@@ -301,8 +304,9 @@ int sol105_model_upload(sol105_ctx *ctx, const sol105_model_desc *d, sol105_mode
     if (!ctx || !d || !out) return set_err(ctx, SOL105_E_INVALID, "null argument");
     *out = nullptr;
     if (d->n_nodes <= 0 || !d->xyz) return set_err(ctx, SOL105_E_INVALID, "model has no nodes");
-    if (d->n_q4 < 0 || d->n_bar < 0) return set_err(ctx, SOL105_E_INVALID, "negative element count");
-    if ((d->n_q4 > 0 && (!d->q4_nodes || !d->q4_prop)) || (d->n_bar > 0 && (!d->bar_nodes || !d->bar_prop || !d->bar_vref)))
+    if (d->n_q4 < 0 || d->n_t3 < 0 || d->n_bar < 0) return set_err(ctx, SOL105_E_INVALID, "negative element count");
+    if ((d->n_q4 > 0 && (!d->q4_nodes || !d->q4_prop)) || (d->n_t3 > 0 && (!d->t3_nodes || !d->t3_prop)) ||
+        (d->n_bar > 0 && (!d->bar_nodes || !d->bar_prop || !d->bar_vref)))
         return set_err(ctx, SOL105_E_INVALID, "missing element arrays");
     if (!d->spc_mask || !d->force || !d->node_has_shell || !d->node_has_bar)
         return set_err(ctx, SOL105_E_INVALID, "missing SPC / load / topology arrays");
@@ -312,10 +316,13 @@ int sol105_model_upload(sol105_ctx *ctx, const sol105_model_desc *d, sol105_mode
     int n = d->n_nodes;
     M->n_nodes = n;
     M->n_q4 = d->n_q4;
+    M->n_t3 = d->n_t3;
     M->n_bar = d->n_bar;
     M->xyz.assign(d->xyz, d->xyz + 3 * (size_t)n);
     M->q4.assign(d->q4_nodes, d->q4_nodes + 4 * (size_t)d->n_q4);
     M->q4prop.assign(d->q4_prop, d->q4_prop + 6 * (size_t)d->n_q4);
+    M->t3.assign(d->t3_nodes, d->t3_nodes + 3 * (size_t)d->n_t3);
+    M->t3prop.assign(d->t3_prop, d->t3_prop + 6 * (size_t)d->n_t3);
     M->bar.assign(d->bar_nodes, d->bar_nodes + 2 * (size_t)d->n_bar);
     M->bar_vref.assign(d->bar_vref, d->bar_vref + 3 * (size_t)d->n_bar);
     M->barprop.assign(d->bar_prop, d->bar_prop + 8 * (size_t)d->n_bar);
@@ -328,19 +335,27 @@ int sol105_model_upload(sol105_ctx *ctx, const sol105_model_desc *d, sol105_mode
     M->nd = d->nd;
     for (int32_t v : M->q4)
         if (v < 0 || v >= n) return set_err(ctx, SOL105_E_INVALID, "CQUAD4 node rank %d out of range", v);
+    for (int32_t v : M->t3)
+        if (v < 0 || v >= n) return set_err(ctx, SOL105_E_INVALID, "CTRIA3 node rank %d out of range", v);
     for (int32_t v : M->bar)
         if (v < 0 || v >= n) return set_err(ctx, SOL105_E_INVALID, "CBAR node rank %d out of range", v);
     for (int e = 0; e < M->n_q4; e++) {
         const double *p = &M->q4prop[6 * (size_t)e];
         if (!(p[2] > 0.0) || !(p[0] > 0.0)) return set_err(ctx, SOL105_E_INVALID, "CQUAD4 %d: T and E must be positive", e);
     }
-    build_block_pattern(n, M->n_q4, M->q4.data(), M->n_bar, M->bar.data(), M->pat);
+    for (int e = 0; e < M->n_t3; e++) {
+        const double *p = &M->t3prop[6 * (size_t)e];
+        if (!(p[2] > 0.0) || !(p[0] > 0.0)) return set_err(ctx, SOL105_E_INVALID, "CTRIA3 %d: T and E must be positive", e);
+    }
+    build_block_pattern(n, M->n_q4, M->q4.data(), M->n_t3, M->t3.data(), M->n_bar, M->bar.data(), M->pat);
     M->diag_blk.resize(n);
     for (int v = 0; v < n; v++) M->diag_blk[v] = (int32_t)M->pat.find(v, v);
     hipStream_t s = ctx->stream;
     HIPCHK(M->d_xyz.upload(M->xyz.data(), M->xyz.size(), s));
     HIPCHK(M->d_q4.upload(M->q4.data(), M->q4.size(), s));
     HIPCHK(M->d_q4prop.upload(M->q4prop.data(), M->q4prop.size(), s));
+    HIPCHK(M->d_t3.upload(M->t3.data(), M->t3.size(), s));
+    HIPCHK(M->d_t3prop.upload(M->t3prop.data(), M->t3prop.size(), s));
     HIPCHK(M->d_bar.upload(M->bar.data(), M->bar.size(), s));
     HIPCHK(M->d_bar_vref.upload(M->bar_vref.data(), M->bar_vref.size(), s));
     HIPCHK(M->d_barprop.upload(M->barprop.data(), M->barprop.size(), s));
@@ -350,7 +365,7 @@ int sol105_model_upload(sol105_ctx *ctx, const sol105_model_desc *d, sol105_mode
     HIPCHK(M->d_cptr.upload(M->pat.cptr.data(), M->pat.cptr.size(), s));
     HIPCHK(M->d_cidx.upload(M->pat.cidx.data(), M->pat.cidx.size(), s));
     HIPCHK(M->d_diag.upload(M->diag_blk.data(), M->diag_blk.size(), s));
-    size_t neb = ((size_t)M->n_q4 * 16 + (size_t)M->n_bar * 4) * 36;
+    size_t neb = ((size_t)M->n_q4 * 16 + (size_t)M->n_t3 * 9 + (size_t)M->n_bar * 4) * 36;
     HIPCHK(M->d_eb.alloc(neb));
     HIPCHK(M->d_K.alloc(M->pat.n_blocks() * 36));
     HIPCHK(M->d_KG.alloc(M->pat.n_blocks() * 36));
@@ -397,6 +412,7 @@ int sol105_assemble_K(sol105_ctx *ctx, sol105_model *M, double *vals) {
     hipStream_t s = ctx->stream;
     ElemArgs a = elem_args(M);
     launch_q4_ke(a, M->d_eb.p, s);
+    launch_t3_ke(a, M->d_eb.p, s);
     launch_bar_ke(a, M->d_eb.p, s);
     launch_block_gather(M->pat.n_blocks(), M->d_cptr.p, M->d_cidx.p, M->d_eb.p, M->d_K.p, s);
     HIPCHK(hipGetLastError());
@@ -497,6 +513,7 @@ int sol105_assemble_KG(sol105_ctx *ctx, sol105_model *M, const double *u, double
     if (!M->u_valid) return set_err(ctx, SOL105_E_STATE, "no static displacement: run sol105_static_solve first");
     ElemArgs a = elem_args(M);
     launch_q4_kg(a, M->d_u.p, M->d_eb.p, s);
+    launch_t3_kg(a, M->d_u.p, M->d_eb.p, s);
     launch_bar_kg(a, M->d_u.p, M->d_eb.p, s);
     launch_block_gather(M->pat.n_blocks(), M->d_cptr.p, M->d_cidx.p, M->d_eb.p, M->d_KG.p, s);
     HIPCHK(hipGetLastError());
@@ -846,12 +863,13 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
     std::stable_sort(pos.begin(), pos.end(), [&](int a, int b) { return lam[a] < lam[b]; });
     // localization filter (defaults: >= 100 shells, share 0.10, scan ND + 16)
     std::vector<int> surv;
-    if (M->n_q4 >= 100 && !pos.empty()) {
+    int n_shell = M->n_q4 + M->n_t3;
+    if (n_shell >= 100 && !pos.empty()) {
         int limit = std::min((int)pos.size(), std::max(nd + 16, nd));
         DevBuf<double> scr, sh;
-        HIPCHK(scr.alloc(2 * ((M->n_q4 + 255) / 256) + 2));
+        HIPCHK(scr.alloc(2 * ((n_shell + 255) / 256) + 2));
         HIPCHK(sh.alloc(2 * (size_t)limit));
-        for (int t = 0; t < limit; t++) elem_energy_share(M->n_q4, M->d_q4.p, ndof, Xf.p, pos[t], scr.p, sh.p + 2 * t, s);
+        for (int t = 0; t < limit; t++) elem_energy_share(M->n_q4, M->d_q4.p, M->n_t3, M->d_t3.p, ndof, Xf.p, pos[t], scr.p, sh.p + 2 * t, s);
         std::vector<double> hsh(2 * (size_t)limit);
         HIPCHK(hipMemcpyAsync(hsh.data(), sh.p, hsh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -940,6 +958,8 @@ int sens_terms(sol105_ctx *ctx, sol105_model *M, int32_t mode, DevBuf<double> &t
                double &pkp) {
     if (!ctx || !M) return set_err(ctx, SOL105_E_INVALID, "null argument");
     if (mode < 0 || mode >= M->n_modes) return set_err(ctx, SOL105_E_STATE, "mode %d not available", mode);
+    if (M->n_t3 > 0)
+        return set_err(ctx, SOL105_E_UNSUPPORTED, "thickness sensitivities of models with CTRIA3 are not supported");
     (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
     int ndof = M->ndof(), n = M->sym.n_free;

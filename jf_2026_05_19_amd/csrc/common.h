@@ -9,15 +9,19 @@
 
 // Element-block layout: every element writes its global stiffness as dense
 // 6x6 node blocks (row-major), CQUAD4 first (16 blocks per element, index
-// e*16 + a*4 + b for corner nodes a, b), then CBAR (4 blocks per bar,
-// index q4_blocks + b*4 + a*2 + c).  The block gather sums these into the
-// 6x6 block-sparse-row matrix in a fixed element order.
+// e*16 + a*4 + b for corner nodes a, b), then CTRIA3 (9 blocks per element,
+// index q4_blocks + e*9 + a*3 + b), then CBAR (4 blocks per bar, index
+// q4_blocks + t3_blocks + b*4 + a*2 + c).  The block gather sums these into
+// the 6x6 block-sparse-row matrix in a fixed element order.
 struct ElemArgs {
     int n_q4;
+    int n_t3;
     int n_bar;
     const double *xyz;        // n_nodes x 3
     const int32_t *q4;        // n_q4 x 4
     const double *q4prop;     // n_q4 x 6
+    const int32_t *t3;        // n_t3 x 3
+    const double *t3prop;     // n_t3 x 6
     const int32_t *bar;       // n_bar x 2
     const double *bar_vref;   // n_bar x 3
     const double *barprop;    // n_bar x 8
@@ -27,6 +31,8 @@ struct ElemArgs {
 // launchers (elements.hip)
 void launch_q4_ke(const ElemArgs &a, double *eb, hipStream_t s);
 void launch_q4_kg(const ElemArgs &a, const double *u, double *eb, hipStream_t s);
+void launch_t3_ke(const ElemArgs &a, double *eb, hipStream_t s);
+void launch_t3_kg(const ElemArgs &a, const double *u, double *eb, hipStream_t s);
 void launch_bar_ke(const ElemArgs &a, double *eb, hipStream_t s);
 void launch_bar_kg(const ElemArgs &a, const double *u, double *eb, hipStream_t s);
 void launch_block_gather(int64_t n_blocks, const int32_t *cptr, const int32_t *cidx, const double *eb,

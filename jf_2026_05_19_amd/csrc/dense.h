@@ -21,8 +21,10 @@ void hash_fill(int n, int cols, double *V, int ldv, uint64_t seed, hipStream_t s
 void trsm_right_upper(int n, int b, double *W, int ldw, const double *R, int ldr, hipStream_t s);
 void upper_mul(int b, const double *R2, int ld2, const double *R1, int ld1, double *out, int ldo, hipStream_t s);
 void start_block(int n, int b, double *V, int ldv, int seed, hipStream_t s);
-void elem_energy_share(int n_q4, const int32_t *q4, int ndof, const double *modes, int k, double *scratch,
-                       double *out2, hipStream_t s);
+// (max, sum) over the shells (CQUAD4, CTRIA3) of the mean |translation|^2
+// at their nodes, for mode column k
+void elem_energy_share(int n_q4, const int32_t *q4, int n_t3, const int32_t *t3, int ndof, const double *modes,
+                       int k, double *scratch, double *out2, hipStream_t s);
 void normalize_maxabs(int64_t n, double *x, double *scratch, hipStream_t s);
 void symmetrize_copy(int m, int ld, const double *T, double *A, hipStream_t s);
 void dot_det(int64_t n, const double *x, const double *y, double *scratch, double *out, hipStream_t s);

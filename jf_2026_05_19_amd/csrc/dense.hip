@@ -282,8 +282,9 @@ __global__ void start_block_kernel(int n, int b, double *V, int ldv, int seed) {
 // per-element translational energy share of each mode (localization filter,
 // JFEM/src/solver/solve_case.jl:2795-2893): e = mean over corner nodes of
 // tx^2+ty^2+tz^2; out[k] = max_e / sum_e for mode k (two-stage).
-__global__ void elem_energy_partial_kernel(int n_q4, const int32_t *q4, int ndof, const double *modes, int k,
-                                           double *pmax, double *psum) {
+// per shell (CQUAD4 then CTRIA3): mean over its nodes of |translation|^2
+__global__ void elem_energy_partial_kernel(int n_q4, const int32_t *q4, int n_t3, const int32_t *t3, int ndof,
+                                           const double *modes, int k, double *pmax, double *psum) {
     __shared__ double smax[256], ssum[256];
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     const double *x = modes + (int64_t)k * ndof;
@@ -294,6 +295,13 @@ __global__ void elem_energy_partial_kernel(int n_q4, const int32_t *q4, int ndof
             v += t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
         }
         v /= 4.0;
+    } else if (e < n_q4 + n_t3) {
+        int f = e - n_q4;
+        for (int a = 0; a < 3; a++) {
+            const double *t = x + 6 * (int64_t)t3[3 * f + a];
+            v += t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+        }
+        v /= 3.0;
     }
     smax[threadIdx.x] = v;
     ssum[threadIdx.x] = v;
@@ -405,10 +413,10 @@ void start_block(int n, int b, double *V, int ldv, int seed, hipStream_t s) {
     start_block_kernel<<<nblk((int64_t)n * b, 256), 256, 0, s>>>(n, b, V, ldv, seed);
 }
 
-void elem_energy_share(int n_q4, const int32_t *q4, int ndof, const double *modes, int k, double *scratch,
-                       double *out2, hipStream_t s) {
-    unsigned nb = nblk(n_q4, 256);
-    elem_energy_partial_kernel<<<nb, 256, 0, s>>>(n_q4, q4, ndof, modes, k, scratch, scratch + nb);
+void elem_energy_share(int n_q4, const int32_t *q4, int n_t3, const int32_t *t3, int ndof, const double *modes,
+                       int k, double *scratch, double *out2, hipStream_t s) {
+    unsigned nb = nblk(n_q4 + n_t3, 256);
+    elem_energy_partial_kernel<<<nb, 256, 0, s>>>(n_q4, q4, n_t3, t3, ndof, modes, k, scratch, scratch + nb);
     final_share_kernel<<<1, 256, 0, s>>>((int)nb, scratch, scratch + nb, out2);
 }
 

@@ -17,6 +17,8 @@
 //   T'KT ........... JFEM/src/solver/assembly.jl:3808-3837
 //   KG ............. JFEM/src/FEMKernels.jl:6341-6539, 6708-6785, 6891-6972,
 //                    7554-7985; JFEM/src/solver/assembly.jl:5175-6007
+//   CTRIA3 ......... JFEM/src/FEMKernels.jl:7129-7209, 7315-7432, 7989-8103;
+//                    JFEM/src/solver/assembly.jl:3862-3955, 6020-6160
 //   CBAR ........... JFEM/src/FEMKernels.jl:140-229, 7441-7471;
 //                    JFEM/src/solver/assembly.jl:3960-4020, 6272-6345
 #include "common.h"
@@ -144,8 +146,10 @@ __device__ __forceinline__ void bm_col(const Q4Stage &S, int g, int a, double *b
     b[1] = (c == 1) ? dy : 0.0;
     b[2] = (c == 0) ? dy : ((c == 1) ? dx : 0.0);
 }
-// bending B column with the MacNeal twist row
-__device__ __forceinline__ void bb_col(const Q4Stage &S, int g, int a, double *b) {
+// bending B column with the MacNeal twist row (S: Q4Stage or a virtual quad
+// of the CTRIA3 macro element, which carry the same geometry members)
+template <class ST>
+__device__ __forceinline__ void bb_col(const ST &S, int g, int a, double *b) {
     int k = a / 6, c = a % 6;
     double dx = S.gdx[g][k], dy = S.gdy[g][k];
     if (c == 4) { b[0] = dx; b[1] = 0.0; b[2] = 2.0 * dy - S.dNdy_c[k]; }
@@ -210,7 +214,8 @@ __device__ void shell_props(const double *prop, double k6rot, ShellProps &P) {
 
 // Stage Gauss-point data (lanes 0-3), centre derivatives (lane 4) and MacNeal
 // sample points (lanes 8-11).
-__device__ void q4_stage_geometry(Q4Stage &S, int lane) {
+template <class ST>
+__device__ void q4_stage_geometry(ST &S, int lane) {
     if (lane < 4) {
         const double gps[4][2] = {{-PT, -PT}, {PT, -PT}, {PT, PT}, {-PT, PT}};
         double r = gps[lane][0], s = gps[lane][1];
@@ -266,7 +271,8 @@ __device__ void q4_stage_geometry(Q4Stage &S, int lane) {
 }
 
 // MacNeal compliance Zs, Zb and Zi = (Zs + Zb)^-1 (single lane).
-__device__ void q4_macneal_z(Q4Stage &S, const ShellProps &P) {
+template <class ST>
+__device__ void q4_macneal_z(ST &S, const ShellProps &P) {
     const double *lc = S.lc;
     double Dx = 0.5 * (lc[2] + lc[4] - lc[0] - lc[6]);
     double Dy = 0.5 * (lc[5] + lc[7] - lc[1] - lc[3]);
@@ -865,6 +871,297 @@ __global__ void bar_kg_kernel(ElemArgs A, const double *u, double *eb) {
 }
 
 // ---------------------------------------------------------------------------
+// CTRIA3, one 64-lane wavefront per element, four elements per workgroup.
+//   Ke = CST membrane Bm' Cm Bm A on (u, v)
+//      + macro plate block on (w, thx, thy): the plate rows/columns of three
+//        virtual MacNeal CQUAD4s on the corners, mid-sides and centroid
+//        (kernel default flags: bending incompatible modes kept), assembled
+//        21 x 21 and the 12 interior DOFs condensed by partially pivoted LU
+//      + drilling alpha (Bd' Bd) A, alpha = (k6rot/1e5) G_ref h, at the centroid.
+//   FEMKernels.jl:7129-7209, 7315-7385; assembly.jl:3862-3955.
+//   KG = h A (sxx b_i b_j + syy c_i c_j + sxy (b_i c_j + c_i b_j)) on u, v, w
+//   from the constant-strain stress.  FEMKernels.jl:7391-7432, 7989-8103;
+//   assembly.jl:6020-6160.
+// ---------------------------------------------------------------------------
+
+// One virtual quad of the macro plate block; member names follow Q4Stage so
+// the CQUAD4 staging helpers apply unchanged.
+struct T3Quad {
+    double lc[8];
+    double gdx[4][4], gdy[4][4], gN[4][4];
+    double gadj[4];
+    double gdp[4][4];
+    double dNdx_c[4], dNdy_c[4];
+    double D[4][12];
+    double Jp[4];
+    double Zi[16], Zs[16], Zb[16];
+    double ZiD[4][12];
+};
+
+struct T3Stage {
+    double R[9], lc[6];
+    double bv[3], cv[3], area;
+    T3Quad q[3];
+    double K21[21 * 21];   // assembled plate matrix of the 7 macro points
+    double M[12 * 21];     // [Kii | Kie] -> LU factors | Kii^-1 Kie
+    double Kc[81];         // condensed plate block (corner w, thx, thy)
+    double out[324];       // 9 global node blocks of 36
+};
+
+// Virtual quads (corner, mid-side, centroid, mid-side) over the points
+// 0-2 corners, 3-5 mid-sides 12/23/31, 6 centroid.
+__constant__ int T3_QUAD[3][4] = {{0, 3, 6, 5}, {1, 4, 6, 3}, {2, 5, 6, 4}};
+
+// position of macro point n in virtual quad q, or -1
+__device__ __forceinline__ int t3_pos(int q, int n) {
+    for (int k = 0; k < 4; k++)
+        if (T3_QUAD[q][k] == n) return k;
+    return -1;
+}
+
+// CTRIA3 frame (v1 = e12, v3 = v1 x (p3 - p1), v2 = v3 x v1) and corner
+// coordinates about the centroid.  helpers.jl:1195-1204, assembly.jl:3927-3930.
+__device__ void t3_frame(const double *p /*9*/, double *R /*9*/, double *lc /*6*/) {
+    double v1[3], v2[3], v3[3], d13[3];
+    for (int i = 0; i < 3; i++) { v1[i] = p[3 + i] - p[i]; d13[i] = p[6 + i] - p[i]; }
+    normalize3(v1);
+    cross3(v1, d13, v3);
+    normalize3(v3);
+    cross3(v3, v1, v2);
+    for (int i = 0; i < 3; i++) { R[i] = v1[i]; R[3 + i] = v2[i]; R[6 + i] = v3[i]; }
+    double c[3];
+    for (int i = 0; i < 3; i++) c[i] = (p[i] + p[3 + i] + p[6 + i]) / 3.0;
+    for (int k = 0; k < 3; k++) {
+        double d[3] = {p[3 * k] - c[0], p[3 * k + 1] - c[1], p[3 * k + 2] - c[2]};
+        lc[2 * k] = dot3(d, v1);
+        lc[2 * k + 1] = dot3(d, v2);
+    }
+}
+
+// area and the constant shape-function derivatives b = dN/dx, c = dN/dy
+__device__ double t3_derivs(const double *lc, double *bv, double *cv) {
+    double x0 = lc[0], y0 = lc[1], x1 = lc[2], y1 = lc[3], x2 = lc[4], y2 = lc[5];
+    double A = 0.5 * fabs(x0 * (y1 - y2) + x1 * (y2 - y0) + x2 * (y0 - y1));
+    bv[0] = (y1 - y2) / (2 * A); bv[1] = (y2 - y0) / (2 * A); bv[2] = (y0 - y1) / (2 * A);
+    cv[0] = (x2 - x1) / (2 * A); cv[1] = (x0 - x2) / (2 * A); cv[2] = (x1 - x0) / (2 * A);
+    return A;
+}
+
+// Plate-block entry (a, b), plate indices 0..11 (w, thx, thy per corner),
+// of a virtual quad: 2x2 Gauss bending with the MacNeal twist row plus the
+// MacNeal shear D' Z^-1 D (symmetrised as the kernel does).
+__device__ double t3_quad_plate_entry(const T3Quad &Q, const ShellProps &P, int a, int b, bool has_shear) {
+    int a24 = 6 * (a / 3) + 2 + a % 3, b24 = 6 * (b / 3) + 2 + b % 3;
+    double kb = 0.0;
+    for (int g = 0; g < 4; g++) {
+        double ba[3], bb[3], t[3];
+        bb_col(Q, g, a24, ba);
+        bb_col(Q, g, b24, bb);
+        mat3v(P.Cb, bb, t);
+        kb += Q.gadj[g] * (ba[0] * t[0] + ba[1] * t[1] + ba[2] * t[2]);
+    }
+    if (has_shear) {
+        double s1 = 0.0, s2 = 0.0;
+        for (int l = 0; l < 4; l++) {
+            s1 += Q.D[l][a] * Q.ZiD[l][b];
+            s2 += Q.D[l][b] * Q.ZiD[l][a];
+        }
+        kb += 0.5 * (s1 + s2);
+    }
+    return kb;
+}
+
+__global__ void __launch_bounds__(256) t3_ke_kernel(ElemArgs A, double *eb) {
+    __shared__ T3Stage stage[EPB];
+    int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
+    int e = blockIdx.x * EPB + w;
+    T3Stage &S = stage[w];
+    bool valid = e < A.n_t3;
+    ShellProps P;
+    double Gref = 0.0;
+    if (valid) {
+        double p[9];
+        for (int k = 0; k < 3; k++) {
+            int nd = A.t3[3 * e + k];
+            for (int i = 0; i < 3; i++) p[3 * k + i] = A.xyz[3 * nd + i];
+        }
+        shell_props(A.t3prop + 6 * e, A.k6rot, P);
+        Gref = A.t3prop[6 * e + 5];
+        if (lane == 0) {
+            t3_frame(p, S.R, S.lc);
+            S.area = t3_derivs(S.lc, S.bv, S.cv);
+            double pts[7][2];
+            for (int k = 0; k < 3; k++) { pts[k][0] = S.lc[2 * k]; pts[k][1] = S.lc[2 * k + 1]; }
+            for (int d = 0; d < 2; d++) {
+                pts[3][d] = (S.lc[d] + S.lc[2 + d]) / 2;
+                pts[4][d] = (S.lc[2 + d] + S.lc[4 + d]) / 2;
+                pts[5][d] = (S.lc[4 + d] + S.lc[d]) / 2;
+                pts[6][d] = (S.lc[d] + S.lc[2 + d] + S.lc[4 + d]) / 3;
+            }
+            for (int q = 0; q < 3; q++)
+                for (int k = 0; k < 4; k++) {
+                    S.q[q].lc[2 * k] = pts[T3_QUAD[q][k]][0];
+                    S.q[q].lc[2 * k + 1] = pts[T3_QUAD[q][k]][1];
+                }
+        }
+    }
+    __syncthreads();
+    bool nonzero = valid && S.area >= 1e-12;
+    bool plate = nonzero && P.bend;
+    bool has_shear = plate && P.h >= 1e-30 && P.gs >= 1e-30;
+    if (plate && lane < 48) q4_stage_geometry(S.q[lane / 16], lane % 16);
+    __syncthreads();
+    if (has_shear && lane < 3) q4_macneal_z(S.q[lane], P);
+    __syncthreads();
+    if (has_shear)
+        for (int t = lane; t < 3 * 48; t += S105_WAVE) {
+            T3Quad &Q = S.q[t / 48];
+            int r = t % 48, i = r / 12, c = r % 12;
+            double s = 0.0;
+            for (int l = 0; l < 4; l++) s += Q.Zi[i * 4 + l] * Q.D[l][c];
+            Q.ZiD[i][c] = s;
+        }
+    __syncthreads();
+    // K21 in quad order (the reference adds the three quads in sequence)
+    if (plate)
+        for (int t = lane; t < 441; t += S105_WAVE) {
+            int i = t / 21, j = t % 21, ni = i / 3, nj = j / 3;
+            double v = 0.0;
+            for (int q = 0; q < 3; q++) {
+                int pa = t3_pos(q, ni), pb = t3_pos(q, nj);
+                if (pa >= 0 && pb >= 0) v += t3_quad_plate_entry(S.q[q], P, 3 * pa + i % 3, 3 * pb + j % 3, has_shear);
+            }
+            S.K21[t] = v;
+        }
+    __syncthreads();
+    if (plate)
+        for (int t = lane; t < 252; t += S105_WAVE) {
+            int i = t / 21, j = t % 21;
+            S.M[t] = j < 12 ? S.K21[(9 + i) * 21 + 9 + j] : S.K21[(9 + i) * 21 + (j - 12)];
+        }
+    __syncthreads();
+    // Gaussian elimination with partial pivoting (first largest |pivot|)
+    for (int k = 0; k < 12; k++) {
+        int piv = k;
+        if (plate) {
+            double best = fabs(S.M[k * 21 + k]);
+            for (int i = k + 1; i < 12; i++) {
+                double v = fabs(S.M[i * 21 + k]);
+                if (v > best) { best = v; piv = i; }
+            }
+        }
+        __syncthreads();
+        if (plate && piv != k && lane < 21) {
+            double tmp = S.M[k * 21 + lane];
+            S.M[k * 21 + lane] = S.M[piv * 21 + lane];
+            S.M[piv * 21 + lane] = tmp;
+        }
+        __syncthreads();
+        if (plate) {
+            int nc = 20 - k, n = (11 - k) * nc;
+            for (int t = lane; t < n; t += S105_WAVE) {
+                int i = k + 1 + t / nc, j = k + 1 + t % nc;
+                double l = S.M[i * 21 + k] / S.M[k * 21 + k];
+                S.M[i * 21 + j] -= l * S.M[k * 21 + j];
+            }
+        }
+        __syncthreads();
+    }
+    // back substitution, one right-hand side (corner DOF) per lane
+    if (plate && lane < 9) {
+        int c = 12 + lane;
+        for (int k = 11; k >= 0; k--) {
+            double s = S.M[k * 21 + c];
+            for (int l = k + 1; l < 12; l++) s -= S.M[k * 21 + l] * S.M[l * 21 + c];
+            S.M[k * 21 + c] = s / S.M[k * 21 + k];
+        }
+    }
+    __syncthreads();
+    if (plate)
+        for (int t = lane; t < 81; t += S105_WAVE) {
+            int i = t / 9, j = t % 9;
+            double s = 0.0;
+            for (int l = 0; l < 12; l++) s += S.K21[i * 21 + 9 + l] * S.M[l * 21 + 12 + j];
+            S.Kc[t] = S.K21[i * 21 + j] - s;
+        }
+    __syncthreads();
+    // local 18 x 18 by 3 x 3 sub-blocks (lane < 36), rotated to the global frame
+    if (valid && lane < 36) {
+        int blk = lane / 4, sub = lane % 4;
+        int I = blk / 3, J = blk % 3, si = sub / 2, sj = sub % 2;
+        double ek6 = P.bend ? A.k6rot : 0.0;
+        double alpha = (ek6 / 1e5) * Gref * P.h;
+        double ar = S.area;
+        double loc[9], g[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                int ca = 3 * si + i, cb = 3 * sj + j;
+                double v = 0.0;
+                if (nonzero) {
+                    if (ca < 2 && cb < 2) {
+                        double ba[3] = {ca == 0 ? S.bv[I] : 0.0, ca == 1 ? S.cv[I] : 0.0, ca == 0 ? S.cv[I] : S.bv[I]};
+                        double bb[3] = {cb == 0 ? S.bv[J] : 0.0, cb == 1 ? S.cv[J] : 0.0, cb == 0 ? S.cv[J] : S.bv[J]};
+                        double t[3];
+                        mat3v(P.Cm, bb, t);
+                        v = (ba[0] * t[0] + ba[1] * t[1] + ba[2] * t[2]) * ar;
+                    } else if (plate && ca >= 2 && ca <= 4 && cb >= 2 && cb <= 4) {
+                        v = S.Kc[(3 * I + ca - 2) * 9 + 3 * J + cb - 2];
+                    }
+                    double da = ca == 0 ? 0.5 * S.cv[I] : (ca == 1 ? -0.5 * S.bv[I] : (ca == 5 ? 1.0 / 3.0 : 0.0));
+                    double db = cb == 0 ? 0.5 * S.cv[J] : (cb == 1 ? -0.5 * S.bv[J] : (cb == 5 ? 1.0 / 3.0 : 0.0));
+                    v += alpha * (da * db) * ar;
+                }
+                loc[i * 3 + j] = v;
+            }
+        rot_block(S.R, loc, g);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) S.out[(I * 3 + J) * 36 + (3 * si + i) * 6 + 3 * sj + j] = g[i * 3 + j];
+    }
+    __syncthreads();
+    if (valid)
+        for (int t = lane; t < 324; t += S105_WAVE) eb[(size_t)e * 324 + t] = S.out[t];
+}
+
+// One thread per (element, node-block pair): CST stress and the 3 x 3
+// translational block v I rotated to the global frame; rotations carry none.
+__global__ void t3_kg_kernel(ElemArgs A, const double *u, double *eb) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (int64_t)A.n_t3 * 9) return;
+    int e = (int)(gid / 9), blk = (int)(gid % 9), I = blk / 3, J = blk % 3;
+    double p[9], R[9], lc[6], bv[3], cv[3];
+    for (int k = 0; k < 3; k++) {
+        int nd = A.t3[3 * e + k];
+        for (int i = 0; i < 3; i++) p[3 * k + i] = A.xyz[3 * nd + i];
+    }
+    t3_frame(p, R, lc);
+    const double *pr = A.t3prop + 6 * e;
+    double E = pr[0], nu = pr[1], h = pr[2];
+    double ar = t3_derivs(lc, bv, cv);
+    double v = 0.0;
+    if (ar >= 1e-12 && h >= 1e-30) {
+        double exx = 0.0, eyy = 0.0, gxy = 0.0;
+        for (int k = 0; k < 3; k++) {
+            const double *ug = u + 6 * (size_t)A.t3[3 * e + k];
+            double ul = R[0] * ug[0] + R[1] * ug[1] + R[2] * ug[2];
+            double vl = R[3] * ug[0] + R[4] * ug[1] + R[5] * ug[2];
+            exx += bv[k] * ul;
+            eyy += cv[k] * vl;
+            gxy += cv[k] * ul + bv[k] * vl;
+        }
+        double c = E / (1 - nu * nu);
+        double D[9] = {c, c * nu, 0.0, c * nu, c, 0.0, 0.0, 0.0, c * ((1 - nu) / 2)};
+        double sig[3];
+        for (int i = 0; i < 3; i++) sig[i] = ((D[3 * i] * exx + D[3 * i + 1] * eyy + D[3 * i + 2] * gxy) * h) / h;
+        v = h * ar * (sig[0] * bv[I] * bv[J] + sig[1] * cv[I] * cv[J] + sig[2] * (bv[I] * cv[J] + cv[I] * bv[J]));
+    }
+    double loc[9] = {v, 0.0, 0.0, 0.0, v, 0.0, 0.0, 0.0, v}, g[9];
+    rot_block(R, loc, g);
+    double *dst = eb + (size_t)gid * 36;
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) dst[i * 6 + j] = (i < 3 && j < 3) ? g[i * 3 + j] : 0.0;
+}
+
+// ---------------------------------------------------------------------------
 // Deterministic block gather: K[blk][t] = sum of contributing element blocks
 // in element order (no atomics: the summation order is fixed run to run).
 // ---------------------------------------------------------------------------
@@ -1167,13 +1464,22 @@ void launch_q4_kg(const ElemArgs &a, const double *u, double *eb, hipStream_t s)
     if (a.n_q4 == 0) return;
     q4_kg_kernel<<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, eb);
 }
+void launch_t3_ke(const ElemArgs &a, double *eb, hipStream_t s) {
+    if (a.n_t3 == 0) return;
+    t3_ke_kernel<<<(a.n_t3 + EPB - 1) / EPB, 256, 0, s>>>(a, eb + (size_t)a.n_q4 * 576);
+}
+void launch_t3_kg(const ElemArgs &a, const double *u, double *eb, hipStream_t s) {
+    if (a.n_t3 == 0) return;
+    int64_t n = (int64_t)a.n_t3 * 9;
+    t3_kg_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, u, eb + (size_t)a.n_q4 * 576);
+}
 void launch_bar_ke(const ElemArgs &a, double *eb, hipStream_t s) {
     if (a.n_bar == 0) return;
-    bar_ke_kernel<<<(a.n_bar + 127) / 128, 128, 0, s>>>(a, eb + (size_t)a.n_q4 * 576);
+    bar_ke_kernel<<<(a.n_bar + 127) / 128, 128, 0, s>>>(a, eb + (size_t)a.n_q4 * 576 + (size_t)a.n_t3 * 324);
 }
 void launch_bar_kg(const ElemArgs &a, const double *u, double *eb, hipStream_t s) {
     if (a.n_bar == 0) return;
-    bar_kg_kernel<<<(a.n_bar + 127) / 128, 128, 0, s>>>(a, u, eb + (size_t)a.n_q4 * 576);
+    bar_kg_kernel<<<(a.n_bar + 127) / 128, 128, 0, s>>>(a, u, eb + (size_t)a.n_q4 * 576 + (size_t)a.n_t3 * 324);
 }
 void launch_block_gather(int64_t n_blocks, const int32_t *cptr, const int32_t *cidx, const double *eb,
                          double *vals, hipStream_t s) {

@@ -21,13 +21,16 @@ int64_t BlockPattern::find(int i, int j) const {
     return (int64_t)(it - col.begin());
 }
 
-void build_block_pattern(int n_nodes, int n_q4, const int32_t *q4, int n_bar, const int32_t *bar,
-                         BlockPattern &P) {
+void build_block_pattern(int n_nodes, int n_q4, const int32_t *q4, int n_t3, const int32_t *t3, int n_bar,
+                         const int32_t *bar, BlockPattern &P) {
     P.n_nodes = n_nodes;
     std::vector<std::vector<int32_t>> nb(n_nodes);
     for (int e = 0; e < n_q4; e++)
         for (int a = 0; a < 4; a++)
             for (int b = 0; b < 4; b++) nb[q4[4 * e + a]].push_back(q4[4 * e + b]);
+    for (int e = 0; e < n_t3; e++)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) nb[t3[3 * e + a]].push_back(t3[3 * e + b]);
     for (int e = 0; e < n_bar; e++)
         for (int a = 0; a < 2; a++)
             for (int b = 0; b < 2; b++) nb[bar[2 * e + a]].push_back(bar[2 * e + b]);
@@ -43,12 +46,19 @@ void build_block_pattern(int n_nodes, int n_q4, const int32_t *q4, int n_bar, co
     for (int i = 0; i < n_nodes; i++) std::copy(nb[i].begin(), nb[i].end(), P.col.begin() + P.row_ptr[i]);
     int64_t nbk = P.n_blocks();
     std::vector<int32_t> cnt(nbk + 1, 0);
-    std::vector<int64_t> q4blk((size_t)n_q4 * 16), barblk((size_t)n_bar * 4);
+    std::vector<int64_t> q4blk((size_t)n_q4 * 16), t3blk((size_t)n_t3 * 9), barblk((size_t)n_bar * 4);
     for (int e = 0; e < n_q4; e++)
         for (int a = 0; a < 4; a++)
             for (int b = 0; b < 4; b++) {
                 int64_t k = P.find(q4[4 * e + a], q4[4 * e + b]);
                 q4blk[(size_t)e * 16 + a * 4 + b] = k;
+                cnt[k + 1]++;
+            }
+    for (int e = 0; e < n_t3; e++)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                int64_t k = P.find(t3[3 * e + a], t3[3 * e + b]);
+                t3blk[(size_t)e * 9 + a * 3 + b] = k;
                 cnt[k + 1]++;
             }
     for (int e = 0; e < n_bar; e++)
@@ -63,8 +73,10 @@ void build_block_pattern(int n_nodes, int n_q4, const int32_t *q4, int n_bar, co
     P.cidx.resize(cnt[nbk]);
     std::vector<int32_t> cur(cnt.begin(), cnt.end() - 1);
     for (int64_t t = 0; t < (int64_t)q4blk.size(); t++) P.cidx[cur[q4blk[t]]++] = (int32_t)t;
+    for (int64_t t = 0; t < (int64_t)t3blk.size(); t++)
+        P.cidx[cur[t3blk[t]]++] = (int32_t)((int64_t)n_q4 * 16 + t);
     for (int64_t t = 0; t < (int64_t)barblk.size(); t++)
-        P.cidx[cur[barblk[t]]++] = (int32_t)((int64_t)n_q4 * 16 + t);
+        P.cidx[cur[barblk[t]]++] = (int32_t)((int64_t)n_q4 * 16 + (int64_t)n_t3 * 9 + t);
 }
 
 namespace {

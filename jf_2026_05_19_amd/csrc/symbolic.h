@@ -15,9 +15,9 @@ struct BlockPattern {
 };
 
 // Build the pattern and the deterministic gather map: CQUAD4 blocks first
-// (element order, a-major), then CBAR blocks.
-void build_block_pattern(int n_nodes, int n_q4, const int32_t *q4, int n_bar, const int32_t *bar,
-                         BlockPattern &P);
+// (element order, a-major), then CTRIA3 blocks, then CBAR blocks.
+void build_block_pattern(int n_nodes, int n_q4, const int32_t *q4, int n_t3, const int32_t *t3, int n_bar,
+                         const int32_t *bar, BlockPattern &P);
 
 struct FrontInfo {
     int ns = 0;          // pivot rows (contiguous elimination indices piv0..piv0+ns-1)

@@ -6,6 +6,12 @@ parser (``JFEM/src/parsing/card_processor.jl``).
 * ``plate_deck``  -- flat square plate, CQUAD4 mesh, simply supported on all
   edges, uniform compressive edge load on x = a reacted at x = 0, consistent
   nodal lumping (half load at the corners).
+* ``tri_frac`` (plate and panel) -- a seeded fraction of the quads is split
+  into two CTRIA3 along an alternating diagonal (mixed CQUAD4/CTRIA3 mesh).
+  A region meshed only with CTRIA3 has two zero-energy drilling patterns
+  (theta_z alternating over a 3-colouring of its nodes: the one-point
+  drilling penalty of each triangle sees only the mean of its three
+  theta_z); ``spc_drill`` constrains theta_z everywhere for such decks.
 * ``panel_deck``  -- flat panel with CBAR stringers (PBARL BAR) on every k-th
   longitudinal mesh line, clamped loaded edges, axial compression.
 * ``variant_sweep`` -- seeded design variants of the plate (mesh size,
@@ -46,8 +52,28 @@ def _header(nd, sol=105):
     ]
 
 
+def _shells(L, nx, ny, gid, tri_frac, tri_seed):
+    """CQUAD4 per mesh cell, or two CTRIA3 for a seeded fraction of cells."""
+    rng = np.random.default_rng(tri_seed)
+    split = rng.random(nx * ny) < tri_frac
+    eid = 1
+    for j in range(ny):
+        for i in range(nx):
+            c = (gid(i, j), gid(i + 1, j), gid(i + 1, j + 1), gid(i, j + 1))
+            if split[j * nx + i]:
+                if (i + j) % 2 == 0:
+                    L.append(_card("CTRIA3", eid, 1, c[0], c[1], c[2]))
+                    L.append(_card("CTRIA3", 5000000 + eid, 1, c[0], c[2], c[3]))
+                else:
+                    L.append(_card("CTRIA3", eid, 1, c[0], c[1], c[3]))
+                    L.append(_card("CTRIA3", 5000000 + eid, 1, c[1], c[2], c[3]))
+            else:
+                L.append(_card("CQUAD4", eid, 1, *c))
+            eid += 1
+
+
 def plate_deck(nx=20, ny=20, a=500.0, b=500.0, t=2.0, E=70e3, nu=0.33, line_load=1.0, nd=5,
-               k6rot=None):
+               k6rot=None, tri_frac=0.0, tri_seed=20260520, spc_drill=False):
     """Simply supported plate under uniaxial edge compression (N/mm)."""
     L = _header(nd)
     if k6rot is not None:
@@ -58,11 +84,7 @@ def plate_deck(nx=20, ny=20, a=500.0, b=500.0, t=2.0, E=70e3, nu=0.33, line_load
     for j in range(ny + 1):
         for i in range(nx + 1):
             L.append(_card("GRID", gid(i, j), "", a * i / nx, b * j / ny, 0.0))
-    eid = 1
-    for j in range(ny):
-        for i in range(nx):
-            L.append(_card("CQUAD4", eid, 1, gid(i, j), gid(i + 1, j), gid(i + 1, j + 1), gid(i, j + 1)))
-            eid += 1
+    _shells(L, nx, ny, gid, tri_frac, tri_seed)
     edge = sorted({gid(i, j) for j in range(ny + 1) for i in range(nx + 1)
                    if i in (0, nx) or j in (0, ny)})
     for g in edge:
@@ -70,6 +92,10 @@ def plate_deck(nx=20, ny=20, a=500.0, b=500.0, t=2.0, E=70e3, nu=0.33, line_load
     for j in range(ny + 1):
         L.append(_card("SPC1", 1, "1", gid(0, j)))
     L.append(_card("SPC1", 1, "2", gid(0, 0)))
+    if spc_drill:
+        for j in range(ny + 1):
+            for i in range(nx + 1):
+                L.append(_card("SPC1", 1, "6", gid(i, j)))
     dy = b / ny
     for j in range(ny + 1):
         w = 0.5 if j in (0, ny) else 1.0
@@ -79,7 +105,7 @@ def plate_deck(nx=20, ny=20, a=500.0, b=500.0, t=2.0, E=70e3, nu=0.33, line_load
 
 
 def panel_deck(nx=200, ny=100, a=1000.0, b=500.0, t=1.5, E=70e3, nu=0.33, stringer_every=10,
-               bar_dims=(3.0, 20.0), line_load=1.0, nd=10):
+               bar_dims=(3.0, 20.0), line_load=1.0, nd=10, tri_frac=0.0, tri_seed=20260520):
     """Stringer-stiffened panel: clamped loaded edges, axial compression."""
     L = _header(nd)
     L.append(_card("MAT1", 1, float(E), "", float(nu)))
@@ -89,11 +115,7 @@ def panel_deck(nx=200, ny=100, a=1000.0, b=500.0, t=1.5, E=70e3, nu=0.33, string
     for j in range(ny + 1):
         for i in range(nx + 1):
             L.append(_card("GRID", gid(i, j), "", a * i / nx, b * j / ny, 0.0))
-    eid = 1
-    for j in range(ny):
-        for i in range(nx):
-            L.append(_card("CQUAD4", eid, 1, gid(i, j), gid(i + 1, j), gid(i + 1, j + 1), gid(i, j + 1)))
-            eid += 1
+    _shells(L, nx, ny, gid, tri_frac, tri_seed)
     bid = 1000001
     for j in range(0, ny + 1, stringer_every):
         for i in range(nx):

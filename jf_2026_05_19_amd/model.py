@@ -7,8 +7,10 @@ Here that derivation happens once, on the host, into contiguous arrays that
 the C-ABI takes by pointer:
 
 * node order = ascending GRID id (``assembly.jl:1958-1961``); DOF ``6*rank+c``;
-* per-CQUAD4 property row ``[E, nu, T, BEND_RATIO, TS_T, G_shear]``
-  (``assembly.jl:2393-2476``: MID3 falls back to MID1 for the shear modulus);
+* per-CQUAD4 and per-CTRIA3 property row ``[E, nu, T, BEND_RATIO, TS_T,
+  G_shear]`` (``assembly.jl:2393-2476``: MID3 falls back to MID1 for the
+  shear modulus; a CTRIA3 uses G_shear as its drilling reference modulus,
+  ``assembly.jl:2454-2460, 2515``);
 * per-CBAR property row ``[A, I1, I2, J, K1, K2, E, G]`` and the resolved
   orientation vector (G0 - GA, or V; ``helpers.jl:8-18``);
 * SPC1/SPC set of the static subcase (``boundary_conditions.jl:634-677``);
@@ -40,6 +42,10 @@ class Sol105Arrays:
     q4_eid: np.ndarray            # (n_q4,) int64
     q4_pid: np.ndarray            # (n_q4,) int64
     q4prop: np.ndarray            # (n_q4, 6) float64  E, nu, T, BR, TS/T, G_shear
+    t3: np.ndarray                # (n_t3, 3) int32 node ranks
+    t3_eid: np.ndarray            # (n_t3,) int64
+    t3_pid: np.ndarray            # (n_t3,) int64
+    t3prop: np.ndarray            # (n_t3, 6) float64, same columns as q4prop
     bar: np.ndarray               # (n_bar, 2) int32
     bar_eid: np.ndarray
     bar_vref: np.ndarray          # (n_bar, 3)
@@ -69,6 +75,14 @@ class Sol105Arrays:
         return int(self.q4.shape[0])
 
     @property
+    def n_t3(self):
+        return int(self.t3.shape[0])
+
+    @property
+    def n_shell(self):
+        return self.n_q4 + self.n_t3
+
+    @property
     def n_bar(self):
         return int(self.bar.shape[0])
 
@@ -78,6 +92,22 @@ class Sol105Arrays:
         out = copy.copy(self)
         out.q4prop = self.q4prop.copy()
         out.q4prop[:, 2] = t
+        return out
+
+    def shell_thickness(self) -> np.ndarray:
+        """Thickness of every shell, CQUAD4 first, then CTRIA3."""
+        return np.concatenate([self.q4prop[:, 2], self.t3prop[:, 2]])
+
+    def with_shell_thickness(self, t: np.ndarray) -> "Sol105Arrays":
+        """Copy with the thickness of every shell replaced (CQUAD4 first,
+        then CTRIA3)."""
+        import copy
+        t = np.asarray(t, dtype=float)
+        out = copy.copy(self)
+        out.q4prop = self.q4prop.copy()
+        out.t3prop = self.t3prop.copy()
+        out.q4prop[:, 2] = t[:self.n_q4]
+        out.t3prop[:, 2] = t[self.n_q4:]
         return out
 
 
@@ -134,11 +164,12 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
     mats = model["MATs"]
     pshells = model["PSHELLs"]
     q4, q4eid, q4pid, q4prop = [], [], [], []
+    t3, t3eid, t3pid, t3prop = [], [], [], []
     for key in sorted(model["CSHELLs"], key=lambda k: int(k)):
         el = model["CSHELLs"][key]
         nodes = el["NODES"]
-        if len(nodes) != 4:
-            raise UnsupportedModel(f"element {el['ID']}: CTRIA3 is not supported by this build")
+        if len(nodes) not in (3, 4):
+            raise UnsupportedModel(f"element {el['ID']}: shells need 3 or 4 nodes")
         prop = pshells.get(str(el["PID"]))
         if prop is None:
             continue
@@ -153,6 +184,14 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
         if Gs <= 0.0:
             E_ = float(smat.get("E", 0.0)); nu_ = float(smat.get("NU", 0.0))
             Gs = E_ / (2 * (1 + nu_)) if E_ > 0 else 0.0
+        row = [float(mat["E"]), float(mat["NU"]), float(prop["T"]),
+               float(prop.get("BEND_RATIO", 1.0)), float(prop.get("TS_T", 5.0 / 6.0)), Gs]
+        if len(nodes) == 3:
+            t3.append([rank[v] for v in nodes])
+            t3eid.append(int(el["ID"]))
+            t3pid.append(int(el["PID"]))
+            t3prop.append(row)
+            continue
         p = xyz[[rank[v] for v in nodes]]
         dev, L = _flat_dev(p)
         if dev >= 1e-6 * max(L, 1e-12):
@@ -160,8 +199,7 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
         q4.append([rank[v] for v in nodes])
         q4eid.append(int(el["ID"]))
         q4pid.append(int(el["PID"]))
-        q4prop.append([float(mat["E"]), float(mat["NU"]), float(prop["T"]),
-                       float(prop.get("BEND_RATIO", 1.0)), float(prop.get("TS_T", 5.0 / 6.0)), Gs])
+        q4prop.append(row)
     pbarls = model.get("PBARLs", {})
     bars, bareid, vref, barprop = [], [], [], []
     for group in ("CBARs", "CBEAMs"):
@@ -246,17 +284,22 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
     if isinstance(snorm, (int, float)) and snorm > 0:
         raise UnsupportedModel("PARAM,SNORM > 0 is not supported")
     q4a = np.asarray(q4, dtype=np.int32).reshape(-1, 4)
+    t3a = np.asarray(t3, dtype=np.int32).reshape(-1, 3)
     bara = np.asarray(bars, dtype=np.int32).reshape(-1, 2)
     has_shell = np.zeros(n, dtype=np.uint8)
     has_bar = np.zeros(n, dtype=np.uint8)
     if q4a.size:
         has_shell[q4a.ravel()] = 1
+    if t3a.size:
+        has_shell[t3a.ravel()] = 1
     if bara.size:
         has_bar[bara.ravel()] = 1
     return Sol105Arrays(
         node_ids=np.asarray(ids, dtype=np.int64), xyz=xyz,
         q4=q4a, q4_eid=np.asarray(q4eid, dtype=np.int64), q4_pid=np.asarray(q4pid, dtype=np.int64),
         q4prop=np.asarray(q4prop, dtype=float).reshape(-1, 6),
+        t3=t3a, t3_eid=np.asarray(t3eid, dtype=np.int64), t3_pid=np.asarray(t3pid, dtype=np.int64),
+        t3prop=np.asarray(t3prop, dtype=float).reshape(-1, 6),
         bar=bara, bar_eid=np.asarray(bareid, dtype=np.int64),
         bar_vref=np.asarray(vref, dtype=float).reshape(-1, 3),
         barprop=np.asarray(barprop, dtype=float).reshape(-1, 8),

@@ -71,6 +71,13 @@ def lib():
         L.or_q4_ke_global.argtypes = [_dp, _dp, ctypes.c_double, _dp, _dp, _dp]
         L.or_q4_frame.argtypes = [_dp, _dp, _dp, _dp]
         L.or_bar_ke_global.argtypes = [_dp, _dp, _dp, _dp, _dp]
+        L.or_t3_frame.argtypes = [_dp, _dp, _dp]
+        L.or_t3_plate_macro.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        _dp, _dp]
+        L.or_t3_ke_local.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, _dp]
+        L.or_t3_ke_global.argtypes = [_dp, _dp, ctypes.c_double, _dp, _dp, _dp]
+        L.or_t3_kg_global.argtypes = [_dp, _dp, _dp, _dp, _dp]
         L.or_frame3d_ke.argtypes = [ctypes.c_double] * 9 + [_dp]
         L.or_frame3d_kg.argtypes = [ctypes.c_double, ctypes.c_double, _dp]
         L.or_q4_kg_global.argtypes = [_dp, _dp, _dp, _dp]
@@ -79,11 +86,11 @@ def lib():
         L.or_q4_condensation_map.argtypes = [_dp, _dp, _dp]
         L.or_bar_kg_global.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.or_assemble_K.restype = ctypes.c_int64
-        L.or_assemble_K.argtypes = [ctypes.c_int, _ip, _dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp,
-                                    _lp, _lp, _dp]
+        L.or_assemble_K.argtypes = [ctypes.c_int, _ip, _dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp,
+                                    ctypes.c_int, _ip, _dp, _dp, _lp, _lp, _dp]
         L.or_assemble_KG.restype = ctypes.c_int64
-        L.or_assemble_KG.argtypes = [ctypes.c_int, _ip, _dp, _dp, ctypes.c_int, _ip, _dp, _dp, _dp,
-                                     _lp, _lp, _dp]
+        L.or_assemble_KG.argtypes = [ctypes.c_int, _ip, _dp, _dp, ctypes.c_int, _ip, _dp, ctypes.c_int, _ip,
+                                     _dp, _dp, _dp, _lp, _lp, _dp]
         _lib = L
     return _lib
 
@@ -133,6 +140,40 @@ def q4_membrane_force_field(lc, u, Cm):
     return out.reshape(4, 3)
 
 
+def t3_frame(p):
+    R = np.zeros(9); lc = np.zeros(6)
+    lib().or_t3_frame(_p(_c(p, float)), _p(R), _p(lc))
+    return R.reshape(3, 3), lc.reshape(3, 2)
+
+
+def t3_plate_macro(lc, Cm, Cb, Cs, h, Gref, k6rot=100.0):
+    """(Kcond 9x9, assembled 21x21) of the CTRIA3 macro plate block."""
+    Kc = np.zeros((9, 9)); K21 = np.zeros((21, 21))
+    lib().or_t3_plate_macro(_p(_c(lc, float)), _p(_c(Cm, float)), _p(_c(Cb, float)), _p(_c(Cs, float)),
+                            float(h), float(Gref), float(k6rot), _p(Kc), _p(K21))
+    return Kc, K21
+
+
+def t3_ke_local(lc, Cm, Cb, Cs, h, Gref, br=1.0, k6rot=100.0):
+    out = np.zeros((18, 18))
+    lib().or_t3_ke_local(_p(_c(lc, float)), _p(_c(Cm, float)), _p(_c(Cb, float)), _p(_c(Cs, float)),
+                         float(h), float(Gref), float(br), float(k6rot), _p(out))
+    return out
+
+
+def t3_ke_global(p, prop, k6rot=100.0):
+    out = np.zeros((18, 18)); R = np.zeros(9); lc = np.zeros(6)
+    lib().or_t3_ke_global(_p(_c(p, float)), _p(_c(prop, float)), float(k6rot), _p(out), _p(R), _p(lc))
+    return out, R.reshape(3, 3), lc.reshape(3, 2)
+
+
+def t3_kg_global(p, prop, ug):
+    """(KG 18x18 global, membrane stress [sxx, syy, sxy] local)."""
+    out = np.zeros((18, 18)); sig = np.zeros(3)
+    lib().or_t3_kg_global(_p(_c(p, float)), _p(_c(prop, float)), _p(_c(ug, float)), _p(out), _p(sig))
+    return out, sig
+
+
 def frame3d_ke(L, A, Iy, Iz, J, E, G, Asy=np.inf, Asz=np.inf):
     out = np.zeros((12, 12))
     lib().or_frame3d_ke(L, A, Iy, Iz, J, E, G, Asy, Asz, _p(out))
@@ -156,22 +197,23 @@ def bar_ke_global(p1, p2, vref, bp):
 # --------------------------------------------------------------------------
 
 def assemble_K(arr):
-    n = 576 * arr.n_q4 + 144 * arr.n_bar
+    n = 576 * arr.n_q4 + 324 * arr.n_t3 + 144 * arr.n_bar
     I = np.zeros(n, np.int64); J = np.zeros(n, np.int64); V = np.zeros(n)
-    q4 = _c(arr.q4, np.int32); bar = _c(arr.bar, np.int32)
+    q4 = _c(arr.q4, np.int32); t3 = _c(arr.t3, np.int32); bar = _c(arr.bar, np.int32)
     cnt = lib().or_assemble_K(arr.n_q4, _p(q4, _ip), _p(_c(arr.xyz, float)), _p(_c(arr.q4prop, float)),
-                              float(arr.k6rot), arr.n_bar, _p(bar, _ip), _p(_c(arr.bar_vref, float)),
+                              float(arr.k6rot), arr.n_t3, _p(t3, _ip), _p(_c(arr.t3prop, float)),
+                              arr.n_bar, _p(bar, _ip), _p(_c(arr.bar_vref, float)),
                               _p(_c(arr.barprop, float)), _p(I, _lp), _p(J, _lp), _p(V))
     assert cnt == n
     return sp.csr_matrix((V, (I, J)), shape=(arr.ndof, arr.ndof))
 
 
 def assemble_KG(arr, u):
-    n = 576 * arr.n_q4 + 144 * arr.n_bar
+    n = 576 * arr.n_q4 + 324 * arr.n_t3 + 144 * arr.n_bar
     I = np.zeros(n, np.int64); J = np.zeros(n, np.int64); V = np.zeros(n)
-    q4 = _c(arr.q4, np.int32); bar = _c(arr.bar, np.int32)
+    q4 = _c(arr.q4, np.int32); t3 = _c(arr.t3, np.int32); bar = _c(arr.bar, np.int32)
     cnt = lib().or_assemble_KG(arr.n_q4, _p(q4, _ip), _p(_c(arr.xyz, float)), _p(_c(arr.q4prop, float)),
-                               arr.n_bar, _p(bar, _ip), _p(_c(arr.bar_vref, float)), _p(_c(arr.barprop, float)),
+                               arr.n_t3, _p(t3, _ip), _p(_c(arr.t3prop, float)), arr.n_bar, _p(bar, _ip), _p(_c(arr.bar_vref, float)), _p(_c(arr.barprop, float)),
                                _p(_c(u, float)), _p(I, _lp), _p(J, _lp), _p(V))
     assert cnt == n
     return sp.csr_matrix((V, (I, J)), shape=(arr.ndof, arr.ndof))
@@ -254,7 +296,7 @@ def buckling(arr, K, KG, free, krylov_tol=1e-13):
     pos = np.flatnonzero(lam > 1e-10)
     order = pos[np.argsort(lam[pos], kind="stable")]
     dropped = []
-    if arr.n_q4 >= 100 and len(order):
+    if arr.n_shell >= 100 and len(order):
         limit = min(len(order), max(nd + 16, nd))
         surv = []
         full = np.zeros(arr.ndof)
@@ -262,7 +304,7 @@ def buckling(arr, K, KG, free, krylov_tol=1e-13):
             full[:] = 0.0
             full[free] = V[:, k]
             t2 = (full.reshape(-1, 6)[:, :3] ** 2).sum(axis=1)
-            e = t2[arr.q4].mean(axis=1)
+            e = np.concatenate([t2[arr.q4].sum(axis=1) / 4, t2[arr.t3].sum(axis=1) / 3])
             tot = e.sum()
             if tot > 0 and e.max() / tot > 0.10:
                 dropped.append(int(k))

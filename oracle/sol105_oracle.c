@@ -11,6 +11,14 @@
  *        Wilson membrane + bending incompatible modes condensed)
  *   - MacNeal residual-bending-flexibility shear  JFEM/src/FEMKernels.jl:4715-4978
  *   - shell T'KT transform and COO order       JFEM/src/solver/assembly.jl:3808-3851
+ *   - CTRIA3 frame                             JFEM/src/solver/helpers.jl:1195-1204
+ *   - CTRIA3 stiffness: CST membrane, 3-quad macro plate block with the
+ *     12 interior DOFs condensed, centroid drilling
+ *                                              JFEM/src/FEMKernels.jl:7129-7209, 7315-7385,
+ *                                              JFEM/src/solver/assembly.jl:3862-3955
+ *   - CTRIA3 geometric stiffness (CST stress; principal-transverse falls
+ *     through to the all-translations branch)  JFEM/src/FEMKernels.jl:7391-7432, 7989-8103,
+ *                                              JFEM/src/solver/assembly.jl:6020-6160
  *   - CBAR Timoshenko 12x12 + frame            JFEM/src/FEMKernels.jl:140-213,
  *                                              JFEM/src/solver/assembly.jl:3960-4020
  *   - Q4 membrane force field (Wilson modes)   JFEM/src/FEMKernels.jl:6341-6539
@@ -400,6 +408,207 @@ void or_q4_ke_global(const double p[12], const double prop[6], double k6rot, dou
 }
 
 /* ------------------------------------------------------------------------ */
+/* CTRIA3                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* CTRIA3 frame: v1 = e12, v3 = normalize(v1 x (p3 - p1)), v2 = v3 x v1
+ * (helpers.jl:1195-1204); local corner coordinates about the centroid
+ * (assembly.jl:3927-3930). */
+void or_t3_frame(const double p[9], double R[9], double lc[6]) {
+    double v1[3], v3[3], v2[3], d13[3];
+    for (int i = 0; i < 3; i++) { v1[i] = p[3 + i] - p[i]; d13[i] = p[6 + i] - p[i]; }
+    v3_normalize(v1);
+    v3_cross(v1, d13, v3);
+    v3_normalize(v3);
+    v3_cross(v3, v1, v2);
+    for (int i = 0; i < 3; i++) { R[i] = v1[i]; R[3 + i] = v2[i]; R[6 + i] = v3[i]; }
+    double c[3];
+    for (int i = 0; i < 3; i++) c[i] = (p[i] + p[3 + i] + p[6 + i]) / 3.0;
+    for (int k = 0; k < 3; k++) {
+        double d[3] = {p[3 * k] - c[0], p[3 * k + 1] - c[1], p[3 * k + 2] - c[2]};
+        lc[2 * k] = v3_dot(d, v1);
+        lc[2 * k + 1] = v3_dot(d, v2);
+    }
+}
+
+/* Gaussian elimination with partial pivoting (Golub & Van Loan 3.4): solves
+ * A X = B in place, A n x n, B n x nrhs, row-major.  Returns 0 on success. */
+static int lu_solve_n(int n, double *A, int nrhs, double *B) {
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        for (int i = k + 1; i < n; i++) if (fabs(A[i * n + k]) > fabs(A[p * n + k])) p = i;
+        if (A[p * n + k] == 0.0) return -1;
+        if (p != k) {
+            for (int j = 0; j < n; j++) { double t = A[k * n + j]; A[k * n + j] = A[p * n + j]; A[p * n + j] = t; }
+            for (int j = 0; j < nrhs; j++) { double t = B[k * nrhs + j]; B[k * nrhs + j] = B[p * nrhs + j]; B[p * nrhs + j] = t; }
+        }
+        for (int i = k + 1; i < n; i++) {
+            double l = A[i * n + k] / A[k * n + k];
+            A[i * n + k] = l;
+            for (int j = k + 1; j < n; j++) A[i * n + j] -= l * A[k * n + j];
+            for (int j = 0; j < nrhs; j++) B[i * nrhs + j] -= l * B[k * nrhs + j];
+        }
+    }
+    for (int k = n - 1; k >= 0; k--)
+        for (int j = 0; j < nrhs; j++) {
+            double s = B[k * nrhs + j];
+            for (int l = k + 1; l < n; l++) s -= A[k * n + l] * B[l * nrhs + j];
+            B[k * nrhs + j] = s / A[k * n + k];
+        }
+    return 0;
+}
+
+/* The condensed plate block of the CTRIA3 macro element.  FEMKernels.jl:
+ * 7129-7209.  Seven points (corners, mid-sides 12/23/31, centroid) carry
+ * (w, thx, thy); three virtual CQUAD4s (1,4,7,6), (2,5,7,4), (3,6,7,5)
+ * contribute the plate rows/columns of the MacNeal stiffness with the
+ * kernel's default flags (bending incompatible modes NOT condensed); the 12
+ * interior DOFs are condensed: Kcond = Kee - Kei Kii^-1 Kie.  Also returns
+ * the 21x21 assembled matrix when K21 is non-NULL. */
+void or_t3_plate_macro(const double lc[6], const double Cm[9], const double Cb[9], const double Cs[4],
+                       double h, double Gref, double k6rot, double *Kcond /*9x9*/, double *K21 /*21x21*/) {
+    static const int quads[3][4] = {{0, 3, 6, 5}, {1, 4, 6, 3}, {2, 5, 6, 4}};
+    static const int pd[12] = {2, 3, 4, 8, 9, 10, 14, 15, 16, 20, 21, 22};
+    double pts[7][2];
+    for (int k = 0; k < 3; k++) { pts[k][0] = lc[2 * k]; pts[k][1] = lc[2 * k + 1]; }
+    for (int d = 0; d < 2; d++) {
+        pts[3][d] = (lc[d] + lc[2 + d]) / 2;
+        pts[4][d] = (lc[2 + d] + lc[4 + d]) / 2;
+        pts[5][d] = (lc[4 + d] + lc[d]) / 2;
+        pts[6][d] = (lc[d] + lc[2 + d] + lc[4 + d]) / 3;
+    }
+    double K[21 * 21];
+    memset(K, 0, sizeof(K));
+    for (int q = 0; q < 3; q++) {
+        double qc[8];
+        for (int k = 0; k < 4; k++) { qc[2 * k] = pts[quads[q][k]][0]; qc[2 * k + 1] = pts[quads[q][k]][1]; }
+        double Kq[576];
+        or_q4_ke_local(qc, Cm, Cb, Cs, h, Gref, k6rot, 1.0, 1, 0, Kq);
+        for (int a = 0; a < 12; a++) for (int b = 0; b < 12; b++) {
+            int ga = 3 * quads[q][a / 3] + a % 3, gb = 3 * quads[q][b / 3] + b % 3;
+            K[ga * 21 + gb] += Kq[pd[a] * 24 + pd[b]];
+        }
+    }
+    if (K21) memcpy(K21, K, sizeof(K));
+    double Kii[144], X[12 * 9];
+    for (int i = 0; i < 12; i++) {
+        for (int j = 0; j < 12; j++) Kii[i * 12 + j] = K[(9 + i) * 21 + 9 + j];
+        for (int j = 0; j < 9; j++) X[i * 9 + j] = K[(9 + i) * 21 + j];
+    }
+    lu_solve_n(12, Kii, 9, X);
+    for (int i = 0; i < 9; i++) for (int j = 0; j < 9; j++) {
+        double s = 0; for (int l = 0; l < 12; l++) s += K[i * 21 + 9 + l] * X[l * 9 + j];
+        Kcond[i * 9 + j] = K[i * 21 + j] - s;
+    }
+}
+
+/* CTRIA3 stiffness in element-local coordinates.  FEMKernels.jl:7315-7385:
+ * CST membrane Bm' Dm Bm A on (u, v), the macro plate block on (w, thx, thy),
+ * Hughes-Brezzi drilling alpha = (k6rot/1e5) G_ref h at the centroid. */
+void or_t3_ke_local(const double lc[6], const double Cm[9], const double Cb[9], const double Cs[4], double h,
+                    double Gref, double br, double k6rot, double *Ke /*18x18*/) {
+    memset(Ke, 0, sizeof(double) * 324);
+    const double *x0 = lc;
+    double x[3] = {x0[0], x0[2], x0[4]}, y[3] = {x0[1], x0[3], x0[5]};
+    double A2 = x[0] * (y[1] - y[2]) + x[1] * (y[2] - y[0]) + x[2] * (y[0] - y[1]);
+    double A = 0.5 * fabs(A2);
+    if (A < 1e-12) return;
+    double bv[3] = {(y[1] - y[2]) / (2 * A), (y[2] - y[0]) / (2 * A), (y[0] - y[1]) / (2 * A)};
+    double cv[3] = {(x[2] - x[1]) / (2 * A), (x[0] - x[2]) / (2 * A), (x[1] - x[0]) / (2 * A)};
+    double Bm[3][6];
+    memset(Bm, 0, sizeof(Bm));
+    for (int i = 0; i < 3; i++) {
+        Bm[0][2 * i] = bv[i]; Bm[1][2 * i + 1] = cv[i]; Bm[2][2 * i] = cv[i]; Bm[2][2 * i + 1] = bv[i];
+    }
+    static const int mi[6] = {0, 1, 6, 7, 12, 13};
+    for (int a = 0; a < 6; a++) for (int b = 0; b < 6; b++) {
+        double s = 0;
+        for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) s += Bm[i][a] * Cm[i * 3 + j] * Bm[j][b];
+        Ke[mi[a] * 18 + mi[b]] = s * A;
+    }
+    if (br > 1e-12) {
+        double Kc[81];
+        or_t3_plate_macro(lc, Cm, Cb, Cs, h, Gref, k6rot, Kc, 0);
+        static const int bi[9] = {2, 3, 4, 8, 9, 10, 14, 15, 16};
+        for (int a = 0; a < 9; a++) for (int b = 0; b < 9; b++) Ke[bi[a] * 18 + bi[b]] += Kc[a * 9 + b];
+    }
+    double alpha = (k6rot / 1e5) * Gref * h;
+    double Bd[18];
+    memset(Bd, 0, sizeof(Bd));
+    for (int i = 0; i < 3; i++) { Bd[6 * i] = 0.5 * cv[i]; Bd[6 * i + 1] = -0.5 * bv[i]; Bd[6 * i + 5] = 1.0 / 3.0; }
+    for (int a = 0; a < 18; a++) for (int b = 0; b < 18; b++) Ke[a * 18 + b] += alpha * (Bd[a] * Bd[b]) * A;
+}
+
+/* Frame + local stiffness + T'KT for one flat isotropic CTRIA3.
+ * prop = [E, nu, h, bend_ratio, ts_t, G_shear]; G_ref = G_shear
+ * (assembly.jl:2454-2460, 2515); k6rot -> 0 for a membrane-only shell
+ * (assembly.jl:3930-3933).  assembly.jl:3862-3955. */
+void or_t3_ke_global(const double p[9], const double prop[6], double k6rot, double *Keg /*18x18*/,
+                     double R_out[9], double lc_out[6]) {
+    double R[9], lc[6];
+    or_t3_frame(p, R, lc);
+    double E = prop[0], nu = prop[1], h = prop[2], br = prop[3], tst = prop[4], Gs = prop[5];
+    double Cm[9], Cb[9], Cs[4];
+    shell_constitutive(E, nu, Gs, h, br, tst, Cm, Cb, Cs);
+    double ek6 = br <= 1e-12 ? 0.0 : k6rot;
+    double Ke[324];
+    or_t3_ke_local(lc, Cm, Cb, Cs, h, Gs, br, ek6, Ke);
+    rotate_blocks(18, R, Ke, Keg);
+    if (R_out) memcpy(R_out, R, sizeof(R));
+    if (lc_out) memcpy(lc_out, lc, sizeof(lc));
+}
+
+/* CTRIA3 geometric stiffness.  Membrane stress of the constant-strain
+ * triangle sigma = N/h with N = (D eps) h, D = E/(1-nu^2)[1 nu 0; nu 1 0;
+ * 0 0 (1-nu)/2] (FEMKernels.jl:7391-7432); KG = h A (sxx b_i b_j + syy c_i c_j
+ * + sxy (b_i c_j + c_i b_j)) on u, v and w of each node pair -- the default
+ * principal-transverse mode falls through to this branch (FEMKernels.jl:
+ * 7989-8103); T'KG T with the drilling rows zeroed (assembly.jl:6020-6160).
+ * Also returns the membrane stress when sig_out is non-NULL. */
+void or_t3_kg_global(const double p[9], const double prop[6], const double *ug /*18*/, double *KGg /*18x18*/,
+                     double sig_out[3]) {
+    double R[9], lc[6];
+    or_t3_frame(p, R, lc);
+    double ul[18];
+    for (int b = 0; b < 6; b++) for (int i = 0; i < 3; i++) {
+        double s = 0; for (int j = 0; j < 3; j++) s += R[i * 3 + j] * ug[b * 3 + j];
+        ul[b * 3 + i] = s;
+    }
+    double E = prop[0], nu = prop[1], h = prop[2];
+    double x[3] = {lc[0], lc[2], lc[4]}, y[3] = {lc[1], lc[3], lc[5]};
+    double A = 0.5 * fabs(x[0] * (y[1] - y[2]) + x[1] * (y[2] - y[0]) + x[2] * (y[0] - y[1]));
+    double KG[324];
+    memset(KG, 0, sizeof(KG));
+    double sig[3] = {0, 0, 0};
+    if (A >= 1e-12 && h >= 1e-30) {
+        double bv[3] = {(y[1] - y[2]) / (2 * A), (y[2] - y[0]) / (2 * A), (y[0] - y[1]) / (2 * A)};
+        double cv[3] = {(x[2] - x[1]) / (2 * A), (x[0] - x[2]) / (2 * A), (x[1] - x[0]) / (2 * A)};
+        double exx = 0, eyy = 0, gxy = 0;
+        for (int i = 0; i < 3; i++) {
+            double ui = ul[6 * i], vi = ul[6 * i + 1];
+            exx += bv[i] * ui;
+            eyy += cv[i] * vi;
+            gxy += cv[i] * ui + bv[i] * vi;
+        }
+        double c = E / (1 - nu * nu);
+        double D[9] = {c, c * nu, 0.0, c * nu, c, 0.0, 0.0, 0.0, c * ((1 - nu) / 2)};
+        double N[3];
+        for (int i = 0; i < 3; i++) N[i] = (D[3 * i] * exx + D[3 * i + 1] * eyy + D[3 * i + 2] * gxy) * h;
+        for (int i = 0; i < 3; i++) sig[i] = N[i] / h;
+        for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) {
+            double v = h * A * (sig[0] * bv[i] * bv[j] + sig[1] * cv[i] * cv[j] + sig[2] * (bv[i] * cv[j] + cv[i] * bv[j]));
+            for (int d = 0; d < 3; d++) KG[(6 * i + d) * 18 + 6 * j + d] += v;
+        }
+    }
+    rotate_blocks(18, R, KG, KGg);
+    for (int k = 0; k < 3; k++) {
+        int d = 6 * k + 5;
+        for (int r = 0; r < 18; r++) { KGg[r * 18 + d] = 0.0; KGg[d * 18 + r] = 0.0; }
+    }
+    if (sig_out) memcpy(sig_out, sig, sizeof(sig));
+}
+
+/* ------------------------------------------------------------------------ */
 /* CBAR                                                                     */
 /* ------------------------------------------------------------------------ */
 
@@ -686,13 +895,15 @@ void or_bar_kg_global(const double p1[3], const double p2[3], const double vref[
 }
 
 /* ------------------------------------------------------------------------ */
-/* model-level COO assembly (element order: all CQUAD4, then all CBAR)       */
+/* model-level COO assembly (element order: CQUAD4, CTRIA3, then CBAR)      */
 /* ------------------------------------------------------------------------ */
 
 /* COO triplets exactly as the reference emits them: 576 per Q4 column-major
- * over (rr, cc), then 144 per bar.  assembly.jl:3839-3851, 4016-4020.
- * Returns the number of triplets written.  Node indices are 0-based ranks. */
+ * over (rr, cc), then 324 per CTRIA3, then 144 per bar.  assembly.jl:3839-3851,
+ * 3949-3953, 4016-4020.  Returns the number of triplets written.  Node
+ * indices are 0-based ranks. */
 int64_t or_assemble_K(int n_q4, const int32_t *q4, const double *xyz, const double *q4prop, double k6rot,
+                      int n_t3, const int32_t *t3, const double *t3prop,
                       int n_bar, const int32_t *bar, const double *bar_vref, const double *barprop,
                       int64_t *I, int64_t *J, double *V) {
     int64_t cnt = 0;
@@ -705,6 +916,18 @@ int64_t or_assemble_K(int n_q4, const int32_t *q4, const double *xyz, const doub
             I[cnt] = 6 * (int64_t)q4[4 * e + rr / 6] + rr % 6;
             J[cnt] = 6 * (int64_t)q4[4 * e + cc / 6] + cc % 6;
             V[cnt] = Keg[rr * 24 + cc];
+            cnt++;
+        }
+    }
+    for (int e = 0; e < n_t3; e++) {
+        double p[9];
+        for (int k = 0; k < 3; k++) for (int i = 0; i < 3; i++) p[3 * k + i] = xyz[3 * t3[3 * e + k] + i];
+        double Keg[324];
+        or_t3_ke_global(p, t3prop + 6 * e, k6rot, Keg, 0, 0);
+        for (int cc = 0; cc < 18; cc++) for (int rr = 0; rr < 18; rr++) {
+            I[cnt] = 6 * (int64_t)t3[3 * e + rr / 6] + rr % 6;
+            J[cnt] = 6 * (int64_t)t3[3 * e + cc / 6] + cc % 6;
+            V[cnt] = Keg[rr * 18 + cc];
             cnt++;
         }
     }
@@ -723,7 +946,7 @@ int64_t or_assemble_K(int n_q4, const int32_t *q4, const double *xyz, const doub
 }
 
 int64_t or_assemble_KG(int n_q4, const int32_t *q4, const double *xyz, const double *q4prop,
-                       int n_bar, const int32_t *bar, const double *bar_vref, const double *barprop,
+                       int n_t3, const int32_t *t3, const double *t3prop, int n_bar, const int32_t *bar, const double *bar_vref, const double *barprop,
                        const double *u, int64_t *I, int64_t *J, double *V) {
     int64_t cnt = 0;
     for (int e = 0; e < n_q4; e++) {
@@ -739,6 +962,22 @@ int64_t or_assemble_KG(int n_q4, const int32_t *q4, const double *xyz, const dou
             I[cnt] = 6 * (int64_t)q4[4 * e + rr / 6] + rr % 6;
             J[cnt] = 6 * (int64_t)q4[4 * e + cc / 6] + cc % 6;
             V[cnt] = KGg[rr * 24 + cc];
+            cnt++;
+        }
+    }
+    for (int e = 0; e < n_t3; e++) {
+        double p[9], ug[18];
+        for (int k = 0; k < 3; k++) {
+            int nd = t3[3 * e + k];
+            for (int i = 0; i < 3; i++) p[3 * k + i] = xyz[3 * nd + i];
+            for (int i = 0; i < 6; i++) ug[6 * k + i] = u[6 * (int64_t)nd + i];
+        }
+        double KGg[324];
+        or_t3_kg_global(p, t3prop + 6 * e, ug, KGg, 0);
+        for (int cc = 0; cc < 18; cc++) for (int rr = 0; rr < 18; rr++) {
+            I[cnt] = 6 * (int64_t)t3[3 * e + rr / 6] + rr % 6;
+            J[cnt] = 6 * (int64_t)t3[3 * e + cc / 6] + cc % 6;
+            V[cnt] = KGg[rr * 18 + cc];
             cnt++;
         }
     }

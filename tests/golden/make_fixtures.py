@@ -4,9 +4,10 @@ The reference holds no golden vectors for this path, so these fixtures are
 produced by the oracle after it has been pinned by tests/test_oracle_pins.py
 (closed forms and invariants).  They freeze the oracle's element outputs for
 seeded inputs: skewed CQUAD4s in arbitrary 3-D orientation with random
-material and thickness, CBARs with random sections and orientation vectors,
-and one displacement per element for KG; plus the eigenvalues of two small
-decks.  The GPU tests compare the device kernels against these vectors
+material and thickness, CTRIA3s of random shape (separate seed, so adding
+them left the other vectors unchanged), CBARs with random sections and
+orientation vectors, and one displacement per element for KG; plus the
+eigenvalues of small decks, two of them with mixed CQUAD4/CTRIA3 meshes.  The GPU tests compare the device kernels against these vectors
 through the C ABI, one element per model.
 
     python tests/golden/make_fixtures.py
@@ -50,6 +51,24 @@ def quads(rng, n=6):
     return out
 
 
+def tris(rng, n=6):
+    out = []
+    for k in range(n):
+        a = rng.uniform(5, 60)
+        pl = np.array([[0, 0], [a, 0], [rng.uniform(-0.5, 1.5) * a, rng.uniform(0.3, 1.2) * a]], float)
+        pts = np.c_[pl, np.zeros(3)] @ _rot(rng).T + rng.uniform(-100, 100, 3)
+        E = float(rng.choice([70e3, 210e3, 3.5e3]))
+        nu = float(rng.uniform(0.2, 0.4))
+        t = float(rng.uniform(0.5, 5.0))
+        br = 0.0 if k == n - 1 else 1.0          # the last one is membrane-only
+        prop = [E, nu, t, br, 5.0 / 6.0, E / (2 * (1 + nu))]
+        ug = rng.standard_normal(18) * 1e-3 * a
+        Ke, _, _ = O.t3_ke_global(pts.ravel(), np.array(prop))
+        KG, _ = O.t3_kg_global(pts.ravel(), np.array(prop), ug)
+        out.append(dict(xyz=pts.tolist(), prop=prop, u=ug.tolist(), Ke=Ke.ravel().tolist(), KG=KG.ravel().tolist()))
+    return out
+
+
 def bars(rng, n=4):
     out = []
     for _ in range(n):
@@ -76,6 +95,9 @@ def solves():
     cases = {
         "plate6x5": decks.plate_deck(nx=6, ny=5, a=300.0, b=250.0, t=1.5, nd=3),
         "panel12x6": decks.panel_deck(nx=12, ny=6, a=240.0, b=120.0, stringer_every=3, nd=4),
+        "mixed_plate8x7": decks.plate_deck(nx=8, ny=7, a=320.0, b=280.0, t=1.5, nd=3, tri_frac=0.4),
+        "mixed_panel12x6": decks.panel_deck(nx=12, ny=6, a=240.0, b=120.0, stringer_every=3, nd=4,
+                                            tri_frac=0.3),
     }
     out = {}
     for name, txt in cases.items():
@@ -88,7 +110,9 @@ def solves():
 
 def main():
     rng = np.random.default_rng(20260519)
-    json.dump(dict(quads=quads(rng), bars=bars(rng)), open(os.path.join(HERE, "elements.json"), "w"))
+    q, b = quads(rng), bars(rng)
+    t = tris(np.random.default_rng(20260520))
+    json.dump(dict(quads=q, bars=b, tris=t), open(os.path.join(HERE, "elements.json"), "w"))
     json.dump(solves(), open(os.path.join(HERE, "solves.json"), "w"), indent=1)
 
 

@@ -12,10 +12,10 @@ struct Probe {
 
 extern "C" {
 
-void *probe_analyze(int n_nodes, int n_q4, const int32_t *q4, int n_bar, const int32_t *bar, const double *xyz,
-                    const uint8_t *free_mask, int leaf) {
+void *probe_analyze(int n_nodes, int n_q4, const int32_t *q4, int n_t3, const int32_t *t3, int n_bar,
+                    const int32_t *bar, const double *xyz, const uint8_t *free_mask, int leaf) {
     Probe *p = new Probe();
-    build_block_pattern(n_nodes, n_q4, q4, n_bar, bar, p->P);
+    build_block_pattern(n_nodes, n_q4, q4, n_t3, t3, n_bar, bar, p->P);
     symbolic_analyze(p->P, xyz, free_mask, leaf, p->S);
     return p;
 }

@@ -29,6 +29,17 @@ def test_oracle_quad_fixtures(i):
         assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("i", range(len(ELEMS["tris"])))
+def test_oracle_tri_fixtures(i):
+    q = ELEMS["tris"][i]
+    p = np.asarray(q["xyz"]).ravel()
+    Ke, _, _ = O.t3_ke_global(p, np.asarray(q["prop"]))
+    KG, _ = O.t3_kg_global(p, np.asarray(q["prop"]), np.asarray(q["u"]))
+    for got, ref in ((Ke.ravel(), q["Ke"]), (KG.ravel(), q["KG"])):
+        ref = np.asarray(ref)
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("i", range(len(ELEMS["bars"])))
 def test_oracle_bar_fixtures(i):
     b = ELEMS["bars"][i]

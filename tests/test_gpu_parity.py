@@ -37,6 +37,13 @@ CASES = {
     "plate20": lambda: decks.plate_deck(),                                     # 20x20 SS plate, dense path
     "plate9x7": lambda: decks.plate_deck(nx=9, ny=7, a=450.0, b=350.0, t=1.2, nd=3),
     "panel": lambda: decks.panel_deck(nx=40, ny=20, a=400.0, b=200.0, stringer_every=5, nd=6),  # Krylov path
+    # mixed CQUAD4/CTRIA3 meshes: all-triangle plate with theta_z constrained
+    # (dense path) and a panel with 15% of the quads split (Krylov path,
+    # localization filter active)
+    "tri_plate": lambda: decks.plate_deck(nx=12, ny=10, a=480.0, b=400.0, t=1.4, nd=3, tri_frac=1.0,
+                                          spc_drill=True),
+    "mixed_panel": lambda: decks.panel_deck(nx=40, ny=20, a=400.0, b=200.0, stringer_every=5, nd=6,
+                                            tri_frac=0.15),
 }
 
 

@@ -118,8 +118,23 @@ def test_panel_arrays_and_sorted_grid_ranks():
     assert arr.node_has_bar.sum() == 3 * 21 and arr.node_has_shell.all()
 
 
+def test_ctria3_rows_and_shell_topology():
+    base = open(os.path.join(GOLDEN, "ref_sol105_quad_buckling.bdf")).read().replace("EIGRL,1,0.,1.0E9,3",
+                                                                                    "EIGRL,1,,,3")
+    arr = M.build_arrays(bdf.bdf_to_model_text(base.replace("CQUAD4,1,1,1,2,3,4", "CTRIA3,1,1,1,2,3")))
+    assert arr.n_q4 == 0 and arr.n_t3 == 1 and arr.n_shell == 1
+    assert list(arr.t3[0]) == [0, 1, 2] and arr.t3_eid[0] == 1
+    assert arr.t3prop.shape == (1, 6) and arr.t3prop[0, 2] > 0
+    assert list(arr.node_has_shell) == [1, 1, 1, 0]
+    mixed = M.build_arrays(bdf.bdf_to_model_text(decks.plate_deck(nx=6, ny=6, tri_frac=0.5)))
+    assert mixed.n_q4 + mixed.n_t3 // 2 == 36 and mixed.n_t3 % 2 == 0 and mixed.n_t3 > 0
+    assert mixed.node_has_shell.all()
+    t = mixed.shell_thickness() * 1.5
+    assert np.array_equal(mixed.with_shell_thickness(t).shell_thickness(), t)
+
+
 @pytest.mark.parametrize("edit,msg", [
-    (lambda t: t.replace("CQUAD4,1,1,1,2,3,4", "CTRIA3,1,1,1,2,3"), "CTRIA3"),
+    (lambda t: t.replace("EIGRL,1,,,3", "EIGRL,1,,,3\nPARAM,SNORM,20."), "SNORM"),
     (lambda t: t.replace("GRID,1,,0.,0.,0.", "GRID,1,,0.,0.,0.,5"), "CP/CD"),
     (lambda t: t.replace("GRID,3,,1.,1.,0.", "GRID,3,,1.,1.,0.1"), "warped"),
     (lambda t: t.replace("FORCE,1,2,0,", "FORCE,1,2,1,"), "CID"),

@@ -8,13 +8,22 @@ plausible slip (dropped term, wrong sign, transposed operand) fails it:
   placement, membrane constant-strain patch (nodal forces equal the boundary
   tractions), Kirchhoff constant-curvature bending energy, and in-plane pure
   bending of a rectangle reproduced exactly by the condensed Wilson modes;
+* CTRIA3 Ke: null space = the six rigid-body modes plus the two drilling
+  patterns that the one-point centroid penalty leaves free (exactly eight),
+  membrane constant-strain patch (nodal forces = edge tractions), the
+  drilling energy of a pure theta_z field (alpha A mean(theta_z)^2 / 2), and
+  the macro plate block against a brute-force Schur complement of three
+  virtual CQUAD4 plate blocks assembled and condensed with numpy;
+* CTRIA3 KG: constant-strain stress recovery sigma = D eps and the
+  second-order work h A grad(w).sigma.grad(w) of linear fields;
 * CBAR Ke: cantilever tip response (axial, torsion, Timoshenko bending in
   both planes); CBAR KG: Euler column load pi^2 EI / L^2;
 * CQUAD4 KG: rigid-translation invariance and the uniaxial-stress closed form
   -p h int dN/dx dN/dx^T on the transverse DOFs;
 * system level: static residual, dense vs Krylov eigen-solutions, Sturm
   inertia count (no missed modes), and the simply supported plate buckling
-  load 4 pi^2 D / b^2 within the FE discretisation band.
+  load 4 pi^2 D / b^2 within the FE discretisation band, for CQUAD4 and
+  (converging under refinement) all-CTRIA3 meshes.
 """
 import numpy as np
 import pytest
@@ -224,6 +233,137 @@ def test_q4_kg_uniaxial_closed_form_and_translation():
     assert np.abs(KGg @ Rm).max() <= 1e-10 * np.abs(KGg).max()
 
 
+# ---------------------------------------------------------------- CTRIA3
+
+TRI = np.array([[0.0, 0.0], [52.0, 3.0], [17.0, 41.0]])
+
+
+def test_all_tri_mesh_has_two_drilling_mechanisms():
+    """The assembled stiffness of an all-CTRIA3 patch (no constraints) has
+    exactly 6 rigid modes + 2 drilling patterns: theta_z = (1, w, w^2)-type
+    values over a 3-colouring of the nodes, summing to zero on every
+    triangle."""
+    arr = M.build_arrays(bdf.bdf_to_model_text(decks.plate_deck(nx=3, ny=3, a=90.0, b=90.0, tri_frac=1.0)))
+    K = O.assemble_K(arr).toarray()
+    ev = np.linalg.eigvalsh(0.5 * (K + K.T))
+    assert np.all(np.abs(ev[:8]) <= 1e-9 * ev[-1]) and ev[8] > 1e-8 * ev[-1]
+
+
+def test_t3_null_space_rigid_plus_two_drilling_modes():
+    R = _rot([0.3, -0.5, 0.8], 0.7)
+    pts = np.c_[TRI, np.zeros(3)] @ R.T + np.array([10.0, -20.0, 5.0])
+    Ke, Rf, _ = O.t3_ke_global(pts.ravel(), PROP)
+    scale = np.abs(Ke).max()
+    assert np.abs(Ke - Ke.T).max() <= 1e-12 * scale
+    # element normal = third frame axis; the drilling patterns rotate about it
+    nrm = Rf[2]
+    drill = []
+    for pat in ((1.0, -1.0, 0.0), (0.0, 1.0, -1.0)):
+        d = np.zeros((3, 6))
+        for i in range(3):
+            d[i, 3:] = pat[i] * nrm
+        drill.append(d.ravel())
+    N = np.c_[_rigid_modes(pts), np.array(drill).T]
+    assert np.abs(Ke @ N).max() <= 1e-9 * scale * np.abs(N).max()
+    ev = np.linalg.eigvalsh(0.5 * (Ke + Ke.T))
+    assert np.all(np.abs(ev[:8]) <= 1e-9 * ev[-1])
+    assert ev[8] > 1e-7 * ev[-1]
+
+
+def test_t3_membrane_patch_constant_strain():
+    Ke, _, _ = O.t3_ke_global(np.c_[TRI, np.zeros(3)].ravel(), PROP)
+    ex, ey, gxy, rot = 1.3e-3, -0.4e-3, 0.9e-3, 0.2e-3
+    d = np.zeros((3, 6))
+    for i, (x, y) in enumerate(TRI):
+        d[i, 0] = ex * x + (0.5 * gxy - rot) * y
+        d[i, 1] = (0.5 * gxy + rot) * x + ey * y
+        d[i, 5] = rot
+    f = (Ke @ d.ravel()).reshape(3, 6)
+    Cm = E * T / (1 - NU ** 2) * np.array([[1, NU, 0], [NU, 1, 0], [0, 0, (1 - NU) / 2]])
+    Nx, Ny, Nxy = Cm @ np.array([ex, ey, gxy])
+    Nt = np.array([[Nx, Nxy], [Nxy, Ny]])
+    fe = np.zeros((3, 2))
+    for i in range(3):
+        a, b = TRI[i], TRI[(i + 1) % 3]
+        t = b - a
+        tr = Nt @ np.array([t[1], -t[0]])
+        fe[i] += 0.5 * tr
+        fe[(i + 1) % 3] += 0.5 * tr
+    scale = np.abs(fe).max()
+    assert np.abs(f[:, :2] - fe).max() <= 1e-9 * scale
+    assert np.abs(f[:, 2:]).max() <= 1e-9 * scale
+
+
+def test_t3_drilling_energy_closed_form():
+    k6rot = 250.0
+    Ke, _, _ = O.t3_ke_global(np.c_[TRI, np.zeros(3)].ravel(), PROP, k6rot=k6rot)
+    th = np.array([0.7e-3, -0.2e-3, 1.1e-3])
+    d = np.zeros((3, 6)); d[:, 5] = th
+    U = 0.5 * d.ravel() @ Ke @ d.ravel()
+    A = 0.5 * abs((TRI[1, 0] - TRI[0, 0]) * (TRI[2, 1] - TRI[0, 1]) - (TRI[2, 0] - TRI[0, 0]) * (TRI[1, 1] - TRI[0, 1]))
+    alpha = k6rot / 1e5 * G * T
+    assert abs(U - 0.5 * alpha * A * th.mean() ** 2) <= 1e-12 * U
+
+
+def _iso(h, br=1.0):
+    Dm = np.array([[1, NU, 0], [NU, 1, 0], [0, 0, (1 - NU) / 2]]) * E / (1 - NU ** 2)
+    return Dm * h, Dm * br * h ** 3 / 12, np.eye(2) * 5.0 / 6.0 * G * h
+
+
+def test_t3_macro_plate_block_is_schur_complement_of_three_quads():
+    _, _, lc = O.t3_ke_global(np.c_[TRI, np.zeros(3)].ravel(), PROP)
+    Cm, Cb, Cs = _iso(T)
+    Kc, K21 = O.t3_plate_macro(lc, Cm, Cb, Cs, T, G)
+    c = lc
+    pts = np.array([c[0], c[1], c[2], (c[0] + c[1]) / 2, (c[1] + c[2]) / 2, (c[2] + c[0]) / 2, c.mean(axis=0)])
+    pd = [2, 3, 4, 8, 9, 10, 14, 15, 16, 20, 21, 22]
+    K = np.zeros((21, 21))
+    for quad in ((0, 3, 6, 5), (1, 4, 6, 3), (2, 5, 6, 4)):
+        Kq = O.q4_ke_local(pts[list(quad)], Cm, Cb, Cs.ravel(), T, G, bending_incomp=False)
+        dof = [3 * n + k for n in quad for k in range(3)]
+        K[np.ix_(dof, dof)] += Kq[np.ix_(pd, pd)]
+    scale = np.abs(K).max()
+    assert np.abs(K21 - K).max() <= 1e-13 * scale
+    ref = K[:9, :9] - K[:9, 9:] @ np.linalg.solve(K[9:, 9:], K[9:, :9])
+    assert np.abs(Kc - ref).max() <= 1e-10 * np.abs(ref).max()
+    # the default-flag quads keep the bending incompatible modes: condensing
+    # them instead changes the block, so the flag is pinned too
+    Kq_bi = O.q4_ke_local(pts[[0, 3, 6, 5]], Cm, Cb, Cs.ravel(), T, G, bending_incomp=True)
+    Kq_nb = O.q4_ke_local(pts[[0, 3, 6, 5]], Cm, Cb, Cs.ravel(), T, G, bending_incomp=False)
+    assert np.abs(Kq_bi[np.ix_(pd, pd)] - Kq_nb[np.ix_(pd, pd)]).max() > 1e-6 * scale
+    # rank 6 = 9 plate DOFs minus w translation and two rigid rotations
+    ev = np.linalg.eigvalsh(0.5 * (Kc + Kc.T))
+    assert np.all(np.abs(ev[:3]) <= 1e-9 * ev[-1]) and ev[3] > 1e-7 * ev[-1]
+
+
+def test_t3_kg_stress_recovery_and_linear_field_work():
+    R = _rot([-0.2, 0.9, 0.4], 1.1)
+    pts = np.c_[TRI, np.zeros(3)] @ R.T + np.array([3.0, 1.0, -7.0])
+    _, Rf, _ = O.t3_ke_global(pts.ravel(), PROP)
+    ex, ey, gxy = -1.1e-3, 0.3e-3, 0.5e-3
+    pts_l = (pts - pts.mean(axis=0)) @ Rf.T   # local coordinates about the centroid
+    ug = np.zeros((3, 6))
+    for i, (x, y, _) in enumerate(pts_l):
+        ul = np.array([ex * x + 0.5 * gxy * y, 0.5 * gxy * x + ey * y, 0.0])
+        ug[i, :3] = Rf.T @ ul
+    KG, sig = O.t3_kg_global(pts.ravel(), PROP, ug.ravel())
+    Dm = np.array([[1, NU, 0], [NU, 1, 0], [0, 0, (1 - NU) / 2]]) * E / (1 - NU ** 2)
+    s_ex = Dm @ np.array([ex, ey, gxy])
+    assert np.abs(sig - s_ex).max() <= 1e-10 * np.abs(s_ex).max()
+    A = 0.5 * np.linalg.norm(np.cross(pts[1] - pts[0], pts[2] - pts[0]))
+    S = np.array([[s_ex[0], s_ex[2]], [s_ex[2], s_ex[1]]])
+    for comp in range(3):                        # u, v and w: the all-translations branch
+        for gx, gy in ((1e-3, 0.0), (0.0, 1e-3), (0.6e-3, -0.8e-3)):
+            d = np.zeros((3, 6))
+            for i, (x, y, _) in enumerate(pts_l):
+                d[i, :3] = Rf[comp] * (gx * x + gy * y)
+            W = d.ravel() @ KG @ d.ravel()
+            g = np.array([gx, gy])
+            assert abs(W - T * A * g @ S @ g) <= 1e-9 * abs(T * A * g @ S @ g) + 1e-18
+    # rotations carry no geometric stiffness
+    assert np.abs(KG.reshape(3, 6, 3, 6)[:, 3:, :, :]).max() == 0.0
+
+
 # ---------------------------------------------------------------- system level
 
 @pytest.fixture(scope="module")
@@ -266,6 +406,23 @@ def test_dense_vs_krylov_and_inertia(plate):
     _, dd, _ = sla.ldl((K + sigma * KG).toarray())
     neg = int(np.sum(np.linalg.eigvalsh(dd) < 0))
     assert neg == 4
+
+
+def test_tri_plate_buckling_converges():
+    """All-CTRIA3 simply supported plate: the buckling load approaches
+    4 pi^2 D / b^2 under refinement (error down at least 4x per halving of h)
+    and is within 6 % on the 20 x 20 split mesh.  theta_z is constrained:
+    an all-triangle mesh has two zero-energy drilling patterns, and the
+    uniform prestress has no in-plane rotation for it to act on."""
+    D = E * T ** 3 / (12 * (1 - NU ** 2))
+    ncr = 4 * np.pi ** 2 * D / 500.0 ** 2
+    err = []
+    for n in (10, 20):
+        arr = M.build_arrays(bdf.bdf_to_model_text(decks.plate_deck(nx=n, ny=n, nd=1, tri_frac=1.0,
+                                                                      spc_drill=True)))
+        assert arr.n_q4 == 0 and arr.n_t3 == 2 * n * n
+        err.append(abs(O.solve(arr)["eigenvalues"][0] - ncr) / ncr)
+    assert err[1] <= 0.06 and err[1] <= err[0] / 4
 
 
 def test_plate_thickness_scaling():

@@ -35,7 +35,8 @@ def _probe():
     L = ctypes.CDLL(LIB)
     vp = ctypes.c_void_p
     L.probe_analyze.restype = vp
-    L.probe_analyze.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, ctypes.c_int]
+    L.probe_analyze.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp,
+                                ctypes.c_int]
     L.probe_free.argtypes = [vp]
     L.probe_sizes.argtypes = [vp, vp]
     L.probe_pattern.argtypes = [vp, vp, vp, vp, vp]
@@ -61,10 +62,12 @@ def _ptr(a):
 
 def analyze(L, arr, free_mask, leaf=8):
     q4 = np.ascontiguousarray(arr.q4, np.int32)
+    t3 = np.ascontiguousarray(arr.t3, np.int32)
     bar = np.ascontiguousarray(arr.bar, np.int32)
     xyz = np.ascontiguousarray(arr.xyz, np.float64)
     fm = np.ascontiguousarray(free_mask, np.uint8)
-    h = L.probe_analyze(arr.n_nodes, arr.n_q4, _ptr(q4), arr.n_bar, _ptr(bar), _ptr(xyz), _ptr(fm), leaf)
+    h = L.probe_analyze(arr.n_nodes, arr.n_q4, _ptr(q4), arr.n_t3, _ptr(t3), arr.n_bar, _ptr(bar), _ptr(xyz),
+                        _ptr(fm), leaf)
     sz = np.zeros(9, np.int64)
     L.probe_sizes(h, _ptr(sz))
     nb, nc, nfree, nf, nrows, nrel, nk, fs, height = [int(x) for x in sz]
@@ -96,7 +99,7 @@ def analyze(L, arr, free_mask, leaf=8):
 
 def element_blocks(arr):
     """Oracle element matrices laid out exactly like the device buffer."""
-    eb = np.zeros((16 * arr.n_q4 + 4 * arr.n_bar, 6, 6))
+    eb = np.zeros((16 * arr.n_q4 + 9 * arr.n_t3 + 4 * arr.n_bar, 6, 6))
     for e in range(arr.n_q4):
         p = arr.xyz[arr.q4[e]].ravel()
         Ke, _, _ = O.q4_ke_global(p, arr.q4prop[e], arr.k6rot)
@@ -104,6 +107,12 @@ def element_blocks(arr):
             for b in range(4):
                 eb[e * 16 + a * 4 + b] = Ke[6 * a:6 * a + 6, 6 * b:6 * b + 6]
     off = 16 * arr.n_q4
+    for e in range(arr.n_t3):
+        Ke, _, _ = O.t3_ke_global(arr.xyz[arr.t3[e]].ravel(), arr.t3prop[e], arr.k6rot)
+        for a in range(3):
+            for b in range(3):
+                eb[off + e * 9 + a * 3 + b] = Ke[6 * a:6 * a + 6, 6 * b:6 * b + 6]
+    off += 9 * arr.n_t3
     for i in range(arr.n_bar):
         n1, n2 = arr.bar[i]
         Kb = O.bar_ke_global(arr.xyz[n1], arr.xyz[n2], arr.bar_vref[i], arr.barprop[i])
@@ -196,10 +205,12 @@ def emulate_solve(S, F, b):
     return x
 
 
-@pytest.mark.parametrize("deck", ["plate", "panel"])
+@pytest.mark.parametrize("deck", ["plate", "panel", "mixed_panel"])
 def test_gather_and_multifrontal_emulation(probe, deck):
     if deck == "plate":
         txt = decks.plate_deck(nx=7, ny=5, a=350.0, b=250.0)
+    elif deck == "mixed_panel":
+        txt = decks.panel_deck(nx=12, ny=8, a=240.0, b=160.0, stringer_every=4, nd=4, tri_frac=0.35)
     else:
         txt = decks.panel_deck(nx=12, ny=8, a=240.0, b=160.0, stringer_every=4, nd=4)
     arr = M.build_arrays(bdf.bdf_to_model_text(txt))
