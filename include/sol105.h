@@ -140,6 +140,35 @@ int sol105_sensitivity_terms(sol105_ctx *ctx, sol105_model *model, int32_t mode,
 int sol105_profile_enable(sol105_ctx *ctx, int32_t on);
 int sol105_profile_read(sol105_ctx *ctx, double *out /* 4 */);
 
+/* Batched solves.  A batch stacks independent models (design variants,
+ * queued requests) into one block-diagonal system: element kernels,
+ * assembly, AUTOSPC (per member), the multifrontal factorisation and the
+ * triangular solves run once per elimination-tree height for all members,
+ * and the block Lanczos iteration runs on every member's own rows
+ * (per-member inner products, orthonormalisation and Rayleigh-Ritz) until
+ * each has converged.  Per member, results follow sol105_buckling's
+ * semantics; a member whose iteration needs the rank-revealing rebuild, or
+ * whose stacked factorisation fails, is re-solved alone.  Replaces the loop
+ * of independent solves of the reference's batch runner
+ * (JFEM/tools/manifest_batch_core.jl:409-478).  Members must share PARAM
+ * K6ROT and AUTOSPC. */
+typedef struct sol105_batch sol105_batch;
+int sol105_batch_create(sol105_ctx *ctx, const sol105_model_desc *members, int32_t n_members, sol105_batch **out);
+int sol105_batch_free(sol105_ctx *ctx, sol105_batch *batch);
+/* K -> u -> KG -> buckling for every member.  lambda_out: n_members x
+ * max_modes (row-major, zero-padded), n_out and status (SOL105_OK or the
+ * member's own error code): n_members each; any may be NULL.  Returns
+ * SOL105_OK when the batch ran, also if single members failed. */
+int sol105_batch_solve(sol105_ctx *ctx, sol105_batch *batch, int32_t max_modes, double *lambda_out, int32_t *n_out,
+                       int32_t *status);
+/* Mode shapes of one member after sol105_batch_solve: n_out[member] modes of
+ * 6*n_nodes[member] values each (mode-major, max |component| = 1). */
+int sol105_batch_modes(sol105_ctx *ctx, const sol105_batch *batch, int32_t member, double *modes_out);
+/* out[8]: device ms of the last batch solve, Lanczos vectors per member at
+ * the end, block steps, members re-solved alone, stacked free DOFs, front
+ * storage bytes, factor flops, tree height.  No reference counterpart. */
+int sol105_batch_stats(const sol105_ctx *ctx, const sol105_batch *batch, double *out /* 8 */);
+
 /* Phase timings of the last sol105_solve in milliseconds (device events):
  * [0] K assembly, [1] AUTOSPC+symbolic, [2] factorization, [3] static solve,
  * [4] KG assembly, [5] eigensolve, [6] post-processing, [7] total. */

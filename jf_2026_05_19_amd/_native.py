@@ -23,7 +23,8 @@ EXPORTS = [
     "sol105_model_set_q4_thickness", "sol105_pattern", "sol105_assemble_K", "sol105_static_solve",
     "sol105_assemble_KG", "sol105_buckling", "sol105_solve", "sol105_sensitivity_thickness",
     "sol105_sensitivity_terms", "sol105_profile_enable", "sol105_profile_read", "sol105_last_timings",
-    "sol105_last_stats",
+    "sol105_last_stats", "sol105_batch_create", "sol105_batch_free", "sol105_batch_solve", "sol105_batch_modes",
+    "sol105_batch_stats",
 ]
 
 
@@ -85,6 +86,12 @@ def lib():
         L.sol105_profile_read.argtypes = [vp, dp]
         L.sol105_last_timings.argtypes = [vp, dp]
         L.sol105_last_stats.argtypes = [vp, dp]
+        ip = ctypes.POINTER(ctypes.c_int32)
+        L.sol105_batch_create.argtypes = [vp, ctypes.POINTER(ModelDesc), ctypes.c_int32, ctypes.POINTER(vp)]
+        L.sol105_batch_free.argtypes = [vp, vp]
+        L.sol105_batch_solve.argtypes = [vp, vp, ctypes.c_int32, dp, ip, ip]
+        L.sol105_batch_modes.argtypes = [vp, vp, ctypes.c_int32, dp]
+        L.sol105_batch_stats.argtypes = [vp, vp, dp]
         _lib = L
     return _lib
 
@@ -140,6 +147,32 @@ class Context:
         return out
 
 
+def _model_desc(arr):
+    """(ModelDesc, arrays it points into) for a Sol105Arrays."""
+    keep = dict(
+        xyz=np.ascontiguousarray(arr.xyz, dtype=np.float64),
+        q4=np.ascontiguousarray(arr.q4, dtype=np.int32),
+        q4prop=np.ascontiguousarray(arr.q4prop, dtype=np.float64),
+        t3=np.ascontiguousarray(arr.t3, dtype=np.int32),
+        t3prop=np.ascontiguousarray(arr.t3prop, dtype=np.float64),
+        bar=np.ascontiguousarray(arr.bar, dtype=np.int32),
+        vref=np.ascontiguousarray(arr.bar_vref, dtype=np.float64),
+        barprop=np.ascontiguousarray(arr.barprop, dtype=np.float64),
+        spc=np.ascontiguousarray(arr.spc_mask, dtype=np.uint8),
+        F=np.ascontiguousarray(arr.F, dtype=np.float64),
+        hs=np.ascontiguousarray(arr.node_has_shell, dtype=np.uint8),
+        hb=np.ascontiguousarray(arr.node_has_bar, dtype=np.uint8),
+    )
+    k = keep
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    up = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    d = ModelDesc(arr.n_nodes, _dp(k["xyz"]), arr.n_q4, ip(k["q4"]), _dp(k["q4prop"]),
+                  arr.n_t3, ip(k["t3"]), _dp(k["t3prop"]), arr.n_bar, ip(k["bar"]), _dp(k["vref"]), _dp(k["barprop"]),
+                  up(k["spc"]), _dp(k["F"]), up(k["hs"]), up(k["hb"]),
+                  float(arr.k6rot), int(arr.autospc), int(arr.nd))
+    return d, keep
+
+
 class DeviceModel:
     """A model uploaded to the device (pattern, maps and ordering built)."""
 
@@ -147,27 +180,7 @@ class DeviceModel:
         self.ctx = ctx
         self.arr = arr
         L = ctx.L
-        self._keep = dict(
-            xyz=np.ascontiguousarray(arr.xyz, dtype=np.float64),
-            q4=np.ascontiguousarray(arr.q4, dtype=np.int32),
-            q4prop=np.ascontiguousarray(arr.q4prop, dtype=np.float64),
-            t3=np.ascontiguousarray(arr.t3, dtype=np.int32),
-            t3prop=np.ascontiguousarray(arr.t3prop, dtype=np.float64),
-            bar=np.ascontiguousarray(arr.bar, dtype=np.int32),
-            vref=np.ascontiguousarray(arr.bar_vref, dtype=np.float64),
-            barprop=np.ascontiguousarray(arr.barprop, dtype=np.float64),
-            spc=np.ascontiguousarray(arr.spc_mask, dtype=np.uint8),
-            F=np.ascontiguousarray(arr.F, dtype=np.float64),
-            hs=np.ascontiguousarray(arr.node_has_shell, dtype=np.uint8),
-            hb=np.ascontiguousarray(arr.node_has_bar, dtype=np.uint8),
-        )
-        k = self._keep
-        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
-        up = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
-        d = ModelDesc(arr.n_nodes, _dp(k["xyz"]), arr.n_q4, ip(k["q4"]), _dp(k["q4prop"]),
-                      arr.n_t3, ip(k["t3"]), _dp(k["t3prop"]), arr.n_bar, ip(k["bar"]), _dp(k["vref"]), _dp(k["barprop"]),
-                      up(k["spc"]), _dp(k["F"]), up(k["hs"]), up(k["hb"]),
-                      float(arr.k6rot), int(arr.autospc), int(arr.nd))
+        d, self._keep = _model_desc(arr)
         h = ctypes.c_void_p()
         ctx.check(L.sol105_model_upload(ctx.h, ctypes.byref(d), ctypes.byref(h)))
         self.h = h
@@ -256,3 +269,60 @@ class DeviceModel:
         I = (6 * brow[:, None] + ci[None, :]).ravel()
         J = (6 * self.col[:, None].astype(np.int64) + cj[None, :]).ravel()
         return sp.csr_matrix((np.asarray(vals).reshape(nb * 36), (I, J)), shape=(self.arr.ndof, self.arr.ndof))
+
+
+class Batch:
+    """Independent models solved together as one block-diagonal system
+    (sol105_batch_*): results per member, as a DeviceModel.solve of each
+    member alone would give them."""
+
+    def __init__(self, ctx: Context, arrs):
+        self.ctx = ctx
+        self.arrs = list(arrs)
+        pairs = [_model_desc(a) for a in self.arrs]
+        self._keep = [k for _, k in pairs]
+        descs = (ModelDesc * len(pairs))(*[d for d, _ in pairs])
+        h = ctypes.c_void_p()
+        ctx.check(ctx.L.sol105_batch_create(ctx.h, descs, len(pairs), ctypes.byref(h)))
+        self.h = h
+        self.n_out = None
+
+    def free(self):
+        if self.h:
+            self.ctx.L.sol105_batch_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def solve(self, max_modes=16):
+        """(list of eigenvalue arrays, status codes); a member's own error is
+        reported in its status, not raised."""
+        n = len(self.arrs)
+        lam = np.zeros((n, max_modes))
+        nout = np.zeros(n, np.int32)
+        st = np.zeros(n, np.int32)
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        self.ctx.check(self.ctx.L.sol105_batch_solve(self.ctx.h, self.h, int(max_modes), _dp(lam), ip(nout), ip(st)))
+        self.n_out = nout
+        return [lam[i, :nout[i]].copy() for i in range(n)], st
+
+    def modes(self, member):
+        k = int(self.n_out[member])
+        out = np.zeros((k, self.arrs[member].ndof))
+        if k:
+            self.ctx.check(self.ctx.L.sol105_batch_modes(self.ctx.h, self.h, int(member), _dp(out)))
+        return out.T.copy()
+
+    def stats(self):
+        """dict of the last batch solve: device ms, Lanczos size, steps,
+        members re-solved alone, stacked free DOFs, front bytes, factor flops,
+        tree height."""
+        out = np.zeros(8)
+        self.ctx.check(self.ctx.L.sol105_batch_stats(self.ctx.h, self.h, _dp(out)))
+        keys = ("ms", "lanczos_vectors", "block_steps", "fallbacks", "n_free", "front_bytes", "factor_flops",
+                "tree_height")
+        return dict(zip(keys, out.tolist()))

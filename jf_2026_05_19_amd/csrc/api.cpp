@@ -7,109 +7,18 @@
 //   sol105_assemble_KG   Solver.assemble_geometric_stiffness  JFEM/src/solver/assembly.jl:4382-6635
 //   sol105_buckling      Solver.solve_buckling         JFEM/src/solver/solve_case.jl:1863-3205
 //   sol105_sensitivity_thickness  solve_adjoint_buckling  JFEM/src/solver/adjoint_buckling.jl:2406-2725
-#include "../../include/sol105.h"
+#include "internal.h"
 
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
-#include <string>
-#include <vector>
 
-#include "common.h"
-#include "dense.h"
-#include "factor.h"
-#include "solve.h"
-#include "symbolic.h"
+using namespace s105;
 
-struct sol105_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    double timings[8] = {0};
-    double stats[8] = {0};
-    hipEvent_t ev[9] = {nullptr};
-    bool profiling = false;
-    ProdTimer ptimer;  // HIP-event timing of the block triangular-solve products
-    ProdTimer *timer() { return profiling ? &ptimer : nullptr; }
-};
-
-namespace {
-
-template <typename T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    void free_() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    ~DevBuf() { free_(); }
-    hipError_t alloc(size_t count) {
-        if (count <= n && p) return hipSuccess;
-        free_();
-        n = count;
-        return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
-    }
-    hipError_t upload(const T *h, size_t count, hipStream_t s) {
-        hipError_t e = alloc(count);
-        if (e != hipSuccess) return e;
-        if (count == 0) return hipSuccess;
-        return hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s);
-    }
-};
-
-}  // namespace
-
-struct sol105_model {
-    // host copies
-    int n_nodes = 0, n_q4 = 0, n_t3 = 0, n_bar = 0;
-    std::vector<double> xyz, q4prop, t3prop, bar_vref, barprop, force;
-    std::vector<int32_t> q4, t3, bar;
-    std::vector<uint8_t> spc_mask, has_shell, has_bar;
-    double k6rot = 100.0;
-    int autospc = 1, nd = 3;
-    BlockPattern pat;
-    std::vector<int32_t> diag_blk;
-    // device: model + assembly
-    DevBuf<double> d_xyz, d_q4prop, d_t3prop, d_bar_vref, d_barprop, d_force;
-    DevBuf<int32_t> d_q4, d_t3, d_bar, d_rp, d_col, d_cptr, d_cidx, d_diag;
-    DevBuf<double> d_eb, d_K, d_KG, d_u, d_kdiag;
-    // factor
-    bool have_sym = false;
-    std::vector<uint8_t> free_mask;
-    Symbolic sym;
-    FactorPlan plan;
-    DevBuf<int32_t> f_ns, f_nu, f_m, f_piv0, f_rows, f_rel, f_cptr, f_cidx, f_flist, f_ksrc, d_elim_of_dof,
-        d_dof_of_elim;
-    DevBuf<int64_t> f_off, f_rowoff, f_vecoff, f_reloff, f_kdst;
-    DevBuf<uint8_t> d_free, d_flag;
-    DevBuf<double> d_front, d_piv, d_W, d_kdiag_elim;
-    DevBuf<int> d_err;
-    DevFronts dev;
-    SolvePlan splan;
-    DevBuf<int64_t> s_poff;
-    DevBuf<int32_t> s_ftiles, s_btiles, s_ftbase, s_btbase, s_fnks, s_bnks, s_fkw, s_bkw, s_pullptr, s_pullsrc, s_lblk;
-    DevBuf<double> d_P, d_PT, d_part;
-    DevSolve dsolve;
-    // state
-    bool K_valid = false, factor_valid = false, u_valid = false, KG_valid = false;
-    // eigen results
-    std::vector<double> lambda;
-    DevBuf<double> d_modes;  // ndof x n_modes (normalised, global frame)
-    int n_modes = 0;
-    double lanczos_m = 0, lanczos_steps = 0;
-    int warm_m = 0, warm_b = 0;  // Lanczos size at the last convergence (first check there next time)
-    int ndof() const { return 6 * n_nodes; }
-};
-
-namespace {
+namespace s105 {
 
 int set_err(sol105_ctx *ctx, int code, const char *fmt, ...) {
     char buf[512];
@@ -120,12 +29,6 @@ int set_err(sol105_ctx *ctx, int code, const char *fmt, ...) {
     if (ctx) ctx->err = buf;
     return code;
 }
-
-#define HIPCHK(call)                                                                                      \
-    do {                                                                                                  \
-        hipError_t e_ = (call);                                                                           \
-        if (e_ != hipSuccess) return set_err(ctx, SOL105_E_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
-    } while (0)
 
 ElemArgs elem_args(sol105_model *M) {
     ElemArgs a;
@@ -144,17 +47,21 @@ ElemArgs elem_args(sol105_model *M) {
     return a;
 }
 
-// AUTOSPC on the diagonal (JFEM/src/solver/boundary_conditions.jl:296-383).
+// AUTOSPC on the diagonal (JFEM/src/solver/boundary_conditions.jl:296-383);
+// the translational / rotational maxima are those of each member's own K.
 void autospc_mask(const sol105_model *M, const std::vector<double> &kd, std::vector<uint8_t> &free_out) {
     int ndof = M->ndof();
     std::vector<uint8_t> fixed(M->spc_mask.begin(), M->spc_mask.end());
-    if (M->autospc) {
+    std::vector<int32_t> off = M->mem_node_off;
+    if (off.empty()) off = {0, M->n_nodes};
+    for (size_t mb = 0; M->autospc && mb + 1 < off.size(); mb++) {
+        int i0 = 6 * off[mb], i1 = 6 * off[mb + 1];
         double max_t = 1.0, max_r = 1.0;
-        for (int i = 0; i < ndof; i++) {
+        for (int i = i0; i < i1; i++) {
             if (i % 6 < 3) max_t = std::max(max_t, std::fabs(kd[i]));
             else max_r = std::max(max_r, std::fabs(kd[i]));
         }
-        for (int i = 0; i < ndof; i++) {
+        for (int i = i0; i < i1; i++) {
             if (fixed[i]) continue;
             int c = i % 6, v = i / 6;
             double th;
@@ -267,42 +174,7 @@ double ev_ms(hipEvent_t a, hipEvent_t b) {
     return (double)ms;
 }
 
-}  // namespace
-
-extern "C" {
-
-int sol105_ctx_create(int device, sol105_ctx **out) {
-    if (!out) return SOL105_E_INVALID;
-    *out = nullptr;
-    auto ctx = std::make_unique<sol105_ctx>();
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SOL105_E_DEVICE;
-    if (device < 0 || device >= ndev) return SOL105_E_INVALID;
-    ctx->device = device;
-    if (hipSetDevice(device) != hipSuccess) return SOL105_E_DEVICE;
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return SOL105_E_DEVICE;
-    for (auto &e : ctx->ev)
-        if (hipEventCreate(&e) != hipSuccess) return SOL105_E_DEVICE;
-    *out = ctx.release();
-    return SOL105_OK;
-}
-
-int sol105_ctx_destroy(sol105_ctx *ctx) {
-    if (!ctx) return SOL105_OK;
-    (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (auto &e : ctx->ev)
-        if (e) (void)hipEventDestroy(e);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
-    return SOL105_OK;
-}
-
-const char *sol105_last_error(const sol105_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
-
-int sol105_model_upload(sol105_ctx *ctx, const sol105_model_desc *d, sol105_model **out) {
-    if (!ctx || !d || !out) return set_err(ctx, SOL105_E_INVALID, "null argument");
-    *out = nullptr;
+int model_init(sol105_ctx *ctx, const sol105_model_desc *d, sol105_model *M) {
     if (d->n_nodes <= 0 || !d->xyz) return set_err(ctx, SOL105_E_INVALID, "model has no nodes");
     if (d->n_q4 < 0 || d->n_t3 < 0 || d->n_bar < 0) return set_err(ctx, SOL105_E_INVALID, "negative element count");
     if ((d->n_q4 > 0 && (!d->q4_nodes || !d->q4_prop)) || (d->n_t3 > 0 && (!d->t3_nodes || !d->t3_prop)) ||
@@ -312,7 +184,6 @@ int sol105_model_upload(sol105_ctx *ctx, const sol105_model_desc *d, sol105_mode
         return set_err(ctx, SOL105_E_INVALID, "missing SPC / load / topology arrays");
     if (d->nd <= 0) return set_err(ctx, SOL105_E_INVALID, "EIGRL ND must be positive");
     (void)hipSetDevice(ctx->device);
-    auto M = std::make_unique<sol105_model>();
     int n = d->n_nodes;
     M->n_nodes = n;
     M->n_q4 = d->n_q4;
@@ -372,6 +243,48 @@ int sol105_model_upload(sol105_ctx *ctx, const sol105_model_desc *d, sol105_mode
     HIPCHK(M->d_u.alloc(6 * (size_t)n));
     HIPCHK(M->d_kdiag.alloc(6 * (size_t)n));
     HIPCHK(hipStreamSynchronize(s));
+    return SOL105_OK;
+}
+
+}  // namespace s105
+
+extern "C" {
+
+int sol105_ctx_create(int device, sol105_ctx **out) {
+    if (!out) return SOL105_E_INVALID;
+    *out = nullptr;
+    auto ctx = std::make_unique<sol105_ctx>();
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SOL105_E_DEVICE;
+    if (device < 0 || device >= ndev) return SOL105_E_INVALID;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) return SOL105_E_DEVICE;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return SOL105_E_DEVICE;
+    for (auto &e : ctx->ev)
+        if (hipEventCreate(&e) != hipSuccess) return SOL105_E_DEVICE;
+    *out = ctx.release();
+    return SOL105_OK;
+}
+
+int sol105_ctx_destroy(sol105_ctx *ctx) {
+    if (!ctx) return SOL105_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SOL105_OK;
+}
+
+const char *sol105_last_error(const sol105_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int sol105_model_upload(sol105_ctx *ctx, const sol105_model_desc *d, sol105_model **out) {
+    if (!ctx || !d || !out) return set_err(ctx, SOL105_E_INVALID, "null argument");
+    *out = nullptr;
+    auto M = std::make_unique<sol105_model>();
+    int rc = model_init(ctx, d, M.get());
+    if (rc) return rc;
     *out = M.release();
     return SOL105_OK;
 }
@@ -425,7 +338,11 @@ int sol105_assemble_K(sol105_ctx *ctx, sol105_model *M, double *vals) {
     return SOL105_OK;
 }
 
-static int factorize(sol105_ctx *ctx, sol105_model *M) {
+}  // extern "C"
+
+namespace s105 {
+
+int factorize(sol105_ctx *ctx, sol105_model *M) {
     hipStream_t s = ctx->stream;
     int ndof = M->ndof();
     // AUTOSPC from the assembled diagonal
@@ -473,6 +390,10 @@ static int factorize(sol105_ctx *ctx, sol105_model *M) {
     }
     return set_err(ctx, SOL105_E_SINGULAR, "stiffness matrix is not positive definite on the free set");
 }
+
+}  // namespace s105
+
+extern "C" {
 
 int sol105_static_solve(sol105_ctx *ctx, sol105_model *M, double *u, uint8_t *free_mask) {
     if (!ctx || !M) return set_err(ctx, SOL105_E_INVALID, "null argument");

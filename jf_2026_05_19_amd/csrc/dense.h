@@ -42,3 +42,56 @@ constexpr int SYMEIG_MAX_M = 1024;
 constexpr int SYMEIG_MAX_K = 64;
 int64_t symeig_work_size(int m, int k);
 void sym_eig_top(int m, double *A, int k, double *theta, double *Zs, double *work, int *info, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Segmented forms for a batch of independent problems stacked by rows (each
+// member's free DOFs in elimination order).  Every operation acts on each
+// active segment alone; per-segment small matrices are strided (base + s *
+// stride).  Chunks never cross a segment and start at multiples of their
+// size from the segment start, so a segment reproduces the single-problem
+// arithmetic.
+// ---------------------------------------------------------------------------
+struct SegTable {
+    int n_seg = 0, n_ch = 0, n_chs = 0;
+    const int32_t *seg_off = nullptr;                                         // n_seg + 1
+    const int32_t *ch_r0 = nullptr, *ch_seg = nullptr, *ch_ptr = nullptr;     // 512-row chunks, ptr n_seg + 1
+    const int32_t *chs_r0 = nullptr, *chs_seg = nullptr, *chs_ptr = nullptr;  // 256-row chunks
+    const uint8_t *active = nullptr;                                          // n_seg
+};
+constexpr int SEG_CH = 512, SEG_CHS = 256;
+void seg_gemm_tn(const SegTable &T, int P, int Q, const double *V, int ldv, const double *W, int ldw, double *out,
+                 int64_t so, int ldo, double alpha, double beta, double *part, hipStream_t s);
+int64_t seg_gemm_tn_part_size(int n_ch, int n_chs, int P, int Q);
+// Q <= 16
+void seg_gemm_nn(const SegTable &T, int P, int Q, const double *V, int ldv, const double *H, int64_t sh, int ldh,
+                 double *Y, int ldy, double alpha, double beta, hipStream_t s);
+// err: two ints per segment, as small_chol_upper(_checked) sets them
+void seg_chol_upper(const SegTable &T, int b, double *G, int64_t sg, int ldg, const double *g0, int64_t sg0,
+                    int ldg0, double tol2, int *err, hipStream_t s);
+void seg_trsm_right_upper(const SegTable &T, int b, double *W, int ldw, const double *R, int64_t sr, int ldr,
+                          hipStream_t s);
+void seg_upper_mul(const SegTable &T, int b, const double *R2, int64_t s2, int ld2, const double *R1, int64_t s1,
+                   int ld1, double *out, int64_t so, int ldo, hipStream_t s);
+// start block of every segment (row index local to the segment)
+void seg_start_block(const SegTable &T, int b, double *V, int ldv, int seed, hipStream_t s);
+// V[rows of the active segments, 0:cols] = v
+void seg_fill(const SegTable &T, int cols, double *V, int ldv, double v, hipStream_t s);
+void seg_symmetrize_copy(const SegTable &T, int m, int ldT, const double *Tm, int64_t sT, double *A, int64_t sA,
+                         hipStream_t s);
+void seg_mat_add(const SegTable &T, int rows, int cols, const double *src, int64_t ss, int lds, double *dst,
+                 int64_t sd, int ldd, hipStream_t s);
+// dst_s = src_s (rows x cols) for the active segments
+void seg_copy(const SegTable &T, int rows, int cols, const double *src, int64_t ss, int lds, double *dst, int64_t sd,
+              int ldd, hipStream_t s);
+// per (member, column) pair: (max, sum) over the member's shells (element
+// ranges q4_off / t3_off) of the mean |translation|^2 at their nodes
+void seg_energy_share(int n_pairs, const int32_t *pairs, const int32_t *q4_off, const int32_t *q4,
+                      const int32_t *t3_off, const int32_t *t3, int64_t ldx, const double *X, double *out2,
+                      hipStream_t s);
+// per (member, column) pair: X[dof_off[m]:dof_off[m+1], col] / its max |x|, to dst + dst_off[pair]
+void seg_normalize(int n_pairs, const int32_t *pairs, const int64_t *dof_off, const int64_t *dst_off, int64_t ldx,
+                   const double *X, double *dst, hipStream_t s);
+// sym_eig_top for the listed problems p (A + p sA, theta + p sT, Zs + p sZ,
+// work + p sW, info + p), one workgroup each, all of size m
+void sym_eig_top_batched(int m, double *A, int64_t sA, int k, double *theta, int64_t sT, double *Zs, int64_t sZ,
+                         double *work, int64_t sW, int *info, const int *list, int nlist, hipStream_t s);

@@ -17,13 +17,12 @@ constexpr int PC = 64;     // columns of V per workgroup
 // V.  Each 64-row tile of V (64 x 64) and W (64 x 16) is staged in LDS with
 // coalesced loads; lane (pl, ql) accumulates the four outputs
 // (pl + 16 i, ql), i = 0..3, so W is re-read once per 64 columns of V.
-__global__ void __launch_bounds__(256) tn_partial_kernel(int n, int P, int Q, const double *V, int ldv, const double *W,
-                                                         int ldw, double *part) {
+__device__ __forceinline__ void tn_partial_tile(int r0, int r1, int p0, int P, int Q, const double *V, int ldv,
+                                                const double *W, int ldw, double *part_c) {
     __shared__ double Vs[TR][PC + 1];
     __shared__ double Ws[TR][17];
-    const int c = blockIdx.x, p0 = blockIdx.y * PC, t = threadIdx.x;
+    const int t = threadIdx.x;
     const int pl = t % 16, ql = t / 16;
-    const int r0 = c * CH, r1 = min(n, r0 + CH);
     const int lr = t % TR, lc = t / TR;  // this lane's load row and first column
     double vreg[PC / 4], wreg[4];
     // tile loads go to registers first, so the next tile is in flight while
@@ -62,34 +61,46 @@ __global__ void __launch_bounds__(256) tn_partial_kernel(int n, int P, int Q, co
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int p = p0 + pl + 16 * i;
-            if (p < P) part[((int64_t)c * P + p) * Q + ql] = acc[i];
+            if (p < P) part_c[(int64_t)p * Q + ql] = acc[i];
         }
+}
+
+__global__ void __launch_bounds__(256) tn_partial_kernel(int n, int P, int Q, const double *V, int ldv, const double *W,
+                                                         int ldw, double *part) {
+    const int c = blockIdx.x;
+    tn_partial_tile(c * CH, min(n, c * CH + CH), blockIdx.y * PC, P, Q, V, ldv, W, ldw, part + (int64_t)c * P * Q);
 }
 
 // P, Q <= 16 (Gram matrices of one block): one 256-row chunk per workgroup,
 // both tiles in LDS after a single coalesced load each
 constexpr int CHS = 256;
-__global__ void __launch_bounds__(256) tn_small_kernel(int n, int P, int Q, const double *V, int ldv, const double *W,
-                                                       int ldw, double *part) {
+__device__ __forceinline__ void tn_small_tile(int r0, int r1, int P, int Q, const double *V, int ldv, const double *W,
+                                              int ldw, double *part_c) {
     __shared__ double Vs[CHS][17];
     __shared__ double Ws[CHS][17];
-    const int c = blockIdx.x, t = threadIdx.x, row = c * CHS + t;
+    const int t = threadIdx.x, row = r0 + t;
 #pragma unroll
-    for (int p = 0; p < 16; p++) Vs[t][p] = (row < n && p < P) ? V[row + (int64_t)p * ldv] : 0.0;
+    for (int p = 0; p < 16; p++) Vs[t][p] = (row < r1 && p < P) ? V[row + (int64_t)p * ldv] : 0.0;
 #pragma unroll
-    for (int q = 0; q < 16; q++) Ws[t][q] = (row < n && q < Q) ? W[row + (int64_t)q * ldw] : 0.0;
+    for (int q = 0; q < 16; q++) Ws[t][q] = (row < r1 && q < Q) ? W[row + (int64_t)q * ldw] : 0.0;
     __syncthreads();
     const int pl = t % 16, ql = t / 16;
     double acc = 0.0;
 #pragma unroll 8
     for (int r = 0; r < CHS; r++) acc += Vs[r][pl] * Ws[r][ql];
-    if (pl < P && ql < Q) part[((int64_t)c * P + pl) * Q + ql] = acc;
+    if (pl < P && ql < Q) part_c[(int64_t)pl * Q + ql] = acc;
+}
+
+__global__ void __launch_bounds__(256) tn_small_kernel(int n, int P, int Q, const double *V, int ldv, const double *W,
+                                                       int ldw, double *part) {
+    const int c = blockIdx.x;
+    tn_small_tile(c * CHS, min(n, c * CHS + CHS), P, Q, V, ldv, W, ldw, part + (int64_t)c * P * Q);
 }
 
 // out[p + q*ldo] = alpha * sum_c part[c][p][q] + beta * out[...]; sixteen lanes
 // per output sum interleaved chunk subsets, added in a fixed order
-__global__ void __launch_bounds__(256) tn_reduce_kernel(int nch, int P, int Q, const double *part, double *out,
-                                                        int ldo, double alpha, double beta) {
+__device__ __forceinline__ void tn_reduce_body(int nch, int P, int Q, const double *part, double *out, int ldo,
+                                               double alpha, double beta) {
     __shared__ double red[16][17];
     const int ol = threadIdx.x % 16, sg = threadIdx.x / 16;
     const int o = blockIdx.x * 16 + ol;
@@ -109,13 +120,16 @@ __global__ void __launch_bounds__(256) tn_reduce_kernel(int nch, int P, int Q, c
     }
 }
 
+__global__ void __launch_bounds__(256) tn_reduce_kernel(int nch, int P, int Q, const double *part, double *out,
+                                                        int ldo, double alpha, double beta) {
+    tn_reduce_body(nch, P, Q, part, out, ldo, alpha, beta);
+}
+
 // Y[row, q] = beta*Y + alpha * sum_p V[row, p] H[p, q]   (Q <= 16), one row
 // per lane, H staged in LDS, eight columns of V in flight per lane
-__global__ void __launch_bounds__(256) nn_kernel(int n, int P, int Q, const double *V, int ldv, const double *H,
-                                                 int ldh, double *Y, int ldy, double alpha, double beta) {
+__device__ __forceinline__ void nn_rows(int row, bool live, int P, int Q, const double *V, int ldv, const double *H,
+                                        int ldh, double *Y, int ldy, double alpha, double beta) {
     __shared__ double Hs[256][16];
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = row < n;
     const double *vr = V + (live ? row : 0);
     double acc[16];
 #pragma unroll
@@ -153,14 +167,20 @@ __global__ void __launch_bounds__(256) nn_kernel(int n, int P, int Q, const doub
         }
 }
 
+__global__ void __launch_bounds__(256) nn_kernel(int n, int P, int Q, const double *V, int ldv, const double *H,
+                                                 int ldh, double *Y, int ldy, double alpha, double beta) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    nn_rows(row, row < n, P, Q, V, ldv, H, ldh, Y, ldy, alpha, beta);
+}
+
 // Cholesky G = R^T R (upper R, b <= 16) in place, one wavefront: step j
 // forms the Schur-complement row j on lanes j..b-1.  Without g0, a pivot
 // d <= 0 sets err[0] |= 2.  With g0 (the Gram matrix of the block before
 // Gram-Schmidt, ld ldg0), a pivot d <= tol2 * max_i g0_ii marks a direction
 // that Gram-Schmidt reduced to rounding noise: err[0] |= 4 and err[1] keeps
 // the smallest (blk + 1) seen.  Pivots are floored so no NaN is produced.
-__global__ void __launch_bounds__(64) chol_upper_kernel(int b, double *G, int ldg, const double *g0, int ldg0,
-                                                        double tol2, int blk, int *err) {
+__device__ __forceinline__ void chol_upper_body(int b, double *G, int ldg, const double *g0, int ldg0, double tol2,
+                                                int blk, int *err) {
     __shared__ double R[16][17];
     __shared__ double dsh;
     const int t = threadIdx.x;
@@ -207,16 +227,20 @@ __global__ void __launch_bounds__(64) chol_upper_kernel(int b, double *G, int ld
     }
 }
 
+__global__ void __launch_bounds__(64) chol_upper_kernel(int b, double *G, int ldg, const double *g0, int ldg0,
+                                                        double tol2, int blk, int *err) {
+    chol_upper_body(b, G, ldg, g0, ldg0, tol2, blk, err);
+}
+
 // W = W R^-1 (R upper b x b), one thread per row
-__global__ void trsm_right_upper_kernel(int n, int b, double *W, int ldw, const double *R, int ldr) {
+__device__ __forceinline__ void trsm_rows(int row, bool live, int b, double *W, int ldw, const double *R, int ldr) {
     __shared__ double Rs[16][16];
     for (int t = threadIdx.x; t < 256; t += blockDim.x) {
         int i = t % 16, j = t / 16;
         Rs[i][j] = (i < b && j < b) ? R[i + j * ldr] : 0.0;
     }
     __syncthreads();
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= n) return;
+    if (!live) return;
     double x[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) x[j] = (j < b) ? W[row + (int64_t)j * ldw] : 0.0;
@@ -234,14 +258,24 @@ __global__ void trsm_right_upper_kernel(int n, int b, double *W, int ldw, const 
         if (j < b) W[row + (int64_t)j * ldw] = x[j];
 }
 
+__global__ void trsm_right_upper_kernel(int n, int b, double *W, int ldw, const double *R, int ldr) {
+    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    trsm_rows(row, row < n, b, W, ldw, R, ldr);
+}
+
 // R_prod = R2 * R1 (upper b x b): used for CholQR2 (R = R2 R1)
-__global__ void upper_mul_kernel(int b, const double *R2, int ld2, const double *R1, int ld1, double *out, int ldo) {
+__device__ __forceinline__ void upper_mul_body(int b, const double *R2, int ld2, const double *R1, int ld1, double *out,
+                                               int ldo) {
     int t = threadIdx.x;
     if (t >= b * b) return;
     int i = t % b, j = t / b;
     double s = 0.0;
     for (int k = 0; k < b; k++) s += R2[i + k * ld2] * R1[k + j * ld1];
     out[i + j * ldo] = s;
+}
+
+__global__ void upper_mul_kernel(int b, const double *R2, int ld2, const double *R1, int ld1, double *out, int ldo) {
+    upper_mul_body(b, R2, ld2, R1, ld1, out, ldo);
 }
 
 // Start block (elimination order).  Column 0 is the reference's deterministic
@@ -265,10 +299,7 @@ __global__ void hash_fill_kernel(int n, int cols, double *V, int ldv, uint64_t s
     V[i + (int64_t)c * ldv] = hash_unit((seed << 40) ^ ((uint64_t)c << 32) ^ (uint64_t)i);
 }
 
-__global__ void start_block_kernel(int n, int b, double *V, int ldv, int seed) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)n * b) return;
-    int i = (int)(t % n), r = (int)(t / n);
+__device__ __forceinline__ double start_value(int i, int r, int seed) {
     double v;
     if (r == 0) {
         double x = (i + 1) + 0.137 * (seed + 1) + 0.731 * 1;
@@ -276,7 +307,14 @@ __global__ void start_block_kernel(int n, int b, double *V, int ldv, int seed) {
     } else {
         v = hash_unit(((uint64_t)(seed + 1) << 48) ^ ((uint64_t)r << 32) ^ (uint64_t)i);
     }
-    V[i + (int64_t)r * ldv] = v;
+    return v;
+}
+
+__global__ void start_block_kernel(int n, int b, double *V, int ldv, int seed) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * b) return;
+    int i = (int)(t % n), r = (int)(t / n);
+    V[i + (int64_t)r * ldv] = start_value(i, r, seed);
 }
 
 // per-element translational energy share of each mode (localization filter,
@@ -526,3 +564,242 @@ void mat_add(int rows, int cols, const double *src, int lds, double *dst, int ld
     if (rows * cols > 0) mat_add_kernel<<<(rows * cols + 255) / 256, 256, 0, s>>>(rows, cols, src, lds, dst, ldd);
 }
 
+
+// ---------------------------------------------------------------------------
+// Segmented forms: a batch of independent problems stacked by rows.  Every
+// chunk lies inside one segment and starts at a multiple of its size from the
+// segment start, and reductions run over a segment's chunks in the same
+// interleaved order as the single-problem kernels, so a segment reproduces
+// the arithmetic of the same problem solved alone.
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ void __launch_bounds__(256) seg_tn_partial_kernel(SegTable T, int P, int Q, const double *V, int ldv,
+                                                             const double *W, int ldw, double *part) {
+    const int c = blockIdx.x, sg = T.ch_seg[c];
+    if (!T.active[sg]) return;
+    const int r0 = T.ch_r0[c], r1 = min(r0 + CH, T.seg_off[sg + 1]);
+    tn_partial_tile(r0, r1, blockIdx.y * PC, P, Q, V, ldv, W, ldw, part + (int64_t)c * P * Q);
+}
+
+__global__ void __launch_bounds__(256) seg_tn_small_kernel(SegTable T, int P, int Q, const double *V, int ldv,
+                                                           const double *W, int ldw, double *part) {
+    const int c = blockIdx.x, sg = T.chs_seg[c];
+    if (!T.active[sg]) return;
+    const int r0 = T.chs_r0[c], r1 = min(r0 + CHS, T.seg_off[sg + 1]);
+    tn_small_tile(r0, r1, P, Q, V, ldv, W, ldw, part + (int64_t)c * P * Q);
+}
+
+__global__ void __launch_bounds__(256) seg_tn_reduce_kernel(SegTable T, bool small, int P, int Q, const double *part,
+                                                            double *out, int64_t so, int ldo, double alpha,
+                                                            double beta) {
+    const int sg = blockIdx.y;
+    if (!T.active[sg]) return;
+    const int32_t *ptr = small ? T.chs_ptr : T.ch_ptr;
+    tn_reduce_body(ptr[sg + 1] - ptr[sg], P, Q, part + (int64_t)ptr[sg] * P * Q, out + sg * so, ldo, alpha, beta);
+}
+
+__global__ void __launch_bounds__(256) seg_nn_kernel(SegTable T, int P, int Q, const double *V, int ldv,
+                                                     const double *H, int64_t sh, int ldh, double *Y, int ldy,
+                                                     double alpha, double beta) {
+    const int c = blockIdx.x, sg = T.chs_seg[c];
+    if (!T.active[sg]) return;
+    const int row = T.chs_r0[c] + threadIdx.x;
+    nn_rows(row, row < min(T.chs_r0[c] + CHS, T.seg_off[sg + 1]), P, Q, V, ldv, H + sg * sh, ldh, Y, ldy, alpha,
+            beta);
+}
+
+__global__ void __launch_bounds__(64) seg_chol_kernel(SegTable T, int b, double *G, int64_t sgs, int ldg,
+                                                      const double *g0, int64_t sg0, int ldg0, double tol2, int *err) {
+    const int sg = blockIdx.x;
+    if (!T.active[sg]) return;
+    chol_upper_body(b, G + sg * sgs, ldg, g0 ? g0 + sg * sg0 : nullptr, ldg0, tol2, 0, err + 2 * sg);
+}
+
+__global__ void seg_trsm_kernel(SegTable T, int b, double *W, int ldw, const double *R, int64_t sr, int ldr) {
+    const int c = blockIdx.x, sg = T.chs_seg[c];
+    if (!T.active[sg]) return;
+    const int row = T.chs_r0[c] + threadIdx.x;
+    trsm_rows(row, row < min(T.chs_r0[c] + CHS, T.seg_off[sg + 1]), b, W, ldw, R + sg * sr, ldr);
+}
+
+__global__ void seg_upper_mul_kernel(SegTable T, int b, const double *R2, int64_t s2, int ld2, const double *R1,
+                                     int64_t s1, int ld1, double *out, int64_t so, int ldo) {
+    const int sg = blockIdx.x;
+    if (!T.active[sg]) return;
+    upper_mul_body(b, R2 + sg * s2, ld2, R1 + sg * s1, ld1, out + sg * so, ldo);
+}
+
+__global__ void seg_start_block_kernel(SegTable T, double *V, int ldv, int seed) {
+    const int c = blockIdx.x, r = blockIdx.y, sg = T.chs_seg[c];
+    const int row = T.chs_r0[c] + threadIdx.x;
+    if (row >= min(T.chs_r0[c] + CHS, T.seg_off[sg + 1])) return;
+    V[row + (int64_t)r * ldv] = start_value(row - T.seg_off[sg], r, seed);
+}
+
+__global__ void seg_symmetrize_kernel(SegTable T, int m, int ldT, const double *Tm, int64_t sT, double *A,
+                                      int64_t sA) {
+    const int sg = blockIdx.y;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!T.active[sg] || t >= m * m) return;
+    const int i = t % m, j = t / m;
+    const double *Ts = Tm + sg * sT;
+    A[sg * sA + t] = 0.5 * (Ts[i + (int64_t)j * ldT] + Ts[j + (int64_t)i * ldT]);
+}
+
+__global__ void seg_mat_add_kernel(SegTable T, int rows, int cols, const double *src, int64_t ss, int lds,
+                                   double *dst, int64_t sd, int ldd, bool acc) {
+    const int sg = blockIdx.y;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!T.active[sg] || t >= rows * cols) return;
+    const int i = t % rows, j = t / rows;
+    double *d = dst + sg * sd + i + (int64_t)j * ldd;
+    const double v = src[sg * ss + i + (int64_t)j * lds];
+    *d = acc ? *d + v : v;
+}
+
+__global__ void seg_fill_kernel(SegTable T, double *V, int ldv, double v) {
+    const int c = blockIdx.x, r = blockIdx.y, sg = T.chs_seg[c];
+    if (!T.active[sg]) return;
+    const int row = T.chs_r0[c] + threadIdx.x;
+    if (row < min(T.chs_r0[c] + CHS, T.seg_off[sg + 1])) V[row + (int64_t)r * ldv] = v;
+}
+
+// one workgroup per (member, mode column) pair: max and sum over the
+// member's shells of the mean |translation|^2 at their nodes
+__global__ void __launch_bounds__(256) seg_energy_kernel(const int32_t *pairs, const int32_t *q4_off,
+                                                         const int32_t *q4, const int32_t *t3_off, const int32_t *t3,
+                                                         int64_t ldx, const double *X, double *out2) {
+    __shared__ double smax[256], ssum[256];
+    const int p = blockIdx.x, mb = pairs[2 * p], col = pairs[2 * p + 1];
+    const double *x = X + (int64_t)col * ldx;
+    double mx = 0.0, sm = 0.0;
+    for (int e = q4_off[mb] + threadIdx.x; e < q4_off[mb + 1]; e += 256) {
+        double v = 0.0;
+        for (int a = 0; a < 4; a++) {
+            const double *t = x + 6 * (int64_t)q4[4 * e + a];
+            v += t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+        }
+        v /= 4.0;
+        mx = fmax(mx, v);
+        sm += v;
+    }
+    for (int e = t3_off[mb] + threadIdx.x; e < t3_off[mb + 1]; e += 256) {
+        double v = 0.0;
+        for (int a = 0; a < 3; a++) {
+            const double *t = x + 6 * (int64_t)t3[3 * e + a];
+            v += t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+        }
+        v /= 3.0;
+        mx = fmax(mx, v);
+        sm += v;
+    }
+    smax[threadIdx.x] = mx;
+    ssum[threadIdx.x] = sm;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+            ssum[threadIdx.x] += ssum[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out2[2 * p] = smax[0]; out2[2 * p + 1] = ssum[0]; }
+}
+
+// one workgroup per (member, column) pair: the member's DOF range of the
+// column divided by its max |component|, written contiguously at dst_off
+__global__ void __launch_bounds__(256) seg_normalize_kernel(const int32_t *pairs, const int64_t *dof_off,
+                                                            const int64_t *dst_off, int64_t ldx, const double *X,
+                                                            double *dst) {
+    __shared__ double smax[256];
+    const int p = blockIdx.x, mb = pairs[2 * p], col = pairs[2 * p + 1];
+    const double *x = X + (int64_t)col * ldx;
+    const int64_t i0 = dof_off[mb], i1 = dof_off[mb + 1];
+    double mx = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) mx = fmax(mx, fabs(x[i]));
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double m = smax[0];
+    double *d = dst + dst_off[p];
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) d[i - i0] = (m > 1e-30) ? x[i] / m : x[i];
+}
+
+}  // namespace
+
+void seg_gemm_tn(const SegTable &T, int P, int Q, const double *V, int ldv, const double *W, int ldw, double *out,
+                 int64_t so, int ldo, double alpha, double beta, double *part, hipStream_t s) {
+    if (P <= 0 || Q <= 0 || T.n_seg == 0) return;
+    const bool small = P <= 16 && Q <= 16;
+    if (small) seg_tn_small_kernel<<<T.n_chs, 256, 0, s>>>(T, P, Q, V, ldv, W, ldw, part);
+    else seg_tn_partial_kernel<<<dim3(T.n_ch, (P + PC - 1) / PC), 256, 0, s>>>(T, P, Q, V, ldv, W, ldw, part);
+    seg_tn_reduce_kernel<<<dim3(nblk((int64_t)P * Q, 16), T.n_seg), 256, 0, s>>>(T, small, P, Q, part, out, so, ldo,
+                                                                                   alpha, beta);
+}
+
+int64_t seg_gemm_tn_part_size(int n_ch, int n_chs, int P, int Q) {
+    return std::max<int64_t>((int64_t)n_ch * P * Q, (int64_t)n_chs * 16 * 16);
+}
+
+void seg_gemm_nn(const SegTable &T, int P, int Q, const double *V, int ldv, const double *H, int64_t sh, int ldh,
+                 double *Y, int ldy, double alpha, double beta, hipStream_t s) {
+    if (T.n_chs == 0 || Q <= 0) return;
+    seg_nn_kernel<<<T.n_chs, 256, 0, s>>>(T, P, Q, V, ldv, H, sh, ldh, Y, ldy, alpha, beta);
+}
+
+void seg_chol_upper(const SegTable &T, int b, double *G, int64_t sg, int ldg, const double *g0, int64_t sg0,
+                    int ldg0, double tol2, int *err, hipStream_t s) {
+    if (T.n_seg) seg_chol_kernel<<<T.n_seg, 64, 0, s>>>(T, b, G, sg, ldg, g0, sg0, ldg0, tol2, err);
+}
+
+void seg_trsm_right_upper(const SegTable &T, int b, double *W, int ldw, const double *R, int64_t sr, int ldr,
+                          hipStream_t s) {
+    if (T.n_chs) seg_trsm_kernel<<<T.n_chs, 256, 0, s>>>(T, b, W, ldw, R, sr, ldr);
+}
+
+void seg_upper_mul(const SegTable &T, int b, const double *R2, int64_t s2, int ld2, const double *R1, int64_t s1,
+                   int ld1, double *out, int64_t so, int ldo, hipStream_t s) {
+    if (T.n_seg) seg_upper_mul_kernel<<<T.n_seg, 256, 0, s>>>(T, b, R2, s2, ld2, R1, s1, ld1, out, so, ldo);
+}
+
+void seg_start_block(const SegTable &T, int b, double *V, int ldv, int seed, hipStream_t s) {
+    if (T.n_chs) seg_start_block_kernel<<<dim3(T.n_chs, b), 256, 0, s>>>(T, V, ldv, seed);
+}
+
+void seg_fill(const SegTable &T, int cols, double *V, int ldv, double v, hipStream_t s) {
+    if (T.n_chs && cols > 0) seg_fill_kernel<<<dim3(T.n_chs, cols), 256, 0, s>>>(T, V, ldv, v);
+}
+
+void seg_symmetrize_copy(const SegTable &T, int m, int ldT, const double *Tm, int64_t sT, double *A, int64_t sA,
+                         hipStream_t s) {
+    if (T.n_seg) seg_symmetrize_kernel<<<dim3(nblk((int64_t)m * m, 256), T.n_seg), 256, 0, s>>>(T, m, ldT, Tm, sT, A, sA);
+}
+
+void seg_mat_add(const SegTable &T, int rows, int cols, const double *src, int64_t ss, int lds, double *dst,
+                 int64_t sd, int ldd, hipStream_t s) {
+    if (T.n_seg && rows * cols > 0)
+        seg_mat_add_kernel<<<dim3(nblk((int64_t)rows * cols, 256), T.n_seg), 256, 0, s>>>(T, rows, cols, src, ss, lds,
+                                                                                           dst, sd, ldd, true);
+}
+
+void seg_copy(const SegTable &T, int rows, int cols, const double *src, int64_t ss, int lds, double *dst, int64_t sd,
+              int ldd, hipStream_t s) {
+    if (T.n_seg && rows * cols > 0)
+        seg_mat_add_kernel<<<dim3(nblk((int64_t)rows * cols, 256), T.n_seg), 256, 0, s>>>(T, rows, cols, src, ss, lds,
+                                                                                           dst, sd, ldd, false);
+}
+
+void seg_energy_share(int n_pairs, const int32_t *pairs, const int32_t *q4_off, const int32_t *q4,
+                      const int32_t *t3_off, const int32_t *t3, int64_t ldx, const double *X, double *out2,
+                      hipStream_t s) {
+    if (n_pairs > 0) seg_energy_kernel<<<n_pairs, 256, 0, s>>>(pairs, q4_off, q4, t3_off, t3, ldx, X, out2);
+}
+
+void seg_normalize(int n_pairs, const int32_t *pairs, const int64_t *dof_off, const int64_t *dst_off, int64_t ldx,
+                   const double *X, double *dst, hipStream_t s) {
+    if (n_pairs > 0) seg_normalize_kernel<<<n_pairs, 256, 0, s>>>(pairs, dof_off, dst_off, ldx, X, dst);
+}

@@ -1,0 +1,130 @@
+// internal.h -- shared internals of the C ABI implementation: the context
+// and model structures, device buffers and the host-side helpers that both
+// the single-model entry points (api.cpp) and the batched entry points
+// (batch.cpp) use.  Not part of the public interface.
+#pragma once
+#include "../../include/sol105.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "dense.h"
+#include "factor.h"
+#include "solve.h"
+#include "symbolic.h"
+
+struct sol105_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    double timings[8] = {0};
+    double stats[8] = {0};
+    hipEvent_t ev[9] = {nullptr};
+    bool profiling = false;
+    ProdTimer ptimer;  // HIP-event timing of the block triangular-solve products
+    ProdTimer *timer() { return profiling ? &ptimer : nullptr; }
+};
+
+namespace s105 {
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void free_() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { free_(); }
+    hipError_t alloc(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        free_();
+        n = count;
+        return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+    }
+    hipError_t upload(const T *h, size_t count, hipStream_t s) {
+        hipError_t e = alloc(count);
+        if (e != hipSuccess) return e;
+        if (count == 0) return hipSuccess;
+        return hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s);
+    }
+};
+
+}  // namespace s105
+
+struct sol105_model {
+    // host copies
+    int n_nodes = 0, n_q4 = 0, n_t3 = 0, n_bar = 0;
+    std::vector<double> xyz, q4prop, t3prop, bar_vref, barprop, force;
+    std::vector<int32_t> q4, t3, bar;
+    std::vector<uint8_t> spc_mask, has_shell, has_bar;
+    double k6rot = 100.0;
+    int autospc = 1, nd = 3;
+    BlockPattern pat;
+    std::vector<int32_t> diag_blk;
+    // device: model + assembly
+    s105::DevBuf<double> d_xyz, d_q4prop, d_t3prop, d_bar_vref, d_barprop, d_force;
+    s105::DevBuf<int32_t> d_q4, d_t3, d_bar, d_rp, d_col, d_cptr, d_cidx, d_diag;
+    s105::DevBuf<double> d_eb, d_K, d_KG, d_u, d_kdiag;
+    // factor
+    bool have_sym = false;
+    std::vector<uint8_t> free_mask;
+    Symbolic sym;
+    FactorPlan plan;
+    s105::DevBuf<int32_t> f_ns, f_nu, f_m, f_piv0, f_rows, f_rel, f_cptr, f_cidx, f_flist, f_ksrc, d_elim_of_dof,
+        d_dof_of_elim;
+    s105::DevBuf<int64_t> f_off, f_rowoff, f_vecoff, f_reloff, f_kdst;
+    s105::DevBuf<uint8_t> d_free, d_flag;
+    s105::DevBuf<double> d_front, d_piv, d_W, d_kdiag_elim;
+    s105::DevBuf<int> d_err;
+    DevFronts dev;
+    SolvePlan splan;
+    s105::DevBuf<int64_t> s_poff;
+    s105::DevBuf<int32_t> s_ftiles, s_btiles, s_ftbase, s_btbase, s_fnks, s_bnks, s_fkw, s_bkw, s_pullptr, s_pullsrc, s_lblk;
+    s105::DevBuf<double> d_P, d_PT, d_part;
+    DevSolve dsolve;
+    // state
+    bool K_valid = false, factor_valid = false, u_valid = false, KG_valid = false;
+    // eigen results
+    std::vector<double> lambda;
+    s105::DevBuf<double> d_modes;  // ndof x n_modes (normalised, global frame)
+    int n_modes = 0;
+    double lanczos_m = 0, lanczos_steps = 0;
+    int warm_m = 0, warm_b = 0;  // Lanczos size at the last convergence (first check there next time)
+    // a batch model is the block-diagonal union of independent members; their
+    // node ranges (n_members + 1 offsets; empty for a single model) scope the
+    // per-model AUTOSPC maxima
+    std::vector<int32_t> mem_node_off;
+    int ndof() const { return 6 * n_nodes; }
+};
+
+namespace s105 {
+
+int set_err(sol105_ctx *ctx, int code, const char *fmt, ...);
+
+#define HIPCHK(call)                                                                                      \
+    do {                                                                                                  \
+        hipError_t e_ = (call);                                                                           \
+        if (e_ != hipSuccess) return s105::set_err(ctx, SOL105_E_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+ElemArgs elem_args(sol105_model *M);
+// AUTOSPC on the diagonal, per member of a batch model
+void autospc_mask(const sol105_model *M, const std::vector<double> &kd, std::vector<uint8_t> &free_out);
+int upload_symbolic(sol105_ctx *ctx, sol105_model *M);
+// x <- K_ff^-1 x (elimination order, nrhs columns)
+void tri_solve(sol105_model *M, double *X, int nrhs, hipStream_t s);
+double ev_ms(hipEvent_t a, hipEvent_t b);
+// host copies, validation, block pattern and device upload of a model
+int model_init(sol105_ctx *ctx, const sol105_model_desc *d, sol105_model *M);
+// AUTOSPC + symbolic (when the free set changed) + numeric factorisation with
+// the pivot-ratio refix
+int factorize(sol105_ctx *ctx, sol105_model *M);
+
+}  // namespace s105

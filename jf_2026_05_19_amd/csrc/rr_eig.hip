@@ -74,9 +74,24 @@ __device__ __forceinline__ double hash_unit(uint64_t k) {
 }
 
 // RPL: rows per lane of a column (m <= 64 RPL + 1); CPI: columns per wavefront round
+// Batched launches: problem list[blockIdx.x] at the given strides (list ==
+// nullptr: the single problem at the base pointers).
+struct SymeigBatch {
+    const int *list;
+    int64_t sA, sT, sZ, sW;
+};
+
 template <int RPL, int CPI>
 __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k, double *theta, double *Zs,
-                                                        double *work, int *info) {
+                                                        double *work, int *info, SymeigBatch bt) {
+    if (bt.list) {
+        const int64_t p = bt.list[blockIdx.x];
+        A += p * bt.sA;
+        theta += p * bt.sT;
+        Zs += p * bt.sZ;
+        work += p * bt.sW;
+        info += p;
+    }
     __shared__ double vsh[NT];
     __shared__ double psh[NT];
     __shared__ double dd[NT], ee[NT], e2[NT];
@@ -428,8 +443,19 @@ int64_t symeig_work_size(int m, int k) {
 }
 
 void sym_eig_top(int m, double *A, int k, double *theta, double *Zs, double *work, int *info, hipStream_t s) {
+    const SymeigBatch one{nullptr, 0, 0, 0, 0};
     if (m <= 385)
-        symeig_top_kernel<6, 4><<<1, NT, 0, s>>>(m, A, k, theta, Zs, work, info);
+        symeig_top_kernel<6, 4><<<1, NT, 0, s>>>(m, A, k, theta, Zs, work, info, one);
     else
-        symeig_top_kernel<16, 1><<<1, NT, 0, s>>>(m, A, k, theta, Zs, work, info);
+        symeig_top_kernel<16, 1><<<1, NT, 0, s>>>(m, A, k, theta, Zs, work, info, one);
+}
+
+void sym_eig_top_batched(int m, double *A, int64_t sA, int k, double *theta, int64_t sT, double *Zs, int64_t sZ,
+                         double *work, int64_t sW, int *info, const int *list, int nlist, hipStream_t s) {
+    if (nlist <= 0) return;
+    const SymeigBatch bt{list, sA, sT, sZ, sW};
+    if (m <= 385)
+        symeig_top_kernel<6, 4><<<nlist, NT, 0, s>>>(m, A, k, theta, Zs, work, info, bt);
+    else
+        symeig_top_kernel<16, 1><<<nlist, NT, 0, s>>>(m, A, k, theta, Zs, work, info, bt);
 }
