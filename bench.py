@@ -9,6 +9,10 @@ solve -> KG assembly -> block-Lanczos buckling eigensolve (10 modes, eigen-
 vectors, reference filters).  The model arrays are resident in HBM before
 the timed region (deck parse and upload excluded, reported separately).
 
+The line also carries the batched-vs-plain comparison of the metric
+("batched"): a sweep of 256 seeded plate variants solved one model at a time
+and as one stacked batch.
+
 Prints ONE JSON line (rank 0).  --gpus N with N > 1 exits: this build runs
 on one GPU.
 """
@@ -39,6 +43,9 @@ def main():
     ap.add_argument("--ny", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-only", action="store_true", help="only time the CPU oracle sample")
+    ap.add_argument("--batch-variants", type=int, default=256,
+                    help="seeded plate variants of the batched-vs-plain comparison (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -108,6 +115,9 @@ def main():
     })
     result["roofline"] = roofline(prof)
     result["factor_rate"] = factor_rate(ctx, stats)
+    dm.free()
+    if args.batch_variants > 0:
+        result["batched"] = batched_sweep(ctx, args.batch_variants, args.batch_steps)
     if cpu is not None:
         result["gpu_over_cpu"] = result["value"] / cpu["value"]
     print(json.dumps(result))
@@ -147,6 +157,78 @@ def factor_rate(ctx, stats):
     flops = stats[2]
     return {"tflops": flops / (fact_ms * 1e-3) / 1e12 if fact_ms > 0 else 0.0, "factor_flops": flops,
             "phase_ms": fact_ms}
+
+
+def _parse_variant(txt):
+    from jf_2026_05_19_amd import bdf, model as M
+    return M.build_arrays(bdf.bdf_to_model_text(txt))
+
+
+def batched_sweep(ctx, n, steps):
+    """Design sweep of n seeded plate variants (mesh 10x10..80x80, t in
+    U(1, 4) mm, E in {70e3, 210e3} MPa, ND = 5): every variant solved one
+    after another (plain: one model per solve) and all of them as one batch
+    (sol105_batch_solve).  Both keep the models resident and re-use their
+    symbolic analysis; the first solve of each is an untimed warmup."""
+    from concurrent.futures import ProcessPoolExecutor
+    from jf_2026_05_19_amd import decks
+    from jf_2026_05_19_amd import _native as N
+    t0 = time.perf_counter()
+    texts = [t for t, _ in decks.variant_sweep(n=n)]
+    with ProcessPoolExecutor(max_workers=8) as ex:
+        arrs = list(ex.map(_parse_variant, texts, chunksize=8))
+    ndofs = [a.ndof for a in arrs]
+    log("batched sweep: %d variants, %d..%d DOF, %d total (parse %.1fs)" % (n, min(ndofs), max(ndofs), sum(ndofs),
+                                                                            time.perf_counter() - t0))
+    # plain: one solve per variant
+    models, lam_plain = [], []
+    for i, a in enumerate(arrs):
+        models.append(N.DeviceModel(ctx, a))
+        lam_plain.append(models[-1].solve(16, want_modes=False)[0])   # warmup (symbolic analysis)
+        if (i + 1) % 64 == 0:
+            log("plain warmup: %d/%d models" % (i + 1, n))
+    lat = []
+    t1 = time.perf_counter()
+    for m in models:
+        ts = time.perf_counter()
+        m.solve(16, want_modes=False)
+        lat.append(time.perf_counter() - ts)
+    t_plain = time.perf_counter() - t1
+    for m in models:
+        m.free()
+    del models
+    log("plain: %d solves in %.3fs" % (n, t_plain))
+    # batched: all variants in one stacked solve
+    B = N.Batch(ctx, arrs)
+    t2 = time.perf_counter()
+    lam_b, st = B.solve(16)
+    log("batch warmup (incl. symbolic analysis): %.2fs" % (time.perf_counter() - t2))
+    bt = []
+    for k in range(steps):
+        ts = time.perf_counter()
+        lam_b, st = B.solve(16)
+        bt.append(time.perf_counter() - ts)
+    t_batch = float(np.mean(bt))
+    stats = B.stats()
+    B.free()
+    err = 0.0
+    for la, lb in zip(lam_plain, lam_b):
+        if len(la) != len(lb):
+            err = float("inf")
+            break
+        if len(la):
+            err = max(err, float(np.max(np.abs(lb - la) / np.abs(la))))
+    lat_ms = np.array(lat) * 1e3
+    log("batched: %d solves in %.3fs (%.1fx plain), max rel err vs plain %.2e" % (n, t_batch, t_plain / t_batch, err))
+    return {"workload": "%d seeded plate variants 500x500 mm, mesh 10x10..80x80 (%d..%d DOF, %d total), ND=5, "
+                        "t U(1,4) mm, E in {70e3, 210e3} MPa, FP64" % (n, min(ndofs), max(ndofs), sum(ndofs)),
+            "batched_solves_per_s": n / t_batch, "plain_solves_per_s": n / t_plain,
+            "speedup_vs_plain": t_plain / t_batch, "batch_ms": 1e3 * t_batch, "batch_steps": steps,
+            "plain_p50_ms": float(np.percentile(lat_ms, 50)), "plain_p99_ms": float(np.percentile(lat_ms, 99)),
+            "batched_latency_ms": 1e3 * t_batch,
+            "max_eig_rel_err_vs_plain": err, "members_ok": int(np.sum(st == 0)),
+            "members_resolved_alone": int(stats["fallbacks"]), "stacked_free_dofs": int(stats["n_free"]),
+            "lanczos_vectors": int(stats["lanczos_vectors"])}
 
 
 def cpu_baseline(arr):

@@ -112,17 +112,19 @@ class Sol105Arrays:
 
 
 def _flat_dev(p):
-    """max out-of-plane corner deviation / diagonal length, assembly.jl:2815-2826."""
-    d13 = p[2] - p[0]
-    d24 = p[3] - p[1]
+    """Max out-of-plane corner deviation and the longer diagonal of each quad
+    (p: E x 4 x 3), assembly.jl:2815-2826; zero deviation where the diagonals
+    are parallel."""
+    d13 = p[:, 2] - p[:, 0]
+    d24 = p[:, 3] - p[:, 1]
     n = np.cross(d13, d24)
-    ln = np.linalg.norm(n)
-    c = p.mean(axis=0)
-    L = max(np.linalg.norm(d13), np.linalg.norm(d24))
-    if ln <= 1e-12:
-        return 0.0, L
-    n = n / ln
-    return float(np.max(np.abs((p - c) @ n))), L
+    ln = np.linalg.norm(n, axis=1)
+    c = p.mean(axis=1)
+    L = np.maximum(np.linalg.norm(d13, axis=1), np.linalg.norm(d24, axis=1))
+    ok = ln > 1e-12
+    nn = np.where(ok[:, None], n / np.where(ok, ln, 1.0)[:, None], 0.0)
+    dev = np.abs(np.einsum("eki,ei->ek", p - c[:, None, :], nn)).max(axis=1)
+    return np.where(ok, dev, 0.0), L
 
 
 def _subcase_pairing(cc):
@@ -192,10 +194,6 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
             t3pid.append(int(el["PID"]))
             t3prop.append(row)
             continue
-        p = xyz[[rank[v] for v in nodes]]
-        dev, L = _flat_dev(p)
-        if dev >= 1e-6 * max(L, 1e-12):
-            raise UnsupportedModel(f"element {el['ID']}: warped CQUAD4 (curved-shell gates) not supported")
         q4.append([rank[v] for v in nodes])
         q4eid.append(int(el["ID"]))
         q4pid.append(int(el["PID"]))
@@ -284,6 +282,11 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
     if isinstance(snorm, (int, float)) and snorm > 0:
         raise UnsupportedModel("PARAM,SNORM > 0 is not supported")
     q4a = np.asarray(q4, dtype=np.int32).reshape(-1, 4)
+    if len(q4a):
+        dev, L = _flat_dev(xyz[q4a])
+        warped = np.flatnonzero(dev >= 1e-6 * np.maximum(L, 1e-12))
+        if len(warped):
+            raise UnsupportedModel(f"element {q4eid[warped[0]]}: warped CQUAD4 (curved-shell gates) not supported")
     t3a = np.asarray(t3, dtype=np.int32).reshape(-1, 3)
     bara = np.asarray(bars, dtype=np.int32).reshape(-1, 2)
     has_shell = np.zeros(n, dtype=np.uint8)
