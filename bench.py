@@ -46,6 +46,8 @@ def main():
     ap.add_argument("--batch-variants", type=int, default=256,
                     help="seeded plate variants of the batched-vs-plain comparison (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--batch-only", action="store_true", help="only the batched-vs-plain sweep (for profiling)")
+    ap.add_argument("--no-batch-plain", action="store_true", help="skip the one-at-a-time leg of the sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -53,6 +55,12 @@ def main():
 
     from jf_2026_05_19_amd import bdf, decks, model as M
 
+    if args.batch_only:
+        from jf_2026_05_19_amd import _native as N
+        print(json.dumps({"metric": METRIC, "batched": batched_sweep(N.Context(0), args.batch_variants,
+                                                                     args.batch_steps,
+                                                                     plain=not args.no_batch_plain)}))
+        return
     t0 = time.perf_counter()
     txt = decks.panel_deck(nx=args.nx, ny=args.ny)
     arr = M.build_arrays(bdf.bdf_to_model_text(txt))
@@ -164,7 +172,7 @@ def _parse_variant(txt):
     return M.build_arrays(bdf.bdf_to_model_text(txt))
 
 
-def batched_sweep(ctx, n, steps):
+def batched_sweep(ctx, n, steps, plain=True):
     """Design sweep of n seeded plate variants (mesh 10x10..80x80, t in
     U(1, 4) mm, E in {70e3, 210e3} MPa, ND = 5): every variant solved one
     after another (plain: one model per solve) and all of them as one batch
@@ -181,23 +189,25 @@ def batched_sweep(ctx, n, steps):
     log("batched sweep: %d variants, %d..%d DOF, %d total (parse %.1fs)" % (n, min(ndofs), max(ndofs), sum(ndofs),
                                                                             time.perf_counter() - t0))
     # plain: one solve per variant
-    models, lam_plain = [], []
-    for i, a in enumerate(arrs):
-        models.append(N.DeviceModel(ctx, a))
-        lam_plain.append(models[-1].solve(16, want_modes=False)[0])   # warmup (symbolic analysis)
-        if (i + 1) % 64 == 0:
-            log("plain warmup: %d/%d models" % (i + 1, n))
-    lat = []
-    t1 = time.perf_counter()
-    for m in models:
-        ts = time.perf_counter()
-        m.solve(16, want_modes=False)
-        lat.append(time.perf_counter() - ts)
-    t_plain = time.perf_counter() - t1
-    for m in models:
-        m.free()
-    del models
-    log("plain: %d solves in %.3fs" % (n, t_plain))
+    lat, lam_plain, t_plain = [float("nan")], None, float("nan")
+    if plain:
+        models, lam_plain = [], []
+        for i, a in enumerate(arrs):
+            models.append(N.DeviceModel(ctx, a))
+            lam_plain.append(models[-1].solve(16, want_modes=False)[0])   # warmup (symbolic analysis)
+            if (i + 1) % 64 == 0:
+                log("plain warmup: %d/%d models" % (i + 1, n))
+        lat = []
+        t1 = time.perf_counter()
+        for m in models:
+            ts = time.perf_counter()
+            m.solve(16, want_modes=False)
+            lat.append(time.perf_counter() - ts)
+        t_plain = time.perf_counter() - t1
+        for m in models:
+            m.free()
+        del models
+        log("plain: %d solves in %.3fs" % (n, t_plain))
     # batched: all variants in one stacked solve
     B = N.Batch(ctx, arrs)
     t2 = time.perf_counter()
@@ -211,8 +221,8 @@ def batched_sweep(ctx, n, steps):
     t_batch = float(np.mean(bt))
     stats = B.stats()
     B.free()
-    err = 0.0
-    for la, lb in zip(lam_plain, lam_b):
+    err = 0.0 if plain else float("nan")
+    for la, lb in zip(lam_plain or [], lam_b):
         if len(la) != len(lb):
             err = float("inf")
             break
