@@ -111,16 +111,17 @@ int sol105_solve(sol105_ctx *ctx, sol105_model *model, int32_t max_modes,
                  double *lambda_out, double *modes_out, int32_t *n_out);
 
 /* Per-element thickness sensitivity d(lambda_m)/dt_e of buckling mode m
- * (0-based, after sol105_buckling) for every CQUAD4, including the prestress
- * (adjoint) term.  Replaces solve_adjoint_buckling with one design variable
- * per element (JFEM/src/solver/adjoint_buckling.jl:2406-2725). */
+ * (0-based, after sol105_buckling) for every shell (CQUAD4 first, then
+ * CTRIA3), including the prestress (adjoint) term.  Replaces
+ * solve_adjoint_buckling with one design variable per element
+ * (JFEM/src/solver/adjoint_buckling.jl:2406-2725). */
 int sol105_sensitivity_thickness(sol105_ctx *ctx, sol105_model *model, int32_t mode,
-                                 double *dlam_dt /* n_q4 */);
+                                 double *dlam_dt /* n_q4 + n_t3 */);
 
 /* The terms of d(lambda_m)/dt_e separately, for checking the adjoint formula
  * term by term (JFEM/src/solver/adjoint_buckling.jl:2611):
- *   terms (n_q4 x 3, may be NULL) = [phi' dKe/dt phi, phi' KGe phi / t,
- *                                    psi' dKe/dt u] per CQUAD4,
+ *   terms ((n_q4 + n_t3) x 3, may be NULL) = [phi' dKe/dt phi,
+ *                 phi' KGe phi / t, psi' dKe/dt u] per shell (CQUAD4 first),
  *   adjoint_load (6*n_nodes, may be NULL) = lambda d(phi' KG(u) phi)/du on
  *                                    the free DOFs (zero elsewhere),
  *   *phi_kg_phi (may be NULL) = phi' KG phi,

@@ -246,13 +246,14 @@ class DeviceModel:
         return lam[:k].copy(), (modes[:k].T.copy() if want_modes else None)
 
     def sensitivity_thickness(self, mode=0):
-        out = np.zeros(self.arr.n_q4)
+        """d(lambda_mode)/dt of every shell (CQUAD4 first, then CTRIA3)."""
+        out = np.zeros(self.arr.n_q4 + self.arr.n_t3)
         self.ctx.check(self.ctx.L.sol105_sensitivity_thickness(self.ctx.h, self.h, int(mode), _dp(out)))
         return out
 
     def sensitivity_terms(self, mode=0):
-        """(terms n_q4 x 3, adjoint load ndof, phi' KG phi) of d(lambda)/dt."""
-        terms = np.zeros((self.arr.n_q4, 3))
+        """(terms (n_q4 + n_t3) x 3, adjoint load ndof, phi' KG phi) of d(lambda)/dt."""
+        terms = np.zeros((self.arr.n_q4 + self.arr.n_t3, 3))
         g = np.zeros(self.arr.ndof)
         pkp = np.zeros(1)
         self.ctx.check(self.ctx.L.sol105_sensitivity_terms(self.ctx.h, self.h, int(mode), _dp(terms), _dp(g),

@@ -879,16 +879,16 @@ int sens_terms(sol105_ctx *ctx, sol105_model *M, int32_t mode, DevBuf<double> &t
                double &pkp) {
     if (!ctx || !M) return set_err(ctx, SOL105_E_INVALID, "null argument");
     if (mode < 0 || mode >= M->n_modes) return set_err(ctx, SOL105_E_STATE, "mode %d not available", mode);
-    if (M->n_t3 > 0)
-        return set_err(ctx, SOL105_E_UNSUPPORTED, "thickness sensitivities of models with CTRIA3 are not supported");
     (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
     int ndof = M->ndof(), n = M->sym.n_free;
     const double *phi = M->d_modes.p + (size_t)mode * ndof;
     double lam = M->lambda[mode];
     ElemArgs a = elem_args(M);
-    DevBuf<double> egq, egb, ge, psi, kgphi, scr, dots, phif;
+    const int n_shell = M->n_q4 + M->n_t3;
+    DevBuf<double> egq, egt, egb, ge, psi, kgphi, scr, dots, phif;
     HIPCHK(egq.alloc((size_t)std::max(M->n_q4, 1) * 24));
+    HIPCHK(egt.alloc((size_t)std::max(M->n_t3, 1) * 18));
     HIPCHK(egb.alloc((size_t)std::max(M->n_bar, 1) * 12));
     HIPCHK(g.alloc(ndof));
     HIPCHK(ge.alloc(std::max(n, 1)));
@@ -897,10 +897,12 @@ int sens_terms(sol105_ctx *ctx, sol105_model *M, int32_t mode, DevBuf<double> &t
     HIPCHK(scr.alloc(512));
     HIPCHK(dots.alloc(2));
     HIPCHK(phif.alloc(ndof));
-    HIPCHK(terms.alloc((size_t)std::max(M->n_q4, 1) * 3));
+    HIPCHK(terms.alloc((size_t)std::max(n_shell, 1) * 3));
     launch_q4_adjoint_rhs(a, M->d_u.p, phi, lam, egq.p, s);
+    launch_t3_kg_sens(a, M->d_u.p, phi, lam, terms.p, egt.p, s);
     launch_bar_adjoint_rhs(a, phi, lam, egb.p, s);
-    node_gather_vec(M->n_nodes, M->n_q4, M->d_diag.p, M->d_cptr.p, M->d_cidx.p, egq.p, egb.p, M->d_free.p, g.p, s);
+    node_gather_vec(M->n_nodes, M->n_q4, M->n_t3, M->d_diag.p, M->d_cptr.p, M->d_cidx.p, egq.p, egt.p, egb.p,
+                    M->d_free.p, g.p, s);
     launch_gather_elim(n, ndof, 1, M->d_dof_of_elim.p, g.p, ge.p, s);
     tri_solve(M, ge.p, 1, s);
     launch_scatter_elim(n, ndof, 1, M->d_elim_of_dof.p, ge.p, psi.p, s);
@@ -910,6 +912,7 @@ int sens_terms(sol105_ctx *ctx, sol105_model *M, int32_t mode, DevBuf<double> &t
     launch_bsr_spmv(M->n_nodes, M->d_rp.p, M->d_col.p, M->d_KG.p, phif.p, kgphi.p, 1, 1.0, s);
     dot_det(ndof, phif.p, kgphi.p, scr.p, dots.p, s);
     launch_q4_sens_terms(a, M->d_u.p, phif.p, psi.p, terms.p, s);
+    launch_t3_sens_terms(a, M->d_u.p, phif.p, psi.p, terms.p, s);
     HIPCHK(hipMemcpyAsync(&pkp, dots.p, sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipGetLastError());
@@ -927,9 +930,10 @@ int sol105_sensitivity_thickness(sol105_ctx *ctx, sol105_model *M, int32_t mode,
     int rc = sens_terms(ctx, M, mode, terms, g, pkp);
     if (rc) return rc;
     hipStream_t s = ctx->stream;
-    HIPCHK(dl.alloc(std::max(M->n_q4, 1)));
-    launch_sens_combine(M->n_q4, terms.p, M->lambda[mode], pkp, dl.p, s);
-    HIPCHK(hipMemcpyAsync(dlam_dt, dl.p, sizeof(double) * M->n_q4, hipMemcpyDeviceToHost, s));
+    const int n_shell = M->n_q4 + M->n_t3;
+    HIPCHK(dl.alloc(std::max(n_shell, 1)));
+    launch_sens_combine(n_shell, terms.p, M->lambda[mode], pkp, dl.p, s);
+    HIPCHK(hipMemcpyAsync(dlam_dt, dl.p, sizeof(double) * n_shell, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return SOL105_OK;
 }
@@ -941,7 +945,8 @@ int sol105_sensitivity_terms(sol105_ctx *ctx, sol105_model *M, int32_t mode, dou
     int rc = sens_terms(ctx, M, mode, terms, g, pkp);
     if (rc) return rc;
     hipStream_t s = ctx->stream;
-    if (terms_out) HIPCHK(hipMemcpyAsync(terms_out, terms.p, sizeof(double) * 3 * M->n_q4, hipMemcpyDeviceToHost, s));
+    if (terms_out)
+        HIPCHK(hipMemcpyAsync(terms_out, terms.p, sizeof(double) * 3 * (M->n_q4 + M->n_t3), hipMemcpyDeviceToHost, s));
     if (adjoint_load) HIPCHK(hipMemcpyAsync(adjoint_load, g.p, sizeof(double) * M->ndof(), hipMemcpyDeviceToHost, s));
     if (phi_kg_phi) *phi_kg_phi = pkp;
     HIPCHK(hipStreamSynchronize(s));

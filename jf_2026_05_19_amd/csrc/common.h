@@ -48,3 +48,9 @@ void launch_q4_adjoint_rhs(const ElemArgs &a, const double *u, const double *phi
                            double *eg /* n_q4 x 24 */, hipStream_t s);
 void launch_bar_adjoint_rhs(const ElemArgs &a, const double *phi, double lam, double *eg /* n_bar x 12 */,
                             hipStream_t s);
+// CTRIA3 rows n_q4 + e of terms: [0], [2] (launch_t3_sens_terms) and [1]
+// with the element adjoint loads eg (n_t3 x 18) (launch_t3_kg_sens)
+void launch_t3_sens_terms(const ElemArgs &a, const double *u, const double *phi, const double *psi, double *terms,
+                          hipStream_t s);
+void launch_t3_kg_sens(const ElemArgs &a, const double *u, const double *phi, double lam, double *terms, double *eg,
+                       hipStream_t s);

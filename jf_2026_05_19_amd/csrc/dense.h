@@ -28,8 +28,12 @@ void elem_energy_share(int n_q4, const int32_t *q4, int n_t3, const int32_t *t3,
 void normalize_maxabs(int64_t n, double *x, double *scratch, hipStream_t s);
 void symmetrize_copy(int m, int ld, const double *T, double *A, hipStream_t s);
 void dot_det(int64_t n, const double *x, const double *y, double *scratch, double *out, hipStream_t s);
-void node_gather_vec(int n_nodes, int n_q4, const int32_t *diag_blk, const int32_t *cptr, const int32_t *cidx,
-                     const double *eg_q4, const double *eg_bar, const uint8_t *free_mask, double *g, hipStream_t s);
+// g[6v + c] = sum of the element vectors (CQUAD4 24, CTRIA3 18, CBAR 12 per
+// element) of the elements at node v, in the block gather's element order;
+// zero on fixed DOFs
+void node_gather_vec(int n_nodes, int n_q4, int n_t3, const int32_t *diag_blk, const int32_t *cptr,
+                     const int32_t *cidx, const double *eg_q4, const double *eg_t3, const double *eg_bar,
+                     const uint8_t *free_mask, double *g, hipStream_t s);
 void extract_kdiag(int n_nodes, const int32_t *diag_blk, const double *kv, double *out, hipStream_t s);
 // dst[i + j*ldd] += src[i + j*lds]
 void mat_add(int rows, int cols, const double *src, int lds, double *dst, int ldd, hipStream_t s);

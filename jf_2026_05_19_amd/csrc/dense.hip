@@ -497,23 +497,28 @@ __global__ void dot_final_kernel(int np, const double *part, double *out) {
 }
 
 // g[6v + c] = sum over the node's incident element vectors (fixed order)
-__global__ void node_gather_kernel(int n_nodes, int n_q4, const int32_t *diag_blk, const int32_t *cptr,
-                                   const int32_t *cidx, const double *eg_q4, const double *eg_bar,
-                                   const uint8_t *free_mask, double *g) {
+__global__ void node_gather_kernel(int n_nodes, int n_q4, int n_t3, const int32_t *diag_blk, const int32_t *cptr,
+                                   const int32_t *cidx, const double *eg_q4, const double *eg_t3,
+                                   const double *eg_bar, const uint8_t *free_mask, double *g) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_nodes * 6) return;
     int v = t / 6, c = t % 6;
     int d = diag_blk[v];
     double s = 0.0;
-    int64_t q4b = (int64_t)n_q4 * 16;
+    int64_t q4b = (int64_t)n_q4 * 16, t3b = q4b + (int64_t)n_t3 * 9;
     for (int k = cptr[d]; k < cptr[d + 1]; k++) {
         int64_t idx = cidx[k];
         if (idx < q4b) {
             int64_t e = idx / 16;
             int a = (int)((idx % 16) / 4);
             s += eg_q4[e * 24 + a * 6 + c];
-        } else {
+        } else if (idx < t3b) {
             int64_t r = idx - q4b;
+            int64_t e = r / 9;
+            int a = (int)((r % 9) / 3);
+            s += eg_t3[e * 18 + a * 6 + c];
+        } else {
+            int64_t r = idx - t3b;
             int64_t b = r / 4;
             int a = (int)((r % 4) / 2);
             s += eg_bar[b * 12 + a * 6 + c];
@@ -541,9 +546,11 @@ void dot_det(int64_t n, const double *x, const double *y, double *scratch, doubl
     dot_final_kernel<<<1, 64, 0, s>>>(np, scratch, out);
 }
 
-void node_gather_vec(int n_nodes, int n_q4, const int32_t *diag_blk, const int32_t *cptr, const int32_t *cidx,
-                     const double *eg_q4, const double *eg_bar, const uint8_t *free_mask, double *g, hipStream_t s) {
-    node_gather_kernel<<<(n_nodes * 6 + 255) / 256, 256, 0, s>>>(n_nodes, n_q4, diag_blk, cptr, cidx, eg_q4, eg_bar,
+void node_gather_vec(int n_nodes, int n_q4, int n_t3, const int32_t *diag_blk, const int32_t *cptr,
+                     const int32_t *cidx, const double *eg_q4, const double *eg_t3, const double *eg_bar,
+                     const uint8_t *free_mask, double *g, hipStream_t s) {
+    node_gather_kernel<<<(n_nodes * 6 + 255) / 256, 256, 0, s>>>(n_nodes, n_q4, n_t3, diag_blk, cptr, cidx, eg_q4,
+                                                                 eg_t3, eg_bar,
                                                                 free_mask, g);
 }
 

@@ -29,6 +29,12 @@ namespace {
 constexpr double PT = 0.57735026918962576451;  // 1/sqrt(3)
 constexpr int EPB = 4;                         // elements (wavefronts) per workgroup
 
+// sum over the 64 lanes of a wavefront (butterfly, same order on every lane)
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 __device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
     c[0] = a[1] * b[2] - a[2] * b[1];
     c[1] = a[2] * b[0] - a[0] * b[2];
@@ -971,14 +977,40 @@ __device__ double t3_quad_plate_entry(const T3Quad &Q, const ShellProps &P, int 
     return kb;
 }
 
-__global__ void __launch_bounds__(256) t3_ke_kernel(ElemArgs A, double *eb) {
-    __shared__ T3Stage stage[EPB];
-    int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
-    int e = blockIdx.x * EPB + w;
-    T3Stage &S = stage[w];
+// Entry (a, b) of the local 18 x 18 CTRIA3 matrix: (membrane + drilling) *
+// md_scale + the plate block Kp (condensed macro block or its derivative).
+__device__ double t3_local_entry(const T3Stage &S, const ShellProps &P, double alpha, bool nonzero, bool plate,
+                                 const double *Kp, double md_scale, int a, int b) {
+    if (!nonzero) return 0.0;
+    int I = a / 6, J = b / 6, ca = a % 6, cb = b % 6;
+    const double ar = S.area;
+    double v = 0.0;
+    if (ca < 2 && cb < 2) {
+        double ba[3] = {ca == 0 ? S.bv[I] : 0.0, ca == 1 ? S.cv[I] : 0.0, ca == 0 ? S.cv[I] : S.bv[I]};
+        double bb[3] = {cb == 0 ? S.bv[J] : 0.0, cb == 1 ? S.cv[J] : 0.0, cb == 0 ? S.cv[J] : S.bv[J]};
+        double t[3];
+        mat3v(P.Cm, bb, t);
+        v = (ba[0] * t[0] + ba[1] * t[1] + ba[2] * t[2]) * ar;
+    }
+    double da = ca == 0 ? 0.5 * S.cv[I] : (ca == 1 ? -0.5 * S.bv[I] : (ca == 5 ? 1.0 / 3.0 : 0.0));
+    double db = cb == 0 ? 0.5 * S.cv[J] : (cb == 1 ? -0.5 * S.bv[J] : (cb == 5 ? 1.0 / 3.0 : 0.0));
+    v += alpha * (da * db) * ar;
+    v *= md_scale;
+    if (plate && ca >= 2 && ca <= 4 && cb >= 2 && cb <= 4) v += Kp[(3 * I + ca - 2) * 9 + 3 * J + cb - 2];
+    return v;
+}
+
+// Shared set-up of the CTRIA3 kernels (all lanes of the element's wavefront
+// take part; contains the workgroup barriers): frame, area and shape
+// derivatives, the three virtual quads, K21, the LU elimination of Kii with
+// X = Kii^-1 Kie left in M[:, 12:21], and the condensed plate block Kc.
+struct T3Flags {
+    bool valid, nonzero, plate, has_shear;
+};
+
+__device__ T3Flags t3_setup(T3Stage &S, ShellProps &P, double &Gref, const ElemArgs &A, int e, int lane) {
     bool valid = e < A.n_t3;
-    ShellProps P;
-    double Gref = 0.0;
+    Gref = 0.0;
     if (valid) {
         double p[9];
         for (int k = 0; k < 3; k++) {
@@ -1085,34 +1117,28 @@ __global__ void __launch_bounds__(256) t3_ke_kernel(ElemArgs A, double *eb) {
             S.Kc[t] = S.K21[i * 21 + j] - s;
         }
     __syncthreads();
+    return T3Flags{valid, nonzero, plate, has_shear};
+}
+
+__global__ void __launch_bounds__(256) t3_ke_kernel(ElemArgs A, double *eb) {
+    __shared__ T3Stage stage[EPB];
+    int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
+    int e = blockIdx.x * EPB + w;
+    T3Stage &S = stage[w];
+    ShellProps P;
+    double Gref;
+    T3Flags F = t3_setup(S, P, Gref, A, e, lane);
+    const bool valid = F.valid, nonzero = F.nonzero, plate = F.plate;
     // local 18 x 18 by 3 x 3 sub-blocks (lane < 36), rotated to the global frame
     if (valid && lane < 36) {
         int blk = lane / 4, sub = lane % 4;
         int I = blk / 3, J = blk % 3, si = sub / 2, sj = sub % 2;
-        double ek6 = P.bend ? A.k6rot : 0.0;
-        double alpha = (ek6 / 1e5) * Gref * P.h;
-        double ar = S.area;
+        double alpha = ((P.bend ? A.k6rot : 0.0) / 1e5) * Gref * P.h;
         double loc[9], g[9];
         for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                int ca = 3 * si + i, cb = 3 * sj + j;
-                double v = 0.0;
-                if (nonzero) {
-                    if (ca < 2 && cb < 2) {
-                        double ba[3] = {ca == 0 ? S.bv[I] : 0.0, ca == 1 ? S.cv[I] : 0.0, ca == 0 ? S.cv[I] : S.bv[I]};
-                        double bb[3] = {cb == 0 ? S.bv[J] : 0.0, cb == 1 ? S.cv[J] : 0.0, cb == 0 ? S.cv[J] : S.bv[J]};
-                        double t[3];
-                        mat3v(P.Cm, bb, t);
-                        v = (ba[0] * t[0] + ba[1] * t[1] + ba[2] * t[2]) * ar;
-                    } else if (plate && ca >= 2 && ca <= 4 && cb >= 2 && cb <= 4) {
-                        v = S.Kc[(3 * I + ca - 2) * 9 + 3 * J + cb - 2];
-                    }
-                    double da = ca == 0 ? 0.5 * S.cv[I] : (ca == 1 ? -0.5 * S.bv[I] : (ca == 5 ? 1.0 / 3.0 : 0.0));
-                    double db = cb == 0 ? 0.5 * S.cv[J] : (cb == 1 ? -0.5 * S.bv[J] : (cb == 5 ? 1.0 / 3.0 : 0.0));
-                    v += alpha * (da * db) * ar;
-                }
-                loc[i * 3 + j] = v;
-            }
+            for (int j = 0; j < 3; j++)
+                loc[i * 3 + j] = t3_local_entry(S, P, alpha, nonzero, plate, S.Kc, 1.0, 6 * I + 3 * si + i,
+                                                6 * J + 3 * sj + j);
         rot_block(S.R, loc, g);
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) S.out[(I * 3 + J) * 36 + (3 * si + i) * 6 + 3 * sj + j] = g[i * 3 + j];
@@ -1120,6 +1146,159 @@ __global__ void __launch_bounds__(256) t3_ke_kernel(ElemArgs A, double *eb) {
     __syncthreads();
     if (valid)
         for (int t = lane; t < 324; t += S105_WAVE) eb[(size_t)e * 324 + t] = S.out[t];
+}
+
+// Thickness derivative of the plate-block entry (a, b) of a virtual quad:
+// bending scales with t^3, the MacNeal shear block is D' Z^-1 D with
+// Z = Zs + Zb, Zs ~ 1/t and Zb ~ 1/t^3.
+__device__ double t3_quad_plate_dentry(const T3Quad &Q, const ShellProps &P, int a, int b, bool has_shear) {
+    const double t = P.h;
+    int a24 = 6 * (a / 3) + 2 + a % 3, b24 = 6 * (b / 3) + 2 + b % 3;
+    double kb = 0.0;
+    for (int g = 0; g < 4; g++) {
+        double ba[3], bb[3], w[3];
+        bb_col(Q, g, a24, ba);
+        bb_col(Q, g, b24, bb);
+        mat3v(P.Cb, bb, w);
+        kb += Q.gadj[g] * (ba[0] * w[0] + ba[1] * w[1] + ba[2] * w[2]);
+    }
+    double v = 3.0 * kb / t;
+    if (has_shear) {
+        double s = 0.0;
+        for (int l = 0; l < 4; l++)
+            for (int q = 0; q < 4; q++)
+                s += Q.ZiD[l][a] * (Q.Zs[l * 4 + q] / t + 3.0 * Q.Zb[l * 4 + q] / t) * Q.ZiD[q][b];
+        v += s;
+    }
+    return v;
+}
+
+// terms[3 (n_q4 + e)] = phi' dKe/dt phi, terms[3 (n_q4 + e) + 2] = psi' dKe/dt u
+// of CTRIA3 e (global-frame vectors).  Membrane and drilling scale with t;
+// the condensed plate block is differentiated as a Schur complement:
+// dKc = dKee - dKei X - X' dKie + X' dKii X, X = Kii^-1 Kie.
+__global__ void __launch_bounds__(256) t3_dke_kernel(ElemArgs A, const double *u, const double *phi,
+                                                     const double *psi, double *terms) {
+    __shared__ T3Stage stage[EPB];
+    __shared__ double dK[EPB][441];
+    __shared__ double Yd[EPB][108];
+    __shared__ double dKc[EPB][81];
+    __shared__ double vl[EPB][3][18];  // phi, psi, u in the element frame
+    int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
+    int e = blockIdx.x * EPB + w;
+    T3Stage &S = stage[w];
+    ShellProps P;
+    double Gref;
+    T3Flags F = t3_setup(S, P, Gref, A, e, lane);
+    if (F.plate)
+        for (int t = lane; t < 441; t += S105_WAVE) {
+            int i = t / 21, j = t % 21, ni = i / 3, nj = j / 3;
+            double v = 0.0;
+            for (int q = 0; q < 3; q++) {
+                int pa = t3_pos(q, ni), pb = t3_pos(q, nj);
+                if (pa >= 0 && pb >= 0)
+                    v += t3_quad_plate_dentry(S.q[q], P, 3 * pa + i % 3, 3 * pb + j % 3, F.has_shear);
+            }
+            dK[w][t] = v;
+        }
+    if (F.valid && lane < 54) {
+        int vec = lane / 18, a = lane % 18, node = a / 6, blk = (a % 6) / 3, i = a % 3;
+        const double *x = vec == 0 ? phi : (vec == 1 ? psi : u);
+        const double *xg = x + 6 * (size_t)A.t3[3 * e + node] + 3 * blk;
+        vl[w][vec][a] = S.R[i * 3] * xg[0] + S.R[i * 3 + 1] * xg[1] + S.R[i * 3 + 2] * xg[2];
+    }
+    __syncthreads();
+    // X = M[:, 12:21]; Yd = dKii X
+    if (F.plate)
+        for (int t = lane; t < 108; t += S105_WAVE) {
+            int i = t / 9, j = t % 9;
+            double sum = 0.0;
+            for (int l = 0; l < 12; l++) sum += dK[w][(9 + i) * 21 + 9 + l] * S.M[l * 21 + 12 + j];
+            Yd[w][t] = sum;
+        }
+    __syncthreads();
+    if (F.plate)
+        for (int t = lane; t < 81; t += S105_WAVE) {
+            int i = t / 9, j = t % 9;
+            double v = dK[w][i * 21 + j];
+            for (int l = 0; l < 12; l++) {
+                double xli = S.M[l * 21 + 12 + i], xlj = S.M[l * 21 + 12 + j];
+                v += -dK[w][i * 21 + 9 + l] * xlj - xli * dK[w][(9 + l) * 21 + j] + xli * Yd[w][l * 9 + j];
+            }
+            dKc[w][t] = v;
+        }
+    __syncthreads();
+    double f1 = 0.0, f2 = 0.0;
+    if (F.valid) {
+        double alpha = ((P.bend ? A.k6rot : 0.0) / 1e5) * Gref * P.h;
+        for (int t = lane; t < 324; t += S105_WAVE) {
+            int a = t / 18, b = t % 18;
+            double v = t3_local_entry(S, P, alpha, F.nonzero, F.plate, dKc[w], 1.0 / P.h, a, b);
+            f1 += vl[w][0][a] * v * vl[w][0][b];
+            f2 += vl[w][1][a] * v * vl[w][2][b];
+        }
+    }
+    f1 = wave_sum(f1);
+    f2 = wave_sum(f2);
+    if (F.valid && lane == 0) {
+        terms[3 * ((size_t)A.n_q4 + e)] = f1;
+        terms[3 * ((size_t)A.n_q4 + e) + 2] = f2;
+    }
+}
+
+// One thread per CTRIA3: terms[3 (n_q4 + e) + 1] = phi' KGe phi / t and the
+// element's adjoint load eg[18 e ..] = lam d(phi' KGe(u) phi)/du (global
+// frame, rotations zero).  With Phx = sum_i b_i phi_i, Phy = sum_i c_i phi_i
+// (translations), phi' KGe phi = t A (sxx |Phx|^2 + syy |Phy|^2 + 2 sxy
+// Phx.Phy) and the stress is linear in u: sigma = D Bm u.
+__global__ void t3_kg_sens_kernel(ElemArgs A, const double *u, const double *phi, double lam, double *terms,
+                                  double *eg) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= A.n_t3) return;
+    double p[9], R[9], lc[6], bv[3], cv[3];
+    for (int k = 0; k < 3; k++) {
+        int nd = A.t3[3 * e + k];
+        for (int i = 0; i < 3; i++) p[3 * k + i] = A.xyz[3 * nd + i];
+    }
+    t3_frame(p, R, lc);
+    const double *pr = A.t3prop + 6 * e;
+    double E = pr[0], nu = pr[1], h = pr[2];
+    double ar = t3_derivs(lc, bv, cv);
+    double *g = eg + 18 * (size_t)e;
+    if (!(ar >= 1e-12 && h >= 1e-30)) {
+        terms[3 * ((size_t)A.n_q4 + e) + 1] = 0.0;
+        for (int i = 0; i < 18; i++) g[i] = 0.0;
+        return;
+    }
+    double exx = 0.0, eyy = 0.0, gxy = 0.0, Px[3] = {0, 0, 0}, Py[3] = {0, 0, 0};
+    for (int k = 0; k < 3; k++) {
+        const size_t base = 6 * (size_t)A.t3[3 * e + k];
+        const double *ug = u + base, *fg = phi + base;
+        double ul = R[0] * ug[0] + R[1] * ug[1] + R[2] * ug[2];
+        double vl = R[3] * ug[0] + R[4] * ug[1] + R[5] * ug[2];
+        exx += bv[k] * ul;
+        eyy += cv[k] * vl;
+        gxy += cv[k] * ul + bv[k] * vl;
+        for (int d = 0; d < 3; d++) { Px[d] += bv[k] * fg[d]; Py[d] += cv[k] * fg[d]; }
+    }
+    double c = E / (1 - nu * nu);
+    double D[9] = {c, c * nu, 0.0, c * nu, c, 0.0, 0.0, 0.0, c * ((1 - nu) / 2)};
+    double sig[3];
+    for (int i = 0; i < 3; i++) sig[i] = ((D[3 * i] * exx + D[3 * i + 1] * eyy + D[3 * i + 2] * gxy) * h) / h;
+    double wv[3] = {Px[0] * Px[0] + Px[1] * Px[1] + Px[2] * Px[2], Py[0] * Py[0] + Py[1] * Py[1] + Py[2] * Py[2],
+                    2.0 * (Px[0] * Py[0] + Px[1] * Py[1] + Px[2] * Py[2])};
+    terms[3 * ((size_t)A.n_q4 + e) + 1] = ar * (sig[0] * wv[0] + sig[1] * wv[1] + sig[2] * wv[2]);
+    double Dw[3];
+    for (int i = 0; i < 3; i++) Dw[i] = D[3 * i] * wv[0] + D[3 * i + 1] * wv[1] + D[3 * i + 2] * wv[2];
+    const double f = lam * h * ar;
+    for (int k = 0; k < 3; k++) {
+        double gu = f * (bv[k] * Dw[0] + cv[k] * Dw[2]);
+        double gv = f * (cv[k] * Dw[1] + bv[k] * Dw[2]);
+        for (int d = 0; d < 3; d++) {
+            g[6 * k + d] = gu * R[d] + gv * R[3 + d];
+            g[6 * k + 3 + d] = 0.0;
+        }
+    }
 }
 
 // One thread per (element, node-block pair): CST stress and the 3 x 3
@@ -1185,11 +1364,6 @@ __global__ void block_gather_kernel(int64_t n_blocks, const int32_t *cptr, const
 // Restates the quantities of JFEM/src/solver/adjoint_buckling.jl:2406-2637
 // for one design variable per element, differentiating the forward kernel.
 // ---------------------------------------------------------------------------
-
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 // element DOF a of element e in the global DOF vector
 __device__ __forceinline__ double q4_gvec(const ElemArgs &A, const double *x, int e, int a) {
@@ -1472,6 +1646,16 @@ void launch_t3_kg(const ElemArgs &a, const double *u, double *eb, hipStream_t s)
     if (a.n_t3 == 0) return;
     int64_t n = (int64_t)a.n_t3 * 9;
     t3_kg_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, u, eb + (size_t)a.n_q4 * 576);
+}
+void launch_t3_sens_terms(const ElemArgs &a, const double *u, const double *phi, const double *psi, double *terms,
+                          hipStream_t s) {
+    if (a.n_t3 == 0) return;
+    t3_dke_kernel<<<(a.n_t3 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, psi, terms);
+}
+void launch_t3_kg_sens(const ElemArgs &a, const double *u, const double *phi, double lam, double *terms, double *eg,
+                       hipStream_t s) {
+    if (a.n_t3 == 0) return;
+    t3_kg_sens_kernel<<<(a.n_t3 + 127) / 128, 128, 0, s>>>(a, u, phi, lam, terms, eg);
 }
 void launch_bar_ke(const ElemArgs &a, double *eb, hipStream_t s) {
     if (a.n_bar == 0) return;

@@ -338,19 +338,21 @@ def mac(a, b):
 
 
 def sensitivity_fd(arr, elems, mode=0, rel_step=1e-5):
-    """Central FD of lambda_mode w.r.t. the thickness of each listed CQUAD4,
-    tracking the mode by MAC against the unperturbed shape."""
+    """Central FD of lambda_mode w.r.t. the thickness of each listed shell
+    (index over CQUAD4 first, then CTRIA3), tracking the mode by MAC against
+    the unperturbed shape."""
     base = solve(arr)
     phi0 = base["modes"][:, mode]
+    t0 = arr.shell_thickness()
     out = []
     for e in elems:
-        h = arr.q4prop[e, 2]
+        h = t0[e]
         d = rel_step * h
         vals = []
         for sgn in (+1, -1):
-            t = arr.q4prop[:, 2].copy()
+            t = t0.copy()
             t[e] = h + sgn * d
-            r = solve(arr.with_q4_thickness(t))
+            r = solve(arr.with_shell_thickness(t))
             macs = [mac(phi0, r["modes"][:, j]) for j in range(r["modes"].shape[1])]
             j = int(np.argmax(macs))
             vals.append(r["eigenvalues"][j])

@@ -5,9 +5,10 @@ through the C ABI.
   single skewed 3-D CQUAD4, CTRIA3 or CBAR equals the fixture's element
   matrix in the global frame (1e-12 of max |entry|);
 * small decks: eigenvalues within 1e-8 of the fixture;
-* thickness sensitivity d(lambda)/dt_e: against central finite differences of
-  the oracle's forward path with MAC mode tracking, and the sum over all
-  elements against a uniform-thickness finite difference.
+* thickness sensitivity d(lambda)/dt_e (CQUAD4, and CTRIA3 on a mixed
+  mesh): against central finite differences of the oracle's forward path
+  with MAC mode tracking, the sum over all elements against a
+  uniform-thickness finite difference, and term by term.
 """
 import json
 import os
@@ -123,7 +124,7 @@ def test_gpu_sensitivity_vs_fd(ctx):
     print("sensitivity plate6x5 lambda", lam[:2], flush=True)
     for mode in (0, 1):
         g = dm.sensitivity_thickness(mode)
-        assert g.shape == (arr.n_q4,) and np.all(np.isfinite(g))
+        assert g.shape == (arr.n_shell,) and np.all(np.isfinite(g))
         elems = [0, 7, 14, 22, 29]
         fd, base = O.sensitivity_fd(arr, elems, mode=mode)
         scale = np.abs(g).max()
@@ -186,4 +187,96 @@ def test_gpu_sensitivity_terms_vs_fd(ctx):
     assert np.abs(g[free] - gref[free]).max() <= 1e-7 * np.abs(gref[free]).max()
     fixed = np.setdiff1d(np.arange(arr.ndof), free)
     assert np.all(g[fixed] == 0.0)
+    dm.free()
+
+
+def test_gpu_sensitivity_mixed_mesh_vs_fd(ctx):
+    """CQUAD4 and CTRIA3 thickness sensitivities on a mixed mesh against
+    central finite differences of the oracle (shell index: quads first)."""
+    from jf_2026_05_19_amd import _native as N
+    from oracle import oracle as O
+    arr = M.build_arrays(bdf.bdf_to_model_text(SOLVES["mixed_plate8x7"]["deck"]))
+    assert arr.n_t3 > 0 and arr.n_q4 > 0
+    dm = N.DeviceModel(ctx, arr)
+    dm.solve()
+    g = dm.sensitivity_thickness(0)
+    assert g.shape == (arr.n_shell,) and np.all(np.isfinite(g))
+    q, t = arr.n_q4, arr.n_t3
+    elems = [0, q // 2, q, q + t // 2, q + t - 1]
+    print("mixed sensitivity elements", elems, flush=True)
+    fd, _ = O.sensitivity_fd(arr, elems, mode=0)
+    scale = np.abs(g).max()
+    assert np.abs(g[elems] - fd).max() <= 1e-5 * scale, (g[elems], fd)
+    t0 = arr.shell_thickness()
+    d = 1e-5
+    lp = O.solve(arr.with_shell_thickness(t0 * (1 + d)))["eigenvalues"][0]
+    lm = O.solve(arr.with_shell_thickness(t0 * (1 - d)))["eigenvalues"][0]
+    dir_fd = (lp - lm) / (2 * d)
+    assert abs(g @ t0 - dir_fd) <= 1e-5 * abs(dir_fd)
+    dm.free()
+
+
+def test_gpu_sensitivity_terms_ctria3_vs_fd(ctx):
+    """The adjoint terms of CTRIA3 elements against central differences of
+    the oracle (re-assembly at t_e +- delta, static re-solves), and the
+    CTRIA3 element adjoint loads against differences of phi' KGe(u) phi."""
+    from jf_2026_05_19_amd import _native as N
+    from oracle import oracle as O
+    arr = M.build_arrays(bdf.bdf_to_model_text(SOLVES["mixed_plate8x7"]["deck"]))
+    dm = N.DeviceModel(ctx, arr)
+    dm.solve()
+    terms, g, pkp = dm.sensitivity_terms(0)
+    r = O.solve(arr)
+    lam, phi, u, free = r["eigenvalues"][0], r["modes"][:, 0], r["u"], r["free"]
+    assert abs(pkp - phi @ (r["KG"] @ phi)) <= 1e-9 * abs(pkp)
+    t0 = arr.shell_thickness()
+    q = arr.n_q4
+    for e in (q, q + 1, q + arr.n_t3 // 2, q + arr.n_t3 - 1):
+        d = 1e-6 * t0[e]
+        tp, tm = t0.copy(), t0.copy()
+        tp[e] += d
+        tm[e] -= d
+        ap, am = arr.with_shell_thickness(tp), arr.with_shell_thickness(tm)
+        Kp, Km = O.assemble_K(ap), O.assemble_K(am)
+        ref0 = phi @ ((Kp - Km) @ phi) / (2 * d)
+        ref1 = phi @ ((O.assemble_KG(ap, u) - O.assemble_KG(am, u)) @ phi) / (2 * d)
+        # the static re-solve term is small on some triangles: a 1e-4 step keeps
+        # its difference above the solver's rounding (truncation ~1e-8)
+        d2 = 1e-4 * t0[e]
+        tp, tm = t0.copy(), t0.copy()
+        tp[e] += d2
+        tm[e] -= d2
+        ap, am = arr.with_shell_thickness(tp), arr.with_shell_thickness(tm)
+        up, _ = O.static_solve(ap, O.assemble_K(ap))
+        um, _ = O.static_solve(am, O.assemble_K(am))
+        ref2 = -lam * (phi @ (O.assemble_KG(arr, up) @ phi) - phi @ (O.assemble_KG(arr, um) @ phi)) / (2 * d2)
+        for got, ref in ((terms[e, 0], ref0), (terms[e, 1], ref1), (terms[e, 2], ref2)):
+            assert abs(got - ref) <= 1e-6 * max(abs(ref), 1e-3 * abs(terms[:, 0]).max()), (e, got, ref)
+    # adjoint load of the triangles alone: g minus the quads' share, compared
+    # with element-wise differences of phi' KGe(u) phi (linear in u: exact)
+    gref = np.zeros(arr.ndof)
+    hh = 1e-3 * np.abs(u).max()
+    for e in range(arr.n_t3):
+        dof = np.concatenate([6 * arr.t3[e][k] + np.arange(6) for k in range(3)])
+        p = arr.xyz[arr.t3[e]].ravel()
+        for a in range(18):
+            if dof[a] % 6 >= 3:
+                continue
+            du = np.zeros(18)
+            du[a] = hh
+            fp = phi[dof] @ O.t3_kg_global(p, arr.t3prop[e], u[dof] + du)[0] @ phi[dof]
+            fm = phi[dof] @ O.t3_kg_global(p, arr.t3prop[e], u[dof] - du)[0] @ phi[dof]
+            gref[dof[a]] += lam * (fp - fm) / (2 * hh)
+    for e in range(arr.n_q4):
+        dof = np.concatenate([6 * arr.q4[e][k] + np.arange(6) for k in range(4)])
+        p = arr.xyz[arr.q4[e]].ravel()
+        for a in range(24):
+            if dof[a] % 6 >= 3:
+                continue
+            du = np.zeros(24)
+            du[a] = 1e-6 * np.abs(u).max()
+            fp = phi[dof] @ O.q4_kg_global(p, arr.q4prop[e], u[dof] + du) @ phi[dof]
+            fm = phi[dof] @ O.q4_kg_global(p, arr.q4prop[e], u[dof] - du) @ phi[dof]
+            gref[dof[a]] += lam * (fp - fm) / (2 * du[a])
+    assert np.abs(g[free] - gref[free]).max() <= 1e-7 * np.abs(gref[free]).max()
     dm.free()
