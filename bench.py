@@ -48,6 +48,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--batch-only", action="store_true", help="only the batched-vs-plain sweep (for profiling)")
     ap.add_argument("--no-batch-plain", action="store_true", help="skip the one-at-a-time leg of the sweep")
+    ap.add_argument("--cylinder-fd", type=int, default=8,
+                    help="seeded shells of the cylinder's finite-difference check (0: no cylinder leg)")
+    ap.add_argument("--cylinder-only", action="store_true", help="only the cylinder leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -55,6 +58,10 @@ def main():
 
     from jf_2026_05_19_amd import bdf, decks, model as M
 
+    if args.cylinder_only:
+        from jf_2026_05_19_amd import _native as N
+        print(json.dumps({"metric": METRIC, "cylinder": cylinder_case(N.Context(0), max(args.cylinder_fd, 1))}))
+        return
     if args.batch_only:
         from jf_2026_05_19_amd import _native as N
         print(json.dumps({"metric": METRIC, "batched": batched_sweep(N.Context(0), args.batch_variants,
@@ -126,6 +133,8 @@ def main():
     dm.free()
     if args.batch_variants > 0:
         result["batched"] = batched_sweep(ctx, args.batch_variants, args.batch_steps)
+    if args.cylinder_fd > 0:
+        result["cylinder"] = cylinder_case(ctx, args.cylinder_fd)
     if cpu is not None:
         result["gpu_over_cpu"] = result["value"] / cpu["value"]
     print(json.dumps(result))
@@ -239,6 +248,69 @@ def batched_sweep(ctx, n, steps, plain=True):
             "max_eig_rel_err_vs_plain": err, "members_ok": int(np.sum(st == 0)),
             "members_resolved_alone": int(stats["fallbacks"]), "stacked_free_dofs": int(stats["n_free"]),
             "lanczos_vectors": int(stats["lanczos_vectors"])}
+
+
+def cylinder_case(ctx, n_fd, steps=3, seed=20260521):
+    """Large single case: cylinder R = 500, L = 1000, t = 1 mm, 416 x 400
+    GRIDs (998,400 DOF) with a 10 %-by-area CTRIA3 band, clamped at z = 0,
+    axial ring load at z = L, ND = 10.  Timed: warm solves (symbolic analysis
+    in the untimed first solve), then d(lambda)/dt of every shell for all
+    modes.  Checked: mode-1 sensitivities of n_fd seeded shells against
+    central differences of the device's own forward path (delta = 1e-4 t,
+    mode tracked by MAC; lambda is resolved to ~1e-13 relative, so smaller
+    steps drown the difference in that noise)."""
+    from jf_2026_05_19_amd import bdf, decks, model as M
+    from jf_2026_05_19_amd import _native as N
+    t0 = time.perf_counter()
+    arr = M.build_arrays(bdf.bdf_to_model_text(decks.cylinder_deck()))
+    log("cylinder: nodes=%d q4=%d t3=%d ndof=%d (parse %.1fs)" % (arr.n_nodes, arr.n_q4, arr.n_t3, arr.ndof,
+                                                                  time.perf_counter() - t0))
+    dm = N.DeviceModel(ctx, arr)
+    t1 = time.perf_counter()
+    lam, modes = dm.solve(16)
+    log("cylinder first solve (incl. symbolic analysis) %.2fs, lambda1 %.6f" % (time.perf_counter() - t1, lam[0]))
+    lat = []
+    for k in range(steps):
+        ts = time.perf_counter()
+        lam, modes = dm.solve(16, want_modes=(k == steps - 1))
+        lat.append(time.perf_counter() - ts)
+    tim, st = ctx.timings(), ctx.stats()
+    ts = time.perf_counter()
+    sens = [dm.sensitivity_thickness(k) for k in range(len(lam))]
+    t_sens = time.perf_counter() - ts
+    log("cylinder: solve %.3fs, sensitivities of %d shells x %d modes %.3fs" % (np.median(lat), arr.n_shell,
+                                                                               len(lam), t_sens))
+    # finite-difference check of mode 1 on seeded shells
+    rng = np.random.default_rng(seed)
+    elems = np.sort(rng.choice(arr.n_shell, size=n_fd, replace=False))
+    t_sh = arr.shell_thickness()
+    phi0 = modes[:, 0]
+    errs = []
+    for e in elems:
+        vals = []
+        for sgn in (1.0, -1.0):
+            t = t_sh.copy()
+            t[e] *= 1.0 + sgn * 1e-4
+            dm.set_shell_thickness(t)
+            l2, m2 = dm.solve(16)
+            macs = [float((phi0 @ m2[:, j]) ** 2 / ((phi0 @ phi0) * (m2[:, j] @ m2[:, j]))) for j in range(len(l2))]
+            vals.append(l2[int(np.argmax(macs))])
+        fd = (vals[0] - vals[1]) / (2e-4 * t_sh[e])
+        errs.append(abs(sens[0][e] - fd))
+        log("cylinder FD shell %d: adjoint %.6e fd %.6e" % (e, sens[0][e], fd))
+    dm.set_shell_thickness(t_sh)
+    scale = float(np.abs(sens[0]).max())
+    dm.free()
+    return {"workload": "cylinder R=500 L=1000 t=1 mm, 416x400 GRIDs, %d CQUAD4 + %d CTRIA3 (10%% band), %d DOF, "
+                        "ND=10, FP64" % (arr.n_q4, arr.n_t3, arr.ndof),
+            "solve_ms": 1e3 * float(np.median(lat)), "phase_ms": {"K_assembly": tim[0], "factor": tim[2],
+                                                               "static_solve": tim[3], "KG_assembly": tim[4],
+                                                               "eigensolve": tim[5]},
+            "lambda": [float(x) for x in lam], "lanczos_vectors": int(st[5]), "factor_flops": float(st[2]),
+            "sensitivity_ms_all_shells_all_modes": 1e3 * t_sens, "shells": arr.n_shell, "modes": len(lam),
+            "fd_check": {"shells": [int(e) for e in elems], "max_abs_err": float(max(errs)),
+                         "max_err_rel_to_max_sensitivity": float(max(errs)) / scale, "step": "1e-4 t, central",
+                         "mode": 1}}
 
 
 def cpu_baseline(arr):

@@ -74,6 +74,9 @@ int sol105_model_free(sol105_ctx *ctx, sol105_model *model);
 
 /* Replace the CQUAD4 thickness column (design update) without re-analysis. */
 int sol105_model_set_q4_thickness(sol105_ctx *ctx, sol105_model *model, const double *t /* n_q4 */);
+/* Replace the thickness of every shell, CQUAD4 first, then CTRIA3 (design
+ * update; the pattern, maps and ordering are kept). */
+int sol105_model_set_shell_thickness(sol105_ctx *ctx, sol105_model *model, const double *t /* n_q4 + n_t3 */);
 
 /* Pattern query: n_blocks, then block row pointers (n_nodes+1) and block
  * columns (n_blocks).  Either array may be NULL. */

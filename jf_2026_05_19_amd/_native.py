@@ -20,7 +20,8 @@ ERRORS = {-1: "SOL105_E_INVALID", -2: "SOL105_E_UNSUPPORTED", -3: "SOL105_E_DEVI
 
 EXPORTS = [
     "sol105_ctx_create", "sol105_ctx_destroy", "sol105_last_error", "sol105_model_upload", "sol105_model_free",
-    "sol105_model_set_q4_thickness", "sol105_pattern", "sol105_assemble_K", "sol105_static_solve",
+    "sol105_model_set_q4_thickness", "sol105_model_set_shell_thickness", "sol105_pattern", "sol105_assemble_K",
+    "sol105_static_solve",
     "sol105_assemble_KG", "sol105_buckling", "sol105_solve", "sol105_sensitivity_thickness",
     "sol105_sensitivity_terms", "sol105_profile_enable", "sol105_profile_read", "sol105_last_timings",
     "sol105_last_stats", "sol105_batch_create", "sol105_batch_free", "sol105_batch_solve", "sol105_batch_modes",
@@ -73,6 +74,7 @@ def lib():
         L.sol105_model_upload.argtypes = [vp, ctypes.POINTER(ModelDesc), ctypes.POINTER(vp)]
         L.sol105_model_free.argtypes = [vp, vp]
         L.sol105_model_set_q4_thickness.argtypes = [vp, vp, dp]
+        L.sol105_model_set_shell_thickness.argtypes = [vp, vp, dp]
         L.sol105_pattern.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
                                      ctypes.POINTER(ctypes.c_int32)]
         L.sol105_assemble_K.argtypes = [vp, vp, dp]
@@ -207,6 +209,12 @@ class DeviceModel:
     def set_q4_thickness(self, t):
         t = np.ascontiguousarray(t, dtype=np.float64)
         self.ctx.check(self.ctx.L.sol105_model_set_q4_thickness(self.ctx.h, self.h, _dp(t)))
+
+    def set_shell_thickness(self, t):
+        """Thickness of every shell (CQUAD4 first, then CTRIA3)."""
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        assert t.shape == (self.arr.n_shell,)
+        self.ctx.check(self.ctx.L.sol105_model_set_shell_thickness(self.ctx.h, self.h, _dp(t)))
 
     def assemble_K(self, want=True):
         vals = np.zeros(self.n_blocks * 36) if want else None

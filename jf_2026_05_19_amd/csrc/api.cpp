@@ -311,6 +311,20 @@ int sol105_model_set_q4_thickness(sol105_ctx *ctx, sol105_model *M, const double
     return SOL105_OK;
 }
 
+int sol105_model_set_shell_thickness(sol105_ctx *ctx, sol105_model *M, const double *t) {
+    if (!ctx || !M || (!t && M->n_q4 + M->n_t3 > 0)) return set_err(ctx, SOL105_E_INVALID, "null argument");
+    for (int e = 0; e < M->n_q4 + M->n_t3; e++)
+        if (!(t[e] > 0.0)) return set_err(ctx, SOL105_E_INVALID, "thickness of shell %d must be positive", e);
+    for (int e = 0; e < M->n_q4; e++) M->q4prop[6 * (size_t)e + 2] = t[e];
+    for (int e = 0; e < M->n_t3; e++) M->t3prop[6 * (size_t)e + 2] = t[M->n_q4 + e];
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(M->d_q4prop.upload(M->q4prop.data(), M->q4prop.size(), ctx->stream));
+    HIPCHK(M->d_t3prop.upload(M->t3prop.data(), M->t3prop.size(), ctx->stream));
+    M->K_valid = M->factor_valid = M->u_valid = M->KG_valid = false;
+    M->n_modes = 0;
+    return SOL105_OK;
+}
+
 int sol105_pattern(sol105_ctx *ctx, const sol105_model *M, int64_t *n_blocks, int32_t *rp, int32_t *col) {
     if (!M) return set_err(ctx, SOL105_E_INVALID, "null model");
     if (n_blocks) *n_blocks = M->pat.n_blocks();
@@ -501,7 +515,7 @@ void small_sym_eig_host(int b, std::vector<double> A, std::vector<double> &w, st
 struct LanczosOut {
     std::vector<double> theta;  // selected Ritz values (descending |theta|)
     std::vector<double> resid;
-    int m = 0, steps = 0, injected = 0;
+    int m = 0, steps = 0, injected = 0, generated = 0, restarts = 0;
 };
 
 // Block Lanczos on C = L^-1 (-KG) L^-T with full CGS2 reorthogonalisation.
@@ -521,13 +535,18 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     int n = M->sym.n_free, ndof = M->ndof();
     const bool debug = std::getenv("SOL105_DEBUG") != nullptr;
     int b = want <= 8 ? 8 : 16;
-    int mmax = std::min(384, (n / b) * b);
+    int mmax = std::min(1024, (n / b) * b);
     if (mmax < b) return set_err(ctx, SOL105_E_NOCONV, "free set too small for the eigensolver (%d)", n);
     int guard = 2;
     int kneed = std::min(want + guard, mmax);
     const double tol = 1e-10;
     const double dep_tol2 = 1e-14;
-    DevBuf<double> V, Wb, xf, yf, T, A, Zs, thd, ework, part, small, H2;
+    // thick restart at mmax vectors: keep the kk Ritz vectors of largest
+    // |theta| and the next block
+    const int kk = mmax >= 128 + b ? (SYMEIG_MAX_K / b) * b : std::max(b, (mmax / 2 / b) * b);
+    const int max_restarts = 40;
+    const int kz = std::max(kneed, kk);
+    DevBuf<double> V, Wb, xf, yf, T, A, Zs, thd, ework, part, small, H2, Ytmp;
     DevBuf<int> ierr, sw;
     HIPCHK(V.alloc((size_t)n * (mmax + b)));
     HIPCHK(Wb.alloc((size_t)n * b));
@@ -539,11 +558,11 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     HIPCHK(small.alloc(5 * 16 * 16));
     HIPCHK(ierr.alloc(4));
     HIPCHK(sw.alloc(1));
-    if (kneed > SYMEIG_MAX_K || mmax > SYMEIG_MAX_M)
-        return set_err(ctx, SOL105_E_UNSUPPORTED, "Rayleigh-Ritz size out of range (k=%d, m=%d)", kneed, mmax);
-    HIPCHK(Zs.alloc((size_t)mmax * kneed));
-    HIPCHK(thd.alloc(kneed));
-    HIPCHK(ework.alloc(symeig_work_size(mmax, kneed)));
+    if (kz > SYMEIG_MAX_K || mmax > SYMEIG_MAX_M)
+        return set_err(ctx, SOL105_E_UNSUPPORTED, "Rayleigh-Ritz size out of range (k=%d, m=%d)", kz, mmax);
+    HIPCHK(Zs.alloc((size_t)mmax * kz));
+    HIPCHK(thd.alloc(kz));
+    HIPCHK(ework.alloc(symeig_work_size(mmax, kz)));
     HIPCHK(H2.alloc((size_t)(mmax + b) * b));
     int ldT = mmax + b;
     dev_fill(T.p, (int64_t)ldT * ldT, 0.0, s);
@@ -650,14 +669,60 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
                           : std::min(mmax, std::max(2 * b, ((kneed * 3 + b - 1) / b) * b));
     std::vector<double> hR;
     bool converged = false;
-    int m = 0, injected = 0;
+    int m = 0, injected = 0, restarts = 0, generated = 0;
+    // Thick restart (the basis is full and the wanted pairs have not
+    // converged): V <- [V Z_kk, Q], T <- [diag(theta_kk) 0; R Z_kk(last b rows, :) ...].
+    // With Z the exact Ritz vectors of T, C V Z = V Z diag(theta) + Q R E^T Z,
+    // so the Lanczos relation holds for the new basis with that arrowhead T;
+    // the next expansion fills T's new column block through CGS2.
+    auto restart = [&]() {
+        symmetrize_copy(m, ldT, T.p, A.p, s);
+        sym_eig_top(m, A.p, kk, thd.p, Zs.p, ework.p, sw.p, s);
+        std::vector<double> th(kk), zl((size_t)b * kk), R((size_t)b * b);
+        HIPCHK(hipMemcpyAsync(th.data(), thd.p, sizeof(double) * kk, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpy2DAsync(zl.data(), b * sizeof(double), Zs.p + (m - b), (size_t)m * sizeof(double),
+                                b * sizeof(double), kk, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpy2DAsync(R.data(), b * sizeof(double), T.p + (size_t)m + (size_t)(m - b) * ldT,
+                                ldT * sizeof(double), b * sizeof(double), b, hipMemcpyDeviceToHost, s));
+        int eig_info = 0;
+        HIPCHK(hipMemcpyAsync(&eig_info, sw.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(Ytmp.alloc((size_t)n * kk));
+        for (int c0 = 0; c0 < kk; c0 += 16)
+            ts_gemm_nn(n, m, std::min(16, kk - c0), V.p, n, Zs.p + (size_t)c0 * m, m, Ytmp.p + (size_t)c0 * n, n,
+                       1.0, 0.0, s);
+        HIPCHK(hipStreamSynchronize(s));
+        if (eig_info != 0) return set_err(ctx, SOL105_E_NOCONV, "restart: inverse iteration failed (m=%d)", m);
+        HIPCHK(hipMemcpyAsync(V.p, Ytmp.p, sizeof(double) * n * kk, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(V.p + (size_t)kk * n, blk(j), sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
+        // arrowhead: (kk + b) x kk leading block of T, column-major
+        std::vector<double> H((size_t)(kk + b) * kk, 0.0);
+        for (int i = 0; i < kk; i++) {
+            H[(size_t)i * (kk + b) + i] = th[i];
+            for (int p = 0; p < b; p++) {
+                double v = 0.0;
+                for (int q = 0; q < b; q++) v += R[(size_t)q * b + p] * zl[(size_t)i * b + q];
+                H[(size_t)i * (kk + b) + kk + p] = v;
+            }
+        }
+        dev_fill(T.p, (int64_t)ldT * ldT, 0.0, s);
+        HIPCHK(hipMemcpy2DAsync(T.p, ldT * sizeof(double), H.data(), (kk + b) * sizeof(double),
+                                (kk + b) * sizeof(double), kk, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(ierr.p, 0, sizeof(int) * 4, s));
+        HIPCHK(hipStreamSynchronize(s));
+        j = kk / b;
+        m = kk;
+        restarts++;
+        return SOL105_OK;
+    };
     while (true) {
         int rc = fast_step(j);
         if (rc) return rc;
         j++;
+        generated += b;
         m = j * b;
-        bool last = (m >= mmax);
-        if (!(m >= next_check || last)) continue;
+        bool last = (m >= mmax) && restarts >= max_restarts;
+        bool full = (m >= mmax);
+        if (!(m >= next_check || full)) continue;
         symmetrize_copy(m, ldT, T.p, A.p, s);
         // the kchk Ritz pairs of largest |theta| (A is overwritten)
         const int kchk = std::min(kneed, m);
@@ -727,13 +792,22 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
             HIPCHK(hipStreamSynchronize(s));
             break;
         }
+        if (full) {
+            rc = restart();
+            if (rc) return rc;
+            if (debug) fprintf(stderr, "[sol105] thick restart %d: kept %d Ritz vectors\n", restarts, kk);
+            next_check = std::min(mmax, m + 4 * b);
+            continue;
+        }
         next_check = std::min(mmax, m + (warm ? b : 4 * b));
     }
     out.m = m;
+    out.generated = generated;
+    out.restarts = restarts;
     out.steps = j;
     out.injected = injected;
     if (!converged) return set_err(ctx, SOL105_E_NOCONV, "block Lanczos did not converge in %d vectors", m);
-    M->warm_m = m;
+    M->warm_m = restarts ? mmax : m;
     M->warm_b = b;
     return SOL105_OK;
 }
@@ -758,7 +832,7 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
     int rc = lanczos(ctx, M, req, Y.p, lo);
     if (rc) return rc;
     int k = (int)lo.theta.size();
-    M->lanczos_m = lo.m;
+    M->lanczos_m = lo.generated;
     M->lanczos_steps = lo.steps;
     // x = L^-T y, expand to all DOFs
     solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Y.p, k, s, ctx->timer());

@@ -16,12 +16,15 @@ parser (``JFEM/src/parsing/card_processor.jl``).
   longitudinal mesh line, clamped loaded edges, axial compression.
 * ``variant_sweep`` -- seeded design variants of the plate (mesh size,
   thickness, modulus).
+* ``cylinder_deck`` -- axially compressed cylindrical shell of flat CQUAD4
+  facets with a circumferential band of CTRIA3 (each quad of the band split
+  in two), clamped at z = 0, axial ring load at z = L.
 """
 from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["plate_deck", "panel_deck", "variant_sweep"]
+__all__ = ["plate_deck", "panel_deck", "variant_sweep", "cylinder_deck"]
 
 
 def _fmt(v):
@@ -143,3 +146,53 @@ def variant_sweep(n=256, seed=20260519, n_min=10, n_max=80):
         E, nu = (210e3, 0.30) if steel else (70e3, 0.33)
         out.append((plate_deck(nx=m, ny=m, t=t, E=E, nu=nu), dict(mesh=m, t=t, E=E, nu=nu)))
     return out
+
+
+def cylinder_deck(n_circ=416, n_axial=400, R=500.0, L=1000.0, t=1.0, E=70e3, nu=0.33, line_load=1.0, nd=10,
+                  tri_band=0.10):
+    """Cylindrical shell under axial compression (N/mm of circumference).
+
+    n_circ x n_axial GRIDs on the radius-R surface (theta_i = 2 pi i / n_circ,
+    z_j = L j / (n_axial - 1)); one flat CQUAD4 per grid cell (a chord of the
+    circle extruded along z, so every facet is planar).  The element rows
+    whose centres lie in the middle tri_band fraction of the length are split
+    into two CTRIA3 each (alternating diagonals), i.e. tri_band of the area.
+    End z = 0 clamped (123456); end z = L: components 1, 2, 4, 5 fixed and a
+    compressive axial FORCE ring, line_load times the node's share of the
+    circumference.
+    """
+    L_ = _header(nd)
+    L_.append(_card("MAT1", 1, float(E), "", float(nu)))
+    L_.append(_card("PSHELL", 1, 1, float(t), 1, "", ""))
+    gid = lambda i, j: j * n_circ + (i % n_circ) + 1
+    for j in range(n_axial):
+        z = L * j / (n_axial - 1)
+        for i in range(n_circ):
+            th = 2.0 * np.pi * i / n_circ
+            L_.append(_card("GRID", gid(i, j), "", R * np.cos(th), R * np.sin(th), z))
+    n_rows = n_axial - 1
+    lo, hi = 0.5 * (1.0 - tri_band), 0.5 * (1.0 + tri_band)
+    eid = 1
+    for j in range(n_rows):
+        zc = (j + 0.5) / n_rows
+        band = tri_band > 0 and lo <= zc < hi
+        for i in range(n_circ):
+            c = (gid(i, j), gid(i + 1, j), gid(i + 1, j + 1), gid(i, j + 1))
+            if band:
+                if (i + j) % 2 == 0:
+                    L_.append(_card("CTRIA3", eid, 1, c[0], c[1], c[2]))
+                    L_.append(_card("CTRIA3", 5000000 + eid, 1, c[0], c[2], c[3]))
+                else:
+                    L_.append(_card("CTRIA3", eid, 1, c[0], c[1], c[3]))
+                    L_.append(_card("CTRIA3", 5000000 + eid, 1, c[1], c[2], c[3]))
+            else:
+                L_.append(_card("CQUAD4", eid, 1, *c))
+            eid += 1
+    for i in range(n_circ):
+        L_.append(_card("SPC1", 1, "123456", gid(i, 0)))
+        L_.append(_card("SPC1", 1, "1245", gid(i, n_axial - 1)))
+    f = line_load * 2.0 * np.pi * R / n_circ
+    for i in range(n_circ):
+        L_.append(_card("FORCE", 10, gid(i, n_axial - 1), 0, f, 0.0, 0.0, -1.0))
+    L_.append("ENDDATA")
+    return "\n".join(L_) + "\n"
