@@ -11,7 +11,9 @@ the timed region (deck parse and upload excluded, reported separately).
 
 The line also carries the batched-vs-plain comparison of the metric
 ("batched"): a sweep of 256 seeded plate variants solved one model at a time
-and as one stacked batch.
+and as one stacked batch; the served request stream ("served": p50/p99
+latency of a Poisson stream of design requests, served vs a plain queue);
+and the large cylinder case with its sensitivities ("cylinder").
 
 Prints ONE JSON line (rank 0).  --gpus N with N > 1 exits: this build runs
 on one GPU.
@@ -51,6 +53,9 @@ def main():
     ap.add_argument("--cylinder-fd", type=int, default=8,
                     help="seeded shells of the cylinder's finite-difference check (0: no cylinder leg)")
     ap.add_argument("--cylinder-only", action="store_true", help="only the cylinder leg")
+    ap.add_argument("--served-requests", type=int, default=120,
+                    help="requests of the served-stream leg (0: no served leg)")
+    ap.add_argument("--served-only", action="store_true", help="only the served-stream leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -58,6 +63,9 @@ def main():
 
     from jf_2026_05_19_amd import bdf, decks, model as M
 
+    if args.served_only:
+        print(json.dumps({"metric": METRIC, "served": served_stream(max(args.served_requests, 1))}))
+        return
     if args.cylinder_only:
         from jf_2026_05_19_amd import _native as N
         print(json.dumps({"metric": METRIC, "cylinder": cylinder_case(N.Context(0), max(args.cylinder_fd, 1))}))
@@ -135,6 +143,8 @@ def main():
         result["batched"] = batched_sweep(ctx, args.batch_variants, args.batch_steps)
     if args.cylinder_fd > 0:
         result["cylinder"] = cylinder_case(ctx, args.cylinder_fd)
+    if args.served_requests > 0:
+        result["served"] = served_stream(args.served_requests)
     if cpu is not None:
         result["gpu_over_cpu"] = result["value"] / cpu["value"]
     print(json.dumps(result))
@@ -248,6 +258,92 @@ def batched_sweep(ctx, n, steps, plain=True):
             "max_eig_rel_err_vs_plain": err, "members_ok": int(np.sum(st == 0)),
             "members_resolved_alone": int(stats["fallbacks"]), "stacked_free_dofs": int(stats["n_free"]),
             "lanczos_vectors": int(stats["lanczos_vectors"])}
+
+
+def _replay(arrs, trace, workers, batching):
+    """Serves a request trace [(arrival s, mesh, thickness)]: submits each
+    request at its arrival time, returns per-request (eigenvalues, info) and
+    the wall time from the first arrival to the last result."""
+    from jf_2026_05_19_amd import _native as N
+    srv = N.Server(workers=workers, batching=batching)
+    ids = [srv.add_model(a) for a in arrs]
+    for i in ids:                      # first design of every mesh: symbolic analysis, untimed
+        srv.wait(srv.submit(i))
+    t0 = time.perf_counter()
+    tickets = []
+    for arrival, f, t in trace:
+        dt = arrival - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        tickets.append(srv.submit(ids[f], t))
+    out = [srv.wait(tk) for tk in tickets]
+    wall = time.perf_counter() - t0
+    stats = srv.stats()
+    srv.close()
+    return out, wall, stats
+
+
+def served_stream(n_req, workers=4, seed=20260520):
+    """Served optimisation stream: ten seeded panel meshes (2k..500k DOF
+    log-uniform, 10 % CTRIA3, CBAR stringers, ND = 5), Poisson arrivals at
+    80 % of the one-at-a-time device capacity, each request a +-5 %
+    per-shell thickness perturbation of the previous design of its mesh.
+    Served (4 worker streams, stacking of small pending requests,
+    HBM-footprint admission) and, as the reference, the same trace through
+    one worker without stacking (a plain queue).  Latency = arrival -> result."""
+    from concurrent.futures import ProcessPoolExecutor
+    from jf_2026_05_19_amd import decks
+    from jf_2026_05_19_amd import _native as N
+    texts = [t for t, _ in decks.served_families(seed=seed)]
+    with ProcessPoolExecutor(max_workers=8) as ex:
+        arrs = list(ex.map(_parse_variant, texts))
+    ndofs = [a.ndof for a in arrs]
+    # one-at-a-time service time of each mesh (warm)
+    ctx = N.Context(0)
+    svc = []
+    for a in arrs:
+        dm = N.DeviceModel(ctx, a)
+        dm.solve(16, want_modes=False)
+        ts = time.perf_counter()
+        dm.solve(16, want_modes=False)
+        svc.append(time.perf_counter() - ts)
+        dm.free()
+    ctx.close()
+    rate = 0.8 / float(np.mean(svc))
+    rng = np.random.default_rng(seed)
+    cur = [a.shell_thickness() for a in arrs]
+    trace, t = [], 0.0
+    for r in range(n_req):
+        t += float(rng.exponential(1.0 / rate))
+        f = int(rng.integers(len(arrs)))
+        cur[f] = cur[f] * rng.uniform(0.95, 1.05, size=cur[f].shape)
+        trace.append((t, f, cur[f].copy()))
+    log("served stream: %d requests over %d meshes (%d..%d DOF), rate %.1f/s (mean solo service %.1f ms)" %
+        (n_req, len(arrs), min(ndofs), max(ndofs), rate, 1e3 * np.mean(svc)))
+    res_s, wall_s, st_s = _replay(arrs, trace, workers, True)
+    res_q, wall_q, st_q = _replay(arrs, trace, 1, False)
+    lat_s = np.array([i["latency_ms"] for _, _, i in res_s])
+    lat_q = np.array([i["latency_ms"] for _, _, i in res_q])
+    err = 0.0
+    for (la, sa, _), (lb, sb, _) in zip(res_s, res_q):
+        if sa != 0 or sb != 0 or len(la) != len(lb):
+            err = float("inf")
+            break
+        if len(la):
+            err = max(err, float(np.max(np.abs(la - lb) / np.abs(lb))))
+    log("served: p50 %.1f ms p99 %.1f ms (%.1f req/s); plain queue: p50 %.1f ms p99 %.1f ms (%.1f req/s)" %
+        (np.percentile(lat_s, 50), np.percentile(lat_s, 99), n_req / wall_s, np.percentile(lat_q, 50),
+         np.percentile(lat_q, 99), n_req / wall_q))
+    return {"workload": "served stream: %d requests, Poisson arrivals at 80%% of one-at-a-time capacity "
+                        "(%.1f/s), 10 panel meshes %d..%d DOF (10%% CTRIA3, CBAR), +-5%% thickness steps per "
+                        "mesh, ND=5, FP64" % (n_req, rate, min(ndofs), max(ndofs)),
+            "served_p50_ms": float(np.percentile(lat_s, 50)), "served_p99_ms": float(np.percentile(lat_s, 99)),
+            "served_requests_per_s": n_req / wall_s,
+            "queue_p50_ms": float(np.percentile(lat_q, 50)), "queue_p99_ms": float(np.percentile(lat_q, 99)),
+            "queue_requests_per_s": n_req / wall_q,
+            "workers": workers, "stacked_solves": int(st_s["stacked_solves"]),
+            "stacked_requests": int(st_s["stacked_requests"]), "max_eig_rel_err_vs_queue": err,
+            "mean_solo_service_ms": 1e3 * float(np.mean(svc))}
 
 
 def cylinder_case(ctx, n_fd, steps=3, seed=20260521):

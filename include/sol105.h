@@ -31,6 +31,7 @@ extern "C" {
 #define SOL105_E_SINGULAR (-4)     /* stiffness not positive definite           */
 #define SOL105_E_NOCONV (-5)       /* eigensolver did not converge              */
 #define SOL105_E_STATE (-6)        /* call order violated (e.g. KG before u)    */
+#define SOL105_E_PENDING (-7)      /* server request not finished yet           */
 
 typedef struct sol105_ctx sol105_ctx;
 typedef struct sol105_model sol105_model;
@@ -172,6 +173,44 @@ int sol105_batch_modes(sol105_ctx *ctx, const sol105_batch *batch, int32_t membe
  * the end, block steps, members re-solved alone, stacked free DOFs, front
  * storage bytes, factor flops, tree height.  No reference counterpart. */
 int sol105_batch_stats(const sol105_ctx *ctx, const sol105_batch *batch, double *out /* 8 */);
+
+/* Request server for design loops and request streams.  The reference
+ * serves them with a JSONL worker that runs one manifest case after another
+ * (JFEM/tools/jfem_worker_jsonl.jl, JFEM/tools/manifest_batch_core.jl:
+ * 409-478).  Here each mesh is registered once (uploaded, pattern and
+ * ordering built once) and designs of it are submitted as new shell
+ * thicknesses, so analysis and factor buffers stay on the device between
+ * design iterations.  n_workers threads, each with its own HIP stream, take
+ * requests as soon as a worker and device memory are free, so requests
+ * overlap on the device; pending requests of small meshes are stacked into
+ * one batched solve when batching != 0.  A request is admitted when its
+ * estimated device footprint fits hbm_budget_bytes beside the work in flight
+ * (<= 0: three quarters of the free device memory).  Requests of one mesh
+ * run in submission order.  No reference counterpart for the entry points
+ * themselves. */
+typedef struct sol105_server sol105_server;
+int sol105_server_create(int device, int32_t n_workers, double hbm_budget_bytes, int32_t batching,
+                         sol105_server **out);
+int sol105_server_destroy(sol105_server *server);  /* finishes queued requests first */
+const char *sol105_server_last_error(const sol105_server *server);
+int sol105_server_add_model(sol105_server *server, const sol105_model_desc *desc, int32_t *model_id);
+/* shell_thickness (n_q4 + n_t3, CQUAD4 first) may be NULL: the mesh's
+ * current design.  max_modes <= 16. */
+int sol105_server_submit(sol105_server *server, int32_t model_id, const double *shell_thickness,
+                         int32_t max_modes, int64_t *ticket);
+/* SOL105_OK with the results (then the ticket is released) or
+ * SOL105_E_PENDING.  lambda_out: max_modes values; status: the request's
+ * own result code; info (may be NULL): latency ms (submit -> done), queue
+ * ms, service ms, requests in its stacked solve, start and finish ms since
+ * the server was created. */
+int sol105_server_poll(sol105_server *server, int64_t ticket, double *lambda_out, int32_t *n_out,
+                       int32_t *status, double *info /* 6 */);
+/* as sol105_server_poll, blocking until the request is done */
+int sol105_server_wait(sol105_server *server, int64_t ticket, double *lambda_out, int32_t *n_out,
+                       int32_t *status, double *info /* 6 */);
+/* out[8]: requests done, stacked solves, requests in them, queued, bytes in
+ * flight, budget, workers, registered meshes */
+int sol105_server_stats(sol105_server *server, double *out /* 8 */);
 
 /* Phase timings of the last sol105_solve in milliseconds (device events):
  * [0] K assembly, [1] AUTOSPC+symbolic, [2] factorization, [3] static solve,

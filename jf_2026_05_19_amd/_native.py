@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "libsol105_hip.so")
 
 SOL105_OK = 0
 ERRORS = {-1: "SOL105_E_INVALID", -2: "SOL105_E_UNSUPPORTED", -3: "SOL105_E_DEVICE", -4: "SOL105_E_SINGULAR",
-          -5: "SOL105_E_NOCONV", -6: "SOL105_E_STATE"}
+          -5: "SOL105_E_NOCONV", -6: "SOL105_E_STATE", -7: "SOL105_E_PENDING"}
+SOL105_E_PENDING = -7
 
 EXPORTS = [
     "sol105_ctx_create", "sol105_ctx_destroy", "sol105_last_error", "sol105_model_upload", "sol105_model_free",
@@ -25,7 +26,9 @@ EXPORTS = [
     "sol105_assemble_KG", "sol105_buckling", "sol105_solve", "sol105_sensitivity_thickness",
     "sol105_sensitivity_terms", "sol105_profile_enable", "sol105_profile_read", "sol105_last_timings",
     "sol105_last_stats", "sol105_batch_create", "sol105_batch_free", "sol105_batch_solve", "sol105_batch_modes",
-    "sol105_batch_stats",
+    "sol105_batch_stats", "sol105_server_create", "sol105_server_destroy", "sol105_server_last_error",
+    "sol105_server_add_model", "sol105_server_submit", "sol105_server_poll", "sol105_server_wait",
+    "sol105_server_stats",
 ]
 
 
@@ -94,6 +97,16 @@ def lib():
         L.sol105_batch_solve.argtypes = [vp, vp, ctypes.c_int32, dp, ip, ip]
         L.sol105_batch_modes.argtypes = [vp, vp, ctypes.c_int32, dp]
         L.sol105_batch_stats.argtypes = [vp, vp, dp]
+        L.sol105_server_create.argtypes = [ctypes.c_int, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
+                                           ctypes.POINTER(vp)]
+        L.sol105_server_destroy.argtypes = [vp]
+        L.sol105_server_last_error.argtypes = [vp]
+        L.sol105_server_last_error.restype = ctypes.c_char_p
+        L.sol105_server_add_model.argtypes = [vp, ctypes.POINTER(ModelDesc), ip]
+        L.sol105_server_submit.argtypes = [vp, ctypes.c_int32, dp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]
+        L.sol105_server_poll.argtypes = [vp, ctypes.c_int64, dp, ip, ip, dp]
+        L.sol105_server_wait.argtypes = [vp, ctypes.c_int64, dp, ip, ip, dp]
+        L.sol105_server_stats.argtypes = [vp, dp]
         _lib = L
     return _lib
 
@@ -334,4 +347,77 @@ class Batch:
         self.ctx.check(self.ctx.L.sol105_batch_stats(self.ctx.h, self.h, _dp(out)))
         keys = ("ms", "lanczos_vectors", "block_steps", "fallbacks", "n_free", "front_bytes", "factor_flops",
                 "tree_height")
+        return dict(zip(keys, out.tolist()))
+
+
+class Server:
+    """Device-resident request server (sol105_server_*): meshes registered
+    once, designs submitted as shell thicknesses, requests served by worker
+    streams with stacking of small pending requests."""
+
+    def __init__(self, device=0, workers=4, hbm_budget=0.0, batching=True):
+        self.L = lib()
+        h = ctypes.c_void_p()
+        rc = self.L.sol105_server_create(int(device), int(workers), float(hbm_budget), 1 if batching else 0,
+                                         ctypes.byref(h))
+        if rc != SOL105_OK:
+            raise Sol105Error(rc, "could not create the server")
+        self.h = h
+        self.arrs = []
+
+    def _check(self, rc):
+        if rc != SOL105_OK:
+            raise Sol105Error(rc, self.L.sol105_server_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.L.sol105_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_model(self, arr):
+        d, keep = _model_desc(arr)
+        mid = ctypes.c_int32()
+        self._check(self.L.sol105_server_add_model(self.h, ctypes.byref(d), ctypes.byref(mid)))
+        self.arrs.append(arr)
+        return int(mid.value)
+
+    def submit(self, model_id, thickness=None, max_modes=16):
+        t = None if thickness is None else np.ascontiguousarray(thickness, dtype=np.float64)
+        if t is not None:
+            assert t.shape == (self.arrs[model_id].n_shell,)
+        tk = ctypes.c_int64()
+        self._check(self.L.sol105_server_submit(self.h, int(model_id), _dp(t) if t is not None else None,
+                                                int(max_modes), ctypes.byref(tk)))
+        return int(tk.value)
+
+    def _result(self, fn, ticket):
+        lam = np.zeros(16)
+        n = ctypes.c_int32()
+        st = ctypes.c_int32()
+        info = np.zeros(6)
+        rc = fn(self.h, ctypes.c_int64(ticket), _dp(lam), ctypes.byref(n), ctypes.byref(st), _dp(info))
+        if rc == SOL105_E_PENDING:
+            return None
+        self._check(rc)
+        keys = ("latency_ms", "queue_ms", "service_ms", "members", "start_ms", "finish_ms")
+        return lam[:n.value].copy(), int(st.value), dict(zip(keys, info.tolist()))
+
+    def poll(self, ticket):
+        """(eigenvalues, status, info) when done, else None."""
+        return self._result(self.L.sol105_server_poll, ticket)
+
+    def wait(self, ticket):
+        return self._result(self.L.sol105_server_wait, ticket)
+
+    def stats(self):
+        out = np.zeros(8)
+        self._check(self.L.sol105_server_stats(self.h, _dp(out)))
+        keys = ("done", "stacked_solves", "stacked_requests", "queued", "inflight_bytes", "budget_bytes",
+                "workers", "meshes")
         return dict(zip(keys, out.tolist()))

@@ -15,10 +15,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsol105_hip.so")
-SOURCES = ["elements.hip", "factor.hip", "solve.hip", "dense.hip", "rr_eig.hip", "symbolic.cpp", "solve_plan.cpp", "api.cpp", "batch.cpp"]
+SOURCES = ["elements.hip", "factor.hip", "solve.hip", "dense.hip", "rr_eig.hip", "symbolic.cpp", "solve_plan.cpp",
+           "api.cpp", "batch.cpp", "server.cpp"]
 ARCH = os.environ.get("SOL105_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wno-pass-failed",
+FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-pthread", "-Wno-pass-failed",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
@@ -36,7 +37,7 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp%d" % os.getpid()
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s" % r.stderr)

@@ -546,8 +546,10 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     const int kk = mmax >= 128 + b ? (SYMEIG_MAX_K / b) * b : std::max(b, (mmax / 2 / b) * b);
     const int max_restarts = 40;
     const int kz = std::max(kneed, kk);
-    DevBuf<double> V, Wb, xf, yf, T, A, Zs, thd, ework, part, small, H2, Ytmp;
-    DevBuf<int> ierr, sw;
+    LanczosWork &W = M->lw;
+    DevBuf<double> &V = W.V, &Wb = W.Wb, &xf = W.xf, &yf = W.yf, &T = W.T, &A = W.A, &Zs = W.Zs, &thd = W.thd,
+                   &ework = W.ework, &part = W.part, &small = W.small, &H2 = W.H2, &Ytmp = W.Ytmp;
+    DevBuf<int> &ierr = W.ierr, &sw = W.sw;
     HIPCHK(V.alloc((size_t)n * (mmax + b)));
     HIPCHK(Wb.alloc((size_t)n * b));
     HIPCHK(xf.alloc((size_t)ndof * b));
@@ -826,7 +828,7 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
     int req = dense ? 3 * nd : nd;
     req = std::min(req, std::max(n - 2, 1));
     if (req > MAXR) return set_err(ctx, SOL105_E_UNSUPPORTED, "more than %d requested modes", MAXR);
-    DevBuf<double> Y, Xf;
+    DevBuf<double> &Y = M->lw.Y, &Xf = M->lw.Xf;
     HIPCHK(Y.alloc((size_t)n * req));
     LanczosOut lo;
     int rc = lanczos(ctx, M, req, Y.p, lo);
@@ -861,7 +863,7 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
     int n_shell = M->n_q4 + M->n_t3;
     if (n_shell >= 100 && !pos.empty()) {
         int limit = std::min((int)pos.size(), std::max(nd + 16, nd));
-        DevBuf<double> scr, sh;
+        DevBuf<double> &scr = M->lw.scr, &sh = M->lw.sh;
         HIPCHK(scr.alloc(2 * ((n_shell + 255) / 256) + 2));
         HIPCHK(sh.alloc(2 * (size_t)limit));
         for (int t = 0; t < limit; t++) elem_energy_share(M->n_q4, M->d_q4.p, M->n_t3, M->d_t3.p, ndof, Xf.p, pos[t], scr.p, sh.p + 2 * t, s);
@@ -881,8 +883,8 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
     nout = std::min(nout, max_modes);
     HIPCHK(M->d_modes.alloc((size_t)ndof * std::max(nout, 1)));
     M->lambda.assign(nout, 0.0);
-    DevBuf<double> scr;
-    HIPCHK(scr.alloc(1024));
+    DevBuf<double> &scr = M->lw.scr;
+    HIPCHK(scr.alloc(std::max<size_t>(1024, scr.n)));
     for (int t = 0; t < nout; t++) {
         int i = surv[t];
         M->lambda[t] = lam[i];

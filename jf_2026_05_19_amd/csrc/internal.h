@@ -58,6 +58,14 @@ struct DevBuf {
 
 }  // namespace s105
 
+// Eigensolver workspace kept with a model, so repeated solves (design
+// loops, served requests) re-use the device buffers instead of allocating
+// and freeing them per solve.
+struct LanczosWork {
+    s105::DevBuf<double> V, Wb, xf, yf, T, A, Zs, thd, ework, part, small, H2, Ytmp, Y, Xf, scr, sh;
+    s105::DevBuf<int> ierr, sw;
+};
+
 struct sol105_model {
     // host copies
     int n_nodes = 0, n_q4 = 0, n_t3 = 0, n_bar = 0;
@@ -101,6 +109,7 @@ struct sol105_model {
     // node ranges (n_members + 1 offsets; empty for a single model) scope the
     // per-model AUTOSPC maxima
     std::vector<int32_t> mem_node_off;
+    LanczosWork lw;
     int ndof() const { return 6 * n_nodes; }
 };
 

@@ -16,6 +16,8 @@ parser (``JFEM/src/parsing/card_processor.jl``).
   longitudinal mesh line, clamped loaded edges, axial compression.
 * ``variant_sweep`` -- seeded design variants of the plate (mesh size,
   thickness, modulus).
+* ``served_families`` -- seeded stiffened-panel meshes for a request stream
+  (DOF log-uniform, 10 % of the quads split into CTRIA3, CBAR stringers).
 * ``cylinder_deck`` -- axially compressed cylindrical shell of flat CQUAD4
   facets with a circumferential band of CTRIA3 (each quad of the band split
   in two), clamped at z = 0, axial ring load at z = L.
@@ -24,7 +26,7 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["plate_deck", "panel_deck", "variant_sweep", "cylinder_deck"]
+__all__ = ["plate_deck", "panel_deck", "variant_sweep", "cylinder_deck", "served_families"]
 
 
 def _fmt(v):
@@ -145,6 +147,24 @@ def variant_sweep(n=256, seed=20260519, n_min=10, n_max=80):
         steel = bool(rng.integers(0, 2))
         E, nu = (210e3, 0.30) if steel else (70e3, 0.33)
         out.append((plate_deck(nx=m, ny=m, t=t, E=E, nu=nu), dict(mesh=m, t=t, E=E, nu=nu)))
+    return out
+
+
+def served_families(n=10, seed=20260520, dof_min=2e3, dof_max=5e5, nd=5):
+    """Seeded panel meshes of a request stream: DOF log-uniform in
+    [dof_min, dof_max] (nx = 2 ny, 5 mm cells), 10 % of the quads split into
+    two CTRIA3, CBAR stringers on about every tenth longitudinal line.
+    Returns (deck text, description) pairs."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        dof = float(np.exp(rng.uniform(np.log(dof_min), np.log(dof_max))))
+        ny = max(4, int(round(np.sqrt(dof / 12.0))))
+        nx = 2 * ny
+        every = max(2, ny // 10)
+        txt = panel_deck(nx=nx, ny=ny, a=5.0 * nx, b=5.0 * ny, stringer_every=every, nd=nd, tri_frac=0.10,
+                         tri_seed=seed + k)
+        out.append((txt, dict(nx=nx, ny=ny, ndof=6 * (nx + 1) * (ny + 1), stringer_every=every)))
     return out
 
 

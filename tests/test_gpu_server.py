@@ -1,0 +1,49 @@
+"""The request server (sol105_server_*) on the GPU, through the C ABI.
+
+A burst of design requests (thickness perturbations of +-5 % chained per
+mesh, as an optimiser's iterations) over several registered panel meshes
+with CTRIA3 and CBAR: every result must equal the same design solved alone
+(eigenvalues within 1e-8 relative), pending small requests must have been
+stacked into shared solves, and solves of different requests must have
+overlapped in time on the device.
+"""
+import numpy as np
+import pytest
+
+from jf_2026_05_19_amd import bdf, decks, model as M
+
+pytestmark = pytest.mark.gpu
+
+
+def test_served_results_equal_solves_alone():
+    from jf_2026_05_19_amd import _native as N
+    arrs = [M.build_arrays(bdf.bdf_to_model_text(t))
+            for t, _ in decks.served_families(n=6, seed=11, dof_min=2e3, dof_max=6e4)]
+    print("meshes", [a.ndof for a in arrs], flush=True)
+    srv = N.Server(workers=4)
+    ids = [srv.add_model(a) for a in arrs]
+    rng = np.random.default_rng(7)
+    cur = [a.shell_thickness() for a in arrs]
+    designs, tickets = [], []
+    for r in range(30):
+        f = int(rng.integers(len(arrs)))
+        cur[f] = cur[f] * rng.uniform(0.95, 1.05, size=cur[f].shape)
+        designs.append((f, cur[f].copy()))
+        tickets.append(srv.submit(ids[f], cur[f]))
+    results = [srv.wait(t) for t in tickets]
+    stats = srv.stats()
+    srv.close()
+    print("server stats", stats, flush=True)
+    ctx = N.Context(0)
+    for k, ((f, t), (lam, st, info)) in enumerate(zip(designs, results)):
+        assert st == 0, (k, st)
+        dm = N.DeviceModel(ctx, arrs[f].with_shell_thickness(t))
+        la, _ = dm.solve(16, want_modes=False)
+        dm.free()
+        assert lam.shape == la.shape, (k, lam, la)
+        assert np.all(np.abs(lam - la) <= 1e-8 * np.abs(la)), (k, lam, la)
+    ctx.close()
+    assert stats["done"] == 30 and stats["stacked_solves"] >= 1
+    iv = sorted({(i["start_ms"], i["finish_ms"]) for _, _, i in results})
+    overlap = any(iv[a][1] > iv[b][0] for a in range(len(iv)) for b in range(a + 1, len(iv)))
+    assert overlap, iv
