@@ -305,6 +305,75 @@ __global__ void __launch_bounds__(256) prod_kernel(DevFronts D, const int32_t *t
     }
 }
 
+// The same product for a block of right-hand sides on the matrix cores:
+// each wavefront owns 16 rows of the tile and runs the whole k-slice with
+// v_mfma_f64_16x16x4f64 (D[16x16] += A[16x4] B[4x16]; lane l supplies
+// A[l%16][l/16] and B[l/16][l%16] and holds D[l/16 + 4v][l%16], v = 0..3).
+// Every factor entry is loaded by exactly one lane and each lane reads one
+// right-hand-side value from LDS per four k, so neither LDS nor the vector
+// FMA pipe limits the stream of factor entries; no cross-wavefront
+// reduction is needed.  Sixteen 4-k steps (64 k) of loads are in flight per
+// lane.  Columns beyond nrhs are zero.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, const int32_t *tiles, const double *A,
+                                                        const int64_t *poff, double *W, double *part, int nrhs,
+                                                        int bwd, const int32_t *kwid, const int32_t *nks, double *X,
+                                                        int n_free) {
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int f = tiles[3 * tile], rt = tiles[3 * tile + 1], ks = tiles[3 * tile + 2];
+    const int ns = D.ns[f], m = D.m[f];
+    const int nrows = bwd ? ns : m, inner = bwd ? m : ns, lda = bwd ? ns : m;
+    const double *Af = A + poff[f];
+    double *V = W + D.vec_off[f] * nrhs;
+    const int kw = kwid[f];
+    const int k0 = ks * kw, k1 = min(inner, k0 + kw);
+    const int kl = k1 - k0, klp = (kl + 3) & ~3;
+    __shared__ double Vs[SOLVE_KMAX][MAXR];
+    for (int t = threadIdx.x; t < klp * MAXR; t += blockDim.x) {
+        const int r = t / klp, k = t % klp;
+        Vs[k][r] = (k < kl && r < nrhs) ? V[(k0 + k) + (int64_t)r * m] : 0.0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i0 = rt * RT + 16 * wv;
+    const int ia = i0 + (lane & 15), kq = lane >> 4, col = lane & 15;
+    const bool live = ia < nrows;
+    const double *a = Af + ia + (int64_t)(k0 + kq) * lda;
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    int k = 0;
+    for (; k + 64 <= klp; k += 64) {
+        double av[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            av[u] = (live && k + 4 * u + kq < kl) ? a[(int64_t)(k + 4 * u) * lda] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], Vs[k + 4 * u + kq][col], acc, 0, 0, 0);
+    }
+    for (; k < klp; k += 4) {
+        const double av = (live && k + kq < kl) ? a[(int64_t)k * lda] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Vs[k + kq][col], acc, 0, 0, 0);
+    }
+    const int rb = 16 * wv + (lane >> 4);  // this lane's rows within the tile: rb, rb + 4, rb + 8, rb + 12
+    if (nks[f] == 1) {
+        // single k-slice: the sum is final (no other workgroup of the launch
+        // reads x_piv or the update rows of V; fwd reads V[0:ns] only)
+        if (col < nrhs) {
+            const int piv0 = D.piv0[f];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = rt * RT + rb + 4 * v;
+                if (i >= nrows) continue;
+                if (bwd || i < ns) X[(piv0 + i) + (int64_t)col * n_free] = acc[v];
+                else V[i + (int64_t)col * m] -= acc[v];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; v++) part[((int64_t)tile * RT + rb + 4 * v) * MAXR + col] = acc[v];
+    }
+}
+
 // fixed-order reduction of the k-slices and write-back
 __global__ void fwd_finish_kernel(DevFronts D, const int32_t *flist, const int32_t *tbase, const int32_t *nks,
                                   const double *part, double *W, double *X, int n_free, int nrhs) {
@@ -418,8 +487,8 @@ void solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, cons
             prod_kernel<1><<<SH.n_fwd, 256, 0, s>>>(D, tl, Pbuf, DS.poff, W, part, nrhs, 0, DS.fwd_kw, DS.fwd_nks, X,
                                                     P.n_free);
         else
-            prod_kernel<MAXR><<<SH.n_fwd, 256, 0, s>>>(D, tl, Pbuf, DS.poff, W, part, nrhs, 0, DS.fwd_kw, DS.fwd_nks,
-                                                       X, P.n_free);
+            prod_mfma_kernel<<<SH.n_fwd, 256, 0, s>>>(D, tl, Pbuf, DS.poff, W, part, nrhs, 0, DS.fwd_kw, DS.fwd_nks,
+                                                      X, P.n_free);
         if (timed) pt->end(s);
         fwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.fwd_tbase_rel,
                                                                                        DS.fwd_nks, part, W, X,
@@ -442,8 +511,8 @@ void solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, cons
             prod_kernel<1><<<SH.n_bwd, 256, 0, s>>>(D, tl, PTbuf, DS.poff, W, part, nrhs, 1, DS.bwd_kw, DS.bwd_nks, X,
                                                     P.n_free);
         else
-            prod_kernel<MAXR><<<SH.n_bwd, 256, 0, s>>>(D, tl, PTbuf, DS.poff, W, part, nrhs, 1, DS.bwd_kw, DS.bwd_nks,
-                                                       X, P.n_free);
+            prod_mfma_kernel<<<SH.n_bwd, 256, 0, s>>>(D, tl, PTbuf, DS.poff, W, part, nrhs, 1, DS.bwd_kw, DS.bwd_nks,
+                                                      X, P.n_free);
         if (timed) pt->end(s);
         bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.bwd_tbase_rel,
                                                                                         DS.bwd_nks, part, X,
