@@ -151,15 +151,15 @@ def main():
 
 
 # HBM bytes per leaf-height launch of the dominant kernel from rocprofv3 --pmc
-# FETCH_SIZE (224.34 MB) + WRITE_SIZE (15.85 MB), each scaled by the same
+# FETCH_SIZE (229.68 MB) + WRITE_SIZE (15.98 MB), each scaled by the same
 # counter's reading for a 256 MiB device copy (factors 2.0 and 1.0); the runs
-# and the per-grid figures are in profiles/prod_kernel_c2_panel_pmc.txt.
-PROD_TRAFFIC_BYTES = 240.18e6
+# and the per-grid figures are in profiles/prod_mfma_kernel_c2_panel_pmc.txt.
+PROD_TRAFFIC_BYTES = 245.66e6
 
 
 def roofline(prof):
     """Roofline of the dominant kernel, the block triangular-solve product
-    (prod_kernel<16>): algorithmic bytes per launch (factor entries streamed
+    (prod_mfma_kernel): algorithmic bytes per launch (factor entries streamed
     once + the right-hand-side rows read and written) over the average launch
     duration, both from HIP events recorded on the solve stream around every
     leaf-height launch (the two largest grids, forward and backward) of the
@@ -171,7 +171,7 @@ def roofline(prof):
     per_launch = nbytes / launches
     achieved = per_launch / avg_s / 1e9
     peak = 8000.0  # MI355X HBM3E GB/s
-    return {"kernel": "prod_kernel<16> (block triangular solve, split-K product with the explicit factor inverse)",
+    return {"kernel": "prod_mfma_kernel (block triangular solve: FP64 MFMA product with the explicit factor inverse)",
             "bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s", "frac": achieved / peak,
             "traffic": PROD_TRAFFIC_BYTES, "algorithmic_bytes_per_launch": per_launch,
             "avg_launch_us": avg_s * 1e6, "launches": launches}

@@ -118,7 +118,7 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
     HIPCHK(M->d_piv.alloc(std::max(S.n_free, 1)));
     HIPCHK(M->d_kdiag_elim.alloc(std::max(S.n_free, 1)));
     HIPCHK(M->d_flag.alloc(std::max(S.n_free, 1)));
-    HIPCHK(M->d_W.alloc(std::max<int64_t>(S.vec_storage * MAXR, 1)));
+    HIPCHK(M->d_W.alloc(solve_w_size(S.vec_storage)));
     HIPCHK(M->d_err.alloc(4));
     DevFronts &D = M->dev;
     D.ns = M->f_ns.p; D.nu = M->f_nu.p; D.m = M->f_m.p; D.piv0 = M->f_piv0.p;
@@ -140,8 +140,10 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
         HIPCHK(M->s_pullptr.upload(SP.pull_ptr.data(), SP.pull_ptr.size(), s));
         HIPCHK(M->s_pullsrc.upload(SP.pull_src.data(), SP.pull_src.size(), s));
         HIPCHK(M->s_lblk.upload(SP.lblk.data(), SP.lblk.size(), s));
-        HIPCHK(M->d_P.alloc(std::max<int64_t>(SP.p_storage, 1)));
-        HIPCHK(M->d_PT.alloc(std::max<int64_t>(SP.p_storage, 1)));
+        HIPCHK(M->d_P.alloc(SP.p_storage + SP.p_pad));
+        HIPCHK(M->d_PT.alloc(SP.p_storage + SP.p_pad));
+        HIPCHK(hipMemsetAsync(M->d_P.p + SP.p_storage, 0, sizeof(double) * SP.p_pad, s));
+        HIPCHK(hipMemsetAsync(M->d_PT.p + SP.p_storage, 0, sizeof(double) * SP.p_pad, s));
         HIPCHK(M->d_part.alloc(std::max<int64_t>(solve_part_size(SP), 1)));
         DevSolve &DS = M->dsolve;
         DS.poff = M->s_poff.p;

@@ -37,6 +37,10 @@ struct SolveHeight {
 struct SolvePlan {
     std::vector<int64_t> poff;                  // per-front offset of P / PT (m * ns doubles)
     int64_t p_storage = 0;
+    // zero-filled tail of P / PT: the block-RHS product loads whole 64 x 64
+    // chunks of a front without masks, reaching at most 64 (max m + 1)
+    // entries past the front's end
+    int64_t p_pad = 0;
     std::vector<int32_t> fwd_tiles, bwd_tiles;  // (front, row tile, k slice) triples
     std::vector<int32_t> fwd_tbase, bwd_tbase;  // per front: first tile, relative to its height
     std::vector<int32_t> fwd_nks, bwd_nks;      // per front: k slices
@@ -53,6 +57,9 @@ struct SolvePlan {
 };
 
 inline int64_t solve_part_size(const SolvePlan &S) { return S.max_height_tiles * SOLVE_RT * MAXR; }
+// working-vector storage: rows of MAXR values plus the 64 rows the block-RHS
+// product may read past the last front (masked, never used)
+inline int64_t solve_w_size(int64_t vec_storage) { return (vec_storage + 64) * MAXR; }
 
 void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std::vector<int> &m,
                       const std::vector<int32_t> &flist, SolvePlan &S);

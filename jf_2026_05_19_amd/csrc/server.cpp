@@ -41,10 +41,10 @@ double model_footprint(const sol105_model *M) {
     const double ndof = M->ndof();
     if (!M->have_sym) return 40e3 * ndof;
     const double nf = M->sym.n_free;
-    double b = 8.0 * (double)M->sym.front_storage + 16.0 * (double)M->splan.p_storage;
+    double b = 8.0 * (double)M->sym.front_storage + 16.0 * (double)(M->splan.p_storage + M->splan.p_pad);
     b += 8.0 * 36.0 * 2.0 * (double)M->pat.n_blocks();
     b += 8.0 * 36.0 * (16.0 * M->n_q4 + 9.0 * M->n_t3 + 4.0 * M->n_bar);
-    b += 8.0 * (double)M->sym.vec_storage * MAXR;
+    b += 8.0 * (double)solve_w_size(M->sym.vec_storage);
     b += 8.0 * nf * std::min(1040.0, nf + 16.0) + 8.0 * 4.0 * ndof * 16.0;
     return b;
 }

@@ -183,44 +183,49 @@ __global__ void transpose_kernel(DevFronts D, const int32_t *flist, const double
 
 // --- solves ----------------------------------------------------------------
 
+// Working vectors: front f owns m_f rows at W + vec_off[f] * ldv, row-major
+// with row stride ldv = MAXR for a block of right-hand sides (one 128-byte
+// line per row) and 1 for a single one.
+
 // forward assembly: V_f = [x_piv ; 0] + sum over children of their w, one
 // thread per (row, rhs); each row pulls its child rows in fixed child order
 __global__ void fwd_pull_kernel(DevFronts D, DevSolve DS, const int32_t *flist, double *W, const double *X,
-                                int n_free, int nrhs) {
+                                int n_free, int nrhs, int ldv) {
     int f = flist[blockIdx.y];
     int ns = D.ns[f], m = D.m[f], piv0 = D.piv0[f];
     int64_t slot0 = D.vec_off[f];
-    double *V = W + slot0 * nrhs;
+    double *V = W + slot0 * ldv;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m * nrhs; t += gridDim.x * blockDim.x) {
-        int i = t % m, r = t / m;
+        int i = t / nrhs, r = t % nrhs;
         double v = (i < ns) ? X[(piv0 + i) + (int64_t)r * n_free] : 0.0;
         for (int q = DS.pull_ptr[slot0 + i]; q < DS.pull_ptr[slot0 + i + 1]; q++) {
             int c = DS.pull_src[2 * q], row = DS.pull_src[2 * q + 1];
-            v += W[D.vec_off[c] * nrhs + (int64_t)r * D.m[c] + row];
+            v += W[(D.vec_off[c] + row) * ldv + r];
         }
-        V[i + (int64_t)r * m] = v;
+        V[(int64_t)i * ldv + r] = v;
     }
 }
 
 // backward gather: V_f = [x_piv ; -x(update rows)]
 __global__ void bwd_gather_kernel(DevFronts D, const int32_t *flist, double *W, const double *X, int n_free,
-                                  int nrhs) {
+                                  int nrhs, int ldv) {
     int f = flist[blockIdx.y];
     int ns = D.ns[f], m = D.m[f];
     const int32_t *rows = D.rows + D.row_off[f];
-    double *V = W + D.vec_off[f] * nrhs;
+    double *V = W + D.vec_off[f] * ldv;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m * nrhs; t += gridDim.x * blockDim.x) {
-        int i = t % m, r = t / m;
+        int i = t / nrhs, r = t % nrhs;
         double v = X[rows[i] + (int64_t)r * n_free];
-        V[i + (int64_t)r * m] = (i < ns) ? v : -v;
+        V[(int64_t)i * ldv + r] = (i < ns) ? v : -v;
     }
 }
 
 // partial[tile][64][nrhs] = A[rows of tile, k-slice] * V[k-slice]
 // fwd: A = P_f (m x ns, ld m), input = V_f[0:ns];  bwd: A = PT_f (ns x m, ld ns), input = V_f[0:m]
-// Wavefront g of the workgroup streams k in [k0 + 32g, k0 + 32g + 32) for the
-// tile's 64 rows (one row per lane, 8 loads in flight per lane); the four
-// partial sums are added in wavefront order (deterministic).
+// Single right-hand side (row stride 1): wavefront g of the workgroup streams
+// a quarter of the k-slice for the tile's 64 rows (one row per lane, 16 loads
+// in flight per lane); the four partial sums are added in wavefront order
+// (deterministic).
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
 // on XCD b % 8), each with its own L2.  This bijection hands every XCD a
 // contiguous run of logical tiles, so the row tiles of one front (which all
@@ -230,10 +235,10 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
     return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
-template <int NR>
-__global__ void __launch_bounds__(256) prod_kernel(DevFronts D, const int32_t *tiles, const double *A,
-                                                   const int64_t *poff, double *W, double *part, int nrhs, int bwd,
-                                                   const int32_t *kwid, const int32_t *nks, double *X, int n_free) {
+__global__ void __launch_bounds__(256) prod1_kernel(DevFronts D, const int32_t *tiles, const double *A,
+                                                    const int64_t *poff, double *W, double *part, int bwd,
+                                                    const int32_t *kwid, const int32_t *nks, double *X, int n_free) {
+    constexpr int NR = 1, nrhs = 1;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     int f = tiles[3 * tile], rt = tiles[3 * tile + 1], ks = tiles[3 * tile + 2];
     int ns = D.ns[f], m = D.m[f];
@@ -253,7 +258,7 @@ __global__ void __launch_bounds__(256) prod_kernel(DevFronts D, const int32_t *t
     double (*red)[RT][NR + 1] = reinterpret_cast<double (*)[RT][NR + 1]>(lds);
     for (int t = threadIdx.x; t < (k1 - k0) * NR; t += blockDim.x) {
         int k = t / NR, r = t % NR;
-        Vs[k][r] = (r < nrhs) ? V[(k0 + k) + (int64_t)r * m] : 0.0;
+        Vs[k][r] = (r < nrhs) ? V[(int64_t)(k0 + k) * nrhs + r] : 0.0;
     }
     __syncthreads();
     int row = threadIdx.x % RT, g = threadIdx.x / RT;
@@ -294,7 +299,7 @@ __global__ void __launch_bounds__(256) prod_kernel(DevFronts D, const int32_t *t
             for (int r = 0; r < nrhs; r++) {
                 double s = acc[r] + red[0][row][r] + red[1][row][r] + red[2][row][r];
                 if (bwd || i < ns) X[(piv0 + i) + (int64_t)r * n_free] = s;
-                else V[i + (int64_t)r * m] -= s;
+                else V[(int64_t)i * nrhs + r] -= s;
             }
         } else {
             for (int r = 0; r < nrhs; r++) {
@@ -305,15 +310,24 @@ __global__ void __launch_bounds__(256) prod_kernel(DevFronts D, const int32_t *t
     }
 }
 
-// The same product for a block of right-hand sides on the matrix cores:
-// each wavefront owns 16 rows of the tile and runs the whole k-slice with
-// v_mfma_f64_16x16x4f64 (D[16x16] += A[16x4] B[4x16]; lane l supplies
-// A[l%16][l/16] and B[l/16][l%16] and holds D[l/16 + 4v][l%16], v = 0..3).
-// Every factor entry is loaded by exactly one lane and each lane reads one
-// right-hand-side value from LDS per four k, so neither LDS nor the vector
-// FMA pipe limits the stream of factor entries; no cross-wavefront
-// reduction is needed.  Sixteen 4-k steps (64 k) of loads are in flight per
-// lane.  Columns beyond nrhs are zero.
+// The same product for a block of right-hand sides (row stride MAXR) on the
+// matrix cores, v_mfma_f64_16x16x4f64 (D[16x16] += A[16x4] B[4x16]; lane l
+// supplies A[l%16][l/16] and B[l/16][l%16] and holds D[l/16 + 4v][l%16],
+// v = 0..3).  Each wavefront covers all 64 rows of the tile (four 16-row
+// blocks, four accumulators) over a quarter of the k-slice, so one B value
+// (V[k + l/16][l%16]: 512 contiguous bytes per wavefront) feeds four
+// products, and the factor is read in the column-major front layout, 128
+// contiguous bytes per block column.  A chunk of 8 four-k steps (32 factor
+// loads and 8 B loads per lane) is issued before its products.  The four
+// partial sums meet once in LDS and are added in wavefront order (the result
+// is bit-reproducible); the write-back is then coalesced, row-fastest into
+// the column-major x and column-fastest into the row-major update rows.
+// The factor loads are not masked.  Row blocks wholly past the front re-read
+// the last live block and steps past a wavefront's range re-read its last
+// step (cache hits, no new memory traffic; B is zero there); the remaining
+// overhang (rows past the front, which reach only discarded output rows, and
+// up to 3 k past the slice, which meet a zero B value) reads factor data of
+// later fronts or the zero tail of the buffer (p_pad), finite either way.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, const int32_t *tiles, const double *A,
@@ -324,72 +338,102 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, const int32
     const int f = tiles[3 * tile], rt = tiles[3 * tile + 1], ks = tiles[3 * tile + 2];
     const int ns = D.ns[f], m = D.m[f];
     const int nrows = bwd ? ns : m, inner = bwd ? m : ns, lda = bwd ? ns : m;
-    const double *Af = A + poff[f];
-    double *V = W + D.vec_off[f] * nrhs;
+    double *V = W + D.vec_off[f] * MAXR;
     const int kw = kwid[f];
-    const int k0 = ks * kw, k1 = min(inner, k0 + kw);
-    const int kl = k1 - k0, klp = (kl + 3) & ~3;
-    __shared__ double Vs[SOLVE_KMAX][MAXR];
-    for (int t = threadIdx.x; t < klp * MAXR; t += blockDim.x) {
-        const int r = t / klp, k = t % klp;
-        Vs[k][r] = (k < kl && r < nrhs) ? V[(k0 + k) + (int64_t)r * m] : 0.0;
+    const int k0 = ks * kw, kl = min(inner, k0 + kw) - k0;
+    const int nst = (kl + 3) >> 2;   // four-k steps of the slice
+    const int q = (nst + 3) >> 2;    // steps per wavefront
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int s0 = min(nst, wv * q), s1 = min(nst, s0 + q);
+    const int nb = min(4, (nrows - rt * RT + 15) >> 4);  // live 16-row blocks, >= 1
+    const int lane = threadIdx.x & 63, kq = lane >> 4, col = lane & 15;
+    const double *Aw = A + poff[f] + rt * RT + (int64_t)k0 * lda;  // wavefront-uniform
+    unsigned aoff[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) aoff[b] = (unsigned)((lane & 15) + 16 * min(b, nb - 1) + kq * lda);
+    const double *Vw = V + (int64_t)k0 * MAXR;
+    // B masks are bitwise (all ones or zero), so no load is placed under
+    // divergent control flow
+    const uint64_t cmask = col < nrhs ? ~0ull : 0ull;
+    dbl4 acc[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[b] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int s = s0; s < s1; s += 8) {
+        double av[8][4], bv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int ss = min(s + u, s1 - 1);
+            const double *pa = Aw + (int64_t)(4 * ss) * lda;
+#pragma unroll
+            for (int b = 0; b < 4; b++) av[u][b] = pa[aoff[b]];
+            const uint64_t msk = (s + u < s1 && 4 * ss + kq < kl) ? cmask : 0ull;
+            bv[u] = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, Vw[4 * ss * MAXR + lane]) & msk);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][b], bv[u], acc[b], 0, 0, 0);
+        // all 40 loads of the chunk are issued before its products
+        __builtin_amdgcn_sched_group_barrier(0x020, 40, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
     }
+    __shared__ double red[4][RT][MAXR + 1];
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) red[wv][16 * b + kq + 4 * v][col] = acc[b][v];
     __syncthreads();
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int i0 = rt * RT + 16 * wv;
-    const int ia = i0 + (lane & 15), kq = lane >> 4, col = lane & 15;
-    const bool live = ia < nrows;
-    const double *a = Af + ia + (int64_t)(k0 + kq) * lda;
-    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-    int k = 0;
-    for (; k + 64 <= klp; k += 64) {
-        double av[16];
-#pragma unroll
-        for (int u = 0; u < 16; u++)
-            av[u] = (live && k + 4 * u + kq < kl) ? a[(int64_t)(k + 4 * u) * lda] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 16; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], Vs[k + 4 * u + kq][col], acc, 0, 0, 0);
-    }
-    for (; k < klp; k += 4) {
-        const double av = (live && k + kq < kl) ? a[(int64_t)k * lda] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Vs[k + kq][col], acc, 0, 0, 0);
-    }
-    const int rb = 16 * wv + (lane >> 4);  // this lane's rows within the tile: rb, rb + 4, rb + 8, rb + 12
+    const int t = threadIdx.x, r0 = rt * RT;
+    auto sum = [&](int row, int c) { return red[0][row][c] + red[1][row][c] + red[2][row][c] + red[3][row][c]; };
     if (nks[f] == 1) {
         // single k-slice: the sum is final (no other workgroup of the launch
         // reads x_piv or the update rows of V; fwd reads V[0:ns] only)
-        if (col < nrhs) {
-            const int piv0 = D.piv0[f];
+        const int piv0 = D.piv0[f];
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int i = rt * RT + rb + 4 * v;
-                if (i >= nrows) continue;
-                if (bwd || i < ns) X[(piv0 + i) + (int64_t)col * n_free] = acc[v];
-                else V[i + (int64_t)col * m] -= acc[v];
+        for (int j = 0; j < 4; j++) {
+            const int e = t + 256 * j, row = e & 63, c = e >> 6, i = r0 + row;
+            if (i < nrows && c < nrhs && (bwd || i < ns)) X[(piv0 + i) + (int64_t)c * n_free] = sum(row, c);
+        }
+        if (!bwd) {
+            // update rows: the four old values are read together (rows
+            // clamped into the front) before any store
+            double old[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int e = t + 256 * j;
+                old[j] = V[(int64_t)min(r0 + (e >> 4), nrows - 1) * MAXR + (e & 15)];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int e = t + 256 * j, row = e >> 4, c = e & 15, i = r0 + row;
+                if (i >= ns && i < nrows && c < nrhs) V[(int64_t)i * MAXR + c] = old[j] - sum(row, c);
             }
         }
     } else {
 #pragma unroll
-        for (int v = 0; v < 4; v++) part[((int64_t)tile * RT + rb + 4 * v) * MAXR + col] = acc[v];
+        for (int j = 0; j < 4; j++) {
+            const int e = t + 256 * j, row = e >> 4, c = e & 15;
+            part[((int64_t)tile * RT + row) * MAXR + c] = sum(row, c);
+        }
     }
 }
 
 // fixed-order reduction of the k-slices and write-back
 __global__ void fwd_finish_kernel(DevFronts D, const int32_t *flist, const int32_t *tbase, const int32_t *nks,
-                                  const double *part, double *W, double *X, int n_free, int nrhs) {
+                                  const double *part, double *W, double *X, int n_free, int nrhs, int ldv) {
     int f = flist[blockIdx.y];
     int ns = D.ns[f], m = D.m[f], piv0 = D.piv0[f];
-    double *V = W + D.vec_off[f] * nrhs;
+    double *V = W + D.vec_off[f] * ldv;
     int nk = nks[f];
     if (nk == 1) return;  // written by the product kernel
     int64_t base = tbase[f];
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m * nrhs; t += gridDim.x * blockDim.x) {
-        int i = t % m, r = t / m;
+        int i = t / nrhs, r = t % nrhs;
         int rt = i / RT, rr = i % RT;
         double s = 0.0;
         for (int k = 0; k < nk; k++) s += part[((base + (int64_t)rt * nk + k) * RT + rr) * MAXR + r];
         if (i < ns) X[(piv0 + i) + (int64_t)r * n_free] = s;
-        else V[i + (int64_t)r * m] -= s;
+        else V[(int64_t)i * ldv + r] -= s;
     }
 }
 
@@ -474,42 +518,44 @@ static double prod_bytes(const SolveHeight &SH, int nrhs) {
 
 void solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *Pbuf,
                double *W, double *part, double *X, int nrhs, hipStream_t s, ProdTimer *pt) {
+    const int ldv = nrhs > 1 ? MAXR : 1;
     for (int h = 0; h <= P.height; h++) {
         const HeightInfo &H = P.heights[h];
         const SolveHeight &SH = S.heights[h];
         if (H.n == 0) continue;
         const int32_t *fl = D.flist + H.list_off;
-        fwd_pull_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, DS, fl, W, X, P.n_free, nrhs);
+        fwd_pull_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, DS, fl, W, X, P.n_free, nrhs,
+                                                                                     ldv);
         const bool timed = pt && nrhs > 1 && h == 0;
         if (timed) pt->begin(s, prod_bytes(SH, nrhs));
         const int32_t *tl = DS.fwd_tiles + 3 * SH.fwd_off;
         if (nrhs == 1)
-            prod_kernel<1><<<SH.n_fwd, 256, 0, s>>>(D, tl, Pbuf, DS.poff, W, part, nrhs, 0, DS.fwd_kw, DS.fwd_nks, X,
-                                                    P.n_free);
+            prod1_kernel<<<SH.n_fwd, 256, 0, s>>>(D, tl, Pbuf, DS.poff, W, part, 0, DS.fwd_kw, DS.fwd_nks, X, P.n_free);
         else
             prod_mfma_kernel<<<SH.n_fwd, 256, 0, s>>>(D, tl, Pbuf, DS.poff, W, part, nrhs, 0, DS.fwd_kw, DS.fwd_nks,
                                                       X, P.n_free);
         if (timed) pt->end(s);
         fwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.fwd_tbase_rel,
                                                                                        DS.fwd_nks, part, W, X,
-                                                                                       P.n_free, nrhs);
+                                                                                       P.n_free, nrhs, ldv);
     }
 }
 
 void solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *PTbuf,
                double *W, double *part, double *X, int nrhs, hipStream_t s, ProdTimer *pt) {
+    const int ldv = nrhs > 1 ? MAXR : 1;
     for (int h = P.height; h >= 0; h--) {
         const HeightInfo &H = P.heights[h];
         const SolveHeight &SH = S.heights[h];
         if (H.n == 0) continue;
         const int32_t *fl = D.flist + H.list_off;
-        bwd_gather_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, fl, W, X, P.n_free, nrhs);
+        bwd_gather_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, fl, W, X, P.n_free, nrhs,
+                                                                                       ldv);
         const bool timed = pt && nrhs > 1 && h == 0;
         if (timed) pt->begin(s, prod_bytes(SH, nrhs));
         const int32_t *tl = DS.bwd_tiles + 3 * SH.bwd_off;
         if (nrhs == 1)
-            prod_kernel<1><<<SH.n_bwd, 256, 0, s>>>(D, tl, PTbuf, DS.poff, W, part, nrhs, 1, DS.bwd_kw, DS.bwd_nks, X,
-                                                    P.n_free);
+            prod1_kernel<<<SH.n_bwd, 256, 0, s>>>(D, tl, PTbuf, DS.poff, W, part, 1, DS.bwd_kw, DS.bwd_nks, X, P.n_free);
         else
             prod_mfma_kernel<<<SH.n_bwd, 256, 0, s>>>(D, tl, PTbuf, DS.poff, W, part, nrhs, 1, DS.bwd_kw, DS.bwd_nks,
                                                       X, P.n_free);

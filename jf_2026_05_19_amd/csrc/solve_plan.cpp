@@ -16,11 +16,14 @@ void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std
     int nf = (int)ns.size();
     S.poff.assign(nf, 0);
     int64_t off = 0;
+    int max_m = 0;
     for (int f = 0; f < nf; f++) {
         S.poff[f] = off;
         off += (int64_t)m[f] * ns[f];
+        max_m = std::max(max_m, m[f]);
     }
     S.p_storage = off;
+    S.p_pad = 64 * ((int64_t)max_m + 1);
     S.fwd_tiles.clear();
     S.bwd_tiles.clear();
     S.fwd_tbase.assign(nf, 0);
