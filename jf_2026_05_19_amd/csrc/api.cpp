@@ -681,7 +681,8 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     // the next expansion fills T's new column block through CGS2.
     auto restart = [&]() {
         symmetrize_copy(m, ldT, T.p, A.p, s);
-        sym_eig_top(m, A.p, kk, thd.p, Zs.p, ework.p, sw.p, s);
+        // block tridiagonal (half-bandwidth b) until the first restart
+        sym_eig_top(m, A.p, kk, thd.p, Zs.p, ework.p, (int64_t)ework.n, sw.p, s, restarts ? 0 : b);
         std::vector<double> th(kk), zl((size_t)b * kk), R((size_t)b * b);
         HIPCHK(hipMemcpyAsync(th.data(), thd.p, sizeof(double) * kk, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpy2DAsync(zl.data(), b * sizeof(double), Zs.p + (m - b), (size_t)m * sizeof(double),
@@ -730,7 +731,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
         symmetrize_copy(m, ldT, T.p, A.p, s);
         // the kchk Ritz pairs of largest |theta| (A is overwritten)
         const int kchk = std::min(kneed, m);
-        sym_eig_top(m, A.p, kchk, thd.p, Zs.p, ework.p, sw.p, s);
+        sym_eig_top(m, A.p, kchk, thd.p, Zs.p, ework.p, (int64_t)ework.n, sw.p, s, restarts ? 0 : b);
         HIPCHK(hipGetLastError());
         std::vector<double> th(kchk), hZl((size_t)b * kchk);
         HIPCHK(hipMemcpyAsync(th.data(), thd.p, sizeof(double) * kchk, hipMemcpyDeviceToHost, s));

@@ -279,7 +279,7 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
         HIPCHK(hipMemcpyAsync(dlist.p, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, s));
         seg_symmetrize_copy(T, m, ldT, Tm.p, sT, A.p, (int64_t)mmax * mmax, s);
         sym_eig_top_batched(m, A.p, (int64_t)mmax * mmax, kchk, thd.p, kdim, Zs.p, (int64_t)mmax * kdim, ework.p,
-                            symeig_work_size(mmax, kdim), info.p, dlist.p, (int)list.size(), s);
+                            symeig_work_size(mmax, kdim), info.p, dlist.p, (int)list.size(), s, b);
         // last block row of the Ritz vectors and the trailing coupling block R_j
         seg_copy(T, b, kchk, Zs.p + (m - b), (int64_t)mmax * kdim, m, zl.p, (int64_t)b * kdim, b, s);
         seg_copy(T, b, b, Tm.p + (size_t)m + (size_t)(m - b) * ldT, sT, ldT, rl.p, (int64_t)b * b, b, s);

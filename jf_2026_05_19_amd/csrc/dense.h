@@ -38,14 +38,21 @@ void extract_kdiag(int n_nodes, const int32_t *diag_blk, const double *kv, doubl
 // dst[i + j*ldd] += src[i + j*lds]
 void mat_add(int rows, int cols, const double *src, int lds, double *dst, int ldd, hipStream_t s);
 // The k eigenpairs of largest |theta| of a symmetric m x m matrix A
-// (column-major, overwritten), one workgroup; m <= SYMEIG_MAX_M,
+// (column-major, may be overwritten), one workgroup; m <= SYMEIG_MAX_M,
 // k <= SYMEIG_MAX_K.  theta[t] in descending |theta| order, Zs (m x k, ld m)
-// the orthonormal eigenvectors; work >= symeig_work_size(m, k) doubles;
-// *info = 0 on success.
+// the orthonormal eigenvectors; work holds work_n >= symeig_work_size(m, k)
+// doubles; *info = 0 on success.  bw > 0 declares A banded with half-bandwidth bw
+// (block Lanczos without restarts): for 4 <= bw <= SYMEIG_BAND_MAX_W and
+// m <= SYMEIG_BAND_MAX_M the pairs then come from a band reduction, which
+// checks the claim (off-band entries within 1e-12 max|A|) and otherwise
+// leaves the matrix to the dense path.
 constexpr int SYMEIG_MAX_M = 1024;
 constexpr int SYMEIG_MAX_K = 64;
+constexpr int SYMEIG_BAND_MAX_M = 384;
+constexpr int SYMEIG_BAND_MAX_W = 16;
 int64_t symeig_work_size(int m, int k);
-void sym_eig_top(int m, double *A, int k, double *theta, double *Zs, double *work, int *info, hipStream_t s);
+void sym_eig_top(int m, double *A, int k, double *theta, double *Zs, double *work, int64_t work_n, int *info,
+                 hipStream_t s, int bw = 0);
 
 // ---------------------------------------------------------------------------
 // Segmented forms for a batch of independent problems stacked by rows (each
@@ -96,6 +103,6 @@ void seg_energy_share(int n_pairs, const int32_t *pairs, const int32_t *q4_off, 
 void seg_normalize(int n_pairs, const int32_t *pairs, const int64_t *dof_off, const int64_t *dst_off, int64_t ldx,
                    const double *X, double *dst, hipStream_t s);
 // sym_eig_top for the listed problems p (A + p sA, theta + p sT, Zs + p sZ,
-// work + p sW, info + p), one workgroup each, all of size m
+// work + p sW with sW doubles each, info + p), one workgroup each, all of size m
 void sym_eig_top_batched(int m, double *A, int64_t sA, int k, double *theta, int64_t sT, double *Zs, int64_t sZ,
-                         double *work, int64_t sW, int *info, const int *list, int nlist, hipStream_t s);
+                         double *work, int64_t sW, int *info, const int *list, int nlist, hipStream_t s, int bw = 0);

@@ -24,6 +24,7 @@
 // basis carries does not slow any step down.
 #include "dense.h"
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 
@@ -81,9 +82,214 @@ struct SymeigBatch {
     int64_t sA, sT, sZ, sW;
 };
 
+// LDS arrays shared by the tridiagonal stages (steps 2 and 3)
+struct TriLds {
+    double *dd, *ee, *e2, *scr, *ev, *wlo, *whi, *wem, *bnd;
+    int *sel, *asc, *ocol, *cl_start, *sint;
+    int scr_n;  // doubles of scr
+};
+
+// Steps 2 and 3 on the tridiagonal (dd, ee; ee[m-1] = 0) held in LDS: the
+// eigenvalues at the 2k extreme indices by multisection, the k of largest
+// |theta| into theta (descending |theta|) and their eigenvectors by inverse
+// iteration into Zs (m x k, ld m).  All NT threads of the workgroup call it;
+// sint[0] collects failures (bit 2: a vector did not come out finite).
+__device__ void tridiag_top_pairs(int m, int k, const TriLds &L, double *theta, double *Zs) {
+    double *dd = L.dd, *ee = L.ee, *e2 = L.e2, *scr = L.scr, *ev = L.ev, *wlo = L.wlo, *whi = L.whi, *wem = L.wem,
+           *bnd = L.bnd;
+    int *sel = L.sel, *asc = L.asc, *ocol = L.ocol, *cl_start = L.cl_start, *sint = L.sint;
+    const int scr_n = L.scr_n;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    // ---- 2. Gershgorin interval, then multisection at the extreme indices ----
+    for (int i = tid; i < m; i += NT) e2[i] = ee[i] * ee[i];
+    {
+        double lo = INFINITY, hi = -INFINITY, em = 0.0;
+        for (int i = tid; i < m; i += NT) {
+            double r = fabs(ee[i]) + (i > 0 ? fabs(ee[i - 1]) : 0.0);
+            lo = fmin(lo, dd[i] - r);
+            hi = fmax(hi, dd[i] + r);
+            em = fmax(em, ee[i] * ee[i]);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = fmin(lo, __shfl_xor(lo, o, 64));
+            hi = fmax(hi, __shfl_xor(hi, o, 64));
+            em = fmax(em, __shfl_xor(em, o, 64));
+        }
+        if (lane == 0) { wlo[wid] = lo; whi[wid] = hi; wem[wid] = em; }
+        __syncthreads();
+        if (tid == 0) {
+            lo = wlo[0]; hi = whi[0]; em = wem[0];
+            for (int i = 1; i < NW; i++) {
+                lo = fmin(lo, wlo[i]);
+                hi = fmax(hi, whi[i]);
+                em = fmax(em, wem[i]);
+            }
+            double tnorm = fmax(fabs(lo), fabs(hi));
+            double pivmin = DBL_MIN * fmax(1.0, em);
+            double pad = 2.0 * EPS * tnorm * m + 4.0 * pivmin;
+            bnd[0] = lo - pad;
+            bnd[1] = hi + pad;
+            bnd[2] = tnorm;
+            bnd[3] = pivmin;
+        }
+        __syncthreads();
+    }
+    const double tnorm = bnd[2], pivmin = bnd[3];
+    const int both = (2 * k >= m);          // every index is an extreme one
+    const int nidx = both ? m : 2 * k;
+    {
+        int G = 32;
+        while (G > 1 && nidx * G > NT) G >>= 1;
+        const int grp = tid / G, gl = tid % G;
+        const bool active = grp < nidx;
+        const int idx = both ? grp : (grp < k ? grp : m - 2 * k + grp);
+        const int shift = (lane / G) * G;
+        const uint64_t gmask = (1ull << G) - 1;
+        double lo = bnd[0], hi = bnd[1];
+        const double abstol = EPS * tnorm;
+        for (int it = 0; it < 200; it++) {
+            const double tol = fmax(abstol, 2.0 * EPS * fmax(fabs(lo), fabs(hi)));
+            if (!active || hi - lo <= tol) break;
+            const double w = hi - lo;
+            const double xg = lo + w * (double)(gl + 1) / (double)(G + 1);
+            const bool le = sturm_count(m, dd, e2, xg, pivmin) <= idx;
+            const uint64_t bal = (uint64_t)__ballot(le);
+            const int s = __popcll((bal >> shift) & gmask);
+            const double nlo = (s > 0) ? lo + w * (double)s / (double)(G + 1) : lo;
+            const double nhi = (s < G) ? lo + w * (double)(s + 1) / (double)(G + 1) : hi;
+            lo = nlo;
+            hi = fmax(nhi, nlo);
+        }
+        if (active && gl == 0) ev[grp] = 0.5 * (lo + hi);
+    }
+    __syncthreads();
+
+    // ---- selection of the k largest |theta| and clusters (one lane) ----
+    // eigenvalue of ascending index i, for i in [0, k) or [m - k, m)
+    auto ev_at = [&](int i) { return both ? ev[i] : (i < k ? ev[i] : ev[i - (m - 2 * k)]); };
+    if (tid == 0) {
+        int a = 0, b = m - 1;
+        for (int t = 0; t < k; t++) {
+            if (fabs(ev_at(b)) >= fabs(ev_at(a))) sel[t] = b--;
+            else sel[t] = a++;
+        }
+        // ascending list: [0, a) then (b, m)
+        int na = 0;
+        for (int i = 0; i < a; i++) asc[na++] = i;
+        for (int i = b + 1; i < m; i++) asc[na++] = i;
+        for (int t = 0; t < k; t++)
+            for (int u = 0; u < k; u++)
+                if (sel[u] == asc[t]) ocol[t] = u;
+        // clusters: consecutive indices with a gap below 1e-3 * ||T||
+        const double ortol = 1e-3 * tnorm;
+        int nc = 0;
+        for (int t = 0; t < k; t++) {
+            bool join = t > 0 && asc[t] == asc[t - 1] + 1 && (ev_at(asc[t]) - ev_at(asc[t - 1])) <= ortol;
+            if (!join) cl_start[nc++] = t;
+        }
+        cl_start[nc] = k;
+        sint[1] = nc;
+        for (int t = 0; t < k; t++) theta[t] = ev_at(sel[t]);
+    }
+    __syncthreads();
+
+    // ---- 3. inverse iteration, one lane per cluster, LU and vector in LDS ----
+    const int nc = sint[1];
+    const int CB = scr_n / (6 * m);  // clusters per batch (>= 2 for m <= 1024)
+    for (int c0 = 0; c0 < nc; c0 += CB) {
+        const int cid = c0 + tid;
+        if (tid < CB && cid < nc) {
+            double *D = scr + (size_t)tid * 6 * m;
+            double *DL = D + m, *DU = DL + m, *DU2 = DU + m, *PV = DU2 + m, *z = PV + m;
+            const double tiny = EPS * tnorm + pivmin;
+            double xprev = 0.0;
+            for (int t = cl_start[cid]; t < cl_start[cid + 1]; t++) {
+                double x = ev_at(asc[t]);
+                if (t > cl_start[cid]) {
+                    double pertol = 10.0 * EPS * fmax(fabs(x), tnorm * 1e-3);
+                    if (x <= xprev + pertol) x = xprev + pertol;
+                }
+                xprev = x;
+                // LU with partial pivoting of T - x I (rows i, i+1 swapped when PV[i] = 1);
+                // dcur/ucur carry D[i]/DU[i] of the row being eliminated
+                double dcur = dd[0] - x, ucur = ee[0];
+                for (int i = 0; i + 1 < m; i++) {
+                    const double l = ee[i];
+                    const double dn = dd[i + 1] - x, un = (i + 2 < m) ? ee[i + 1] : 0.0;
+                    if (fabs(dcur) >= fabs(l)) {
+                        const double f = (dcur != 0.0) ? l / dcur : 0.0;
+                        D[i] = dcur; DL[i] = f; DU[i] = ucur; DU2[i] = 0.0; PV[i] = 0.0;
+                        dcur = dn - f * ucur;
+                        ucur = un;
+                    } else {
+                        const double f = dcur / l;
+                        D[i] = l; DL[i] = f; DU[i] = dn; DU2[i] = un; PV[i] = 1.0;
+                        dcur = ucur - f * dn;
+                        ucur = -f * un;
+                    }
+                }
+                D[m - 1] = dcur;
+                DU[m - 1] = 0.0;
+                for (int i = 0; i < m; i++)
+                    if (fabs(D[i]) < tiny) D[i] = (D[i] < 0.0) ? -tiny : tiny;
+                for (int i = 0; i < m; i++) z[i] = hash_unit(((uint64_t)(asc[t] + 1) << 32) ^ (uint64_t)i);
+                // the eigenvalue is accurate to rounding, so two solves give the
+                // vector to working precision; a clustered one gets a third with
+                // re-orthogonalisation.  Solves are linear: normalise once at the end.
+                const int nit = (cl_start[cid + 1] - cl_start[cid] > 1) ? 3 : 2;
+                for (int it = 0; it < nit; it++) {
+                    // L solve, carrying the current row's value
+                    double cur = z[0];
+                    for (int i = 0; i + 1 < m; i++) {
+                        const double nxt = z[i + 1];
+                        if (PV[i] == 0.0) {
+                            z[i] = cur;
+                            cur = nxt - DL[i] * cur;
+                        } else {
+                            z[i] = nxt;
+                            cur = cur - DL[i] * nxt;
+                        }
+                    }
+                    z[m - 1] = cur;
+                    // U solve, carrying z[i+1], z[i+2]
+                    double z1 = z[m - 1] / D[m - 1], z2 = 0.0;
+                    z[m - 1] = z1;
+                    for (int i = m - 2; i >= 0; i--) {
+                        const double zi = (z[i] - DU[i] * z1 - DU2[i] * z2) / D[i];
+                        z[i] = zi;
+                        z2 = z1;
+                        z1 = zi;
+                    }
+                    // orthogonalise against the cluster's earlier (unit) vectors
+                    for (int u = cl_start[cid]; u < t; u++) {
+                        const double *zu = Zs + (int64_t)ocol[u] * m;
+                        double sdot = 0.0;
+                        for (int i = 0; i < m; i++) sdot += zu[i] * z[i];
+                        for (int i = 0; i < m; i++) z[i] -= sdot * zu[i];
+                    }
+                }
+                double nrm = 0.0;
+                for (int i = 0; i < m; i++) nrm += z[i] * z[i];
+                nrm = sqrt(nrm);
+                if (!(nrm > 0.0) || !isfinite(nrm)) {
+                    atomicOr(&sint[0], 2);
+                    nrm = 1.0;
+                }
+                const double inv = 1.0 / nrm;
+                for (int i = 0; i < m; i++) z[i] *= inv;
+                double *zo = Zs + (int64_t)ocol[t] * m;
+                for (int i = 0; i < m; i++) zo[i] = z[i];
+            }
+        }
+        __syncthreads();
+    }
+
+}
+
 template <int RPL, int CPI>
 __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k, double *theta, double *Zs,
-                                                        double *work, int *info, SymeigBatch bt) {
+                                                        double *work, int *info, SymeigBatch bt, int after_band) {
     if (bt.list) {
         const int64_t p = bt.list[blockIdx.x];
         A += p * bt.sA;
@@ -92,6 +298,8 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
         work += p * bt.sW;
         info += p;
     }
+    // behind the band kernel: run only where it declined (info < 0)
+    if (after_band && *info >= 0) return;
     __shared__ double vsh[NT];
     __shared__ double psh[NT];
     __shared__ double dd[NT], ee[NT], e2[NT];
@@ -213,189 +421,9 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
     }
     __syncthreads();
 
-    // ---- 2. Gershgorin interval, then multisection at the extreme indices ----
-    for (int i = tid; i < m; i += NT) e2[i] = ee[i] * ee[i];
     {
-        double lo = INFINITY, hi = -INFINITY, em = 0.0;
-        for (int i = tid; i < m; i += NT) {
-            double r = fabs(ee[i]) + (i > 0 ? fabs(ee[i - 1]) : 0.0);
-            lo = fmin(lo, dd[i] - r);
-            hi = fmax(hi, dd[i] + r);
-            em = fmax(em, ee[i] * ee[i]);
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            lo = fmin(lo, __shfl_xor(lo, o, 64));
-            hi = fmax(hi, __shfl_xor(hi, o, 64));
-            em = fmax(em, __shfl_xor(em, o, 64));
-        }
-        if (lane == 0) { wlo[wid] = lo; whi[wid] = hi; wem[wid] = em; }
-        __syncthreads();
-        if (tid == 0) {
-            lo = wlo[0]; hi = whi[0]; em = wem[0];
-            for (int i = 1; i < NW; i++) {
-                lo = fmin(lo, wlo[i]);
-                hi = fmax(hi, whi[i]);
-                em = fmax(em, wem[i]);
-            }
-            double tnorm = fmax(fabs(lo), fabs(hi));
-            double pivmin = DBL_MIN * fmax(1.0, em);
-            double pad = 2.0 * EPS * tnorm * m + 4.0 * pivmin;
-            bnd[0] = lo - pad;
-            bnd[1] = hi + pad;
-            bnd[2] = tnorm;
-            bnd[3] = pivmin;
-        }
-        __syncthreads();
-    }
-    const double tnorm = bnd[2], pivmin = bnd[3];
-    const int both = (2 * k >= m);          // every index is an extreme one
-    const int nidx = both ? m : 2 * k;
-    {
-        int G = 32;
-        while (G > 1 && nidx * G > NT) G >>= 1;
-        const int grp = tid / G, gl = tid % G;
-        const bool active = grp < nidx;
-        const int idx = both ? grp : (grp < k ? grp : m - 2 * k + grp);
-        const int shift = (lane / G) * G;
-        const uint64_t gmask = (1ull << G) - 1;
-        double lo = bnd[0], hi = bnd[1];
-        const double abstol = EPS * tnorm;
-        for (int it = 0; it < 200; it++) {
-            const double tol = fmax(abstol, 2.0 * EPS * fmax(fabs(lo), fabs(hi)));
-            if (!active || hi - lo <= tol) break;
-            const double w = hi - lo;
-            const double xg = lo + w * (double)(gl + 1) / (double)(G + 1);
-            const bool le = sturm_count(m, dd, e2, xg, pivmin) <= idx;
-            const uint64_t bal = (uint64_t)__ballot(le);
-            const int s = __popcll((bal >> shift) & gmask);
-            const double nlo = (s > 0) ? lo + w * (double)s / (double)(G + 1) : lo;
-            const double nhi = (s < G) ? lo + w * (double)(s + 1) / (double)(G + 1) : hi;
-            lo = nlo;
-            hi = fmax(nhi, nlo);
-        }
-        if (active && gl == 0) ev[grp] = 0.5 * (lo + hi);
-    }
-    __syncthreads();
-
-    // ---- selection of the k largest |theta| and clusters (one lane) ----
-    // eigenvalue of ascending index i, for i in [0, k) or [m - k, m)
-    auto ev_at = [&](int i) { return both ? ev[i] : (i < k ? ev[i] : ev[i - (m - 2 * k)]); };
-    if (tid == 0) {
-        int a = 0, b = m - 1;
-        for (int t = 0; t < k; t++) {
-            if (fabs(ev_at(b)) >= fabs(ev_at(a))) sel[t] = b--;
-            else sel[t] = a++;
-        }
-        // ascending list: [0, a) then (b, m)
-        int na = 0;
-        for (int i = 0; i < a; i++) asc[na++] = i;
-        for (int i = b + 1; i < m; i++) asc[na++] = i;
-        for (int t = 0; t < k; t++)
-            for (int u = 0; u < k; u++)
-                if (sel[u] == asc[t]) ocol[t] = u;
-        // clusters: consecutive indices with a gap below 1e-3 * ||T||
-        const double ortol = 1e-3 * tnorm;
-        int nc = 0;
-        for (int t = 0; t < k; t++) {
-            bool join = t > 0 && asc[t] == asc[t - 1] + 1 && (ev_at(asc[t]) - ev_at(asc[t - 1])) <= ortol;
-            if (!join) cl_start[nc++] = t;
-        }
-        cl_start[nc] = k;
-        sint[1] = nc;
-        for (int t = 0; t < k; t++) theta[t] = ev_at(sel[t]);
-    }
-    __syncthreads();
-
-    // ---- 3. inverse iteration, one lane per cluster, LU and vector in LDS ----
-    const int nc = sint[1];
-    const int CB = SCR / (6 * m);  // clusters per batch (>= 2 for m <= 1024)
-    for (int c0 = 0; c0 < nc; c0 += CB) {
-        const int cid = c0 + tid;
-        if (tid < CB && cid < nc) {
-            double *D = scr + (size_t)tid * 6 * m;
-            double *DL = D + m, *DU = DL + m, *DU2 = DU + m, *PV = DU2 + m, *z = PV + m;
-            const double tiny = EPS * tnorm + pivmin;
-            double xprev = 0.0;
-            for (int t = cl_start[cid]; t < cl_start[cid + 1]; t++) {
-                double x = ev_at(asc[t]);
-                if (t > cl_start[cid]) {
-                    double pertol = 10.0 * EPS * fmax(fabs(x), tnorm * 1e-3);
-                    if (x <= xprev + pertol) x = xprev + pertol;
-                }
-                xprev = x;
-                // LU with partial pivoting of T - x I (rows i, i+1 swapped when PV[i] = 1);
-                // dcur/ucur carry D[i]/DU[i] of the row being eliminated
-                double dcur = dd[0] - x, ucur = ee[0];
-                for (int i = 0; i + 1 < m; i++) {
-                    const double l = ee[i];
-                    const double dn = dd[i + 1] - x, un = (i + 2 < m) ? ee[i + 1] : 0.0;
-                    if (fabs(dcur) >= fabs(l)) {
-                        const double f = (dcur != 0.0) ? l / dcur : 0.0;
-                        D[i] = dcur; DL[i] = f; DU[i] = ucur; DU2[i] = 0.0; PV[i] = 0.0;
-                        dcur = dn - f * ucur;
-                        ucur = un;
-                    } else {
-                        const double f = dcur / l;
-                        D[i] = l; DL[i] = f; DU[i] = dn; DU2[i] = un; PV[i] = 1.0;
-                        dcur = ucur - f * dn;
-                        ucur = -f * un;
-                    }
-                }
-                D[m - 1] = dcur;
-                DU[m - 1] = 0.0;
-                for (int i = 0; i < m; i++)
-                    if (fabs(D[i]) < tiny) D[i] = (D[i] < 0.0) ? -tiny : tiny;
-                for (int i = 0; i < m; i++) z[i] = hash_unit(((uint64_t)(asc[t] + 1) << 32) ^ (uint64_t)i);
-                // the eigenvalue is accurate to rounding, so two solves give the
-                // vector to working precision; a clustered one gets a third with
-                // re-orthogonalisation.  Solves are linear: normalise once at the end.
-                const int nit = (cl_start[cid + 1] - cl_start[cid] > 1) ? 3 : 2;
-                for (int it = 0; it < nit; it++) {
-                    // L solve, carrying the current row's value
-                    double cur = z[0];
-                    for (int i = 0; i + 1 < m; i++) {
-                        const double nxt = z[i + 1];
-                        if (PV[i] == 0.0) {
-                            z[i] = cur;
-                            cur = nxt - DL[i] * cur;
-                        } else {
-                            z[i] = nxt;
-                            cur = cur - DL[i] * nxt;
-                        }
-                    }
-                    z[m - 1] = cur;
-                    // U solve, carrying z[i+1], z[i+2]
-                    double z1 = z[m - 1] / D[m - 1], z2 = 0.0;
-                    z[m - 1] = z1;
-                    for (int i = m - 2; i >= 0; i--) {
-                        const double zi = (z[i] - DU[i] * z1 - DU2[i] * z2) / D[i];
-                        z[i] = zi;
-                        z2 = z1;
-                        z1 = zi;
-                    }
-                    // orthogonalise against the cluster's earlier (unit) vectors
-                    for (int u = cl_start[cid]; u < t; u++) {
-                        const double *zu = Zs + (int64_t)ocol[u] * m;
-                        double sdot = 0.0;
-                        for (int i = 0; i < m; i++) sdot += zu[i] * z[i];
-                        for (int i = 0; i < m; i++) z[i] -= sdot * zu[i];
-                    }
-                }
-                double nrm = 0.0;
-                for (int i = 0; i < m; i++) nrm += z[i] * z[i];
-                nrm = sqrt(nrm);
-                if (!(nrm > 0.0) || !isfinite(nrm)) {
-                    atomicOr(&sint[0], 2);
-                    nrm = 1.0;
-                }
-                const double inv = 1.0 / nrm;
-                for (int i = 0; i < m; i++) z[i] *= inv;
-                double *zo = Zs + (int64_t)ocol[t] * m;
-                for (int i = 0; i < m; i++) zo[i] = z[i];
-            }
-        }
-        __syncthreads();
+        const TriLds L{dd, ee, e2, scr, ev, wlo, whi, wem, bnd, sel, asc, ocol, cl_start, sint, SCR};
+        tridiag_top_pairs(m, k, L, theta, Zs);
     }
 
     // ---- 4. back-transformation Zs <- H_0 ... H_{m-3} Zs, vectors in LDS ----
@@ -435,27 +463,289 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
     if (tid == 0) *info = sint[0];
 }
 
-}  // namespace
 
-int64_t symeig_work_size(int m, int k) {
-    (void)k;
-    return (int64_t)m;
+// ---------------------------------------------------------------------------
+// Banded projected matrices.  Without restarts the block Lanczos matrix is
+// block tridiagonal: half-bandwidth bw = the block size.  Its top pairs come
+// from a band-to-tridiagonal reduction by bulge chasing (Schwarz's method in
+// the Householder form of Bischof, Lang and Sun's successive band reduction,
+// the task structure of LAPACK's dsb2st_kernels): sweep i annihilates column
+// i below the subdiagonal with one reflector acting on rows [i+1, i+bw],
+// then chases the bulge down in blocks of bw rows; each step applies the
+// previous reflector from the right to the block below, annihilates that
+// block's first column with a new reflector and applies it from both sides
+// to the next diagonal block.  Sweeps run pipelined, sweep i three steps
+// behind sweep i-1 (their blocks are then disjoint and every overlapping pair
+// of tasks keeps its sequential order), one wavefront per active sweep and
+// one workgroup barrier per step, the band in LDS.  Steps 2 and 3 are those
+// of the dense path; the vectors are then mapped back through the stored
+// reflectors, sweep by sweep in reverse (one sweep's reflectors act on
+// disjoint rows and are applied together).  O(m^2 bw) work instead of the
+// dense path's O(m^3).
+
+constexpr int LDB = 2 * SYMEIG_BAND_MAX_W + 1;                 // band column: diagonal + bw + bulge
+constexpr int BAND_LDS = SYMEIG_BAND_MAX_M * LDB > SCR ? SYMEIG_BAND_MAX_M * LDB : SCR;
+constexpr int NSLOT = 64;                                      // previous-reflector slots (>= active sweeps)
+
+__device__ __forceinline__ int band_nsteps(int m, int bw, int i) { return (m - 1 - i + bw - 1) / bw; }
+// sum_{j=1}^{n} ceil(j / bw)
+__device__ __forceinline__ int64_t band_F(int n, int bw) {
+    const int64_t q = n / bw, r = n % bw;
+    return bw * q * (q + 1) / 2 + r * (q + 1);
+}
+// first task index of sweep i (tasks of sweeps 0..i-1)
+__device__ __forceinline__ int64_t band_toff(int m, int bw, int i) { return band_F(m - 1, bw) - band_F(m - 1 - i, bw); }
+
+// sum over the 16 lanes of a row group (lanes sharing l / 16)
+__device__ __forceinline__ double sum16(double v) {
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// sum over the 4 lanes sharing l % 16
+__device__ __forceinline__ double sum4(double v) {
+    v += __shfl_xor(v, 16, 64);
+    return v + __shfl_xor(v, 32, 64);
 }
 
-void sym_eig_top(int m, double *A, int k, double *theta, double *Zs, double *work, int *info, hipStream_t s) {
+// Householder reflector H = I - tau v v^T with v[0] = 1 and H x = mu e_0 for
+// the vector x (lane l holds x[l % 16], length L); Golub & Van Loan Alg. 5.1.1.
+__device__ __forceinline__ void house16(double x, int p, int L, double &v, double &tau, double &mu) {
+    const double alpha = __shfl(x, 0, 64);
+    const double sigma = sum16((p >= 1 && p < L) ? x * x : 0.0);
+    if (sigma > 0.0) {
+        mu = sqrt(alpha * alpha + sigma);
+        const double v1 = (alpha <= 0.0) ? alpha - mu : -sigma / (alpha + mu);
+        tau = 2.0 * v1 * v1 / (sigma + v1 * v1);
+        v = (p == 0) ? 1.0 : ((p < L) ? x / v1 : 0.0);
+    } else {
+        tau = 0.0;
+        mu = alpha;
+        v = (p == 0) ? 1.0 : 0.0;
+    }
+}
+
+// One task of sweep i, step s, by one wavefront.  AB: lower band, A[r][c]
+// (r >= c) at AB[c * LDB + r - c].  slot: the sweep's previous reflector
+// (v[0..15], tau at [16]); rec: this task's record for the back-transformation.
+__device__ void band_task(int m, int bw, int i, int s, double *AB, double *slot, double *rec) {
+    const int lane = threadIdx.x & 63, p = lane & 15, qg = lane >> 4;
+    auto at = [&](int r, int c) -> double & { return AB[c * LDB + (r - c)]; };
+    double v, tau, mu;
+    int r0, L;
+    if (s == 0) {
+        r0 = i + 1;
+        L = min(bw, m - r0);
+        const double x = (p < L) ? at(r0 + p, i) : 0.0;
+        house16(x, p, L, v, tau, mu);
+        if (qg == 0 && p < L) at(r0 + p, i) = (p == 0) ? mu : 0.0;
+    } else {
+        const int cs = i + 1 + (s - 1) * bw, Lp = min(bw, m - cs);
+        r0 = i + 1 + s * bw;
+        L = min(bw, m - r0);
+        const double tp = slot[16];
+        double c[4], vq[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int q = qg + 4 * j;
+            vq[j] = (q < Lp) ? slot[q] : 0.0;
+            c[j] = (p < L && q < Lp) ? at(r0 + p, cs + q) : 0.0;
+        }
+        // the block below the previous reflector's rows: C <- C H_prev
+        const double y = sum4(c[0] * vq[0] + c[1] * vq[1] + c[2] * vq[2] + c[3] * vq[3]);
+#pragma unroll
+        for (int j = 0; j < 4; j++) c[j] -= tp * y * vq[j];
+        // annihilate C's first column below its first row (lanes with qg == 0 hold it in c[0])
+        const double x = __shfl(c[0], p, 64);
+        house16(x, p, L, v, tau, mu);
+        if (qg == 0) c[0] = (p == 0) ? mu : 0.0;
+        // and apply the new reflector from the left to C's other columns
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const double z = sum16(v * c[j]);
+            if (qg + 4 * j >= 1) c[j] -= tau * v * z;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int q = qg + 4 * j;
+            if (p < L && q < Lp) at(r0 + p, cs + q) = c[j];
+        }
+    }
+    // two-sided on the diagonal block S = A[r0:r0+L, r0:r0+L]:
+    // p = tau S v, K = tau/2 p^T v, w = p - K v, S <- S - v w^T - w v^T
+    double sv[4], vq[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int q = qg + 4 * j;
+        vq[j] = __shfl(v, q, 64);
+        sv[j] = (p < L && q < L) ? at(r0 + max(p, q), r0 + min(p, q)) : 0.0;
+    }
+    const double pv = tau * sum4(sv[0] * vq[0] + sv[1] * vq[1] + sv[2] * vq[2] + sv[3] * vq[3]);
+    const double K = 0.5 * tau * sum16((p < L) ? pv * v : 0.0);
+    const double w = pv - K * v;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int q = qg + 4 * j;
+        const double wq = __shfl(w, q, 64);
+        if (p < L && q < L && p >= q) at(r0 + p, r0 + q) = sv[j] - (v * wq + w * vq[j]);
+    }
+    if (qg == 0) {
+        slot[p] = v;
+        if (p < bw) rec[p] = v;
+    }
+    if (lane == 0) {
+        slot[16] = tau;
+        rec[bw] = tau;
+    }
+}
+
+__global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const double *A, int k, double *theta,
+                                                         double *Zs, double *work, int *info, SymeigBatch bt) {
+    if (bt.list) {
+        const int64_t p = bt.list[blockIdx.x];
+        A += p * bt.sA;
+        theta += p * bt.sT;
+        Zs += p * bt.sZ;
+        work += p * bt.sW;
+        info += p;
+    }
+    __shared__ double AB[BAND_LDS];  // the band; afterwards scratch of steps 3 and 4
+    __shared__ double dd[SYMEIG_BAND_MAX_M], ee[SYMEIG_BAND_MAX_M], e2[SYMEIG_BAND_MAX_M];
+    __shared__ double slots[NSLOT][17];
+    __shared__ double ev[2 * SYMEIG_MAX_K];
+    __shared__ int sel[SYMEIG_MAX_K], asc[SYMEIG_MAX_K], ocol[SYMEIG_MAX_K], cl_start[SYMEIG_MAX_K + 1];
+    __shared__ double wlo[NW], whi[NW], wem[NW];
+    __shared__ double bnd[4];
+    __shared__ int sint[4];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+
+    // ---- 1. band into LDS (bulge room zeroed); decline a matrix that is not banded ----
+    for (int t = tid; t < m * LDB; t += NT) {
+        const int c = t / LDB, d = t % LDB;
+        AB[t] = (d <= bw && c + d < m) ? A[(c + d) + (int64_t)c * m] : 0.0;
+    }
+    {
+        double offm = 0.0, allm = 0.0;
+        for (int c = wid; c < m; c += NW)
+            for (int r = c + lane; r < m; r += 64) {
+                const double a = fabs(A[r + (int64_t)c * m]);
+                allm = fmax(allm, a);
+                if (r - c > bw) offm = fmax(offm, a);
+            }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            offm = fmax(offm, __shfl_xor(offm, o, 64));
+            allm = fmax(allm, __shfl_xor(allm, o, 64));
+        }
+        if (lane == 0) { wlo[wid] = offm; whi[wid] = allm; }
+        __syncthreads();
+        if (tid == 0) {
+            double o = 0.0, a = 0.0;
+            for (int i = 0; i < NW; i++) { o = fmax(o, wlo[i]); a = fmax(a, whi[i]); }
+            sint[0] = 0;
+            sint[2] = !(o <= 1e-12 * a);  // off-band entries beyond rounding (or NaN): the dense path
+        }
+        __syncthreads();
+        if (sint[2]) {
+            if (tid == 0) *info = -1;
+            return;
+        }
+    }
+
+    // ---- bulge chasing: time step ts runs step ts - 3i of every active sweep i ----
+    const int nsw = m - 2;  // sweeps 0 .. m-3
+    if (nsw > 0) {
+        const int tend = 3 * (nsw - 1) + band_nsteps(m, bw, nsw - 1);
+        for (int ts = 0; ts < tend; ts++) {
+            const int ihi = min(nsw - 1, ts / 3);
+            // this wavefront's sweeps are i == wid (mod NW); older sweeps finish first
+            for (int i = ihi - ((ihi - wid) % NW + NW) % NW; i >= 0; i -= NW) {
+                const int st = ts - 3 * i;
+                if (st >= band_nsteps(m, bw, i)) break;
+                band_task(m, bw, i, st, AB, slots[i % NSLOT], work + (band_toff(m, bw, i) + st) * (bw + 1));
+            }
+            __syncthreads();
+        }
+    }
+    for (int j = tid; j < m; j += NT) {
+        dd[j] = AB[j * LDB];
+        ee[j] = (j + 1 < m) ? AB[j * LDB + 1] : 0.0;
+    }
+    __syncthreads();
+
+    // ---- 2 and 3 on the tridiagonal ----
+    {
+        const TriLds L{dd, ee, e2, AB, ev, wlo, whi, wem, bnd, sel, asc, ocol, cl_start, sint, BAND_LDS};
+        tridiag_top_pairs(m, k, L, theta, Zs);
+    }
+    __syncthreads();
+
+    // ---- 4. back-transformation Zs <- Q Zs, Q the product of the sweeps' reflectors ----
+    const int cbat = BAND_LDS / m;
+    for (int c0 = 0; c0 < k; c0 += cbat) {
+        const int nv = min(cbat, k - c0);
+        double *Z = AB;  // nv columns of length m
+        for (int t = tid; t < nv * m; t += NT) Z[t] = Zs[(int64_t)c0 * m + t];
+        __syncthreads();
+        for (int i = nsw - 1; i >= 0; i--) {
+            const int nst = band_nsteps(m, bw, i);
+            const double *recs = work + band_toff(m, bw, i) * (bw + 1);
+            for (int t = tid; t < nst * nv; t += NT) {
+                const int st = t / nv, c = t % nv;
+                const int r0 = i + 1 + st * bw, L = min(bw, m - r0);
+                const double *rv = recs + st * (bw + 1);
+                const double tau = rv[bw];
+                if (tau == 0.0) continue;
+                double *z = Z + (size_t)c * m + r0;
+                double dot = 0.0;
+                for (int q = 0; q < L; q++) dot += rv[q] * z[q];
+                dot *= tau;
+                for (int q = 0; q < L; q++) z[q] -= dot * rv[q];
+            }
+            __syncthreads();
+        }
+        for (int t = tid; t < nv * m; t += NT) Zs[(int64_t)c0 * m + t] = Z[t];
+        __syncthreads();
+    }
+    if (tid == 0) *info = sint[0];
+}
+
+}  // namespace
+
+// band path (4 <= bw <= 16) at size m: one record of bw + 1 doubles per task,
+// at most m^2 (bw + 1) / (2 bw) + m (bw + 1) <= 5 m^2 / 8 + 17 m
+static int64_t band_work(int m) { return (int64_t)m + (5 * (int64_t)m * m) / 8 + 17 * (int64_t)m + 64; }
+
+// a workspace sized for m serves every call at sizes up to m: the dense path
+// needs m doubles, the band path band_work(m') for m' <= min(m, SYMEIG_BAND_MAX_M)
+int64_t symeig_work_size(int m, int k) {
+    (void)k;
+    return std::max<int64_t>(m, band_work(std::min(m, SYMEIG_BAND_MAX_M)));
+}
+
+static bool band_path(int m, int bw, int64_t work_n) {
+    return bw >= 4 && bw <= SYMEIG_BAND_MAX_W && m >= 3 && m <= SYMEIG_BAND_MAX_M && work_n >= band_work(m);
+}
+
+void sym_eig_top(int m, double *A, int k, double *theta, double *Zs, double *work, int64_t work_n, int *info,
+                 hipStream_t s, int bw) {
     const SymeigBatch one{nullptr, 0, 0, 0, 0};
+    const int band = band_path(m, bw, work_n);
+    if (band) symeig_band_kernel<<<1, NT, 0, s>>>(m, bw, A, k, theta, Zs, work, info, one);
     if (m <= 385)
-        symeig_top_kernel<6, 4><<<1, NT, 0, s>>>(m, A, k, theta, Zs, work, info, one);
+        symeig_top_kernel<6, 4><<<1, NT, 0, s>>>(m, A, k, theta, Zs, work, info, one, band);
     else
-        symeig_top_kernel<16, 1><<<1, NT, 0, s>>>(m, A, k, theta, Zs, work, info, one);
+        symeig_top_kernel<16, 1><<<1, NT, 0, s>>>(m, A, k, theta, Zs, work, info, one, band);
 }
 
 void sym_eig_top_batched(int m, double *A, int64_t sA, int k, double *theta, int64_t sT, double *Zs, int64_t sZ,
-                         double *work, int64_t sW, int *info, const int *list, int nlist, hipStream_t s) {
+                         double *work, int64_t sW, int *info, const int *list, int nlist, hipStream_t s, int bw) {
     if (nlist <= 0) return;
     const SymeigBatch bt{list, sA, sT, sZ, sW};
+    const int band = band_path(m, bw, sW);
+    if (band) symeig_band_kernel<<<nlist, NT, 0, s>>>(m, bw, A, k, theta, Zs, work, info, bt);
     if (m <= 385)
-        symeig_top_kernel<6, 4><<<nlist, NT, 0, s>>>(m, A, k, theta, Zs, work, info, bt);
+        symeig_top_kernel<6, 4><<<nlist, NT, 0, s>>>(m, A, k, theta, Zs, work, info, bt, band);
     else
-        symeig_top_kernel<16, 1><<<nlist, NT, 0, s>>>(m, A, k, theta, Zs, work, info, bt);
+        symeig_top_kernel<16, 1><<<nlist, NT, 0, s>>>(m, A, k, theta, Zs, work, info, bt, band);
 }
