@@ -12,11 +12,15 @@ constexpr int CH = 512;    // rows per partial-sum chunk
 constexpr int TR = 64;     // rows per LDS tile of the V^T W product
 constexpr int PC = 64;     // columns of V per workgroup
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
 // partial[c][p][q] = sum_{rows of chunk c} V[row, p] * W[row, q]   (Q <= 16)
 // Workgroup (c, g) takes rows [c CH, (c+1) CH) and columns [g PC, (g+1) PC) of
 // V.  Each 64-row tile of V (64 x 64) and W (64 x 16) is staged in LDS with
-// coalesced loads; lane (pl, ql) accumulates the four outputs
-// (pl + 16 i, ql), i = 0..3, so W is re-read once per 64 columns of V.
+// coalesced loads; wavefront w accumulates the 16 x 16 block of V columns
+// [16 w, 16 w + 16) against W on the matrix cores (v_mfma_f64_16x16x4f64,
+// A = V^T from LDS, B = W from LDS, 16 products per tile), so W is re-read
+// once per 64 columns of V.
 __device__ __forceinline__ void tn_partial_tile(int r0, int r1, int p0, int P, int Q, const double *V, int ldv,
                                                 const double *W, int ldw, double *part_c) {
     __shared__ double Vs[TR][PC + 1];
@@ -40,7 +44,8 @@ __device__ __forceinline__ void tn_partial_tile(int r0, int r1, int p0, int P, i
             wreg[u] = (row < r1 && q < Q) ? W[row + (int64_t)q * ldw] : 0.0;
         }
     };
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int lane = t & 63, wv = t >> 6, kq = lane >> 4, col = lane & 15;
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
     if (r0 < r1) load_tile(r0);
     for (int rb = r0; rb < r1; rb += TR) {
 #pragma unroll
@@ -49,19 +54,21 @@ __device__ __forceinline__ void tn_partial_tile(int r0, int r1, int p0, int P, i
         for (int u = 0; u < 4; u++) Ws[lr][lc + 4 * u] = wreg[u];
         __syncthreads();
         if (rb + TR < r1) load_tile(rb + TR);
-#pragma unroll 8
-        for (int r = 0; r < TR; r++) {
-            const double w = Ws[r][ql];
+        // D[p][q] += sum_r V[r][16 w + p] W[r][q]: lane l supplies A[l%16][l/16]
+        // = V[r + l/16][16 w + l%16] and B[l/16][l%16] = W[r + l/16][l%16]
 #pragma unroll
-            for (int i = 0; i < 4; i++) acc[i] += Vs[r][pl + 16 * i] * w;
-        }
+        for (int kk = 0; kk < TR / 4; kk++)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Vs[4 * kk + kq][16 * wv + col], Ws[4 * kk + kq][col], acc, 0, 0, 0);
         __syncthreads();
     }
-    if (ql < Q)
+    // lane l holds D[l/16 + 4 v][l%16]
+    (void)pl;
+    (void)ql;
+    if (col < Q)
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int p = p0 + pl + 16 * i;
-            if (p < P) part_c[(int64_t)p * Q + ql] = acc[i];
+        for (int v = 0; v < 4; v++) {
+            const int p = p0 + 16 * wv + kq + 4 * v;
+            if (p < P) part_c[(int64_t)p * Q + col] = acc[v];
         }
 }
 
@@ -72,23 +79,33 @@ __global__ void __launch_bounds__(256) tn_partial_kernel(int n, int P, int Q, co
 }
 
 // P, Q <= 16 (Gram matrices of one block): one 256-row chunk per workgroup,
-// both tiles in LDS after a single coalesced load each
+// both tiles in LDS after a single coalesced load each; wavefront w takes rows
+// [64 w, 64 w + 64) on the matrix cores and the four 16 x 16 partial sums are
+// added in wavefront order
 constexpr int CHS = 256;
 __device__ __forceinline__ void tn_small_tile(int r0, int r1, int P, int Q, const double *V, int ldv, const double *W,
                                               int ldw, double *part_c) {
     __shared__ double Vs[CHS][17];
     __shared__ double Ws[CHS][17];
+    __shared__ double red[4][16][17];
     const int t = threadIdx.x, row = r0 + t;
 #pragma unroll
     for (int p = 0; p < 16; p++) Vs[t][p] = (row < r1 && p < P) ? V[row + (int64_t)p * ldv] : 0.0;
 #pragma unroll
     for (int q = 0; q < 16; q++) Ws[t][q] = (row < r1 && q < Q) ? W[row + (int64_t)q * ldw] : 0.0;
     __syncthreads();
+    const int lane = t & 63, wv = t >> 6, kq = lane >> 4, col = lane & 15;
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+        const int r = 64 * wv + 4 * kk + kq;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Vs[r][col], Ws[r][col], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) red[wv][kq + 4 * v][col] = acc[v];
+    __syncthreads();
     const int pl = t % 16, ql = t / 16;
-    double acc = 0.0;
-#pragma unroll 8
-    for (int r = 0; r < CHS; r++) acc += Vs[r][pl] * Ws[r][ql];
-    if (pl < P && ql < Q) part_c[(int64_t)pl * Q + ql] = acc;
+    if (pl < P && ql < Q) part_c[(int64_t)pl * Q + ql] = red[0][pl][ql] + red[1][pl][ql] + red[2][pl][ql] + red[3][pl][ql];
 }
 
 __global__ void __launch_bounds__(256) tn_small_kernel(int n, int P, int Q, const double *V, int ldv, const double *W,
@@ -125,52 +142,72 @@ __global__ void __launch_bounds__(256) tn_reduce_kernel(int nch, int P, int Q, c
     tn_reduce_body(nch, P, Q, part, out, ldo, alpha, beta);
 }
 
-// Y[row, q] = beta*Y + alpha * sum_p V[row, p] H[p, q]   (Q <= 16), one row
-// per lane, H staged in LDS, eight columns of V in flight per lane
-__device__ __forceinline__ void nn_rows(int row, bool live, int P, int Q, const double *V, int ldv, const double *H,
+// Y[r0:r1, q] = beta*Y + alpha * sum_p V[row, p] H[p, q]   (Q <= 16,
+// r1 - r0 <= 256): four wavefronts of 64 rows, each as four 16-row blocks on
+// the matrix cores (v_mfma_f64_16x16x4f64; lane l supplies A[l%16][l/16] =
+// V[row + l%16, p + l/16] and B[l/16][l%16] = H[p + l/16, l%16] and holds
+// D[l/16 + 4 v][l%16]), one B value from LDS feeding four products.  H is
+// staged in LDS 256 rows at a time; a chunk of 8 four-p steps (32 loads of V
+// per lane) is issued before its products.  Row and column indices of the V
+// loads are clamped into [r0, r1) x [0, P): the extra rows only reach output
+// rows that are not written, the extra columns meet a zero B value.
+__device__ __forceinline__ void nn_rows(int r0, int r1, int P, int Q, const double *V, int ldv, const double *H,
                                         int ldh, double *Y, int ldy, double alpha, double beta) {
     __shared__ double Hs[256][16];
-    const double *vr = V + (live ? row : 0);
-    double acc[16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, kq = lane >> 4, col = lane & 15;
+    const int rw = r0 + 64 * wv;
+    int rc[4];
 #pragma unroll
-    for (int q = 0; q < 16; q++) acc[q] = 0.0;
+    for (int b = 0; b < 4; b++) rc[b] = min(rw + 16 * b + col, r1 - 1);
+    dbl4 acc[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[b] = dbl4{0.0, 0.0, 0.0, 0.0};
     for (int pb = 0; pb < P; pb += 256) {
         const int pn = min(256, P - pb);
         __syncthreads();
         for (int t = threadIdx.x; t < 256 * 16; t += blockDim.x) {
-            int p = t / 16, q = t % 16;
+            const int p = t / 16, q = t % 16;
             Hs[p][q] = (p < pn && q < Q) ? H[(pb + p) + (int64_t)q * ldh] : 0.0;
         }
         __syncthreads();
-        if (live) {
-            int p = 0;
-            for (; p + 8 <= pn; p += 8) {
-                double v[8];
+        if (rw >= r1) continue;  // wavefront-uniform: no rows here
+        const int nst = (pn + 3) >> 2;
+        for (int st = 0; st < nst; st += 8) {
+            double av[8][4], bv[8];
 #pragma unroll
-                for (int u = 0; u < 8; u++) v[u] = vr[(int64_t)(pb + p + u) * ldv];
+            for (int u = 0; u < 8; u++) {
+                const int ss = min(st + u, nst - 1);
+                const double *vc = V + (int64_t)min(pb + 4 * ss + kq, P - 1) * ldv;
 #pragma unroll
-                for (int u = 0; u < 8; u++)
-#pragma unroll
-                    for (int q = 0; q < 16; q++) acc[q] += v[u] * Hs[p + u][q];
+                for (int b = 0; b < 4; b++) av[u][b] = vc[rc[b]];
+                bv[u] = (st + u < nst) ? Hs[4 * ss + kq][col] : 0.0;
             }
-            for (; p < pn; p++) {
-                const double v = vr[(int64_t)(pb + p) * ldv];
 #pragma unroll
-                for (int q = 0; q < 16; q++) acc[q] += v * Hs[p][q];
-            }
+            for (int u = 0; u < 8; u++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][b], bv[u], acc[b], 0, 0, 0);
+            __builtin_amdgcn_sched_group_barrier(0x020, 32, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
         }
     }
-    if (live)
-        for (int q = 0; q < Q; q++) {
-            double *y = Y + row + (int64_t)q * ldy;
-            *y = alpha * acc[q] + (beta == 0.0 ? 0.0 : beta * *y);
-        }
+    if (col < Q && rw < r1)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = rw + 16 * b + kq + 4 * v;
+                if (r < r1) {
+                    double *y = Y + r + (int64_t)col * ldy;
+                    *y = alpha * acc[b][v] + (beta == 0.0 ? 0.0 : beta * *y);
+                }
+            }
 }
 
 __global__ void __launch_bounds__(256) nn_kernel(int n, int P, int Q, const double *V, int ldv, const double *H,
                                                  int ldh, double *Y, int ldy, double alpha, double beta) {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    nn_rows(row, row < n, P, Q, V, ldv, H, ldh, Y, ldy, alpha, beta);
+    const int r0 = blockIdx.x * 256;
+    nn_rows(r0, min(n, r0 + 256), P, Q, V, ldv, H, ldh, Y, ldy, alpha, beta);
 }
 
 // Cholesky G = R^T R (upper R, b <= 16) in place, one wavefront: step j
@@ -611,9 +648,8 @@ __global__ void __launch_bounds__(256) seg_nn_kernel(SegTable T, int P, int Q, c
                                                      double alpha, double beta) {
     const int c = blockIdx.x, sg = T.chs_seg[c];
     if (!T.active[sg]) return;
-    const int row = T.chs_r0[c] + threadIdx.x;
-    nn_rows(row, row < min(T.chs_r0[c] + CHS, T.seg_off[sg + 1]), P, Q, V, ldv, H + sg * sh, ldh, Y, ldy, alpha,
-            beta);
+    const int r0 = T.chs_r0[c];
+    nn_rows(r0, min(r0 + CHS, T.seg_off[sg + 1]), P, Q, V, ldv, H + sg * sh, ldh, Y, ldy, alpha, beta);
 }
 
 __global__ void __launch_bounds__(64) seg_chol_kernel(SegTable T, int b, double *G, int64_t sgs, int ldg,
