@@ -6,7 +6,7 @@
 // matrices are extend-added into their parent in a fixed child order (no
 // atomics, bit-reproducible), then each front is partially factored with a
 // right-looking blocked algorithm (panel width 32):
-//   potrf of the 32x32 diagonal block (one workgroup per front),
+//   potrf of the 32x32 diagonal block (one wavefront per front),
 //   trsm of the panel rows below it, and
 //   a 64x64-tiled syrk update of the trailing lower triangle.
 // The first ns columns of every front then hold L11 / L21.
@@ -24,6 +24,8 @@ namespace {
 
 constexpr int NB = 32;   // panel width
 constexpr int TS = 64;   // syrk tile
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __global__ void fill_kernel(double *p, int64_t n, double v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -53,40 +55,45 @@ __global__ void extend_add_kernel(DevFronts D, const int32_t *flist, int slot, d
     }
 }
 
-__global__ void potrf_diag_kernel(DevFronts D, const int32_t *flist, int k0, double *F, double *piv, int *err) {
+// Cholesky of the nb x nb diagonal block (nb <= 32) by one wavefront: lane l
+// holds row l of the block in registers; column j is finished by lane j's
+// diagonal and broadcast by shuffles, so the 32 columns need no barrier.
+__global__ void __launch_bounds__(64) potrf_diag_kernel(DevFronts D, const int32_t *flist, int k0, double *F,
+                                                        double *piv, int *err) {
     int f = flist[blockIdx.x];
     int ns = D.ns[f];
     if (k0 >= ns) return;
-    int nb = min(NB, ns - k0), m = D.m[f];
+    const int nb = min(NB, ns - k0), m = D.m[f];
     double *A = F + D.off[f];
-    __shared__ double L[NB][NB + 1];
-    int tid = threadIdx.x;
-    for (int t = tid; t < nb * nb; t += blockDim.x) {
-        int i = t % nb, j = t / nb;
-        L[i][j] = (i >= j) ? A[(k0 + i) + (int64_t)(k0 + j) * m] : 0.0;
-    }
-    __syncthreads();
-    for (int j = 0; j < nb; j++) {
-        if (tid == 0) {
-            double d = L[j][j];
-            piv[D.piv0[f] + k0 + j] = d;
-            if (!(d > 0.0)) { atomicOr(err, 1); d = 1.0; }
-            L[j][j] = sqrt(d);
+    const int l = threadIdx.x, lr = min(l, nb - 1);
+    double a[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) a[j] = A[(k0 + lr) + (int64_t)(k0 + min(j, nb - 1)) * m];
+#pragma unroll
+    for (int j = 0; j < NB; j++) a[j] = (l < nb && j <= l && j < nb) ? a[j] : 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        if (j < nb) {
+            double d = __shfl(a[j], j, 64);
+            if (l == 0) {
+                piv[D.piv0[f] + k0 + j] = d;
+                if (!(d > 0.0)) atomicOr(err, 1);
+            }
+            if (!(d > 0.0)) d = 1.0;
+            const double sd = sqrt(d);
+            const double lij = (l == j) ? sd : ((l > j) ? a[j] / sd : 0.0);
+            a[j] = lij;
+#pragma unroll
+            for (int k = j + 1; k < NB; k++) {
+                const double lkj = __shfl(lij, k, 64);
+                if (k <= l) a[k] -= lij * lkj;
+            }
         }
-        __syncthreads();
-        for (int i = j + 1 + tid; i < nb; i += blockDim.x) L[i][j] /= L[j][j];
-        __syncthreads();
-        int w = nb - j - 1;
-        for (int t = tid; t < w * w; t += blockDim.x) {
-            int ii = j + 1 + t % w, ll = j + 1 + t / w;
-            if (ii >= ll) L[ii][ll] -= L[ii][j] * L[ll][j];
-        }
-        __syncthreads();
     }
-    for (int t = tid; t < nb * nb; t += blockDim.x) {
-        int i = t % nb, j = t / nb;
-        if (i >= j) A[(k0 + i) + (int64_t)(k0 + j) * m] = L[i][j];
-    }
+    if (l < nb)
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            if (j <= l) A[(k0 + l) + (int64_t)(k0 + j) * m] = a[j];
 }
 
 __global__ void trsm_panel_kernel(DevFronts D, const int32_t *flist, int k0, double *F) {
@@ -96,9 +103,20 @@ __global__ void trsm_panel_kernel(DevFronts D, const int32_t *flist, int k0, dou
     int nb = min(NB, ns - k0), m = D.m[f];
     double *A = F + D.off[f];
     __shared__ double L[NB][NB + 1];
-    for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
-        int i = t % NB, j = t / NB;
-        L[i][j] = (i < nb && j < nb && i >= j) ? A[(k0 + i) + (int64_t)(k0 + j) * m] : 0.0;
+    {
+        // all loads issued together (clamped indices), masks applied after
+        constexpr int U = NB * NB / 256;
+        double tmp[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = threadIdx.x + 256 * u, i = t % NB, j = t / NB;
+            tmp[u] = A[(k0 + min(i, nb - 1)) + (int64_t)(k0 + min(j, nb - 1)) * m];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = threadIdx.x + 256 * u, i = t % NB, j = t / NB;
+            L[i][j] = (i < nb && j < nb && i >= j) ? tmp[u] : 0.0;
+        }
     }
     __syncthreads();
     int i = k0 + nb + blockIdx.x * blockDim.x + threadIdx.x;
@@ -121,9 +139,15 @@ __global__ void trsm_panel_kernel(DevFronts D, const int32_t *flist, int k0, dou
 }
 
 // Trailing update A[i][j] -= sum_l A[i][k0+l] A[j][k0+l] for i >= j >= k1 on
-// 64x64 tiles of the lower triangle.  256 threads, each owning a 4 x 4
-// micro-tile (rows tx + 16a, columns ty + 16b) fed from LDS copies of the
-// two 64 x 32 panel slices.
+// 64x64 tiles of the lower triangle, on the matrix cores.  The two 64 x 32
+// panel slices are staged in LDS; wavefront w takes the 32 x 32 quadrant
+// (rows 32 (w & 1), columns 32 (w >> 1)) as 2 x 2 blocks of
+// v_mfma_f64_16x16x4f64 over the panel width, computing the transposed block
+// D^T[j][i] = sum_l Lc[l][j] Lr[l][i] so that each lane's outputs
+// (D^T[l/16 + 4 v][l%16]) run down consecutive rows of the column-major front.
+// The old tile is read before the product (indices clamped into the front, no
+// conditional read) and parked in LDS, so the read-modify-write costs one
+// memory latency per tile however few tiles a CU holds.
 __global__ void __launch_bounds__(256) syrk_update_kernel(DevFronts D, const int32_t *flist, int k0, double *F) {
     int f = flist[blockIdx.y];
     int ns = D.ns[f];
@@ -141,41 +165,66 @@ __global__ void __launch_bounds__(256) syrk_update_kernel(DevFronts D, const int
     int tj = t - ti * (ti + 1) / 2;
     int r0 = k1 + ti * TS, c0 = k1 + tj * TS;
     double *A = F + D.off[f];
-    __shared__ double Lr[NB][TS + 1];
-    __shared__ double Lc[NB][TS + 1];
-    for (int q = threadIdx.x; q < NB * TS; q += blockDim.x) {
-        int r = q % TS, l = q / TS;
-        bool ok = l < nb;
-        Lr[l][r] = (ok && r0 + r < m) ? A[(r0 + r) + (int64_t)(k0 + l) * m] : 0.0;
-        Lc[l][r] = (ok && c0 + r < m) ? A[(c0 + r) + (int64_t)(k0 + l) * m] : 0.0;
+    __shared__ double lds[2 * NB * (TS + 1)];  // the panels, then the old tile (TS x (TS + 1))
+    double (*Lr)[TS + 1] = reinterpret_cast<double (*)[TS + 1]>(lds);
+    double (*Lc)[TS + 1] = reinterpret_cast<double (*)[TS + 1]>(lds + NB * (TS + 1));
+    double (*Ot)[TS + 1] = reinterpret_cast<double (*)[TS + 1]>(lds);
+    const int tid = threadIdx.x;
+    double oldr[TS / 4];
+    {
+        const int64_t i = min(r0 + (tid & 63), m - 1);
+#pragma unroll
+        for (int u = 0; u < TS / 4; u++) oldr[u] = A[i + (int64_t)min(c0 + (tid >> 6) + 4 * u, m - 1) * m];
+    }
+    // panel slices: all loads issued together (clamped indices, the masks
+    // applied to the loaded values) before the LDS writes
+    constexpr int PL = NB * TS / 256;
+    double lr[PL], lc[PL];
+#pragma unroll
+    for (int u = 0; u < PL; u++) {
+        const int q = tid + 256 * u, r = q % TS, l = q / TS;
+        const int64_t cl = (int64_t)(k0 + min(l, nb - 1)) * m;
+        lr[u] = A[min(r0 + r, m - 1) + cl];
+        lc[u] = A[min(c0 + r, m - 1) + cl];
+    }
+#pragma unroll
+    for (int u = 0; u < PL; u++) {
+        const int q = tid + 256 * u, r = q % TS, l = q / TS;
+        Lr[l][r] = (l < nb && r0 + r < m) ? lr[u] : 0.0;
+        Lc[l][r] = (l < nb && c0 + r < m) ? lc[u] : 0.0;
     }
     __syncthreads();
-    int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-    double acc[4][4];
+    const int lane = tid & 63, wv = tid >> 6, kq = lane >> 4, col = lane & 15;
+    const int qi = 32 * (wv & 1), qj = 32 * (wv >> 1);
+    dbl4 acc[2][2];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int jb = 0; jb < 2; jb++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
-    for (int l = 0; l < NB; l++) {
-        double ra[4], cb[4];
+        for (int ib = 0; ib < 2; ib++) acc[jb][ib] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int a = 0; a < 4; a++) ra[a] = Lr[l][tx + 16 * a];
-#pragma unroll
-        for (int b = 0; b < 4; b++) cb[b] = Lc[l][ty + 16 * b];
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int b = 0; b < 4; b++) acc[a][b] += ra[a] * cb[b];
+    for (int kk = 0; kk < NB / 4; kk++) {
+        const int l = 4 * kk + kq;
+        const double a0 = Lc[l][qj + col], a1 = Lc[l][qj + 16 + col];
+        const double b0 = Lr[l][qi + col], b1 = Lr[l][qi + 16 + col];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
+    __syncthreads();  // the panels are no longer read
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
-        int j = c0 + ty + 16 * b;
+    for (int u = 0; u < TS / 4; u++) Ot[tid & 63][(tid >> 6) + 4 * u] = oldr[u];
+    __syncthreads();
 #pragma unroll
-        for (int a = 0; a < 4; a++) {
-            int i = r0 + tx + 16 * a;
-            if (i < m && j < m && i >= j) A[i + (int64_t)j * m] -= acc[a][b];
-        }
-    }
+    for (int jb = 0; jb < 2; jb++)
+#pragma unroll
+        for (int ib = 0; ib < 2; ib++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int il = qi + 16 * ib + col, jl = qj + 16 * jb + kq + 4 * v;
+                const int i = r0 + il, j = c0 + jl;
+                if (i < m && j < m && i >= j) A[i + (int64_t)j * m] = Ot[il][jl] - acc[jb][ib][v];
+            }
 }
 
 // pivot ratio check of the post-factorisation refix (boundary_conditions.jl:742-782):
@@ -288,7 +337,7 @@ void factor_numeric(const FactorPlan &P, const DevFronts &D, const double *kvals
                 extend_add_kernel<<<dim3(gx, H.n), 256, 0, s>>>(D, fl, slot, F);
         }
         for (int k0 = 0; k0 < H.max_ns; k0 += NB) {
-            potrf_diag_kernel<<<H.n, 256, 0, s>>>(D, fl, k0, F, piv, err);
+            potrf_diag_kernel<<<H.n, 64, 0, s>>>(D, fl, k0, F, piv, err);
             // rows below the panel: m - k0 - nb for each front (nb <= NB), so
             // max_m - k0 bounds every front of this height; kernels clip to m.
             int rows = H.max_m - k0;

@@ -38,9 +38,20 @@ __global__ void __launch_bounds__(64) diag_inv_kernel(DevFronts D, DevSolve DS, 
     const double *A = F + D.off[f];
     double *Pf = P + DS.poff[f];
     __shared__ double L[TB][TB + 1];
-    for (int t = threadIdx.x; t < TB * TB; t += 64) {
-        int i = t % TB, j = t / TB;
-        L[i][j] = (i < nb && j < nb && i >= j) ? A[(b0 + i) + (int64_t)(b0 + j) * m] : ((i == j) ? 1.0 : 0.0);
+    {
+        // all loads issued together (clamped indices), masks applied after
+        constexpr int U = TB * TB / 64;
+        double tmp[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = threadIdx.x + 64 * u, i = t % TB, j = t / TB;
+            tmp[u] = A[(b0 + min(i, nb - 1)) + (int64_t)(b0 + min(j, nb - 1)) * m];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = threadIdx.x + 64 * u, i = t % TB, j = t / TB;
+            L[i][j] = (i < nb && j < nb && i >= j) ? tmp[u] : ((i == j) ? 1.0 : 0.0);
+        }
     }
     __syncthreads();
     const int j = threadIdx.x;
@@ -76,10 +87,22 @@ __global__ void __launch_bounds__(256) linv_col_kernel(DevFronts D, DevSolve DS,
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         for (int k0 = j0; k0 < i0; k0 += TB) {
             const int nbk = min(TB, ns - k0);
-            for (int t = tid; t < TB * TB; t += blockDim.x) {
-                int i = t % TB, jj = t / TB;
-                Lt[i][jj] = (i < nbi && jj < nbk) ? A[(i0 + i) + (int64_t)(k0 + jj) * m] : 0.0;
-                Xt[i][jj] = (i < nbk && jj < nbj) ? Pf[(k0 + i) + (int64_t)(j0 + jj) * m] : 0.0;
+            {
+                // all loads issued together (clamped indices), masks applied after
+                constexpr int U = TB * TB / 256;
+                double tl[U], tx[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int t = tid + 256 * u, i = t % TB, jj = t / TB;
+                    tl[u] = A[(i0 + min(i, nbi - 1)) + (int64_t)(k0 + min(jj, nbk - 1)) * m];
+                    tx[u] = Pf[(k0 + min(i, nbk - 1)) + (int64_t)(j0 + min(jj, nbj - 1)) * m];
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int t = tid + 256 * u, i = t % TB, jj = t / TB;
+                    Lt[i][jj] = (i < nbi && jj < nbk) ? tl[u] : 0.0;
+                    Xt[i][jj] = (i < nbk && jj < nbj) ? tx[u] : 0.0;
+                }
             }
             __syncthreads();
 #pragma unroll 8
@@ -93,9 +116,19 @@ __global__ void __launch_bounds__(256) linv_col_kernel(DevFronts D, DevSolve DS,
         // X[i0, j0] = -inv(L[i0,i0]) acc, inv(L[i0,i0]) from the diagonal-block pass
 #pragma unroll
         for (int u = 0; u < 4; u++) Xt[ti][4 * tq + u] = acc[u];
-        for (int t = tid; t < TB * TB; t += blockDim.x) {
-            int i = t % TB, jj = t / TB;
-            Lt[i][jj] = (i < nbi && jj < nbi && jj <= i) ? Pf[(i0 + i) + (int64_t)(i0 + jj) * m] : 0.0;
+        {
+            constexpr int U = TB * TB / 256;
+            double tl[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = tid + 256 * u, i = t % TB, jj = t / TB;
+                tl[u] = Pf[(i0 + min(i, nbi - 1)) + (int64_t)(i0 + min(jj, nbi - 1)) * m];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = tid + 256 * u, i = t % TB, jj = t / TB;
+                Lt[i][jj] = (i < nbi && jj < nbi && jj <= i) ? tl[u] : 0.0;
+            }
         }
         __syncthreads();
         if (ti < nbi)
@@ -131,11 +164,24 @@ __global__ void __launch_bounds__(256) m21_kernel(DevFronts D, const int32_t *fl
 #pragma unroll
         for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
     for (int k0 = c0; k0 < ns; k0 += 32) {
-        for (int t = threadIdx.x; t < 64 * 32; t += 256) {
-            int r = t % 64, k = t / 64;
-            As[r][k] = (r0 + r < m && k0 + k < ns) ? A[(r0 + r) + (int64_t)(k0 + k) * m] : 0.0;
-            int kk = t % 32, c = t / 32;
-            Bs[kk][c] = (k0 + kk < ns && c0 + c < ns) ? Pf[(k0 + kk) + (int64_t)(c0 + c) * m] : 0.0;
+        {
+            // all loads issued together (clamped indices), masks applied after
+            constexpr int U = 64 * 32 / 256;
+            double ta[U], tb[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = threadIdx.x + 256 * u;
+                const int r = t % 64, k = t / 64, kk = t % 32, c = t / 32;
+                ta[u] = A[min(r0 + r, m - 1) + (int64_t)min(k0 + k, ns - 1) * m];
+                tb[u] = Pf[min(k0 + kk, ns - 1) + (int64_t)min(c0 + c, ns - 1) * m];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = threadIdx.x + 256 * u;
+                const int r = t % 64, k = t / 64, kk = t % 32, c = t / 32;
+                As[r][k] = (r0 + r < m && k0 + k < ns) ? ta[u] : 0.0;
+                Bs[kk][c] = (k0 + kk < ns && c0 + c < ns) ? tb[u] : 0.0;
+            }
         }
         __syncthreads();
 #pragma unroll 4
@@ -170,9 +216,20 @@ __global__ void transpose_kernel(DevFronts D, const int32_t *flist, const double
     const double *Pf = P + poff[f];
     double *Tf = PT + poff[f];
     __shared__ double T[64][65];
-    for (int t = threadIdx.x; t < 64 * 64; t += blockDim.x) {
-        int i = t % 64, j = t / 64;
-        T[i][j] = (i0 + i < m && j0 + j < ns) ? Pf[(i0 + i) + (int64_t)(j0 + j) * m] : 0.0;
+    {
+        // all loads issued together (clamped indices), masks applied after
+        constexpr int U = 64 * 64 / 256;
+        double tv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = threadIdx.x + 256 * u, i = t % 64, j = t / 64;
+            tv[u] = Pf[min(i0 + i, m - 1) + (int64_t)min(j0 + j, ns - 1) * m];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = threadIdx.x + 256 * u, i = t % 64, j = t / 64;
+            T[i][j] = (i0 + i < m && j0 + j < ns) ? tv[u] : 0.0;
+        }
     }
     __syncthreads();
     for (int t = threadIdx.x; t < 64 * 64; t += blockDim.x) {
