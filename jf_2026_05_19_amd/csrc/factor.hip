@@ -55,13 +55,21 @@ __global__ void extend_add_kernel(DevFronts D, const int32_t *flist, int slot, d
     }
 }
 
+__device__ __forceinline__ double readlane_d(double x, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 // Cholesky of the nb x nb diagonal block (nb <= 32) by one wavefront: lane l
-// holds row l of the block in registers; column j is finished by lane j's
-// diagonal and broadcast by shuffles, so the 32 columns need no barrier.
+// holds row l of the block in registers; the diagonal and column j are
+// broadcast with v_readlane (scalar registers), so the 32 columns need no
+// barrier and no LDS round trip.
 __global__ void __launch_bounds__(64) potrf_diag_kernel(DevFronts D, const int32_t *flist, int k0, double *F,
                                                         double *piv, int *err) {
-    int f = flist[blockIdx.x];
-    int ns = D.ns[f];
+    const int f = flist[blockIdx.x];
+    const int ns = D.ns[f];
     if (k0 >= ns) return;
     const int nb = min(NB, ns - k0), m = D.m[f];
     double *A = F + D.off[f];
@@ -71,21 +79,21 @@ __global__ void __launch_bounds__(64) potrf_diag_kernel(DevFronts D, const int32
     for (int j = 0; j < NB; j++) a[j] = A[(k0 + lr) + (int64_t)(k0 + min(j, nb - 1)) * m];
 #pragma unroll
     for (int j = 0; j < NB; j++) a[j] = (l < nb && j <= l && j < nb) ? a[j] : 0.0;
+    double *pv = piv + D.piv0[f] + k0;
 #pragma unroll
     for (int j = 0; j < NB; j++) {
         if (j < nb) {
-            double d = __shfl(a[j], j, 64);
+            const double d = readlane_d(a[j], j);
             if (l == 0) {
-                piv[D.piv0[f] + k0 + j] = d;
+                pv[j] = d;
                 if (!(d > 0.0)) atomicOr(err, 1);
             }
-            if (!(d > 0.0)) d = 1.0;
-            const double sd = sqrt(d);
-            const double lij = (l == j) ? sd : ((l > j) ? a[j] / sd : 0.0);
+            const double sd = sqrt((d > 0.0) ? d : 1.0), rs = 1.0 / sd;
+            const double lij = (l == j) ? sd : ((l > j) ? a[j] * rs : 0.0);
             a[j] = lij;
 #pragma unroll
             for (int k = j + 1; k < NB; k++) {
-                const double lkj = __shfl(lij, k, 64);
+                const double lkj = readlane_d(lij, k);
                 if (k <= l) a[k] -= lij * lkj;
             }
         }
@@ -96,6 +104,8 @@ __global__ void __launch_bounds__(64) potrf_diag_kernel(DevFronts D, const int32
             if (j <= l) A[(k0 + l) + (int64_t)(k0 + j) * m] = a[j];
 }
 
+// rows below the diagonal block: x L^T = a, one row per thread, the factored
+// block and its reciprocal diagonal in LDS
 __global__ void trsm_panel_kernel(DevFronts D, const int32_t *flist, int k0, double *F) {
     int f = flist[blockIdx.y];
     int ns = D.ns[f];
@@ -103,34 +113,37 @@ __global__ void trsm_panel_kernel(DevFronts D, const int32_t *flist, int k0, dou
     int nb = min(NB, ns - k0), m = D.m[f];
     double *A = F + D.off[f];
     __shared__ double L[NB][NB + 1];
+    __shared__ double rdiag[NB];
+    const int tid = threadIdx.x;
+    const int i = k0 + nb + blockIdx.x * blockDim.x + tid;
+    // this thread's row is loaded with the block (clamped), before the barrier
+    double x[NB];
+#pragma unroll
+    for (int l = 0; l < NB; l++) x[l] = A[min(i, m - 1) + (int64_t)(k0 + min(l, nb - 1)) * m];
     {
-        // all loads issued together (clamped indices), masks applied after
         constexpr int U = NB * NB / 256;
         double tmp[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int t = threadIdx.x + 256 * u, i = t % NB, j = t / NB;
-            tmp[u] = A[(k0 + min(i, nb - 1)) + (int64_t)(k0 + min(j, nb - 1)) * m];
+            const int t = tid + 256 * u, r = t % NB, c = t / NB;
+            tmp[u] = A[(k0 + min(r, nb - 1)) + (int64_t)(k0 + min(c, nb - 1)) * m];
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int t = threadIdx.x + 256 * u, i = t % NB, j = t / NB;
-            L[i][j] = (i < nb && j < nb && i >= j) ? tmp[u] : 0.0;
+            const int t = tid + 256 * u, r = t % NB, c = t / NB;
+            L[r][c] = (r < nb && c < nb && r >= c) ? tmp[u] : 0.0;
+            if (r == c) rdiag[r] = (r < nb) ? 1.0 / tmp[u] : 0.0;
         }
     }
     __syncthreads();
-    int i = k0 + nb + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    double x[NB];
-#pragma unroll
-    for (int l = 0; l < NB; l++) x[l] = (l < nb) ? A[i + (int64_t)(k0 + l) * m] : 0.0;
 #pragma unroll
     for (int j = 0; j < NB; j++) {
         if (j < nb) {
-            double s = x[j];
+            double sacc = x[j];
 #pragma unroll
-            for (int l = 0; l < j; l++) s -= x[l] * L[j][l];
-            x[j] = s / L[j][j];
+            for (int l = 0; l < j; l++) sacc -= x[l] * L[j][l];
+            x[j] = sacc * rdiag[j];
         }
     }
 #pragma unroll

@@ -58,6 +58,8 @@ __device__ int sturm_count(int m, const double *d, const double *e2, double x, d
     double q = d[0] - x;
     if (fabs(q) < pivmin) q = -pivmin;
     int c = (q < 0.0);
+    // unrolled so the LDS reads of d and e2 run ahead of the division chain
+#pragma unroll 8
     for (int i = 1; i < m; i++) {
         q = d[i] - x - e2[i - 1] / q;
         if (fabs(q) < pivmin) q = -pivmin;
@@ -197,10 +199,14 @@ __device__ void tridiag_top_pairs(int m, int k, const TriLds &L, double *theta, 
     // ---- 3. inverse iteration, one lane per cluster, LU and vector in LDS ----
     const int nc = sint[1];
     const int CB = scr_n / (6 * m);  // clusters per batch (>= 2 for m <= 1024)
+    // slot s of a batch runs on lane s / NW of wavefront s % NW: the chains of
+    // a batch sit in different wavefronts first, so their pivoting branches
+    // do not serialise each other
+    const int slot = lane * NW + wid;
     for (int c0 = 0; c0 < nc; c0 += CB) {
-        const int cid = c0 + tid;
-        if (tid < CB && cid < nc) {
-            double *D = scr + (size_t)tid * 6 * m;
+        const int cid = c0 + slot;
+        if (slot < CB && cid < nc) {
+            double *D = scr + (size_t)slot * 6 * m;
             double *DL = D + m, *DU = DL + m, *DU2 = DU + m, *PV = DU2 + m, *z = PV + m;
             const double tiny = EPS * tnorm + pivmin;
             double xprev = 0.0;
@@ -214,25 +220,34 @@ __device__ void tridiag_top_pairs(int m, int k, const TriLds &L, double *theta, 
                 // LU with partial pivoting of T - x I (rows i, i+1 swapped when PV[i] = 1);
                 // dcur/ucur carry D[i]/DU[i] of the row being eliminated
                 double dcur = dd[0] - x, ucur = ee[0];
+                // the O(m) loops below are unrolled so that their LDS reads run
+                // ahead of the recurrence
+                // (branch-free: both pivot choices are selects, so the loop
+                // body is one block and the reads run ahead of the recurrence)
+#pragma unroll 4
                 for (int i = 0; i + 1 < m; i++) {
                     const double l = ee[i];
                     const double dn = dd[i + 1] - x, un = (i + 2 < m) ? ee[i + 1] : 0.0;
-                    if (fabs(dcur) >= fabs(l)) {
-                        const double f = (dcur != 0.0) ? l / dcur : 0.0;
-                        D[i] = dcur; DL[i] = f; DU[i] = ucur; DU2[i] = 0.0; PV[i] = 0.0;
-                        dcur = dn - f * ucur;
-                        ucur = un;
-                    } else {
-                        const double f = dcur / l;
-                        D[i] = l; DL[i] = f; DU[i] = dn; DU2[i] = un; PV[i] = 1.0;
-                        dcur = ucur - f * dn;
-                        ucur = -f * un;
-                    }
+                    const bool keep = fabs(dcur) >= fabs(l);  // no row swap
+                    const double num = keep ? l : dcur, den = keep ? dcur : l;
+                    const double f = (den != 0.0) ? num / den : 0.0;
+                    D[i] = den;
+                    DL[i] = f;
+                    DU[i] = keep ? ucur : dn;
+                    DU2[i] = keep ? 0.0 : un;
+                    PV[i] = keep ? 0.0 : 1.0;
+                    const double nd = keep ? dn - f * ucur : ucur - f * dn;
+                    ucur = keep ? un : -f * un;
+                    dcur = nd;
                 }
                 D[m - 1] = dcur;
                 DU[m - 1] = 0.0;
-                for (int i = 0; i < m; i++)
-                    if (fabs(D[i]) < tiny) D[i] = (D[i] < 0.0) ? -tiny : tiny;
+                // D keeps the reciprocal pivots from here on: the U solves multiply
+#pragma unroll 4
+                for (int i = 0; i < m; i++) {
+                    const double di = D[i];
+                    D[i] = 1.0 / ((fabs(di) < tiny) ? ((di < 0.0) ? -tiny : tiny) : di);
+                }
                 for (int i = 0; i < m; i++) z[i] = hash_unit(((uint64_t)(asc[t] + 1) << 32) ^ (uint64_t)i);
                 // the eigenvalue is accurate to rounding, so two solves give the
                 // vector to working precision; a clustered one gets a third with
@@ -241,22 +256,20 @@ __device__ void tridiag_top_pairs(int m, int k, const TriLds &L, double *theta, 
                 for (int it = 0; it < nit; it++) {
                     // L solve, carrying the current row's value
                     double cur = z[0];
+#pragma unroll 4
                     for (int i = 0; i + 1 < m; i++) {
-                        const double nxt = z[i + 1];
-                        if (PV[i] == 0.0) {
-                            z[i] = cur;
-                            cur = nxt - DL[i] * cur;
-                        } else {
-                            z[i] = nxt;
-                            cur = cur - DL[i] * nxt;
-                        }
+                        const double nxt = z[i + 1], li = DL[i];
+                        const bool keep = PV[i] == 0.0;
+                        z[i] = keep ? cur : nxt;
+                        cur = keep ? nxt - li * cur : cur - li * nxt;
                     }
                     z[m - 1] = cur;
                     // U solve, carrying z[i+1], z[i+2]
-                    double z1 = z[m - 1] / D[m - 1], z2 = 0.0;
+                    double z1 = z[m - 1] * D[m - 1], z2 = 0.0;
                     z[m - 1] = z1;
+#pragma unroll 4
                     for (int i = m - 2; i >= 0; i--) {
-                        const double zi = (z[i] - DU[i] * z1 - DU2[i] * z2) / D[i];
+                        const double zi = (z[i] - DU[i] * z1 - DU2[i] * z2) * D[i];
                         z[i] = zi;
                         z2 = z1;
                         z1 = zi;
@@ -694,13 +707,23 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
                 const int st = t / nv, c = t % nv;
                 const int r0 = i + 1 + st * bw, L = min(bw, m - r0);
                 const double *rv = recs + st * (bw + 1);
+                // all record loads issued together (fixed width, clamped, masked)
+                double v[SYMEIG_BAND_MAX_W];
+#pragma unroll
+                for (int q = 0; q < SYMEIG_BAND_MAX_W; q++) v[q] = rv[min(q, bw)];
                 const double tau = rv[bw];
-                if (tau == 0.0) continue;
                 double *z = Z + (size_t)c * m + r0;
+                double zz[SYMEIG_BAND_MAX_W];
                 double dot = 0.0;
-                for (int q = 0; q < L; q++) dot += rv[q] * z[q];
+#pragma unroll
+                for (int q = 0; q < SYMEIG_BAND_MAX_W; q++) {
+                    zz[q] = (q < L) ? z[min(q, L - 1)] : 0.0;
+                    dot += ((q < L) ? v[q] : 0.0) * zz[q];
+                }
                 dot *= tau;
-                for (int q = 0; q < L; q++) z[q] -= dot * rv[q];
+#pragma unroll
+                for (int q = 0; q < SYMEIG_BAND_MAX_W; q++)
+                    if (q < L) z[q] = zz[q] - dot * v[q];
             }
             __syncthreads();
         }
