@@ -124,7 +124,14 @@ __device__ __forceinline__ void tn_reduce_body(int nch, int P, int Q, const doub
     const int PQ = P * Q;
     double s = 0.0;
     if (o < PQ)
-        for (int c = sg; c < nch; c += 16) s += part[(int64_t)c * PQ + o];
+        // eight partial sums in flight, added in chunk order
+        for (int c = sg; c < nch; c += 128) {
+            double pv[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) pv[u] = part[(int64_t)min(c + 16 * u, nch - 1) * PQ + o];
+#pragma unroll
+            for (int u = 0; u < 8; u++) s = (c + 16 * u < nch) ? s + pv[u] : s;
+        }
     red[sg][ol] = s;
     __syncthreads();
     if (sg == 0 && o < PQ) {
