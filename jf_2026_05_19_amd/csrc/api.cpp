@@ -138,7 +138,15 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
         HIPCHK(M->s_fkw.upload(SP.fwd_kw.data(), SP.fwd_kw.size(), s));
         HIPCHK(M->s_bkw.upload(SP.bwd_kw.data(), SP.bwd_kw.size(), s));
         HIPCHK(M->s_pullptr.upload(SP.pull_ptr.data(), SP.pull_ptr.size(), s));
-        HIPCHK(M->s_pullsrc.upload(SP.pull_src.data(), SP.pull_src.size(), s));
+        {
+            // the device pull map holds absolute vector slots (child's vec_off
+            // + row): one dependent load less per contribution
+            std::vector<int32_t> slot(SP.pull_src.size() / 2);
+            for (size_t q = 0; q < slot.size(); q++)
+                slot[q] = (int32_t)(S.fronts[SP.pull_src[2 * q]].vec_off + SP.pull_src[2 * q + 1]);
+            HIPCHK(M->s_pullsrc.upload(slot.data(), slot.size(), s));
+            HIPCHK(hipStreamSynchronize(s));  // slot is a host temporary
+        }
         HIPCHK(M->s_lblk.upload(SP.lblk.data(), SP.lblk.size(), s));
         HIPCHK(M->d_P.alloc(SP.p_storage + SP.p_pad));
         HIPCHK(M->d_PT.alloc(SP.p_storage + SP.p_pad));

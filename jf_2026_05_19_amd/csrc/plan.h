@@ -32,6 +32,7 @@ struct SolveHeight {
     int n_fwd = 0, n_bwd = 0;          // tiles of this height
     int max_m = 0, max_ns = 0;
     int64_t sum_mns = 0, sum_m = 0, sum_ns = 0;  // factor entries and vector rows of this height
+    bool fwd_split = false, bwd_split = false;   // some front's product is split over k (needs a finish pass)
 };
 
 struct SolvePlan {

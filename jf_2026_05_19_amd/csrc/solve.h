@@ -13,7 +13,7 @@ struct DevSolve {
     const int32_t *fwd_tbase_rel, *bwd_tbase_rel;
     const int32_t *fwd_nks, *bwd_nks;
     const int32_t *fwd_kw, *bwd_kw;
-    const int32_t *pull_ptr, *pull_src;
+    const int32_t *pull_ptr, *pull_src;  // pull map: per vector slot, the child slots it adds (absolute)
     const int32_t *lblk;  // (front, first column) of every 32-wide L11 column block
     int n_lblk;
 };

@@ -255,10 +255,8 @@ __global__ void fwd_pull_kernel(DevFronts D, DevSolve DS, const int32_t *flist, 
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m * nrhs; t += gridDim.x * blockDim.x) {
         int i = t / nrhs, r = t % nrhs;
         double v = (i < ns) ? X[(piv0 + i) + (int64_t)r * n_free] : 0.0;
-        for (int q = DS.pull_ptr[slot0 + i]; q < DS.pull_ptr[slot0 + i + 1]; q++) {
-            int c = DS.pull_src[2 * q], row = DS.pull_src[2 * q + 1];
-            v += W[(D.vec_off[c] + row) * ldv + r];
-        }
+        for (int q = DS.pull_ptr[slot0 + i]; q < DS.pull_ptr[slot0 + i + 1]; q++)
+            v += W[(int64_t)DS.pull_src[q] * ldv + r];
         V[(int64_t)i * ldv + r] = v;
     }
 }
@@ -592,7 +590,9 @@ void solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, cons
             prod_mfma_kernel<<<SH.n_fwd, 256, 0, s>>>(D, tl, Pbuf, DS.poff, W, part, nrhs, 0, DS.fwd_kw, DS.fwd_nks,
                                                       X, P.n_free);
         if (timed) pt->end(s);
-        fwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.fwd_tbase_rel,
+        // single-slice fronts were finished by the product itself
+        if (SH.fwd_split)
+            fwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.fwd_tbase_rel,
                                                                                        DS.fwd_nks, part, W, X,
                                                                                        P.n_free, nrhs, ldv);
     }
@@ -617,7 +617,8 @@ void solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, cons
             prod_mfma_kernel<<<SH.n_bwd, 256, 0, s>>>(D, tl, PTbuf, DS.poff, W, part, nrhs, 1, DS.bwd_kw, DS.bwd_nks,
                                                       X, P.n_free);
         if (timed) pt->end(s);
-        bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.bwd_tbase_rel,
+        if (SH.bwd_split)
+            bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.bwd_tbase_rel,
                                                                                         DS.bwd_nks, part, X,
                                                                                         P.n_free, nrhs);
     }

@@ -52,6 +52,7 @@ void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std
             int nrt = (m[f] + RT - 1) / RT, nk = (ns[f] + kwf - 1) / kwf;
             S.fwd_tbase[f] = (int32_t)(S.fwd_tiles.size() / 3 - SH.fwd_off);
             S.fwd_nks[f] = nk;
+            SH.fwd_split = SH.fwd_split || nk > 1;
             for (int rt = 0; rt < nrt; rt++)
                 for (int k = 0; k < nk; k++) {
                     S.fwd_tiles.push_back(f);
@@ -61,6 +62,7 @@ void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std
             int nrtb = (ns[f] + RT - 1) / RT, nkb = (m[f] + kwb - 1) / kwb;
             S.bwd_tbase[f] = (int32_t)(S.bwd_tiles.size() / 3 - SH.bwd_off);
             S.bwd_nks[f] = nkb;
+            SH.bwd_split = SH.bwd_split || nkb > 1;
             for (int rt = 0; rt < nrtb; rt++)
                 for (int k = 0; k < nkb; k++) {
                     S.bwd_tiles.push_back(f);
