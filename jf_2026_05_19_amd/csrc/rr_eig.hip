@@ -509,11 +509,32 @@ __device__ __forceinline__ int64_t band_F(int n, int bw) {
 // first task index of sweep i (tasks of sweeps 0..i-1)
 __device__ __forceinline__ int64_t band_toff(int m, int bw, int i) { return band_F(m - 1, bw) - band_F(m - 1 - i, bw); }
 
-// sum over the 16 lanes of a row group (lanes sharing l / 16)
+// v from the lane that the DPP control ctrl selects within the row of 16
+template <int ctrl>
+__device__ __forceinline__ double dpp_d(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, ctrl, 0xF, 0xF, false);
+    const uint32_t hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), ctrl, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// sum over the 16 lanes of a row group (lanes sharing l / 16) by a DPP
+// butterfly: quad_perm(1,0,3,2), quad_perm(2,3,0,1), row_half_mirror,
+// row_mirror; every step adds a pair symmetrically, so all 16 lanes hold the
+// same bits
 __device__ __forceinline__ double sum16(double v) {
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
     return v;
+}
+
+__device__ __forceinline__ double readlane_d(double x, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 // sum over the 4 lanes sharing l % 16
 __device__ __forceinline__ double sum4(double v) {
@@ -524,7 +545,7 @@ __device__ __forceinline__ double sum4(double v) {
 // Householder reflector H = I - tau v v^T with v[0] = 1 and H x = mu e_0 for
 // the vector x (lane l holds x[l % 16], length L); Golub & Van Loan Alg. 5.1.1.
 __device__ __forceinline__ void house16(double x, int p, int L, double &v, double &tau, double &mu) {
-    const double alpha = __shfl(x, 0, 64);
+    const double alpha = readlane_d(x, 0);
     const double sigma = sum16((p >= 1 && p < L) ? x * x : 0.0);
     if (sigma > 0.0) {
         mu = sqrt(alpha * alpha + sigma);
@@ -625,6 +646,7 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
     __shared__ double AB[BAND_LDS];  // the band; afterwards scratch of steps 3 and 4
     __shared__ double dd[SYMEIG_BAND_MAX_M], ee[SYMEIG_BAND_MAX_M], e2[SYMEIG_BAND_MAX_M];
     __shared__ double slots[NSLOT][17];
+    __shared__ int sw_nst[SYMEIG_BAND_MAX_M], sw_off[SYMEIG_BAND_MAX_M];  // per sweep: steps, first record
     __shared__ double ev[2 * SYMEIG_MAX_K];
     __shared__ int sel[SYMEIG_MAX_K], asc[SYMEIG_MAX_K], ocol[SYMEIG_MAX_K], cl_start[SYMEIG_MAX_K + 1];
     __shared__ double wlo[NW], whi[NW], wem[NW];
@@ -667,17 +689,27 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
 
     // ---- bulge chasing: time step ts runs step ts - 3i of every active sweep i ----
     const int nsw = m - 2;  // sweeps 0 .. m-3
+    for (int i = tid; i < nsw; i += NT) {
+        sw_nst[i] = band_nsteps(m, bw, i);
+        sw_off[i] = (int)band_toff(m, bw, i);
+    }
+    __syncthreads();
     if (nsw > 0) {
-        const int tend = 3 * (nsw - 1) + band_nsteps(m, bw, nsw - 1);
+        const int tend = 3 * (nsw - 1) + sw_nst[nsw - 1];
+        // this wavefront's sweeps are i == wid (mod NW); ihi = min(nsw - 1, ts / 3)
+        // is tracked incrementally, and older sweeps finish first
+        int ihi = 0, ph = 0;
         for (int ts = 0; ts < tend; ts++) {
-            const int ihi = min(nsw - 1, ts / 3);
-            // this wavefront's sweeps are i == wid (mod NW); older sweeps finish first
             for (int i = ihi - ((ihi - wid) % NW + NW) % NW; i >= 0; i -= NW) {
                 const int st = ts - 3 * i;
-                if (st >= band_nsteps(m, bw, i)) break;
-                band_task(m, bw, i, st, AB, slots[i % NSLOT], work + (band_toff(m, bw, i) + st) * (bw + 1));
+                if (st >= sw_nst[i]) break;
+                band_task(m, bw, i, st, AB, slots[i % NSLOT], work + (int64_t)(sw_off[i] + st) * (bw + 1));
             }
             __syncthreads();
+            if (++ph == 3) {
+                ph = 0;
+                ihi = min(nsw - 1, ihi + 1);
+            }
         }
     }
     for (int j = tid; j < m; j += NT) {
@@ -701,8 +733,8 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
         for (int t = tid; t < nv * m; t += NT) Z[t] = Zs[(int64_t)c0 * m + t];
         __syncthreads();
         for (int i = nsw - 1; i >= 0; i--) {
-            const int nst = band_nsteps(m, bw, i);
-            const double *recs = work + band_toff(m, bw, i) * (bw + 1);
+            const int nst = sw_nst[i];
+            const double *recs = work + (int64_t)sw_off[i] * (bw + 1);
             for (int t = tid; t < nst * nv; t += NT) {
                 const int st = t / nv, c = t % nv;
                 const int r0 = i + 1 + st * bw, L = min(bw, m - r0);
