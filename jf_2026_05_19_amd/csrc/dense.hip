@@ -223,44 +223,61 @@ __global__ void __launch_bounds__(256) nn_kernel(int n, int P, int Q, const doub
 // Gram-Schmidt, ld ldg0), a pivot d <= tol2 * max_i g0_ii marks a direction
 // that Gram-Schmidt reduced to rounding noise: err[0] |= 4 and err[1] keeps
 // the smallest (blk + 1) seen.  Pivots are floored so no NaN is produced.
+__device__ __forceinline__ double readlane_dd(double x, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Upper Cholesky G = R^T R of a b x b Gram block (b <= 16) by one wavefront:
+// lane t holds column t in registers (c[i] = G[i][t], then R[i][t]); each
+// column step takes the diagonal and row j of the trailing update with
+// v_readlane, so no barrier is needed.  g0 non-null: a pivot at or below
+// tol2 * max diag(g0) flags a rank-deficient block (err[0] |= 4, first block
+// index in err[1]); otherwise a non-positive pivot sets err[0] |= 2.  Flagged
+// pivots are clamped to max(d, threshold, 1e-300).
 __device__ __forceinline__ void chol_upper_body(int b, double *G, int ldg, const double *g0, int ldg0, double tol2,
                                                 int blk, int *err) {
-    __shared__ double R[16][17];
-    __shared__ double dsh;
-    const int t = threadIdx.x;
-    for (int q = t; q < 256; q += 64) {
-        int i = q % 16, j = q / 16;
-        R[i][j] = (i < b && j < b) ? G[i + j * ldg] : 0.0;
-    }
+    const int t = threadIdx.x, tc = min(t, b - 1);
+    double c[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) c[i] = G[min(i, b - 1) + tc * ldg];
+#pragma unroll
+    for (int i = 0; i < 16; i++) c[i] = (t < b && i < b) ? c[i] : 0.0;
     double thr = 0.0;
-    if (g0)
-        for (int i = 0; i < b; i++) thr = fmax(thr, g0[i + i * ldg0]);
-    thr *= tol2;
+    if (g0) {
+        double gd[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) gd[i] = g0[min(i, b - 1) * (ldg0 + 1)];
+#pragma unroll
+        for (int i = 0; i < 16; i++) thr = (i < b) ? fmax(thr, gd[i]) : thr;
+        thr *= tol2;
+    }
     int flag = 0;
-    __syncthreads();
-    for (int j = 0; j < b; j++) {
-        double sv = 0.0;
-        if (t >= j && t < b) {
-            sv = R[j][t];
-            for (int k = 0; k < j; k++) sv -= R[k][j] * R[k][t];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        if (j < b) {
+            double d = readlane_dd(c[j], j);
+            if (g0 ? !(d > thr) : !(d > 0.0)) {
+                flag = 1;
+                d = fmax(fmax(d, thr), 1e-300);
+            }
+            const double r = sqrt(d), rinv = 1.0 / r;
+            // row j of R: R[j][t] = c[j] / r for t > j, r on the diagonal, 0 left of it
+            c[j] = (t > j) ? c[j] * rinv : ((t == j) ? r : 0.0);
+            // trailing rows i > j of column t: c[i] -= R[j][i] R[j][t]
+#pragma unroll
+            for (int i = j + 1; i < 16; i++) {
+                const double rji = readlane_dd(c[j], i);
+                if (i < b && i <= t) c[i] -= rji * c[j];
+            }
         }
-        if (t == j) dsh = sv;
-        __syncthreads();
-        double d = dsh;
-        if (g0 ? !(d > thr) : !(d > 0.0)) {
-            flag = 1;
-            d = fmax(fmax(d, thr), 1e-300);
-        }
-        const double r = sqrt(d);
-        if (t > j && t < b) R[j][t] = sv / r;
-        if (t == j) R[j][j] = r;
-        if (t < j) R[j][t] = 0.0;
-        __syncthreads();
     }
-    for (int q = t; q < b * b; q += 64) {
-        int i = q % b, j = q / b;
-        G[i + j * ldg] = (i <= j) ? R[i][j] : 0.0;
-    }
+    if (t < b)
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if (i < b) G[i + t * ldg] = (i <= t) ? c[i] : 0.0;
     if (t == 0 && flag) {
         if (g0) {
             err[0] |= 4;
@@ -431,14 +448,16 @@ __global__ void colmaxabs_partial_kernel(int64_t n, const double *x, double *par
 }
 
 __global__ void scale_by_max_kernel(int64_t n, double *x, const double *part, int np) {
-    __shared__ double mx;
-    if (threadIdx.x == 0) {
-        double v = 0.0;
-        for (int i = 0; i < np; i++) v = fmax(v, part[i]);
-        mx = v;
-    }
+    // every workgroup reduces the partial maxima itself: all threads read a
+    // share, then a wavefront max and a 4-entry LDS max
+    __shared__ double wm[4];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) v = fmax(v, part[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
     __syncthreads();
-    double m = mx;
+    double m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
     if (!(m > 1e-30)) return;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] /= m;
 }

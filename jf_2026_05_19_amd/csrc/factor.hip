@@ -272,20 +272,46 @@ __global__ void scatter_elim_kernel(int n_free, int ndof, int nrhs, const int32_
 // Interleaved DOF-order vectors (x[d * nrhs + r]) for the BSR product: the
 // nrhs values of one DOF are adjacent, so a block's 6 x nrhs inputs are
 // 6 * nrhs contiguous doubles.
-__global__ void scatter_elim_il_kernel(int n_free, int ndof, int nrhs, const int32_t *elim_of_dof, const double *xe,
-                                       double *xf) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)ndof * nrhs) return;
-    int d = (int)(t / nrhs), r = (int)(t % nrhs);
-    int e = elim_of_dof[d];
-    xf[t] = e >= 0 ? xe[e + (int64_t)r * n_free] : 0.0;
+// Both permutations go through a 64-DOF tile in LDS so that each side is
+// read or written along its contiguous direction: the elimination-order side
+// (column-major, runs of a node's DOFs) with the DOF index fastest, the
+// interleaved side (one row of nrhs values per DOF) with the column fastest.
+__global__ void __launch_bounds__(256) scatter_elim_il_kernel(int n_free, int ndof, int nrhs, const int32_t *elim_of_dof,
+                                                              const double *xe, double *xf) {
+    __shared__ double tile[64][17];
+    const int d0 = blockIdx.x * 64, t = threadIdx.x, dl = t & 63, rg = t >> 6;
+    const int d = min(d0 + dl, ndof - 1);
+    const int e = elim_of_dof[d];
+    double v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int r = rg + 4 * j;
+        v[j] = (e >= 0 && r < nrhs) ? xe[max(e, 0) + (int64_t)min(r, nrhs - 1) * n_free] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) tile[dl][rg + 4 * j] = v[j];
+    __syncthreads();
+    const int nd = min(64, ndof - d0);
+    for (int idx = t; idx < nd * nrhs; idx += 256) xf[(int64_t)d0 * nrhs + idx] = tile[idx / nrhs][idx % nrhs];
 }
 
-__global__ void gather_elim_il_kernel(int n_free, int nrhs, const int32_t *dof_of_elim, const double *xf, double *xe) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)n_free * nrhs) return;
-    int e = (int)(t % n_free), r = (int)(t / n_free);
-    xe[t] = xf[(int64_t)dof_of_elim[e] * nrhs + r];
+__global__ void __launch_bounds__(256) gather_elim_il_kernel(int n_free, int nrhs, const int32_t *dof_of_elim,
+                                                             const double *xf, double *xe) {
+    __shared__ double tile[64][17];
+    const int e0 = blockIdx.x * 64, t = threadIdx.x;
+    const int ne = min(64, n_free - e0);
+    for (int idx = t; idx < ne * nrhs; idx += 256) {
+        const int el = idx / nrhs, r = idx % nrhs;
+        tile[el][r] = xf[(int64_t)dof_of_elim[e0 + el] * nrhs + r];
+    }
+    __syncthreads();
+    const int el = t & 63, rg = t >> 6;
+    if (el < ne)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int r = rg + 4 * j;
+            if (r < nrhs) xe[(e0 + el) + (int64_t)r * n_free] = tile[el][r];
+        }
 }
 
 // y = s * A x for a 6x6 BSR matrix, x and y interleaved (x[d * nrhs + r]).
@@ -383,14 +409,12 @@ void launch_bsr_spmv(int n_nodes, const int32_t *rp, const int32_t *col, const d
 
 void launch_scatter_elim_il(int n_free, int ndof, int nrhs, const int32_t *elim_of_dof, const double *xe, double *xf,
                             hipStream_t s) {
-    int64_t n = (int64_t)ndof * nrhs;
-    if (n > 0) scatter_elim_il_kernel<<<nblk(n, 256), 256, 0, s>>>(n_free, ndof, nrhs, elim_of_dof, xe, xf);
+    if (ndof > 0) scatter_elim_il_kernel<<<nblk(ndof, 64), 256, 0, s>>>(n_free, ndof, nrhs, elim_of_dof, xe, xf);
 }
 
 void launch_gather_elim_il(int n_free, int nrhs, const int32_t *dof_of_elim, const double *xf, double *xe,
                            hipStream_t s) {
-    int64_t n = (int64_t)n_free * nrhs;
-    if (n > 0) gather_elim_il_kernel<<<nblk(n, 256), 256, 0, s>>>(n_free, nrhs, dof_of_elim, xf, xe);
+    if (n_free > 0) gather_elim_il_kernel<<<nblk(n_free, 64), 256, 0, s>>>(n_free, nrhs, dof_of_elim, xf, xe);
 }
 
 void launch_kdiag_elim(int n_free, const int32_t *dof_of_elim, const int32_t *diag_blk, const double *kv, double *out,
