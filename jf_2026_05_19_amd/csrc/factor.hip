@@ -342,6 +342,78 @@ __global__ void __launch_bounds__(192) bsr_spmv_kernel(int n_nodes, const int32_
     y[((int64_t)I * 6 + ci) * nrhs + r] = s * acc;
 }
 
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// y = s * A x for 16 right-hand sides: one wavefront per block row, blocks
+// taken two at a time.  The wave stages the two blocks' 72 values in LDS
+// (column-major inside a block) with two coalesced loads; lane (r, q) then
+// takes the (block, column) pairs 3q..3q+2 of the 12, reads x[6J + cj][r]
+// (each input of the two blocks is read by exactly one lane) and adds the
+// column times that input into six row sums.  The four partial sums of each
+// output are added through LDS at the end of the row, and the row's 96
+// outputs are written as one contiguous run.  The next pair's values and
+// inputs are loaded before the current pair is multiplied.
+__global__ void __launch_bounds__(256) bsr_spmv16_kernel(int n_nodes, const int32_t *__restrict__ rp,
+                                                         const int32_t *__restrict__ col,
+                                                         const double *__restrict__ val,
+                                                         const double *__restrict__ x, double *__restrict__ y,
+                                                         double s) {
+    __shared__ double sA[4][72];
+    __shared__ double sR[4][384];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+    const int I = blockIdx.x * 4 + w;
+    if (I >= n_nodes) return;  // the whole wave leaves; no block barrier follows
+    const int r = l & 15, q = l >> 4, kk = q >> 1, cb = 3 * (q & 1);
+    const int k0 = rp[I], k1 = rp[I + 1];
+    double *A = sA[w];
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    // values of the pair at k (zero past the row end) and this lane's inputs
+    auto load = [&](int k, double &a0, double &a1, double (&xv)[3]) {
+        const int nv = min(2, k1 - k) * 36;
+        const double *v = val + (int64_t)k * 36;
+        a0 = v[min(l, nv - 1)];
+        a1 = v[min(l + 64, nv - 1)];
+        a0 = l < nv ? a0 : 0.0;
+        a1 = l + 64 < nv ? a1 : 0.0;
+        const int J = col[min(k + kk, k1 - 1)];
+        const double *xb = x + (int64_t)J * 96 + cb * 16 + r;
+#pragma unroll
+        for (int j = 0; j < 3; j++) xv[j] = xb[j * 16];
+#pragma unroll
+        for (int j = 0; j < 3; j++) xv[j] = k + kk < k1 ? xv[j] : 0.0;
+    };
+    // value t = kk*36 + ci*6 + cj of a pair goes to A[kk*36 + cj*6 + ci]
+    const int t0 = l, t1 = l + 64;
+    const int d0 = (t0 / 36) * 36 + (t0 % 6) * 6 + (t0 % 36) / 6;
+    const int d1 = (t1 / 36) * 36 + (t1 % 6) * 6 + (t1 % 36) / 6;
+    double a0 = 0.0, a1 = 0.0, xv[3] = {0.0, 0.0, 0.0};
+    if (k0 < k1) load(k0, a0, a1, xv);
+    for (int k = k0; k < k1; k += 2) {
+        A[d0] = a0;
+        if (t1 < 72) A[d1] = a1;
+        wave_sync();
+        double cx[3] = {xv[0], xv[1], xv[2]};
+        if (k + 2 < k1) load(k + 2, a0, a1, xv);
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const double *a = A + kk * 36 + (cb + j) * 6;
+#pragma unroll
+            for (int ci = 0; ci < 6; ci++) acc[ci] += a[ci] * cx[j];
+        }
+        wave_sync();
+    }
+    double *R = sR[w];
+#pragma unroll
+    for (int ci = 0; ci < 6; ci++) R[q * 96 + ci * 16 + r] = acc[ci];
+    wave_sync();
+    y[(int64_t)I * 96 + l] = s * ((R[l] + R[96 + l]) + (R[192 + l] + R[288 + l]));
+    if (l < 32) y[(int64_t)I * 96 + 64 + l] = s * ((R[64 + l] + R[160 + l]) + (R[256 + l] + R[352 + l]));
+}
+
 __global__ void kdiag_elim_kernel(int n_free, const int32_t *dof_of_elim, const int32_t *diag_blk, const double *kv,
                                   double *out) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -403,6 +475,10 @@ void launch_scatter_elim(int n_free, int ndof, int nrhs, const int32_t *elim_of_
 
 void launch_bsr_spmv(int n_nodes, const int32_t *rp, const int32_t *col, const double *val, const double *x, double *y,
                      int nrhs, double scale, hipStream_t s) {
+    if (nrhs == 16) {
+        bsr_spmv16_kernel<<<nblk(n_nodes, 4), 256, 0, s>>>(n_nodes, rp, col, val, x, y, scale);
+        return;
+    }
     const int rpw = std::max(1, 192 / (6 * nrhs));
     bsr_spmv_kernel<<<nblk(n_nodes, rpw), 192, 0, s>>>(n_nodes, rp, col, val, x, y, nrhs, scale);
 }
