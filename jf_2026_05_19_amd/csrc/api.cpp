@@ -172,10 +172,13 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
     return SOL105_OK;
 }
 
-// x <- K_ff^-1 x (elimination order, nrhs columns)
-void tri_solve(sol105_model *M, double *X, int nrhs, hipStream_t s) {
-    solve_fwd(M->plan, M->dev, M->dsolve, M->splan, M->d_P.p, M->d_W.p, M->d_part.p, X, nrhs, s);
-    solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, X, nrhs, s);
+// x <- K_ff^-1 x (elimination order, nrhs <= MAXR columns)
+hipError_t tri_solve(sol105_model *M, double *X, int nrhs, hipStream_t s) {
+    hipError_t e = M->d_tsol.alloc((size_t)std::max(M->sym.n_free, 1) * nrhs);
+    if (e != hipSuccess) return e;
+    e = solve_fwd(M->plan, M->dev, M->dsolve, M->splan, M->d_P.p, M->d_W.p, M->d_part.p, X, M->d_tsol.p, nrhs, s);
+    if (e != hipSuccess) return e;
+    return solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, M->d_tsol.p, X, nrhs, s);
 }
 
 double ev_ms(hipEvent_t a, hipEvent_t b) {
@@ -435,9 +438,10 @@ int sol105_static_solve(sol105_ctx *ctx, sol105_model *M, double *u, uint8_t *fr
     double *X = nullptr;
     HIPCHK(hipMallocAsync((void **)&X, sizeof(double) * nfree, s));
     launch_gather_elim(nfree, ndof, 1, M->d_dof_of_elim.p, M->d_force.p, X, s);
-    tri_solve(M, X, 1, s);
-    launch_scatter_elim(nfree, ndof, 1, M->d_elim_of_dof.p, X, M->d_u.p, s);
+    const hipError_t te = tri_solve(M, X, 1, s);
+    if (te == hipSuccess) launch_scatter_elim(nfree, ndof, 1, M->d_elim_of_dof.p, X, M->d_u.p, s);
     HIPCHK(hipFreeAsync(X, s));
+    HIPCHK(te);
     HIPCHK(hipGetLastError());
     M->u_valid = true;
     M->KG_valid = false;
@@ -593,36 +597,41 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
         trsm_right_upper(n, b, Q, n, G2, 16, s);
         if (Rout) upper_mul(b, G2, 16, G1, 16, Rout, ldr, s);
     };
-    // Wb = C Q_jj, orthogonalised (CGS2) against V[:, 0:(jj+1)b]; coefficients
-    // into T's column block jj; G0 = Gram matrix after the first pass
-    auto expand = [&](int jj) {
-        HIPCHK(hipMemcpyAsync(Wb.p, blk(jj), sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
-        solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Wb.p, b, s, ctx->timer());
+    // dst = C Q_jj, orthogonalised (CGS2) against V[:, 0:(jj+1)b]; coefficients
+    // into T's column block jj; G0 = Gram matrix after the first pass.  The
+    // backward solve goes from Q_jj into Wb, the KG product back into Wb and
+    // the forward solve from there into dst.
+    auto expand = [&](int jj, double *dst) {
+        HIPCHK(solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, blk(jj), Wb.p, b, s,
+                         ctx->timer()));
         launch_scatter_elim_il(n, ndof, b, M->d_elim_of_dof.p, Wb.p, xf.p, s);
         launch_bsr_spmv(M->n_nodes, M->d_rp.p, M->d_col.p, M->d_KG.p, xf.p, yf.p, b, -1.0, s);
         launch_gather_elim_il(n, b, M->d_dof_of_elim.p, yf.p, Wb.p, s);
-        solve_fwd(M->plan, M->dev, M->dsolve, M->splan, M->d_P.p, M->d_W.p, M->d_part.p, Wb.p, b, s, ctx->timer());
+        HIPCHK(solve_fwd(M->plan, M->dev, M->dsolve, M->splan, M->d_P.p, M->d_W.p, M->d_part.p, Wb.p, dst, b, s,
+                         ctx->timer()));
         int kcols = (jj + 1) * b;
         double *Tcol = T.p + (size_t)jj * b * ldT;
-        ts_gemm_tn(n, kcols, b, V.p, n, Wb.p, n, Tcol, ldT, 1.0, 0.0, part.p, s);
-        ts_gemm_nn(n, kcols, b, V.p, n, Tcol, ldT, Wb.p, n, -1.0, 1.0, s);
-        ts_gemm_tn(n, b, b, Wb.p, n, Wb.p, n, G0, 16, 1.0, 0.0, part.p, s);
-        ts_gemm_tn(n, kcols, b, V.p, n, Wb.p, n, H2.p, kcols, 1.0, 0.0, part.p, s);
-        ts_gemm_nn(n, kcols, b, V.p, n, H2.p, kcols, Wb.p, n, -1.0, 1.0, s);
+        ts_gemm_tn(n, kcols, b, V.p, n, dst, n, Tcol, ldT, 1.0, 0.0, part.p, s);
+        ts_gemm_nn(n, kcols, b, V.p, n, Tcol, ldT, dst, n, -1.0, 1.0, s);
+        ts_gemm_tn(n, b, b, dst, n, dst, n, G0, 16, 1.0, 0.0, part.p, s);
+        ts_gemm_tn(n, kcols, b, V.p, n, dst, n, H2.p, kcols, 1.0, 0.0, part.p, s);
+        ts_gemm_nn(n, kcols, b, V.p, n, H2.p, kcols, dst, n, -1.0, 1.0, s);
         mat_add(kcols, b, H2.p, kcols, Tcol, ldT, s);
         return SOL105_OK;
     };
+    // the new block is built in place: column block jj+1 of V is not part of
+    // the basis that the products of step jj read
     auto fast_step = [&](int jj) {
-        int rc = expand(jj);
+        int rc = expand(jj, blk(jj + 1));
         if (rc) return rc;
-        HIPCHK(hipMemcpyAsync(blk(jj + 1), Wb.p, sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
         cholqr2(blk(jj + 1), Tsub(jj), ldT, G0, jj + 1);
         return SOL105_OK;
     };
     // rank-revealing rebuild of block jj+1 (host-assisted; rare)
     auto robust_step = [&](int jj, int &rank) {
-        int rc = expand(jj);
+        int rc = expand(jj, blk(jj + 1));
         if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(Wb.p, blk(jj + 1), sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
         ts_gemm_tn(n, b, b, Wb.p, n, Wb.p, n, G1, 16, 1.0, 0.0, part.p, s);
         std::vector<double> hG0(256), hG1(256);
         HIPCHK(hipMemcpyAsync(hG0.data(), G0, 256 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -848,9 +857,14 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
     M->lanczos_m = lo.generated;
     M->lanczos_steps = lo.steps;
     // x = L^-T y, expand to all DOFs
-    solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Y.p, k, s, ctx->timer());
     HIPCHK(Xf.alloc((size_t)ndof * k));
-    launch_scatter_elim(n, ndof, k, M->d_elim_of_dof.p, Y.p, Xf.p, s);
+    if (k > 0) {
+        DevBuf<double> &Ys = M->lw.Ys;
+        HIPCHK(Ys.alloc((size_t)n * k));
+        HIPCHK(solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Y.p, Ys.p, k, s,
+                         ctx->timer()));
+        launch_scatter_elim(n, ndof, k, M->d_elim_of_dof.p, Ys.p, Xf.p, s);
+    }
     // eigenvalue semantics: lambda = 1/theta, |theta| > 1e-12 (dense) or
     // >= 1e-14 and |lambda| > 1e-6 (Krylov); order by |lambda|, keep req
     std::vector<int> cand;
@@ -991,7 +1005,7 @@ int sens_terms(sol105_ctx *ctx, sol105_model *M, int32_t mode, DevBuf<double> &t
     node_gather_vec(M->n_nodes, M->n_q4, M->n_t3, M->d_diag.p, M->d_cptr.p, M->d_cidx.p, egq.p, egt.p, egb.p,
                     M->d_free.p, g.p, s);
     launch_gather_elim(n, ndof, 1, M->d_dof_of_elim.p, g.p, ge.p, s);
-    tri_solve(M, ge.p, 1, s);
+    HIPCHK(tri_solve(M, ge.p, 1, s));
     launch_scatter_elim(n, ndof, 1, M->d_elim_of_dof.p, ge.p, psi.p, s);
     // phi' KG phi on the free set
     launch_gather_elim(n, ndof, 1, M->d_dof_of_elim.p, phi, ge.p, s);

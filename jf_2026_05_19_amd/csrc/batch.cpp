@@ -236,21 +236,21 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
         if (Rout) seg_upper_mul(T, b, G2.p, s16, 16, G1.p, s16, 16, Rout, sR, ldr, s);
     };
     auto step = [&](int jj) {
-        HIPCHK(hipMemcpyAsync(Wb.p, blk(jj), sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
-        solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Wb.p, b, s);
+        HIPCHK(solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, blk(jj), Wb.p, b, s));
         launch_scatter_elim_il(n, ndof, b, M->d_elim_of_dof.p, Wb.p, xf.p, s);
         launch_bsr_spmv(M->n_nodes, M->d_rp.p, M->d_col.p, M->d_KG.p, xf.p, yf.p, b, -1.0, s);
         launch_gather_elim_il(n, b, M->d_dof_of_elim.p, yf.p, Wb.p, s);
-        solve_fwd(M->plan, M->dev, M->dsolve, M->splan, M->d_P.p, M->d_W.p, M->d_part.p, Wb.p, b, s);
+        // into column block jj+1 of V, which the products of this step do not read
+        double *Wn = blk(jj + 1);
+        HIPCHK(solve_fwd(M->plan, M->dev, M->dsolve, M->splan, M->d_P.p, M->d_W.p, M->d_part.p, Wb.p, Wn, b, s));
         int kcols = (jj + 1) * b;
         double *Tcol = Tm.p + (size_t)jj * b * ldT;
-        seg_gemm_tn(T, kcols, b, V.p, n, Wb.p, n, Tcol, sT, ldT, 1.0, 0.0, part.p, s);
-        seg_gemm_nn(T, kcols, b, V.p, n, Tcol, sT, ldT, Wb.p, n, -1.0, 1.0, s);
-        seg_gemm_tn(T, b, b, Wb.p, n, Wb.p, n, G0.p, s16, 16, 1.0, 0.0, part.p, s);
-        seg_gemm_tn(T, kcols, b, V.p, n, Wb.p, n, H2.p, sH2, kcols, 1.0, 0.0, part.p, s);
-        seg_gemm_nn(T, kcols, b, V.p, n, H2.p, sH2, kcols, Wb.p, n, -1.0, 1.0, s);
+        seg_gemm_tn(T, kcols, b, V.p, n, Wn, n, Tcol, sT, ldT, 1.0, 0.0, part.p, s);
+        seg_gemm_nn(T, kcols, b, V.p, n, Tcol, sT, ldT, Wn, n, -1.0, 1.0, s);
+        seg_gemm_tn(T, b, b, Wn, n, Wn, n, G0.p, s16, 16, 1.0, 0.0, part.p, s);
+        seg_gemm_tn(T, kcols, b, V.p, n, Wn, n, H2.p, sH2, kcols, 1.0, 0.0, part.p, s);
+        seg_gemm_nn(T, kcols, b, V.p, n, H2.p, sH2, kcols, Wn, n, -1.0, 1.0, s);
         seg_mat_add(T, kcols, b, H2.p, sH2, kcols, Tcol, sT, ldT, s);
-        HIPCHK(hipMemcpyAsync(blk(jj + 1), Wb.p, sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
         cholqr2(blk(jj + 1), Tm.p + (size_t)(jj + 1) * b + (size_t)jj * b * ldT, sT, ldT, G0.p);
         return SOL105_OK;
     };
@@ -370,8 +370,10 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
     DevBuf<double> Xf;
     HIPCHK(Xf.alloc((size_t)ndof * std::max(kcols, 1)));
     if (kcols > 0) {
-        solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Y.p, kcols, s);
-        launch_scatter_elim(n, ndof, kcols, M->d_elim_of_dof.p, Y.p, Xf.p, s);
+        DevBuf<double> Ys;
+        HIPCHK(Ys.alloc((size_t)n * kcols));
+        HIPCHK(solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Y.p, Ys.p, kcols, s));
+        launch_scatter_elim(n, ndof, kcols, M->d_elim_of_dof.p, Ys.p, Xf.p, s);
     }
     // eigenvalue semantics per member
     std::vector<std::vector<int>> pos(nm);

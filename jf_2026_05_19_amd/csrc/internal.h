@@ -62,7 +62,7 @@ struct DevBuf {
 // loops, served requests) re-use the device buffers instead of allocating
 // and freeing them per solve.
 struct LanczosWork {
-    s105::DevBuf<double> V, Wb, xf, yf, T, A, Zs, thd, ework, part, small, H2, Ytmp, Y, Xf, scr, sh;
+    s105::DevBuf<double> V, Wb, xf, yf, T, A, Zs, thd, ework, part, small, H2, Ytmp, Y, Ys, Xf, scr, sh;
     s105::DevBuf<int> ierr, sw;
 };
 
@@ -89,7 +89,7 @@ struct sol105_model {
         d_dof_of_elim;
     s105::DevBuf<int64_t> f_off, f_rowoff, f_vecoff, f_reloff, f_kdst;
     s105::DevBuf<uint8_t> d_free, d_flag;
-    s105::DevBuf<double> d_front, d_piv, d_W, d_kdiag_elim;
+    s105::DevBuf<double> d_front, d_piv, d_W, d_kdiag_elim, d_tsol;  // d_tsol: the intermediate y of tri_solve
     s105::DevBuf<int> d_err;
     DevFronts dev;
     SolvePlan splan;
@@ -127,8 +127,8 @@ ElemArgs elem_args(sol105_model *M);
 // AUTOSPC on the diagonal, per member of a batch model
 void autospc_mask(const sol105_model *M, const std::vector<double> &kd, std::vector<uint8_t> &free_out);
 int upload_symbolic(sol105_ctx *ctx, sol105_model *M);
-// x <- K_ff^-1 x (elimination order, nrhs columns)
-void tri_solve(sol105_model *M, double *X, int nrhs, hipStream_t s);
+// x <- K_ff^-1 x (elimination order, nrhs <= MAXR columns)
+hipError_t tri_solve(sol105_model *M, double *X, int nrhs, hipStream_t s);
 double ev_ms(hipEvent_t a, hipEvent_t b);
 // host copies, validation, block pattern and device upload of a model
 int model_init(sol105_ctx *ctx, const sol105_model_desc *d, sol105_model *M);

@@ -37,7 +37,13 @@ struct ProdTimer {
 
 void solve_prepare(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *F,
                    double *Pbuf, double *PTbuf, hipStream_t s);
-void solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *Pbuf,
-               double *W, double *part, double *X, int nrhs, hipStream_t s, ProdTimer *pt = nullptr);
-void solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *PTbuf,
-               double *W, double *part, double *X, int nrhs, hipStream_t s, ProdTimer *pt = nullptr);
+// Forward (L y = b) and backward (L^T x = y) solves of nrhs <= MAXR
+// right-hand sides, column-major with ld n_free, from xin into xout.  The two
+// arrays must be distinct (hipErrorInvalidValue otherwise): every product
+// launch reads xin while it writes xout.
+hipError_t solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S,
+                     const double *Pbuf, double *W, double *part, const double *xin, double *xout, int nrhs,
+                     hipStream_t s, ProdTimer *pt = nullptr);
+hipError_t solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S,
+                     const double *PTbuf, double *W, double *part, const double *xin, double *xout, int nrhs,
+                     hipStream_t s, ProdTimer *pt = nullptr);

@@ -141,7 +141,9 @@ __global__ void __launch_bounds__(256) linv_col_kernel(DevFronts D, DevSolve DS,
                     Pf[(i0 + ti) + (int64_t)(j0 + c) * m] = -sacc;
                 }
             }
-        __threadfence();
+        // the block just written is read back (rows k0 of later steps) only by
+        // this workgroup: the barrier's workgroup-scope release/acquire orders
+        // it; a device-scope fence here would write back and invalidate L2
         __syncthreads();
     }
 }
@@ -242,37 +244,24 @@ __global__ void transpose_kernel(DevFronts D, const int32_t *flist, const double
 
 // Working vectors: front f owns m_f rows at W + vec_off[f] * ldv, row-major
 // with row stride ldv = MAXR for a block of right-hand sides (one 128-byte
-// line per row) and 1 for a single one.
+// line per row) and 1 for a single one.  Only the update rows [ns, m) are
+// written: w_f, which the parent's forward product adds in.
+//
+// The vector operand of a product is assembled by each workgroup for its own
+// k-slice, in LDS, from
+//   forward:  v[k] = x_in[piv0 + k] + the child update rows that the pull map
+//             lists for row k (fixed child order);
+//   backward: v[k] = x_in[piv0 + k] for k < ns and -x_out[rows[k]] for the
+//             update rows (ancestor unknowns, solved by earlier launches).
+// x_in and x_out are distinct column-major arrays (ld n_free): while some
+// workgroups of a launch read x_in, others write their pivots' x_out.
 
-// forward assembly: V_f = [x_piv ; 0] + sum over children of their w, one
-// thread per (row, rhs); each row pulls its child rows in fixed child order
-__global__ void fwd_pull_kernel(DevFronts D, DevSolve DS, const int32_t *flist, double *W, const double *X,
-                                int n_free, int nrhs, int ldv) {
-    int f = flist[blockIdx.y];
-    int ns = D.ns[f], m = D.m[f], piv0 = D.piv0[f];
-    int64_t slot0 = D.vec_off[f];
-    double *V = W + slot0 * ldv;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m * nrhs; t += gridDim.x * blockDim.x) {
-        int i = t / nrhs, r = t % nrhs;
-        double v = (i < ns) ? X[(piv0 + i) + (int64_t)r * n_free] : 0.0;
-        for (int q = DS.pull_ptr[slot0 + i]; q < DS.pull_ptr[slot0 + i + 1]; q++)
-            v += W[(int64_t)DS.pull_src[q] * ldv + r];
-        V[(int64_t)i * ldv + r] = v;
-    }
-}
-
-// backward gather: V_f = [x_piv ; -x(update rows)]
-__global__ void bwd_gather_kernel(DevFronts D, const int32_t *flist, double *W, const double *X, int n_free,
-                                  int nrhs, int ldv) {
-    int f = flist[blockIdx.y];
-    int ns = D.ns[f], m = D.m[f];
-    const int32_t *rows = D.rows + D.row_off[f];
-    double *V = W + D.vec_off[f] * ldv;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m * nrhs; t += gridDim.x * blockDim.x) {
-        int i = t / nrhs, r = t % nrhs;
-        double v = X[rows[i] + (int64_t)r * n_free];
-        V[(int64_t)i * ldv + r] = (i < ns) ? v : -v;
-    }
+// v plus the child update rows pulled into vector slot `slot`, column c
+__device__ __forceinline__ double pulled(const DevSolve &DS, const double *W, int ldv, int64_t slot, int c,
+                                         double v) {
+    const int q0 = DS.pull_ptr[slot], q1 = DS.pull_ptr[slot + 1];
+    for (int q = q0; q < q1; q++) v += W[(int64_t)DS.pull_src[q] * ldv + c];
+    return v;
 }
 
 // partial[tile][64][nrhs] = A[rows of tile, k-slice] * V[k-slice]
@@ -290,30 +279,35 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
     return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
-__global__ void __launch_bounds__(256) prod1_kernel(DevFronts D, const int32_t *tiles, const double *A,
-                                                    const int64_t *poff, double *W, double *part, int bwd,
-                                                    const int32_t *kwid, const int32_t *nks, double *X, int n_free) {
+__global__ void __launch_bounds__(256) prod1_kernel(DevFronts D, DevSolve DS, const int32_t *tiles, const double *A,
+                                                    double *W, double *part, int bwd, const int32_t *kwid,
+                                                    const int32_t *nks, const double *xin, double *xout, int n_free) {
     constexpr int NR = 1, nrhs = 1;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     int f = tiles[3 * tile], rt = tiles[3 * tile + 1], ks = tiles[3 * tile + 2];
-    int ns = D.ns[f], m = D.m[f];
+    int ns = D.ns[f], m = D.m[f], piv0 = D.piv0[f];
     int nrows = bwd ? ns : m;
     int inner = bwd ? m : ns;
     int lda = bwd ? ns : m;
-    const double *Af = A + poff[f];
-    double *V = W + D.vec_off[f] * nrhs;
+    const double *Af = A + DS.poff[f];
+    const int64_t slot0 = D.vec_off[f];
+    double *V = W + slot0 * nrhs;
     const int kw = kwid[f];
     const int KW = (kw + 3) / 4;  // k per wavefront
     int k0 = ks * kw, k1 = min(inner, k0 + kw);
-    // the V slice and, after the products, the cross-wavefront partial sums
-    // share one LDS buffer
+    // the vector slice and, after the products, the cross-wavefront partial
+    // sums share one LDS buffer
     constexpr int VS_N = SOLVE_KMAX * NR, RED_N = 3 * RT * (NR + 1);
     __shared__ double lds[VS_N > RED_N ? VS_N : RED_N];
     double (*Vs)[NR] = reinterpret_cast<double (*)[NR]>(lds);
     double (*red)[RT][NR + 1] = reinterpret_cast<double (*)[RT][NR + 1]>(lds);
-    for (int t = threadIdx.x; t < (k1 - k0) * NR; t += blockDim.x) {
-        int k = t / NR, r = t % NR;
-        Vs[k][r] = (r < nrhs) ? V[(int64_t)(k0 + k) * nrhs + r] : 0.0;
+    const int32_t *rows = D.rows + D.row_off[f];
+    for (int k = threadIdx.x; k < k1 - k0; k += blockDim.x) {
+        const int kk = k0 + k;
+        double v;
+        if (bwd) v = (kk < ns) ? xin[piv0 + kk] : -xout[rows[kk]];
+        else v = pulled(DS, W, 1, slot0 + kk, 0, xin[piv0 + kk]);
+        Vs[k][0] = v;
     }
     __syncthreads();
     int row = threadIdx.x % RT, g = threadIdx.x / RT;
@@ -349,12 +343,11 @@ __global__ void __launch_bounds__(256) prod1_kernel(DevFronts D, const int32_t *
     if (g == 0 && i < nrows) {
         if (nks[f] == 1) {
             // single k-slice: the sum is final.  Nothing else in this launch
-            // reads x_piv or the update rows of V (fwd reads V[0:ns] only).
-            const int piv0 = D.piv0[f];
+            // reads this front's pivots of x_out or its update rows.
             for (int r = 0; r < nrhs; r++) {
                 double s = acc[r] + red[0][row][r] + red[1][row][r] + red[2][row][r];
-                if (bwd || i < ns) X[(piv0 + i) + (int64_t)r * n_free] = s;
-                else V[(int64_t)i * nrhs + r] -= s;
+                if (bwd || i < ns) xout[(piv0 + i) + (int64_t)r * n_free] = s;
+                else V[(int64_t)i * nrhs + r] = pulled(DS, W, 1, slot0 + i, r, 0.0) - s;
             }
         } else {
             for (int r = 0; r < nrhs; r++) {
@@ -370,13 +363,14 @@ __global__ void __launch_bounds__(256) prod1_kernel(DevFronts D, const int32_t *
 // supplies A[l%16][l/16] and B[l/16][l%16] and holds D[l/16 + 4v][l%16],
 // v = 0..3).  Each wavefront covers all 64 rows of the tile (four 16-row
 // blocks, four accumulators) over a quarter of the k-slice, so one B value
-// (V[k + l/16][l%16]: 512 contiguous bytes per wavefront) feeds four
-// products, and the factor is read in the column-major front layout, 128
-// contiguous bytes per block column.  A chunk of 8 four-k steps (32 factor
-// loads and 8 B loads per lane) is issued before its products.  The four
-// partial sums meet once in LDS and are added in wavefront order (the result
-// is bit-reproducible); the write-back is then coalesced, row-fastest into
-// the column-major x and column-fastest into the row-major update rows.
+// (the slice row k + l/16, column l%16, from LDS) feeds four products, and the
+// factor is read in the column-major front layout, 128 contiguous bytes per
+// block column.  A chunk of 8 four-k steps (32 factor loads per lane) is
+// issued before its products; the first chunk goes out before the vector
+// slice is assembled, so the slice's loads overlap it.  The four partial sums
+// meet once in LDS and are added in wavefront order (the result is
+// bit-reproducible); the write-back is then coalesced, row-fastest into the
+// column-major x_out and column-fastest into the row-major update rows.
 // The factor loads are not masked.  Row blocks wholly past the front re-read
 // the last live block and steps past a wavefront's range re-read its last
 // step (cache hits, no new memory traffic; B is zero there); the remaining
@@ -385,15 +379,17 @@ __global__ void __launch_bounds__(256) prod1_kernel(DevFronts D, const int32_t *
 // later fronts or the zero tail of the buffer (p_pad), finite either way.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, const int32_t *tiles, const double *A,
-                                                        const int64_t *poff, double *W, double *part, int nrhs,
-                                                        int bwd, const int32_t *kwid, const int32_t *nks, double *X,
-                                                        int n_free) {
+__global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS, const int32_t *tiles,
+                                                        const double *A, double *W, double *part, int nrhs, int bwd,
+                                                        const int32_t *kwid, const int32_t *nks, const double *xin,
+                                                        double *xout, int n_free) {
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int f = tiles[3 * tile], rt = tiles[3 * tile + 1], ks = tiles[3 * tile + 2];
-    const int ns = D.ns[f], m = D.m[f];
+    const int ns = D.ns[f], m = D.m[f], piv0 = D.piv0[f];
     const int nrows = bwd ? ns : m, inner = bwd ? m : ns, lda = bwd ? ns : m;
-    double *V = W + D.vec_off[f] * MAXR;
+    const bool whole = nks[f] == 1;
+    const int64_t slot0 = D.vec_off[f];
+    double *V = W + slot0 * MAXR;
     const int kw = kwid[f];
     const int k0 = ks * kw, kl = min(inner, k0 + kw) - k0;
     const int nst = (kl + 3) >> 2;   // four-k steps of the slice
@@ -401,69 +397,122 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, const int32
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int s0 = min(nst, wv * q), s1 = min(nst, s0 + q);
     const int nb = min(4, (nrows - rt * RT + 15) >> 4);  // live 16-row blocks, >= 1
-    const int lane = threadIdx.x & 63, kq = lane >> 4, col = lane & 15;
-    const double *Aw = A + poff[f] + rt * RT + (int64_t)k0 * lda;  // wavefront-uniform
+    const int t = threadIdx.x, lane = t & 63, kq = lane >> 4, col = lane & 15;
+    const int r0 = rt * RT;
+    const double *Aw = A + DS.poff[f] + r0 + (int64_t)k0 * lda;  // wavefront-uniform
     unsigned aoff[4];
 #pragma unroll
     for (int b = 0; b < 4; b++) aoff[b] = (unsigned)((lane & 15) + 16 * min(b, nb - 1) + kq * lda);
-    const double *Vw = V + (int64_t)k0 * MAXR;
-    // B masks are bitwise (all ones or zero), so no load is placed under
-    // divergent control flow
-    const uint64_t cmask = col < nrhs ? ~0ull : 0ull;
-    dbl4 acc[4];
-#pragma unroll
-    for (int b = 0; b < 4; b++) acc[b] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for (int s = s0; s < s1; s += 8) {
-        double av[8][4], bv[8];
+    double av[8][4];
+    auto load_a = [&](int s) {
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int ss = min(s + u, s1 - 1);
             const double *pa = Aw + (int64_t)(4 * ss) * lda;
 #pragma unroll
             for (int b = 0; b < 4; b++) av[u][b] = pa[aoff[b]];
-            const uint64_t msk = (s + u < s1 && 4 * ss + kq < kl) ? cmask : 0ull;
-            bv[u] = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, Vw[4 * ss * MAXR + lane]) & msk);
+        }
+    };
+    if (s0 < s1) load_a(s0);
+    // forward, single k-slice: the child rows pulled into this tile's update
+    // rows (written back below), loaded now so they arrive with the factor
+    double upd[4] = {0.0, 0.0, 0.0, 0.0};
+    if (!bwd && whole && r0 + RT > ns)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int i = r0 + (t >> 4) + 16 * j, c = t & 15;
+            if (i >= ns && i < nrows && c < nrhs) upd[j] = pulled(DS, W, MAXR, slot0 + i, c, 0.0);
+        }
+    // LDS: the vector slice (rows up to 4 nst <= SOLVE_KMAX, padded columns),
+    // after the products the cross-wavefront partial sums
+    constexpr int VS_N = SOLVE_KMAX * (MAXR + 1), RED_N = 4 * RT * (MAXR + 1);
+    __shared__ double lds[VS_N > RED_N ? VS_N : RED_N];
+    double (*Vs)[MAXR + 1] = reinterpret_cast<double (*)[MAXR + 1]>(lds);
+    double (*red)[RT][MAXR + 1] = reinterpret_cast<double (*)[RT][MAXR + 1]>(lds);
+    {
+        // right-hand side or ancestor values, k fastest (x columns are
+        // contiguous); rows past the slice and columns past nrhs are zero
+        const int kpad = 4 * nst;
+        const int32_t *rows = D.rows + D.row_off[f];
+        for (int k = t; k < kpad; k += 256) {
+            const int kk = k0 + min(k, kl - 1);
+            const double *src = xin + piv0 + kk;
+            double sg = 1.0;
+            if (bwd && kk >= ns) {
+                src = xout + rows[kk];
+                sg = -1.0;
+            }
+            double xv[MAXR];
+#pragma unroll
+            for (int c = 0; c < MAXR; c++) xv[c] = src[(int64_t)min(c, nrhs - 1) * n_free];
+#pragma unroll
+            for (int c = 0; c < MAXR; c++) Vs[k][c] = (k < kl && c < nrhs) ? sg * xv[c] : 0.0;
+        }
+        // forward: add the pulled child rows, one slice row per thread (a
+        // child row is 128 contiguous bytes), children in their fixed order
+        if (!bwd && D.child_ptr[f + 1] > D.child_ptr[f]) {
+            __syncthreads();
+            for (int k = t; k < kl; k += 256) {
+                const int64_t slot = slot0 + k0 + k;
+                const int q0 = DS.pull_ptr[slot], q1 = DS.pull_ptr[slot + 1];
+                double v[MAXR];
+#pragma unroll
+                for (int c = 0; c < MAXR; c++) v[c] = Vs[k][c];
+                for (int qq = q0; qq < q1; qq++) {
+                    const double *w = W + (int64_t)DS.pull_src[qq] * MAXR;
+                    double wv2[MAXR];
+#pragma unroll
+                    for (int c = 0; c < MAXR; c++) wv2[c] = w[c];
+#pragma unroll
+                    for (int c = 0; c < MAXR; c++) v[c] += wv2[c];
+                }
+#pragma unroll
+                for (int c = 0; c < MAXR; c++)
+                    if (c < nrhs) Vs[k][c] = v[c];
+            }
+        }
+    }
+    __syncthreads();
+    dbl4 acc[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[b] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int s = s0; s < s1; s += 8) {
+        if (s > s0) load_a(s);
+        double bv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int ss = min(s + u, s1 - 1);
+            bv[u] = (s + u < s1) ? Vs[4 * ss + kq][col] : 0.0;  // wavefront-uniform condition
         }
 #pragma unroll
         for (int u = 0; u < 8; u++)
 #pragma unroll
             for (int b = 0; b < 4; b++) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][b], bv[u], acc[b], 0, 0, 0);
-        // all 40 loads of the chunk are issued before its products
-        __builtin_amdgcn_sched_group_barrier(0x020, 40, 0);
+        __builtin_amdgcn_sched_group_barrier(0x020, 32, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
         __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
     }
-    __shared__ double red[4][RT][MAXR + 1];
+    __syncthreads();  // the slice is dead; its LDS takes the partial sums
 #pragma unroll
     for (int b = 0; b < 4; b++)
 #pragma unroll
         for (int v = 0; v < 4; v++) red[wv][16 * b + kq + 4 * v][col] = acc[b][v];
     __syncthreads();
-    const int t = threadIdx.x, r0 = rt * RT;
     auto sum = [&](int row, int c) { return red[0][row][c] + red[1][row][c] + red[2][row][c] + red[3][row][c]; };
-    if (nks[f] == 1) {
+    if (whole) {
         // single k-slice: the sum is final (no other workgroup of the launch
-        // reads x_piv or the update rows of V; fwd reads V[0:ns] only)
-        const int piv0 = D.piv0[f];
+        // reads this front's pivots of x_out or its update rows)
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int e = t + 256 * j, row = e & 63, c = e >> 6, i = r0 + row;
-            if (i < nrows && c < nrhs && (bwd || i < ns)) X[(piv0 + i) + (int64_t)c * n_free] = sum(row, c);
+            if (i < nrows && c < nrhs && (bwd || i < ns)) xout[(piv0 + i) + (int64_t)c * n_free] = sum(row, c);
         }
-        if (!bwd) {
-            // update rows: the four old values are read together (rows
-            // clamped into the front) before any store
-            double old[4];
+        if (!bwd)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const int e = t + 256 * j;
-                old[j] = V[(int64_t)min(r0 + (e >> 4), nrows - 1) * MAXR + (e & 15)];
+                const int row = (t >> 4) + 16 * j, c = t & 15, i = r0 + row;
+                if (i >= ns && i < nrows && c < nrhs) V[(int64_t)i * MAXR + c] = upd[j] - sum(row, c);
             }
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int e = t + 256 * j, row = e >> 4, c = e & 15, i = r0 + row;
-                if (i >= ns && i < nrows && c < nrhs) V[(int64_t)i * MAXR + c] = old[j] - sum(row, c);
-            }
-        }
     } else {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -474,11 +523,13 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, const int32
 }
 
 // fixed-order reduction of the k-slices and write-back
-__global__ void fwd_finish_kernel(DevFronts D, const int32_t *flist, const int32_t *tbase, const int32_t *nks,
-                                  const double *part, double *W, double *X, int n_free, int nrhs, int ldv) {
+__global__ void fwd_finish_kernel(DevFronts D, DevSolve DS, const int32_t *flist, const int32_t *tbase,
+                                  const int32_t *nks, const double *part, double *W, double *xout, int n_free,
+                                  int nrhs, int ldv) {
     int f = flist[blockIdx.y];
     int ns = D.ns[f], m = D.m[f], piv0 = D.piv0[f];
-    double *V = W + D.vec_off[f] * ldv;
+    const int64_t slot0 = D.vec_off[f];
+    double *V = W + slot0 * ldv;
     int nk = nks[f];
     if (nk == 1) return;  // written by the product kernel
     int64_t base = tbase[f];
@@ -487,13 +538,13 @@ __global__ void fwd_finish_kernel(DevFronts D, const int32_t *flist, const int32
         int rt = i / RT, rr = i % RT;
         double s = 0.0;
         for (int k = 0; k < nk; k++) s += part[((base + (int64_t)rt * nk + k) * RT + rr) * MAXR + r];
-        if (i < ns) X[(piv0 + i) + (int64_t)r * n_free] = s;
-        else V[(int64_t)i * ldv + r] -= s;
+        if (i < ns) xout[(piv0 + i) + (int64_t)r * n_free] = s;
+        else V[(int64_t)i * ldv + r] = pulled(DS, W, ldv, slot0 + i, r, 0.0) - s;
     }
 }
 
 __global__ void bwd_finish_kernel(DevFronts D, const int32_t *flist, const int32_t *tbase, const int32_t *nks,
-                                  const double *part, double *X, int n_free, int nrhs) {
+                                  const double *part, double *xout, int n_free, int nrhs) {
     int f = flist[blockIdx.y];
     int ns = D.ns[f], piv0 = D.piv0[f];
     int nk = nks[f];
@@ -504,7 +555,7 @@ __global__ void bwd_finish_kernel(DevFronts D, const int32_t *flist, const int32
         int rt = i / RT, rr = i % RT;
         double s = 0.0;
         for (int k = 0; k < nk; k++) s += part[((base + (int64_t)rt * nk + k) * RT + rr) * MAXR + r];
-        X[(piv0 + i) + (int64_t)r * n_free] = s;
+        xout[(piv0 + i) + (int64_t)r * n_free] = s;
     }
 }
 
@@ -571,55 +622,56 @@ static double prod_bytes(const SolveHeight &SH, int nrhs) {
     return 8.0 * ((double)SH.sum_mns + (double)nrhs * (double)(SH.sum_m + SH.sum_ns));
 }
 
-void solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *Pbuf,
-               double *W, double *part, double *X, int nrhs, hipStream_t s, ProdTimer *pt) {
+hipError_t solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S,
+                     const double *Pbuf, double *W, double *part, const double *xin, double *xout, int nrhs,
+                     hipStream_t s, ProdTimer *pt) {
+    if (xin == xout || nrhs < 1 || nrhs > MAXR) return hipErrorInvalidValue;
     const int ldv = nrhs > 1 ? MAXR : 1;
     for (int h = 0; h <= P.height; h++) {
         const HeightInfo &H = P.heights[h];
         const SolveHeight &SH = S.heights[h];
         if (H.n == 0) continue;
         const int32_t *fl = D.flist + H.list_off;
-        fwd_pull_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, DS, fl, W, X, P.n_free, nrhs,
-                                                                                     ldv);
         const bool timed = pt && nrhs > 1 && h == 0;
         if (timed) pt->begin(s, prod_bytes(SH, nrhs));
         const int32_t *tl = DS.fwd_tiles + 3 * SH.fwd_off;
         if (nrhs == 1)
-            prod1_kernel<<<SH.n_fwd, 256, 0, s>>>(D, tl, Pbuf, DS.poff, W, part, 0, DS.fwd_kw, DS.fwd_nks, X, P.n_free);
+            prod1_kernel<<<SH.n_fwd, 256, 0, s>>>(D, DS, tl, Pbuf, W, part, 0, DS.fwd_kw, DS.fwd_nks, xin, xout,
+                                                  P.n_free);
         else
-            prod_mfma_kernel<<<SH.n_fwd, 256, 0, s>>>(D, tl, Pbuf, DS.poff, W, part, nrhs, 0, DS.fwd_kw, DS.fwd_nks,
-                                                      X, P.n_free);
+            prod_mfma_kernel<<<SH.n_fwd, 256, 0, s>>>(D, DS, tl, Pbuf, W, part, nrhs, 0, DS.fwd_kw, DS.fwd_nks, xin,
+                                                      xout, P.n_free);
         if (timed) pt->end(s);
         // single-slice fronts were finished by the product itself
         if (SH.fwd_split)
-            fwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.fwd_tbase_rel,
-                                                                                       DS.fwd_nks, part, W, X,
-                                                                                       P.n_free, nrhs, ldv);
+            fwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(
+                D, DS, fl, DS.fwd_tbase_rel, DS.fwd_nks, part, W, xout, P.n_free, nrhs, ldv);
     }
+    return hipSuccess;
 }
 
-void solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, const double *PTbuf,
-               double *W, double *part, double *X, int nrhs, hipStream_t s, ProdTimer *pt) {
-    const int ldv = nrhs > 1 ? MAXR : 1;
+hipError_t solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S,
+                     const double *PTbuf, double *W, double *part, const double *xin, double *xout, int nrhs,
+                     hipStream_t s, ProdTimer *pt) {
+    if (xin == xout || nrhs < 1 || nrhs > MAXR) return hipErrorInvalidValue;
     for (int h = P.height; h >= 0; h--) {
         const HeightInfo &H = P.heights[h];
         const SolveHeight &SH = S.heights[h];
         if (H.n == 0) continue;
         const int32_t *fl = D.flist + H.list_off;
-        bwd_gather_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(D, fl, W, X, P.n_free, nrhs,
-                                                                                       ldv);
         const bool timed = pt && nrhs > 1 && h == 0;
         if (timed) pt->begin(s, prod_bytes(SH, nrhs));
         const int32_t *tl = DS.bwd_tiles + 3 * SH.bwd_off;
         if (nrhs == 1)
-            prod1_kernel<<<SH.n_bwd, 256, 0, s>>>(D, tl, PTbuf, DS.poff, W, part, 1, DS.bwd_kw, DS.bwd_nks, X, P.n_free);
+            prod1_kernel<<<SH.n_bwd, 256, 0, s>>>(D, DS, tl, PTbuf, W, part, 1, DS.bwd_kw, DS.bwd_nks, xin, xout,
+                                                  P.n_free);
         else
-            prod_mfma_kernel<<<SH.n_bwd, 256, 0, s>>>(D, tl, PTbuf, DS.poff, W, part, nrhs, 1, DS.bwd_kw, DS.bwd_nks,
-                                                      X, P.n_free);
+            prod_mfma_kernel<<<SH.n_bwd, 256, 0, s>>>(D, DS, tl, PTbuf, W, part, nrhs, 1, DS.bwd_kw, DS.bwd_nks, xin,
+                                                      xout, P.n_free);
         if (timed) pt->end(s);
         if (SH.bwd_split)
-            bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(D, fl, DS.bwd_tbase_rel,
-                                                                                        DS.bwd_nks, part, X,
-                                                                                        P.n_free, nrhs);
+            bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(
+                D, fl, DS.bwd_tbase_rel, DS.bwd_nks, part, xout, P.n_free, nrhs);
     }
+    return hipSuccess;
 }
