@@ -145,7 +145,19 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
             for (size_t q = 0; q < slot.size(); q++)
                 slot[q] = (int32_t)(S.fronts[SP.pull_src[2 * q]].vec_off + SP.pull_src[2 * q + 1]);
             HIPCHK(M->s_pullsrc.upload(slot.data(), slot.size(), s));
-            HIPCHK(hipStreamSynchronize(s));  // slot is a host temporary
+            // the same map as (first, second) child slot per row, -1 for none,
+            // when no row has more than two entries (a binary tree): one
+            // dependent load per row instead of two
+            std::vector<int32_t> pair(2 * (size_t)S.vec_storage, -1);
+            bool pairs_ok = true;
+            for (int64_t v = 0; v < S.vec_storage && pairs_ok; v++) {
+                const int q0 = SP.pull_ptr[v], cnt = SP.pull_ptr[v + 1] - q0;
+                if (cnt > 2) pairs_ok = false;
+                for (int j = 0; j < cnt && j < 2; j++) pair[2 * v + j] = slot[q0 + j];
+            }
+            if (pairs_ok) HIPCHK(M->s_pull2.upload(pair.data(), pair.size(), s));
+            M->pull2_ok = pairs_ok;
+            HIPCHK(hipStreamSynchronize(s));  // slot and pair are host temporaries
         }
         HIPCHK(M->s_lblk.upload(SP.lblk.data(), SP.lblk.size(), s));
         HIPCHK(M->d_P.alloc(SP.p_storage + SP.p_pad));
@@ -167,6 +179,7 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
         DS.pull_src = M->s_pullsrc.p;
         DS.lblk = M->s_lblk.p;
         DS.n_lblk = (int)(SP.lblk.size() / 2);
+        DS.pull2 = M->pull2_ok ? M->s_pull2.p : nullptr;
     }
     M->have_sym = true;
     return SOL105_OK;

@@ -33,6 +33,7 @@ struct SolveHeight {
     int max_m = 0, max_ns = 0;
     int64_t sum_mns = 0, sum_m = 0, sum_ns = 0;  // factor entries and vector rows of this height
     bool fwd_split = false, bwd_split = false;   // some front's product is split over k (needs a finish pass)
+    int fwd_kmax = 0, bwd_kmax = 0;              // widest k-slice of this height's products
 };
 
 struct SolvePlan {

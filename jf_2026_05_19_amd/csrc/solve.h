@@ -16,6 +16,9 @@ struct DevSolve {
     const int32_t *pull_ptr, *pull_src;  // pull map: per vector slot, the child slots it adds (absolute)
     const int32_t *lblk;  // (front, first column) of every 32-wide L11 column block
     int n_lblk;
+    // the pull map as (first, second) child slot per vector slot, -1 for
+    // none; nullptr when some row has more than two entries
+    const int32_t *pull2;
 };
 
 // Optional HIP-event timing of the solve product launches (diagnostics):

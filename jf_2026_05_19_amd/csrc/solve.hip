@@ -14,6 +14,7 @@
 #include "factor.h"
 #include "solve.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace {
@@ -256,20 +257,21 @@ __global__ void transpose_kernel(DevFronts D, const int32_t *flist, const double
 // x_in and x_out are distinct column-major arrays (ld n_free): while some
 // workgroups of a launch read x_in, others write their pivots' x_out.
 
-// v plus the child update rows pulled into vector slot `slot`, column c
+// v plus the child update rows pulled into vector slot `slot`, column c, in
+// the pull map's order.  With the pair table both rows are loaded together
+// (slot 0 stands in for a missing one and is masked).
 __device__ __forceinline__ double pulled(const DevSolve &DS, const double *W, int ldv, int64_t slot, int c,
                                          double v) {
-    const int q0 = DS.pull_ptr[slot], q1 = DS.pull_ptr[slot + 1];
-    for (int q = q0; q < q1; q++) v += W[(int64_t)DS.pull_src[q] * ldv + c];
+    if (DS.pull2) {
+        const int2 e = reinterpret_cast<const int2 *>(DS.pull2)[slot];
+        const double wa = W[(int64_t)max(e.x, 0) * ldv + c], wb = W[(int64_t)max(e.y, 0) * ldv + c];
+        v = e.x >= 0 ? v + wa : v;
+        return e.y >= 0 ? v + wb : v;
+    }
+    for (int q = DS.pull_ptr[slot]; q < DS.pull_ptr[slot + 1]; q++) v += W[(int64_t)DS.pull_src[q] * ldv + c];
     return v;
 }
 
-// partial[tile][64][nrhs] = A[rows of tile, k-slice] * V[k-slice]
-// fwd: A = P_f (m x ns, ld m), input = V_f[0:ns];  bwd: A = PT_f (ns x m, ld ns), input = V_f[0:m]
-// Single right-hand side (row stride 1): wavefront g of the workgroup streams
-// a quarter of the k-slice for the tile's 64 rows (one row per lane, 16 loads
-// in flight per lane); the four partial sums are added in wavefront order
-// (deterministic).
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
 // on XCD b % 8), each with its own L2.  This bijection hands every XCD a
 // contiguous run of logical tiles, so the row tiles of one front (which all
@@ -416,17 +418,18 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
     if (s0 < s1) load_a(s0);
     // forward, single k-slice: the child rows pulled into this tile's update
     // rows (written back below), loaded now so they arrive with the factor
+    const bool parent = D.child_ptr[f + 1] > D.child_ptr[f];  // leaves pull nothing
     double upd[4] = {0.0, 0.0, 0.0, 0.0};
-    if (!bwd && whole && r0 + RT > ns)
+    if (!bwd && whole && parent && r0 + RT > ns)
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int i = r0 + (t >> 4) + 16 * j, c = t & 15;
             if (i >= ns && i < nrows && c < nrhs) upd[j] = pulled(DS, W, MAXR, slot0 + i, c, 0.0);
         }
-    // LDS: the vector slice (rows up to 4 nst <= SOLVE_KMAX, padded columns),
-    // after the products the cross-wavefront partial sums
-    constexpr int VS_N = SOLVE_KMAX * (MAXR + 1), RED_N = 4 * RT * (MAXR + 1);
-    __shared__ double lds[VS_N > RED_N ? VS_N : RED_N];
+    // LDS (dynamic, sized for the launch's widest slice): the vector slice
+    // (4 nst rows, padded columns), after the products the cross-wavefront
+    // partial sums
+    extern __shared__ double lds[];
     double (*Vs)[MAXR + 1] = reinterpret_cast<double (*)[MAXR + 1]>(lds);
     double (*red)[RT][MAXR + 1] = reinterpret_cast<double (*)[RT][MAXR + 1]>(lds);
     {
@@ -448,27 +451,35 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
 #pragma unroll
             for (int c = 0; c < MAXR; c++) Vs[k][c] = (k < kl && c < nrhs) ? sg * xv[c] : 0.0;
         }
-        // forward: add the pulled child rows, one slice row per thread (a
-        // child row is 128 contiguous bytes), children in their fixed order
-        if (!bwd && D.child_ptr[f + 1] > D.child_ptr[f]) {
+        // forward: add the pulled child rows (128 contiguous bytes each, in
+        // the pull map's order), one slice row per thread, in two halves
+        if (!bwd && parent) {
             __syncthreads();
+            const int2 *pairs = reinterpret_cast<const int2 *>(DS.pull2);
             for (int k = t; k < kl; k += 256) {
                 const int64_t slot = slot0 + k0 + k;
-                const int q0 = DS.pull_ptr[slot], q1 = DS.pull_ptr[slot + 1];
-                double v[MAXR];
+                if (pairs) {
+                    const int2 e = pairs[slot];
+                    const double *wa = W + (int64_t)max(e.x, 0) * MAXR, *wb = W + (int64_t)max(e.y, 0) * MAXR;
 #pragma unroll
-                for (int c = 0; c < MAXR; c++) v[c] = Vs[k][c];
-                for (int qq = q0; qq < q1; qq++) {
-                    const double *w = W + (int64_t)DS.pull_src[qq] * MAXR;
-                    double wv2[MAXR];
+                    for (int h = 0; h < MAXR; h += 8) {
+                        double xa[8], xb[8];
 #pragma unroll
-                    for (int c = 0; c < MAXR; c++) wv2[c] = w[c];
+                        for (int c = 0; c < 8; c++) {
+                            xa[c] = wa[h + c];
+                            xb[c] = wb[h + c];
+                        }
 #pragma unroll
-                    for (int c = 0; c < MAXR; c++) v[c] += wv2[c];
+                        for (int c = 0; c < 8; c++) {
+                            double v = Vs[k][h + c];
+                            v = e.x >= 0 ? v + xa[c] : v;
+                            v = e.y >= 0 ? v + xb[c] : v;
+                            if (h + c < nrhs) Vs[k][h + c] = v;
+                        }
+                    }
+                } else {
+                    for (int c = 0; c < nrhs; c++) Vs[k][c] = pulled(DS, W, MAXR, slot, c, Vs[k][c]);
                 }
-#pragma unroll
-                for (int c = 0; c < MAXR; c++)
-                    if (c < nrhs) Vs[k][c] = v[c];
             }
         }
     }
@@ -617,6 +628,13 @@ ProdTimer::~ProdTimer() {
     for (hipEvent_t e : pool) (void)hipEventDestroy(e);
 }
 
+// LDS bytes of a block-RHS product launch whose widest k-slice is kmax: the
+// slice rows (rounded up to four-k steps) or the partial sums, the larger
+static size_t prod_lds(int kmax) {
+    const size_t slice = (size_t)4 * ((kmax + 3) / 4) * (MAXR + 1), red = (size_t)4 * RT * (MAXR + 1);
+    return sizeof(double) * std::max(slice, red);
+}
+
 // algorithmic bytes of one product launch: factor entries, vector rows in and out
 static double prod_bytes(const SolveHeight &SH, int nrhs) {
     return 8.0 * ((double)SH.sum_mns + (double)nrhs * (double)(SH.sum_m + SH.sum_ns));
@@ -639,8 +657,8 @@ hipError_t solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS
             prod1_kernel<<<SH.n_fwd, 256, 0, s>>>(D, DS, tl, Pbuf, W, part, 0, DS.fwd_kw, DS.fwd_nks, xin, xout,
                                                   P.n_free);
         else
-            prod_mfma_kernel<<<SH.n_fwd, 256, 0, s>>>(D, DS, tl, Pbuf, W, part, nrhs, 0, DS.fwd_kw, DS.fwd_nks, xin,
-                                                      xout, P.n_free);
+            prod_mfma_kernel<<<SH.n_fwd, 256, prod_lds(SH.fwd_kmax), s>>>(D, DS, tl, Pbuf, W, part, nrhs, 0,
+                                                                          DS.fwd_kw, DS.fwd_nks, xin, xout, P.n_free);
         if (timed) pt->end(s);
         // single-slice fronts were finished by the product itself
         if (SH.fwd_split)
@@ -666,8 +684,8 @@ hipError_t solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS
             prod1_kernel<<<SH.n_bwd, 256, 0, s>>>(D, DS, tl, PTbuf, W, part, 1, DS.bwd_kw, DS.bwd_nks, xin, xout,
                                                   P.n_free);
         else
-            prod_mfma_kernel<<<SH.n_bwd, 256, 0, s>>>(D, DS, tl, PTbuf, W, part, nrhs, 1, DS.bwd_kw, DS.bwd_nks, xin,
-                                                      xout, P.n_free);
+            prod_mfma_kernel<<<SH.n_bwd, 256, prod_lds(SH.bwd_kmax), s>>>(D, DS, tl, PTbuf, W, part, nrhs, 1,
+                                                                          DS.bwd_kw, DS.bwd_nks, xin, xout, P.n_free);
         if (timed) pt->end(s);
         if (SH.bwd_split)
             bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(

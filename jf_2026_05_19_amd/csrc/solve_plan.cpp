@@ -49,6 +49,8 @@ void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std
             const int kwb = m[f] <= SOLVE_KMAX ? std::max(m[f], 1) : KS;
             S.fwd_kw[f] = kwf;
             S.bwd_kw[f] = kwb;
+            SH.fwd_kmax = std::max(SH.fwd_kmax, std::min(kwf, std::max(ns[f], 1)));
+            SH.bwd_kmax = std::max(SH.bwd_kmax, std::min(kwb, std::max(m[f], 1)));
             int nrt = (m[f] + RT - 1) / RT, nk = (ns[f] + kwf - 1) / kwf;
             S.fwd_tbase[f] = (int32_t)(S.fwd_tiles.size() / 3 - SH.fwd_off);
             S.fwd_nks[f] = nk;
