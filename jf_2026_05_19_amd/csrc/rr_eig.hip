@@ -487,10 +487,13 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
 // then chases the bulge down in blocks of bw rows; each step applies the
 // previous reflector from the right to the block below, annihilates that
 // block's first column with a new reflector and applies it from both sides
-// to the next diagonal block.  Sweeps run pipelined, sweep i three steps
-// behind sweep i-1 (their blocks are then disjoint and every overlapping pair
-// of tasks keeps its sequential order), one wavefront per active sweep and
-// one workgroup barrier per step, the band in LDS.  Steps 2 and 3 are those
+// to the next diagonal block.  Sweeps run pipelined, sweep i two steps behind
+// sweep i-1: step s of sweep i touches rows up to i + (s+1) bw, step s+2 of
+// sweep i-1 rows from i + (s+2) bw on (their row ranges meet only at index
+// i + (s+1) bw, a column of the one and the last diagonal row of the other,
+// no common entry), and every pair of tasks that does share entries keeps
+// its sequential order; one wavefront per active sweep and one workgroup
+// barrier per step, the band in LDS.  Steps 2 and 3 are those
 // of the dense path; the vectors are then mapped back through the stored
 // reflectors, sweep by sweep in reverse (one sweep's reflectors act on
 // disjoint rows and are applied together).  O(m^2 bw) work instead of the
@@ -687,7 +690,7 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
         }
     }
 
-    // ---- bulge chasing: time step ts runs step ts - 3i of every active sweep i ----
+    // ---- bulge chasing: time step ts runs step ts - 2i of every active sweep i ----
     const int nsw = m - 2;  // sweeps 0 .. m-3
     for (int i = tid; i < nsw; i += NT) {
         sw_nst[i] = band_nsteps(m, bw, i);
@@ -695,18 +698,19 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
     }
     __syncthreads();
     if (nsw > 0) {
-        const int tend = 3 * (nsw - 1) + sw_nst[nsw - 1];
-        // this wavefront's sweeps are i == wid (mod NW); ihi = min(nsw - 1, ts / 3)
-        // is tracked incrementally, and older sweeps finish first
+        const int tend = 2 * (nsw - 1) + sw_nst[nsw - 1];
+        // this wavefront's sweeps are i == wid (mod NW); ihi = min(nsw - 1, ts / 2)
+        // is tracked incrementally, and older sweeps finish first; at most
+        // max steps / 2 + 1 <= 49 sweeps are active (< NSLOT)
         int ihi = 0, ph = 0;
         for (int ts = 0; ts < tend; ts++) {
             for (int i = ihi - ((ihi - wid) % NW + NW) % NW; i >= 0; i -= NW) {
-                const int st = ts - 3 * i;
+                const int st = ts - 2 * i;
                 if (st >= sw_nst[i]) break;
                 band_task(m, bw, i, st, AB, slots[i % NSLOT], work + (int64_t)(sw_off[i] + st) * (bw + 1));
             }
             __syncthreads();
-            if (++ph == 3) {
+            if (++ph == 2) {
                 ph = 0;
                 ihi = min(nsw - 1, ihi + 1);
             }

@@ -7,8 +7,8 @@ What the mathematics fixes, on random banded symmetric matrices:
 the reduced matrix is exactly tridiagonal, its eigenvalues equal numpy's
 eigenvalues of the original (an orthogonal similarity), and the
 back-transformed tridiagonal eigenvectors are orthonormal eigenvectors of the
-original matrix.  The schedule's same-step tasks touch disjoint index
-ranges; a planted mistake in it (sweeps one step apart, run concurrently)
+original matrix.  The schedule's same-step tasks touch disjoint band
+entries; a planted mistake in it (sweeps one step apart, run concurrently)
 must break the similarity.
 """
 import numpy as np
@@ -40,7 +40,21 @@ def task_rows(m, bw, i, s):
     return lo, min(i + (s + 1) * bw, m - 1)
 
 
-def band_reduce(T, bw, stagger=3, clashes=None):
+def task_entries(m, bw, i, s):
+    """Band entries (r, c), r >= c, that task (i, s) reads or writes: column i
+    below the diagonal (s == 0) or the block below the previous reflector's
+    rows, and the lower triangle of the diagonal block."""
+    r0 = i + 1 + s * bw
+    L = min(bw, m - r0)
+    if s == 0:
+        cols = [(r0 + p, i) for p in range(L)]
+    else:
+        cs = i + 1 + (s - 1) * bw
+        cols = [(r0 + p, cs + q) for p in range(L) for q in range(min(bw, m - cs))]
+    return set(cols) | {(r0 + p, r0 + q) for p in range(L) for q in range(p + 1)}
+
+
+def band_reduce(T, bw, stagger=2, clashes=None, entry_clashes=None):
     """The kernel's schedule.  Tasks of one time step run concurrently on the
     device; here they run in decreasing sweep order (the reverse of the
     sequential order), so a schedule that lets them interact gives a wrong
@@ -57,6 +71,10 @@ def band_reduce(T, bw, stagger=3, clashes=None):
             rng = [task_rows(m, bw, i, ts - stagger * i) for i in active]
             clashes += [(ts, a, b) for a in range(len(rng)) for b in range(a + 1, len(rng))
                         if rng[a][0] <= rng[b][1] and rng[b][0] <= rng[a][1]]
+        if entry_clashes is not None:
+            ent = [task_entries(m, bw, i, ts - stagger * i) for i in active]
+            entry_clashes += [(ts, a, b) for a in range(len(ent)) for b in range(a + 1, len(ent))
+                              if ent[a] & ent[b]]
         for i in reversed(active):
             s = ts - stagger * i
             if s == 0:
@@ -103,8 +121,8 @@ def banded(m, bw, seed):
 def test_band_reduction_is_an_orthogonal_similarity(m, bw):
     T = banded(m, bw, m * 31 + bw)
     clashes = []
-    d, e, refl, A = band_reduce(T, bw, clashes=clashes)
-    assert clashes == []                                            # same-step tasks are disjoint
+    d, e, refl, A = band_reduce(T, bw, entry_clashes=clashes)
+    assert clashes == []                                            # same-step tasks touch disjoint entries
     assert np.abs(np.tril(A, -2)).max(initial=0.0) == 0.0          # exactly tridiagonal
     Tt = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
     w_ref = np.linalg.eigvalsh(T)
@@ -121,19 +139,23 @@ def test_band_reduction_is_an_orthogonal_similarity(m, bw):
 
 
 def test_overlapping_schedule_is_detected():
-    """Sweeps one step apart share rows within a time step: the overlap is
-    reported and, run in the reverse order, the result is no longer a
-    tridiagonal similarity.  Two steps apart the ranges still touch (one shared
-    index), which the disjointness check also reports."""
+    """Sweeps one step apart share band entries within a time step: the
+    overlap is reported and, run in the reverse order, the result is no longer
+    a tridiagonal similarity.  Two steps apart (the kernel's schedule) the
+    row ranges still touch at one index, but the entries are disjoint (the
+    shared index is a column of one task and the diagonal of the other), and
+    the result is bitwise that of three steps apart."""
     m, bw = 64, 8
     T = banded(m, bw, 7)
-    clashes = []
-    d, e, refl, A = band_reduce(T, bw, stagger=1, clashes=clashes)
-    assert clashes
+    ent = []
+    d, e, refl, A = band_reduce(T, bw, stagger=1, entry_clashes=ent)
+    assert ent
     Tt = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
     off = np.abs(np.tril(A, -2)).max()
     err = np.abs(np.linalg.eigvalsh(Tt) - np.linalg.eigvalsh(T)).max()
     assert off > 1e-8 or err > 1e-8
-    clashes2 = []
-    band_reduce(T, bw, stagger=2, clashes=clashes2)
-    assert clashes2
+    rng, ent2 = [], []
+    d2, e2, _, _ = band_reduce(T, bw, stagger=2, clashes=rng, entry_clashes=ent2)
+    assert rng and not ent2
+    d3, e3, _, _ = band_reduce(T, bw, stagger=3)
+    assert np.array_equal(d2, d3) and np.array_equal(e2, e3)
