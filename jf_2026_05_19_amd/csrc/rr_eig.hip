@@ -198,15 +198,17 @@ __device__ void tridiag_top_pairs(int m, int k, const TriLds &L, double *theta, 
 
     // ---- 3. inverse iteration, one lane per cluster, LU and vector in LDS ----
     const int nc = sint[1];
-    const int CB = scr_n / (6 * m);  // clusters per batch (>= 2 for m <= 1024)
-    // slot s of a batch runs on lane s / NW of wavefront s % NW: the chains of
-    // a batch sit in different wavefronts first, so their pivoting branches
-    // do not serialise each other
-    const int slot = lane * NW + wid;
+    // clusters per batch (>= 2 for m <= 1024); a chain's six arrays take
+    // 6 m + 1 doubles, so the chains of a batch start on different LDS banks
+    const int CB = scr_n / (6 * m + 1);
+    // slot s of a batch runs on lane s of wavefront 0: the LU and the solves
+    // are branch-free, so one instruction advances every chain of the batch
+    // (one lane per wavefront would pay a whole wavefront's issue per chain)
+    const int slot = tid;
     for (int c0 = 0; c0 < nc; c0 += CB) {
         const int cid = c0 + slot;
         if (slot < CB && cid < nc) {
-            double *D = scr + (size_t)slot * 6 * m;
+            double *D = scr + (size_t)slot * (6 * m + 1);
             double *DL = D + m, *DU = DL + m, *DU2 = DU + m, *PV = DU2 + m, *z = PV + m;
             const double tiny = EPS * tnorm + pivmin;
             double xprev = 0.0;
@@ -503,6 +505,12 @@ constexpr int LDB = 2 * SYMEIG_BAND_MAX_W + 1;                 // band column: d
 constexpr int BAND_LDS = SYMEIG_BAND_MAX_M * LDB > SCR ? SYMEIG_BAND_MAX_M * LDB : SCR;
 constexpr int NSLOT = 64;                                      // previous-reflector slots (>= active sweeps)
 
+// Workgroup barrier that orders LDS only: this wavefront's LDS operations
+// complete, then s_barrier.  Global loads and stores in flight stay in flight
+// (__syncthreads() starts with a release fence that waits for them); used
+// where the barrier guards LDS data alone.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 __device__ __forceinline__ int band_nsteps(int m, int bw, int i) { return (m - 1 - i + bw - 1) / bw; }
 // sum_{j=1}^{n} ceil(j / bw)
 __device__ __forceinline__ int64_t band_F(int n, int bw) {
@@ -709,7 +717,9 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
                 if (st >= sw_nst[i]) break;
                 band_task(m, bw, i, st, AB, slots[i % NSLOT], work + (int64_t)(sw_off[i] + st) * (bw + 1));
             }
-            __syncthreads();
+            // the band and the slots are in LDS; the record stores (read only
+            // after the full barriers below) need not complete here
+            lds_barrier();
             if (++ph == 2) {
                 ph = 0;
                 ihi = min(nsw - 1, ihi + 1);
@@ -730,40 +740,69 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
     __syncthreads();
 
     // ---- 4. back-transformation Zs <- Q Zs, Q the product of the sweeps' reflectors ----
-    const int cbat = BAND_LDS / m;
+    // task (st, c) applies step st of the current sweep to vector c.  When a
+    // sweep's tasks fit the workgroup (always for bw = 16), thread tid keeps
+    // task (tid / nv, tid % nv) and loads the records of the next sweep
+    // (i - 1) while this one is applied; otherwise the tasks are strided.
+    // vectors in LDS with column stride m + 1 (odd), so that the tasks of a
+    // sweep, which differ in the vector, hit different banks
+    const int ldz = m + 1, cbat = BAND_LDS / ldz;
+    auto apply = [&](int i, int st, int c, const double *vr, double tau, double *Z) {
+        const int r0 = i + 1 + st * bw, L = min(bw, m - r0);
+        double *z = Z + (size_t)c * ldz + r0;
+        double dot = 0.0;
+#pragma unroll
+        for (int q = 0; q < SYMEIG_BAND_MAX_W; q++) dot += ((q < L) ? vr[q] : 0.0) * z[min(q, L - 1)];
+        dot *= tau;
+        // z is read again from LDS rather than kept in registers
+#pragma unroll
+        for (int q = 0; q < SYMEIG_BAND_MAX_W; q++)
+            if (q < L) z[q] -= dot * vr[q];
+    };
     for (int c0 = 0; c0 < k; c0 += cbat) {
         const int nv = min(cbat, k - c0);
-        double *Z = AB;  // nv columns of length m
-        for (int t = tid; t < nv * m; t += NT) Z[t] = Zs[(int64_t)c0 * m + t];
+        double *Z = AB;  // nv columns of length m, stride ldz
+        for (int t = tid; t < nv * m; t += NT) Z[(t / m) * ldz + t % m] = Zs[(int64_t)c0 * m + t];
         __syncthreads();
-        for (int i = nsw - 1; i >= 0; i--) {
-            const int nst = sw_nst[i];
-            const double *recs = work + (int64_t)sw_off[i] * (bw + 1);
-            for (int t = tid; t < nst * nv; t += NT) {
-                const int st = t / nv, c = t % nv;
-                const int r0 = i + 1 + st * bw, L = min(bw, m - r0);
-                const double *rv = recs + st * (bw + 1);
-                // all record loads issued together (fixed width, clamped, masked)
-                double v[SYMEIG_BAND_MAX_W];
+        if (nsw > 0 && sw_nst[0] * nv <= NT) {
+            const int st = tid / nv, c = tid % nv;
+            const bool act = tid < sw_nst[0] * nv;  // threads without a task load nothing
+            double v[SYMEIG_BAND_MAX_W], tau = 0.0;
+#pragma unroll
+            for (int q = 0; q < SYMEIG_BAND_MAX_W; q++) v[q] = 0.0;
+            // fixed-width record loads, clamped into sweep i's records
+            auto load_rec = [&](int i) {
+                const double *rv = work + ((int64_t)sw_off[i] + min(st, sw_nst[i] - 1)) * (bw + 1);
 #pragma unroll
                 for (int q = 0; q < SYMEIG_BAND_MAX_W; q++) v[q] = rv[min(q, bw)];
-                const double tau = rv[bw];
-                double *z = Z + (size_t)c * m + r0;
-                double zz[SYMEIG_BAND_MAX_W];
-                double dot = 0.0;
+                tau = rv[bw];
+            };
+            if (act) load_rec(nsw - 1);
+            for (int i = nsw - 1; i >= 0; i--) {
+                double vc[SYMEIG_BAND_MAX_W];
 #pragma unroll
-                for (int q = 0; q < SYMEIG_BAND_MAX_W; q++) {
-                    zz[q] = (q < L) ? z[min(q, L - 1)] : 0.0;
-                    dot += ((q < L) ? v[q] : 0.0) * zz[q];
-                }
-                dot *= tau;
-#pragma unroll
-                for (int q = 0; q < SYMEIG_BAND_MAX_W; q++)
-                    if (q < L) z[q] = zz[q] - dot * v[q];
+                for (int q = 0; q < SYMEIG_BAND_MAX_W; q++) vc[q] = v[q];
+                const double tc = tau;
+                if (act && i > 0) load_rec(i - 1);
+                if (st < sw_nst[i]) apply(i, st, c, vc, tc, Z);
+                lds_barrier();  // Z is in LDS; the next records stay in flight
             }
-            __syncthreads();
+        } else {
+            for (int i = nsw - 1; i >= 0; i--) {
+                const int nst = sw_nst[i];
+                const double *recs = work + (int64_t)sw_off[i] * (bw + 1);
+                for (int t = tid; t < nst * nv; t += NT) {
+                    const int st = t / nv, c = t % nv;
+                    const double *rv = recs + st * (bw + 1);
+                    double v[SYMEIG_BAND_MAX_W];
+#pragma unroll
+                    for (int q = 0; q < SYMEIG_BAND_MAX_W; q++) v[q] = rv[min(q, bw)];
+                    apply(i, st, c, v, rv[bw], Z);
+                }
+                __syncthreads();
+            }
         }
-        for (int t = tid; t < nv * m; t += NT) Zs[(int64_t)c0 * m + t] = Z[t];
+        for (int t = tid; t < nv * m; t += NT) Zs[(int64_t)c0 * m + t] = Z[(t / m) * ldz + t % m];
         __syncthreads();
     }
     if (tid == 0) *info = sint[0];
