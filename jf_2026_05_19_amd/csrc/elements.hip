@@ -102,25 +102,45 @@ __device__ __forceinline__ void jacobian(const double *lc, const double *dNr, co
 
 // 4x4 inverse by Gauss-Jordan with partial pivoting (single lane).
 __device__ void inv4(const double *a, double *ai) {
+    // Gauss-Jordan with partial pivoting; every index is a compile-time
+    // constant after unrolling (the row swap is a select over the candidate
+    // rows), so the working matrix stays in registers
     double m[4][8];
+#pragma unroll
     for (int i = 0; i < 4; i++)
+#pragma unroll
         for (int j = 0; j < 4; j++) { m[i][j] = a[i * 4 + j]; m[i][4 + j] = (i == j) ? 1.0 : 0.0; }
+#pragma unroll
     for (int c = 0; c < 4; c++) {
         int p = c;
         double best = fabs(m[c][c]);
+#pragma unroll
         for (int r = c + 1; r < 4; r++)
             if (fabs(m[r][c]) > best) { best = fabs(m[r][c]); p = r; }
-        if (p != c)
-            for (int j = 0; j < 8; j++) { double t = m[c][j]; m[c][j] = m[p][j]; m[p][j] = t; }
+#pragma unroll
+        for (int r = c + 1; r < 4; r++) {
+            const bool sw = (r == p);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const double t = m[c][j];
+                m[c][j] = sw ? m[r][j] : t;
+                m[r][j] = sw ? t : m[r][j];
+            }
+        }
         double d = m[c][c];
+#pragma unroll
         for (int j = 0; j < 8; j++) m[c][j] /= d;
+#pragma unroll
         for (int r = 0; r < 4; r++) {
             if (r == c) continue;
             double f = m[r][c];
+#pragma unroll
             for (int j = 0; j < 8; j++) m[r][j] -= f * m[c][j];
         }
     }
+#pragma unroll
     for (int i = 0; i < 4; i++)
+#pragma unroll
         for (int j = 0; j < 4; j++) ai[i * 4 + j] = m[i][4 + j];
 }
 
@@ -223,8 +243,8 @@ __device__ void shell_props(const double *prop, double k6rot, ShellProps &P) {
 template <class ST>
 __device__ void q4_stage_geometry(ST &S, int lane) {
     if (lane < 4) {
-        const double gps[4][2] = {{-PT, -PT}, {PT, -PT}, {PT, PT}, {-PT, PT}};
-        double r = gps[lane][0], s = gps[lane][1];
+        // Gauss points (-,-), (+,-), (+,+), (-,+)
+        const double r = (lane == 1 || lane == 2) ? PT : -PT, s = (lane >= 2) ? PT : -PT;
         double dNr[4], dNs[4], J11, J12, J21, J22;
         shape_derivs(r, s, dNr, dNs);
         jacobian(S.lc, dNr, dNs, J11, J12, J21, J22);
@@ -233,6 +253,7 @@ __device__ void q4_stage_geometry(ST &S, int lane) {
         if (adj < 1e-12) adj = 1e-12;
         double id = 1.0 / detJ;
         double iJ11 = J22 * id, iJ12 = -J12 * id, iJ21 = -J21 * id, iJ22 = J11 * id;
+#pragma unroll
         for (int k = 0; k < 4; k++) {
             S.gdx[lane][k] = iJ11 * dNr[k] + iJ12 * dNs[k];
             S.gdy[lane][k] = iJ21 * dNr[k] + iJ22 * dNs[k];
@@ -255,8 +276,8 @@ __device__ void q4_stage_geometry(ST &S, int lane) {
         }
     } else if (lane >= 8 && lane < 12) {
         int p = lane - 8;
-        const double sp[4][2] = {{0.0, -PT}, {0.0, PT}, {-PT, 0.0}, {PT, 0.0}};
-        double xi = sp[p][0], eta = sp[p][1];
+        // sample points (0,-), (0,+), (-,0), (+,0)
+        const double xi = p < 2 ? 0.0 : (p == 2 ? -PT : PT), eta = p == 0 ? -PT : (p == 1 ? PT : 0.0);
         double dNr[4], dNs[4], J11, J12, J21, J22;
         shape_derivs(xi, eta, dNr, dNs);
         jacobian(S.lc, dNr, dNs, J11, J12, J21, J22);
@@ -464,7 +485,11 @@ __device__ bool q4_setup(Q4Stage &S, ShellProps &P, const ElemArgs &A, int e, in
 __device__ void q4_local_block(const Q4Stage &S, const ShellProps &P, int lane, bool bend_c, bool has_shear,
                                double *blk) {
     int bi = lane / 8, bj = lane % 8;
+    // kept rolled: unrolled, the nine entries' Gauss-point sums need more
+    // registers than a lane has (spills, one wavefront per SIMD)
+#pragma unroll 1
     for (int i = 0; i < 3; i++)
+#pragma unroll 1
         for (int j = 0; j < 3; j++) {
             int a = 3 * bi + i, b = 3 * bj + j;
             double km, kb, kd, cm, cb;
@@ -530,8 +555,7 @@ struct Q4KGStage {
 };
 
 __device__ void q4kg_gp_stage(Q4KGStage &S, int lane) {
-    const double gps[4][2] = {{-PT, -PT}, {PT, -PT}, {PT, PT}, {-PT, PT}};
-    double r = gps[lane][0], s = gps[lane][1];
+    const double r = (lane == 1 || lane == 2) ? PT : -PT, s = (lane >= 2) ? PT : -PT;
     double dNr[4], dNs[4], J11, J12, J21, J22;
     shape_derivs(r, s, dNr, dNs);
     jacobian(S.lc, dNr, dNs, J11, J12, J21, J22);
