@@ -599,16 +599,18 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     double *G1 = small.p, *G2 = small.p + 256, *Rt = small.p + 512, *G0 = small.p + 768, *Hs = small.p + 1024;
     auto blk = [&](int jj) { return V.p + (size_t)jj * b * n; };
     auto Tsub = [&](int jj) { return T.p + (size_t)(jj + 1) * b + (size_t)jj * b * ldT; };  // R of block jj+1
-    // CholQR2 of Q (n x b); with g0 the first Cholesky checks for dependent directions
-    auto cholqr2 = [&](double *Q, double *Rout, int ldr, const double *g0, int bidx) {
-        ts_gemm_tn(n, b, b, Q, n, Q, n, G1, 16, 1.0, 0.0, part.p, s);
+    // CholQR2 of Q (n x b); with g0 the first Cholesky checks for dependent
+    // directions.  g1_ready: G1 = Q^T Q was formed by the kernel that wrote Q.
+    // The first triangular solve forms the second pass's Gram matrix, the
+    // second one R = R2 R1.
+    auto cholqr2 = [&](double *Q, double *Rout, int ldr, const double *g0, int bidx, bool g1_ready) {
+        if (!g1_ready) ts_gemm_tn(n, b, b, Q, n, Q, n, G1, 16, 1.0, 0.0, part.p, s);
         if (g0) small_chol_upper_checked(b, G1, 16, g0, 16, dep_tol2, bidx, ierr.p, s);
         else small_chol_upper(b, G1, 16, ierr.p, s);
-        trsm_right_upper(n, b, Q, n, G1, 16, s);
-        ts_gemm_tn(n, b, b, Q, n, Q, n, G2, 16, 1.0, 0.0, part.p, s);
+        trsm_right_upper_gram(n, b, Q, n, G1, 16, part.p, G2, 16, s);
         small_chol_upper(b, G2, 16, ierr.p, s);
-        trsm_right_upper(n, b, Q, n, G2, 16, s);
-        if (Rout) upper_mul(b, G2, 16, G1, 16, Rout, ldr, s);
+        if (Rout) trsm_right_upper_mul(n, b, Q, n, G2, 16, G1, 16, Rout, ldr, s);
+        else trsm_right_upper(n, b, Q, n, G2, 16, s);
     };
     // dst = C Q_jj, orthogonalised (CGS2) against V[:, 0:(jj+1)b]; coefficients
     // into T's column block jj; G0 = Gram matrix after the first pass.  The
@@ -624,12 +626,13 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
                          ctx->timer()));
         int kcols = (jj + 1) * b;
         double *Tcol = T.p + (size_t)jj * b * ldT;
+        // first pass: coefficients into T, update; G0 = Gram matrix after it
         ts_gemm_tn(n, kcols, b, V.p, n, dst, n, Tcol, ldT, 1.0, 0.0, part.p, s);
-        ts_gemm_nn(n, kcols, b, V.p, n, Tcol, ldT, dst, n, -1.0, 1.0, s);
-        ts_gemm_tn(n, b, b, dst, n, dst, n, G0, 16, 1.0, 0.0, part.p, s);
-        ts_gemm_tn(n, kcols, b, V.p, n, dst, n, H2.p, kcols, 1.0, 0.0, part.p, s);
-        ts_gemm_nn(n, kcols, b, V.p, n, H2.p, kcols, dst, n, -1.0, 1.0, s);
-        mat_add(kcols, b, H2.p, kcols, Tcol, ldT, s);
+        ts_gemm_nn_gram(n, kcols, b, V.p, n, Tcol, ldT, dst, n, -1.0, 1.0, part.p, G0, 16, s);
+        // second pass: its coefficients are also added into T; G1 = Gram
+        // matrix of the result (the first Cholesky-QR pass of the new block)
+        ts_gemm_tn_add(n, kcols, b, V.p, n, dst, n, H2.p, kcols, Tcol, ldT, part.p, s);
+        ts_gemm_nn_gram(n, kcols, b, V.p, n, H2.p, kcols, dst, n, -1.0, 1.0, part.p, G1, 16, s);
         return SOL105_OK;
     };
     // the new block is built in place: column block jj+1 of V is not part of
@@ -637,7 +640,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     auto fast_step = [&](int jj) {
         int rc = expand(jj, blk(jj + 1));
         if (rc) return rc;
-        cholqr2(blk(jj + 1), Tsub(jj), ldT, G0, jj + 1);
+        cholqr2(blk(jj + 1), Tsub(jj), ldT, G0, jj + 1, true);
         return SOL105_OK;
     };
     // rank-revealing rebuild of block jj+1 (host-assisted; rare)
@@ -694,7 +697,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
         return SOL105_OK;
     };
     start_block(n, b, V.p, n, 0, s);
-    cholqr2(V.p, nullptr, 0, nullptr, 0);
+    cholqr2(V.p, nullptr, 0, nullptr, 0, false);
     int j = 0;
     // first Rayleigh-Ritz check: where the last solve of this model converged
     // (design loops and repeated solves), else at about 3 vectors per wanted pair

@@ -8,10 +8,18 @@
 void ts_gemm_tn(int n, int P, int Q, const double *V, int ldv, const double *W, int ldw, double *out, int ldo,
                 double alpha, double beta, double *part, hipStream_t s);
 int64_t ts_gemm_tn_part_size(int n, int P, int Q);
+// out = V^T W and out2 += V^T W (Q <= 16)
+void ts_gemm_tn_add(int n, int P, int Q, const double *V, int ldv, const double *W, int ldw, double *out, int ldo,
+                    double *out2, int ldo2, double *part, hipStream_t s);
 // Y = alpha * V H + beta * Y   (Q <= 16)
 void ts_gemm_nn(int n, int P, int Q, const double *V, int ldv, const double *H, int ldh, double *Y, int ldy,
                 double alpha, double beta, hipStream_t s);
+// as ts_gemm_nn, and G = Y^T Y (Q x Q, ld ldg) of the result; part holds
+// ts_gemm_tn_part_size(n, 16, 16) doubles
+void ts_gemm_nn_gram(int n, int P, int Q, const double *V, int ldv, const double *H, int ldh, double *Y, int ldy,
+                     double alpha, double beta, double *part, double *G, int ldg, hipStream_t s);
 void small_chol_upper(int b, double *G, int ldg, int *err, hipStream_t s);
+
 // as small_chol_upper, but a pivot <= tol2 * max diag(g0) flags block blk as
 // rank-deficient (err[0] |= 4, err[1] = min(blk + 1))
 void small_chol_upper_checked(int b, double *G, int ldg, const double *g0, int ldg0, double tol2, int blk, int *err,
@@ -19,6 +27,12 @@ void small_chol_upper_checked(int b, double *G, int ldg, const double *g0, int l
 // V[:, 0:cols] = seeded uniform values in [-1, 1)
 void hash_fill(int n, int cols, double *V, int ldv, uint64_t seed, hipStream_t s);
 void trsm_right_upper(int n, int b, double *W, int ldw, const double *R, int ldr, hipStream_t s);
+// as trsm_right_upper, and Rout = R R1 (R1 upper b x b)
+void trsm_right_upper_mul(int n, int b, double *W, int ldw, const double *R, int ldr, const double *R1, int ld1,
+                          double *Rout, int ldo, hipStream_t s);
+// as trsm_right_upper, and G = W^T W (b x b, ld ldg) of the result
+void trsm_right_upper_gram(int n, int b, double *W, int ldw, const double *R, int ldr, double *part, double *G,
+                           int ldg, hipStream_t s);
 void upper_mul(int b, const double *R2, int ld2, const double *R1, int ld1, double *out, int ldo, hipStream_t s);
 void start_block(int n, int b, double *V, int ldv, int seed, hipStream_t s);
 // (max, sum) over the shells (CQUAD4, CTRIA3) of the mean |translation|^2

@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256) tn_small_kernel(int n, int P, int Q, cons
 // out[p + q*ldo] = alpha * sum_c part[c][p][q] + beta * out[...]; sixteen lanes
 // per output sum interleaved chunk subsets, added in a fixed order
 __device__ __forceinline__ void tn_reduce_body(int nch, int P, int Q, const double *part, double *out, int ldo,
-                                               double alpha, double beta) {
+                                               double alpha, double beta, double *out2 = nullptr, int ldo2 = 0) {
     __shared__ double red[16][17];
     const int ol = threadIdx.x % 16, sg = threadIdx.x / 16;
     const int o = blockIdx.x * 16 + ol;
@@ -141,12 +141,35 @@ __device__ __forceinline__ void tn_reduce_body(int nch, int P, int Q, const doub
         const int p = o / Q, q = o % Q;
         double *y = out + p + (int64_t)q * ldo;
         *y = alpha * tot + (beta == 0.0 ? 0.0 : beta * *y);
+        if (out2) out2[p + (int64_t)q * ldo2] += alpha * tot;
     }
 }
 
 __global__ void __launch_bounds__(256) tn_reduce_kernel(int nch, int P, int Q, const double *part, double *out,
-                                                        int ldo, double alpha, double beta) {
-    tn_reduce_body(nch, P, Q, part, out, ldo, alpha, beta);
+                                                        int ldo, double alpha, double beta, double *out2, int ldo2) {
+    tn_reduce_body(nch, P, Q, part, out, ldo, alpha, beta, out2, ldo2);
+}
+
+// Gram partial of a 256-row chunk from the MFMA operand layout: wavefront w
+// covers rows [64 w, 64 w + 64) and lane l holds y[b][v] = Y[row][l%16] for
+// row = 64 w + 16 b + 4 v + l/16 (zero outside the chunk or past Q columns),
+// which is at once the A (Y^T) and the B (Y) operand of the four-row step
+// (16 b + 4 v): G += Y^T Y on the matrix cores without moving a value.  The
+// four wavefronts' 16 x 16 sums are added in LDS in wavefront order.
+__device__ __forceinline__ void gram_partial(const double (&y)[4][4], int Q, double *gpart_c) {
+    __shared__ double gred[4][16][17];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, kq = lane >> 4, col = lane & 15;
+    dbl4 g = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) g = __builtin_amdgcn_mfma_f64_16x16x4f64(y[b][v], y[b][v], g, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; v++) gred[wv][kq + 4 * v][col] = g[v];
+    __syncthreads();
+    const int t = threadIdx.x, pl = t % 16, ql = t / 16;
+    if (pl < Q && ql < Q)
+        gpart_c[(int64_t)pl * Q + ql] = gred[0][pl][ql] + gred[1][pl][ql] + gred[2][pl][ql] + gred[3][pl][ql];
 }
 
 // Y[r0:r1, q] = beta*Y + alpha * sum_p V[row, p] H[p, q]   (Q <= 16,
@@ -159,7 +182,8 @@ __global__ void __launch_bounds__(256) tn_reduce_kernel(int nch, int P, int Q, c
 // loads are clamped into [r0, r1) x [0, P): the extra rows only reach output
 // rows that are not written, the extra columns meet a zero B value.
 __device__ __forceinline__ void nn_rows(int r0, int r1, int P, int Q, const double *V, int ldv, const double *H,
-                                        int ldh, double *Y, int ldy, double alpha, double beta) {
+                                        int ldh, double *Y, int ldy, double alpha, double beta,
+                                        double *gpart_c = nullptr) {
     __shared__ double Hs[256][16];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, kq = lane >> 4, col = lane & 15;
     const int rw = r0 + 64 * wv;
@@ -198,6 +222,11 @@ __device__ __forceinline__ void nn_rows(int r0, int r1, int P, int Q, const doub
             __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
         }
     }
+    double yv[4][4];
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) yv[b][v] = 0.0;
     if (col < Q && rw < r1)
 #pragma unroll
         for (int b = 0; b < 4; b++)
@@ -206,15 +235,19 @@ __device__ __forceinline__ void nn_rows(int r0, int r1, int P, int Q, const doub
                 const int r = rw + 16 * b + kq + 4 * v;
                 if (r < r1) {
                     double *y = Y + r + (int64_t)col * ldy;
-                    *y = alpha * acc[b][v] + (beta == 0.0 ? 0.0 : beta * *y);
+                    yv[b][v] = alpha * acc[b][v] + (beta == 0.0 ? 0.0 : beta * *y);
+                    *y = yv[b][v];
                 }
             }
+    if (gpart_c) gram_partial(yv, Q, gpart_c);
 }
 
 __global__ void __launch_bounds__(256) nn_kernel(int n, int P, int Q, const double *V, int ldv, const double *H,
-                                                 int ldh, double *Y, int ldy, double alpha, double beta) {
+                                                 int ldh, double *Y, int ldy, double alpha, double beta,
+                                                 double *gpart) {
     const int r0 = blockIdx.x * 256;
-    nn_rows(r0, min(n, r0 + 256), P, Q, V, ldv, H, ldh, Y, ldy, alpha, beta);
+    nn_rows(r0, min(n, r0 + 256), P, Q, V, ldv, H, ldh, Y, ldy, alpha, beta,
+            gpart ? gpart + (int64_t)blockIdx.x * Q * Q : nullptr);
 }
 
 // Cholesky G = R^T R (upper R, b <= 16) in place, one wavefront: step j
@@ -293,14 +326,23 @@ __global__ void __launch_bounds__(64) chol_upper_kernel(int b, double *G, int ld
     chol_upper_body(b, G, ldg, g0, ldg0, tol2, blk, err);
 }
 
-// W = W R^-1 (R upper b x b), one thread per row
-__device__ __forceinline__ void trsm_rows(int row, bool live, int b, double *W, int ldw, const double *R, int ldr) {
+// W = W R^-1 (R upper b x b), one thread per row.  R1 non-null: the first
+// workgroup also forms Rout = R R1 (R1 upper), from R in LDS
+__device__ __forceinline__ void trsm_rows(int row, bool live, int b, double *W, int ldw, const double *R, int ldr,
+                                          const double *R1 = nullptr, int ld1 = 0, double *Rout = nullptr,
+                                          int ldo = 0) {
     __shared__ double Rs[16][16];
     for (int t = threadIdx.x; t < 256; t += blockDim.x) {
         int i = t % 16, j = t / 16;
         Rs[i][j] = (i < b && j < b) ? R[i + j * ldr] : 0.0;
     }
     __syncthreads();
+    if (R1 && blockIdx.x == 0 && threadIdx.x < b * b) {
+        const int i = threadIdx.x % b, j = threadIdx.x / b;
+        double sacc = 0.0;
+        for (int k = 0; k < b; k++) sacc += Rs[i][k] * R1[k + j * ld1];
+        Rout[i + j * ldo] = sacc;
+    }
     if (!live) return;
     double x[16];
 #pragma unroll
@@ -319,9 +361,51 @@ __device__ __forceinline__ void trsm_rows(int row, bool live, int b, double *W, 
         if (j < b) W[row + (int64_t)j * ldw] = x[j];
 }
 
-__global__ void trsm_right_upper_kernel(int n, int b, double *W, int ldw, const double *R, int ldr) {
+__global__ void trsm_right_upper_kernel(int n, int b, double *W, int ldw, const double *R, int ldr, const double *R1,
+                                        int ld1, double *Rout, int ldo) {
     int row = blockIdx.x * blockDim.x + threadIdx.x;
-    trsm_rows(row, row < n, b, W, ldw, R, ldr);
+    trsm_rows(row, row < n, b, W, ldw, R, ldr, R1, ld1, Rout, ldo);
+}
+
+// The same for a 256-row chunk per workgroup, then the chunk's Gram partial
+// of the new rows (their next Cholesky-QR pass) on the matrix cores, read
+// back from LDS in the operand layout
+__global__ void __launch_bounds__(256) trsm_gram_kernel(int n, int b, double *W, int ldw, const double *R, int ldr,
+                                                        double *gpart) {
+    __shared__ double Rs[16][16];
+    __shared__ double Ws[256][17];
+    const int t = threadIdx.x, row = blockIdx.x * 256 + t;
+    for (int e = t; e < 256; e += blockDim.x) {
+        const int i = e % 16, j = e / 16;
+        Rs[i][j] = (i < b && j < b) ? R[i + j * ldr] : 0.0;
+    }
+    __syncthreads();
+    double x[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) x[j] = (j < b && row < n) ? W[row + (int64_t)j * ldw] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        if (j < b) {
+            double sacc = x[j];
+#pragma unroll
+            for (int k = 0; k < j; k++) sacc -= x[k] * Rs[k][j];
+            x[j] = sacc / Rs[j][j];
+        }
+    }
+    if (row < n)
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if (j < b) W[row + (int64_t)j * ldw] = x[j];
+#pragma unroll
+    for (int j = 0; j < 16; j++) Ws[t][j] = (j < b && row < n) ? x[j] : 0.0;
+    __syncthreads();
+    const int lane = t & 63, wv = t >> 6, kq = lane >> 4, col = lane & 15;
+    double y[4][4];
+#pragma unroll
+    for (int bb = 0; bb < 4; bb++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) y[bb][v] = Ws[64 * wv + 16 * bb + 4 * v + kq][col];
+    gram_partial(y, b, gpart + (int64_t)blockIdx.x * b * b);
 }
 
 // R_prod = R2 * R1 (upper b x b): used for CholQR2 (R = R2 R1)
@@ -477,7 +561,21 @@ void ts_gemm_tn(int n, int P, int Q, const double *V, int ldv, const double *W, 
         nch = (n + CH - 1) / CH;
         tn_partial_kernel<<<dim3(nch, (P + PC - 1) / PC), 256, 0, s>>>(n, P, Q, V, ldv, W, ldw, part);
     }
-    tn_reduce_kernel<<<nblk((int64_t)P * Q, 16), 256, 0, s>>>(nch, P, Q, part, out, ldo, alpha, beta);
+    tn_reduce_kernel<<<nblk((int64_t)P * Q, 16), 256, 0, s>>>(nch, P, Q, part, out, ldo, alpha, beta, nullptr, 0);
+}
+
+void ts_gemm_tn_add(int n, int P, int Q, const double *V, int ldv, const double *W, int ldw, double *out, int ldo,
+                    double *out2, int ldo2, double *part, hipStream_t s) {
+    if (P <= 0 || Q <= 0) return;
+    if (P <= 16 && Q <= 16) {
+        const int nch = (n + CHS - 1) / CHS;
+        tn_small_kernel<<<nch, 256, 0, s>>>(n, P, Q, V, ldv, W, ldw, part);
+        tn_reduce_kernel<<<nblk((int64_t)P * Q, 16), 256, 0, s>>>(nch, P, Q, part, out, ldo, 1.0, 0.0, out2, ldo2);
+    } else {
+        const int nch = (n + CH - 1) / CH;
+        tn_partial_kernel<<<dim3(nch, (P + PC - 1) / PC), 256, 0, s>>>(n, P, Q, V, ldv, W, ldw, part);
+        tn_reduce_kernel<<<nblk((int64_t)P * Q, 16), 256, 0, s>>>(nch, P, Q, part, out, ldo, 1.0, 0.0, out2, ldo2);
+    }
 }
 
 int64_t ts_gemm_tn_part_size(int n, int P, int Q) {
@@ -486,12 +584,20 @@ int64_t ts_gemm_tn_part_size(int n, int P, int Q) {
 
 void ts_gemm_nn(int n, int P, int Q, const double *V, int ldv, const double *H, int ldh, double *Y, int ldy,
                 double alpha, double beta, hipStream_t s) {
-    nn_kernel<<<nblk(n, 256), 256, 0, s>>>(n, P, Q, V, ldv, H, ldh, Y, ldy, alpha, beta);
+    nn_kernel<<<nblk(n, 256), 256, 0, s>>>(n, P, Q, V, ldv, H, ldh, Y, ldy, alpha, beta, nullptr);
+}
+
+void ts_gemm_nn_gram(int n, int P, int Q, const double *V, int ldv, const double *H, int ldh, double *Y, int ldy,
+                     double alpha, double beta, double *part, double *G, int ldg, hipStream_t s) {
+    const unsigned nch = nblk(n, 256);
+    nn_kernel<<<nch, 256, 0, s>>>(n, P, Q, V, ldv, H, ldh, Y, ldy, alpha, beta, part);
+    tn_reduce_kernel<<<nblk((int64_t)Q * Q, 16), 256, 0, s>>>((int)nch, Q, Q, part, G, ldg, 1.0, 0.0, nullptr, 0);
 }
 
 void small_chol_upper(int b, double *G, int ldg, int *err, hipStream_t s) {
     chol_upper_kernel<<<1, 64, 0, s>>>(b, G, ldg, nullptr, 0, 0.0, 0, err);
 }
+
 
 void small_chol_upper_checked(int b, double *G, int ldg, const double *g0, int ldg0, double tol2, int blk, int *err,
                               hipStream_t s) {
@@ -503,7 +609,19 @@ void hash_fill(int n, int cols, double *V, int ldv, uint64_t seed, hipStream_t s
 }
 
 void trsm_right_upper(int n, int b, double *W, int ldw, const double *R, int ldr, hipStream_t s) {
-    trsm_right_upper_kernel<<<nblk(n, 256), 256, 0, s>>>(n, b, W, ldw, R, ldr);
+    trsm_right_upper_kernel<<<nblk(n, 256), 256, 0, s>>>(n, b, W, ldw, R, ldr, nullptr, 0, nullptr, 0);
+}
+
+void trsm_right_upper_mul(int n, int b, double *W, int ldw, const double *R, int ldr, const double *R1, int ld1,
+                          double *Rout, int ldo, hipStream_t s) {
+    trsm_right_upper_kernel<<<nblk(n, 256), 256, 0, s>>>(n, b, W, ldw, R, ldr, R1, ld1, Rout, ldo);
+}
+
+void trsm_right_upper_gram(int n, int b, double *W, int ldw, const double *R, int ldr, double *part, double *G,
+                           int ldg, hipStream_t s) {
+    const unsigned nch = nblk(n, 256);
+    trsm_gram_kernel<<<nch, 256, 0, s>>>(n, b, W, ldw, R, ldr, part);
+    tn_reduce_kernel<<<nblk((int64_t)b * b, 16), 256, 0, s>>>((int)nch, b, b, part, G, ldg, 1.0, 0.0, nullptr, 0);
 }
 
 void upper_mul(int b, const double *R2, int ld2, const double *R1, int ld1, double *out, int ldo, hipStream_t s) {
