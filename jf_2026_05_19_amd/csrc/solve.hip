@@ -363,23 +363,24 @@ __global__ void __launch_bounds__(256) prod1_kernel(DevFronts D, DevSolve DS, co
 // The same product for a block of right-hand sides (row stride MAXR) on the
 // matrix cores, v_mfma_f64_16x16x4f64 (D[16x16] += A[16x4] B[4x16]; lane l
 // supplies A[l%16][l/16] and B[l/16][l%16] and holds D[l/16 + 4v][l%16],
-// v = 0..3).  Each wavefront covers all 64 rows of the tile (four 16-row
-// blocks, four accumulators) over a quarter of the k-slice, so one B value
-// (the slice row k + l/16, column l%16, from LDS) feeds four products, and the
-// factor is read in the column-major front layout, 128 contiguous bytes per
-// block column.  A chunk of 8 four-k steps (32 factor loads per lane) is
+// v = 0..3).  Wavefront w owns rows [16 w, 16 w + 16) of the 64-row tile
+// over the whole k-slice, so no partial sums cross wavefronts: the factor is
+// read in the column-major front layout (128 contiguous bytes per block
+// column), the B values (slice row k + l/16, column l%16) come from the
+// slice in LDS, and a chunk of 16 four-k steps (16 factor loads per lane) is
 // issued before its products; the first chunk goes out before the vector
-// slice is assembled, so the slice's loads overlap it.  The four partial sums
-// meet once in LDS and are added in wavefront order (the result is
-// bit-reproducible); the write-back is then coalesced, row-fastest into the
-// column-major x_out and column-fastest into the row-major update rows.
-// The factor loads are not masked.  Row blocks wholly past the front re-read
-// the last live block and steps past a wavefront's range re-read its last
-// step (cache hits, no new memory traffic; B is zero there); the remaining
-// overhang (rows past the front, which reach only discarded output rows, and
-// up to 3 k past the slice, which meet a zero B value) reads factor data of
-// later fronts or the zero tail of the buffer (p_pad), finite either way.
+// slice is assembled, so the slice's loads overlap it.  Results go out from
+// the accumulator layout: update rows and partial sums are row-major (one
+// 128-byte row per 16 lanes); rows of the column-major x_out pass through a
+// 16 x 16 transpose in LDS so that each column's 16 rows leave together.
+// The factor loads are not masked: rows past the front re-read the front's
+// last row (cache hits; they reach only discarded outputs), steps past the
+// slice re-read its last step (B is zero there), and the last step's up to 3
+// k past the slice read factor data of later fronts or the zero tail of the
+// buffer (p_pad), finite either way and met by a zero B value.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+constexpr int PCH = 16;  // four-k steps per chunk of factor loads
 
 __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS, const int32_t *tiles,
                                                         const double *A, double *W, double *part, int nrhs, int bwd,
@@ -394,44 +395,33 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
     double *V = W + slot0 * MAXR;
     const int kw = kwid[f];
     const int k0 = ks * kw, kl = min(inner, k0 + kw) - k0;
-    const int nst = (kl + 3) >> 2;   // four-k steps of the slice
-    const int q = (nst + 3) >> 2;    // steps per wavefront
+    const int nst = (kl + 3) >> 2;  // four-k steps of the slice
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int s0 = min(nst, wv * q), s1 = min(nst, s0 + q);
-    const int nb = min(4, (nrows - rt * RT + 15) >> 4);  // live 16-row blocks, >= 1
     const int t = threadIdx.x, lane = t & 63, kq = lane >> 4, col = lane & 15;
-    const int r0 = rt * RT;
-    const double *Aw = A + DS.poff[f] + r0 + (int64_t)k0 * lda;  // wavefront-uniform
-    unsigned aoff[4];
-#pragma unroll
-    for (int b = 0; b < 4; b++) aoff[b] = (unsigned)((lane & 15) + 16 * min(b, nb - 1) + kq * lda);
-    double av[8][4];
+    const int r0 = rt * RT, rb = r0 + 16 * wv;  // this wavefront's first row
+    const bool live = rb < nrows;                // wavefront-uniform
+    const double *Aw = A + DS.poff[f] + (int64_t)k0 * lda;
+    const unsigned aoff = (unsigned)(min(rb + (lane & 15), nrows - 1) + kq * lda);
+    double av[PCH];
     auto load_a = [&](int s) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int ss = min(s + u, s1 - 1);
-            const double *pa = Aw + (int64_t)(4 * ss) * lda;
-#pragma unroll
-            for (int b = 0; b < 4; b++) av[u][b] = pa[aoff[b]];
-        }
+        for (int u = 0; u < PCH; u++) av[u] = Aw[(int64_t)(4 * min(s + u, nst - 1)) * lda + aoff];
     };
-    if (s0 < s1) load_a(s0);
-    // forward, single k-slice: the child rows pulled into this tile's update
-    // rows (written back below), loaded now so they arrive with the factor
+    if (live) load_a(0);
+    // forward, single k-slice: the child rows pulled into this wavefront's
+    // update rows (written back below), loaded now so they arrive with the factor
     const bool parent = D.child_ptr[f + 1] > D.child_ptr[f];  // leaves pull nothing
     double upd[4] = {0.0, 0.0, 0.0, 0.0};
-    if (!bwd && whole && parent && r0 + RT > ns)
+    if (!bwd && whole && parent && live && rb + 16 > ns)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int i = r0 + (t >> 4) + 16 * j, c = t & 15;
-            if (i >= ns && i < nrows && c < nrhs) upd[j] = pulled(DS, W, MAXR, slot0 + i, c, 0.0);
+        for (int v = 0; v < 4; v++) {
+            const int i = rb + kq + 4 * v;
+            if (i >= ns && i < nrows && col < nrhs) upd[v] = pulled(DS, W, MAXR, slot0 + i, col, 0.0);
         }
     // LDS (dynamic, sized for the launch's widest slice): the vector slice
-    // (4 nst rows, padded columns), after the products the cross-wavefront
-    // partial sums
+    // (4 nst rows, padded columns), then one 16 x 17 transpose tile per wavefront
     extern __shared__ double lds[];
     double (*Vs)[MAXR + 1] = reinterpret_cast<double (*)[MAXR + 1]>(lds);
-    double (*red)[RT][MAXR + 1] = reinterpret_cast<double (*)[RT][MAXR + 1]>(lds);
     {
         // right-hand side or ancestor values, k fastest (x columns are
         // contiguous); rows past the slice and columns past nrhs are zero
@@ -484,51 +474,48 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
         }
     }
     __syncthreads();
-    dbl4 acc[4];
+    if (!live) return;  // a whole wavefront past the front; no barrier follows
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int s = 0; s < nst; s += PCH) {
+        if (s > 0) load_a(s);
+        double bv[PCH];
 #pragma unroll
-    for (int b = 0; b < 4; b++) acc[b] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for (int s = s0; s < s1; s += 8) {
-        if (s > s0) load_a(s);
-        double bv[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int ss = min(s + u, s1 - 1);
-            bv[u] = (s + u < s1) ? Vs[4 * ss + kq][col] : 0.0;  // wavefront-uniform condition
+        for (int u = 0; u < PCH; u++) {
+            const int ss = min(s + u, nst - 1);
+            bv[u] = (s + u < nst) ? Vs[4 * ss + kq][col] : 0.0;  // wavefront-uniform condition
         }
 #pragma unroll
-        for (int u = 0; u < 8; u++)
-#pragma unroll
-            for (int b = 0; b < 4; b++) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][b], bv[u], acc[b], 0, 0, 0);
-        __builtin_amdgcn_sched_group_barrier(0x020, 32, 0);
-        __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
-        __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
+        for (int u = 0; u < PCH; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+        __builtin_amdgcn_sched_group_barrier(0x020, PCH, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, PCH, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, PCH, 0);
     }
-    __syncthreads();  // the slice is dead; its LDS takes the partial sums
+    // lane holds D[kq + 4 v][col]: row rb + kq + 4 v, right-hand side col
+    if (!whole) {
 #pragma unroll
-    for (int b = 0; b < 4; b++)
+        for (int v = 0; v < 4; v++)
+            part[((int64_t)tile * RT + 16 * wv + kq + 4 * v) * MAXR + col] = acc[v];
+        return;
+    }
+    if (!bwd && rb + 16 > ns)
 #pragma unroll
-        for (int v = 0; v < 4; v++) red[wv][16 * b + kq + 4 * v][col] = acc[b][v];
-    __syncthreads();
-    auto sum = [&](int row, int c) { return red[0][row][c] + red[1][row][c] + red[2][row][c] + red[3][row][c]; };
-    if (whole) {
-        // single k-slice: the sum is final (no other workgroup of the launch
-        // reads this front's pivots of x_out or its update rows)
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int e = t + 256 * j, row = e & 63, c = e >> 6, i = r0 + row;
-            if (i < nrows && c < nrhs && (bwd || i < ns)) xout[(piv0 + i) + (int64_t)c * n_free] = sum(row, c);
+        for (int v = 0; v < 4; v++) {
+            const int i = rb + kq + 4 * v;
+            if (i >= ns && i < nrows && col < nrhs) V[(int64_t)i * MAXR + col] = upd[v] - acc[v];
         }
-        if (!bwd)
+    if (rb < (bwd ? nrows : min(nrows, ns))) {
+        // pivot rows to x_out through the wavefront's transpose tile
+        double (*Ts)[MAXR + 1] = reinterpret_cast<double (*)[MAXR + 1]>(lds + (size_t)4 * nst * (MAXR + 1)) + 16 * wv;
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int row = (t >> 4) + 16 * j, c = t & 15, i = r0 + row;
-                if (i >= ns && i < nrows && c < nrhs) V[(int64_t)i * MAXR + c] = upd[j] - sum(row, c);
-            }
-    } else {
+        for (int v = 0; v < 4; v++) Ts[kq + 4 * v][col] = acc[v];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int r = lane & 15, i = rb + r;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const int e = t + 256 * j, row = e >> 4, c = e & 15;
-            part[((int64_t)tile * RT + row) * MAXR + c] = sum(row, c);
+            const int c = (lane >> 4) + 4 * j;
+            if (i < nrows && c < nrhs && (bwd || i < ns)) xout[(piv0 + i) + (int64_t)c * n_free] = Ts[r][c];
         }
     }
 }
@@ -629,10 +616,9 @@ ProdTimer::~ProdTimer() {
 }
 
 // LDS bytes of a block-RHS product launch whose widest k-slice is kmax: the
-// slice rows (rounded up to four-k steps) or the partial sums, the larger
+// slice rows (rounded up to four-k steps), then four 16 x 17 transpose tiles
 static size_t prod_lds(int kmax) {
-    const size_t slice = (size_t)4 * ((kmax + 3) / 4) * (MAXR + 1), red = (size_t)4 * RT * (MAXR + 1);
-    return sizeof(double) * std::max(slice, red);
+    return sizeof(double) * ((size_t)4 * ((kmax + 3) / 4) * (MAXR + 1) + (size_t)4 * 16 * (MAXR + 1));
 }
 
 // algorithmic bytes of one product launch: factor entries, vector rows in and out
