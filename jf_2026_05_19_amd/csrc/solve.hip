@@ -367,8 +367,8 @@ __global__ void __launch_bounds__(256) prod1_kernel(DevFronts D, DevSolve DS, co
 // over the whole k-slice, so no partial sums cross wavefronts: the factor is
 // read in the column-major front layout (128 contiguous bytes per block
 // column), the B values (slice row k + l/16, column l%16) come from the
-// slice in LDS, and a chunk of 16 four-k steps (16 factor loads per lane) is
-// issued before its products; the first chunk goes out before the vector
+// slice in LDS, and a chunk of PCH four-k steps (PCH factor loads per lane)
+// is issued before its products; the first chunk goes out before the vector
 // slice is assembled, so the slice's loads overlap it.  Results go out from
 // the accumulator layout: update rows and partial sums are row-major (one
 // 128-byte row per 16 lanes); rows of the column-major x_out pass through a
@@ -380,8 +380,9 @@ __global__ void __launch_bounds__(256) prod1_kernel(DevFronts D, DevSolve DS, co
 // buffer (p_pad), finite either way and met by a zero B value.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-constexpr int PCH = 16;  // four-k steps per chunk of factor loads
-
+// PCH: four-k steps per chunk of factor loads (8: more wavefronts per SIMD
+// for short slices on large grids; 16: fewer dependent rounds otherwise)
+template <int PCH>
 __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS, const int32_t *tiles,
                                                         const double *A, double *W, double *part, int nrhs, int bwd,
                                                         const int32_t *kwid, const int32_t *nks, const double *xin,
@@ -621,6 +622,18 @@ static size_t prod_lds(int kmax) {
     return sizeof(double) * ((size_t)4 * ((kmax + 3) / 4) * (MAXR + 1) + (size_t)4 * 16 * (MAXR + 1));
 }
 
+// one block-RHS product launch over n_tiles tiles whose widest k-slice is kmax
+static void launch_prod(int n_tiles, int kmax, hipStream_t s, const DevFronts &D, const DevSolve &DS,
+                        const int32_t *tl, const double *A, double *W, double *part, int nrhs, int bwd,
+                        const int32_t *kw, const int32_t *nks, const double *xin, double *xout, int n_free) {
+    if (kmax <= 192 && n_tiles >= 380)
+        prod_mfma_kernel<8><<<n_tiles, 256, prod_lds(kmax), s>>>(D, DS, tl, A, W, part, nrhs, bwd, kw, nks, xin, xout,
+                                                                 n_free);
+    else
+        prod_mfma_kernel<16><<<n_tiles, 256, prod_lds(kmax), s>>>(D, DS, tl, A, W, part, nrhs, bwd, kw, nks, xin,
+                                                                  xout, n_free);
+}
+
 // algorithmic bytes of one product launch: factor entries, vector rows in and out
 static double prod_bytes(const SolveHeight &SH, int nrhs) {
     return 8.0 * ((double)SH.sum_mns + (double)nrhs * (double)(SH.sum_m + SH.sum_ns));
@@ -643,8 +656,8 @@ hipError_t solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS
             prod1_kernel<<<SH.n_fwd, 256, 0, s>>>(D, DS, tl, Pbuf, W, part, 0, DS.fwd_kw, DS.fwd_nks, xin, xout,
                                                   P.n_free);
         else
-            prod_mfma_kernel<<<SH.n_fwd, 256, prod_lds(SH.fwd_kmax), s>>>(D, DS, tl, Pbuf, W, part, nrhs, 0,
-                                                                          DS.fwd_kw, DS.fwd_nks, xin, xout, P.n_free);
+            launch_prod(SH.n_fwd, SH.fwd_kmax, s, D, DS, tl, Pbuf, W, part, nrhs, 0, DS.fwd_kw, DS.fwd_nks, xin, xout,
+                        P.n_free);
         if (timed) pt->end(s);
         // single-slice fronts were finished by the product itself
         if (SH.fwd_split)
@@ -670,8 +683,8 @@ hipError_t solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS
             prod1_kernel<<<SH.n_bwd, 256, 0, s>>>(D, DS, tl, PTbuf, W, part, 1, DS.bwd_kw, DS.bwd_nks, xin, xout,
                                                   P.n_free);
         else
-            prod_mfma_kernel<<<SH.n_bwd, 256, prod_lds(SH.bwd_kmax), s>>>(D, DS, tl, PTbuf, W, part, nrhs, 1,
-                                                                          DS.bwd_kw, DS.bwd_nks, xin, xout, P.n_free);
+            launch_prod(SH.n_bwd, SH.bwd_kmax, s, D, DS, tl, PTbuf, W, part, nrhs, 1, DS.bwd_kw, DS.bwd_nks, xin,
+                        xout, P.n_free);
         if (timed) pt->end(s);
         if (SH.bwd_split)
             bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(
