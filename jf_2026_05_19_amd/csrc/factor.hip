@@ -6,7 +6,8 @@
 // matrices are extend-added into their parent in a fixed child order (no
 // atomics, bit-reproducible), then each front is partially factored with a
 // right-looking blocked algorithm (panel width 32):
-//   potrf of the 32x32 diagonal block (one wavefront per front),
+//   potrf of the 32x32 diagonal block (one wavefront per front; after the
+//   first panel, by the update that completed the block),
 //   trsm of the panel rows below it, and
 //   a 64x64-tiled syrk update of the trailing lower triangle.
 // The first ns columns of every front then hold L11 / L21.
@@ -62,24 +63,13 @@ __device__ __forceinline__ double readlane_d(double x, int lane) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// Cholesky of the nb x nb diagonal block (nb <= 32) by one wavefront: lane l
-// holds row l of the block in registers; the diagonal and column j are
-// broadcast with v_readlane (scalar registers), so the 32 columns need no
-// barrier and no LDS round trip.
-__global__ void __launch_bounds__(64) potrf_diag_kernel(DevFronts D, const int32_t *flist, int k0, double *F,
-                                                        double *piv, int *err) {
-    const int f = flist[blockIdx.x];
-    const int ns = D.ns[f];
-    if (k0 >= ns) return;
-    const int nb = min(NB, ns - k0), m = D.m[f];
-    double *A = F + D.off[f];
-    const int l = threadIdx.x, lr = min(l, nb - 1);
-    double a[NB];
-#pragma unroll
-    for (int j = 0; j < NB; j++) a[j] = A[(k0 + lr) + (int64_t)(k0 + min(j, nb - 1)) * m];
-#pragma unroll
-    for (int j = 0; j < NB; j++) a[j] = (l < nb && j <= l && j < nb) ? a[j] : 0.0;
-    double *pv = piv + D.piv0[f] + k0;
+// Cholesky of an nb x nb diagonal block (nb <= 32) by one wavefront: lane l
+// holds row l of the block in registers (a[j], j <= l); the diagonal and
+// column j are broadcast with v_readlane (scalar registers), so the 32
+// columns need no barrier and no LDS round trip.  Pivots go to pv, a
+// non-positive one sets err |= 1.
+__device__ __forceinline__ void potrf_regs(double (&a)[NB], int nb, double *pv, int *err) {
+    const int l = threadIdx.x & 63;
 #pragma unroll
     for (int j = 0; j < NB; j++) {
         if (j < nb) {
@@ -98,6 +88,24 @@ __global__ void __launch_bounds__(64) potrf_diag_kernel(DevFronts D, const int32
             }
         }
     }
+}
+
+// The first panel's diagonal block of every front of a height (the later
+// panels' blocks are factored by the trailing update that completes them).
+__global__ void __launch_bounds__(64) potrf_diag_kernel(DevFronts D, const int32_t *flist, int k0, double *F,
+                                                        double *piv, int *err) {
+    const int f = flist[blockIdx.x];
+    const int ns = D.ns[f];
+    if (k0 >= ns) return;
+    const int nb = min(NB, ns - k0), m = D.m[f];
+    double *A = F + D.off[f];
+    const int l = threadIdx.x, lr = min(l, nb - 1);
+    double a[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) a[j] = A[(k0 + lr) + (int64_t)(k0 + min(j, nb - 1)) * m];
+#pragma unroll
+    for (int j = 0; j < NB; j++) a[j] = (l < nb && j <= l && j < nb) ? a[j] : 0.0;
+    potrf_regs(a, nb, piv + D.piv0[f] + k0, err);
     if (l < nb)
 #pragma unroll
         for (int j = 0; j < NB; j++)
@@ -161,7 +169,13 @@ __global__ void trsm_panel_kernel(DevFronts D, const int32_t *flist, int k0, dou
 // The old tile is read before the product (indices clamped into the front, no
 // conditional read) and parked in LDS, so the read-modify-write costs one
 // memory latency per tile however few tiles a CU holds.
-__global__ void __launch_bounds__(256) syrk_update_kernel(DevFronts D, const int32_t *flist, int k0, double *F) {
+// Tile 0 (rows and columns [k1, k1 + 64)) holds the next panel's diagonal
+// block once updated: its quadrant (0, 0) is wavefront 0's, which keeps the
+// new values in LDS and factors the block there (pivots, error flag), so the
+// next panel needs no separate diagonal launch.  No other workgroup of the
+// launch reads that block.
+__global__ void __launch_bounds__(256) syrk_update_kernel(DevFronts D, const int32_t *flist, int k0, double *F,
+                                                          double *piv, int *err) {
     int f = flist[blockIdx.y];
     int ns = D.ns[f];
     if (k0 >= ns) return;
@@ -228,6 +242,8 @@ __global__ void __launch_bounds__(256) syrk_update_kernel(DevFronts D, const int
 #pragma unroll
     for (int u = 0; u < TS / 4; u++) Ot[tid & 63][(tid >> 6) + 4 * u] = oldr[u];
     __syncthreads();
+    const bool next = t == 0 && k1 < ns;  // workgroup-uniform
+    const int nb2 = min(NB, ns - k1);
 #pragma unroll
     for (int jb = 0; jb < 2; jb++)
 #pragma unroll
@@ -236,8 +252,25 @@ __global__ void __launch_bounds__(256) syrk_update_kernel(DevFronts D, const int
             for (int v = 0; v < 4; v++) {
                 const int il = qi + 16 * ib + col, jl = qj + 16 * jb + kq + 4 * v;
                 const int i = r0 + il, j = c0 + jl;
-                if (i < m && j < m && i >= j) A[i + (int64_t)j * m] = Ot[il][jl] - acc[jb][ib][v];
+                const double val = Ot[il][jl] - acc[jb][ib][v];
+                const bool diag = next && il < nb2 && jl < nb2;  // written factored below
+                if (i < m && j < m && i >= j && !diag) A[i + (int64_t)j * m] = val;
+                if (next && wv == 0) Ot[il][jl] = val;  // quadrant (0, 0): wavefront 0's own entries
             }
+    if (next && wv == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int l = lane, lr = min(l, nb2 - 1);
+        double a[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) a[j] = (l < nb2 && j <= l && j < nb2) ? Ot[lr][min(j, nb2 - 1)] : 0.0;
+        potrf_regs(a, nb2, piv + D.piv0[f] + k1, err);
+        if (l < nb2)
+#pragma unroll
+            for (int j = 0; j < NB; j++)
+                if (j <= l) A[(k1 + l) + (int64_t)(k1 + j) * m] = a[j];
+    }
 }
 
 // pivot ratio check of the post-factorisation refix (boundary_conditions.jl:742-782):
@@ -448,14 +481,15 @@ void factor_numeric(const FactorPlan &P, const DevFronts &D, const double *kvals
                 extend_add_kernel<<<dim3(gx, H.n), 256, 0, s>>>(D, fl, slot, F);
         }
         for (int k0 = 0; k0 < H.max_ns; k0 += NB) {
-            potrf_diag_kernel<<<H.n, 64, 0, s>>>(D, fl, k0, F, piv, err);
+            // later panels' diagonal blocks: factored by the previous update
+            if (k0 == 0) potrf_diag_kernel<<<H.n, 64, 0, s>>>(D, fl, k0, F, piv, err);
             // rows below the panel: m - k0 - nb for each front (nb <= NB), so
             // max_m - k0 bounds every front of this height; kernels clip to m.
             int rows = H.max_m - k0;
             if (rows > 0) {
                 trsm_panel_kernel<<<dim3(nblk(rows, 256), H.n), 256, 0, s>>>(D, fl, k0, F);
                 int T = (rows + TS - 1) / TS;
-                syrk_update_kernel<<<dim3((unsigned)(T * (T + 1) / 2), H.n), 256, 0, s>>>(D, fl, k0, F);
+                syrk_update_kernel<<<dim3((unsigned)(T * (T + 1) / 2), H.n), 256, 0, s>>>(D, fl, k0, F, piv, err);
             }
         }
     }
