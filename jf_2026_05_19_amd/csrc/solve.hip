@@ -23,6 +23,8 @@ constexpr int TB = 32;        // inversion block
 constexpr int RT = SOLVE_RT;  // rows per product tile
 constexpr int KS = SOLVE_KS;  // k-slice of the product
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
 // --- inverse of L11 ----------------------------------------------------------
 // Every front is final once the factorization has finished, so all fronts'
 // inverses are built in two launches: the 32 x 32 diagonal blocks first, then
@@ -149,8 +151,17 @@ __global__ void __launch_bounds__(256) linv_col_kernel(DevFronts D, DevSolve DS,
     }
 }
 
+// Workgroup barrier that orders LDS only (this wavefront's LDS operations
+// complete, then s_barrier): global loads in flight stay in flight, where
+// __syncthreads() would wait for them.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // P[bottom] = L21 * L11^-1 : 64 x 64 output tiles, k from the tile column
-// start (L11^-1 is lower triangular).
+// start (L11^-1 is lower triangular), on the matrix cores.  Wavefront w takes
+// the 32 x 32 quadrant (rows 32 (w & 1), columns 32 (w >> 1)) as 2 x 2 blocks
+// of v_mfma_f64_16x16x4f64, computing the transposed block
+// D^T[j][i] = sum_k Bs[k][j] As[i][k] so that each lane's outputs
+// (D^T[l/16 + 4 v][l%16]) run down consecutive rows of the column-major P.
 __global__ void __launch_bounds__(256) m21_kernel(DevFronts D, const int32_t *flist, const double *F, double *P,
                                                   const int64_t *poff) {
     int f = flist[blockIdx.z];
@@ -160,54 +171,58 @@ __global__ void __launch_bounds__(256) m21_kernel(DevFronts D, const int32_t *fl
     const double *A = F + D.off[f];
     double *Pf = P + poff[f];
     __shared__ double As[64][33], Bs[32][65];
-    int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-    double acc[4][4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, kq = lane >> 4, col = lane & 15;
+    const int qi = 32 * (wv & 1), qj = 32 * (wv >> 1);
+    dbl4 acc[2][2];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int jb = 0; jb < 2; jb++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+        for (int ib = 0; ib < 2; ib++) acc[jb][ib] = dbl4{0.0, 0.0, 0.0, 0.0};
+    // the next k-step's tiles are loaded (all loads together, clamped
+    // indices, masks applied after) while the current one is multiplied
+    constexpr int U = 64 * 32 / 256;
+    double ta[U], tb[U];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = tid + 256 * u;
+            const int r = t % 64, k = t / 64, kk = t % 32, c = t / 32;
+            ta[u] = A[min(r0 + r, m - 1) + (int64_t)min(k0 + k, ns - 1) * m];
+            tb[u] = Pf[min(k0 + kk, ns - 1) + (int64_t)min(c0 + c, ns - 1) * m];
+        }
+    };
+    load(c0);
     for (int k0 = c0; k0 < ns; k0 += 32) {
-        {
-            // all loads issued together (clamped indices), masks applied after
-            constexpr int U = 64 * 32 / 256;
-            double ta[U], tb[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int t = threadIdx.x + 256 * u;
-                const int r = t % 64, k = t / 64, kk = t % 32, c = t / 32;
-                ta[u] = A[min(r0 + r, m - 1) + (int64_t)min(k0 + k, ns - 1) * m];
-                tb[u] = Pf[min(k0 + kk, ns - 1) + (int64_t)min(c0 + c, ns - 1) * m];
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int t = threadIdx.x + 256 * u;
-                const int r = t % 64, k = t / 64, kk = t % 32, c = t / 32;
-                As[r][k] = (r0 + r < m && k0 + k < ns) ? ta[u] : 0.0;
-                Bs[kk][c] = (k0 + kk < ns && c0 + c < ns) ? tb[u] : 0.0;
-            }
+        for (int u = 0; u < U; u++) {
+            const int t = tid + 256 * u;
+            const int r = t % 64, k = t / 64, kk = t % 32, c = t / 32;
+            As[r][k] = (r0 + r < m && k0 + k < ns) ? ta[u] : 0.0;
+            Bs[kk][c] = (k0 + kk < ns && c0 + c < ns) ? tb[u] : 0.0;
         }
-        __syncthreads();
-#pragma unroll 4
-        for (int k = 0; k < 32; k++) {
-            double ra[4], cb[4];
+        lds_barrier();
+        if (k0 + 32 < ns) load(k0 + 32);
 #pragma unroll
-            for (int a = 0; a < 4; a++) ra[a] = As[tx + 16 * a][k];
-#pragma unroll
-            for (int b = 0; b < 4; b++) cb[b] = Bs[k][ty + 16 * b];
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 4; b++) acc[a][b] += ra[a] * cb[b];
+        for (int kk = 0; kk < 8; kk++) {
+            const int k = 4 * kk + kq;
+            const double b0 = Bs[k][qj + col], b1 = Bs[k][qj + 16 + col];
+            const double a0 = As[qi + col][k], a1 = As[qi + 16 + col][k];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
         }
-        __syncthreads();
+        lds_barrier();  // the tiles are read; the next loads stay in flight
     }
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int jb = 0; jb < 2; jb++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            int i = r0 + tx + 16 * a, j = c0 + ty + 16 * b;
-            if (i < m && j < ns) Pf[i + (int64_t)j * m] = acc[a][b];
-        }
+        for (int ib = 0; ib < 2; ib++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = r0 + qi + 16 * ib + col, j = c0 + qj + 16 * jb + kq + 4 * v;
+                if (i < m && j < ns) Pf[i + (int64_t)j * m] = acc[jb][ib][v];
+            }
 }
 
 // PT[j + i*ns] = P[i + j*m]  (64 x 64 tiles through LDS)
@@ -378,8 +393,6 @@ __global__ void __launch_bounds__(256) prod1_kernel(DevFronts D, DevSolve DS, co
 // slice re-read its last step (B is zero there), and the last step's up to 3
 // k past the slice read factor data of later fronts or the zero tail of the
 // buffer (p_pad), finite either way and met by a zero B value.
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-
 // PCH: four-k steps per chunk of factor loads (8: more wavefronts per SIMD
 // for short slices on large grids; 16: fewer dependent rounds otherwise)
 template <int PCH>
