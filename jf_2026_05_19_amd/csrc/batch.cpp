@@ -226,14 +226,16 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
     }
     const int64_t sT = (int64_t)ldT * ldT, s16 = 256, sH2 = (int64_t)(mmax + b) * b;
     auto blk = [&](int jj) { return V.p + (size_t)jj * b * n; };
-    auto cholqr2 = [&](double *Q, double *Rout, int64_t sR, int ldr, const double *g0) {
-        seg_gemm_tn(T, b, b, Q, n, Q, n, G1.p, s16, 16, 1.0, 0.0, part.p, s);
+    // CholQR2 per segment; g1_ready: G1 = Q^T Q was formed by the kernel
+    // that wrote Q.  The first triangular solve forms the second pass's Gram
+    // matrix, the second one R = R2 R1.
+    auto cholqr2 = [&](double *Q, double *Rout, int64_t sR, int ldr, const double *g0, bool g1_ready) {
+        if (!g1_ready) seg_gemm_tn(T, b, b, Q, n, Q, n, G1.p, s16, 16, 1.0, 0.0, part.p, s);
         seg_chol_upper(T, b, G1.p, s16, 16, g0, s16, 16, g0 ? dep_tol2 : 0.0, ierr.p, s);
-        seg_trsm_right_upper(T, b, Q, n, G1.p, s16, 16, s);
-        seg_gemm_tn(T, b, b, Q, n, Q, n, G2.p, s16, 16, 1.0, 0.0, part.p, s);
+        seg_trsm_right_upper_gram(T, b, Q, n, G1.p, s16, 16, part.p, G2.p, s16, 16, s);
         seg_chol_upper(T, b, G2.p, s16, 16, nullptr, 0, 0, 0.0, ierr.p, s);
-        seg_trsm_right_upper(T, b, Q, n, G2.p, s16, 16, s);
-        if (Rout) seg_upper_mul(T, b, G2.p, s16, 16, G1.p, s16, 16, Rout, sR, ldr, s);
+        if (Rout) seg_trsm_right_upper_mul(T, b, Q, n, G2.p, s16, 16, G1.p, s16, 16, Rout, sR, ldr, s);
+        else seg_trsm_right_upper(T, b, Q, n, G2.p, s16, 16, s);
     };
     auto step = [&](int jj) {
         HIPCHK(solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, blk(jj), Wb.p, b, s));
@@ -245,20 +247,21 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
         HIPCHK(solve_fwd(M->plan, M->dev, M->dsolve, M->splan, M->d_P.p, M->d_W.p, M->d_part.p, Wb.p, Wn, b, s));
         int kcols = (jj + 1) * b;
         double *Tcol = Tm.p + (size_t)jj * b * ldT;
+        // first pass: coefficients into T, update; G0 = Gram matrix after it
         seg_gemm_tn(T, kcols, b, V.p, n, Wn, n, Tcol, sT, ldT, 1.0, 0.0, part.p, s);
-        seg_gemm_nn(T, kcols, b, V.p, n, Tcol, sT, ldT, Wn, n, -1.0, 1.0, s);
-        seg_gemm_tn(T, b, b, Wn, n, Wn, n, G0.p, s16, 16, 1.0, 0.0, part.p, s);
-        seg_gemm_tn(T, kcols, b, V.p, n, Wn, n, H2.p, sH2, kcols, 1.0, 0.0, part.p, s);
-        seg_gemm_nn(T, kcols, b, V.p, n, H2.p, sH2, kcols, Wn, n, -1.0, 1.0, s);
-        seg_mat_add(T, kcols, b, H2.p, sH2, kcols, Tcol, sT, ldT, s);
-        cholqr2(blk(jj + 1), Tm.p + (size_t)(jj + 1) * b + (size_t)jj * b * ldT, sT, ldT, G0.p);
+        seg_gemm_nn_gram(T, kcols, b, V.p, n, Tcol, sT, ldT, Wn, n, -1.0, 1.0, part.p, G0.p, s16, 16, s);
+        // second pass: coefficients also added into T; G1 = Gram matrix of
+        // the result (the first Cholesky-QR pass of the new block)
+        seg_gemm_tn_add(T, kcols, b, V.p, n, Wn, n, H2.p, sH2, kcols, Tcol, sT, ldT, part.p, s);
+        seg_gemm_nn_gram(T, kcols, b, V.p, n, H2.p, sH2, kcols, Wn, n, -1.0, 1.0, part.p, G1.p, s16, 16, s);
+        cholqr2(blk(jj + 1), Tm.p + (size_t)(jj + 1) * b + (size_t)jj * b * ldT, sT, ldT, G0.p, true);
         return SOL105_OK;
     };
     int j = 0, m = 0;
     if (nact > 0) {
         seg_start_block(T, b, V.p, n, 0, s);
         seg_fill(Ti, b, V.p, n, 0.0, s);
-        cholqr2(V.p, nullptr, 0, 0, nullptr);
+        cholqr2(V.p, nullptr, 0, 0, nullptr, false);
     }
     int next_check = std::min(mmax, std::max(2 * b, ((kmax * 3 + b - 1) / b) * b));
     std::vector<double> hth, hzl, hrl;

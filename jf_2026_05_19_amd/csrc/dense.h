@@ -87,14 +87,27 @@ constexpr int SEG_CH = 512, SEG_CHS = 256;
 void seg_gemm_tn(const SegTable &T, int P, int Q, const double *V, int ldv, const double *W, int ldw, double *out,
                  int64_t so, int ldo, double alpha, double beta, double *part, hipStream_t s);
 int64_t seg_gemm_tn_part_size(int n_ch, int n_chs, int P, int Q);
+// out = V^T W and out2 += V^T W per segment
+void seg_gemm_tn_add(const SegTable &T, int P, int Q, const double *V, int ldv, const double *W, int ldw, double *out,
+                     int64_t so, int ldo, double *out2, int64_t so2, int ldo2, double *part, hipStream_t s);
 // Q <= 16
 void seg_gemm_nn(const SegTable &T, int P, int Q, const double *V, int ldv, const double *H, int64_t sh, int ldh,
                  double *Y, int ldy, double alpha, double beta, hipStream_t s);
+// as seg_gemm_nn, and G = Y^T Y (Q x Q per segment, stride sg) of the result
+void seg_gemm_nn_gram(const SegTable &T, int P, int Q, const double *V, int ldv, const double *H, int64_t sh, int ldh,
+                      double *Y, int ldy, double alpha, double beta, double *part, double *G, int64_t sg, int ldg,
+                      hipStream_t s);
 // err: two ints per segment, as small_chol_upper(_checked) sets them
 void seg_chol_upper(const SegTable &T, int b, double *G, int64_t sg, int ldg, const double *g0, int64_t sg0,
                     int ldg0, double tol2, int *err, hipStream_t s);
 void seg_trsm_right_upper(const SegTable &T, int b, double *W, int ldw, const double *R, int64_t sr, int ldr,
                           hipStream_t s);
+// as seg_trsm_right_upper, and Rout = R R1 per segment
+void seg_trsm_right_upper_mul(const SegTable &T, int b, double *W, int ldw, const double *R, int64_t sr, int ldr,
+                              const double *R1, int64_t s1, int ld1, double *Rout, int64_t so, int ldo, hipStream_t s);
+// as seg_trsm_right_upper, and G = W^T W (b x b per segment) of the result
+void seg_trsm_right_upper_gram(const SegTable &T, int b, double *W, int ldw, const double *R, int64_t sr, int ldr,
+                               double *part, double *G, int64_t sg, int ldg, hipStream_t s);
 void seg_upper_mul(const SegTable &T, int b, const double *R2, int64_t s2, int ld2, const double *R1, int64_t s1,
                    int ld1, double *out, int64_t so, int ldo, hipStream_t s);
 // start block of every segment (row index local to the segment)

@@ -330,14 +330,14 @@ __global__ void __launch_bounds__(64) chol_upper_kernel(int b, double *G, int ld
 // workgroup also forms Rout = R R1 (R1 upper), from R in LDS
 __device__ __forceinline__ void trsm_rows(int row, bool live, int b, double *W, int ldw, const double *R, int ldr,
                                           const double *R1 = nullptr, int ld1 = 0, double *Rout = nullptr,
-                                          int ldo = 0) {
+                                          int ldo = 0, bool first = true) {
     __shared__ double Rs[16][16];
     for (int t = threadIdx.x; t < 256; t += blockDim.x) {
         int i = t % 16, j = t / 16;
         Rs[i][j] = (i < b && j < b) ? R[i + j * ldr] : 0.0;
     }
     __syncthreads();
-    if (R1 && blockIdx.x == 0 && threadIdx.x < b * b) {
+    if (R1 && first && threadIdx.x < b * b) {
         const int i = threadIdx.x % b, j = threadIdx.x / b;
         double sacc = 0.0;
         for (int k = 0; k < b; k++) sacc += Rs[i][k] * R1[k + j * ld1];
@@ -364,17 +364,18 @@ __device__ __forceinline__ void trsm_rows(int row, bool live, int b, double *W, 
 __global__ void trsm_right_upper_kernel(int n, int b, double *W, int ldw, const double *R, int ldr, const double *R1,
                                         int ld1, double *Rout, int ldo) {
     int row = blockIdx.x * blockDim.x + threadIdx.x;
-    trsm_rows(row, row < n, b, W, ldw, R, ldr, R1, ld1, Rout, ldo);
+    trsm_rows(row, row < n, b, W, ldw, R, ldr, R1, ld1, Rout, ldo, blockIdx.x == 0);
 }
 
-// The same for a 256-row chunk per workgroup, then the chunk's Gram partial
-// of the new rows (their next Cholesky-QR pass) on the matrix cores, read
-// back from LDS in the operand layout
-__global__ void __launch_bounds__(256) trsm_gram_kernel(int n, int b, double *W, int ldw, const double *R, int ldr,
-                                                        double *gpart) {
+// The same for a 256-row chunk [r0, r1) per workgroup, then the chunk's Gram
+// partial of the new rows (their next Cholesky-QR pass) on the matrix cores,
+// read back from LDS in the operand layout
+__device__ __forceinline__ void trsm_gram_body(int r0, int r1, int b, double *W, int ldw, const double *R, int ldr,
+                                               double *gpart_c) {
     __shared__ double Rs[16][16];
     __shared__ double Ws[256][17];
-    const int t = threadIdx.x, row = blockIdx.x * 256 + t;
+    const int t = threadIdx.x, row = r0 + t;
+    const bool live = row < r1;
     for (int e = t; e < 256; e += blockDim.x) {
         const int i = e % 16, j = e / 16;
         Rs[i][j] = (i < b && j < b) ? R[i + j * ldr] : 0.0;
@@ -382,7 +383,7 @@ __global__ void __launch_bounds__(256) trsm_gram_kernel(int n, int b, double *W,
     __syncthreads();
     double x[16];
 #pragma unroll
-    for (int j = 0; j < 16; j++) x[j] = (j < b && row < n) ? W[row + (int64_t)j * ldw] : 0.0;
+    for (int j = 0; j < 16; j++) x[j] = (j < b && live) ? W[row + (int64_t)j * ldw] : 0.0;
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         if (j < b) {
@@ -392,12 +393,12 @@ __global__ void __launch_bounds__(256) trsm_gram_kernel(int n, int b, double *W,
             x[j] = sacc / Rs[j][j];
         }
     }
-    if (row < n)
+    if (live)
 #pragma unroll
         for (int j = 0; j < 16; j++)
             if (j < b) W[row + (int64_t)j * ldw] = x[j];
 #pragma unroll
-    for (int j = 0; j < 16; j++) Ws[t][j] = (j < b && row < n) ? x[j] : 0.0;
+    for (int j = 0; j < 16; j++) Ws[t][j] = (j < b && live) ? x[j] : 0.0;
     __syncthreads();
     const int lane = t & 63, wv = t >> 6, kq = lane >> 4, col = lane & 15;
     double y[4][4];
@@ -405,7 +406,13 @@ __global__ void __launch_bounds__(256) trsm_gram_kernel(int n, int b, double *W,
     for (int bb = 0; bb < 4; bb++)
 #pragma unroll
         for (int v = 0; v < 4; v++) y[bb][v] = Ws[64 * wv + 16 * bb + 4 * v + kq][col];
-    gram_partial(y, b, gpart + (int64_t)blockIdx.x * b * b);
+    gram_partial(y, b, gpart_c);
+}
+
+__global__ void __launch_bounds__(256) trsm_gram_kernel(int n, int b, double *W, int ldw, const double *R, int ldr,
+                                                        double *gpart) {
+    const int r0 = blockIdx.x * 256;
+    trsm_gram_body(r0, min(n, r0 + 256), b, W, ldw, R, ldr, gpart + (int64_t)blockIdx.x * b * b);
 }
 
 // R_prod = R2 * R1 (upper b x b): used for CholQR2 (R = R2 R1)
@@ -780,20 +787,22 @@ __global__ void __launch_bounds__(256) seg_tn_small_kernel(SegTable T, int P, in
 
 __global__ void __launch_bounds__(256) seg_tn_reduce_kernel(SegTable T, bool small, int P, int Q, const double *part,
                                                             double *out, int64_t so, int ldo, double alpha,
-                                                            double beta) {
+                                                            double beta, double *out2, int64_t so2, int ldo2) {
     const int sg = blockIdx.y;
     if (!T.active[sg]) return;
     const int32_t *ptr = small ? T.chs_ptr : T.ch_ptr;
-    tn_reduce_body(ptr[sg + 1] - ptr[sg], P, Q, part + (int64_t)ptr[sg] * P * Q, out + sg * so, ldo, alpha, beta);
+    tn_reduce_body(ptr[sg + 1] - ptr[sg], P, Q, part + (int64_t)ptr[sg] * P * Q, out + sg * so, ldo, alpha, beta,
+                   out2 ? out2 + sg * so2 : nullptr, ldo2);
 }
 
 __global__ void __launch_bounds__(256) seg_nn_kernel(SegTable T, int P, int Q, const double *V, int ldv,
                                                      const double *H, int64_t sh, int ldh, double *Y, int ldy,
-                                                     double alpha, double beta) {
+                                                     double alpha, double beta, double *gpart) {
     const int c = blockIdx.x, sg = T.chs_seg[c];
     if (!T.active[sg]) return;
     const int r0 = T.chs_r0[c];
-    nn_rows(r0, min(r0 + CHS, T.seg_off[sg + 1]), P, Q, V, ldv, H + sg * sh, ldh, Y, ldy, alpha, beta);
+    nn_rows(r0, min(r0 + CHS, T.seg_off[sg + 1]), P, Q, V, ldv, H + sg * sh, ldh, Y, ldy, alpha, beta,
+            gpart ? gpart + (int64_t)c * Q * Q : nullptr);
 }
 
 __global__ void __launch_bounds__(64) seg_chol_kernel(SegTable T, int b, double *G, int64_t sgs, int ldg,
@@ -803,11 +812,22 @@ __global__ void __launch_bounds__(64) seg_chol_kernel(SegTable T, int b, double 
     chol_upper_body(b, G + sg * sgs, ldg, g0 ? g0 + sg * sg0 : nullptr, ldg0, tol2, 0, err + 2 * sg);
 }
 
-__global__ void seg_trsm_kernel(SegTable T, int b, double *W, int ldw, const double *R, int64_t sr, int ldr) {
+__global__ void seg_trsm_kernel(SegTable T, int b, double *W, int ldw, const double *R, int64_t sr, int ldr,
+                                const double *R1, int64_t s1, int ld1, double *Rout, int64_t so, int ldo) {
     const int c = blockIdx.x, sg = T.chs_seg[c];
     if (!T.active[sg]) return;
     const int row = T.chs_r0[c] + threadIdx.x;
-    trsm_rows(row, row < min(T.chs_r0[c] + CHS, T.seg_off[sg + 1]), b, W, ldw, R + sg * sr, ldr);
+    // the segment's first chunk forms its Rout = R R1
+    trsm_rows(row, row < min(T.chs_r0[c] + CHS, T.seg_off[sg + 1]), b, W, ldw, R + sg * sr, ldr,
+              R1 ? R1 + sg * s1 : nullptr, ld1, R1 ? Rout + sg * so : nullptr, ldo, c == T.chs_ptr[sg]);
+}
+
+__global__ void __launch_bounds__(256) seg_trsm_gram_kernel(SegTable T, int b, double *W, int ldw, const double *R,
+                                                            int64_t sr, int ldr, double *gpart) {
+    const int c = blockIdx.x, sg = T.chs_seg[c];
+    if (!T.active[sg]) return;
+    const int r0 = T.chs_r0[c];
+    trsm_gram_body(r0, min(r0 + CHS, T.seg_off[sg + 1]), b, W, ldw, R + sg * sr, ldr, gpart + (int64_t)c * b * b);
 }
 
 __global__ void seg_upper_mul_kernel(SegTable T, int b, const double *R2, int64_t s2, int ld2, const double *R1,
@@ -925,7 +945,17 @@ void seg_gemm_tn(const SegTable &T, int P, int Q, const double *V, int ldv, cons
     if (small) seg_tn_small_kernel<<<T.n_chs, 256, 0, s>>>(T, P, Q, V, ldv, W, ldw, part);
     else seg_tn_partial_kernel<<<dim3(T.n_ch, (P + PC - 1) / PC), 256, 0, s>>>(T, P, Q, V, ldv, W, ldw, part);
     seg_tn_reduce_kernel<<<dim3(nblk((int64_t)P * Q, 16), T.n_seg), 256, 0, s>>>(T, small, P, Q, part, out, so, ldo,
-                                                                                   alpha, beta);
+                                                                                   alpha, beta, nullptr, 0, 0);
+}
+
+void seg_gemm_tn_add(const SegTable &T, int P, int Q, const double *V, int ldv, const double *W, int ldw, double *out,
+                     int64_t so, int ldo, double *out2, int64_t so2, int ldo2, double *part, hipStream_t s) {
+    if (P <= 0 || Q <= 0 || T.n_seg == 0) return;
+    const bool small = P <= 16 && Q <= 16;
+    if (small) seg_tn_small_kernel<<<T.n_chs, 256, 0, s>>>(T, P, Q, V, ldv, W, ldw, part);
+    else seg_tn_partial_kernel<<<dim3(T.n_ch, (P + PC - 1) / PC), 256, 0, s>>>(T, P, Q, V, ldv, W, ldw, part);
+    seg_tn_reduce_kernel<<<dim3(nblk((int64_t)P * Q, 16), T.n_seg), 256, 0, s>>>(T, small, P, Q, part, out, so, ldo,
+                                                                                   1.0, 0.0, out2, so2, ldo2);
 }
 
 int64_t seg_gemm_tn_part_size(int n_ch, int n_chs, int P, int Q) {
@@ -935,7 +965,16 @@ int64_t seg_gemm_tn_part_size(int n_ch, int n_chs, int P, int Q) {
 void seg_gemm_nn(const SegTable &T, int P, int Q, const double *V, int ldv, const double *H, int64_t sh, int ldh,
                  double *Y, int ldy, double alpha, double beta, hipStream_t s) {
     if (T.n_chs == 0 || Q <= 0) return;
-    seg_nn_kernel<<<T.n_chs, 256, 0, s>>>(T, P, Q, V, ldv, H, sh, ldh, Y, ldy, alpha, beta);
+    seg_nn_kernel<<<T.n_chs, 256, 0, s>>>(T, P, Q, V, ldv, H, sh, ldh, Y, ldy, alpha, beta, nullptr);
+}
+
+void seg_gemm_nn_gram(const SegTable &T, int P, int Q, const double *V, int ldv, const double *H, int64_t sh, int ldh,
+                      double *Y, int ldy, double alpha, double beta, double *part, double *G, int64_t sg, int ldg,
+                      hipStream_t s) {
+    if (T.n_chs == 0 || Q <= 0) return;
+    seg_nn_kernel<<<T.n_chs, 256, 0, s>>>(T, P, Q, V, ldv, H, sh, ldh, Y, ldy, alpha, beta, part);
+    seg_tn_reduce_kernel<<<dim3(nblk((int64_t)Q * Q, 16), T.n_seg), 256, 0, s>>>(T, true, Q, Q, part, G, sg, ldg, 1.0,
+                                                                                   0.0, nullptr, 0, 0);
 }
 
 void seg_chol_upper(const SegTable &T, int b, double *G, int64_t sg, int ldg, const double *g0, int64_t sg0,
@@ -945,7 +984,20 @@ void seg_chol_upper(const SegTable &T, int b, double *G, int64_t sg, int ldg, co
 
 void seg_trsm_right_upper(const SegTable &T, int b, double *W, int ldw, const double *R, int64_t sr, int ldr,
                           hipStream_t s) {
-    if (T.n_chs) seg_trsm_kernel<<<T.n_chs, 256, 0, s>>>(T, b, W, ldw, R, sr, ldr);
+    if (T.n_chs) seg_trsm_kernel<<<T.n_chs, 256, 0, s>>>(T, b, W, ldw, R, sr, ldr, nullptr, 0, 0, nullptr, 0, 0);
+}
+
+void seg_trsm_right_upper_mul(const SegTable &T, int b, double *W, int ldw, const double *R, int64_t sr, int ldr,
+                              const double *R1, int64_t s1, int ld1, double *Rout, int64_t so, int ldo, hipStream_t s) {
+    if (T.n_chs) seg_trsm_kernel<<<T.n_chs, 256, 0, s>>>(T, b, W, ldw, R, sr, ldr, R1, s1, ld1, Rout, so, ldo);
+}
+
+void seg_trsm_right_upper_gram(const SegTable &T, int b, double *W, int ldw, const double *R, int64_t sr, int ldr,
+                               double *part, double *G, int64_t sg, int ldg, hipStream_t s) {
+    if (T.n_chs == 0) return;
+    seg_trsm_gram_kernel<<<T.n_chs, 256, 0, s>>>(T, b, W, ldw, R, sr, ldr, part);
+    seg_tn_reduce_kernel<<<dim3(nblk((int64_t)b * b, 16), T.n_seg), 256, 0, s>>>(T, true, b, b, part, G, sg, ldg, 1.0,
+                                                                                   0.0, nullptr, 0, 0);
 }
 
 void seg_upper_mul(const SegTable &T, int b, const double *R2, int64_t s2, int ld2, const double *R1, int64_t s1,
