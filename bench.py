@@ -151,10 +151,10 @@ def main():
 
 
 # HBM bytes per leaf-height launch of the dominant kernel from rocprofv3 --pmc
-# FETCH_SIZE (219.64 MB) + WRITE_SIZE (15.92 MB), each scaled by the same
+# FETCH_SIZE (219.51 MB) + WRITE_SIZE (16.07 MB), each scaled by the same
 # counter's reading for a 256 MiB device copy (factors 2.0 and 1.0); the runs
 # and the per-grid figures are in profiles/prod_mfma_kernel_c2_panel_pmc.txt.
-PROD_TRAFFIC_BYTES = 235.56e6
+PROD_TRAFFIC_BYTES = 235.58e6
 
 
 def roofline(prof):
