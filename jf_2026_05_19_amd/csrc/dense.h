@@ -56,10 +56,11 @@ void mat_add(int rows, int cols, const double *src, int lds, double *dst, int ld
 // k <= SYMEIG_MAX_K.  theta[t] in descending |theta| order, Zs (m x k, ld m)
 // the orthonormal eigenvectors; work holds work_n >= symeig_work_size(m, k)
 // doubles; *info = 0 on success.  bw > 0 declares A banded with half-bandwidth bw
-// (block Lanczos without restarts): for 4 <= bw <= SYMEIG_BAND_MAX_W and
-// m <= SYMEIG_BAND_MAX_M the pairs then come from a band reduction, which
-// checks the claim (off-band entries within 1e-12 max|A|) and otherwise
-// leaves the matrix to the dense path.
+// (block Lanczos without restarts): for 4 <= bw <= SYMEIG_BAND_MAX_W the
+// pairs then come from a band reduction (the band in LDS up to
+// SYMEIG_BAND_MAX_M, in the workspace above), which checks the claim
+// (off-band entries within 1e-12 max|A|) and otherwise leaves the matrix to
+// the dense path.
 constexpr int SYMEIG_MAX_M = 1024;
 constexpr int SYMEIG_MAX_K = 64;
 constexpr int SYMEIG_BAND_MAX_M = 384;

@@ -503,6 +503,7 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
 
 constexpr int LDB = 2 * SYMEIG_BAND_MAX_W + 1;                 // band column: diagonal + bw + bulge
 constexpr int BAND_LDS = SYMEIG_BAND_MAX_M * LDB > SCR ? SYMEIG_BAND_MAX_M * LDB : SCR;
+constexpr int SCR_GB = 14336;  // LDS scratch of steps 3 and 4 when the band lives in the workspace
 constexpr int NSLOT = 64;                                      // previous-reflector slots (>= active sweeps)
 
 // Workgroup barrier that orders LDS only: this wavefront's LDS operations
@@ -644,6 +645,10 @@ __device__ void band_task(int m, int bw, int i, int s, double *AB, double *slot,
     }
 }
 
+// GB: the band lives in the workspace (L2-resident; m up to SYMEIG_MAX_M)
+// instead of LDS (m up to SYMEIG_BAND_MAX_M), and the step barriers then
+// order global memory too.
+template <bool GB>
 __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const double *A, int k, double *theta,
                                                          double *Zs, double *work, int *info, SymeigBatch bt) {
     if (bt.list) {
@@ -654,18 +659,22 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
         work += p * bt.sW;
         info += p;
     }
-    __shared__ double AB[BAND_LDS];  // the band; afterwards scratch of steps 3 and 4
-    __shared__ double dd[SYMEIG_BAND_MAX_M], ee[SYMEIG_BAND_MAX_M], e2[SYMEIG_BAND_MAX_M];
+    constexpr int MAXM = GB ? SYMEIG_MAX_M : SYMEIG_BAND_MAX_M;
+    constexpr int SCRN = GB ? SCR_GB : BAND_LDS;
+    __shared__ double lscr[SCRN];  // the band (LDS form); afterwards scratch of steps 3 and 4
+    __shared__ double dd[MAXM], ee[MAXM], e2[MAXM];
     __shared__ double slots[NSLOT][17];
-    __shared__ int sw_nst[SYMEIG_BAND_MAX_M], sw_off[SYMEIG_BAND_MAX_M];  // per sweep: steps, first record
+    __shared__ int sw_nst[MAXM], sw_off[MAXM];  // per sweep: steps, first record
     __shared__ double ev[2 * SYMEIG_MAX_K];
     __shared__ int sel[SYMEIG_MAX_K], asc[SYMEIG_MAX_K], ocol[SYMEIG_MAX_K], cl_start[SYMEIG_MAX_K + 1];
     __shared__ double wlo[NW], whi[NW], wem[NW];
     __shared__ double bnd[4];
     __shared__ int sint[4];
+    double *AB = GB ? work : lscr;                          // the band
+    double *recs0 = GB ? work + (int64_t)m * LDB : work;    // reflector records
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 
-    // ---- 1. band into LDS (bulge room zeroed); decline a matrix that is not banded ----
+    // ---- 1. band into place (bulge room zeroed); decline a matrix that is not banded ----
     for (int t = tid; t < m * LDB; t += NT) {
         const int c = t / LDB, d = t % LDB;
         AB[t] = (d <= bw && c + d < m) ? A[(c + d) + (int64_t)c * m] : 0.0;
@@ -709,17 +718,19 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
         const int tend = 2 * (nsw - 1) + sw_nst[nsw - 1];
         // this wavefront's sweeps are i == wid (mod NW); ihi = min(nsw - 1, ts / 2)
         // is tracked incrementally, and older sweeps finish first; at most
-        // max steps / 2 + 1 <= 49 sweeps are active (< NSLOT)
+        // ceil(steps of sweep 0 / 2) + 1 sweeps are active, <= NSLOT (band_path)
         int ihi = 0, ph = 0;
         for (int ts = 0; ts < tend; ts++) {
             for (int i = ihi - ((ihi - wid) % NW + NW) % NW; i >= 0; i -= NW) {
                 const int st = ts - 2 * i;
                 if (st >= sw_nst[i]) break;
-                band_task(m, bw, i, st, AB, slots[i % NSLOT], work + (int64_t)(sw_off[i] + st) * (bw + 1));
+                band_task(m, bw, i, st, AB, slots[i % NSLOT], recs0 + (int64_t)(sw_off[i] + st) * (bw + 1));
             }
-            // the band and the slots are in LDS; the record stores (read only
-            // after the full barriers below) need not complete here
-            lds_barrier();
+            // an LDS band and the slots need LDS order only (the record stores,
+            // read after the full barriers below, need not complete here); a
+            // band in the workspace needs the full barrier
+            if (GB) __syncthreads();
+            else lds_barrier();
             if (++ph == 2) {
                 ph = 0;
                 ihi = min(nsw - 1, ihi + 1);
@@ -734,7 +745,7 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
 
     // ---- 2 and 3 on the tridiagonal ----
     {
-        const TriLds L{dd, ee, e2, AB, ev, wlo, whi, wem, bnd, sel, asc, ocol, cl_start, sint, BAND_LDS};
+        const TriLds L{dd, ee, e2, lscr, ev, wlo, whi, wem, bnd, sel, asc, ocol, cl_start, sint, SCRN};
         tridiag_top_pairs(m, k, L, theta, Zs);
     }
     __syncthreads();
@@ -746,7 +757,7 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
     // (i - 1) while this one is applied; otherwise the tasks are strided.
     // vectors in LDS with column stride m + 1 (odd), so that the tasks of a
     // sweep, which differ in the vector, hit different banks
-    const int ldz = m + 1, cbat = BAND_LDS / ldz;
+    const int ldz = m + 1, cbat = SCRN / ldz;
     auto apply = [&](int i, int st, int c, const double *vr, double tau, double *Z) {
         const int r0 = i + 1 + st * bw, L = min(bw, m - r0);
         double *z = Z + (size_t)c * ldz + r0;
@@ -761,7 +772,7 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
     };
     for (int c0 = 0; c0 < k; c0 += cbat) {
         const int nv = min(cbat, k - c0);
-        double *Z = AB;  // nv columns of length m, stride ldz
+        double *Z = lscr;  // nv columns of length m, stride ldz
         for (int t = tid; t < nv * m; t += NT) Z[(t / m) * ldz + t % m] = Zs[(int64_t)c0 * m + t];
         __syncthreads();
         if (nsw > 0 && sw_nst[0] * nv <= NT) {
@@ -772,7 +783,7 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
             for (int q = 0; q < SYMEIG_BAND_MAX_W; q++) v[q] = 0.0;
             // fixed-width record loads, clamped into sweep i's records
             auto load_rec = [&](int i) {
-                const double *rv = work + ((int64_t)sw_off[i] + min(st, sw_nst[i] - 1)) * (bw + 1);
+                const double *rv = recs0 + ((int64_t)sw_off[i] + min(st, sw_nst[i] - 1)) * (bw + 1);
 #pragma unroll
                 for (int q = 0; q < SYMEIG_BAND_MAX_W; q++) v[q] = rv[min(q, bw)];
                 tau = rv[bw];
@@ -790,7 +801,7 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
         } else {
             for (int i = nsw - 1; i >= 0; i--) {
                 const int nst = sw_nst[i];
-                const double *recs = work + (int64_t)sw_off[i] * (bw + 1);
+                const double *recs = recs0 + (int64_t)sw_off[i] * (bw + 1);
                 for (int t = tid; t < nst * nv; t += NT) {
                     const int st = t / nv, c = t % nv;
                     const double *rv = recs + st * (bw + 1);
@@ -811,25 +822,34 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
 }  // namespace
 
 // band path (4 <= bw <= 16) at size m: one record of bw + 1 doubles per task,
-// at most m^2 (bw + 1) / (2 bw) + m (bw + 1) <= 5 m^2 / 8 + 17 m
-static int64_t band_work(int m) { return (int64_t)m + (5 * (int64_t)m * m) / 8 + 17 * (int64_t)m + 64; }
-
-// a workspace sized for m serves every call at sizes up to m: the dense path
-// needs m doubles, the band path band_work(m') for m' <= min(m, SYMEIG_BAND_MAX_M)
-int64_t symeig_work_size(int m, int k) {
-    (void)k;
-    return std::max<int64_t>(m, band_work(std::min(m, SYMEIG_BAND_MAX_M)));
+// at most m^2 (bw + 1) / (2 bw) + m (bw + 1) <= 5 m^2 / 8 + 17 m; above
+// SYMEIG_BAND_MAX_M the band itself (m columns of LDB) comes first
+static int64_t band_work(int m) {
+    return (m > SYMEIG_BAND_MAX_M ? (int64_t)m * LDB : 0) + (int64_t)m + (5 * (int64_t)m * m) / 8 +
+           17 * (int64_t)m + 64;
 }
 
+// a workspace sized for m serves every call at sizes up to m: the dense path
+// needs m doubles, the band path band_work(m') for m' <= m (band_work grows with m)
+int64_t symeig_work_size(int m, int k) {
+    (void)k;
+    return std::max<int64_t>(m, band_work(std::min(m, SYMEIG_MAX_M)));
+}
+
+// the sweeps active in one time step (at most ceil(steps of sweep 0 / 2) + 1)
+// must fit the previous-reflector slots
 static bool band_path(int m, int bw, int64_t work_n) {
-    return bw >= 4 && bw <= SYMEIG_BAND_MAX_W && m >= 3 && m <= SYMEIG_BAND_MAX_M && work_n >= band_work(m);
+    if (bw < 4 || bw > SYMEIG_BAND_MAX_W || m < 3 || m > SYMEIG_MAX_M || work_n < band_work(m)) return false;
+    const int nst0 = (m - 1 + bw - 1) / bw;
+    return (nst0 + 1) / 2 + 1 <= NSLOT;
 }
 
 void sym_eig_top(int m, double *A, int k, double *theta, double *Zs, double *work, int64_t work_n, int *info,
                  hipStream_t s, int bw) {
     const SymeigBatch one{nullptr, 0, 0, 0, 0};
     const int band = band_path(m, bw, work_n);
-    if (band) symeig_band_kernel<<<1, NT, 0, s>>>(m, bw, A, k, theta, Zs, work, info, one);
+    if (band && m <= SYMEIG_BAND_MAX_M) symeig_band_kernel<false><<<1, NT, 0, s>>>(m, bw, A, k, theta, Zs, work, info, one);
+    else if (band) symeig_band_kernel<true><<<1, NT, 0, s>>>(m, bw, A, k, theta, Zs, work, info, one);
     if (m <= 385)
         symeig_top_kernel<6, 4><<<1, NT, 0, s>>>(m, A, k, theta, Zs, work, info, one, band);
     else
@@ -841,7 +861,10 @@ void sym_eig_top_batched(int m, double *A, int64_t sA, int k, double *theta, int
     if (nlist <= 0) return;
     const SymeigBatch bt{list, sA, sT, sZ, sW};
     const int band = band_path(m, bw, sW);
-    if (band) symeig_band_kernel<<<nlist, NT, 0, s>>>(m, bw, A, k, theta, Zs, work, info, bt);
+    if (band && m <= SYMEIG_BAND_MAX_M)
+        symeig_band_kernel<false><<<nlist, NT, 0, s>>>(m, bw, A, k, theta, Zs, work, info, bt);
+    else if (band)
+        symeig_band_kernel<true><<<nlist, NT, 0, s>>>(m, bw, A, k, theta, Zs, work, info, bt);
     if (m <= 385)
         symeig_top_kernel<6, 4><<<nlist, NT, 0, s>>>(m, A, k, theta, Zs, work, info, bt, band);
     else
