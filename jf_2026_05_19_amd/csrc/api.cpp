@@ -583,7 +583,8 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     HIPCHK(yf.alloc((size_t)ndof * b));
     HIPCHK(T.alloc((size_t)(mmax + b) * (mmax + b)));
     HIPCHK(A.alloc((size_t)mmax * mmax));
-    HIPCHK(part.alloc(std::max<int64_t>(ts_gemm_tn_part_size(n, mmax + b, b), 1)));
+    HIPCHK(part.alloc(std::max<int64_t>(
+        std::max(ts_gemm_tn_part_size(n, mmax + b, b), ts_gemm_nn_tn_part_size(n, mmax + b, b)), 1)));
     HIPCHK(small.alloc(5 * 16 * 16));
     HIPCHK(ierr.alloc(4));
     HIPCHK(sw.alloc(1));
@@ -628,10 +629,11 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
         double *Tcol = T.p + (size_t)jj * b * ldT;
         // first pass: coefficients into T, update; G0 = Gram matrix after it
         ts_gemm_tn(n, kcols, b, V.p, n, dst, n, Tcol, ldT, 1.0, 0.0, part.p, s);
-        ts_gemm_nn_gram(n, kcols, b, V.p, n, Tcol, ldT, dst, n, -1.0, 1.0, part.p, G0, 16, s);
-        // second pass: its coefficients are also added into T; G1 = Gram
-        // matrix of the result (the first Cholesky-QR pass of the new block)
-        ts_gemm_tn_add(n, kcols, b, V.p, n, dst, n, H2.p, kcols, Tcol, ldT, part.p, s);
+        // ... and the second pass's coefficients (also added into T), in
+        // the same pass over V as the update
+        ts_gemm_nn_gram_tn(n, kcols, b, V.p, n, Tcol, ldT, dst, n, part.p, G0, 16, H2.p, kcols, Tcol, ldT, s);
+        // second pass update; G1 = Gram matrix of the result (the first
+        // Cholesky-QR pass of the new block)
         ts_gemm_nn_gram(n, kcols, b, V.p, n, H2.p, kcols, dst, n, -1.0, 1.0, part.p, G1, 16, s);
         return SOL105_OK;
     };

@@ -18,6 +18,12 @@ void ts_gemm_nn(int n, int P, int Q, const double *V, int ldv, const double *H, 
 // ts_gemm_tn_part_size(n, 16, 16) doubles
 void ts_gemm_nn_gram(int n, int P, int Q, const double *V, int ldv, const double *H, int ldh, double *Y, int ldy,
                      double alpha, double beta, double *part, double *G, int ldg, hipStream_t s);
+// Y -= V H, G = Y^T Y (Q x Q, ld ldg), C = V^T Y (P x Q, ld ldc) and
+// Cadd += C, in one pass over V for P <= 512 a multiple of 16 (Q <= 16); part holds
+// max(ts_gemm_nn_tn_part_size, ts_gemm_tn_part_size) doubles
+void ts_gemm_nn_gram_tn(int n, int P, int Q, const double *V, int ldv, const double *H, int ldh, double *Y, int ldy,
+                        double *part, double *G, int ldg, double *C, int ldc, double *Cadd, int ldca, hipStream_t s);
+int64_t ts_gemm_nn_tn_part_size(int n, int P, int Q);
 void small_chol_upper(int b, double *G, int ldg, int *err, hipStream_t s);
 
 // as small_chol_upper, but a pivot <= tol2 * max diag(g0) flags block blk as

@@ -250,6 +250,139 @@ __global__ void __launch_bounds__(256) nn_kernel(int n, int P, int Q, const doub
             gpart ? gpart + (int64_t)blockIdx.x * Q * Q : nullptr);
 }
 
+// The end of one classical Gram-Schmidt pass fused with the start of the
+// next, reading V once for both:  Y -= V H,  then per workgroup g the partial
+// sums G_g = Y^T Y (Q x Q) and C_g = V^T Y (P x Q) over the workgroup's rows.
+// A workgroup walks a contiguous run of 16-row chunks.  Wavefront w owns
+// the 16-column blocks w, w + 4, ..., w + 4 (NTB - 1) of V (P <= 64 NTB): it
+// loads the chunk's rows of those columns once, as the A operand of Y -= V H
+// (lane l: V[row l%16][p + l/16]), and the four wavefronts' 16 x 16 sums are
+// added in LDS in wavefront order.  The updated chunk Y (lane l holds
+// Y[4 v + l/16][l%16], the B operand of the four-row steps) then meets the
+// same V values transposed through a per-wavefront 16 x 17 LDS tile
+// (lane l: V[4 s + l/16][p + l%16]) in C += V^T Y, and wavefront 0 adds the
+// Gram products.  The next chunk's loads (Y first, then V) are issued before
+// the LDS exchange, and the barriers order LDS only, so they stay in flight.
+// Every load is unconditional (indices clamped: rows past n, blocks past P,
+// the chunk after the run's last), so the compiler can count them and waits only
+// for the values it uses; the masks act on the loaded values: rows past n
+// give Y = 0, blocks past P meet zero H rows and are not written.  P is a
+// multiple of 16.
+constexpr int NT_GRID = 512;
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <int NTB>
+__global__ void __launch_bounds__(256) nn_tn_kernel(int n, int P, int Q, const double *V, int ldv, const double *H,
+                                                    int ldh, double *Y, int ldy, double *gpart, double *cpart) {
+    extern __shared__ double lds[];
+    const int P16 = P >> 4;
+    double (*Hs)[16] = reinterpret_cast<double (*)[16]>(lds);                            // 64 NTB x 16
+    double (*red)[4][16][17] = reinterpret_cast<double (*)[4][16][17]>(lds + 1024 * NTB);  // two buffers
+    const int lane = threadIdx.x & 63, kq = lane >> 4, col = lane & 15;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double (*Ts)[16][17] = reinterpret_cast<double (*)[16][17]>(lds + 1024 * NTB + 2 * 4 * 16 * 17) + 2 * wv;
+    for (int t = threadIdx.x; t < 1024 * NTB; t += 256) {
+        const int p = t >> 4, q = t & 15;
+        Hs[p][q] = (p < P && q < Q) ? H[p + (int64_t)q * ldh] : 0.0;
+    }
+    const int nchunk = (n + 15) >> 4;
+    // a lane's V loads share one row base and differ by wavefront-uniform
+    // column offsets
+    auto load_v = [&](int c, double (&a)[NTB][4]) {
+        const double *vb = V + min(16 * c + col, n - 1) + (int64_t)kq * ldv;
+#pragma unroll
+        for (int i = 0; i < NTB; i++) {
+            const int pb = min(wv + 4 * i, P16 - 1);
+#pragma unroll
+            for (int s = 0; s < 4; s++) a[i][s] = vb[(int64_t)(16 * pb + 4 * s) * ldv];
+        }
+    };
+    auto load_y = [&](int c, double (&y0)[4]) {
+        const double *yb = Y + (int64_t)min(col, Q - 1) * ldy;
+#pragma unroll
+        for (int v = 0; v < 4; v++) y0[v] = yb[min(16 * c + kq + 4 * v, n - 1)];
+    };
+    dbl4 cacc[NTB], g = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < NTB; i++) cacc[i] = dbl4{0.0, 0.0, 0.0, 0.0};
+    // one chunk from the registers cur (V) and yc (Y); the next chunk's loads
+    // go to nxt and yn
+    auto chunk = [&](int c, int par, double (&cur)[NTB][4], double (&yc)[4], double (&nxt)[NTB][4],
+                     double (&yn)[4]) {
+        const int r0 = 16 * c;
+        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < NTB; i++)
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[i][s], Hs[16 * (wv + 4 * i) + 4 * s + kq][col], acc, 0,
+                                                           0, 0);
+        const int cn = min(c + 1, nchunk - 1);
+        load_y(cn, yn);
+        load_v(cn, nxt);
+#pragma unroll
+        for (int v = 0; v < 4; v++) red[par][wv][kq + 4 * v][col] = acc[v];
+        lds_barrier();
+        double y[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int r = r0 + kq + 4 * v;
+            const double sv = red[par][0][kq + 4 * v][col] + red[par][1][kq + 4 * v][col] +
+                              red[par][2][kq + 4 * v][col] + red[par][3][kq + 4 * v][col];
+            y[v] = (r < n && col < Q) ? yc[v] - sv : 0.0;
+        }
+        if (wv == 0) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = r0 + kq + 4 * v;
+                if (r < n && col < Q) Y[r + (int64_t)col * ldy] = y[v];
+                g = __builtin_amdgcn_mfma_f64_16x16x4f64(y[v], y[v], g, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NTB; i++) {
+            double (*T)[17] = Ts[i & 1];
+#pragma unroll
+            for (int s = 0; s < 4; s++) T[col][4 * s + kq] = cur[i][s];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            double at[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) at[s] = T[4 * s + kq][col];
+#pragma unroll
+            for (int s = 0; s < 4; s++) cacc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(at[s], y[s], cacc[i], 0, 0, 0);
+        }
+    };
+    // a contiguous run of chunks per workgroup (possibly empty)
+    const int cpw = (nchunk + gridDim.x - 1) / gridDim.x;
+    const int cb = min(nchunk, (int)blockIdx.x * cpw), ce = min(nchunk, cb + cpw);
+    double a0[NTB][4], a1[NTB][4], y0[4], y1[4];
+    load_y(min(cb, nchunk - 1), y0);
+    load_v(min(cb, nchunk - 1), a0);
+    __syncthreads();
+    // two chunks per trip, the register sets trading places
+    for (int c = cb; c < ce; c += 2) {
+        chunk(c, 0, a0, y0, a1, y1);
+        if (c + 1 < ce) chunk(c + 1, 1, a1, y1, a0, y0);
+    }
+    // lane l holds D[l/16 + 4 v][l%16]
+#pragma unroll
+    for (int i = 0; i < NTB; i++) {
+        const int pb = wv + 4 * i;
+        if (pb < P16 && col < Q)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                cpart[((int64_t)blockIdx.x * P + 16 * pb + kq + 4 * v) * Q + col] = cacc[i][v];
+    }
+    if (wv == 0 && col < Q)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int p = kq + 4 * v;
+            if (p < Q) gpart[((int64_t)blockIdx.x * Q + p) * Q + col] = g[v];
+        }
+}
+
 // Cholesky G = R^T R (upper R, b <= 16) in place, one wavefront: step j
 // forms the Schur-complement row j on lanes j..b-1.  Without g0, a pivot
 // d <= 0 sets err[0] |= 2.  With g0 (the Gram matrix of the block before
@@ -599,6 +732,37 @@ void ts_gemm_nn_gram(int n, int P, int Q, const double *V, int ldv, const double
     const unsigned nch = nblk(n, 256);
     nn_kernel<<<nch, 256, 0, s>>>(n, P, Q, V, ldv, H, ldh, Y, ldy, alpha, beta, part);
     tn_reduce_kernel<<<nblk((int64_t)Q * Q, 16), 256, 0, s>>>((int)nch, Q, Q, part, G, ldg, 1.0, 0.0, nullptr, 0);
+}
+
+static int nn_tn_grid(int n) { return std::max(1, std::min(NT_GRID, (n + 15) / 16)); }
+
+// for every P' <= P (the fused form runs up to P' = 512)
+int64_t ts_gemm_nn_tn_part_size(int n, int P, int Q) {
+    P = std::min(P, 512);
+    Q = std::min(Q, 16);
+    return (int64_t)nn_tn_grid(n) * ((int64_t)P * Q + Q * Q);
+}
+
+void ts_gemm_nn_gram_tn(int n, int P, int Q, const double *V, int ldv, const double *H, int ldh, double *Y, int ldy,
+                        double *part, double *G, int ldg, double *C, int ldc, double *Cadd, int ldca, hipStream_t s) {
+    if (P <= 0 || P > 512 || P % 16 != 0 || Q > 16) {
+        ts_gemm_nn_gram(n, P, Q, V, ldv, H, ldh, Y, ldy, -1.0, 1.0, part, G, ldg, s);
+        ts_gemm_tn_add(n, P, Q, V, ldv, Y, ldy, C, ldc, Cadd, ldca, part, s);
+        return;
+    }
+    const int grid = nn_tn_grid(n), ntb = (P / 16 + 3) / 4;
+    double *cpart = part, *gpart = part + (int64_t)grid * P * Q;
+    const size_t lds = sizeof(double) * ((size_t)1024 * ntb + 2 * 4 * 16 * 17 + 4 * 2 * 16 * 17);
+#define NN_TN(K)                                                                                     \
+    case K:                                                                                          \
+        nn_tn_kernel<K><<<grid, 256, lds, s>>>(n, P, Q, V, ldv, H, ldh, Y, ldy, gpart, cpart); \
+        break;
+    switch (ntb) {
+        NN_TN(1) NN_TN(2) NN_TN(3) NN_TN(4) NN_TN(5) NN_TN(6) NN_TN(7) NN_TN(8)
+    }
+#undef NN_TN
+    tn_reduce_kernel<<<nblk((int64_t)Q * Q, 16), 256, 0, s>>>(grid, Q, Q, gpart, G, ldg, 1.0, 0.0, nullptr, 0);
+    tn_reduce_kernel<<<nblk((int64_t)P * Q, 16), 256, 0, s>>>(grid, P, Q, cpart, C, ldc, 1.0, 0.0, Cadd, ldca);
 }
 
 void small_chol_upper(int b, double *G, int ldg, int *err, hipStream_t s) {
