@@ -585,7 +585,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     HIPCHK(A.alloc((size_t)mmax * mmax));
     HIPCHK(part.alloc(std::max<int64_t>(
         std::max(ts_gemm_tn_part_size(n, mmax + b, b), ts_gemm_nn_tn_part_size(n, mmax + b, b)), 1)));
-    HIPCHK(small.alloc(5 * 16 * 16));
+    HIPCHK(small.alloc(6 * 16 * 16));
     HIPCHK(ierr.alloc(4));
     HIPCHK(sw.alloc(1));
     if (kz > SYMEIG_MAX_K || mmax > SYMEIG_MAX_M)
@@ -597,21 +597,20 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     int ldT = mmax + b;
     dev_fill(T.p, (int64_t)ldT * ldT, 0.0, s);
     HIPCHK(hipMemsetAsync(ierr.p, 0, sizeof(int) * 4, s));
-    double *G1 = small.p, *G2 = small.p + 256, *Rt = small.p + 512, *G0 = small.p + 768, *Hs = small.p + 1024;
+    double *G1 = small.p, *G2 = small.p + 256, *Rt = small.p + 512, *G0 = small.p + 768, *Hs = small.p + 1024,
+           *R1 = small.p + 1280;
     auto blk = [&](int jj) { return V.p + (size_t)jj * b * n; };
     auto Tsub = [&](int jj) { return T.p + (size_t)(jj + 1) * b + (size_t)jj * b * ldT; };  // R of block jj+1
     // CholQR2 of Q (n x b); with g0 the first Cholesky checks for dependent
     // directions.  g1_ready: G1 = Q^T Q was formed by the kernel that wrote Q.
-    // The first triangular solve forms the second pass's Gram matrix, the
-    // second one R = R2 R1.
+    // Each triangular solve factors its Gram block itself; the first one
+    // forms the second pass's Gram matrix and keeps R1, the second one forms
+    // R = R2 R1.
     auto cholqr2 = [&](double *Q, double *Rout, int ldr, const double *g0, int bidx, bool g1_ready) {
         if (!g1_ready) ts_gemm_tn(n, b, b, Q, n, Q, n, G1, 16, 1.0, 0.0, part.p, s);
-        if (g0) small_chol_upper_checked(b, G1, 16, g0, 16, dep_tol2, bidx, ierr.p, s);
-        else small_chol_upper(b, G1, 16, ierr.p, s);
-        trsm_right_upper_gram(n, b, Q, n, G1, 16, part.p, G2, 16, s);
-        small_chol_upper(b, G2, 16, ierr.p, s);
-        if (Rout) trsm_right_upper_mul(n, b, Q, n, G2, 16, G1, 16, Rout, ldr, s);
-        else trsm_right_upper(n, b, Q, n, G2, 16, s);
+        chol_trsm_right_upper_gram(n, b, Q, n, G1, 16, g0, 16, g0 ? dep_tol2 : 0.0, bidx, ierr.p, R1, part.p, G2, 16,
+                                   s);
+        chol_trsm_right_upper(n, b, Q, n, G2, 16, ierr.p, Rout ? R1 : nullptr, 16, Rout, ldr, s);
     };
     // dst = C Q_jj, orthogonalised (CGS2) against V[:, 0:(jj+1)b]; coefficients
     // into T's column block jj; G0 = Gram matrix after the first pass.  The

@@ -39,6 +39,17 @@ void trsm_right_upper_mul(int n, int b, double *W, int ldw, const double *R, int
 // as trsm_right_upper, and G = W^T W (b x b, ld ldg) of the result
 void trsm_right_upper_gram(int n, int b, double *W, int ldw, const double *R, int ldr, double *part, double *G,
                            int ldg, hipStream_t s);
+// The two Cholesky-QR passes without separate factorisation launches: every
+// workgroup factors the b x b Gram block G itself (R = chol(G), the
+// arithmetic of small_chol_upper(_checked), same flags in err).
+// chol_trsm_right_upper_gram: W = W R^-1, G2 = W^T W, R stored to Rsave (ld 16);
+// g0 non-null checks for dependent directions as small_chol_upper_checked.
+void chol_trsm_right_upper_gram(int n, int b, double *W, int ldw, const double *G, int ldg, const double *g0,
+                                int ldg0, double tol2, int blk, int *err, double *Rsave, double *part, double *G2,
+                                int ldg2, hipStream_t s);
+// W = W R^-1 and, with R1 non-null, Rout = R R1
+void chol_trsm_right_upper(int n, int b, double *W, int ldw, const double *G, int ldg, int *err, const double *R1,
+                           int ld1, double *Rout, int ldo, hipStream_t s);
 void upper_mul(int b, const double *R2, int ld2, const double *R1, int ld1, double *out, int ldo, hipStream_t s);
 void start_block(int n, int b, double *V, int ldv, int seed, hipStream_t s);
 // (max, sum) over the shells (CQUAD4, CTRIA3) of the mean |translation|^2

@@ -117,10 +117,11 @@ __global__ void __launch_bounds__(256) tn_small_kernel(int n, int P, int Q, cons
 // out[p + q*ldo] = alpha * sum_c part[c][p][q] + beta * out[...]; sixteen lanes
 // per output sum interleaved chunk subsets, added in a fixed order
 __device__ __forceinline__ void tn_reduce_body(int nch, int P, int Q, const double *part, double *out, int ldo,
-                                               double alpha, double beta, double *out2 = nullptr, int ldo2 = 0) {
+                                               double alpha, double beta, double *out2 = nullptr, int ldo2 = 0,
+                                               int bx = -1) {
     __shared__ double red[16][17];
     const int ol = threadIdx.x % 16, sg = threadIdx.x / 16;
-    const int o = blockIdx.x * 16 + ol;
+    const int o = (bx < 0 ? (int)blockIdx.x : bx) * 16 + ol;
     const int PQ = P * Q;
     double s = 0.0;
     if (o < PQ)
@@ -148,6 +149,17 @@ __device__ __forceinline__ void tn_reduce_body(int nch, int P, int Q, const doub
 __global__ void __launch_bounds__(256) tn_reduce_kernel(int nch, int P, int Q, const double *part, double *out,
                                                         int ldo, double alpha, double beta, double *out2, int ldo2) {
     tn_reduce_body(nch, P, Q, part, out, ldo, alpha, beta, out2, ldo2);
+}
+
+// two reductions over the same chunks in one launch: C = sum cpart (P x Q,
+// also added into Cadd) on the first workgroups, G = sum gpart (Q x Q) on the
+// rest
+__global__ void __launch_bounds__(256) tn_reduce2_kernel(int nch, int P, int Q, const double *cpart, double *C,
+                                                         int ldc, double *Cadd, int ldca, const double *gpart,
+                                                         double *G, int ldg) {
+    const int nbc = (P * Q + 15) / 16;
+    if ((int)blockIdx.x < nbc) tn_reduce_body(nch, P, Q, cpart, C, ldc, 1.0, 0.0, Cadd, ldca);
+    else tn_reduce_body(nch, Q, Q, gpart, G, ldg, 1.0, 0.0, nullptr, 0, (int)blockIdx.x - nbc);
 }
 
 // Gram partial of a 256-row chunk from the MFMA operand layout: wavefront w
@@ -403,10 +415,9 @@ __device__ __forceinline__ double readlane_dd(double x, int lane) {
 // tol2 * max diag(g0) flags a rank-deficient block (err[0] |= 4, first block
 // index in err[1]); otherwise a non-positive pivot sets err[0] |= 2.  Flagged
 // pivots are clamped to max(d, threshold, 1e-300).
-__device__ __forceinline__ void chol_upper_body(int b, double *G, int ldg, const double *g0, int ldg0, double tol2,
-                                                int blk, int *err) {
+__device__ __forceinline__ int chol_upper_regs(int b, const double *G, int ldg, const double *g0, int ldg0,
+                                               double tol2, double (&c)[16]) {
     const int t = threadIdx.x, tc = min(t, b - 1);
-    double c[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) c[i] = G[min(i, b - 1) + tc * ldg];
 #pragma unroll
@@ -440,11 +451,11 @@ __device__ __forceinline__ void chol_upper_body(int b, double *G, int ldg, const
             }
         }
     }
-    if (t < b)
-#pragma unroll
-        for (int i = 0; i < 16; i++)
-            if (i < b) G[i + t * ldg] = (i <= t) ? c[i] : 0.0;
-    if (t == 0 && flag) {
+    return flag;
+}
+
+__device__ __forceinline__ void chol_flag(int flag, const double *g0, int blk, int *err) {
+    if (threadIdx.x == 0 && flag) {
         if (g0) {
             err[0] |= 4;
             if (err[1] == 0 || blk + 1 < err[1]) err[1] = blk + 1;
@@ -454,22 +465,67 @@ __device__ __forceinline__ void chol_upper_body(int b, double *G, int ldg, const
     }
 }
 
+__device__ __forceinline__ void chol_upper_body(int b, double *G, int ldg, const double *g0, int ldg0, double tol2,
+                                                int blk, int *err) {
+    const int t = threadIdx.x;
+    double c[16];
+    const int flag = chol_upper_regs(b, G, ldg, g0, ldg0, tol2, c);
+    if (t < b)
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if (i < b) G[i + t * ldg] = (i <= t) ? c[i] : 0.0;
+    chol_flag(flag, g0, blk, err);
+}
+
+// R of a b x b block into LDS (Rs[i][j] = R[i][j], zero outside b x b)
+__device__ __forceinline__ void load_r(int b, const double *R, int ldr, double (*Rs)[16]) {
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) {
+        int i = t % 16, j = t / 16;
+        Rs[i][j] = (i < b && j < b) ? R[i + j * ldr] : 0.0;
+    }
+}
+
+// The same R factored from the Gram block G by the workgroup's wavefront 0,
+// with chol_upper_body's arithmetic (so bit for bit the factor that kernel
+// writes).  first: that workgroup also stores R (ld 16) to Rsave (if given)
+// and sets the error flags.  A barrier follows the call.
+__device__ __forceinline__ void chol_upper_lds(int b, const double *G, int ldg, const double *g0, int ldg0,
+                                               double tol2, int blk, int *err, double (*Rs)[16], double *Rsave,
+                                               bool first) {
+    if (threadIdx.x >= 64) return;
+    const int t = threadIdx.x;
+    double c[16];
+    const int flag = chol_upper_regs(b, G, ldg, g0, ldg0, tol2, c);
+    if (t < 16)
+#pragma unroll
+        for (int i = 0; i < 16; i++) Rs[i][t] = (t < b && i < b && i <= t) ? c[i] : 0.0;
+    if (first) {
+        if (Rsave && t < b)
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                if (i < b) Rsave[i + t * 16] = (i <= t) ? c[i] : 0.0;
+        chol_flag(flag, g0, blk, err);
+    }
+}
+
 __global__ void __launch_bounds__(64) chol_upper_kernel(int b, double *G, int ldg, const double *g0, int ldg0,
                                                         double tol2, int blk, int *err) {
     chol_upper_body(b, G, ldg, g0, ldg0, tol2, blk, err);
 }
 
-// W = W R^-1 (R upper b x b), one thread per row.  R1 non-null: the first
-// workgroup also forms Rout = R R1 (R1 upper), from R in LDS
-__device__ __forceinline__ void trsm_rows(int row, bool live, int b, double *W, int ldw, const double *R, int ldr,
-                                          const double *R1 = nullptr, int ld1 = 0, double *Rout = nullptr,
-                                          int ldo = 0, bool first = true) {
-    __shared__ double Rs[16][16];
-    for (int t = threadIdx.x; t < 256; t += blockDim.x) {
-        int i = t % 16, j = t / 16;
-        Rs[i][j] = (i < b && j < b) ? R[i + j * ldr] : 0.0;
-    }
-    __syncthreads();
+// row `row` of W (b columns; zero when not live), loaded ahead of the barrier
+// that publishes R so its latency overlaps R's staging or factorisation
+__device__ __forceinline__ void load_row16(int row, bool live, int b, const double *W, int ldw, double (&x)[16]) {
+#pragma unroll
+    for (int j = 0; j < 16; j++) x[j] = (live && j < b) ? W[row + (int64_t)j * ldw] : 0.0;
+}
+
+// W = W R^-1 (R upper b x b, in LDS; x = the thread's row, from load_row16),
+// one thread per row.  R1 non-null: the first workgroup also forms
+// Rout = R R1 (R1 upper)
+__device__ __forceinline__ void trsm_rows(int row, bool live, int b, double *W, int ldw, const double (*Rs)[16],
+                                          double (&x)[16], const double *R1 = nullptr, int ld1 = 0,
+                                          double *Rout = nullptr, int ldo = 0, bool first = true) {
     if (R1 && first && threadIdx.x < b * b) {
         const int i = threadIdx.x % b, j = threadIdx.x / b;
         double sacc = 0.0;
@@ -477,9 +533,6 @@ __device__ __forceinline__ void trsm_rows(int row, bool live, int b, double *W, 
         Rout[i + j * ldo] = sacc;
     }
     if (!live) return;
-    double x[16];
-#pragma unroll
-    for (int j = 0; j < 16; j++) x[j] = (j < b) ? W[row + (int64_t)j * ldw] : 0.0;
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         if (j < b) {
@@ -496,27 +549,36 @@ __device__ __forceinline__ void trsm_rows(int row, bool live, int b, double *W, 
 
 __global__ void trsm_right_upper_kernel(int n, int b, double *W, int ldw, const double *R, int ldr, const double *R1,
                                         int ld1, double *Rout, int ldo) {
+    __shared__ double Rs[16][16];
     int row = blockIdx.x * blockDim.x + threadIdx.x;
-    trsm_rows(row, row < n, b, W, ldw, R, ldr, R1, ld1, Rout, ldo, blockIdx.x == 0);
+    double x[16];
+    load_row16(row, row < n, b, W, ldw, x);
+    load_r(b, R, ldr, Rs);
+    __syncthreads();
+    trsm_rows(row, row < n, b, W, ldw, Rs, x, R1, ld1, Rout, ldo, blockIdx.x == 0);
+}
+
+// as trsm_right_upper_kernel with R = chol(G) factored in every workgroup
+// (workgroup 0 keeps the error flags)
+__global__ void __launch_bounds__(256) trsm_chol_kernel(int n, int b, double *W, int ldw, const double *G, int ldg,
+                                                        int *err, const double *R1, int ld1, double *Rout, int ldo) {
+    __shared__ double Rs[16][16];
+    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    double x[16];
+    load_row16(row, row < n, b, W, ldw, x);
+    chol_upper_lds(b, G, ldg, nullptr, 0, 0.0, 0, err, Rs, nullptr, blockIdx.x == 0);
+    __syncthreads();
+    trsm_rows(row, row < n, b, W, ldw, Rs, x, R1, ld1, Rout, ldo, blockIdx.x == 0);
 }
 
 // The same for a 256-row chunk [r0, r1) per workgroup, then the chunk's Gram
 // partial of the new rows (their next Cholesky-QR pass) on the matrix cores,
 // read back from LDS in the operand layout
-__device__ __forceinline__ void trsm_gram_body(int r0, int r1, int b, double *W, int ldw, const double *R, int ldr,
-                                               double *gpart_c) {
-    __shared__ double Rs[16][16];
+__device__ __forceinline__ void trsm_gram_body(int r0, int r1, int b, double *W, int ldw, const double (*Rs)[16],
+                                               double (&x)[16], double *gpart_c) {
     __shared__ double Ws[256][17];
     const int t = threadIdx.x, row = r0 + t;
     const bool live = row < r1;
-    for (int e = t; e < 256; e += blockDim.x) {
-        const int i = e % 16, j = e / 16;
-        Rs[i][j] = (i < b && j < b) ? R[i + j * ldr] : 0.0;
-    }
-    __syncthreads();
-    double x[16];
-#pragma unroll
-    for (int j = 0; j < 16; j++) x[j] = (j < b && live) ? W[row + (int64_t)j * ldw] : 0.0;
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         if (j < b) {
@@ -544,8 +606,27 @@ __device__ __forceinline__ void trsm_gram_body(int r0, int r1, int b, double *W,
 
 __global__ void __launch_bounds__(256) trsm_gram_kernel(int n, int b, double *W, int ldw, const double *R, int ldr,
                                                         double *gpart) {
-    const int r0 = blockIdx.x * 256;
-    trsm_gram_body(r0, min(n, r0 + 256), b, W, ldw, R, ldr, gpart + (int64_t)blockIdx.x * b * b);
+    __shared__ double Rs[16][16];
+    const int r0 = blockIdx.x * 256, r1 = min(n, r0 + 256);
+    double x[16];
+    load_row16(r0 + threadIdx.x, r0 + (int)threadIdx.x < r1, b, W, ldw, x);
+    load_r(b, R, ldr, Rs);
+    __syncthreads();
+    trsm_gram_body(r0, r1, b, W, ldw, Rs, x, gpart + (int64_t)blockIdx.x * b * b);
+}
+
+// as trsm_gram_kernel with R = chol(G) factored in every workgroup
+// (workgroup 0 stores R to Rsave and keeps the error flags)
+__global__ void __launch_bounds__(256) trsm_gram_chol_kernel(int n, int b, double *W, int ldw, const double *G,
+                                                             int ldg, const double *g0, int ldg0, double tol2, int blk,
+                                                             int *err, double *Rsave, double *gpart) {
+    __shared__ double Rs[16][16];
+    const int r0 = blockIdx.x * 256, r1 = min(n, r0 + 256);
+    double x[16];
+    load_row16(r0 + threadIdx.x, r0 + (int)threadIdx.x < r1, b, W, ldw, x);
+    chol_upper_lds(b, G, ldg, g0, ldg0, tol2, blk, err, Rs, Rsave, blockIdx.x == 0);
+    __syncthreads();
+    trsm_gram_body(r0, r1, b, W, ldw, Rs, x, gpart + (int64_t)blockIdx.x * b * b);
 }
 
 // R_prod = R2 * R1 (upper b x b): used for CholQR2 (R = R2 R1)
@@ -761,8 +842,8 @@ void ts_gemm_nn_gram_tn(int n, int P, int Q, const double *V, int ldv, const dou
         NN_TN(1) NN_TN(2) NN_TN(3) NN_TN(4) NN_TN(5) NN_TN(6) NN_TN(7) NN_TN(8)
     }
 #undef NN_TN
-    tn_reduce_kernel<<<nblk((int64_t)Q * Q, 16), 256, 0, s>>>(grid, Q, Q, gpart, G, ldg, 1.0, 0.0, nullptr, 0);
-    tn_reduce_kernel<<<nblk((int64_t)P * Q, 16), 256, 0, s>>>(grid, P, Q, cpart, C, ldc, 1.0, 0.0, Cadd, ldca);
+    tn_reduce2_kernel<<<nblk((int64_t)P * Q, 16) + nblk((int64_t)Q * Q, 16), 256, 0, s>>>(grid, P, Q, cpart, C, ldc,
+                                                                                          Cadd, ldca, gpart, G, ldg);
 }
 
 void small_chol_upper(int b, double *G, int ldg, int *err, hipStream_t s) {
@@ -793,6 +874,19 @@ void trsm_right_upper_gram(int n, int b, double *W, int ldw, const double *R, in
     const unsigned nch = nblk(n, 256);
     trsm_gram_kernel<<<nch, 256, 0, s>>>(n, b, W, ldw, R, ldr, part);
     tn_reduce_kernel<<<nblk((int64_t)b * b, 16), 256, 0, s>>>((int)nch, b, b, part, G, ldg, 1.0, 0.0, nullptr, 0);
+}
+
+void chol_trsm_right_upper_gram(int n, int b, double *W, int ldw, const double *G, int ldg, const double *g0,
+                                int ldg0, double tol2, int blk, int *err, double *Rsave, double *part, double *G2,
+                                int ldg2, hipStream_t s) {
+    const unsigned nch = nblk(n, 256);
+    trsm_gram_chol_kernel<<<nch, 256, 0, s>>>(n, b, W, ldw, G, ldg, g0, ldg0, tol2, blk, err, Rsave, part);
+    tn_reduce_kernel<<<nblk((int64_t)b * b, 16), 256, 0, s>>>((int)nch, b, b, part, G2, ldg2, 1.0, 0.0, nullptr, 0);
+}
+
+void chol_trsm_right_upper(int n, int b, double *W, int ldw, const double *G, int ldg, int *err, const double *R1,
+                           int ld1, double *Rout, int ldo, hipStream_t s) {
+    trsm_chol_kernel<<<nblk(n, 256), 256, 0, s>>>(n, b, W, ldw, G, ldg, err, R1, ld1, Rout, ldo);
 }
 
 void upper_mul(int b, const double *R2, int ld2, const double *R1, int ld1, double *out, int ldo, hipStream_t s) {
@@ -980,9 +1074,15 @@ __global__ void seg_trsm_kernel(SegTable T, int b, double *W, int ldw, const dou
                                 const double *R1, int64_t s1, int ld1, double *Rout, int64_t so, int ldo) {
     const int c = blockIdx.x, sg = T.chs_seg[c];
     if (!T.active[sg]) return;
+    __shared__ double Rs[16][16];
     const int row = T.chs_r0[c] + threadIdx.x;
+    const bool live = row < min(T.chs_r0[c] + CHS, T.seg_off[sg + 1]);
+    double x[16];
+    load_row16(row, live, b, W, ldw, x);
+    load_r(b, R + sg * sr, ldr, Rs);
+    __syncthreads();
     // the segment's first chunk forms its Rout = R R1
-    trsm_rows(row, row < min(T.chs_r0[c] + CHS, T.seg_off[sg + 1]), b, W, ldw, R + sg * sr, ldr,
+    trsm_rows(row, live, b, W, ldw, Rs, x,
               R1 ? R1 + sg * s1 : nullptr, ld1, R1 ? Rout + sg * so : nullptr, ldo, c == T.chs_ptr[sg]);
 }
 
@@ -990,8 +1090,13 @@ __global__ void __launch_bounds__(256) seg_trsm_gram_kernel(SegTable T, int b, d
                                                             int64_t sr, int ldr, double *gpart) {
     const int c = blockIdx.x, sg = T.chs_seg[c];
     if (!T.active[sg]) return;
-    const int r0 = T.chs_r0[c];
-    trsm_gram_body(r0, min(r0 + CHS, T.seg_off[sg + 1]), b, W, ldw, R + sg * sr, ldr, gpart + (int64_t)c * b * b);
+    __shared__ double Rs[16][16];
+    const int r0 = T.chs_r0[c], r1 = min(r0 + CHS, T.seg_off[sg + 1]);
+    double x[16];
+    load_row16(r0 + threadIdx.x, r0 + (int)threadIdx.x < r1, b, W, ldw, x);
+    load_r(b, R + sg * sr, ldr, Rs);
+    __syncthreads();
+    trsm_gram_body(r0, r1, b, W, ldw, Rs, x, gpart + (int64_t)c * b * b);
 }
 
 __global__ void seg_upper_mul_kernel(SegTable T, int b, const double *R2, int64_t s2, int ld2, const double *R1,
