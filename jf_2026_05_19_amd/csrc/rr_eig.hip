@@ -198,9 +198,12 @@ __device__ void tridiag_top_pairs(int m, int k, const TriLds &L, double *theta, 
 
     // ---- 3. inverse iteration, one lane per cluster, LU and vector in LDS ----
     const int nc = sint[1];
-    // clusters per batch (>= 2 for m <= 1024); a chain's six arrays take
-    // 6 m + 1 doubles, so the chains of a batch start on different LDS banks
-    const int CB = scr_n / (6 * m + 1);
+    // clusters per batch (>= 2 for m <= 1024): a chain keeps four arrays of
+    // m doubles and m pivot bytes; its stride is odd, so the chains of a
+    // batch start on different LDS banks.  The second superdiagonal of U is
+    // not stored: it is ee[i+1] where rows i, i+1 were swapped, else 0.
+    const int CS = (4 * m + (m + 7) / 8) | 1;
+    const int CB = scr_n / CS;
     // slot s of a batch runs on lane s of wavefront 0: the LU and the solves
     // are branch-free, so one instruction advances every chain of the batch
     // (one lane per wavefront would pay a whole wavefront's issue per chain)
@@ -208,8 +211,9 @@ __device__ void tridiag_top_pairs(int m, int k, const TriLds &L, double *theta, 
     for (int c0 = 0; c0 < nc; c0 += CB) {
         const int cid = c0 + slot;
         if (slot < CB && cid < nc) {
-            double *D = scr + (size_t)slot * (6 * m + 1);
-            double *DL = D + m, *DU = DL + m, *DU2 = DU + m, *PV = DU2 + m, *z = PV + m;
+            double *D = scr + (size_t)slot * CS;
+            double *DL = D + m, *DU = DL + m, *z = DU + m;
+            unsigned char *PV = reinterpret_cast<unsigned char *>(z + m);
             const double tiny = EPS * tnorm + pivmin;
             double xprev = 0.0;
             for (int t = cl_start[cid]; t < cl_start[cid + 1]; t++) {
@@ -236,8 +240,7 @@ __device__ void tridiag_top_pairs(int m, int k, const TriLds &L, double *theta, 
                     D[i] = den;
                     DL[i] = f;
                     DU[i] = keep ? ucur : dn;
-                    DU2[i] = keep ? 0.0 : un;
-                    PV[i] = keep ? 0.0 : 1.0;
+                    PV[i] = keep ? 0 : 1;
                     const double nd = keep ? dn - f * ucur : ucur - f * dn;
                     ucur = keep ? un : -f * un;
                     dcur = nd;
@@ -261,7 +264,7 @@ __device__ void tridiag_top_pairs(int m, int k, const TriLds &L, double *theta, 
 #pragma unroll 4
                     for (int i = 0; i + 1 < m; i++) {
                         const double nxt = z[i + 1], li = DL[i];
-                        const bool keep = PV[i] == 0.0;
+                        const bool keep = PV[i] == 0;
                         z[i] = keep ? cur : nxt;
                         cur = keep ? nxt - li * cur : cur - li * nxt;
                     }
@@ -271,7 +274,8 @@ __device__ void tridiag_top_pairs(int m, int k, const TriLds &L, double *theta, 
                     z[m - 1] = z1;
 #pragma unroll 4
                     for (int i = m - 2; i >= 0; i--) {
-                        const double zi = (z[i] - DU[i] * z1 - DU2[i] * z2) * D[i];
+                        const double du2 = (PV[i] && i + 2 < m) ? ee[i + 1] : 0.0;
+                        const double zi = (z[i] - DU[i] * z1 - du2 * z2) * D[i];
                         z[i] = zi;
                         z2 = z1;
                         z1 = zi;
@@ -504,6 +508,9 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
 constexpr int LDB = 2 * SYMEIG_BAND_MAX_W + 1;                 // band column: diagonal + bw + bulge
 constexpr int BAND_LDS = SYMEIG_BAND_MAX_M * LDB > SCR ? SYMEIG_BAND_MAX_M * LDB : SCR;
 constexpr int SCR_GB = 14336;  // LDS scratch of steps 3 and 4 when the band lives in the workspace
+// LDS scratch of steps 3 and 4 with the band in LDS: 12 inverse-iteration
+// chains of m = 304 (4 m + m/8 doubles each) in one batch
+constexpr int SCR_BAND = BAND_LDS > 15104 ? BAND_LDS : 15104;
 constexpr int NSLOT = 64;                                      // previous-reflector slots (>= active sweeps)
 
 // Workgroup barrier that orders LDS only: this wavefront's LDS operations
@@ -660,7 +667,7 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
         info += p;
     }
     constexpr int MAXM = GB ? SYMEIG_MAX_M : SYMEIG_BAND_MAX_M;
-    constexpr int SCRN = GB ? SCR_GB : BAND_LDS;
+    constexpr int SCRN = GB ? SCR_GB : SCR_BAND;
     __shared__ double lscr[SCRN];  // the band (LDS form); afterwards scratch of steps 3 and 4
     __shared__ double dd[MAXM], ee[MAXM], e2[MAXM];
     __shared__ double slots[NSLOT][17];
