@@ -53,15 +53,28 @@ __device__ double block_sum(double v, double *wred) {
     return r;
 }
 
-// number of eigenvalues of the tridiagonal (d, e2 = e^2) below x
+// 1/q from the hardware reciprocal estimate and two Newton-Raphson steps
+// r <- r + r (1 - q r) (each doubles the correct bits; the result is within
+// an ulp or so, not correctly rounded): a shorter dependent chain than the
+// correctly rounded division.  |q| >= pivmin keeps 1/q finite.
+__device__ __forceinline__ double recip_nr(double q) {
+    double r = __builtin_amdgcn_rcp(q);
+    r = fma(fma(-q, r, 1.0), r, r);
+    r = fma(fma(-q, r, 1.0), r, r);
+    return r;
+}
+
+// number of eigenvalues of the tridiagonal (d, e2 = e^2) below x; each step
+// multiplies by the reciprocal above, so the count is that of a matrix
+// perturbed at the rounding level, as with a division
 __device__ int sturm_count(int m, const double *d, const double *e2, double x, double pivmin) {
     double q = d[0] - x;
     if (fabs(q) < pivmin) q = -pivmin;
     int c = (q < 0.0);
-    // unrolled so the LDS reads of d and e2 run ahead of the division chain
+    // unrolled so the LDS reads of d and e2 run ahead of the recurrence
 #pragma unroll 8
     for (int i = 1; i < m; i++) {
-        q = d[i] - x - e2[i - 1] / q;
+        q = d[i] - x - e2[i - 1] * recip_nr(q);
         if (fabs(q) < pivmin) q = -pivmin;
         c += (q < 0.0);
     }
