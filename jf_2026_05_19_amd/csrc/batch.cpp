@@ -208,7 +208,9 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
         HIPCHK(Zs.alloc((size_t)nm * mmax * kdim));
         HIPCHK(thd.alloc((size_t)nm * kdim));
         HIPCHK(ework.alloc((size_t)nm * symeig_work_size(mmax, kdim)));
-        HIPCHK(part.alloc(std::max<int64_t>(seg_gemm_tn_part_size(T.n_ch, T.n_chs, mmax + b, b), 1)));
+        HIPCHK(part.alloc(std::max<int64_t>(
+            std::max(seg_gemm_tn_part_size(T.n_ch, T.n_chs, mmax + b, b), seg_gemm_nn_tn_part_size(T.n_ch, mmax + b, b)),
+            1)));
         HIPCHK(G1.alloc((size_t)nm * 256));
         HIPCHK(G2.alloc((size_t)nm * 256));
         HIPCHK(G0.alloc((size_t)nm * 256));
@@ -249,10 +251,12 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
         double *Tcol = Tm.p + (size_t)jj * b * ldT;
         // first pass: coefficients into T, update; G0 = Gram matrix after it
         seg_gemm_tn(T, kcols, b, V.p, n, Wn, n, Tcol, sT, ldT, 1.0, 0.0, part.p, s);
-        seg_gemm_nn_gram(T, kcols, b, V.p, n, Tcol, sT, ldT, Wn, n, -1.0, 1.0, part.p, G0.p, s16, 16, s);
-        // second pass: coefficients also added into T; G1 = Gram matrix of
-        // the result (the first Cholesky-QR pass of the new block)
-        seg_gemm_tn_add(T, kcols, b, V.p, n, Wn, n, H2.p, sH2, kcols, Tcol, sT, ldT, part.p, s);
+        // ... and the second pass's coefficients (also added into T), in the
+        // same pass over V as the update
+        seg_gemm_nn_gram_tn(T, kcols, b, V.p, n, Tcol, sT, ldT, Wn, n, part.p, G0.p, s16, 16, H2.p, sH2, kcols, Tcol, sT,
+                            ldT, s);
+        // second pass update; G1 = Gram matrix of the result (the first
+        // Cholesky-QR pass of the new block)
         seg_gemm_nn_gram(T, kcols, b, V.p, n, H2.p, sH2, kcols, Wn, n, -1.0, 1.0, part.p, G1.p, s16, 16, s);
         cholqr2(blk(jj + 1), Tm.p + (size_t)(jj + 1) * b + (size_t)jj * b * ldT, sT, ldT, G0.p, true);
         return SOL105_OK;

@@ -115,6 +115,13 @@ void seg_gemm_nn(const SegTable &T, int P, int Q, const double *V, int ldv, cons
 void seg_gemm_nn_gram(const SegTable &T, int P, int Q, const double *V, int ldv, const double *H, int64_t sh, int ldh,
                       double *Y, int ldy, double alpha, double beta, double *part, double *G, int64_t sg, int ldg,
                       hipStream_t s);
+// Y -= V H, G = Y^T Y, C = V^T Y and Cadd += C per segment in one pass over
+// V (ts_gemm_nn_gram_tn per segment); part holds
+// max(seg_gemm_nn_tn_part_size, seg_gemm_tn_part_size) doubles
+void seg_gemm_nn_gram_tn(const SegTable &T, int P, int Q, const double *V, int ldv, const double *H, int64_t sh,
+                         int ldh, double *Y, int ldy, double *part, double *G, int64_t sg, int ldg, double *C,
+                         int64_t sc, int ldc, double *Cadd, int64_t sca, int ldca, hipStream_t s);
+int64_t seg_gemm_nn_tn_part_size(int n_ch, int P, int Q);
 // err: two ints per segment, as small_chol_upper(_checked) sets them
 void seg_chol_upper(const SegTable &T, int b, double *G, int64_t sg, int ldg, const double *g0, int64_t sg0,
                     int ldg0, double tol2, int *err, hipStream_t s);

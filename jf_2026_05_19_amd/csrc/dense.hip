@@ -283,9 +283,12 @@ __global__ void __launch_bounds__(256) nn_kernel(int n, int P, int Q, const doub
 constexpr int NT_GRID = 512;
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// rows [rbeg, rend) of the workgroup, partial sums to gpart_c (Q x Q) and
+// cpart_c (P x Q); an empty range (rend <= rbeg, rend >= 1) writes zeros
 template <int NTB>
-__global__ void __launch_bounds__(256) nn_tn_kernel(int n, int P, int Q, const double *V, int ldv, const double *H,
-                                                    int ldh, double *Y, int ldy, double *gpart, double *cpart) {
+__device__ __forceinline__ void nn_tn_rows(int rbeg, int rend, int P, int Q, const double *V, int ldv,
+                                           const double *H, int ldh, double *Y, int ldy, double *gpart_c,
+                                           double *cpart_c) {
     extern __shared__ double lds[];
     const int P16 = P >> 4;
     double (*Hs)[16] = reinterpret_cast<double (*)[16]>(lds);                            // 64 NTB x 16
@@ -297,11 +300,11 @@ __global__ void __launch_bounds__(256) nn_tn_kernel(int n, int P, int Q, const d
         const int p = t >> 4, q = t & 15;
         Hs[p][q] = (p < P && q < Q) ? H[p + (int64_t)q * ldh] : 0.0;
     }
-    const int nchunk = (n + 15) >> 4;
+    const int nsub = max(0, (rend - rbeg + 15) >> 4);  // 16-row chunks
     // a lane's V loads share one row base and differ by wavefront-uniform
     // column offsets
     auto load_v = [&](int c, double (&a)[NTB][4]) {
-        const double *vb = V + min(16 * c + col, n - 1) + (int64_t)kq * ldv;
+        const double *vb = V + min(rbeg + 16 * c + col, rend - 1) + (int64_t)kq * ldv;
 #pragma unroll
         for (int i = 0; i < NTB; i++) {
             const int pb = min(wv + 4 * i, P16 - 1);
@@ -312,7 +315,7 @@ __global__ void __launch_bounds__(256) nn_tn_kernel(int n, int P, int Q, const d
     auto load_y = [&](int c, double (&y0)[4]) {
         const double *yb = Y + (int64_t)min(col, Q - 1) * ldy;
 #pragma unroll
-        for (int v = 0; v < 4; v++) y0[v] = yb[min(16 * c + kq + 4 * v, n - 1)];
+        for (int v = 0; v < 4; v++) y0[v] = yb[min(rbeg + 16 * c + kq + 4 * v, rend - 1)];
     };
     dbl4 cacc[NTB], g = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -321,7 +324,7 @@ __global__ void __launch_bounds__(256) nn_tn_kernel(int n, int P, int Q, const d
     // go to nxt and yn
     auto chunk = [&](int c, int par, double (&cur)[NTB][4], double (&yc)[4], double (&nxt)[NTB][4],
                      double (&yn)[4]) {
-        const int r0 = 16 * c;
+        const int r0 = rbeg + 16 * c;
         dbl4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int i = 0; i < NTB; i++)
@@ -329,7 +332,7 @@ __global__ void __launch_bounds__(256) nn_tn_kernel(int n, int P, int Q, const d
             for (int s = 0; s < 4; s++)
                 acc = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[i][s], Hs[16 * (wv + 4 * i) + 4 * s + kq][col], acc, 0,
                                                            0, 0);
-        const int cn = min(c + 1, nchunk - 1);
+        const int cn = min(c + 1, max(nsub - 1, 0));
         load_y(cn, yn);
         load_v(cn, nxt);
 #pragma unroll
@@ -341,13 +344,13 @@ __global__ void __launch_bounds__(256) nn_tn_kernel(int n, int P, int Q, const d
             const int r = r0 + kq + 4 * v;
             const double sv = red[par][0][kq + 4 * v][col] + red[par][1][kq + 4 * v][col] +
                               red[par][2][kq + 4 * v][col] + red[par][3][kq + 4 * v][col];
-            y[v] = (r < n && col < Q) ? yc[v] - sv : 0.0;
+            y[v] = (r < rend && col < Q) ? yc[v] - sv : 0.0;
         }
         if (wv == 0) {
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int r = r0 + kq + 4 * v;
-                if (r < n && col < Q) Y[r + (int64_t)col * ldy] = y[v];
+                if (r < rend && col < Q) Y[r + (int64_t)col * ldy] = y[v];
                 g = __builtin_amdgcn_mfma_f64_16x16x4f64(y[v], y[v], g, 0, 0, 0);
             }
         }
@@ -366,17 +369,14 @@ __global__ void __launch_bounds__(256) nn_tn_kernel(int n, int P, int Q, const d
             for (int s = 0; s < 4; s++) cacc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(at[s], y[s], cacc[i], 0, 0, 0);
         }
     };
-    // a contiguous run of chunks per workgroup (possibly empty)
-    const int cpw = (nchunk + gridDim.x - 1) / gridDim.x;
-    const int cb = min(nchunk, (int)blockIdx.x * cpw), ce = min(nchunk, cb + cpw);
     double a0[NTB][4], a1[NTB][4], y0[4], y1[4];
-    load_y(min(cb, nchunk - 1), y0);
-    load_v(min(cb, nchunk - 1), a0);
+    load_y(0, y0);
+    load_v(0, a0);
     __syncthreads();
     // two chunks per trip, the register sets trading places
-    for (int c = cb; c < ce; c += 2) {
+    for (int c = 0; c < nsub; c += 2) {
         chunk(c, 0, a0, y0, a1, y1);
-        if (c + 1 < ce) chunk(c + 1, 1, a1, y1, a0, y0);
+        if (c + 1 < nsub) chunk(c + 1, 1, a1, y1, a0, y0);
     }
     // lane l holds D[l/16 + 4 v][l%16]
 #pragma unroll
@@ -384,15 +384,25 @@ __global__ void __launch_bounds__(256) nn_tn_kernel(int n, int P, int Q, const d
         const int pb = wv + 4 * i;
         if (pb < P16 && col < Q)
 #pragma unroll
-            for (int v = 0; v < 4; v++)
-                cpart[((int64_t)blockIdx.x * P + 16 * pb + kq + 4 * v) * Q + col] = cacc[i][v];
+            for (int v = 0; v < 4; v++) cpart_c[(int64_t)(16 * pb + kq + 4 * v) * Q + col] = cacc[i][v];
     }
     if (wv == 0 && col < Q)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int p = kq + 4 * v;
-            if (p < Q) gpart[((int64_t)blockIdx.x * Q + p) * Q + col] = g[v];
+            if (p < Q) gpart_c[p * Q + col] = g[v];
         }
+}
+
+// a contiguous run of 16-row chunks per workgroup (possibly empty)
+template <int NTB>
+__global__ void __launch_bounds__(256) nn_tn_kernel(int n, int P, int Q, const double *V, int ldv, const double *H,
+                                                    int ldh, double *Y, int ldy, double *gpart, double *cpart) {
+    const int nchunk = (n + 15) >> 4;
+    const int cpw = (nchunk + gridDim.x - 1) / gridDim.x;
+    const int cb = min(nchunk, (int)blockIdx.x * cpw), ce = min(nchunk, cb + cpw);
+    nn_tn_rows<NTB>(16 * cb, min(n, 16 * ce), P, Q, V, ldv, H, ldh, Y, ldy, gpart + (int64_t)blockIdx.x * Q * Q,
+                    cpart + (int64_t)blockIdx.x * P * Q);
 }
 
 // Cholesky G = R^T R (upper R, b <= 16) in place, one wavefront: step j
@@ -1063,6 +1073,19 @@ __global__ void __launch_bounds__(256) seg_nn_kernel(SegTable T, int P, int Q, c
             gpart ? gpart + (int64_t)c * Q * Q : nullptr);
 }
 
+// the fused pass per segment: one 512-row chunk per workgroup, partial sums
+// per chunk (the layout of seg_tn_partial_kernel)
+template <int NTB>
+__global__ void __launch_bounds__(256) seg_nn_tn_kernel(SegTable T, int P, int Q, const double *V, int ldv,
+                                                        const double *H, int64_t sh, int ldh, double *Y, int ldy,
+                                                        double *gpart, double *cpart) {
+    const int c = blockIdx.x, sg = T.ch_seg[c];
+    if (!T.active[sg]) return;
+    const int r0 = T.ch_r0[c];
+    nn_tn_rows<NTB>(r0, min(r0 + SEG_CH, T.seg_off[sg + 1]), P, Q, V, ldv, H + sg * sh, ldh, Y, ldy,
+                    gpart + (int64_t)c * Q * Q, cpart + (int64_t)c * P * Q);
+}
+
 __global__ void __launch_bounds__(64) seg_chol_kernel(SegTable T, int b, double *G, int64_t sgs, int ldg,
                                                       const double *g0, int64_t sg0, int ldg0, double tol2, int *err) {
     const int sg = blockIdx.x;
@@ -1244,6 +1267,38 @@ void seg_gemm_nn_gram(const SegTable &T, int P, int Q, const double *V, int ldv,
     seg_nn_kernel<<<T.n_chs, 256, 0, s>>>(T, P, Q, V, ldv, H, sh, ldh, Y, ldy, alpha, beta, part);
     seg_tn_reduce_kernel<<<dim3(nblk((int64_t)Q * Q, 16), T.n_seg), 256, 0, s>>>(T, true, Q, Q, part, G, sg, ldg, 1.0,
                                                                                    0.0, nullptr, 0, 0);
+}
+
+int64_t seg_gemm_nn_tn_part_size(int n_ch, int P, int Q) {
+    P = std::min(P, 512);
+    Q = std::min(Q, 16);
+    return (int64_t)n_ch * ((int64_t)P * Q + Q * Q);
+}
+
+void seg_gemm_nn_gram_tn(const SegTable &T, int P, int Q, const double *V, int ldv, const double *H, int64_t sh,
+                         int ldh, double *Y, int ldy, double *part, double *G, int64_t sg, int ldg, double *C,
+                         int64_t sc, int ldc, double *Cadd, int64_t sca, int ldca, hipStream_t s) {
+    if (T.n_ch == 0 || Q <= 0) return;
+    if (P <= 0 || P > 512 || P % 16 != 0 || Q > 16) {
+        seg_gemm_nn_gram(T, P, Q, V, ldv, H, sh, ldh, Y, ldy, -1.0, 1.0, part, G, sg, ldg, s);
+        seg_gemm_tn_add(T, P, Q, V, ldv, Y, ldy, C, sc, ldc, Cadd, sca, ldca, part, s);
+        return;
+    }
+    const int ntb = (P / 16 + 3) / 4;
+    double *cpart = part, *gpart = part + (int64_t)T.n_ch * P * Q;
+    const size_t lds = sizeof(double) * ((size_t)1024 * ntb + 2 * 4 * 16 * 17 + 4 * 2 * 16 * 17);
+#define SEG_NN_TN(K)                                                                                              \
+    case K:                                                                                                       \
+        seg_nn_tn_kernel<K><<<T.n_ch, 256, lds, s>>>(T, P, Q, V, ldv, H, sh, ldh, Y, ldy, gpart, cpart); \
+        break;
+    switch (ntb) {
+        SEG_NN_TN(1) SEG_NN_TN(2) SEG_NN_TN(3) SEG_NN_TN(4) SEG_NN_TN(5) SEG_NN_TN(6) SEG_NN_TN(7) SEG_NN_TN(8)
+    }
+#undef SEG_NN_TN
+    seg_tn_reduce_kernel<<<dim3(nblk((int64_t)Q * Q, 16), T.n_seg), 256, 0, s>>>(T, false, Q, Q, gpart, G, sg, ldg,
+                                                                                   1.0, 0.0, nullptr, 0, 0);
+    seg_tn_reduce_kernel<<<dim3(nblk((int64_t)P * Q, 16), T.n_seg), 256, 0, s>>>(T, false, P, Q, cpart, C, sc, ldc,
+                                                                                   1.0, 0.0, Cadd, sca, ldca);
 }
 
 void seg_chol_upper(const SegTable &T, int b, double *G, int64_t sg, int ldg, const double *g0, int64_t sg0,
