@@ -89,6 +89,22 @@ int sol105_pattern(sol105_ctx *ctx, const sol105_model *model, int64_t *n_blocks
  * vals (n_blocks*36) may be NULL (keeps K on the device). */
 int sol105_assemble_K(sol105_ctx *ctx, sol105_model *model, double *vals);
 
+/* No counterpart in the reference.  K assembly as sol105_assemble_K for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the call allocates
+ * nothing.  The element kernels stage their 6x6 node blocks in a
+ * library-owned static device array of SOL105_NOALLOC_SCRATCH_BYTES per GPU
+ * (held until the process ends) instead of the model's own buffer; a model
+ * whose staging ((16 n_q4 + 9 n_t3 + 4 n_bar) x 36 doubles) exceeds it is
+ * rejected with SOL105_E_INVALID.  K lands in the model's device buffer (as
+ * sol105_assemble_K with vals = NULL) and, when d_vals is not NULL, is copied
+ * to d_vals, a DEVICE pointer to n_blocks x 36 doubles; the call then
+ * returns after the copy.  The values are those of sol105_assemble_K, bit for
+ * bit.  All calls on one GPU share its array: calls that overlap on one GPU
+ * race. */
+#define SOL105_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int sol105_assemble_K_noalloc(sol105_ctx *ctx, sol105_model *model, double *d_vals);
+
 /* Static prestress solve: AUTOSPC, sparse Cholesky (device), solve.
  * Replaces Solver.solve_case / apply_bc_and_solve
  * (JFEM/src/solver/solve_case.jl:845-871, boundary_conditions.jl:530-1012).

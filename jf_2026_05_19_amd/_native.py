@@ -22,7 +22,7 @@ SOL105_E_PENDING = -7
 EXPORTS = [
     "sol105_ctx_create", "sol105_ctx_destroy", "sol105_last_error", "sol105_model_upload", "sol105_model_free",
     "sol105_model_set_q4_thickness", "sol105_model_set_shell_thickness", "sol105_pattern", "sol105_assemble_K",
-    "sol105_static_solve",
+    "sol105_assemble_K_noalloc", "sol105_static_solve",
     "sol105_assemble_KG", "sol105_buckling", "sol105_solve", "sol105_sensitivity_thickness",
     "sol105_sensitivity_terms", "sol105_profile_enable", "sol105_profile_read", "sol105_last_timings",
     "sol105_last_stats", "sol105_batch_create", "sol105_batch_free", "sol105_batch_solve", "sol105_batch_modes",
@@ -81,6 +81,7 @@ def lib():
         L.sol105_pattern.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
                                      ctypes.POINTER(ctypes.c_int32)]
         L.sol105_assemble_K.argtypes = [vp, vp, dp]
+        L.sol105_assemble_K_noalloc.argtypes = [vp, vp, ctypes.c_void_p]
         L.sol105_static_solve.argtypes = [vp, vp, dp, ctypes.POINTER(ctypes.c_uint8)]
         L.sol105_assemble_KG.argtypes = [vp, vp, dp, dp]
         L.sol105_buckling.argtypes = [vp, vp, ctypes.c_int32, dp, dp, ctypes.POINTER(ctypes.c_int32)]

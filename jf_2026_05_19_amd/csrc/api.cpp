@@ -286,6 +286,9 @@ int sol105_ctx_create(int device, sol105_ctx **out) {
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess) return SOL105_E_DEVICE;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return SOL105_E_DEVICE;
+    // device address of the no-allocation scratch array, once per device; a
+    // failure is kept and reported by sol105_assemble_K_noalloc
+    (void)noalloc_setup(device);
     for (auto &e : ctx->ev)
         if (hipEventCreate(&e) != hipSuccess) return SOL105_E_DEVICE;
     *out = ctx.release();

@@ -136,5 +136,8 @@ int model_init(sol105_ctx *ctx, const sol105_model_desc *d, sol105_model *M);
 // AUTOSPC + symbolic (when the free set changed) + numeric factorisation with
 // the pivot-ratio refix
 int factorize(sol105_ctx *ctx, sol105_model *M);
+// device address of the no-allocation scratch array on `device` (current),
+// looked up once per device (noalloc.hip)
+hipError_t noalloc_setup(int device);
 
 }  // namespace s105
