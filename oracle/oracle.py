@@ -140,6 +140,15 @@ def q4_membrane_force_field(lc, u, Cm):
     return out.reshape(4, 3)
 
 
+def q4_condensation_map(lc, Cm):
+    """A = -Kbb^-1 Kab^T (4 x 24): the condensed Wilson-mode amplitudes
+    [ux (1-xi^2), uy (1-xi^2), ux (1-eta^2), uy (1-eta^2)] per unit nodal
+    displacement (FEMKernels.jl:6708-6785)."""
+    out = np.zeros(96)
+    lib().or_q4_condensation_map(_p(_c(lc, float)), _p(_c(Cm, float)), _p(out))
+    return out.reshape(4, 24)
+
+
 def t3_frame(p):
     R = np.zeros(9); lc = np.zeros(6)
     lib().or_t3_frame(_p(_c(p, float)), _p(R), _p(lc))
@@ -295,22 +304,7 @@ def buckling(arr, K, KG, free, krylov_tol=1e-13):
     raw_lam = lam.copy()
     pos = np.flatnonzero(lam > 1e-10)
     order = pos[np.argsort(lam[pos], kind="stable")]
-    dropped = []
-    if arr.n_shell >= 100 and len(order):
-        limit = min(len(order), max(nd + 16, nd))
-        surv = []
-        full = np.zeros(arr.ndof)
-        for k in order[:limit]:
-            full[:] = 0.0
-            full[free] = V[:, k]
-            t2 = (full.reshape(-1, 6)[:, :3] ** 2).sum(axis=1)
-            e = np.concatenate([t2[arr.q4].sum(axis=1) / 4, t2[arr.t3].sum(axis=1) / 3])
-            tot = e.sum()
-            if tot > 0 and e.max() / tot > 0.10:
-                dropped.append(int(k))
-                continue
-            surv.append(int(k))
-        order = np.array(surv + list(order[limit:]), dtype=int)
+    order, dropped = localization_filter(arr, free, V, order, nd, arr.n_shell)
     n_out = min(req, len(order))
     sel = order[:n_out]
     modes = np.zeros((arr.ndof, n_out))
@@ -321,6 +315,34 @@ def buckling(arr, K, KG, free, krylov_tol=1e-13):
             modes[:, m] /= mx
     return dict(eigenvalues=lam[sel], modes=modes, raw_eigenvalues=raw_lam, dense=dense,
                 localization_dropped=len(dropped), requested=req)
+
+
+def localization_filter(arr, free, V, order, num_modes, n_shell_cards, max_share=0.10, min_elements=100,
+                        scan_buffer=16):
+    """Default localization filter of the buckling modes
+    (solve_case.jl:2582-2608, 2670-2850): when the model has at least
+    ``min_elements`` shell cards, each of the first max(num_modes + 16,
+    num_modes) candidates (ascending lambda) gets per-shell energies
+    e = mean over the shell's nodes of |translation|^2 (free-DOF vector
+    expanded with zeros) and is dropped when max(e) / sum(e) > max_share.
+    Returns (surviving order, dropped indices)."""
+    order = np.asarray(order, dtype=int)
+    if n_shell_cards < min_elements or len(order) == 0 or len(free) == 0:
+        return order, []
+    limit = min(len(order), max(num_modes + scan_buffer, num_modes))
+    surv, dropped = [], []
+    full = np.zeros(arr.ndof)
+    for k in order[:limit]:
+        full[:] = 0.0
+        full[free] = V[:, k]
+        t2 = (full.reshape(-1, 6)[:, :3] ** 2).sum(axis=1)
+        e = np.concatenate([t2[arr.q4].sum(axis=1) / 4, t2[arr.t3].sum(axis=1) / 3])
+        tot = e.sum()
+        if tot > 0 and e.max() / tot > max_share:
+            dropped.append(int(k))
+            continue
+        surv.append(int(k))
+    return np.array(surv + list(order[limit:]), dtype=int), dropped
 
 
 def solve(arr):
