@@ -67,6 +67,18 @@ int sol105_ctx_create(int device, sol105_ctx **out);
 int sol105_ctx_destroy(sol105_ctx *ctx);
 const char *sol105_last_error(const sol105_ctx *ctx);
 
+/* Eigensolver limits of a context, the counterparts of the reference's
+ * KrylovKit krylovdim and maxiter (JFEM/src/solver/solve_case.jl:1847-1861):
+ *   SOL105_OPT_LANCZOS_MAX_VECTORS   basis vectors before a thick restart,
+ *                                    32 .. 1024 (default 1024; rounded down
+ *                                    to a multiple of the block size);
+ *   SOL105_OPT_LANCZOS_MAX_RESTARTS  thick restarts before SOL105_E_NOCONV,
+ *                                    0 .. 1000 (default 40).
+ * SOL105_E_INVALID for an unknown option or a value out of range. */
+#define SOL105_OPT_LANCZOS_MAX_VECTORS 1
+#define SOL105_OPT_LANCZOS_MAX_RESTARTS 2
+int sol105_ctx_set_option(sol105_ctx *ctx, int32_t option, double value);
+
 /* Upload a model, build the BSR pattern, the deterministic element->block
  * scatter map and the fill-reducing ordering.  Replaces the indexing part of
  * Solver.assemble_stiffness (JFEM/src/solver/assembly.jl:1957-1989). */

@@ -20,7 +20,7 @@ ERRORS = {-1: "SOL105_E_INVALID", -2: "SOL105_E_UNSUPPORTED", -3: "SOL105_E_DEVI
 SOL105_E_PENDING = -7
 
 EXPORTS = [
-    "sol105_ctx_create", "sol105_ctx_destroy", "sol105_last_error", "sol105_model_upload", "sol105_model_free",
+    "sol105_ctx_create", "sol105_ctx_destroy", "sol105_last_error", "sol105_ctx_set_option", "sol105_model_upload", "sol105_model_free",
     "sol105_model_set_q4_thickness", "sol105_model_set_shell_thickness", "sol105_pattern", "sol105_assemble_K",
     "sol105_assemble_K_noalloc", "sol105_static_solve",
     "sol105_assemble_KG", "sol105_buckling", "sol105_solve", "sol105_sensitivity_thickness",
@@ -74,6 +74,7 @@ def lib():
         L.sol105_ctx_destroy.argtypes = [vp]
         L.sol105_last_error.argtypes = [vp]
         L.sol105_last_error.restype = ctypes.c_char_p
+        L.sol105_ctx_set_option.argtypes = [vp, ctypes.c_int32, ctypes.c_double]
         L.sol105_model_upload.argtypes = [vp, ctypes.POINTER(ModelDesc), ctypes.POINTER(vp)]
         L.sol105_model_free.argtypes = [vp, vp]
         L.sol105_model_set_q4_thickness.argtypes = [vp, vp, dp]
@@ -142,6 +143,14 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    OPT_LANCZOS_MAX_VECTORS = 1
+    OPT_LANCZOS_MAX_RESTARTS = 2
+
+    def set_option(self, option, value):
+        """Eigensolver limits (SOL105_OPT_*): basis size before a thick
+        restart, thick restarts before SOL105_E_NOCONV."""
+        self.check(self.L.sol105_ctx_set_option(self.h, int(option), float(value)))
 
     def profile_enable(self, on=True):
         self.check(self.L.sol105_profile_enable(self.h, 1 if on else 0))

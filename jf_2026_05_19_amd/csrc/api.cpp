@@ -308,6 +308,26 @@ int sol105_ctx_destroy(sol105_ctx *ctx) {
 
 const char *sol105_last_error(const sol105_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int sol105_ctx_set_option(sol105_ctx *ctx, int32_t option, double value) {
+    if (!ctx) return SOL105_E_INVALID;
+    if (!(value == std::floor(value)))
+        return set_err(ctx, SOL105_E_INVALID, "option %d: value %g is not an integer", option, value);
+    switch (option) {
+        case SOL105_OPT_LANCZOS_MAX_VECTORS:
+            if (value < 32 || value > 1024)
+                return set_err(ctx, SOL105_E_INVALID, "Lanczos basis size %g out of range 32..1024", value);
+            ctx->lanczos_max_vectors = (int)value;
+            return SOL105_OK;
+        case SOL105_OPT_LANCZOS_MAX_RESTARTS:
+            if (value < 0 || value > 1000)
+                return set_err(ctx, SOL105_E_INVALID, "Lanczos restarts %g out of range 0..1000", value);
+            ctx->lanczos_max_restarts = (int)value;
+            return SOL105_OK;
+        default:
+            return set_err(ctx, SOL105_E_INVALID, "unknown option %d", option);
+    }
+}
+
 int sol105_model_upload(sol105_ctx *ctx, const sol105_model_desc *d, sol105_model **out) {
     if (!ctx || !d || !out) return set_err(ctx, SOL105_E_INVALID, "null argument");
     *out = nullptr;
@@ -385,7 +405,152 @@ int sol105_assemble_K(sol105_ctx *ctx, sol105_model *M, double *vals) {
 
 namespace s105 {
 
-int factorize(sol105_ctx *ctx, sol105_model *M) {
+namespace {
+
+// One numeric factorisation of K_ff + shift I on the free set fm (a new
+// symbolic analysis when fm changed).  herr[0] != 0: a pivot was not
+// positive (the reference's cholesky throws); with shift == 0, herr[1]
+// counts the pivots with d_i / K_ii < 1e-7 (flags per elimination index) and
+// the explicit-inverse solve operators are formed.
+int factor_free_set(sol105_ctx *ctx, sol105_model *M, const std::vector<uint8_t> &fm, double shift, int herr[4]) {
+    hipStream_t s = ctx->stream;
+    if (!M->have_sym || fm != M->free_mask) {
+        M->free_mask = fm;
+        M->factor_valid = false;
+        try {
+            symbolic_analyze(M->pat, M->xyz.data(), M->free_mask.data(), 32, M->sym);
+        } catch (const std::exception &ex) {
+            return set_err(ctx, SOL105_E_INVALID, "symbolic analysis failed: %s", ex.what());
+        }
+        if (M->sym.n_free == 0) return set_err(ctx, SOL105_E_SINGULAR, "no free DOFs");
+        int rc = upload_symbolic(ctx, M);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemsetAsync(M->d_err.p, 0, sizeof(int) * 4, s));
+    const double *kv = M->d_K.p;
+    if (shift != 0.0) {
+        const size_t nv = M->pat.n_blocks() * 36;
+        HIPCHK(M->d_Kshift.alloc(nv));
+        HIPCHK(hipMemcpyAsync(M->d_Kshift.p, M->d_K.p, nv * sizeof(double), hipMemcpyDeviceToDevice, s));
+        launch_add_diag(M->n_nodes, M->d_diag.p, shift, M->d_Kshift.p, s);
+        kv = M->d_Kshift.p;
+    }
+    factor_numeric(M->plan, M->dev, kv, M->d_front.p, M->d_piv.p, M->d_err.p, s);
+    if (shift == 0.0) {
+        // post-factorisation singular-pivot check: d_i / K_ii < 1e-7
+        launch_kdiag_elim(M->sym.n_free, M->d_dof_of_elim.p, M->d_diag.p, M->d_K.p, M->d_kdiag_elim.p, s);
+        launch_pivot_ratio(M->sym.n_free, M->d_piv.p, M->d_kdiag_elim.p, M->d_err.p + 1, M->d_flag.p, s);
+        solve_prepare(M->plan, M->dev, M->dsolve, M->splan, M->d_front.p, M->d_P.p, M->d_PT.p, s);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(herr, M->d_err.p, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SOL105_OK;
+}
+
+// Largest |entry| of the element stiffness matrices before assembly
+// (max_elem_stiff, JFEM/src/solver/assembly.jl:4352-4356): the element
+// kernels run again into the model's staging buffer.
+int max_elem_stiff(sol105_ctx *ctx, sol105_model *M, double &out) {
+    hipStream_t s = ctx->stream;
+    ElemArgs a = elem_args(M);
+    launch_q4_ke(a, M->d_eb.p, s);
+    launch_t3_ke(a, M->d_eb.p, s);
+    launch_bar_ke(a, M->d_eb.p, s);
+    const int64_t neb = ((int64_t)M->n_q4 * 16 + (int64_t)M->n_t3 * 9 + (int64_t)M->n_bar * 4) * 36;
+    HIPCHK(M->d_Kshift.alloc(std::max<size_t>(M->pat.n_blocks() * 36, 1)));
+    launch_maxabs(neb, M->d_eb.p, M->d_Kshift.p, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&out, M->d_Kshift.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SOL105_OK;
+}
+
+// DOFs (by DOF index) of the last unshifted factorisation whose pivot is
+// zero to working precision, |d_i / K_ii| < 1e-7 (non-positive pivots of a
+// failed factorisation included only when they are that small).
+int near_zero_pivots(sol105_ctx *ctx, sol105_model *M, std::vector<int32_t> &dofs) {
+    const int nf = M->sym.n_free;
+    std::vector<double> piv(nf), kd(nf);
+    HIPCHK(hipMemcpy(piv.data(), M->d_piv.p, nf * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(kd.data(), M->d_kdiag_elim.p, nf * sizeof(double), hipMemcpyDeviceToHost));
+    dofs.clear();
+    for (int i = 0; i < nf; i++) {
+        const double k = std::fabs(kd[i]);
+        if (k > 1e-30 && std::fabs(piv[i]) < 1e-7 * k) dofs.push_back(M->sym.dof_of_elim[i]);
+    }
+    return SOL105_OK;
+}
+
+// Factorisation AUTOSPC after a failed Cholesky (boundary_conditions.jl:
+// 787-881): probe K_ff + shift I with shift = max(max_elem_stiff, 1) 10^e,
+// e = -12, -10, -8, -6, until a probe factors; fix the DOFs whose pivot
+// L_ii is below min(3 sqrt(shift), 1e-4 median(L_ii)) unless they are more
+// than half of the free set; then factor K_ff once more.  Where that still
+// fails the reference turns to an LU factorisation of K_ff (and, for the
+// buckling solve, to a dense generalized eigensolver); here the DOFs whose
+// pivots came out zero to working precision in the failed factorisation
+// (`zero_piv`, the refix rule on |d_i / K_ii|) are fixed instead and K_ff is
+// factored once more.  On exact mechanisms (zero-energy patterns with no
+// load and no geometric stiffness) that gives the reference's translations
+// and buckling loads; an indefinite K_ff (large negative pivots) stays
+// SOL105_E_SINGULAR.
+int factor_autospc(sol105_ctx *ctx, sol105_model *M, std::vector<uint8_t> &fm,
+                   const std::vector<int32_t> &zero_piv) {
+    double mes = 0.0;
+    int rc = max_elem_stiff(ctx, M, mes);
+    if (rc) return rc;
+    bool found = false;
+    int herr[4];
+    for (int e : {-12, -10, -8, -6}) {
+        const double shift = std::max(mes, 1.0) * std::pow(10.0, (double)e);
+        rc = factor_free_set(ctx, M, fm, shift, herr);
+        if (rc) return rc;
+        if (herr[0]) continue;
+        const int nf = M->sym.n_free;
+        std::vector<double> piv(nf), L(nf);
+        HIPCHK(hipMemcpy(piv.data(), M->d_piv.p, nf * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < nf; i++) L[i] = std::sqrt(std::fabs(piv[i]));
+        std::vector<double> srt(L);
+        std::sort(srt.begin(), srt.end());
+        const double med = (nf % 2) ? srt[nf / 2] : 0.5 * (srt[nf / 2 - 1] + srt[nf / 2]);
+        const double thr = std::min(3.0 * std::sqrt(shift), 1e-4 * med);
+        int n_mech = 0;
+        for (int i = 0; i < nf; i++) n_mech += L[i] < thr;
+        found = true;
+        if (n_mech > 0.5 * nf) break;  // threshold too aggressive for this shift: nothing fixed
+        for (int i = 0; i < nf; i++)
+            if (L[i] < thr) fm[M->sym.dof_of_elim[i]] = 0;
+        break;
+    }
+    if (found) {
+        rc = factor_free_set(ctx, M, fm, 0.0, herr);
+        if (rc) return rc;
+        if (herr[0] == 0) {
+            M->factor_valid = true;
+            return SOL105_OK;
+        }
+    }
+    int n_fix = 0, n_free = 0;
+    for (int32_t d : zero_piv)
+        if (fm[d]) { fm[d] = 0; n_fix++; }
+    for (uint8_t f : fm) n_free += f;
+    if (n_fix > 0 && n_fix <= n_free) {
+        rc = factor_free_set(ctx, M, fm, 0.0, herr);
+        if (rc) return rc;
+        if (herr[0] == 0) {
+            M->factor_valid = true;
+            return SOL105_OK;
+        }
+    }
+    return set_err(ctx, SOL105_E_SINGULAR,
+                   "stiffness matrix is not positive definite on the free set after the factorisation AUTOSPC "
+                   "(no zero pivots to fix: K_ff is indefinite)");
+}
+
+}  // namespace
+
+int factorize(sol105_ctx *ctx, sol105_model *M, bool shifted_fallback) {
     hipStream_t s = ctx->stream;
     int ndof = M->ndof();
     // AUTOSPC from the assembled diagonal
@@ -395,43 +560,33 @@ int factorize(sol105_ctx *ctx, sol105_model *M) {
     HIPCHK(hipStreamSynchronize(s));
     std::vector<uint8_t> fm;
     autospc_mask(M, kd, fm);
-    for (int attempt = 0; attempt < 2; attempt++) {
-        if (!M->have_sym || fm != M->free_mask) {
-            M->free_mask = fm;
-            try {
-                symbolic_analyze(M->pat, M->xyz.data(), M->free_mask.data(), 32, M->sym);
-            } catch (const std::exception &ex) {
-                return set_err(ctx, SOL105_E_INVALID, "symbolic analysis failed: %s", ex.what());
-            }
-            if (M->sym.n_free == 0) return set_err(ctx, SOL105_E_SINGULAR, "no free DOFs");
-            int rc = upload_symbolic(ctx, M);
-            if (rc) return rc;
-        }
-        HIPCHK(hipMemsetAsync(M->d_err.p, 0, sizeof(int) * 4, s));
-        factor_numeric(M->plan, M->dev, M->d_K.p, M->d_front.p, M->d_piv.p, M->d_err.p, s);
-        // post-factorisation singular-pivot check: d_i / K_ii < 1e-7
-        launch_kdiag_elim(M->sym.n_free, M->d_dof_of_elim.p, M->d_diag.p, M->d_K.p, M->d_kdiag_elim.p, s);
-        launch_pivot_ratio(M->sym.n_free, M->d_piv.p, M->d_kdiag_elim.p, M->d_err.p + 1, M->d_flag.p, s);
-        // explicit-inverse solve operators (only used when the factor is accepted)
-        solve_prepare(M->plan, M->dev, M->dsolve, M->splan, M->d_front.p, M->d_P.p, M->d_PT.p, s);
-        HIPCHK(hipGetLastError());
-        int herr[4];
-        HIPCHK(hipMemcpyAsync(herr, M->d_err.p, sizeof(herr), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        if (herr[1] == 0 && herr[0] == 0) {
+    int herr[4];
+    int rc = factor_free_set(ctx, M, fm, 0.0, herr);
+    if (rc) return rc;
+    if (herr[0] == 0) {
+        if (herr[1] == 0) {
             M->factor_valid = true;
             return SOL105_OK;
         }
-        if (attempt == 1) break;
-        // fix the flagged DOFs (reference: boundary_conditions.jl:742-782) and refactor
+        // singular-pivot refix (boundary_conditions.jl:742-782): fix the
+        // flagged DOFs and factor once more; the ratios are not checked again
         std::vector<uint8_t> flag(M->sym.n_free);
         HIPCHK(hipMemcpy(flag.data(), M->d_flag.p, flag.size(), hipMemcpyDeviceToHost));
-        int nfix = 0;
         for (int e = 0; e < M->sym.n_free; e++)
-            if (flag[e]) { fm[M->sym.dof_of_elim[e]] = 0; nfix++; }
-        if (nfix == 0) break;
+            if (flag[e]) fm[M->sym.dof_of_elim[e]] = 0;
+        rc = factor_free_set(ctx, M, fm, 0.0, herr);
+        if (rc) return rc;
+        if (herr[0] == 0) {
+            M->factor_valid = true;
+            return SOL105_OK;
+        }
     }
-    return set_err(ctx, SOL105_E_SINGULAR, "stiffness matrix is not positive definite on the free set");
+    if (!shifted_fallback)
+        return set_err(ctx, SOL105_E_SINGULAR, "stiffness matrix is not positive definite on the free set");
+    std::vector<int32_t> zero_piv;
+    rc = near_zero_pivots(ctx, M, zero_piv);
+    if (rc) return rc;
+    return factor_autospc(ctx, M, fm, zero_piv);
 }
 
 }  // namespace s105
@@ -447,7 +602,7 @@ int sol105_static_solve(sol105_ctx *ctx, sol105_model *M, double *u, uint8_t *fr
         if (rc) return rc;
     }
     if (!M->factor_valid) {
-        int rc = factorize(ctx, M);
+        int rc = factorize(ctx, M, true);
         if (rc) return rc;
     }
     int nfree = M->sym.n_free, ndof = M->ndof();
@@ -565,7 +720,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     int n = M->sym.n_free, ndof = M->ndof();
     const bool debug = std::getenv("SOL105_DEBUG") != nullptr;
     int b = want <= 8 ? 8 : 16;
-    int mmax = std::min(1024, (n / b) * b);
+    int mmax = std::min((ctx->lanczos_max_vectors / b) * b, (n / b) * b);
     if (mmax < b) return set_err(ctx, SOL105_E_NOCONV, "free set too small for the eigensolver (%d)", n);
     int guard = 2;
     int kneed = std::min(want + guard, mmax);
@@ -574,7 +729,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     // thick restart at mmax vectors: keep the kk Ritz vectors of largest
     // |theta| and the next block
     const int kk = mmax >= 128 + b ? (SYMEIG_MAX_K / b) * b : std::max(b, (mmax / 2 / b) * b);
-    const int max_restarts = 40;
+    const int max_restarts = ctx->lanczos_max_restarts;
     const int kz = std::max(kneed, kk);
     LanczosWork &W = M->lw;
     DevBuf<double> &V = W.V, &Wb = W.Wb, &xf = W.xf, &yf = W.yf, &T = W.T, &A = W.A, &Zs = W.Zs, &thd = W.thd,
@@ -957,7 +1112,7 @@ int sol105_solve(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double *la
     int rc = sol105_assemble_K(ctx, M, nullptr);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->ev[1], s));
-    rc = factorize(ctx, M);
+    rc = factorize(ctx, M, true);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->ev[3], s));
     rc = sol105_static_solve(ctx, M, nullptr, nullptr);

@@ -613,7 +613,7 @@ int sol105_batch_solve(sol105_ctx *ctx, sol105_batch *B, int32_t max_modes, doub
     HIPCHK(hipEventRecord(ctx->ev[7], s));  // events 0-6 belong to sol105_solve (member fallbacks)
     sol105_model *M = &B->M;
     int rc = sol105_assemble_K(ctx, M, nullptr);
-    if (rc == SOL105_OK) rc = factorize(ctx, M);
+    if (rc == SOL105_OK) rc = factorize(ctx, M, false);  // a failed stack: members re-solved alone
     if (rc == SOL105_OK) rc = sol105_static_solve(ctx, M, nullptr, nullptr);
     if (rc == SOL105_OK) rc = sol105_assemble_KG(ctx, M, nullptr, nullptr);
     if (rc == SOL105_OK) {

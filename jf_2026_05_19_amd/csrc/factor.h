@@ -35,3 +35,7 @@ void launch_gather_elim_il(int n_free, int nrhs, const int32_t *dof_of_elim, con
 void launch_kdiag_elim(int n_free, const int32_t *dof_of_elim, const int32_t *diag_blk, const double *kv, double *out,
                        hipStream_t s);
 void launch_pivot_ratio(int n, const double *piv, const double *kdiag_elim, int *count, uint8_t *flag, hipStream_t s);
+// kv (BSR values) += shift on every diagonal entry
+void launch_add_diag(int n_nodes, const int32_t *diag_blk, double shift, double *kv, hipStream_t s);
+// *out = max |x_i| (device scalar)
+void launch_maxabs(int64_t n, const double *x, double *out, hipStream_t s);

@@ -456,9 +456,45 @@ __global__ void kdiag_elim_kernel(int n_free, const int32_t *dof_of_elim, const 
     out[e] = kv[(int64_t)diag_blk[d / 6] * 36 + c * 6 + c];
 }
 
+// K + shift I on the 6x6 diagonal blocks (the shifted probes of the
+// factorisation AUTOSPC, boundary_conditions.jl:795-801)
+__global__ void add_diag_kernel(int n_nodes, const int32_t *diag_blk, double shift, double *kv) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 6 * n_nodes) return;
+    const int v = t / 6, c = t % 6;
+    kv[(int64_t)diag_blk[v] * 36 + c * 7] += shift;
+}
+
+// max |x| into *out (initialised to +0 by the caller); the bit patterns of
+// non-negative doubles order like the values, so an integer atomic max works
+__global__ void maxabs_kernel(int64_t n, const double *x, unsigned long long *out) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(x[i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
+}
+
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
+
+void launch_add_diag(int n_nodes, const int32_t *diag_blk, double shift, double *kv, hipStream_t s) {
+    if (n_nodes > 0) add_diag_kernel<<<nblk(6 * (int64_t)n_nodes, 256), 256, 0, s>>>(n_nodes, diag_blk, shift, kv);
+}
+
+void launch_maxabs(int64_t n, const double *x, double *out, hipStream_t s) {
+    dev_fill(out, 1, 0.0, s);
+    if (n <= 0) return;
+    int64_t blocks = std::min<int64_t>((n + 255) / 256, 2048);
+    maxabs_kernel<<<(unsigned)blocks, 256, 0, s>>>(n, x, reinterpret_cast<unsigned long long *>(out));
+}
 
 void dev_fill(double *p, int64_t n, double v, hipStream_t s) {
     if (n <= 0) return;

@@ -26,6 +26,8 @@ struct sol105_ctx {
     double stats[8] = {0};
     hipEvent_t ev[9] = {nullptr};
     bool profiling = false;
+    int lanczos_max_vectors = 1024;  // SOL105_OPT_LANCZOS_MAX_VECTORS
+    int lanczos_max_restarts = 40;   // SOL105_OPT_LANCZOS_MAX_RESTARTS
     ProdTimer ptimer;  // HIP-event timing of the block triangular-solve products
     ProdTimer *timer() { return profiling ? &ptimer : nullptr; }
 };
@@ -80,6 +82,7 @@ struct sol105_model {
     s105::DevBuf<double> d_xyz, d_q4prop, d_t3prop, d_bar_vref, d_barprop, d_force;
     s105::DevBuf<int32_t> d_q4, d_t3, d_bar, d_rp, d_col, d_cptr, d_cidx, d_diag;
     s105::DevBuf<double> d_eb, d_K, d_KG, d_u, d_kdiag;
+    s105::DevBuf<double> d_Kshift;  // K + shift I of the factorisation AUTOSPC probes (allocated on first use)
     // factor
     bool have_sym = false;
     std::vector<uint8_t> free_mask;
@@ -134,8 +137,9 @@ double ev_ms(hipEvent_t a, hipEvent_t b);
 // host copies, validation, block pattern and device upload of a model
 int model_init(sol105_ctx *ctx, const sol105_model_desc *d, sol105_model *M);
 // AUTOSPC + symbolic (when the free set changed) + numeric factorisation with
-// the pivot-ratio refix
-int factorize(sol105_ctx *ctx, sol105_model *M);
+// the pivot-ratio refix and, when shifted_fallback is set, the factorisation
+// AUTOSPC of a failed Cholesky (shifted probes)
+int factorize(sol105_ctx *ctx, sol105_model *M, bool shifted_fallback);
 // device address of the no-allocation scratch array on `device` (current),
 // looked up once per device (noalloc.hip)
 hipError_t noalloc_setup(int device);

@@ -9,8 +9,11 @@ restatement, see its header for the reference lines).  The solves use SciPy
 as a textbook reference for what the reference's third-party libraries
 compute on the same matrices:
 
-* static solve: sparse LU (SuperLU) of ``K_ff`` -- the reference factors the
-  same matrix with CHOLMOD (``JFEM/src/solver/boundary_conditions.jl:740``);
+* static solve: the reference's Cholesky path with its singular-pivot refix
+  and factorisation AUTOSPC (``JFEM/src/solver/boundary_conditions.jl:
+  734-881``) on a dense Cholesky in ascending free-DOF order, for up to
+  ``DENSE_CHOLESKY_MAX`` free DOFs; above that, sparse LU (SuperLU) of
+  ``K_ff`` on the AUTOSPC free set (the reference factors it with CHOLMOD);
 * buckling, ``n_free <= 4000``: dense symmetric-definite ``eigh(-KG, K)``,
   i.e. the reference's ``C = L^-1 (-KG) L^-T`` route
   (``JFEM/src/solver/solve_case.jl:2131-2196``);
@@ -205,7 +208,9 @@ def bar_ke_global(p1, p2, vref, bp):
 # assembly
 # --------------------------------------------------------------------------
 
-def assemble_K(arr):
+def assemble_K(arr, with_max_elem_stiff=False):
+    """K (CSR).  With ``with_max_elem_stiff``: (K, largest |entry| of the
+    element matrices before assembly, assembly.jl:4352-4356)."""
     n = 576 * arr.n_q4 + 324 * arr.n_t3 + 144 * arr.n_bar
     I = np.zeros(n, np.int64); J = np.zeros(n, np.int64); V = np.zeros(n)
     q4 = _c(arr.q4, np.int32); t3 = _c(arr.t3, np.int32); bar = _c(arr.bar, np.int32)
@@ -214,7 +219,10 @@ def assemble_K(arr):
                               arr.n_bar, _p(bar, _ip), _p(_c(arr.bar_vref, float)),
                               _p(_c(arr.barprop, float)), _p(I, _lp), _p(J, _lp), _p(V))
     assert cnt == n
-    return sp.csr_matrix((V, (I, J)), shape=(arr.ndof, arr.ndof))
+    K = sp.csr_matrix((V, (I, J)), shape=(arr.ndof, arr.ndof))
+    if with_max_elem_stiff:
+        return K, float(np.abs(V).max(initial=0.0))
+    return K
 
 
 def assemble_KG(arr, u):
@@ -262,6 +270,129 @@ def static_solve(arr, K):
     lu = spla.splu(Kff)
     u = np.zeros(arr.ndof)
     u[free] = lu.solve(arr.F[free])
+    return u, free
+
+
+class SingularModel(RuntimeError):
+    """K_ff stays indefinite after the factorisation AUTOSPC (the reference
+    would go on with an LU factorisation, which is not restated here)."""
+
+
+def _chol(A):
+    try:
+        return sla.cholesky(A, lower=True, check_finite=False)
+    except (sla.LinAlgError, ValueError):
+        return None
+
+
+def _chol_pivots(A):
+    """Right-looking Cholesky that runs to the end: a non-positive pivot is
+    taken as 1 with its column left unscaled, as the device factorisation
+    does.  Returns the pivots d_i (before the square root)."""
+    A = np.array(A, dtype=float, copy=True)
+    n = len(A)
+    d = np.empty(n)
+    for j in range(n):
+        dj = A[j, j]
+        d[j] = dj
+        sd = np.sqrt(dj) if dj > 0 else 1.0
+        col = A[j + 1:, j] / sd
+        A[j + 1:, j + 1:] -= np.outer(col, col)
+    return d
+
+
+DENSE_CHOLESKY_MAX = 6000
+
+
+def static_solve_cholesky(arr, K, max_elem_stiff, info=None):
+    """Static solve along the reference's Cholesky path, dense and in
+    ascending free-DOF order (no fill-reducing permutation), for
+    n_free <= DENSE_CHOLESKY_MAX (boundary_conditions.jl:734-881):
+
+    * AUTOSPC, then L = chol(K_ff);
+    * singular-pivot refix: DOFs with L_ii^2 / |K_ii| < 1e-7 are fixed and
+      K_ff is factored once more (ratios not checked again).  The reference
+      divides the i-th pivot of CHOLMOD's permuted factor by the i-th
+      diagonal entry in unpermuted order; with no permutation both are DOF i;
+    * a Cholesky that fails (here or after the refix) starts the
+      factorisation AUTOSPC: shifted probes K_ff + s I, s =
+      max(max_elem_stiff, 1) 10^e for e = -12, -10, -8, -6; the first probe
+      that factors fixes the DOFs with L_ii < min(3 sqrt(s), 1e-4
+      median(L_ii)) unless they are more than half of the free set; then a
+      clean Cholesky;
+    * where that still fails the reference goes on with an LU factorisation
+      (boundary_conditions.jl:862-879); instead, the DOFs whose pivots in the
+      failed unshifted factorisation are zero to working precision
+      (|d_i / K_ii| < 1e-7) are fixed and K_ff is factored once more, which
+      on exact mechanisms gives the same translations and buckling loads.
+      Failing that (an indefinite K_ff), SingularModel.
+
+    Returns (u, free).  ``info`` (a dict) receives the path taken and the
+    DOFs each step fixed."""
+    fixed = autospc(arr, K.diagonal())
+    free = np.flatnonzero(~fixed)
+    if len(free) > DENSE_CHOLESKY_MAX:
+        raise ValueError("static_solve_cholesky is dense: %d free DOFs" % len(free))
+    Kd = K.toarray()
+    Kd = 0.5 * (Kd + Kd.T)
+    info = {} if info is None else info
+    info.update(path="cholesky", refix=[], mechanism=[])
+
+    def kff(fr):
+        return Kd[np.ix_(fr, fr)]
+
+    def zero_pivots(fr):
+        A = kff(fr)
+        d, k = _chol_pivots(A), np.abs(np.diag(A))
+        return fr[(k > 1e-30) & (np.abs(d) < 1e-7 * k)]
+
+    L = _chol(kff(free))
+    zero_piv = None
+    if L is not None:
+        kdiag = np.abs(np.diag(kff(free)))
+        ratio = np.where(kdiag > 1e-30, np.diag(L) ** 2 / np.where(kdiag > 1e-30, kdiag, 1.0), 1.0)
+        sing = free[ratio < 1e-7]
+        if len(sing):
+            info["path"] = "refix"
+            info["refix"] = [int(d) for d in sing]
+            fixed[sing] = True
+            free = np.flatnonzero(~fixed)
+            L = _chol(kff(free))
+            if L is None:
+                zero_piv = zero_pivots(free)
+    else:
+        zero_piv = zero_pivots(free)
+    if L is None:
+        info["path"] = "factorization_autospc"
+        found = False
+        for e in (-12, -10, -8, -6):
+            shift = max(max_elem_stiff, 1.0) * 10.0 ** e
+            Ls = _chol(kff(free) + shift * np.eye(len(free)))
+            if Ls is None:
+                continue
+            ld = np.abs(np.diag(Ls))
+            small = ld < min(3.0 * np.sqrt(shift), float(np.median(ld)) * 1e-4)
+            found = True
+            if small.sum() > 0.5 * len(free):
+                break
+            if small.any():
+                info["mechanism"] = [int(d) for d in free[small]]
+                fixed[free[small]] = True
+                free = np.flatnonzero(~fixed)
+            break
+        L = _chol(kff(free)) if found else None
+        if L is None:
+            fix = np.array([d for d in zero_piv if not fixed[d]], dtype=int)
+            if len(fix):
+                info["path"] = "zero_pivots"
+                info["zero_pivots"] = [int(d) for d in fix]
+                fixed[fix] = True
+                free = np.flatnonzero(~fixed)
+                L = _chol(kff(free))
+        if L is None:
+            raise SingularModel("K_ff is not positive definite after the factorisation AUTOSPC")
+    u = np.zeros(arr.ndof)
+    u[free] = sla.cho_solve((L, True), arr.F[free], check_finite=False)
     return u, free
 
 
@@ -346,12 +477,21 @@ def localization_filter(arr, free, V, order, num_modes, n_shell_cards, max_share
 
 
 def solve(arr):
-    """Full forward SOL 105 path: K -> u -> KG -> (lambda, phi)."""
-    K = assemble_K(arr)
-    u, free = static_solve(arr, K)
+    """Full forward SOL 105 path: K -> u -> KG -> (lambda, phi).  The static
+    solve follows the reference's Cholesky path (refix, factorisation
+    AUTOSPC) densely up to DENSE_CHOLESKY_MAX free DOFs, above that a sparse
+    LU of the AUTOSPC free set."""
+    K, mes = assemble_K(arr, with_max_elem_stiff=True)
+    n_free = int(np.sum(~autospc(arr, K.diagonal())))
+    info = {}
+    if n_free <= DENSE_CHOLESKY_MAX:
+        u, free = static_solve_cholesky(arr, K, mes, info)
+    else:
+        u, free = static_solve(arr, K)
+        info["path"] = "sparse_lu"
     KG = assemble_KG(arr, u)
     res = buckling(arr, K, KG, free)
-    res.update(K=K, KG=KG, u=u, free=free)
+    res.update(K=K, KG=KG, u=u, free=free, static_info=info)
     return res
 
 
