@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--batch-variants", type=int, default=256,
                     help="seeded plate variants of the batched-vs-plain comparison (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--batch-cpu-sample", type=int, default=32,
+                    help="variants of the sweep also solved by the CPU oracle on all host cores (0: none)")
     ap.add_argument("--batch-only", action="store_true", help="only the batched-vs-plain sweep (for profiling)")
     ap.add_argument("--no-batch-plain", action="store_true", help="skip the one-at-a-time leg of the sweep")
     ap.add_argument("--cylinder-fd", type=int, default=8,
@@ -74,7 +76,8 @@ def main():
         from jf_2026_05_19_amd import _native as N
         print(json.dumps({"metric": METRIC, "batched": batched_sweep(N.Context(0), args.batch_variants,
                                                                      args.batch_steps,
-                                                                     plain=not args.no_batch_plain)}))
+                                                                     plain=not args.no_batch_plain,
+                                                                     cpu_sample=0)}))
         return
     t0 = time.perf_counter()
     txt = decks.panel_deck(nx=args.nx, ny=args.ny)
@@ -90,9 +93,9 @@ def main():
               "config": {"workload": workload, "nx": args.nx, "ny": args.ny, "ndof": arr.ndof, "nd": arr.nd,
                          "parallelism": "single GPU"}}
 
-    cpu = None
+    cpu, cpu_ref = None, None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(arr)
+        cpu, cpu_ref = cpu_baseline(arr)
         result["cpu_baseline"] = cpu
     if args.cpu_only:
         print(json.dumps(result))
@@ -111,7 +114,6 @@ def main():
         log("warmup %d: %.2f ms lambda1=%.6f" % (w, 1e3 * (time.perf_counter() - t1), lam[0] if len(lam) else -1))
     lat = []
     phase = np.zeros(8)
-    ctx.profile_enable(True)   # HIP events around the dominant kernel's launches
     t_start = time.perf_counter()
     for k in range(args.steps):
         t1 = time.perf_counter()
@@ -119,6 +121,12 @@ def main():
         lat.append(time.perf_counter() - t1)
         phase += ctx.timings()
     t_total = time.perf_counter() - t_start
+    # the roofline pass: the same solves again with HIP events around every
+    # launch of the dominant kernel (outside the timed region: the events
+    # themselves cost time between launches)
+    ctx.profile_enable(True)
+    for k in range(args.steps):
+        dm.solve(want_modes=False)
     prof = ctx.profile_read()
     ctx.profile_enable(False)
     phase /= max(args.steps, 1)
@@ -138,9 +146,12 @@ def main():
     })
     result["roofline"] = roofline(prof)
     result["factor_rate"] = factor_rate(ctx, stats)
+    if cpu_ref is not None:
+        result["accuracy_vs_cpu"] = accuracy_vs_cpu(dm, cpu_ref)
     dm.free()
     if args.batch_variants > 0:
-        result["batched"] = batched_sweep(ctx, args.batch_variants, args.batch_steps)
+        result["batched"] = batched_sweep(ctx, args.batch_variants, args.batch_steps,
+                                          cpu_sample=0 if args.no_cpu_baseline else args.batch_cpu_sample)
     if args.cylinder_fd > 0:
         result["cylinder"] = cylinder_case(ctx, args.cylinder_fd)
     if args.served_requests > 0:
@@ -150,31 +161,65 @@ def main():
     print(json.dumps(result))
 
 
-# HBM bytes per leaf-height launch of the dominant kernel from rocprofv3 --pmc
-# FETCH_SIZE (219.51 MB) + WRITE_SIZE (16.07 MB), each scaled by the same
-# counter's reading for a 256 MiB device copy (factors 2.0 and 1.0); the runs
-# and the per-grid figures are in profiles/prod_mfma_kernel_c2_panel_pmc.txt.
-PROD_TRAFFIC_BYTES = 235.58e6
+# HBM bytes per launch of the dominant kernel, averaged over all its launches
+# in a solve, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by
+# the same counter's reading for a 256 MiB device copy (factors 2.0 and 1.0);
+# the runs and the per-height figures are in
+# profiles/prod_mfma_kernel_c2_panel_pmc.txt.
+PROD_TRAFFIC_BYTES = None
+
+
+def _rate(launches, ms, nbytes):
+    if launches == 0 or ms <= 0:
+        return None
+    avg_s = ms * 1e-3 / launches
+    achieved = nbytes / launches / avg_s / 1e9
+    return {"launches": launches, "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": nbytes / launches,
+            "achieved": achieved, "frac": achieved / 8000.0}
 
 
 def roofline(prof):
     """Roofline of the dominant kernel, the block triangular-solve product
-    (prod_mfma_kernel): algorithmic bytes per launch (factor entries streamed
-    once + the right-hand-side rows read and written) over the average launch
-    duration, both from HIP events recorded on the solve stream around every
-    leaf-height launch (the two largest grids, forward and backward) of the
-    block Lanczos steps in the timed region."""
-    launches, ms, nbytes = prof
-    if launches == 0 or ms <= 0:
+    (prod_mfma_kernel), over ALL of its launches in the timed region (every
+    tree height, both solve directions, every block Lanczos step):
+    achieved = sum of algorithmic bytes (factor entries streamed once + the
+    right-hand-side rows read and written) / sum of launch durations, both
+    from HIP events recorded on the solve stream around each launch.  The
+    leaf height (the two largest grids) and the heights above it are also
+    given separately."""
+    la, ms, nb = prof["all"]
+    r = _rate(la, ms, nb)
+    if r is None:
         return None
-    avg_s = ms * 1e-3 / launches
-    per_launch = nbytes / launches
-    achieved = per_launch / avg_s / 1e9
     peak = 8000.0  # MI355X HBM3E GB/s
     return {"kernel": "prod_mfma_kernel (block triangular solve: FP64 MFMA product with the explicit factor inverse)",
-            "bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s", "frac": achieved / peak,
-            "traffic": PROD_TRAFFIC_BYTES, "algorithmic_bytes_per_launch": per_launch,
-            "avg_launch_us": avg_s * 1e6, "launches": launches}
+            "bound": "hbm", "achieved": r["achieved"], "peak": peak, "unit": "GB/s", "frac": r["achieved"] / peak,
+            "traffic": PROD_TRAFFIC_BYTES, "algorithmic_bytes_per_launch": r["algorithmic_bytes_per_launch"],
+            "avg_launch_us": r["avg_launch_us"], "launches": la, "total_ms": ms,
+            "leaf_height": _rate(*prof["leaf"]), "upper_heights": _rate(*prof["upper"])}
+
+
+def accuracy_vs_cpu(dm, ref):
+    """Eigenvalues and mode shapes of the device solve against the CPU
+    oracle's solve of the same deck (the cpu_baseline leg): largest relative
+    eigenvalue error and smallest MAC over modes of non-degenerate pairs."""
+    lam, modes = dm.solve()
+    lr, mr = ref["eigenvalues"], ref["modes"]
+    out = {"n_modes": [len(lam), len(lr)]}
+    if len(lam) != len(lr) or len(lam) == 0:
+        out["max_eig_rel_err_vs_cpu"] = float("inf")
+        return out
+    out["max_eig_rel_err_vs_cpu"] = float(np.max(np.abs(lam - lr) / np.abs(lr)))
+    macs = []
+    for k in range(len(lam)):
+        gap = min([abs(lr[k] - lr[j]) / abs(lr[k]) for j in range(len(lr)) if j != k] + [1.0])
+        if gap >= 1e-6:
+            a, b = modes[:, k], mr[:, k]
+            macs.append(float((a @ b) ** 2 / ((a @ a) * (b @ b))))
+    out["min_mac_vs_cpu"] = min(macs) if macs else None
+    log("accuracy vs CPU oracle: max eig rel err %.2e, min MAC %s" % (out["max_eig_rel_err_vs_cpu"],
+                                                                      out["min_mac_vs_cpu"]))
+    return out
 
 
 def factor_rate(ctx, stats):
@@ -191,7 +236,7 @@ def _parse_variant(txt):
     return M.build_arrays(bdf.bdf_to_model_text(txt))
 
 
-def batched_sweep(ctx, n, steps, plain=True):
+def batched_sweep(ctx, n, steps, plain=True, cpu_sample=32):
     """Design sweep of n seeded plate variants (mesh 10x10..80x80, t in
     U(1, 4) mm, E in {70e3, 210e3} MPa, ND = 5): every variant solved one
     after another (plain: one model per solve) and all of them as one batch
@@ -249,6 +294,7 @@ def batched_sweep(ctx, n, steps, plain=True):
             err = max(err, float(np.max(np.abs(lb - la) / np.abs(la))))
     lat_ms = np.array(lat) * 1e3
     log("batched: %d solves in %.3fs (%.1fx plain), max rel err vs plain %.2e" % (n, t_batch, t_plain / t_batch, err))
+    cpu = batched_cpu_sample(arrs, lam_b, cpu_sample) if cpu_sample > 0 else None
     return {"workload": "%d seeded plate variants 500x500 mm, mesh 10x10..80x80 (%d..%d DOF, %d total), ND=5, "
                         "t U(1,4) mm, E in {70e3, 210e3} MPa, FP64" % (n, min(ndofs), max(ndofs), sum(ndofs)),
             "batched_solves_per_s": n / t_batch, "plain_solves_per_s": n / t_plain,
@@ -257,7 +303,36 @@ def batched_sweep(ctx, n, steps, plain=True):
             "batched_latency_ms": 1e3 * t_batch,
             "max_eig_rel_err_vs_plain": err, "members_ok": int(np.sum(st == 0)),
             "members_resolved_alone": int(stats["fallbacks"]), "stacked_free_dofs": int(stats["n_free"]),
-            "lanczos_vectors": int(stats["lanczos_vectors"])}
+            "lanczos_vectors": int(stats["lanczos_vectors"]), "cpu": cpu,
+            "batched_over_cpu": (n / t_batch) / cpu["solves_per_s"] if cpu else None,
+            "mean_dof_sweep": float(np.mean(ndofs))}
+
+
+def batched_cpu_sample(arrs, lam_b, k):
+    """The CPU oracle on the first k variants of the sweep, spread over all
+    host cores (one process, one BLAS thread each): its throughput, and the
+    batched eigenvalues of those variants against it."""
+    from oracle import oracle as O
+    O.lib()
+    cores = host_cores()
+    k = min(k, len(arrs))
+    t0 = time.perf_counter()
+    ref = O.eigenvalues_many(arrs[:k], workers=cores)
+    wall = time.perf_counter() - t0
+    err = 0.0
+    for la, lr in zip(lam_b[:k], ref):
+        if len(la) != len(lr):
+            err = float("inf")
+            break
+        if len(lr):
+            err = max(err, float(np.max(np.abs(la - lr) / np.abs(lr))))
+    log("batched CPU sample: %d variants on %d cores in %.1fs, max eig rel err batched vs CPU %.2e" %
+        (k, cores, wall, err))
+    return {"solves_per_s": k / wall, "variants": k, "cores": cores, "wall_s": wall, "kind": "port",
+            "mean_dof_sample": float(np.mean([a.ndof for a in arrs[:k]])),
+            "max_eig_rel_err_batched_vs_cpu": err,
+            "sample": "the first %d variants of the sweep, one oracle solve per process on %d cores "
+                      "(one BLAS thread each)" % (k, cores)}
 
 
 def _replay(arrs, trace, workers, batching):
@@ -409,18 +484,50 @@ def cylinder_case(ctx, n_fd, steps=3, seed=20260521):
                          "mode": 1}}
 
 
+def host_cores():
+    """Host cores this process may use, capped at 16 (the box's CPU share per GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(arr):
-    """One full SOL 105 solve of the same deck with the CPU oracle (C element
-    kernels + SciPy SuperLU/ARPACK), single-threaded."""
+    """The CPU oracle's throughput on the host: one full SOL 105 solve of
+    the same deck per core, all concurrently (one process and one BLAS
+    thread each: C element kernels, SciPy SuperLU static solve, ARPACK
+    shift-invert eigensolve), solves/s = cores / wall time.  Also returns the
+    first process's eigenpairs for the accuracy check."""
     from oracle import oracle as O
     O.lib()
-    t0 = time.perf_counter()
-    r = O.solve(arr)
-    dt = time.perf_counter() - t0
-    log("cpu oracle: %.1fs lambda1=%.6f" % (dt, r["eigenvalues"][0]))
-    return {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": "1 full solve of the same deck (C element kernels, SciPy SuperLU COLAMD static solve, "
-                      "ARPACK shift-invert eigsh), %.1f s" % dt}
+    cores = host_cores()
+    wall, outs = O.concurrent_solves(arr, cores)
+    single = [o["times"]["total"] for o in outs]
+    t0 = outs[0]["times"]
+    log("cpu oracle: %d concurrent solves in %.1fs (single solve %.1f..%.1f s), lambda1=%.6f" %
+        (cores, wall, min(single), max(single), outs[0]["eigenvalues"][0]))
+    out = {"value": cores / wall, "unit": "solves/s", "cores": cores, "kind": "port",
+           "sample": "%d concurrent full solves of the same deck, one per core (one process and one BLAS thread each: "
+                     "C element kernels, SciPy SuperLU static solve, ARPACK shift-invert eigsh), %.1f s wall"
+                     % (cores, wall),
+           "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+           "single_solve_s_median": float(np.median(single)),
+           "phase_s": {k: float(v) for k, v in t0.items() if k != "superlu_fill_nnz"},
+           "superlu_factor_s": float(t0.get("superlu_factor", float("nan"))),
+           "superlu_fill_nnz": int(t0.get("superlu_fill_nnz", 0))}
+    return out, outs[0]
 
 
 if __name__ == "__main__":

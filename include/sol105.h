@@ -169,9 +169,11 @@ int sol105_sensitivity_terms(sol105_ctx *ctx, sol105_model *model, int32_t mode,
  * on (on != 0) or off; sol105_profile_read returns
  *   out[0] launches timed, out[1] their summed duration in ms,
  *   out[2] their algorithmic bytes (8 x (factor entries + nrhs x vector
- *          rows read and written)), out[3] = 0. */
+ *          rows read and written)),
+ *   out[3], out[4], out[5]: launches, ms and bytes of the leaf height alone,
+ *   out[6], out[7]: ms and bytes of the heights above it. */
 int sol105_profile_enable(sol105_ctx *ctx, int32_t on);
-int sol105_profile_read(sol105_ctx *ctx, double *out /* 4 */);
+int sol105_profile_read(sol105_ctx *ctx, double *out /* 8 */);
 
 /* Batched solves.  A batch stacks independent models (design variants,
  * queued requests) into one block-diagonal system: element kernels,

@@ -156,10 +156,14 @@ class Context:
         self.check(self.L.sol105_profile_enable(self.h, 1 if on else 0))
 
     def profile_read(self):
-        """(launches, total ms, algorithmic bytes) of the timed product launches."""
-        out = np.zeros(4)
+        """Timed product launches: dict of launches, ms and algorithmic bytes
+        over all heights ("all"), the leaf height ("leaf") and the heights
+        above it ("upper")."""
+        out = np.zeros(8)
         self.check(self.L.sol105_profile_read(self.h, _dp(out)))
-        return int(out[0]), float(out[1]), float(out[2])
+        return {"all": (int(out[0]), float(out[1]), float(out[2])),
+                "leaf": (int(out[3]), float(out[4]), float(out[5])),
+                "upper": (int(out[0] - out[3]), float(out[6]), float(out[7]))}
 
     def timings(self):
         out = np.zeros(8)

@@ -1241,10 +1241,15 @@ int sol105_profile_read(sol105_ctx *ctx, double *out) {
     if (!ctx || !out) return SOL105_E_INVALID;
     (void)hipSetDevice(ctx->device);
     ctx->ptimer.collect();
-    out[0] = (double)ctx->ptimer.launches;
-    out[1] = ctx->ptimer.total_ms;
-    out[2] = ctx->ptimer.total_bytes;
-    out[3] = 0.0;
+    const ProdTimer &T = ctx->ptimer;
+    out[0] = (double)(T.launches[0] + T.launches[1]);
+    out[1] = T.total_ms[0] + T.total_ms[1];
+    out[2] = T.total_bytes[0] + T.total_bytes[1];
+    out[3] = (double)T.launches[0];
+    out[4] = T.total_ms[0];
+    out[5] = T.total_bytes[0];
+    out[6] = T.total_ms[1];
+    out[7] = T.total_bytes[1];
     return SOL105_OK;
 }
 

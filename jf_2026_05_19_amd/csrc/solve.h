@@ -28,10 +28,12 @@ struct DevSolve {
 struct ProdTimer {
     std::vector<hipEvent_t> pool;
     std::vector<double> pending_bytes;
+    std::vector<int> pending_tag;
     size_t used = 0;
-    double total_ms = 0.0, total_bytes = 0.0;
-    long launches = 0;
-    void begin(hipStream_t s, double bytes);
+    // per tag: 0 = leaf height, 1 = the heights above it
+    double total_ms[2] = {0.0, 0.0}, total_bytes[2] = {0.0, 0.0};
+    long launches[2] = {0, 0};
+    void begin(hipStream_t s, double bytes, int tag);
     void end(hipStream_t s);
     void collect();
     void reset();

@@ -591,7 +591,7 @@ void solve_prepare(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, 
     }
 }
 
-void ProdTimer::begin(hipStream_t s, double bytes) {
+void ProdTimer::begin(hipStream_t s, double bytes, int tag) {
     while (pool.size() < used + 2) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return;
@@ -599,6 +599,7 @@ void ProdTimer::begin(hipStream_t s, double bytes) {
     }
     (void)hipEventRecord(pool[used], s);
     pending_bytes.push_back(bytes);
+    pending_tag.push_back(tag);
 }
 
 void ProdTimer::end(hipStream_t s) {
@@ -610,19 +611,21 @@ void ProdTimer::end(hipStream_t s) {
 void ProdTimer::collect() {
     for (size_t i = 0; i + 1 < used; i += 2) {
         float ms = 0.f;
+        const int tag = pending_tag[i / 2];
         (void)hipEventSynchronize(pool[i + 1]);
-        if (hipEventElapsedTime(&ms, pool[i], pool[i + 1]) == hipSuccess) total_ms += ms;
-        total_bytes += pending_bytes[i / 2];
-        launches++;
+        if (hipEventElapsedTime(&ms, pool[i], pool[i + 1]) == hipSuccess) total_ms[tag] += ms;
+        total_bytes[tag] += pending_bytes[i / 2];
+        launches[tag]++;
     }
     used = 0;
     pending_bytes.clear();
+    pending_tag.clear();
 }
 
 void ProdTimer::reset() {
     collect();
-    total_ms = total_bytes = 0.0;
-    launches = 0;
+    total_ms[0] = total_ms[1] = total_bytes[0] = total_bytes[1] = 0.0;
+    launches[0] = launches[1] = 0;
 }
 
 ProdTimer::~ProdTimer() {
@@ -662,8 +665,8 @@ hipError_t solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS
         const SolveHeight &SH = S.heights[h];
         if (H.n == 0) continue;
         const int32_t *fl = D.flist + H.list_off;
-        const bool timed = pt && nrhs > 1 && h == 0;
-        if (timed) pt->begin(s, prod_bytes(SH, nrhs));
+        const bool timed = pt && nrhs > 1;
+        if (timed) pt->begin(s, prod_bytes(SH, nrhs), h == 0 ? 0 : 1);
         const int32_t *tl = DS.fwd_tiles + 3 * SH.fwd_off;
         if (nrhs == 1)
             prod1_kernel<<<SH.n_fwd, 256, 0, s>>>(D, DS, tl, Pbuf, W, part, 0, DS.fwd_kw, DS.fwd_nks, xin, xout,
@@ -689,8 +692,8 @@ hipError_t solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS
         const SolveHeight &SH = S.heights[h];
         if (H.n == 0) continue;
         const int32_t *fl = D.flist + H.list_off;
-        const bool timed = pt && nrhs > 1 && h == 0;
-        if (timed) pt->begin(s, prod_bytes(SH, nrhs));
+        const bool timed = pt && nrhs > 1;
+        if (timed) pt->begin(s, prod_bytes(SH, nrhs), h == 0 ? 0 : 1);
         const int32_t *tl = DS.bwd_tiles + 3 * SH.bwd_off;
         if (nrhs == 1)
             prod1_kernel<<<SH.n_bwd, 256, 0, s>>>(D, DS, tl, PTbuf, W, part, 1, DS.bwd_kw, DS.bwd_nks, xin, xout,

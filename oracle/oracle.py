@@ -262,12 +262,17 @@ def autospc(arr, Kdiag):
     return fixed | new
 
 
-def static_solve(arr, K):
+def static_solve(arr, K, times=None):
+    import time
     Kd = K.diagonal()
     fixed = autospc(arr, Kd)
     free = np.flatnonzero(~fixed)
     Kff = K[free][:, free].tocsc()
+    t0 = time.perf_counter()
     lu = spla.splu(Kff)
+    if times is not None:
+        times["superlu_factor"] = time.perf_counter() - t0
+        times["superlu_fill_nnz"] = int(lu.L.nnz + lu.U.nnz)
     u = np.zeros(arr.ndof)
     u[free] = lu.solve(arr.F[free])
     return u, free
@@ -480,19 +485,68 @@ def solve(arr):
     """Full forward SOL 105 path: K -> u -> KG -> (lambda, phi).  The static
     solve follows the reference's Cholesky path (refix, factorisation
     AUTOSPC) densely up to DENSE_CHOLESKY_MAX free DOFs, above that a sparse
-    LU of the AUTOSPC free set."""
+    LU of the AUTOSPC free set.  ``res["times"]`` holds the phase wall times
+    (s) and, on the sparse path, the SuperLU factorisation time and fill."""
+    import time
+    tm = {}
+    t0 = time.perf_counter()
     K, mes = assemble_K(arr, with_max_elem_stiff=True)
+    t1 = time.perf_counter()
+    tm["K_assembly"] = t1 - t0
     n_free = int(np.sum(~autospc(arr, K.diagonal())))
     info = {}
     if n_free <= DENSE_CHOLESKY_MAX:
         u, free = static_solve_cholesky(arr, K, mes, info)
     else:
-        u, free = static_solve(arr, K)
+        u, free = static_solve(arr, K, tm)
         info["path"] = "sparse_lu"
+    t2 = time.perf_counter()
+    tm["static_solve"] = t2 - t1
     KG = assemble_KG(arr, u)
+    t3 = time.perf_counter()
+    tm["KG_assembly"] = t3 - t2
     res = buckling(arr, K, KG, free)
-    res.update(K=K, KG=KG, u=u, free=free, static_info=info)
+    tm["eigensolve"] = time.perf_counter() - t3
+    tm["total"] = time.perf_counter() - t0
+    res.update(K=K, KG=KG, u=u, free=free, static_info=info, times=tm)
     return res
+
+
+def _eigenvalues(arr):
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1):      # one BLAS thread per worker process
+        return solve(arr)["eigenvalues"]
+
+
+def _timed_solve(arr, keep_modes):
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1):
+        r = solve(arr)
+    return dict(eigenvalues=r["eigenvalues"], modes=r["modes"] if keep_modes else None, times=r["times"])
+
+
+def concurrent_solves(arr, workers):
+    """`workers` concurrent solves of the same model, one process each, one
+    BLAS thread each (every host core busy with its own solve).  Returns
+    (wall seconds, per-process results; the first keeps its modes)."""
+    import multiprocessing as mp
+    import time
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        list(ex.map(time.sleep, [1.0] * workers))           # every worker up before the clock starts
+        t0 = time.perf_counter()
+        futs = [ex.submit(_timed_solve, arr, i == 0) for i in range(workers)]
+        out = [f.result() for f in futs]
+        return time.perf_counter() - t0, out
+
+
+def eigenvalues_many(arrs, workers=8):
+    """Eigenvalues of several models, solved in parallel by spawned worker
+    processes (not forked: the caller may already hold a GPU context)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=max(1, min(workers, len(arrs))), mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_eigenvalues, arrs))
 
 
 def mac(a, b):

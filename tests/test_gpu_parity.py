@@ -127,6 +127,26 @@ def test_solve_repeatable(solved):
     assert m1.tobytes() == m2.tobytes()
 
 
+def _check_residual_bound(K, KG, lam, modes, free, tol=1e-10, slack=10.0):
+    """The eigensolver stops when ||C y - theta y|| <= tol |theta| ||y|| for
+    C = L^-1 (-KG) L^-T (K_ff = L L^T), y = L^T x.  Multiplying by L gives
+    K x + lambda KG x = -lambda L (C y - theta y), so on the free DOFs
+        ||K x + lambda KG x|| <= tol ||L|| sqrt(x' K x),  ||L|| <= sqrt(||K_ff||_inf);
+    checked with a factor `slack` for rounding."""
+    Kf = K[free][:, free]
+    KGf = KG[free][:, free]
+    normL = np.sqrt(abs(Kf).sum(axis=1).max())
+    worst = 0.0
+    for k in range(len(lam)):
+        x = modes[free, k]
+        Kx = Kf @ x
+        r = np.linalg.norm(Kx + lam[k] * (KGf @ x))
+        bound = tol * normL * np.sqrt(x @ Kx)
+        worst = max(worst, r / bound)
+        assert r <= slack * bound, (k, r, bound)
+    return worst
+
+
 def test_restarted_eigensolve_residuals(ctx):
     """A cylinder without the triangle band at 208 x 200 GRIDs (249,600 DOF)
     has a tightly clustered lowest spectrum that needs a thick restart of
@@ -146,10 +166,7 @@ def test_restarted_eigensolve_residuals(ctx):
     K = dm.bsr_to_csr(dm.assemble_K())
     u, free = dm.static_solve()
     KG = dm.bsr_to_csr(dm.assemble_KG())
-    for k in range(len(lam)):
-        p = modes[:, k]
-        r = (K @ p + lam[k] * (KG @ p))[free]
-        assert np.linalg.norm(r) <= 1e-6 * np.linalg.norm((K @ p)[free]), (k, np.linalg.norm(r))
+    print("residual / bound:", _check_residual_bound(K, KG, lam, modes, np.flatnonzero(free)), flush=True)
     ncr = 70e3 * 1.0 / (500.0 * np.sqrt(3 * (1 - 0.33 ** 2)))
     assert abs(lam[0] - ncr) <= 0.02 * ncr
     dm.free()
@@ -160,8 +177,9 @@ def test_banded_rayleigh_ritz_beyond_lds(ctx):
     ND = 10, Lanczos block 16) converges at 752 vectors without a restart, so
     its Rayleigh-Ritz matrix is banded at m > 384: the band reduction keeps
     the band in the workspace instead of LDS.  Checks: every pair's
-    eigen-residual ||K phi + lambda KG phi|| / ||K phi|| on the free DOFs
-    (device K and KG, host products), ascending positive loads, and the first
+    eigen-residual ||K phi + lambda KG phi|| on the free DOFs (device K and
+    KG, host products) against the bound the convergence rule implies,
+    ascending positive loads, and the first
     load against 60.60834312831407, what the dense Rayleigh-Ritz path
     (O(m^3) tridiagonalisation, no band) gives for this model."""
     from jf_2026_05_19_amd import _native as N
@@ -175,9 +193,8 @@ def test_banded_rayleigh_ritz_beyond_lds(ctx):
     K = dm.bsr_to_csr(dm.assemble_K())
     u, free = dm.static_solve()
     KG = dm.bsr_to_csr(dm.assemble_KG())
-    for k in range(len(lam)):
-        p = modes[:, k]
-        r = (K @ p + lam[k] * (KG @ p))[free]
-        assert np.linalg.norm(r) <= 1e-6 * np.linalg.norm((K @ p)[free]), (k, np.linalg.norm(r))
+    print("residual / bound:", _check_residual_bound(K, KG, lam, modes, np.flatnonzero(free)), flush=True)
+    # regression value of this build's dense Rayleigh-Ritz path (not a parity
+    # reference: the oracle's parity runs on this geometry at 48 x 25 GRIDs)
     assert abs(lam[0] - 60.60834312831407) <= 1e-9 * 60.60834312831407
     dm.free()
