@@ -335,12 +335,12 @@ def batched_cpu_sample(arrs, lam_b, k):
                       "(one BLAS thread each)" % (k, cores)}
 
 
-def _replay(arrs, trace, workers, batching):
+def _replay(arrs, trace, workers, batching, warm=False):
     """Serves a request trace [(arrival s, mesh, thickness)]: submits each
     request at its arrival time, returns per-request (eigenvalues, info) and
     the wall time from the first arrival to the last result."""
     from jf_2026_05_19_amd import _native as N
-    srv = N.Server(workers=workers, batching=batching)
+    srv = N.Server(workers=workers, batching=batching, warm_start=warm)
     ids = [srv.add_model(a) for a in arrs]
     for i in ids:                      # first design of every mesh: symbolic analysis, untimed
         srv.wait(srv.submit(i))
@@ -358,14 +358,39 @@ def _replay(arrs, trace, workers, batching):
     return out, wall, stats
 
 
-def served_stream(n_req, workers=4, seed=20260520):
+def _trace(arrs, n_req, rate, rng):
+    """Poisson arrivals at `rate`, each a +-5 % per-shell perturbation of the
+    previous design of a uniformly drawn mesh."""
+    cur = [a.shell_thickness() for a in arrs]
+    trace, t = [], 0.0
+    for r in range(n_req):
+        t += float(rng.exponential(1.0 / rate))
+        f = int(rng.integers(len(arrs)))
+        cur[f] = cur[f] * rng.uniform(0.95, 1.05, size=cur[f].shape)
+        trace.append((t, f, cur[f].copy()))
+    return trace
+
+
+def _stream_summary(res, wall, stats, n_req):
+    lat = np.array([i["latency_ms"] for _, _, i in res])
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "requests_per_s": n_req / wall, "stacked_solves": int(stats["stacked_solves"]),
+            "stacked_requests": int(stats["stacked_requests"]), "cached": int(stats["cached"]),
+            "peak_inflight_bytes": stats["peak_inflight_bytes"],
+            "lanczos_vectors_mean": float(np.mean([i["lanczos_vectors"] for _, _, i in res]))}
+
+
+def served_stream(n_req, workers=4, seed=20260520, oracle_max_dof=30000):
     """Served optimisation stream: ten seeded panel meshes (2k..500k DOF
-    log-uniform, 10 % CTRIA3, CBAR stringers, ND = 5), Poisson arrivals at
-    80 % of the one-at-a-time device capacity, each request a +-5 %
+    log-uniform, 10 % CTRIA3, CBAR stringers, ND = 5); each request a +-5 %
     per-shell thickness perturbation of the previous design of its mesh.
     Served (4 worker streams, stacking of small pending requests,
     HBM-footprint admission) and, as the reference, the same trace through
-    one worker without stacking (a plain queue).  Latency = arrival -> result."""
+    one worker without stacking (a plain queue).  Two legs: Poisson arrivals at 80 % of the
+    one-at-a-time device capacity (latency), and at twice that capacity
+    (saturation: throughput).  Latency = arrival -> result.  The served
+    results of the meshes up to `oracle_max_dof` DOF are checked against the
+    CPU oracle."""
     from concurrent.futures import ProcessPoolExecutor
     from jf_2026_05_19_amd import decks
     from jf_2026_05_19_amd import _native as N
@@ -384,41 +409,54 @@ def served_stream(n_req, workers=4, seed=20260520):
         svc.append(time.perf_counter() - ts)
         dm.free()
     ctx.close()
-    rate = 0.8 / float(np.mean(svc))
+    capacity = 1.0 / float(np.mean(svc))
     rng = np.random.default_rng(seed)
-    cur = [a.shell_thickness() for a in arrs]
-    trace, t = [], 0.0
-    for r in range(n_req):
-        t += float(rng.exponential(1.0 / rate))
-        f = int(rng.integers(len(arrs)))
-        cur[f] = cur[f] * rng.uniform(0.95, 1.05, size=cur[f].shape)
-        trace.append((t, f, cur[f].copy()))
-    log("served stream: %d requests over %d meshes (%d..%d DOF), rate %.1f/s (mean solo service %.1f ms)" %
-        (n_req, len(arrs), min(ndofs), max(ndofs), rate, 1e3 * np.mean(svc)))
-    res_s, wall_s, st_s = _replay(arrs, trace, workers, True)
-    res_q, wall_q, st_q = _replay(arrs, trace, 1, False)
-    lat_s = np.array([i["latency_ms"] for _, _, i in res_s])
-    lat_q = np.array([i["latency_ms"] for _, _, i in res_q])
+    out = {"workload": "served stream: %d requests per leg over 10 panel meshes %d..%d DOF (10%% CTRIA3, CBAR), "
+                       "+-5%% thickness steps per mesh, ND=5, FP64; Poisson arrivals at 0.8x and 2x the "
+                       "one-at-a-time capacity (%.1f req/s)" % (n_req, min(ndofs), max(ndofs), capacity),
+           "workers": workers, "mean_solo_service_ms": 1e3 * float(np.mean(svc))}
+    for leg, load in (("load_0.8", 0.8), ("saturated_2x", 2.0)):
+        trace = _trace(arrs, n_req, load * capacity, rng)
+        res_s, wall_s, st_s = _replay(arrs, trace, workers, True)
+        res_q, wall_q, st_q = _replay(arrs, trace, 1, False)
+        err = 0.0
+        for (la, sa, _), (lb, sb, _) in zip(res_s, res_q):
+            if sa != 0 or sb != 0 or len(la) != len(lb):
+                err = float("inf")
+                break
+            if len(la):
+                err = max(err, float(np.max(np.abs(la - lb) / np.abs(lb))))
+        S, Q = _stream_summary(res_s, wall_s, st_s, n_req), _stream_summary(res_q, wall_q, st_q, n_req)
+        log("served %s: p50 %.1f ms p99 %.1f ms %.1f req/s (stacked %d/%d, cached %d, %.0f vectors/solve); "
+            "plain queue: p50 %.1f ms p99 %.1f ms %.1f req/s (%.0f vectors/solve)" %
+            (leg, S["p50_ms"], S["p99_ms"], S["requests_per_s"], S["stacked_requests"], S["stacked_solves"],
+             S["cached"], S["lanczos_vectors_mean"], Q["p50_ms"], Q["p99_ms"], Q["requests_per_s"],
+             Q["lanczos_vectors_mean"]))
+        out[leg] = {"arrival_rate_per_s": load * capacity, "served": S, "plain_queue": Q,
+                    "throughput_served_over_plain": S["requests_per_s"] / Q["requests_per_s"],
+                    "max_eig_rel_err_served_vs_queue": err}
+        if leg == "load_0.8":
+            out[leg]["vs_cpu_oracle"] = _served_vs_oracle(arrs, trace, res_s, oracle_max_dof)
+    return out
+
+
+def _served_vs_oracle(arrs, trace, res, max_dof):
+    """The served eigenvalues of every request on a mesh of at most max_dof
+    DOF against the CPU oracle's solve of the same design (on all host cores)."""
+    from oracle import oracle as O
+    idx = [k for k, (_, f, _) in enumerate(trace) if arrs[f].ndof <= max_dof]
+    if not idx:
+        return None
+    ref = O.eigenvalues_many([arrs[trace[k][1]].with_shell_thickness(trace[k][2]) for k in idx], workers=host_cores())
     err = 0.0
-    for (la, sa, _), (lb, sb, _) in zip(res_s, res_q):
-        if sa != 0 or sb != 0 or len(la) != len(lb):
-            err = float("inf")
-            break
-        if len(la):
-            err = max(err, float(np.max(np.abs(la - lb) / np.abs(lb))))
-    log("served: p50 %.1f ms p99 %.1f ms (%.1f req/s); plain queue: p50 %.1f ms p99 %.1f ms (%.1f req/s)" %
-        (np.percentile(lat_s, 50), np.percentile(lat_s, 99), n_req / wall_s, np.percentile(lat_q, 50),
-         np.percentile(lat_q, 99), n_req / wall_q))
-    return {"workload": "served stream: %d requests, Poisson arrivals at 80%% of one-at-a-time capacity "
-                        "(%.1f/s), 10 panel meshes %d..%d DOF (10%% CTRIA3, CBAR), +-5%% thickness steps per "
-                        "mesh, ND=5, FP64" % (n_req, rate, min(ndofs), max(ndofs)),
-            "served_p50_ms": float(np.percentile(lat_s, 50)), "served_p99_ms": float(np.percentile(lat_s, 99)),
-            "served_requests_per_s": n_req / wall_s,
-            "queue_p50_ms": float(np.percentile(lat_q, 50)), "queue_p99_ms": float(np.percentile(lat_q, 99)),
-            "queue_requests_per_s": n_req / wall_q,
-            "workers": workers, "stacked_solves": int(st_s["stacked_solves"]),
-            "stacked_requests": int(st_s["stacked_requests"]), "max_eig_rel_err_vs_queue": err,
-            "mean_solo_service_ms": 1e3 * float(np.mean(svc))}
+    for k, lr in zip(idx, ref):
+        la = res[k][0]
+        if len(la) != len(lr):
+            return {"requests": len(idx), "max_eig_rel_err": float("inf")}
+        if len(lr):
+            err = max(err, float(np.max(np.abs(la - lr) / np.abs(lr))))
+    log("served vs CPU oracle: %d requests on meshes <= %d DOF, max eig rel err %.2e" % (len(idx), max_dof, err))
+    return {"requests": len(idx), "max_dof": max_dof, "max_eig_rel_err": err}
 
 
 def cylinder_case(ctx, n_fd, steps=3, seed=20260521):

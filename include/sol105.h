@@ -73,10 +73,19 @@ const char *sol105_last_error(const sol105_ctx *ctx);
  *                                    32 .. 1024 (default 1024; rounded down
  *                                    to a multiple of the block size);
  *   SOL105_OPT_LANCZOS_MAX_RESTARTS  thick restarts before SOL105_E_NOCONV,
- *                                    0 .. 1000 (default 40).
+ *                                    0 .. 1000 (default 40);
+ *   SOL105_OPT_WARM_START            0 or 1 (default 0): start the block
+ *                                    Lanczos iteration of a model from the
+ *                                    Ritz vectors of its previous solve (same
+ *                                    free set), as a design loop or a served
+ *                                    stream moves them little.  The results
+ *                                    then depend on that history within the
+ *                                    convergence tolerance, no longer bitwise.
+ *                                    No counterpart in the reference.
  * SOL105_E_INVALID for an unknown option or a value out of range. */
 #define SOL105_OPT_LANCZOS_MAX_VECTORS 1
 #define SOL105_OPT_LANCZOS_MAX_RESTARTS 2
+#define SOL105_OPT_WARM_START 3
 int sol105_ctx_set_option(sol105_ctx *ctx, int32_t option, double value);
 
 /* Upload a model, build the BSR pattern, the deterministic element->block
@@ -213,13 +222,21 @@ int sol105_batch_stats(const sol105_ctx *ctx, const sol105_batch *batch, double 
  * design iterations.  n_workers threads, each with its own HIP stream, take
  * requests as soon as a worker and device memory are free, so requests
  * overlap on the device; pending requests of small meshes are stacked into
- * one batched solve when batching != 0.  A request is admitted when its
+ * one batched solve with SOL105_SERVER_STACKING.  A request is admitted when its
  * estimated device footprint fits hbm_budget_bytes beside the work in flight
  * (<= 0: three quarters of the free device memory).  Requests of one mesh
- * run in submission order.  No reference counterpart for the entry points
- * themselves. */
+ * run in submission order.  With SOL105_SERVER_WARM_START a worker starts a
+ * mesh's eigensolve from the Ritz vectors of that mesh's previous design
+ * (SOL105_OPT_WARM_START), and a
+ * request repeating its mesh's last solved design is answered from that
+ * solve.  No reference counterpart for the entry points themselves. */
 typedef struct sol105_server sol105_server;
-int sol105_server_create(int device, int32_t n_workers, double hbm_budget_bytes, int32_t batching,
+/* flags: SOL105_SERVER_STACKING (stack small pending requests into batched
+ * solves) | SOL105_SERVER_WARM_START (start each eigensolve of a mesh from
+ * the Ritz vectors of its previous design) */
+#define SOL105_SERVER_STACKING 1
+#define SOL105_SERVER_WARM_START 2
+int sol105_server_create(int device, int32_t n_workers, double hbm_budget_bytes, int32_t flags,
                          sol105_server **out);
 int sol105_server_destroy(sol105_server *server);  /* finishes queued requests first */
 const char *sol105_server_last_error(const sol105_server *server);
@@ -232,15 +249,19 @@ int sol105_server_submit(sol105_server *server, int32_t model_id, const double *
  * SOL105_E_PENDING.  lambda_out: max_modes values; status: the request's
  * own result code; info (may be NULL): latency ms (submit -> done), queue
  * ms, service ms, requests in its stacked solve, start and finish ms since
- * the server was created. */
+ * the server was created, Lanczos basis vectors its eigensolve generated,
+ * 1 if it repeated its mesh's last solved design and was answered from that
+ * solve (else 0). */
 int sol105_server_poll(sol105_server *server, int64_t ticket, double *lambda_out, int32_t *n_out,
-                       int32_t *status, double *info /* 6 */);
-/* as sol105_server_poll, blocking until the request is done */
+                       int32_t *status, double *info /* 8 */);
+/* as sol105_server_poll, blocking until the request is done; SOL105_E_INVALID
+ * when another caller released the ticket meanwhile */
 int sol105_server_wait(sol105_server *server, int64_t ticket, double *lambda_out, int32_t *n_out,
-                       int32_t *status, double *info /* 6 */);
-/* out[8]: requests done, stacked solves, requests in them, queued, bytes in
- * flight, budget, workers, registered meshes */
-int sol105_server_stats(sol105_server *server, double *out /* 8 */);
+                       int32_t *status, double *info /* 8 */);
+/* out[10]: requests done, stacked solves, requests in them, queued, bytes in
+ * flight, budget, workers, registered meshes, requests answered from a
+ * repeated design, largest bytes in flight so far */
+int sol105_server_stats(sol105_server *server, double *out /* 10 */);
 
 /* Phase timings of the last sol105_solve in milliseconds (device events):
  * [0] K assembly, [1] AUTOSPC+symbolic, [2] factorization, [3] static solve,

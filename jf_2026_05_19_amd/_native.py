@@ -367,13 +367,14 @@ class Batch:
 class Server:
     """Device-resident request server (sol105_server_*): meshes registered
     once, designs submitted as shell thicknesses, requests served by worker
-    streams with stacking of small pending requests."""
+    streams with stacking of small pending requests and, on request,
+    warm-started eigensolves."""
 
-    def __init__(self, device=0, workers=4, hbm_budget=0.0, batching=True):
+    def __init__(self, device=0, workers=4, hbm_budget=0.0, batching=True, warm_start=False):
         self.L = lib()
         h = ctypes.c_void_p()
-        rc = self.L.sol105_server_create(int(device), int(workers), float(hbm_budget), 1 if batching else 0,
-                                         ctypes.byref(h))
+        flags = (1 if batching else 0) | (2 if warm_start else 0)   # SOL105_SERVER_STACKING, _WARM_START
+        rc = self.L.sol105_server_create(int(device), int(workers), float(hbm_budget), flags, ctypes.byref(h))
         if rc != SOL105_OK:
             raise Sol105Error(rc, "could not create the server")
         self.h = h
@@ -414,12 +415,13 @@ class Server:
         lam = np.zeros(16)
         n = ctypes.c_int32()
         st = ctypes.c_int32()
-        info = np.zeros(6)
+        info = np.zeros(8)
         rc = fn(self.h, ctypes.c_int64(ticket), _dp(lam), ctypes.byref(n), ctypes.byref(st), _dp(info))
         if rc == SOL105_E_PENDING:
             return None
         self._check(rc)
-        keys = ("latency_ms", "queue_ms", "service_ms", "members", "start_ms", "finish_ms")
+        keys = ("latency_ms", "queue_ms", "service_ms", "members", "start_ms", "finish_ms", "lanczos_vectors",
+                "cached")
         return lam[:n.value].copy(), int(st.value), dict(zip(keys, info.tolist()))
 
     def poll(self, ticket):
@@ -430,8 +432,8 @@ class Server:
         return self._result(self.L.sol105_server_wait, ticket)
 
     def stats(self):
-        out = np.zeros(8)
+        out = np.zeros(10)
         self._check(self.L.sol105_server_stats(self.h, _dp(out)))
         keys = ("done", "stacked_solves", "stacked_requests", "queued", "inflight_bytes", "budget_bytes",
-                "workers", "meshes")
+                "workers", "meshes", "cached", "peak_inflight_bytes")
         return dict(zip(keys, out.tolist()))

@@ -318,6 +318,10 @@ int sol105_ctx_set_option(sol105_ctx *ctx, int32_t option, double value) {
                 return set_err(ctx, SOL105_E_INVALID, "Lanczos basis size %g out of range 32..1024", value);
             ctx->lanczos_max_vectors = (int)value;
             return SOL105_OK;
+        case SOL105_OPT_WARM_START:
+            if (value != 0 && value != 1) return set_err(ctx, SOL105_E_INVALID, "warm start is 0 or 1");
+            ctx->warm_start = value != 0;
+            return SOL105_OK;
         case SOL105_OPT_LANCZOS_MAX_RESTARTS:
             if (value < 0 || value > 1000)
                 return set_err(ctx, SOL105_E_INVALID, "Lanczos restarts %g out of range 0..1000", value);
@@ -423,6 +427,7 @@ int factor_free_set(sol105_ctx *ctx, sol105_model *M, const std::vector<uint8_t>
             return set_err(ctx, SOL105_E_INVALID, "symbolic analysis failed: %s", ex.what());
         }
         if (M->sym.n_free == 0) return set_err(ctx, SOL105_E_SINGULAR, "no free DOFs");
+        M->sym_gen++;
         int rc = upload_symbolic(ctx, M);
         if (rc) return rc;
     }
@@ -700,7 +705,7 @@ void small_sym_eig_host(int b, std::vector<double> A, std::vector<double> &w, st
 struct LanczosOut {
     std::vector<double> theta;  // selected Ritz values (descending |theta|)
     std::vector<double> resid;
-    int m = 0, steps = 0, injected = 0, generated = 0, restarts = 0;
+    int m = 0, steps = 0, injected = 0, generated = 0, restarts = 0, seeded = 0;
 };
 
 // Block Lanczos on C = L^-1 (-KG) L^-T with full CGS2 reorthogonalisation.
@@ -856,13 +861,30 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
         return SOL105_OK;
     };
     start_block(n, b, V.p, n, 0, s);
+    // warm start: the Ritz vectors of the last solve of this model (same
+    // free set and elimination order) replace the first columns of the start
+    // block; design iterations move them little, so fewer steps are needed
+    int seeded = 0;
+    if (ctx->warm_start && M->y_cols > 0 && M->y_sym == M->sym_gen && d_Y) {
+        seeded = std::min(b, M->y_cols);
+        HIPCHK(hipMemcpyAsync(V.p, d_Y, sizeof(double) * n * seeded, hipMemcpyDeviceToDevice, s));
+    }
+    M->y_cols = 0;
     cholqr2(V.p, nullptr, 0, nullptr, 0, false);
     int j = 0;
     // first Rayleigh-Ritz check: where the last solve of this model converged
-    // (design loops and repeated solves), else at about 3 vectors per wanted pair
+    // (design loops and repeated solves; one block earlier when warm-started,
+    // since a check costs about two steps), else at about 3 vectors per
+    // wanted pair
     const bool warm = M->warm_m > 0 && M->warm_b == b;
-    int next_check = warm ? std::min(mmax, std::max(b, M->warm_m))
-                          : std::min(mmax, std::max(2 * b, ((kneed * 3 + b - 1) / b) * b));
+    // a warm-started solve tries one block earlier only while that keeps
+    // working (the previous warm-started solve converged at its first check)
+    const bool early = seeded && warm && M->warm_first_ok;
+    int next_check = early  ? std::min(mmax, std::max(2 * b, M->warm_m - b))
+                     : warm ? std::min(mmax, std::max(b, M->warm_m))
+                            : std::min(mmax, std::max(2 * b, ((kneed * 3 + b - 1) / b) * b));
+    const int first_check = next_check;
+    out.seeded = seeded;
     std::vector<double> hR;
     bool converged = false;
     int m = 0, injected = 0, restarts = 0, generated = 0;
@@ -1004,8 +1026,11 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     out.steps = j;
     out.injected = injected;
     if (!converged) return set_err(ctx, SOL105_E_NOCONV, "block Lanczos did not converge in %d vectors", m);
+    M->warm_first_ok = seeded && warm && restarts == 0 && m == first_check;
     M->warm_m = restarts ? mmax : m;
     M->warm_b = b;
+    M->y_cols = (int)out.theta.size();
+    M->y_sym = M->sym_gen;
     return SOL105_OK;
 }
 
@@ -1024,6 +1049,7 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
     req = std::min(req, std::max(n - 2, 1));
     if (req > MAXR) return set_err(ctx, SOL105_E_UNSUPPORTED, "more than %d requested modes", MAXR);
     DevBuf<double> &Y = M->lw.Y, &Xf = M->lw.Xf;
+    if (Y.n < (size_t)n * req) M->y_cols = 0;  // reallocated: no kept Ritz vectors
     HIPCHK(Y.alloc((size_t)n * req));
     LanczosOut lo;
     int rc = lanczos(ctx, M, req, Y.p, lo);

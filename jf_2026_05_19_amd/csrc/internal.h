@@ -28,6 +28,7 @@ struct sol105_ctx {
     bool profiling = false;
     int lanczos_max_vectors = 1024;  // SOL105_OPT_LANCZOS_MAX_VECTORS
     int lanczos_max_restarts = 40;   // SOL105_OPT_LANCZOS_MAX_RESTARTS
+    bool warm_start = false;         // SOL105_OPT_WARM_START
     ProdTimer ptimer;  // HIP-event timing of the block triangular-solve products
     ProdTimer *timer() { return profiling ? &ptimer : nullptr; }
 };
@@ -109,6 +110,12 @@ struct sol105_model {
     int n_modes = 0;
     double lanczos_m = 0, lanczos_steps = 0;
     int warm_m = 0, warm_b = 0;  // Lanczos size at the last convergence (first check there next time)
+    // symbolic analyses so far (the free set and elimination order change
+    // with each); the Ritz vectors kept in lw.Y (y_cols columns, elimination
+    // order) belong to analysis y_sym
+    uint64_t sym_gen = 0, y_sym = 0;
+    int y_cols = 0;
+    bool warm_first_ok = true;  // the last warm-started solve converged at its first check
     // a batch model is the block-diagonal union of independent members; their
     // node ranges (n_members + 1 offsets; empty for a single model) scope the
     // per-model AUTOSPC maxima

@@ -12,7 +12,10 @@
 // the device.  Requests of small meshes that are pending together are
 // stacked into one batched solve (sol105_batch_*).  A request is admitted
 // when its estimated device footprint fits the budget beside the work in
-// flight.
+// flight.  On request (SOL105_SERVER_WARM_START) each worker context starts
+// a mesh's eigensolve from the Ritz vectors of that mesh's previous design
+// (SOL105_OPT_WARM_START).  A request that repeats a mesh's last solved
+// design is answered from that solve's results.
 #include "internal.h"
 
 #include <chrono>
@@ -53,6 +56,11 @@ struct SrvModel {
     std::unique_ptr<sol105_model> M;
     bool busy = false;
     int nd = 3;
+    // the last design solved alone and its results (a request that repeats
+    // it is answered without a solve)
+    bool last_valid = false;
+    std::vector<double> last_t, last_lambda;
+    int last_max_modes = 0;
 };
 
 struct Request {
@@ -63,7 +71,15 @@ struct Request {
     bool done = false;
     int status = SOL105_OK;
     int members = 1;
+    double lanczos_vectors = 0.0;  // basis vectors its eigensolve generated
+    bool cached = false;           // answered from the last solve of the same design
     std::vector<double> lambda;
+};
+
+// a request and its mesh, both resolved under the server lock
+struct Job {
+    Request *r;
+    SrvModel *m;
 };
 
 }  // namespace
@@ -84,7 +100,9 @@ struct sol105_server {
     bool stop = false;
     Clock::time_point t0 = Clock::now();
     std::string err;
-    int64_t n_done = 0, n_batches = 0, n_batched = 0;
+    int64_t n_done = 0, n_batches = 0, n_batched = 0, n_cached = 0;
+    double peak_inflight = 0.0;
+    bool warm_start = false;
 };
 
 namespace {
@@ -127,35 +145,60 @@ std::vector<int64_t> pick(sol105_server *S, double &fp) {
     return group;
 }
 
-void run_group(sol105_server *S, sol105_ctx *ctx, const std::vector<int64_t> &group) {
-    // requests are not touched by other threads while their models are busy
-    std::vector<Request *> R;
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        for (int64_t tk : group) R.push_back(&S->reqs[tk]);
-    }
-    for (Request *r : R) r->started = Clock::now();
-    if (R.size() == 1) {
-        Request &r = *R[0];
-        sol105_model *M = S->models[r.model]->M.get();
-        int rc = SOL105_OK;
-        if (!r.t.empty()) rc = sol105_model_set_shell_thickness(ctx, M, r.t.data());
-        std::vector<double> lam(std::max(r.max_modes, 1));
-        int32_t n = 0;
-        if (rc == SOL105_OK) rc = sol105_solve(ctx, M, r.max_modes, lam.data(), nullptr, &n);
-        r.status = rc;
-        if (rc == SOL105_OK) r.lambda.assign(lam.begin(), lam.begin() + n);
+// The design a request asks for: its own thicknesses, or the mesh's current
+// ones when it gave none.
+std::vector<double> design_of(const Request &r, const sol105_model *M) {
+    if (!r.t.empty()) return r.t;
+    std::vector<double> t(M->n_q4 + M->n_t3);
+    for (int e = 0; e < M->n_q4; e++) t[e] = M->q4prop[6 * (size_t)e + 2];
+    for (int e = 0; e < M->n_t3; e++) t[M->n_q4 + e] = M->t3prop[6 * (size_t)e + 2];
+    return t;
+}
+
+// Runs a group outside the server lock.  Requests and meshes of the group
+// belong to this worker until it marks them done (their meshes are busy), so
+// no other thread touches them meanwhile.
+void run_group(sol105_ctx *ctx, const std::vector<Job> &jobs) {
+    for (const Job &j : jobs) j.r->started = Clock::now();
+    if (jobs.size() == 1) {
+        Request &r = *jobs[0].r;
+        SrvModel &SM = *jobs[0].m;
+        sol105_model *M = SM.M.get();
+        const std::vector<double> t = design_of(r, M);
         r.members = 1;
+        if (SM.last_valid && t == SM.last_t && r.max_modes <= SM.last_max_modes) {
+            r.status = SOL105_OK;
+            r.cached = true;
+            r.lambda.assign(SM.last_lambda.begin(),
+                            SM.last_lambda.begin() + std::min<size_t>(SM.last_lambda.size(), (size_t)r.max_modes));
+        } else {
+            int rc = SOL105_OK;
+            if (!r.t.empty()) rc = sol105_model_set_shell_thickness(ctx, M, r.t.data());
+            std::vector<double> lam(std::max(r.max_modes, 1));
+            int32_t n = 0;
+            if (rc == SOL105_OK) rc = sol105_solve(ctx, M, r.max_modes, lam.data(), nullptr, &n);
+            r.status = rc;
+            SM.last_valid = rc == SOL105_OK;
+            if (rc == SOL105_OK) {
+                r.lambda.assign(lam.begin(), lam.begin() + n);
+                r.lanczos_vectors = M->lanczos_m;
+                SM.last_t = t;
+                SM.last_lambda = r.lambda;
+                SM.last_max_modes = r.max_modes;
+            }
+        }
     } else {
         // stacked solve of the group: member descriptions point into the
         // registered models' host arrays with the requested thicknesses
-        std::vector<sol105_model_desc> d(R.size());
+        std::vector<sol105_model_desc> d(jobs.size());
         int mm = 1;
-        for (size_t i = 0; i < R.size(); i++) {
-            sol105_model *M = S->models[R[i]->model]->M.get();
+        for (size_t i = 0; i < jobs.size(); i++) {
+            sol105_model *M = jobs[i].m->M.get();
+            const Request &r = *jobs[i].r;
             // the registered model takes the design too, so a later solve of
             // it alone starts from the same thicknesses
-            if (!R[i]->t.empty()) (void)sol105_model_set_shell_thickness(ctx, M, R[i]->t.data());
+            if (!r.t.empty()) (void)sol105_model_set_shell_thickness(ctx, M, r.t.data());
+            jobs[i].m->last_valid = false;
             sol105_model_desc &x = d[i];
             x.n_nodes = M->n_nodes;
             x.xyz = M->xyz.data();
@@ -176,17 +219,20 @@ void run_group(sol105_server *S, sol105_ctx *ctx, const std::vector<int64_t> &gr
             x.k6rot = M->k6rot;
             x.autospc = M->autospc;
             x.nd = M->nd;
-            mm = std::max(mm, R[i]->max_modes);
+            mm = std::max(mm, r.max_modes);
         }
         sol105_batch *B = nullptr;
         int rc = sol105_batch_create(ctx, d.data(), (int32_t)d.size(), &B);
-        std::vector<double> lam((size_t)mm * R.size());
-        std::vector<int32_t> n(R.size()), st(R.size());
+        std::vector<double> lam((size_t)mm * jobs.size());
+        std::vector<int32_t> n(jobs.size()), st(jobs.size());
         if (rc == SOL105_OK) rc = sol105_batch_solve(ctx, B, mm, lam.data(), n.data(), st.data());
-        for (size_t i = 0; i < R.size(); i++) {
-            Request &r = *R[i];
-            r.members = (int)R.size();
+        double bst[8] = {0};
+        if (rc == SOL105_OK) (void)sol105_batch_stats(ctx, B, bst);
+        for (size_t i = 0; i < jobs.size(); i++) {
+            Request &r = *jobs[i].r;
+            r.members = (int)jobs.size();
             r.status = rc != SOL105_OK ? rc : st[i];
+            r.lanczos_vectors = bst[1];
             if (r.status == SOL105_OK) {
                 int k = std::min<int>(n[i], r.max_modes);
                 r.lambda.assign(lam.begin() + (size_t)i * mm, lam.begin() + (size_t)i * mm + k);
@@ -195,12 +241,13 @@ void run_group(sol105_server *S, sol105_ctx *ctx, const std::vector<int64_t> &gr
         if (B) sol105_batch_free(ctx, B);
     }
     Clock::time_point fin = Clock::now();
-    for (Request *r : R) r->finished = fin;
+    for (const Job &j : jobs) j.r->finished = fin;
 }
 
 void worker(sol105_server *S) {
     sol105_ctx *ctx = nullptr;
     if (sol105_ctx_create(S->device, &ctx) != SOL105_OK) return;
+    if (S->warm_start) (void)sol105_ctx_set_option(ctx, SOL105_OPT_WARM_START, 1);
     std::unique_lock<std::mutex> lk(S->mu);
     while (true) {
         double fp = 0.0;
@@ -211,20 +258,25 @@ void worker(sol105_server *S) {
             return !group.empty();
         });
         if (group.empty()) break;  // stopping with an empty queue
+        std::vector<Job> jobs;
         for (int64_t tk : group) {
             S->queue.erase(std::find(S->queue.begin(), S->queue.end(), tk));
-            S->models[S->reqs[tk].model]->busy = true;
+            Request *r = &S->reqs.at(tk);  // element addresses survive rehashing
+            SrvModel *m = S->models[r->model].get();
+            m->busy = true;
+            jobs.push_back({r, m});
         }
         S->inflight += fp;
+        S->peak_inflight = std::max(S->peak_inflight, S->inflight);
         lk.unlock();
-        run_group(S, ctx, group);
+        run_group(ctx, jobs);
         lk.lock();
         S->inflight -= fp;
         if (S->inflight < 1.0) S->inflight = 0.0;
-        for (int64_t tk : group) {
-            Request &r = S->reqs[tk];
-            r.done = true;
-            S->models[r.model]->busy = false;
+        for (const Job &j : jobs) {
+            j.r->done = true;
+            j.m->busy = false;
+            S->n_cached += j.r->cached;
         }
         S->n_done += (int64_t)group.size();
         if (group.size() > 1) {
@@ -251,6 +303,8 @@ void fill_result(const Request &r, const sol105_server *S, double *lambda_out, i
         info[3] = r.members;
         info[4] = ms_since(S->t0, r.started);
         info[5] = ms_since(S->t0, r.finished);
+        info[6] = r.lanczos_vectors;
+        info[7] = r.cached ? 1.0 : 0.0;
     }
 }
 
@@ -258,7 +312,7 @@ void fill_result(const Request &r, const sol105_server *S, double *lambda_out, i
 
 extern "C" {
 
-int sol105_server_create(int device, int32_t n_workers, double hbm_budget_bytes, int32_t batching,
+int sol105_server_create(int device, int32_t n_workers, double hbm_budget_bytes, int32_t flags,
                          sol105_server **out) {
     if (!out || n_workers <= 0) return SOL105_E_INVALID;
     *out = nullptr;
@@ -268,7 +322,8 @@ int sol105_server_create(int device, int32_t n_workers, double hbm_budget_bytes,
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
     S->budget = hbm_budget_bytes > 0.0 ? hbm_budget_bytes : 0.75 * (double)free_b;
-    S->batching = batching != 0;
+    S->batching = (flags & SOL105_SERVER_STACKING) != 0;
+    S->warm_start = (flags & SOL105_SERVER_WARM_START) != 0;
     for (int w = 0; w < n_workers; w++) S->workers.emplace_back(worker, S.get());
     *out = S.release();
     return SOL105_OK;
@@ -336,6 +391,7 @@ int sol105_server_poll(sol105_server *S, int64_t ticket, double *lambda_out, int
     if (!it->second.done) return SOL105_E_PENDING;
     fill_result(it->second, S, lambda_out, n_out, status, info);
     S->reqs.erase(it);
+    S->cv_done.notify_all();  // a waiter of this ticket sees it released
     return SOL105_OK;
 }
 
@@ -343,11 +399,20 @@ int sol105_server_wait(sol105_server *S, int64_t ticket, double *lambda_out, int
                        double *info) {
     if (!S) return SOL105_E_INVALID;
     std::unique_lock<std::mutex> lk(S->mu);
+    if (S->reqs.find(ticket) == S->reqs.end()) return srv_err(S, SOL105_E_INVALID, "unknown ticket");
+    // look the ticket up again after every wake-up: submits may rehash the
+    // map, and a concurrent poll or wait of the same ticket may release it
+    bool released = false;
+    S->cv_done.wait(lk, [&] {
+        auto it = S->reqs.find(ticket);
+        released = it == S->reqs.end();
+        return released || it->second.done;
+    });
+    if (released) return srv_err(S, SOL105_E_INVALID, "ticket released by another caller");
     auto it = S->reqs.find(ticket);
-    if (it == S->reqs.end()) return srv_err(S, SOL105_E_INVALID, "unknown ticket");
-    S->cv_done.wait(lk, [&] { return it->second.done; });
     fill_result(it->second, S, lambda_out, n_out, status, info);
     S->reqs.erase(it);
+    S->cv_done.notify_all();  // other waiters of this ticket see it released
     return SOL105_OK;
 }
 
@@ -362,6 +427,8 @@ int sol105_server_stats(sol105_server *S, double *out) {
     out[5] = S->budget;
     out[6] = (double)S->workers.size();
     out[7] = (double)S->models.size();
+    out[8] = (double)S->n_cached;
+    out[9] = S->peak_inflight;
     return SOL105_OK;
 }
 
