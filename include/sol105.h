@@ -245,6 +245,11 @@ int sol105_server_add_model(sol105_server *server, const sol105_model_desc *desc
  * current design.  max_modes <= 16. */
 int sol105_server_submit(sol105_server *server, int32_t model_id, const double *shell_thickness,
                          int32_t max_modes, int64_t *ticket);
+/* As sol105_server_submit; flags SOL105_REQ_MODES also keeps the request's
+ * mode shapes for sol105_server_wait_modes. */
+#define SOL105_REQ_MODES 1
+int sol105_server_submit_ex(sol105_server *server, int32_t model_id, const double *shell_thickness,
+                            int32_t max_modes, int32_t flags, int64_t *ticket);
 /* SOL105_OK with the results (then the ticket is released) or
  * SOL105_E_PENDING.  lambda_out: max_modes values; status: the request's
  * own result code; info (may be NULL): latency ms (submit -> done), queue
@@ -258,6 +263,11 @@ int sol105_server_poll(sol105_server *server, int64_t ticket, double *lambda_out
  * when another caller released the ticket meanwhile */
 int sol105_server_wait(sol105_server *server, int64_t ticket, double *lambda_out, int32_t *n_out,
                        int32_t *status, double *info /* 8 */);
+/* as sol105_server_wait, also copying the mode shapes of a request submitted
+ * with SOL105_REQ_MODES (n_out x 6 n_nodes doubles, mode-major, max |component|
+ * = 1) into modes_out (room for max_modes x 6 n_nodes; may be NULL) */
+int sol105_server_wait_modes(sol105_server *server, int64_t ticket, double *lambda_out, int32_t *n_out,
+                             int32_t *status, double *info /* 8 */, double *modes_out);
 /* out[10]: requests done, stacked solves, requests in them, queued, bytes in
  * flight, budget, workers, registered meshes, requests answered from a
  * repeated design, largest bytes in flight so far */

@@ -27,7 +27,8 @@ EXPORTS = [
     "sol105_sensitivity_terms", "sol105_profile_enable", "sol105_profile_read", "sol105_last_timings",
     "sol105_last_stats", "sol105_batch_create", "sol105_batch_free", "sol105_batch_solve", "sol105_batch_modes",
     "sol105_batch_stats", "sol105_server_create", "sol105_server_destroy", "sol105_server_last_error",
-    "sol105_server_add_model", "sol105_server_submit", "sol105_server_poll", "sol105_server_wait",
+    "sol105_server_add_model", "sol105_server_submit", "sol105_server_submit_ex", "sol105_server_poll",
+    "sol105_server_wait", "sol105_server_wait_modes",
     "sol105_server_stats",
 ]
 
@@ -108,6 +109,9 @@ def lib():
         L.sol105_server_submit.argtypes = [vp, ctypes.c_int32, dp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]
         L.sol105_server_poll.argtypes = [vp, ctypes.c_int64, dp, ip, ip, dp]
         L.sol105_server_wait.argtypes = [vp, ctypes.c_int64, dp, ip, ip, dp]
+        L.sol105_server_submit_ex.argtypes = [vp, ctypes.c_int32, dp, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.POINTER(ctypes.c_int64)]
+        L.sol105_server_wait_modes.argtypes = [vp, ctypes.c_int64, dp, ip, ip, dp, dp]
         L.sol105_server_stats.argtypes = [vp, dp]
         _lib = L
     return _lib
@@ -379,6 +383,7 @@ class Server:
             raise Sol105Error(rc, "could not create the server")
         self.h = h
         self.arrs = []
+        self._tickets = {}
 
     def _check(self, rc):
         if rc != SOL105_OK:
@@ -402,14 +407,32 @@ class Server:
         self.arrs.append(arr)
         return int(mid.value)
 
-    def submit(self, model_id, thickness=None, max_modes=16):
+    def submit(self, model_id, thickness=None, max_modes=16, modes=False):
+        """Ticket of a new request; with modes=True the request keeps its mode
+        shapes for wait_modes()."""
         t = None if thickness is None else np.ascontiguousarray(thickness, dtype=np.float64)
         if t is not None:
             assert t.shape == (self.arrs[model_id].n_shell,)
         tk = ctypes.c_int64()
-        self._check(self.L.sol105_server_submit(self.h, int(model_id), _dp(t) if t is not None else None,
-                                                int(max_modes), ctypes.byref(tk)))
+        self._check(self.L.sol105_server_submit_ex(self.h, int(model_id), _dp(t) if t is not None else None,
+                                                   int(max_modes), 1 if modes else 0, ctypes.byref(tk)))
+        self._tickets[int(tk.value)] = (int(model_id), int(max_modes))
         return int(tk.value)
+
+    def wait_modes(self, ticket):
+        """(eigenvalues, status, info, modes 6 n_nodes x n) of a request
+        submitted with modes=True, blocking until it is done."""
+        mid, mm = self._tickets.pop(ticket, (0, 16))
+        ndof = self.arrs[mid].ndof
+        lam = np.zeros(16)
+        modes = np.zeros((max(mm, 1), ndof))
+        n, st, info = ctypes.c_int32(), ctypes.c_int32(), np.zeros(8)
+        self._check(self.L.sol105_server_wait_modes(self.h, ctypes.c_int64(ticket), _dp(lam), ctypes.byref(n),
+                                                    ctypes.byref(st), _dp(info), _dp(modes)))
+        keys = ("latency_ms", "queue_ms", "service_ms", "members", "start_ms", "finish_ms", "lanczos_vectors",
+                "cached")
+        k = n.value
+        return lam[:k].copy(), int(st.value), dict(zip(keys, info.tolist())), modes[:k].T.copy()
 
     def _result(self, fn, ticket):
         lam = np.zeros(16)
@@ -419,6 +442,7 @@ class Server:
         rc = fn(self.h, ctypes.c_int64(ticket), _dp(lam), ctypes.byref(n), ctypes.byref(st), _dp(info))
         if rc == SOL105_E_PENDING:
             return None
+        self._tickets.pop(ticket, None)
         self._check(rc)
         keys = ("latency_ms", "queue_ms", "service_ms", "members", "start_ms", "finish_ms", "lanczos_vectors",
                 "cached")

@@ -59,7 +59,7 @@ struct SrvModel {
     // the last design solved alone and its results (a request that repeats
     // it is answered without a solve)
     bool last_valid = false;
-    std::vector<double> last_t, last_lambda;
+    std::vector<double> last_t, last_lambda, last_modes;  // last_modes empty when not kept
     int last_max_modes = 0;
 };
 
@@ -73,7 +73,9 @@ struct Request {
     int members = 1;
     double lanczos_vectors = 0.0;  // basis vectors its eigensolve generated
     bool cached = false;           // answered from the last solve of the same design
+    bool want_modes = false;       // keep the mode shapes (SOL105_REQ_MODES)
     std::vector<double> lambda;
+    std::vector<double> modes;     // n_out x 6 n_nodes, mode-major
 };
 
 // a request and its mesh, both resolved under the server lock
@@ -166,24 +168,32 @@ void run_group(sol105_ctx *ctx, const std::vector<Job> &jobs) {
         sol105_model *M = SM.M.get();
         const std::vector<double> t = design_of(r, M);
         r.members = 1;
-        if (SM.last_valid && t == SM.last_t && r.max_modes <= SM.last_max_modes) {
+        const size_t ndof = (size_t)M->ndof();
+        if (SM.last_valid && t == SM.last_t && r.max_modes <= SM.last_max_modes &&
+            (!r.want_modes || !SM.last_modes.empty())) {
+            const size_t k = std::min<size_t>(SM.last_lambda.size(), (size_t)r.max_modes);
             r.status = SOL105_OK;
             r.cached = true;
-            r.lambda.assign(SM.last_lambda.begin(),
-                            SM.last_lambda.begin() + std::min<size_t>(SM.last_lambda.size(), (size_t)r.max_modes));
+            r.lambda.assign(SM.last_lambda.begin(), SM.last_lambda.begin() + k);
+            if (r.want_modes) r.modes.assign(SM.last_modes.begin(), SM.last_modes.begin() + k * ndof);
         } else {
             int rc = SOL105_OK;
             if (!r.t.empty()) rc = sol105_model_set_shell_thickness(ctx, M, r.t.data());
             std::vector<double> lam(std::max(r.max_modes, 1));
+            std::vector<double> modes(r.want_modes ? (size_t)std::max(r.max_modes, 1) * ndof : 0);
             int32_t n = 0;
-            if (rc == SOL105_OK) rc = sol105_solve(ctx, M, r.max_modes, lam.data(), nullptr, &n);
+            if (rc == SOL105_OK)
+                rc = sol105_solve(ctx, M, r.max_modes, lam.data(), r.want_modes ? modes.data() : nullptr, &n);
             r.status = rc;
             SM.last_valid = rc == SOL105_OK;
             if (rc == SOL105_OK) {
                 r.lambda.assign(lam.begin(), lam.begin() + n);
+                if (r.want_modes) modes.resize((size_t)n * ndof);
+                r.modes = modes;
                 r.lanczos_vectors = M->lanczos_m;
                 SM.last_t = t;
                 SM.last_lambda = r.lambda;
+                SM.last_modes = std::move(modes);
                 SM.last_max_modes = r.max_modes;
             }
         }
@@ -236,6 +246,16 @@ void run_group(sol105_ctx *ctx, const std::vector<Job> &jobs) {
             if (r.status == SOL105_OK) {
                 int k = std::min<int>(n[i], r.max_modes);
                 r.lambda.assign(lam.begin() + (size_t)i * mm, lam.begin() + (size_t)i * mm + k);
+                if (r.want_modes) {
+                    const size_t ndof = (size_t)jobs[i].m->M->ndof();
+                    std::vector<double> all((size_t)std::max(n[i], 1) * ndof);
+                    if (sol105_batch_modes(ctx, B, (int32_t)i, all.data()) == SOL105_OK) {
+                        all.resize((size_t)k * ndof);
+                        r.modes = std::move(all);
+                    } else {
+                        r.status = SOL105_E_DEVICE;
+                    }
+                }
             }
         }
         if (B) sol105_batch_free(ctx, B);
@@ -291,7 +311,9 @@ void worker(sol105_server *S) {
 }
 
 void fill_result(const Request &r, const sol105_server *S, double *lambda_out, int32_t *n_out, int32_t *status,
-                 double *info) {
+                 double *info, double *modes_out = nullptr) {
+    if (modes_out)
+        for (size_t t = 0; t < r.modes.size(); t++) modes_out[t] = r.modes[t];
     if (n_out) *n_out = (int32_t)r.lambda.size();
     if (status) *status = r.status;
     if (lambda_out)
@@ -360,6 +382,11 @@ int sol105_server_add_model(sol105_server *S, const sol105_model_desc *desc, int
 
 int sol105_server_submit(sol105_server *S, int32_t model_id, const double *shell_thickness, int32_t max_modes,
                          int64_t *ticket) {
+    return sol105_server_submit_ex(S, model_id, shell_thickness, max_modes, 0, ticket);
+}
+
+int sol105_server_submit_ex(sol105_server *S, int32_t model_id, const double *shell_thickness, int32_t max_modes,
+                            int32_t flags, int64_t *ticket) {
     if (!S || !ticket || max_modes <= 0) return srv_err(S, SOL105_E_INVALID, "null argument");
     std::lock_guard<std::mutex> lk(S->mu);
     if (model_id < 0 || model_id >= (int32_t)S->models.size())
@@ -368,6 +395,7 @@ int sol105_server_submit(sol105_server *S, int32_t model_id, const double *shell
     Request r;
     r.model = model_id;
     r.max_modes = std::min<int32_t>(max_modes, MAXR);
+    r.want_modes = (flags & SOL105_REQ_MODES) != 0;
     if (shell_thickness) {
         r.t.assign(shell_thickness, shell_thickness + M->n_q4 + M->n_t3);
         for (double v : r.t)
@@ -397,6 +425,11 @@ int sol105_server_poll(sol105_server *S, int64_t ticket, double *lambda_out, int
 
 int sol105_server_wait(sol105_server *S, int64_t ticket, double *lambda_out, int32_t *n_out, int32_t *status,
                        double *info) {
+    return sol105_server_wait_modes(S, ticket, lambda_out, n_out, status, info, nullptr);
+}
+
+int sol105_server_wait_modes(sol105_server *S, int64_t ticket, double *lambda_out, int32_t *n_out, int32_t *status,
+                             double *info, double *modes_out) {
     if (!S) return SOL105_E_INVALID;
     std::unique_lock<std::mutex> lk(S->mu);
     if (S->reqs.find(ticket) == S->reqs.end()) return srv_err(S, SOL105_E_INVALID, "unknown ticket");
@@ -410,7 +443,7 @@ int sol105_server_wait(sol105_server *S, int64_t ticket, double *lambda_out, int
     });
     if (released) return srv_err(S, SOL105_E_INVALID, "ticket released by another caller");
     auto it = S->reqs.find(ticket);
-    fill_result(it->second, S, lambda_out, n_out, status, info);
+    fill_result(it->second, S, lambda_out, n_out, status, info, modes_out);
     S->reqs.erase(it);
     S->cv_done.notify_all();  // other waiters of this ticket see it released
     return SOL105_OK;

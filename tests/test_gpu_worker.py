@@ -1,0 +1,61 @@
+"""The JSONL worker process on the GPU through the client module (the
+interface of the reference's jfem_client.py): two iterations of a design
+loop, each a manifest of decks; buckling loads against the oracle within
+1e-8 and mode shapes by MAC; the second iteration's designs of the same
+meshes reuse the registered meshes."""
+import json
+
+import numpy as np
+import pytest
+
+from jf_2026_05_19_amd import bdf, decks, model as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _mac(a, b):
+    return float((a @ b) ** 2 / ((a @ a) * (b @ b)))
+
+
+def test_worker_design_loop_against_oracle(tmp_path):
+    from jf_2026_05_19_amd import client
+    from oracle import oracle as O
+    texts = {"panel": lambda t: decks.panel_deck(nx=24, ny=12, a=240.0, b=120.0, t=t, stringer_every=4, nd=4,
+                                                 tri_frac=0.1),
+             "plate": lambda t: decks.plate_deck(nx=10, ny=8, a=400.0, b=320.0, t=t, nd=3)}
+    with client.JFEMWorker() as w:
+        assert w.ready["status"] == "ready"
+        for it, thick in enumerate(((1.5, 2.0), (1.55, 1.9))):
+            cases = []
+            for name, mk in texts.items():
+                for k, t in enumerate(thick):
+                    p = tmp_path / ("%s_%d_it%d.bdf" % (name, k, it))
+                    p.write_text(mk(t))
+                    cases.append({"input": str(p), "case_id": "%s_%d" % (name, k)})
+            cases[0]["output_options"] = {"eigenvectors": True}
+            man = client.write_batch_manifest(tmp_path / ("it%d.json" % it), cases, tmp_path / ("out%d" % it),
+                                              batch_id="it%d" % it,
+                                              output_options={"json": True, "eigenvalues_only": True,
+                                                              "report": False})
+            print("iteration", it, flush=True)
+            r = w.run_batch(man, request_id="r%d" % it)
+            assert r["status"] == "ok" and r["completed"] == 4 and r["id"] == "r%d" % it
+            rows = client.load_summary(r["summary_json"])["cases"]
+            for row in rows:
+                arr = M.build_arrays(bdf.bdf_to_model_text(open(row["input"]).read()))
+                ref = O.solve(arr)
+                lam = np.array(row["eigenvalues"])
+                assert lam.shape == ref["eigenvalues"].shape
+                assert np.max(np.abs(lam - ref["eigenvalues"]) / np.abs(ref["eigenvalues"])) <= 1e-8
+                if row["case_id"] == "panel_0":
+                    bj = json.load(open(row["result_json"]))
+                    lr = ref["eigenvalues"]
+                    for i, m in enumerate(bj["modes"]):
+                        gap = min([abs(lr[i] - lr[j]) / abs(lr[i]) for j in range(len(lr)) if j != i] + [1.0])
+                        if gap < 1e-6:
+                            continue
+                        shape = np.array([[d[c] for c in ("t1", "t2", "t3", "r1", "r2", "r3")]
+                                          for d in m["mode_shape"]]).ravel()
+                        assert _mac(shape, ref["modes"][:, i]) >= 1 - 1e-8
+            st = w.status()
+            assert st["meshes"] == 2          # two meshes, every deck a design of one of them
