@@ -59,6 +59,10 @@ typedef struct sol105_model_desc {
     double k6rot;               /* PARAM,K6ROT (default 100)                      */
     int32_t autospc;            /* PARAM,AUTOSPC (default 1)                      */
     int32_t nd;                 /* EIGRL ND                                       */
+    double eigrl_v1;            /* EIGRL V1, V2: a range when V2 > V1 and either  */
+    double eigrl_v2;            /*   is non-zero (blank fields: 0)                */
+    int32_t n_shell_cards;      /* shell cards of the deck (0: n_q4 + n_t3); the  */
+                                /*   mode filters act on decks with >= 100        */
 } sol105_model_desc;
 
 /* Device selection and context.  No reference counterpart (the reference is
@@ -138,12 +142,32 @@ int sol105_static_solve(sol105_ctx *ctx, sol105_model *model, double *u, uint8_t
 int sol105_assemble_KG(sol105_ctx *ctx, sol105_model *model, const double *u, double *vals);
 
 /* Buckling eigensolve with the reference's output semantics (mode count,
- * positive and localization filters, max-abs normalisation).  Replaces
- * Solver.solve_buckling (JFEM/src/solver/solve_case.jl:1863-3205).
- * lambda_out has room for max_modes values, modes_out (may be NULL) for
- * max_modes x 6*n_nodes doubles (mode-major).  *n_out receives the count. */
+ * positive, EIGRL range, localization and cluster filters, max-abs
+ * normalisation).  Replaces Solver.solve_buckling
+ * (JFEM/src/solver/solve_case.jl:1863-3205).  lambda_out has room for
+ * max_modes values, modes_out (may be NULL) for max_modes x 6*n_nodes
+ * doubles (mode-major).  *n_out receives the count.
+ * EIGRL range: the reference keeps the whole spectrum of a free set of at
+ * most 4000 DOFs, and above that takes the ceil(3 ND) roots of smallest
+ * |lambda| plus a shift-invert search near V2 (solve_case.jl:2131-2196,
+ * 2422-2560).  Here a free set of at most 64 DOFs gets every pair; up to 4000
+ * DOFs the eigensolve returns the smallest positive roots (ceil(3 ND), then
+ * 62 when those do not decide the output, SOL105_E_UNSUPPORTED when 62 do
+ * not either); above 4000 DOFs the ceil(3 ND) roots of smallest |lambda| and
+ * no shift-invert search.  sol105_buckling_info tells whether the computed
+ * roots decided the output. */
 int sol105_buckling(sol105_ctx *ctx, sol105_model *model, int32_t max_modes,
                     double *lambda_out, double *modes_out, int32_t *n_out);
+
+/* Selection record of the model's last buckling eigensolve:
+ * [0] eigenpairs computed, [1] positive candidates in the EIGRL range,
+ * [2] modes skipped by the cluster filter, [3] 1 when the computed pairs
+ * decide the output as the complete in-range spectrum would (always without
+ * a range), 0 when further in-range eigenvalues above the computed ones
+ * could change it (the output then is the reference's with
+ * JFEM_SOL105_SKIP_RANGE_AUGMENTATION=true on a free set above 4000 DOFs).
+ * No counterpart in the reference (its diagnostics dictionary). */
+int sol105_buckling_info(const sol105_ctx *ctx, const sol105_model *model, double *out /* 4 */);
 
 /* End-to-end SOL 105 solve (K -> u -> KG -> buckling) on the device.
  * Replaces the SOL 105 part of solve_model/_solve_sol105
@@ -242,7 +266,7 @@ int sol105_server_destroy(sol105_server *server);  /* finishes queued requests f
 const char *sol105_server_last_error(const sol105_server *server);
 int sol105_server_add_model(sol105_server *server, const sol105_model_desc *desc, int32_t *model_id);
 /* shell_thickness (n_q4 + n_t3, CQUAD4 first) may be NULL: the mesh's
- * current design.  max_modes <= 16. */
+ * current design.  max_modes above 64 is taken as 64. */
 int sol105_server_submit(sol105_server *server, int32_t model_id, const double *shell_thickness,
                          int32_t max_modes, int64_t *ticket);
 /* As sol105_server_submit; flags SOL105_REQ_MODES also keeps the request's

@@ -18,12 +18,13 @@ SOL105_OK = 0
 ERRORS = {-1: "SOL105_E_INVALID", -2: "SOL105_E_UNSUPPORTED", -3: "SOL105_E_DEVICE", -4: "SOL105_E_SINGULAR",
           -5: "SOL105_E_NOCONV", -6: "SOL105_E_STATE", -7: "SOL105_E_PENDING"}
 SOL105_E_PENDING = -7
+SERVER_MAX_MODES = 64   # a server request keeps at most this many modes
 
 EXPORTS = [
     "sol105_ctx_create", "sol105_ctx_destroy", "sol105_last_error", "sol105_ctx_set_option", "sol105_model_upload", "sol105_model_free",
     "sol105_model_set_q4_thickness", "sol105_model_set_shell_thickness", "sol105_pattern", "sol105_assemble_K",
     "sol105_assemble_K_noalloc", "sol105_static_solve",
-    "sol105_assemble_KG", "sol105_buckling", "sol105_solve", "sol105_sensitivity_thickness",
+    "sol105_assemble_KG", "sol105_buckling", "sol105_buckling_info", "sol105_solve", "sol105_sensitivity_thickness",
     "sol105_sensitivity_terms", "sol105_profile_enable", "sol105_profile_read", "sol105_last_timings",
     "sol105_last_stats", "sol105_batch_create", "sol105_batch_free", "sol105_batch_solve", "sol105_batch_modes",
     "sol105_batch_stats", "sol105_server_create", "sol105_server_destroy", "sol105_server_last_error",
@@ -55,6 +56,7 @@ class ModelDesc(ctypes.Structure):
         ("spc_mask", ctypes.POINTER(ctypes.c_uint8)), ("force", ctypes.POINTER(ctypes.c_double)),
         ("node_has_shell", ctypes.POINTER(ctypes.c_uint8)), ("node_has_bar", ctypes.POINTER(ctypes.c_uint8)),
         ("k6rot", ctypes.c_double), ("autospc", ctypes.c_int32), ("nd", ctypes.c_int32),
+        ("eigrl_v1", ctypes.c_double), ("eigrl_v2", ctypes.c_double), ("n_shell_cards", ctypes.c_int32),
     ]
 
 
@@ -88,6 +90,7 @@ def lib():
         L.sol105_assemble_KG.argtypes = [vp, vp, dp, dp]
         L.sol105_buckling.argtypes = [vp, vp, ctypes.c_int32, dp, dp, ctypes.POINTER(ctypes.c_int32)]
         L.sol105_solve.argtypes = [vp, vp, ctypes.c_int32, dp, dp, ctypes.POINTER(ctypes.c_int32)]
+        L.sol105_buckling_info.argtypes = [vp, vp, dp]
         L.sol105_sensitivity_thickness.argtypes = [vp, vp, ctypes.c_int32, dp]
         L.sol105_sensitivity_terms.argtypes = [vp, vp, ctypes.c_int32, dp, dp, dp]
         L.sol105_profile_enable.argtypes = [vp, ctypes.c_int32]
@@ -202,7 +205,8 @@ def _model_desc(arr):
     d = ModelDesc(arr.n_nodes, _dp(k["xyz"]), arr.n_q4, ip(k["q4"]), _dp(k["q4prop"]),
                   arr.n_t3, ip(k["t3"]), _dp(k["t3prop"]), arr.n_bar, ip(k["bar"]), _dp(k["vref"]), _dp(k["barprop"]),
                   up(k["spc"]), _dp(k["F"]), up(k["hs"]), up(k["hb"]),
-                  float(arr.k6rot), int(arr.autospc), int(arr.nd))
+                  float(arr.k6rot), int(arr.autospc), int(arr.nd),
+                  float(arr.v1), float(arr.v2), int(arr.n_shell_cards))
     return d, keep
 
 
@@ -274,6 +278,15 @@ class DeviceModel:
                                                   _dp(modes) if want_modes else None, ctypes.byref(n)))
         k = n.value
         return lam[:k].copy(), (modes[:k].T.copy() if want_modes else None)
+
+    def buckling_info(self):
+        """Selection record of the last eigensolve: eigenpairs computed,
+        positive candidates in the EIGRL range, modes the cluster filter
+        skipped, and whether the computed pairs decide the output as the
+        complete in-range spectrum would."""
+        out = np.zeros(4)
+        self.ctx.check(self.ctx.L.sol105_buckling_info(self.ctx.h, self.h, _dp(out)))
+        return dict(pairs=int(out[0]), in_range=int(out[1]), cluster_skip=int(out[2]), decided=bool(out[3]))
 
     def solve(self, max_modes=64, want_modes=True):
         lam = np.zeros(max_modes)
@@ -416,15 +429,15 @@ class Server:
         tk = ctypes.c_int64()
         self._check(self.L.sol105_server_submit_ex(self.h, int(model_id), _dp(t) if t is not None else None,
                                                    int(max_modes), 1 if modes else 0, ctypes.byref(tk)))
-        self._tickets[int(tk.value)] = (int(model_id), int(max_modes))
+        self._tickets[int(tk.value)] = (int(model_id), min(int(max_modes), SERVER_MAX_MODES))
         return int(tk.value)
 
     def wait_modes(self, ticket):
         """(eigenvalues, status, info, modes 6 n_nodes x n) of a request
         submitted with modes=True, blocking until it is done."""
-        mid, mm = self._tickets.pop(ticket, (0, 16))
+        mid, mm = self._tickets.pop(ticket, (0, SERVER_MAX_MODES))
         ndof = self.arrs[mid].ndof
-        lam = np.zeros(16)
+        lam = np.zeros(SERVER_MAX_MODES)
         modes = np.zeros((max(mm, 1), ndof))
         n, st, info = ctypes.c_int32(), ctypes.c_int32(), np.zeros(8)
         self._check(self.L.sol105_server_wait_modes(self.h, ctypes.c_int64(ticket), _dp(lam), ctypes.byref(n),
@@ -435,7 +448,7 @@ class Server:
         return lam[:k].copy(), int(st.value), dict(zip(keys, info.tolist())), modes[:k].T.copy()
 
     def _result(self, fn, ticket):
-        lam = np.zeros(16)
+        lam = np.zeros(SERVER_MAX_MODES)
         n = ctypes.c_int32()
         st = ctypes.c_int32()
         info = np.zeros(8)

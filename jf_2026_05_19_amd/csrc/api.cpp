@@ -8,6 +8,7 @@
 //   sol105_buckling      Solver.solve_buckling         JFEM/src/solver/solve_case.jl:1863-3205
 //   sol105_sensitivity_thickness  solve_adjoint_buckling  JFEM/src/solver/adjoint_buckling.jl:2406-2725
 #include "internal.h"
+#include "select.h"
 
 #include <cmath>
 #include <cstdarg>
@@ -209,6 +210,8 @@ int model_init(sol105_ctx *ctx, const sol105_model_desc *d, sol105_model *M) {
     if (!d->spc_mask || !d->force || !d->node_has_shell || !d->node_has_bar)
         return set_err(ctx, SOL105_E_INVALID, "missing SPC / load / topology arrays");
     if (d->nd <= 0) return set_err(ctx, SOL105_E_INVALID, "EIGRL ND must be positive");
+    if (!std::isfinite(d->eigrl_v1) || !std::isfinite(d->eigrl_v2) || d->n_shell_cards < 0)
+        return set_err(ctx, SOL105_E_INVALID, "EIGRL V1/V2 must be finite and the shell card count non-negative");
     (void)hipSetDevice(ctx->device);
     int n = d->n_nodes;
     M->n_nodes = n;
@@ -230,6 +233,9 @@ int model_init(sol105_ctx *ctx, const sol105_model_desc *d, sol105_model *M) {
     M->k6rot = d->k6rot;
     M->autospc = d->autospc;
     M->nd = d->nd;
+    M->v1 = d->eigrl_v1;
+    M->v2 = d->eigrl_v2;
+    M->n_shell_cards = d->n_shell_cards > 0 ? d->n_shell_cards : d->n_q4 + d->n_t3;
     for (int32_t v : M->q4)
         if (v < 0 || v >= n) return set_err(ctx, SOL105_E_INVALID, "CQUAD4 node rank %d out of range", v);
     for (int32_t v : M->t3)
@@ -719,8 +725,10 @@ struct LanczosOut {
 // seeded random vectors orthogonalised against the whole basis.  The
 // Lanczos relation C V = V T + Q R E^T stays exact (the new directions enter
 // with zero coefficients), so Rayleigh-Ritz and the residual bound hold.
+// With algebraic set the wanted pairs are those of largest theta (smallest
+// positive lambda) instead of largest |theta|.
 int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want, elimination order */,
-            LanczosOut &out) {
+            LanczosOut &out, bool algebraic = false) {
     hipStream_t s = ctx->stream;
     int n = M->sym.n_free, ndof = M->ndof();
     const bool debug = std::getenv("SOL105_DEBUG") != nullptr;
@@ -896,7 +904,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     auto restart = [&]() {
         symmetrize_copy(m, ldT, T.p, A.p, s);
         // block tridiagonal (half-bandwidth b) until the first restart
-        sym_eig_top(m, A.p, kk, thd.p, Zs.p, ework.p, (int64_t)ework.n, sw.p, s, restarts ? 0 : b);
+        sym_eig_top(m, A.p, kk, thd.p, Zs.p, ework.p, (int64_t)ework.n, sw.p, s, restarts ? 0 : b, algebraic);
         std::vector<double> th(kk), zl((size_t)b * kk), R((size_t)b * b);
         HIPCHK(hipMemcpyAsync(th.data(), thd.p, sizeof(double) * kk, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpy2DAsync(zl.data(), b * sizeof(double), Zs.p + (m - b), (size_t)m * sizeof(double),
@@ -945,7 +953,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
         symmetrize_copy(m, ldT, T.p, A.p, s);
         // the kchk Ritz pairs of largest |theta| (A is overwritten)
         const int kchk = std::min(kneed, m);
-        sym_eig_top(m, A.p, kchk, thd.p, Zs.p, ework.p, (int64_t)ework.n, sw.p, s, restarts ? 0 : b);
+        sym_eig_top(m, A.p, kchk, thd.p, Zs.p, ework.p, (int64_t)ework.n, sw.p, s, restarts ? 0 : b, algebraic);
         HIPCHK(hipGetLastError());
         std::vector<double> th(kchk), hZl((size_t)b * kchk);
         HIPCHK(hipMemcpyAsync(th.data(), thd.p, sizeof(double) * kchk, hipMemcpyDeviceToHost, s));
@@ -1006,8 +1014,10 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
             out.theta.assign(th.begin(), th.begin() + k);
             out.resid.clear();
             for (int t = 0; t < k; t++) out.resid.push_back(resid(t));
-            // Ritz vectors Y = V_m Z[:, 0:k]
-            ts_gemm_nn(n, m, k, V.p, n, Zs.p, m, d_Y, n, 1.0, 0.0, s);
+            // Ritz vectors Y = V_m Z[:, 0:k], 16 columns per product
+            for (int c0 = 0; c0 < k; c0 += 16)
+                ts_gemm_nn(n, m, std::min(16, k - c0), V.p, n, Zs.p + (size_t)c0 * m, m, d_Y + (size_t)c0 * n, n,
+                           1.0, 0.0, s);
             HIPCHK(hipStreamSynchronize(s));
             break;
         }
@@ -1034,6 +1044,57 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     return SOL105_OK;
 }
 
+// Every eigenpair of C = L^-1 (-KG) L^-T for a free set of at most
+// SYMEIG_MAX_K DOFs: C is formed column block by column block from the
+// operator applied to the identity (the same products as a Lanczos step),
+// symmetrised, and handed whole to the projected eigensolver; Y = the
+// eigenvectors (the basis is the identity), descending |theta|.
+int dense_pairs(sol105_ctx *ctx, sol105_model *M, double *d_Y, LanczosOut &out) {
+    hipStream_t s = ctx->stream;
+    const int n = M->sym.n_free, ndof = M->ndof();
+    LanczosWork &W = M->lw;
+    HIPCHK(W.V.alloc((size_t)n * n));
+    HIPCHK(W.Wb.alloc((size_t)n * MAXR));
+    HIPCHK(W.xf.alloc((size_t)ndof * MAXR));
+    HIPCHK(W.yf.alloc((size_t)ndof * MAXR));
+    HIPCHK(W.A.alloc((size_t)n * n));
+    HIPCHK(W.Zs.alloc((size_t)n * n));
+    HIPCHK(W.thd.alloc(n));
+    HIPCHK(W.ework.alloc(symeig_work_size(n, n)));
+    HIPCHK(W.sw.alloc(1));
+    HIPCHK(W.Ytmp.alloc((size_t)n * MAXR));
+    std::vector<double> eye((size_t)n * MAXR);
+    for (int c0 = 0; c0 < n; c0 += MAXR) {
+        const int b = std::min(MAXR, n - c0);
+        std::fill(eye.begin(), eye.end(), 0.0);
+        for (int j = 0; j < b; j++) eye[(size_t)j * n + c0 + j] = 1.0;
+        HIPCHK(hipMemcpyAsync(W.Ytmp.p, eye.data(), sizeof(double) * n * b, hipMemcpyHostToDevice, s));
+        HIPCHK(solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, W.Ytmp.p, W.Wb.p,
+                         b, s, ctx->timer()));
+        launch_scatter_elim_il(n, ndof, b, M->d_elim_of_dof.p, W.Wb.p, W.xf.p, s);
+        launch_bsr_spmv(M->n_nodes, M->d_rp.p, M->d_col.p, M->d_KG.p, W.xf.p, W.yf.p, b, -1.0, s);
+        launch_gather_elim_il(n, b, M->d_dof_of_elim.p, W.yf.p, W.Wb.p, s);
+        HIPCHK(solve_fwd(M->plan, M->dev, M->dsolve, M->splan, M->d_P.p, M->d_W.p, M->d_part.p, W.Wb.p,
+                         W.V.p + (size_t)c0 * n, b, s, ctx->timer()));
+        HIPCHK(hipStreamSynchronize(s));  // eye is reused by the next block
+    }
+    symmetrize_copy(n, n, W.V.p, W.A.p, s);
+    sym_eig_top(n, W.A.p, n, W.thd.p, W.Zs.p, W.ework.p, (int64_t)W.ework.n, W.sw.p, s, 0);
+    HIPCHK(hipGetLastError());
+    std::vector<double> th(n);
+    int info = 0;
+    HIPCHK(hipMemcpyAsync(th.data(), W.thd.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&info, W.sw.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(d_Y, W.Zs.p, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (info != 0) return set_err(ctx, SOL105_E_NOCONV, "dense eigensolve failed (n=%d)", n);
+    out.theta = th;
+    out.m = n;
+    out.generated = n;
+    M->y_cols = 0;
+    return SOL105_OK;
+}
+
 }  // namespace
 
 int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double *lambda_out, double *modes_out,
@@ -1043,76 +1104,109 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
     hipStream_t s = ctx->stream;
     if (!M->factor_valid) return set_err(ctx, SOL105_E_STATE, "no factorization: run sol105_static_solve first");
     if (!M->KG_valid) return set_err(ctx, SOL105_E_STATE, "no KG: run sol105_assemble_KG first");
-    int n = M->sym.n_free, ndof = M->ndof(), nd = M->nd;
-    bool dense = n <= 4000;
-    int req = dense ? 3 * nd : nd;
-    req = std::min(req, std::max(n - 2, 1));
-    if (req > MAXR) return set_err(ctx, SOL105_E_UNSUPPORTED, "more than %d requested modes", MAXR);
+    const int n = M->sym.n_free, ndof = M->ndof();
+    ModeSelect sel;
+    sel.nd = M->nd;
+    sel.n_free = n;
+    sel.v1 = M->v1;
+    sel.v2 = M->v2;
+    sel.n_shell_cards = M->n_shell_cards;
+    // a free set of at most SYMEIG_MAX_K DOFs: the dense path on the device,
+    // every pair of C = L^-1 (-KG) L^-T, as the reference's dense eigensolve
+    sel.all_pairs = n <= SYMEIG_MAX_K;
+    // the dense path under a range keeps the whole spectrum, of which only
+    // the positive roots in [V1, V2] can be output: the eigensolve targets
+    // the largest theta (the smallest positive lambda) there, ceil(3 ND)
+    // pairs first and, when those do not decide the output, as many as the
+    // projected eigensolver takes
+    const bool grow = sel.dense() && sel.has_range() && !sel.all_pairs;
+    const int req_max = std::min(SYMEIG_MAX_K - 2, std::max(n - 2, 1));
+    int req = sel.requested();
+    if (!sel.all_pairs && req > SYMEIG_MAX_K - 2) {
+        if (!grow) return set_err(ctx, SOL105_E_UNSUPPORTED, "more than %d requested modes", SYMEIG_MAX_K - 2);
+        req = req_max;
+    }
     DevBuf<double> &Y = M->lw.Y, &Xf = M->lw.Xf;
-    if (Y.n < (size_t)n * req) M->y_cols = 0;  // reallocated: no kept Ritz vectors
-    HIPCHK(Y.alloc((size_t)n * req));
-    LanczosOut lo;
-    int rc = lanczos(ctx, M, req, Y.p, lo);
-    if (rc) return rc;
-    int k = (int)lo.theta.size();
-    M->lanczos_m = lo.generated;
-    M->lanczos_steps = lo.steps;
-    // x = L^-T y, expand to all DOFs
-    HIPCHK(Xf.alloc((size_t)ndof * k));
-    if (k > 0) {
-        DevBuf<double> &Ys = M->lw.Ys;
-        HIPCHK(Ys.alloc((size_t)n * k));
-        HIPCHK(solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p, Y.p, Ys.p, k, s,
-                         ctx->timer()));
-        launch_scatter_elim(n, ndof, k, M->d_elim_of_dof.p, Ys.p, Xf.p, s);
-    }
-    // eigenvalue semantics: lambda = 1/theta, |theta| > 1e-12 (dense) or
-    // >= 1e-14 and |lambda| > 1e-6 (Krylov); order by |lambda|, keep req
-    std::vector<int> cand;
-    std::vector<double> lam(k);
-    for (int i = 0; i < k; i++) {
-        double th = lo.theta[i];
-        lam[i] = 1.0 / th;
-        bool ok = dense ? (std::isfinite(th) && std::fabs(th) > 1e-12)
-                        : (std::fabs(th) >= 1e-14 && std::fabs(lam[i]) > 1e-6);
-        if (ok) cand.push_back(i);
-    }
-    std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) { return std::fabs(lam[a]) < std::fabs(lam[b]); });
-    if ((int)cand.size() > req) cand.resize(req);
-    // positive filter (lambda > 1e-10), ascending
-    std::vector<int> pos;
-    for (int i : cand)
-        if (lam[i] > 1e-10) pos.push_back(i);
-    std::stable_sort(pos.begin(), pos.end(), [&](int a, int b) { return lam[a] < lam[b]; });
-    // localization filter (defaults: >= 100 shells, share 0.10, scan ND + 16)
-    std::vector<int> surv;
-    int n_shell = M->n_q4 + M->n_t3;
-    if (n_shell >= 100 && !pos.empty()) {
-        int limit = std::min((int)pos.size(), std::max(nd + 16, nd));
-        DevBuf<double> &scr = M->lw.scr, &sh = M->lw.sh;
-        HIPCHK(scr.alloc(2 * ((n_shell + 255) / 256) + 2));
-        HIPCHK(sh.alloc(2 * (size_t)limit));
-        for (int t = 0; t < limit; t++) elem_energy_share(M->n_q4, M->d_q4.p, M->n_t3, M->d_t3.p, ndof, Xf.p, pos[t], scr.p, sh.p + 2 * t, s);
-        std::vector<double> hsh(2 * (size_t)limit);
-        HIPCHK(hipMemcpyAsync(hsh.data(), sh.p, hsh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        for (int t = 0; t < limit; t++) {
-            double mx = hsh[2 * t], tot = hsh[2 * t + 1];
-            if (tot > 0 && mx / tot > 0.10) continue;
-            surv.push_back(pos[t]);
+    std::vector<double> lam;
+    std::vector<int> pos, surv;
+    bool decided = true;
+    for (;;) {
+        const int ycols = sel.all_pairs ? n : req;
+        if (Y.n < (size_t)n * ycols) M->y_cols = 0;  // reallocated: no kept Ritz vectors
+        HIPCHK(Y.alloc((size_t)n * ycols));
+        LanczosOut lo;
+        int rc = sel.all_pairs ? dense_pairs(ctx, M, Y.p, lo) : lanczos(ctx, M, req, Y.p, lo, grow);
+        if (rc) return rc;
+        const int k = (int)lo.theta.size();
+        M->lanczos_m = lo.generated;
+        M->lanczos_steps = lo.steps;
+        // x = L^-T y, expand to all DOFs (16 right-hand sides per triangular solve)
+        HIPCHK(Xf.alloc((size_t)ndof * std::max(k, 1)));
+        if (k > 0) {
+            DevBuf<double> &Ys = M->lw.Ys;
+            HIPCHK(Ys.alloc((size_t)n * k));
+            for (int c0 = 0; c0 < k; c0 += MAXR) {
+                const int nc = std::min(MAXR, k - c0);
+                HIPCHK(solve_bwd(M->plan, M->dev, M->dsolve, M->splan, M->d_PT.p, M->d_W.p, M->d_part.p,
+                                 Y.p + (size_t)c0 * n, Ys.p + (size_t)c0 * n, nc, s, ctx->timer()));
+            }
+            launch_scatter_elim(n, ndof, k, M->d_elim_of_dof.p, Ys.p, Xf.p, s);
         }
-        for (int t = limit; t < (int)pos.size(); t++) surv.push_back(pos[t]);
-    } else {
-        surv = pos;
+        // candidates: lambda = 1/theta, the requested count, positive, in range, ascending
+        pos = select_candidates(sel, lo.theta.data(), k, lam);
+        // localization filter (>= 100 shell cards, share 0.10, scan ND + 16)
+        surv.clear();
+        const int limit = localization_scan_limit(sel, (int)pos.size());
+        if (limit > 0) {
+            const int n_shell = M->n_q4 + M->n_t3;
+            DevBuf<double> &scr = M->lw.scr, &sh = M->lw.sh;
+            HIPCHK(scr.alloc(2 * ((n_shell + 255) / 256) + 2));
+            HIPCHK(sh.alloc(2 * (size_t)limit));
+            for (int t = 0; t < limit; t++)
+                elem_energy_share(M->n_q4, M->d_q4.p, M->n_t3, M->d_t3.p, ndof, Xf.p, pos[t], scr.p, sh.p + 2 * t, s);
+            std::vector<double> hsh(2 * (size_t)limit);
+            HIPCHK(hipMemcpyAsync(hsh.data(), sh.p, hsh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (int t = 0; t < limit; t++) {
+                double mx = hsh[2 * t], tot = hsh[2 * t + 1];
+                if (tot > 0 && mx / tot > 0.10) continue;
+                surv.push_back(pos[t]);
+            }
+            for (int t = limit; t < (int)pos.size(); t++) surv.push_back(pos[t]);
+        } else {
+            surv = pos;
+        }
+        // every pair with |lambda| below the smallest computed |theta|'s
+        // inverse is known (for the largest theta: every positive root below
+        // the smallest computed theta's inverse, all of them once a computed
+        // theta is not positive)
+        double cover = INFINITY;
+        if (!sel.all_pairs)
+            for (int i = 0; i < k; i++) {
+                if (grow && !(lo.theta[i] > 0.0)) { cover = INFINITY; break; }
+                cover = std::min(cover, 1.0 / std::fabs(lo.theta[i]));
+            }
+        decided = selection_decided(sel, surv, lam, cover);
+        M->sel_info[0] = k;
+        M->sel_info[1] = (double)pos.size();
+        if (decided || !grow || req >= req_max) break;
+        req = req_max;
     }
-    int nout = std::min(req, (int)surv.size());
-    nout = std::min(nout, max_modes);
+    if (!decided && grow)
+        return set_err(ctx, SOL105_E_UNSUPPORTED,
+                       "EIGRL range: the %d smallest positive roots do not decide the output (%zu in range)", req,
+                       pos.size());
+    M->sel_info[2] = cluster_skip(sel, surv, lam);
+    M->sel_info[3] = decided ? 1.0 : 0.0;
+    // mode count and cluster filter
+    std::vector<int> outp = select_output(sel, surv, lam);
+    const int nout = std::min((int)outp.size(), (int)max_modes);
     HIPCHK(M->d_modes.alloc((size_t)ndof * std::max(nout, 1)));
     M->lambda.assign(nout, 0.0);
     DevBuf<double> &scr = M->lw.scr;
     HIPCHK(scr.alloc(std::max<size_t>(1024, scr.n)));
     for (int t = 0; t < nout; t++) {
-        int i = surv[t];
+        int i = outp[t];
         M->lambda[t] = lam[i];
         HIPCHK(hipMemcpyAsync(M->d_modes.p + (size_t)t * ndof, Xf.p + (size_t)i * ndof, ndof * sizeof(double),
                               hipMemcpyDeviceToDevice, s));
@@ -1126,6 +1220,13 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
         HIPCHK(hipMemcpyAsync(modes_out, M->d_modes.p, (size_t)ndof * nout * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipGetLastError());
+    return SOL105_OK;
+}
+
+int sol105_buckling_info(const sol105_ctx *ctx, const sol105_model *M, double *out) {
+    (void)ctx;
+    if (!M || !out) return SOL105_E_INVALID;
+    std::memcpy(out, M->sel_info, sizeof(M->sel_info));
     return SOL105_OK;
 }
 

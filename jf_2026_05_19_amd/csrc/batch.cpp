@@ -16,6 +16,7 @@
 // re-solved alone by the single-model path, which has the rank-revealing
 // rebuild.
 #include "internal.h"
+#include "select.h"
 
 #include <cmath>
 #include <cstring>
@@ -27,7 +28,8 @@ struct sol105_batch {
     sol105_model M;  // block-diagonal union of the members
     int n_mem = 0;
     std::vector<int32_t> node_off, q4_off, t3_off, bar_off;  // n_mem + 1
-    std::vector<int32_t> nd;
+    std::vector<int32_t> nd, n_shell_cards;
+    std::vector<double> v1, v2;  // EIGRL range per member
     // results of the last solve
     std::vector<std::vector<double>> lambda;
     std::vector<int32_t> status, n_out;
@@ -91,13 +93,26 @@ void member_desc(const sol105_batch *B, int i, MemberDesc &D) {
     d.k6rot = M.k6rot;
     d.autospc = M.autospc;
     d.nd = B->nd[i];
+    d.eigrl_v1 = B->v1[i];
+    d.eigrl_v2 = B->v2[i];
+    d.n_shell_cards = B->n_shell_cards[i];
+}
+
+ModeSelect member_select(const sol105_batch *B, int i, int n_free) {
+    ModeSelect sel;
+    sel.nd = B->nd[i];
+    sel.n_free = n_free;
+    sel.v1 = B->v1[i];
+    sel.v2 = B->v2[i];
+    sel.n_shell_cards = B->n_shell_cards[i];
+    return sel;
 }
 
 // Per-member eigen semantics of sol105_buckling (solve_case.jl:1938-1945,
 // 2131-2196, 2244-2252, 2414, 2568).
 struct MemberEig {
     int n = 0;             // free DOFs (segment length)
-    bool dense = false;
+    ModeSelect sel;
     int req = 0, kneed = 0, mmax = 0;
     bool active = false, fallback = false;
     int code = SOL105_OK;  // error of this member (when not re-solved)
@@ -137,10 +152,12 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
     int b = 8;
     for (int i = 0; i < nm; i++) {
         MemberEig &e = E[i];
-        e.dense = e.n <= 4000;
-        e.req = std::min(e.dense ? 3 * B->nd[i] : B->nd[i], std::max(e.n - 2, 1));
+        e.sel = member_select(B, i, e.n);
+        e.req = e.sel.requested();
         if (e.n == 0) { e.code = SOL105_E_SINGULAR; continue; }
-        if (e.req > MAXR) { e.code = SOL105_E_UNSUPPORTED; continue; }
+        // a range (whole-spectrum semantics) or more pairs than the stacked
+        // iteration carries: the member is solved alone
+        if (e.sel.has_range() || e.req > MAXR) { e.fallback = true; continue; }
         if (e.req > 8) b = 16;
     }
     int mmax = 0, kmax = 0;
@@ -387,30 +404,14 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
     std::vector<std::vector<double>> lam(nm);
     for (int i = 0; i < nm; i++) {
         MemberEig &e = E[i];
-        int k = (int)e.theta.size();
-        lam[i].resize(k);
-        std::vector<int> cand;
-        for (int t = 0; t < k; t++) {
-            double th = e.theta[t];
-            lam[i][t] = 1.0 / th;
-            bool ok = e.dense ? (std::isfinite(th) && std::fabs(th) > 1e-12)
-                              : (std::fabs(th) >= 1e-14 && std::fabs(lam[i][t]) > 1e-6);
-            if (ok) cand.push_back(t);
-        }
-        std::stable_sort(cand.begin(), cand.end(),
-                         [&](int a, int c) { return std::fabs(lam[i][a]) < std::fabs(lam[i][c]); });
-        if ((int)cand.size() > e.req) cand.resize(e.req);
-        for (int t : cand)
-            if (lam[i][t] > 1e-10) pos[i].push_back(t);
-        std::stable_sort(pos[i].begin(), pos[i].end(), [&](int a, int c) { return lam[i][a] < lam[i][c]; });
+        pos[i] = select_candidates(e.sel, e.theta.data(), (int)e.theta.size(), lam[i]);
     }
     // localization filter (>= 100 shells per member, share 0.10, scan ND + 16)
     std::vector<int32_t> pairs;
     std::vector<int> pair_mem, pair_t;
     for (int i = 0; i < nm; i++) {
-        int nsh = (B->q4_off[i + 1] - B->q4_off[i]) + (B->t3_off[i + 1] - B->t3_off[i]);
-        if (nsh < 100 || pos[i].empty()) continue;
-        int limit = std::min((int)pos[i].size(), std::max(B->nd[i] + 16, B->nd[i]));
+        const int limit = localization_scan_limit(E[i].sel, (int)pos[i].size());
+        if (limit == 0) continue;
         for (int t = 0; t < limit; t++) {
             pairs.push_back(i);
             pairs.push_back(pos[i][t]);
@@ -451,10 +452,13 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
     B->mode_off.assign(nm + 1, 0);
     for (int i = 0; i < nm; i++) {
         int no = 0;
-        if (!E[i].fallback && E[i].code == SOL105_OK) no = std::min({E[i].req, (int)surv[i].size(), max_modes});
+        if (!E[i].fallback && E[i].code == SOL105_OK) {
+            surv[i] = select_output(E[i].sel, surv[i], lam[i]);
+            no = std::min((int)surv[i].size(), max_modes);
+        }
         B->n_out[i] = no;
         B->mode_off[i + 1] = B->mode_off[i] + (int64_t)6 * (B->node_off[i + 1] - B->node_off[i]) *
-                                                  std::max(no, E[i].fallback ? std::min(max_modes, MAXR) : 0);
+                                                  std::max(no, E[i].fallback ? std::min(max_modes, SYMEIG_MAX_K) : 0);
     }
     HIPCHK(B->d_modes.alloc(std::max<int64_t>(B->mode_off[nm], 1)));
     std::vector<int32_t> npairs;
@@ -491,7 +495,7 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
         sol105_model Mi;
         int rc = model_init(ctx, &D.d, &Mi);
         int32_t no = 0;
-        std::vector<double> l(std::min(max_modes, MAXR));
+        std::vector<double> l(std::min(max_modes, SYMEIG_MAX_K));
         if (rc == SOL105_OK) rc = sol105_solve(ctx, &Mi, (int32_t)l.size(), l.data(), nullptr, &no);
         B->status[i] = rc;
         if (rc != SOL105_OK) continue;
@@ -518,6 +522,9 @@ int sol105_batch_create(sol105_ctx *ctx, const sol105_model_desc *mem, int32_t n
     B->t3_off.assign(n_members + 1, 0);
     B->bar_off.assign(n_members + 1, 0);
     B->nd.resize(n_members);
+    B->v1.resize(n_members);
+    B->v2.resize(n_members);
+    B->n_shell_cards.resize(n_members);
     for (int i = 0; i < n_members; i++) {
         const sol105_model_desc &d = mem[i];
         if (d.n_nodes <= 0 || d.n_q4 < 0 || d.n_t3 < 0 || d.n_bar < 0 || d.nd <= 0)
@@ -528,7 +535,12 @@ int sol105_batch_create(sol105_ctx *ctx, const sol105_model_desc *mem, int32_t n
         B->q4_off[i + 1] = B->q4_off[i] + d.n_q4;
         B->t3_off[i + 1] = B->t3_off[i] + d.n_t3;
         B->bar_off[i + 1] = B->bar_off[i] + d.n_bar;
+        if (!std::isfinite(d.eigrl_v1) || !std::isfinite(d.eigrl_v2) || d.n_shell_cards < 0)
+            return set_err(ctx, SOL105_E_INVALID, "batch member %d: invalid EIGRL range or shell card count", i);
         B->nd[i] = d.nd;
+        B->v1[i] = d.eigrl_v1;
+        B->v2[i] = d.eigrl_v2;
+        B->n_shell_cards[i] = d.n_shell_cards > 0 ? d.n_shell_cards : d.n_q4 + d.n_t3;
     }
     const int N = B->node_off[n_members];
     std::vector<double> xyz(3 * (size_t)N), q4p(6 * (size_t)B->q4_off[n_members]), t3p(6 * (size_t)B->t3_off[n_members]);
@@ -626,7 +638,7 @@ int sol105_batch_solve(sol105_ctx *ctx, sol105_batch *B, int32_t max_modes, doub
         B->n_out.assign(nm, 0);
         B->status.assign(nm, SOL105_OK);
         B->mode_off.assign(nm + 1, 0);
-        const int mm = std::min(max_modes, MAXR);
+        const int mm = std::min(max_modes, SYMEIG_MAX_K);
         for (int i = 0; i < nm; i++) B->mode_off[i + 1] = B->mode_off[i] + 6 * (int64_t)(B->node_off[i + 1] - B->node_off[i]) * mm;
         HIPCHK(B->d_modes.alloc(std::max<int64_t>(B->mode_off[nm], 1)));
         B->n_fallback = nm;

@@ -68,9 +68,10 @@ void node_gather_vec(int n_nodes, int n_q4, int n_t3, const int32_t *diag_blk, c
 void extract_kdiag(int n_nodes, const int32_t *diag_blk, const double *kv, double *out, hipStream_t s);
 // dst[i + j*ldd] += src[i + j*lds]
 void mat_add(int rows, int cols, const double *src, int lds, double *dst, int ldd, hipStream_t s);
-// The k eigenpairs of largest |theta| of a symmetric m x m matrix A
-// (column-major, may be overwritten), one workgroup; m <= SYMEIG_MAX_M,
-// k <= SYMEIG_MAX_K.  theta[t] in descending |theta| order, Zs (m x k, ld m)
+// The k eigenpairs of largest |theta| (algebraic != 0: of largest theta) of a
+// symmetric m x m matrix A (column-major, may be overwritten), one workgroup;
+// m <= SYMEIG_MAX_M, k <= SYMEIG_MAX_K.  theta[t] in descending |theta|
+// (descending) order, Zs (m x k, ld m)
 // the orthonormal eigenvectors; work holds work_n >= symeig_work_size(m, k)
 // doubles; *info = 0 on success.  bw > 0 declares A banded with half-bandwidth bw
 // (block Lanczos without restarts): for 4 <= bw <= SYMEIG_BAND_MAX_W the
@@ -84,7 +85,7 @@ constexpr int SYMEIG_BAND_MAX_M = 384;
 constexpr int SYMEIG_BAND_MAX_W = 16;
 int64_t symeig_work_size(int m, int k);
 void sym_eig_top(int m, double *A, int k, double *theta, double *Zs, double *work, int64_t work_n, int *info,
-                 hipStream_t s, int bw = 0);
+                 hipStream_t s, int bw = 0, int algebraic = 0);
 
 // ---------------------------------------------------------------------------
 // Segmented forms for a batch of independent problems stacked by rows (each

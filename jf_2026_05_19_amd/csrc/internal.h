@@ -77,6 +77,9 @@ struct sol105_model {
     std::vector<uint8_t> spc_mask, has_shell, has_bar;
     double k6rot = 100.0;
     int autospc = 1, nd = 3;
+    double v1 = 0.0, v2 = 0.0;  // EIGRL V1, V2
+    int n_shell_cards = 0;      // shell cards of the deck (the mode filters' mesh-size guard)
+    double sel_info[4] = {0, 0, 0, 1};  // last buckling: pairs, in-range candidates, cluster skip, decided
     BlockPattern pat;
     std::vector<int32_t> diag_blk;
     // device: model + assembly

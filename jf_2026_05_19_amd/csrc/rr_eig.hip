@@ -95,6 +95,7 @@ __device__ __forceinline__ double hash_unit(uint64_t k) {
 struct SymeigBatch {
     const int *list;
     int64_t sA, sT, sZ, sW;
+    int alg;  // 1: the k largest theta instead of the k largest |theta|
 };
 
 // LDS arrays shared by the tridiagonal stages (steps 2 and 3)
@@ -102,12 +103,14 @@ struct TriLds {
     double *dd, *ee, *e2, *scr, *ev, *wlo, *whi, *wem, *bnd;
     int *sel, *asc, *ocol, *cl_start, *sint;
     int scr_n;  // doubles of scr
+    int alg;    // select the k largest theta (descending) instead of |theta|
 };
 
 // Steps 2 and 3 on the tridiagonal (dd, ee; ee[m-1] = 0) held in LDS: the
 // eigenvalues at the 2k extreme indices by multisection, the k of largest
-// |theta| into theta (descending |theta|) and their eigenvectors by inverse
-// iteration into Zs (m x k, ld m).  All NT threads of the workgroup call it;
+// |theta| (or, with L.alg, the k largest) into theta (descending |theta|,
+// or descending) and their eigenvectors by inverse iteration into Zs
+// (m x k, ld m).  All NT threads of the workgroup call it;
 // sint[0] collects failures (bit 2: a vector did not come out finite).
 __device__ void tridiag_top_pairs(int m, int k, const TriLds &L, double *theta, double *Zs) {
     double *dd = L.dd, *ee = L.ee, *e2 = L.e2, *scr = L.scr, *ev = L.ev, *wlo = L.wlo, *whi = L.whi, *wem = L.wem,
@@ -186,7 +189,7 @@ __device__ void tridiag_top_pairs(int m, int k, const TriLds &L, double *theta, 
     if (tid == 0) {
         int a = 0, b = m - 1;
         for (int t = 0; t < k; t++) {
-            if (fabs(ev_at(b)) >= fabs(ev_at(a))) sel[t] = b--;
+            if (L.alg || fabs(ev_at(b)) >= fabs(ev_at(a))) sel[t] = b--;
             else sel[t] = a++;
         }
         // ascending list: [0, a) then (b, m)
@@ -454,7 +457,7 @@ __global__ void __launch_bounds__(NT) symeig_top_kernel(int m, double *A, int k,
     __syncthreads();
 
     {
-        const TriLds L{dd, ee, e2, scr, ev, wlo, whi, wem, bnd, sel, asc, ocol, cl_start, sint, SCR};
+        const TriLds L{dd, ee, e2, scr, ev, wlo, whi, wem, bnd, sel, asc, ocol, cl_start, sint, SCR, bt.alg};
         tridiag_top_pairs(m, k, L, theta, Zs);
     }
 
@@ -765,7 +768,7 @@ __global__ void __launch_bounds__(NT) symeig_band_kernel(int m, int bw, const do
 
     // ---- 2 and 3 on the tridiagonal ----
     {
-        const TriLds L{dd, ee, e2, lscr, ev, wlo, whi, wem, bnd, sel, asc, ocol, cl_start, sint, SCRN};
+        const TriLds L{dd, ee, e2, lscr, ev, wlo, whi, wem, bnd, sel, asc, ocol, cl_start, sint, SCRN, bt.alg};
         tridiag_top_pairs(m, k, L, theta, Zs);
     }
     __syncthreads();
@@ -865,8 +868,8 @@ static bool band_path(int m, int bw, int64_t work_n) {
 }
 
 void sym_eig_top(int m, double *A, int k, double *theta, double *Zs, double *work, int64_t work_n, int *info,
-                 hipStream_t s, int bw) {
-    const SymeigBatch one{nullptr, 0, 0, 0, 0};
+                 hipStream_t s, int bw, int algebraic) {
+    const SymeigBatch one{nullptr, 0, 0, 0, 0, algebraic ? 1 : 0};
     const int band = band_path(m, bw, work_n);
     if (band && m <= SYMEIG_BAND_MAX_M) symeig_band_kernel<false><<<1, NT, 0, s>>>(m, bw, A, k, theta, Zs, work, info, one);
     else if (band) symeig_band_kernel<true><<<1, NT, 0, s>>>(m, bw, A, k, theta, Zs, work, info, one);
@@ -879,7 +882,7 @@ void sym_eig_top(int m, double *A, int k, double *theta, double *Zs, double *wor
 void sym_eig_top_batched(int m, double *A, int64_t sA, int k, double *theta, int64_t sT, double *Zs, int64_t sZ,
                          double *work, int64_t sW, int *info, const int *list, int nlist, hipStream_t s, int bw) {
     if (nlist <= 0) return;
-    const SymeigBatch bt{list, sA, sT, sZ, sW};
+    const SymeigBatch bt{list, sA, sT, sZ, sW, 0};
     const int band = band_path(m, bw, sW);
     if (band && m <= SYMEIG_BAND_MAX_M)
         symeig_band_kernel<false><<<nlist, NT, 0, s>>>(m, bw, A, k, theta, Zs, work, info, bt);

@@ -229,6 +229,9 @@ void run_group(sol105_ctx *ctx, const std::vector<Job> &jobs) {
             x.k6rot = M->k6rot;
             x.autospc = M->autospc;
             x.nd = M->nd;
+            x.eigrl_v1 = M->v1;
+            x.eigrl_v2 = M->v2;
+            x.n_shell_cards = M->n_shell_cards;
             mm = std::max(mm, r.max_modes);
         }
         sol105_batch *B = nullptr;
@@ -394,7 +397,7 @@ int sol105_server_submit_ex(sol105_server *S, int32_t model_id, const double *sh
     const sol105_model *M = S->models[model_id]->M.get();
     Request r;
     r.model = model_id;
-    r.max_modes = std::min<int32_t>(max_modes, MAXR);
+    r.max_modes = std::min<int32_t>(max_modes, SYMEIG_MAX_K);
     r.want_modes = (flags & SOL105_REQ_MODES) != 0;
     if (shell_thickness) {
         r.t.assign(shell_thickness, shell_thickness + M->n_q4 + M->n_t3);

@@ -61,6 +61,7 @@ class Sol105Arrays:
     node_has_bar: np.ndarray = field(default=None)
     buckling_subcase: int = 1
     static_subcase: int = 1
+    shell_cards: int = -1         # CSHELL cards of the deck, assembled or not (-1: n_shell)
 
     @property
     def n_nodes(self):
@@ -81,6 +82,18 @@ class Sol105Arrays:
     @property
     def n_shell(self):
         return self.n_q4 + self.n_t3
+
+    @property
+    def n_shell_cards(self):
+        """Shell cards the mode filters' mesh-size guards count
+        (``length(model["CSHELLs"])`` in solve_case.jl:2609, 3071), including
+        cards the assembly skips."""
+        return self.n_shell if self.shell_cards < 0 else int(self.shell_cards)
+
+    @property
+    def has_range(self):
+        """EIGRL V1/V2 range mode (solve_case.jl:1938)."""
+        return (self.v1 != 0.0 or self.v2 != 0.0) and self.v2 > self.v1
 
     @property
     def n_bar(self):
@@ -267,8 +280,6 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
         e = model["EIGRLs"].get(str(int(method)))
         if e is not None:
             nd, v1, v2 = int(e.get("ND", 3)), float(e.get("V1", 0.0)), float(e.get("V2", 0.0))
-    if (v1 != 0.0 or v2 != 0.0) and v2 > v1:
-        raise UnsupportedModel("EIGRL V1/V2 range mode is not supported")
     k6 = model.get("PARAM_K6ROT", 100.0)
     k6rot = float(k6) if isinstance(k6, (int, float)) else 100.0
     asp = model.get("PARAM_AUTOSPC", True)
@@ -309,4 +320,5 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
         spc_mask=spc_mask, F=F, nd=nd, v1=v1, v2=v2, k6rot=k6rot, autospc=autospc,
         node_has_shell=has_shell, node_has_bar=has_bar,
         buckling_subcase=int(buck_sid), static_subcase=int(stat_sid),
+        shell_cards=len(model["CSHELLs"]),
     )

@@ -20,9 +20,13 @@ compute on the same matrices:
 * buckling, ``n_free > 4000``: ARPACK ``eigsh(-KG, M=K, which='LM')`` for the
   largest |theta| of ``K^-1 (-KG)``, the operator KrylovKit iterates
   (``solve_case.jl:2202-2315``; KrylovKit 0.10.2, Arnoldi, tol 1e-12);
-* post-processing restated from ``solve_case.jl:2388-2420`` (positive filter),
-  ``:2582-2933`` (localization filter defaults), ``:2966-2987`` (mode count),
-  ``:3146-3192`` (expansion + max-abs normalisation);
+* post-processing restated from ``solve_case.jl:2388-2436`` (positive and
+  EIGRL range filters; under a range the dense path keeps its whole
+  spectrum, ``:2131-2196``), ``:2582-2933`` (localization filter defaults),
+  ``:2966-2987`` (mode count), ``:2995-3139`` (cluster filter),
+  ``:3146-3192`` (expansion + max-abs normalisation).  The range's
+  shift-invert search near V2 on the Krylov path (``:2437-2560``) is not
+  restated;
 * AUTOSPC restated from ``boundary_conditions.jl:205-383``.
 
 Sensitivities are central finite differences of this forward path with MAC
@@ -406,22 +410,31 @@ def static_solve_cholesky(arr, K, max_elem_stiff, info=None):
 # --------------------------------------------------------------------------
 
 def buckling(arr, K, KG, free, krylov_tol=1e-13):
+    """Buckling eigenpairs with the reference's output semantics
+    (solve_case.jl:1938-1951 request count, 2131-2196 dense path keeping the
+    whole spectrum under an EIGRL range, 2202-2315 Krylov path, 2414-2436
+    positive and range filters, 2568 ascending order, 2582-2933 localization
+    filter, 2966-2987 mode count, 2995-3139 cluster filter).  The shift-invert
+    search near V2 that the reference adds on the Krylov path under a range
+    (solve_case.jl:2437-2560) is not restated: there the result is the
+    reference's with JFEM_SOL105_SKIP_RANGE_AUGMENTATION=true."""
     nd = arr.nd
     nf = len(free)
+    v1, v2 = float(arr.v1), float(arr.v2)
+    has_range = (v1 != 0.0 or v2 != 0.0) and v2 > v1
     Kff = K[free][:, free]
     Kff = (0.5 * (Kff + Kff.T)).tocsc()
     KGff = KG[free][:, free]
     KGff = (0.5 * (KGff + KGff.T)).tocsc()
     dense = nf <= 4000
-    req = 3 * nd if dense else nd
-    req = min(req, max(nf - 2, 1))
+    req = request_count(nd, nf, v1, v2)
     if dense:
         th, V = sla.eigh(-KGff.toarray(), Kff.toarray())
         ok = np.isfinite(th) & (np.abs(th) > 1e-12)
         th, V = th[ok], V[:, ok]
         lam = 1.0 / th
         perm = np.argsort(np.abs(lam), kind="stable")
-        n_out = min(req, len(perm))
+        n_out = len(perm) if has_range else min(req, len(perm))
         lam, V = lam[perm[:n_out]], V[:, perm[:n_out]]
     else:
         nev = min(req + 5, nf - 1)
@@ -438,19 +451,135 @@ def buckling(arr, K, KG, free, krylov_tol=1e-13):
         n_out = min(req, len(perm))
         lam, V = lam[perm[:n_out]], V[:, perm[:n_out]]
     raw_lam = lam.copy()
-    pos = np.flatnonzero(lam > 1e-10)
-    order = pos[np.argsort(lam[pos], kind="stable")]
-    order, dropped = localization_filter(arr, free, V, order, nd, arr.n_shell)
-    n_out = min(req, len(order))
-    sel = order[:n_out]
+    sel, rec = select_ranked(lam, nd, v1, v2, arr.n_shell_cards, req,
+                             lambda order: localization_filter(arr, free, V, order, nd, arr.n_shell_cards))
+    n_out = len(sel)
     modes = np.zeros((arr.ndof, n_out))
     modes[free] = V[:, sel]
     for m in range(n_out):
         mx = np.abs(modes[:, m]).max()
         if mx > 1e-30:
             modes[:, m] /= mx
-    return dict(eigenvalues=lam[sel], modes=modes, raw_eigenvalues=raw_lam, dense=dense,
-                localization_dropped=len(dropped), requested=req)
+    return dict(eigenvalues=lam[sel], modes=modes, raw_eigenvalues=raw_lam, dense=dense, requested=req, **rec)
+
+
+def request_count(nd, n_free, v1=0.0, v2=0.0):
+    """Eigenpairs the reference asks its eigensolver for (solve_case.jl:
+    1938-1951): ceil(3 ND) under a range, 3 ND on the dense path
+    (n_free <= 4000), else ND; at most max(n_free - 2, 1)."""
+    has_range = (v1 != 0.0 or v2 != 0.0) and v2 > v1
+    req = int(np.ceil(3.0 * nd)) if has_range else (3 * nd if n_free <= 4000 else nd)
+    return min(req, max(n_free - 2, 1))
+
+
+def select_ranked(lam, nd, v1, v2, n_shell_cards, req, localization=None):
+    """Output pairs from eigenvalues ``lam`` already ranked and cut as the
+    eigensolver path leaves them: positive filter (> 1e-10), EIGRL range
+    [max(V1, 1e-10), V2] (solve_case.jl:2414-2436), ascending order
+    (:2568), localization filter (``localization(order) -> (order,
+    dropped)``), cluster filter, then ND under a range, else ``req`` modes
+    (:2966-2987, 3127-3139).  Returns (indices into lam, record)."""
+    lam = np.asarray(lam, dtype=float)
+    has_range = (v1 != 0.0 or v2 != 0.0) and v2 > v1
+    tol = 1e-10
+    ok = lam > tol
+    if has_range:
+        ok &= (lam >= max(v1, tol)) & (lam <= v2)
+    pos = np.flatnonzero(ok)
+    order = pos[np.argsort(lam[pos], kind="stable")]
+    dropped = []
+    if localization is not None:
+        order, dropped = localization(order)
+    n_skip = cluster_filter_skip(lam[order], has_range, v2, n_shell_cards)
+    cap = nd if has_range else req
+    sel = np.asarray(order[n_skip:][:cap], dtype=int)
+    return sel, dict(localization_dropped=len(dropped), in_range=len(pos), cluster_skip=n_skip)
+
+
+def select_modes(theta, nd, n_free, v1, v2, n_shell_cards, shares=None):
+    """The reference's choice of output pairs from eigenvalues theta of
+    C = L^-1 (-KG) L^-T (lambda = 1/theta), as the whole buckling path makes
+    it: the dense path (n_free <= 4000) drops non-finite and |theta| <=
+    1e-12 and keeps the requested count by |lambda|, or every pair under a
+    range (solve_case.jl:2131-2196); the Krylov path drops |theta| < 1e-14
+    and |lambda| <= 1e-6 and keeps the requested count (:2236-2256); then
+    select_ranked.  ``shares[k]``: the localization share max(e)/sum(e) of
+    pair k (None: 0).  Returns (indices into theta in output order,
+    record)."""
+    theta = np.asarray(theta, dtype=float)
+    has_range = (v1 != 0.0 or v2 != 0.0) and v2 > v1
+    dense = n_free <= 4000
+    req = request_count(nd, n_free, v1, v2)
+    with np.errstate(divide="ignore"):
+        lam_all = 1.0 / theta
+    if dense:
+        keep = np.flatnonzero(np.isfinite(theta) & (np.abs(theta) > 1e-12))
+    else:
+        keep = np.flatnonzero((np.abs(theta) >= 1e-14) & (np.abs(lam_all) > 1e-6))
+    keep = keep[np.argsort(np.abs(lam_all[keep]), kind="stable")]
+    if not (dense and has_range):
+        keep = keep[:req]
+    lam = lam_all[keep]
+
+    def loc(order):
+        return localization_scan(keep[order] if len(order) else order, order, nd, n_shell_cards,
+                                 lambda k: 0.0 if shares is None else float(shares[k]))
+    sel, rec = select_ranked(lam, nd, v1, v2, n_shell_cards, req, loc)
+    return keep[sel], rec
+
+
+def localization_scan(pair_ids, order, num_modes, n_shell_cards, share_of, max_share=0.10, min_elements=100,
+                      scan_buffer=16):
+    """The localization filter's scan (solve_case.jl:2670-2680, 2850-2896)
+    with the share of each candidate given: of the ascending candidates
+    ``order`` (pair ``pair_ids[t]`` for position t) the first max(num_modes +
+    16, num_modes) are dropped when their share exceeds max_share, on decks
+    of at least min_elements shell cards.  Returns (order, dropped)."""
+    order = np.asarray(order, dtype=int)
+    if n_shell_cards < min_elements or len(order) == 0:
+        return order, []
+    limit = min(len(order), max(num_modes + scan_buffer, num_modes))
+    surv, dropped = [], []
+    for t in range(limit):
+        if share_of(int(pair_ids[t])) > max_share:
+            dropped.append(int(order[t]))
+        else:
+            surv.append(int(order[t]))
+    return np.array(surv + list(order[limit:]), dtype=int), dropped
+
+
+def cluster_filter_skip(eigs, has_range, v2, n_shell_cards, jump=1.25, singleton=3.0, jump_max=8.0, min_v2=1.0,
+                        dense_n=3, spread=0.30, max_pre_spread=5.0, min_elements=100, last=True):
+    """Leading modes the default cluster filter skips (solve_case.jl:
+    2995-3139): over the full ascending in-range survivor list, a jump
+    e[i+1]/e[i] (i counted from 1) of at least ``singleton`` for i = 1, else
+    ``jump``, at most ``jump_max``; for i >= 2 the pre-jump modes within a
+    factor ``max_pre_spread`` of e[1]; the ``dense_n`` modes after the jump
+    within a relative ``spread``.  The last (default) or first qualifying i.
+    Only under an EIGRL range with V2 >= ``min_v2``, a deck of at least
+    ``min_elements`` shell cards, and at least dense_n + 2 modes, all
+    positive."""
+    e = [float(x) for x in eigs]
+    n = len(e)
+    if not (has_range and v2 >= min_v2 and n_shell_cards >= min_elements and n >= dense_n + 2):
+        return 0
+    if not all(x > 0.0 for x in e):
+        return 0
+    thr_single = max(jump, singleton)
+    n_skip = 0
+    for i in range(1, n - dense_n + 1):          # i counted from 1 as in the reference
+        ei, ej = e[i - 1], e[i]
+        r = ej / ei
+        if r < (thr_single if i == 1 else jump) or r > jump_max:
+            continue
+        if i >= 2 and e[i - 1] / e[0] > max_pre_spread:
+            continue
+        window = e[i:i + dense_n]
+        if (max(window) - min(window)) / min(window) <= spread:
+            n_skip = i
+            if not last:
+                break
+    return n_skip
 
 
 def localization_filter(arr, free, V, order, num_modes, n_shell_cards, max_share=0.10, min_elements=100,
@@ -463,22 +592,18 @@ def localization_filter(arr, free, V, order, num_modes, n_shell_cards, max_share
     expanded with zeros) and is dropped when max(e) / sum(e) > max_share.
     Returns (surviving order, dropped indices)."""
     order = np.asarray(order, dtype=int)
-    if n_shell_cards < min_elements or len(order) == 0 or len(free) == 0:
+    if len(free) == 0:
         return order, []
-    limit = min(len(order), max(num_modes + scan_buffer, num_modes))
-    surv, dropped = [], []
     full = np.zeros(arr.ndof)
-    for k in order[:limit]:
+
+    def share(k):
         full[:] = 0.0
         full[free] = V[:, k]
         t2 = (full.reshape(-1, 6)[:, :3] ** 2).sum(axis=1)
         e = np.concatenate([t2[arr.q4].sum(axis=1) / 4, t2[arr.t3].sum(axis=1) / 3])
         tot = e.sum()
-        if tot > 0 and e.max() / tot > max_share:
-            dropped.append(int(k))
-            continue
-        surv.append(int(k))
-    return np.array(surv + list(order[limit:]), dtype=int), dropped
+        return e.max() / tot if tot > 0 else 0.0
+    return localization_scan(order, order, num_modes, n_shell_cards, share, max_share, min_elements, scan_buffer)
 
 
 def solve(arr):

@@ -61,18 +61,28 @@ def test_reference_bundled_sol105_deck():
     subs = m["CASE_CONTROL"]["SUBCASES"]
     assert subs[2]["STATSUB"] == 1 and subs[2]["METHOD"] == 1 and subs[1]["LOAD"] == 1
     assert m["EIGRLs"]["1"]["ND"] == 3 and m["EIGRLs"]["1"]["V2"] == 1.0e9
-    with pytest.raises(M.UnsupportedModel, match="EIGRL"):
-        M.build_arrays(m)
-    # the same deck without the range builds: check the arrays
+    ranged = M.build_arrays(m)
+    assert ranged.has_range and (ranged.v1, ranged.v2) == (0.0, 1.0e9) and ranged.n_shell_cards == 1
+    # the same deck without the range: check the arrays
     txt = open(os.path.join(GOLDEN, "ref_sol105_quad_buckling.bdf")).read().replace("EIGRL,1,0.,1.0E9,3",
                                                                                    "EIGRL,1,,,3")
     arr = M.build_arrays(bdf.bdf_to_model_text(txt))
+    assert not arr.has_range
     assert arr.n_nodes == 4 and arr.n_q4 == 1 and arr.nd == 3
     assert arr.q4prop[0, 3] == 0.0                                # BR = 0
     assert arr.spc_mask.reshape(4, 6)[[0, 3]].all() and not arr.spc_mask.reshape(4, 6)[[1, 2]].any()
     F = arr.F.reshape(4, 6)
     assert F[1, 0] == -1000.0 and F[2, 0] == -1000.0 and np.count_nonzero(F) == 2
     assert arr.autospc
+    # membrane only: AUTOSPC leaves 4 free DOFs.  Without the range the dense
+    # path keeps min(3 ND, n_free - 2) = 2 pairs; with it the whole spectrum,
+    # so the third positive root is reported too (ND = 3)
+    from oracle import oracle as O
+    r_plain, r_range = O.solve(arr), O.solve(ranged)
+    assert len(r_plain["free"]) == 4 and r_plain["requested"] == 2
+    assert len(r_plain["eigenvalues"]) == 2 and len(r_range["eigenvalues"]) == 3
+    assert np.array_equal(r_range["eigenvalues"][:2], r_plain["eigenvalues"])
+    assert np.all(np.diff(r_range["eigenvalues"]) > 0) and r_range["eigenvalues"][0] > 0
 
 
 def _as_fixed(txt, large=False):
