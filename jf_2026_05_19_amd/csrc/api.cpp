@@ -733,7 +733,9 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     int n = M->sym.n_free, ndof = M->ndof();
     const bool debug = std::getenv("SOL105_DEBUG") != nullptr;
     int b = want <= 8 ? 8 : 16;
-    int mmax = std::min((ctx->lanczos_max_vectors / b) * b, (n / b) * b);
+    // the basis and its residual block stay inside the free space (a basis
+    // of more than n - b vectors leaves the residual block rank-deficient)
+    int mmax = std::min((ctx->lanczos_max_vectors / b) * b, ((n - b) / b) * b);
     if (mmax < b) return set_err(ctx, SOL105_E_NOCONV, "free set too small for the eigensolver (%d)", n);
     int guard = 2;
     int kneed = std::min(want + guard, mmax);
@@ -941,12 +943,16 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
         restarts++;
         return SOL105_OK;
     };
+    bool step = true;
     while (true) {
-        int rc = fast_step(j);
-        if (rc) return rc;
-        j++;
-        generated += b;
-        m = j * b;
+        if (step) {
+            int rc = fast_step(j);
+            if (rc) return rc;
+            j++;
+            generated += b;
+            m = j * b;
+        }
+        step = true;
         bool last = (m >= mmax) && restarts >= max_restarts;
         bool full = (m >= mmax);
         if (!(m >= next_check || full)) continue;
@@ -963,6 +969,12 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
         HIPCHK(hipMemcpy2DAsync(hZl.data(), b * sizeof(double), Zs.p + (m - b), (size_t)m * sizeof(double),
                                 b * sizeof(double), kchk, hipMemcpyDeviceToHost, s));
         hR.assign((size_t)b * b, 0.0);
+        if (m < b || m > mmax || (size_t)ldT * ldT > T.n)
+            return set_err(ctx, SOL105_E_DEVICE, "block Lanczos: basis size %d outside [%d, %d] (T %zu, ld %d)", m,
+                           b, mmax, T.n, ldT);
+        if (debug)
+            fprintf(stderr, "[sol105] check m=%d j=%d mmax=%d kchk=%d restarts=%d T.n=%zu ldT=%d\n", m, j, mmax, kchk,
+                    restarts, T.n, ldT);
         HIPCHK(hipMemcpy2DAsync(hR.data(), b * sizeof(double), T.p + (size_t)m + (size_t)(m - b) * ldT,
                                 ldT * sizeof(double), b * sizeof(double), b, hipMemcpyDeviceToHost, s));
         int herr[4];
@@ -973,13 +985,16 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
             int jb = herr[1] - 1;
             HIPCHK(hipMemsetAsync(ierr.p, 0, sizeof(int) * 4, s));
             int rank = b;
-            rc = robust_step(jb - 1, rank);
+            int rc = robust_step(jb - 1, rank);
             if (rc) return rc;
             injected += b - rank;
             if (debug) fprintf(stderr, "[sol105] block %d rank-deficient: rank %d, %d directions injected\n", jb, rank,
                                b - rank);
             j = jb;
             m = j * b;
+            // the rebuilt block was the residual block of a full basis: that
+            // basis is checked again (and restarted), not extended past V
+            if (m >= mmax) step = false;
             continue;
         }
         if (herr[0] & 2)
@@ -1022,7 +1037,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
             break;
         }
         if (full) {
-            rc = restart();
+            int rc = restart();
             if (rc) return rc;
             if (debug) fprintf(stderr, "[sol105] thick restart %d: kept %d Ritz vectors\n", restarts, kk);
             next_check = std::min(mmax, m + 4 * b);
@@ -1044,12 +1059,13 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     return SOL105_OK;
 }
 
-// Every eigenpair of C = L^-1 (-KG) L^-T for a free set of at most
-// SYMEIG_MAX_K DOFs: C is formed column block by column block from the
-// operator applied to the identity (the same products as a Lanczos step),
+// The k eigenpairs of largest |theta| (algebraic: of largest theta) of
+// C = L^-1 (-KG) L^-T for a free set of at most SYMEIG_MAX_M DOFs, k <=
+// SYMEIG_MAX_K: C is formed column block by column block from the operator
+// applied to the identity (the same products as a Lanczos step),
 // symmetrised, and handed whole to the projected eigensolver; Y = the
-// eigenvectors (the basis is the identity), descending |theta|.
-int dense_pairs(sol105_ctx *ctx, sol105_model *M, double *d_Y, LanczosOut &out) {
+// eigenvectors (the basis is the identity).
+int dense_pairs(sol105_ctx *ctx, sol105_model *M, int k, bool algebraic, double *d_Y, LanczosOut &out) {
     hipStream_t s = ctx->stream;
     const int n = M->sym.n_free, ndof = M->ndof();
     LanczosWork &W = M->lw;
@@ -1058,9 +1074,9 @@ int dense_pairs(sol105_ctx *ctx, sol105_model *M, double *d_Y, LanczosOut &out) 
     HIPCHK(W.xf.alloc((size_t)ndof * MAXR));
     HIPCHK(W.yf.alloc((size_t)ndof * MAXR));
     HIPCHK(W.A.alloc((size_t)n * n));
-    HIPCHK(W.Zs.alloc((size_t)n * n));
-    HIPCHK(W.thd.alloc(n));
-    HIPCHK(W.ework.alloc(symeig_work_size(n, n)));
+    HIPCHK(W.Zs.alloc((size_t)n * k));
+    HIPCHK(W.thd.alloc(k));
+    HIPCHK(W.ework.alloc(symeig_work_size(n, k)));
     HIPCHK(W.sw.alloc(1));
     HIPCHK(W.Ytmp.alloc((size_t)n * MAXR));
     std::vector<double> eye((size_t)n * MAXR);
@@ -1079,13 +1095,13 @@ int dense_pairs(sol105_ctx *ctx, sol105_model *M, double *d_Y, LanczosOut &out) 
         HIPCHK(hipStreamSynchronize(s));  // eye is reused by the next block
     }
     symmetrize_copy(n, n, W.V.p, W.A.p, s);
-    sym_eig_top(n, W.A.p, n, W.thd.p, W.Zs.p, W.ework.p, (int64_t)W.ework.n, W.sw.p, s, 0);
+    sym_eig_top(n, W.A.p, k, W.thd.p, W.Zs.p, W.ework.p, (int64_t)W.ework.n, W.sw.p, s, 0, algebraic ? 1 : 0);
     HIPCHK(hipGetLastError());
-    std::vector<double> th(n);
+    std::vector<double> th(k);
     int info = 0;
-    HIPCHK(hipMemcpyAsync(th.data(), W.thd.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(th.data(), W.thd.p, sizeof(double) * k, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&info, W.sw.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(d_Y, W.Zs.p, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_Y, W.Zs.p, sizeof(double) * n * k, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
     if (info != 0) return set_err(ctx, SOL105_E_NOCONV, "dense eigensolve failed (n=%d)", n);
     out.theta = th;
@@ -1116,10 +1132,14 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
     sel.all_pairs = n <= SYMEIG_MAX_K;
     // the dense path under a range keeps the whole spectrum, of which only
     // the positive roots in [V1, V2] can be output: the eigensolve targets
-    // the largest theta (the smallest positive lambda) there, ceil(3 ND)
-    // pairs first and, when those do not decide the output, as many as the
-    // projected eigensolver takes
+    // the largest theta (the smallest positive lambda) there.  Up to
+    // SYMEIG_MAX_M free DOFs C is formed whole and its SYMEIG_MAX_K largest
+    // pairs taken at once (a Lanczos basis would span most of the space);
+    // above that block Lanczos, ceil(3 ND) pairs first and, when those do not
+    // decide the output, as many as the projected eigensolver takes
     const bool grow = sel.dense() && sel.has_range() && !sel.all_pairs;
+    const bool whole = sel.all_pairs || (grow && n <= SYMEIG_MAX_M);
+    const int k_whole = sel.all_pairs ? n : std::min(n, SYMEIG_MAX_K);
     const int req_max = std::min(SYMEIG_MAX_K - 2, std::max(n - 2, 1));
     int req = sel.requested();
     if (!sel.all_pairs && req > SYMEIG_MAX_K - 2) {
@@ -1131,11 +1151,11 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
     std::vector<int> pos, surv;
     bool decided = true;
     for (;;) {
-        const int ycols = sel.all_pairs ? n : req;
+        const int ycols = whole ? k_whole : req;
         if (Y.n < (size_t)n * ycols) M->y_cols = 0;  // reallocated: no kept Ritz vectors
         HIPCHK(Y.alloc((size_t)n * ycols));
         LanczosOut lo;
-        int rc = sel.all_pairs ? dense_pairs(ctx, M, Y.p, lo) : lanczos(ctx, M, req, Y.p, lo, grow);
+        int rc = whole ? dense_pairs(ctx, M, k_whole, grow, Y.p, lo) : lanczos(ctx, M, req, Y.p, lo, grow);
         if (rc) return rc;
         const int k = (int)lo.theta.size();
         M->lanczos_m = lo.generated;
@@ -1189,13 +1209,13 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
         decided = selection_decided(sel, surv, lam, cover);
         M->sel_info[0] = k;
         M->sel_info[1] = (double)pos.size();
-        if (decided || !grow || req >= req_max) break;
+        if (decided || !grow || whole || req >= req_max) break;
         req = req_max;
     }
     if (!decided && grow)
         return set_err(ctx, SOL105_E_UNSUPPORTED,
-                       "EIGRL range: the %d smallest positive roots do not decide the output (%zu in range)", req,
-                       pos.size());
+                       "EIGRL range: the %d smallest positive roots do not decide the output (%zu in range)",
+                       whole ? k_whole : req, pos.size());
     M->sel_info[2] = cluster_skip(sel, surv, lam);
     M->sel_info[3] = decided ? 1.0 : 0.0;
     // mode count and cluster filter

@@ -98,13 +98,17 @@ def test_batch_repeatable(solved):
 def test_batch_member_error_is_isolated(ctx):
     from jf_2026_05_19_amd import _native as N
     good = _arr(decks.plate_deck(nx=6, ny=5, a=300.0, b=250.0, nd=3))
-    # ND = 8 on the dense path asks for 24 modes: beyond this build's 16
-    bad = _arr(decks.plate_deck(nx=5, ny=5, a=250.0, b=250.0, nd=8))
-    B = N.Batch(ctx, [good, bad, good])
+    # ND = 25 on the dense path asks for 75 modes: beyond the eigensolver's 62
+    bad = _arr(decks.plate_deck(nx=8, ny=6, a=320.0, b=240.0, nd=25))
+    # ND = 8 asks for 24: more than the stacked iteration carries, solved alone
+    big = _arr(decks.plate_deck(nx=5, ny=5, a=250.0, b=250.0, nd=8))
+    B = N.Batch(ctx, [good, bad, good, big])
     lam, st = B.solve(16)
     assert st[1] == -2 and len(lam[1]) == 0
-    assert st[0] == 0 and st[2] == 0
-    dm = N.DeviceModel(ctx, good)
-    la, _ = dm.solve(16)
-    for i in (0, 2):
-        assert np.all(np.abs(lam[i] - la) <= RTOL_LAM * np.abs(la))
+    assert st[0] == 0 and st[2] == 0 and st[3] == 0
+    for i, arr in ((0, good), (2, good), (3, big)):
+        dm = N.DeviceModel(ctx, arr)
+        la, _ = dm.solve(16)
+        dm.free()
+        assert lam[i].shape == la.shape and np.all(np.abs(lam[i] - la) <= RTOL_LAM * np.abs(la))
+    assert len(lam[3]) == 16
