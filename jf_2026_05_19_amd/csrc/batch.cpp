@@ -163,7 +163,7 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
     int mmax = 0, kmax = 0;
     for (int i = 0; i < nm; i++) {
         MemberEig &e = E[i];
-        if (e.code != SOL105_OK) continue;
+        if (e.code != SOL105_OK || e.fallback) continue;
         e.mmax = std::min(384, (e.n / b) * b);
         if (e.mmax < b) { e.fallback = true; continue; }  // too small for the stacked block size
         e.kneed = std::min(e.req + 2, e.mmax);
