@@ -27,13 +27,23 @@ struct FactorPlan {
     std::vector<HeightInfo> heights;
 };
 
+// Shape of one height's product launch: fronts whose inner dimension is at
+// most kmax form one k-slice, wider ones are cut in ks-wide slices (reduced
+// in slice order by the last slice to finish); pch four-k steps per chunk of
+// factor loads (0: chosen from the launch's size).
+struct ProdCfg {
+    int kmax = SOLVE_KMAX, ks = SOLVE_KS, pch = 0;
+};
+
 struct SolveHeight {
     int64_t fwd_off = 0, bwd_off = 0;  // first tile of this height
     int n_fwd = 0, n_bwd = 0;          // tiles of this height
     int max_m = 0, max_ns = 0;
     int64_t sum_mns = 0, sum_m = 0, sum_ns = 0;  // factor entries and vector rows of this height
+    int64_t sum_nz = 0;                          // nonzero factor entries: ns (ns + 1) / 2 + (m - ns) ns
     bool fwd_split = false, bwd_split = false;   // some front's product is split over k (needs a finish pass)
     int fwd_kmax = 0, bwd_kmax = 0;              // widest k-slice of this height's products
+    ProdCfg fwd_cfg, bwd_cfg;
 };
 
 struct SolvePlan {
@@ -63,8 +73,10 @@ inline int64_t solve_part_size(const SolvePlan &S) { return S.max_height_tiles *
 // product may read past the last front (masked, never used)
 inline int64_t solve_w_size(int64_t vec_storage) { return (vec_storage + 64) * MAXR; }
 
+// cfg: per height (fwd, bwd) launch shapes; empty = defaults everywhere
 void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std::vector<int> &m,
-                      const std::vector<int32_t> &flist, SolvePlan &S);
+                      const std::vector<int32_t> &flist, SolvePlan &S,
+                      const std::vector<std::pair<ProdCfg, ProdCfg>> &cfg = {});
 
 struct Symbolic;
 void build_plans(const Symbolic &S, FactorPlan &P, SolvePlan &SP, std::vector<int32_t> &flist);

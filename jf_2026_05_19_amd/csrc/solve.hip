@@ -393,8 +393,12 @@ __global__ void __launch_bounds__(256) prod1_kernel(DevFronts D, DevSolve DS, co
 // slice re-read its last step (B is zero there), and the last step's up to 3
 // k past the slice read factor data of later fronts or the zero tail of the
 // buffer (p_pad), finite either way and met by a zero B value.
+// A front cut in several k-slices writes partial sums, which the finish
+// kernels add in slice order.  (Reducing them in the last slice to finish,
+// behind a device-scope release and acquire per wavefront, measured slower:
+// each such release writes back the XCD's L2.)
 // PCH: four-k steps per chunk of factor loads (8: more wavefronts per SIMD
-// for short slices on large grids; 16: fewer dependent rounds otherwise)
+// for short slices on large grids; 16 or 32: fewer dependent rounds)
 template <int PCH>
 __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS, const int32_t *tiles,
                                                         const double *A, double *W, double *part, int nrhs, int bwd,
@@ -416,12 +420,19 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
     const bool live = rb < nrows;                // wavefront-uniform
     const double *Aw = A + DS.poff[f] + (int64_t)k0 * lda;
     const unsigned aoff = (unsigned)(min(rb + (lane & 15), nrows - 1) + kq * lda);
+    // L11^-1 is lower triangular: forward pivot rows i use columns k <= i,
+    // backward rows use k >= i below ns.  Four-k steps that are zero for all
+    // 16 rows of the wavefront are skipped (they would add exact zeros):
+    // steps [s_lo, s_hi) remain.
+    int s_lo = 0, s_hi = nst;
+    if (!bwd && rb + 16 <= ns) s_hi = rb + 15 < k0 ? 0 : min(nst, ((rb + 15 - k0) >> 2) + 1);
+    if (bwd && rb > k0) s_lo = min(nst, (rb - k0) >> 2);
     double av[PCH];
     auto load_a = [&](int s) {
 #pragma unroll
-        for (int u = 0; u < PCH; u++) av[u] = Aw[(int64_t)(4 * min(s + u, nst - 1)) * lda + aoff];
+        for (int u = 0; u < PCH; u++) av[u] = Aw[(int64_t)(4 * min(s + u, s_hi - 1)) * lda + aoff];
     };
-    if (live) load_a(0);
+    if (live && s_lo < s_hi) load_a(s_lo);
     // forward, single k-slice: the child rows pulled into this wavefront's
     // update rows (written back below), loaded now so they arrive with the factor
     const bool parent = D.child_ptr[f + 1] > D.child_ptr[f];  // leaves pull nothing
@@ -490,13 +501,13 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
     __syncthreads();
     if (!live) return;  // a whole wavefront past the front; no barrier follows
     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int s = 0; s < nst; s += PCH) {
-        if (s > 0) load_a(s);
+    for (int s = s_lo; s < s_hi; s += PCH) {
+        if (s > s_lo) load_a(s);
         double bv[PCH];
 #pragma unroll
         for (int u = 0; u < PCH; u++) {
-            const int ss = min(s + u, nst - 1);
-            bv[u] = (s + u < nst) ? Vs[4 * ss + kq][col] : 0.0;  // wavefront-uniform condition
+            const int ss = min(s + u, s_hi - 1);
+            bv[u] = (s + u < s_hi) ? Vs[4 * ss + kq][col] : 0.0;  // wavefront-uniform condition
         }
 #pragma unroll
         for (int u = 0; u < PCH; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
@@ -638,21 +649,27 @@ static size_t prod_lds(int kmax) {
     return sizeof(double) * ((size_t)4 * ((kmax + 3) / 4) * (MAXR + 1) + (size_t)4 * 16 * (MAXR + 1));
 }
 
-// one block-RHS product launch over n_tiles tiles whose widest k-slice is kmax
-static void launch_prod(int n_tiles, int kmax, hipStream_t s, const DevFronts &D, const DevSolve &DS,
+// one block-RHS product launch over n_tiles tiles whose widest k-slice is
+// kmax, pch four-k steps per chunk of factor loads (0: by the launch's size)
+static void launch_prod(int n_tiles, int kmax, int pch, hipStream_t s, const DevFronts &D, const DevSolve &DS,
                         const int32_t *tl, const double *A, double *W, double *part, int nrhs, int bwd,
                         const int32_t *kw, const int32_t *nks, const double *xin, double *xout, int n_free) {
-    if (kmax <= 192 && n_tiles >= 380)
+    if (pch == 0) pch = (kmax <= 192 && n_tiles >= 380) ? 8 : 16;
+    if (pch == 8)
         prod_mfma_kernel<8><<<n_tiles, 256, prod_lds(kmax), s>>>(D, DS, tl, A, W, part, nrhs, bwd, kw, nks, xin, xout,
                                                                  n_free);
+    else if (pch == 32)
+        prod_mfma_kernel<32><<<n_tiles, 256, prod_lds(kmax), s>>>(D, DS, tl, A, W, part, nrhs, bwd, kw, nks, xin,
+                                                                  xout, n_free);
     else
         prod_mfma_kernel<16><<<n_tiles, 256, prod_lds(kmax), s>>>(D, DS, tl, A, W, part, nrhs, bwd, kw, nks, xin,
                                                                   xout, n_free);
 }
 
-// algorithmic bytes of one product launch: factor entries, vector rows in and out
+// algorithmic bytes of one product launch: the factor's nonzero entries
+// (the lower triangle of L11^-1 and L21 L11^-1), vector rows in and out
 static double prod_bytes(const SolveHeight &SH, int nrhs) {
-    return 8.0 * ((double)SH.sum_mns + (double)nrhs * (double)(SH.sum_m + SH.sum_ns));
+    return 8.0 * ((double)SH.sum_nz + (double)nrhs * (double)(SH.sum_m + SH.sum_ns));
 }
 
 hipError_t solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S,
@@ -672,8 +689,8 @@ hipError_t solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS
             prod1_kernel<<<SH.n_fwd, 256, 0, s>>>(D, DS, tl, Pbuf, W, part, 0, DS.fwd_kw, DS.fwd_nks, xin, xout,
                                                   P.n_free);
         else
-            launch_prod(SH.n_fwd, SH.fwd_kmax, s, D, DS, tl, Pbuf, W, part, nrhs, 0, DS.fwd_kw, DS.fwd_nks, xin, xout,
-                        P.n_free);
+            launch_prod(SH.n_fwd, SH.fwd_kmax, SH.fwd_cfg.pch, s, D, DS, tl, Pbuf, W, part, nrhs, 0, DS.fwd_kw,
+                        DS.fwd_nks, xin, xout, P.n_free);
         if (timed) pt->end(s);
         // single-slice fronts were finished by the product itself
         if (SH.fwd_split)
@@ -699,8 +716,8 @@ hipError_t solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS
             prod1_kernel<<<SH.n_bwd, 256, 0, s>>>(D, DS, tl, PTbuf, W, part, 1, DS.bwd_kw, DS.bwd_nks, xin, xout,
                                                   P.n_free);
         else
-            launch_prod(SH.n_bwd, SH.bwd_kmax, s, D, DS, tl, PTbuf, W, part, nrhs, 1, DS.bwd_kw, DS.bwd_nks, xin,
-                        xout, P.n_free);
+            launch_prod(SH.n_bwd, SH.bwd_kmax, SH.bwd_cfg.pch, s, D, DS, tl, PTbuf, W, part, nrhs, 1, DS.bwd_kw,
+                        DS.bwd_nks, xin, xout, P.n_free);
         if (timed) pt->end(s);
         if (SH.bwd_split)
             bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(

@@ -11,7 +11,8 @@
 #include <algorithm>
 
 void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std::vector<int> &m,
-                      const std::vector<int32_t> &flist, SolvePlan &S) {
+                      const std::vector<int32_t> &flist, SolvePlan &S,
+                      const std::vector<std::pair<ProdCfg, ProdCfg>> &cfg) {
     const int RT = SOLVE_RT, KS = SOLVE_KS;
     int nf = (int)ns.size();
     S.poff.assign(nf, 0);
@@ -37,16 +38,22 @@ void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std
     for (int h = 0; h <= P.height; h++) {
         const HeightInfo &H = P.heights[h];
         SolveHeight &SH = S.heights[h];
+        if (h < (int)cfg.size()) {
+            SH.fwd_cfg = cfg[h].first;
+            SH.bwd_cfg = cfg[h].second;
+        }
+        const ProdCfg &CF = SH.fwd_cfg, &CB = SH.bwd_cfg;
         SH.fwd_off = (int64_t)S.fwd_tiles.size() / 3;
         SH.bwd_off = (int64_t)S.bwd_tiles.size() / 3;
         for (int q = 0; q < H.n; q++) {
             int f = flist[H.list_off + q];
             SH.sum_mns += (int64_t)m[f] * ns[f];
+            SH.sum_nz += (int64_t)ns[f] * (ns[f] + 1) / 2 + (int64_t)(m[f] - ns[f]) * ns[f];
             SH.sum_m += m[f];
             SH.sum_ns += ns[f];
             // one k-slice (and no partial sums) when the inner dimension fits
-            const int kwf = ns[f] <= SOLVE_KMAX ? std::max(ns[f], 1) : KS;
-            const int kwb = m[f] <= SOLVE_KMAX ? std::max(m[f], 1) : KS;
+            const int kwf = ns[f] <= CF.kmax ? std::max(ns[f], 1) : CF.ks;
+            const int kwb = m[f] <= CB.kmax ? std::max(m[f], 1) : CB.ks;
             S.fwd_kw[f] = kwf;
             S.bwd_kw[f] = kwb;
             SH.fwd_kmax = std::max(SH.fwd_kmax, std::min(kwf, std::max(ns[f], 1)));
