@@ -79,12 +79,18 @@ def main():
                                                                      plain=not args.no_batch_plain,
                                                                      cpu_sample=0)}))
         return
-    t0 = time.perf_counter()
+    from jf_2026_05_19_amd import _native as N
     txt = decks.panel_deck(nx=args.nx, ny=args.ny)
-    arr = M.build_arrays(bdf.bdf_to_model_text(txt))
+    # deck text to model arrays: the native reader (the product path) and,
+    # for comparison, the host builder it is tested against
+    t0 = time.perf_counter()
+    arr = N.read_deck(txt)
     t_parse = time.perf_counter() - t0
-    log("model: nodes=%d q4=%d bars=%d ndof=%d (parse+arrays %.2fs)" % (arr.n_nodes, arr.n_q4, arr.n_bar, arr.ndof,
-                                                                       t_parse))
+    t0 = time.perf_counter()
+    M.build_arrays(bdf.bdf_to_model_text(txt))
+    t_parse_host = time.perf_counter() - t0
+    log("model: nodes=%d q4=%d bars=%d ndof=%d (deck read %.3fs native, %.3fs host builder)" %
+        (arr.n_nodes, arr.n_q4, arr.n_bar, arr.ndof, t_parse, t_parse_host))
     workload = ("stiffened panel %dx%d CQUAD4 + %d CBAR stringers, %d DOF, ND=%d, FP64" %
                 (args.nx, args.ny, arr.n_bar, arr.ndof, arr.nd))
     result = {"metric": METRIC, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -142,7 +148,7 @@ def main():
                      "KG_assembly": phase[4], "eigensolve": phase[5], "total": phase[7]},
         "factor_stats": {"n_free": stats[0], "nnz_L": stats[1], "factor_flops": stats[2], "fronts": stats[3],
                          "tree_height": stats[4], "lanczos_vectors": stats[5], "lanczos_block_steps": stats[6]},
-        "parse_s": t_parse, "upload_s": t_upload,
+        "parse_s": t_parse, "parse_s_host_builder": t_parse_host, "upload_s": t_upload,
     })
     result["roofline"] = roofline(prof)
     result["factor_rate"] = factor_rate(ctx, stats)
@@ -232,8 +238,8 @@ def factor_rate(ctx, stats):
 
 
 def _parse_variant(txt):
-    from jf_2026_05_19_amd import bdf, model as M
-    return M.build_arrays(bdf.bdf_to_model_text(txt))
+    from jf_2026_05_19_amd import _native as N
+    return N.read_deck(txt)
 
 
 def batched_sweep(ctx, n, steps, plain=True, cpu_sample=32):
@@ -471,9 +477,11 @@ def cylinder_case(ctx, n_fd, steps=3, seed=20260521):
     from jf_2026_05_19_amd import bdf, decks, model as M
     from jf_2026_05_19_amd import _native as N
     t0 = time.perf_counter()
-    arr = M.build_arrays(bdf.bdf_to_model_text(decks.cylinder_deck()))
-    log("cylinder: nodes=%d q4=%d t3=%d ndof=%d (parse %.1fs)" % (arr.n_nodes, arr.n_q4, arr.n_t3, arr.ndof,
-                                                                  time.perf_counter() - t0))
+    txt = decks.cylinder_deck()
+    t0 = time.perf_counter()
+    arr = N.read_deck(txt)
+    log("cylinder: nodes=%d q4=%d t3=%d ndof=%d (deck read %.2fs)" % (arr.n_nodes, arr.n_q4, arr.n_t3, arr.ndof,
+                                                                      time.perf_counter() - t0))
     dm = N.DeviceModel(ctx, arr)
     t1 = time.perf_counter()
     lam, modes = dm.solve(16)

@@ -65,6 +65,30 @@ typedef struct sol105_model_desc {
                                 /*   mode filters act on decks with >= 100        */
 } sol105_model_desc;
 
+/* Native deck reader: Nastran deck text (executive and case control, bulk
+ * data) to the model arrays of this path, host code only.  Replaces the
+ * reference's parse and model build for the supported card subset
+ * (JFEM/src/parsing/card_processor.jl:51-212, JFEM/src/ModelBuilder.jl:
+ * 421-823) and the per-element array derivation of assemble_stiffness
+ * (JFEM/src/solver/assembly.jl:1953-2530).  buckling_index picks the
+ * buckling subcase (METHOD) when there are several.  Decks outside the
+ * supported branch return SOL105_E_UNSUPPORTED, malformed ones
+ * SOL105_E_INVALID, with the message in err (err_len bytes, may be NULL). */
+typedef struct sol105_deck sol105_deck;
+int sol105_deck_read(const char *text, int64_t len, int32_t buckling_index, sol105_deck **out, char *err,
+                     int32_t err_len);
+int sol105_deck_free(sol105_deck *deck);
+/* out: n_nodes, n_q4, n_t3, n_bar, EIGRL ND, buckling subcase id, static
+ * subcase id, shell cards of the deck, SOL (101 when the deck names none) */
+int sol105_deck_sizes(const sol105_deck *deck, int64_t *out /* 9 */);
+/* A model description pointing into the deck's arrays (valid until
+ * sol105_deck_free), ready for sol105_model_upload. */
+int sol105_deck_desc(const sol105_deck *deck, sol105_model_desc *desc);
+/* GRID ids (ascending, the node ranks' ids), element ids and PSHELL ids in
+ * array order; any pointer may be NULL. */
+int sol105_deck_ids(const sol105_deck *deck, int64_t *node_ids, int64_t *q4_eid, int64_t *q4_pid, int64_t *t3_eid,
+                    int64_t *t3_pid, int64_t *bar_eid);
+
 /* Device selection and context.  No reference counterpart (the reference is
  * CPU-only); one context owns one HIP stream and its scratch. */
 int sol105_ctx_create(int device, sol105_ctx **out);

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libsol105_hip.so")
 SOL105_OK = 0
 ERRORS = {-1: "SOL105_E_INVALID", -2: "SOL105_E_UNSUPPORTED", -3: "SOL105_E_DEVICE", -4: "SOL105_E_SINGULAR",
           -5: "SOL105_E_NOCONV", -6: "SOL105_E_STATE", -7: "SOL105_E_PENDING"}
+SOL105_E_UNSUPPORTED = -2
 SOL105_E_PENDING = -7
 SERVER_MAX_MODES = 64   # a server request keeps at most this many modes
 
@@ -30,7 +31,8 @@ EXPORTS = [
     "sol105_batch_stats", "sol105_server_create", "sol105_server_destroy", "sol105_server_last_error",
     "sol105_server_add_model", "sol105_server_submit", "sol105_server_submit_ex", "sol105_server_poll",
     "sol105_server_wait", "sol105_server_wait_modes",
-    "sol105_server_stats",
+    "sol105_server_stats", "sol105_deck_read", "sol105_deck_free", "sol105_deck_sizes", "sol105_deck_desc",
+    "sol105_deck_ids",
 ]
 
 
@@ -116,12 +118,68 @@ def lib():
                                               ctypes.POINTER(ctypes.c_int64)]
         L.sol105_server_wait_modes.argtypes = [vp, ctypes.c_int64, dp, ip, ip, dp, dp]
         L.sol105_server_stats.argtypes = [vp, dp]
+        lp = ctypes.POINTER(ctypes.c_int64)
+        L.sol105_deck_read.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(vp),
+                                       ctypes.c_char_p, ctypes.c_int32]
+        L.sol105_deck_free.argtypes = [vp]
+        L.sol105_deck_sizes.argtypes = [vp, lp]
+        L.sol105_deck_desc.argtypes = [vp, ctypes.POINTER(ModelDesc)]
+        L.sol105_deck_ids.argtypes = [vp, lp, lp, lp, lp, lp, lp]
         _lib = L
     return _lib
 
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def read_deck(text, buckling_index=0, with_sol=False):
+    """Sol105Arrays of a Nastran deck (text or bytes) read by the native
+    reader (sol105_deck_read), with the deck's SOL as (arrays, sol) when
+    with_sol; raises model.UnsupportedModel for decks outside the supported
+    branch, ValueError for malformed ones."""
+    from .model import Sol105Arrays, UnsupportedModel
+    L = lib()
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    h = ctypes.c_void_p()
+    err = ctypes.create_string_buffer(1024)
+    rc = L.sol105_deck_read(data, len(data), int(buckling_index), ctypes.byref(h), err, len(err))
+    if rc == SOL105_E_UNSUPPORTED:
+        raise UnsupportedModel(err.value.decode())
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    try:
+        sz = np.zeros(9, np.int64)
+        lp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        L.sol105_deck_sizes(h, lp(sz))
+        n, nq, nt, nb = (int(x) for x in sz[:4])
+        d = ModelDesc()
+        L.sol105_deck_desc(h, ctypes.byref(d))
+        ids = [np.zeros(k, np.int64) for k in (n, nq, nq, nt, nt, nb)]
+        L.sol105_deck_ids(h, *[lp(a) for a in ids])
+
+        def arr(ptr, count, dtype, shape):
+            out = np.empty(count, dtype)
+            if count:
+                ctypes.memmove(out.ctypes.data, ptr, count * out.itemsize)
+            return out.reshape(shape)
+        dbl, i32, u8 = np.float64, np.int32, np.uint8
+        arrays = Sol105Arrays(
+            node_ids=ids[0], xyz=arr(d.xyz, 3 * n, dbl, (n, 3)),
+            q4=arr(d.q4_nodes, 4 * nq, i32, (nq, 4)), q4_eid=ids[1], q4_pid=ids[2],
+            q4prop=arr(d.q4_prop, 6 * nq, dbl, (nq, 6)),
+            t3=arr(d.t3_nodes, 3 * nt, i32, (nt, 3)), t3_eid=ids[3], t3_pid=ids[4],
+            t3prop=arr(d.t3_prop, 6 * nt, dbl, (nt, 6)),
+            bar=arr(d.bar_nodes, 2 * nb, i32, (nb, 2)), bar_eid=ids[5],
+            bar_vref=arr(d.bar_vref, 3 * nb, dbl, (nb, 3)), barprop=arr(d.bar_prop, 8 * nb, dbl, (nb, 8)),
+            spc_mask=arr(d.spc_mask, 6 * n, u8, (6 * n,)), F=arr(d.force, 6 * n, dbl, (6 * n,)),
+            nd=int(d.nd), v1=float(d.eigrl_v1), v2=float(d.eigrl_v2), k6rot=float(d.k6rot),
+            autospc=bool(d.autospc), node_has_shell=arr(d.node_has_shell, n, u8, (n,)),
+            node_has_bar=arr(d.node_has_bar, n, u8, (n,)),
+            buckling_subcase=int(sz[5]), static_subcase=int(sz[6]), shell_cards=int(sz[7]))
+        return (arrays, int(sz[8])) if with_sol else arrays
+    finally:
+        L.sol105_deck_free(h)
 
 
 class Context:

@@ -42,7 +42,8 @@ import time
 
 import numpy as np
 
-from . import bdf, model as M
+from . import model as M
+from ._native import read_deck
 
 # manifest_batch_core.jl:10-15
 FAST_DEFAULT_FLAGS = {
@@ -386,12 +387,11 @@ def run_manifest(manifest, solver, manifest_path=None, quiet=True, command_extra
         t0 = time.perf_counter()
         arr, err, sol = None, None, ""
         try:
-            with open(case["input"]) as fh:
-                mdl = bdf.bdf_to_model_text(fh.read())
-            sol = int(mdl.get("SOL", 101))
+            with open(case["input"], "rb") as fh:
+                arr, sol = read_deck(fh.read(), with_sol=True)
             if sol != 105:
+                arr = None
                 raise M.UnsupportedModel("SOL %d: this path solves SOL 105 only" % sol)
-            arr = M.build_arrays(mdl)
         except Exception as ex:  # the case fails, the batch goes on (manifest_batch_core.jl:452-472)
             err = "%s: %s" % (type(ex).__name__, ex)
         with open(log_path, "w") as fh:

@@ -1,0 +1,193 @@
+"""The native deck reader (sol105_deck_read, jf_2026_05_19_amd/csrc/deck.cpp)
+against the host model builder (bdf.py + model.py, which restate the
+reference's parser and model build: JFEM/src/parsing/card_processor.jl,
+JFEM/src/parsing/extract_properties.jl, extract_materials.jl,
+JFEM/src/ModelBuilder.jl:421-823), on the CPU.
+
+Every array and scalar of the model must be identical (bit for bit) over a
+corpus of decks: the generated plates, mixed CQUAD4/CTRIA3 meshes, stiffened
+panels and cylinders, the reference's bundled example, the same decks in
+small and large fixed fields, and edits that exercise each card rule (MAT1
+completion, PSHELL blank MID2 and MID3, PBARL, CBAR G0 and V, SPC1 THRU, SPC,
+GRID PS, number shorthand, continuations, comments, case-control
+inheritance and subcase pairing, EIGRL ranges, PARAM values).  Decks outside
+the supported branch must be refused with the same message."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from jf_2026_05_19_amd import bdf, decks, model as M
+
+N = pytest.importorskip("jf_2026_05_19_amd._native")
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUNDLED = open(os.path.join(HERE, "golden", "ref_sol105_quad_buckling.bdf")).read()
+
+ARRAYS = ("node_ids", "xyz", "q4", "q4_eid", "q4_pid", "q4prop", "t3", "t3_eid", "t3_pid", "t3prop", "bar",
+          "bar_eid", "bar_vref", "barprop", "spc_mask", "F", "node_has_shell", "node_has_bar")
+SCALARS = ("nd", "v1", "v2", "k6rot", "autospc", "buckling_subcase", "static_subcase", "n_shell_cards")
+
+
+def _lib_or_skip():
+    try:
+        N.lib()
+    except N.NativeLibraryMissing:
+        pytest.skip("native library not built")
+
+
+def _same(txt, buckling_index=0):
+    _lib_or_skip()
+    a = M.build_arrays(bdf.bdf_to_model_text(txt), buckling_index)
+    b = N.read_deck(txt, buckling_index)
+    for k in ARRAYS:
+        x, y = np.asarray(getattr(a, k)), np.asarray(getattr(b, k))
+        assert x.shape == y.shape and x.dtype.kind == y.dtype.kind and np.array_equal(x, y), k
+    for k in SCALARS:
+        assert getattr(a, k) == getattr(b, k), (k, getattr(a, k), getattr(b, k))
+    return b
+
+
+def _as_fixed(txt, large=False):
+    """Re-emit a free-field deck in small (8) or large (16, '*') fixed format."""
+    out, bulk = [], False
+    for line in txt.splitlines():
+        if line.strip().upper() == "BEGIN BULK":
+            bulk = True
+            out.append(line)
+            continue
+        if not bulk or "," not in line:
+            out.append(line)
+            continue
+        f = line.split(",")
+        name, data = f[0].strip(), [x.strip() for x in f[1:]]
+        w, per = (16, 4) if large else (8, 8)
+        chunks = [data[i:i + per] for i in range(0, len(data), per)] or [[]]
+        for ci, ch in enumerate(chunks):
+            head = (name + "*" if large else name) if ci == 0 else ("*" if large else "+")
+            out.append(head.ljust(8) + "".join(x.rjust(w)[:w] for x in ch))
+    return "\n".join(out) + "\n"
+
+
+CORPUS = {
+    "plate": lambda: decks.plate_deck(nx=7, ny=5, a=350.0, b=250.0, nd=4),
+    "plate_k6rot_drill": lambda: decks.plate_deck(nx=5, ny=5, k6rot=250.0, spc_drill=True),
+    "mixed": lambda: decks.plate_deck(nx=9, ny=6, tri_frac=0.4),
+    "all_tri": lambda: decks.plate_deck(nx=6, ny=4, tri_frac=1.0),
+    "panel": lambda: decks.panel_deck(nx=24, ny=12, stringer_every=4, nd=6, tri_frac=0.2),
+    "cylinder": lambda: decks.cylinder_deck(n_circ=24, n_axial=10, nd=4),
+    "bundled": lambda: BUNDLED,
+    "fixed_small": lambda: _as_fixed(decks.plate_deck(nx=4, ny=4, a=400.0, b=400.0, nd=2)),
+    "fixed_large": lambda: _as_fixed(decks.plate_deck(nx=4, ny=4, a=400.0, b=400.0, nd=2), large=True),
+    "fixed_panel_large": lambda: _as_fixed(decks.panel_deck(nx=8, ny=4, stringer_every=2, nd=3), large=True),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CORPUS))
+def test_corpus_identical(name):
+    _same(CORPUS[name]())
+
+
+def _edit(old, new, base=BUNDLED):
+    assert old in base, old
+    return base.replace(old, new)
+
+
+EDITS = {
+    # MAT1 completion: E + G gives nu, G + nu gives E, E alone nu = 0.3
+    "mat1_e_g": _edit("MAT1,1,7.0E10,2.6923E10,0.3,2700.", "MAT1,1,7.0E10,2.6923E10,,2700."),
+    "mat1_g_nu": _edit("MAT1,1,7.0E10,2.6923E10,0.3,2700.", "MAT1,1,,2.6923E10,0.3"),
+    "mat1_e": _edit("MAT1,1,7.0E10,2.6923E10,0.3,2700.", "MAT1,1,7.0E10"),
+    # PSHELL with MID2 (bending), MID3 pointing at another material, TS/T
+    "pshell_mid2_mid3": _edit("PSHELL,1,1,0.02", "PSHELL,1,1,0.02,1,0.9,2,0.7\nMAT1,2,7.0E10,2.0E10,0.3"),
+    "pshell_missing_mat": _edit("PSHELL,1,1,0.02", "PSHELL,1,9,0.02"),
+    # number forms: Nastran shorthand, D exponent, leading sign and dot
+    "numbers": _edit("GRID,2,,1.,0.,0.", "GRID,2,,1.0+0,-.0-3,0.D0")
+                .replace("FORCE,1,2,0,-1000.,1.,0.,0.", "FORCE,1,2,0,-1.+3,+1.,0.,0."),
+    # comments, blank lines, a continuation
+    "comments_continuation": _edit("CQUAD4,1,1,1,2,3,4", "$ a comment line\n\nCQUAD4,1,1,1,2,  $ split\n+,3,4"),
+    # SPC instead of SPC1, GRID PS, SPC1 THRU
+    "spc_forms": _edit("SPC1,1,123456,1,4", "SPC,1,1,123456,0.,4,123,0.\nSPC1,1,456,4,THRU,4")
+                 .replace("GRID,2,,1.,0.,0.", "GRID,2,,1.,0.,0.,,3"),
+    # case control: global keys inherited, the buckling subcase's own SPC
+    "case_global": _edit("SUBCASE 1\n  SPC = 1\n  LOAD = 1", "SPC = 1\nSUBCASE 1\n  LOAD = 1")
+                   .replace("SUBCASE 2\n  SPC = 1", "SUBCASE 2\n  SPC = 2")
+                   .replace("SPC1,1,123456,1,4", "SPC1,1,123456,1,4\nSPC1,2,123456,1"),
+    # pairing without STATSUB: the first subcase with a LOAD
+    "case_no_statsub": _edit("  STATSUB = 1\n", ""),
+    # EIGRL without a range, and ND blank (default 3)
+    "eigrl_plain": _edit("EIGRL,1,0.,1.0E9,3", "EIGRL,1,,,5"),
+    "eigrl_nd_blank": _edit("EIGRL,1,0.,1.0E9,3", "EIGRL,1,0.,1.0E9"),
+    # PARAM AUTOSPC spelled NO, and as a number; K6ROT
+    "autospc_no": _edit("PARAM,AUTOSPC,YES", "PARAM,AUTOSPC,NO\nPARAM,K6ROT,50."),
+    "autospc_zero": _edit("PARAM,AUTOSPC,YES", "PARAM,AUTOSPC,0"),
+    # a CTRIA3 beside the quad, an element on an unknown GRID (skipped)
+    "tria_and_orphan": _edit("CQUAD4,1,1,1,2,3,4", "CQUAD4,1,1,1,2,3,4\nCTRIA3,2,1,1,2,3\nCTRIA3,3,1,1,2,99"),
+    # bulk data only (no executive or case control)
+    "bulk_only": "\n".join(l for l in BUNDLED.splitlines()
+                           if l.strip() and l.split(",")[0] in ("GRID", "CQUAD4", "PSHELL", "MAT1", "SPC1",
+                                                                  "FORCE", "EIGRL", "PARAM")),
+}
+
+
+@pytest.mark.parametrize("name", sorted(EDITS))
+def test_card_rules_identical(name):
+    _same(EDITS[name])
+
+
+def test_bars_g0_and_vector():
+    base = decks.panel_deck(nx=6, ny=4, stringer_every=2, nd=3)
+    # orientation by a G0 node instead of a vector on every other bar
+    lines = []
+    for k, l in enumerate(base.splitlines()):
+        if l.startswith("CBAR") and k % 2:
+            f = l.split(",")
+            l = ",".join(f[:5] + [str(int(f[3]) + 1)])
+        lines.append(l)
+    b = _same("\n".join(lines) + "\n")
+    assert b.n_bar > 0
+
+
+def test_two_buckling_subcases():
+    txt = _edit("BEGIN BULK", "SUBCASE 3\n  SPC = 1\n  METHOD = 2\n  STATSUB = 1\nBEGIN BULK") \
+        .replace("EIGRL,1,0.,1.0E9,3", "EIGRL,1,0.,1.0E9,3\nEIGRL,2,,,6")
+    a, b = _same(txt, 0), _same(txt, 1)
+    assert (a.buckling_subcase, a.nd) == (2, 3) and (b.buckling_subcase, b.nd) == (3, 6)
+    with pytest.raises(ValueError):
+        N.read_deck(txt, 2)
+
+
+REFUSALS = [
+    (_edit("EIGRL,1,0.,1.0E9,3", "EIGRL,1,0.,1.0E9,3\nPARAM,SNORM,20."), "SNORM"),
+    (_edit("GRID,1,,0.,0.,0.", "GRID,1,,0.,0.,0.,5"), "CP/CD"),
+    (_edit("GRID,1,,0.,0.,0.", "GRID,1,2,0.,0.,0."), "CP/CD"),
+    (_edit("GRID,3,,1.,1.,0.", "GRID,3,,1.,1.,0.1"), "warped"),
+    (_edit("FORCE,1,2,0,", "FORCE,1,2,1,"), "CID"),
+    (_edit("SPC1,1,123456,1,4", "SPC1,1,123456,1\nSPC,1,4,3,0.5"), "enforced"),
+    (_edit("CQUAD4,1,1,1,2,3,4", "CQUAD4,1,1,1,2,3,4\nRBE2,9,1,123456,2\nCHEXA,5,1,1,2,3,4"), "unsupported cards"),
+    (_edit("CQUAD4,1,1,1,2,3,4", "CQUAD4,1,1,1,2,3,4\nCBAR,7,5,1,2,0.,0.,1.\nPBARL,5,1,,TUBE,1.,0.5"),
+     "only TYPE=BAR"),
+    (_edit("CQUAD4,1,1,1,2,3,4", "CQUAD4,1,1,1,2,3,4\nCBAR,7,5,1,2,0.,0.,1.,,1\nPBARL,5,1,,BAR,1.,0.5"),
+     "pin flags"),
+    (_edit("CQUAD4,1,1,1,2,3,4", "CQUAD4,1,1,1,2,3,4\nCBAR,7,5,1,2,0.,0.,1.,,,,0.,0.,1.\nPBARL,5,1,,BAR,1.,0.5"),
+     "offsets"),
+]
+
+
+@pytest.mark.parametrize("txt,msg", REFUSALS)
+def test_refusals_match(txt, msg):
+    _lib_or_skip()
+    with pytest.raises(M.UnsupportedModel, match=msg) as host:
+        M.build_arrays(bdf.bdf_to_model_text(txt))
+    with pytest.raises(M.UnsupportedModel) as native:
+        N.read_deck(txt)
+    assert str(native.value) == str(host.value)
+
+
+def test_malformed_deck_is_invalid():
+    _lib_or_skip()
+    with pytest.raises(ValueError):
+        N.read_deck(_edit("FORCE,1,2,0,", "FORCE,1,77,0,"))        # a load on an unknown GRID
+    with pytest.raises(ValueError):
+        N.read_deck(_edit("GRID,4,,0.,1.,0.", "GRID,x,,0.,1.,0."))  # a GRID without an id
