@@ -88,6 +88,10 @@ int sol105_deck_desc(const sol105_deck *deck, sol105_model_desc *desc);
  * array order; any pointer may be NULL. */
 int sol105_deck_ids(const sol105_deck *deck, int64_t *node_ids, int64_t *q4_eid, int64_t *q4_pid, int64_t *t3_eid,
                     int64_t *t3_pid, int64_t *bar_eid);
+/* Stress recovery points of each CBAR's PBARL section (n_bar x 8: C1 C2 D1 D2
+ * E1 E2 F1 F2, JFEM/src/parsing/extract_properties.jl:167-187), the
+ * bar_points argument of sol105_static_recover. */
+int sol105_deck_bar_points(const sol105_deck *deck, double *out);
 
 /* Device selection and context.  No reference counterpart (the reference is
  * CPU-only); one context owns one HIP stream and its scratch. */
@@ -159,6 +163,28 @@ int sol105_assemble_K_noalloc(sol105_ctx *ctx, sol105_model *model, double *d_va
  * (JFEM/src/solver/solve_case.jl:845-871, boundary_conditions.jl:530-1012).
  * u (6*n_nodes) may be NULL; free_mask (6*n_nodes) may be NULL. */
 int sol105_static_solve(sol105_ctx *ctx, sol105_model *model, double *u, uint8_t *free_mask);
+
+/* SOL 101 result recovery on the device-resident static state (after
+ * sol105_static_solve): the reactions r = K u - F (6*n_nodes, analysis
+ * frame) and the element results of the current displacement.  Replaces the
+ * result build of Solver.solve_case (JFEM/src/solver/solve_case.jl:126-186,
+ * 866-868) and recover_shell_stresses! / recover_bar_stresses!
+ * (JFEM/src/solver/stress_recovery.jl:163-513).  Element outputs are
+ * component-major, column c of element e at out[c * n + e]:
+ *   q4_out  (44 x n_q4): N (fx fy fxy), M (mx my mxy), Q (qx qy) at the
+ *           centroid, stresses at z1 = -T/2 and z2 = +T/2 (normal_x,
+ *           normal_y, shear_xy), strains at z1 and z2, corner N (4 x 3) and
+ *           corner M (4 x 3, mxy held at the centroid value), element frame;
+ *   t3_out  (20 x n_t3): the first 20 columns of the CQUAD4 layout;
+ *   bar_out (18 x n_bar): axial, shear_1, shear_2, torque, moment_a1,
+ *           moment_a2, moment_b1, moment_b2, axial stress, end A stresses at
+ *           C D E F, end B stresses at C D E F, 1 (0: bar shorter than 1e-9,
+ *           which the reference leaves out).
+ * bar_points (n_bar x 8: C1 C2 D1 D2 E1 E2 F1 F2 of the PBARL section) may be
+ * NULL (all zero).  Any output may be NULL; kernel_ms (4: reactions, CQUAD4,
+ * CTRIA3, CBAR device times in ms) may be NULL. */
+int sol105_static_recover(sol105_ctx *ctx, sol105_model *model, const double *bar_points, double *residual,
+                          double *q4_out, double *t3_out, double *bar_out, double *kernel_ms);
 
 /* KG assembly from the device-resident static displacement (or from u if
  * non-NULL).  Replaces Solver.assemble_geometric_stiffness

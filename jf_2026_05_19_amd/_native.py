@@ -32,7 +32,7 @@ EXPORTS = [
     "sol105_server_add_model", "sol105_server_submit", "sol105_server_submit_ex", "sol105_server_poll",
     "sol105_server_wait", "sol105_server_wait_modes",
     "sol105_server_stats", "sol105_deck_read", "sol105_deck_free", "sol105_deck_sizes", "sol105_deck_desc",
-    "sol105_deck_ids",
+    "sol105_deck_ids", "sol105_deck_bar_points", "sol105_static_recover",
 ]
 
 
@@ -125,6 +125,8 @@ def lib():
         L.sol105_deck_sizes.argtypes = [vp, lp]
         L.sol105_deck_desc.argtypes = [vp, ctypes.POINTER(ModelDesc)]
         L.sol105_deck_ids.argtypes = [vp, lp, lp, lp, lp, lp, lp]
+        L.sol105_deck_bar_points.argtypes = [vp, dp]
+        L.sol105_static_recover.argtypes = [vp, vp, dp, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -157,6 +159,8 @@ def read_deck(text, buckling_index=0, with_sol=False):
         L.sol105_deck_desc(h, ctypes.byref(d))
         ids = [np.zeros(k, np.int64) for k in (n, nq, nq, nt, nt, nb)]
         L.sol105_deck_ids(h, *[lp(a) for a in ids])
+        bar_points = np.zeros((nb, 8))
+        L.sol105_deck_bar_points(h, bar_points.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
 
         def arr(ptr, count, dtype, shape):
             out = np.empty(count, dtype)
@@ -176,7 +180,8 @@ def read_deck(text, buckling_index=0, with_sol=False):
             nd=int(d.nd), v1=float(d.eigrl_v1), v2=float(d.eigrl_v2), k6rot=float(d.k6rot),
             autospc=bool(d.autospc), node_has_shell=arr(d.node_has_shell, n, u8, (n,)),
             node_has_bar=arr(d.node_has_bar, n, u8, (n,)),
-            buckling_subcase=int(sz[5]), static_subcase=int(sz[6]), shell_cards=int(sz[7]))
+            buckling_subcase=int(sz[5]), static_subcase=int(sz[6]), shell_cards=int(sz[7]),
+            bar_points=bar_points)
         return (arrays, int(sz[8])) if with_sol else arrays
     finally:
         L.sol105_deck_free(h)
@@ -320,6 +325,24 @@ class DeviceModel:
         self.ctx.check(self.ctx.L.sol105_static_solve(self.ctx.h, self.h, _dp(u),
                                                       fm.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
         return u, fm.astype(bool)
+
+    def static_recover(self, bar_points=None, timings=False):
+        """SOL 101 result recovery on the device-resident static state:
+        ``(residual K u - F, CQUAD4 rows (n_q4, 44), CTRIA3 rows (n_t3, 20),
+        CBAR rows (n_bar, 18))`` in the column layout of
+        sol105_static_recover, plus the four kernel times in ms when
+        ``timings``."""
+        a = self.arr
+        pts = a.bar_points if bar_points is None else bar_points
+        pts = None if pts is None or a.n_bar == 0 else np.ascontiguousarray(pts, dtype=np.float64).reshape(a.n_bar, 8)
+        res = np.zeros(a.ndof)
+        q4 = np.zeros((44, a.n_q4)); t3 = np.zeros((20, a.n_t3)); bar = np.zeros((18, a.n_bar))
+        ms = np.zeros(4)
+        self.ctx.check(self.ctx.L.sol105_static_recover(
+            self.ctx.h, self.h, _dp(pts) if pts is not None else None, _dp(res), _dp(q4), _dp(t3), _dp(bar),
+            _dp(ms) if timings else None))
+        out = (res, q4.T.copy(), t3.T.copy(), bar.T.copy())
+        return out + (ms,) if timings else out
 
     def assemble_KG(self, u=None, want=True):
         vals = np.zeros(self.n_blocks * 36) if want else None

@@ -38,6 +38,17 @@ void launch_bar_kg(const ElemArgs &a, const double *u, double *eb, hipStream_t s
 void launch_block_gather(int64_t n_blocks, const int32_t *cptr, const int32_t *cidx, const double *eb,
                          double *vals, hipStream_t s);
 
+// SOL 101 result recovery (elements.hip); outputs component-major
+// (column c of element e at out[c * n + e]): CQUAD4 44, CTRIA3 20, CBAR 18
+// columns.  r = K u - F over the 6x6 BSR matrix.
+void launch_bsr_residual(int n_nodes, const int32_t *rp, const int32_t *col, const double *K, const double *u,
+                         const double *F, double *r, hipStream_t s);
+void launch_q4_recover(const ElemArgs &a, const double *u, double *out, hipStream_t s);
+void launch_t3_recover(const ElemArgs &a, const double *u, const double *t3w /* n_t3 */, double *out,
+                       hipStream_t s);
+void launch_bar_recover(const ElemArgs &a, const double *u, const double *pts /* n_bar x 8 */, double *out,
+                        hipStream_t s);
+
 // dlambda/dt kernels (elements.hip)
 // terms[3e..3e+2] = phi' dKe/dt phi, phi' KGe phi / t, psi' dKe/dt u (global frame vectors)
 void launch_q4_sens_terms(const ElemArgs &a, const double *u, const double *phi, const double *psi,

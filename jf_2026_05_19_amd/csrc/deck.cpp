@@ -44,7 +44,7 @@
 
 struct sol105_deck {
     std::vector<int64_t> node_ids, q4_eid, q4_pid, t3_eid, t3_pid, bar_eid;
-    std::vector<double> xyz, q4prop, t3prop, bar_vref, barprop, force;
+    std::vector<double> xyz, q4prop, t3prop, bar_vref, barprop, bar_points, force;
     std::vector<int32_t> q4, t3, bar;
     std::vector<uint8_t> spc_mask, has_shell, has_bar;
     double k6rot = 100.0, v1 = 0.0, v2 = 0.0;
@@ -427,6 +427,7 @@ struct PBarl {
     Key mid;
     bool bar = false;
     double A = 0, I1 = 0, I2 = 0, J = 0, K = 0;
+    double pts[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // stress points C1 C2 D1 D2 E1 E2 F1 F2
 };
 struct Shell {
     Num pid;
@@ -624,6 +625,11 @@ void build(const Cards &cards, const CaseControl &cc, int buckling_index, sol105
             else if (h > 0) p.J = h * std::pow(w, 3) / 3 * (1 - 0.63 * w / h);
             else p.J = p.I1 + p.I2;
             p.K = 10.0 * (1.0 + nu) / (12.0 + 11.0 * nu);
+            // (y, z) = C (h/2, w/2), D (-h/2, w/2), E (-h/2, -w/2), F (h/2, -w/2)
+            // (extract_properties.jl:176-178)
+            const double w2 = w / 2, h2 = h / 2;
+            const double pts[8] = {h2, w2, -h2, w2, -h2, -w2, h2, -w2};
+            std::copy(pts, pts + 8, p.pts);
         }
         pbarls[key_of(p.pid)] = p;
     }
@@ -746,7 +752,7 @@ void build(const Cards &cards, const CaseControl &cc, int buckling_index, sol105
             D.q4prop.insert(D.q4prop.end(), row, row + 6);
         }
     }
-    D.bar.clear(); D.bar_eid.clear(); D.bar_vref.clear(); D.barprop.clear();
+    D.bar.clear(); D.bar_eid.clear(); D.bar_vref.clear(); D.barprop.clear(); D.bar_points.clear();
     for (int gi = 0; gi < 2; gi++)
         for (const auto &kv : bars_by_group[gi]) {
             const Bar &b = kv.second;
@@ -779,6 +785,7 @@ void build(const Cards &cards, const CaseControl &cc, int buckling_index, sol105
             D.bar_vref.insert(D.bar_vref.end(), v, v + 3);
             const double row[8] = {prop.A, prop.I1, prop.I2, prop.J, prop.K, prop.K, mit->second.E, mit->second.G};
             D.barprop.insert(D.barprop.end(), row, row + 8);
+            D.bar_points.insert(D.bar_points.end(), prop.pts, prop.pts + 8);
         }
     const size_t ndof = 6 * (size_t)n;
     const auto pairs = subcase_pairing(cc);
@@ -1013,6 +1020,12 @@ int sol105_deck_ids(const sol105_deck *D, int64_t *node_ids, int64_t *q4_eid, in
     cp(D->t3_eid, t3_eid);
     cp(D->t3_pid, t3_pid);
     cp(D->bar_eid, bar_eid);
+    return SOL105_OK;
+}
+
+int sol105_deck_bar_points(const sol105_deck *D, double *out) {
+    if (!D || (!out && !D->bar_points.empty())) return SOL105_E_INVALID;
+    if (!D->bar_points.empty()) std::memcpy(out, D->bar_points.data(), D->bar_points.size() * sizeof(double));
     return SOL105_OK;
 }
 

@@ -1032,7 +1032,11 @@ struct T3Flags {
     bool valid, nonzero, plate, has_shear;
 };
 
-__device__ T3Flags t3_setup(T3Stage &S, ShellProps &P, double &Gref, const ElemArgs &A, int e, int lane) {
+// With `recovery` the plate block takes the result-recovery constitutive set
+// (Cb = D h^3/12, Cs = 5/6 G h, G = E/2(1+nu), FEMKernels.jl:7271-7279)
+// instead of the element's stiffness set.
+__device__ T3Flags t3_setup(T3Stage &S, ShellProps &P, double &Gref, const ElemArgs &A, int e, int lane,
+                           bool recovery = false) {
     bool valid = e < A.n_t3;
     Gref = 0.0;
     if (valid) {
@@ -1041,8 +1045,15 @@ __device__ T3Flags t3_setup(T3Stage &S, ShellProps &P, double &Gref, const ElemA
             int nd = A.t3[3 * e + k];
             for (int i = 0; i < 3; i++) p[3 * k + i] = A.xyz[3 * nd + i];
         }
-        shell_props(A.t3prop + 6 * e, A.k6rot, P);
-        Gref = A.t3prop[6 * e + 5];
+        if (recovery) {
+            const double *pr = A.t3prop + 6 * e;
+            const double rp[6] = {pr[0], pr[1], pr[2], 1.0, 5.0 / 6.0, pr[0] / (2.0 * (1.0 + pr[1]))};
+            shell_props(rp, 100.0, P);
+            Gref = rp[5];
+        } else {
+            shell_props(A.t3prop + 6 * e, A.k6rot, P);
+            Gref = A.t3prop[6 * e + 5];
+        }
         if (lane == 0) {
             t3_frame(p, S.R, S.lc);
             S.area = t3_derivs(S.lc, S.bv, S.cv);
@@ -1652,6 +1663,444 @@ __global__ void sens_combine_kernel(int n, const double *terms, double lam, doub
     dldt[e] = -num / phiKGphi;
 }
 
+// ---------------------------------------------------------------------------
+// SOL 101 result recovery (stress_recovery.jl:163-513).  Outputs are
+// component-major: column c of element e at out[c * n + e], so the threads of
+// a wavefront store each column contiguously.
+//   CQUAD4 (44): N(3) M(3) Q(2) sigma_z1(3) sigma_z2(3) eps_z1(3) eps_z2(3)
+//                corner N (4 x 3) corner M (4 x 3)
+//   CTRIA3 (20): the first 20 columns of the CQUAD4 layout
+//   CBAR   (18): axial shear_1 shear_2 torque moment_a1 moment_a2 moment_b1
+//                moment_b2, axial stress, end A stresses at C D E F, end B
+//                stresses at C D E F, 1 (0 for a bar shorter than 1e-9)
+// ---------------------------------------------------------------------------
+
+// MITC4 centre transverse-shear resultant of a quad with corner coordinates
+// lc and plate DOFs (w, thx, thy) of corner k at wp[3k..3k+2]: covariant
+// shears tied at the edge mid-points (0, -+1) and (-+1, 0), mapped through
+// the centre Jacobian, phi2 = min(1, 10 h^2 / (4 |det Jc|)), ts/t = 5/6.
+// FEMKernels.jl:5927-6001.
+__device__ void mitc4_center_shear(const double *lc, const double *wp, double G, double h, double *Q) {
+    const double tx[4] = {0.0, 0.0, -1.0, 1.0}, ty[4] = {-1.0, 1.0, 0.0, 0.0};
+    double g[2] = {0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const double xi = tx[t], eta = ty[t];
+        double dNr[4], dNs[4], J11, J12, J21, J22;
+        shape_derivs(xi, eta, dNr, dNs);
+        jacobian(lc, dNr, dNs, J11, J12, J21, J22);
+        const double Nk[4] = {0.25 * (1.0 - xi) * (1.0 - eta), 0.25 * (1.0 + xi) * (1.0 - eta),
+                              0.25 * (1.0 + xi) * (1.0 + eta), 0.25 * (1.0 - xi) * (1.0 + eta)};
+        const bool along_xi = t < 2;
+        const double cx = along_xi ? -J12 : -J22, cy = along_xi ? J11 : J21;
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            v += (along_xi ? dNr[k] : dNs[k]) * wp[3 * k] + cx * Nk[k] * wp[3 * k + 1] + cy * Nk[k] * wp[3 * k + 2];
+        g[along_xi ? 0 : 1] += 0.5 * v;
+    }
+    double dNr[4], dNs[4], J11, J12, J21, J22;
+    shape_derivs(0.0, 0.0, dNr, dNs);
+    jacobian(lc, dNr, dNs, J11, J12, J21, J22);
+    const double det = J11 * J22 - J12 * J21;
+    const double adet = fmax(fabs(det), 1e-30);
+    const double id = 1.0 / det;
+    const double i11 = J22 * id, i12 = -J12 * id, i21 = -J21 * id, i22 = J11 * id;
+    const double phi2 = fmin(1.0, 10.0 * h * h / fmax(4.0 * adet, 1e-30));
+    const double k = (5.0 / 6.0) * G * h;
+    const double t11 = k * i11, t12 = k * i12, t21 = k * i21, t22 = k * i22;
+    const double q0 = phi2 * ((i11 * t11 + i21 * t21) * g[0] + (i11 * t12 + i21 * t22) * g[1]);
+    const double q1 = phi2 * ((i12 * t11 + i22 * t21) * g[0] + (i12 * t12 + i22 * t22) * g[1]);
+    Q[0] = J11 * q0 + J21 * q1;
+    Q[1] = J12 * q0 + J22 * q1;
+}
+
+// One thread per CQUAD4: frame and local displacements, the Wilson membrane
+// modes condensed at the |det J|-weighted 2x2 Gauss points, Gauss-averaged
+// membrane strain and curvature (N = Cm eps, M = -br D kappa h^3/12), MITC4
+// shear, fibre stresses/strains at -+h/2, corner N and M extrapolated from
+// the Gauss points, the equilibrium shear (dMx/dx, dMy/dy) replacing each
+// shear component it undercuts, and the twisting moment held at its centroid
+// value over the corners.  FEMKernels.jl:6003-6338; stress_recovery.jl:89-133,
+// 242-311.
+__global__ void __launch_bounds__(128) q4_recover_kernel(ElemArgs A, const double *u, double *out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= A.n_q4) return;
+    const int n = A.n_q4;
+    double p[12];
+    int nd[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        nd[k] = A.q4[4 * e + k];
+#pragma unroll
+        for (int i = 0; i < 3; i++) p[3 * k + i] = A.xyz[3 * nd[k] + i];
+    }
+    double R[9], lc[8];
+    q4_frame(p, R, lc);
+    double ul[24];
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const double *ug = u + 6 * (size_t)nd[b / 2] + 3 * (b % 2);
+        const double x = ug[0], y = ug[1], z = ug[2];
+#pragma unroll
+        for (int i = 0; i < 3; i++) ul[3 * b + i] = R[3 * i] * x + R[3 * i + 1] * y + R[3 * i + 2] * z;
+    }
+    const double *prop = A.q4prop + 6 * e;
+    const double E = prop[0], nu = prop[1], h = prop[2], br = prop[3];
+    const double dm = E / (1.0 - nu * nu);
+    const double D[9] = {dm, dm * nu, 0.0, dm * nu, dm, 0.0, 0.0, 0.0, dm * ((1.0 - nu) / 2.0)};
+    double Cm[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) Cm[i] = D[i] * h;
+    // Gauss points (-,-), (+,-), (+,+), (-,+): compatible strain, curvature,
+    // Wilson-mode derivatives and the condensation system
+    double adj[4], ec[4][3], kp[4][3], dp[4][4];
+    double Kbb[16], rhs[4];
+#pragma unroll
+    for (int i = 0; i < 16; i++) Kbb[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) rhs[i] = 0.0;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const double r = (g == 1 || g == 2) ? PT : -PT, s = (g >= 2) ? PT : -PT;
+        double dNr[4], dNs[4], J11, J12, J21, J22;
+        shape_derivs(r, s, dNr, dNs);
+        jacobian(lc, dNr, dNs, J11, J12, J21, J22);
+        const double det = J11 * J22 - J12 * J21;
+        adj[g] = fmax(fabs(det), 1e-12);
+        const double id = 1.0 / det;
+        const double i11 = J22 * id, i12 = -J12 * id, i21 = -J21 * id, i22 = J11 * id;
+        double e0 = 0.0, e1 = 0.0, e2 = 0.0, k0 = 0.0, k1 = 0.0, k2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double dx = i11 * dNr[k] + i12 * dNs[k], dy = i21 * dNr[k] + i22 * dNs[k];
+            const double ux = ul[6 * k], uy = ul[6 * k + 1], tx = ul[6 * k + 3], ty = ul[6 * k + 4];
+            e0 += dx * ux;
+            e1 += dy * uy;
+            e2 += dy * ux + dx * uy;
+            k0 += dx * ty;
+            k1 += -dy * tx;
+            k2 += dy * ty - dx * tx;
+        }
+        ec[g][0] = e0; ec[g][1] = e1; ec[g][2] = e2;
+        kp[g][0] = k0; kp[g][1] = k1; kp[g][2] = k2;
+        dp[g][0] = i11 * (-2.0 * r);  // d(1 - xi^2)/dx
+        dp[g][1] = i21 * (-2.0 * r);  // d(1 - xi^2)/dy
+        dp[g][2] = i12 * (-2.0 * s);  // d(1 - eta^2)/dx
+        dp[g][3] = i22 * (-2.0 * s);  // d(1 - eta^2)/dy
+        const double cols[4][3] = {{dp[g][0], 0.0, dp[g][1]}, {0.0, dp[g][1], dp[g][0]},
+                                   {dp[g][2], 0.0, dp[g][3]}, {0.0, dp[g][3], dp[g][2]}};
+        double t[3];
+        mat3v(Cm, ec[g], t);
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            rhs[m] += adj[g] * (cols[m][0] * t[0] + cols[m][1] * t[1] + cols[m][2] * t[2]);
+            double cm[3];
+            mat3v(Cm, cols[m], cm);
+#pragma unroll
+            for (int l = 0; l < 4; l++)
+                Kbb[l * 4 + m] += adj[g] * (cols[l][0] * cm[0] + cols[l][1] * cm[1] + cols[l][2] * cm[2]);
+        }
+    }
+    double Kbi[16], al[4];
+    inv4(Kbb, Kbi);
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+        al[l] = -(Kbi[4 * l] * rhs[0] + Kbi[4 * l + 1] * rhs[1] + Kbi[4 * l + 2] * rhs[2] + Kbi[4 * l + 3] * rhs[3]);
+    double Cb[9];
+    const double cbs = h * h * h / 12.0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) Cb[i] = (br * D[i]) * cbs;
+    double eps[3] = {0.0, 0.0, 0.0}, kap[3] = {0.0, 0.0, 0.0}, area = 0.0;
+    double Ng[4][3], Mg[4][3];
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        double eg[3] = {ec[g][0] + dp[g][0] * al[0] + dp[g][2] * al[2],
+                        ec[g][1] + dp[g][1] * al[1] + dp[g][3] * al[3],
+                        ec[g][2] + dp[g][1] * al[0] + dp[g][0] * al[1] + dp[g][3] * al[2] + dp[g][2] * al[3]};
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            eps[i] += eg[i] * adj[g];
+            kap[i] += kp[g][i] * adj[g];
+        }
+        area += adj[g];
+        double mk[3];
+        mat3v(Cm, eg, Ng[g]);
+        mat3v(Cb, kp[g], mk);
+#pragma unroll
+        for (int i = 0; i < 3; i++) Mg[g][i] = -mk[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) { eps[i] /= area; kap[i] /= area; }
+    double N[3], Dk[3], M[3];
+    mat3v(Cm, eps, N);
+    mat3v(D, kap, Dk);
+#pragma unroll
+    for (int i = 0; i < 3; i++) M[i] = (-br * Dk[i]) * cbs;
+    double Q[2] = {0.0, 0.0};
+    if (br > 1e-12) {
+        double wp[12];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) wp[3 * k + i] = ul[6 * k + 2 + i];
+        mitc4_center_shear(lc, wp, E / (2.0 * (1.0 + nu)), h, Q);
+    }
+    // corner values: 2x2 Gauss extrapolation
+    double Nc[4][3], Mc[4][3];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const double xi = (c == 1 || c == 2) ? 1.0 : -1.0, eta = (c >= 2) ? 1.0 : -1.0;
+        const double lx1 = (PT - xi) / (2.0 * PT), lx2 = (xi + PT) / (2.0 * PT);
+        const double ly1 = (PT - eta) / (2.0 * PT), ly2 = (eta + PT) / (2.0 * PT);
+        const double w[4] = {lx1 * ly1, lx2 * ly1, lx2 * ly2, lx1 * ly2};
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            Nc[c][i] = w[0] * Ng[0][i] + w[1] * Ng[1][i] + w[2] * Ng[2][i] + w[3] * Ng[3][i];
+            Mc[c][i] = w[0] * Mg[0][i] + w[1] * Mg[1][i] + w[2] * Mg[2][i] + w[3] * Mg[3][i];
+        }
+    }
+    // equilibrium shear from the corner moments
+    if (fabs(br) > 1e-12) {
+        double dNr[4], dNs[4], J11, J12, J21, J22;
+        shape_derivs(0.0, 0.0, dNr, dNs);
+        jacobian(lc, dNr, dNs, J11, J12, J21, J22);
+        const double det = J11 * J22 - J12 * J21;
+        if (fabs(det) >= 1e-12) {
+            double rx = 0.0, sx = 0.0, ry = 0.0, sy = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                rx += dNr[k] * Mc[k][0]; sx += dNs[k] * Mc[k][0];
+                ry += dNr[k] * Mc[k][1]; sy += dNs[k] * Mc[k][1];
+            }
+            const double eq[2] = {(J22 / det) * rx + (-J12 / det) * sx, (-J21 / det) * ry + (J11 / det) * sy};
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+                if (isfinite(eq[i]) && fabs(eq[i]) < fabs(Q[i])) Q[i] = eq[i];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) Mc[c][2] = M[2];
+    double row[20];
+    const double z1 = -h / 2.0, z2 = h / 2.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        row[i] = N[i];
+        row[3 + i] = M[i];
+        row[14 + i] = eps[i] + z1 * kap[i];
+        row[17 + i] = eps[i] + z2 * kap[i];
+    }
+    row[6] = Q[0];
+    row[7] = Q[1];
+    mat3v(D, row + 14, row + 8);
+    mat3v(D, row + 17, row + 11);
+#pragma unroll
+    for (int c = 0; c < 20; c++) out[(size_t)c * n + e] = row[c];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            out[(size_t)(20 + 3 * c + i) * n + e] = Nc[c][i];
+            out[(size_t)(32 + 3 * c + i) * n + e] = Mc[c][i];
+        }
+}
+
+// One 64-lane wavefront per CTRIA3 (the macro plate set-up of the stiffness
+// kernel with the recovery constitutive set): constant-strain membrane and
+// curvature, the macro shear (interior DOFs -Kii^-1 Kie u, the three
+// virtual quads' MITC4 shears averaged by area) and, with weight w > 0, the
+// macro moment blend M = (1 - w) M + w M_macro with the fibre strains
+// rebuilt from the blended moment.  FEMKernels.jl:7213-7311, 7391-7432;
+// stress_recovery.jl:52-87, 356-366.
+__global__ void __launch_bounds__(256) t3_recover_kernel(ElemArgs A, const double *u, const double *wt,
+                                                         double *out) {
+    __shared__ T3Stage stage[EPB];
+    int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
+    int e = blockIdx.x * EPB + w;
+    T3Stage &S = stage[w];
+    ShellProps P;
+    double Gref;
+    T3Flags F = t3_setup(S, P, Gref, A, e, lane, true);
+    // S.out is free here: local DOFs at [0, 18), the 7-point plate field at
+    // [24, 45), per virtual quad (Qx, Qy, Mx, My, Mxy, area) at [48, 66)
+    double *ul = S.out, *ua = S.out + 24, *qr = S.out + 48;
+    const double br = F.valid ? A.t3prop[6 * e + 3] : 0.0;
+    const double E = F.valid ? A.t3prop[6 * e] : 0.0, nu = F.valid ? A.t3prop[6 * e + 1] : 0.0;
+    const double h = F.valid ? A.t3prop[6 * e + 2] : 1.0;
+    if (F.valid && lane < 6) {
+        const double *ug = u + 6 * (size_t)A.t3[3 * e + lane / 2] + 3 * (lane % 2);
+        for (int i = 0; i < 3; i++) ul[3 * lane + i] = S.R[i * 3] * ug[0] + S.R[i * 3 + 1] * ug[1] + S.R[i * 3 + 2] * ug[2];
+    }
+    __syncthreads();
+    if (F.plate && lane < 21) {
+        if (lane < 9) {
+            ua[lane] = ul[6 * (lane / 3) + 2 + lane % 3];
+        } else {
+            const int i = lane - 9;
+            double s = 0.0;
+            for (int j = 0; j < 9; j++) s += S.M[i * 21 + 12 + j] * ul[6 * (j / 3) + 2 + j % 3];
+            ua[lane] = -s;
+        }
+    }
+    __syncthreads();
+    if (F.plate && lane < 3) {
+        const T3Quad &Q = S.q[lane];
+        double wp[12];
+        for (int a = 0; a < 4; a++)
+            for (int i = 0; i < 3; i++) wp[3 * a + i] = ua[3 * T3_QUAD[lane][a] + i];
+        double kap[3] = {0.0, 0.0, 0.0}, area = 0.0;
+        for (int g = 0; g < 4; g++) {
+            for (int k = 0; k < 4; k++) {
+                const double dx = Q.gdx[g][k], dy = Q.gdy[g][k], tx = wp[3 * k + 1], ty = wp[3 * k + 2];
+                kap[0] += Q.gadj[g] * (dx * ty);
+                kap[1] += Q.gadj[g] * (-dy * tx);
+                kap[2] += Q.gadj[g] * (dy * ty - dx * tx);
+            }
+            area += Q.gadj[g];
+        }
+        const double dm = E / (1.0 - nu * nu);
+        const double k0 = kap[0] / area, k1 = kap[1] / area, k2 = kap[2] / area;
+        const double cbs = h * h * h / 12.0;
+        double Qs[2];
+        mitc4_center_shear(Q.lc, wp, E / (2.0 * (1.0 + nu)), h, Qs);
+        const double *qc = Q.lc;
+        const double a1 = (qc[2] - qc[0]) * (qc[7] - qc[1]) - (qc[6] - qc[0]) * (qc[3] - qc[1]);
+        const double a2 = (qc[4] - qc[2]) * (qc[7] - qc[3]) - (qc[6] - qc[2]) * (qc[5] - qc[3]);
+        qr[6 * lane] = Qs[0];
+        qr[6 * lane + 1] = Qs[1];
+        qr[6 * lane + 2] = (-br * (dm * k0 + dm * nu * k1)) * cbs;
+        qr[6 * lane + 3] = (-br * (dm * nu * k0 + dm * k1)) * cbs;
+        qr[6 * lane + 4] = (-br * (dm * ((1.0 - nu) / 2.0) * k2)) * cbs;
+        qr[6 * lane + 5] = (fabs(a1) + fabs(a2)) / 2.0;
+    }
+    __syncthreads();
+    if (F.valid && lane == 0) {
+        const int n = A.n_t3;
+        double row[20];
+        for (int c = 0; c < 20; c++) row[c] = 0.0;
+        if (F.nonzero) {
+            const double dm = E / (1.0 - nu * nu);
+            const double D[9] = {dm, dm * nu, 0.0, dm * nu, dm, 0.0, 0.0, 0.0, dm * ((1.0 - nu) / 2.0)};
+            double eps[3] = {0.0, 0.0, 0.0}, kap[3] = {0.0, 0.0, 0.0};
+            for (int i = 0; i < 3; i++) {
+                const double b = S.bv[i], c = S.cv[i];
+                const double ux = ul[6 * i], uy = ul[6 * i + 1], tx = ul[6 * i + 3], ty = ul[6 * i + 4];
+                eps[0] += b * ux; eps[1] += c * uy; eps[2] += c * ux + b * uy;
+                kap[0] += b * ty; kap[1] += -c * tx; kap[2] += c * ty - b * tx;
+            }
+            const double cbs = h * h * h / 12.0;
+            double De[3], Dk[3], M[3];
+            mat3v(D, eps, De);
+            mat3v(D, kap, Dk);
+            for (int i = 0; i < 3; i++) {
+                row[i] = De[i] * h;
+                M[i] = (-br * Dk[i]) * cbs;
+            }
+            double Qm[2] = {0.0, 0.0}, Mm[3] = {0.0, 0.0, 0.0};
+            if (br > 1e-12) {
+                const double As = qr[5] + qr[11] + qr[17];
+                if (As > 1e-12) {
+                    for (int i = 0; i < 2; i++) Qm[i] = (qr[5] * qr[i] + qr[11] * qr[6 + i] + qr[17] * qr[12 + i]) / As;
+                    for (int i = 0; i < 3; i++)
+                        Mm[i] = (qr[5] * qr[2 + i] + qr[11] * qr[8 + i] + qr[17] * qr[14 + i]) / As;
+                }
+            }
+            double e1[3], e2[3];
+            for (int i = 0; i < 3; i++) {
+                e1[i] = eps[i] + (-h / 2.0) * kap[i];
+                e2[i] = eps[i] + (h / 2.0) * kap[i];
+            }
+            const double wgt = wt[e];
+            if (wgt > 0.0 && fabs(br) > 1e-12 && isfinite(Mm[0]) && isfinite(Mm[1]) && isfinite(Mm[2])) {
+                for (int i = 0; i < 3; i++) M[i] = (1.0 - wgt) * M[i] + wgt * Mm[i];
+                // kappa = -(12 / (br h^3)) D^-1 M, D^-1 of the plane-stress matrix
+                const double f = -(12.0 / (br * h * h * h)), dinv = 1.0 / dm;
+                const double den = 1.0 - nu * nu;
+                const double kb[3] = {f * dinv * (M[0] - nu * M[1]) / den, f * dinv * (M[1] - nu * M[0]) / den,
+                                      f * dinv * M[2] / ((1.0 - nu) / 2.0)};
+                for (int i = 0; i < 3; i++) {
+                    const double em = (e1[i] + e2[i]) / 2.0;
+                    e1[i] = em + (-h / 2.0) * kb[i];
+                    e2[i] = em + (h / 2.0) * kb[i];
+                }
+            }
+            for (int i = 0; i < 3; i++) {
+                row[3 + i] = M[i];
+                row[14 + i] = e1[i];
+                row[17 + i] = e2[i];
+            }
+            row[6] = Qm[0];
+            row[7] = Qm[1];
+            mat3v(D, e1, row + 8);
+            mat3v(D, e2, row + 11);
+        }
+        for (int c = 0; c < 20; c++) out[(size_t)c * n + e] = row[c];
+    }
+}
+
+// One thread per CBAR: end forces k u in the bar frame and the stresses at the
+// recovery points C, D, E, F (y, z pairs in pts, n_bar x 8) with the BAR
+// section's sign -1 and I12 = 0.  stress_recovery.jl:432-513,
+// FEMKernels.jl:215-229.
+__global__ void __launch_bounds__(128) bar_recover_kernel(ElemArgs A, const double *u, const double *pts,
+                                                          double *out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= A.n_bar) return;
+    const int n = A.n_bar;
+    BarLocal B;
+    bar_setup(A, b, B);
+    double ul[12];
+    for (int blk = 0; blk < 4; blk++) {
+        const double *ug = u + 6 * (size_t)A.bar[2 * b + blk / 2] + 3 * (blk % 2);
+        for (int i = 0; i < 3; i++) ul[3 * blk + i] = B.R[i * 3] * ug[0] + B.R[i * 3 + 1] * ug[1] + B.R[i * 3 + 2] * ug[2];
+    }
+    double f[12];
+    for (int i = 0; i < 12; i++) {
+        double s = 0.0;
+        for (int j = 0; j < 12; j++) s += bar_k(B, i, j) * ul[j];
+        f[i] = s;
+    }
+    const double *bp = A.barprop + 8 * b;
+    const double Ar = bp[0];
+    double Iy = bp[2], Iz = bp[1];
+    if (Iy == 0.0) Iy = Iz;
+    if (Iz == 0.0) Iz = Iy;
+    double row[18];
+    row[0] = f[6]; row[1] = -f[1]; row[2] = -f[2]; row[3] = -f[3];
+    row[4] = -f[5]; row[5] = f[4]; row[6] = f[11]; row[7] = -f[10];
+    row[8] = fabs(Ar) > 1e-30 ? f[6] / Ar : 0.0;
+    const double den = Iz * Iy;
+    for (int j = 0; j < 4; j++) {
+        const double y = pts[8 * b + 2 * j], z = pts[8 * b + 2 * j + 1];
+        for (int end = 0; end < 2; end++) {
+            const double m1 = row[4 + 2 * end], m2 = row[5 + 2 * end];
+            const double s = fabs(den) > 1e-30 ? (m1 * Iy * y + m2 * Iz * z) / den
+                                              : m1 * y / fmax(Iz, 1e-30) + m2 * z / fmax(Iy, 1e-30);
+            row[9 + 4 * end + j] = -s;
+        }
+    }
+    const bool ok = B.L >= 1e-9;
+    for (int c = 0; c < 17; c++) out[(size_t)c * n + b] = ok ? row[c] : 0.0;
+    out[(size_t)17 * n + b] = ok ? 1.0 : 0.0;
+}
+
+// r = K u - F over the 6x6 block rows: 6 lanes per block row (one per row of
+// the block), 32 block rows per 192-thread workgroup.
+__global__ void __launch_bounds__(192) bsr_residual_kernel(int n_nodes, const int32_t *rp, const int32_t *col,
+                                                           const double *val, const double *x, const double *F,
+                                                           double *r) {
+    const int I = blockIdx.x * 32 + threadIdx.x / 6, ci = threadIdx.x % 6;
+    if (I >= n_nodes) return;
+    double acc = 0.0;
+    for (int k = rp[I]; k < rp[I + 1]; k++) {
+        const double *v = val + (int64_t)k * 36 + ci * 6;
+        const double *xb = x + (int64_t)col[k] * 6;
+#pragma unroll
+        for (int cj = 0; cj < 6; cj++) acc += v[cj] * xb[cj];
+    }
+    r[(int64_t)I * 6 + ci] = acc - F[(int64_t)I * 6 + ci];
+}
+
 }  // namespace
 
 void launch_q4_ke(const ElemArgs &a, double *eb, hipStream_t s) {
@@ -1713,4 +2162,17 @@ void launch_q4_adjoint_rhs(const ElemArgs &a, const double *u, const double *phi
 void launch_bar_adjoint_rhs(const ElemArgs &a, const double *phi, double lam, double *eg, hipStream_t s) {
     if (a.n_bar == 0) return;
     bar_adjrhs_kernel<<<(a.n_bar + 127) / 128, 128, 0, s>>>(a, phi, lam, eg);
+}
+void launch_bsr_residual(int n_nodes, const int32_t *rp, const int32_t *col, const double *K, const double *u,
+                         const double *F, double *r, hipStream_t s) {
+    if (n_nodes > 0) bsr_residual_kernel<<<(n_nodes + 31) / 32, 192, 0, s>>>(n_nodes, rp, col, K, u, F, r);
+}
+void launch_q4_recover(const ElemArgs &a, const double *u, double *out, hipStream_t s) {
+    if (a.n_q4 > 0) q4_recover_kernel<<<(a.n_q4 + 127) / 128, 128, 0, s>>>(a, u, out);
+}
+void launch_t3_recover(const ElemArgs &a, const double *u, const double *t3w, double *out, hipStream_t s) {
+    if (a.n_t3 > 0) t3_recover_kernel<<<(a.n_t3 + EPB - 1) / EPB, 256, 0, s>>>(a, u, t3w, out);
+}
+void launch_bar_recover(const ElemArgs &a, const double *u, const double *pts, double *out, hipStream_t s) {
+    if (a.n_bar > 0) bar_recover_kernel<<<(a.n_bar + 127) / 128, 128, 0, s>>>(a, u, pts, out);
 }

@@ -124,6 +124,10 @@ struct sol105_model {
     // per-model AUTOSPC maxima
     std::vector<int32_t> mem_node_off;
     LanczosWork lw;
+    // result recovery: CTRIA3 macro-blend weights (host copy sized n_t3 once
+    // computed), bar recovery points and the device outputs
+    std::vector<double> t3w;
+    s105::DevBuf<double> d_t3w, d_barpts, d_rec;
     int ndof() const { return 6 * n_nodes; }
 };
 

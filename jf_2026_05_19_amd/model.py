@@ -62,6 +62,7 @@ class Sol105Arrays:
     buckling_subcase: int = 1
     static_subcase: int = 1
     shell_cards: int = -1         # CSHELL cards of the deck, assembled or not (-1: n_shell)
+    bar_points: np.ndarray = field(default=None)   # (n_bar, 8) stress points C1 C2 D1 D2 E1 E2 F1 F2
 
     @property
     def n_nodes(self):
@@ -212,7 +213,7 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
         q4pid.append(int(el["PID"]))
         q4prop.append(row)
     pbarls = model.get("PBARLs", {})
-    bars, bareid, vref, barprop = [], [], [], []
+    bars, bareid, vref, barprop, barpts = [], [], [], [], []
     for group in ("CBARs", "CBEAMs"):
         for key in sorted(model.get(group, {}), key=lambda k: int(k)):
             b = model[group][key]
@@ -238,6 +239,10 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
             vref.append(v)
             barprop.append([prop["A"], prop.get("I1", prop.get("I", 0.0)), prop.get("I2", prop.get("I", 0.0)),
                             prop["J"], prop.get("K1", 0.0), prop.get("K2", 0.0), float(mat["E"]), float(mat["G"])])
+            # stress recovery points of the BAR section (y, z) = C (h/2, w/2),
+            # D (-h/2, w/2), E (-h/2, -w/2), F (h/2, -w/2), extract_properties.jl:176-178
+            w2, h2 = prop["DIMS"][0] / 2, prop["DIMS"][1] / 2
+            barpts.append([h2, w2, -h2, w2, -h2, -w2, h2, -w2])
     ndof = 6 * n
     cc = model["CASE_CONTROL"]
     pairs = _subcase_pairing(cc)
@@ -321,4 +326,5 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
         node_has_shell=has_shell, node_has_bar=has_bar,
         buckling_subcase=int(buck_sid), static_subcase=int(stat_sid),
         shell_cards=len(model["CSHELLs"]),
+        bar_points=np.asarray(barpts, dtype=float).reshape(-1, 8),
     )

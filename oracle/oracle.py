@@ -78,6 +78,8 @@ def lib():
         L.or_q4_ke_global.argtypes = [_dp, _dp, ctypes.c_double, _dp, _dp, _dp]
         L.or_q4_frame.argtypes = [_dp, _dp, _dp, _dp]
         L.or_bar_ke_global.argtypes = [_dp, _dp, _dp, _dp, _dp]
+        L.or_q4_local_coords.argtypes = [_dp, _dp, _dp, _dp]
+        L.or_bar_frame.argtypes = [_dp, _dp, _dp, _dp, _dp]
         L.or_t3_frame.argtypes = [_dp, _dp, _dp]
         L.or_t3_plate_macro.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         _dp, _dp]
@@ -200,6 +202,23 @@ def frame3d_kg(L, P):
     out = np.zeros((12, 12))
     lib().or_frame3d_kg(L, P, _p(out))
     return out
+
+
+def q4_frame(p):
+    """(R rows v1, v2, v3; local corner coordinates 4 x 2) of a CQUAD4."""
+    v = [np.zeros(3) for _ in range(3)]
+    lc = np.zeros(8)
+    pp = _c(p, float)
+    lib().or_q4_frame(_p(pp), _p(v[0]), _p(v[1]), _p(v[2]))
+    lib().or_q4_local_coords(_p(pp), _p(v[0]), _p(v[1]), _p(lc))
+    return np.array(v), lc.reshape(4, 2)
+
+
+def bar_frame(p1, p2, vref):
+    """(R rows vx, vy, vz; length) of a CBAR without offsets."""
+    R = np.zeros(9); L = ctypes.c_double()
+    lib().or_bar_frame(_p(_c(p1, float)), _p(_c(p2, float)), _p(_c(vref, float)), _p(R), ctypes.byref(L))
+    return R.reshape(3, 3), L.value
 
 
 def bar_ke_global(p1, p2, vref, bp):

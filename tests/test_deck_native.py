@@ -26,7 +26,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 BUNDLED = open(os.path.join(HERE, "golden", "ref_sol105_quad_buckling.bdf")).read()
 
 ARRAYS = ("node_ids", "xyz", "q4", "q4_eid", "q4_pid", "q4prop", "t3", "t3_eid", "t3_pid", "t3prop", "bar",
-          "bar_eid", "bar_vref", "barprop", "spc_mask", "F", "node_has_shell", "node_has_bar")
+          "bar_eid", "bar_vref", "barprop", "bar_points", "spc_mask", "F", "node_has_shell", "node_has_bar")
 SCALARS = ("nd", "v1", "v2", "k6rot", "autospc", "buckling_subcase", "static_subcase", "n_shell_cards")
 
 

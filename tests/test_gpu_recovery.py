@@ -1,0 +1,131 @@
+"""GPU: SOL 101 result recovery (sol105_static_recover) against the oracle's
+restatement (oracle/recovery.py), on the device's own static displacement.
+
+Models: a mixed CQUAD4/CTRIA3 plate (CTRIA3 macro blend weights 0.5 and 1),
+a stringer panel with CBARs and CTRIA3s, a faceted cylinder (element frames
+off the basic axes), and the bundled deck.  Element rows: every quantity
+(N, M, Q, fibre stresses, fibre strains, corner N, corner M; bar forces,
+bar stresses) within 1e-9 of its largest magnitude over the model (FP64 in
+a different order of operations).  Reactions: within 1e-10 of
+max_i sum_j |K_ij| |u_j|.  Also: the reactions balance the applied load, a
+thickness change followed by a new solve recovers the new state, the results
+layout built from the device rows, and recovery before a solve is refused.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from jf_2026_05_19_amd import bdf, decks, model as M
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from jf_2026_05_19_amd import _native as N
+    c = N.Context(0)
+    yield c
+    c.close()
+
+
+CASES = {
+    "mixed_plate": lambda: decks.plate_deck(nx=14, ny=10, a=420.0, b=300.0, t=2.0, tri_frac=0.4, nd=3),
+    "panel_bars": lambda: decks.panel_deck(nx=24, ny=12, stringer_every=4, nd=4, tri_frac=0.2),
+    "cylinder": lambda: decks.cylinder_deck(n_circ=32, n_axial=12, nd=3),
+    "bundled": lambda: open(os.path.join(HERE, "golden", "ref_sol105_quad_buckling.bdf")).read(),
+}
+
+
+# column groups of one physical quantity (a component that is zero up to
+# round-off is measured against its group): N, M, Q, stresses, strains,
+# corner N, corner M; bar forces and moments, bar stresses
+SHELL_GROUPS = [(0, 3), (3, 6), (6, 8), (8, 14), (14, 20), (20, 32), (32, 44)]
+BAR_GROUPS = [(0, 8), (8, 17), (17, 18)]
+
+
+def _close_cols(dev, ref, rel=1e-9):
+    if len(ref) == 0:
+        assert len(dev) == 0
+        return
+    groups = BAR_GROUPS if ref.shape[1] == 18 else SHELL_GROUPS
+    for a, b in groups:
+        if a >= ref.shape[1]:
+            continue
+        scale = max(np.abs(ref[:, a:b]).max(), 1e-300)
+        err = np.abs(dev[:, a:b] - ref[:, a:b]).max() / scale
+        assert err <= rel, (a, b, err)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_recovery_parity(ctx, name):
+    from jf_2026_05_19_amd import _native as N
+    from oracle import oracle as O
+    from oracle import recovery as RC
+    arr = M.build_arrays(bdf.bdf_to_model_text(CASES[name]()))
+    print("recovery parity", name, "ndof", arr.ndof, "q4", arr.n_q4, "t3", arr.n_t3, "bar", arr.n_bar, flush=True)
+    dm = N.DeviceModel(ctx, arr)
+    try:
+        u, free = dm.static_solve()
+        res, q4, t3, bar, ms = dm.static_recover(timings=True)
+    finally:
+        dm.free()
+    K = O.assemble_K(arr)
+    rres, rq4, rt3, rbar = RC.recover(arr, u, K=K, bar_points=arr.bar_points)
+    scale = (abs(K) @ np.abs(u)).max()
+    assert np.abs(res - rres).max() <= 1e-10 * scale
+    _close_cols(q4, rq4)
+    _close_cols(t3, rt3)
+    _close_cols(bar, rbar)
+    if arr.n_t3:
+        w = RC.t3_macro_weights(arr.q4, arr.t3)
+        assert (w > 0).any()
+    # reactions balance the load; free rows carry only round-off
+    F = np.asarray(arr.F)
+    for c in range(3):
+        assert abs(res[c::6].sum() + F[c::6].sum()) <= 1e-8 * np.abs(F).sum()
+    assert np.abs(res[free]).max() <= 1e-9 * scale
+    assert np.all(ms >= 0)
+
+
+def test_results_layout_from_device(ctx):
+    from jf_2026_05_19_amd import results as R
+    arr = M.build_arrays(bdf.bdf_to_model_text(CASES["panel_bars"]()))
+    u_out, scal, js = R.solve_static(arr, ctx=ctx)
+    assert len(js["displacements"]) == arr.n_nodes
+    assert [d["grid_id"] for d in js["displacements"]] == [int(g) for g in arr.node_ids]
+    assert len(js["forces"]["quad4"]) == arr.n_q4 and len(js["forces"]["tria3"]) == arr.n_t3
+    assert len(js["forces"]["cbar"]) == arr.n_bar and len(js["stresses"]["cbar"]) == arr.n_bar
+    assert len(js["forces_bilin"]["quad4"]) == 5 * arr.n_q4
+    eq = js["solver_diagnostics"]["equilibrium"]["spc_reaction_sum"]
+    F = np.asarray(arr.F)
+    assert abs(eq["fx"] + F[0::6].sum()) <= 1e-8 * np.abs(F).sum()
+    assert len(scal) == arr.n_q4 + arr.n_t3 + arr.n_bar
+
+
+def test_recovery_follows_a_thickness_change(ctx):
+    from jf_2026_05_19_amd import _native as N
+    from oracle import recovery as RC
+    arr = M.build_arrays(bdf.bdf_to_model_text(CASES["mixed_plate"]()))
+    dm = N.DeviceModel(ctx, arr)
+    try:
+        with pytest.raises(N.Sol105Error):
+            dm.static_recover()                       # no static state yet
+        dm.static_solve()
+        t = np.linspace(1.5, 2.5, arr.n_shell)
+        dm.set_shell_thickness(t)
+        with pytest.raises(N.Sol105Error):
+            dm.static_recover()                       # the thickness change invalidated u
+        u, _ = dm.static_solve()
+        res, q4, t3, bar = dm.static_recover()
+    finally:
+        dm.free()
+    arr2 = M.build_arrays(bdf.bdf_to_model_text(CASES["mixed_plate"]()))
+    arr2.q4prop = np.array(arr2.q4prop); arr2.t3prop = np.array(arr2.t3prop)
+    arr2.q4prop[:, 2] = t[:arr.n_q4]
+    arr2.t3prop[:, 2] = t[arr.n_q4:]
+    _, rq4, rt3, _ = RC.recover(arr2, u)
+    _close_cols(q4, rq4)
+    _close_cols(t3, rt3)
