@@ -58,6 +58,7 @@ def main():
     ap.add_argument("--served-requests", type=int, default=120,
                     help="requests of the served-stream leg (0: no served leg)")
     ap.add_argument("--served-only", action="store_true", help="only the served-stream leg")
+    ap.add_argument("--no-static", action="store_true", help="skip the SOL 101 result-recovery leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -154,6 +155,8 @@ def main():
     result["factor_rate"] = factor_rate(ctx, stats)
     if cpu_ref is not None:
         result["accuracy_vs_cpu"] = accuracy_vs_cpu(dm, cpu_ref)
+    if not args.no_static:
+        result["static_recovery"] = static_recovery(dm, arr, cpu_sample=0 if args.no_cpu_baseline else 300)
     dm.free()
     if args.batch_variants > 0:
         result["batched"] = batched_sweep(ctx, args.batch_variants, args.batch_steps,
@@ -203,6 +206,65 @@ def roofline(prof):
             "traffic": PROD_TRAFFIC_BYTES, "algorithmic_bytes_per_launch": r["algorithmic_bytes_per_launch"],
             "avg_launch_us": r["avg_launch_us"], "launches": la, "total_ms": ms,
             "leaf_height": _rate(*prof["leaf"]), "upper_heights": _rate(*prof["upper"])}
+
+
+def recovery_bytes(arr, n_blocks):
+    """Algorithmic HBM bytes of the result-recovery launches: every input
+    read once and every output written once (node coordinates and
+    displacements once per node).  Per CQUAD4 4 node indices + 4 property
+    words + 44 output words; per CTRIA3 3 + 4 + 20 + the macro-blend weight;
+    per CBAR 2 indices + 3 + 8 + 8 words in and 18 out; per node 3 + 6 words;
+    reactions: the K blocks (36 words + a column index each), u, F and r."""
+    nodes = 72 * arr.n_nodes
+    return {"q4": (16 * arr.n_q4 + 8 * (4 + 44) * arr.n_q4 + nodes) if arr.n_q4 else 0,
+            "t3": (12 * arr.n_t3 + 8 * (4 + 20 + 1) * arr.n_t3 + nodes) if arr.n_t3 else 0,
+            "bar": (8 * arr.n_bar + 8 * (3 + 8 + 8 + 18) * arr.n_bar + nodes) if arr.n_bar else 0,
+            "residual": n_blocks * (36 * 8 + 4) + 4 * (arr.n_nodes + 1) + 3 * 8 * arr.ndof}
+
+
+def static_recovery(dm, arr, reps=5, cpu_sample=300):
+    """SOL 101 leg on the solved static state: the device's result recovery
+    (reactions, every CQUAD4/CTRIA3/CBAR row; sol105_static_recover) timed by
+    its own HIP events per kernel, against the algorithmic bytes; and the CPU
+    restatement's per-element rate on a sample of elements (the oracle, one
+    core) for scale."""
+    dm.static_recover()
+    ms = np.zeros(4)
+    wall = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = dm.static_recover(timings=True)
+        wall.append(time.perf_counter() - t0)
+        ms += out[4]
+    ms /= reps
+    nb = recovery_bytes(arr, dm.n_blocks)
+    kern = {}
+    for i, k in enumerate(("residual", "q4", "t3", "bar")):
+        if ms[i] > 0 and nb[k] > 0:
+            kern[k] = {"us": 1e3 * ms[i], "algorithmic_bytes": nb[k],
+                       "achieved_GBps": nb[k] / (ms[i] * 1e-3) / 1e9,
+                       "frac_hbm": nb[k] / (ms[i] * 1e-3) / 1e9 / 8000.0}
+    res = {"kernels": kern, "wall_ms_median": 1e3 * float(np.median(wall)),
+           "elements": {"q4": arr.n_q4, "t3": arr.n_t3, "bar": arr.n_bar}}
+    if cpu_sample > 0 and arr.n_q4 > 0:
+        from oracle import oracle as O
+        from oracle import recovery as RC
+        u = dm.static_solve()[0]
+        xyz = np.asarray(arr.xyz)
+        n = min(cpu_sample, arr.n_q4)
+        t0 = time.perf_counter()
+        for e in range(n):
+            nodes = [int(v) for v in arr.q4[e]]
+            R, lc = O.q4_frame(xyz[nodes].ravel())
+            E, nu, h, br = (float(v) for v in arr.q4prop[e][:4])
+            RC.q4_recover(lc, RC._local_u(R, u, nodes), E, nu, h, br)
+        dt = time.perf_counter() - t0
+        res["cpu_oracle_q4_per_s"] = n / dt
+        if "q4" in kern:
+            res["q4_gpu_over_cpu_oracle"] = arr.n_q4 / (kern["q4"]["us"] * 1e-6) / (n / dt)
+    log("static recovery: %s, wall %.2f ms" % (", ".join("%s %.1f us" % (k, v["us"]) for k, v in kern.items()),
+                                               res["wall_ms_median"]))
+    return res
 
 
 def accuracy_vs_cpu(dm, ref):
@@ -492,6 +554,7 @@ def cylinder_case(ctx, n_fd, steps=3, seed=20260521):
         lam, modes = dm.solve(16, want_modes=(k == steps - 1))
         lat.append(time.perf_counter() - ts)
     tim, st = ctx.timings(), ctx.stats()
+    rec = static_recovery(dm, arr, cpu_sample=0)
     ts = time.perf_counter()
     sens = [dm.sensitivity_thickness(k) for k in range(len(lam))]
     t_sens = time.perf_counter() - ts
@@ -525,6 +588,7 @@ def cylinder_case(ctx, n_fd, steps=3, seed=20260521):
                                                                "eigensolve": tim[5]},
             "lambda": [float(x) for x in lam], "lanczos_vectors": int(st[5]), "factor_flops": float(st[2]),
             "sensitivity_ms_all_shells_all_modes": 1e3 * t_sens, "shells": arr.n_shell, "modes": len(lam),
+            "static_recovery": rec,
             "fd_check": {"shells": [int(e) for e in elems], "max_abs_err": float(max(errs)),
                          "max_err_rel_to_max_sensitivity": float(max(errs)) / scale, "step": "1e-4 t, central",
                          "mode": 1}}

@@ -2054,12 +2054,21 @@ __global__ void __launch_bounds__(128) bar_recover_kernel(ElemArgs A, const doub
         const double *ug = u + 6 * (size_t)A.bar[2 * b + blk / 2] + 3 * (blk % 2);
         for (int i = 0; i < 3; i++) ul[3 * blk + i] = B.R[i * 3] * ug[0] + B.R[i * 3 + 1] * ug[1] + B.R[i * 3 + 2] * ug[2];
     }
+    // f = k u with the nonzero pattern of the local Timoshenko matrix (bar_k)
+    const double *q = ul;
     double f[12];
-    for (int i = 0; i < 12; i++) {
-        double s = 0.0;
-        for (int j = 0; j < 12; j++) s += bar_k(B, i, j) * ul[j];
-        f[i] = s;
-    }
+    f[0] = B.X * (q[0] - q[6]);
+    f[6] = B.X * (q[6] - q[0]);
+    f[3] = B.T * (q[3] - q[9]);
+    f[9] = B.T * (q[9] - q[3]);
+    f[2] = B.ay * (q[2] - q[8]) - B.by * (q[4] + q[10]);
+    f[8] = B.ay * (q[8] - q[2]) + B.by * (q[4] + q[10]);
+    f[4] = B.by * (q[8] - q[2]) + B.cy * q[4] + B.dy * q[10];
+    f[10] = B.by * (q[8] - q[2]) + B.dy * q[4] + B.cy * q[10];
+    f[1] = B.az * (q[1] - q[7]) + B.bz * (q[5] + q[11]);
+    f[7] = B.az * (q[7] - q[1]) - B.bz * (q[5] + q[11]);
+    f[5] = B.bz * (q[1] - q[7]) + B.cz * q[5] + B.dz * q[11];
+    f[11] = B.bz * (q[1] - q[7]) + B.dz * q[5] + B.cz * q[11];
     const double *bp = A.barprop + 8 * b;
     const double Ar = bp[0];
     double Iy = bp[2], Iz = bp[1];
