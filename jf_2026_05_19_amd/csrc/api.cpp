@@ -160,6 +160,28 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
             M->pull2_ok = pairs_ok;
             HIPCHK(hipStreamSynchronize(s));  // slot and pair are host temporaries
         }
+        {
+            // tile descriptors (the factor offset is that of P and of PT alike)
+            auto build = [&](const std::vector<int32_t> &tl, const std::vector<int32_t> &kw,
+                             const std::vector<int32_t> &nks, std::vector<TileDesc> &out) {
+                out.resize(tl.size() / 3);
+                for (size_t t = 0; t < out.size(); t++) {
+                    const int f = tl[3 * t];
+                    TileDesc &d = out[t];
+                    d.f = f; d.rt = tl[3 * t + 1]; d.ks = tl[3 * t + 2];
+                    d.ns = ns[f]; d.m = m[f]; d.piv0 = piv0[f]; d.kw = kw[f]; d.nks = nks[f];
+                    d.parent = cptr[f + 1] > cptr[f] ? 1 : 0;
+                    d.pad = 0;
+                    d.vec_off = vecoff[f]; d.poff = SP.poff[f]; d.row_off = rowoff[f];
+                }
+            };
+            std::vector<TileDesc> fd, bd;
+            build(SP.fwd_tiles, SP.fwd_kw, SP.fwd_nks, fd);
+            build(SP.bwd_tiles, SP.bwd_kw, SP.bwd_nks, bd);
+            HIPCHK(M->s_fdesc.upload(fd.data(), fd.size(), s));
+            HIPCHK(M->s_bdesc.upload(bd.data(), bd.size(), s));
+            HIPCHK(hipStreamSynchronize(s));  // fd and bd are host temporaries
+        }
         HIPCHK(M->s_lblk.upload(SP.lblk.data(), SP.lblk.size(), s));
         HIPCHK(M->d_P.alloc(SP.p_storage + SP.p_pad));
         HIPCHK(M->d_PT.alloc(SP.p_storage + SP.p_pad));
@@ -170,6 +192,8 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
         DS.poff = M->s_poff.p;
         DS.fwd_tiles = M->s_ftiles.p;
         DS.bwd_tiles = M->s_btiles.p;
+        DS.fwd_desc = M->s_fdesc.p;
+        DS.bwd_desc = M->s_bdesc.p;
         DS.fwd_tbase_rel = M->s_ftbase.p;
         DS.bwd_tbase_rel = M->s_btbase.p;
         DS.fwd_nks = M->s_fnks.p;

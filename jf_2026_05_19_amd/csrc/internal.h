@@ -103,6 +103,7 @@ struct sol105_model {
     s105::DevBuf<int64_t> s_poff;
     s105::DevBuf<int32_t> s_ftiles, s_btiles, s_ftbase, s_btbase, s_fnks, s_bnks, s_fkw, s_bkw, s_pullptr, s_pullsrc, s_pull2, s_lblk;
     bool pull2_ok = false;
+    s105::DevBuf<TileDesc> s_fdesc, s_bdesc;
     s105::DevBuf<double> d_P, d_PT, d_part;
     DevSolve dsolve;
     // state

@@ -7,8 +7,20 @@
 #include "factor.h"
 #include "plan.h"
 
+// One product tile and the front data its workgroup needs, in one 64-byte
+// record (the workgroup reads it with one scalar load): front, row tile,
+// k-slice, pivots ns, rows m, first pivot, k-slice width, k-slices of the
+// front, whether the front has children, and the front's vector slot, factor
+// offset (P or PT) and row-list offset.
+struct alignas(16) TileDesc {
+    int32_t f, rt, ks, ns, m, piv0, kw, nks, parent, pad;
+    int64_t vec_off, poff, row_off;
+};
+static_assert(sizeof(TileDesc) == 64, "TileDesc is one 64-byte record");
+
 struct DevSolve {
     const int64_t *poff;
+    const TileDesc *fwd_desc, *bwd_desc;  // per tile, in the order of fwd_tiles / bwd_tiles
     const int32_t *fwd_tiles, *bwd_tiles;
     const int32_t *fwd_tbase_rel, *bwd_tbase_rel;
     const int32_t *fwd_nks, *bwd_nks;

@@ -399,26 +399,25 @@ __global__ void __launch_bounds__(256) prod1_kernel(DevFronts D, DevSolve DS, co
 // each such release writes back the XCD's L2.)
 // PCH: four-k steps per chunk of factor loads (8: more wavefronts per SIMD
 // for short slices on large grids; 16 or 32: fewer dependent rounds)
-template <int PCH>
-__global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS, const int32_t *tiles,
+template <int PCH, bool FPULL>
+__global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS, const TileDesc *desc,
                                                         const double *A, double *W, double *part, int nrhs, int bwd,
-                                                        const int32_t *kwid, const int32_t *nks, const double *xin,
-                                                        double *xout, int n_free) {
+                                                        const double *xin, double *xout, int n_free) {
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int f = tiles[3 * tile], rt = tiles[3 * tile + 1], ks = tiles[3 * tile + 2];
-    const int ns = D.ns[f], m = D.m[f], piv0 = D.piv0[f];
+    const TileDesc T = desc[tile];  // workgroup-uniform: one scalar load
+    const int rt = T.rt, ks = T.ks, ns = T.ns, m = T.m, piv0 = T.piv0;
     const int nrows = bwd ? ns : m, inner = bwd ? m : ns, lda = bwd ? ns : m;
-    const bool whole = nks[f] == 1;
-    const int64_t slot0 = D.vec_off[f];
+    const bool whole = T.nks == 1;
+    const int64_t slot0 = T.vec_off;
     double *V = W + slot0 * MAXR;
-    const int kw = kwid[f];
+    const int kw = T.kw;
     const int k0 = ks * kw, kl = min(inner, k0 + kw) - k0;
     const int nst = (kl + 3) >> 2;  // four-k steps of the slice
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int t = threadIdx.x, lane = t & 63, kq = lane >> 4, col = lane & 15;
     const int r0 = rt * RT, rb = r0 + 16 * wv;  // this wavefront's first row
     const bool live = rb < nrows;                // wavefront-uniform
-    const double *Aw = A + DS.poff[f] + (int64_t)k0 * lda;
+    const double *Aw = A + T.poff + (int64_t)k0 * lda;
     const unsigned aoff = (unsigned)(min(rb + (lane & 15), nrows - 1) + kq * lda);
     // L11^-1 is lower triangular: forward pivot rows i use columns k <= i,
     // backward rows use k >= i below ns.  Four-k steps that are zero for all
@@ -427,75 +426,110 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
     int s_lo = 0, s_hi = nst;
     if (!bwd && rb + 16 <= ns) s_hi = rb + 15 < k0 ? 0 : min(nst, ((rb + 15 - k0) >> 2) + 1);
     if (bwd && rb > k0) s_lo = min(nst, (rb - k0) >> 2);
+    // Loads go out in dependency order so that every wait covers only what it
+    // needs (vector loads complete in order): first the index loads (pull
+    // pairs of the forward slice rows and of this wavefront's update rows,
+    // ancestor row numbers of the backward slice), then the first chunk of
+    // factor values and the right-hand-side values, then the loads that
+    // depend on the indices (child update rows, ancestor values).
+    const int2 *pairs = reinterpret_cast<const int2 *>(DS.pull2);
+    const bool parent = T.parent != 0;
+    // FPULL: a forward launch above the leaves (every front has children)
+    const bool fpull = FPULL && parent && pairs;  // forward slice rows add child rows (pair table)
+    const int kpad = 4 * nst;                     // slice rows, <= 4 * ceil(SOLVE_KMAX / 4) < 512
+    const int32_t *rows = D.rows + T.row_off;
+    int2 ep[2] = {{-1, -1}, {-1, -1}};
+    int arow[2] = {0, 0};
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int k = t + 256 * j, kk = k0 + min(k, kl - 1);
+        if (fpull && k < kl) ep[j] = pairs[slot0 + k0 + k];
+        if (bwd && k < kpad && kk >= ns) arow[j] = rows[kk];
+    }
+    // forward, single k-slice: the child rows pulled into this wavefront's
+    // update rows (written back below)
+    const bool uprows = FPULL && whole && parent && live && rb + 16 > ns;
+    int2 eu[4] = {{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};
+    if (uprows && pairs)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = rb + kq + 4 * v;
+            if (i >= ns && i < nrows && col < nrhs) eu[v] = pairs[slot0 + i];
+        }
     double av[PCH];
     auto load_a = [&](int s) {
 #pragma unroll
         for (int u = 0; u < PCH; u++) av[u] = Aw[(int64_t)(4 * min(s + u, s_hi - 1)) * lda + aoff];
     };
     if (live && s_lo < s_hi) load_a(s_lo);
-    // forward, single k-slice: the child rows pulled into this wavefront's
-    // update rows (written back below), loaded now so they arrive with the factor
-    const bool parent = D.child_ptr[f + 1] > D.child_ptr[f];  // leaves pull nothing
+    // child update rows (128 contiguous bytes each, in the pull map's order)
     double upd[4] = {0.0, 0.0, 0.0, 0.0};
-    if (!bwd && whole && parent && live && rb + 16 > ns)
+    if (uprows) {
+        if (pairs) {
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int i = rb + kq + 4 * v;
-            if (i >= ns && i < nrows && col < nrhs) upd[v] = pulled(DS, W, MAXR, slot0 + i, col, 0.0);
+            for (int v = 0; v < 4; v++) {
+                const double wa = W[(int64_t)max(eu[v].x, 0) * MAXR + col], wb = W[(int64_t)max(eu[v].y, 0) * MAXR + col];
+                upd[v] = eu[v].x >= 0 ? upd[v] + wa : upd[v];
+                upd[v] = eu[v].y >= 0 ? upd[v] + wb : upd[v];
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = rb + kq + 4 * v;
+                if (i >= ns && i < nrows && col < nrhs) upd[v] = pulled(DS, W, MAXR, slot0 + i, col, 0.0);
+            }
         }
+    }
     // LDS (dynamic, sized for the launch's widest slice): the vector slice
     // (4 nst rows, padded columns), then one 16 x 17 transpose tile per wavefront
     extern __shared__ double lds[];
     double (*Vs)[MAXR + 1] = reinterpret_cast<double (*)[MAXR + 1]>(lds);
-    {
-        // right-hand side or ancestor values, k fastest (x columns are
-        // contiguous); rows past the slice and columns past nrhs are zero
-        const int kpad = 4 * nst;
-        const int32_t *rows = D.rows + D.row_off[f];
-        for (int k = t; k < kpad; k += 256) {
-            const int kk = k0 + min(k, kl - 1);
-            const double *src = xin + piv0 + kk;
-            double sg = 1.0;
-            if (bwd && kk >= ns) {
-                src = xout + rows[kk];
-                sg = -1.0;
-            }
-            double xv[MAXR];
+    // the slice rows of this thread (t, and t + 256 when the slice is wider):
+    // right-hand side or ancestor values, k fastest (x columns are
+    // contiguous), plus the pulled child rows; rows past the slice and
+    // columns past nrhs are zero
+#pragma unroll 1
+    for (int j = 0; j < 2; j++) {
+        const int k = t + 256 * j;
+        if (k >= kpad) break;
+        const int kk = k0 + min(k, kl - 1);
+        const bool anc = bwd && kk >= ns;
+        const double *src = anc ? xout + (j ? arow[1] : arow[0]) : xin + piv0 + kk;
+        const double sg = anc ? -1.0 : 1.0;
+        double xv[MAXR];
 #pragma unroll
-            for (int c = 0; c < MAXR; c++) xv[c] = src[(int64_t)min(c, nrhs - 1) * n_free];
+        for (int c = 0; c < MAXR; c++) xv[c] = src[(int64_t)min(c, nrhs - 1) * n_free];
+        if (fpull) {
+            const int2 e = j ? ep[1] : ep[0];
+            const double *wa = W + (int64_t)max(e.x, 0) * MAXR, *wb = W + (int64_t)max(e.y, 0) * MAXR;
 #pragma unroll
-            for (int c = 0; c < MAXR; c++) Vs[k][c] = (k < kl && c < nrhs) ? sg * xv[c] : 0.0;
-        }
-        // forward: add the pulled child rows (128 contiguous bytes each, in
-        // the pull map's order), one slice row per thread, in two halves
-        if (!bwd && parent) {
-            __syncthreads();
-            const int2 *pairs = reinterpret_cast<const int2 *>(DS.pull2);
-            for (int k = t; k < kl; k += 256) {
-                const int64_t slot = slot0 + k0 + k;
-                if (pairs) {
-                    const int2 e = pairs[slot];
-                    const double *wa = W + (int64_t)max(e.x, 0) * MAXR, *wb = W + (int64_t)max(e.y, 0) * MAXR;
+            for (int h = 0; h < MAXR; h += 8) {
+                double xa[8], xb[8];
 #pragma unroll
-                    for (int h = 0; h < MAXR; h += 8) {
-                        double xa[8], xb[8];
+                for (int c = 0; c < 8; c++) {
+                    xa[c] = wa[h + c];
+                    xb[c] = wb[h + c];
+                }
 #pragma unroll
-                        for (int c = 0; c < 8; c++) {
-                            xa[c] = wa[h + c];
-                            xb[c] = wb[h + c];
-                        }
-#pragma unroll
-                        for (int c = 0; c < 8; c++) {
-                            double v = Vs[k][h + c];
-                            v = e.x >= 0 ? v + xa[c] : v;
-                            v = e.y >= 0 ? v + xb[c] : v;
-                            if (h + c < nrhs) Vs[k][h + c] = v;
-                        }
-                    }
-                } else {
-                    for (int c = 0; c < nrhs; c++) Vs[k][c] = pulled(DS, W, MAXR, slot, c, Vs[k][c]);
+                for (int c = 0; c < 8; c++) {
+                    double v = xv[h + c];
+                    v = e.x >= 0 ? v + xa[c] : v;
+                    v = e.y >= 0 ? v + xb[c] : v;
+                    Vs[k][h + c] = (k < kl && h + c < nrhs) ? v : 0.0;
                 }
             }
+        } else if (FPULL && parent) {
+            // a row with more than two child contributions somewhere: the
+            // general pull map
+#pragma unroll
+            for (int c = 0; c < MAXR; c++) {
+                double v = xv[c];
+                if (k < kl && c < nrhs) v = pulled(DS, W, MAXR, slot0 + k0 + k, c, v);
+                Vs[k][c] = (k < kl && c < nrhs) ? v : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < MAXR; c++) Vs[k][c] = (k < kl && c < nrhs) ? sg * xv[c] : 0.0;
         }
     }
     __syncthreads();
@@ -651,19 +685,28 @@ static size_t prod_lds(int kmax) {
 
 // one block-RHS product launch over n_tiles tiles whose widest k-slice is
 // kmax, pch four-k steps per chunk of factor loads (0: by the launch's size)
-static void launch_prod(int n_tiles, int kmax, int pch, hipStream_t s, const DevFronts &D, const DevSolve &DS,
-                        const int32_t *tl, const double *A, double *W, double *part, int nrhs, int bwd,
-                        const int32_t *kw, const int32_t *nks, const double *xin, double *xout, int n_free) {
-    if (pch == 0) pch = (kmax <= 192 && n_tiles >= 380) ? 8 : 16;
+template <bool FPULL>
+static void launch_prod_t(int n_tiles, int kmax, int pch, hipStream_t s, const DevFronts &D, const DevSolve &DS,
+                          const TileDesc *td, const double *A, double *W, double *part, int nrhs, int bwd,
+                          const double *xin, double *xout, int n_free) {
+    const size_t lds = prod_lds(kmax);
     if (pch == 8)
-        prod_mfma_kernel<8><<<n_tiles, 256, prod_lds(kmax), s>>>(D, DS, tl, A, W, part, nrhs, bwd, kw, nks, xin, xout,
-                                                                 n_free);
+        prod_mfma_kernel<8, FPULL><<<n_tiles, 256, lds, s>>>(D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
     else if (pch == 32)
-        prod_mfma_kernel<32><<<n_tiles, 256, prod_lds(kmax), s>>>(D, DS, tl, A, W, part, nrhs, bwd, kw, nks, xin,
-                                                                  xout, n_free);
+        prod_mfma_kernel<32, FPULL><<<n_tiles, 256, lds, s>>>(D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
     else
-        prod_mfma_kernel<16><<<n_tiles, 256, prod_lds(kmax), s>>>(D, DS, tl, A, W, part, nrhs, bwd, kw, nks, xin,
-                                                                  xout, n_free);
+        prod_mfma_kernel<16, FPULL><<<n_tiles, 256, lds, s>>>(D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
+}
+
+// pull: a forward launch above the leaves (its fronts add child update rows)
+static void launch_prod(int n_tiles, int kmax, int pch, bool pull, hipStream_t s, const DevFronts &D,
+                        const DevSolve &DS, const TileDesc *td, const double *A, double *W, double *part, int nrhs,
+                        int bwd, const double *xin, double *xout, int n_free) {
+    if (pch == 0) pch = (kmax <= 192 && n_tiles >= 380) ? 8 : 16;
+    if (pull && !bwd)
+        launch_prod_t<true>(n_tiles, kmax, pch, s, D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
+    else
+        launch_prod_t<false>(n_tiles, kmax, pch, s, D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
 }
 
 // algorithmic bytes of one product launch: the factor's nonzero entries
@@ -689,8 +732,8 @@ hipError_t solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS
             prod1_kernel<<<SH.n_fwd, 256, 0, s>>>(D, DS, tl, Pbuf, W, part, 0, DS.fwd_kw, DS.fwd_nks, xin, xout,
                                                   P.n_free);
         else
-            launch_prod(SH.n_fwd, SH.fwd_kmax, SH.fwd_cfg.pch, s, D, DS, tl, Pbuf, W, part, nrhs, 0, DS.fwd_kw,
-                        DS.fwd_nks, xin, xout, P.n_free);
+            launch_prod(SH.n_fwd, SH.fwd_kmax, SH.fwd_cfg.pch, h > 0, s, D, DS, DS.fwd_desc + SH.fwd_off, Pbuf, W,
+                        part, nrhs, 0, xin, xout, P.n_free);
         if (timed) pt->end(s);
         // single-slice fronts were finished by the product itself
         if (SH.fwd_split)
@@ -716,8 +759,8 @@ hipError_t solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS
             prod1_kernel<<<SH.n_bwd, 256, 0, s>>>(D, DS, tl, PTbuf, W, part, 1, DS.bwd_kw, DS.bwd_nks, xin, xout,
                                                   P.n_free);
         else
-            launch_prod(SH.n_bwd, SH.bwd_kmax, SH.bwd_cfg.pch, s, D, DS, tl, PTbuf, W, part, nrhs, 1, DS.bwd_kw,
-                        DS.bwd_nks, xin, xout, P.n_free);
+            launch_prod(SH.n_bwd, SH.bwd_kmax, SH.bwd_cfg.pch, false, s, D, DS, DS.bwd_desc + SH.bwd_off, PTbuf, W,
+                        part, nrhs, 1, xin, xout, P.n_free);
         if (timed) pt->end(s);
         if (SH.bwd_split)
             bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(

@@ -42,6 +42,12 @@ void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std
             SH.fwd_cfg = cfg[h].first;
             SH.bwd_cfg = cfg[h].second;
         }
+        // the product kernel stages at most SOLVE_KMAX slice rows (its LDS
+        // and its two slice rows per thread are sized for that)
+        for (ProdCfg *c : {&SH.fwd_cfg, &SH.bwd_cfg}) {
+            c->kmax = std::max(1, std::min(c->kmax, SOLVE_KMAX));
+            c->ks = std::max(4, std::min(c->ks, SOLVE_KMAX));
+        }
         const ProdCfg &CF = SH.fwd_cfg, &CB = SH.bwd_cfg;
         SH.fwd_off = (int64_t)S.fwd_tiles.size() / 3;
         SH.bwd_off = (int64_t)S.bwd_tiles.size() / 3;
