@@ -59,6 +59,7 @@ def main():
                     help="requests of the served-stream leg (0: no served leg)")
     ap.add_argument("--served-only", action="store_true", help="only the served-stream leg")
     ap.add_argument("--no-static", action="store_true", help="skip the SOL 101 result-recovery leg")
+    ap.add_argument("--no-autotune", action="store_true", help="time the default solve-product shapes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -119,6 +120,18 @@ def main():
         t1 = time.perf_counter()
         lam, _ = dm.solve(want_modes=False)
         log("warmup %d: %.2f ms lambda1=%.6f" % (w, 1e3 * (time.perf_counter() - t1), lam[0] if len(lam) else -1))
+    # default product shapes, then the autotuner (setup, untimed like the
+    # symbolic analysis): the timed steps below run the tuned shapes
+    t1 = time.perf_counter()
+    for k in range(max(args.steps // 2, 1)):
+        dm.solve(want_modes=False)
+    t_default = (time.perf_counter() - t1) / max(args.steps // 2, 1)
+    t1 = time.perf_counter()
+    tune = dm.autotune(7) if not args.no_autotune else None
+    if tune:
+        log("autotune: block solve %.3f -> %.3f ms (%d launches timed, %.2fs)" % (tune[0], tune[1], tune[2],
+                                                                             time.perf_counter() - t1))
+        dm.solve(want_modes=False)
     lat = []
     phase = np.zeros(8)
     t_start = time.perf_counter()
@@ -150,6 +163,9 @@ def main():
         "factor_stats": {"n_free": stats[0], "nnz_L": stats[1], "factor_flops": stats[2], "fronts": stats[3],
                          "tree_height": stats[4], "lanczos_vectors": stats[5], "lanczos_block_steps": stats[6]},
         "parse_s": t_parse, "parse_s_host_builder": t_parse_host, "upload_s": t_upload,
+        "ms_per_step_default_shapes": 1e3 * t_default,
+        "autotune": None if not tune else {"block_solve_ms_before": tune[0], "block_solve_ms_after": tune[1],
+                                           "launches_timed": tune[2], "heights": tune[3]},
     })
     result["roofline"] = roofline(prof)
     result["factor_rate"] = factor_rate(ctx, stats)
@@ -350,6 +366,16 @@ def batched_sweep(ctx, n, steps, plain=True, cpu_sample=32):
         ts = time.perf_counter()
         lam_b, st = B.solve(16)
         bt.append(time.perf_counter() - ts)
+    t_batch_default = float(np.mean(bt))
+    # the solve-product autotuner on the stacked model, then the same steps
+    tune = B.autotune(5)
+    log("batch autotune: block solve %.3f -> %.3f ms (%d launches timed)" % tune[:3])
+    lam_b, st = B.solve(16)
+    bt = []
+    for k in range(steps):
+        ts = time.perf_counter()
+        lam_b, st = B.solve(16)
+        bt.append(time.perf_counter() - ts)
     t_batch = float(np.mean(bt))
     stats = B.stats()
     B.free()
@@ -366,6 +392,9 @@ def batched_sweep(ctx, n, steps, plain=True, cpu_sample=32):
     return {"workload": "%d seeded plate variants 500x500 mm, mesh 10x10..80x80 (%d..%d DOF, %d total), ND=5, "
                         "t U(1,4) mm, E in {70e3, 210e3} MPa, FP64" % (n, min(ndofs), max(ndofs), sum(ndofs)),
             "batched_solves_per_s": n / t_batch, "plain_solves_per_s": n / t_plain,
+            "batched_solves_per_s_default_shapes": n / t_batch_default,
+            "autotune": {"block_solve_ms_before": tune[0], "block_solve_ms_after": tune[1],
+                         "launches_timed": tune[2], "heights": tune[3]},
             "speedup_vs_plain": t_plain / t_batch, "batch_ms": 1e3 * t_batch, "batch_steps": steps,
             "plain_p50_ms": float(np.percentile(lat_ms, 50)), "plain_p99_ms": float(np.percentile(lat_ms, 99)),
             "batched_latency_ms": 1e3 * t_batch,
@@ -548,6 +577,8 @@ def cylinder_case(ctx, n_fd, steps=3, seed=20260521):
     t1 = time.perf_counter()
     lam, modes = dm.solve(16)
     log("cylinder first solve (incl. symbolic analysis) %.2fs, lambda1 %.6f" % (time.perf_counter() - t1, lam[0]))
+    tune = dm.autotune(5)
+    log("cylinder autotune: block solve %.3f -> %.3f ms" % tune[:2])
     lat = []
     for k in range(steps):
         ts = time.perf_counter()
@@ -589,6 +620,8 @@ def cylinder_case(ctx, n_fd, steps=3, seed=20260521):
             "lambda": [float(x) for x in lam], "lanczos_vectors": int(st[5]), "factor_flops": float(st[2]),
             "sensitivity_ms_all_shells_all_modes": 1e3 * t_sens, "shells": arr.n_shell, "modes": len(lam),
             "static_recovery": rec,
+            "autotune": {"block_solve_ms_before": tune[0], "block_solve_ms_after": tune[1],
+                         "launches_timed": tune[2], "heights": tune[3]},
             "fd_check": {"shells": [int(e) for e in elems], "max_abs_err": float(max(errs)),
                          "max_err_rel_to_max_sensitivity": float(max(errs)) / scale, "step": "1e-4 t, central",
                          "mode": 1}}

@@ -32,7 +32,9 @@ EXPORTS = [
     "sol105_server_add_model", "sol105_server_submit", "sol105_server_submit_ex", "sol105_server_poll",
     "sol105_server_wait", "sol105_server_wait_modes",
     "sol105_server_stats", "sol105_deck_read", "sol105_deck_free", "sol105_deck_sizes", "sol105_deck_desc",
-    "sol105_deck_ids", "sol105_deck_bar_points", "sol105_static_recover",
+    "sol105_deck_ids", "sol105_deck_bar_points", "sol105_static_recover", "sol105_solve_cfg_get",
+    "sol105_solve_cfg_set", "sol105_solve_autotune", "sol105_solve_heights", "sol105_batch_autotune",
+    "sol105_batch_model",
 ]
 
 
@@ -127,6 +129,13 @@ def lib():
         L.sol105_deck_ids.argtypes = [vp, lp, lp, lp, lp, lp, lp]
         L.sol105_deck_bar_points.argtypes = [vp, dp]
         L.sol105_static_recover.argtypes = [vp, vp, dp, dp, dp, dp, dp, dp]
+        L.sol105_solve_cfg_get.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+        L.sol105_solve_cfg_set.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
+        L.sol105_solve_autotune.argtypes = [vp, vp, ctypes.c_int32, dp]
+        L.sol105_batch_autotune.argtypes = [vp, vp, ctypes.c_int32, dp]
+        L.sol105_batch_model.argtypes = [vp]
+        L.sol105_batch_model.restype = vp
+        L.sol105_solve_heights.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
 
@@ -273,6 +282,23 @@ def _model_desc(arr):
     return d, keep
 
 
+class _ModelRef:
+    """A model handle owned elsewhere (a batch's stacked model): the
+    solve-shape calls only."""
+
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, ctypes.c_void_p(h)
+
+    def solve_heights(self):
+        return DeviceModel.solve_heights(self)
+
+    def solve_cfg(self):
+        return DeviceModel.solve_cfg(self)
+
+    def set_solve_cfg(self, cfg):
+        DeviceModel.set_solve_cfg(self, cfg)
+
+
 class DeviceModel:
     """A model uploaded to the device (pattern, maps and ordering built)."""
 
@@ -343,6 +369,42 @@ class DeviceModel:
             _dp(ms) if timings else None))
         out = (res, q4.T.copy(), t3.T.copy(), bar.T.copy())
         return out + (ms,) if timings else out
+
+    def solve_cfg(self):
+        """Launch shapes of the triangular-solve products per tree height:
+        array (heights, 2, 3) of (kmax, ks, pch), forward then backward."""
+        L, c = self.ctx.L, self.ctx
+        n = ctypes.c_int32(0)
+        c.check(L.sol105_solve_cfg_get(c.h, self.h, None, ctypes.byref(n)))
+        cfg = np.zeros((n.value, 2, 3), np.int32)
+        c.check(L.sol105_solve_cfg_get(c.h, self.h, cfg.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                       ctypes.byref(n)))
+        return cfg
+
+    def solve_heights(self):
+        """Per tree height: (fronts, largest m, largest ns, nonzero factor
+        entries)."""
+        L, c = self.ctx.L, self.ctx
+        n = ctypes.c_int32(0)
+        c.check(L.sol105_solve_heights(c.h, self.h, None, ctypes.byref(n)))
+        sh = np.zeros((n.value, 4), np.int64)
+        c.check(L.sol105_solve_heights(c.h, self.h, sh.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                       ctypes.byref(n)))
+        return sh
+
+    def set_solve_cfg(self, cfg):
+        cfg = np.ascontiguousarray(cfg, dtype=np.int32).reshape(-1, 2, 3)
+        self.ctx.check(self.ctx.L.sol105_solve_cfg_set(self.ctx.h, self.h,
+                                                       cfg.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                       len(cfg)))
+
+    def autotune(self, reps=7):
+        """Time every height's product launch for candidate shapes on this
+        model's factor and keep the fastest; returns (block solve ms before,
+        after, launches timed, heights)."""
+        rep = np.zeros(4)
+        self.ctx.check(self.ctx.L.sol105_solve_autotune(self.ctx.h, self.h, int(reps), _dp(rep)))
+        return float(rep[0]), float(rep[1]), int(rep[2]), int(rep[3])
 
     def assemble_KG(self, u=None, want=True):
         vals = np.zeros(self.n_blocks * 36) if want else None
@@ -460,6 +522,26 @@ class Batch:
         keys = ("ms", "lanczos_vectors", "block_steps", "fallbacks", "n_free", "front_bytes", "factor_flops",
                 "tree_height")
         return dict(zip(keys, out.tolist()))
+
+    def _shapes(self):
+        m = _ModelRef(self.ctx, self.ctx.L.sol105_batch_model(self.h))
+        return m
+
+    def solve_heights(self):
+        return self._shapes().solve_heights()
+
+    def solve_cfg(self):
+        return self._shapes().solve_cfg()
+
+    def set_solve_cfg(self, cfg):
+        self._shapes().set_solve_cfg(cfg)
+
+    def autotune(self, reps=7):
+        """The solve-product autotuner on the stacked model (after a solve):
+        (block solve ms before, after, launches timed, heights)."""
+        rep = np.zeros(4)
+        self.ctx.check(self.ctx.L.sol105_batch_autotune(self.ctx.h, self.h, int(reps), _dp(rep)))
+        return float(rep[0]), float(rep[1]), int(rep[2]), int(rep[3])
 
 
 class Server:

@@ -82,6 +82,57 @@ void autospc_mask(const sol105_model *M, const std::vector<double> &kd, std::vec
     for (int i = 0; i < ndof; i++) free_out[i] = fixed[i] ? 0 : 1;
 }
 
+// Device copies of the tile-dependent parts of the solve plan (tiles, their
+// descriptors, per-front slice data, the partial-sum buffer): at the symbolic
+// analysis and whenever the product launch shapes change.
+int upload_solve_tiles(sol105_ctx *ctx, sol105_model *M) {
+    hipStream_t s = ctx->stream;
+    const SolvePlan &SP = M->splan;
+    const Symbolic &S = M->sym;
+    HIPCHK(M->s_ftiles.upload(SP.fwd_tiles.data(), SP.fwd_tiles.size(), s));
+    HIPCHK(M->s_btiles.upload(SP.bwd_tiles.data(), SP.bwd_tiles.size(), s));
+    HIPCHK(M->s_ftbase.upload(SP.fwd_tbase.data(), SP.fwd_tbase.size(), s));
+    HIPCHK(M->s_btbase.upload(SP.bwd_tbase.data(), SP.bwd_tbase.size(), s));
+    HIPCHK(M->s_fnks.upload(SP.fwd_nks.data(), SP.fwd_nks.size(), s));
+    HIPCHK(M->s_bnks.upload(SP.bwd_nks.data(), SP.bwd_nks.size(), s));
+    HIPCHK(M->s_fkw.upload(SP.fwd_kw.data(), SP.fwd_kw.size(), s));
+    HIPCHK(M->s_bkw.upload(SP.bwd_kw.data(), SP.bwd_kw.size(), s));
+    // tile descriptors (the factor offset is that of P and of PT alike)
+    auto build = [&](const std::vector<int32_t> &tl, const std::vector<int32_t> &kw, const std::vector<int32_t> &nks,
+                     std::vector<TileDesc> &out) {
+        out.resize(tl.size() / 3);
+        for (size_t t = 0; t < out.size(); t++) {
+            const int f = tl[3 * t];
+            const FrontInfo &F = S.fronts[f];
+            TileDesc &d = out[t];
+            d.f = f; d.rt = tl[3 * t + 1]; d.ks = tl[3 * t + 2];
+            d.ns = F.ns; d.m = F.m; d.piv0 = F.piv0; d.kw = kw[f]; d.nks = nks[f];
+            d.parent = S.children[f].empty() ? 0 : 1;
+            d.pad = 0;
+            d.vec_off = F.vec_off; d.poff = SP.poff[f]; d.row_off = F.row_off;
+        }
+    };
+    std::vector<TileDesc> fd, bd;
+    build(SP.fwd_tiles, SP.fwd_kw, SP.fwd_nks, fd);
+    build(SP.bwd_tiles, SP.bwd_kw, SP.bwd_nks, bd);
+    HIPCHK(M->s_fdesc.upload(fd.data(), fd.size(), s));
+    HIPCHK(M->s_bdesc.upload(bd.data(), bd.size(), s));
+    HIPCHK(M->d_part.alloc(std::max<int64_t>(solve_part_size(SP), 1)));
+    HIPCHK(hipStreamSynchronize(s));  // fd and bd are host temporaries
+    DevSolve &DS = M->dsolve;
+    DS.fwd_tiles = M->s_ftiles.p;
+    DS.bwd_tiles = M->s_btiles.p;
+    DS.fwd_desc = M->s_fdesc.p;
+    DS.bwd_desc = M->s_bdesc.p;
+    DS.fwd_tbase_rel = M->s_ftbase.p;
+    DS.bwd_tbase_rel = M->s_btbase.p;
+    DS.fwd_nks = M->s_fnks.p;
+    DS.bwd_nks = M->s_bnks.p;
+    DS.fwd_kw = M->s_fkw.p;
+    DS.bwd_kw = M->s_bkw.p;
+    return SOL105_OK;
+}
+
 int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
     hipStream_t s = ctx->stream;
     Symbolic &S = M->sym;
@@ -97,6 +148,8 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
     }
     FactorPlan &P = M->plan;
     build_plans(S, P, M->splan, flist);
+    if (!M->prod_cfg.empty()) solve_plan_build(P, ns, m, flist, M->splan, M->prod_cfg);
+    M->flist_host = flist;
     HIPCHK(M->f_ns.upload(ns.data(), nf, s));
     HIPCHK(M->f_nu.upload(nu.data(), nf, s));
     HIPCHK(M->f_m.upload(m.data(), nf, s));
@@ -130,14 +183,6 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
     {
         SolvePlan &SP = M->splan;
         HIPCHK(M->s_poff.upload(SP.poff.data(), SP.poff.size(), s));
-        HIPCHK(M->s_ftiles.upload(SP.fwd_tiles.data(), SP.fwd_tiles.size(), s));
-        HIPCHK(M->s_btiles.upload(SP.bwd_tiles.data(), SP.bwd_tiles.size(), s));
-        HIPCHK(M->s_ftbase.upload(SP.fwd_tbase.data(), SP.fwd_tbase.size(), s));
-        HIPCHK(M->s_btbase.upload(SP.bwd_tbase.data(), SP.bwd_tbase.size(), s));
-        HIPCHK(M->s_fnks.upload(SP.fwd_nks.data(), SP.fwd_nks.size(), s));
-        HIPCHK(M->s_bnks.upload(SP.bwd_nks.data(), SP.bwd_nks.size(), s));
-        HIPCHK(M->s_fkw.upload(SP.fwd_kw.data(), SP.fwd_kw.size(), s));
-        HIPCHK(M->s_bkw.upload(SP.bwd_kw.data(), SP.bwd_kw.size(), s));
         HIPCHK(M->s_pullptr.upload(SP.pull_ptr.data(), SP.pull_ptr.size(), s));
         {
             // the device pull map holds absolute vector slots (child's vec_off
@@ -160,46 +205,15 @@ int upload_symbolic(sol105_ctx *ctx, sol105_model *M) {
             M->pull2_ok = pairs_ok;
             HIPCHK(hipStreamSynchronize(s));  // slot and pair are host temporaries
         }
-        {
-            // tile descriptors (the factor offset is that of P and of PT alike)
-            auto build = [&](const std::vector<int32_t> &tl, const std::vector<int32_t> &kw,
-                             const std::vector<int32_t> &nks, std::vector<TileDesc> &out) {
-                out.resize(tl.size() / 3);
-                for (size_t t = 0; t < out.size(); t++) {
-                    const int f = tl[3 * t];
-                    TileDesc &d = out[t];
-                    d.f = f; d.rt = tl[3 * t + 1]; d.ks = tl[3 * t + 2];
-                    d.ns = ns[f]; d.m = m[f]; d.piv0 = piv0[f]; d.kw = kw[f]; d.nks = nks[f];
-                    d.parent = cptr[f + 1] > cptr[f] ? 1 : 0;
-                    d.pad = 0;
-                    d.vec_off = vecoff[f]; d.poff = SP.poff[f]; d.row_off = rowoff[f];
-                }
-            };
-            std::vector<TileDesc> fd, bd;
-            build(SP.fwd_tiles, SP.fwd_kw, SP.fwd_nks, fd);
-            build(SP.bwd_tiles, SP.bwd_kw, SP.bwd_nks, bd);
-            HIPCHK(M->s_fdesc.upload(fd.data(), fd.size(), s));
-            HIPCHK(M->s_bdesc.upload(bd.data(), bd.size(), s));
-            HIPCHK(hipStreamSynchronize(s));  // fd and bd are host temporaries
-        }
         HIPCHK(M->s_lblk.upload(SP.lblk.data(), SP.lblk.size(), s));
         HIPCHK(M->d_P.alloc(SP.p_storage + SP.p_pad));
         HIPCHK(M->d_PT.alloc(SP.p_storage + SP.p_pad));
         HIPCHK(hipMemsetAsync(M->d_P.p + SP.p_storage, 0, sizeof(double) * SP.p_pad, s));
         HIPCHK(hipMemsetAsync(M->d_PT.p + SP.p_storage, 0, sizeof(double) * SP.p_pad, s));
-        HIPCHK(M->d_part.alloc(std::max<int64_t>(solve_part_size(SP), 1)));
         DevSolve &DS = M->dsolve;
         DS.poff = M->s_poff.p;
-        DS.fwd_tiles = M->s_ftiles.p;
-        DS.bwd_tiles = M->s_btiles.p;
-        DS.fwd_desc = M->s_fdesc.p;
-        DS.bwd_desc = M->s_bdesc.p;
-        DS.fwd_tbase_rel = M->s_ftbase.p;
-        DS.bwd_tbase_rel = M->s_btbase.p;
-        DS.fwd_nks = M->s_fnks.p;
-        DS.bwd_nks = M->s_bnks.p;
-        DS.fwd_kw = M->s_fkw.p;
-        DS.bwd_kw = M->s_bkw.p;
+        int rc = upload_solve_tiles(ctx, M);
+        if (rc) return rc;
         DS.pull_ptr = M->s_pullptr.p;
         DS.pull_src = M->s_pullsrc.p;
         DS.lblk = M->s_lblk.p;

@@ -685,6 +685,13 @@ int sol105_batch_modes(sol105_ctx *ctx, const sol105_batch *B, int32_t member, d
     return SOL105_OK;
 }
 
+sol105_model *sol105_batch_model(sol105_batch *B) { return B ? &B->M : nullptr; }
+
+int sol105_batch_autotune(sol105_ctx *ctx, sol105_batch *B, int32_t reps, double *report) {
+    if (!ctx || !B) return set_err(ctx, SOL105_E_INVALID, "null argument");
+    return sol105_solve_autotune(ctx, &B->M, reps, report);
+}
+
 int sol105_batch_stats(const sol105_ctx *ctx, const sol105_batch *B, double *out) {
     (void)ctx;
     if (!B || !out) return SOL105_E_INVALID;

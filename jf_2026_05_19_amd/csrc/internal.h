@@ -104,6 +104,10 @@ struct sol105_model {
     s105::DevBuf<int32_t> s_ftiles, s_btiles, s_ftbase, s_btbase, s_fnks, s_bnks, s_fkw, s_bkw, s_pullptr, s_pullsrc, s_pull2, s_lblk;
     bool pull2_ok = false;
     s105::DevBuf<TileDesc> s_fdesc, s_bdesc;
+    // product launch shapes per height (forward, backward; empty: defaults)
+    // and the front ids grouped by height of the current analysis
+    std::vector<std::pair<ProdCfg, ProdCfg>> prod_cfg;
+    std::vector<int32_t> flist_host;
     s105::DevBuf<double> d_P, d_PT, d_part;
     DevSolve dsolve;
     // state
@@ -146,6 +150,7 @@ ElemArgs elem_args(sol105_model *M);
 // AUTOSPC on the diagonal, per member of a batch model
 void autospc_mask(const sol105_model *M, const std::vector<double> &kd, std::vector<uint8_t> &free_out);
 int upload_symbolic(sol105_ctx *ctx, sol105_model *M);
+int upload_solve_tiles(sol105_ctx *ctx, sol105_model *M);
 // x <- K_ff^-1 x (elimination order, nrhs <= MAXR columns)
 hipError_t tri_solve(sol105_model *M, double *X, int nrhs, hipStream_t s);
 double ev_ms(hipEvent_t a, hipEvent_t b);

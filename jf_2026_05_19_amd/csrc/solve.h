@@ -64,3 +64,10 @@ hipError_t solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS
 hipError_t solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S,
                      const double *PTbuf, double *W, double *part, const double *xin, double *xout, int nrhs,
                      hipStream_t s, ProdTimer *pt = nullptr);
+// One tree height of one direction (the product launch and, for split
+// fronts, the finish launch); A is P (forward) or PT (backward).  With the
+// lower heights (forward) or the higher ones (backward) done, repeating a
+// height's launches rewrites the same values.
+hipError_t solve_height(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, int bwd,
+                        int h, const double *A, double *W, double *part, const double *xin, double *xout, int nrhs,
+                        hipStream_t s, ProdTimer *pt = nullptr);

@@ -456,12 +456,14 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
             const int i = rb + kq + 4 * v;
             if (i >= ns && i < nrows && col < nrhs) eu[v] = pairs[slot0 + i];
         }
-    double av[PCH];
-    auto load_a = [&](int s) {
+    // factor values in two register sets: the next chunk's loads are in
+    // flight while the current chunk is multiplied
+    double av[PCH], an[PCH];
+    auto load_a = [&](double (&dst)[PCH], int s) {
 #pragma unroll
-        for (int u = 0; u < PCH; u++) av[u] = Aw[(int64_t)(4 * min(s + u, s_hi - 1)) * lda + aoff];
+        for (int u = 0; u < PCH; u++) dst[u] = Aw[(int64_t)(4 * min(s + u, s_hi - 1)) * lda + aoff];
     };
-    if (live && s_lo < s_hi) load_a(s_lo);
+    if (live && s_lo < s_hi) load_a(av, s_lo);
     // child update rows (128 contiguous bytes each, in the pull map's order)
     double upd[4] = {0.0, 0.0, 0.0, 0.0};
     if (uprows) {
@@ -535,8 +537,7 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
     __syncthreads();
     if (!live) return;  // a whole wavefront past the front; no barrier follows
     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int s = s_lo; s < s_hi; s += PCH) {
-        if (s > s_lo) load_a(s);
+    auto mult = [&](const double (&a)[PCH], int s) {
         double bv[PCH];
 #pragma unroll
         for (int u = 0; u < PCH; u++) {
@@ -544,10 +545,15 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
             bv[u] = (s + u < s_hi) ? Vs[4 * ss + kq][col] : 0.0;  // wavefront-uniform condition
         }
 #pragma unroll
-        for (int u = 0; u < PCH; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
-        __builtin_amdgcn_sched_group_barrier(0x020, PCH, 0);
-        __builtin_amdgcn_sched_group_barrier(0x100, PCH, 0);
-        __builtin_amdgcn_sched_group_barrier(0x008, PCH, 0);
+        for (int u = 0; u < PCH; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bv[u], acc, 0, 0, 0);
+    };
+    // chunks alternate between the two register sets, each chunk's loads
+    // issued one chunk ahead
+    for (int s = s_lo; s < s_hi; s += 2 * PCH) {
+        if (s + PCH < s_hi) load_a(an, s + PCH);
+        mult(av, s);
+        if (s + 2 * PCH < s_hi) load_a(av, s + 2 * PCH);
+        if (s + PCH < s_hi) mult(an, s + PCH);
     }
     // lane holds D[kq + 4 v][col]: row rb + kq + 4 v, right-hand side col
     if (!whole) {
@@ -715,30 +721,52 @@ static double prod_bytes(const SolveHeight &SH, int nrhs) {
     return 8.0 * ((double)SH.sum_nz + (double)nrhs * (double)(SH.sum_m + SH.sum_ns));
 }
 
-hipError_t solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S,
-                     const double *Pbuf, double *W, double *part, const double *xin, double *xout, int nrhs,
-                     hipStream_t s, ProdTimer *pt) {
-    if (xin == xout || nrhs < 1 || nrhs > MAXR) return hipErrorInvalidValue;
-    const int ldv = nrhs > 1 ? MAXR : 1;
-    for (int h = 0; h <= P.height; h++) {
-        const HeightInfo &H = P.heights[h];
-        const SolveHeight &SH = S.heights[h];
-        if (H.n == 0) continue;
-        const int32_t *fl = D.flist + H.list_off;
-        const bool timed = pt && nrhs > 1;
-        if (timed) pt->begin(s, prod_bytes(SH, nrhs), h == 0 ? 0 : 1);
+hipError_t solve_height(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S, int bwd,
+                        int h, const double *A, double *W, double *part, const double *xin, double *xout, int nrhs,
+                        hipStream_t s, ProdTimer *pt) {
+    if (xin == xout || nrhs < 1 || nrhs > MAXR || h < 0 || h > P.height) return hipErrorInvalidValue;
+    const HeightInfo &H = P.heights[h];
+    const SolveHeight &SH = S.heights[h];
+    if (H.n == 0) return hipSuccess;
+    const int32_t *fl = D.flist + H.list_off;
+    const bool timed = pt && nrhs > 1;
+    if (timed) pt->begin(s, prod_bytes(SH, nrhs), h == 0 ? 0 : 1);
+    if (!bwd) {
         const int32_t *tl = DS.fwd_tiles + 3 * SH.fwd_off;
         if (nrhs == 1)
-            prod1_kernel<<<SH.n_fwd, 256, 0, s>>>(D, DS, tl, Pbuf, W, part, 0, DS.fwd_kw, DS.fwd_nks, xin, xout,
+            prod1_kernel<<<SH.n_fwd, 256, 0, s>>>(D, DS, tl, A, W, part, 0, DS.fwd_kw, DS.fwd_nks, xin, xout,
                                                   P.n_free);
         else
-            launch_prod(SH.n_fwd, SH.fwd_kmax, SH.fwd_cfg.pch, h > 0, s, D, DS, DS.fwd_desc + SH.fwd_off, Pbuf, W,
-                        part, nrhs, 0, xin, xout, P.n_free);
+            launch_prod(SH.n_fwd, SH.fwd_kmax, SH.fwd_cfg.pch, h > 0, s, D, DS, DS.fwd_desc + SH.fwd_off, A, W, part,
+                        nrhs, 0, xin, xout, P.n_free);
         if (timed) pt->end(s);
         // single-slice fronts were finished by the product itself
         if (SH.fwd_split)
             fwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_m * nrhs, 256), H.n), 256, 0, s>>>(
-                D, DS, fl, DS.fwd_tbase_rel, DS.fwd_nks, part, W, xout, P.n_free, nrhs, ldv);
+                D, DS, fl, DS.fwd_tbase_rel, DS.fwd_nks, part, W, xout, P.n_free, nrhs, nrhs > 1 ? MAXR : 1);
+    } else {
+        const int32_t *tl = DS.bwd_tiles + 3 * SH.bwd_off;
+        if (nrhs == 1)
+            prod1_kernel<<<SH.n_bwd, 256, 0, s>>>(D, DS, tl, A, W, part, 1, DS.bwd_kw, DS.bwd_nks, xin, xout,
+                                                  P.n_free);
+        else
+            launch_prod(SH.n_bwd, SH.bwd_kmax, SH.bwd_cfg.pch, false, s, D, DS, DS.bwd_desc + SH.bwd_off, A, W,
+                        part, nrhs, 1, xin, xout, P.n_free);
+        if (timed) pt->end(s);
+        if (SH.bwd_split)
+            bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(
+                D, fl, DS.bwd_tbase_rel, DS.bwd_nks, part, xout, P.n_free, nrhs);
+    }
+    return hipSuccess;
+}
+
+hipError_t solve_fwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS, const SolvePlan &S,
+                     const double *Pbuf, double *W, double *part, const double *xin, double *xout, int nrhs,
+                     hipStream_t s, ProdTimer *pt) {
+    if (xin == xout || nrhs < 1 || nrhs > MAXR) return hipErrorInvalidValue;
+    for (int h = 0; h <= P.height; h++) {
+        const hipError_t e = solve_height(P, D, DS, S, 0, h, Pbuf, W, part, xin, xout, nrhs, s, pt);
+        if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
@@ -748,23 +776,8 @@ hipError_t solve_bwd(const FactorPlan &P, const DevFronts &D, const DevSolve &DS
                      hipStream_t s, ProdTimer *pt) {
     if (xin == xout || nrhs < 1 || nrhs > MAXR) return hipErrorInvalidValue;
     for (int h = P.height; h >= 0; h--) {
-        const HeightInfo &H = P.heights[h];
-        const SolveHeight &SH = S.heights[h];
-        if (H.n == 0) continue;
-        const int32_t *fl = D.flist + H.list_off;
-        const bool timed = pt && nrhs > 1;
-        if (timed) pt->begin(s, prod_bytes(SH, nrhs), h == 0 ? 0 : 1);
-        const int32_t *tl = DS.bwd_tiles + 3 * SH.bwd_off;
-        if (nrhs == 1)
-            prod1_kernel<<<SH.n_bwd, 256, 0, s>>>(D, DS, tl, PTbuf, W, part, 1, DS.bwd_kw, DS.bwd_nks, xin, xout,
-                                                  P.n_free);
-        else
-            launch_prod(SH.n_bwd, SH.bwd_kmax, SH.bwd_cfg.pch, false, s, D, DS, DS.bwd_desc + SH.bwd_off, PTbuf, W,
-                        part, nrhs, 1, xin, xout, P.n_free);
-        if (timed) pt->end(s);
-        if (SH.bwd_split)
-            bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(
-                D, fl, DS.bwd_tbase_rel, DS.bwd_nks, part, xout, P.n_free, nrhs);
+        const hipError_t e = solve_height(P, D, DS, S, 1, h, PTbuf, W, part, xin, xout, nrhs, s, pt);
+        if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
