@@ -349,13 +349,15 @@ int sol105_server_stats(sol105_server *server, double *out /* 10 */);
 
 /* Launch shapes of the triangular-solve products per tree height (no
  * reference counterpart: the reference's solves are CHOLMOD's).  cfg holds,
- * per height, forward then backward (kmax, ks, pch): fronts whose inner
- * dimension is at most kmax form one k-slice, wider ones are cut in ks-wide
- * slices (kmax and ks at most 384), pch four-k steps of factor loads per
- * chunk (8, 16, 32; 0 = chosen by the launch size).  _get needs a symbolic
- * analysis (a first solve); cfg may be NULL to query n_heights.  _set with
- * values <= 0 for kmax or ks keeps the default; the shapes persist over
- * re-analyses of the model. */
+ * per height, forward then backward (kmax, ks, pch, kg): fronts whose
+ * inner dimension is at most kmax form one k-slice, wider ones are cut in
+ * ks-wide slices (kmax and ks at most 384), pch four-k steps of factor loads
+ * per chunk (8, 16, 32; 0 = chosen by the launch size), kg groups of four
+ * wavefronts per workgroup that split a tile's k-steps and sum their parts
+ * in a fixed order (1, 2, 4; 0 = 1; pch 32 only with kg 1).  _get needs a
+ * symbolic analysis (a first solve); cfg may be NULL to query n_heights.
+ * _set with values <= 0 for kmax or ks keeps the default; the shapes persist
+ * over re-analyses of the model. */
 int sol105_solve_cfg_get(sol105_ctx *ctx, const sol105_model *model, int32_t *cfg, int32_t *n_heights);
 /* Shape of each height (heights x 4: fronts, largest m, largest ns, nonzero
  * factor entries), the key of a persisted table of tuned shapes. */
@@ -363,8 +365,10 @@ int sol105_solve_heights(sol105_ctx *ctx, const sol105_model *model, int64_t *sh
 int sol105_solve_cfg_set(sol105_ctx *ctx, sol105_model *model, const int32_t *cfg, int32_t n_heights);
 /* Autotuner: on the model's current factor, times every height of both
  * directions for candidate shapes (median of reps, 3..50) and keeps the
- * fastest (a candidate must win by 3 %).  report (may be NULL): block solve
- * ms before and after, launches timed, heights. */
+ * fastest (a candidate must win by 3 %, and again in an alternating
+ * re-timing against the starting shape); if the whole block solve is not
+ * faster with the new shapes, the starting shapes are restored.  report (may
+ * be NULL): block solve ms before and after, launches timed, heights. */
 int sol105_solve_autotune(sol105_ctx *ctx, sol105_model *model, int32_t reps, double *report);
 /* The same for the stacked model of a batch (after a batch solve). */
 int sol105_batch_autotune(sol105_ctx *ctx, sol105_batch *batch, int32_t reps, double *report);

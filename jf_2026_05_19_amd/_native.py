@@ -372,11 +372,11 @@ class DeviceModel:
 
     def solve_cfg(self):
         """Launch shapes of the triangular-solve products per tree height:
-        array (heights, 2, 3) of (kmax, ks, pch), forward then backward."""
+        array (heights, 2, 4) of (kmax, ks, pch, kg), forward then backward."""
         L, c = self.ctx.L, self.ctx
         n = ctypes.c_int32(0)
         c.check(L.sol105_solve_cfg_get(c.h, self.h, None, ctypes.byref(n)))
-        cfg = np.zeros((n.value, 2, 3), np.int32)
+        cfg = np.zeros((n.value, 2, 4), np.int32)
         c.check(L.sol105_solve_cfg_get(c.h, self.h, cfg.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                        ctypes.byref(n)))
         return cfg
@@ -393,7 +393,7 @@ class DeviceModel:
         return sh
 
     def set_solve_cfg(self, cfg):
-        cfg = np.ascontiguousarray(cfg, dtype=np.int32).reshape(-1, 2, 3)
+        cfg = np.ascontiguousarray(cfg, dtype=np.int32).reshape(-1, 2, 4)
         self.ctx.check(self.ctx.L.sol105_solve_cfg_set(self.ctx.h, self.h,
                                                        cfg.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                                        len(cfg)))

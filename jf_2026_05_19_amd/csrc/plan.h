@@ -29,10 +29,11 @@ struct FactorPlan {
 
 // Shape of one height's product launch: fronts whose inner dimension is at
 // most kmax form one k-slice, wider ones are cut in ks-wide slices (reduced
-// in slice order by the last slice to finish); pch four-k steps per chunk of
-// factor loads (0: chosen from the launch's size).
+// in slice order by a finish launch); pch four-k steps per chunk of factor
+// loads (0: chosen from the launch's size); kg k-groups of four wavefronts
+// per workgroup (1, 2 or 4).
 struct ProdCfg {
-    int kmax = SOLVE_KMAX, ks = SOLVE_KS, pch = 0;
+    int kmax = SOLVE_KMAX, ks = SOLVE_KS, pch = 0, kg = 1;
 };
 
 struct SolveHeight {

@@ -399,8 +399,14 @@ __global__ void __launch_bounds__(256) prod1_kernel(DevFronts D, DevSolve DS, co
 // each such release writes back the XCD's L2.)
 // PCH: four-k steps per chunk of factor loads (8: more wavefronts per SIMD
 // for short slices on large grids; 16 or 32: fewer dependent rounds)
-template <int PCH, bool FPULL>
-__global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS, const TileDesc *desc,
+// KG: k-groups of four wavefronts per workgroup (256 KG threads).  The
+// wavefronts of k-group g take the g-th contiguous part of the slice's
+// four-k steps for their 16 rows; k-groups 1..KG-1 leave their partial sums
+// in LDS and k-group 0 adds them in group order (fixed order, bit-
+// reproducible for a given KG) before the write-back.  Long slices on
+// launches with few tiles (near the root) finish in 1 / KG of the time.
+template <int PCH, bool FPULL, int KG>
+__global__ void __launch_bounds__(256 * KG) prod_mfma_kernel(DevFronts D, DevSolve DS, const TileDesc *desc,
                                                         const double *A, double *W, double *part, int nrhs, int bwd,
                                                         const double *xin, double *xout, int n_free) {
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -413,7 +419,8 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
     const int kw = T.kw;
     const int k0 = ks * kw, kl = min(inner, k0 + kw) - k0;
     const int nst = (kl + 3) >> 2;  // four-k steps of the slice
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wall = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wv = wall & 3, kg = wall >> 2;     // row block, k-group
     const int t = threadIdx.x, lane = t & 63, kq = lane >> 4, col = lane & 15;
     const int r0 = rt * RT, rb = r0 + 16 * wv;  // this wavefront's first row
     const bool live = rb < nrows;                // wavefront-uniform
@@ -426,6 +433,13 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
     int s_lo = 0, s_hi = nst;
     if (!bwd && rb + 16 <= ns) s_hi = rb + 15 < k0 ? 0 : min(nst, ((rb + 15 - k0) >> 2) + 1);
     if (bwd && rb > k0) s_lo = min(nst, (rb - k0) >> 2);
+    if (KG > 1) {
+        // this k-group's contiguous part of the steps
+        const int per = (max(s_hi - s_lo, 0) + KG - 1) / KG;
+        const int g_lo = min(s_hi, s_lo + kg * per);
+        s_hi = min(s_hi, g_lo + per);
+        s_lo = g_lo;
+    }
     // Loads go out in dependency order so that every wait covers only what it
     // needs (vector loads complete in order): first the index loads (pull
     // pairs of the forward slice rows and of this wavefront's update rows,
@@ -442,13 +456,13 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
     int arow[2] = {0, 0};
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-        const int k = t + 256 * j, kk = k0 + min(k, kl - 1);
+        const int k = t + 256 * KG * j, kk = k0 + min(k, kl - 1);
         if (fpull && k < kl) ep[j] = pairs[slot0 + k0 + k];
         if (bwd && k < kpad && kk >= ns) arow[j] = rows[kk];
     }
     // forward, single k-slice: the child rows pulled into this wavefront's
     // update rows (written back below)
-    const bool uprows = FPULL && whole && parent && live && rb + 16 > ns;
+    const bool uprows = FPULL && whole && parent && live && kg == 0 && rb + 16 > ns;
     int2 eu[4] = {{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};
     if (uprows && pairs)
 #pragma unroll
@@ -492,7 +506,7 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
     // columns past nrhs are zero
 #pragma unroll 1
     for (int j = 0; j < 2; j++) {
-        const int k = t + 256 * j;
+        const int k = t + 256 * KG * j;
         if (k >= kpad) break;
         const int kk = k0 + min(k, kl - 1);
         const bool anc = bwd && kk >= ns;
@@ -535,7 +549,7 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
         }
     }
     __syncthreads();
-    if (!live) return;  // a whole wavefront past the front; no barrier follows
+    if (KG == 1 && !live) return;  // a whole wavefront past the front; no barrier follows
     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
     auto mult = [&](const double (&a)[PCH], int s) {
         double bv[PCH];
@@ -549,11 +563,27 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
     };
     // chunks alternate between the two register sets, each chunk's loads
     // issued one chunk ahead
-    for (int s = s_lo; s < s_hi; s += 2 * PCH) {
-        if (s + PCH < s_hi) load_a(an, s + PCH);
-        mult(av, s);
-        if (s + 2 * PCH < s_hi) load_a(av, s + 2 * PCH);
-        if (s + PCH < s_hi) mult(an, s + PCH);
+    if (live)
+        for (int s = s_lo; s < s_hi; s += 2 * PCH) {
+            if (s + PCH < s_hi) load_a(an, s + PCH);
+            mult(av, s);
+            if (s + 2 * PCH < s_hi) load_a(av, s + 2 * PCH);
+            if (s + PCH < s_hi) mult(an, s + PCH);
+        }
+    if (KG > 1) {
+        // partial sums of k-groups 1..KG-1 (in the slice's LDS, which every
+        // wavefront has finished reading), added by k-group 0 in group order
+        double (*red)[64][MAXR] = reinterpret_cast<double (*)[64][MAXR]>(lds);
+        __syncthreads();
+        if (kg > 0 && live)
+#pragma unroll
+            for (int v = 0; v < 4; v++) red[kg - 1][16 * wv + kq + 4 * v][col] = acc[v];
+        __syncthreads();
+        if (kg > 0 || !live) return;  // no barrier follows
+#pragma unroll
+        for (int g = 0; g < KG - 1; g++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[v] += red[g][16 * wv + kq + 4 * v][col];
     }
     // lane holds D[kq + 4 v][col]: row rb + kq + 4 v, right-hand side col
     if (!whole) {
@@ -570,7 +600,9 @@ __global__ void __launch_bounds__(256) prod_mfma_kernel(DevFronts D, DevSolve DS
         }
     if (rb < (bwd ? nrows : min(nrows, ns))) {
         // pivot rows to x_out through the wavefront's transpose tile
-        double (*Ts)[MAXR + 1] = reinterpret_cast<double (*)[MAXR + 1]>(lds + (size_t)4 * nst * (MAXR + 1)) + 16 * wv;
+        // past the slice and the k-groups' partial sums (both may still be read)
+        const size_t ts_off = max((size_t)4 * nst * (MAXR + 1), (size_t)(KG - 1) * 64 * MAXR);
+        double (*Ts)[MAXR + 1] = reinterpret_cast<double (*)[MAXR + 1]>(lds + ts_off) + 16 * wv;
 #pragma unroll
         for (int v = 0; v < 4; v++) Ts[kq + 4 * v][col] = acc[v];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -685,34 +717,50 @@ ProdTimer::~ProdTimer() {
 
 // LDS bytes of a block-RHS product launch whose widest k-slice is kmax: the
 // slice rows (rounded up to four-k steps), then four 16 x 17 transpose tiles
-static size_t prod_lds(int kmax) {
-    return sizeof(double) * ((size_t)4 * ((kmax + 3) / 4) * (MAXR + 1) + (size_t)4 * 16 * (MAXR + 1));
+static size_t prod_lds(int kmax, int kg) {
+    const size_t slice = (size_t)4 * ((kmax + 3) / 4) * (MAXR + 1), red = (size_t)(kg - 1) * 64 * MAXR;
+    return sizeof(double) * (std::max(slice, red) + (size_t)4 * 16 * (MAXR + 1));
 }
 
 // one block-RHS product launch over n_tiles tiles whose widest k-slice is
 // kmax, pch four-k steps per chunk of factor loads (0: by the launch's size)
-template <bool FPULL>
-static void launch_prod_t(int n_tiles, int kmax, int pch, hipStream_t s, const DevFronts &D, const DevSolve &DS,
+template <bool FPULL, int KG>
+static void launch_prod_k(int n_tiles, int kmax, int pch, hipStream_t s, const DevFronts &D, const DevSolve &DS,
                           const TileDesc *td, const double *A, double *W, double *part, int nrhs, int bwd,
                           const double *xin, double *xout, int n_free) {
-    const size_t lds = prod_lds(kmax);
+    const size_t lds = prod_lds(kmax, KG);
     if (pch == 8)
-        prod_mfma_kernel<8, FPULL><<<n_tiles, 256, lds, s>>>(D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
-    else if (pch == 32)
-        prod_mfma_kernel<32, FPULL><<<n_tiles, 256, lds, s>>>(D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
+        prod_mfma_kernel<8, FPULL, KG><<<n_tiles, 256 * KG, lds, s>>>(D, DS, td, A, W, part, nrhs, bwd, xin, xout,
+                                                                      n_free);
+    else if (pch == 32 && KG == 1)
+        prod_mfma_kernel<32, FPULL, 1><<<n_tiles, 256, lds, s>>>(D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
     else
-        prod_mfma_kernel<16, FPULL><<<n_tiles, 256, lds, s>>>(D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
+        prod_mfma_kernel<16, FPULL, KG><<<n_tiles, 256 * KG, lds, s>>>(D, DS, td, A, W, part, nrhs, bwd, xin, xout,
+                                                                       n_free);
 }
 
-// pull: a forward launch above the leaves (its fronts add child update rows)
-static void launch_prod(int n_tiles, int kmax, int pch, bool pull, hipStream_t s, const DevFronts &D,
+template <bool FPULL>
+static void launch_prod_t(int n_tiles, int kmax, int pch, int kg, hipStream_t s, const DevFronts &D,
+                          const DevSolve &DS, const TileDesc *td, const double *A, double *W, double *part, int nrhs,
+                          int bwd, const double *xin, double *xout, int n_free) {
+    if (kg == 4)
+        launch_prod_k<FPULL, 4>(n_tiles, kmax, pch, s, D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
+    else if (kg == 2)
+        launch_prod_k<FPULL, 2>(n_tiles, kmax, pch, s, D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
+    else
+        launch_prod_k<FPULL, 1>(n_tiles, kmax, pch, s, D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
+}
+
+// pull: a forward launch above the leaves (its fronts add child update rows);
+// kg: k-groups per workgroup (1, 2, 4)
+static void launch_prod(int n_tiles, int kmax, int pch, int kg, bool pull, hipStream_t s, const DevFronts &D,
                         const DevSolve &DS, const TileDesc *td, const double *A, double *W, double *part, int nrhs,
                         int bwd, const double *xin, double *xout, int n_free) {
     if (pch == 0) pch = (kmax <= 192 && n_tiles >= 380) ? 8 : 16;
     if (pull && !bwd)
-        launch_prod_t<true>(n_tiles, kmax, pch, s, D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
+        launch_prod_t<true>(n_tiles, kmax, pch, kg, s, D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
     else
-        launch_prod_t<false>(n_tiles, kmax, pch, s, D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
+        launch_prod_t<false>(n_tiles, kmax, pch, kg, s, D, DS, td, A, W, part, nrhs, bwd, xin, xout, n_free);
 }
 
 // algorithmic bytes of one product launch: the factor's nonzero entries
@@ -737,8 +785,8 @@ hipError_t solve_height(const FactorPlan &P, const DevFronts &D, const DevSolve 
             prod1_kernel<<<SH.n_fwd, 256, 0, s>>>(D, DS, tl, A, W, part, 0, DS.fwd_kw, DS.fwd_nks, xin, xout,
                                                   P.n_free);
         else
-            launch_prod(SH.n_fwd, SH.fwd_kmax, SH.fwd_cfg.pch, h > 0, s, D, DS, DS.fwd_desc + SH.fwd_off, A, W, part,
-                        nrhs, 0, xin, xout, P.n_free);
+            launch_prod(SH.n_fwd, SH.fwd_kmax, SH.fwd_cfg.pch, SH.fwd_cfg.kg, h > 0, s, D, DS,
+                        DS.fwd_desc + SH.fwd_off, A, W, part, nrhs, 0, xin, xout, P.n_free);
         if (timed) pt->end(s);
         // single-slice fronts were finished by the product itself
         if (SH.fwd_split)
@@ -750,8 +798,8 @@ hipError_t solve_height(const FactorPlan &P, const DevFronts &D, const DevSolve 
             prod1_kernel<<<SH.n_bwd, 256, 0, s>>>(D, DS, tl, A, W, part, 1, DS.bwd_kw, DS.bwd_nks, xin, xout,
                                                   P.n_free);
         else
-            launch_prod(SH.n_bwd, SH.bwd_kmax, SH.bwd_cfg.pch, false, s, D, DS, DS.bwd_desc + SH.bwd_off, A, W,
-                        part, nrhs, 1, xin, xout, P.n_free);
+            launch_prod(SH.n_bwd, SH.bwd_kmax, SH.bwd_cfg.pch, SH.bwd_cfg.kg, false, s, D, DS,
+                        DS.bwd_desc + SH.bwd_off, A, W, part, nrhs, 1, xin, xout, P.n_free);
         if (timed) pt->end(s);
         if (SH.bwd_split)
             bwd_finish_kernel<<<dim3(cdiv((int64_t)H.max_ns * nrhs, 256), H.n), 256, 0, s>>>(

@@ -47,6 +47,10 @@ void solve_plan_build(const FactorPlan &P, const std::vector<int> &ns, const std
         for (ProdCfg *c : {&SH.fwd_cfg, &SH.bwd_cfg}) {
             c->kmax = std::max(1, std::min(c->kmax, SOLVE_KMAX));
             c->ks = std::max(4, std::min(c->ks, SOLVE_KMAX));
+            // one, two or four k-groups; the 32-step chunk has the registers
+            // of one group only
+            c->kg = c->kg >= 4 ? 4 : c->kg >= 2 ? 2 : 1;
+            if (c->pch == 32) c->kg = 1;
         }
         const ProdCfg &CF = SH.fwd_cfg, &CB = SH.bwd_cfg;
         SH.fwd_off = (int64_t)S.fwd_tiles.size() / 3;

@@ -1,7 +1,7 @@
 """Persisted table of tuned solve-product launch shapes.
 
 The device autotuner (``sol105_solve_autotune``) picks, per tree height and
-direction, the product launch shape (kmax, ks, pch) that runs fastest on a
+direction, the product launch shape (kmax, ks, pch, kg) that runs fastest on a
 model's own factor.  The best shape depends only on the height's shape
 class: how many fronts it has and how large they are.  This module keys the
 tuned shapes by that class and keeps them in a JSON file, so a new model
@@ -23,7 +23,7 @@ import numpy as np
 
 __all__ = ["shape_keys", "load", "save", "apply", "tune"]
 
-DEFAULT = (384, 128, 0)
+DEFAULT = (384, 128, 0, 1)
 
 
 def _bucket_pow2(n):
@@ -46,13 +46,14 @@ def load(path):
         return {}
     with open(path) as fh:
         data = json.load(fh)
-    return {k: tuple(int(x) for x in v) for k, v in data.get("shapes", {}).items()}
+    # tables written before the k-group count existed hold three values
+    return {k: tuple(int(x) for x in v) + (1,) * (4 - len(v)) for k, v in data.get("shapes", {}).items()}
 
 
 def save(path, table):
     tmp = path + ".tmp%d" % os.getpid()
     with open(tmp, "w") as fh:
-        json.dump({"version": 1, "shapes": {k: list(v) for k, v in sorted(table.items())}}, fh, indent=1)
+        json.dump({"version": 2, "shapes": {k: list(v) for k, v in sorted(table.items())}}, fh, indent=1)
     os.replace(tmp, path)
 
 
@@ -60,7 +61,7 @@ def apply(dm, table):
     """Set the tabled shapes on a device model (or batch's model) that has
     been solved once; returns the number of heights the table covered."""
     keys = shape_keys(dm.solve_heights())
-    cfg = np.zeros((len(keys), 2, 3), np.int32)
+    cfg = np.zeros((len(keys), 2, 4), np.int32)
     hit = 0
     for h, (kf, kb) in enumerate(keys):
         for d, k in enumerate((kf, kb)):

@@ -5,7 +5,8 @@ persisted table (jf_2026_05_19_amd/tuning.py).
 Every shape computes the same solve up to the summation order of k-slices:
 eigenvalues of a stiffened panel within 1e-12 relative of the default
 shapes' (and of the oracle within the parity bound 1e-8) for forced splits
-(kmax = ks = 128), wide slices (384) and each chunk length; a tuned model and
+(kmax = ks = 128), wide slices (384), each chunk length and each count of
+k-groups per workgroup (their parts summed in a fixed order); a tuned model and
 a tuned batch give the same eigenvalues as before tuning; a table written
 from one model's tuning sets the same shapes on a fresh model of the mesh.
 """
@@ -43,16 +44,18 @@ def test_shapes_give_the_same_solve(ctx):
         lam0, _ = dm.solve(want_modes=False)
         assert np.max(np.abs(lam0 - ref) / np.abs(ref)) <= 1e-8
         nh = len(dm.solve_heights())
-        for kmax, ks, pch in [(128, 128, 8), (384, 384, 16), (256, 128, 32), (384, 256, 8), (0, 0, 0)]:
-            cfg = np.tile(np.array([kmax, ks, pch], np.int32), (nh, 2, 1))
+        for shape in [(128, 128, 8, 1), (384, 384, 16, 1), (256, 128, 32, 1), (384, 256, 8, 1),
+                      (384, 128, 8, 2), (384, 384, 16, 4), (128, 128, 0, 4), (256, 256, 16, 2), (0, 0, 0, 0)]:
+            cfg = np.tile(np.array(shape, np.int32), (nh, 2, 1))
             dm.set_solve_cfg(cfg)
             got = dm.solve_cfg()
-            if kmax:
-                assert (got[:, :, 0] == kmax).all() and (got[:, :, 1] == ks).all() and (got[:, :, 2] == pch).all()
+            if shape[0]:
+                assert (got == cfg).all()
             lam, _ = dm.solve(want_modes=False)
-            assert np.max(np.abs(lam - lam0) / np.abs(lam0)) <= 1e-12, (kmax, ks, pch)
-        with pytest.raises(N.Sol105Error):
-            dm.set_solve_cfg(np.tile(np.array([384, 128, 12], np.int32), (nh, 2, 1)))   # pch not 8/16/32
+            assert np.max(np.abs(lam - lam0) / np.abs(lam0)) <= 1e-12, shape
+        for bad in [(384, 128, 12, 1), (384, 128, 8, 3), (384, 128, 32, 2)]:   # pch, kg, pch 32 with kg > 1
+            with pytest.raises(N.Sol105Error):
+                dm.set_solve_cfg(np.tile(np.array(bad, np.int32), (nh, 2, 1)))
     finally:
         dm.free()
 

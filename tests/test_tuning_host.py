@@ -11,7 +11,7 @@ from jf_2026_05_19_amd import tuning
 class FakeModel:
     def __init__(self, heights):
         self.h = np.asarray(heights, np.int64)
-        self.cfg = np.zeros((len(heights), 2, 3), np.int32)
+        self.cfg = np.zeros((len(heights), 2, 4), np.int32)
 
     def solve_heights(self):
         return self.h
@@ -20,7 +20,7 @@ class FakeModel:
         return self.cfg
 
     def set_solve_cfg(self, cfg):
-        self.cfg = np.asarray(cfg, np.int32).reshape(-1, 2, 3)
+        self.cfg = np.asarray(cfg, np.int32).reshape(-1, 2, 4)
 
 
 def test_keys_bucket_counts_and_sizes():
@@ -32,7 +32,7 @@ def test_keys_bucket_counts_and_sizes():
 
 def test_table_roundtrip_and_apply(tmp_path):
     dm = FakeModel([[566, 336, 180, 0], [256, 264, 36, 0]])
-    dm.cfg[:] = [[[384, 128, 8], [256, 256, 16]], [[128, 128, 32], [384, 384, 8]]]
+    dm.cfg[:] = [[[384, 128, 8, 2], [256, 256, 16, 1]], [[128, 128, 32, 1], [384, 384, 8, 4]]]
     # tune() merges whatever the device autotuner left on the model
     dm.autotune = lambda reps: (1.0, 0.9, 10, 2)
     table, rep = tuning.tune(dm, {})
@@ -45,3 +45,9 @@ def test_table_roundtrip_and_apply(tmp_path):
     assert np.array_equal(fresh.cfg[:2], dm.cfg)
     assert np.array_equal(fresh.cfg[2], [list(tuning.DEFAULT)] * 2)
     assert tuning.load(str(tmp_path / "missing.json")) == {}
+
+
+def test_three_value_table_loads_with_one_k_group(tmp_path):
+    path = tmp_path / "old.json"
+    path.write_text('{"version": 1, "shapes": {"fwd:8:1:2": [256, 128, 16]}}')
+    assert tuning.load(str(path)) == {"fwd:8:1:2": (256, 128, 16, 1)}
