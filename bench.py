@@ -131,7 +131,10 @@ def main():
     if tune:
         log("autotune: block solve %.3f -> %.3f ms (%d launches timed, %.2fs)" % (tune[0], tune[1], tune[2],
                                                                              time.perf_counter() - t1))
-        dm.solve(want_modes=False)
+    # untimed: the first solve under the tuned shapes launches directly, the
+    # second captures the Lanczos steps as HIP graphs; the timed ones replay
+    dm.solve(want_modes=False)
+    dm.solve(want_modes=False)
     lat = []
     phase = np.zeros(8)
     t_start = time.perf_counter()
@@ -141,6 +144,15 @@ def main():
         lat.append(time.perf_counter() - t1)
         phase += ctx.timings()
     t_total = time.perf_counter() - t_start
+    # the same solves with every kernel launched directly (no step graphs)
+    ctx.set_option(N.Context.OPT_STEP_GRAPHS, 0)
+    t1 = time.perf_counter()
+    for k in range(args.steps):
+        dm.solve(want_modes=False)
+    t_direct = (time.perf_counter() - t1) / max(args.steps, 1)
+    ctx.set_option(N.Context.OPT_STEP_GRAPHS, 1)
+    log("steps: %.2f ms/solve with step graphs, %.2f ms with direct launches" % (1e3 * t_total / args.steps,
+                                                                                 1e3 * t_direct))
     # the roofline pass: the same solves again with HIP events around every
     # launch of the dominant kernel (outside the timed region: the events
     # themselves cost time between launches)
@@ -164,6 +176,7 @@ def main():
                          "tree_height": stats[4], "lanczos_vectors": stats[5], "lanczos_block_steps": stats[6]},
         "parse_s": t_parse, "parse_s_host_builder": t_parse_host, "upload_s": t_upload,
         "ms_per_step_default_shapes": 1e3 * t_default,
+        "ms_per_step_direct_launches": 1e3 * t_direct,
         "autotune": None if not tune else {"block_solve_ms_before": tune[0], "block_solve_ms_after": tune[1],
                                            "launches_timed": tune[2], "heights": tune[3]},
     })
@@ -370,13 +383,22 @@ def batched_sweep(ctx, n, steps, plain=True, cpu_sample=32):
     # the solve-product autotuner on the stacked model, then the same steps
     tune = B.autotune(5)
     log("batch autotune: block solve %.3f -> %.3f ms (%d launches timed)" % tune[:3])
-    lam_b, st = B.solve(16)
+    lam_b, st = B.solve(16)   # direct launches under the tuned shapes
+    lam_b, st = B.solve(16)   # captures the step graphs
     bt = []
     for k in range(steps):
         ts = time.perf_counter()
         lam_b, st = B.solve(16)
         bt.append(time.perf_counter() - ts)
     t_batch = float(np.mean(bt))
+    ctx.set_option(N.Context.OPT_STEP_GRAPHS, 0)
+    bt = []
+    for k in range(steps):
+        ts = time.perf_counter()
+        B.solve(16)
+        bt.append(time.perf_counter() - ts)
+    t_batch_direct = float(np.mean(bt))
+    ctx.set_option(N.Context.OPT_STEP_GRAPHS, 1)
     stats = B.stats()
     B.free()
     err = 0.0 if plain else float("nan")
@@ -393,6 +415,7 @@ def batched_sweep(ctx, n, steps, plain=True, cpu_sample=32):
                         "t U(1,4) mm, E in {70e3, 210e3} MPa, FP64" % (n, min(ndofs), max(ndofs), sum(ndofs)),
             "batched_solves_per_s": n / t_batch, "plain_solves_per_s": n / t_plain,
             "batched_solves_per_s_default_shapes": n / t_batch_default,
+            "batched_solves_per_s_direct_launches": n / t_batch_direct,
             "autotune": {"block_solve_ms_before": tune[0], "block_solve_ms_after": tune[1],
                          "launches_timed": tune[2], "heights": tune[3]},
             "speedup_vs_plain": t_plain / t_batch, "batch_ms": 1e3 * t_batch, "batch_steps": steps,

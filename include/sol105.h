@@ -114,10 +114,20 @@ const char *sol105_last_error(const sol105_ctx *ctx);
  *                                    then depend on that history within the
  *                                    convergence tolerance, no longer bitwise.
  *                                    No counterpart in the reference.
+ *   SOL105_OPT_STEP_GRAPHS           0 or 1 (default 1): replay the block
+ *                                    Lanczos expansion steps (both triangular
+ *                                    solves, the KG product, CGS2 and
+ *                                    Cholesky-QR) as HIP graphs, captured per
+ *                                    block index from the second solve of a
+ *                                    model (or batch) with unchanged buffers
+ *                                    and launch shapes.  Results are bitwise
+ *                                    those of direct launches.  No
+ *                                    counterpart in the reference.
  * SOL105_E_INVALID for an unknown option or a value out of range. */
 #define SOL105_OPT_LANCZOS_MAX_VECTORS 1
 #define SOL105_OPT_LANCZOS_MAX_RESTARTS 2
 #define SOL105_OPT_WARM_START 3
+#define SOL105_OPT_STEP_GRAPHS 4
 int sol105_ctx_set_option(sol105_ctx *ctx, int32_t option, double value);
 
 /* Upload a model, build the BSR pattern, the deterministic element->block

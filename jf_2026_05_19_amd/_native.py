@@ -225,10 +225,12 @@ class Context:
 
     OPT_LANCZOS_MAX_VECTORS = 1
     OPT_LANCZOS_MAX_RESTARTS = 2
+    OPT_WARM_START = 3
+    OPT_STEP_GRAPHS = 4
 
     def set_option(self, option, value):
-        """Eigensolver limits (SOL105_OPT_*): basis size before a thick
-        restart, thick restarts before SOL105_E_NOCONV."""
+        """Context options (SOL105_OPT_*): basis size before a thick restart,
+        thick restarts before SOL105_E_NOCONV, warm starts, step graphs."""
         self.check(self.L.sol105_ctx_set_option(self.h, int(option), float(value)))
 
     def profile_enable(self, on=True):

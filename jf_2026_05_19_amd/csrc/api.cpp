@@ -82,6 +82,52 @@ void autospc_mask(const sol105_model *M, const std::vector<double> &kd, std::vec
     for (int i = 0; i < ndof; i++) free_out[i] = fixed[i] ? 0 : 1;
 }
 
+void hash_solve_state(KeyHash &k, const sol105_model *M) {
+    static_assert(sizeof(DevFronts) == 15 * sizeof(void *), "DevFronts holds device pointers only");
+    k.add(M->dev);
+    const DevSolve &DS = M->dsolve;
+    for (const void *p : {(const void *)DS.poff, (const void *)DS.fwd_desc, (const void *)DS.bwd_desc,
+                          (const void *)DS.fwd_tiles, (const void *)DS.bwd_tiles, (const void *)DS.fwd_tbase_rel,
+                          (const void *)DS.bwd_tbase_rel, (const void *)DS.fwd_nks, (const void *)DS.bwd_nks,
+                          (const void *)DS.fwd_kw, (const void *)DS.bwd_kw, (const void *)DS.pull_ptr,
+                          (const void *)DS.pull_src, (const void *)DS.lblk, (const void *)DS.pull2})
+        k.add(p);
+    k.add(DS.n_lblk);
+    const FactorPlan &P = M->plan;
+    k.add(P.n_free);
+    k.add(P.height);
+    for (const HeightInfo &H : P.heights) {
+        k.add(H.n);
+        k.add(H.list_off);
+        k.add(H.max_ns);
+        k.add(H.max_m);
+    }
+    for (const SolveHeight &S : M->splan.heights) {
+        k.add(S.fwd_off);
+        k.add(S.bwd_off);
+        k.add(S.n_fwd);
+        k.add(S.n_bwd);
+        k.add(S.max_m);
+        k.add(S.max_ns);
+        k.add(S.fwd_split);
+        k.add(S.bwd_split);
+        k.add(S.fwd_kmax);
+        k.add(S.bwd_kmax);
+        for (const ProdCfg *c : {&S.fwd_cfg, &S.bwd_cfg}) {
+            k.add(c->kmax);
+            k.add(c->ks);
+            k.add(c->pch);
+            k.add(c->kg);
+        }
+    }
+    for (const void *p : {(const void *)M->d_P.p, (const void *)M->d_PT.p, (const void *)M->d_W.p,
+                          (const void *)M->d_part.p, (const void *)M->d_KG.p, (const void *)M->d_rp.p,
+                          (const void *)M->d_col.p, (const void *)M->d_elim_of_dof.p, (const void *)M->d_dof_of_elim.p})
+        k.add(p);
+    k.add(M->n_nodes);
+    k.add(M->sym_gen);
+}
+
 // Device copies of the tile-dependent parts of the solve plan (tiles, their
 // descriptors, per-front slice data, the partial-sum buffer): at the symbolic
 // analysis and whenever the product launch shapes change.
@@ -365,6 +411,10 @@ int sol105_ctx_set_option(sol105_ctx *ctx, int32_t option, double value) {
         case SOL105_OPT_WARM_START:
             if (value != 0 && value != 1) return set_err(ctx, SOL105_E_INVALID, "warm start is 0 or 1");
             ctx->warm_start = value != 0;
+            return SOL105_OK;
+        case SOL105_OPT_STEP_GRAPHS:
+            if (value != 0 && value != 1) return set_err(ctx, SOL105_E_INVALID, "step graphs is 0 or 1");
+            ctx->graphs = value != 0;
             return SOL105_OK;
         case SOL105_OPT_LANCZOS_MAX_RESTARTS:
             if (value < 0 || value > 1000)
@@ -908,6 +958,45 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
         upper_mul(b, G2, 16, Rt, 16, Tsub(jj), ldT, s);
         return SOL105_OK;
     };
+    // step graphs: valid while the key (addresses, sizes, launch shapes) holds
+    StepGraphs &G = W.graphs;
+    const bool use_graphs = ctx->graphs && !ctx->profiling;
+    if (use_graphs) {
+        KeyHash k;
+        hash_solve_state(k, M);
+        for (const void *p : {(const void *)V.p, (const void *)Wb.p, (const void *)xf.p, (const void *)yf.p,
+                              (const void *)T.p, (const void *)part.p, (const void *)small.p, (const void *)H2.p,
+                              (const void *)ierr.p})
+            k.add(p);
+        k.add(n);
+        k.add(ndof);
+        k.add(b);
+        k.add(ldT);
+        if (G.key != k.h) {
+            G.clear();
+            G.key = k.h;
+        } else {
+            G.armed = true;
+        }
+    }
+    auto run_step = [&](int jj) {
+        if (!use_graphs || !G.armed) return fast_step(jj);
+        if ((size_t)jj >= G.exec.size()) G.exec.resize(jj + 1, nullptr);
+        if (!G.exec[jj]) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rc = fast_step(jj);
+            hipError_t e = hipStreamEndCapture(s, &g);
+            hipGraphExec_t ex = nullptr;
+            if (rc == SOL105_OK && e == hipSuccess) e = hipGraphInstantiateWithFlags(&ex, g, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (rc) return rc;
+            HIPCHK(e);
+            G.exec[jj] = ex;
+        }
+        HIPCHK(hipGraphLaunch(G.exec[jj], s));
+        return SOL105_OK;
+    };
     start_block(n, b, V.p, n, 0, s);
     // warm start: the Ritz vectors of the last solve of this model (same
     // free set and elimination order) replace the first columns of the start
@@ -984,7 +1073,7 @@ int lanczos(sol105_ctx *ctx, sol105_model *M, int want, double *d_Y /* n x want,
     bool step = true;
     while (true) {
         if (step) {
-            int rc = fast_step(j);
+            int rc = run_step(j);
             if (rc) return rc;
             j++;
             generated += b;

@@ -39,6 +39,11 @@ struct sol105_batch {
     DevBuf<int32_t> d_seg_off, d_ch_r0, d_ch_seg, d_ch_ptr, d_chs_r0, d_chs_seg, d_chs_ptr;
     DevBuf<int32_t> d_q4_off, d_t3_off;
     DevBuf<uint8_t> d_active, d_conv, d_idle;
+    // stacked-iteration workspace, kept between solves of the batch (the
+    // step graphs replay on its addresses)
+    DevBuf<double> V, Wb, xf, yf, Tm, A, Zs, thd, ework, part, G1, G2, G0, Rt, H2, Y, zl, rl;
+    DevBuf<int> ierr, info, dlist;
+    StepGraphs graphs;
     double last_ms = 0.0;
     int last_m = 0, last_steps = 0, n_fallback = 0;
 };
@@ -208,8 +213,10 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
     Ti.active = B->d_idle.p;
     const int n = ntot;
     const int ldT = mmax + b;
-    DevBuf<double> V, Wb, xf, yf, Tm, A, Zs, thd, ework, part, G1, G2, G0, Rt, H2, Y, zl, rl;
-    DevBuf<int> ierr, info, dlist;
+    DevBuf<double> &V = B->V, &Wb = B->Wb, &xf = B->xf, &yf = B->yf, &Tm = B->Tm, &A = B->A, &Zs = B->Zs,
+                   &thd = B->thd, &ework = B->ework, &part = B->part, &G1 = B->G1, &G2 = B->G2, &G0 = B->G0,
+                   &Rt = B->Rt, &H2 = B->H2, &Y = B->Y, &zl = B->zl, &rl = B->rl;
+    DevBuf<int> &ierr = B->ierr, &info = B->info, &dlist = B->dlist;
     int nact = 0;
     for (int i = 0; i < nm; i++) nact += E[i].active;
     const int kdim = std::max(kmax, 1);
@@ -278,6 +285,47 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
         cholqr2(blk(jj + 1), Tm.p + (size_t)(jj + 1) * b + (size_t)jj * b * ldT, sT, ldT, G0.p, true);
         return SOL105_OK;
     };
+    // step graphs (as for a single model): the key covers the solve state,
+    // the workspace and segment-table addresses and the stacked sizes; the
+    // members' activity flags are read by the kernels at run time
+    StepGraphs &G = B->graphs;
+    const bool use_graphs = ctx->graphs && !ctx->profiling && nact > 0;
+    if (use_graphs) {
+        KeyHash k;
+        hash_solve_state(k, M);
+        for (const void *p : {(const void *)V.p, (const void *)Wb.p, (const void *)xf.p, (const void *)yf.p,
+                              (const void *)Tm.p, (const void *)part.p, (const void *)G0.p, (const void *)G1.p,
+                              (const void *)G2.p, (const void *)H2.p, (const void *)ierr.p, (const void *)T.seg_off,
+                              (const void *)T.ch_r0, (const void *)T.ch_seg, (const void *)T.ch_ptr,
+                              (const void *)T.chs_r0, (const void *)T.chs_seg, (const void *)T.chs_ptr,
+                              (const void *)T.active})
+            k.add(p);
+        for (int v : {T.n_seg, T.n_ch, T.n_chs, n, ndof, b, ldT, mmax}) k.add(v);
+        if (G.key != k.h) {
+            G.clear();
+            G.key = k.h;
+        } else {
+            G.armed = true;
+        }
+    }
+    auto run_step = [&](int jj) {
+        if (!use_graphs || !G.armed) return step(jj);
+        if ((size_t)jj >= G.exec.size()) G.exec.resize(jj + 1, nullptr);
+        if (!G.exec[jj]) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rc = step(jj);
+            hipError_t e = hipStreamEndCapture(s, &g);
+            hipGraphExec_t ex = nullptr;
+            if (rc == SOL105_OK && e == hipSuccess) e = hipGraphInstantiateWithFlags(&ex, g, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (rc) return rc;
+            HIPCHK(e);
+            G.exec[jj] = ex;
+        }
+        HIPCHK(hipGraphLaunch(G.exec[jj], s));
+        return (int)SOL105_OK;
+    };
     int j = 0, m = 0;
     if (nact > 0) {
         seg_start_block(T, b, V.p, n, 0, s);
@@ -288,7 +336,7 @@ int batch_buckling(sol105_ctx *ctx, sol105_batch *B, int max_modes) {
     std::vector<double> hth, hzl, hrl;
     std::vector<int> herr, hinfo, list;
     while (nact > 0) {
-        int rc = step(j);
+        int rc = run_step(j);
         if (rc) return rc;
         j++;
         m = j * b;

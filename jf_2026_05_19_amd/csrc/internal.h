@@ -29,6 +29,7 @@ struct sol105_ctx {
     int lanczos_max_vectors = 1024;  // SOL105_OPT_LANCZOS_MAX_VECTORS
     int lanczos_max_restarts = 40;   // SOL105_OPT_LANCZOS_MAX_RESTARTS
     bool warm_start = false;         // SOL105_OPT_WARM_START
+    bool graphs = true;              // SOL105_OPT_STEP_GRAPHS
     ProdTimer ptimer;  // HIP-event timing of the block triangular-solve products
     ProdTimer *timer() { return profiling ? &ptimer : nullptr; }
 };
@@ -61,12 +62,47 @@ struct DevBuf {
 
 }  // namespace s105
 
+// HIP graphs of the block Lanczos expansion step, one per block index.  A
+// step's launches (both triangular solves over every tree height, the KG
+// product, the two orthogonalisation passes and Cholesky-QR) depend only on
+// the block index and on buffer addresses, sizes and launch shapes that stay
+// fixed for a model between analyses; `key` hashes all of them.  The first
+// solve under a key launches directly and arms the cache, the next one
+// captures each step as it reaches it, and later steps and solves replay.
+struct StepGraphs {
+    std::vector<hipGraphExec_t> exec;  // per block index (nullptr: not captured)
+    uint64_t key = 0;
+    bool armed = false;
+    void clear() {
+        for (auto &g : exec)
+            if (g) (void)hipGraphExecDestroy(g);
+        exec.clear();
+        key = 0;
+        armed = false;
+    }
+    ~StepGraphs() { clear(); }
+};
+
 // Eigensolver workspace kept with a model, so repeated solves (design
 // loops, served requests) re-use the device buffers instead of allocating
 // and freeing them per solve.
 struct LanczosWork {
     s105::DevBuf<double> V, Wb, xf, yf, T, A, Zs, thd, ework, part, small, H2, Ytmp, Y, Ys, Xf, scr, sh;
     s105::DevBuf<int> ierr, sw;
+    StepGraphs graphs;
+};
+
+// FNV-1a over the values a captured launch sequence depends on.
+struct KeyHash {
+    uint64_t h = 1469598103934665603ull;
+    void bytes(const void *p, size_t n) {
+        const unsigned char *c = (const unsigned char *)p;
+        for (size_t i = 0; i < n; i++) h = (h ^ c[i]) * 1099511628211ull;
+    }
+    template <typename T>
+    void add(const T &v) {
+        bytes(&v, sizeof(T));
+    }
 };
 
 struct sol105_model {
@@ -160,6 +196,9 @@ int model_init(sol105_ctx *ctx, const sol105_model_desc *d, sol105_model *M);
 // the pivot-ratio refix and, when shifted_fallback is set, the factorisation
 // AUTOSPC of a failed Cholesky (shifted probes)
 int factorize(sol105_ctx *ctx, sol105_model *M, bool shifted_fallback);
+// the launch shapes and device addresses that the triangular solves and the
+// KG product of a model depend on (key of its Lanczos step graphs)
+void hash_solve_state(KeyHash &k, const sol105_model *M);
 // device address of the no-allocation scratch array on `device` (current),
 // looked up once per device (noalloc.hip)
 hipError_t noalloc_setup(int device);
