@@ -123,11 +123,30 @@ const char *sol105_last_error(const sol105_ctx *ctx);
  *                                    and launch shapes.  Results are bitwise
  *                                    those of direct launches.  No
  *                                    counterpart in the reference.
+ *   SOL105_OPT_SENS_MODE             0 or 1 (default 0): the CQUAD4 terms of
+ *                                    sol105_sensitivity_thickness /
+ *                                    _terms.  0: the exact derivative of the
+ *                                    forward path (dKe/dt of the MacNeal
+ *                                    element, d(phi' KG(u) phi)/du through
+ *                                    the principal-stress split).  1: the
+ *                                    reference's own analytic path
+ *                                    (JFEM/src/solver/adjoint_buckling.jl:
+ *                                    2406-2637 with its defaults): dKe/dt of
+ *                                    the legacy MITC4 element
+ *                                    stiffness_quad4_generic (dKdx.jl:219-
+ *                                    247), the unit-basis adjoint load
+ *                                    phi' KG(sigma(e_j)) phi (adjoint_
+ *                                    buckling.jl:1024-1042) and phi' KG phi
+ *                                    / t, with six condensed membrane modes
+ *                                    when the model has no CBAR, else four.
+ *                                    CTRIA3 and CBAR terms are the same in
+ *                                    both modes.
  * SOL105_E_INVALID for an unknown option or a value out of range. */
 #define SOL105_OPT_LANCZOS_MAX_VECTORS 1
 #define SOL105_OPT_LANCZOS_MAX_RESTARTS 2
 #define SOL105_OPT_WARM_START 3
 #define SOL105_OPT_STEP_GRAPHS 4
+#define SOL105_OPT_SENS_MODE 5
 int sol105_ctx_set_option(sol105_ctx *ctx, int32_t option, double value);
 
 /* Upload a model, build the BSR pattern, the deterministic element->block

@@ -227,6 +227,7 @@ class Context:
     OPT_LANCZOS_MAX_RESTARTS = 2
     OPT_WARM_START = 3
     OPT_STEP_GRAPHS = 4
+    OPT_SENS_MODE = 5
 
     def set_option(self, option, value):
         """Context options (SOL105_OPT_*): basis size before a thick restart,

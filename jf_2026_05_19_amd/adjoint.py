@@ -18,6 +18,15 @@ empty dictionary (adjoint_buckling.jl:2479-2484).  The configuration follows
 ``parse_adjoint_config`` (JFEM/src/solver/adjoint.jl:16-38) and the design
 variable values ``get_design_variable_values`` (adjoint.jl:789-800).
 
+``mode="exact"`` (default) takes the per-shell terms of the exact derivative
+of the forward path.  ``mode="reference"`` takes the reference's own analytic
+path for CQUAD4 (SOL105_OPT_SENS_MODE = 1: dKe/dt of the legacy MITC4 element
+stiffness_quad4_generic, the unit-basis adjoint load, the context KG with six
+condensed membrane modes when the model has no bars) and, like
+``_dKg_dx_phi_thickness`` (adjoint_buckling.jl:2059-2089), replaces the
+group's sum of phi' KGe phi / t by phi' KG phi / T when the design variable
+is one PSHELL that owns every shell and the model has no bars.
+
 Other design-variable types (material E and NU, bar area, node coordinates,
 topology density) have no device path and raise ``UnsupportedModel``.
 """
@@ -83,40 +92,63 @@ def design_variable_values(dv, model):
     return out
 
 
-def group_sensitivities(terms, lam, phi_kg_phi, shell_pids, pids):
+def group_sensitivities(terms, lam, phi_kg_phi, shell_pids, pids, whole_kg_over_t=None):
     """d lambda / d T_p for each listed PSHELL id p from the per-shell terms
     ``[phi' dKe/dt phi, phi' KGe phi / t, psi' dKe/dt u]`` (rows in shell
     order, ``shell_pids`` the property of each row).  A PSHELL without shells
-    has derivative zero."""
+    has derivative zero.  ``whole_kg_over_t`` (phi' KG phi / T of the
+    assembled KG) replaces the group's second term when given."""
     terms = np.asarray(terms, dtype=float)
     shell_pids = np.asarray(shell_pids, dtype=np.int64)
     out = {}
     for pid in pids:
         rows = shell_pids == int(pid)
         t = terms[rows].sum(axis=0) if rows.any() else np.zeros(3)
-        out["PID_%d" % int(pid)] = float(-(t[0] + lam * t[1] - t[2]) / phi_kg_phi) if rows.any() else 0.0
+        t2 = t[1] if whole_kg_over_t is None else whole_kg_over_t
+        out["PID_%d" % int(pid)] = float(-(t[0] + lam * t2 - t[2]) / phi_kg_phi) if rows.any() else 0.0
     return out
 
 
-def solve_adjoint_buckling(dm, lam, model, config):
+def _whole_kg_shortcut(dv, shell_pids, n_bar):
+    """The reference's single-PSHELL shortcut of the dKG/dT term applies."""
+    pids = [int(p) for p in dv["pids"]]
+    return len(pids) == 1 and set(np.asarray(shell_pids).tolist()) <= {pids[0]} and n_bar == 0
+
+
+def solve_adjoint_buckling(dm, lam, model, config, mode="exact"):
     """The reference's result dictionary for the solved device model ``dm``
     (its last buckling solve gave the eigenvalues ``lam``): sensitivities,
-    eigenvalue_values and design_variable_values."""
+    eigenvalue_values and design_variable_values.  ``mode``: "exact" or
+    "reference" (see the module docstring)."""
+    if mode not in ("exact", "reference"):
+        raise ValueError("mode is 'exact' or 'reference'")
     config = parse_adjoint_config(config)
     dvs = config["design_variables"]
     for dv in dvs:
         _check_supported(dv)
     arr = dm.arr
     shell_pids = np.concatenate([np.asarray(arr.q4_pid, np.int64), np.asarray(arr.t3_pid, np.int64)])
+    thick = np.concatenate([np.asarray(arr.q4prop, float).reshape(-1, 6)[:, 2],
+                            np.asarray(arr.t3prop, float).reshape(-1, 6)[:, 2]])
+    ref = mode == "reference"
     sens, eig = {}, {}
-    for i, li in enumerate(np.asarray(lam, dtype=float)):
-        mode_id = "mode_%d" % (i + 1)
-        eig[mode_id] = float(li)
-        terms, _, pkp = dm.sensitivity_terms(i)
-        if abs(pkp) < 1e-30:
+    dm.ctx.set_option(dm.ctx.OPT_SENS_MODE, 1 if ref else 0)
+    try:
+        for i, li in enumerate(np.asarray(lam, dtype=float)):
+            mode_id = "mode_%d" % (i + 1)
+            eig[mode_id] = float(li)
+            terms, _, pkp = dm.sensitivity_terms(i)
+            if abs(pkp) < 1e-30:
+                sens[mode_id] = {}
+                continue
             sens[mode_id] = {}
-            continue
-        sens[mode_id] = {dv["id"]: group_sensitivities(terms, li, pkp, shell_pids, dv["pids"]) for dv in dvs}
+            for dv in dvs:
+                whole = None
+                if ref and len(thick) and _whole_kg_shortcut(dv, shell_pids, arr.n_bar):
+                    whole = pkp / float(thick[0])
+                sens[mode_id][dv["id"]] = group_sensitivities(terms, li, pkp, shell_pids, dv["pids"], whole)
+    finally:
+        dm.ctx.set_option(dm.ctx.OPT_SENS_MODE, 0)
     return {
         "sensitivities": sens,
         "eigenvalue_values": eig,

@@ -416,6 +416,10 @@ int sol105_ctx_set_option(sol105_ctx *ctx, int32_t option, double value) {
             if (value != 0 && value != 1) return set_err(ctx, SOL105_E_INVALID, "step graphs is 0 or 1");
             ctx->graphs = value != 0;
             return SOL105_OK;
+        case SOL105_OPT_SENS_MODE:
+            if (value != 0 && value != 1) return set_err(ctx, SOL105_E_INVALID, "sensitivity mode is 0 or 1");
+            ctx->sens_mode = (int)value;
+            return SOL105_OK;
         case SOL105_OPT_LANCZOS_MAX_RESTARTS:
             if (value < 0 || value > 1000)
                 return set_err(ctx, SOL105_E_INVALID, "Lanczos restarts %g out of range 0..1000", value);
@@ -1448,7 +1452,11 @@ int sens_terms(sol105_ctx *ctx, sol105_model *M, int32_t mode, DevBuf<double> &t
     HIPCHK(dots.alloc(2));
     HIPCHK(phif.alloc(ndof));
     HIPCHK(terms.alloc((size_t)std::max(n_shell, 1) * 3));
-    launch_q4_adjoint_rhs(a, M->d_u.p, phi, lam, egq.p, s);
+    // reference mode: six condensed membrane modes in a model without line
+    // elements (adjoint_buckling.jl:950-955), else the four of the forward KG
+    const bool ref = ctx->sens_mode == 1, six = M->n_bar == 0;
+    if (ref) launch_q4_adjoint_rhs_ref(a, M->d_u.p, phi, lam, six, egq.p, s);
+    else launch_q4_adjoint_rhs(a, M->d_u.p, phi, lam, egq.p, s);
     launch_t3_kg_sens(a, M->d_u.p, phi, lam, terms.p, egt.p, s);
     launch_bar_adjoint_rhs(a, phi, lam, egb.p, s);
     node_gather_vec(M->n_nodes, M->n_q4, M->n_t3, M->d_diag.p, M->d_cptr.p, M->d_cidx.p, egq.p, egt.p, egb.p,
@@ -1461,7 +1469,8 @@ int sens_terms(sol105_ctx *ctx, sol105_model *M, int32_t mode, DevBuf<double> &t
     launch_scatter_elim(n, ndof, 1, M->d_elim_of_dof.p, ge.p, phif.p, s);
     launch_bsr_spmv(M->n_nodes, M->d_rp.p, M->d_col.p, M->d_KG.p, phif.p, kgphi.p, 1, 1.0, s);
     dot_det(ndof, phif.p, kgphi.p, scr.p, dots.p, s);
-    launch_q4_sens_terms(a, M->d_u.p, phif.p, psi.p, terms.p, s);
+    if (ref) launch_q4_sens_terms_ref(a, M->d_u.p, phif.p, psi.p, six, terms.p, s);
+    else launch_q4_sens_terms(a, M->d_u.p, phif.p, psi.p, terms.p, s);
     launch_t3_sens_terms(a, M->d_u.p, phif.p, psi.p, terms.p, s);
     HIPCHK(hipMemcpyAsync(&pkp, dots.p, sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));

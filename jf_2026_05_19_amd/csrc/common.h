@@ -57,6 +57,12 @@ void launch_q4_sens_terms(const ElemArgs &a, const double *u, const double *phi,
 void launch_sens_combine(int n_q4, const double *terms, double lam, double phiKGphi, double *dldt, hipStream_t s);
 void launch_q4_adjoint_rhs(const ElemArgs &a, const double *u, const double *phi, double lam,
                            double *eg /* n_q4 x 24 */, hipStream_t s);
+// the reference's own analytic path (legacy dKe/dt, unit-basis adjoint load;
+// six condensed membrane modes when the model has no bars, else four)
+void launch_q4_sens_terms_ref(const ElemArgs &a, const double *u, const double *phi, const double *psi,
+                              bool six_modes, double *terms, hipStream_t s);
+void launch_q4_adjoint_rhs_ref(const ElemArgs &a, const double *u, const double *phi, double lam, bool six_modes,
+                               double *eg, hipStream_t s);
 void launch_bar_adjoint_rhs(const ElemArgs &a, const double *phi, double lam, double *eg /* n_bar x 12 */,
                             hipStream_t s);
 // CTRIA3 rows n_q4 + e of terms: [0], [2] (launch_t3_sens_terms) and [1]

@@ -537,24 +537,30 @@ __global__ void __launch_bounds__(256) q4_ke_kernel(ElemArgs A, double *eb) {
 // CQUAD4 geometric stiffness
 // ---------------------------------------------------------------------------
 
-struct Q4KGStage {
+// NM condensed membrane modes: the four Wilson modes (1 - r^2), (1 - s^2) on
+// ux and uy of the forward KG, or six with the r s mode added (the
+// "enhanced" set of the reference's adjoint path, FEMKernels.jl:1861-1896)
+template <int NM>
+struct Q4KGStageT {
     double lc[8];
     double R[9];
     double ul[24];
-    double gdx[4][4], gdy[4][4], gadj[4], gdp[4][4];
-    double Kab[24][4];
-    double Kbb[16], Kbbi[16];
-    double Am[4][24];   // condensation map A = -Kbb^-1 Kab^T
-    double alpha[4];
+    double gdx[4][4], gdy[4][4], gadj[4], gdp[4][NM == 6 ? 6 : 4];  // + d(rs)/dx, d(rs)/dy with six modes
+    double Kab[24][NM];
+    double Kbb[NM * NM], Kbbi[NM * NM];
+    double Am[NM][24];  // condensation map A = -Kbb^-1 Kab^T
+    double alpha[NM];
     double sig[4][3];
     double pr[4][6];    // principal stresses per GP: l1 c1 s1 l2 c2 s2
     double va[8][24];   // (s gx - c gy) per (gp, direction)
     double vb[8][24];   // (c gzx + s gzy)
     double wt[8];
-    double out[576];
+    double out[NM == 4 ? 576 : 1];  // the six-mode stage (adjoint kernels) writes no element block
 };
+using Q4KGStage = Q4KGStageT<4>;
 
-__device__ void q4kg_gp_stage(Q4KGStage &S, int lane) {
+template <int NM>
+__device__ void q4kg_gp_stage(Q4KGStageT<NM> &S, int lane) {
     const double r = (lane == 1 || lane == 2) ? PT : -PT, s = (lane >= 2) ? PT : -PT;
     double dNr[4], dNs[4], J11, J12, J21, J22;
     shape_derivs(r, s, dNr, dNs);
@@ -573,6 +579,10 @@ __device__ void q4kg_gp_stage(Q4KGStage &S, int lane) {
     S.gdp[lane][1] = iJ21 * (-2.0 * r);
     S.gdp[lane][2] = iJ12 * (-2.0 * s);
     S.gdp[lane][3] = iJ22 * (-2.0 * s);
+    if constexpr (NM == 6) {
+        S.gdp[lane][4] = iJ11 * s + iJ12 * r;
+        S.gdp[lane][5] = iJ21 * s + iJ22 * r;
+    }
 }
 
 // principal stress decomposition, radius <= 1e-30 returns the axes
@@ -585,9 +595,19 @@ __device__ void principal2d(double sxx, double syy, double sxy, double *o) {
     o[0] = mean + rad; o[1] = c; o[2] = s; o[3] = mean - rad; o[4] = -s; o[5] = c;
 }
 
+// membrane strain column (3-vector) of condensed mode m at GP g
+template <int NM>
+__device__ __forceinline__ void q4kg_mode_col(const Q4KGStageT<NM> &S, int g, int m, double *b) {
+    const int j = 2 * (m / 2);  // first gdp entry of the mode's shape function
+    const double px = S.gdp[g][j], py = S.gdp[g][j + 1];
+    if (m % 2 == 0) { b[0] = px; b[1] = 0.0; b[2] = py; }
+    else { b[0] = 0.0; b[1] = py; b[2] = px; }
+}
+
 // gradient vectors of (ux, uy, uz) along x/y at GP g for element DOF a,
-// including the condensed Wilson modes.
-__device__ __forceinline__ void q4kg_grads(const Q4KGStage &S, int g, int a, double *gr /*6*/) {
+// including the condensed modes.
+template <int NM>
+__device__ __forceinline__ void q4kg_grads(const Q4KGStageT<NM> &S, int g, int a, double *gr /*6*/) {
     int k = a / 6, c = a % 6;
     double dx = S.gdx[g][k], dy = S.gdy[g][k];
     double uxx = (c == 0) ? dx : 0.0, uxy = (c == 0) ? dy : 0.0;
@@ -595,16 +615,51 @@ __device__ __forceinline__ void q4kg_grads(const Q4KGStage &S, int g, int a, dou
     double uzx = (c == 2) ? dx : 0.0, uzy = (c == 2) ? dy : 0.0;
     double p1x = S.gdp[g][0], p1y = S.gdp[g][1], p2x = S.gdp[g][2], p2y = S.gdp[g][3];
     double a1 = S.Am[0][a], a2 = S.Am[1][a], a3 = S.Am[2][a], a4 = S.Am[3][a];
-    uxx += p1x * a1 + p2x * a3;
-    uxy += p1y * a1 + p2y * a3;
-    uyx += p1x * a2 + p2x * a4;
-    uyy += p1y * a2 + p2y * a4;
+    if constexpr (NM == 6) {
+        double qx = S.gdp[g][4], qy = S.gdp[g][5], a5 = S.Am[4][a], a6 = S.Am[5][a];
+        uxx += p1x * a1 + p2x * a3 + qx * a5;
+        uxy += p1y * a1 + p2y * a3 + qy * a5;
+        uyx += p1x * a2 + p2x * a4 + qx * a6;
+        uyy += p1y * a2 + p2y * a4 + qy * a6;
+    } else {
+        uxx += p1x * a1 + p2x * a3;
+        uxy += p1y * a1 + p2y * a3;
+        uyx += p1x * a2 + p2x * a4;
+        uyy += p1y * a2 + p2y * a4;
+    }
     gr[0] = uxx; gr[1] = uxy; gr[2] = uyx; gr[3] = uyy; gr[4] = uzx; gr[5] = uzy;
+}
+
+// n x n inverse by Gauss-Jordan with partial pivoting (single lane; the
+// six-mode Kbb)
+template <int N>
+__device__ void inv_gj(const double *a, double *ai) {
+    double m[N][2 * N];
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) { m[i][j] = a[i * N + j]; m[i][N + j] = (i == j) ? 1.0 : 0.0; }
+    for (int c = 0; c < N; c++) {
+        int p = c;
+        double best = fabs(m[c][c]);
+        for (int r = c + 1; r < N; r++)
+            if (fabs(m[r][c]) > best) { best = fabs(m[r][c]); p = r; }
+        if (p != c)
+            for (int j = 0; j < 2 * N; j++) { double t = m[c][j]; m[c][j] = m[p][j]; m[p][j] = t; }
+        double d = m[c][c];
+        for (int j = 0; j < 2 * N; j++) m[c][j] /= d;
+        for (int r = 0; r < N; r++) {
+            if (r == c) continue;
+            double f = m[r][c];
+            for (int j = 0; j < 2 * N; j++) m[r][j] -= f * m[c][j];
+        }
+    }
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) ai[i * N + j] = m[i][N + j];
 }
 
 // Shared set-up of the KG and sensitivity kernels: frame, local u, GP data,
 // condensation map, membrane stresses and principal directions.
-__device__ bool q4kg_setup(Q4KGStage &S, const ElemArgs &A, const double *u, int e, int lane, double &h) {
+template <int NM>
+__device__ bool q4kg_setup(Q4KGStageT<NM> &S, const ElemArgs &A, const double *u, int e, int lane, double &h) {
     bool valid = e < A.n_q4;
     double Cm[9];
     h = 1.0;
@@ -636,54 +691,51 @@ __device__ bool q4kg_setup(Q4KGStage &S, const ElemArgs &A, const double *u, int
     }
     __syncthreads();
     if (valid) {
-        for (int t = lane; t < 112; t += S105_WAVE) {
-            if (t < 96) {
-                int a = t / 4, m = t % 4;
+        for (int t = lane; t < 24 * NM + NM * NM; t += S105_WAVE) {
+            if (t < 24 * NM) {
+                int a = t / NM, m = t % NM;
                 int k = a / 6, c = a % 6;
                 double s = 0.0;
                 for (int g = 0; g < 4; g++) {
                     double dx = S.gdx[g][k], dy = S.gdy[g][k];
                     double ba[3] = {(c == 0) ? dx : 0.0, (c == 1) ? dy : 0.0, (c == 0) ? dy : ((c == 1) ? dx : 0.0)};
-                    double p1x = S.gdp[g][0], p1y = S.gdp[g][1], p2x = S.gdp[g][2], p2y = S.gdp[g][3];
                     double bi[3];
-                    switch (m) {
-                        case 0: bi[0] = p1x; bi[1] = 0.0; bi[2] = p1y; break;
-                        case 1: bi[0] = 0.0; bi[1] = p1y; bi[2] = p1x; break;
-                        case 2: bi[0] = p2x; bi[1] = 0.0; bi[2] = p2y; break;
-                        default: bi[0] = 0.0; bi[1] = p2y; bi[2] = p2x; break;
-                    }
+                    q4kg_mode_col(S, g, m, bi);
                     double cbi[3];
                     mat3v(Cm, bi, cbi);
                     s += (ba[0] * cbi[0] + ba[1] * cbi[1] + ba[2] * cbi[2]) * S.gadj[g];
                 }
                 S.Kab[a][m] = s;
             } else {
-                int idx = t - 96;
-                int l = idx / 4, m = idx % 4;
+                int idx = t - 24 * NM;
+                int l = idx / NM, m = idx % NM;
                 double s = 0.0;
                 for (int g = 0; g < 4; g++) {
-                    double p1x = S.gdp[g][0], p1y = S.gdp[g][1], p2x = S.gdp[g][2], p2y = S.gdp[g][3];
-                    double cols[4][3] = {{p1x, 0.0, p1y}, {0.0, p1y, p1x}, {p2x, 0.0, p2y}, {0.0, p2y, p2x}};
-                    double cbm[3];
-                    mat3v(Cm, cols[m], cbm);
-                    s += (cols[l][0] * cbm[0] + cols[l][1] * cbm[1] + cols[l][2] * cbm[2]) * S.gadj[g];
+                    double bl[3], bm[3], cbm[3];
+                    q4kg_mode_col(S, g, l, bl);
+                    q4kg_mode_col(S, g, m, bm);
+                    mat3v(Cm, bm, cbm);
+                    s += (bl[0] * cbm[0] + bl[1] * cbm[1] + bl[2] * cbm[2]) * S.gadj[g];
                 }
-                S.Kbb[l * 4 + m] = s;
+                S.Kbb[l * NM + m] = s;
             }
         }
     }
     __syncthreads();
-    if (valid && lane == 0) inv4(S.Kbb, S.Kbbi);
+    if (valid && lane == 0) {
+        if constexpr (NM == 4) inv4(S.Kbb, S.Kbbi);
+        else inv_gj<NM>(S.Kbb, S.Kbbi);
+    }
     __syncthreads();
     if (valid)
-        for (int t = lane; t < 96; t += S105_WAVE) {
+        for (int t = lane; t < NM * 24; t += S105_WAVE) {
             int l = t / 24, a = t % 24;
             double s = 0.0;
-            for (int q = 0; q < 4; q++) s += S.Kbbi[l * 4 + q] * S.Kab[a][q];
+            for (int q = 0; q < NM; q++) s += S.Kbbi[l * NM + q] * S.Kab[a][q];
             S.Am[l][a] = -s;
         }
     __syncthreads();
-    if (valid && lane < 4) {
+    if (valid && lane < NM) {
         double s = 0.0;
         for (int a = 0; a < 24; a++) s += S.Am[lane][a] * S.ul[a];
         S.alpha[lane] = s;
@@ -702,6 +754,12 @@ __device__ bool q4kg_setup(Q4KGStage &S, const ElemArgs &A, const double *u, int
         eps[0] += p1x * S.alpha[0] + p2x * S.alpha[2];
         eps[1] += p1y * S.alpha[1] + p2y * S.alpha[3];
         eps[2] += p1y * S.alpha[0] + p1x * S.alpha[1] + p2y * S.alpha[2] + p2x * S.alpha[3];
+        if constexpr (NM == 6) {
+            double qx = S.gdp[g][4], qy = S.gdp[g][5];
+            eps[0] += qx * S.alpha[4];
+            eps[1] += qy * S.alpha[5];
+            eps[2] += qy * S.alpha[4] + qx * S.alpha[5];
+        }
         double N[3];
         mat3v(Cm, eps, N);
         for (int i = 0; i < 3; i++) S.sig[g][i] = N[i] / h;
@@ -712,7 +770,8 @@ __device__ bool q4kg_setup(Q4KGStage &S, const ElemArgs &A, const double *u, int
 }
 
 // va/vb vectors and weights for the 8 (gp, direction) rank-2 updates.
-__device__ void q4kg_vectors(Q4KGStage &S, int lane, double h) {
+template <int NM>
+__device__ void q4kg_vectors(Q4KGStageT<NM> &S, int lane, double h) {
     for (int t = lane; t < 8 * 24; t += S105_WAVE) {
         int q = t / 24, a = t % 24;
         int g = q / 2, k = q % 2;
@@ -1455,13 +1514,15 @@ __global__ void __launch_bounds__(256) q4_dke_kernel(ElemArgs A, const double *u
     if (valid && lane == 0) { terms[3 * e] = f1; terms[3 * e + 2] = f2; }
 }
 
-// phi' KGe phi / t per element (the dKG/dt|u term)
+// phi' KGe phi / t per element (the dKG/dt|u term), KGe with NM condensed
+// membrane modes (four: the forward KG)
+template <int NM>
 __global__ void __launch_bounds__(256) q4_kgt_kernel(ElemArgs A, const double *u, const double *phi,
                                                      double *terms /* n_q4 x 3 */) {
-    __shared__ Q4KGStage stage[EPB];
+    __shared__ Q4KGStageT<NM> stage[EPB];
     int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
     int e = blockIdx.x * EPB + w;
-    Q4KGStage &S = stage[w];
+    Q4KGStageT<NM> &S = stage[w];
     double h;
     bool valid = q4kg_setup(S, A, u, e, lane, h);
     if (valid) q4kg_vectors(S, lane, h);
@@ -1623,6 +1684,291 @@ __global__ void __launch_bounds__(256) q4_adjrhs_kernel(ElemArgs A, const double
             for (int g = 0; g < 4; g++)
                 for (int q = 0; q < 3; q++) s += dphi_s[w][g][q] * dsdu[w][g][q][a];
             gl[i] = s;
+        }
+        for (int j = 0; j < 3; j++) {
+            double v = S.R[j] * gl[0] + S.R[3 + j] * gl[1] + S.R[6 + j] * gl[2];
+            eg[(size_t)e * 24 + 3 * b + j] = lam * v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The reference's own analytic thickness sensitivity (its default adjoint
+// path for flat isotropic CQUAD4, adjoint_buckling.jl:2406-2637), selected by
+// SOL105_OPT_SENS_MODE:
+//  * dKe/dt of the legacy element stiffness_quad4_generic (dKdx.jl:219-247,
+//    FEMKernels.jl:1473-1658), not of the forward MacNeal element: Wilson
+//    membrane modes, standard bending B, MITC4 shear with
+//    phi^2 = min(1, 10 t^2 / (4 |det J_c|)), drilling, with the generic
+//    defaults (bending ratio 1, TS/T 5/6, K6ROT 100).  Membrane, drilling and
+//    the condensation scale with t, bending with t^3, shear with phi^2(t) t,
+//    so dKe/dt = (Km + Kd - Kc)/t + 3 Kb/t + d(phi^2 t)/dt Ks/t exactly;
+//  * the adjoint load f_j = phi' KG(sigma(e_j)) phi over the 24 unit nodal
+//    displacements (_flat_iso_quad4_rhs_local, adjoint_buckling.jl:1024-1042)
+//    and phi' KG phi / t, with four Wilson modes when the model has bars and
+//    six (plus the r s mode) otherwise (adjoint_buckling.jl:930-1005).
+// ---------------------------------------------------------------------------
+
+struct Q4LegStage {
+    double lc[8];
+    double R[9];
+    double gdx[4][4], gdy[4][4], gN[4][4], gadj[4], gdp[4][4];
+    double giJ[4][4];   // inverse Jacobian per GP: iJ11 iJ12 iJ21 iJ22
+    double btp[4][12];  // MITC4 tying rows on (w, thx, thy) of the corners
+    double Kab[24][4], Kbb[16], Kbbi[16];
+    double dphi2t;      // d(phi^2 t)/dt
+};
+
+__global__ void __launch_bounds__(256) q4_dke_ref_kernel(ElemArgs A, const double *u, const double *phi,
+                                                         const double *psi, double *terms /* n_q4 x 3 */) {
+    __shared__ Q4LegStage stage[EPB];
+    int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
+    int e = blockIdx.x * EPB + w;
+    Q4LegStage &S = stage[w];
+    const bool valid = e < A.n_q4;
+    double E = 1.0, nu = 0.0, t = 1.0;
+    if (valid) {
+        const double *prop = A.q4prop + 6 * e;
+        E = prop[0];
+        nu = prop[1];
+        t = prop[2];
+        if (lane == 0) {
+            double p[12];
+            for (int k = 0; k < 4; k++) {
+                int nd = A.q4[4 * e + k];
+                for (int i = 0; i < 3; i++) p[3 * k + i] = A.xyz[3 * nd + i];
+            }
+            q4_frame(p, S.R, S.lc);
+        }
+    }
+    __syncthreads();
+    const double G = E / (2.0 * (1.0 + nu));
+    const double cmf = E * t / (1 - nu * nu), cbf = E * t * t * t / (12 * (1 - nu * nu));
+    const double D[9] = {1.0, nu, 0.0, nu, 1.0, 0.0, 0.0, 0.0, (1 - nu) / 2};
+    double Cm[9], Cb[9];
+    for (int i = 0; i < 9; i++) { Cm[i] = cmf * D[i]; Cb[i] = cbf * D[i]; }
+    const double tsg = (5.0 / 6.0) * G * t, alpha = (100.0 / 1e5) * G * t;
+    if (valid) {
+        if (lane < 4) {
+            const double r = (lane == 1 || lane == 2) ? PT : -PT, s = (lane >= 2) ? PT : -PT;
+            double dNr[4], dNs[4], J11, J12, J21, J22;
+            shape_derivs(r, s, dNr, dNs);
+            jacobian(S.lc, dNr, dNs, J11, J12, J21, J22);
+            double detJ = J11 * J22 - J12 * J21;
+            double adj = fabs(detJ);
+            if (adj < 1e-12) adj = 1e-12;
+            double id = 1.0 / detJ;
+            double iJ11 = J22 * id, iJ12 = -J12 * id, iJ21 = -J21 * id, iJ22 = J11 * id;
+            for (int k = 0; k < 4; k++) {
+                S.gdx[lane][k] = iJ11 * dNr[k] + iJ12 * dNs[k];
+                S.gdy[lane][k] = iJ21 * dNr[k] + iJ22 * dNs[k];
+                S.gN[lane][k] = 0.25 * (1 + ((k == 1 || k == 2) ? r : -r)) * (1 + (k >= 2 ? s : -s));
+            }
+            S.gadj[lane] = adj;
+            S.gdp[lane][0] = iJ11 * (-2.0 * r);
+            S.gdp[lane][1] = iJ21 * (-2.0 * r);
+            S.gdp[lane][2] = iJ12 * (-2.0 * s);
+            S.gdp[lane][3] = iJ22 * (-2.0 * s);
+            S.giJ[lane][0] = iJ11; S.giJ[lane][1] = iJ12; S.giJ[lane][2] = iJ21; S.giJ[lane][3] = iJ22;
+        } else if (lane < 8) {
+            // tying points (0,-1), (0,1) (r-shear) and (-1,0), (1,0) (s-shear)
+            const int tp = lane - 4;
+            const double r = tp < 2 ? 0.0 : (tp == 2 ? -1.0 : 1.0), s = tp == 0 ? -1.0 : (tp == 1 ? 1.0 : 0.0);
+            double dNr[4], dNs[4], J11, J12, J21, J22;
+            shape_derivs(r, s, dNr, dNs);
+            jacobian(S.lc, dNr, dNs, J11, J12, J21, J22);
+            const double N[4] = {0.25 * (1 - r) * (1 - s), 0.25 * (1 + r) * (1 - s), 0.25 * (1 + r) * (1 + s),
+                                 0.25 * (1 - r) * (1 + s)};
+            for (int k = 0; k < 4; k++) {
+                if (tp < 2) { S.btp[tp][3 * k] = dNr[k]; S.btp[tp][3 * k + 1] = -J12 * N[k]; S.btp[tp][3 * k + 2] = J11 * N[k]; }
+                else { S.btp[tp][3 * k] = dNs[k]; S.btp[tp][3 * k + 1] = -J22 * N[k]; S.btp[tp][3 * k + 2] = J21 * N[k]; }
+            }
+        } else if (lane == 8) {
+            const double dNr_c[4] = {-0.25, 0.25, 0.25, -0.25}, dNs_c[4] = {-0.25, -0.25, 0.25, 0.25};
+            double J11, J12, J21, J22;
+            jacobian(S.lc, dNr_c, dNs_c, J11, J12, J21, J22);
+            double trial = 10.0 * t * t / fmax(4.0 * fabs(J11 * J22 - J12 * J21), 1e-30);
+            S.dphi2t = trial < 1.0 ? 3.0 * trial : 1.0;
+        }
+    }
+    __syncthreads();
+    if (valid)
+        for (int q = lane; q < 112; q += S105_WAVE) {
+            if (q < 96) {
+                int a = q / 4, m = q % 4, k = a / 6, c = a % 6;
+                double sacc = 0.0;
+                for (int g = 0; g < 4; g++) {
+                    double dx = S.gdx[g][k], dy = S.gdy[g][k];
+                    double ba[3] = {(c == 0) ? dx : 0.0, (c == 1) ? dy : 0.0, (c == 0) ? dy : ((c == 1) ? dx : 0.0)};
+                    const int j = 2 * (m / 2);
+                    const double px = S.gdp[g][j], py = S.gdp[g][j + 1];
+                    double bi[3] = {m % 2 == 0 ? px : 0.0, m % 2 == 0 ? 0.0 : py, m % 2 == 0 ? py : px}, cbi[3];
+                    mat3v(Cm, bi, cbi);
+                    sacc += (ba[0] * cbi[0] + ba[1] * cbi[1] + ba[2] * cbi[2]) * S.gadj[g];
+                }
+                S.Kab[a][m] = sacc;
+            } else {
+                int l = (q - 96) / 4, m = (q - 96) % 4;
+                double sacc = 0.0;
+                for (int g = 0; g < 4; g++) {
+                    const int jl = 2 * (l / 2), jm = 2 * (m / 2);
+                    double bl[3] = {l % 2 == 0 ? S.gdp[g][jl] : 0.0, l % 2 == 0 ? 0.0 : S.gdp[g][jl + 1],
+                                    l % 2 == 0 ? S.gdp[g][jl + 1] : S.gdp[g][jl]};
+                    double bm[3] = {m % 2 == 0 ? S.gdp[g][jm] : 0.0, m % 2 == 0 ? 0.0 : S.gdp[g][jm + 1],
+                                    m % 2 == 0 ? S.gdp[g][jm + 1] : S.gdp[g][jm]}, cbm[3];
+                    mat3v(Cm, bm, cbm);
+                    sacc += (bl[0] * cbm[0] + bl[1] * cbm[1] + bl[2] * cbm[2]) * S.gadj[g];
+                }
+                S.Kbb[l * 4 + m] = sacc;
+            }
+        }
+    __syncthreads();
+    if (valid && lane == 0) inv4(S.Kbb, S.Kbbi);
+    __syncthreads();
+    double f1 = 0.0, f2 = 0.0;
+    if (valid) {
+        const int bi = lane / 8, bj = lane % 8;
+        double blk[9], gbl[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const int a = 3 * bi + i, b = 3 * bj + j;
+                const int ka = a / 6, ca = a % 6, kb = b / 6, cb = b % 6;
+                double km = 0.0, kbend = 0.0, kd = 0.0, ks = 0.0;
+                for (int g = 0; g < 4; g++) {
+                    const double adj = S.gadj[g];
+                    const double dxa = S.gdx[g][ka], dya = S.gdy[g][ka], dxb = S.gdx[g][kb], dyb = S.gdy[g][kb];
+                    // membrane
+                    double ma[3] = {ca == 0 ? dxa : 0.0, ca == 1 ? dya : 0.0, ca == 0 ? dya : (ca == 1 ? dxa : 0.0)};
+                    double mb[3] = {cb == 0 ? dxb : 0.0, cb == 1 ? dyb : 0.0, cb == 0 ? dyb : (cb == 1 ? dxb : 0.0)};
+                    double cmb[3];
+                    mat3v(Cm, mb, cmb);
+                    km += adj * (ma[0] * cmb[0] + ma[1] * cmb[1] + ma[2] * cmb[2]);
+                    // standard bending: kx = d thy/dx, ky = -d thx/dy, kxy = d thy/dy - d thx/dx
+                    double ba_[3] = {ca == 4 ? dxa : 0.0, ca == 3 ? -dya : 0.0, ca == 4 ? dya : (ca == 3 ? -dxa : 0.0)};
+                    double bb_[3] = {cb == 4 ? dxb : 0.0, cb == 3 ? -dyb : 0.0, cb == 4 ? dyb : (cb == 3 ? -dxb : 0.0)};
+                    double cbb[3];
+                    mat3v(Cb, bb_, cbb);
+                    kbend += adj * (ba_[0] * cbb[0] + ba_[1] * cbb[1] + ba_[2] * cbb[2]);
+                    // drilling
+                    double da = ca == 0 ? 0.5 * dya : (ca == 1 ? -0.5 * dxa : (ca == 5 ? S.gN[g][ka] : 0.0));
+                    double db = cb == 0 ? 0.5 * dyb : (cb == 1 ? -0.5 * dxb : (cb == 5 ? S.gN[g][kb] : 0.0));
+                    kd += adj * alpha * da * db;
+                    // MITC4 shear: covariant strains interpolated from the tying points
+                    if (ca >= 2 && ca <= 4 && cb >= 2 && cb <= 4) {
+                        const double r = (g == 1 || g == 2) ? PT : -PT, s = (g >= 2) ? PT : -PT;
+                        const int ia = 3 * ka + (ca - 2), ib = 3 * kb + (cb - 2);
+                        const double gra = 0.5 * (1 - s) * S.btp[0][ia] + 0.5 * (1 + s) * S.btp[1][ia];
+                        const double gsa = 0.5 * (1 - r) * S.btp[2][ia] + 0.5 * (1 + r) * S.btp[3][ia];
+                        const double grb = 0.5 * (1 - s) * S.btp[0][ib] + 0.5 * (1 + s) * S.btp[1][ib];
+                        const double gsb = 0.5 * (1 - r) * S.btp[2][ib] + 0.5 * (1 + r) * S.btp[3][ib];
+                        const double i11 = S.giJ[g][0], i12 = S.giJ[g][1], i21 = S.giJ[g][2], i22 = S.giJ[g][3];
+                        const double c11 = tsg * (i11 * i11 + i21 * i21), c12 = tsg * (i11 * i12 + i21 * i22),
+                                     c22 = tsg * (i12 * i12 + i22 * i22);
+                        ks += adj * (gra * (c11 * grb + c12 * gsb) + gsa * (c12 * grb + c22 * gsb));
+                    }
+                }
+                double kc = 0.0;
+                for (int l = 0; l < 4; l++)
+                    for (int m = 0; m < 4; m++) kc += S.Kab[a][l] * S.Kbbi[l * 4 + m] * S.Kab[b][m];
+                blk[i * 3 + j] = (km + kd - kc) / t + 3.0 * kbend / t + S.dphi2t * ks / t;
+            }
+        rot_block(S.R, blk, gbl);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double pa = q4_gvec(A, phi, e, 3 * bi + i), pb = q4_gvec(A, phi, e, 3 * bj + j);
+                double sa = q4_gvec(A, psi, e, 3 * bi + i), ub = q4_gvec(A, u, e, 3 * bj + j);
+                f1 += pa * gbl[i * 3 + j] * pb;
+                f2 += sa * gbl[i * 3 + j] * ub;
+            }
+    }
+    f1 = wave_sum(f1);
+    f2 = wave_sum(f2);
+    if (valid && lane == 0) { terms[3 * e] = f1; terms[3 * e + 2] = f2; }
+}
+
+// Unit-basis adjoint load per element: f_a = phi_l' KG(sigma(e_a)) phi_l for
+// the local unit displacements e_a (zero for w and the rotations), rotated to
+// the global frame and scaled by lambda.
+template <int NM>
+__global__ void __launch_bounds__(256) q4_adjrhs_ref_kernel(ElemArgs A, const double *u, const double *phi,
+                                                            double lam, double *eg /* n_q4 x 24 */) {
+    __shared__ Q4KGStageT<NM> stage[EPB];
+    __shared__ double grd[EPB][4][6];        // gradients of phi per GP
+    __shared__ double dsdu[EPB][4][3][24];   // sigma_gp(e_a)
+    __shared__ double phil[EPB][24];
+    __shared__ double fc[EPB][4][24];        // per-GP parts of f_a
+    int w = threadIdx.x / S105_WAVE, lane = threadIdx.x % S105_WAVE;
+    int e = blockIdx.x * EPB + w;
+    Q4KGStageT<NM> &S = stage[w];
+    double h;
+    bool valid = q4kg_setup(S, A, u, e, lane, h);
+    if (valid && lane < 8) {
+        int b = lane;
+        int nd = A.q4[4 * e + b / 2];
+        double pg[3];
+        for (int i = 0; i < 3; i++) pg[i] = phi[6 * (size_t)nd + 3 * (b % 2) + i];
+        for (int i = 0; i < 3; i++) phil[w][3 * b + i] = S.R[i * 3] * pg[0] + S.R[i * 3 + 1] * pg[1] + S.R[i * 3 + 2] * pg[2];
+    }
+    __syncthreads();
+    if (valid && lane < 4) {
+        int g = lane;
+        double X[6] = {0, 0, 0, 0, 0, 0};
+        for (int a = 0; a < 24; a++) {
+            double gr[6];
+            q4kg_grads(S, g, a, gr);
+            for (int i = 0; i < 6; i++) X[i] += gr[i] * phil[w][a];
+        }
+        for (int i = 0; i < 6; i++) grd[w][g][i] = X[i];
+    }
+    if (valid) {
+        const double *prop = A.q4prop + 6 * e;
+        double E = prop[0], nu = prop[1];
+        double cm = E / (1 - nu * nu);
+        const double D[9] = {1.0, nu, 0.0, nu, 1.0, 0.0, 0.0, 0.0, (1 - nu) / 2};
+        double Cm[9];
+        for (int i = 0; i < 9; i++) Cm[i] = (cm * D[i]) * h;
+        for (int q = lane; q < 96; q += S105_WAVE) {
+            int g = q / 24, a = q % 24;
+            int k = a / 6, cpt = a % 6;
+            double dx = S.gdx[g][k], dy = S.gdy[g][k];
+            double eps[3] = {(cpt == 0) ? dx : 0.0, (cpt == 1) ? dy : 0.0, (cpt == 0) ? dy : ((cpt == 1) ? dx : 0.0)};
+            for (int m = 0; m < NM; m++) {
+                double bm[3];
+                q4kg_mode_col(S, g, m, bm);
+                const double am = S.Am[m][a];
+                eps[0] += bm[0] * am;
+                eps[1] += bm[1] * am;
+                eps[2] += bm[2] * am;
+            }
+            double N[3];
+            mat3v(Cm, eps, N);
+            for (int i = 0; i < 3; i++) dsdu[w][g][i][a] = N[i] / h;
+        }
+    }
+    __syncthreads();
+    if (valid)
+        for (int q = lane; q < 96; q += S105_WAVE) {
+            int g = q / 24, a = q % 24;
+            double pr[6];
+            principal2d(dsdu[w][g][0][a], dsdu[w][g][1][a], dsdu[w][g][2][a], pr);
+            const double *X = grd[w][g];
+            double val = 0.0;
+            for (int k = 0; k < 2; k++) {
+                const double l = pr[3 * k], c = pr[3 * k + 1], sn = pr[3 * k + 2];
+                if (fabs(l) <= 1e-30) continue;
+                const double gx = c * X[0] + sn * X[1], gy = c * X[2] + sn * X[3], gz = c * X[4] + sn * X[5];
+                const double av = sn * gx - c * gy;
+                val += l * (av * av + gz * gz);
+            }
+            fc[w][g][a] = h * S.gadj[g] * val;
+        }
+    __syncthreads();
+    if (valid && lane < 8) {
+        int b = lane;
+        double gl[3];
+        for (int i = 0; i < 3; i++) {
+            int a = 3 * b + i;
+            gl[i] = fc[w][0][a] + fc[w][1][a] + fc[w][2][a] + fc[w][3][a];
         }
         for (int j = 0; j < 3; j++) {
             double v = S.R[j] * gl[0] + S.R[3 + j] * gl[1] + S.R[6 + j] * gl[2];
@@ -2157,7 +2503,20 @@ void launch_q4_sens_terms(const ElemArgs &a, const double *u, const double *phi,
                           double *terms /* n_q4 x 3 */, hipStream_t s) {
     if (a.n_q4 == 0) return;
     q4_dke_kernel<<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, psi, terms);
-    q4_kgt_kernel<<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, terms);
+    q4_kgt_kernel<4><<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, terms);
+}
+void launch_q4_sens_terms_ref(const ElemArgs &a, const double *u, const double *phi, const double *psi,
+                              bool six_modes, double *terms /* n_q4 x 3 */, hipStream_t s) {
+    if (a.n_q4 == 0) return;
+    q4_dke_ref_kernel<<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, psi, terms);
+    if (six_modes) q4_kgt_kernel<6><<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, terms);
+    else q4_kgt_kernel<4><<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, terms);
+}
+void launch_q4_adjoint_rhs_ref(const ElemArgs &a, const double *u, const double *phi, double lam, bool six_modes,
+                               double *eg, hipStream_t s) {
+    if (a.n_q4 == 0) return;
+    if (six_modes) q4_adjrhs_ref_kernel<6><<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, lam, eg);
+    else q4_adjrhs_ref_kernel<4><<<(a.n_q4 + EPB - 1) / EPB, 256, 0, s>>>(a, u, phi, lam, eg);
 }
 void launch_sens_combine(int n_q4, const double *terms, double lam, double phiKGphi, double *dldt, hipStream_t s) {
     if (n_q4 == 0) return;

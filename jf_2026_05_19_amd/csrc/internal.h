@@ -30,6 +30,7 @@ struct sol105_ctx {
     int lanczos_max_restarts = 40;   // SOL105_OPT_LANCZOS_MAX_RESTARTS
     bool warm_start = false;         // SOL105_OPT_WARM_START
     bool graphs = true;              // SOL105_OPT_STEP_GRAPHS
+    int sens_mode = 0;               // SOL105_OPT_SENS_MODE
     ProdTimer ptimer;  // HIP-event timing of the block triangular-solve products
     ProdTimer *timer() { return profiling ? &ptimer : nullptr; }
 };

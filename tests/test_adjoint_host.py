@@ -44,11 +44,23 @@ def test_design_variable_values_and_unsupported_types():
         A.design_variable_values({"id": "e", "type": "material_E", "mids": [1]}, m)
 
 
+class _Ctx:
+    OPT_SENS_MODE = 5
+
+    def __init__(self):
+        self.modes = []
+
+    def set_option(self, opt, value):
+        assert opt == self.OPT_SENS_MODE
+        self.modes.append(value)
+
+
 class _Stub:
     """Returns fixed per-shell terms and phi' KG phi for every mode."""
 
     def __init__(self, arr, terms, pkp):
         self.arr, self.terms, self.pkp = arr, terms, pkp
+        self.ctx = _Ctx()
 
     def sensitivity_terms(self, mode):
         return self.terms[mode], None, self.pkp[mode]
@@ -74,3 +86,29 @@ def test_group_formula_and_layout():
             assert abs(s["PID_%d" % p] - per_elem[arr.q4_pid == p].sum()) <= 1e-12 * np.abs(per_elem).sum()
         # the groups partition the shells: their sum is the all-shell derivative
         assert abs(s["PID_1"] + s["PID_2"] - per_elem.sum()) <= 1e-12 * np.abs(per_elem).sum()
+
+
+def test_reference_mode_whole_kg_shortcut():
+    """Reference mode: a design variable that is the only PSHELL of a model
+    without bars takes phi' KG phi / T for its dKG term (the reference's
+    _dKg_dx_phi_thickness shortcut); two PSHELLs keep the element sums.  The
+    context's sensitivity mode is set for the terms and restored after."""
+    m = bdf.bdf_to_model_text(decks.plate_deck(nx=4, ny=3, a=200.0, b=150.0, t=1.25, nd=2))
+    arr = M.build_arrays(m)
+    rng = np.random.default_rng(5)
+    terms = [rng.normal(size=(arr.n_shell, 3))]
+    pkp = [-3.0]
+    lam = np.array([7.0])
+    cfg1 = {"responses": [{"id": "b", "type": "buckling"}],
+            "design_variables": [{"id": "t", "type": "shell_thickness", "pids": [1]}]}
+    stub = _Stub(arr, terms, pkp)
+    res = A.solve_adjoint_buckling(stub, lam, m, cfg1, mode="reference")
+    assert stub.ctx.modes == [1, 0]
+    t = terms[0].sum(axis=0)
+    expect = -(t[0] + lam[0] * pkp[0] / 1.25 - t[2]) / pkp[0]
+    assert abs(res["sensitivities"]["mode_1"]["t"]["PID_1"] - expect) <= 1e-12 * abs(expect)
+    exact = A.solve_adjoint_buckling(stub, lam, m, cfg1)
+    assert abs(exact["sensitivities"]["mode_1"]["t"]["PID_1"] - (-(t[0] + lam[0] * t[1] - t[2]) / pkp[0])) <= \
+        1e-12 * abs(expect)
+    with pytest.raises(ValueError):
+        A.solve_adjoint_buckling(stub, lam, m, cfg1, mode="legacy")
