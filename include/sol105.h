@@ -121,7 +121,11 @@ const char *sol105_last_error(const sol105_ctx *ctx);
  *                                    block index from the second solve of a
  *                                    model (or batch) with unchanged buffers
  *                                    and launch shapes.  Results are bitwise
- *                                    those of direct launches.  No
+ *                                    those of direct launches.  The
+ *                                    environment variable SOL105_STEP_GRAPHS=0
+ *                                    makes 0 the default of new contexts
+ *                                    (kernel-trace profilers that do not
+ *                                    follow stream capture).  No
  *                                    counterpart in the reference.
  *   SOL105_OPT_SENS_MODE             0 or 1 (default 0): the CQUAD4 terms of
  *                                    sol105_sensitivity_thickness /
@@ -286,6 +290,19 @@ int sol105_sensitivity_terms(sol105_ctx *ctx, sol105_model *model, int32_t mode,
  *   out[6], out[7]: ms and bytes of the heights above it. */
 int sol105_profile_enable(sol105_ctx *ctx, int32_t on);
 int sol105_profile_read(sol105_ctx *ctx, double *out /* 8 */);
+
+/* Diagnostics: the projected (Rayleigh-Ritz) eigensolver of the block
+ * Lanczos iteration on a caller's symmetric matrix.  No reference
+ * counterpart (the reference's projected problems are KrylovKit's); for
+ * tests of the device eigensolver.  A: m x m, column-major, host memory
+ * (1 <= m <= 1024); the k (1 <= k <= min(m, 64)) pairs of largest |theta|
+ * (algebraic != 0: largest theta) into theta_out (k, descending) and z_out
+ * (m x k, column-major).  bw: 0 for a dense matrix, 4..16 to declare a
+ * half-bandwidth (the band-reduction path, which falls back to the dense one
+ * when the band claim fails).  SOL105_E_NOCONV when the solver reports
+ * failure. */
+int sol105_symeig_top(sol105_ctx *ctx, int32_t m, const double *a, int32_t k, int32_t bw, int32_t algebraic,
+                      double *theta_out, double *z_out);
 
 /* Batched solves.  A batch stacks independent models (design variants,
  * queued requests) into one block-diagonal system: element kernels,

@@ -26,7 +26,7 @@ EXPORTS = [
     "sol105_model_set_q4_thickness", "sol105_model_set_shell_thickness", "sol105_pattern", "sol105_assemble_K",
     "sol105_assemble_K_noalloc", "sol105_static_solve",
     "sol105_assemble_KG", "sol105_buckling", "sol105_buckling_info", "sol105_solve", "sol105_sensitivity_thickness",
-    "sol105_sensitivity_terms", "sol105_profile_enable", "sol105_profile_read", "sol105_last_timings",
+    "sol105_sensitivity_terms", "sol105_profile_enable", "sol105_profile_read", "sol105_symeig_top", "sol105_last_timings",
     "sol105_last_stats", "sol105_batch_create", "sol105_batch_free", "sol105_batch_solve", "sol105_batch_modes",
     "sol105_batch_stats", "sol105_server_create", "sol105_server_destroy", "sol105_server_last_error",
     "sol105_server_add_model", "sol105_server_submit", "sol105_server_submit_ex", "sol105_server_poll",
@@ -99,6 +99,7 @@ def lib():
         L.sol105_sensitivity_terms.argtypes = [vp, vp, ctypes.c_int32, dp, dp, dp]
         L.sol105_profile_enable.argtypes = [vp, ctypes.c_int32]
         L.sol105_profile_read.argtypes = [vp, dp]
+        L.sol105_symeig_top.argtypes = [vp, ctypes.c_int32, dp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, dp, dp]
         L.sol105_last_timings.argtypes = [vp, dp]
         L.sol105_last_stats.argtypes = [vp, dp]
         ip = ctypes.POINTER(ctypes.c_int32)
@@ -233,6 +234,17 @@ class Context:
         """Context options (SOL105_OPT_*): basis size before a thick restart,
         thick restarts before SOL105_E_NOCONV, warm starts, step graphs."""
         self.check(self.L.sol105_ctx_set_option(self.h, int(option), float(value)))
+
+    def symeig_top(self, A, k, bw=0, algebraic=False):
+        """(theta (k,), Z (m, k)) of the projected eigensolver on a symmetric
+        matrix A (diagnostics, sol105_symeig_top)."""
+        A = np.asfortranarray(np.asarray(A, dtype=np.float64))
+        m = A.shape[0]
+        th = np.zeros(k)
+        Z = np.zeros((k, m))
+        self.check(self.L.sol105_symeig_top(self.h, int(m), A.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(k),
+                                            int(bw), 1 if algebraic else 0, _dp(th), _dp(Z)))
+        return th, Z.T.copy()
 
     def profile_enable(self, on=True):
         self.check(self.L.sol105_profile_enable(self.h, 1 if on else 0))

@@ -374,6 +374,10 @@ int sol105_ctx_create(int device, sol105_ctx **out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SOL105_E_DEVICE;
     if (device < 0 || device >= ndev) return SOL105_E_INVALID;
     ctx->device = device;
+    // SOL105_STEP_GRAPHS=0 in the environment: new contexts launch the
+    // Lanczos steps directly (for kernel-trace profilers that do not follow
+    // stream capture)
+    if (const char *g = std::getenv("SOL105_STEP_GRAPHS")) ctx->graphs = std::strcmp(g, "0") != 0;
     if (hipSetDevice(device) != hipSuccess) return SOL105_E_DEVICE;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return SOL105_E_DEVICE;
     // device address of the no-allocation scratch array, once per device; a
@@ -1517,6 +1521,36 @@ int sol105_profile_enable(sol105_ctx *ctx, int32_t on) {
     (void)hipSetDevice(ctx->device);
     ctx->ptimer.reset();
     ctx->profiling = on != 0;
+    return SOL105_OK;
+}
+
+int sol105_symeig_top(sol105_ctx *ctx, int32_t m, const double *a, int32_t k, int32_t bw, int32_t algebraic,
+                      double *theta_out, double *z_out) {
+    if (!ctx || !a || !theta_out || !z_out) return set_err(ctx, SOL105_E_INVALID, "null argument");
+    if (m < 1 || m > SYMEIG_MAX_M) return set_err(ctx, SOL105_E_INVALID, "m = %d out of range 1..%d", m, SYMEIG_MAX_M);
+    if (k < 1 || k > std::min<int>(m, SYMEIG_MAX_K))
+        return set_err(ctx, SOL105_E_INVALID, "k = %d out of range 1..%d", k, std::min<int>(m, SYMEIG_MAX_K));
+    if (bw != 0 && (bw < 4 || bw > SYMEIG_BAND_MAX_W))
+        return set_err(ctx, SOL105_E_INVALID, "bw = %d: 0 or 4..%d", bw, SYMEIG_BAND_MAX_W);
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    DevBuf<double> A, th, Z, work;
+    DevBuf<int> info;
+    HIPCHK(A.upload(a, (size_t)m * m, s));
+    HIPCHK(th.alloc(k));
+    HIPCHK(Z.alloc((size_t)m * k));
+    const int64_t wn = symeig_work_size(m, k);
+    HIPCHK(work.alloc(wn));
+    HIPCHK(info.alloc(1));
+    HIPCHK(hipMemsetAsync(info.p, 0, sizeof(int), s));
+    sym_eig_top(m, A.p, k, th.p, Z.p, work.p, wn, info.p, s, bw, algebraic);
+    HIPCHK(hipGetLastError());
+    int hinfo = 0;
+    HIPCHK(hipMemcpyAsync(theta_out, th.p, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(z_out, Z.p, sizeof(double) * m * k, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hinfo != 0) return set_err(ctx, SOL105_E_NOCONV, "projected eigensolver failed (info %d)", hinfo);
     return SOL105_OK;
 }
 
