@@ -63,6 +63,21 @@ typedef struct sol105_model_desc {
     double eigrl_v2;            /*   is non-zero (blank fields: 0)                */
     int32_t n_shell_cards;      /* shell cards of the deck (0: n_q4 + n_t3); the  */
                                 /*   mode filters act on decks with >= 100        */
+    /* Rigid and interpolation constraints, eliminated as dependent DOFs
+     * (JFEM/src/solver/constraints.jl:37-364): RBE2 (and RBAR, converted
+     * to RBE2 as ModelBuilder.jl:708-725 does) as master-slave pairs, RBE3
+     * as a reference grid with weight groups.  Component sets are the
+     * Nastran digit strings read as integers (123456).  All counts may be 0. */
+    int32_t n_rbe2;             /* master-slave pairs                             */
+    const int32_t *rbe2;        /* n_rbe2 x 3: master rank, CM, slave rank        */
+    int32_t n_rbe3;             /* RBE3 elements                                  */
+    const int32_t *rbe3;        /* n_rbe3 x 4: REFGRID rank, REFC, first group,   */
+                                /*   group count                                  */
+    int32_t n_rbe3_grp;         /* weight groups                                  */
+    const int32_t *rbe3_grp;    /* n_rbe3_grp x 3: components, first node, count  */
+    const double *rbe3_wt;      /* n_rbe3_grp weights                             */
+    int32_t n_rbe3_node;
+    const int32_t *rbe3_node;   /* node ranks of the groups                       */
 } sol105_model_desc;
 
 /* Native deck reader: Nastran deck text (executive and case control, bulk

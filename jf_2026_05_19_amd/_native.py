@@ -61,6 +61,11 @@ class ModelDesc(ctypes.Structure):
         ("node_has_shell", ctypes.POINTER(ctypes.c_uint8)), ("node_has_bar", ctypes.POINTER(ctypes.c_uint8)),
         ("k6rot", ctypes.c_double), ("autospc", ctypes.c_int32), ("nd", ctypes.c_int32),
         ("eigrl_v1", ctypes.c_double), ("eigrl_v2", ctypes.c_double), ("n_shell_cards", ctypes.c_int32),
+        ("n_rbe2", ctypes.c_int32), ("rbe2", ctypes.POINTER(ctypes.c_int32)),
+        ("n_rbe3", ctypes.c_int32), ("rbe3", ctypes.POINTER(ctypes.c_int32)),
+        ("n_rbe3_grp", ctypes.c_int32), ("rbe3_grp", ctypes.POINTER(ctypes.c_int32)),
+        ("rbe3_wt", ctypes.POINTER(ctypes.c_double)),
+        ("n_rbe3_node", ctypes.c_int32), ("rbe3_node", ctypes.POINTER(ctypes.c_int32)),
     ]
 
 
@@ -191,7 +196,11 @@ def read_deck(text, buckling_index=0, with_sol=False):
             autospc=bool(d.autospc), node_has_shell=arr(d.node_has_shell, n, u8, (n,)),
             node_has_bar=arr(d.node_has_bar, n, u8, (n,)),
             buckling_subcase=int(sz[5]), static_subcase=int(sz[6]), shell_cards=int(sz[7]),
-            bar_points=bar_points)
+            bar_points=bar_points,
+            rbe2=arr(d.rbe2, 3 * d.n_rbe2, i32, (d.n_rbe2, 3)), rbe3=arr(d.rbe3, 4 * d.n_rbe3, i32, (d.n_rbe3, 4)),
+            rbe3_grp=arr(d.rbe3_grp, 3 * d.n_rbe3_grp, i32, (d.n_rbe3_grp, 3)),
+            rbe3_wt=arr(d.rbe3_wt, d.n_rbe3_grp, dbl, (d.n_rbe3_grp,)),
+            rbe3_node=arr(d.rbe3_node, d.n_rbe3_node, i32, (d.n_rbe3_node,)))
         return (arrays, int(sz[8])) if with_sol else arrays
     finally:
         L.sol105_deck_free(h)
@@ -285,6 +294,11 @@ def _model_desc(arr):
         F=np.ascontiguousarray(arr.F, dtype=np.float64),
         hs=np.ascontiguousarray(arr.node_has_shell, dtype=np.uint8),
         hb=np.ascontiguousarray(arr.node_has_bar, dtype=np.uint8),
+        rbe2=np.ascontiguousarray(arr.rbe2, dtype=np.int32).reshape(-1),
+        rbe3=np.ascontiguousarray(arr.rbe3, dtype=np.int32).reshape(-1),
+        rbe3_grp=np.ascontiguousarray(arr.rbe3_grp, dtype=np.int32).reshape(-1),
+        rbe3_wt=np.ascontiguousarray(arr.rbe3_wt, dtype=np.float64),
+        rbe3_node=np.ascontiguousarray(arr.rbe3_node, dtype=np.int32),
     )
     k = keep
     ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
@@ -293,7 +307,9 @@ def _model_desc(arr):
                   arr.n_t3, ip(k["t3"]), _dp(k["t3prop"]), arr.n_bar, ip(k["bar"]), _dp(k["vref"]), _dp(k["barprop"]),
                   up(k["spc"]), _dp(k["F"]), up(k["hs"]), up(k["hb"]),
                   float(arr.k6rot), int(arr.autospc), int(arr.nd),
-                  float(arr.v1), float(arr.v2), int(arr.n_shell_cards))
+                  float(arr.v1), float(arr.v2), int(arr.n_shell_cards),
+                  len(arr.rbe2), ip(k["rbe2"]), len(arr.rbe3), ip(k["rbe3"]), len(arr.rbe3_grp), ip(k["rbe3_grp"]),
+                  _dp(k["rbe3_wt"]), len(arr.rbe3_node), ip(k["rbe3_node"]))
     return d, keep
 
 

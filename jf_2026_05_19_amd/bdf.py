@@ -3,7 +3,7 @@
 Builds the same model dictionary shape as the reference's ModelBuilder
 (``JFEM/src/ModelBuilder.jl:421-823``) for the cards this path supports:
 GRID, CQUAD4, CTRIA3, CBAR, CBEAM, PSHELL, PBARL, MAT1, SPC1, SPC, FORCE,
-EIGRL, PARAM and the case-control keys SOL / SUBCASE / SPC / LOAD / METHOD /
+EIGRL, PARAM, RBE2, RBE3, RBAR and the case-control keys SOL / SUBCASE / SPC / LOAD / METHOD /
 STATSUB.  Field parsing follows ``JFEM/src/parsing/utilities.jl`` (small and
 large fixed fields, free-field commas, ``+``/``*`` continuations, Nastran
 exponent shorthand such as ``1.-5``).
@@ -13,7 +13,11 @@ Property and material extraction restate
 membrane only), ``:40-190`` (PBARL BAR section constants), ``:540-560``
 (Cowper shear factor for a solid rectangle with nu = 0.3),
 ``JFEM/src/parsing/extract_materials.jl:3-27`` (MAT1 completion) and
-``:465-480`` (EIGRL, ND defaults to 3).
+``:465-480`` (EIGRL, ND defaults to 3).  RBE2, RBE3 and RBAR follow
+``JFEM/src/parsing/extract_constraints.jl:46-141`` (RBE3 weight groups: a
+field with '.', an exponent or an inner sign starts a new group; continuation
+markers are skipped) and RBAR becomes an RBE2 as ``ModelBuilder.jl:708-725``
+converts it.
 """
 from __future__ import annotations
 
@@ -159,6 +163,71 @@ def _f(card, i, default=None):
     return parse_number(card[i], default) if i < len(card) else default
 
 
+def _to_id(v):
+    """to_id(parse_nastran_number(v)): integers as they are, reals rounded,
+    anything else 0 (JFEM/src/parsing/utilities.jl:8-27)."""
+    x = parse_number(v, None)
+    if x is None:
+        return 0
+    if isinstance(x, float):
+        if abs(x) >= 0.5 and abs(x - round(x)) < 1e-8:
+            return int(round(x))
+        return int(round(x))
+    return int(x)
+
+
+def _is_cont(sv):
+    sv = sv.strip()
+    return bool(sv) and sv[0] in "+*"
+
+
+def _is_real_field(sv):
+    sv = sv.strip()
+    if not sv or _is_cont(sv):
+        return False
+    return "." in sv or re.search(r"[EeDd]", sv) is not None or (len(sv) > 1 and re.search(r"[+-]", sv[1:]) is not None)
+
+
+def _rbe3_groups(c):
+    """Weight groups of an RBE3 card from its fifth field on
+    (extract_constraints.jl:88-122)."""
+    groups, pos, n = [], 4, len(c)
+    while pos < n:
+        wt_str = str(c[pos]).strip()
+        if not wt_str or _is_cont(wt_str):
+            pos += 1
+            continue
+        wt = parse_number(wt_str, None)
+        if wt is None:
+            break
+        pos += 1
+        while pos < n:
+            cs = str(c[pos]).strip()
+            if cs and not _is_cont(cs):
+                break
+            pos += 1
+        if pos >= n:
+            break
+        comps = _to_id(c[pos])
+        pos += 1
+        grids = []
+        while pos < n:
+            g = str(c[pos]).strip()
+            if not g or _is_cont(g):
+                pos += 1
+                continue
+            if _is_real_field(g):
+                break
+            gid = parse_number(g, None)
+            if gid is None:
+                break
+            grids.append(int(gid))
+            pos += 1
+        if grids:
+            groups.append({"wt": float(wt), "comps": comps, "grids": grids})
+    return groups
+
+
 def _pbarl_bar(dims, nu=0.3):
     """PBARL TYPE=BAR section constants (extract_properties.jl:109-121, 554-557)."""
     w, h = dims[0], dims[1]
@@ -295,8 +364,29 @@ def cards_to_model(cards, cc):
         val = str(c[1]).strip() if len(c) > 1 else ""
         num = parse_number(val, None)
         model["PARAM_" + key] = num if num is not None else val
+    model["RBE2s"], model["RBE3s"] = {}, {}
+    for c in cards.get("RBE2", []):
+        eid, gn, cm = _to_id(c[0] if c else ""), _to_id(c[1] if len(c) > 1 else ""), _to_id(c[2] if len(c) > 2 else "")
+        slaves = [g for g in (_to_id(v) for v in c[3:]) if g > 0]
+        if eid > 0 and gn > 0:
+            model["RBE2s"][str(eid)] = {"ID": eid, "GN": gn, "CM": cm, "GM": slaves}
+    for c in cards.get("RBAR", []):
+        f = [_to_id(c[k]) if k < len(c) else 0 for k in range(7)]
+        eid, ga, gb, cna, cnb = f[0], f[1], f[2], f[3], f[4]
+        if eid > 0 and ga > 0 and gb > 0:
+            if cnb > 0 and cna == 0:
+                master, slave, cm = gb, ga, cnb
+            elif cna > 0 and cnb == 0:
+                master, slave, cm = ga, gb, cna
+            else:
+                master, slave, cm = ga, gb, 123456
+            model["RBE2s"][str(eid)] = {"ID": eid, "GN": master, "CM": cm, "GM": [slave]}
+    for c in cards.get("RBE3", []):
+        eid = _to_id(c[0] if c else "")
+        model["RBE3s"][str(eid)] = {"ID": eid, "REFGRID": _to_id(c[2] if len(c) > 2 else ""),
+                                    "REFC": _to_id(c[3] if len(c) > 3 else ""), "WT_GROUPS": _rbe3_groups(c)}
     unsupported = sorted(set(cards) - {"GRID", "CQUAD4", "CTRIA3", "CBAR", "CBEAM", "PSHELL", "PBARL",
-                                       "MAT1", "SPC1", "SPC", "FORCE", "EIGRL", "PARAM"})
+                                       "MAT1", "SPC1", "SPC", "FORCE", "EIGRL", "PARAM", "RBE2", "RBE3", "RBAR"})
     model["_UNSUPPORTED_CARDS"] = unsupported
     return model
 

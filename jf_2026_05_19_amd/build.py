@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsol105_hip.so")
 SOURCES = ["elements.hip", "factor.hip", "solve.hip", "dense.hip", "rr_eig.hip", "symbolic.cpp", "solve_plan.cpp",
-           "api.cpp", "select.cpp", "recover.cpp", "tune.cpp", "deck.cpp", "batch.cpp", "server.cpp", "noalloc.hip"]
+           "api.cpp", "constraints.cpp", "select.cpp", "recover.cpp", "tune.cpp", "deck.cpp", "batch.cpp", "server.cpp", "noalloc.hip"]
 ARCH = os.environ.get("SOL105_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-pthread", "-Wno-pass-failed",

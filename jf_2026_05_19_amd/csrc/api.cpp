@@ -334,10 +334,45 @@ int model_init(sol105_ctx *ctx, const sol105_model_desc *d, sol105_model *M) {
         const double *p = &M->t3prop[6 * (size_t)e];
         if (!(p[2] > 0.0) || !(p[0] > 0.0)) return set_err(ctx, SOL105_E_INVALID, "CTRIA3 %d: T and E must be positive", e);
     }
-    build_block_pattern(n, M->n_q4, M->q4.data(), M->n_t3, M->t3.data(), M->n_bar, M->bar.data(), M->pat);
+    {
+        std::string cerr;
+        int crc = build_constraint_map(d, M->mpc, cerr);
+        if (crc) return set_err(ctx, crc, "%s", cerr.c_str());
+    }
+    ConstraintPlan cplan;
+    if (M->has_mpc()) {
+        // the element pattern, then the pattern with the constraint couplings
+        BlockPattern elem;
+        build_block_pattern(n, M->n_q4, M->q4.data(), M->n_t3, M->t3.data(), M->n_bar, M->bar.data(), elem);
+        std::vector<std::pair<int32_t, int32_t>> extra;
+        constraint_pattern_pairs(M->mpc, elem, n, extra);
+        build_block_pattern(n, M->n_q4, M->q4.data(), M->n_t3, M->t3.data(), M->n_bar, M->bar.data(), M->pat,
+                            &extra);
+        build_constraint_plan(M->mpc, elem, M->pat, n, cplan);
+        // dependent DOFs are fixed; their loads move to the independent DOFs
+        for (int32_t dd : M->mpc.dep) M->spc_mask[dd] = 1;
+        redistribute_force(M->mpc, M->force);
+    } else {
+        build_block_pattern(n, M->n_q4, M->q4.data(), M->n_t3, M->t3.data(), M->n_bar, M->bar.data(), M->pat);
+    }
     M->diag_blk.resize(n);
     for (int v = 0; v < n; v++) M->diag_blk[v] = (int32_t)M->pat.find(v, v);
     hipStream_t s = ctx->stream;
+    if (M->has_mpc()) {
+        const ConstraintMap &C = M->mpc;
+        HIPCHK(M->d_mpc_dep.upload(C.dep.data(), C.dep.size(), s));
+        HIPCHK(M->d_mpc_ptr.upload(C.ptr.data(), C.ptr.size(), s));
+        HIPCHK(M->d_mpc_ind.upload(C.ind.data(), std::max<size_t>(C.ind.size(), 1), s));
+        HIPCHK(M->d_mpc_coef.upload(C.coef.data(), std::max<size_t>(C.coef.size(), 1), s));
+        M->mpc_nt = (int64_t)cplan.tgt.size();
+        M->mpc_nz = (int64_t)cplan.zero.size();
+        HIPCHK(M->d_mpc_tgt.upload(cplan.tgt.data(), cplan.tgt.size(), s));
+        HIPCHK(M->d_mpc_tptr.upload(cplan.tptr.data(), cplan.tptr.size(), s));
+        HIPCHK(M->d_mpc_src.upload(cplan.src.data(), cplan.src.size(), s));
+        HIPCHK(M->d_mpc_tcoef.upload(cplan.coef.data(), cplan.coef.size(), s));
+        HIPCHK(M->d_mpc_zero.upload(cplan.zero.data(), cplan.zero.size(), s));
+        HIPCHK(hipStreamSynchronize(s));  // cplan is a host temporary
+    }
     HIPCHK(M->d_xyz.upload(M->xyz.data(), M->xyz.size(), s));
     HIPCHK(M->d_q4.upload(M->q4.data(), M->q4.size(), s));
     HIPCHK(M->d_q4prop.upload(M->q4prop.data(), M->q4prop.size(), s));
@@ -497,6 +532,9 @@ int sol105_assemble_K(sol105_ctx *ctx, sol105_model *M, double *vals) {
     launch_t3_ke(a, M->d_eb.p, s);
     launch_bar_ke(a, M->d_eb.p, s);
     launch_block_gather(M->pat.n_blocks(), M->d_cptr.p, M->d_cidx.p, M->d_eb.p, M->d_K.p, s);
+    if (M->has_mpc())
+        launch_mpc_apply(M->mpc_nt, M->d_mpc_tgt.p, M->d_mpc_tptr.p, M->d_mpc_src.p, M->d_mpc_tcoef.p, M->mpc_nz,
+                         M->d_mpc_zero.p, M->d_K.p, s);
     HIPCHK(hipGetLastError());
     M->K_valid = true;
     M->factor_valid = M->u_valid = M->KG_valid = false;
@@ -718,6 +756,9 @@ int sol105_static_solve(sol105_ctx *ctx, sol105_model *M, double *u, uint8_t *fr
     launch_gather_elim(nfree, ndof, 1, M->d_dof_of_elim.p, M->d_force.p, X, s);
     const hipError_t te = tri_solve(M, X, 1, s);
     if (te == hipSuccess) launch_scatter_elim(nfree, ndof, 1, M->d_elim_of_dof.p, X, M->d_u.p, s);
+    if (te == hipSuccess && M->has_mpc())
+        launch_mpc_recover((int)M->mpc.dep.size(), M->d_mpc_dep.p, M->d_mpc_ptr.p, M->d_mpc_ind.p, M->d_mpc_coef.p,
+                           M->d_u.p, ndof, 1, s);
     HIPCHK(hipFreeAsync(X, s));
     HIPCHK(te);
     HIPCHK(hipGetLastError());
@@ -743,6 +784,10 @@ int sol105_assemble_KG(sol105_ctx *ctx, sol105_model *M, const double *u, double
     launch_t3_kg(a, M->d_u.p, M->d_eb.p, s);
     launch_bar_kg(a, M->d_u.p, M->d_eb.p, s);
     launch_block_gather(M->pat.n_blocks(), M->d_cptr.p, M->d_cidx.p, M->d_eb.p, M->d_KG.p, s);
+    // KG takes the same constraint redistribution as K (assembly.jl:6617-6620)
+    if (M->has_mpc())
+        launch_mpc_apply(M->mpc_nt, M->d_mpc_tgt.p, M->d_mpc_tptr.p, M->d_mpc_src.p, M->d_mpc_tcoef.p, M->mpc_nz,
+                         M->d_mpc_zero.p, M->d_KG.p, s);
     HIPCHK(hipGetLastError());
     M->KG_valid = true;
     if (vals) {
@@ -1365,6 +1410,10 @@ int sol105_buckling(sol105_ctx *ctx, sol105_model *M, int32_t max_modes, double 
         M->lambda[t] = lam[i];
         HIPCHK(hipMemcpyAsync(M->d_modes.p + (size_t)t * ndof, Xf.p + (size_t)i * ndof, ndof * sizeof(double),
                               hipMemcpyDeviceToDevice, s));
+        // dependent DOFs of the mode before its normalisation (solve_case.jl:3153-3162)
+        if (M->has_mpc())
+            launch_mpc_recover((int)M->mpc.dep.size(), M->d_mpc_dep.p, M->d_mpc_ptr.p, M->d_mpc_ind.p,
+                               M->d_mpc_coef.p, M->d_modes.p + (size_t)t * ndof, ndof, 1, s);
         normalize_maxabs(ndof, M->d_modes.p + (size_t)t * ndof, scr.p, s);
     }
     M->n_modes = nout;

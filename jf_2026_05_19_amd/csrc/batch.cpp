@@ -579,6 +579,8 @@ int sol105_batch_create(sol105_ctx *ctx, const sol105_model_desc *mem, int32_t n
             return set_err(ctx, SOL105_E_INVALID, "batch member %d: invalid sizes", i);
         if (d.k6rot != mem[0].k6rot || (d.autospc != 0) != (mem[0].autospc != 0))
             return set_err(ctx, SOL105_E_UNSUPPORTED, "batch members must share PARAM K6ROT and AUTOSPC");
+        if (d.n_rbe2 > 0 || d.n_rbe3 > 0)
+            return set_err(ctx, SOL105_E_UNSUPPORTED, "batch member %d: RBE2/RBE3 constraints are solved alone", i);
         B->node_off[i + 1] = B->node_off[i] + d.n_nodes;
         B->q4_off[i + 1] = B->q4_off[i] + d.n_q4;
         B->t3_off[i + 1] = B->t3_off[i] + d.n_t3;

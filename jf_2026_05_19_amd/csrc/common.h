@@ -37,6 +37,13 @@ void launch_bar_ke(const ElemArgs &a, double *eb, hipStream_t s);
 void launch_bar_kg(const ElemArgs &a, const double *u, double *eb, hipStream_t s);
 void launch_block_gather(int64_t n_blocks, const int32_t *cptr, const int32_t *cidx, const double *eb,
                          double *vals, hipStream_t s);
+// constraint elimination: K'[tgt] += coef K[src] (CSR by target), then the
+// dependent entries zpos are cleared; dependent DOFs of ncols columns of x
+// (leading dimension ld) recovered from the independent ones
+void launch_mpc_apply(int64_t nt, const int64_t *tgt, const int32_t *tptr, const int64_t *src, const double *coef,
+                      int64_t nz, const int64_t *zpos, double *K, hipStream_t s);
+void launch_mpc_recover(int ndep, const int32_t *dep, const int32_t *ptr, const int32_t *ind, const double *coef,
+                        double *x, int64_t ld, int ncols, hipStream_t s);
 
 // SOL 101 result recovery (elements.hip); outputs component-major
 // (column c of element e at out[c * n + e]): CQUAD4 44, CTRIA3 20, CBAR 18

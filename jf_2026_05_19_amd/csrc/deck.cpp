@@ -13,7 +13,8 @@
 //     the Nastran exponent shorthand (1.-5);
 //   * GRID, CQUAD4, CTRIA3, CBAR, CBEAM, PSHELL (blank MID2: membrane only),
 //     PBARL TYPE=BAR (section constants, Cowper shear factor at nu = 0.3),
-//     MAT1 (completion of E, G, nu), SPC1 (THRU), SPC, FORCE, EIGRL, PARAM;
+//     MAT1 (completion of E, G, nu), SPC1 (THRU), SPC, FORCE, EIGRL, PARAM,
+//     RBE2, RBE3 and RBAR (as an RBE2);
 //   * case control SOL / SUBCASE / key = value, global keys inherited by the
 //     subcases, the buckling and static subcase pairing (METHOD / STATSUB /
 //     LOAD / SPC);
@@ -44,6 +45,8 @@
 
 struct sol105_deck {
     std::vector<int64_t> node_ids, q4_eid, q4_pid, t3_eid, t3_pid, bar_eid;
+    std::vector<int32_t> rbe2, rbe3, rbe3_grp, rbe3_node;  // constraint arrays (sol105_model_desc)
+    std::vector<double> rbe3_wt;
     std::vector<double> xyz, q4prop, t3prop, bar_vref, barprop, bar_points, force;
     std::vector<int32_t> q4, t3, bar;
     std::vector<uint8_t> spc_mask, has_shell, has_bar;
@@ -461,7 +464,8 @@ void sort_last_wins(std::vector<std::pair<int64_t, T>> &v) {
     size_t w = 0;
     for (size_t i = 0; i < v.size(); i++) {
         if (i + 1 < v.size() && v[i + 1].first == v[i].first) continue;
-        v[w++] = std::move(v[i]);
+        if (w != i) v[w] = std::move(v[i]);  // no self-move (it would empty a vector member)
+        w++;
     }
     v.resize(w);
 }
@@ -481,7 +485,7 @@ std::string fmt_g(double x) {
 
 void build(const Cards &cards, const CaseControl &cc, int buckling_index, sol105_deck &D) {
     static const char *supported[] = {"GRID", "CQUAD4", "CTRIA3", "CBAR", "CBEAM", "PSHELL", "PBARL",
-                                      "MAT1", "SPC1", "SPC", "FORCE", "EIGRL", "PARAM"};
+                                      "MAT1", "SPC1", "SPC", "FORCE", "EIGRL", "PARAM", "RBE2", "RBE3", "RBAR"};
     {
         std::string list;
         for (const auto &kv : cards.by_name) {
@@ -643,6 +647,138 @@ void build(const Cards &cards, const CaseControl &cc, int buckling_index, sol105
         else if (G > 0 && nu >= 0 && E <= 0) E = 2 * G * (1 + nu);
         if (nu < 0) nu = 0.3;
         mats[mid] = Mat{E, G, nu};
+    }
+    // RBE2, RBAR (as RBE2) and RBE3 (extract_constraints.jl:46-141,
+    // ModelBuilder.jl:708-725); ids as to_id: integers, reals rounded, else 0
+    auto to_id = [](const Num &x) -> int64_t {
+        if (x.kind == Num::INT) return x.i;
+        if (x.kind == Num::REAL && std::isfinite(x.d) && std::fabs(x.d) < 9.0e18) return (int64_t)std::nearbyint(x.d);
+        return 0;
+    };
+    auto id_at = [&](const CardView &c, size_t i) { return to_id(field(c, i)); };
+    struct Rbe2 {
+        int64_t gn, cm;
+        std::vector<int64_t> gm;
+    };
+    std::vector<std::pair<int64_t, Rbe2>> rbe2s;
+    for (const CardView &c : get("RBE2")) {
+        const int64_t eid = id_at(c, 0), gn = id_at(c, 1), cm = id_at(c, 2);
+        Rbe2 r{gn, cm, {}};
+        for (size_t k = 3; k < c.size(); k++) {
+            const int64_t g = id_at(c, k);
+            if (g > 0) r.gm.push_back(g);
+        }
+        if (eid > 0 && gn > 0) rbe2s.push_back({eid, r});
+    }
+    for (const CardView &c : get("RBAR")) {
+        const int64_t eid = id_at(c, 0), ga = id_at(c, 1), gb = id_at(c, 2), cna = id_at(c, 3), cnb = id_at(c, 4);
+        if (eid > 0 && ga > 0 && gb > 0) {
+            if (cnb > 0 && cna == 0) rbe2s.push_back({eid, Rbe2{gb, cnb, {ga}}});
+            else if (cna > 0 && cnb == 0) rbe2s.push_back({eid, Rbe2{ga, cna, {gb}}});
+            else rbe2s.push_back({eid, Rbe2{ga, 123456, {gb}}});
+        }
+    }
+    sort_last_wins(rbe2s);
+    D.rbe2.clear();
+    for (const auto &kv : rbe2s) {
+        int m, r;
+        Num gn;
+        gn.kind = Num::INT;
+        gn.i = kv.second.gn;
+        if (!rank_of(gn, m)) continue;
+        for (int64_t g : kv.second.gm) {
+            Num x;
+            x.kind = Num::INT;
+            x.i = g;
+            if (!rank_of(x, r)) continue;
+            D.rbe2.insert(D.rbe2.end(), {m, (int32_t)kv.second.cm, r});
+        }
+    }
+    auto is_cont = [](sv t) {
+        t = strip(t);
+        return !t.empty() && (t[0] == '+' || t[0] == '*');
+    };
+    auto is_real_field = [&](sv t) {
+        t = strip(t);
+        if (t.empty() || is_cont(t)) return false;
+        for (char ch : t)
+            if (ch == '.' || ch == 'E' || ch == 'e' || ch == 'D' || ch == 'd') return true;
+        for (size_t k = 1; k < t.size(); k++)
+            if (t[k] == '+' || t[k] == '-') return true;
+        return false;
+    };
+    struct Rbe3Group {
+        double wt;
+        int64_t comps;
+        std::vector<int64_t> grids;
+    };
+    struct Rbe3 {
+        int64_t ref, refc;
+        std::vector<Rbe3Group> groups;
+    };
+    std::vector<std::pair<int64_t, Rbe3>> rbe3s;
+    for (const CardView &c : get("RBE3")) {
+        Rbe3 r{id_at(c, 2), id_at(c, 3), {}};
+        size_t pos = 4;
+        const size_t nf = c.size();
+        while (pos < nf) {
+            const sv w = strip(c[pos]);
+            if (w.empty() || is_cont(w)) {
+                pos++;
+                continue;
+            }
+            const Num wt = parse_number(w);
+            if (!wt.ok()) break;
+            pos++;
+            while (pos < nf) {
+                const sv t = strip(c[pos]);
+                if (!t.empty() && !is_cont(t)) break;
+                pos++;
+            }
+            if (pos >= nf) break;
+            Rbe3Group g{wt.val(), id_at(c, pos), {}};
+            pos++;
+            while (pos < nf) {
+                const sv t = strip(c[pos]);
+                if (t.empty() || is_cont(t)) {
+                    pos++;
+                    continue;
+                }
+                if (is_real_field(t)) break;
+                const Num x = parse_number(t);
+                if (!x.ok()) break;
+                g.grids.push_back((int64_t)std::trunc(x.val()));
+                pos++;
+            }
+            if (!g.grids.empty()) r.groups.push_back(std::move(g));
+        }
+        rbe3s.push_back({id_at(c, 0), std::move(r)});
+    }
+    sort_last_wins(rbe3s);
+    D.rbe3.clear();
+    D.rbe3_grp.clear();
+    D.rbe3_wt.clear();
+    D.rbe3_node.clear();
+    for (const auto &kv : rbe3s) {
+        int ref;
+        Num x;
+        x.kind = Num::INT;
+        x.i = kv.second.ref;
+        if (!rank_of(x, ref)) continue;
+        const int32_t g0 = (int32_t)D.rbe3_wt.size();
+        for (const Rbe3Group &g : kv.second.groups) {
+            const int32_t first = (int32_t)D.rbe3_node.size();
+            for (int64_t gid : g.grids) {
+                Num y;
+                y.kind = Num::INT;
+                y.i = gid;
+                int r;
+                if (rank_of(y, r)) D.rbe3_node.push_back(r);
+            }
+            D.rbe3_grp.insert(D.rbe3_grp.end(), {(int32_t)g.comps, first, (int32_t)D.rbe3_node.size() - first});
+            D.rbe3_wt.push_back(g.wt);
+        }
+        D.rbe3.insert(D.rbe3.end(), {ref, (int32_t)kv.second.refc, g0, (int32_t)D.rbe3_wt.size() - g0});
     }
     std::vector<Spc> spcs;
     for (const CardView &c : get("SPC1")) {
@@ -1005,6 +1141,15 @@ int sol105_deck_desc(const sol105_deck *D, sol105_model_desc *d) {
     d->eigrl_v1 = D->v1;
     d->eigrl_v2 = D->v2;
     d->n_shell_cards = D->shell_cards;
+    d->n_rbe2 = (int32_t)(D->rbe2.size() / 3);
+    d->rbe2 = D->rbe2.data();
+    d->n_rbe3 = (int32_t)(D->rbe3.size() / 4);
+    d->rbe3 = D->rbe3.data();
+    d->n_rbe3_grp = (int32_t)D->rbe3_wt.size();
+    d->rbe3_grp = D->rbe3_grp.data();
+    d->rbe3_wt = D->rbe3_wt.data();
+    d->n_rbe3_node = (int32_t)D->rbe3_node.size();
+    d->rbe3_node = D->rbe3_node.data();
     return SOL105_OK;
 }
 

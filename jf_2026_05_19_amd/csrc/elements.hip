@@ -2493,6 +2493,50 @@ void launch_bar_kg(const ElemArgs &a, const double *u, double *eb, hipStream_t s
     if (a.n_bar == 0) return;
     bar_kg_kernel<<<(a.n_bar + 127) / 128, 128, 0, s>>>(a, u, eb + (size_t)a.n_q4 * 576 + (size_t)a.n_t3 * 324);
 }
+// ---------------------------------------------------------------------------
+// Constraint elimination (RBE2 / RBE3): entries of dependent rows and columns
+// move to the independent DOFs in a fixed order per target (targets are
+// entries of independent rows and columns, never sources, so the update is
+// in place without races), then the dependent rows and columns are cleared;
+// dependent DOFs of a solution are recovered from the independent ones.
+// ---------------------------------------------------------------------------
+__global__ void mpc_apply_kernel(int64_t nt, const int64_t *tgt, const int32_t *tptr, const int64_t *src,
+                                 const double *coef, double *K) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double acc = K[tgt[t]];
+    for (int32_t p = tptr[t]; p < tptr[t + 1]; p++) acc += coef[p] * K[src[p]];
+    K[tgt[t]] = acc;
+}
+
+__global__ void mpc_zero_kernel(int64_t nz, const int64_t *pos, double *K) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nz) K[pos[t]] = 0.0;
+}
+
+// x[dep[i] + c ld] = sum_p coef[p] x[ind[p] + c ld] for columns c < ncols
+__global__ void mpc_recover_kernel(int ndep, const int32_t *dep, const int32_t *ptr, const int32_t *ind,
+                                   const double *coef, double *x, int64_t ld, int ncols) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)ndep * ncols) return;
+    const int i = (int)(t % ndep), c = (int)(t / ndep);
+    double acc = 0.0;
+    for (int32_t p = ptr[i]; p < ptr[i + 1]; p++) acc += coef[p] * x[ind[p] + c * ld];
+    x[dep[i] + c * ld] = acc;
+}
+
+void launch_mpc_apply(int64_t nt, const int64_t *tgt, const int32_t *tptr, const int64_t *src, const double *coef,
+                      int64_t nz, const int64_t *zpos, double *K, hipStream_t s) {
+    if (nt > 0) mpc_apply_kernel<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(nt, tgt, tptr, src, coef, K);
+    if (nz > 0) mpc_zero_kernel<<<(unsigned)((nz + 255) / 256), 256, 0, s>>>(nz, zpos, K);
+}
+
+void launch_mpc_recover(int ndep, const int32_t *dep, const int32_t *ptr, const int32_t *ind, const double *coef,
+                        double *x, int64_t ld, int ncols, hipStream_t s) {
+    const int64_t n = (int64_t)ndep * ncols;
+    if (n > 0) mpc_recover_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ndep, dep, ptr, ind, coef, x, ld, ncols);
+}
+
 void launch_block_gather(int64_t n_blocks, const int32_t *cptr, const int32_t *cidx, const double *eb,
                          double *vals, hipStream_t s) {
     int64_t n = n_blocks * 36;

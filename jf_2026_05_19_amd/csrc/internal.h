@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "common.h"
+#include "constraints.h"
 #include "dense.h"
 #include "factor.h"
 #include "solve.h"
@@ -119,6 +120,14 @@ struct sol105_model {
     double sel_info[4] = {0, 0, 0, 1};  // last buckling: pairs, in-range candidates, cluster skip, decided
     BlockPattern pat;
     std::vector<int32_t> diag_blk;
+    // RBE2 / RBE3 constraints: the map (dependent DOF -> independent DOFs)
+    // and its device copies, the redistribution plan of K on the pattern
+    ConstraintMap mpc;
+    int64_t mpc_nt = 0, mpc_nz = 0;
+    s105::DevBuf<int32_t> d_mpc_dep, d_mpc_ptr, d_mpc_ind, d_mpc_tptr;
+    s105::DevBuf<double> d_mpc_coef, d_mpc_tcoef;
+    s105::DevBuf<int64_t> d_mpc_tgt, d_mpc_src, d_mpc_zero;
+    bool has_mpc() const { return !mpc.empty(); }
     // device: model + assembly
     s105::DevBuf<double> d_xyz, d_q4prop, d_t3prop, d_bar_vref, d_barprop, d_force;
     s105::DevBuf<int32_t> d_q4, d_t3, d_bar, d_rp, d_col, d_cptr, d_cidx, d_diag;

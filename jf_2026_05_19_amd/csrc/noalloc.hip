@@ -66,6 +66,9 @@ extern "C" int sol105_assemble_K_noalloc(sol105_ctx *ctx, sol105_model *M, doubl
                                  hipGetErrorString(g_status[dev]));
         ptr = g_ptr[dev];
     }
+    if (M->has_mpc())
+        return s105::set_err(ctx, SOL105_E_UNSUPPORTED,
+                             "models with RBE2/RBE3 constraints are assembled by sol105_assemble_K");
     const size_t need = ((size_t)M->n_q4 * 16 + (size_t)M->n_t3 * 9 + (size_t)M->n_bar * 4) * 36;
     if (need > kScratchDoubles)
         return s105::set_err(ctx, SOL105_E_INVALID,

@@ -134,9 +134,9 @@ std::vector<int64_t> pick(sol105_server *S, double &fp) {
             used.push_back(R.model);
             fp = f;
             lead = SM.M.get();
-            small = S->batching && SM.M->ndof() <= S->small_dof;
+            small = S->batching && SM.M->ndof() <= S->small_dof && !SM.M->has_mpc();
             if (!small) break;
-        } else if (SM.M->ndof() <= S->small_dof && (int)group.size() < S->max_members &&
+        } else if (SM.M->ndof() <= S->small_dof && !SM.M->has_mpc() && (int)group.size() < S->max_members &&
                    S->inflight + fp + f <= S->budget && SM.M->k6rot == lead->k6rot &&
                    (SM.M->autospc != 0) == (lead->autospc != 0)) {
             group.push_back(*it);

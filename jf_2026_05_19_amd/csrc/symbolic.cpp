@@ -22,7 +22,8 @@ int64_t BlockPattern::find(int i, int j) const {
 }
 
 void build_block_pattern(int n_nodes, int n_q4, const int32_t *q4, int n_t3, const int32_t *t3, int n_bar,
-                         const int32_t *bar, BlockPattern &P) {
+                         const int32_t *bar, BlockPattern &P,
+                         const std::vector<std::pair<int32_t, int32_t>> *extra) {
     P.n_nodes = n_nodes;
     std::vector<std::vector<int32_t>> nb(n_nodes);
     for (int e = 0; e < n_q4; e++)
@@ -34,6 +35,8 @@ void build_block_pattern(int n_nodes, int n_q4, const int32_t *q4, int n_t3, con
     for (int e = 0; e < n_bar; e++)
         for (int a = 0; a < 2; a++)
             for (int b = 0; b < 2; b++) nb[bar[2 * e + a]].push_back(bar[2 * e + b]);
+    if (extra)
+        for (const auto &pr : *extra) nb[pr.first].push_back(pr.second);
     P.row_ptr.assign(n_nodes + 1, 0);
     for (int i = 0; i < n_nodes; i++) {
         auto &v = nb[i];

@@ -2,6 +2,7 @@
 // nested-dissection ordering and the supernodal (multifrontal) structure.
 #pragma once
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 struct BlockPattern {
@@ -16,8 +17,11 @@ struct BlockPattern {
 
 // Build the pattern and the deterministic gather map: CQUAD4 blocks first
 // (element order, a-major), then CTRIA3 blocks, then CBAR blocks.
+// extra: further node pairs to include (constraint couplings), no element
+// contributions.
 void build_block_pattern(int n_nodes, int n_q4, const int32_t *q4, int n_t3, const int32_t *t3, int n_bar,
-                         const int32_t *bar, BlockPattern &P);
+                         const int32_t *bar, BlockPattern &P,
+                         const std::vector<std::pair<int32_t, int32_t>> *extra = nullptr);
 
 struct FrontInfo {
     int ns = 0;          // pivot rows (contiguous elimination indices piv0..piv0+ns-1)

@@ -21,12 +21,21 @@ parser (``JFEM/src/parsing/card_processor.jl``).
 * ``cylinder_deck`` -- axially compressed cylindrical shell of flat CQUAD4
   facets with a circumferential band of CTRIA3 (each quad of the band split
   in two), clamped at z = 0, axial ring load at z = L.
+* ``glued_plate_deck`` -- the plate meshed as two halves with duplicated
+  seam GRIDs tied by RBE2 (123456, coincident nodes): the same structure as
+  the one-piece plate.
+* ``eccentric_panel_deck`` -- a plate with CBAR stringers on their own GRIDs
+  at an eccentricity above the skin, each tied to the skin node beneath by an
+  RBE2 (123456).
+* ``rbe3_load_plate_deck`` -- the plate with its edge load applied at one
+  reference GRID and spread over the loaded edge by an RBE3.
 """
 from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["plate_deck", "panel_deck", "variant_sweep", "cylinder_deck", "served_families"]
+__all__ = ["plate_deck", "panel_deck", "variant_sweep", "cylinder_deck", "served_families", "glued_plate_deck",
+           "eccentric_panel_deck", "rbe3_load_plate_deck"]
 
 
 def _fmt(v):
@@ -216,3 +225,98 @@ def cylinder_deck(n_circ=416, n_axial=400, R=500.0, L=1000.0, t=1.0, E=70e3, nu=
         L_.append(_card("FORCE", 10, gid(i, n_axial - 1), 0, f, 0.0, 0.0, -1.0))
     L_.append("ENDDATA")
     return "\n".join(L_) + "\n"
+
+
+def _plate_bc_and_load(L, nx, ny, b, gid_at, line_load, load_edge=True):
+    """plate_deck's supports and edge load on the GRID ids gid_at(i, j)."""
+    edge = sorted({gid_at(i, j) for j in range(ny + 1) for i in range(nx + 1) if i in (0, nx) or j in (0, ny)})
+    for g in edge:
+        L.append(_card("SPC1", 1, "3", g))
+    for j in range(ny + 1):
+        L.append(_card("SPC1", 1, "1", gid_at(0, j)))
+    L.append(_card("SPC1", 1, "2", gid_at(0, 0)))
+    if load_edge:
+        dy = b / ny
+        for j in range(ny + 1):
+            w = 0.5 if j in (0, ny) else 1.0
+            L.append(_card("FORCE", 10, gid_at(nx, j), 0, line_load * dy * w, -1.0, 0.0, 0.0))
+
+
+def glued_plate_deck(nx=12, ny=8, a=400.0, b=300.0, t=1.5, E=70e3, nu=0.33, line_load=1.0, nd=4):
+    """plate_deck's plate (nx even) as two half meshes: the cells right of
+    x = a/2 use their own GRIDs (ids + 100000) and each seam GRID of the
+    right half is the RBE2 (123456) slave of the coincident left one."""
+    assert nx % 2 == 0
+    L = _header(nd)
+    L.append(_card("MAT1", 1, float(E), "", float(nu)))
+    L.append(_card("PSHELL", 1, 1, float(t), 1, "", ""))
+    h = nx // 2
+    gl = lambda i, j: j * (nx + 1) + i + 1
+    gr = lambda i, j: 100000 + j * (nx + 1) + i + 1
+    gid_at = lambda i, j: gl(i, j) if i <= h else gr(i, j)
+    for j in range(ny + 1):
+        for i in range(nx + 1):
+            if i <= h:
+                L.append(_card("GRID", gl(i, j), "", a * i / nx, b * j / ny, 0.0))
+            if i >= h:
+                L.append(_card("GRID", gr(i, j), "", a * i / nx, b * j / ny, 0.0))
+    eid = 1
+    for j in range(ny):
+        for i in range(nx):
+            g = gl if i < h else gr
+            L.append(_card("CQUAD4", eid, 1, g(i, j), g(i + 1, j), g(i + 1, j + 1), g(i, j + 1)))
+            eid += 1
+    for j in range(ny + 1):
+        L.append(_card("RBE2", 900000 + j, gl(h, j), 123456, gr(h, j)))
+    _plate_bc_and_load(L, nx, ny, b, gid_at, line_load)
+    L.append("ENDDATA")
+    return "\n".join(L) + "\n"
+
+
+def eccentric_panel_deck(nx=16, ny=8, a=400.0, b=200.0, t=1.5, E=70e3, nu=0.33, stringer_every=4, ecc=12.0,
+                         dims=(3.0, 20.0), line_load=1.0, nd=4):
+    """A simply supported plate with CBAR stringers (PBARL BAR) on every
+    stringer_every-th longitudinal mesh line, on their own GRIDs at z = ecc
+    (ids + 200000), each tied to the skin GRID beneath by an RBE2 (123456,
+    the skin node independent)."""
+    L = _header(nd)
+    L.append(_card("MAT1", 1, float(E), "", float(nu)))
+    L.append(_card("PSHELL", 1, 1, float(t), 1, "", ""))
+    L.append(_card("PBARL", 2, 1, "", "BAR", float(dims[0]), float(dims[1])))
+    gid = lambda i, j: j * (nx + 1) + i + 1
+    for j in range(ny + 1):
+        for i in range(nx + 1):
+            L.append(_card("GRID", gid(i, j), "", a * i / nx, b * j / ny, 0.0))
+    _shells(L, nx, ny, gid, 0.0, 0)
+    beid = 100000
+    for j in range(stringer_every, ny, stringer_every):
+        for i in range(nx + 1):
+            L.append(_card("GRID", 200000 + gid(i, j), "", a * i / nx, b * j / ny, float(ecc)))
+            L.append(_card("RBE2", 300000 + gid(i, j), gid(i, j), 123456, 200000 + gid(i, j)))
+        for i in range(nx):
+            L.append(_card("CBAR", beid, 2, 200000 + gid(i, j), 200000 + gid(i + 1, j), 0.0, 0.0, 1.0))
+            beid += 1
+    _plate_bc_and_load(L, nx, ny, b, gid, line_load)
+    L.append("ENDDATA")
+    return "\n".join(L) + "\n"
+
+
+def rbe3_load_plate_deck(nx=10, ny=8, a=400.0, b=300.0, t=1.5, E=70e3, nu=0.33, line_load=1.0, nd=4,
+                         offset=10.0):
+    """plate_deck's plate whose edge load is one FORCE at a reference GRID
+    (id 999999) beside the middle of the loaded edge, spread over the edge
+    GRIDs by an RBE3 (REFC 1; one group: weight 1.0, component 1)."""
+    L = _header(nd)
+    L.append(_card("MAT1", 1, float(E), "", float(nu)))
+    L.append(_card("PSHELL", 1, 1, float(t), 1, "", ""))
+    gid = lambda i, j: j * (nx + 1) + i + 1
+    for j in range(ny + 1):
+        for i in range(nx + 1):
+            L.append(_card("GRID", gid(i, j), "", a * i / nx, b * j / ny, 0.0))
+    L.append(_card("GRID", 999999, "", a + float(offset), b / 2, 0.0))
+    _shells(L, nx, ny, gid, 0.0, 0)
+    L.append(_card("RBE3", 999000, "", 999999, 1, 1.0, 1, *[gid(nx, j) for j in range(ny + 1)]))
+    _plate_bc_and_load(L, nx, ny, b, gid, line_load, load_edge=False)
+    L.append(_card("FORCE", 10, 999999, 0, line_load * b, -1.0, 0.0, 0.0))
+    L.append("ENDDATA")
+    return "\n".join(L) + "\n"

@@ -17,6 +17,10 @@ the C-ABI takes by pointer:
 * FORCE vector of the static subcase LOAD (``solve_case.jl:3-57``,
   ``loads.jl:50-53``), rotated into the analysis frame (identity for CD = 0);
 * subcase pairing and EIGRL ND/V1/V2 (``JFEMSolver.jl:1302-1326, 1464-1472``).
+* RBE2 (and RBAR) master-slave pairs and RBE3 weight groups by node rank, in
+  element-id order; grids absent from the model are dropped as
+  ``assemble_constraints`` skips them (``constraints.jl:44-60, 236-266``).
+  The device builds the dependent-DOF map from them.
 
 Features outside the supported default branch raise ``UnsupportedModel``
 (the C-ABI returns SOL105_E_UNSUPPORTED for the same cases).
@@ -63,6 +67,13 @@ class Sol105Arrays:
     static_subcase: int = 1
     shell_cards: int = -1         # CSHELL cards of the deck, assembled or not (-1: n_shell)
     bar_points: np.ndarray = field(default=None)   # (n_bar, 8) stress points C1 C2 D1 D2 E1 E2 F1 F2
+    # constraints (RBE2 / RBAR as master-slave pairs, RBE3 elements with
+    # weight groups), node ranks; component sets as digit integers (123456)
+    rbe2: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.int32))      # master, CM, slave
+    rbe3: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int32))      # ref, REFC, first group, groups
+    rbe3_grp: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.int32))  # comps, first node, nodes
+    rbe3_wt: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    rbe3_node: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
 
     @property
     def n_nodes(self):
@@ -243,6 +254,26 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
             # D (-h/2, w/2), E (-h/2, -w/2), F (h/2, -w/2), extract_properties.jl:176-178
             w2, h2 = prop["DIMS"][0] / 2, prop["DIMS"][1] / 2
             barpts.append([h2, w2, -h2, w2, -h2, -w2, h2, -w2])
+    rbe2 = []
+    for key in sorted(model.get("RBE2s", {}), key=lambda k: int(k)):
+        r = model["RBE2s"][key]
+        if r["GN"] not in rank:
+            continue
+        for g in r["GM"]:
+            if g in rank:
+                rbe2.append([rank[r["GN"]], int(r["CM"]), rank[g]])
+    rbe3, rbe3_grp, rbe3_wt, rbe3_node = [], [], [], []
+    for key in sorted(model.get("RBE3s", {}), key=lambda k: int(k)):
+        r = model["RBE3s"][key]
+        if r["REFGRID"] not in rank:
+            continue
+        g0 = len(rbe3_grp)
+        for grp in r["WT_GROUPS"]:
+            nodes = [rank[g] for g in grp["grids"] if g in rank]
+            rbe3_grp.append([int(grp["comps"]), len(rbe3_node), len(nodes)])
+            rbe3_wt.append(float(grp["wt"]))
+            rbe3_node.extend(nodes)
+        rbe3.append([rank[r["REFGRID"]], int(r["REFC"]), g0, len(rbe3_grp) - g0])
     ndof = 6 * n
     cc = model["CASE_CONTROL"]
     pairs = _subcase_pairing(cc)
@@ -327,4 +358,7 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
         buckling_subcase=int(buck_sid), static_subcase=int(stat_sid),
         shell_cards=len(model["CSHELLs"]),
         bar_points=np.asarray(barpts, dtype=float).reshape(-1, 8),
+        rbe2=np.asarray(rbe2, dtype=np.int32).reshape(-1, 3), rbe3=np.asarray(rbe3, dtype=np.int32).reshape(-1, 4),
+        rbe3_grp=np.asarray(rbe3_grp, dtype=np.int32).reshape(-1, 3), rbe3_wt=np.asarray(rbe3_wt, dtype=float),
+        rbe3_node=np.asarray(rbe3_node, dtype=np.int32),
     )

@@ -231,6 +231,124 @@ def bar_ke_global(p1, p2, vref, bp):
 # assembly
 # --------------------------------------------------------------------------
 
+# --------------------------------------------------------------------------
+# RBE2 / RBE3 constraints by dependent-DOF elimination
+# --------------------------------------------------------------------------
+
+def _digits(code):
+    return [int(ch) for ch in str(abs(int(code))) if ch in "123456"]
+
+
+def _rigid_row(d, c):
+    """Rigid row of component c (1..6) of a point at offset d from the
+    reference point on the reference's six DOFs (constraints.jl:7-24)."""
+    dx, dy, dz = d
+    row = np.zeros(6)
+    row[c - 1] = 1.0
+    if c <= 3:
+        O = np.array([[0.0, dz, -dy], [-dz, 0.0, dx], [dy, -dx, 0.0]])
+        row[3:] = O[c - 1]
+    return row
+
+
+def constraint_map(arr):
+    """{dependent DOF: [(independent DOF, coefficient), ...]} of the model's
+    RBE2 (RBAR) pairs and RBE3 elements, JFEM/src/solver/constraints.jl:
+    37-364 with node_R = I: RBE2 slave components follow the master rigidly,
+    an RBE3 reference DOF is the weighted least-squares rigid fit of its
+    groups' components (pinv of A6 = sum w G'G with relative cut 1e-10),
+    coefficients <= 1e-15 dropped, RBE2 entries replacing RBE3 ones."""
+    xyz = np.asarray(arr.xyz, dtype=float)
+    rbe3m = {}
+    for ref, refc, g0, ng in np.asarray(arr.rbe3).reshape(-1, 4):
+        rd = sorted(_digits(refc))
+        if not rd:
+            continue
+        A6 = np.zeros((6, 6))
+        rows = []
+        for g in range(g0, g0 + ng):
+            comps, first, cnt = arr.rbe3_grp[g]
+            wt = float(arr.rbe3_wt[g])
+            cd = sorted(_digits(comps))
+            if not cd:
+                continue
+            for k in range(first, first + cnt):
+                di = int(arr.rbe3_node[k])
+                G = np.array([_rigid_row(xyz[di] - xyz[ref], c) for c in cd])
+                A6 += wt * (G.T @ G)
+                rows.append((di, G, wt, cd))
+        if not rows:
+            continue
+        Ai = np.linalg.pinv(A6, rcond=1e-10)
+        for di, G, wt, cd in rows:
+            C = Ai @ (wt * G.T)
+            for r in rd:
+                lst = rbe3m.setdefault(6 * int(ref) + r - 1, [])
+                for jj, c in enumerate(cd):
+                    if abs(C[r - 1, jj]) > 1e-15:
+                        lst.append((6 * di + c - 1, float(C[r - 1, jj])))
+    rbe2m = {}
+    for m, cm, sl in np.asarray(arr.rbe2).reshape(-1, 3):
+        for c in _digits(cm):
+            row = _rigid_row(xyz[sl] - xyz[m], c)
+            pairs = [(6 * int(m) + k, float(row[k])) for k in range(6) if abs(row[k]) > 1e-15]
+            if pairs:
+                rbe2m[6 * int(sl) + c - 1] = pairs
+    out = dict(rbe3m)
+    out.update(rbe2m)
+    return out
+
+
+def _redistribute(I, J, V, cmap):
+    """Triplets with a dependent row or column moved to the independent DOFs
+    in triplet order (constraints.jl:318-360), for K and for KG
+    (assembly.jl:4367, 6617-6620)."""
+    if not cmap:
+        return I, J, V
+    nI, nJ, nV = [], [], []
+    for i, j, v in zip(I.tolist(), J.tolist(), V.tolist()):
+        pi, pj = cmap.get(i), cmap.get(j)
+        if pi is None and pj is None:
+            nI.append(i); nJ.append(j); nV.append(v)
+        elif pj is None:
+            for a, c in pi:
+                nI.append(a); nJ.append(j); nV.append(v * c)
+        elif pi is None:
+            for b, c in pj:
+                nI.append(i); nJ.append(b); nV.append(v * c)
+        else:
+            for a, ca in pi:
+                for b, cb in pj:
+                    nI.append(a); nJ.append(b); nV.append(v * ca * cb)
+    return np.asarray(nI, np.int64), np.asarray(nJ, np.int64), np.asarray(nV, float)
+
+
+def constrained_arrays(arr, cmap):
+    """Copy of the arrays with the dependent DOFs fixed and their loads moved
+    to the independent DOFs (solve_case.jl:40-55, boundary_conditions.jl:
+    634-640)."""
+    import copy
+    out = copy.copy(arr)
+    out.spc_mask = np.asarray(arr.spc_mask, dtype=np.uint8).copy()
+    F = np.asarray(arr.F, dtype=float).copy()
+    for dep, pairs in cmap.items():
+        out.spc_mask[dep] = 1
+        f = F[dep]
+        if abs(f) > 1e-30:
+            for ind, c in pairs:
+                F[ind] += f * c
+            F[dep] = 0.0
+    out.F = F
+    return out
+
+
+def recover_dependent(cmap, x):
+    """x[dep] = sum c x[ind] (columns of a 2-D x alike)."""
+    for dep, pairs in cmap.items():
+        x[dep] = sum(c * x[ind] for ind, c in pairs) if pairs else 0.0 * x[dep]
+    return x
+
+
 def assemble_K(arr, with_max_elem_stiff=False):
     """K (CSR).  With ``with_max_elem_stiff``: (K, largest |entry| of the
     element matrices before assembly, assembly.jl:4352-4356)."""
@@ -242,9 +360,11 @@ def assemble_K(arr, with_max_elem_stiff=False):
                               arr.n_bar, _p(bar, _ip), _p(_c(arr.bar_vref, float)),
                               _p(_c(arr.barprop, float)), _p(I, _lp), _p(J, _lp), _p(V))
     assert cnt == n
+    mes = float(np.abs(V).max(initial=0.0))
+    I, J, V = _redistribute(I, J, V, constraint_map(arr) if _has_constraints(arr) else {})
     K = sp.csr_matrix((V, (I, J)), shape=(arr.ndof, arr.ndof))
     if with_max_elem_stiff:
-        return K, float(np.abs(V).max(initial=0.0))
+        return K, mes
     return K
 
 
@@ -256,6 +376,8 @@ def assemble_KG(arr, u):
                                arr.n_t3, _p(t3, _ip), _p(_c(arr.t3prop, float)), arr.n_bar, _p(bar, _ip), _p(_c(arr.bar_vref, float)), _p(_c(arr.barprop, float)),
                                _p(_c(u, float)), _p(I, _lp), _p(J, _lp), _p(V))
     assert cnt == n
+    # the same constraint redistribution as K (assembly.jl:6617-6620)
+    I, J, V = _redistribute(I, J, V, constraint_map(arr) if _has_constraints(arr) else {})
     return sp.csr_matrix((V, (I, J)), shape=(arr.ndof, arr.ndof))
 
 
@@ -428,7 +550,7 @@ def static_solve_cholesky(arr, K, max_elem_stiff, info=None):
 # buckling
 # --------------------------------------------------------------------------
 
-def buckling(arr, K, KG, free, krylov_tol=1e-13):
+def buckling(arr, K, KG, free, krylov_tol=1e-13, cmap=None):
     """Buckling eigenpairs with the reference's output semantics
     (solve_case.jl:1938-1951 request count, 2131-2196 dense path keeping the
     whole spectrum under an EIGRL range, 2202-2315 Krylov path, 2414-2436
@@ -475,6 +597,8 @@ def buckling(arr, K, KG, free, krylov_tol=1e-13):
     n_out = len(sel)
     modes = np.zeros((arr.ndof, n_out))
     modes[free] = V[:, sel]
+    if cmap:   # dependent DOFs before the normalisation (solve_case.jl:3153-3162)
+        recover_dependent(cmap, modes)
     for m in range(n_out):
         mx = np.abs(modes[:, m]).max()
         if mx > 1e-30:
@@ -625,6 +749,10 @@ def localization_filter(arr, free, V, order, num_modes, n_shell_cards, max_share
     return localization_scan(order, order, num_modes, n_shell_cards, share, max_share, min_elements, scan_buffer)
 
 
+def _has_constraints(arr):
+    return len(getattr(arr, "rbe2", ())) > 0 or len(getattr(arr, "rbe3", ())) > 0
+
+
 def solve(arr):
     """Full forward SOL 105 path: K -> u -> KG -> (lambda, phi).  The static
     solve follows the reference's Cholesky path (refix, factorisation
@@ -634,7 +762,9 @@ def solve(arr):
     import time
     tm = {}
     t0 = time.perf_counter()
+    cmap = constraint_map(arr) if _has_constraints(arr) else {}
     K, mes = assemble_K(arr, with_max_elem_stiff=True)
+    arr = constrained_arrays(arr, cmap) if cmap else arr
     t1 = time.perf_counter()
     tm["K_assembly"] = t1 - t0
     n_free = int(np.sum(~autospc(arr, K.diagonal())))
@@ -644,12 +774,14 @@ def solve(arr):
     else:
         u, free = static_solve(arr, K, tm)
         info["path"] = "sparse_lu"
+    if cmap:
+        recover_dependent(cmap, u)
     t2 = time.perf_counter()
     tm["static_solve"] = t2 - t1
     KG = assemble_KG(arr, u)
     t3 = time.perf_counter()
     tm["KG_assembly"] = t3 - t2
-    res = buckling(arr, K, KG, free)
+    res = buckling(arr, K, KG, free, cmap=cmap)
     tm["eigensolve"] = time.perf_counter() - t3
     tm["total"] = time.perf_counter() - t0
     res.update(K=K, KG=KG, u=u, free=free, static_info=info, times=tm)

@@ -42,7 +42,7 @@ def _decks(tmp_path):
     plate = _deck(tmp_path, "plate_a.bdf", decks.plate_deck(nx=6, ny=5, a=300.0, b=250.0, t=1.5, nd=2))
     plate_b = _deck(tmp_path, "plate_b.bdf", decks.plate_deck(nx=6, ny=5, a=300.0, b=250.0, t=1.8, nd=2))
     panel = _deck(tmp_path, "panel.dat", decks.panel_deck(nx=10, ny=6, a=200.0, b=120.0, stringer_every=3, nd=3))
-    bad = _deck(tmp_path, "bad.bdf", decks.plate_deck(nx=3, ny=3).replace("ENDDATA", "RBE2,900,1,123456,2\nENDDATA"))
+    bad = _deck(tmp_path, "bad.bdf", decks.plate_deck(nx=3, ny=3).replace("ENDDATA", "CONM2,900,1,,5.0\nENDDATA"))
     return plate, plate_b, panel, bad
 
 
@@ -106,7 +106,7 @@ def test_manifest_outputs_against_oracle(tmp_path):
     assert rows[0]["output_dir"] == os.path.join(root, "p_1") and rows[1]["output_dir"] == os.path.join(root, "p_1_2")
     assert rows[2]["output_dir"] == str(mpath.parent / "custom_dir")
     # the unsupported deck fails with its error, the batch goes on
-    assert rows[3]["status"] == "failed" and "RBE2" in rows[3]["error"]
+    assert rows[3]["status"] == "failed" and "CONM2" in rows[3]["error"]
     # eigenvalues against the oracle, summary columns
     for k, deck in ((0, plate), (1, plate_b), (2, panel), (4, plate)):
         ref = O.solve(M.build_arrays(bdf.bdf_to_model_text(open(deck).read())))
