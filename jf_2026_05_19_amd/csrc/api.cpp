@@ -371,7 +371,14 @@ int model_init(sol105_ctx *ctx, const sol105_model_desc *d, sol105_model *M) {
         HIPCHK(M->d_mpc_src.upload(cplan.src.data(), cplan.src.size(), s));
         HIPCHK(M->d_mpc_tcoef.upload(cplan.coef.data(), cplan.coef.size(), s));
         HIPCHK(M->d_mpc_zero.upload(cplan.zero.data(), cplan.zero.size(), s));
-        HIPCHK(hipStreamSynchronize(s));  // cplan is a host temporary
+        ConstraintFold fold;
+        build_constraint_fold(C, fold);
+        M->mpc_nrecv = (int32_t)fold.recv.size();
+        HIPCHK(M->d_mpc_frecv.upload(fold.recv.data(), fold.recv.size(), s));
+        HIPCHK(M->d_mpc_fptr.upload(fold.ptr.data(), fold.ptr.size(), s));
+        HIPCHK(M->d_mpc_fdep.upload(fold.dep.data(), fold.dep.size(), s));
+        HIPCHK(M->d_mpc_fcoef.upload(fold.coef.data(), fold.coef.size(), s));
+        HIPCHK(hipStreamSynchronize(s));  // cplan and fold are host temporaries
     }
     HIPCHK(M->d_xyz.upload(M->xyz.data(), M->xyz.size(), s));
     HIPCHK(M->d_q4.upload(M->q4.data(), M->q4.size(), s));
@@ -1512,14 +1519,30 @@ int sens_terms(sol105_ctx *ctx, sol105_model *M, int32_t mode, DevBuf<double> &t
     else launch_q4_adjoint_rhs(a, M->d_u.p, phi, lam, egq.p, s);
     launch_t3_kg_sens(a, M->d_u.p, phi, lam, terms.p, egt.p, s);
     launch_bar_adjoint_rhs(a, phi, lam, egb.p, s);
+    // with RBE2 / RBE3 constraints the exact adjoint load is T' g: the
+    // entries on dependent DOFs fold onto their independent DOFs, and psi
+    // gets its dependent DOFs back (psi = T psi_ind).  The reference's path
+    // drops those entries and keeps psi zero there (adjoint_buckling.jl:
+    // 2490-2493 restrict g to the free DOFs, which exclude dependent ones).
+    const bool fold = M->has_mpc() && !ref;
     node_gather_vec(M->n_nodes, M->n_q4, M->n_t3, M->d_diag.p, M->d_cptr.p, M->d_cidx.p, egq.p, egt.p, egb.p,
-                    M->d_free.p, g.p, s);
+                    fold ? nullptr : M->d_free.p, g.p, s);
+    if (fold)
+        launch_mpc_fold(M->mpc_nrecv, M->d_mpc_frecv.p, M->d_mpc_fptr.p, M->d_mpc_fdep.p, M->d_mpc_fcoef.p,
+                        M->d_free.p, ndof, g.p, s);
     launch_gather_elim(n, ndof, 1, M->d_dof_of_elim.p, g.p, ge.p, s);
     HIPCHK(tri_solve(M, ge.p, 1, s));
     launch_scatter_elim(n, ndof, 1, M->d_elim_of_dof.p, ge.p, psi.p, s);
-    // phi' KG phi on the free set
+    if (fold)
+        launch_mpc_recover((int)M->mpc.dep.size(), M->d_mpc_dep.p, M->d_mpc_ptr.p, M->d_mpc_ind.p, M->d_mpc_coef.p,
+                           psi.p, ndof, 1, s);
+    // phi' KG phi on the free set; phi on the free set plus its dependent
+    // DOFs (the element terms take the whole mode, adjoint_buckling.jl:2472)
     launch_gather_elim(n, ndof, 1, M->d_dof_of_elim.p, phi, ge.p, s);
     launch_scatter_elim(n, ndof, 1, M->d_elim_of_dof.p, ge.p, phif.p, s);
+    if (M->has_mpc())
+        launch_mpc_recover((int)M->mpc.dep.size(), M->d_mpc_dep.p, M->d_mpc_ptr.p, M->d_mpc_ind.p, M->d_mpc_coef.p,
+                           phif.p, ndof, 1, s);
     launch_bsr_spmv(M->n_nodes, M->d_rp.p, M->d_col.p, M->d_KG.p, phif.p, kgphi.p, 1, 1.0, s);
     dot_det(ndof, phif.p, kgphi.p, scr.p, dots.p, s);
     if (ref) launch_q4_sens_terms_ref(a, M->d_u.p, phif.p, psi.p, six, terms.p, s);

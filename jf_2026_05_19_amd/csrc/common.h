@@ -44,6 +44,10 @@ void launch_mpc_apply(int64_t nt, const int64_t *tgt, const int32_t *tptr, const
                       int64_t nz, const int64_t *zpos, double *K, hipStream_t s);
 void launch_mpc_recover(int ndep, const int32_t *dep, const int32_t *ptr, const int32_t *ind, const double *coef,
                         double *x, int64_t ld, int ncols, hipStream_t s);
+// a vector with entries on the dependent DOFs folded onto the independent
+// ones (v[ind] += coef v[dep] on free receivers), then zeroed off the free set
+void launch_mpc_fold(int nrecv, const int32_t *recv, const int32_t *ptr, const int32_t *dep, const double *coef,
+                     const uint8_t *free_mask, int ndof, double *v, hipStream_t s);
 
 // SOL 101 result recovery (elements.hip); outputs component-major
 // (column c of element e at out[c * n + e]): CQUAD4 44, CTRIA3 20, CBAR 18

@@ -296,6 +296,22 @@ void build_constraint_plan(const ConstraintMap &C, const BlockPattern &elem, con
         }
 }
 
+void build_constraint_fold(const ConstraintMap &C, ConstraintFold &out) {
+    std::map<int32_t, std::vector<std::pair<int32_t, double>>> by_ind;
+    for (size_t i = 0; i < C.dep.size(); i++)
+        for (int32_t p = C.ptr[i]; p < C.ptr[i + 1]; p++) by_ind[C.ind[p]].push_back({C.dep[i], C.coef[p]});
+    out = ConstraintFold();
+    out.ptr.push_back(0);
+    for (const auto &kv : by_ind) {
+        out.recv.push_back(kv.first);
+        for (const auto &t : kv.second) {
+            out.dep.push_back(t.first);
+            out.coef.push_back(t.second);
+        }
+        out.ptr.push_back((int32_t)out.dep.size());
+    }
+}
+
 void redistribute_force(const ConstraintMap &C, std::vector<double> &F) {
     for (size_t i = 0; i < C.dep.size(); i++) {
         const double f = F[C.dep[i]];

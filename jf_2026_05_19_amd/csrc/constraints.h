@@ -42,6 +42,15 @@ struct ConstraintPlan {
 void build_constraint_plan(const ConstraintMap &C, const BlockPattern &elem, const BlockPattern &P, int n_nodes,
                            ConstraintPlan &out);
 
+// The map transposed for folding a vector onto the independent DOFs:
+// v[recv[k]] += sum_p coef[p] v[dep[p]], p in [ptr[k], ptr[k+1]); recv
+// ascending, the terms of one receiver in ascending dependent order.
+struct ConstraintFold {
+    std::vector<int32_t> recv, ptr, dep;
+    std::vector<double> coef;
+};
+void build_constraint_fold(const ConstraintMap &C, ConstraintFold &out);
+
 // F'[ind] += coef F[dep], F'[dep] = 0 (the load redistribution of
 // _assemble_applied_force, JFEM/src/solver/solve_case.jl:40-55)
 void redistribute_force(const ConstraintMap &C, std::vector<double> &F);

@@ -980,7 +980,7 @@ __global__ void node_gather_kernel(int n_nodes, int n_q4, int n_t3, const int32_
             s += eg_bar[b * 12 + a * 6 + c];
         }
     }
-    g[t] = free_mask[t] ? s : 0.0;
+    g[t] = (!free_mask || free_mask[t]) ? s : 0.0;
 }
 
 __global__ void kdiag_kernel(int n_nodes, const int32_t *diag_blk, const double *kv, double *out) {

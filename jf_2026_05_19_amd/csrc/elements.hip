@@ -2525,6 +2525,30 @@ __global__ void mpc_recover_kernel(int ndep, const int32_t *dep, const int32_t *
     x[dep[i] + c * ld] = acc;
 }
 
+// v[recv[k]] += sum_p coef[p] v[dep[p]] on free receivers (fixed ones get
+// zero); dependent DOFs are never receivers, so reads and writes do not meet
+__global__ void mpc_fold_kernel(int nrecv, const int32_t *recv, const int32_t *ptr, const int32_t *dep,
+                                const double *coef, const uint8_t *free_mask, double *v) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nrecv) return;
+    const int a = recv[k];
+    double acc = v[a];
+    for (int32_t p = ptr[k]; p < ptr[k + 1]; p++) acc += coef[p] * v[dep[p]];
+    v[a] = free_mask[a] ? acc : 0.0;
+}
+
+__global__ void vec_free_mask_kernel(int n, const uint8_t *free_mask, double *v) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n && !free_mask[t]) v[t] = 0.0;
+}
+
+void launch_mpc_fold(int nrecv, const int32_t *recv, const int32_t *ptr, const int32_t *dep, const double *coef,
+                     const uint8_t *free_mask, int ndof, double *v, hipStream_t s) {
+    if (nrecv > 0) mpc_fold_kernel<<<(unsigned)((nrecv + 255) / 256), 256, 0, s>>>(nrecv, recv, ptr, dep, coef,
+                                                                                    free_mask, v);
+    vec_free_mask_kernel<<<(unsigned)((ndof + 255) / 256), 256, 0, s>>>(ndof, free_mask, v);
+}
+
 void launch_mpc_apply(int64_t nt, const int64_t *tgt, const int32_t *tptr, const int64_t *src, const double *coef,
                       int64_t nz, const int64_t *zpos, double *K, hipStream_t s) {
     if (nt > 0) mpc_apply_kernel<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(nt, tgt, tptr, src, coef, K);

@@ -127,6 +127,10 @@ struct sol105_model {
     s105::DevBuf<int32_t> d_mpc_dep, d_mpc_ptr, d_mpc_ind, d_mpc_tptr;
     s105::DevBuf<double> d_mpc_coef, d_mpc_tcoef;
     s105::DevBuf<int64_t> d_mpc_tgt, d_mpc_src, d_mpc_zero;
+    // the map transposed (receiving independent DOFs), for the adjoint load
+    int32_t mpc_nrecv = 0;
+    s105::DevBuf<int32_t> d_mpc_frecv, d_mpc_fptr, d_mpc_fdep;
+    s105::DevBuf<double> d_mpc_fcoef;
     bool has_mpc() const { return !mpc.empty(); }
     // device: model + assembly
     s105::DevBuf<double> d_xyz, d_q4prop, d_t3prop, d_bar_vref, d_barprop, d_force;
