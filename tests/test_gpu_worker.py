@@ -1,6 +1,7 @@
 """The JSONL worker process on the GPU through the client module (the
 interface of the reference's jfem_client.py): two iterations of a design
-loop, each a manifest of decks; buckling loads against the oracle within
+loop, each a manifest of decks (a stiffened panel, a plate and a plate
+glued from two halves by RBE2); buckling loads against the oracle within
 1e-8 and mode shapes by MAC; the second iteration's designs of the same
 meshes reuse the registered meshes."""
 import json
@@ -22,7 +23,8 @@ def test_worker_design_loop_against_oracle(tmp_path):
     from oracle import oracle as O
     texts = {"panel": lambda t: decks.panel_deck(nx=24, ny=12, a=240.0, b=120.0, t=t, stringer_every=4, nd=4,
                                                  tri_frac=0.1),
-             "plate": lambda t: decks.plate_deck(nx=10, ny=8, a=400.0, b=320.0, t=t, nd=3)}
+             "plate": lambda t: decks.plate_deck(nx=10, ny=8, a=400.0, b=320.0, t=t, nd=3),
+             "glued": lambda t: decks.glued_plate_deck(nx=8, ny=6, t=t, nd=3)}
     with client.JFEMWorker() as w:
         assert w.ready["status"] == "ready"
         for it, thick in enumerate(((1.5, 2.0), (1.55, 1.9))):
@@ -39,7 +41,7 @@ def test_worker_design_loop_against_oracle(tmp_path):
                                                               "report": False})
             print("iteration", it, flush=True)
             r = w.run_batch(man, request_id="r%d" % it)
-            assert r["status"] == "ok" and r["completed"] == 4 and r["id"] == "r%d" % it
+            assert r["status"] == "ok" and r["completed"] == 6 and r["id"] == "r%d" % it
             rows = client.load_summary(r["summary_json"])["cases"]
             for row in rows:
                 arr = M.build_arrays(bdf.bdf_to_model_text(open(row["input"]).read()))
@@ -58,4 +60,4 @@ def test_worker_design_loop_against_oracle(tmp_path):
                                           for d in m["mode_shape"]]).ravel()
                         assert _mac(shape, ref["modes"][:, i]) >= 1 - 1e-8
             st = w.status()
-            assert st["meshes"] == 2          # two meshes, every deck a design of one of them
+            assert st["meshes"] == 3          # three meshes, every deck a design of one of them

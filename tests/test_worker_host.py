@@ -180,3 +180,19 @@ def test_mesh_fingerprint_ignores_thickness_only():
     b = M.build_arrays(bdf.bdf_to_model_text(decks.plate_deck(nx=5, ny=4, t=2.5)))
     c = M.build_arrays(bdf.bdf_to_model_text(decks.plate_deck(nx=5, ny=4, t=1.5, line_load=2.0)))
     assert W.mesh_fingerprint(a) == W.mesh_fingerprint(b) != W.mesh_fingerprint(c)
+
+
+def test_mesh_fingerprint_sees_constraints_and_range():
+    import copy
+    g = M.build_arrays(bdf.bdf_to_model_text(decks.glued_plate_deck(nx=4, ny=2)))
+    fewer = copy.copy(g)
+    fewer.rbe2 = g.rbe2[:-1]
+    ranged = copy.copy(g)
+    ranged.v1, ranged.v2 = 0.5, 50.0
+    thick = g.with_shell_thickness(g.shell_thickness() * 1.1)
+    prints = [W.mesh_fingerprint(x) for x in (g, fewer, ranged)]
+    assert len(set(prints)) == 3 and W.mesh_fingerprint(thick) == prints[0]
+    r3 = M.build_arrays(bdf.bdf_to_model_text(decks.rbe3_load_plate_deck(nx=4, ny=2)))
+    heavier = copy.copy(r3)
+    heavier.rbe3_wt = r3.rbe3_wt * 2.0
+    assert W.mesh_fingerprint(r3) != W.mesh_fingerprint(heavier)
