@@ -3,7 +3,9 @@ restatement (oracle/recovery.py), on the device's own static displacement.
 
 Models: a mixed CQUAD4/CTRIA3 plate (CTRIA3 macro blend weights 0.5 and 1),
 a stringer panel with CBARs and CTRIA3s, a faceted cylinder (element frames
-off the basic axes), and the bundled deck.  Element rows: every quantity
+off the basic axes), the bundled deck, and three constraint decks (a plate
+glued from two halves by RBE2, eccentric CBAR stringers tied by RBE2, an
+edge load through an RBE3).  Element rows: every quantity
 (N, M, Q, fibre stresses, fibre strains, corner N, corner M; bar forces,
 bar stresses) within 1e-9 of its largest magnitude over the model (FP64 in
 a different order of operations).  Reactions: within 1e-10 of
@@ -36,6 +38,9 @@ CASES = {
     "panel_bars": lambda: decks.panel_deck(nx=24, ny=12, stringer_every=4, nd=4, tri_frac=0.2),
     "cylinder": lambda: decks.cylinder_deck(n_circ=32, n_axial=12, nd=3),
     "bundled": lambda: open(os.path.join(HERE, "golden", "ref_sol105_quad_buckling.bdf")).read(),
+    "glued_rbe2": lambda: decks.glued_plate_deck(nx=12, ny=8, nd=3),
+    "eccentric_rbe2": lambda: decks.eccentric_panel_deck(nx=16, ny=8, stringer_every=4, nd=3),
+    "rbe3_load": lambda: decks.rbe3_load_plate_deck(nx=10, ny=8, nd=3),
 }
 
 
@@ -73,7 +78,11 @@ def test_recovery_parity(ctx, name):
     finally:
         dm.free()
     K = O.assemble_K(arr)
-    rres, rq4, rt3, rbar = RC.recover(arr, u, K=K, bar_points=arr.bar_points)
+    # with RBE2 / RBE3 the residual is K' u - F' on the redistributed K and
+    # load (solve_case.jl:853-866); element rows see the recovered u
+    cmap = O.constraint_map(arr)
+    rarr = O.constrained_arrays(arr, cmap) if cmap else arr
+    rres, rq4, rt3, rbar = RC.recover(rarr, u, K=K, bar_points=arr.bar_points)
     scale = (abs(K) @ np.abs(u)).max()
     assert np.abs(res - rres).max() <= 1e-10 * scale
     _close_cols(q4, rq4)
@@ -83,7 +92,7 @@ def test_recovery_parity(ctx, name):
         w = RC.t3_macro_weights(arr.q4, arr.t3)
         assert (w > 0).any()
     # reactions balance the load; free rows carry only round-off
-    F = np.asarray(arr.F)
+    F = np.asarray(rarr.F)
     for c in range(3):
         assert abs(res[c::6].sum() + F[c::6].sum()) <= 1e-8 * np.abs(F).sum()
     assert np.abs(res[free]).max() <= 1e-9 * scale
