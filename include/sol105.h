@@ -78,6 +78,15 @@ typedef struct sol105_model_desc {
     const double *rbe3_wt;      /* n_rbe3_grp weights                             */
     int32_t n_rbe3_node;
     const int32_t *rbe3_node;   /* node ranks of the groups                       */
+    /* Explicit MPC equations sum_i A_i u(G_i, C_i) = 0, the first term the
+     * dependent DOF (constraints.jl:290-316): u_dep = sum_{i>1} -A_i/A_1
+     * u_i.  An MPC replaces RBE2 / RBE3 entries of its dependent DOF, and
+     * a later MPC of the same DOF replaces an earlier one (card order). */
+    int32_t n_mpc;              /* equations                                      */
+    const int32_t *mpc_eq;      /* n_mpc x 2: first term, term count (>= 2)       */
+    int32_t n_mpc_term;
+    const int32_t *mpc_term;    /* n_mpc_term x 2: node rank, component 1..6      */
+    const double *mpc_coef;     /* n_mpc_term coefficients A_i                    */
 } sol105_model_desc;
 
 /* Native deck reader: Nastran deck text (executive and case control, bulk

@@ -66,6 +66,9 @@ class ModelDesc(ctypes.Structure):
         ("n_rbe3_grp", ctypes.c_int32), ("rbe3_grp", ctypes.POINTER(ctypes.c_int32)),
         ("rbe3_wt", ctypes.POINTER(ctypes.c_double)),
         ("n_rbe3_node", ctypes.c_int32), ("rbe3_node", ctypes.POINTER(ctypes.c_int32)),
+        ("n_mpc", ctypes.c_int32), ("mpc_eq", ctypes.POINTER(ctypes.c_int32)),
+        ("n_mpc_term", ctypes.c_int32), ("mpc_term", ctypes.POINTER(ctypes.c_int32)),
+        ("mpc_coef", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -200,7 +203,10 @@ def read_deck(text, buckling_index=0, with_sol=False):
             rbe2=arr(d.rbe2, 3 * d.n_rbe2, i32, (d.n_rbe2, 3)), rbe3=arr(d.rbe3, 4 * d.n_rbe3, i32, (d.n_rbe3, 4)),
             rbe3_grp=arr(d.rbe3_grp, 3 * d.n_rbe3_grp, i32, (d.n_rbe3_grp, 3)),
             rbe3_wt=arr(d.rbe3_wt, d.n_rbe3_grp, dbl, (d.n_rbe3_grp,)),
-            rbe3_node=arr(d.rbe3_node, d.n_rbe3_node, i32, (d.n_rbe3_node,)))
+            rbe3_node=arr(d.rbe3_node, d.n_rbe3_node, i32, (d.n_rbe3_node,)),
+            mpc_eq=arr(d.mpc_eq, 2 * d.n_mpc, i32, (d.n_mpc, 2)),
+            mpc_term=arr(d.mpc_term, 2 * d.n_mpc_term, i32, (d.n_mpc_term, 2)),
+            mpc_coef=arr(d.mpc_coef, d.n_mpc_term, dbl, (d.n_mpc_term,)))
         return (arrays, int(sz[8])) if with_sol else arrays
     finally:
         L.sol105_deck_free(h)
@@ -299,6 +305,9 @@ def _model_desc(arr):
         rbe3_grp=np.ascontiguousarray(arr.rbe3_grp, dtype=np.int32).reshape(-1),
         rbe3_wt=np.ascontiguousarray(arr.rbe3_wt, dtype=np.float64),
         rbe3_node=np.ascontiguousarray(arr.rbe3_node, dtype=np.int32),
+        mpc_eq=np.ascontiguousarray(arr.mpc_eq, dtype=np.int32).reshape(-1),
+        mpc_term=np.ascontiguousarray(arr.mpc_term, dtype=np.int32).reshape(-1),
+        mpc_coef=np.ascontiguousarray(arr.mpc_coef, dtype=np.float64),
     )
     k = keep
     ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
@@ -309,7 +318,8 @@ def _model_desc(arr):
                   float(arr.k6rot), int(arr.autospc), int(arr.nd),
                   float(arr.v1), float(arr.v2), int(arr.n_shell_cards),
                   len(arr.rbe2), ip(k["rbe2"]), len(arr.rbe3), ip(k["rbe3"]), len(arr.rbe3_grp), ip(k["rbe3_grp"]),
-                  _dp(k["rbe3_wt"]), len(arr.rbe3_node), ip(k["rbe3_node"]))
+                  _dp(k["rbe3_wt"]), len(arr.rbe3_node), ip(k["rbe3_node"]),
+                  len(arr.mpc_eq), ip(k["mpc_eq"]), len(arr.mpc_term), ip(k["mpc_term"]), _dp(k["mpc_coef"]))
     return d, keep
 
 

@@ -3,7 +3,7 @@
 Builds the same model dictionary shape as the reference's ModelBuilder
 (``JFEM/src/ModelBuilder.jl:421-823``) for the cards this path supports:
 GRID, CQUAD4, CTRIA3, CBAR, CBEAM, PSHELL, PBARL, MAT1, SPC1, SPC, FORCE,
-EIGRL, PARAM, RBE2, RBE3, RBAR and the case-control keys SOL / SUBCASE / SPC / LOAD / METHOD /
+EIGRL, PARAM, RBE2, RBE3, RBAR, MPC and the case-control keys SOL / SUBCASE / SPC / LOAD / METHOD /
 STATSUB.  Field parsing follows ``JFEM/src/parsing/utilities.jl`` (small and
 large fixed fields, free-field commas, ``+``/``*`` continuations, Nastran
 exponent shorthand such as ``1.-5``).
@@ -16,8 +16,8 @@ membrane only), ``:40-190`` (PBARL BAR section constants), ``:540-560``
 ``:465-480`` (EIGRL, ND defaults to 3).  RBE2, RBE3 and RBAR follow
 ``JFEM/src/parsing/extract_constraints.jl:46-141`` (RBE3 weight groups: a
 field with '.', an exponent or an inner sign starts a new group; continuation
-markers are skipped) and RBAR becomes an RBE2 as ``ModelBuilder.jl:708-725``
-converts it.
+markers are skipped), RBAR becomes an RBE2 as ``ModelBuilder.jl:708-725``
+converts it, and MPC follows ``extract_constraints.jl:251-271``.
 """
 from __future__ import annotations
 
@@ -385,8 +385,24 @@ def cards_to_model(cards, cc):
         eid = _to_id(c[0] if c else "")
         model["RBE3s"][str(eid)] = {"ID": eid, "REFGRID": _to_id(c[2] if len(c) > 2 else ""),
                                     "REFC": _to_id(c[3] if len(c) > 3 else ""), "WT_GROUPS": _rbe3_groups(c)}
+    # MPC: SID, then (G, C, A) triples while a whole triple remains; terms
+    # with G > 0 and C > 0 kept, equations with two or more terms kept
+    # (extract_constraints.jl:251-271); every MPC card applies (constraints.jl:291)
+    model["MPCs"] = []
+    for c in cards.get("MPC", []):
+        terms = []
+        k = 1
+        while k + 2 < len(c):
+            gid, comp = _to_id(c[k]), _to_id(c[k + 1])
+            coeff = float(parse_number(c[k + 2], 0.0) or 0.0)
+            if gid > 0 and comp > 0:
+                terms.append({"G": gid, "C": comp, "A": coeff})
+            k += 3
+        if len(terms) >= 2:
+            model["MPCs"].append({"SID": _to_id(c[0] if c else ""), "TERMS": terms})
     unsupported = sorted(set(cards) - {"GRID", "CQUAD4", "CTRIA3", "CBAR", "CBEAM", "PSHELL", "PBARL",
-                                       "MAT1", "SPC1", "SPC", "FORCE", "EIGRL", "PARAM", "RBE2", "RBE3", "RBAR"})
+                                       "MAT1", "SPC1", "SPC", "FORCE", "EIGRL", "PARAM", "RBE2", "RBE3", "RBAR",
+                                       "MPC"})
     model["_UNSUPPORTED_CARDS"] = unsupported
     return model
 

@@ -1,4 +1,4 @@
-// constraints.cpp -- RBE2 / RBE3 constraint map, pattern pairs and the
+// constraints.cpp -- RBE2 / RBE3 / MPC constraint map, pattern pairs and the
 // deterministic redistribution plan (host code).
 //
 // Restates JFEM/src/solver/constraints.jl:37-364 for RBE2 (and RBAR as
@@ -198,6 +198,31 @@ int build_constraint_map(const sol105_model_desc *d, ConstraintMap &C, std::stri
         }
     }
     for (auto &kv : rbe2m) rbe3m[kv.first] = kv.second;
+    // explicit MPC equations (constraints.jl:290-316): u_dep = sum -A_i/A_1
+    // u_i; they replace RBE2 / RBE3 entries, a later equation an earlier one
+    if (d->n_mpc > 0 && (!d->mpc_eq || !d->mpc_term || !d->mpc_coef)) {
+        err = "missing MPC arrays";
+        return SOL105_E_INVALID;
+    }
+    for (int e = 0; e < d->n_mpc; e++) {
+        const int t0 = d->mpc_eq[2 * (size_t)e], nt = d->mpc_eq[2 * (size_t)e + 1];
+        bool ok = t0 >= 0 && nt >= 2 && t0 + nt <= d->n_mpc_term;
+        for (int t = t0; ok && t < t0 + nt; t++) {
+            const int g = d->mpc_term[2 * (size_t)t], c = d->mpc_term[2 * (size_t)t + 1];
+            ok = g >= 0 && g < n && c >= 1 && c <= 6;
+        }
+        if (!ok || !(std::fabs(d->mpc_coef[t0]) >= 1e-30)) {
+            std::snprintf(buf, sizeof buf, "MPC %d: terms out of range or a zero dependent coefficient", e);
+            err = buf;
+            return SOL105_E_INVALID;
+        }
+        const double a0 = d->mpc_coef[t0];
+        std::vector<std::pair<int32_t, double>> pairs;
+        for (int t = t0 + 1; t < t0 + nt; t++)
+            pairs.emplace_back(6 * d->mpc_term[2 * (size_t)t] + d->mpc_term[2 * (size_t)t + 1] - 1,
+                               -d->mpc_coef[t] / a0);
+        rbe3m[6 * d->mpc_term[2 * (size_t)t0] + d->mpc_term[2 * (size_t)t0 + 1] - 1] = std::move(pairs);
+    }
     C.ptr.push_back(0);
     for (auto &kv : rbe3m) {
         C.dep.push_back(kv.first);

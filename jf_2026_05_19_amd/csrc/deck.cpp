@@ -14,7 +14,7 @@
 //   * GRID, CQUAD4, CTRIA3, CBAR, CBEAM, PSHELL (blank MID2: membrane only),
 //     PBARL TYPE=BAR (section constants, Cowper shear factor at nu = 0.3),
 //     MAT1 (completion of E, G, nu), SPC1 (THRU), SPC, FORCE, EIGRL, PARAM,
-//     RBE2, RBE3 and RBAR (as an RBE2);
+//     RBE2, RBE3, RBAR (as an RBE2) and MPC;
 //   * case control SOL / SUBCASE / key = value, global keys inherited by the
 //     subcases, the buckling and static subcase pairing (METHOD / STATSUB /
 //     LOAD / SPC);
@@ -47,6 +47,8 @@ struct sol105_deck {
     std::vector<int64_t> node_ids, q4_eid, q4_pid, t3_eid, t3_pid, bar_eid;
     std::vector<int32_t> rbe2, rbe3, rbe3_grp, rbe3_node;  // constraint arrays (sol105_model_desc)
     std::vector<double> rbe3_wt;
+    std::vector<int32_t> mpc_eq, mpc_term;  // explicit MPC equations (sol105_model_desc)
+    std::vector<double> mpc_coef;
     std::vector<double> xyz, q4prop, t3prop, bar_vref, barprop, bar_points, force;
     std::vector<int32_t> q4, t3, bar;
     std::vector<uint8_t> spc_mask, has_shell, has_bar;
@@ -485,7 +487,8 @@ std::string fmt_g(double x) {
 
 void build(const Cards &cards, const CaseControl &cc, int buckling_index, sol105_deck &D) {
     static const char *supported[] = {"GRID", "CQUAD4", "CTRIA3", "CBAR", "CBEAM", "PSHELL", "PBARL",
-                                      "MAT1", "SPC1", "SPC", "FORCE", "EIGRL", "PARAM", "RBE2", "RBE3", "RBAR"};
+                                      "MAT1", "SPC1", "SPC", "FORCE", "EIGRL", "PARAM", "RBE2", "RBE3", "RBAR",
+                                      "MPC"};
     {
         std::string list;
         for (const auto &kv : cards.by_name) {
@@ -779,6 +782,44 @@ void build(const Cards &cards, const CaseControl &cc, int buckling_index, sol105
             D.rbe3_wt.push_back(g.wt);
         }
         D.rbe3.insert(D.rbe3.end(), {ref, (int32_t)kv.second.refc, g0, (int32_t)D.rbe3_wt.size() - g0});
+    }
+    // MPC (extract_constraints.jl:251-271): (G, C, A) triples after the SID
+    // while a whole triple remains, terms with G > 0 and C > 0, equations of
+    // two or more terms; in card order, filtered as constraints.jl:294-312
+    D.mpc_eq.clear();
+    D.mpc_term.clear();
+    D.mpc_coef.clear();
+    for (const CardView &c : get("MPC")) {
+        struct Term { int64_t g, comp; double a; };
+        std::vector<Term> terms;
+        for (size_t k = 1; k + 2 < c.size(); k += 3) {
+            const int64_t g = id_at(c, k), comp = id_at(c, k + 1);
+            const double a = real_or(c, k + 2, 0.0);
+            if (g > 0 && comp > 0) terms.push_back({g, comp, a});
+        }
+        if (terms.size() < 2) continue;
+        auto rank_id = [&](int64_t g, int &r) {
+            Num x;
+            x.kind = Num::INT;
+            x.i = g;
+            return rank_of(x, r);
+        };
+        int r0;
+        if (!rank_id(terms[0].g, r0) || terms[0].comp < 1 || terms[0].comp > 6 || !(std::fabs(terms[0].a) >= 1e-30))
+            continue;
+        std::vector<std::pair<int, const Term *>> rest;
+        for (size_t t = 1; t < terms.size(); t++) {
+            int r;
+            if (rank_id(terms[t].g, r) && terms[t].comp >= 1 && terms[t].comp <= 6) rest.push_back({r, &terms[t]});
+        }
+        if (rest.empty()) continue;
+        D.mpc_eq.insert(D.mpc_eq.end(), {(int32_t)(D.mpc_term.size() / 2), (int32_t)(1 + rest.size())});
+        D.mpc_term.insert(D.mpc_term.end(), {r0, (int32_t)terms[0].comp});
+        D.mpc_coef.push_back(terms[0].a);
+        for (const auto &rt : rest) {
+            D.mpc_term.insert(D.mpc_term.end(), {rt.first, (int32_t)rt.second->comp});
+            D.mpc_coef.push_back(rt.second->a);
+        }
     }
     std::vector<Spc> spcs;
     for (const CardView &c : get("SPC1")) {
@@ -1150,6 +1191,11 @@ int sol105_deck_desc(const sol105_deck *D, sol105_model_desc *d) {
     d->rbe3_wt = D->rbe3_wt.data();
     d->n_rbe3_node = (int32_t)D->rbe3_node.size();
     d->rbe3_node = D->rbe3_node.data();
+    d->n_mpc = (int32_t)(D->mpc_eq.size() / 2);
+    d->mpc_eq = D->mpc_eq.data();
+    d->n_mpc_term = (int32_t)D->mpc_coef.size();
+    d->mpc_term = D->mpc_term.data();
+    d->mpc_coef = D->mpc_coef.data();
     return SOL105_OK;
 }
 

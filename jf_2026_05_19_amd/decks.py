@@ -242,10 +242,12 @@ def _plate_bc_and_load(L, nx, ny, b, gid_at, line_load, load_edge=True):
             L.append(_card("FORCE", 10, gid_at(nx, j), 0, line_load * dy * w, -1.0, 0.0, 0.0))
 
 
-def glued_plate_deck(nx=12, ny=8, a=400.0, b=300.0, t=1.5, E=70e3, nu=0.33, line_load=1.0, nd=4):
+def glued_plate_deck(nx=12, ny=8, a=400.0, b=300.0, t=1.5, E=70e3, nu=0.33, line_load=1.0, nd=4, tie="rbe2"):
     """plate_deck's plate (nx even) as two half meshes: the cells right of
     x = a/2 use their own GRIDs (ids + 100000) and each seam GRID of the
-    right half is the RBE2 (123456) slave of the coincident left one."""
+    right half is the RBE2 (123456) slave of the coincident left one
+    (tie="rbe2"), or tied to it by six MPC equations u_r - u_l = 0
+    (tie="mpc")."""
     assert nx % 2 == 0
     L = _header(nd)
     L.append(_card("MAT1", 1, float(E), "", float(nu)))
@@ -267,7 +269,11 @@ def glued_plate_deck(nx=12, ny=8, a=400.0, b=300.0, t=1.5, E=70e3, nu=0.33, line
             L.append(_card("CQUAD4", eid, 1, g(i, j), g(i + 1, j), g(i + 1, j + 1), g(i, j + 1)))
             eid += 1
     for j in range(ny + 1):
-        L.append(_card("RBE2", 900000 + j, gl(h, j), 123456, gr(h, j)))
+        if tie == "mpc":
+            for c in range(1, 7):
+                L.append(_card("MPC", 1, gr(h, j), c, 1.0, gl(h, j), c, -1.0))
+        else:
+            L.append(_card("RBE2", 900000 + j, gl(h, j), 123456, gr(h, j)))
     _plate_bc_and_load(L, nx, ny, b, gid_at, line_load)
     L.append("ENDDATA")
     return "\n".join(L) + "\n"

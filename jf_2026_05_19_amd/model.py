@@ -74,6 +74,9 @@ class Sol105Arrays:
     rbe3_grp: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.int32))  # comps, first node, nodes
     rbe3_wt: np.ndarray = field(default_factory=lambda: np.zeros(0))
     rbe3_node: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    mpc_eq: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int32))    # first term, terms
+    mpc_term: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int32))  # node rank, component
+    mpc_coef: np.ndarray = field(default_factory=lambda: np.zeros(0))
 
     @property
     def n_nodes(self):
@@ -274,6 +277,25 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
             rbe3_wt.append(float(grp["wt"]))
             rbe3_node.extend(nodes)
         rbe3.append([rank[r["REFGRID"]], int(r["REFC"]), g0, len(rbe3_grp) - g0])
+    # explicit MPC equations in card order, filtered as constraints.jl:
+    # 294-312: the dependent term needs a known GRID, a component 1..6 and
+    # |A| >= 1e-30; other terms with an unknown GRID or component are
+    # skipped; an equation left without independent terms is dropped
+    mpc_eq, mpc_term, mpc_coef = [], [], []
+    for mpc in model.get("MPCs", []):
+        terms = mpc["TERMS"]
+        if len(terms) < 2:
+            continue
+        d0 = terms[0]
+        if d0["G"] not in rank or not 1 <= d0["C"] <= 6 or abs(d0["A"]) < 1e-30:
+            continue
+        rest = [t for t in terms[1:] if t["G"] in rank and 1 <= t["C"] <= 6]
+        if not rest:
+            continue
+        mpc_eq.append([len(mpc_term), 1 + len(rest)])
+        for t in [d0] + rest:
+            mpc_term.append([rank[t["G"]], int(t["C"])])
+            mpc_coef.append(float(t["A"]))
     ndof = 6 * n
     cc = model["CASE_CONTROL"]
     pairs = _subcase_pairing(cc)
@@ -361,4 +383,6 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
         rbe2=np.asarray(rbe2, dtype=np.int32).reshape(-1, 3), rbe3=np.asarray(rbe3, dtype=np.int32).reshape(-1, 4),
         rbe3_grp=np.asarray(rbe3_grp, dtype=np.int32).reshape(-1, 3), rbe3_wt=np.asarray(rbe3_wt, dtype=float),
         rbe3_node=np.asarray(rbe3_node, dtype=np.int32),
+        mpc_eq=np.asarray(mpc_eq, dtype=np.int32).reshape(-1, 2),
+        mpc_term=np.asarray(mpc_term, dtype=np.int32).reshape(-1, 2), mpc_coef=np.asarray(mpc_coef, dtype=float),
     )

@@ -296,6 +296,15 @@ def constraint_map(arr):
                 rbe2m[6 * int(sl) + c - 1] = pairs
     out = dict(rbe3m)
     out.update(rbe2m)
+    # explicit MPC equations, card order (constraints.jl:290-316): the first
+    # term is dependent, u_dep = sum -A_i / A_1 u_i over the others
+    eqs = np.asarray(getattr(arr, "mpc_eq", np.zeros((0, 2), np.int32))).reshape(-1, 2)
+    for t0, nt in eqs:
+        g0, c0 = arr.mpc_term[t0]
+        a0 = float(arr.mpc_coef[t0])
+        out[6 * int(g0) + int(c0) - 1] = [(6 * int(g) + int(c) - 1, -float(a) / a0)
+                                          for (g, c), a in zip(arr.mpc_term[t0 + 1:t0 + nt],
+                                                               arr.mpc_coef[t0 + 1:t0 + nt])]
     return out
 
 
@@ -750,7 +759,8 @@ def localization_filter(arr, free, V, order, num_modes, n_shell_cards, max_share
 
 
 def _has_constraints(arr):
-    return len(getattr(arr, "rbe2", ())) > 0 or len(getattr(arr, "rbe3", ())) > 0
+    return (len(getattr(arr, "rbe2", ())) > 0 or len(getattr(arr, "rbe3", ())) > 0
+            or len(getattr(arr, "mpc_eq", ())) > 0)
 
 
 def solve(arr):

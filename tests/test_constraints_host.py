@@ -1,4 +1,4 @@
-"""RBE2 / RBE3 constraints on the CPU: pins of the oracle's restatement of
+"""RBE2 / RBE3 / MPC constraints on the CPU: pins of the oracle's restatement of
 the reference's dependent-DOF elimination (JFEM/src/solver/constraints.jl:
 37-364, the load redistribution of solve_case.jl:40-55 and the recovery of
 boundary_conditions.jl:978-988, solve_case.jl:3153-3162), the library's host
@@ -11,6 +11,10 @@ Pins (expectations from mechanics, not from the restatement):
 * a cantilever CBAR with an RBE2 arm of length e at its tip, loaded at the
   arm's end: tip deflection P L^3/(3 E I) + P L/(k G A), twist e P L/(G J),
   and the arm end moves rigidly with the tip (w + e theta_x);
+* two half meshes tied by MPC equations u_r - u_l = 0 are the one-piece
+  plate; a load at a GRID defined by an MPC average of two GRIDs acts as
+  half the load at each of them; an MPC replaces the RBE2 entries of its
+  DOF, and a later MPC an earlier one;
 * RBE3: a rigid motion of the independent grids gives the reference the
   same rigid motion (the least-squares fit is exact for it), and a load at
   the reference spreads into grid forces with the same resultant force and
@@ -133,6 +137,48 @@ def test_rbe3_reproduces_rigid_motion_and_balances_load():
     assert np.allclose(mom, Fr[3:], rtol=1e-10, atol=1e-10)
 
 
+def test_mpc_glued_halves_equal_one_piece_plate():
+    g = _arr(decks.glued_plate_deck(nx=12, ny=8, tie="mpc"))
+    p = _arr(decks.plate_deck(nx=12, ny=8, a=400.0, b=300.0, t=1.5, nd=4))
+    assert len(g.mpc_eq) == 6 * 9 and len(g.rbe2) == 0
+    rg, rp = O.solve(g), O.solve(p)
+    assert np.allclose(rg["eigenvalues"], rp["eigenvalues"], rtol=1e-10, atol=0)
+
+
+def test_mpc_equation_moves_a_load_by_its_coefficients():
+    """u_R1 = 0.5 u_A1 + 0.5 u_B1 (MPC 2 u_R1 - u_A1 - u_B1 = 0): a force P
+    at the otherwise free GRID R acts as P/2 at A and at B (the transposed
+    equation), so the model equals the plate loaded at A and B directly,
+    and R moves with the average of A and B."""
+    nx, ny, P = 10, 8, -150.0
+    base = decks.plate_deck(nx=nx, ny=ny, a=400.0, b=300.0, t=1.5, nd=4)
+    A, B = 3 * (nx + 1) + nx + 1, 5 * (nx + 1) + nx + 1     # GRIDs on the loaded edge
+    viampc = base.replace("ENDDATA", "GRID,999999,,410.0,150.0,0.0\nMPC,1,999999,1,2.0,%d,1,-1.0,%d,1,-1.0\n"
+                          "FORCE,10,999999,0,%r,1.0,0.0,0.0\nENDDATA" % (A, B, P))
+    direct = base.replace("ENDDATA", "FORCE,10,%d,0,%r,1.0,0.0,0.0\nFORCE,10,%d,0,%r,1.0,0.0,0.0\nENDDATA"
+                          % (A, P / 2, B, P / 2))
+    m, d = _arr(viampc), _arr(direct)
+    rm, rd = O.solve(m), O.solve(d)
+    assert np.allclose(rm["eigenvalues"], rd["eigenvalues"], rtol=1e-11, atol=0)
+    nd = 6 * d.n_nodes                                       # GRID 999999 is the last rank
+    assert np.allclose(rm["u"][:nd], rd["u"], rtol=0, atol=1e-12 * np.abs(rd["u"]).max())
+    ia, ib = 6 * (A - 1), 6 * (B - 1)
+    assert np.isclose(rm["u"][nd], 0.5 * (rm["u"][ia] + rm["u"][ib]), rtol=1e-14)
+
+
+def test_mpc_replaces_rbe2_entries_and_last_card_wins():
+    txt = decks.glued_plate_deck(nx=4, ny=2)
+    h = 2
+    gl, gr = h + 1, 100000 + h + 1                            # seam GRIDs of row 0
+    extra = "MPC,7,%d,3,1.0,%d,3,-0.5\nMPC,8,%d,3,1.0,%d,3,-0.25\nENDDATA" % (gr, gl, gr, gl)
+    arr = _arr(txt.replace("ENDDATA", extra))
+    cmap = O.constraint_map(arr)
+    rank = {int(g): i for i, g in enumerate(arr.node_ids)}
+    dep = 6 * rank[gr] + 2
+    assert cmap[dep] == [(6 * rank[gl] + 2, 0.25)]           # the later MPC, not the RBE2 row
+    assert cmap[6 * rank[gr] + 3] == [(6 * rank[gl] + 3, 1.0)]  # other components keep the RBE2
+
+
 # --------------------------------------------------------------------------
 # the library's constraint code (host build) against the oracle
 # --------------------------------------------------------------------------
@@ -155,7 +201,10 @@ def _desc(arr):
 
 DECKS = [("glued", lambda: decks.glued_plate_deck(nx=8, ny=4)),
          ("eccentric", lambda: decks.eccentric_panel_deck(nx=8, ny=8, stringer_every=4)),
-         ("rbe3_load", lambda: decks.rbe3_load_plate_deck(nx=6, ny=4))]
+         ("rbe3_load", lambda: decks.rbe3_load_plate_deck(nx=6, ny=4)),
+         ("glued_mpc", lambda: decks.glued_plate_deck(nx=8, ny=4, tie="mpc")),
+         ("mpc_over_rbe2", lambda: decks.glued_plate_deck(nx=4, ny=2).replace(
+             "ENDDATA", "MPC,7,100003,3,1.0,3,3,-0.5,5,3,0.25\nMPC,8,100003,3,2.0,3,3,-1.0\nENDDATA"))]
 
 
 @pytest.mark.parametrize("name,deck", DECKS, ids=[d[0] for d in DECKS])
@@ -184,6 +233,7 @@ def test_library_map_and_redistribution_match_oracle(probe, name, deck):
     plain = copy.copy(arr)
     plain.rbe2 = np.zeros((0, 3), np.int32)
     plain.rbe3 = np.zeros((0, 4), np.int32)
+    plain.mpc_eq = np.zeros((0, 2), np.int32)
     K0 = O.assemble_K(plain).toarray()
     K1 = O.assemble_K(arr).toarray()
     out = np.zeros_like(K0)
@@ -215,6 +265,7 @@ def test_native_reader_matches_host_builder(name, deck):
     from jf_2026_05_19_amd import _native as N
     txt = deck()
     a, b = _arr(txt), N.read_deck(txt)
-    for f in ("rbe2", "rbe3", "rbe3_grp", "rbe3_wt", "rbe3_node", "xyz", "F", "spc_mask", "q4", "bar"):
+    for f in ("rbe2", "rbe3", "rbe3_grp", "rbe3_wt", "rbe3_node", "mpc_eq", "mpc_term", "mpc_coef", "xyz", "F",
+              "spc_mask", "q4", "bar"):
         x, y = getattr(a, f), getattr(b, f)
         assert x.shape == y.shape and np.array_equal(x, y), (name, f)

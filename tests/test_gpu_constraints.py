@@ -1,7 +1,8 @@
-"""GPU: RBE2 / RBE3 constraints (dependent-DOF elimination) through the C
+"""GPU: RBE2 / RBE3 / MPC constraints (dependent-DOF elimination) through the C
 ABI against the oracle (constraints.jl:37-364 restated in oracle.py):
 
-* glued half meshes (RBE2 123456 pairs of coincident GRIDs), a plate with
+* glued half meshes (RBE2 123456 pairs of coincident GRIDs, or six MPC
+  equations per pair), a plate with
   eccentric CBAR stringers on their own GRIDs tied by RBE2, and a plate
   loaded through an RBE3 at one reference GRID;
 * K and KG (redistributed, dependent rows and columns cleared) within 1e-12
@@ -22,6 +23,7 @@ CASES = {
     "glued": lambda: decks.glued_plate_deck(nx=12, ny=8),
     "eccentric": lambda: decks.eccentric_panel_deck(nx=16, ny=8, stringer_every=4),
     "rbe3_load": lambda: decks.rbe3_load_plate_deck(nx=10, ny=8),
+    "glued_mpc": lambda: decks.glued_plate_deck(nx=12, ny=8, tie="mpc"),
 }
 
 
@@ -85,12 +87,13 @@ def test_buckling_parity(solved, name):
 
 def test_glued_equals_one_piece(solved, ctx):
     from jf_2026_05_19_amd import _native as N
-    arr, dm, ref = solved["glued"]
-    lam_g, _ = dm.solve()
     dp = N.DeviceModel(ctx, _arr(decks.plate_deck(nx=12, ny=8, a=400.0, b=300.0, t=1.5, nd=4)))
     lam_p, _ = dp.solve()
     dp.free()
-    assert np.allclose(lam_g, lam_p, rtol=1e-9, atol=0), (lam_g, lam_p)
+    for name in ("glued", "glued_mpc"):
+        arr, dm, ref = solved[name]
+        lam_g, _ = dm.solve()
+        assert np.allclose(lam_g, lam_p, rtol=1e-9, atol=0), (name, lam_g, lam_p)
 
 
 def test_batch_and_noalloc_refuse_constraints(solved, ctx):
