@@ -87,6 +87,17 @@ typedef struct sol105_model_desc {
     int32_t n_mpc_term;
     const int32_t *mpc_term;    /* n_mpc_term x 2: node rank, component 1..6      */
     const double *mpc_coef;     /* n_mpc_term coefficients A_i                    */
+    /* RSPLINE (constraints.jl:146-213): each dependent (GRID, component) is
+     * projected on the polyline of the independent GRIDs and follows the
+     * two ends of the nearest segment linearly.  Replaces RBE2 / RBE3
+     * entries; MPC equations replace it. */
+    int32_t n_rspline;
+    const int32_t *rspline;     /* n_rspline x 4: first independent, count,       */
+                                /*   first dependent, count                       */
+    int32_t n_rspline_indep;
+    const int32_t *rspline_indep; /* independent node ranks, polyline order       */
+    int32_t n_rspline_dep;
+    const int32_t *rspline_dep; /* n_rspline_dep x 2: node rank, component 1..6   */
 } sol105_model_desc;
 
 /* Native deck reader: Nastran deck text (executive and case control, bulk

@@ -69,6 +69,9 @@ class ModelDesc(ctypes.Structure):
         ("n_mpc", ctypes.c_int32), ("mpc_eq", ctypes.POINTER(ctypes.c_int32)),
         ("n_mpc_term", ctypes.c_int32), ("mpc_term", ctypes.POINTER(ctypes.c_int32)),
         ("mpc_coef", ctypes.POINTER(ctypes.c_double)),
+        ("n_rspline", ctypes.c_int32), ("rspline", ctypes.POINTER(ctypes.c_int32)),
+        ("n_rspline_indep", ctypes.c_int32), ("rspline_indep", ctypes.POINTER(ctypes.c_int32)),
+        ("n_rspline_dep", ctypes.c_int32), ("rspline_dep", ctypes.POINTER(ctypes.c_int32)),
     ]
 
 
@@ -206,7 +209,10 @@ def read_deck(text, buckling_index=0, with_sol=False):
             rbe3_node=arr(d.rbe3_node, d.n_rbe3_node, i32, (d.n_rbe3_node,)),
             mpc_eq=arr(d.mpc_eq, 2 * d.n_mpc, i32, (d.n_mpc, 2)),
             mpc_term=arr(d.mpc_term, 2 * d.n_mpc_term, i32, (d.n_mpc_term, 2)),
-            mpc_coef=arr(d.mpc_coef, d.n_mpc_term, dbl, (d.n_mpc_term,)))
+            mpc_coef=arr(d.mpc_coef, d.n_mpc_term, dbl, (d.n_mpc_term,)),
+            rspline=arr(d.rspline, 4 * d.n_rspline, i32, (d.n_rspline, 4)),
+            rspline_indep=arr(d.rspline_indep, d.n_rspline_indep, i32, (d.n_rspline_indep,)),
+            rspline_dep=arr(d.rspline_dep, 2 * d.n_rspline_dep, i32, (d.n_rspline_dep, 2)))
         return (arrays, int(sz[8])) if with_sol else arrays
     finally:
         L.sol105_deck_free(h)
@@ -308,6 +314,9 @@ def _model_desc(arr):
         mpc_eq=np.ascontiguousarray(arr.mpc_eq, dtype=np.int32).reshape(-1),
         mpc_term=np.ascontiguousarray(arr.mpc_term, dtype=np.int32).reshape(-1),
         mpc_coef=np.ascontiguousarray(arr.mpc_coef, dtype=np.float64),
+        rspline=np.ascontiguousarray(arr.rspline, dtype=np.int32).reshape(-1),
+        rspline_indep=np.ascontiguousarray(arr.rspline_indep, dtype=np.int32),
+        rspline_dep=np.ascontiguousarray(arr.rspline_dep, dtype=np.int32).reshape(-1),
     )
     k = keep
     ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
@@ -319,7 +328,9 @@ def _model_desc(arr):
                   float(arr.v1), float(arr.v2), int(arr.n_shell_cards),
                   len(arr.rbe2), ip(k["rbe2"]), len(arr.rbe3), ip(k["rbe3"]), len(arr.rbe3_grp), ip(k["rbe3_grp"]),
                   _dp(k["rbe3_wt"]), len(arr.rbe3_node), ip(k["rbe3_node"]),
-                  len(arr.mpc_eq), ip(k["mpc_eq"]), len(arr.mpc_term), ip(k["mpc_term"]), _dp(k["mpc_coef"]))
+                  len(arr.mpc_eq), ip(k["mpc_eq"]), len(arr.mpc_term), ip(k["mpc_term"]), _dp(k["mpc_coef"]),
+                  len(arr.rspline), ip(k["rspline"]), len(arr.rspline_indep), ip(k["rspline_indep"]),
+                  len(arr.rspline_dep), ip(k["rspline_dep"]))
     return d, keep
 
 

@@ -3,7 +3,7 @@
 Builds the same model dictionary shape as the reference's ModelBuilder
 (``JFEM/src/ModelBuilder.jl:421-823``) for the cards this path supports:
 GRID, CQUAD4, CTRIA3, CBAR, CBEAM, PSHELL, PBARL, MAT1, SPC1, SPC, FORCE,
-EIGRL, PARAM, RBE2, RBE3, RBAR, MPC and the case-control keys SOL / SUBCASE / SPC / LOAD / METHOD /
+EIGRL, PARAM, RBE2, RBE3, RBAR, MPC, RSPLINE and the case-control keys SOL / SUBCASE / SPC / LOAD / METHOD /
 STATSUB.  Field parsing follows ``JFEM/src/parsing/utilities.jl`` (small and
 large fixed fields, free-field commas, ``+``/``*`` continuations, Nastran
 exponent shorthand such as ``1.-5``).
@@ -17,7 +17,8 @@ membrane only), ``:40-190`` (PBARL BAR section constants), ``:540-560``
 ``JFEM/src/parsing/extract_constraints.jl:46-141`` (RBE3 weight groups: a
 field with '.', an exponent or an inner sign starts a new group; continuation
 markers are skipped), RBAR becomes an RBE2 as ``ModelBuilder.jl:708-725``
-converts it, and MPC follows ``extract_constraints.jl:251-271``.
+converts it, MPC follows ``extract_constraints.jl:251-271`` and RSPLINE
+``:202-250``.
 """
 from __future__ import annotations
 
@@ -400,9 +401,35 @@ def cards_to_model(cards, cc):
             k += 3
         if len(terms) >= 2:
             model["MPCs"].append({"SID": _to_id(c[0] if c else ""), "TERMS": terms})
+    # RSPLINE: EID, D/L, then GRIDs in polyline order; a GRID followed by a
+    # component code (digits 0-6, at most 123456) is dependent in each
+    # non-zero digit, any other GRID independent (extract_constraints.jl:202-250)
+    model["RSPLINEs"] = {}
+    for c in cards.get("RSPLINE", []):
+        eid = _to_id(c[0] if c else "")
+        g1 = _to_id(c[2] if len(c) > 2 else "")
+        if eid <= 0 or g1 <= 0:
+            continue
+        indep, dep = [g1], []
+        k = 3
+        while k < len(c):
+            g = _to_id(c[k])
+            if g <= 0:
+                k += 1
+                continue
+            if k + 1 < len(c):
+                nxt = _to_id(c[k + 1]) if parse_number(c[k + 1], None) is not None else -1
+                if 0 < nxt <= 123456 and all(ch in "0123456" for ch in str(nxt)):
+                    dep.extend((g, int(ch)) for ch in str(nxt) if ch != "0")
+                    k += 2
+                    continue
+            indep.append(g)
+            k += 1
+        if dep and len(indep) >= 2:
+            model["RSPLINEs"][str(eid)] = {"ID": eid, "INDEP_GRIDS": indep, "DEP": dep}
     unsupported = sorted(set(cards) - {"GRID", "CQUAD4", "CTRIA3", "CBAR", "CBEAM", "PSHELL", "PBARL",
                                        "MAT1", "SPC1", "SPC", "FORCE", "EIGRL", "PARAM", "RBE2", "RBE3", "RBAR",
-                                       "MPC"})
+                                       "MPC", "RSPLINE"})
     model["_UNSUPPORTED_CARDS"] = unsupported
     return model
 

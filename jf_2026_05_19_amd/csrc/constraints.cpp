@@ -1,4 +1,4 @@
-// constraints.cpp -- RBE2 / RBE3 / MPC constraint map, pattern pairs and the
+// constraints.cpp -- RBE2 / RBE3 / RSPLINE / MPC constraint map, pattern pairs and the
 // deterministic redistribution plan (host code).
 //
 // Restates JFEM/src/solver/constraints.jl:37-364 for RBE2 (and RBAR as
@@ -198,6 +198,71 @@ int build_constraint_map(const sol105_model_desc *d, ConstraintMap &C, std::stri
         }
     }
     for (auto &kv : rbe2m) rbe3m[kv.first] = kv.second;
+    // RSPLINE (constraints.jl:146-213): a dependent point is projected on the
+    // polyline of the independent GRIDs (the first nearest segment, t
+    // clamped to [0, 1]) and follows that segment's ends with 1 - t and t
+    if (d->n_rspline > 0 && (!d->rspline || !d->rspline_indep || !d->rspline_dep)) {
+        err = "missing RSPLINE arrays";
+        return SOL105_E_INVALID;
+    }
+    std::map<int32_t, std::vector<std::pair<int32_t, double>>> rsplm;
+    for (int e = 0; e < d->n_rspline; e++) {
+        const int32_t *r = d->rspline + 4 * (size_t)e;
+        const int i0 = r[0], ni = r[1], d0 = r[2], ndp = r[3];
+        bool ok = i0 >= 0 && ni >= 0 && i0 + ni <= d->n_rspline_indep && d0 >= 0 && ndp >= 0 &&
+                  d0 + ndp <= d->n_rspline_dep;
+        for (int k = 0; ok && k < ni; k++) ok = d->rspline_indep[i0 + k] >= 0 && d->rspline_indep[i0 + k] < n;
+        for (int k = 0; ok && k < ndp; k++) {
+            const int g = d->rspline_dep[2 * (size_t)(d0 + k)], c = d->rspline_dep[2 * (size_t)(d0 + k) + 1];
+            ok = g >= 0 && g < n && c >= 1 && c <= 6;
+        }
+        if (!ok) {
+            std::snprintf(buf, sizeof buf, "RSPLINE %d: ranks or components out of range", e);
+            err = buf;
+            return SOL105_E_INVALID;
+        }
+        if (ni < 2) continue;
+        const int32_t *ind = d->rspline_indep + i0;
+        double total = 0.0;
+        for (int s = 1; s < ni; s++) {
+            const double dx = P(ind[s], 0) - P(ind[s - 1], 0), dy = P(ind[s], 1) - P(ind[s - 1], 1),
+                         dz = P(ind[s], 2) - P(ind[s - 1], 2);
+            total += std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+        if (total < 1e-30) continue;
+        for (int k = 0; k < ndp; k++) {
+            const int g = d->rspline_dep[2 * (size_t)(d0 + k)], c = d->rspline_dep[2 * (size_t)(d0 + k) + 1];
+            const double px = P(g, 0), py = P(g, 1), pz = P(g, 2);
+            int best = 0;
+            double bt = 0.0, bd = INFINITY;
+            for (int s = 0; s + 1 < ni; s++) {
+                const double ax = P(ind[s], 0), ay = P(ind[s], 1), az = P(ind[s], 2);
+                const double dx = P(ind[s + 1], 0) - ax, dy = P(ind[s + 1], 1) - ay, dz = P(ind[s + 1], 2) - az;
+                const double l2 = dx * dx + dy * dy + dz * dz;
+                if (l2 < 1e-30) continue;
+                const double t = std::min(std::max(((px - ax) * dx + (py - ay) * dy + (pz - az) * dz) / l2, 0.0), 1.0);
+                const double cx = ax + t * dx, cy = ay + t * dy, cz = az + t * dz;
+                const double dist = std::sqrt((px - cx) * (px - cx) + (py - cy) * (py - cy) + (pz - cz) * (pz - cz));
+                if (dist < bd) {
+                    best = s;
+                    bt = t;
+                    bd = dist;
+                }
+            }
+            std::vector<std::pair<int32_t, double>> pairs;
+            if (std::fabs(1.0 - bt) > 1e-15) pairs.emplace_back(6 * ind[best] + c - 1, 1.0 - bt);
+            if (std::fabs(bt) > 1e-15) pairs.emplace_back(6 * ind[best + 1] + c - 1, bt);
+            if (pairs.empty()) continue;
+            const int32_t dd = 6 * g + c - 1;
+            if (rsplm.count(dd)) {
+                std::snprintf(buf, sizeof buf, "RSPLINE: DOF %d is dependent in two elements", dd);
+                err = buf;
+                return SOL105_E_UNSUPPORTED;
+            }
+            rsplm.emplace(dd, std::move(pairs));
+        }
+    }
+    for (auto &kv : rsplm) rbe3m[kv.first] = kv.second;
     // explicit MPC equations (constraints.jl:290-316): u_dep = sum -A_i/A_1
     // u_i; they replace RBE2 / RBE3 entries, a later equation an earlier one
     if (d->n_mpc > 0 && (!d->mpc_eq || !d->mpc_term || !d->mpc_coef)) {

@@ -14,7 +14,7 @@
 //   * GRID, CQUAD4, CTRIA3, CBAR, CBEAM, PSHELL (blank MID2: membrane only),
 //     PBARL TYPE=BAR (section constants, Cowper shear factor at nu = 0.3),
 //     MAT1 (completion of E, G, nu), SPC1 (THRU), SPC, FORCE, EIGRL, PARAM,
-//     RBE2, RBE3, RBAR (as an RBE2) and MPC;
+//     RBE2, RBE3, RBAR (as an RBE2), RSPLINE and MPC;
 //   * case control SOL / SUBCASE / key = value, global keys inherited by the
 //     subcases, the buckling and static subcase pairing (METHOD / STATSUB /
 //     LOAD / SPC);
@@ -49,6 +49,7 @@ struct sol105_deck {
     std::vector<double> rbe3_wt;
     std::vector<int32_t> mpc_eq, mpc_term;  // explicit MPC equations (sol105_model_desc)
     std::vector<double> mpc_coef;
+    std::vector<int32_t> rspline, rspline_indep, rspline_dep;  // RSPLINE elements (sol105_model_desc)
     std::vector<double> xyz, q4prop, t3prop, bar_vref, barprop, bar_points, force;
     std::vector<int32_t> q4, t3, bar;
     std::vector<uint8_t> spc_mask, has_shell, has_bar;
@@ -488,7 +489,7 @@ std::string fmt_g(double x) {
 void build(const Cards &cards, const CaseControl &cc, int buckling_index, sol105_deck &D) {
     static const char *supported[] = {"GRID", "CQUAD4", "CTRIA3", "CBAR", "CBEAM", "PSHELL", "PBARL",
                                       "MAT1", "SPC1", "SPC", "FORCE", "EIGRL", "PARAM", "RBE2", "RBE3", "RBAR",
-                                      "MPC"};
+                                      "MPC", "RSPLINE"};
     {
         std::string list;
         for (const auto &kv : cards.by_name) {
@@ -782,6 +783,70 @@ void build(const Cards &cards, const CaseControl &cc, int buckling_index, sol105
             D.rbe3_wt.push_back(g.wt);
         }
         D.rbe3.insert(D.rbe3.end(), {ref, (int32_t)kv.second.refc, g0, (int32_t)D.rbe3_wt.size() - g0});
+    }
+    // RSPLINE (extract_constraints.jl:202-250): EID, D/L, then GRIDs in
+    // polyline order; a GRID followed by a component code (digits 0-6, at
+    // most 123456) is dependent in each non-zero digit, any other GRID is
+    // independent; kept with a dependent and two independent GRIDs
+    struct Rspline {
+        std::vector<int64_t> indep;
+        std::vector<std::pair<int64_t, int>> dep;
+    };
+    std::vector<std::pair<int64_t, Rspline>> rsplines;
+    for (const CardView &c : get("RSPLINE")) {
+        const int64_t eid = id_at(c, 0), g1 = id_at(c, 2);
+        if (eid <= 0 || g1 <= 0) continue;
+        Rspline r;
+        r.indep.push_back(g1);
+        size_t k = 3;
+        while (k < c.size()) {
+            const int64_t g = id_at(c, k);
+            if (g <= 0) {
+                k++;
+                continue;
+            }
+            if (k + 1 < c.size()) {
+                const Num nx = field(c, k + 1);
+                const int64_t nxt = nx.ok() ? to_id(nx) : -1;
+                if (nxt > 0 && nxt <= 123456) {
+                    const std::string sd = std::to_string(nxt);
+                    if (sd.find_first_not_of("0123456") == std::string::npos) {
+                        for (char ch : sd)
+                            if (ch != '0') r.dep.push_back({g, ch - '0'});
+                        k += 2;
+                        continue;
+                    }
+                }
+            }
+            r.indep.push_back(g);
+            k++;
+        }
+        if (!r.dep.empty() && r.indep.size() >= 2) rsplines.push_back({eid, std::move(r)});
+    }
+    sort_last_wins(rsplines);
+    D.rspline.clear();
+    D.rspline_indep.clear();
+    D.rspline_dep.clear();
+    for (const auto &kv : rsplines) {
+        auto rank_id = [&](int64_t g, int &rr) {
+            Num x;
+            x.kind = Num::INT;
+            x.i = g;
+            return rank_of(x, rr);
+        };
+        std::vector<int32_t> ind;
+        for (int64_t g : kv.second.indep) {
+            int rr;
+            if (rank_id(g, rr)) ind.push_back(rr);
+        }
+        if (ind.size() < 2) continue;
+        const int32_t i0 = (int32_t)D.rspline_indep.size(), d0 = (int32_t)(D.rspline_dep.size() / 2);
+        D.rspline_indep.insert(D.rspline_indep.end(), ind.begin(), ind.end());
+        for (const auto &gc : kv.second.dep) {
+            int rr;
+            if (rank_id(gc.first, rr)) D.rspline_dep.insert(D.rspline_dep.end(), {rr, (int32_t)gc.second});
+        }
+        D.rspline.insert(D.rspline.end(), {i0, (int32_t)ind.size(), d0, (int32_t)(D.rspline_dep.size() / 2) - d0});
     }
     // MPC (extract_constraints.jl:251-271): (G, C, A) triples after the SID
     // while a whole triple remains, terms with G > 0 and C > 0, equations of
@@ -1196,6 +1261,12 @@ int sol105_deck_desc(const sol105_deck *D, sol105_model_desc *d) {
     d->n_mpc_term = (int32_t)D->mpc_coef.size();
     d->mpc_term = D->mpc_term.data();
     d->mpc_coef = D->mpc_coef.data();
+    d->n_rspline = (int32_t)(D->rspline.size() / 4);
+    d->rspline = D->rspline.data();
+    d->n_rspline_indep = (int32_t)D->rspline_indep.size();
+    d->rspline_indep = D->rspline_indep.data();
+    d->n_rspline_dep = (int32_t)(D->rspline_dep.size() / 2);
+    d->rspline_dep = D->rspline_dep.data();
     return SOL105_OK;
 }
 

@@ -326,3 +326,21 @@ def rbe3_load_plate_deck(nx=10, ny=8, a=400.0, b=300.0, t=1.5, E=70e3, nu=0.33, 
     L.append(_card("FORCE", 10, 999999, 0, line_load * b, -1.0, 0.0, 0.0))
     L.append("ENDDATA")
     return "\n".join(L) + "\n"
+
+
+def rspline_load_plate_deck(nx=10, ny=8, a=400.0, b=300.0, t=1.5, E=70e3, nu=0.33, line_load=1.0, nd=4,
+                            offset=10.0, frac=0.25, P=-150.0):
+    """plate_deck's plate with an extra x force P at a GRID (id 999999)
+    beside the loaded edge, tied in x by an RSPLINE to the last three edge
+    GRIDs (fraction frac along the last segment): the force reaches those
+    GRIDs with weights 1 - frac and frac.  With nx = 10 the edge GRIDs of
+    the spline are 77, 88 and 99, ids that the RSPLINE field rule cannot
+    take for component codes."""
+    base = plate_deck(nx=nx, ny=ny, a=a, b=b, t=t, E=E, nu=nu, line_load=line_load, nd=nd)
+    gid = lambda i, j: j * (nx + 1) + i + 1
+    g = [gid(nx, ny - 2), gid(nx, ny - 1), gid(nx, ny)]
+    y = b * (ny - 1) / ny + frac * b / ny
+    extra = [_card("GRID", 999999, "", a + float(offset), y, 0.0),
+             _card("RSPLINE", 1, 0.1, g[0], g[1], g[2], 999999, 1),
+             _card("FORCE", 10, 999999, 0, float(P), 1.0, 0.0, 0.0)]
+    return base.replace("ENDDATA", "\n".join(extra) + "\nENDDATA")

@@ -77,6 +77,9 @@ class Sol105Arrays:
     mpc_eq: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int32))    # first term, terms
     mpc_term: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int32))  # node rank, component
     mpc_coef: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    rspline: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int32))   # indep first/count, dep first/count
+    rspline_indep: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    rspline_dep: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int32))  # node rank, component
 
     @property
     def n_nodes(self):
@@ -296,6 +299,19 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
         for t in [d0] + rest:
             mpc_term.append([rank[t["G"]], int(t["C"])])
             mpc_coef.append(float(t["A"]))
+    # RSPLINE elements in element-id order: independent GRIDs known to the
+    # model (constraints.jl:155-160), dependent (GRID, component) pairs on
+    # known GRIDs; elements with fewer than two known independent GRIDs skipped
+    rspline, rspline_indep, rspline_dep = [], [], []
+    for key in sorted(model.get("RSPLINEs", {}), key=lambda k: int(k)):
+        r = model["RSPLINEs"][key]
+        ind = [rank[g] for g in r["INDEP_GRIDS"] if g in rank]
+        if len(ind) < 2:
+            continue
+        dep = [[rank[g], int(c)] for g, c in r["DEP"] if g in rank]
+        rspline.append([len(rspline_indep), len(ind), len(rspline_dep), len(dep)])
+        rspline_indep.extend(ind)
+        rspline_dep.extend(dep)
     ndof = 6 * n
     cc = model["CASE_CONTROL"]
     pairs = _subcase_pairing(cc)
@@ -385,4 +401,7 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
         rbe3_node=np.asarray(rbe3_node, dtype=np.int32),
         mpc_eq=np.asarray(mpc_eq, dtype=np.int32).reshape(-1, 2),
         mpc_term=np.asarray(mpc_term, dtype=np.int32).reshape(-1, 2), mpc_coef=np.asarray(mpc_coef, dtype=float),
+        rspline=np.asarray(rspline, dtype=np.int32).reshape(-1, 4),
+        rspline_indep=np.asarray(rspline_indep, dtype=np.int32),
+        rspline_dep=np.asarray(rspline_dep, dtype=np.int32).reshape(-1, 2),
     )

@@ -264,7 +264,8 @@ def mesh_fingerprint(arr):
     t3p = np.delete(np.asarray(arr.t3prop, dtype=np.float64), 2, axis=1) if arr.n_t3 else np.zeros((0, 5))
     for a in (arr.node_ids, arr.xyz, arr.q4, q4p, arr.t3, t3p, arr.bar, arr.bar_vref, arr.barprop, arr.spc_mask,
               arr.F, arr.node_has_shell, arr.node_has_bar, arr.rbe2, arr.rbe3, arr.rbe3_grp, arr.rbe3_wt,
-              arr.rbe3_node, arr.mpc_eq, arr.mpc_term, arr.mpc_coef):
+              arr.rbe3_node, arr.mpc_eq, arr.mpc_term, arr.mpc_coef, arr.rspline, arr.rspline_indep,
+              arr.rspline_dep):
         a = np.ascontiguousarray(a)
         h.update(str((a.dtype.str, a.shape)).encode())
         h.update(a.tobytes())

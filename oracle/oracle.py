@@ -296,6 +296,33 @@ def constraint_map(arr):
                 rbe2m[6 * int(sl) + c - 1] = pairs
     out = dict(rbe3m)
     out.update(rbe2m)
+    # RSPLINE (constraints.jl:146-213): each dependent point projected on the
+    # polyline of the independent GRIDs (first nearest segment, parameter
+    # clamped to [0, 1]) follows that segment's ends with weights 1 - t, t;
+    # it replaces RBE2 / RBE3 entries, MPC equations replace it
+    sp = np.asarray(getattr(arr, "rspline", np.zeros((0, 4), np.int32))).reshape(-1, 4)
+    for i0, ni, d0, ndp in sp:
+        ind = [int(v) for v in arr.rspline_indep[i0:i0 + ni]]
+        if len(ind) < 2:
+            continue
+        seg = [xyz[ind[s + 1]] - xyz[ind[s]] for s in range(len(ind) - 1)]
+        if sum(float(np.sqrt(v @ v)) for v in seg) < 1e-30:
+            continue
+        for g, c in np.asarray(arr.rspline_dep[d0:d0 + ndp]).reshape(-1, 2):
+            p = xyz[int(g)]
+            best, bt, bd = 0, 0.0, np.inf
+            for s, v in enumerate(seg):
+                l2 = float(v @ v)
+                if l2 < 1e-30:
+                    continue
+                t = min(max(float((p - xyz[ind[s]]) @ v) / l2, 0.0), 1.0)
+                dist = float(np.linalg.norm(p - (xyz[ind[s]] + t * v)))
+                if dist < bd:
+                    best, bt, bd = s, t, dist
+            pairs = [(6 * ind[best] + int(c) - 1, 1.0 - bt), (6 * ind[best + 1] + int(c) - 1, bt)]
+            pairs = [pc for pc in pairs if abs(pc[1]) > 1e-15]
+            if pairs:
+                out[6 * int(g) + int(c) - 1] = pairs
     # explicit MPC equations, card order (constraints.jl:290-316): the first
     # term is dependent, u_dep = sum -A_i / A_1 u_i over the others
     eqs = np.asarray(getattr(arr, "mpc_eq", np.zeros((0, 2), np.int32))).reshape(-1, 2)
@@ -760,7 +787,7 @@ def localization_filter(arr, free, V, order, num_modes, n_shell_cards, max_share
 
 def _has_constraints(arr):
     return (len(getattr(arr, "rbe2", ())) > 0 or len(getattr(arr, "rbe3", ())) > 0
-            or len(getattr(arr, "mpc_eq", ())) > 0)
+            or len(getattr(arr, "mpc_eq", ())) > 0 or len(getattr(arr, "rspline", ())) > 0)
 
 
 def solve(arr):

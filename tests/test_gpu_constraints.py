@@ -1,10 +1,11 @@
-"""GPU: RBE2 / RBE3 / MPC constraints (dependent-DOF elimination) through the C
-ABI against the oracle (constraints.jl:37-364 restated in oracle.py):
+"""GPU: RBE2 / RBE3 / RSPLINE / MPC constraints (dependent-DOF elimination)
+through the C ABI against the oracle (constraints.jl:37-364 restated in
+oracle.py):
 
 * glued half meshes (RBE2 123456 pairs of coincident GRIDs, or six MPC
-  equations per pair), a plate with
-  eccentric CBAR stringers on their own GRIDs tied by RBE2, and a plate
-  loaded through an RBE3 at one reference GRID;
+  equations per pair), a plate with eccentric CBAR stringers on their own
+  GRIDs tied by RBE2, a plate loaded through an RBE3 at one reference GRID,
+  and one with a force at a GRID tied to the loaded edge by an RSPLINE;
 * K and KG (redistributed, dependent rows and columns cleared) within 1e-12
   of their largest entry, bit-identical run to run; the free set, u within
   1e-10 (dependent DOFs recovered); lambda within 1e-8 and modes at MAC >=
@@ -24,6 +25,7 @@ CASES = {
     "eccentric": lambda: decks.eccentric_panel_deck(nx=16, ny=8, stringer_every=4),
     "rbe3_load": lambda: decks.rbe3_load_plate_deck(nx=10, ny=8),
     "glued_mpc": lambda: decks.glued_plate_deck(nx=12, ny=8, tie="mpc"),
+    "rspline_load": lambda: decks.rspline_load_plate_deck(nx=10, ny=8),
 }
 
 
