@@ -98,6 +98,17 @@ typedef struct sol105_model_desc {
     const int32_t *rspline_indep; /* independent node ranks, polyline order       */
     int32_t n_rspline_dep;
     const int32_t *rspline_dep; /* n_rspline_dep x 2: node rank, component 1..6   */
+    /* RBE1 (constraints.jl:76-143): the rigid motion about the centroid of
+     * the independent GRIDs fitted to the independent DOFs (pseudo-inverse,
+     * 1 to 6 DOFs) drives the dependent DOFs.  Replaces RBE2 / RBE3
+     * entries; RSPLINE and MPC entries replace it. */
+    int32_t n_rbe1;
+    const int32_t *rbe1;        /* n_rbe1 x 4: first independent, count,          */
+                                /*   first dependent, count                       */
+    int32_t n_rbe1_indep;
+    const int32_t *rbe1_indep;  /* n_rbe1_indep x 2: node rank, component 1..6    */
+    int32_t n_rbe1_dep;
+    const int32_t *rbe1_dep;    /* n_rbe1_dep x 2: node rank, component 1..6      */
 } sol105_model_desc;
 
 /* Native deck reader: Nastran deck text (executive and case control, bulk

@@ -72,6 +72,9 @@ class ModelDesc(ctypes.Structure):
         ("n_rspline", ctypes.c_int32), ("rspline", ctypes.POINTER(ctypes.c_int32)),
         ("n_rspline_indep", ctypes.c_int32), ("rspline_indep", ctypes.POINTER(ctypes.c_int32)),
         ("n_rspline_dep", ctypes.c_int32), ("rspline_dep", ctypes.POINTER(ctypes.c_int32)),
+        ("n_rbe1", ctypes.c_int32), ("rbe1", ctypes.POINTER(ctypes.c_int32)),
+        ("n_rbe1_indep", ctypes.c_int32), ("rbe1_indep", ctypes.POINTER(ctypes.c_int32)),
+        ("n_rbe1_dep", ctypes.c_int32), ("rbe1_dep", ctypes.POINTER(ctypes.c_int32)),
     ]
 
 
@@ -212,7 +215,10 @@ def read_deck(text, buckling_index=0, with_sol=False):
             mpc_coef=arr(d.mpc_coef, d.n_mpc_term, dbl, (d.n_mpc_term,)),
             rspline=arr(d.rspline, 4 * d.n_rspline, i32, (d.n_rspline, 4)),
             rspline_indep=arr(d.rspline_indep, d.n_rspline_indep, i32, (d.n_rspline_indep,)),
-            rspline_dep=arr(d.rspline_dep, 2 * d.n_rspline_dep, i32, (d.n_rspline_dep, 2)))
+            rspline_dep=arr(d.rspline_dep, 2 * d.n_rspline_dep, i32, (d.n_rspline_dep, 2)),
+            rbe1=arr(d.rbe1, 4 * d.n_rbe1, i32, (d.n_rbe1, 4)),
+            rbe1_indep=arr(d.rbe1_indep, 2 * d.n_rbe1_indep, i32, (d.n_rbe1_indep, 2)),
+            rbe1_dep=arr(d.rbe1_dep, 2 * d.n_rbe1_dep, i32, (d.n_rbe1_dep, 2)))
         return (arrays, int(sz[8])) if with_sol else arrays
     finally:
         L.sol105_deck_free(h)
@@ -317,6 +323,9 @@ def _model_desc(arr):
         rspline=np.ascontiguousarray(arr.rspline, dtype=np.int32).reshape(-1),
         rspline_indep=np.ascontiguousarray(arr.rspline_indep, dtype=np.int32),
         rspline_dep=np.ascontiguousarray(arr.rspline_dep, dtype=np.int32).reshape(-1),
+        rbe1=np.ascontiguousarray(arr.rbe1, dtype=np.int32).reshape(-1),
+        rbe1_indep=np.ascontiguousarray(arr.rbe1_indep, dtype=np.int32).reshape(-1),
+        rbe1_dep=np.ascontiguousarray(arr.rbe1_dep, dtype=np.int32).reshape(-1),
     )
     k = keep
     ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
@@ -330,7 +339,9 @@ def _model_desc(arr):
                   _dp(k["rbe3_wt"]), len(arr.rbe3_node), ip(k["rbe3_node"]),
                   len(arr.mpc_eq), ip(k["mpc_eq"]), len(arr.mpc_term), ip(k["mpc_term"]), _dp(k["mpc_coef"]),
                   len(arr.rspline), ip(k["rspline"]), len(arr.rspline_indep), ip(k["rspline_indep"]),
-                  len(arr.rspline_dep), ip(k["rspline_dep"]))
+                  len(arr.rspline_dep), ip(k["rspline_dep"]),
+                  len(arr.rbe1), ip(k["rbe1"]), len(arr.rbe1_indep), ip(k["rbe1_indep"]),
+                  len(arr.rbe1_dep), ip(k["rbe1_dep"]))
     return d, keep
 
 

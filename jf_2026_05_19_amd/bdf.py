@@ -3,7 +3,7 @@
 Builds the same model dictionary shape as the reference's ModelBuilder
 (``JFEM/src/ModelBuilder.jl:421-823``) for the cards this path supports:
 GRID, CQUAD4, CTRIA3, CBAR, CBEAM, PSHELL, PBARL, MAT1, SPC1, SPC, FORCE,
-EIGRL, PARAM, RBE2, RBE3, RBAR, MPC, RSPLINE and the case-control keys SOL / SUBCASE / SPC / LOAD / METHOD /
+EIGRL, PARAM, RBE1, RBE2, RBE3, RBAR, MPC, RSPLINE and the case-control keys SOL / SUBCASE / SPC / LOAD / METHOD /
 STATSUB.  Field parsing follows ``JFEM/src/parsing/utilities.jl`` (small and
 large fixed fields, free-field commas, ``+``/``*`` continuations, Nastran
 exponent shorthand such as ``1.-5``).
@@ -17,8 +17,8 @@ membrane only), ``:40-190`` (PBARL BAR section constants), ``:540-560``
 ``JFEM/src/parsing/extract_constraints.jl:46-141`` (RBE3 weight groups: a
 field with '.', an exponent or an inner sign starts a new group; continuation
 markers are skipped), RBAR becomes an RBE2 as ``ModelBuilder.jl:708-725``
-converts it, MPC follows ``extract_constraints.jl:251-271`` and RSPLINE
-``:202-250``.
+converts it, MPC follows ``extract_constraints.jl:251-271``, RSPLINE
+``:202-250`` and RBE1 ``:147-200``.
 """
 from __future__ import annotations
 
@@ -427,9 +427,30 @@ def cards_to_model(cards, cc):
             k += 1
         if dep and len(indep) >= 2:
             model["RSPLINEs"][str(eid)] = {"ID": eid, "INDEP_GRIDS": indep, "DEP": dep}
+    # RBE1: EID, (GRID, components) pairs up to the "UM" field, then the
+    # dependent pairs; every digit of a component field is one DOF
+    # (extract_constraints.jl:147-200)
+    model["RBE1s"] = {}
+    for c in cards.get("RBE1", []):
+        eid = _to_id(c[0] if c else "")
+        um = next((k for k in range(len(c)) if str(c[k]).strip().upper() == "UM"), -1)
+        if eid <= 0 or um < 0:
+            continue
+
+        def pairs(k, stop):
+            out = []
+            while k + 1 < len(c) and k < stop:
+                g, cn = _to_id(c[k]), _to_id(c[k + 1])
+                if g > 0 and cn > 0:
+                    out.extend((g, int(ch)) for ch in str(cn) if ch.isdigit())
+                k += 2
+            return out
+        indep, dep = pairs(1, um), pairs(um + 1, len(c))
+        if indep and dep:
+            model["RBE1s"][str(eid)] = {"ID": eid, "INDEP": indep, "DEP": dep}
     unsupported = sorted(set(cards) - {"GRID", "CQUAD4", "CTRIA3", "CBAR", "CBEAM", "PSHELL", "PBARL",
                                        "MAT1", "SPC1", "SPC", "FORCE", "EIGRL", "PARAM", "RBE2", "RBE3", "RBAR",
-                                       "MPC", "RSPLINE"})
+                                       "MPC", "RSPLINE", "RBE1"})
     model["_UNSUPPORTED_CARDS"] = unsupported
     return model
 

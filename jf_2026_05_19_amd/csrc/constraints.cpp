@@ -1,4 +1,4 @@
-// constraints.cpp -- RBE2 / RBE3 / RSPLINE / MPC constraint map, pattern pairs and the
+// constraints.cpp -- RBE1 / RBE2 / RBE3 / RSPLINE / MPC constraint map, pattern pairs and the
 // deterministic redistribution plan (host code).
 //
 // Restates JFEM/src/solver/constraints.jl:37-364 for RBE2 (and RBAR as
@@ -48,8 +48,8 @@ void rigid_row(double dx, double dy, double dz, int c, double out[6]) {
 }
 
 // pseudo-inverse of a symmetric positive semi-definite 6x6 matrix (cyclic
-// Jacobi; eigenvalues at or below 1e-10 of the largest are dropped)
-void pinv6(const double A[36], double P[36]) {
+// Jacobi; eigenvalues at or below rcut times the largest are dropped)
+void pinv6(const double A[36], double P[36], double rcut = 1e-10) {
     double a[6][6], v[6][6];
     for (int i = 0; i < 6; i++)
         for (int j = 0; j < 6; j++) { a[i][j] = 0.5 * (A[i * 6 + j] + A[j * 6 + i]); v[i][j] = i == j ? 1.0 : 0.0; }
@@ -83,7 +83,7 @@ void pinv6(const double A[36], double P[36]) {
     }
     double lmax = 0.0;
     for (int i = 0; i < 6; i++) lmax = std::max(lmax, std::fabs(a[i][i]));
-    const double cut = 1e-10 * lmax;
+    const double cut = rcut * lmax;
     for (int i = 0; i < 36; i++) P[i] = 0.0;
     for (int k = 0; k < 6; k++) {
         double l = a[k][k];
@@ -198,6 +198,72 @@ int build_constraint_map(const sol105_model_desc *d, ConstraintMap &C, std::stri
         }
     }
     for (auto &kv : rbe2m) rbe3m[kv.first] = kv.second;
+    // RBE1 (constraints.jl:76-143): the rigid motion q about the centroid of
+    // the independent GRIDs with A q = u_indep (A's rows the rigid rows of
+    // the independent DOFs), q = pinv(A) u_indep; a dependent DOF with rigid
+    // row b takes pinv(A)' b.  pinv(A) = pinv(A'A) A', singular values below
+    // 1e-10 of the largest dropped (eigenvalues of A'A below 1e-20).
+    if (d->n_rbe1 > 0 && (!d->rbe1 || !d->rbe1_indep || !d->rbe1_dep)) {
+        err = "missing RBE1 arrays";
+        return SOL105_E_INVALID;
+    }
+    std::map<int32_t, std::vector<std::pair<int32_t, double>>> rbe1m;
+    for (int e = 0; e < d->n_rbe1; e++) {
+        const int32_t *r = d->rbe1 + 4 * (size_t)e;
+        const int i0 = r[0], ni = r[1], d0 = r[2], ndp = r[3];
+        bool ok = i0 >= 0 && ni >= 1 && ni <= 6 && i0 + ni <= d->n_rbe1_indep && d0 >= 0 && ndp >= 0 &&
+                  d0 + ndp <= d->n_rbe1_dep;
+        for (int k = 0; ok && k < ni + ndp; k++) {
+            const int32_t *gc = k < ni ? d->rbe1_indep + 2 * (size_t)(i0 + k) : d->rbe1_dep + 2 * (size_t)(d0 + k - ni);
+            ok = gc[0] >= 0 && gc[0] < n && gc[1] >= 1 && gc[1] <= 6;
+        }
+        if (!ok) {
+            std::snprintf(buf, sizeof buf, "RBE1 %d: ranks, components or counts out of range", e);
+            err = buf;
+            return SOL105_E_INVALID;
+        }
+        const int32_t *ind = d->rbe1_indep + 2 * (size_t)i0;
+        std::vector<int> grids;
+        for (int k = 0; k < ni; k++)
+            if (std::find(grids.begin(), grids.end(), ind[2 * k]) == grids.end()) grids.push_back(ind[2 * k]);
+        double xr[3] = {0.0, 0.0, 0.0};
+        for (int g : grids)
+            for (int a = 0; a < 3; a++) xr[a] += P(g, a);
+        for (int a = 0; a < 3; a++) xr[a] /= (double)grids.size();
+        std::vector<double> A((size_t)ni * 6);
+        for (int k = 0; k < ni; k++)
+            rigid_row(P(ind[2 * k], 0) - xr[0], P(ind[2 * k], 1) - xr[1], P(ind[2 * k], 2) - xr[2], ind[2 * k + 1],
+                      &A[(size_t)k * 6]);
+        double AtA[36] = {0}, Pi[36];
+        for (int a = 0; a < 6; a++)
+            for (int b = 0; b < 6; b++)
+                for (int k = 0; k < ni; k++) AtA[a * 6 + b] += A[(size_t)k * 6 + a] * A[(size_t)k * 6 + b];
+        pinv6(AtA, Pi, 1e-20);
+        for (int k = 0; k < ndp; k++) {
+            const int g = d->rbe1_dep[2 * (size_t)(d0 + k)], c = d->rbe1_dep[2 * (size_t)(d0 + k) + 1];
+            double b[6], w[6];
+            rigid_row(P(g, 0) - xr[0], P(g, 1) - xr[1], P(g, 2) - xr[2], c, b);
+            for (int a = 0; a < 6; a++) {
+                w[a] = 0.0;
+                for (int q = 0; q < 6; q++) w[a] += Pi[a * 6 + q] * b[q];
+            }
+            std::vector<std::pair<int32_t, double>> pairs;
+            for (int j = 0; j < ni; j++) {
+                double cj = 0.0;
+                for (int a = 0; a < 6; a++) cj += A[(size_t)j * 6 + a] * w[a];
+                if (std::fabs(cj) > 1e-15) pairs.emplace_back(6 * ind[2 * j] + ind[2 * j + 1] - 1, cj);
+            }
+            if (pairs.empty()) continue;
+            const int32_t dd = 6 * g + c - 1;
+            if (rbe1m.count(dd)) {
+                std::snprintf(buf, sizeof buf, "RBE1: DOF %d is dependent in two elements", dd);
+                err = buf;
+                return SOL105_E_UNSUPPORTED;
+            }
+            rbe1m.emplace(dd, std::move(pairs));
+        }
+    }
+    for (auto &kv : rbe1m) rbe3m[kv.first] = kv.second;
     // RSPLINE (constraints.jl:146-213): a dependent point is projected on the
     // polyline of the independent GRIDs (the first nearest segment, t
     // clamped to [0, 1]) and follows that segment's ends with 1 - t and t

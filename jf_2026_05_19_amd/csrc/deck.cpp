@@ -14,7 +14,7 @@
 //   * GRID, CQUAD4, CTRIA3, CBAR, CBEAM, PSHELL (blank MID2: membrane only),
 //     PBARL TYPE=BAR (section constants, Cowper shear factor at nu = 0.3),
 //     MAT1 (completion of E, G, nu), SPC1 (THRU), SPC, FORCE, EIGRL, PARAM,
-//     RBE2, RBE3, RBAR (as an RBE2), RSPLINE and MPC;
+//     RBE1, RBE2, RBE3, RBAR (as an RBE2), RSPLINE and MPC;
 //   * case control SOL / SUBCASE / key = value, global keys inherited by the
 //     subcases, the buckling and static subcase pairing (METHOD / STATSUB /
 //     LOAD / SPC);
@@ -50,6 +50,7 @@ struct sol105_deck {
     std::vector<int32_t> mpc_eq, mpc_term;  // explicit MPC equations (sol105_model_desc)
     std::vector<double> mpc_coef;
     std::vector<int32_t> rspline, rspline_indep, rspline_dep;  // RSPLINE elements (sol105_model_desc)
+    std::vector<int32_t> rbe1, rbe1_indep, rbe1_dep;           // RBE1 elements (sol105_model_desc)
     std::vector<double> xyz, q4prop, t3prop, bar_vref, barprop, bar_points, force;
     std::vector<int32_t> q4, t3, bar;
     std::vector<uint8_t> spc_mask, has_shell, has_bar;
@@ -489,7 +490,7 @@ std::string fmt_g(double x) {
 void build(const Cards &cards, const CaseControl &cc, int buckling_index, sol105_deck &D) {
     static const char *supported[] = {"GRID", "CQUAD4", "CTRIA3", "CBAR", "CBEAM", "PSHELL", "PBARL",
                                       "MAT1", "SPC1", "SPC", "FORCE", "EIGRL", "PARAM", "RBE2", "RBE3", "RBAR",
-                                      "MPC", "RSPLINE"};
+                                      "MPC", "RSPLINE", "RBE1"};
     {
         std::string list;
         for (const auto &kv : cards.by_name) {
@@ -783,6 +784,74 @@ void build(const Cards &cards, const CaseControl &cc, int buckling_index, sol105
             D.rbe3_wt.push_back(g.wt);
         }
         D.rbe3.insert(D.rbe3.end(), {ref, (int32_t)kv.second.refc, g0, (int32_t)D.rbe3_wt.size() - g0});
+    }
+    // RBE1 (extract_constraints.jl:147-200): EID, (GRID, components) pairs
+    // up to the "UM" field, then the dependent pairs; every digit of a
+    // component field is one DOF; kept with both lists non-empty
+    struct Rbe1 {
+        std::vector<std::pair<int64_t, int>> indep, dep;
+    };
+    std::vector<std::pair<int64_t, Rbe1>> rbe1s;
+    for (const CardView &c : get("RBE1")) {
+        const int64_t eid = id_at(c, 0);
+        size_t um = c.size();
+        for (size_t k = 0; k < c.size(); k++) {
+            std::string t(strip(c[k]));
+            for (char &ch : t) ch = (char)std::toupper((unsigned char)ch);
+            if (t == "UM") {
+                um = k;
+                break;
+            }
+        }
+        if (eid <= 0 || um == c.size()) continue;
+        auto pairs = [&](size_t k, size_t stop, std::vector<std::pair<int64_t, int>> &out) {
+            for (; k + 1 < c.size() && k < stop; k += 2) {
+                const int64_t g = id_at(c, k), cn = id_at(c, k + 1);
+                if (g > 0 && cn > 0)
+                    for (char ch : std::to_string(cn)) out.push_back({g, ch - '0'});
+            }
+        };
+        Rbe1 r;
+        pairs(1, um, r.indep);
+        pairs(um + 1, c.size(), r.dep);
+        if (!r.indep.empty() && !r.dep.empty()) rbe1s.push_back({eid, std::move(r)});
+    }
+    sort_last_wins(rbe1s);
+    D.rbe1.clear();
+    D.rbe1_indep.clear();
+    D.rbe1_dep.clear();
+    for (const auto &kv : rbe1s) {
+        for (const auto *lst : {&kv.second.indep, &kv.second.dep})
+            for (const auto &gc : *lst)
+                if (gc.second < 1 || gc.second > 6)
+                    throw Unsupported("RBE1 " + std::to_string(kv.first) + ": component " +
+                                      std::to_string(gc.second) + " of GRID " + std::to_string(gc.first));
+        auto rank_id = [&](int64_t g, int &rr) {
+            Num x;
+            x.kind = Num::INT;
+            x.i = g;
+            return rank_of(x, rr);
+        };
+        const auto &in = kv.second.indep;
+        if (in.size() > 6) continue;
+        std::vector<int32_t> ind;
+        bool all = true;
+        for (const auto &gc : in) {
+            int rr;
+            if (!rank_id(gc.first, rr)) {
+                all = false;
+                break;
+            }
+            ind.insert(ind.end(), {rr, (int32_t)gc.second});
+        }
+        if (!all) continue;
+        const int32_t i0 = (int32_t)(D.rbe1_indep.size() / 2), d0 = (int32_t)(D.rbe1_dep.size() / 2);
+        D.rbe1_indep.insert(D.rbe1_indep.end(), ind.begin(), ind.end());
+        for (const auto &gc : kv.second.dep) {
+            int rr;
+            if (rank_id(gc.first, rr)) D.rbe1_dep.insert(D.rbe1_dep.end(), {rr, (int32_t)gc.second});
+        }
+        D.rbe1.insert(D.rbe1.end(), {i0, (int32_t)in.size(), d0, (int32_t)(D.rbe1_dep.size() / 2) - d0});
     }
     // RSPLINE (extract_constraints.jl:202-250): EID, D/L, then GRIDs in
     // polyline order; a GRID followed by a component code (digits 0-6, at
@@ -1267,6 +1336,12 @@ int sol105_deck_desc(const sol105_deck *D, sol105_model_desc *d) {
     d->rspline_indep = D->rspline_indep.data();
     d->n_rspline_dep = (int32_t)(D->rspline_dep.size() / 2);
     d->rspline_dep = D->rspline_dep.data();
+    d->n_rbe1 = (int32_t)(D->rbe1.size() / 4);
+    d->rbe1 = D->rbe1.data();
+    d->n_rbe1_indep = (int32_t)(D->rbe1_indep.size() / 2);
+    d->rbe1_indep = D->rbe1_indep.data();
+    d->n_rbe1_dep = (int32_t)(D->rbe1_dep.size() / 2);
+    d->rbe1_dep = D->rbe1_dep.data();
     return SOL105_OK;
 }
 

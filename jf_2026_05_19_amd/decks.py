@@ -246,8 +246,9 @@ def glued_plate_deck(nx=12, ny=8, a=400.0, b=300.0, t=1.5, E=70e3, nu=0.33, line
     """plate_deck's plate (nx even) as two half meshes: the cells right of
     x = a/2 use their own GRIDs (ids + 100000) and each seam GRID of the
     right half is the RBE2 (123456) slave of the coincident left one
-    (tie="rbe2"), or tied to it by six MPC equations u_r - u_l = 0
-    (tie="mpc")."""
+    (tie="rbe2"), tied to it by six MPC equations u_r - u_l = 0
+    (tie="mpc"), or the dependent GRID of an RBE1 whose six independent
+    DOFs are the left GRID's (tie="rbe1", the same rigid link)."""
     assert nx % 2 == 0
     L = _header(nd)
     L.append(_card("MAT1", 1, float(E), "", float(nu)))
@@ -269,7 +270,9 @@ def glued_plate_deck(nx=12, ny=8, a=400.0, b=300.0, t=1.5, E=70e3, nu=0.33, line
             L.append(_card("CQUAD4", eid, 1, g(i, j), g(i + 1, j), g(i + 1, j + 1), g(i, j + 1)))
             eid += 1
     for j in range(ny + 1):
-        if tie == "mpc":
+        if tie == "rbe1":
+            L.append(_card("RBE1", 900000 + j, gl(h, j), 123456, "UM", gr(h, j), 123456))
+        elif tie == "mpc":
             for c in range(1, 7):
                 L.append(_card("MPC", 1, gr(h, j), c, 1.0, gl(h, j), c, -1.0))
         else:

@@ -80,6 +80,9 @@ class Sol105Arrays:
     rspline: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int32))   # indep first/count, dep first/count
     rspline_indep: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     rspline_dep: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int32))  # node rank, component
+    rbe1: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int32))       # indep first/count, dep first/count
+    rbe1_indep: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int32))  # node rank, component
+    rbe1_dep: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int32))    # node rank, component
 
     @property
     def n_nodes(self):
@@ -312,6 +315,22 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
         rspline.append([len(rspline_indep), len(ind), len(rspline_dep), len(dep)])
         rspline_indep.extend(ind)
         rspline_dep.extend(dep)
+    # RBE1 elements in element-id order (constraints.jl:80-112): one to six
+    # independent DOFs, all on known GRIDs (else the element is skipped);
+    # dependent DOFs on unknown GRIDs skipped.  Components outside 1..6 make
+    # the reference's rigid row fail, so they are refused here.
+    rbe1, rbe1_indep, rbe1_dep = [], [], []
+    for key in sorted(model.get("RBE1s", {}), key=lambda k: int(k)):
+        r = model["RBE1s"][key]
+        for g, c in list(r["INDEP"]) + list(r["DEP"]):
+            if not 1 <= int(c) <= 6:
+                raise UnsupportedModel(f"RBE1 {r['ID']}: component {c} of GRID {g}")
+        if not 1 <= len(r["INDEP"]) <= 6 or any(g not in rank for g, _ in r["INDEP"]):
+            continue
+        dep = [[rank[g], int(c)] for g, c in r["DEP"] if g in rank]
+        rbe1.append([len(rbe1_indep), len(r["INDEP"]), len(rbe1_dep), len(dep)])
+        rbe1_indep.extend([rank[g], int(c)] for g, c in r["INDEP"])
+        rbe1_dep.extend(dep)
     ndof = 6 * n
     cc = model["CASE_CONTROL"]
     pairs = _subcase_pairing(cc)
@@ -404,4 +423,7 @@ def build_arrays(model: dict, buckling_index: int = 0) -> Sol105Arrays:
         rspline=np.asarray(rspline, dtype=np.int32).reshape(-1, 4),
         rspline_indep=np.asarray(rspline_indep, dtype=np.int32),
         rspline_dep=np.asarray(rspline_dep, dtype=np.int32).reshape(-1, 2),
+        rbe1=np.asarray(rbe1, dtype=np.int32).reshape(-1, 4),
+        rbe1_indep=np.asarray(rbe1_indep, dtype=np.int32).reshape(-1, 2),
+        rbe1_dep=np.asarray(rbe1_dep, dtype=np.int32).reshape(-1, 2),
     )

@@ -265,7 +265,7 @@ def mesh_fingerprint(arr):
     for a in (arr.node_ids, arr.xyz, arr.q4, q4p, arr.t3, t3p, arr.bar, arr.bar_vref, arr.barprop, arr.spc_mask,
               arr.F, arr.node_has_shell, arr.node_has_bar, arr.rbe2, arr.rbe3, arr.rbe3_grp, arr.rbe3_wt,
               arr.rbe3_node, arr.mpc_eq, arr.mpc_term, arr.mpc_coef, arr.rspline, arr.rspline_indep,
-              arr.rspline_dep):
+              arr.rspline_dep, arr.rbe1, arr.rbe1_indep, arr.rbe1_dep):
         a = np.ascontiguousarray(a)
         h.update(str((a.dtype.str, a.shape)).encode())
         h.update(a.tobytes())

@@ -296,6 +296,21 @@ def constraint_map(arr):
                 rbe2m[6 * int(sl) + c - 1] = pairs
     out = dict(rbe3m)
     out.update(rbe2m)
+    # RBE1 (constraints.jl:76-143): q = pinv(A, 1e-10) u_indep is the rigid
+    # motion about the centroid of the independent GRIDs, A's rows the rigid
+    # rows of the independent DOFs; a dependent DOF takes pinv(A)' b
+    r1 = np.asarray(getattr(arr, "rbe1", np.zeros((0, 4), np.int32))).reshape(-1, 4)
+    for i0, ni, d0, ndp in r1:
+        ind = [(int(g), int(c)) for g, c in np.asarray(arr.rbe1_indep[i0:i0 + ni]).reshape(-1, 2)]
+        grids = list(dict.fromkeys(g for g, _ in ind))
+        xr = xyz[grids].mean(axis=0)
+        A = np.array([_rigid_row(xyz[g] - xr, c) for g, c in ind])
+        Ai = np.linalg.pinv(A, rcond=1e-10)
+        for g, c in np.asarray(arr.rbe1_dep[d0:d0 + ndp]).reshape(-1, 2):
+            cv = Ai.T @ _rigid_row(xyz[int(g)] - xr, int(c))
+            pairs = [(6 * gi + ci - 1, float(v)) for (gi, ci), v in zip(ind, cv) if abs(v) > 1e-15]
+            if pairs:
+                out[6 * int(g) + int(c) - 1] = pairs
     # RSPLINE (constraints.jl:146-213): each dependent point projected on the
     # polyline of the independent GRIDs (first nearest segment, parameter
     # clamped to [0, 1]) follows that segment's ends with weights 1 - t, t;
@@ -787,7 +802,8 @@ def localization_filter(arr, free, V, order, num_modes, n_shell_cards, max_share
 
 def _has_constraints(arr):
     return (len(getattr(arr, "rbe2", ())) > 0 or len(getattr(arr, "rbe3", ())) > 0
-            or len(getattr(arr, "mpc_eq", ())) > 0 or len(getattr(arr, "rspline", ())) > 0)
+            or len(getattr(arr, "mpc_eq", ())) > 0 or len(getattr(arr, "rspline", ())) > 0
+            or len(getattr(arr, "rbe1", ())) > 0)
 
 
 def solve(arr):

@@ -1,4 +1,4 @@
-"""RBE2 / RBE3 / MPC constraints on the CPU: pins of the oracle's restatement of
+"""RBE1 / RBE2 / RBE3 / RSPLINE / MPC constraints on the CPU: pins of the oracle's restatement of
 the reference's dependent-DOF elimination (JFEM/src/solver/constraints.jl:
 37-364, the load redistribution of solve_case.jl:40-55 and the recovery of
 boundary_conditions.jl:978-988, solve_case.jl:3153-3162), the library's host
@@ -199,6 +199,50 @@ def test_rspline_projects_and_moves_a_load_by_its_weights():
     assert np.isclose(rs["u"][nd], 0.75 * rs["u"][6 * 87] + 0.25 * rs["u"][6 * 98], rtol=1e-14)
 
 
+def test_rbe1_on_one_grid_is_the_rbe2_link():
+    """An RBE1 whose six independent DOFs are one GRID's is the rigid link
+    of an RBE2 from that GRID: the same map and the one-piece plate."""
+    e1 = _arr(decks.glued_plate_deck(nx=8, ny=4, tie="rbe1"))
+    e2 = _arr(decks.glued_plate_deck(nx=8, ny=4))
+    m1, m2 = O.constraint_map(e1), O.constraint_map(e2)
+    assert sorted(m1) == sorted(m2) and len(e1.rbe1) == 5
+    for k in m2:
+        a = dict(m1[k]); b = dict(m2[k])
+        assert sorted(a) == sorted(b) and all(abs(a[i] - b[i]) <= 1e-13 for i in a)
+    p = _arr(decks.plate_deck(nx=8, ny=4, a=400.0, b=300.0, t=1.5, nd=4))
+    assert np.allclose(O.solve(e1)["eigenvalues"], O.solve(p)["eigenvalues"], rtol=1e-10, atol=0)
+
+
+def test_rbe1_reproduces_rigid_motion_and_balances_load():
+    """3-2-1 independent DOFs on three GRIDs: a rigid motion of the grids
+    gives the dependent GRID the same rigid motion, and a load there moves
+    onto the independent DOFs with the same resultant force and moment."""
+    rng = np.random.default_rng(7)
+    xyz = rng.normal(size=(4, 3)) * 40.0
+    ns = types.SimpleNamespace(xyz=xyz, rbe2=np.zeros((0, 3), np.int32), rbe3=np.zeros((0, 4), np.int32),
+                               rbe1=np.array([[0, 6, 0, 6]], np.int32),
+                               rbe1_indep=np.array([[0, 1], [0, 2], [0, 3], [1, 2], [1, 3], [2, 3]], np.int32),
+                               rbe1_dep=np.array([[3, c] for c in range(1, 7)], np.int32))
+    cmap = O.constraint_map(ns)
+    assert sorted(cmap) == [18 + c for c in range(6)]
+    t, th = rng.normal(size=3), rng.normal(size=3) * 1e-2
+    u = _rigid(xyz, t, th)
+    ref = u.copy()
+    ref[18:] = 0.0
+    O.recover_dependent(cmap, ref)
+    assert np.allclose(ref[18:24], u[18:24], rtol=0, atol=1e-12 * np.abs(u).max())
+    Fr = rng.normal(size=6)
+    F = np.zeros(24)
+    F[18:24] = Fr
+    for dep, pairs in cmap.items():
+        for ind, c in pairs:
+            F[ind] += F[dep] * c
+    grid = F[:18].reshape(3, 6)
+    assert np.allclose(grid[:, :3].sum(axis=0), Fr[:3], rtol=1e-10, atol=1e-10)
+    mom = sum(np.cross(xyz[i] - xyz[3], grid[i, :3]) for i in range(3))
+    assert np.allclose(mom, Fr[3:], rtol=1e-9, atol=1e-9)
+
+
 def test_rspline_field_rule_takes_small_digit_ids_for_components():
     """The reference's RSPLINE reader calls a GRID dependent when the next
     field reads as a component code (extract_constraints.jl:222-237): in
@@ -238,6 +282,7 @@ DECKS = [("glued", lambda: decks.glued_plate_deck(nx=8, ny=4)),
          ("rbe3_load", lambda: decks.rbe3_load_plate_deck(nx=6, ny=4)),
          ("glued_mpc", lambda: decks.glued_plate_deck(nx=8, ny=4, tie="mpc")),
          ("rspline_load", lambda: decks.rspline_load_plate_deck(nx=10, ny=8)),
+         ("glued_rbe1", lambda: decks.glued_plate_deck(nx=8, ny=4, tie="rbe1")),
          ("mpc_over_rbe2", lambda: decks.glued_plate_deck(nx=4, ny=2).replace(
              "ENDDATA", "MPC,7,100003,3,1.0,3,3,-0.5,5,3,0.25\nMPC,8,100003,3,2.0,3,3,-1.0\nENDDATA"))]
 
@@ -270,6 +315,7 @@ def test_library_map_and_redistribution_match_oracle(probe, name, deck):
     plain.rbe3 = np.zeros((0, 4), np.int32)
     plain.mpc_eq = np.zeros((0, 2), np.int32)
     plain.rspline = np.zeros((0, 4), np.int32)
+    plain.rbe1 = np.zeros((0, 4), np.int32)
     K0 = O.assemble_K(plain).toarray()
     K1 = O.assemble_K(arr).toarray()
     out = np.zeros_like(K0)
@@ -302,7 +348,7 @@ def test_native_reader_matches_host_builder(name, deck):
     txt = deck()
     a, b = _arr(txt), N.read_deck(txt)
     for f in ("rbe2", "rbe3", "rbe3_grp", "rbe3_wt", "rbe3_node", "mpc_eq", "mpc_term", "mpc_coef", "rspline",
-              "rspline_indep", "rspline_dep", "xyz", "F",
+              "rspline_indep", "rspline_dep", "rbe1", "rbe1_indep", "rbe1_dep", "xyz", "F",
               "spc_mask", "q4", "bar"):
         x, y = getattr(a, f), getattr(b, f)
         assert x.shape == y.shape and np.array_equal(x, y), (name, f)

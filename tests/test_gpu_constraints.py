@@ -1,9 +1,9 @@
-"""GPU: RBE2 / RBE3 / RSPLINE / MPC constraints (dependent-DOF elimination)
+"""GPU: RBE1 / RBE2 / RBE3 / RSPLINE / MPC constraints (dependent-DOF elimination)
 through the C ABI against the oracle (constraints.jl:37-364 restated in
 oracle.py):
 
-* glued half meshes (RBE2 123456 pairs of coincident GRIDs, or six MPC
-  equations per pair), a plate with eccentric CBAR stringers on their own
+* glued half meshes (RBE2 123456 pairs of coincident GRIDs, six MPC
+  equations per pair, or one RBE1 per pair), a plate with eccentric CBAR stringers on their own
   GRIDs tied by RBE2, a plate loaded through an RBE3 at one reference GRID,
   and one with a force at a GRID tied to the loaded edge by an RSPLINE;
 * K and KG (redistributed, dependent rows and columns cleared) within 1e-12
@@ -26,6 +26,7 @@ CASES = {
     "rbe3_load": lambda: decks.rbe3_load_plate_deck(nx=10, ny=8),
     "glued_mpc": lambda: decks.glued_plate_deck(nx=12, ny=8, tie="mpc"),
     "rspline_load": lambda: decks.rspline_load_plate_deck(nx=10, ny=8),
+    "glued_rbe1": lambda: decks.glued_plate_deck(nx=12, ny=8, tie="rbe1"),
 }
 
 
@@ -92,7 +93,7 @@ def test_glued_equals_one_piece(solved, ctx):
     dp = N.DeviceModel(ctx, _arr(decks.plate_deck(nx=12, ny=8, a=400.0, b=300.0, t=1.5, nd=4)))
     lam_p, _ = dp.solve()
     dp.free()
-    for name in ("glued", "glued_mpc"):
+    for name in ("glued", "glued_mpc", "glued_rbe1"):
         arr, dm, ref = solved[name]
         lam_g, _ = dm.solve()
         assert np.allclose(lam_g, lam_p, rtol=1e-9, atol=0), (name, lam_g, lam_p)
