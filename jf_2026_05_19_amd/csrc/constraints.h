@@ -1,6 +1,7 @@
-// constraints.h -- rigid (RBE2) and interpolation (RBE3) constraints as
-// dependent-DOF elimination: the constraint map, the extra block-pattern
-// pairs it needs and the entry-level redistribution plan of K.  Host code.
+// constraints.h -- rigid (RBE1, RBE2), interpolation (RBE3, RSPLINE) and
+// explicit MPC constraints as dependent-DOF elimination: the constraint
+// map, the extra block-pattern pairs it needs and the entry-level
+// redistribution plan of K.  Host code.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -19,8 +20,10 @@ struct ConstraintMap {
 // Builds the map of a model description.  Returns SOL105_OK, or an error
 // code with the reason in err: SOL105_E_INVALID for out-of-range ranks or
 // groups, SOL105_E_UNSUPPORTED for a dependent DOF claimed twice by one
-// constraint family or an independent DOF that is itself dependent (the
-// reference resolves neither deterministically).
+// RBE1 / RBE2 / RBE3 / RSPLINE family or an independent DOF that is itself
+// dependent (the reference resolves neither deterministically).  Families
+// merge in the order RBE3, RBE2, RBE1, RSPLINE, MPC, each replacing the
+// entries before it; among MPC equations the later card wins.
 int build_constraint_map(const sol105_model_desc *d, ConstraintMap &C, std::string &err);
 
 // Node pairs (i, j) to add to the element block pattern so that the
